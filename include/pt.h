@@ -1,0 +1,114 @@
+/* libpt: the C ABI of the MI355X wavefront SDF path tracer, a drop-in for the engine behind ShadyTracer's
+ * lib/pathtracer. The JavaScript shim (shadytracer_amd/js/raytrace_main.js) keeps the reference's
+ * RAYTRACER(WG, _U, Q, scene) object and calls these through the N-API addon (shadytracer_amd/js/addon.cc);
+ * Python reaches them through ctypes (shadytracer_amd/pt.py).
+ *
+ * Every call returns 0 on success or a PT_ERR_* code; pt_last_error(ctx) then says what went wrong. The
+ * library owns all device memory and its stream; readbacks copy into caller-provided arrays.
+ * Threading: a context is used from one thread at a time. pt_trace only enqueues; pt_sync waits.
+ */
+#ifndef PT_H
+#define PT_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "pt_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PT_OK 0
+#define PT_ERR_ARG 1        /* a bad argument: null pointer, size out of range, wrong struct size */
+#define PT_ERR_HIP 2        /* the HIP runtime failed (device lost, out of memory, launch failure) */
+#define PT_ERR_COMPILE 3    /* the scene did not compile; the previous scene is still loaded */
+#define PT_ERR_STATE 4      /* the call is not valid now: trace with no scene, or during a scene build */
+
+#define PT_ABI_VERSION 1
+
+typedef struct pt_ctx pt_ctx;
+
+typedef struct pt_opts {
+    int device;                 /* HIP device ordinal */
+    uint32_t blocks_per_cu;     /* persistent kernels' blocks per compute unit (0: library default) */
+    const char* cache_dir;      /* code-object cache (NULL: $PT_KCACHE, else kcache/ beside the library) */
+    uint32_t reserved[4];
+} pt_opts;
+
+/* A scene as the HIP emitter (shadytracer_amd/js/hipgen.js) produces it: HIP C++ defining sceneDist and
+ * sceneFull, the parameter block they read, the material table and the light faces. */
+typedef struct pt_scene_desc {
+    const char* hip_src;
+    const float* params;
+    uint32_t nparams;
+    const PtMaterial* materials;
+    uint32_t nmaterials;
+    const PtLightFace* faces;
+    uint32_t nfaces;
+} pt_scene_desc;
+
+/* ABI version of this library (PT_ABI_VERSION). */
+int pt_abi_version(void);
+
+/* sizeof of the ABI's structures, for bindings to check their mirrors: 0 PtUniforms, 1 PtMaterial,
+ * 2 PtLightFace, 3 PtCamera, 4 PtEnv, 5 PtHaze, 6 pt_scene_desc, 7 pt_opts; 0 for anything else.
+ * No counterpart in the reference (its uniform packing lives in the absent tinywebgpu submodule). */
+size_t pt_struct_size(int which);
+
+/* Creates an engine on a device and integrates its GGX albedo tables.
+ * Replaces RAYTRACER(WG, _U, Q, scene) and R.init (lib/pathtracer/core/raytrace_main.js:121, :1291-1313). */
+int pt_create(pt_ctx** out, const pt_opts* opts);
+
+/* Frees everything the engine holds. Replaces R.releaseBuffers (raytrace_main.js:234-302). */
+void pt_destroy(pt_ctx* ctx);
+
+/* The last error's text, never NULL. Replaces the Error objects R.setScene / R.trace throw
+ * (raytrace_main.js:1162-1176, :1579-1580). ctx may be NULL for errors of pt_create. */
+const char* pt_last_error(const pt_ctx* ctx);
+
+/* Compiles a scene's kernels for gfx950 into the code-object cache without a GPU, and writes the cached
+ * file's path into path_out. The build step of _buildWavefrontPipelines (raytrace_main.js:585-658). */
+int pt_compile_scene(const char* hip_src, const char* cache_dir, char* path_out, size_t path_cap);
+
+/* Loads a scene: compiles (or finds in the cache) and loads its kernels, uploads its tables and clears the
+ * accumulation. On a compile failure the previous scene stays loaded and PT_ERR_COMPILE is returned (the
+ * reference's e.restored). Replaces R.setScene (raytrace_main.js:1145-1183). */
+int pt_set_scene(pt_ctx* ctx, const pt_scene_desc* scene);
+
+/* Updates the scene's parameter block in place (same length): the builder's per-node uniforms, without a
+ * rebuild. Replaces the scene-uniform half of the uniform bag (raytrace_main.js:1997-2023). */
+int pt_set_params(pt_ctx* ctx, const float* params, uint32_t nparams);
+
+/* Sets the engine uniforms (size must be sizeof(PtUniforms)); width/height resize the buffers.
+ * Replaces the base half of the uniform bag and R.ensureBuffers (raytrace_main.js:234-302, :1997-2023). */
+int pt_set_uniforms(pt_ctx* ctx, const PtUniforms* u, size_t size);
+
+/* Clears the per-pixel accumulators and the sample counter. Replaces R.clearAccums (raytrace_main.js:1131). */
+int pt_clear_accum(pt_ctx* ctx);
+
+/* Enqueues `samples` samples (one camera ray per pixel each) on the engine's stream; returns at once.
+ * Replaces R.trace(Q, uniforms, count) (raytrace_main.js:1574-2342). */
+int pt_trace(pt_ctx* ctx, uint32_t samples);
+
+/* Waits for everything enqueued. Replaces the drain on queue.onSubmittedWorkDone (lib/pathtracer/index.js:1005-1007). */
+int pt_sync(pt_ctx* ctx);
+
+/* Copies the FP64 accumulators: accum 4 doubles per pixel (mean r, g, b, samples), moments 2 per pixel (V, S
+ * of the luminance; standard error sqrt(S V / (1 - S))). Either pointer may be NULL. npx must be width*height.
+ * Replaces the readbacks of R.imageStats / R.readRegion (raytrace_main.js:876-956, :1417-1448). */
+int pt_read_accum(pt_ctx* ctx, double* accum, double* moments, size_t npx);
+
+/* Copies the engine's counters (PT_S_* indices, 64-bit each) since the last pt_reset_stats; n <= PT_S_WORDS.
+ * Replaces the counter readback and wavefront log (raytrace_main.js:106-116, :2335-2341). */
+int pt_read_stats(pt_ctx* ctx, uint64_t* stats, size_t n);
+int pt_reset_stats(pt_ctx* ctx);
+
+/* Runs `samples` samples kernel by kernel with HIP events around each launch on the engine's stream and
+ * returns the summed milliseconds per kernel: ms[0] genrays, [1] march, [2] shade, [3] occlude, [4] resolve.
+ * Replaces R.timeKernels (raytrace_main.js:549-557). */
+int pt_time_kernels(pt_ctx* ctx, uint32_t samples, double* ms, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

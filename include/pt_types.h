@@ -1,0 +1,145 @@
+// Plain-old-data structures of the C ABI (include/pt.h), shared by the host library and the device code that
+// hipRTC compiles at scene load. Only fixed-width scalars and explicit padding: the same bytes on both sides.
+//
+// Conventions (the reference's): world Z-up, metres, linear Rec.709 RGB whose Y is cd/m^2
+// (lib/pathtracer/README.md:237-238, :290-309; core/sdf.js:619-621).
+#ifndef PT_TYPES_H
+#define PT_TYPES_H
+
+#define PT_MAX_MATERIALS 64
+#define PT_MAX_FACES 64
+#define PT_MF_NMU 32
+#define PT_MF_NR 32
+#define PT_MF_NIOR 16
+// Ec[ior][r][mu], EcAvg[ior][r], E1[r][mu]
+#define PT_MF_EC 0
+#define PT_MF_ECAVG (PT_MF_NIOR * PT_MF_NR * PT_MF_NMU)
+#define PT_MF_E1 (PT_MF_ECAVG + PT_MF_NIOR * PT_MF_NR)
+#define PT_MF_WORDS (PT_MF_E1 + PT_MF_NR * PT_MF_NMU)
+
+// Path states (the reference's RAY_* of core/tracebasics.js, renumbered)
+#define PT_ACTIVE 0u
+#define PT_ESCAPED 1u
+#define PT_MAXITER 2u
+#define PT_TERMINATED 3u
+#define PT_SCATTER 4u   // a collision in the haze box, to be shaded as a medium vertex
+
+// Device counters (u32 words)
+#define PT_C_MARCH0 0
+#define PT_C_MARCH1 1
+#define PT_C_MARCH_HEAD 2
+#define PT_C_SHADE 3
+#define PT_C_SHADOW 4
+#define PT_C_SHADOW_HEAD 5
+#define PT_C_SAMPLE 6      // samples resolved since the accumulators were cleared (drives the per-sample seed)
+#define PT_C_PHOTON 7      // photons stored in the SPPM map this pass
+#define PT_C_WORDS 16
+
+// 64-bit statistics
+#define PT_S_SEGMENTS 0      // path segments: march events handed to shade (hit, escape, cap, haze collision)
+#define PT_S_SHADOWSEGS 1    // shadow-ray segments traced
+#define PT_S_STEPS 2         // sphere-tracing steps of path rays
+#define PT_S_SHADOWSTEPS 3   // sphere-tracing steps of shadow rays
+#define PT_S_NONFINITE 4     // deposits refused as NaN/Inf/negative luminance
+#define PT_S_MAXITER 5       // rays that ran out of steps or length
+#define PT_S_PATHS 6         // camera paths completed (deposited)
+#define PT_S_LEAVING 7       // hits from behind a surface, moved on without a bounce
+#define PT_S_GATHERS 8       // SPPM camera gathers
+#define PT_S_PHOTONS 9       // SPPM photons stored
+#define PT_S_WORDS 16
+
+typedef struct PtCamera {
+    float pos[3];
+    float fx;            // invProjection[0][0]: aspect * tan(fov/2)
+    float right[3];
+    float fy;            // invProjection[1][1]: tan(fov/2)
+    float up[3];
+    unsigned int type;   // 0 perspective, 1 orthographic
+    float fwd[3];        // from the target towards the camera; the view direction is -fwd
+    float pad;
+} PtCamera;
+
+// A material as the shade kernel reads it (the reference's Material2 subset, core/materials.js:12-110).
+typedef struct PtMaterial {
+    float base[3];        // baseColor
+    float metallic;
+    float roughness;
+    float transmission;
+    float ior;
+    float specular;       // 0.5 is the dielectric default (F0 = 0.04 at ior 1.5)
+    float emissive[3];    // radiance, cd/m^2 as Y (colour scaled to emissiveLuminance)
+    float abbe;           // > 0: dispersive (Cauchy through n_d and V_d)
+    int face;             // light face this emissive material belongs to, -1 none
+    float specularTint;
+    float pad[2];
+} PtMaterial;
+
+// A light's face for next-event estimation (the reference's LightFace, core/tracebasics.js:266-277).
+typedef struct PtLightFace {
+    float centre[3];
+    float halfx;
+    float u[3];
+    float halfy;
+    float v[3];
+    unsigned int shape;   // 1 disc (halfx = radius), 2 rectangle
+    float n[3];
+    unsigned int twoSided;
+    float le[3];
+    float weight;
+} PtLightFace;
+
+// Environment: 0 black, 1 uniform sphere, 2 uniform upper hemisphere (z > 0), plus an optional sun.
+typedef struct PtEnv {
+    unsigned int type;
+    float L;              // luminance, cd/m^2
+    float col[3];         // colour with luma 1
+    float sunDir[3];      // towards the sun
+    float sunCos;         // cos of the sun's angular radius
+    float sunE;           // illuminance on a surface facing the sun, lux (0: no sun)
+    float sunRadius;      // aperture radius around sunAxisPoint (0: unlimited)
+    float sunPoint[3];    // a point on the aperture's axis
+    float pad[2];
+} PtEnv;
+
+// The homogeneous haze box (the reference's haze.js hazeUniforms): axis-aligned, centre and half size.
+typedef struct PtHaze {
+    float centre[3];
+    float sigmaS;         // scattering coefficient, 1/m (0: no haze)
+    float half[3];
+    float g;              // Henyey-Greenstein asymmetry
+    float albedo;
+    float pad[3];
+} PtHaze;
+
+// The engine's uniforms: the subset of the reference's schema (core/uniforms.schema.js:37-418) this path reads.
+typedef struct PtUniforms {
+    unsigned int width, height;
+    unsigned int maxbounce;     // 10
+    unsigned int maxiter;       // 400 march steps per path
+    float stepscale;            // 0.9
+    float eps;                  // hit tolerance setting, metres (0: epsAuto)
+    float epsAuto;
+    float envRadius;            // metres
+    unsigned int nee;           // 1
+    unsigned int rr;            // 0
+    unsigned int rrStart;       // 3
+    unsigned int diffuseCap;    // 0: none
+    float fireflyClamp;
+    float fireflyWhite;
+    unsigned int maxaverages;   // 0: unlimited
+    unsigned int seed;
+    PtCamera cam;
+    PtEnv env;
+    PtHaze haze;
+    float faceShare;            // share of light sampling that goes to light faces (0.5)
+    unsigned int sppm;          // 1: camera paths after delta-only chains gather from the photon map
+    float sppmRadius;           // current gather radius, metres (host schedule)
+    unsigned int sppmPhotons;   // photons emitted in the map's pass (normalises the gather)
+    unsigned int marchRefill;   // lanes a wave lets go idle before it fetches more rays
+    unsigned int guide;         // 1: MLP path guiding mixed with BSDF sampling
+    float guideFrac;            // share of guided directions
+    unsigned int pad0;
+} PtUniforms;
+
+
+#endif

@@ -1,0 +1,53 @@
+// The device pointer table: everything a kernel needs, by one pointer (a kernel argument, read through scalar
+// loads). Internal to the library; the host fills it (pt_api.cpp).
+#pragma once
+
+struct PtDev {
+    PtUniforms u;
+    unsigned int tilesX, tilesY;
+    float invTX, invTY;
+    unsigned int nrays;         // camera rays per sample
+    unsigned int capacity;      // path slots
+    unsigned int nmats, nfaces;
+    float faceTotal;
+    unsigned int gridBlocks;    // persistent kernels' grid
+    unsigned int mfReady;
+    unsigned int pad1;
+    // path state, structure of arrays indexed by slot
+    float* pos;        // 4/slot: xyz, ltot (optical length so far)
+    float* dir;        // 4/slot: xyz, n (index of the medium the ray is in)
+    float* beta;       // 4/slot: rgb, prevPdf
+    float* rad;        // 4/slot: rgb, vtxLtot
+    unsigned int* meta;// 4/slot: pix, counts (steps | diffuse << 16), bounces | state << 8 | flags << 16, tag
+    float* aux;        // 4/slot: sunmis, tmax (haze free flight), lambda, spare
+    // queues of slots
+    unsigned int* qMarch0;
+    unsigned int* qMarch1;
+    unsigned int* qShade;
+    unsigned int* qShadow;
+    // shadow rays by slot
+    float* shOrig;     // 4/slot: xyz, tmax
+    float* shDir;      // 4/slot: xyz, n
+    float* shRad;      // 4/slot: rgb contribution, pix (bits)
+    unsigned int* counters;
+    unsigned long long* stats;
+    // per-sample frame and FP64 accumulators
+    float* frame;      // 4/px: rgb sum, count (u32 bits)
+    double* accum;     // 4/px: mean rgb, count
+    double* moments;   // 2/px: V, S
+    // scene
+    const float* params;
+    const PtMaterial* mats;
+    const PtLightFace* faces;
+    const float* mf;   // GGX albedo tables
+    unsigned int* seedCur;  // 4 words: this sample's seed
+    // SPPM photon map
+    unsigned int* sppmHeads;
+    float* sppmRecs;
+    unsigned int sppmCap, sppmHashSize;
+    float* sppmAcc;    // per-pixel gathered flux (rgb) and count for the current pass
+    // guiding
+    const void* guideNet;
+    float* guideBuf;
+    unsigned int guideCap, guidePad;
+};

@@ -1,0 +1,238 @@
+// The wavefront kernels. The emitted scene (namespace-free functions sceneDist / sceneFull and the constant
+// PT_SCENE_MATERIALS) is compiled in front of this file by hipRTC.
+//
+// Per sample: pt_genrays, then maxbounce + 1 rounds of pt_march -> pt_shade -> pt_occlude, then pt_resolve.
+// The reference's dispatch chain (core/raytrace_main.js:2043-2179) has more passes per round because its march
+// stops after 32 steps and re-queues; here march and occlude are persistent kernels whose waves refill idle lanes
+// from the queue, so a ray marches to its event in one launch and a round is one segment of every live path.
+//
+// Path state lives in structure-of-arrays by slot (16-byte lanes, coalesced when the queue holds ascending
+// slots); queues hold slots; every append is wave-aggregated (one atomic per wave, ballot + prefix).
+#pragma once
+
+#define PT_BLOCK 256
+
+// ---------------------------------------------------------------------------------------------------
+// wave helpers
+__device__ __forceinline__ unsigned int laneId() { return __lane_id(); }
+__device__ __forceinline__ unsigned int rankIn(unsigned long long m) {
+    unsigned int l = laneId();
+    return (unsigned int)__popcll(m & ((1ull << l) - 1ull));
+}
+// Reserve one slot per predicated lane at *counter; returns this lane's index (meaningless where !pred).
+__device__ __forceinline__ unsigned int waveAppend(unsigned int* counter, bool pred) {
+    unsigned long long m = __ballot(pred);
+    if (m == 0ull) return 0u;
+    int leader = __ffsll((long long)m) - 1;
+    unsigned int base = 0u;
+    if ((int)laneId() == leader) base = atomicAdd(counter, (unsigned int)__popcll(m));
+    base = __shfl(base, leader);
+    return base + rankIn(m);
+}
+__device__ __forceinline__ unsigned long long waveSum64(unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+__device__ __forceinline__ void statAdd(const PtDev* __restrict__ D, int which, unsigned long long v) {
+    v = waveSum64(v);
+    if (laneId() == 0 && v) atomicAdd(&D->stats[which], v);
+}
+
+__device__ __forceinline__ float4 ld4(const float* p, unsigned int i) { return reinterpret_cast<const float4*>(p)[i]; }
+__device__ __forceinline__ uint4 ld4u(const unsigned int* p, unsigned int i) { return reinterpret_cast<const uint4*>(p)[i]; }
+__device__ __forceinline__ void st4(float* p, unsigned int i, float a, float b, float c, float d) {
+    reinterpret_cast<float4*>(p)[i] = make_float4(a, b, c, d);
+}
+__device__ __forceinline__ void st4u(unsigned int* p, unsigned int i, uint4 v) { reinterpret_cast<uint4*>(p)[i] = v; }
+
+// This sample's seed: the base seed and the number of samples resolved so far, through PCG4D.
+__device__ __forceinline__ void sampleSeed(const PtDev* __restrict__ D, unsigned int* s) {
+    Rng r; r.a = D->u.seed; r.b = D->counters[PT_C_SAMPLE]; r.c = 0x5EED5EEDu; r.d = 0x2026F00Du;
+    pcg4d(r); pcg4d(r);
+    s[0] = r.a; s[1] = r.b; s[2] = r.c; s[3] = r.d;
+}
+
+__device__ __forceinline__ float tolSetting(const PtDev* __restrict__ D) { return D->u.eps > 0.0f ? D->u.eps : D->u.epsAuto; }
+__device__ __forceinline__ float offsetTol(const PtDev* __restrict__ D, V3 p) { return 10.0f * marchTol(tolSetting(D), p); }
+
+// meta.z: bounces (8 bits) | state << 8 | flags << 16
+#define PT_F_DELTAONLY 0x1u     // every scatter so far was a delta event (SPPM gathers at the next diffuse hit)
+#define PT_F_NEEPREV 0x2u       // the previous vertex ran light sampling (prevPdf meaningful)
+__device__ __forceinline__ unsigned int mzBounces(unsigned int z) { return z & 0xFFu; }
+__device__ __forceinline__ unsigned int mzState(unsigned int z) { return (z >> 8) & 0xFFu; }
+__device__ __forceinline__ unsigned int mzFlags(unsigned int z) { return z >> 16; }
+__device__ __forceinline__ unsigned int mzPack(unsigned int b, unsigned int s, unsigned int f) { return (b & 0xFFu) | ((s & 0xFFu) << 8) | (f << 16); }
+
+// ---------------------------------------------------------------------------------------------------
+// environment (uniform sphere or hemisphere, and a sun disc), its light-sampling draw and density
+__device__ __forceinline__ float sunSolidAngle(const PtEnv& e) { return 2.0f * PT_PI * (1.0f - e.sunCos); }
+__device__ __forceinline__ bool inSun(const PtEnv& e, V3 d) { return e.sunE > 0.0f && dot(d, ld3(e.sunDir)) >= e.sunCos; }
+__device__ __forceinline__ V3 envBackground(const PtEnv& e, V3 d) {
+    if (e.type == 1u || (e.type == 2u && d.z > 0.0f)) return ld3(e.col) * e.L;
+    return v3s(0.0f);
+}
+// The sun's radiance where it is seen through its aperture (a beam of radius sunRadius about sunPoint).
+__device__ __forceinline__ V3 envSun(const PtEnv& e, V3 p, V3 d) {
+    if (!inSun(e, d)) return v3s(0.0f);
+    if (e.sunRadius > 0.0f) {
+        V3 a = ld3(e.sunDir);
+        V3 q = p - ld3(e.sunPoint);
+        V3 perp = q - a * dot(q, a);
+        if (dot(perp, perp) > e.sunRadius * e.sunRadius) return v3s(0.0f);
+    }
+    return v3s(e.sunE / sunSolidAngle(e));
+}
+__device__ __forceinline__ float sunPick(const PtEnv& e) { return e.sunE > 0.0f ? 0.5f : 0.0f; }
+__device__ __forceinline__ V3 sampleCosine(V3 n, float u1, float u2) {
+    V3 t, b; frameOf(n, t, b);
+    float r = sqrtf(u1), phi = 2.0f * PT_PI * u2;
+    float sp, cp; __sincosf(phi, &sp, &cp);
+    return normalize(t * (r * cp) + b * (r * sp) + n * sqrtf(fmaxf(0.0f, 1.0f - u1)));
+}
+__device__ __forceinline__ V3 sampleSunCone(const PtEnv& e, float u1, float u2) {
+    V3 a = ld3(e.sunDir);
+    V3 t, b; frameOf(a, t, b);
+    float c = 1.0f - u1 * (1.0f - e.sunCos);
+    float s = sqrtf(fmaxf(0.0f, 1.0f - c * c));
+    float phi = 2.0f * PT_PI * u2;
+    float sp, cp; __sincosf(phi, &sp, &cp);
+    return normalize(t * (s * cp) + b * (s * sp) + a * c);
+}
+__device__ __forceinline__ float envPdf(const PtEnv& e, V3 L, V3 n) {
+    float ps = sunPick(e);
+    float pSun = inSun(e, L) ? 1.0f / sunSolidAngle(e) : 0.0f;
+    return ps * pSun + (1.0f - ps) * fmaxf(dot(L, n), 0.0f) * PT_INV_PI;
+}
+__device__ __forceinline__ bool envLit(const PtEnv& e) { return (e.type != 0u && e.L > 0.0f) || e.sunE > 0.0f; }
+__device__ __forceinline__ float misWeight(float a, float b) { a *= a; b *= b; return a / fmaxf(a + b, 1e-20f); }
+
+// ---------------------------------------------------------------------------------------------------
+// light faces (NEE towards emissive rectangles and discs): area sampling, density in solid angle
+__device__ __forceinline__ float faceArea(const PtLightFace& f) {
+    return f.shape == 1u ? PT_PI * f.halfx * f.halfx : 4.0f * f.halfx * f.halfy;
+}
+__device__ __forceinline__ float faceShareAt(const PtDev* __restrict__ D, unsigned int bounces) {
+    if (bounces == 0u || D->nfaces == 0u || D->faceTotal <= 0.0f) return 0.0f;
+    return envLit(D->u.env) ? D->u.faceShare : 1.0f;
+}
+__device__ __forceinline__ unsigned int facePick(const PtDev* __restrict__ D, float u) {
+    float acc = 0.0f;
+    unsigned int last = 0u;
+    for (unsigned int k = 0; k < D->nfaces; ++k) {
+        float w = D->faces[k].weight;
+        if (w > 0.0f) { last = k; acc += w; if (u < acc) return k; }
+    }
+    return last;
+}
+__device__ __forceinline__ float faceDensity(const PtDev* __restrict__ D, float share, const PtLightFace& f, float dist, float cosY) {
+    if (share <= 0.0f || f.weight <= 0.0f) return 0.0f;
+    return share * (f.weight / D->faceTotal) * dist * dist / fmaxf(faceArea(f) * fabsf(cosY), 1e-20f);
+}
+// Where a ray from x along d meets face f's plane inside the face, the distance; < 0 if it does not.
+__device__ __forceinline__ float faceHitDist(const PtLightFace& f, V3 x, V3 d, float& cosY) {
+    V3 n = ld3(f.n);
+    cosY = -dot(n, d);
+    if (fabsf(cosY) < 1e-8f) return -1.0f;
+    float t = dot(ld3(f.centre) - x, n) / (-cosY);
+    if (t <= 0.0f) return -1.0f;
+    V3 q = x + d * t - ld3(f.centre);
+    float a = dot(q, ld3(f.u)), b = dot(q, ld3(f.v));
+    bool in = f.shape == 1u ? (a * a + b * b <= f.halfx * f.halfx) : (fabsf(a) <= f.halfx && fabsf(b) <= f.halfy);
+    return in ? t : -1.0f;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// deposits into the per-sample frame: refused (and counted) when not finite or of negative luminance
+// (core/wavefront.js:202-245 depositWGSL)
+__device__ __forceinline__ bool badDeposit(V3 c) {
+    return !(isfinite(c.x) && isfinite(c.y) && isfinite(c.z)) || luma(c) < 0.0f;
+}
+__device__ __forceinline__ void depositRGB(const PtDev* __restrict__ D, unsigned int pix, V3 c, bool direct) {
+    if (D->u.fireflyClamp > 0.0f && !direct) {
+        float cap = fmaxf(D->u.fireflyClamp, 1.0f) * D->u.fireflyWhite;
+        float y = luma(c);
+        if (y > cap) c = c * (cap / y);
+    }
+    float* f = D->frame + 4ull * pix;
+    if (c.x != 0.0f) unsafeAtomicAdd(f + 0, c.x);
+    if (c.y != 0.0f) unsafeAtomicAdd(f + 1, c.y);
+    if (c.z != 0.0f) unsafeAtomicAdd(f + 2, c.z);
+}
+
+// ---------------------------------------------------------------------------------------------------
+// SPECTRAL: Cauchy's n(lambda) = A + B / lambda^2 through n_d (587.6 nm) and the Abbe number
+// V_d = (n_d - 1) / (n_F - n_C) (core/spectral.js:121-128), lambda in micrometres.
+__device__ __forceinline__ float cauchyN(float nd, float abbe, float lam) {
+    const float iF = 1.0f / (0.4861f * 0.4861f), iC = 1.0f / (0.6563f * 0.6563f), id = 1.0f / (0.5876f * 0.5876f);
+    float B = (nd - 1.0f) / (abbe * (iF - iC));
+    float A = nd - B * id;
+    return A + B / (lam * lam);
+}
+// The weight of one wavelength drawn uniformly over 380-780 nm, per linear sRGB channel: the CIE 1931
+// observer (Wyman, Sloan & Shirley 2013 multi-lobe fit) to XYZ to sRGB, negative lobes clipped, normalised so
+// that its mean over the draw is (1, 1, 1): white light stays white on average.
+__device__ __forceinline__ float wssLobe(float l, float mu, float s1, float s2) {
+    float t = (l - mu) / (l < mu ? s1 : s2);
+    return __expf(-0.5f * t * t);
+}
+__device__ __forceinline__ V3 spectralWeight(float lamUm) {
+    float l = lamUm * 1000.0f;
+    float x = 1.056f * wssLobe(l, 599.8f, 37.9f, 31.0f) + 0.362f * wssLobe(l, 442.0f, 16.0f, 26.7f) - 0.065f * wssLobe(l, 501.1f, 20.4f, 26.2f);
+    float y = 0.821f * wssLobe(l, 568.8f, 46.9f, 40.5f) + 0.286f * wssLobe(l, 530.9f, 16.3f, 31.1f);
+    float z = 1.217f * wssLobe(l, 437.0f, 11.8f, 36.0f) + 0.681f * wssLobe(l, 459.0f, 26.0f, 13.8f);
+    float r = fmaxf(3.2406f * x - 1.5372f * y - 0.4986f * z, 0.0f);
+    float g = fmaxf(-0.9689f * x + 1.8758f * y + 0.0415f * z, 0.0f);
+    float b = fmaxf(0.0557f * x - 0.2040f * y + 1.0570f * z, 0.0f);
+    return v3(r * (1.0f / 0.44045557f), g * (1.0f / 0.28846547f), b * (1.0f / 0.27323972f));
+}
+
+// ---------------------------------------------------------------------------------------------------
+// HAZE: one homogeneous axis-aligned box (core/haze.js:191-290): free flight drawn where a leg starts, over
+// the leg's overlap with the box; Henyey-Greenstein phase function.
+__device__ __forceinline__ bool hazeOn(const PtDev* __restrict__ D) { return D->u.haze.sigmaS > 0.0f; }
+// The ray's span [t0, t1] inside the box; false if none ahead.
+__device__ __forceinline__ bool hazeSpan(const PtHaze& h, V3 o, V3 d, float& t0, float& t1) {
+    float lo = 0.0f, hi = 3.0e38f;
+    const float* c = h.centre;
+    const float* e = h.half;
+    float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float inv = 1.0f / dd[k];
+        float a = (c[k] - e[k] - oo[k]) * inv, b = (c[k] + e[k] - oo[k]) * inv;
+        float mn = fminf(a, b), mx = fmaxf(a, b);
+        if (dd[k] == 0.0f) { if (oo[k] < c[k] - e[k] || oo[k] > c[k] + e[k]) return false; continue; }
+        lo = fmaxf(lo, mn); hi = fminf(hi, mx);
+    }
+    t0 = lo; t1 = hi;
+    return hi > lo;
+}
+// Transmittance of the box along [0, len] of a ray.
+__device__ __forceinline__ float hazeTr(const PtHaze& h, V3 o, V3 d, float len) {
+    float t0, t1;
+    if (!hazeSpan(h, o, d, t0, t1)) return 1.0f;
+    float l = fmaxf(fminf(t1, len) - t0, 0.0f);
+    return __expf(-h.sigmaS * l);
+}
+// Draw where the leg from o along d collides: the distance (3e38: no collision before the box is left).
+__device__ __forceinline__ float hazeDraw(const PtHaze& h, V3 o, V3 d, float u) {
+    float t0, t1;
+    if (!hazeSpan(h, o, d, t0, t1)) return 3.0e38f;
+    float t = t0 - __logf(1.0f - u) / h.sigmaS;
+    return t < t1 ? t : 3.0e38f;
+}
+__device__ __forceinline__ float hgEval(float g, float cosT) {
+    float d = 1.0f + g * g - 2.0f * g * cosT;
+    return (1.0f - g * g) / (4.0f * PT_PI * d * sqrtf(d));
+}
+// Sample HG about the travel direction w (cosT between w and the new direction).
+__device__ __forceinline__ V3 hgSample(float g, V3 w, float u1, float u2) {
+    float c;
+    if (fabsf(g) < 1e-3f) c = 1.0f - 2.0f * u1;
+    else { float s = (1.0f - g * g) / (1.0f - g + 2.0f * g * u1); c = (1.0f + g * g - s * s) / (2.0f * g); }
+    c = fminf(fmaxf(c, -1.0f), 1.0f);
+    float sn = sqrtf(fmaxf(0.0f, 1.0f - c * c));
+    V3 t, b; frameOf(w, t, b);
+    float sp, cp; __sincosf(2.0f * PT_PI * u2, &sp, &cp);
+    return normalize(t * (sn * cp) + b * (sn * sp) + w * c);
+}

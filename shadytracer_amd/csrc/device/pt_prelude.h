@@ -1,0 +1,279 @@
+// Device prelude: vector math, the PCG4D generator, the SDF primitives and CSG joins the emitted scene
+// code calls, and the closest-surface fold. Compiled by hipRTC together with the emitted scene and the
+// kernels (pt_kernels.h). FP32 throughout, as the reference's WGSL.
+#pragma once
+
+#define PT_PI 3.14159265358979f
+#define PT_INV_PI 0.318309886183791f
+#define PT_F32_STEP 1.1920928955078125e-7f
+
+struct V3 { float x, y, z; };
+struct V4 { float x, y, z, w; };
+
+__device__ __forceinline__ V3 v3(float x, float y, float z) { V3 r; r.x = x; r.y = y; r.z = z; return r; }
+__device__ __forceinline__ V3 v3s(float s) { return v3(s, s, s); }
+__device__ __forceinline__ V4 v4(V3 a, float w) { V4 r; r.x = a.x; r.y = a.y; r.z = a.z; r.w = w; return r; }
+__device__ __forceinline__ V3 xyz(V4 a) { return v3(a.x, a.y, a.z); }
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ V3 operator-(V3 a) { return v3(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ V3 operator*(V3 a, V3 b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ V3 operator*(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ V3 operator*(float s, V3 a) { return v3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ V3 operator/(V3 a, V3 b) { return v3(a.x / b.x, a.y / b.y, a.z / b.z); }
+__device__ __forceinline__ V3& operator+=(V3& a, V3 b) { a = a + b; return a; }
+__device__ __forceinline__ V3& operator*=(V3& a, V3 b) { a = a * b; return a; }
+__device__ __forceinline__ V3& operator*=(V3& a, float s) { a = a * s; return a; }
+__device__ __forceinline__ V4 operator-(V4 a) { V4 r; r.x = -a.x; r.y = -a.y; r.z = -a.z; r.w = -a.w; return r; }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) { return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+__device__ __forceinline__ float length(V3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ V3 normalize(V3 a) { return a * (1.0f / sqrtf(dot(a, a))); }
+__device__ __forceinline__ V3 vabs(V3 a) { return v3(fabsf(a.x), fabsf(a.y), fabsf(a.z)); }
+__device__ __forceinline__ V3 vmax(V3 a, V3 b) { return v3(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)); }
+__device__ __forceinline__ V3 vmin(V3 a, V3 b) { return v3(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)); }
+__device__ __forceinline__ float max3(V3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
+__device__ __forceinline__ float luma(V3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+__device__ __forceinline__ float saturate(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
+__device__ __forceinline__ float signf(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
+__device__ __forceinline__ float mixf(float a, float b, float t) { return a + (b - a) * t; }
+__device__ __forceinline__ V3 vmix(V3 a, V3 b, float t) { return a + (b - a) * t; }
+__device__ __forceinline__ V3 ld3(const float* p) { return v3(p[0], p[1], p[2]); }
+
+// ---------------------------------------------------------------------------------------------------
+// PCG4D (Jarzynski & Olano, "Hash Functions for GPU Rendering", JCGT 9(3) 2020): the reference's state
+// update (core/tracebasics.js:369-380), and its output, the top 24 bits of each lane over 2^24.
+struct Rng { unsigned int a, b, c, d; };
+
+__device__ __forceinline__ void pcg4d(Rng& s) {
+    unsigned int x = s.a * 1664525u + 1013904223u, y = s.b * 1664525u + 1013904223u;
+    unsigned int z = s.c * 1664525u + 1013904223u, w = s.d * 1664525u + 1013904223u;
+    x += y * w; y += z * x; z += x * y; w += y * z;
+    x ^= x >> 16; y ^= y >> 16; z ^= z >> 16; w ^= w >> 16;
+    x += y * w; y += z * x; z += x * y; w += y * z;
+    s.a = x; s.b = y; s.c = z; s.d = w;
+}
+__device__ __forceinline__ V4 rngNext(Rng& s) {
+    pcg4d(s);
+    const float k = 1.0f / 16777216.0f;
+    V4 r; r.x = (float)(s.a >> 8) * k; r.y = (float)(s.b >> 8) * k; r.z = (float)(s.c >> 8) * k; r.w = (float)(s.d >> 8) * k;
+    return r;
+}
+// A path's generator at one event: keyed by the sample's seed, the path's slot, its bounce and the kernel,
+// never by queue position, so a path draws the same numbers whatever order the queues hold it in.
+__device__ __forceinline__ Rng rngAt(const unsigned int* seed, unsigned int slot, unsigned int bounce, unsigned int stream) {
+    Rng s; s.a = seed[0] ^ slot; s.b = seed[1] ^ (bounce * 0x9E3779B9u); s.c = seed[2] ^ stream; s.d = seed[3];
+    pcg4d(s);
+    return s;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// THE HIT TOLERANCE (core/tolerances.js:10-34): the setting, never under 1e-9 m nor under 128 f32 spacings of
+// the largest coordinate at p.
+__device__ __forceinline__ float marchTol(float setting, V3 p) {
+    return fmaxf(fmaxf(setting, 1e-9f), 128.0f * PT_F32_STEP * fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z))));
+}
+
+// ---------------------------------------------------------------------------------------------------
+// SDF primitives, (outward normal, signed distance) in the primitive's frame; the reference's N* family
+// (core/sdf.js:352-628), restated.
+__device__ __forceinline__ V4 Nsphere(V3 p, float r) {
+    float l = length(p);
+    V3 n = l > 1e-30f ? p * (1.0f / l) : v3(0.0f, 0.0f, 1.0f);
+    return v4(n, l - r);
+}
+__device__ __forceinline__ float Dsphere(V3 p, float r) { return length(p) - r; }
+
+__device__ __forceinline__ V4 Nroundedbox(V3 p, V3 b, float r) {
+    V3 q = vabs(p) - b;
+    V3 g = vmax(q, v3s(0.0f));
+    float d = length(g) + fminf(fmaxf(q.x, fmaxf(q.y, q.z)), 0.0f) - r;
+    V3 n;
+    if (g.x > 0.0f || g.y > 0.0f || g.z > 0.0f) {
+        n = normalize(g) * v3(signf(p.x), signf(p.y), signf(p.z));
+    } else if (q.x > q.y && q.x > q.z) {
+        n = v3(signf(p.x), 0.0f, 0.0f);
+    } else if (q.y > q.z) {
+        n = v3(0.0f, signf(p.y), 0.0f);
+    } else {
+        n = v3(0.0f, 0.0f, signf(p.z));
+    }
+    return v4(n, d);
+}
+__device__ __forceinline__ float Droundedbox(V3 p, V3 b, float r) {
+    V3 q = vabs(p) - b;
+    return length(vmax(q, v3s(0.0f))) + fminf(fmaxf(q.x, fmaxf(q.y, q.z)), 0.0f) - r;
+}
+// A box of half size b whose edges are rounded by r, the rounding inside the box (sdf.js NboxRounded).
+__device__ __forceinline__ V4 NboxRounded(V3 p, V3 b, float r) {
+    float rr = fminf(fmaxf(r, 0.0f), fminf(b.x, fminf(b.y, b.z)));
+    return Nroundedbox(p, b - v3s(rr), rr);
+}
+__device__ __forceinline__ float DboxRounded(V3 p, V3 b, float r) {
+    float rr = fminf(fmaxf(r, 0.0f), fminf(b.x, fminf(b.y, b.z)));
+    return Droundedbox(p, b - v3s(rr), rr);
+}
+
+// Cylinder along z: half height h, radius r (sdf.js Ncylinder).
+__device__ __forceinline__ V4 Ncylinder(V3 p, float h, float r) {
+    float qx = sqrtf(p.x * p.x + p.y * p.y), qy = fabsf(p.z);
+    float dx = qx - r, dy = qy - h;
+    float ox = fmaxf(dx, 0.0f), oy = fmaxf(dy, 0.0f);
+    float ol = sqrtf(ox * ox + oy * oy);
+    float dist = fminf(fmaxf(dx, dy), 0.0f) + ol;
+    float lxy = fmaxf(qx, 1e-8f);
+    V3 nside = v3(p.x / lxy, p.y / lxy, 0.0f);
+    V3 ncap = v3(0.0f, 0.0f, signf(p.z));
+    V3 n;
+    if (dx > 0.0f && dy > 0.0f) {
+        float il = 1.0f / fmaxf(ol, 1e-8f);
+        n = normalize(v3(nside.x * ox * il, nside.y * ox * il, 0.0f) + ncap * (oy * il));
+    } else if (dx > dy) {
+        n = nside;
+    } else {
+        n = ncap;
+    }
+    return v4(n, dist);
+}
+__device__ __forceinline__ float Dcylinder(V3 p, float h, float r) {
+    float qx = sqrtf(p.x * p.x + p.y * p.y), qy = fabsf(p.z);
+    float dx = qx - r, dy = qy - h;
+    float ox = fmaxf(dx, 0.0f), oy = fmaxf(dy, 0.0f);
+    return fminf(fmaxf(dx, dy), 0.0f) + sqrtf(ox * ox + oy * oy);
+}
+// A cylinder whose rim is rounded by rr (the rounded cylinder of the configs: the cylinder shrunk by rr, then
+// inflated by it).
+__device__ __forceinline__ V4 NroundCylinder(V3 p, float h, float r, float rr) {
+    float k = fminf(fmaxf(rr, 0.0f), fminf(h, r));
+    V4 c = Ncylinder(p, h - k, r - k);
+    c.w -= k;
+    return c;
+}
+__device__ __forceinline__ float DroundCylinder(V3 p, float h, float r, float rr) {
+    float k = fminf(fmaxf(rr, 0.0f), fminf(h, r));
+    return Dcylinder(p, h - k, r - k) - k;
+}
+
+// Torus about z: ring radius R, tube radius r (sdf.js Ntorus).
+__device__ __forceinline__ V4 Ntorus(V3 p, float R, float r) {
+    float lxy = sqrtf(p.x * p.x + p.y * p.y);
+    float rx = lxy > 1e-12f ? p.x / lxy : 1.0f, ry = lxy > 1e-12f ? p.y / lxy : 0.0f;
+    float qx = lxy - R, qy = p.z;
+    float ql = sqrtf(qx * qx + qy * qy);
+    V3 n = ql > 1e-12f ? v3(rx * qx / ql, ry * qx / ql, qy / ql) : v3(0.0f, 0.0f, 1.0f);
+    return v4(n, ql - r);
+}
+__device__ __forceinline__ float Dtorus(V3 p, float R, float r) {
+    float qx = sqrtf(p.x * p.x + p.y * p.y) - R;
+    return sqrtf(qx * qx + p.z * p.z) - r;
+}
+
+// Capsule along z: half length h of the segment, radius r (sdf.js Ncapsule).
+__device__ __forceinline__ V4 Ncapsule(V3 p, float h, float r) {
+    V3 v = p - v3(0.0f, 0.0f, fminf(fmaxf(p.z, -h), h));
+    float l = length(v);
+    V3 n = l > 1e-12f ? v * (1.0f / l) : v3(1.0f, 0.0f, 0.0f);
+    return v4(n, l - r);
+}
+__device__ __forceinline__ float Dcapsule(V3 p, float h, float r) {
+    return length(p - v3(0.0f, 0.0f, fminf(fmaxf(p.z, -h), h))) - r;
+}
+
+// The plane z = 0, solid below (sdf.js Nplane).
+__device__ __forceinline__ V4 Nplane(V3 p) { return v4(v3(0.0f, 0.0f, 1.0f), p.z); }
+__device__ __forceinline__ float Dplane(V3 p) { return p.z; }
+
+// A rectangle sheet in z = 0 of half sizes hx, hy: unsigned distance, normal away from the sheet
+// (sdf.js Nrectangle / sheetNormal).
+__device__ __forceinline__ V4 Nrectangle(V3 p, float hx, float hy) {
+    float ex = fmaxf(fabsf(p.x) - hx, 0.0f), ey = fmaxf(fabsf(p.y) - hy, 0.0f);
+    float s = p.z < 0.0f ? -1.0f : 1.0f;
+    float d = sqrtf(ex * ex + ey * ey + p.z * p.z);
+    if (fmaxf(ex, ey) <= 0.0f || d < 1e-12f) return v4(v3(0.0f, 0.0f, s), fabsf(p.z));
+    return v4(v3(ex / d * signf(p.x), ey / d * signf(p.y), p.z / d), d);
+}
+__device__ __forceinline__ float Drectangle(V3 p, float hx, float hy) {
+    float ex = fmaxf(fabsf(p.x) - hx, 0.0f), ey = fmaxf(fabsf(p.y) - hy, 0.0f);
+    return sqrtf(ex * ex + ey * ey + p.z * p.z);
+}
+// A disc sheet of radius r in z = 0.
+__device__ __forceinline__ V4 Ndisc(V3 p, float r) {
+    float l = sqrtf(p.x * p.x + p.y * p.y);
+    float e = fmaxf(l - r, 0.0f);
+    float s = p.z < 0.0f ? -1.0f : 1.0f;
+    float d = sqrtf(e * e + p.z * p.z);
+    if (e <= 0.0f || d < 1e-12f) return v4(v3(0.0f, 0.0f, s), fabsf(p.z));
+    float il = 1.0f / fmaxf(l, 1e-12f);
+    return v4(v3(e * p.x * il / d, e * p.y * il / d, p.z / d), d);
+}
+__device__ __forceinline__ float Ddisc(V3 p, float r) {
+    float e = fmaxf(sqrtf(p.x * p.x + p.y * p.y) - r, 0.0f);
+    return sqrtf(e * e + p.z * p.z);
+}
+
+// ---------------------------------------------------------------------------------------------------
+// CSG joins (scene_api.js:352-373): a k > 0 blend is the polynomial smooth min/max, the normal the gradient of
+// the blend (mix of the two), none where either side has none.
+__device__ __forceinline__ V3 csgNormal(V3 a, V3 b, float t) {
+    if (dot(a, a) < 1e-12f || dot(b, b) < 1e-12f) return v3s(0.0f);
+    V3 n = vmix(a, b, t);
+    float l = length(n);
+    return l > 1e-12f ? n * (1.0f / l) : v3s(0.0f);
+}
+__device__ __forceinline__ V4 csgUnion(V4 a, V4 b, float k) {
+    if (k <= 0.0f) return a.w < b.w ? a : b;
+    float h = fmaxf(k - fabsf(a.w - b.w), 0.0f);
+    float m = a.w < b.w ? 0.0f : 1.0f;
+    float t = m + (1.0f - 2.0f * m) * 0.5f * h / k;
+    return v4(csgNormal(xyz(a), xyz(b), t), fminf(a.w, b.w) - h * h * 0.25f / k);
+}
+__device__ __forceinline__ V4 csgIntersect(V4 a, V4 b, float k) {
+    if (k <= 0.0f) return a.w > b.w ? a : b;
+    float h = fmaxf(k - fabsf(a.w - b.w), 0.0f);
+    float m = a.w > b.w ? 0.0f : 1.0f;
+    float t = m + (1.0f - 2.0f * m) * 0.5f * h / k;
+    return v4(csgNormal(xyz(a), xyz(b), t), fmaxf(a.w, b.w) + h * h * 0.25f / k);
+}
+__device__ __forceinline__ V4 csgSubtract(V4 a, V4 b, float k) { return csgIntersect(a, -b, k); }
+__device__ __forceinline__ float dUnion(float a, float b, float k) {
+    if (k <= 0.0f) return fminf(a, b);
+    float h = fmaxf(k - fabsf(a - b), 0.0f);
+    return fminf(a, b) - h * h * 0.25f / k;
+}
+__device__ __forceinline__ float dIntersect(float a, float b, float k) {
+    if (k <= 0.0f) return fmaxf(a, b);
+    float h = fmaxf(k - fabsf(a - b), 0.0f);
+    return fmaxf(a, b) + h * h * 0.25f / k;
+}
+__device__ __forceinline__ float dSubtract(float a, float b, float k) { return dIntersect(a, -b, k); }
+
+// ---------------------------------------------------------------------------------------------------
+// A node's frame: params t[0..2] position, t[3..11] rotation (columns: local axes in world), t[12] scale.
+// Local point q = R^T (p - pos) / s; a local (n, d) goes back as (R n, d s) (scene_api.js add_new).
+__device__ __forceinline__ V3 xfIn(const float* t, V3 p) {
+    V3 d = p - v3(t[0], t[1], t[2]);
+    float is = 1.0f / t[12];
+    return v3(dot(d, v3(t[3], t[4], t[5])) * is, dot(d, v3(t[6], t[7], t[8])) * is, dot(d, v3(t[9], t[10], t[11])) * is);
+}
+__device__ __forceinline__ V4 xfOut(const float* t, V4 s) {
+    V3 n = v3(t[3], t[4], t[5]) * s.x + v3(t[6], t[7], t[8]) * s.y + v3(t[9], t[10], t[11]) * s.z;
+    return v4(n, s.w * t[12]);
+}
+
+// The closest-surface fold (scene_api.js:39-63): the closest |d| gives the surface and its material, the
+// deepest negative d the material the point is inside of.
+struct SdfHit {
+    V3 n;
+    float d;
+    unsigned int m;      // material of the closest surface
+    unsigned int a;      // material the point is inside of
+    float insideW;
+};
+__device__ __forceinline__ SdfHit sdfEmpty() {
+    SdfHit h; h.n = v3(0.0f, 0.0f, 1.0f); h.d = 1.0e7f; h.m = 0u; h.a = 0u; h.insideW = -1.0e7f; return h;
+}
+__device__ __forceinline__ void foldFull(SdfHit& h, V4 s, unsigned int mat) {
+    if (s.w < 0.0f && s.w > h.insideW) { h.insideW = s.w; h.a = mat; }
+    if (fabsf(s.w) < fabsf(h.d)) { h.n = xyz(s); h.d = s.w; h.m = mat; }
+}
+__device__ __forceinline__ void foldDist(float& best, float d) { if (fabsf(d) < fabsf(best)) best = d; }

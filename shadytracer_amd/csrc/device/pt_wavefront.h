@@ -1,0 +1,573 @@
+// The wavefront kernels proper (genrays, march, shade, occlude, resolve) and the start-up table integration.
+#pragma once
+
+// ---------------------------------------------------------------------------------------------------
+// GENRAYS (core/genrays.js:36-107): one jittered pinhole ray per tile of the raygen grid (raygenTiles, rounded
+// not floored: genrays.js:22-34), pix from the jittered uv. Also resets the queues for the sample.
+extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_genrays(const PtDev* __restrict__ D) {
+    unsigned int i = blockIdx.x * PT_BLOCK + threadIdx.x;
+    unsigned int* C = D->counters;
+    if (i == 0) {
+        C[PT_C_MARCH0] = D->nrays; C[PT_C_MARCH1] = 0u; C[PT_C_MARCH_HEAD] = 0u;
+        C[PT_C_SHADE] = 0u; C[PT_C_SHADOW] = 0u; C[PT_C_SHADOW_HEAD] = 0u;
+    }
+    if (i >= D->nrays) return;
+    unsigned int seed[4];
+    sampleSeed(D, seed);
+    Rng r = rngAt(seed, i, 0u, 0xCA3E8Au);
+    const PtCamera& cam = D->u.cam;
+    unsigned int tx = i % D->tilesX, ty = i / D->tilesX;
+    V4 rnd = rngNext(r);
+    float u = ((float)tx + 0.5f) * D->invTX + (rnd.x - 0.5f) * D->invTX;
+    float v = ((float)ty + 0.5f) * D->invTY + (rnd.y - 0.5f) * D->invTY;
+    u = fminf(fmaxf(u, 0.0f), 1.0f);
+    v = fminf(fmaxf(v, 0.0f), 1.0f);
+    float x = u * 2.0f - 1.0f, y = v * 2.0f - 1.0f;
+    V3 right = ld3(cam.right), up = ld3(cam.up), fwd = ld3(cam.fwd);
+    V3 o, d;
+    if (cam.type == 0u) {
+        o = ld3(cam.pos);
+        d = normalize(right * (x * cam.fx) + up * (y * cam.fy) - fwd);
+    } else {
+        o = ld3(cam.pos) + right * (x * cam.fx) + up * (y * cam.fy);
+        d = -fwd;
+    }
+    float fw = (float)D->u.width, fh = (float)D->u.height;
+    unsigned int px = (unsigned int)fminf(fmaxf(u * fw, 0.0f), fw - 1.0f);
+    unsigned int py = (unsigned int)fminf(fmaxf(v * fh, 0.0f), fh - 1.0f);
+    float collide = 3.0e38f;
+    if (hazeOn(D)) collide = hazeDraw(D->u.haze, o, d, rnd.z);
+    st4(D->pos, i, o.x, o.y, o.z, 0.0f);
+    st4(D->dir, i, d.x, d.y, d.z, 1.0f);
+    st4(D->beta, i, 1.0f, 1.0f, 1.0f, -1.0f);
+    st4(D->rad, i, 0.0f, 0.0f, 0.0f, 0.0f);
+    st4u(D->meta, i, make_uint4(px + py * D->u.width, 0u, mzPack(0u, PT_ACTIVE, PT_F_DELTAONLY), 0u));
+    st4(D->aux, i, 1.0f, collide, 0.0f, 0.0f);
+    D->qMarch0[i] = i;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// MARCH (core/wavefront.js:834-1005): sphere tracing by |d| * stepscale; a hit where |d| < marchTol(p); escaped
+// when outside envRadius and heading out; out of steps past maxiter or an optical length of 1000; a haze
+// collision when the leg reaches its drawn free flight. Persistent: each wave refills its idle lanes from the
+// queue (one atomic per refill) and appends events to the shade queue (one atomic per wave and step batch).
+#define PT_MARCH_STEPS 4
+extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_march(const PtDev* __restrict__ D, unsigned int qsel) {
+    unsigned int* C = D->counters;
+    if (blockIdx.x == 0 && threadIdx.x == 0) { C[PT_C_SHADOW] = 0u; C[PT_C_SHADOW_HEAD] = 0u; }
+    const unsigned int count = C[qsel ? PT_C_MARCH1 : PT_C_MARCH0];
+    const unsigned int* __restrict__ q = qsel ? D->qMarch1 : D->qMarch0;
+    const float setting = tolSetting(D);
+    const float R2 = D->u.envRadius * D->u.envRadius;
+    const float stepscale = D->u.stepscale;
+    const unsigned int maxiter = D->u.maxiter;
+    const unsigned int refill = D->u.marchRefill;
+    const float* __restrict__ U = D->params;
+
+    int slot = -1;
+    V3 p = v3s(0.0f), dir = v3s(0.0f);
+    float ltot = 0.0f, n = 1.0f, collide = 3.0e38f;
+    unsigned int steps = 0u, z = 0u;
+    bool exhausted = false;
+    unsigned long long nsteps = 0ull, nmax = 0ull;
+    for (;;) {
+        unsigned long long idle = __ballot(slot < 0);
+        if (!exhausted && (unsigned int)__popcll(idle) >= refill) {
+            unsigned int n_idle = (unsigned int)__popcll(idle);
+            int leader = __ffsll((long long)idle) - 1;
+            unsigned int base = 0u;
+            if ((int)laneId() == leader) base = atomicAdd(&C[PT_C_MARCH_HEAD], n_idle);
+            base = __shfl(base, leader);
+            if (base + n_idle >= count) exhausted = true;
+            if (slot < 0) {
+                unsigned int k = base + rankIn(idle);
+                if (k < count) {
+                    slot = (int)q[k];
+                    float4 P = ld4(D->pos, slot), Dd = ld4(D->dir, slot);
+                    uint4 M = ld4u(D->meta, slot);
+                    p = v3(P.x, P.y, P.z); ltot = P.w;
+                    dir = v3(Dd.x, Dd.y, Dd.z); n = Dd.w;
+                    steps = M.y; z = M.z;
+                    collide = D->aux[4u * slot + 1u];
+                }
+            }
+        }
+        if (__ballot(slot >= 0) == 0ull) {
+            if (exhausted) break;
+            continue;
+        }
+        unsigned int state = PT_ACTIVE;
+        bool event = false;
+        if (slot >= 0) {
+            for (int k = 0; k < PT_MARCH_STEPS; ++k) {
+                if (ltot >= collide) {
+                    // back to the collision point, to be shaded as a point in the haze
+                    float back = (ltot - collide) / n;
+                    p = p - dir * back;
+                    ltot = collide;
+                    state = PT_SCATTER;
+                    event = true;
+                    break;
+                }
+                float d = sceneDist(U, p);
+                float ea = fabsf(d);
+                float eps = marchTol(setting, p);
+                if (ltot > 1000.0f || (steps & 0xFFFFu) > maxiter) state = PT_MAXITER;
+                if (dot(p, dir) > 0.0f && dot(p, p) > R2) state = PT_ESCAPED;
+                if (ea < eps || state != PT_ACTIVE) { event = true; break; }
+                steps += 1u;
+                nsteps += 1ull;
+                float t = ea * stepscale;
+                p = p + dir * t;
+                ltot += t * n;
+            }
+        }
+        unsigned int qi = waveAppend(&C[PT_C_SHADE], event);
+        if (event) {
+            if (state == PT_MAXITER) nmax += 1ull;
+            D->qShade[qi] = (unsigned int)slot;
+            st4(D->pos, slot, p.x, p.y, p.z, ltot);
+            D->meta[4u * slot + 1u] = steps;
+            D->meta[4u * slot + 2u] = mzPack(mzBounces(z), state, mzFlags(z));
+            slot = -1;
+        }
+    }
+    statAdd(D, PT_S_STEPS, nsteps);
+    statAdd(D, PT_S_MAXITER, nmax);
+}
+
+// ---------------------------------------------------------------------------------------------------
+// SHADE (core/wavefront.js:1244-1442 with the surface kernel :1037-1147 folded in): escape to the environment
+// weighted by the last vertex's MIS; the hit's normal, facing and material; an emitter's light weighted against
+// face sampling; light sampling (faces, or the sky and sun) with the power heuristic; the BSDF's next direction;
+// the bounce cap and Russian roulette; a point in the haze scatters by Henyey-Greenstein.
+__device__ __forceinline__ V3 normalAt(const float* __restrict__ U, V3 p, float h) {
+    const float a = sceneDist(U, p + v3(h, -h, -h)), b = sceneDist(U, p + v3(-h, -h, h));
+    const float c = sceneDist(U, p + v3(-h, h, -h)), d = sceneDist(U, p + v3(h, h, h));
+    V3 g = v3(a - b - c + d, -a - b + c + d, -a + b - c + d);
+    float l = length(g);
+    return l > 0.0f ? g * (1.0f / l) : v3(0.0f, 0.0f, 1.0f);
+}
+
+struct ShadeOut { bool requeue, shadow; };
+
+__device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const PtMaterial* __restrict__ smat,
+                                             const unsigned int* seed, unsigned int slot) {
+    ShadeOut out; out.requeue = false; out.shadow = false;
+    float4 P = ld4(D->pos, slot), Dd = ld4(D->dir, slot), B = ld4(D->beta, slot), R = ld4(D->rad, slot), A = ld4(D->aux, slot);
+    uint4 M = ld4u(D->meta, slot);
+    V3 p = v3(P.x, P.y, P.z), dir = v3(Dd.x, Dd.y, Dd.z), beta = v3(B.x, B.y, B.z), rad = v3(R.x, R.y, R.z);
+    float ltot = P.w, n = Dd.w, prevPdf = B.w, vtxLtot = R.w, sunmis = A.x, collide = A.y, lambda = A.z;
+    unsigned int pix = M.x, counts = M.y, z = M.z, tag = M.w;
+    unsigned int bounces = mzBounces(z), state = mzState(z), flags = mzFlags(z);
+    const PtEnv& env = D->u.env;
+    Rng rng = rngAt(seed, slot, bounces, 0x5AADEu ^ ((counts & 0xFFFFu) << 12));
+    bool terminate = false;
+    bool direct = false;
+
+    if (state == PT_ESCAPED) {
+        V3 Le = envBackground(env, dir) + envSun(env, p, dir);
+        rad += beta * Le * sunmis;
+        direct = bounces == 0u;
+        terminate = true;
+    } else if (state == PT_MAXITER) {
+        terminate = true;
+    } else if (state == PT_SCATTER) {
+        // A POINT IN THE HAZE (core/haze.js:292-600 mediumVertex): albedo, light sampling with the phase function,
+        // a new direction from it.
+        const PtHaze& hz = D->u.haze;
+        bounces += 1u;
+        beta *= hz.albedo;
+        float fs = faceShareAt(D, bounces);
+        V4 u0 = rngNext(rng), u1 = rngNext(rng);
+        if (D->u.nee != 0u) {
+            V3 L = v3s(0.0f), le = v3s(0.0f), orig = p;
+            float pS = 0.0f, tmax = 3.0e38f;
+            if (u0.x < fs) {
+                const PtLightFace& f = D->faces[facePick(D, u1.w * D->faceTotal)];
+                V3 y = ld3(f.centre);
+                if (f.shape == 1u) {
+                    float rr = f.halfx * sqrtf(u0.y), sp, cp; __sincosf(2.0f * PT_PI * u0.z, &sp, &cp);
+                    y += (ld3(f.u) * cp + ld3(f.v) * sp) * rr;
+                } else {
+                    y += ld3(f.u) * ((2.0f * u0.y - 1.0f) * f.halfx) + ld3(f.v) * ((2.0f * u0.z - 1.0f) * f.halfy);
+                }
+                V3 dy = y - p; float dist = length(dy); L = dy * (1.0f / fmaxf(dist, 1e-20f));
+                float cosY = -dot(ld3(f.n), L);
+                if (dist > 0.0f && fabsf(cosY) > 1e-6f && (cosY > 0.0f || f.twoSided)) {
+                    pS = faceDensity(D, fs, f, dist, cosY); le = ld3(f.le);
+                    tmax = dist - 2.0f * offsetTol(D, y);
+                }
+            } else if (envLit(env)) {
+                float pick = (u0.x - fs) / fmaxf(1.0f - fs, 1e-6f);
+                L = pick < sunPick(env) ? sampleSunCone(env, u0.y, u0.z) : sampleCosine(v3(0.0f, 0.0f, 1.0f), u0.y, u0.z);
+                float pe = envPdf(env, L, v3(0.0f, 0.0f, 1.0f));
+                pS = (1.0f - fs) * pe;
+                le = envBackground(env, L) + envSun(env, p, L);
+            }
+            if (pS > 0.0f) {
+                float ph = hgEval(hz.g, dot(dir, L));
+                V3 contrib = beta * le * (ph * misWeight(pS, ph) / pS);
+                if (max3(contrib) > 1e-7f) {
+                    st4(D->shOrig, slot, orig.x, orig.y, orig.z, tmax);
+                    st4(D->shDir, slot, L.x, L.y, L.z, n);
+                    st4(D->shRad, slot, contrib.x, contrib.y, contrib.z, __uint_as_float(pix));
+                    out.shadow = true;
+                }
+            }
+        }
+        V3 nd = hgSample(hz.g, dir, u1.x, u1.y);
+        float ph = hgEval(hz.g, dot(dir, nd));
+        float pLout = (1.0f - fs) * (envLit(env) ? envPdf(env, nd, v3(0.0f, 0.0f, 1.0f)) : 0.0f);
+        sunmis = (D->u.nee != 0u && pLout > 0.0f) ? misWeight(ph, pLout) : 1.0f;
+        prevPdf = D->u.nee != 0u ? ph : -1.0f;
+        vtxLtot = ltot;
+        dir = nd;
+        flags &= ~PT_F_DELTAONLY;
+        if (bounces > D->u.maxbounce) terminate = true;
+        else {
+            collide = ltot + hazeDraw(hz, p, dir, u1.z) * n;
+            out.requeue = true;
+        }
+    } else {
+        // A SURFACE HIT
+        const float* __restrict__ U = D->params;
+        SdfHit h = sceneFull(U, p);
+        V3 ns = h.n;
+        if (dot(ns, ns) < 0.25f) ns = normalAt(U, p, marchTol(tolSetting(D), p));
+        bool inside = h.d < 0.0f;
+        V3 nrm = inside ? -ns : ns;
+        if (dot(dir, nrm) > 0.0f) {
+            // hit from behind (core/scene_api.js:127-136 pathFacing): moved on, no bounce
+            float off = offsetTol(D, p);
+            p = p + dir * off; ltot += off * n;
+            out.requeue = true;
+            atomicAdd(&D->stats[PT_S_LEAVING], 1ull);
+        } else {
+            bounces += 1u;
+            const PtMaterial& m = smat[h.m < D->nmats ? h.m : 0u];
+            V3 Le = ld3(m.emissive);
+            if (max3(Le) > 0.0f) {
+                // AN EMITTER HIT: weighted against the light sampling the previous vertex ran
+                float w = 1.0f;
+                if (m.face >= 0) {
+                    const PtLightFace& f = D->faces[m.face];
+                    float cosY = -dot(ld3(f.n), dir);
+                    if (!(cosY > 0.0f || f.twoSided)) Le = v3s(0.0f);
+                    else if (prevPdf >= 0.0f) {
+                        float dist = fmaxf(ltot - vtxLtot, 0.0f) / fmaxf(n, 1e-4f);
+                        w = misWeight(prevPdf, faceDensity(D, faceShareAt(D, bounces - 1u), f, dist, cosY));
+                    }
+                }
+                rad += beta * Le * w;
+                direct = bounces == 1u;
+                terminate = true;
+            } else {
+                // the medium beyond the surface: the material's own index going in, air (or the enclosing
+                // material) going out
+                float n0 = n;
+                float n1 = fmaxf(m.ior, 1.0f);
+                if (inside) {
+                    n1 = 1.0f;
+                    if (m.transmission > 0.0f) {
+                        SdfHit hb = sceneFull(U, p + dir * (2.0f * offsetTol(D, p)));
+                        if (hb.insideW < 0.0f && hb.a < D->nmats) n1 = fmaxf(smat[hb.a].ior, 1.0f);
+                    }
+                }
+                // DISPERSION (core/spectral.js:121-135): a wavelength drawn at the first refraction into a
+                // dispersive medium; its index n(lambda) by Cauchy through n_d and V_d, the ray's own n after
+                V4 ud = rngNext(rng);
+                bool draw = false;
+                V3 specW = v3s(1.0f);
+                float lambda0 = lambda;
+                if (m.abbe > 0.0f && m.transmission > 0.0f && !inside) {
+                    if (lambda <= 0.0f) {
+                        draw = true;
+                        lambda = 0.38f + 0.40f * ud.w;   // micrometres, uniform over the visible
+                        specW = spectralWeight(lambda);
+                    }
+                    n1 = cauchyN(n1, m.abbe, lambda);
+                }
+                Lobes lb = makeLobes(nrm, -dir, m, n0 / n1, D->mf);
+                float fs = faceShareAt(D, bounces);
+                bool neeOn = D->u.nee != 0u;
+                if (neeOn && !lobesNone(m)) {
+                    V4 u0 = rngNext(rng), u1 = rngNext(rng);
+                    V3 L = v3s(0.0f), le = v3s(0.0f);
+                    float pS = 0.0f, tmax = 3.0e38f;
+                    if (u0.x < fs) {
+                        const PtLightFace& f = D->faces[facePick(D, u1.x * D->faceTotal)];
+                        V3 y = ld3(f.centre);
+                        if (f.shape == 1u) {
+                            float rr = f.halfx * sqrtf(u0.y), sp, cp; __sincosf(2.0f * PT_PI * u0.z, &sp, &cp);
+                            y += (ld3(f.u) * cp + ld3(f.v) * sp) * rr;
+                        } else {
+                            y += ld3(f.u) * ((2.0f * u0.y - 1.0f) * f.halfx) + ld3(f.v) * ((2.0f * u0.z - 1.0f) * f.halfy);
+                        }
+                        V3 dy = y - p; float dist = length(dy); L = dy * (1.0f / fmaxf(dist, 1e-20f));
+                        float cosY = -dot(ld3(f.n), L);
+                        if (dist > 0.0f && dot(nrm, L) > 0.0f && fabsf(cosY) > 1e-6f && (cosY > 0.0f || f.twoSided)) {
+                            pS = faceDensity(D, fs, f, dist, cosY);
+                            le = ld3(f.le) * (1.0f / fmaxf(n0 * n0, 1.0f));
+                            V3 o = p + nrm * (2.0f * offsetTol(D, p));
+                            tmax = length(y - o) - 2.0f * offsetTol(D, p);
+                        }
+                    } else if (envLit(env)) {
+                        float pick = (u0.x - fs) / fmaxf(1.0f - fs, 1e-6f);
+                        L = pick < sunPick(env) ? sampleSunCone(env, u0.y, u0.z) : sampleCosine(nrm, u0.y, u0.z);
+                        if (dot(nrm, L) > 0.0f) {
+                            pS = (1.0f - fs) * envPdf(env, L, nrm);
+                            le = envBackground(env, L) + envSun(env, p, L);
+                        }
+                    }
+                    if (pS > 0.0f && max3(le) > 0.0f) {
+                        V3 f = lobesEval(lb, D->mf, L);
+                        float pB = lobesPdf(lb, L);
+                        float NoL = dot(nrm, L);
+                        V3 contrib = beta * f * le * (NoL * misWeight(pS, pB) / pS);
+                        if (max3(contrib) > 1e-7f) {
+                            V3 o = p + nrm * (2.0f * offsetTol(D, p));
+                            if (tmax < 3.0e38f) L = normalize((o + L * (tmax + 2.0f * offsetTol(D, p))) - o);
+                            st4(D->shOrig, slot, o.x, o.y, o.z, tmax);
+                            st4(D->shDir, slot, L.x, L.y, L.z, n0);
+                            st4(D->shRad, slot, contrib.x, contrib.y, contrib.z, __uint_as_float(pix));
+                            out.shadow = true;
+                        }
+                    }
+                }
+                LobeSample s = lobesSample(lb, D->mf, rngNext(rng));
+                dir = s.dir;
+                beta *= s.w;
+                if (s.transmitted) {
+                    n = n1;
+                    if (draw) beta *= specW;
+                } else if (draw) {
+                    lambda = lambda0;   // reflected: the wavelength is not the path's
+                }
+                if (!(max3(s.w) > 0.0f)) terminate = true;
+                bool misOut = neeOn && !s.isDelta;
+                float pOut = misOut ? lobesPdf(lb, dir) : 0.0f;
+                bool above = dot(nrm, dir) > 0.0f && !s.isDelta;
+                if (misOut) {
+                    float pLout = above && envLit(env) ? (1.0f - fs) * envPdf(env, dir, nrm) : 0.0f;
+                    sunmis = pLout > 0.0f ? misWeight(pOut, pLout) : 1.0f;
+                    prevPdf = above ? pOut : -1.0f;
+                } else {
+                    sunmis = 1.0f;
+                    prevPdf = -1.0f;
+                }
+                vtxLtot = ltot;
+                if (s.ev == PT_EV_EVAL) counts += 0x10000u;
+                if (!s.isDelta) flags &= ~PT_F_DELTAONLY;
+                if (bounces > D->u.maxbounce) terminate = true;
+                if (!terminate && D->u.diffuseCap != 0u && (counts >> 16) > D->u.diffuseCap) terminate = true;
+                if (!terminate && D->u.rr != 0u && bounces >= D->u.rrStart) {
+                    float pr = fminf(fmaxf(max3(beta), 0.05f), 0.95f);
+                    if (ud.x > pr) terminate = true; else beta *= 1.0f / pr;
+                }
+                if (!terminate) {
+                    float off = offsetTol(D, p);
+                    p = p + dir * off; ltot += off * n;
+                    collide = 3.0e38f;
+                    if (hazeOn(D) && n <= 1.0f) collide = ltot + hazeDraw(D->u.haze, p, dir, ud.y) * n;
+                    out.requeue = true;
+                }
+            }
+        }
+    }
+    if (terminate) {
+        out.requeue = false;
+        if (badDeposit(rad)) atomicAdd(&D->stats[PT_S_NONFINITE], 1ull);
+        else depositRGB(D, pix, rad, direct);
+        atomicAdd(reinterpret_cast<unsigned int*>(D->frame) + 4ull * pix + 3ull, 1u);
+    } else if (out.requeue) {
+        st4(D->pos, slot, p.x, p.y, p.z, ltot);
+        st4(D->dir, slot, dir.x, dir.y, dir.z, n);
+        st4(D->beta, slot, beta.x, beta.y, beta.z, prevPdf);
+        st4(D->rad, slot, rad.x, rad.y, rad.z, vtxLtot);
+        st4u(D->meta, slot, make_uint4(pix, counts, mzPack(bounces, PT_ACTIVE, flags), tag));
+        st4(D->aux, slot, sunmis, collide, lambda, 0.0f);
+    }
+    return out;
+}
+
+extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_shade(const PtDev* __restrict__ D, unsigned int qsel) {
+    __shared__ PtMaterial smat[PT_MAX_MATERIALS];
+    {
+        const unsigned int words = D->nmats * (unsigned int)(sizeof(PtMaterial) / 4);
+        const unsigned int* src = reinterpret_cast<const unsigned int*>(D->mats);
+        unsigned int* dst = reinterpret_cast<unsigned int*>(smat);
+        for (unsigned int k = threadIdx.x; k < words; k += PT_BLOCK) dst[k] = src[k];
+    }
+    __syncthreads();
+    unsigned int* C = D->counters;
+    // the queue march read this round, and its fetch head, start empty for the round after next
+    if (blockIdx.x == 0 && threadIdx.x == 0) { C[qsel ? PT_C_MARCH0 : PT_C_MARCH1] = 0u; C[PT_C_MARCH_HEAD] = 0u; }
+    const unsigned int count = C[PT_C_SHADE];
+    unsigned int seed[4];
+    sampleSeed(D, seed);
+    unsigned int* qout = qsel ? D->qMarch1 : D->qMarch0;
+    unsigned int* cout = &C[qsel ? PT_C_MARCH1 : PT_C_MARCH0];
+    unsigned long long segs = 0ull;
+    for (unsigned int base = blockIdx.x * PT_BLOCK; base < count; base += gridDim.x * PT_BLOCK) {
+        unsigned int i = base + threadIdx.x;
+        ShadeOut o; o.requeue = false; o.shadow = false;
+        unsigned int slot = 0u;
+        if (i < count) {
+            slot = D->qShade[i];
+            o = shadeOne(D, smat, seed, slot);
+            segs += 1ull;
+        }
+        unsigned int qi = waveAppend(cout, o.requeue);
+        if (o.requeue) qout[qi] = slot;
+        unsigned int si = waveAppend(&C[PT_C_SHADOW], o.shadow);
+        if (o.shadow) D->qShadow[si] = slot;
+    }
+    statAdd(D, PT_S_SEGMENTS, segs);
+}
+
+// ---------------------------------------------------------------------------------------------------
+// OCCLUDE (core/wavefront.js:1481-1692): a shadow ray is blocked where |d| < marchTol, reaches its light when it
+// leaves the scene (sky, sun) or when the sphere of free space around it covers what is left to the face; a
+// reached ray deposits its contribution, dimmed by the haze it crossed. Persistent, like march.
+extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_occlude(const PtDev* __restrict__ D) {
+    unsigned int* C = D->counters;
+    if (blockIdx.x == 0 && threadIdx.x == 0) C[PT_C_SHADE] = 0u;
+    const unsigned int count = C[PT_C_SHADOW];
+    const float setting = tolSetting(D);
+    const float R2 = D->u.envRadius * D->u.envRadius;
+    const float stepscale = D->u.stepscale;
+    const unsigned int maxiter = D->u.maxiter;
+    const unsigned int refill = D->u.marchRefill;
+    const float* __restrict__ U = D->params;
+    int slot = -1;
+    V3 p = v3s(0.0f), dir = v3s(0.0f), o = v3s(0.0f);
+    float tmax = 0.0f, trav = 0.0f;
+    unsigned int steps = 0u;
+    bool exhausted = false;
+    unsigned long long nsteps = 0ull, nsegs = 0ull;
+    for (;;) {
+        unsigned long long idle = __ballot(slot < 0);
+        if (!exhausted && (unsigned int)__popcll(idle) >= refill) {
+            unsigned int n_idle = (unsigned int)__popcll(idle);
+            int leader = __ffsll((long long)idle) - 1;
+            unsigned int base = 0u;
+            if ((int)laneId() == leader) base = atomicAdd(&C[PT_C_SHADOW_HEAD], n_idle);
+            base = __shfl(base, leader);
+            if (base + n_idle >= count) exhausted = true;
+            if (slot < 0) {
+                unsigned int k = base + rankIn(idle);
+                if (k < count) {
+                    slot = (int)D->qShadow[k];
+                    float4 Oo = ld4(D->shOrig, slot), Dd = ld4(D->shDir, slot);
+                    o = v3(Oo.x, Oo.y, Oo.z); p = o; tmax = Oo.w;
+                    dir = v3(Dd.x, Dd.y, Dd.z);
+                    trav = 0.0f; steps = 0u;
+                    nsegs += 1ull;
+                }
+            }
+        }
+        if (__ballot(slot >= 0) == 0ull) {
+            if (exhausted) break;
+            continue;
+        }
+        if (slot >= 0) {
+            int result = 0;   // 1 reached, 2 blocked
+            for (int k = 0; k < PT_MARCH_STEPS; ++k) {
+                float d = sceneDist(U, p);
+                float ea = fabsf(d);
+                if (ea < marchTol(setting, p) || steps > maxiter) { result = 2; break; }
+                if (ea >= tmax - trav || (dot(p, dir) > 0.0f && dot(p, p) > R2)) { result = 1; break; }
+                float t = ea * stepscale;
+                p = p + dir * t; trav += t;
+                steps += 1u; nsteps += 1ull;
+            }
+            if (result != 0) {
+                if (result == 1) {
+                    float4 Rr = ld4(D->shRad, slot);
+                    V3 c = v3(Rr.x, Rr.y, Rr.z);
+                    if (hazeOn(D)) c *= hazeTr(D->u.haze, o, dir, tmax < 3.0e38f ? tmax : 1.0e30f);
+                    depositRGB(D, __float_as_uint(Rr.w), c, false);
+                }
+                slot = -1;
+            }
+        }
+    }
+    statAdd(D, PT_S_SHADOWSTEPS, nsteps);
+    statAdd(D, PT_S_SHADOWSEGS, nsegs);
+}
+
+// ---------------------------------------------------------------------------------------------------
+// RESOLVE (core/wavefront.js:414-494; core/measure.js:36-57 stepMoments): the sample's mean per pixel into the
+// running mean, and the exponentially weighted moments (V, S) of its luminance that the error bar reads,
+// all in FP64; the frame cleared for the next sample.
+extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_resolve(const PtDev* __restrict__ D) {
+    unsigned int i = blockIdx.x * PT_BLOCK + threadIdx.x;
+    if (i == 0) D->counters[PT_C_SAMPLE] += 1u;
+    if (i >= D->u.width * D->u.height) return;
+    float4 fr = ld4(D->frame, i);
+    unsigned int cnt = __float_as_uint(fr.w);
+    st4(D->frame, i, 0.0f, 0.0f, 0.0f, 0.0f);
+    if (cnt == 0u) return;
+    double inv = 1.0 / (double)cnt;
+    double s0 = (double)fr.x * inv, s1 = (double)fr.y * inv, s2 = (double)fr.z * inv;
+    double* acc = D->accum + 4ull * i;
+    double prevn = acc[3];
+    if (D->u.maxaverages > 0u) prevn = fmin(prevn, (double)D->u.maxaverages);
+    double c = prevn + 1.0;
+    double m0 = acc[0], m1 = acc[1], m2 = acc[2];
+    double y = 0.2126 * s0 + 0.7152 * s1 + 0.0722 * s2;
+    double yPrev = 0.2126 * m0 + 0.7152 * m1 + 0.0722 * m2;
+    acc[0] = (m0 * prevn + s0) / c;
+    acc[1] = (m1 * prevn + s1) / c;
+    acc[2] = (m2 * prevn + s2) / c;
+    acc[3] = c;
+    double w = 1.0 / c;
+    double dlt = prevn > 0.0 ? y - yPrev : 0.0;
+    double* mo = D->moments + 2ull * i;
+    double V = mo[0], S = mo[1];
+    mo[0] = (1.0 - w) * (V + w * dlt * dlt);
+    mo[1] = (1.0 - w) * (1.0 - w) * S + w * w;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// THE ALBEDO TABLES (core/microfacet.js:245-330 prog_mf_lut): the directional albedo Ec(mu; r, n) of the GGX
+// dielectric reflection lobe, and E1(mu; r) with F = 1, by stratified visible-normal quadrature (32 x 32); then
+// the cosine-weighted averages Ec_avg = 2 int Ec mu dmu.
+extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_mf_lut(float* T) {
+    unsigned int t = blockIdx.x * PT_BLOCK + threadIdx.x;
+    const unsigned int nEc = PT_MF_NIOR * PT_MF_NR * PT_MF_NMU;
+    const unsigned int nE1 = PT_MF_NR * PT_MF_NMU;
+    if (t >= nEc + nE1) return;
+    bool e1 = t >= nEc;
+    unsigned int k = e1 ? t - nEc : t;
+    unsigned int im = k % PT_MF_NMU, ir = (k / PT_MF_NMU) % PT_MF_NR, ii = e1 ? 0u : k / (PT_MF_NMU * PT_MF_NR);
+    float mu = ((float)im + 0.5f) / PT_MF_NMU;
+    float r = (float)ir / (PT_MF_NR - 1);
+    float nrel = 1.0f + 2.0f * (float)ii / (PT_MF_NIOR - 1);
+    float a = fminf(fmaxf(r * r, PT_ALPHA_MIN), 1.0f);
+    V3 o = v3(sqrtf(fmaxf(0.0f, 1.0f - mu * mu)), 0.0f, mu);
+    float acc = 0.0f;
+    for (int sa = 0; sa < 32; ++sa) {
+        for (int sb = 0; sb < 32; ++sb) {
+            V3 h = mfSampleVNDF(o, a, ((float)sa + 0.5f) / 32.0f, ((float)sb + 0.5f) / 32.0f);
+            V3 i = reflectLocal(o, h);
+            if (i.z <= 0.0f) continue;
+            float F = e1 ? 1.0f : fresnelDielectric(dot(o, h), 1.0f / nrel);
+            acc += F * mfG2(o, i, a) / mfG1(o, a);
+        }
+    }
+    acc *= 1.0f / 1024.0f;
+    if (e1) T[PT_MF_E1 + ir * PT_MF_NMU + im] = acc;
+    else T[PT_MF_EC + (ii * PT_MF_NR + ir) * PT_MF_NMU + im] = acc;
+}
+extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_mf_lut_avg(float* T) {
+    unsigned int t = blockIdx.x * PT_BLOCK + threadIdx.x;
+    if (t >= PT_MF_NIOR * PT_MF_NR) return;
+    float acc = 0.0f;
+    for (int im = 0; im < PT_MF_NMU; ++im) {
+        float mu = ((float)im + 0.5f) / PT_MF_NMU;
+        acc += T[PT_MF_EC + t * PT_MF_NMU + im] * mu;
+    }
+    T[PT_MF_ECAVG + t] = 2.0f * acc / PT_MF_NMU;
+}

@@ -1,0 +1,574 @@
+// libpt: the host side of the C ABI (include/pt.h). It compiles each scene with hipRTC for gfx950 (device code
+// embedded at build time from csrc/device/*.h, the emitted scene spliced in), caches the code objects, owns the
+// structure-of-arrays path state in HBM and replays one captured hipGraph per sample.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <dlfcn.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pt.h"
+#include "device/pt_dev.h"
+
+namespace {
+
+#include "pt_device_src.inc"   // kPtDevicePre, kPtDevicePost: the device sources as strings
+
+const char* kKernelNames[] = {"pt_genrays", "pt_march", "pt_shade", "pt_occlude", "pt_resolve", "pt_mf_lut", "pt_mf_lut_avg"};
+enum { K_GENRAYS, K_MARCH, K_SHADE, K_OCCLUDE, K_RESOLVE, K_MFLUT, K_MFAVG, K_COUNT };
+
+thread_local std::string g_createError;
+
+uint64_t fnv1a(const void* data, size_t n, uint64_t h = 1469598103934665603ull) {
+    const unsigned char* p = static_cast<const unsigned char*>(data);
+    for (size_t i = 0; i < n; ++i) { h ^= p[i]; h *= 1099511628211ull; }
+    return h;
+}
+
+std::string libraryDir() {
+    Dl_info info;
+    if (dladdr(reinterpret_cast<void*>(&fnv1a), &info) && info.dli_fname) {
+        std::string p = info.dli_fname;
+        size_t s = p.rfind('/');
+        if (s != std::string::npos) return p.substr(0, s);
+    }
+    return ".";
+}
+
+std::string defaultCacheDir(const char* given) {
+    if (given && *given) return given;
+    const char* env = getenv("PT_KCACHE");
+    if (env && *env) return env;
+    return libraryDir() + "/kcache";
+}
+
+void mkdirs(const std::string& d) {
+    std::string cur;
+    for (size_t i = 0; i < d.size(); ++i) {
+        cur += d[i];
+        if (d[i] == '/' && cur.size() > 1) mkdir(cur.c_str(), 0755);
+    }
+    mkdir(d.c_str(), 0755);
+}
+
+bool readFile(const std::string& path, std::vector<char>& out) {
+    FILE* f = fopen(path.c_str(), "rb");
+    if (!f) return false;
+    fseek(f, 0, SEEK_END);
+    long n = ftell(f);
+    fseek(f, 0, SEEK_SET);
+    out.resize(n > 0 ? size_t(n) : 0);
+    size_t got = n > 0 ? fread(out.data(), 1, size_t(n), f) : 0;
+    fclose(f);
+    return got == out.size() && !out.empty();
+}
+
+bool writeFileAtomic(const std::string& path, const std::vector<char>& data) {
+    std::string tmp = path + ".tmp" + std::to_string(getpid());
+    FILE* f = fopen(tmp.c_str(), "wb");
+    if (!f) return false;
+    bool ok = fwrite(data.data(), 1, data.size(), f) == data.size();
+    ok = (fclose(f) == 0) && ok;
+    if (!ok) { unlink(tmp.c_str()); return false; }
+    return rename(tmp.c_str(), path.c_str()) == 0;
+}
+
+const char* kRtcOpts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+const int kRtcNopts = sizeof(kRtcOpts) / sizeof(kRtcOpts[0]);
+
+std::string fullSource(const char* scene) {
+    std::string s;
+    s.reserve(strlen(kPtDevicePre) + strlen(scene) + strlen(kPtDevicePost) + 64);
+    s += kPtDevicePre;
+    s += "\n// ---- emitted scene ----\n";
+    s += scene;
+    s += "\n// ---- end of scene ----\n";
+    s += kPtDevicePost;
+    return s;
+}
+
+// Compiles (or finds) the code object for a scene. Returns false with the log in err.
+bool compileScene(const char* scene, const std::string& cacheDir, std::vector<char>& code, std::string& path, std::string& err) {
+    std::string src = fullSource(scene);
+    uint64_t h = fnv1a(src.data(), src.size());
+    for (int i = 0; i < kRtcNopts; ++i) h = fnv1a(kRtcOpts[i], strlen(kRtcOpts[i]), h);
+    int ver_major = 0, ver_minor = 0;
+    hiprtcVersion(&ver_major, &ver_minor);
+    h = fnv1a(&ver_major, sizeof ver_major, h);
+    h = fnv1a(&ver_minor, sizeof ver_minor, h);
+    char name[64];
+    snprintf(name, sizeof name, "pt-%016llx.hsaco", (unsigned long long)h);
+    path = cacheDir + "/" + name;
+    if (readFile(path, code)) return true;
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "pt_scene.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
+        err = "hiprtcCreateProgram failed";
+        return false;
+    }
+    hiprtcResult rc = hiprtcCompileProgram(prog, kRtcNopts, kRtcOpts);
+    size_t logSize = 0;
+    hiprtcGetProgramLogSize(prog, &logSize);
+    std::string log(logSize, '\0');
+    if (logSize) hiprtcGetProgramLog(prog, &log[0]);
+    if (rc != HIPRTC_SUCCESS) {
+        err = std::string("scene compile failed: ") + hiprtcGetErrorString(rc) + "\n" + log;
+        hiprtcDestroyProgram(&prog);
+        return false;
+    }
+    size_t codeSize = 0;
+    hiprtcGetCodeSize(prog, &codeSize);
+    code.resize(codeSize);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    mkdirs(cacheDir);
+    writeFileAtomic(path, code);   // a cache miss that cannot be stored still loads
+    return true;
+}
+
+}  // namespace
+
+struct pt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::string cacheDir;
+    unsigned int cus = 256, blocksPerCu = 8;
+
+    hipModule_t module = nullptr;
+    hipFunction_t fn[K_COUNT] = {};
+    bool building = false;
+
+    PtDev host{};           // host mirror of the pointer table
+    PtDev* dev = nullptr;   // device copy
+    bool devDirty = true;
+
+    unsigned int capacity = 0;   // path slots allocated
+    unsigned int pixels = 0;     // pixels the frame/accumulators hold
+    unsigned int nparams = 0;
+
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+    unsigned int graphRounds = 0;
+
+    std::vector<void*> owned;
+};
+
+namespace {
+
+int fail(pt_ctx* c, int code, const char* fmt, ...) {
+    char buf[4096];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf; else g_createError = buf;
+    return code;
+}
+
+#define HIPCHK(c, expr)                                                                             \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess) return fail((c), PT_ERR_HIP, "%s: %s (%s:%d)", #expr,                 \
+                                          hipGetErrorString(e_), __FILE__, __LINE__);               \
+    } while (0)
+
+void freeBuf(pt_ctx* c, void*& p) {
+    if (!p) return;
+    hipFree(p);
+    for (auto& q : c->owned) if (q == p) q = nullptr;
+    p = nullptr;
+}
+
+template <typename T>
+int allocBuf(pt_ctx* c, T*& p, size_t bytes) {
+    void* raw = nullptr;
+    hipError_t e = hipMalloc(&raw, bytes ? bytes : 16);
+    if (e != hipSuccess) return fail(c, PT_ERR_HIP, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    p = static_cast<T*>(raw);
+    c->owned.push_back(raw);
+    return PT_OK;
+}
+
+void destroyGraph(pt_ctx* c) {
+    if (c->graphExec) { hipGraphExecDestroy(c->graphExec); c->graphExec = nullptr; }
+    if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
+}
+
+// The reference's raygen tile grid (core/genrays.js:22-34): rounded, so a 1-spp render is one tile per pixel.
+void raygenTiles(unsigned int N, unsigned int w, unsigned int h, unsigned int& tx, unsigned int& ty) {
+    double t = std::round(std::sqrt(double(N) * (double(w) / double(h ? h : 1))));
+    tx = t < 1 ? 1u : (unsigned int)t;
+    ty = (N + tx - 1) / tx;
+    if (ty < 1) ty = 1;
+}
+
+int ensureBuffers(pt_ctx* c, unsigned int w, unsigned int h) {
+    unsigned int px = w * h;
+    if (px == 0) return fail(c, PT_ERR_ARG, "width and height must be positive (got %u x %u)", w, h);
+    if (px > c->pixels) {
+        destroyGraph(c);
+        void* p;
+        p = c->host.frame; freeBuf(c, p);
+        p = c->host.accum; freeBuf(c, p);
+        p = c->host.moments; freeBuf(c, p);
+        int rc;
+        if ((rc = allocBuf(c, c->host.frame, size_t(px) * 16))) return rc;
+        if ((rc = allocBuf(c, c->host.accum, size_t(px) * 32))) return rc;
+        if ((rc = allocBuf(c, c->host.moments, size_t(px) * 16))) return rc;
+        HIPCHK(c, hipMemsetAsync(c->host.frame, 0, size_t(px) * 16, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->host.accum, 0, size_t(px) * 32, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->host.moments, 0, size_t(px) * 16, c->stream));
+        c->pixels = px;
+    }
+    if (px > c->capacity) {
+        destroyGraph(c);
+        float** f4[] = {&c->host.pos, &c->host.dir, &c->host.beta, &c->host.rad, &c->host.aux, &c->host.shOrig, &c->host.shDir, &c->host.shRad};
+        for (auto pp : f4) { void* p = *pp; freeBuf(c, p); *pp = nullptr; }
+        unsigned int** u1[] = {&c->host.qMarch0, &c->host.qMarch1, &c->host.qShade, &c->host.qShadow};
+        for (auto pp : u1) { void* p = *pp; freeBuf(c, p); *pp = nullptr; }
+        { void* p = c->host.meta; freeBuf(c, p); c->host.meta = nullptr; }
+        int rc;
+        for (auto pp : f4) if ((rc = allocBuf(c, *pp, size_t(px) * 16))) return rc;
+        if ((rc = allocBuf(c, c->host.meta, size_t(px) * 16))) return rc;
+        for (auto pp : u1) if ((rc = allocBuf(c, *pp, size_t(px) * 4))) return rc;
+        c->capacity = px;
+    }
+    c->host.capacity = c->capacity;
+    c->devDirty = true;
+    return PT_OK;
+}
+
+int syncDev(pt_ctx* c) {
+    if (!c->devDirty) return PT_OK;
+    HIPCHK(c, hipMemcpyAsync(c->dev, &c->host, sizeof(PtDev), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->devDirty = false;
+    return PT_OK;
+}
+
+unsigned int rounds(const pt_ctx* c) { return c->host.u.maxbounce + 1u; }
+
+hipError_t launch(pt_ctx* c, int k, unsigned int grid, unsigned int qsel) {
+    void* args1[] = {&c->dev};
+    void* args2[] = {&c->dev, &qsel};
+    bool two = (k == K_MARCH || k == K_SHADE);
+    return hipModuleLaunchKernel(c->fn[k], grid, 1, 1, 256, 1, 1, 0, c->stream, two ? args2 : args1, nullptr);
+}
+
+unsigned int gridFor(const pt_ctx* c, int k) {
+    switch (k) {
+        case K_GENRAYS: return (c->host.nrays + 255u) / 256u;
+        case K_RESOLVE: return (c->host.u.width * c->host.u.height + 255u) / 256u;
+        default: return c->host.gridBlocks;
+    }
+}
+
+// One sample's launches, in order; `events` (2 per launch) brackets each when given.
+hipError_t enqueueSample(pt_ctx* c, std::vector<hipEvent_t>* events, std::vector<int>* kinds) {
+    hipError_t e;
+    auto one = [&](int k, unsigned int qsel) -> hipError_t {
+        if (events) { hipEventRecord((*events)[kinds->size() * 2], c->stream); }
+        hipError_t r = launch(c, k, gridFor(c, k), qsel);
+        if (events) { hipEventRecord((*events)[kinds->size() * 2 + 1], c->stream); kinds->push_back(k); }
+        return r;
+    };
+    if ((e = one(K_GENRAYS, 0))) return e;
+    unsigned int R = rounds(c);
+    for (unsigned int it = 0; it < R; ++it) {
+        unsigned int in = it & 1u;
+        if ((e = one(K_MARCH, in))) return e;
+        if ((e = one(K_SHADE, in ^ 1u))) return e;
+        if ((e = one(K_OCCLUDE, 0))) return e;
+    }
+    return one(K_RESOLVE, 0);
+}
+
+int buildGraph(pt_ctx* c) {
+    if (c->graphExec && c->graphRounds == rounds(c)) return PT_OK;
+    destroyGraph(c);
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = enqueueSample(c, nullptr, nullptr);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        return fail(c, PT_ERR_HIP, "graph capture failed: %s / %s", hipGetErrorString(e), hipGetErrorString(e2));
+    }
+    c->graph = g;
+    HIPCHK(c, hipGraphInstantiate(&c->graphExec, g, nullptr, nullptr, 0));
+    c->graphRounds = rounds(c);
+    return PT_OK;
+}
+
+void setDefaults(PtUniforms& u) {
+    memset(&u, 0, sizeof u);
+    u.width = 256; u.height = 256;
+    u.maxbounce = 10; u.maxiter = 400; u.stepscale = 0.9f;
+    u.eps = 0.0f; u.epsAuto = 1e-4f; u.envRadius = 400.0f;
+    u.nee = 1; u.rr = 0; u.rrStart = 3; u.diffuseCap = 0;
+    u.fireflyClamp = 0.0f; u.fireflyWhite = 1.0f; u.maxaverages = 0; u.seed = 20261203u;
+    u.cam.pos[1] = -3.0f; u.cam.pos[2] = 1.5f;
+    u.cam.right[0] = 1.0f; u.cam.up[2] = 1.0f; u.cam.fwd[1] = -1.0f;
+    u.cam.fx = 0.57735f; u.cam.fy = 0.57735f;
+    u.faceShare = 0.5f;
+    u.marchRefill = 16;
+    u.guideFrac = 0.5f;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pt_abi_version(void) { return PT_ABI_VERSION; }
+
+size_t pt_struct_size(int which) {
+    switch (which) {
+        case 0: return sizeof(PtUniforms);
+        case 1: return sizeof(PtMaterial);
+        case 2: return sizeof(PtLightFace);
+        case 3: return sizeof(PtCamera);
+        case 4: return sizeof(PtEnv);
+        case 5: return sizeof(PtHaze);
+        case 6: return sizeof(pt_scene_desc);
+        case 7: return sizeof(pt_opts);
+        default: return 0;
+    }
+}
+
+const char* pt_last_error(const pt_ctx* ctx) { return ctx ? ctx->err.c_str() : g_createError.c_str(); }
+
+int pt_compile_scene(const char* hip_src, const char* cache_dir, char* path_out, size_t path_cap) {
+    if (!hip_src) return fail(nullptr, PT_ERR_ARG, "hip_src is NULL");
+    std::vector<char> code;
+    std::string path, err;
+    if (!compileScene(hip_src, defaultCacheDir(cache_dir), code, path, err)) return fail(nullptr, PT_ERR_COMPILE, "%s", err.c_str());
+    if (path_out && path_cap) snprintf(path_out, path_cap, "%s", path.c_str());
+    return PT_OK;
+}
+
+int pt_create(pt_ctx** out, const pt_opts* opts) {
+    if (!out) return fail(nullptr, PT_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    pt_ctx* c = new pt_ctx();
+    c->device = opts ? opts->device : 0;
+    c->cacheDir = defaultCacheDir(opts ? opts->cache_dir : nullptr);
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= c->device) {
+        int rc = fail(nullptr, PT_ERR_HIP, "no HIP device %d (%d found): %s", c->device, ndev, hipGetErrorString(e));
+        delete c;
+        return rc;
+    }
+    if ((e = hipSetDevice(c->device)) != hipSuccess || (e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) {
+        int rc = fail(nullptr, PT_ERR_HIP, "device setup: %s", hipGetErrorString(e));
+        delete c;
+        return rc;
+    }
+    int cus = 0;
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
+    c->cus = cus > 0 ? (unsigned int)cus : 256u;
+    c->blocksPerCu = (opts && opts->blocks_per_cu) ? opts->blocks_per_cu : 8u;
+    setDefaults(c->host.u);
+    c->host.gridBlocks = c->cus * c->blocksPerCu;
+    int rc;
+    if ((rc = allocBuf(c, c->dev, sizeof(PtDev))) || (rc = allocBuf(c, c->host.counters, PT_C_WORDS * 4)) ||
+        (rc = allocBuf(c, c->host.stats, PT_S_WORDS * 8)) || (rc = allocBuf(c, c->host.mf, PT_MF_WORDS * 4)) ||
+        (rc = allocBuf(c, c->host.mats, PT_MAX_MATERIALS * sizeof(PtMaterial))) ||
+        (rc = allocBuf(c, c->host.faces, PT_MAX_FACES * sizeof(PtLightFace)))) {
+        g_createError = c->err;
+        pt_destroy(c);
+        return rc;
+    }
+    hipMemsetAsync(c->host.counters, 0, PT_C_WORDS * 4, c->stream);
+    hipMemsetAsync(c->host.stats, 0, PT_S_WORDS * 8, c->stream);
+    if ((rc = ensureBuffers(c, c->host.u.width, c->host.u.height))) {
+        g_createError = c->err;
+        pt_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return PT_OK;
+}
+
+void pt_destroy(pt_ctx* c) {
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    destroyGraph(c);
+    for (void* p : c->owned) if (p) hipFree(p);
+    if (c->module) hipModuleUnload(c->module);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int pt_set_scene(pt_ctx* c, const pt_scene_desc* s) {
+    if (!c || !s || !s->hip_src) return fail(c, PT_ERR_ARG, "scene or its hip_src is NULL");
+    if (s->nmaterials == 0 || s->nmaterials > PT_MAX_MATERIALS) return fail(c, PT_ERR_ARG, "materials: %u (1..%d)", s->nmaterials, PT_MAX_MATERIALS);
+    if (s->nfaces > PT_MAX_FACES) return fail(c, PT_ERR_ARG, "light faces: %u (max %d)", s->nfaces, PT_MAX_FACES);
+    if (s->nparams && !s->params) return fail(c, PT_ERR_ARG, "params is NULL");
+    for (uint32_t i = 0; i < s->nmaterials; ++i)
+        if (s->materials[i].face >= (int)s->nfaces) return fail(c, PT_ERR_ARG, "material %u names light face %d of %u", i, s->materials[i].face, s->nfaces);
+    c->building = true;
+    std::vector<char> code;
+    std::string path, err;
+    if (!compileScene(s->hip_src, c->cacheDir, code, path, err)) {
+        c->building = false;
+        return fail(c, PT_ERR_COMPILE, "%s", err.c_str());
+    }
+    hipModule_t mod = nullptr;
+    hipError_t e = hipModuleLoadData(&mod, code.data());
+    if (e != hipSuccess) { c->building = false; return fail(c, PT_ERR_COMPILE, "hipModuleLoadData(%s): %s", path.c_str(), hipGetErrorString(e)); }
+    hipFunction_t fns[K_COUNT];
+    for (int k = 0; k < K_COUNT; ++k) {
+        if ((e = hipModuleGetFunction(&fns[k], mod, kKernelNames[k])) != hipSuccess) {
+            hipModuleUnload(mod);
+            c->building = false;
+            return fail(c, PT_ERR_COMPILE, "kernel %s missing: %s", kKernelNames[k], hipGetErrorString(e));
+        }
+    }
+    hipStreamSynchronize(c->stream);
+    destroyGraph(c);
+    if (c->module) hipModuleUnload(c->module);
+    c->module = mod;
+    memcpy(c->fn, fns, sizeof fns);
+    c->building = false;
+    // tables
+    if (s->nparams > c->nparams) {
+        void* p = (void*)c->host.params; freeBuf(c, p); c->host.params = nullptr;
+        float* np = nullptr;
+        int rc;
+        if ((rc = allocBuf(c, np, size_t(s->nparams) * 4))) return rc;
+        c->host.params = np;
+    }
+    c->nparams = s->nparams;
+    if (!c->host.params) { float* np = nullptr; int rc; if ((rc = allocBuf(c, np, 16))) return rc; c->host.params = np; }
+    if (s->nparams) HIPCHK(c, hipMemcpyAsync((void*)c->host.params, s->params, size_t(s->nparams) * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync((void*)c->host.mats, s->materials, size_t(s->nmaterials) * sizeof(PtMaterial), hipMemcpyHostToDevice, c->stream));
+    if (s->nfaces) HIPCHK(c, hipMemcpyAsync((void*)c->host.faces, s->faces, size_t(s->nfaces) * sizeof(PtLightFace), hipMemcpyHostToDevice, c->stream));
+    c->host.nmats = s->nmaterials;
+    c->host.nfaces = s->nfaces;
+    float tot = 0.0f;
+    for (uint32_t i = 0; i < s->nfaces; ++i) tot += s->faces[i].weight > 0.0f ? s->faces[i].weight : 0.0f;
+    c->host.faceTotal = tot;
+    c->devDirty = true;
+    if (!c->host.mfReady) {
+        void* args[] = {&c->host.mf};
+        unsigned int n = PT_MF_NIOR * PT_MF_NR * PT_MF_NMU + PT_MF_NR * PT_MF_NMU;
+        HIPCHK(c, hipModuleLaunchKernel(c->fn[K_MFLUT], (n + 255) / 256, 1, 1, 256, 1, 1, 0, c->stream, args, nullptr));
+        HIPCHK(c, hipModuleLaunchKernel(c->fn[K_MFAVG], (PT_MF_NIOR * PT_MF_NR + 255) / 256, 1, 1, 256, 1, 1, 0, c->stream, args, nullptr));
+        c->host.mfReady = 1;
+    }
+    return pt_clear_accum(c);
+}
+
+int pt_set_params(pt_ctx* c, const float* params, uint32_t n) {
+    if (!c || (n && !params)) return fail(c, PT_ERR_ARG, "params is NULL");
+    if (!c->module) return fail(c, PT_ERR_STATE, "no scene loaded");
+    if (n != c->nparams) return fail(c, PT_ERR_ARG, "params: %u values, the scene has %u", n, c->nparams);
+    if (n) HIPCHK(c, hipMemcpyAsync((void*)c->host.params, params, size_t(n) * 4, hipMemcpyHostToDevice, c->stream));
+    return PT_OK;
+}
+
+int pt_set_uniforms(pt_ctx* c, const PtUniforms* u, size_t size) {
+    if (!c || !u) return fail(c, PT_ERR_ARG, "uniforms is NULL");
+    if (size != sizeof(PtUniforms)) return fail(c, PT_ERR_ARG, "uniforms: %zu bytes, expected %zu", size, sizeof(PtUniforms));
+    if (u->width == 0 || u->height == 0 || u->width > 16384 || u->height > 16384) return fail(c, PT_ERR_ARG, "size %u x %u out of range", u->width, u->height);
+    if (u->maxbounce > 64) return fail(c, PT_ERR_ARG, "maxbounce %u above 64", u->maxbounce);
+    if (u->marchRefill == 0 || u->marchRefill > 64) return fail(c, PT_ERR_ARG, "marchRefill %u not in 1..64", u->marchRefill);
+    if (!(u->stepscale > 0.0f && u->stepscale <= 1.0f)) return fail(c, PT_ERR_ARG, "stepscale %g not in (0, 1]", u->stepscale);
+    int rc;
+    if ((rc = ensureBuffers(c, u->width, u->height))) return rc;
+    bool resized = u->width != c->host.u.width || u->height != c->host.u.height;
+    c->host.u = *u;
+    unsigned int N = u->width * u->height;
+    raygenTiles(N, u->width, u->height, c->host.tilesX, c->host.tilesY);
+    c->host.invTX = 1.0f / float(c->host.tilesX);
+    c->host.invTY = 1.0f / float(c->host.tilesY);
+    c->host.nrays = N;
+    c->devDirty = true;
+    if (resized) return pt_clear_accum(c);
+    return PT_OK;
+}
+
+int pt_clear_accum(pt_ctx* c) {
+    if (!c) return fail(c, PT_ERR_ARG, "ctx is NULL");
+    HIPCHK(c, hipMemsetAsync(c->host.frame, 0, size_t(c->pixels) * 16, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->host.accum, 0, size_t(c->pixels) * 32, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->host.moments, 0, size_t(c->pixels) * 16, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->host.counters, 0, PT_C_WORDS * 4, c->stream));
+    return PT_OK;
+}
+
+int pt_trace(pt_ctx* c, uint32_t samples) {
+    if (!c) return fail(c, PT_ERR_ARG, "ctx is NULL");
+    if (c->building) return fail(c, PT_ERR_STATE, "trace during a scene build");
+    if (!c->module) return fail(c, PT_ERR_STATE, "no scene loaded");
+    int rc;
+    if ((rc = syncDev(c)) || (rc = buildGraph(c))) return rc;
+    for (uint32_t s = 0; s < samples; ++s) HIPCHK(c, hipGraphLaunch(c->graphExec, c->stream));
+    return PT_OK;
+}
+
+int pt_sync(pt_ctx* c) {
+    if (!c) return fail(c, PT_ERR_ARG, "ctx is NULL");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PT_OK;
+}
+
+int pt_read_accum(pt_ctx* c, double* accum, double* moments, size_t npx) {
+    if (!c) return fail(c, PT_ERR_ARG, "ctx is NULL");
+    size_t px = size_t(c->host.u.width) * c->host.u.height;
+    if (npx != px) return fail(c, PT_ERR_ARG, "npx %zu, the image has %zu pixels", npx, px);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (accum) HIPCHK(c, hipMemcpy(accum, c->host.accum, px * 32, hipMemcpyDeviceToHost));
+    if (moments) HIPCHK(c, hipMemcpy(moments, c->host.moments, px * 16, hipMemcpyDeviceToHost));
+    return PT_OK;
+}
+
+int pt_read_stats(pt_ctx* c, uint64_t* stats, size_t n) {
+    if (!c || !stats || n > PT_S_WORDS) return fail(c, PT_ERR_ARG, "stats: NULL or n > %d", PT_S_WORDS);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(stats, c->host.stats, n * 8, hipMemcpyDeviceToHost));
+    return PT_OK;
+}
+
+int pt_reset_stats(pt_ctx* c) {
+    if (!c) return fail(c, PT_ERR_ARG, "ctx is NULL");
+    HIPCHK(c, hipMemsetAsync(c->host.stats, 0, PT_S_WORDS * 8, c->stream));
+    return PT_OK;
+}
+
+int pt_time_kernels(pt_ctx* c, uint32_t samples, double* ms, size_t n) {
+    if (!c || !ms || n < 5) return fail(c, PT_ERR_ARG, "ms: NULL or fewer than 5 slots");
+    if (!c->module) return fail(c, PT_ERR_STATE, "no scene loaded");
+    int rc;
+    if ((rc = syncDev(c))) return rc;
+    for (size_t k = 0; k < n; ++k) ms[k] = 0.0;
+    size_t launches = 2 + 3 * size_t(rounds(c));
+    std::vector<hipEvent_t> ev(2 * launches);
+    for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
+    for (uint32_t s = 0; s < samples; ++s) {
+        std::vector<int> kinds;
+        hipError_t e = enqueueSample(c, &ev, &kinds);
+        if (e != hipSuccess) { for (auto& x : ev) hipEventDestroy(x); return fail(c, PT_ERR_HIP, "launch: %s", hipGetErrorString(e)); }
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (size_t i = 0; i < kinds.size(); ++i) {
+            float t = 0.0f;
+            hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]);
+            int slot = kinds[i] == K_GENRAYS ? 0 : kinds[i] == K_MARCH ? 1 : kinds[i] == K_SHADE ? 2 : kinds[i] == K_OCCLUDE ? 3 : 4;
+            ms[slot] += t;
+        }
+    }
+    for (auto& e : ev) hipEventDestroy(e);
+    return PT_OK;
+}
+
+}  // extern "C"

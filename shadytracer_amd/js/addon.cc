@@ -1,0 +1,312 @@
+// The Node N-API addon: libpt's C ABI (include/pt.h) as JavaScript functions, one to one. Structures cross as
+// ArrayBuffers that the JS shim packs (raytrace_main.js); trace returns a Promise resolved on a worker thread
+// once the GPU has finished, so a trace never blocks the event loop (the reference's step() resolves on
+// queue.onSubmittedWorkDone, lib/pathtracer/index.js:1005-1007).
+#include <node_api.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+
+#include "pt.h"
+
+namespace {
+
+#define NAPI_OK(env, call)                                                              \
+    do {                                                                                \
+        if ((call) != napi_ok) {                                                        \
+            napi_throw_error((env), nullptr, "N-API call failed: " #call);              \
+            return nullptr;                                                             \
+        }                                                                               \
+    } while (0)
+
+napi_value throwPt(napi_env env, pt_ctx* ctx, int code) {
+    napi_value msg, err, codeV, restored;
+    std::string text = std::string("libpt: ") + pt_last_error(ctx);
+    napi_create_string_utf8(env, text.c_str(), text.size(), &msg);
+    napi_create_error(env, nullptr, msg, &err);
+    napi_create_int32(env, code, &codeV);
+    napi_set_named_property(env, err, "code", codeV);
+    napi_get_boolean(env, code == PT_ERR_COMPILE, &restored);
+    napi_set_named_property(env, err, "restored", restored);
+    napi_throw(env, err);
+    return nullptr;
+}
+
+bool args(napi_env env, napi_callback_info info, size_t want, napi_value* out) {
+    size_t n = want;
+    if (napi_get_cb_info(env, info, &n, out, nullptr, nullptr) != napi_ok || n < want) {
+        napi_throw_type_error(env, nullptr, "too few arguments");
+        return false;
+    }
+    return true;
+}
+
+pt_ctx* ctxOf(napi_env env, napi_value v) {
+    void* p = nullptr;
+    if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+        napi_throw_type_error(env, nullptr, "not an engine handle (or destroyed)");
+        return nullptr;
+    }
+    return *static_cast<pt_ctx**>(p);
+}
+
+bool bytesOf(napi_env env, napi_value v, void** data, size_t* len) {
+    bool isAB = false, isTA = false;
+    napi_is_arraybuffer(env, v, &isAB);
+    if (isAB) return napi_get_arraybuffer_info(env, v, data, len) == napi_ok;
+    napi_is_typedarray(env, v, &isTA);
+    if (isTA) {
+        napi_typedarray_type t;
+        size_t n, off;
+        napi_value ab;
+        if (napi_get_typedarray_info(env, v, &t, &n, data, &ab, &off) != napi_ok) return false;
+        size_t el = (t == napi_float64_array || t == napi_bigint64_array || t == napi_biguint64_array) ? 8
+                  : (t == napi_float32_array || t == napi_int32_array || t == napi_uint32_array) ? 4
+                  : (t == napi_int16_array || t == napi_uint16_array) ? 2 : 1;
+        *len = n * el;
+        return true;
+    }
+    napi_throw_type_error(env, nullptr, "expected an ArrayBuffer or a typed array");
+    return false;
+}
+
+napi_value num(napi_env env, double v) { napi_value r; napi_create_double(env, v, &r); return r; }
+napi_value undef(napi_env env) { napi_value r; napi_get_undefined(env, &r); return r; }
+
+std::string str(napi_env env, napi_value v) {
+    size_t n = 0;
+    napi_valuetype t;
+    napi_typeof(env, v, &t);
+    if (t != napi_string) return std::string();
+    napi_get_value_string_utf8(env, v, nullptr, 0, &n);
+    std::string s(n, '\0');
+    napi_get_value_string_utf8(env, v, &s[0], n + 1, &n);
+    return s;
+}
+
+void finalizeHandle(napi_env, void* data, void*) {
+    pt_ctx** box = static_cast<pt_ctx**>(data);
+    if (*box) pt_destroy(*box);
+    delete box;
+}
+
+// create(device, blocksPerCu, cacheDir) -> handle
+napi_value Create(napi_env env, napi_callback_info info) {
+    napi_value a[3];
+    if (!args(env, info, 3, a)) return nullptr;
+    int32_t dev = 0; uint32_t bpc = 0;
+    napi_get_value_int32(env, a[0], &dev);
+    napi_get_value_uint32(env, a[1], &bpc);
+    std::string cache = str(env, a[2]);
+    pt_opts o;
+    memset(&o, 0, sizeof o);
+    o.device = dev; o.blocks_per_cu = bpc; o.cache_dir = cache.empty() ? nullptr : cache.c_str();
+    pt_ctx* ctx = nullptr;
+    int rc = pt_create(&ctx, &o);
+    if (rc) return throwPt(env, nullptr, rc);
+    pt_ctx** box = new pt_ctx*(ctx);
+    napi_value h;
+    NAPI_OK(env, napi_create_external(env, box, finalizeHandle, nullptr, &h));
+    return h;
+}
+
+napi_value Destroy(napi_env env, napi_callback_info info) {
+    napi_value a[1];
+    if (!args(env, info, 1, a)) return nullptr;
+    void* p = nullptr;
+    napi_get_value_external(env, a[0], &p);
+    pt_ctx** box = static_cast<pt_ctx**>(p);
+    if (box && *box) { pt_destroy(*box); *box = nullptr; }
+    return undef(env);
+}
+
+napi_value AbiVersion(napi_env env, napi_callback_info) { return num(env, pt_abi_version()); }
+
+napi_value StructSize(napi_env env, napi_callback_info info) {
+    napi_value a[1];
+    if (!args(env, info, 1, a)) return nullptr;
+    int32_t w = 0;
+    napi_get_value_int32(env, a[0], &w);
+    return num(env, (double)pt_struct_size(w));
+}
+
+// compileScene(src, cacheDir) -> path (no GPU)
+napi_value CompileScene(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    if (!args(env, info, 2, a)) return nullptr;
+    std::string src = str(env, a[0]), cache = str(env, a[1]);
+    char path[4096];
+    int rc = pt_compile_scene(src.c_str(), cache.empty() ? nullptr : cache.c_str(), path, sizeof path);
+    if (rc) return throwPt(env, nullptr, rc);
+    napi_value r;
+    napi_create_string_utf8(env, path, strlen(path), &r);
+    return r;
+}
+
+// setScene(h, src, paramsF32, materialsBuf, nmat, facesBuf, nfaces)
+napi_value SetScene(napi_env env, napi_callback_info info) {
+    napi_value a[7];
+    if (!args(env, info, 7, a)) return nullptr;
+    pt_ctx* ctx = ctxOf(env, a[0]);
+    if (!ctx) return nullptr;
+    std::string src = str(env, a[1]);
+    void *pp, *mp, *fp; size_t pl, ml, fl;
+    if (!bytesOf(env, a[2], &pp, &pl) || !bytesOf(env, a[3], &mp, &ml) || !bytesOf(env, a[5], &fp, &fl)) return nullptr;
+    uint32_t nm = 0, nf = 0;
+    napi_get_value_uint32(env, a[4], &nm);
+    napi_get_value_uint32(env, a[6], &nf);
+    if (ml < nm * sizeof(PtMaterial) || fl < nf * sizeof(PtLightFace)) {
+        napi_throw_range_error(env, nullptr, "material or face buffer shorter than its count");
+        return nullptr;
+    }
+    pt_scene_desc d;
+    d.hip_src = src.c_str();
+    d.params = static_cast<const float*>(pp); d.nparams = (uint32_t)(pl / 4);
+    d.materials = static_cast<const PtMaterial*>(mp); d.nmaterials = nm;
+    d.faces = static_cast<const PtLightFace*>(fp); d.nfaces = nf;
+    int rc = pt_set_scene(ctx, &d);
+    if (rc) return throwPt(env, ctx, rc);
+    return undef(env);
+}
+
+napi_value SetParams(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    if (!args(env, info, 2, a)) return nullptr;
+    pt_ctx* ctx = ctxOf(env, a[0]);
+    if (!ctx) return nullptr;
+    void* p; size_t l;
+    if (!bytesOf(env, a[1], &p, &l)) return nullptr;
+    int rc = pt_set_params(ctx, static_cast<const float*>(p), (uint32_t)(l / 4));
+    if (rc) return throwPt(env, ctx, rc);
+    return undef(env);
+}
+
+napi_value SetUniforms(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    if (!args(env, info, 2, a)) return nullptr;
+    pt_ctx* ctx = ctxOf(env, a[0]);
+    if (!ctx) return nullptr;
+    void* p; size_t l;
+    if (!bytesOf(env, a[1], &p, &l)) return nullptr;
+    int rc = pt_set_uniforms(ctx, static_cast<const PtUniforms*>(p), l);
+    if (rc) return throwPt(env, ctx, rc);
+    return undef(env);
+}
+
+napi_value ClearAccum(napi_env env, napi_callback_info info) {
+    napi_value a[1];
+    if (!args(env, info, 1, a)) return nullptr;
+    pt_ctx* ctx = ctxOf(env, a[0]);
+    if (!ctx) return nullptr;
+    int rc = pt_clear_accum(ctx);
+    if (rc) return throwPt(env, ctx, rc);
+    return undef(env);
+}
+
+struct TraceWork {
+    napi_async_work work;
+    napi_deferred deferred;
+    pt_ctx* ctx;
+    uint32_t samples;
+    int rc;
+    std::string err;
+};
+
+void traceExecute(napi_env, void* data) {
+    TraceWork* w = static_cast<TraceWork*>(data);
+    w->rc = pt_trace(w->ctx, w->samples);
+    if (!w->rc) w->rc = pt_sync(w->ctx);
+    if (w->rc) w->err = pt_last_error(w->ctx);
+}
+
+void traceComplete(napi_env env, napi_status, void* data) {
+    TraceWork* w = static_cast<TraceWork*>(data);
+    if (w->rc) {
+        napi_value msg, err;
+        std::string text = "libpt: " + w->err;
+        napi_create_string_utf8(env, text.c_str(), text.size(), &msg);
+        napi_create_error(env, nullptr, msg, &err);
+        napi_reject_deferred(env, w->deferred, err);
+    } else {
+        napi_resolve_deferred(env, w->deferred, undef(env));
+    }
+    napi_delete_async_work(env, w->work);
+    delete w;
+}
+
+// trace(h, samples) -> Promise, resolved once the GPU has run them
+napi_value Trace(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    if (!args(env, info, 2, a)) return nullptr;
+    pt_ctx* ctx = ctxOf(env, a[0]);
+    if (!ctx) return nullptr;
+    TraceWork* w = new TraceWork();
+    w->ctx = ctx;
+    napi_get_value_uint32(env, a[1], &w->samples);
+    napi_value promise, name;
+    NAPI_OK(env, napi_create_promise(env, &w->deferred, &promise));
+    napi_create_string_utf8(env, "pt_trace", NAPI_AUTO_LENGTH, &name);
+    NAPI_OK(env, napi_create_async_work(env, nullptr, name, traceExecute, traceComplete, w, &w->work));
+    NAPI_OK(env, napi_queue_async_work(env, w->work));
+    return promise;
+}
+
+// readAccum(h, f64accum | null, f64moments | null)
+napi_value ReadAccum(napi_env env, napi_callback_info info) {
+    napi_value a[3];
+    if (!args(env, info, 3, a)) return nullptr;
+    pt_ctx* ctx = ctxOf(env, a[0]);
+    if (!ctx) return nullptr;
+    void *ap = nullptr, *mp = nullptr; size_t al = 0, ml = 0;
+    napi_valuetype t;
+    napi_typeof(env, a[1], &t);
+    if (t != napi_null && !bytesOf(env, a[1], &ap, &al)) return nullptr;
+    napi_typeof(env, a[2], &t);
+    if (t != napi_null && !bytesOf(env, a[2], &mp, &ml)) return nullptr;
+    size_t npx = ap ? al / 32 : ml / 16;
+    if (ap && mp && al / 32 != ml / 16) { napi_throw_range_error(env, nullptr, "accum and moments sizes disagree"); return nullptr; }
+    int rc = pt_read_accum(ctx, static_cast<double*>(ap), static_cast<double*>(mp), npx);
+    if (rc) return throwPt(env, ctx, rc);
+    return undef(env);
+}
+
+// readStats(h) -> Float64Array of PT_S_WORDS counts
+napi_value ReadStats(napi_env env, napi_callback_info info) {
+    napi_value a[1];
+    if (!args(env, info, 1, a)) return nullptr;
+    pt_ctx* ctx = ctxOf(env, a[0]);
+    if (!ctx) return nullptr;
+    uint64_t s[PT_S_WORDS];
+    int rc = pt_read_stats(ctx, s, PT_S_WORDS);
+    if (rc) return throwPt(env, ctx, rc);
+    napi_value ab, arr;
+    void* data;
+    NAPI_OK(env, napi_create_arraybuffer(env, PT_S_WORDS * 8, &data, &ab));
+    for (int i = 0; i < PT_S_WORDS; ++i) static_cast<double*>(data)[i] = (double)s[i];
+    NAPI_OK(env, napi_create_typedarray(env, napi_float64_array, PT_S_WORDS, ab, 0, &arr));
+    return arr;
+}
+
+napi_value Init(napi_env env, napi_value exports) {
+    napi_property_descriptor props[] = {
+        {"create", nullptr, Create, nullptr, nullptr, nullptr, napi_default, nullptr},
+        {"destroy", nullptr, Destroy, nullptr, nullptr, nullptr, napi_default, nullptr},
+        {"abiVersion", nullptr, AbiVersion, nullptr, nullptr, nullptr, napi_default, nullptr},
+        {"structSize", nullptr, StructSize, nullptr, nullptr, nullptr, napi_default, nullptr},
+        {"compileScene", nullptr, CompileScene, nullptr, nullptr, nullptr, napi_default, nullptr},
+        {"setScene", nullptr, SetScene, nullptr, nullptr, nullptr, napi_default, nullptr},
+        {"setParams", nullptr, SetParams, nullptr, nullptr, nullptr, napi_default, nullptr},
+        {"setUniforms", nullptr, SetUniforms, nullptr, nullptr, nullptr, napi_default, nullptr},
+        {"clearAccum", nullptr, ClearAccum, nullptr, nullptr, nullptr, napi_default, nullptr},
+        {"trace", nullptr, Trace, nullptr, nullptr, nullptr, napi_default, nullptr},
+        {"readAccum", nullptr, ReadAccum, nullptr, nullptr, nullptr, napi_default, nullptr},
+        {"readStats", nullptr, ReadStats, nullptr, nullptr, nullptr, napi_default, nullptr},
+    };
+    napi_define_properties(env, exports, sizeof props / sizeof props[0], props);
+    return exports;
+}
+
+}  // namespace
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, Init)

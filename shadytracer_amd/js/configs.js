@@ -1,0 +1,179 @@
+'use strict'
+/*
+ * The benchmark and parity scenes as builder documents (format 'raytracer-scene', version 31), built from a seed.
+ * World Z-up, metres. Each returns { doc, camera, settings }: the document for hipgen, the pinhole camera as the
+ * engine's uniforms take it (genrays.js:61-92: right, up, fwd columns of invView, fx, fy of invProjection), and
+ * the render settings of the configuration.
+ *
+ *   plane-c1     infinite SDF plane z = 0, Lambertian albedo 0.8, uniform sky of 1000 cd/m^2 over the upper
+ *                hemisphere, camera at (0, -3, 1.5) aimed at the origin, 60 deg vertical, 256 x 256, 64 spp, depth 4:
+ *                the floor reads 0.8 x 1000 = 800 cd/m^2 exactly.
+ *   room-c2      a 5 x 4 x 3 m room (walls albedo 0.7) holding 32 primitives (spheres, boxes, tori, rounded
+ *                cylinders) placed from the seed, sizes U(0.1, 0.5) m, 8 smooth-union pairs; GGX materials with
+ *                roughness U(0.05, 0.9), metallic 0 or 1 (p = 0.3), base U(0.2, 0.9) per channel; a 0.5 x 0.5 m
+ *                ceiling panel of 5000 cd/m^2; 1920 x 1080, 256 spp, depth 8, NEE on.
+ *   caustic-c3   the room with a 2 x 2 x 2 m haze box (sigma_s 0.05 /m, g 0.6, albedo 1), a dispersive glass sphere
+ *                (r 0.4 m, n = 1.5 + 0.0042 / lambda^2 um) and a 100 klx sun through a 0.3 m aperture in the ceiling;
+ *                1280 x 720.
+ *   furnace-*    closed-form checks: a white furnace (albedo-1 Lambertian plane under a uniform sky, and a haze box
+ *                under a uniform sphere of sky).
+ */
+
+const SEED = 20261203
+
+// mulberry32: a small seeded generator for scene layout (not the renderer's)
+function rng(seed) {
+    let a = seed >>> 0
+    return () => {
+        a = (a + 0x6D2B79F5) >>> 0
+        let t = a
+        t = Math.imul(t ^ (t >>> 15), t | 1)
+        t ^= t + Math.imul(t ^ (t >>> 7), t | 61)
+        return ((t ^ (t >>> 14)) >>> 0) / 4294967296
+    }
+}
+
+function camera(pos, target, vfovDeg, width, height) {
+    const v = [target[0] - pos[0], target[1] - pos[1], target[2] - pos[2]]
+    const lv = Math.hypot(v[0], v[1], v[2])
+    const f = v.map(x => x / lv)
+    let r = [f[1], -f[0], 0]                      // cross(f, z)
+    const lr = Math.hypot(r[0], r[1], r[2]) || 1
+    r = r.map(x => x / lr)
+    const u = [r[1] * f[2] - r[2] * f[1], r[2] * f[0] - r[0] * f[2], r[0] * f[1] - r[1] * f[0]]
+    const fy = Math.tan(0.5 * vfovDeg * Math.PI / 180)
+    return { pos, right: r, up: u, fwd: f.map(x => -x), fx: fy * width / height, fy, type: 0 }
+}
+
+const xf = (x, y, z, rx = 0, ry = 0, rz = 0, s = 1) => ({ pos: { x, y, z }, rot: { x: rx, y: ry, z: rz }, scale: s })
+const BLOCKS = [
+    { id: 'b-plane', label: 'plane', kind: 'object', builtin: 'plane@1', params: {} },
+    { id: 'b-box', label: 'box', kind: 'object', builtin: 'box@1', params: { size: { x: 0.5, y: 0.5, z: 0.5 }, rounding: 0.02 }, lengths: ['size', 'rounding'] },
+    { id: 'b-sphere', label: 'sphere', kind: 'object', builtin: 'sphere@1', params: { radius: 0.5 }, lengths: ['radius'] },
+    { id: 'b-cylinder', label: 'cylinder', kind: 'object', builtin: 'cylinder@1', params: { radius: 0.5, height: 1 }, lengths: ['radius', 'height'] },
+    { id: 'b-torus', label: 'torus', kind: 'object', builtin: 'torus@1', params: { ring: 0.4, tube: 0.1 }, lengths: ['ring', 'tube'] },
+]
+function material(id, base, roughness, metallic, extra) {
+    return { id, label: id, params: Object.assign({ baseColor: { r: base[0], g: base[1], b: base[2] }, metallic, roughness, specular: 0.5, transmission: 0, ior: 1.5,
+        emissive: { r: 1, g: 1, b: 1 }, emissiveLuminance: 0, abbe: 0 }, extra || {}) }
+}
+function doc(title, materials, children, environment, envRadius) {
+    return { format: 'raytracer-scene', version: 31, title, unit: 'm', envRadius: envRadius || 400, materials, blocks: BLOCKS,
+        root: { id: 'root', kind: 'group', label: title, enabled: true, blend: 0, children }, environment: environment || {} }
+}
+const obj = (id, block, material, transform, params, extra) => Object.assign({ id, kind: 'object', type: 'code', label: id, enabled: true, block, material, transform, params: params || {} }, extra || {})
+
+function planeC1(opts = {}) {
+    const albedo = opts.albedo !== undefined ? opts.albedo : 0.8
+    const mats = [material('floor', [albedo, albedo, albedo], 1, 0, { specular: 0 })]
+    const d = doc('plane-c1', mats, [obj('floor', 'b-plane', 'floor', xf(0, 0, 0))], { sky: 'hemisphere', uniformSky: opts.sky || 1000 })
+    const w = opts.width || 256, h = opts.height || 256
+    return { doc: d, camera: camera([0, -3, 1.5], [0, 0, 0], 60, w, h), settings: { width: w, height: h, spp: 64, maxbounce: 4, nee: 1 } }
+}
+
+// The room: a box shell (outer minus inner) of albedo-0.7 walls, inner 5 x 4 x 3 m, floor at z = 0.
+function roomChildren(extraCut) {
+    const shell = { id: 'room', kind: 'group', label: 'room', enabled: true, blend: 0, transform: xf(0, 0, 1.5), children: [
+        obj('room-outer', 'b-box', 'walls', xf(0, 0, 0), { size: { x: 2.7, y: 2.2, z: 1.7 }, rounding: 0 }),
+        obj('room-inner', 'b-box', 'walls', xf(0, 0, 0), { size: { x: 2.5, y: 2.0, z: 1.5 }, rounding: 0 }, { join: 'subtract' }),
+    ] }
+    if (extraCut) shell.children.push(extraCut)
+    return shell
+}
+
+function roomObjects(r, mats) {
+    const kinds = ['sphere', 'box', 'torus', 'cylinder']
+    const out = []
+    const place = (size) => [(-2.5 + size + r() * (5 - 2 * size)) * 0.9, (-2.0 + size + r() * (4 - 2 * size)) * 0.9, size + r() * (2.2 - 2 * size)]
+    const prim = (id, kind, size, at, matId) => {
+        const rot = [r() * 360, r() * 360, r() * 360]
+        const t = xf(at[0], at[1], at[2], rot[0], rot[1], rot[2])
+        if (kind === 'sphere') return obj(id, 'b-sphere', matId, t, { radius: size })
+        if (kind === 'box') return obj(id, 'b-box', matId, t, { size: { x: size, y: size * (0.5 + 0.5 * r()), z: size * (0.5 + 0.5 * r()) }, rounding: 0.02 })
+        if (kind === 'torus') return obj(id, 'b-torus', matId, t, { ring: size * 0.7, tube: size * 0.3 })
+        return obj(id, 'b-cylinder', matId, t, { radius: size * 0.6, height: size * 1.6, rounding: 0.03 })
+    }
+    const newMat = (i) => {
+        const base = [0.2 + 0.7 * r(), 0.2 + 0.7 * r(), 0.2 + 0.7 * r()]
+        const m = material('m' + i, base, 0.05 + 0.85 * r(), r() < 0.3 ? 1 : 0)
+        mats.push(m)
+        return m.id
+    }
+    let k = 0
+    for (let pair = 0; pair < 8; pair++) {
+        const s1 = 0.1 + 0.4 * r(), s2 = 0.1 + 0.4 * r()
+        const at = place(Math.max(s1, s2))
+        const dir = [r() - 0.5, r() - 0.5, r() - 0.5]
+        const l = Math.hypot(dir[0], dir[1], dir[2]) || 1
+        const at2 = at.map((v, i) => v + dir[i] / l * 0.8 * (s1 + s2) * 0.5)
+        at2[2] = Math.max(at2[2], s2)
+        const a = prim('p' + k, kinds[k % 4], s1, at, newMat(k)); k++
+        const b = prim('p' + k, kinds[k % 4], s2, at2, newMat(k)); k++
+        out.push({ id: 'pair' + pair, kind: 'group', label: 'smooth pair', enabled: true, blend: 0.05, transform: xf(0, 0, 0), children: [a, b] })
+    }
+    while (k < 32) {
+        const s = 0.1 + 0.4 * r()
+        out.push(prim('p' + k, kinds[k % 4], s, place(s), newMat(k)))
+        k++
+    }
+    return out
+}
+
+function panel() {
+    return { id: 'panel', kind: 'object', type: 'light', label: 'ceiling panel', enabled: true, material: null,
+        transform: xf(0, 0, 2.99, 180, 0, 0), params: { width: 0.5, height: 0.5 }, light: { shape: 'rectangle', luminance: 5000 } }
+}
+
+function roomC2(opts = {}) {
+    const r = rng(opts.seed || SEED)
+    const mats = [material('walls', [0.7, 0.7, 0.7], 1, 0, { specular: 0 })]
+    const objs = roomObjects(r, mats)
+    const d = doc('room-c2', mats, [roomChildren(), ...objs, panel()], { sky: 'none' }, 10)
+    const w = opts.width || 1920, h = opts.height || 1080
+    return { doc: d, camera: camera([0, -1.95, 1.6], [0, 0.5, 0.8], 70, w, h), settings: { width: w, height: h, spp: 256, maxbounce: 8, nee: 1 } }
+}
+
+function causticC3(opts = {}) {
+    const r = rng(opts.seed || SEED)
+    const mats = [material('walls', [0.7, 0.7, 0.7], 1, 0, { specular: 0 }),
+        material('glass', [1, 1, 1], 0, 0, { transmission: 1, ior: 1.512166, abbe: 63.857, specular: 0.5 })]
+    const objs = roomObjects(r, mats)
+    // the aperture: a 0.3 m hole through the ceiling above the glass
+    const hole = obj('aperture', 'b-cylinder', 'walls', xf(0.6, 0.3, 1.6), { radius: 0.15, height: 0.6 }, { join: 'subtract' })
+    const glass = obj('glass-sphere', 'b-sphere', 'glass', xf(0.3, 0.15, 0.9), { radius: 0.4 })
+    const hazeBox = { id: 'haze', kind: 'object', type: 'haze', label: 'haze', enabled: true, material: null, transform: xf(0, 0, 1.2),
+        params: { size: { x: 2, y: 2, z: 2 } }, haze: { mfp: 20, g: 0.6, albedo: { r: 1, g: 1, b: 1 } } }
+    // keep the glass and the beam clear of the random objects: drop those near them
+    const clear = objs.filter(o => {
+        const c = o.kind === 'group' ? o.children[0].transform.pos : o.transform.pos
+        return Math.hypot(c.x - 0.3, c.y - 0.15) > 0.9
+    })
+    const env = { sky: 'none', sunIlluminance: 100000, uSunAzimuth: 26.57, uSunElevation: 70, sunDiameter: 0.53 }
+    const d = doc('caustic-c3', mats, [roomChildren(hole), ...clear, glass, hazeBox, panel()], env, 10)
+    const w = opts.width || 1280, h = opts.height || 720
+    return { doc: d, camera: camera([0, -1.95, 1.6], [0.2, 0.5, 0.5], 70, w, h),
+        settings: { width: w, height: h, spp: 64, maxbounce: 8, nee: 1, sppmIterations: 10, sppmPhotons: 1 << 22, sppmRadius: 0.02, sppmAlpha: 0.7 } }
+}
+
+function furnacePlane(opts = {}) {
+    const mats = [material('white', [1, 1, 1], opts.roughness !== undefined ? opts.roughness : 1, opts.metallic || 0, { specular: opts.specular !== undefined ? opts.specular : 0 })]
+    const d = doc('furnace-plane', mats, [obj('floor', 'b-plane', 'white', xf(0, 0, 0))], { sky: 'hemisphere', uniformSky: 1000 })
+    const w = opts.width || 64, h = opts.height || 64
+    return { doc: d, camera: camera([0, -3, 1.5], [0, 0, 0], 60, w, h), settings: { width: w, height: h, spp: 64, maxbounce: 16, nee: 1 } }
+}
+
+function furnaceHaze(opts = {}) {
+    // a haze box of albedo 1 in a uniform sphere of sky: every pixel reads the sky's luminance (energy conservation)
+    const g = opts.g !== undefined ? opts.g : 0.6
+    const hazeBox = { id: 'haze', kind: 'object', type: 'haze', label: 'haze', enabled: true, material: null, transform: xf(0, 0, 0),
+        params: { size: { x: 2, y: 2, z: 2 } }, haze: { mfp: opts.mfp || 0.5, g, albedo: { r: 1, g: 1, b: 1 } } }
+    // a far sphere marks nothing: the scene needs a surface, so a tiny sphere far outside the view
+    const mats = [material('none', [0, 0, 0], 1, 0, { specular: 0 })]
+    const d = doc('furnace-haze', mats, [obj('speck', 'b-sphere', 'none', xf(0, 0, -50), { radius: 0.01 }), hazeBox], { sky: 'sphere', uniformSky: 1000 }, 20)
+    const w = opts.width || 64, h = opts.height || 64
+    return { doc: d, camera: camera([0, -4, 0], [0, 0, 0], 30, w, h), settings: { width: w, height: h, spp: 64, maxbounce: 256, nee: 1 } }
+}
+
+const CONFIGS = { 'plane-c1': planeC1, 'room-c2': roomC2, 'caustic-c3': causticC3, 'furnace-plane': furnacePlane, 'furnace-haze': furnaceHaze }
+
+module.exports = { CONFIGS, camera, rng, SEED }

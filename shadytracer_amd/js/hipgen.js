@@ -1,0 +1,355 @@
+'use strict'
+/*
+ * HIPGEN -- the scene builder's document becomes HIP C++ for the MI355X engine, beside the reference's WGSL
+ * emitter (builder/codegen.js generate :974, emitScene :1238, emitMaterials :648, emitLightFaces :861).
+ *
+ * Input: a builder document (format 'raytracer-scene', model version 31: builder/model.js:183), the same JSON the
+ * reference's scenes/*.json hold. Output: the scene contract the engine loads (lib/pathtracer/README.md:126-145),
+ * with `hip` in place of `wgsl`:
+ *   hip        sceneDist(U, p) (distance only, the march's) and sceneFull(U, p) (normal, distance, closest and
+ *              enclosing material), straight-line code over the document's tree
+ *   params     the parameter block U both read: every node's world transform and its own parameters, in metres
+ *   layout     where each node's parameters sit in U (for updates without a rebuild)
+ *   materials  the material table; lights' faces; the haze box; the environment; envRadius
+ *   program    the same tree as a postfix list, for the CPU oracle
+ *   flops      FP32 operations of one sceneDist call (the march's roofline count)
+ *
+ * Scope: the builder's builtin shapes (plane, box, sphere, cylinder, capsule, torus, rectangle, disc), groups with
+ * their joins (union, subtract, intersect; smooth blends), transforms (rotation Rz*Ry*Rx in degrees, uniform scale:
+ * builder/transform.js:9-38), lights (rectangle and disc faces), one haze box and the studio environment. A code
+ * block that is not a builtin -- user WGSL -- is refused with its node named.
+ */
+
+const UNIT_METRES = { m: 1, cm: 0.01, mm: 0.001, um: 1e-6, in: 0.0254 }
+const LUMA = [0.2126, 0.7152, 0.0722]
+
+// FP32 operations per distance-only primitive evaluation in its frame (the transform in and out counted apart).
+const PRIM_FLOPS = { plane: 1, sphere: 7, box: 27, cylinder: 22, capsule: 14, torus: 14, rectangle: 14, disc: 13 }
+const XF_FLOPS = 20      // p - pos, three dots, 1/s, the distance times s
+const FOLD_FLOPS = 2
+const JOIN_FLOPS = 9
+
+// The builtin shapes: their parameters in U (after the 13 of the transform), and the HIP calls.
+const SHAPES = {
+    plane: { params: () => [], dist: () => 'Dplane(q)', full: () => 'Nplane(q)' },
+    sphere: { params: P => [P.radius], dist: o => `Dsphere(q, U[${o}])`, full: o => `Nsphere(q, U[${o}])` },
+    box: {
+        params: P => [P.size.x, P.size.y, P.size.z, P.rounding || 0],
+        dist: o => `DboxRounded(q, v3(U[${o}], U[${o + 1}], U[${o + 2}]), U[${o + 3}])`,
+        full: o => `NboxRounded(q, v3(U[${o}], U[${o + 1}], U[${o + 2}]), U[${o + 3}])`,
+    },
+    cylinder: {
+        // half height, radius, edge rounding (0: the builder's sharp cylinder)
+        params: P => [0.5 * P.height, P.radius, P.rounding || 0],
+        dist: o => `DroundCylinder(q, U[${o}], U[${o + 1}], U[${o + 2}])`,
+        full: o => `NroundCylinder(q, U[${o}], U[${o + 1}], U[${o + 2}])`,
+    },
+    capsule: { params: P => [0.5 * P.height, P.radius], dist: o => `Dcapsule(q, U[${o}], U[${o + 1}])`, full: o => `Ncapsule(q, U[${o}], U[${o + 1}])` },
+    torus: { params: P => [P.ring, P.tube], dist: o => `Dtorus(q, U[${o}], U[${o + 1}])`, full: o => `Ntorus(q, U[${o}], U[${o + 1}])` },
+    rectangle: {
+        params: P => [0.5 * P.width, 0.5 * P.height],
+        dist: o => `Drectangle(q, U[${o}], U[${o + 1}])`, full: o => `Nrectangle(q, U[${o}], U[${o + 1}])`,
+    },
+    disc: { params: P => [P.radius], dist: o => `Ddisc(q, U[${o}])`, full: o => `Ndisc(q, U[${o}])` },
+}
+// the length-valued parameters of each shape (scaled by the unit); the rest are plain numbers
+const LENGTHS = { sphere: ['radius'], box: ['size', 'rounding'], cylinder: ['radius', 'height', 'rounding'], capsule: ['radius', 'height'],
+    torus: ['ring', 'tube'], rectangle: ['width', 'height'], disc: ['radius'], plane: [] }
+
+// ---- transforms: world = parent * local; R = Rz Ry Rx (columns: local axes in the parent)
+function rotationOf(t) {
+    const r = Math.PI / 180
+    const rot = (t && t.rot) || { x: 0, y: 0, z: 0 }
+    const cx = Math.cos(rot.x * r), sx = Math.sin(rot.x * r)
+    const cy = Math.cos(rot.y * r), sy = Math.sin(rot.y * r)
+    const cz = Math.cos(rot.z * r), sz = Math.sin(rot.z * r)
+    return [
+        [cz * cy, cz * sy * sx - sz * cx, cz * sy * cx + sz * sx],
+        [sz * cy, sz * sy * sx + cz * cx, sz * sy * cx - cz * sx],
+        [-sy, cy * sx, cy * cx],
+    ]
+}
+const IDENT = { R: [[1, 0, 0], [0, 1, 0], [0, 0, 1]], t: [0, 0, 0], s: 1 }
+function mul3(A, B) {
+    const C = [[0, 0, 0], [0, 0, 0], [0, 0, 0]]
+    for (let i = 0; i < 3; i++) for (let j = 0; j < 3; j++) for (let k = 0; k < 3; k++) C[i][j] += A[i][k] * B[k][j]
+    return C
+}
+function apply3(A, v) { return [0, 1, 2].map(i => A[i][0] * v[0] + A[i][1] * v[1] + A[i][2] * v[2]) }
+function compose(parent, t, unit) {
+    if (!t) return parent
+    const pos = t.pos || { x: 0, y: 0, z: 0 }
+    const local = [pos.x * unit, pos.y * unit, pos.z * unit]
+    const R = rotationOf(t)
+    const s = t.scale === undefined ? 1 : t.scale
+    if (!(s > 0)) throw new Error('hipgen: a transform scale must be > 0')
+    const off = apply3(parent.R, local).map(v => v * parent.s)
+    return { R: mul3(parent.R, R), t: [0, 1, 2].map(i => parent.t[i] + off[i]), s: parent.s * s }
+}
+// 13 floats: position, the rotation's three columns, scale
+function xfParams(X) {
+    const R = X.R
+    return [X.t[0], X.t[1], X.t[2], R[0][0], R[1][0], R[2][0], R[0][1], R[1][1], R[2][1], R[0][2], R[1][2], R[2][2], X.s]
+}
+
+function fmt(x) { return Number.isFinite(x) ? x : 0 }
+
+function colourOf(c, d) { return c ? [fmt(c.r), fmt(c.g), fmt(c.b)] : d }
+
+// ---- the document's materials as the engine's table
+function materialOf(m) {
+    const P = (m && m.params) || {}
+    const base = colourOf(P.baseColor, [0.8, 0.8, 0.8])
+    const ecol = colourOf(P.emissive, [1, 1, 1])
+    const el = fmt(P.emissiveLuminance || 0)
+    const y = LUMA[0] * ecol[0] + LUMA[1] * ecol[1] + LUMA[2] * ecol[2]
+    const emissive = el > 0 && y > 0 ? ecol.map(c => c / y * el) : [0, 0, 0]
+    return {
+        id: m ? m.id : null,
+        base,
+        metallic: fmt(P.metallic || 0),
+        roughness: P.roughness === undefined ? 0.5 : fmt(P.roughness),
+        transmission: fmt(P.transmission || 0),
+        ior: P.ior === undefined ? 1.5 : fmt(P.ior),
+        specular: P.specular === undefined ? 0.5 : fmt(P.specular),
+        emissive,
+        abbe: fmt(P.abbe || 0),
+        face: -1,
+        specularTint: fmt(P.specularTint || 0),
+    }
+}
+
+function generate(doc, opts) {
+    opts = opts || {}
+    if (!doc || !doc.root) throw new Error('hipgen: not a builder document (no root)')
+    const unit = UNIT_METRES[doc.unit || 'm']
+    if (!unit) throw new Error(`hipgen: unknown unit '${doc.unit}'`)
+    const blocks = {}
+    for (const b of doc.blocks || []) blocks[b.id] = b
+    const materials = [materialOf(null)]           // 0: the default
+    const matIndex = {}
+    for (const m of doc.materials || []) { matIndex[m.id] = materials.length; materials.push(materialOf(m)) }
+    if (materials.length > 64) throw new Error('hipgen: more than 64 materials')
+
+    const params = []
+    const layout = {}
+    const program = []
+    const faces = []
+    let haze = null
+    let flops = 0
+    const dist = [], full = []
+    let tmp = 0
+
+    const push = vals => { const o = params.length; for (const v of vals) params.push(fmt(v)); return o }
+
+    function shapeOf(n) {
+        if (n.type === 'light') return n.light && n.light.shape === 'disc' ? 'disc' : 'rectangle'
+        const b = blocks[n.block]
+        if (!b) throw new Error(`hipgen: node '${n.id}' has no block '${n.block}'`)
+        const key = b.builtin ? String(b.builtin).split('@')[0] : null
+        if (!key || !SHAPES[key]) throw new Error(`hipgen: node '${n.id}' (${n.label || ''}) is user code, not a builtin shape: WGSL translation is not supported`)
+        return key
+    }
+    function paramsOf(n, key) {
+        const b = blocks[n.block] || {}
+        const P = Object.assign({}, b.params || {}, n.params || {})
+        const L = LENGTHS[key]
+        const out = {}
+        for (const k of Object.keys(P)) {
+            const v = P[k]
+            const isLen = L.indexOf(k) >= 0
+            if (v && typeof v === 'object') out[k] = { x: v.x * (isLen ? unit : 1), y: v.y * (isLen ? unit : 1), z: v.z * (isLen ? unit : 1) }
+            else out[k] = isLen ? v * unit : v
+        }
+        return out
+    }
+
+    // one shape: its world transform and parameters into U; the expressions of its (local) value
+    function shape(n, X) {
+        const key = shapeOf(n)
+        let P = paramsOf(n, key)
+        if (n.type === 'light') {
+            P = n.light.shape === 'disc' ? { radius: (n.params.radius || 0.5) * unit } : { width: (n.params.width || 1) * unit, height: (n.params.height || 1) * unit }
+        }
+        const xo = push(xfParams(X))
+        const po = push(SHAPES[key].params(P))
+        layout[n.id] = { xf: xo, params: po, shape: key }
+        flops += PRIM_FLOPS[key] + XF_FLOPS
+        return { key, xo, po, P }
+    }
+    function matOf(n) {
+        if (n.type === 'light') return lightMaterial(n)
+        if (n.material && matIndex[n.material] !== undefined) return matIndex[n.material]
+        return 0
+    }
+    function lightMaterial(n) {
+        // a light: an emissive face (one-sided, front along +z of its frame) and an entry in the face table
+        const L = n.light || {}
+        const X = n.__world
+        const shape = L.shape === 'disc' ? 1 : 2
+        const half = shape === 1 ? [(n.params.radius || 0.5) * unit * X.s, 0] : [0.5 * (n.params.width || 1) * unit * X.s, 0.5 * (n.params.height || 1) * unit * X.s]
+        const area = shape === 1 ? Math.PI * half[0] * half[0] : 4 * half[0] * half[1]
+        const col = colourOf(L.color, [1, 1, 1])
+        const y = LUMA[0] * col[0] + LUMA[1] * col[1] + LUMA[2] * col[2]
+        // luminance given, or from the flux of a one-sided Lambertian face: L = flux / (pi A)
+        const lum = L.luminance !== undefined ? L.luminance : (L.flux || 0) / (Math.PI * area * (L.twoSided ? 2 : 1))
+        const le = col.map(c => c / (y || 1) * lum)
+        const R = X.R
+        faces.push({
+            centre: X.t.slice(), halfx: half[0], u: [R[0][0], R[1][0], R[2][0]], halfy: half[1], v: [R[0][1], R[1][1], R[2][1]], shape,
+            n: [R[0][2], R[1][2], R[2][2]], twoSided: L.twoSided ? 1 : 0, le, weight: 1,
+        })
+        const m = materialOf(null)
+        m.base = [0, 0, 0]
+        m.emissive = le
+        m.face = faces.length - 1
+        m.id = 'light:' + n.id
+        materials.push(m)
+        return materials.length - 1
+    }
+
+    const isCsg = g => (g.blend > 0) || (g.children || []).some((c, i) => i > 0 && ((c.join && c.join !== 'union') || c.joinBlend > 0))
+    const drawn = c => c.enabled !== false && c.kind !== undefined && c.type !== 'haze'
+
+    // a CSG group's children joined in order into one value; returns the names of its expressions
+    function csgGroup(g, X) {
+        const kids = (g.children || []).filter(drawn)
+        if (!kids.length) return null
+        const blendG = (g.blend || 0) * unit * X.s
+        let acc = null
+        for (let i = 0; i < kids.length; i++) {
+            const c = kids[i]
+            const v = valueOf(c, X)
+            if (!v) continue
+            if (!acc) { acc = v; continue }
+            const join = i === 0 ? 'union' : (c.join || 'union')
+            const k = c.joinBlend !== undefined && c.joinBlend !== null ? c.joinBlend * unit * X.s : blendG
+            const ko = push([k])
+            const fn = { union: ['dUnion', 'csgUnion'], subtract: ['dSubtract', 'csgSubtract'], intersect: ['dIntersect', 'csgIntersect'] }[join]
+            if (!fn) throw new Error(`hipgen: node '${c.id}': unknown join '${join}'`)
+            const wins = { union: `${v.f}.w < ${acc.f}.w`, subtract: `-${v.f}.w > ${acc.f}.w`, intersect: `${v.f}.w > ${acc.f}.w` }[join]
+            dist.push(`    ${acc.d} = ${fn[0]}(${acc.d}, ${v.d}, U[${ko}]);`)
+            full.push(`    if (${wins}) ${acc.m} = ${v.m};`)
+            full.push(`    ${acc.f} = ${fn[1]}(${acc.f}, ${v.f}, U[${ko}]);`)
+            program.push({ op: 'join', join, k: ko })
+            flops += JOIN_FLOPS
+        }
+        return acc
+    }
+    // a node's value as expressions (for use inside a CSG group)
+    function valueOf(n, parentX) {
+        const X = compose(parentX, n.transform, unit)
+        if (n.kind === 'group') {
+            if (isCsg(n)) return csgGroup(n, X)
+            // a plain group inside a CSG group: its children unioned
+            return csgGroup(Object.assign({}, n, { blend: 0, children: (n.children || []).map(c => Object.assign({}, c, { join: 'union', joinBlend: undefined })) }), X)
+        }
+        n.__world = X
+        const s = shape(n, X)
+        const mat = matOf(n)
+        const i = tmp++
+        const d = `d${i}`, f = `f${i}`, m = `m${i}`
+        dist.push(`    float ${d} = ${SHAPES[s.key].dist(s.po)} * U[${s.xo + 12}];`.replace(/\bq\b/, `xfIn(U + ${s.xo}, p)`))
+        full.push(`    V4 ${f} = xfOut(U + ${s.xo}, ${SHAPES[s.key].full(s.po)});`.replace(/\bq\b/, `xfIn(U + ${s.xo}, p)`))
+        full.push(`    unsigned int ${m} = ${mat}u;`)
+        program.push({ op: 'prim', shape: s.key, xf: s.xo, p: s.po, mat })
+        return { d, f, m }
+    }
+    // the closest-surface fold over a plain group's descendants
+    function foldNode(n, parentX) {
+        if (n.enabled === false) return
+        if (n.type === 'haze') { hazeOf(n, compose(parentX, n.transform, unit)); return }
+        if (n.kind === 'group' && !isCsg(n)) {
+            const X = compose(parentX, n.transform, unit)
+            for (const c of n.children || []) foldNode(c, X)
+            return
+        }
+        const v = valueOf(n, parentX)
+        if (!v) return
+        dist.push(`    foldDist(best, ${v.d});`)
+        full.push(`    foldFull(h, ${v.f}, ${v.m});`)
+        program.push({ op: 'fold' })
+        flops += FOLD_FLOPS
+    }
+    function hazeOf(n, X) {
+        if (haze) throw new Error('hipgen: one haze box a scene')
+        const H = Object.assign({ mfp: 1, albedo: { r: 0.9, g: 0.9, b: 0.9 }, g: 0.7 }, n.haze || {})
+        const size = (n.params && n.params.size) || { x: 1, y: 1, z: 1 }
+        const rotated = Math.abs(X.R[0][0] - 1) + Math.abs(X.R[1][1] - 1) + Math.abs(X.R[2][2] - 1) > 1e-6
+        if (rotated) throw new Error(`hipgen: haze box '${n.id}' is turned; boxes are axis-aligned here`)
+        const alb = colourOf(H.albedo, [0.9, 0.9, 0.9])
+        haze = {
+            centre: X.t.slice(), half: [size.x, size.y, size.z].map(v => 0.5 * v * unit * X.s),
+            sigmaS: 1 / H.mfp, g: Math.max(-0.95, Math.min(0.95, H.g)),
+            albedo: (alb[0] + alb[1] + alb[2]) / 3,
+        }
+    }
+
+    for (const c of doc.root.children || []) foldNode(c, compose(IDENT, doc.root.transform, unit))
+
+    const envRadius = (doc.envRadius !== undefined ? doc.envRadius * unit : 400)
+    const env = environmentOf(doc.environment || {}, doc)
+
+    const hip = [
+        `// emitted by shadytracer_amd/js/hipgen.js from '${(doc.title || 'scene').replace(/[\n\r]/g, ' ')}'`,
+        `#define PT_SCENE_MATERIALS ${materials.length}`,
+        '__device__ __forceinline__ float sceneDist(const float* __restrict__ U, V3 p) {',
+        '    float best = 1.0e7f;',
+        ...dist,
+        '    return best;',
+        '}',
+        '__device__ __forceinline__ SdfHit sceneFull(const float* __restrict__ U, V3 p) {',
+        '    SdfHit h = sdfEmpty();',
+        ...full,
+        '    return h;',
+        '}',
+        '',
+    ].join('\n')
+    return {
+        id: doc.title || 'scene',
+        hip,
+        params: Float32Array.from(params),
+        layout,
+        materials,
+        faces,
+        haze,
+        env,
+        envRadius,
+        unit,
+        program,
+        flops: { dist: flops },
+    }
+}
+
+// The studio environment (the builder's envMode 0, studioIlluminance lux on a horizontal surface): here a uniform
+// sky over the upper hemisphere whose luminance gives that illuminance (E = pi L); `uniformSky` (cd/m^2) and
+// `sky: 'sphere' | 'hemisphere' | 'none'` name it directly. An optional sun: direction from azimuth/elevation
+// (sdf.js:72-76, Z-up), illuminance in lux, angular diameter in degrees, and an aperture (a beam radius about a point).
+function environmentOf(E, doc) {
+    const env = { type: 0, L: 0, col: [1, 1, 1], sunDir: [0, 0, 1], sunCos: 1, sunE: 0, sunRadius: 0, sunPoint: [0, 0, 0] }
+    const kind = E.sky || (E.envMode === 0 || E.envMode === undefined ? 'hemisphere' : 'none')
+    let L = E.uniformSky !== undefined ? E.uniformSky : (E.studioIlluminance !== undefined ? E.studioIlluminance / Math.PI : 0)
+    if (kind === 'none') L = 0
+    env.type = L > 0 ? (kind === 'sphere' ? 1 : 2) : 0
+    env.L = L
+    if (E.studioColor) {
+        const c = colourOf(E.studioColor, [1, 1, 1])
+        const y = LUMA[0] * c[0] + LUMA[1] * c[1] + LUMA[2] * c[2]
+        env.col = c.map(v => v / (y || 1))
+    }
+    if (E.sunIlluminance > 0) {
+        const r = Math.PI / 180
+        const az = (E.uSunAzimuth || 0) * r, el = (E.uSunElevation || 45) * r
+        env.sunDir = [Math.cos(el) * Math.cos(az), Math.cos(el) * Math.sin(az), Math.sin(el)]
+        env.sunCos = Math.cos(0.5 * (E.sunDiameter || 0.53) * r)
+        env.sunE = E.sunIlluminance
+        const unit = UNIT_METRES[doc.unit || 'm']
+        if (E.sunAperture) {
+            env.sunRadius = E.sunAperture.radius * unit
+            env.sunPoint = [E.sunAperture.point.x * unit, E.sunAperture.point.y * unit, E.sunAperture.point.z * unit]
+        }
+    }
+    return env
+}
+
+module.exports = { generate, environmentOf, rotationOf, UNIT_METRES, PRIM_FLOPS }

@@ -1,0 +1,301 @@
+"""ctypes binding of libpt (include/pt.h): the engine as the tests and the benchmark drive it.
+
+It mirrors the reference's engine object (lib/pathtracer/core/raytrace_main.js:121 RAYTRACER): set_scene,
+set_uniforms, trace, image_stats, read_region, clear_accums. There is no CPU fallback: without the built
+library or a GPU the calls raise.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, 'libpt.so')
+
+F3 = ctypes.c_float * 3
+F2 = ctypes.c_float * 2
+
+
+class PtCamera(ctypes.Structure):
+    _fields_ = [('pos', F3), ('fx', ctypes.c_float), ('right', F3), ('fy', ctypes.c_float), ('up', F3),
+                ('type', ctypes.c_uint), ('fwd', F3), ('pad', ctypes.c_float)]
+
+
+class PtMaterial(ctypes.Structure):
+    _fields_ = [('base', F3), ('metallic', ctypes.c_float), ('roughness', ctypes.c_float), ('transmission', ctypes.c_float),
+                ('ior', ctypes.c_float), ('specular', ctypes.c_float), ('emissive', F3), ('abbe', ctypes.c_float),
+                ('face', ctypes.c_int), ('specularTint', ctypes.c_float), ('pad', F2)]
+
+
+class PtLightFace(ctypes.Structure):
+    _fields_ = [('centre', F3), ('halfx', ctypes.c_float), ('u', F3), ('halfy', ctypes.c_float), ('v', F3), ('shape', ctypes.c_uint),
+                ('n', F3), ('twoSided', ctypes.c_uint), ('le', F3), ('weight', ctypes.c_float)]
+
+
+class PtEnv(ctypes.Structure):
+    _fields_ = [('type', ctypes.c_uint), ('L', ctypes.c_float), ('col', F3), ('sunDir', F3), ('sunCos', ctypes.c_float),
+                ('sunE', ctypes.c_float), ('sunRadius', ctypes.c_float), ('sunPoint', F3), ('pad', F2)]
+
+
+class PtHaze(ctypes.Structure):
+    _fields_ = [('centre', F3), ('sigmaS', ctypes.c_float), ('half', F3), ('g', ctypes.c_float), ('albedo', ctypes.c_float),
+                ('pad', F3)]
+
+
+class PtUniforms(ctypes.Structure):
+    _fields_ = [('width', ctypes.c_uint), ('height', ctypes.c_uint), ('maxbounce', ctypes.c_uint), ('maxiter', ctypes.c_uint),
+                ('stepscale', ctypes.c_float), ('eps', ctypes.c_float), ('epsAuto', ctypes.c_float), ('envRadius', ctypes.c_float),
+                ('nee', ctypes.c_uint), ('rr', ctypes.c_uint), ('rrStart', ctypes.c_uint), ('diffuseCap', ctypes.c_uint),
+                ('fireflyClamp', ctypes.c_float), ('fireflyWhite', ctypes.c_float), ('maxaverages', ctypes.c_uint),
+                ('seed', ctypes.c_uint), ('cam', PtCamera), ('env', PtEnv), ('haze', PtHaze), ('faceShare', ctypes.c_float),
+                ('sppm', ctypes.c_uint), ('sppmRadius', ctypes.c_float), ('sppmPhotons', ctypes.c_uint),
+                ('marchRefill', ctypes.c_uint), ('guide', ctypes.c_uint), ('guideFrac', ctypes.c_float), ('pad0', ctypes.c_uint)]
+
+
+class PtSceneDesc(ctypes.Structure):
+    _fields_ = [('hip_src', ctypes.c_char_p), ('params', ctypes.POINTER(ctypes.c_float)), ('nparams', ctypes.c_uint32),
+                ('materials', ctypes.POINTER(PtMaterial)), ('nmaterials', ctypes.c_uint32),
+                ('faces', ctypes.POINTER(PtLightFace)), ('nfaces', ctypes.c_uint32)]
+
+
+class PtOpts(ctypes.Structure):
+    _fields_ = [('device', ctypes.c_int), ('blocks_per_cu', ctypes.c_uint32), ('cache_dir', ctypes.c_char_p),
+                ('reserved', ctypes.c_uint32 * 4)]
+
+
+STRUCTS = [PtUniforms, PtMaterial, PtLightFace, PtCamera, PtEnv, PtHaze, PtSceneDesc, PtOpts]
+
+# the C ABI, name -> (restype, argtypes)
+SIGNATURES = {
+    'pt_abi_version': (ctypes.c_int, []),
+    'pt_struct_size': (ctypes.c_size_t, [ctypes.c_int]),
+    'pt_create': (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(PtOpts)]),
+    'pt_destroy': (None, [ctypes.c_void_p]),
+    'pt_last_error': (ctypes.c_char_p, [ctypes.c_void_p]),
+    'pt_compile_scene': (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]),
+    'pt_set_scene': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PtSceneDesc)]),
+    'pt_set_params': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_uint32]),
+    'pt_set_uniforms': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PtUniforms), ctypes.c_size_t]),
+    'pt_clear_accum': (ctypes.c_int, [ctypes.c_void_p]),
+    'pt_trace': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    'pt_sync': (ctypes.c_int, [ctypes.c_void_p]),
+    'pt_read_accum': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ctypes.c_size_t]),
+    'pt_read_stats': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.c_size_t]),
+    'pt_reset_stats': (ctypes.c_int, [ctypes.c_void_p]),
+    'pt_time_kernels': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double), ctypes.c_size_t]),
+}
+
+STAT_NAMES = ['segments', 'shadow_segments', 'steps', 'shadow_steps', 'nonfinite', 'maxiter', 'paths', 'leaving',
+              'gathers', 'photons']
+LUMA = (0.2126, 0.7152, 0.0722)
+
+_lib = None
+
+
+class PtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__('libpt error %d: %s' % (code, msg))
+        self.code = code
+        self.restored = code == 3
+
+
+def load_library(path=LIB_PATH):
+    """Loads libpt and binds the C ABI. Raises if the library is not built: there is no fallback."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError('libpt.so is not built (%s): run `python -m shadytracer_amd.build`' % path)
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    for i, s in enumerate(STRUCTS):
+        if lib.pt_struct_size(i) != ctypes.sizeof(s):
+            raise RuntimeError('ABI mismatch: %s is %d bytes in libpt, %d here' % (s.__name__, lib.pt_struct_size(i), ctypes.sizeof(s)))
+    _lib = lib
+    return lib
+
+
+def _set3(arr, v):
+    for i in range(3):
+        arr[i] = float(v[i])
+
+
+def make_camera(cam):
+    c = PtCamera()
+    _set3(c.pos, cam['pos']); _set3(c.right, cam['right']); _set3(c.up, cam['up']); _set3(c.fwd, cam['fwd'])
+    c.fx, c.fy, c.type = cam['fx'], cam['fy'], int(cam.get('type', 0))
+    return c
+
+
+def make_env(e):
+    v = PtEnv()
+    v.type, v.L = int(e['type']), float(e['L'])
+    _set3(v.col, e['col']); _set3(v.sunDir, e['sunDir']); _set3(v.sunPoint, e['sunPoint'])
+    v.sunCos, v.sunE, v.sunRadius = e['sunCos'], e['sunE'], e['sunRadius']
+    return v
+
+
+def make_haze(h):
+    v = PtHaze()
+    if h:
+        _set3(v.centre, h['centre']); _set3(v.half, h['half'])
+        v.sigmaS, v.g, v.albedo = h['sigmaS'], h['g'], h['albedo']
+    return v
+
+
+def default_uniforms():
+    u = PtUniforms()
+    u.width, u.height, u.maxbounce, u.maxiter = 256, 256, 10, 400
+    u.stepscale, u.eps, u.epsAuto, u.envRadius = 0.9, 0.0, 1e-4, 400.0
+    u.nee, u.rr, u.rrStart, u.diffuseCap = 1, 0, 3, 0
+    u.fireflyClamp, u.fireflyWhite, u.maxaverages, u.seed = 0.0, 1.0, 0, 20261203
+    u.faceShare, u.marchRefill, u.guideFrac = 0.5, 16, 0.5
+    return u
+
+
+def uniforms_for(made, **over):
+    """The engine uniforms of an emitted configuration ({scene, camera, settings}), with overrides."""
+    s = dict(made.get('settings') or {})
+    s.update(over)
+    sc = made['scene']
+    u = default_uniforms()
+    u.width, u.height = int(s.get('width', 256)), int(s.get('height', 256))
+    u.maxbounce = int(s.get('maxbounce', 10))
+    u.nee = int(s.get('nee', 1))
+    u.seed = int(s.get('seed', 20261203))
+    u.envRadius = float(sc['envRadius'])
+    u.cam = make_camera(made['camera'])
+    u.env = make_env(sc['env'])
+    u.haze = make_haze(sc.get('haze'))
+    for k in ('maxiter', 'marchRefill', 'rr', 'rrStart', 'diffuseCap', 'maxaverages'):
+        if k in s:
+            setattr(u, k, int(s[k]))
+    for k in ('stepscale', 'eps', 'epsAuto', 'faceShare', 'fireflyClamp'):
+        if k in s:
+            setattr(u, k, float(s[k]))
+    return u
+
+
+class Engine:
+    """One libpt context on a GPU."""
+
+    def __init__(self, device=0, blocks_per_cu=0, cache_dir=None):
+        self.lib = load_library()
+        o = PtOpts()
+        o.device, o.blocks_per_cu = device, blocks_per_cu
+        o.cache_dir = cache_dir.encode() if cache_dir else None
+        h = ctypes.c_void_p()
+        rc = self.lib.pt_create(ctypes.byref(h), ctypes.byref(o))
+        if rc:
+            raise PtError(rc, self.lib.pt_last_error(None).decode())
+        self.h = h
+        self.width = self.height = 0
+        self._keep = []
+
+    def _chk(self, rc):
+        if rc:
+            raise PtError(rc, self.lib.pt_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            self.lib.pt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_scene(self, scene):
+        mats = (PtMaterial * len(scene['materials']))()
+        for i, m in enumerate(scene['materials']):
+            _set3(mats[i].base, m['base']); _set3(mats[i].emissive, m['emissive'])
+            mats[i].metallic, mats[i].roughness, mats[i].transmission = m['metallic'], m['roughness'], m['transmission']
+            mats[i].ior, mats[i].specular, mats[i].abbe = m['ior'], m['specular'], m['abbe']
+            mats[i].face, mats[i].specularTint = int(m['face']), m.get('specularTint', 0.0)
+        nf = len(scene['faces'])
+        faces = (PtLightFace * max(nf, 1))()
+        for i, f in enumerate(scene['faces']):
+            for k in ('centre', 'u', 'v', 'n', 'le'):
+                _set3(getattr(faces[i], k), f[k])
+            faces[i].halfx, faces[i].halfy, faces[i].shape = f['halfx'], f['halfy'], int(f['shape'])
+            faces[i].twoSided, faces[i].weight = int(f['twoSided']), f['weight']
+        params = np.ascontiguousarray(np.asarray(scene['params'], dtype=np.float32))
+        d = PtSceneDesc()
+        d.hip_src = scene['hip'].encode()
+        d.params = params.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        d.nparams = len(params)
+        d.materials, d.nmaterials = mats, len(scene['materials'])
+        d.faces, d.nfaces = faces, nf
+        self._keep = [params, mats, faces, d]
+        self._chk(self.lib.pt_set_scene(self.h, ctypes.byref(d)))
+
+    def set_params(self, params):
+        a = np.ascontiguousarray(np.asarray(params, dtype=np.float32))
+        self._chk(self.lib.pt_set_params(self.h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(a)))
+
+    def set_uniforms(self, u):
+        self._chk(self.lib.pt_set_uniforms(self.h, ctypes.byref(u), ctypes.sizeof(u)))
+        self.width, self.height = u.width, u.height
+
+    def clear_accums(self):
+        self._chk(self.lib.pt_clear_accum(self.h))
+
+    def trace(self, samples=1):
+        self._chk(self.lib.pt_trace(self.h, samples))
+
+    def sync(self):
+        self._chk(self.lib.pt_sync(self.h))
+
+    def read_accum(self):
+        """(accum [h, w, 4] mean rgb + samples, moments [h, w, 2] V, S) as float64 arrays; row 0 is the bottom."""
+        n = self.width * self.height
+        acc = np.zeros((n, 4), np.float64)
+        mom = np.zeros((n, 2), np.float64)
+        self._chk(self.lib.pt_read_accum(self.h, acc.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                         mom.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n))
+        return acc.reshape(self.height, self.width, 4), mom.reshape(self.height, self.width, 2)
+
+    def stats(self):
+        a = (ctypes.c_uint64 * 16)()
+        self._chk(self.lib.pt_read_stats(self.h, a, 16))
+        return {k: int(a[i]) for i, k in enumerate(STAT_NAMES)}
+
+    def reset_stats(self):
+        self._chk(self.lib.pt_reset_stats(self.h))
+
+    def time_kernels(self, samples=1):
+        ms = (ctypes.c_double * 5)()
+        self._chk(self.lib.pt_time_kernels(self.h, samples, ms, 5))
+        return dict(zip(['genrays', 'march', 'shade', 'occlude', 'resolve'], list(ms)))
+
+    def image_stats(self):
+        """Luminance statistics of the accumulated image (R.imageStats, raytrace_main.js:876-956): mean, and the
+        per-pixel standard error sqrt(S V / (1 - S)) (core/measure.js:49-52)."""
+        acc, mom = self.read_accum()
+        return image_stats(acc, mom)
+
+
+def luminance(acc):
+    return LUMA[0] * acc[..., 0] + LUMA[1] * acc[..., 1] + LUMA[2] * acc[..., 2]
+
+
+def standard_error(mom):
+    V, S = mom[..., 0], mom[..., 1]
+    ok = (S > 0) & (S < 1) & (V >= 0)
+    se = np.full(V.shape, np.nan)
+    se[ok] = np.sqrt(S[ok] * V[ok] / (1 - S[ok]))
+    return se
+
+
+def image_stats(acc, mom):
+    L = luminance(acc)
+    se = standard_error(mom)
+    have = acc[..., 3] > 0
+    return {'meanLuma': float(L[have].mean()) if have.any() else 0.0, 'samples': float(acc[..., 3].max()),
+            'relSE': float(np.nanmean(se[have] / np.maximum(L[have], 1e-30))) if have.any() else math.nan}
