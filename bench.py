@@ -1,0 +1,128 @@
+"""Benchmark: Mpath-segments/s of the wavefront SDF path tracer on one MI355X.
+
+Workload (the plain method at scale): the 5 x 4 x 3 m room with 32 seeded primitives, 8 smooth-union pairs, GGX
+materials and a 5000 cd/m^2 ceiling panel; 1920 x 1080, depth 8, NEE on, FP32 path state, FP64 accumulators. A
+step is one sample: one camera path per pixel to its end (<= 9 segments), with its shadow rays, resolved into the
+FP64 accumulators. `value` counts path segments (a camera ray or bounce marched to its next event and shaded) over
+the timed steps; shadow rays are reported beside it.
+
+    python bench.py [--gpus 1] [--steps 32] [--warmup 4] [--no-cpu]
+
+Prints one JSON line. The roofline is the dominant kernel's (sphere tracing: FP32 VALU), from HIP events on the
+engine's stream; `cpu_baseline` times the FP64 C oracle on this host's cores on a bounded slice of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 vector peak (published)
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(made):
+    """The oracle (FP64, OpenMP) on a bounded slice: the same room at 480 x 270 (1/16 of the pixels), 4 samples."""
+    sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+    import oracle
+    from shadytracer_amd import pt, scenes
+    small = scenes.emit('room-c2', width=480, height=270)
+    u = pt.uniforms_for(small)
+    threads = int(os.environ.get('OMP_NUM_THREADS', '0') or 0) or os.cpu_count() or 1
+    t0 = time.time()
+    _, _, st = oracle.render(small['scene'], u, 4, first=0, threads=threads)
+    dt = time.time() - t0
+    return {'value': st['segments'] / dt / 1e6, 'unit': 'Mpath-segments/s', 'cores': threads, 'kind': 'port',
+            'sample': 'the same room at 480x270 (1/16 of the 1920x1080 pixels), 4 samples, FP64 C oracle with OpenMP; '
+                      '%d segments in %.1f s' % (st['segments'], dt)}
+
+
+def traffic_from_profiles():
+    """HBM bytes per march launch, from the committed rocprofv3 --pmc summary (profiles/), if there is one."""
+    p = os.path.join(ROOT, 'profiles', 'march_room-c2_1080p_traffic.json')
+    if not os.path.exists(p):
+        return None
+    try:
+        return json.load(open(p))['bytes_per_launch']
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=32)
+    ap.add_argument('--warmup', type=int, default=4)
+    ap.add_argument('--width', type=int, default=1920)
+    ap.add_argument('--height', type=int, default=1080)
+    ap.add_argument('--no-cpu', action='store_true')
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
+        print('bench.py: this build runs on one GPU; --gpus %d is not supported (samples would shard across GPUs with an '
+              'all-reduce of the FP64 accumulators, not built here)' % a.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from shadytracer_amd import pt, scenes
+    made = scenes.emit('room-c2', width=a.width, height=a.height)
+    log('bench: room-c2 %dx%d, warmup %d, steps %d' % (a.width, a.height, a.warmup, a.steps))
+    e = pt.Engine()
+    e.set_scene(made['scene'])
+    u = pt.uniforms_for(made)
+    e.set_uniforms(u)
+    e.clear_accums()
+    e.trace(a.warmup)
+    e.sync()
+    log('bench: warm')
+    e.reset_stats()
+    e.sync()
+    t0 = time.perf_counter()
+    e.trace(a.steps)
+    e.sync()
+    dt = time.perf_counter() - t0
+    st = e.stats()
+    log('bench: timed %.3f s, %s' % (dt, st))
+
+    # the dominant kernel, timed launch by launch with HIP events on the engine's stream
+    e.reset_stats()
+    ksamples = 4
+    ms = e.time_kernels(ksamples)
+    kst = e.stats()
+    rounds = u.maxbounce + 1
+    flops_per_step = made['scene']['flops']['dist'] + 12      # the scene's SDF plus the step's own arithmetic
+    dom = max(('march', 'occlude'), key=lambda k: ms[k])
+    steps_k = kst['steps'] if dom == 'march' else kst['shadow_steps']
+    launches = ksamples * rounds
+    avg_launch_ms = ms[dom] / launches
+    flops_per_launch = steps_k * flops_per_step / launches
+    achieved = flops_per_launch / (avg_launch_ms * 1e-3) / 1e12
+    traffic = traffic_from_profiles()
+    roof = {'kernel': 'pt_' + dom, 'bound': 'valu', 'achieved': round(achieved, 3), 'peak': PEAK_FP32_TFLOPS, 'unit': 'TFLOP/s',
+            'frac': round(achieved / PEAK_FP32_TFLOPS, 4), 'traffic': traffic,
+            'flops_per_launch': flops_per_launch, 'avg_launch_ms': avg_launch_ms,
+            'flops_per_step': flops_per_step, 'kernel_ms_per_sample': {k: v / ksamples for k, v in ms.items()}}
+    cpu = None if a.no_cpu else cpu_baseline(made)
+    segs = st['segments']
+    out = {
+        'metric': 'Mpath-segments/s', 'value': segs / dt / 1e6, 'unit': 'Mpath-segments/s', 'n_gpus': 1, 'steps': a.steps,
+        'warmup': a.warmup, 'ms_per_step': dt * 1e3 / a.steps, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f32', 'data': 'synthetic: seeded procedural room (room-c2), emitted by the project\'s JS emitter',
+        'config': {'workload': 'room-c2: 5x4x3 m room, 32 SDF primitives, 8 smooth unions, GGX, 0.5 m panel 5000 cd/m^2, '
+                               '%dx%d, depth 8, NEE on, 1 sample per step' % (a.width, a.height),
+                   'width': a.width, 'height': a.height, 'maxbounce': u.maxbounce, 'accumulation': 'f64'},
+        'shadow_segments_per_s': st['shadow_segments'] / dt, 'march_steps_per_s': st['steps'] / dt,
+        'mean_steps_per_segment': st['steps'] / max(segs, 1),
+        'roofline': roof, 'cpu_baseline': cpu,
+    }
+    print(json.dumps(out), flush=True)
+    e.close()
+
+
+if __name__ == '__main__':
+    main()

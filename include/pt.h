@@ -108,6 +108,12 @@ int pt_reset_stats(pt_ctx* ctx);
  * Replaces R.timeKernels (raytrace_main.js:549-557). */
 int pt_time_kernels(pt_ctx* ctx, uint32_t samples, double* ms, size_t n);
 
+/* Runs one sample's camera rays to their first event and reports, per ray, where each stopped (pos4: x, y, z,
+ * optical length) and what it hit (info: state | material << 8, then the primitive; 0xFFFFFFFF none). n must be
+ * the camera rays of a sample (width*height). The sample counter is not advanced. Replaces R.pick
+ * (lib/pathtracer/core/pick.js) for every pixel at once. */
+int pt_pick_rays(pt_ctx* ctx, float* pos4, uint32_t* info, size_t n);
+
 #ifdef __cplusplus
 }
 #endif

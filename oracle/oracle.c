@@ -1,0 +1,907 @@
+/*
+ * THE ORACLE -- test infrastructure only: a CPU restatement, in FP64, of the estimators of ShadyTracer's
+ * wavefront path tracer (lib/pathtracer), against which the HIP engine is checked. Nothing in the product
+ * links or calls it; tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg do, through ctypes.
+ *
+ * It reads a scene as the emitter's postfix program (shadytracer_amd/js/hipgen.js `program`), the same
+ * parameter block, materials, light faces and uniforms (include/pt_types.h), and renders it path by path
+ * (depth first, a shadow ray traced where it is made) rather than in wavefronts. Each function cites the
+ * reference file:line it restates. Random numbers: PCG4D keyed like the engine's (sample seed, ray, bounce,
+ * stream), so a camera ray's jitter -- and with it its first march -- is the engine's own.
+ *
+ * Pinned by (tests/test_oracle.py): closed forms (Lambertian floor = rho L; white furnace = 1; haze furnace =
+ * 1; Fresnel at normal incidence and total internal reflection; Cauchy through n_d and V_d), brute-force
+ * ray-primitive intersection on small inputs, quadrature invariants of the BSDF (pdf integrates to its
+ * non-delta share; sampling weights agree with eval/pdf), the reference's own measured vectors
+ * (lib/pathtracer/test/fixtures/run-0.77.0-pascal.json via tests/golden), and the error-bar identity of
+ * core/measure.js (tests/golden/measure_vectors.json).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/pt_types.h"
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef struct { double x, y, z; } d3;
+typedef struct { double x, y, z, w; } d4;
+
+static inline d3 v(double x, double y, double z) { d3 r = {x, y, z}; return r; }
+static inline d3 add(d3 a, d3 b) { return v(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline d3 sub(d3 a, d3 b) { return v(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline d3 mul(d3 a, d3 b) { return v(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline d3 scl(d3 a, double s) { return v(a.x * s, a.y * s, a.z * s); }
+static inline double dotp(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline double len(d3 a) { return sqrt(dotp(a, a)); }
+static inline d3 unit(d3 a) { double l = len(a); return l > 0 ? scl(a, 1.0 / l) : a; }
+static inline d3 neg(d3 a) { return v(-a.x, -a.y, -a.z); }
+static inline d3 crossp(d3 a, d3 b) { return v(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static inline double maxd(double a, double b) { return a > b ? a : b; }
+static inline double mind(double a, double b) { return a < b ? a : b; }
+static inline double clampd(double x, double a, double b) { return x < a ? a : (x > b ? b : x); }
+static inline double sgn(double x) { return x > 0 ? 1.0 : (x < 0 ? -1.0 : 0.0); }
+static inline double max3d(d3 a) { return maxd(a.x, maxd(a.y, a.z)); }
+static inline double lumad(d3 c) { return 0.2126 * c.x + 0.7152 * c.y + 0.0722 * c.z; }
+static inline d3 f3(const float* p) { return v(p[0], p[1], p[2]); }
+static inline d4 d4of(d3 n, double w) { d4 r = {n.x, n.y, n.z, w}; return r; }
+
+#define PI_D 3.14159265358979323846
+
+/* ---- PCG4D (Jarzynski & Olano, JCGT 2020), as core/tracebasics.js:369-380 uses it ---- */
+typedef struct { uint32_t a, b, c, d; } OrRng;
+static void pcg4d(OrRng* s) {
+    uint32_t x = s->a * 1664525u + 1013904223u, y = s->b * 1664525u + 1013904223u;
+    uint32_t z = s->c * 1664525u + 1013904223u, w = s->d * 1664525u + 1013904223u;
+    x += y * w; y += z * x; z += x * y; w += y * z;
+    x ^= x >> 16; y ^= y >> 16; z ^= z >> 16; w ^= w >> 16;
+    x += y * w; y += z * x; z += x * y; w += y * z;
+    s->a = x; s->b = y; s->c = z; s->d = w;
+}
+static void rng4(OrRng* s, double* u) {
+    pcg4d(s);
+    u[0] = (double)(s->a >> 8) / 16777216.0; u[1] = (double)(s->b >> 8) / 16777216.0;
+    u[2] = (double)(s->c >> 8) / 16777216.0; u[3] = (double)(s->d >> 8) / 16777216.0;
+}
+static OrRng rngAt(const uint32_t* seed, uint32_t ray, uint32_t bounce, uint32_t stream) {
+    OrRng s = {seed[0] ^ ray, seed[1] ^ (bounce * 0x9E3779B9u), seed[2] ^ stream, seed[3]};
+    pcg4d(&s);
+    return s;
+}
+static void sampleSeed(uint32_t base, uint32_t sample, uint32_t* out) {
+    OrRng r = {base, sample, 0x5EED5EEDu, 0x2026F00Du};
+    pcg4d(&r); pcg4d(&r);
+    out[0] = r.a; out[1] = r.b; out[2] = r.c; out[3] = r.d;
+}
+
+/* ---- the scene program (hipgen.js `program`) ---- */
+enum { OP_PRIM = 0, OP_JOIN = 1, OP_FOLD = 2 };
+enum { SH_PLANE = 0, SH_SPHERE, SH_BOX, SH_CYLINDER, SH_CAPSULE, SH_TORUS, SH_RECT, SH_DISC };
+enum { J_UNION = 0, J_SUBTRACT, J_INTERSECT };
+typedef struct { int op, shape, xf, p, mat, prim, join, k; } OrOp;
+
+typedef struct {
+    const OrOp* ops;
+    int nops;
+    const float* U;
+    const PtMaterial* mats;
+    int nmats;
+    const PtLightFace* faces;
+    int nfaces;
+    double faceTotal;
+    const PtUniforms* u;
+} OrScene;
+
+/* Primitive (outward normal, distance) in its own frame: core/sdf.js:352-628 restated. */
+static d4 primLocal(int shape, d3 p, const float* P) {
+    switch (shape) {
+    case SH_PLANE: return d4of(v(0, 0, 1), p.z);
+    case SH_SPHERE: { double l = len(p); return d4of(l > 1e-300 ? scl(p, 1 / l) : v(0, 0, 1), l - P[0]); }
+    case SH_BOX: {
+        double rr = clampd(P[3], 0, mind(P[0], mind(P[1], P[2])));
+        d3 b = v(P[0] - rr, P[1] - rr, P[2] - rr);
+        d3 q = v(fabs(p.x) - b.x, fabs(p.y) - b.y, fabs(p.z) - b.z);
+        d3 g = v(maxd(q.x, 0), maxd(q.y, 0), maxd(q.z, 0));
+        double d = len(g) + mind(maxd(q.x, maxd(q.y, q.z)), 0) - rr;
+        d3 n;
+        if (g.x > 0 || g.y > 0 || g.z > 0) n = mul(unit(g), v(sgn(p.x), sgn(p.y), sgn(p.z)));
+        else if (q.x > q.y && q.x > q.z) n = v(sgn(p.x), 0, 0);
+        else if (q.y > q.z) n = v(0, sgn(p.y), 0);
+        else n = v(0, 0, sgn(p.z));
+        return d4of(n, d);
+    }
+    case SH_CYLINDER: {
+        double k = clampd(P[2], 0, mind(P[0], P[1]));
+        double h = P[0] - k, r = P[1] - k;
+        double qx = sqrt(p.x * p.x + p.y * p.y), qy = fabs(p.z);
+        double dx = qx - r, dy = qy - h, ox = maxd(dx, 0), oy = maxd(dy, 0), ol = sqrt(ox * ox + oy * oy);
+        double dist = mind(maxd(dx, dy), 0) + ol - k;
+        double l = maxd(qx, 1e-8);
+        d3 side = v(p.x / l, p.y / l, 0), cap = v(0, 0, sgn(p.z)), n;
+        if (dx > 0 && dy > 0) { double il = 1 / maxd(ol, 1e-8); n = unit(add(scl(side, ox * il), scl(cap, oy * il))); }
+        else if (dx > dy) n = side;
+        else n = cap;
+        return d4of(n, dist);
+    }
+    case SH_CAPSULE: {
+        d3 w = sub(p, v(0, 0, clampd(p.z, -P[0], P[0])));
+        double l = len(w);
+        return d4of(l > 1e-12 ? scl(w, 1 / l) : v(1, 0, 0), l - P[1]);
+    }
+    case SH_TORUS: {
+        double lxy = sqrt(p.x * p.x + p.y * p.y);
+        double rx = lxy > 1e-12 ? p.x / lxy : 1, ry = lxy > 1e-12 ? p.y / lxy : 0;
+        double qx = lxy - P[0], qy = p.z, ql = sqrt(qx * qx + qy * qy);
+        d3 n = ql > 1e-12 ? v(rx * qx / ql, ry * qx / ql, qy / ql) : v(0, 0, 1);
+        return d4of(n, ql - P[1]);
+    }
+    case SH_RECT: {
+        double ex = maxd(fabs(p.x) - P[0], 0), ey = maxd(fabs(p.y) - P[1], 0);
+        double s = p.z < 0 ? -1 : 1, d = sqrt(ex * ex + ey * ey + p.z * p.z);
+        if (maxd(ex, ey) <= 0 || d < 1e-12) return d4of(v(0, 0, s), fabs(p.z));
+        return d4of(v(ex / d * sgn(p.x), ey / d * sgn(p.y), p.z / d), d);
+    }
+    case SH_DISC: {
+        double l = sqrt(p.x * p.x + p.y * p.y), e = maxd(l - P[0], 0);
+        double s = p.z < 0 ? -1 : 1, d = sqrt(e * e + p.z * p.z);
+        if (e <= 0 || d < 1e-12) return d4of(v(0, 0, s), fabs(p.z));
+        double il = 1 / maxd(l, 1e-12);
+        return d4of(v(e * p.x * il / d, e * p.y * il / d, p.z / d), d);
+    }
+    }
+    return d4of(v(0, 0, 1), 1e30);
+}
+
+/* a node's frame: U[xf..xf+12] = position, rotation columns, scale (hipgen.js xfParams) */
+static d4 primWorld(const OrScene* S, const OrOp* o, d3 p) {
+    const float* t = S->U + o->xf;
+    d3 c0 = v(t[3], t[4], t[5]), c1 = v(t[6], t[7], t[8]), c2 = v(t[9], t[10], t[11]);
+    d3 d = sub(p, v(t[0], t[1], t[2]));
+    double s = t[12];
+    d3 q = v(dotp(d, c0) / s, dotp(d, c1) / s, dotp(d, c2) / s);
+    d4 l = primLocal(o->shape, q, S->U + o->p);
+    d3 n = add(add(scl(c0, l.x), scl(c1, l.y)), scl(c2, l.z));
+    return d4of(n, l.w * s);
+}
+
+/* CSG joins (core/scene_api.js:352-373): polynomial smooth min/max for k > 0, the normal mixed */
+static d3 csgN(d3 a, d3 b, double t) {
+    if (dotp(a, a) < 1e-12 || dotp(b, b) < 1e-12) return v(0, 0, 0);
+    d3 n = add(a, scl(sub(b, a), t));
+    double l = len(n);
+    return l > 1e-12 ? scl(n, 1 / l) : v(0, 0, 0);
+}
+static d4 joinU(d4 a, d4 b, double k) {
+    if (k <= 0) return a.w < b.w ? a : b;
+    double h = maxd(k - fabs(a.w - b.w), 0), m = a.w < b.w ? 0 : 1, t = m + (1 - 2 * m) * 0.5 * h / k;
+    return d4of(csgN(v(a.x, a.y, a.z), v(b.x, b.y, b.z), t), mind(a.w, b.w) - h * h * 0.25 / k);
+}
+static d4 joinI(d4 a, d4 b, double k) {
+    if (k <= 0) return a.w > b.w ? a : b;
+    double h = maxd(k - fabs(a.w - b.w), 0), m = a.w > b.w ? 0 : 1, t = m + (1 - 2 * m) * 0.5 * h / k;
+    return d4of(csgN(v(a.x, a.y, a.z), v(b.x, b.y, b.z), t), maxd(a.w, b.w) + h * h * 0.25 / k);
+}
+
+typedef struct { d3 n; double d; int m, a, prim; double insideW; } OrHit;
+
+/* The closest-surface fold over the program (scene_api.js:39-63 fold; codegen.js:1821-1828 joins). */
+static OrHit sceneAt(const OrScene* S, d3 p) {
+    d4 st[64]; int sm[64], sk[64]; int top = 0;
+    OrHit h; h.n = v(0, 0, 1); h.d = 1e7; h.m = 0; h.a = 0; h.prim = -1; h.insideW = -1e7;
+    for (int i = 0; i < S->nops; ++i) {
+        const OrOp* o = &S->ops[i];
+        if (o->op == OP_PRIM) { st[top] = primWorld(S, o, p); sm[top] = o->mat; sk[top] = o->prim; ++top; }
+        else if (o->op == OP_JOIN) {
+            d4 b = st[--top], a = st[top - 1];
+            double k = S->U[o->k];
+            int wins = o->join == J_UNION ? b.w < a.w : o->join == J_SUBTRACT ? -b.w > a.w : b.w > a.w;
+            if (wins) { sm[top - 1] = sm[top]; sk[top - 1] = sk[top]; }
+            if (o->join == J_UNION) st[top - 1] = joinU(a, b, k);
+            else if (o->join == J_SUBTRACT) { d4 nb = {-b.x, -b.y, -b.z, -b.w}; st[top - 1] = joinI(a, nb, k); }
+            else st[top - 1] = joinI(a, b, k);
+        } else {
+            d4 s = st[--top];
+            if (s.w < 0 && s.w > h.insideW) { h.insideW = s.w; h.a = sm[top]; }
+            if (fabs(s.w) < fabs(h.d)) { h.n = v(s.x, s.y, s.z); h.d = s.w; h.m = sm[top]; h.prim = sk[top]; }
+        }
+    }
+    return h;
+}
+static double sceneDist(const OrScene* S, d3 p) { return sceneAt(S, p).d; }
+
+/* THE HIT TOLERANCE (core/tolerances.js:10-34) */
+static double tolSetting(const OrScene* S) { return S->u->eps > 0 ? S->u->eps : S->u->epsAuto; }
+static double marchTol(const OrScene* S, d3 p) {
+    return maxd(maxd(tolSetting(S), 1e-9), 128.0 * pow(2.0, -23) * maxd(fabs(p.x), maxd(fabs(p.y), fabs(p.z))));
+}
+static double offsetTol(const OrScene* S, d3 p) { return 10.0 * marchTol(S, p); }
+
+/* ---- GGX, Fresnel, albedo tables (core/microfacet.js:72-140, :245-330) ---- */
+#define ALPHA_MIN 1e-3
+static double mfLambda(d3 w, double a) { double z2 = maxd(w.z * w.z, 1e-12); return 0.5 * (-1 + sqrt(1 + a * a * (w.x * w.x + w.y * w.y) / z2)); }
+static double mfG1(d3 w, double a) { return 1 / (1 + mfLambda(w, a)); }
+static double mfG2(d3 o, d3 i, double a) { return 1 / (1 + mfLambda(o, a) + mfLambda(i, a)); }
+static double mfD(d3 h, double a) {
+    if (h.z <= 0) return 0;
+    double t = (h.x * h.x + h.y * h.y) / (a * a) + h.z * h.z;
+    return 1 / (PI_D * a * a * t * t);
+}
+static d3 mfVNDF(d3 o, double a, double u1, double u2) {
+    d3 os = unit(v(o.x * a, o.y * a, o.z));
+    double phi = 2 * PI_D * u1, z = (1 - u2) * (1 + os.z) - os.z, s = sqrt(clampd(1 - z * z, 0, 1));
+    d3 hs = add(v(s * cos(phi), s * sin(phi), z), os);
+    return unit(v(hs.x * a, hs.y * a, maxd(hs.z, 1e-7)));
+}
+static double mfPdfR(d3 o, d3 i, double a) {
+    if (o.z <= 0 || i.z <= 0) return 0;
+    return mfG1(o, a) * mfD(unit(add(o, i)), a) / (4 * o.z);
+}
+double or_fresnel(double cosI, double eta) {
+    cosI = clampd(cosI, 0, 1);
+    double s2 = eta * eta * (1 - cosI * cosI);
+    if (s2 >= 1) return 1;
+    double ct = sqrt(1 - s2), rs = (eta * cosI - ct) / (eta * cosI + ct), rp = (cosI - eta * ct) / (cosI + eta * ct);
+    return 0.5 * (rs * rs + rp * rp);
+}
+static d3 schlick3(double c, d3 f0) { double m = pow(1 - clampd(c, 0, 1), 5); return add(f0, scl(sub(v(1, 1, 1), f0), m)); }
+static d3 reflL(d3 o, d3 h) { return sub(scl(h, 2 * dotp(o, h)), o); }
+
+static double g_mf[PT_MF_WORDS];
+static int g_mfReady = 0;
+/* The tables by stratified visible-normal quadrature (64 x 64 per entry), FP64. */
+void or_mf_tables(double* out) {
+    if (!g_mfReady) {
+        #pragma omp parallel for schedule(dynamic)
+        for (int t = 0; t < PT_MF_NIOR * PT_MF_NR * PT_MF_NMU + PT_MF_NR * PT_MF_NMU; ++t) {
+            int e1 = t >= PT_MF_NIOR * PT_MF_NR * PT_MF_NMU;
+            int k = e1 ? t - PT_MF_NIOR * PT_MF_NR * PT_MF_NMU : t;
+            int im = k % PT_MF_NMU, ir = (k / PT_MF_NMU) % PT_MF_NR, ii = e1 ? 0 : k / (PT_MF_NMU * PT_MF_NR);
+            double mu = (im + 0.5) / PT_MF_NMU, r = (double)ir / (PT_MF_NR - 1), nrel = 1 + 2.0 * ii / (PT_MF_NIOR - 1);
+            double a = clampd(r * r, ALPHA_MIN, 1);
+            d3 o = v(sqrt(maxd(0, 1 - mu * mu)), 0, mu);
+            double acc = 0;
+            for (int sa = 0; sa < 64; ++sa) for (int sb = 0; sb < 64; ++sb) {
+                d3 h = mfVNDF(o, a, (sa + 0.5) / 64, (sb + 0.5) / 64);
+                d3 i = reflL(o, h);
+                if (i.z <= 0) continue;
+                acc += (e1 ? 1 : or_fresnel(dotp(o, h), 1 / nrel)) * mfG2(o, i, a) / mfG1(o, a);
+            }
+            acc /= 4096.0;
+            if (e1) g_mf[PT_MF_E1 + ir * PT_MF_NMU + im] = acc;
+            else g_mf[PT_MF_EC + (ii * PT_MF_NR + ir) * PT_MF_NMU + im] = acc;
+        }
+        for (int t = 0; t < PT_MF_NIOR * PT_MF_NR; ++t) {
+            double acc = 0;
+            for (int im = 0; im < PT_MF_NMU; ++im) acc += g_mf[PT_MF_EC + t * PT_MF_NMU + im] * (im + 0.5) / PT_MF_NMU;
+            g_mf[PT_MF_ECAVG + t] = 2 * acc / PT_MF_NMU;
+        }
+        g_mfReady = 1;
+    }
+    if (out) memcpy(out, g_mf, sizeof g_mf);
+}
+static void axis(double x, int n, int* i0, double* f) {
+    double c = clampd(x, 0, n - 1), fl = mind(floor(c), n - 2);
+    *i0 = (int)fl; *f = c - fl;
+}
+static double lutEc(double mu, double r, double nrel) {
+    int im, ir, in; double fm, fr, fn;
+    axis(mu * PT_MF_NMU - 0.5, PT_MF_NMU, &im, &fm); axis(r * (PT_MF_NR - 1), PT_MF_NR, &ir, &fr);
+    axis((nrel - 1) * (PT_MF_NIOR - 1) * 0.5, PT_MF_NIOR, &in, &fn);
+    double acc = 0;
+    for (int c = 0; c < 8; ++c) {
+        int di = c & 1, dk = (c >> 1) & 1, dj = (c >> 2) & 1;
+        acc += (di ? fn : 1 - fn) * (dk ? fr : 1 - fr) * (dj ? fm : 1 - fm) * g_mf[PT_MF_EC + ((in + di) * PT_MF_NR + ir + dk) * PT_MF_NMU + im + dj];
+    }
+    return acc;
+}
+static double lutEcAvg(double r, double nrel) {
+    int ir, in; double fr, fn;
+    axis(r * (PT_MF_NR - 1), PT_MF_NR, &ir, &fr); axis((nrel - 1) * (PT_MF_NIOR - 1) * 0.5, PT_MF_NIOR, &in, &fn);
+    double acc = 0;
+    for (int c = 0; c < 4; ++c) { int di = c & 1, dk = (c >> 1) & 1; acc += (di ? fn : 1 - fn) * (dk ? fr : 1 - fr) * g_mf[PT_MF_ECAVG + (in + di) * PT_MF_NR + ir + dk]; }
+    return acc;
+}
+static double lutE1(double mu, double r) {
+    int im, ir; double fm, fr;
+    axis(mu * PT_MF_NMU - 0.5, PT_MF_NMU, &im, &fm); axis(r * (PT_MF_NR - 1), PT_MF_NR, &ir, &fr);
+    double acc = 0;
+    for (int c = 0; c < 4; ++c) { int dk = c & 1, dj = (c >> 1) & 1; acc += (dk ? fr : 1 - fr) * (dj ? fm : 1 - fm) * g_mf[PT_MF_E1 + (ir + dk) * PT_MF_NMU + im + dj]; }
+    return acc;
+}
+
+/* ---- the layered BSDF (core/materials_pbr.js:74-553) ---- */
+typedef struct {
+    d3 tX, tY, nZ, o, base, tint, cm;
+    double a, r, metallic, trans, coatK, eta, nrel, ps, ecO, ecAvg;
+    int isDelta;
+} OrLobes;
+static void basis(d3 n, d3* t, d3* b) {
+    /* any orthonormal frame will do for isotropic lobes; Frisvad's, restated */
+    if (n.z < -0.9999999) { *t = v(0, -1, 0); *b = v(-1, 0, 0); return; }
+    double a = 1 / (1 + n.z), c = -n.x * n.y * a;
+    *t = v(1 - n.x * n.x * a, c, -n.x);
+    *b = v(c, 1 - n.y * n.y * a, -n.y);
+}
+static double coatF(const OrLobes* L, double c) { return mind(L->coatK * or_fresnel(c, L->eta), 1); }
+static double coatAlb(const OrLobes* L, double mu) { return mind(L->coatK * lutEc(mu, L->r, L->nrel), 1); }
+static OrLobes makeLobes(d3 n, d3 view, const PtMaterial* m, double eta) {
+    OrLobes L;
+    basis(n, &L.tX, &L.tY); L.nZ = n;
+    L.o = v(dotp(view, L.tX), dotp(view, L.tY), maxd(dotp(view, n), 1e-4));
+    L.r = clampd(m->roughness, 0, 1);
+    L.isDelta = L.r * L.r < ALPHA_MIN;
+    L.a = clampd(L.r * L.r, ALPHA_MIN, 1);
+    L.metallic = clampd(m->metallic, 0, 1);
+    L.trans = clampd(m->transmission, 0, 1);
+    L.base = f3(m->base); L.tint = L.base;
+    L.coatK = 2 * m->specular + (1 - 2 * m->specular) * L.trans;
+    L.eta = eta; L.nrel = clampd(1 / eta, 1, 3);
+    double fo = coatF(&L, L.o.z);
+    L.ps = mind(fo, or_fresnel(L.o.z, L.eta) >= 1 ? 1 : 0.999);
+    L.ecO = coatAlb(&L, L.o.z);
+    L.ecAvg = mind(L.coatK * lutEcAvg(L.r, L.nrel), 1);
+    double e1 = L.metallic > 0 ? lutE1(L.o.z, L.r) : 1;
+    L.cm = add(v(1, 1, 1), scl(L.base, 1 / maxd(e1, 1e-3) - 1));
+    return L;
+}
+static d3 toLoc(const OrLobes* L, d3 w) { return v(dotp(w, L->tX), dotp(w, L->tY), dotp(w, L->nZ)); }
+static d3 toWld(const OrLobes* L, d3 w) { return unit(add(add(scl(L->tX, w.x), scl(L->tY, w.y)), scl(L->nZ, w.z))); }
+static d3 lobesEval(const OrLobes* L, d3 l) {
+    d3 i = toLoc(L, l);
+    if (i.z <= 0) return v(0, 0, 0);
+    d3 f = v(0, 0, 0);
+    if (!L->isDelta) {
+        d3 h = unit(add(L->o, i));
+        double spec = mfD(h, L->a) * mfG2(L->o, i, L->a) / (4 * L->o.z * i.z), oh = dotp(L->o, h);
+        if (L->metallic > 0) f = add(f, scl(mul(L->cm, schlick3(oh, L->base)), L->metallic * spec));
+        double c = (1 - L->metallic) * coatF(L, oh) * spec;
+        f = add(f, v(c, c, c));
+    }
+    double wd = (1 - L->metallic) * (1 - L->trans);
+    if (wd > 0) f = add(f, scl(L->base, wd * (1 - L->ecO) * (1 - coatAlb(L, i.z)) / (PI_D * maxd(1 - L->ecAvg, 1e-4))));
+    return f;
+}
+static double reflP(const OrLobes* L, double oh) { double fh = coatF(L, oh), den = fh + (1 - fh) * L->trans; return den > 1e-6 ? fh / den : 1; }
+static double lobesPdf(const OrLobes* L, d3 l) {
+    d3 i = toLoc(L, l);
+    if (i.z <= 0) return 0;
+    double wd = 1 - L->metallic, pd = (1 - L->ps) * (1 - L->trans), p = wd * pd * i.z / PI_D;
+    if (!L->isDelta) p += (L->metallic + wd * (1 - pd) * reflP(L, dotp(L->o, unit(add(L->o, i))))) * mfPdfR(L->o, i, L->a);
+    return p;
+}
+static int lobesNone(const PtMaterial* m) {
+    double r = clampd(m->roughness, 0, 1);
+    return r * r < ALPHA_MIN && (1 - clampd(m->metallic, 0, 1)) * (1 - clampd(m->transmission, 0, 1)) <= 0;
+}
+typedef struct { d3 dir, w; int delta, transmitted, eval; } OrSample;
+static OrSample lobesSample(const OrLobes* L, const double* u) {
+    OrSample s; s.w = v(0, 0, 0); s.dir = L->nZ; s.delta = 0; s.transmitted = 0; s.eval = 1;
+    d3 o = L->o;
+    if (u[0] < L->metallic) {
+        d3 h = L->isDelta ? v(0, 0, 1) : mfVNDF(o, L->a, u[1], u[2]);
+        d3 i = reflL(o, h);
+        if (i.z <= 0) return s;
+        double oh = dotp(o, h);
+        if (L->isDelta) { s.w = schlick3(oh, L->base); s.delta = 1; s.eval = 0; }
+        else s.w = scl(mul(L->cm, schlick3(oh, L->base)), mfG2(o, i, L->a) / mfG1(o, L->a));
+        s.dir = toWld(L, i);
+        return s;
+    }
+    double ux = (u[0] - L->metallic) / maxd(1 - L->metallic, 1e-7), pd = (1 - L->ps) * (1 - L->trans);
+    if (ux < pd) {
+        double r = sqrt(u[1]), ph = 2 * PI_D * u[2];
+        d3 i = v(r * cos(ph), r * sin(ph), sqrt(maxd(0, 1 - u[1])));
+        s.w = scl(L->base, (1 - L->trans) * (1 - L->ecO) * (1 - coatAlb(L, i.z)) / (maxd(1 - L->ecAvg, 1e-4) * pd));
+        s.dir = toWld(L, i);
+        return s;
+    }
+    d3 h = L->isDelta ? v(0, 0, 1) : mfVNDF(o, L->a, u[1], u[2]);
+    double oh = dotp(o, h), pR = reflP(L, oh), fh = coatF(L, oh), uw = (ux - pd) / maxd(1 - pd, 1e-7);
+    if (uw < pR) {
+        d3 i = reflL(o, h);
+        if (i.z <= 0) return s;
+        double geo = L->isDelta ? 1 : mfG2(o, i, L->a) / mfG1(o, L->a);
+        double w = fh * geo / ((1 - pd) * pR);
+        s.w = v(w, w, w); s.dir = toWld(L, i); s.delta = L->isDelta; s.eval = !L->isDelta;
+        return s;
+    }
+    double c = dotp(o, h), k = 1 - L->eta * L->eta * (1 - c * c);
+    if (k <= 0) return s;
+    d3 t = sub(scl(h, L->eta * c - sqrt(k)), scl(o, L->eta));
+    if (t.z >= 0) return s;
+    double geo = L->isDelta ? 1 : mfG2(o, v(t.x, t.y, -t.z), L->a) / mfG1(o, L->a);
+    s.w = scl(L->tint, (1 - fh) * L->trans * geo / ((1 - pd) * (1 - pR)));
+    s.dir = toWld(L, t); s.transmitted = 1; s.delta = L->isDelta; s.eval = 0;
+    return s;
+}
+
+/* ---- spectral (core/spectral.js:121-135) ---- */
+double or_cauchy(double nd, double abbe, double lam) {
+    double iF = 1 / (0.4861 * 0.4861), iC = 1 / (0.6563 * 0.6563), id = 1 / (0.5876 * 0.5876);
+    double B = (nd - 1) / (abbe * (iF - iC)), A = nd - B * id;
+    return A + B / (lam * lam);
+}
+static double wss(double l, double mu, double s1, double s2) { double t = (l - mu) / (l < mu ? s1 : s2); return exp(-0.5 * t * t); }
+void or_spectral_weight(double lamUm, double* rgb) {
+    double l = lamUm * 1000;
+    double x = 1.056 * wss(l, 599.8, 37.9, 31.0) + 0.362 * wss(l, 442.0, 16.0, 26.7) - 0.065 * wss(l, 501.1, 20.4, 26.2);
+    double y = 0.821 * wss(l, 568.8, 46.9, 40.5) + 0.286 * wss(l, 530.9, 16.3, 31.1);
+    double z = 1.217 * wss(l, 437.0, 11.8, 36.0) + 0.681 * wss(l, 459.0, 26.0, 13.8);
+    rgb[0] = maxd(3.2406 * x - 1.5372 * y - 0.4986 * z, 0) / 0.44045557;
+    rgb[1] = maxd(-0.9689 * x + 1.8758 * y + 0.0415 * z, 0) / 0.28846547;
+    rgb[2] = maxd(0.0557 * x - 0.2040 * y + 1.0570 * z, 0) / 0.27323972;
+}
+
+/* ---- environment, sun, light faces ---- */
+static int envLit(const PtEnv* e) { return (e->type != 0 && e->L > 0) || e->sunE > 0; }
+static double sunOmega(const PtEnv* e) { return 2 * PI_D * (1 - e->sunCos); }
+static int inSun(const PtEnv* e, d3 d) { return e->sunE > 0 && dotp(d, f3(e->sunDir)) >= e->sunCos; }
+static d3 envBg(const PtEnv* e, d3 d) {
+    if (e->type == 1 || (e->type == 2 && d.z > 0)) return scl(f3(e->col), e->L);
+    return v(0, 0, 0);
+}
+static d3 envSun(const PtEnv* e, d3 p, d3 d) {
+    if (!inSun(e, d)) return v(0, 0, 0);
+    if (e->sunRadius > 0) {
+        d3 a = f3(e->sunDir), q = sub(p, f3(e->sunPoint)), perp = sub(q, scl(a, dotp(q, a)));
+        if (dotp(perp, perp) > (double)e->sunRadius * e->sunRadius) return v(0, 0, 0);
+    }
+    double L = e->sunE / sunOmega(e);
+    return v(L, L, L);
+}
+static double sunPick(const PtEnv* e) { return e->sunE > 0 ? 0.5 : 0; }
+static double envPdf(const PtEnv* e, d3 L, d3 n) {
+    double ps = sunPick(e);
+    return ps * (inSun(e, L) ? 1 / sunOmega(e) : 0) + (1 - ps) * maxd(dotp(L, n), 0) / PI_D;
+}
+static d3 cosDir(d3 n, double u1, double u2) {
+    d3 t, b; basis(n, &t, &b);
+    double r = sqrt(u1), ph = 2 * PI_D * u2;
+    return unit(add(add(scl(t, r * cos(ph)), scl(b, r * sin(ph))), scl(n, sqrt(maxd(0, 1 - u1)))));
+}
+static d3 sunDir(const PtEnv* e, double u1, double u2) {
+    d3 a = f3(e->sunDir), t, b; basis(a, &t, &b);
+    double c = 1 - u1 * (1 - e->sunCos), s = sqrt(maxd(0, 1 - c * c)), ph = 2 * PI_D * u2;
+    return unit(add(add(scl(t, s * cos(ph)), scl(b, s * sin(ph))), scl(a, c)));
+}
+static double misW(double a, double b) { a *= a; b *= b; return a / maxd(a + b, 1e-20); }
+static double faceArea(const PtLightFace* f) { return f->shape == 1 ? PI_D * f->halfx * f->halfx : 4.0 * f->halfx * f->halfy; }
+static double faceShare(const OrScene* S, int bounces) {
+    if (bounces == 0 || S->nfaces == 0 || S->faceTotal <= 0) return 0;
+    return envLit(&S->u->env) ? S->u->faceShare : 1;
+}
+static int facePick(const OrScene* S, double u) {
+    double acc = 0; int last = 0;
+    for (int k = 0; k < S->nfaces; ++k) if (S->faces[k].weight > 0) { last = k; acc += S->faces[k].weight; if (u < acc) return k; }
+    return last;
+}
+static double faceDensity(const OrScene* S, double share, const PtLightFace* f, double dist, double cosY) {
+    if (share <= 0 || f->weight <= 0) return 0;
+    return share * (f->weight / S->faceTotal) * dist * dist / maxd(faceArea(f) * fabs(cosY), 1e-20);
+}
+
+/* ---- haze (core/haze.js:191-290) ---- */
+static int hazeOn(const OrScene* S) { return S->u->haze.sigmaS > 0; }
+static int hazeSpan(const PtHaze* h, d3 o, d3 d, double* t0, double* t1) {
+    double lo = 0, hi = 1e300, oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+    for (int k = 0; k < 3; ++k) {
+        double c = h->centre[k], e = h->half[k];
+        if (dd[k] == 0) { if (oo[k] < c - e || oo[k] > c + e) return 0; continue; }
+        double a = (c - e - oo[k]) / dd[k], b = (c + e - oo[k]) / dd[k];
+        lo = maxd(lo, mind(a, b)); hi = mind(hi, maxd(a, b));
+    }
+    *t0 = lo; *t1 = hi;
+    return hi > lo;
+}
+static double hazeTr(const PtHaze* h, d3 o, d3 d, double lenMax) {
+    double t0, t1;
+    if (!hazeSpan(h, o, d, &t0, &t1)) return 1;
+    return exp(-h->sigmaS * maxd(mind(t1, lenMax) - t0, 0));
+}
+static double hazeDraw(const PtHaze* h, d3 o, d3 d, double u) {
+    double t0, t1;
+    if (!hazeSpan(h, o, d, &t0, &t1)) return 1e300;
+    double t = t0 - log(1 - u) / h->sigmaS;
+    return t < t1 ? t : 1e300;
+}
+static double hg(double g, double c) { double d = 1 + g * g - 2 * g * c; return (1 - g * g) / (4 * PI_D * d * sqrt(d)); }
+static d3 hgDir(double g, d3 w, double u1, double u2) {
+    double c;
+    if (fabs(g) < 1e-3) c = 1 - 2 * u1;
+    else { double s = (1 - g * g) / (1 - g + 2 * g * u1); c = (1 + g * g - s * s) / (2 * g); }
+    c = clampd(c, -1, 1);
+    double sn = sqrt(maxd(0, 1 - c * c)), ph = 2 * PI_D * u2;
+    d3 t, b; basis(w, &t, &b);
+    return unit(add(add(scl(t, sn * cos(ph)), scl(b, sn * sin(ph))), scl(w, c)));
+}
+
+/* ---- march (core/wavefront.js:834-1005): returns the state; moves p, ltot, steps ---- */
+enum { ST_ACTIVE = 0, ST_ESCAPED = 1, ST_MAXITER = 2, ST_SCATTER = 4 };
+static int march(const OrScene* S, d3* p, d3 dir, double n, double* ltot, uint32_t* steps, double collide) {
+    const PtUniforms* u = S->u;
+    double R2 = (double)u->envRadius * u->envRadius;
+    for (;;) {
+        if (*ltot >= collide) { *p = sub(*p, scl(dir, (*ltot - collide) / n)); *ltot = collide; return ST_SCATTER; }
+        double d = sceneDist(S, *p), ea = fabs(d), eps = marchTol(S, *p);
+        int st = ST_ACTIVE;
+        if (*ltot > 1000 || (*steps & 0xFFFFu) > u->maxiter) st = ST_MAXITER;
+        if (dotp(*p, dir) > 0 && dotp(*p, *p) > R2) st = ST_ESCAPED;
+        if (ea < eps || st != ST_ACTIVE) return st;
+        *steps += 1;
+        double t = ea * u->stepscale;
+        *p = add(*p, scl(dir, t));
+        *ltot += t * n;
+    }
+}
+/* a shadow ray (core/wavefront.js:1481-1692): 1 reached, 0 blocked */
+static int occluded(const OrScene* S, d3 o, d3 dir, double tmax, long long* steps) {
+    const PtUniforms* u = S->u;
+    double R2 = (double)u->envRadius * u->envRadius, trav = 0;
+    d3 p = o;
+    uint32_t k = 0;
+    for (;;) {
+        double d = sceneDist(S, p), ea = fabs(d);
+        if (ea < marchTol(S, p) || k > u->maxiter) return 0;
+        if (ea >= tmax - trav || (dotp(p, dir) > 0 && dotp(p, p) > R2)) return 1;
+        double t = ea * u->stepscale;
+        p = add(p, scl(dir, t)); trav += t; ++k; ++*steps;
+    }
+}
+
+static d3 normalAt(const OrScene* S, d3 p, double h) {
+    double a = sceneDist(S, add(p, v(h, -h, -h))), b = sceneDist(S, add(p, v(-h, -h, h)));
+    double c = sceneDist(S, add(p, v(-h, h, -h))), d = sceneDist(S, add(p, v(h, h, h)));
+    return unit(v(a - b - c + d, -a - b + c + d, -a + b - c + d));
+}
+
+/* ---- camera (core/genrays.js:36-107) ---- */
+typedef struct { uint32_t tilesX, tilesY; double invTX, invTY; } OrTiles;
+static OrTiles tilesOf(uint32_t N, uint32_t w, uint32_t h) {
+    OrTiles t;
+    double r = floor(sqrt((double)N * ((double)w / (double)(h ? h : 1))) + 0.5);
+    t.tilesX = r < 1 ? 1 : (uint32_t)r;
+    t.tilesY = (N + t.tilesX - 1) / t.tilesX;
+    t.invTX = 1.0 / t.tilesX; t.invTY = 1.0 / t.tilesY;
+    return t;
+}
+static void cameraRay(const OrScene* S, const OrTiles* T, const uint32_t* seed, uint32_t i, d3* o, d3* d, uint32_t* pix, double* uhaze) {
+    const PtCamera* c = &S->u->cam;
+    OrRng r = rngAt(seed, i, 0, 0xCA3E8Au);
+    double u4[4]; rng4(&r, u4);
+    uint32_t tx = i % T->tilesX, ty = i / T->tilesX;
+    double uu = clampd((tx + 0.5) * T->invTX + (u4[0] - 0.5) * T->invTX, 0, 1);
+    double vv = clampd((ty + 0.5) * T->invTY + (u4[1] - 0.5) * T->invTY, 0, 1);
+    double x = 2 * uu - 1, y = 2 * vv - 1;
+    d3 right = f3(c->right), up = f3(c->up), fwd = f3(c->fwd);
+    if (c->type == 0) { *o = f3(c->pos); *d = unit(sub(add(scl(right, x * c->fx), scl(up, y * c->fy)), fwd)); }
+    else { *o = add(f3(c->pos), add(scl(right, x * c->fx), scl(up, y * c->fy))); *d = neg(fwd); }
+    double fw = S->u->width, fh = S->u->height;
+    uint32_t px = (uint32_t)clampd(uu * fw, 0, fw - 1), py = (uint32_t)clampd(vv * fh, 0, fh - 1);
+    *pix = px + py * S->u->width;
+    *uhaze = u4[2];
+}
+
+typedef struct { long long segments, shadow, steps, shadowSteps, maxiter, nonfinite; } OrStats;
+
+/* One camera path to its end (core/wavefront.js:1244-1442 shade; core/tracebasics.js:834-1108 path phases;
+ * core/materials_pbr.js:1014-1070 emitterHit, :1110-1454 processMaterial_advanced). Adds to rgb. */
+static void tracePath(const OrScene* S, const OrTiles* T, const uint32_t* seed, uint32_t ray, double* frame, OrStats* st) {
+    const PtUniforms* u = S->u;
+    const PtEnv* env = &u->env;
+    d3 p, dir; uint32_t pix; double uh;
+    cameraRay(S, T, seed, ray, &p, &dir, &pix, &uh);
+    d3 beta = v(1, 1, 1), rad = v(0, 0, 0);
+    double n = 1, ltot = 0, prevPdf = -1, vtxLtot = 0, sunmis = 1, lambda = 0;
+    uint32_t steps = 0, bounces = 0;
+    double collide = hazeOn(S) ? hazeDraw(&u->haze, p, dir, uh) : 1e300;
+    int direct = 0;
+    for (;;) {
+        uint32_t s0 = steps;
+        int state = march(S, &p, dir, n, &ltot, &steps, collide);
+        st->steps += steps - s0;
+        st->segments++;
+        if (state == ST_ESCAPED) {
+            rad = add(rad, scl(mul(beta, add(envBg(env, dir), envSun(env, p, dir))), sunmis));
+            direct = bounces == 0;
+            break;
+        }
+        if (state == ST_MAXITER) { st->maxiter++; break; }
+        OrRng rng = rngAt(seed, ray, bounces, 0x5AADEu ^ ((steps & 0xFFFFu) << 12));
+        if (state == ST_SCATTER) {
+            const PtHaze* hz = &u->haze;
+            bounces++;
+            beta = scl(beta, hz->albedo);
+            double fs = faceShare(S, bounces), u0[4], u1[4];
+            rng4(&rng, u0); rng4(&rng, u1);
+            if (u->nee) {
+                d3 L = v(0, 0, 0), le = v(0, 0, 0); double pS = 0, tmax = 1e300;
+                if (u0[0] < fs) {
+                    const PtLightFace* f = &S->faces[facePick(S, u1[3] * S->faceTotal)];
+                    d3 y = f3(f->centre);
+                    if (f->shape == 1) { double rr = f->halfx * sqrt(u0[1]), ph = 2 * PI_D * u0[2]; y = add(y, scl(add(scl(f3(f->u), cos(ph)), scl(f3(f->v), sin(ph))), rr)); }
+                    else y = add(y, add(scl(f3(f->u), (2 * u0[1] - 1) * f->halfx), scl(f3(f->v), (2 * u0[2] - 1) * f->halfy)));
+                    d3 dy = sub(y, p); double dist = len(dy); L = scl(dy, 1 / maxd(dist, 1e-20));
+                    double cosY = -dotp(f3(f->n), L);
+                    if (dist > 0 && fabs(cosY) > 1e-6 && (cosY > 0 || f->twoSided)) { pS = faceDensity(S, fs, f, dist, cosY); le = f3(f->le); tmax = dist - 2 * offsetTol(S, y); }
+                } else if (envLit(env)) {
+                    double pick = (u0[0] - fs) / maxd(1 - fs, 1e-6);
+                    L = pick < sunPick(env) ? sunDir(env, u0[1], u0[2]) : cosDir(v(0, 0, 1), u0[1], u0[2]);
+                    pS = (1 - fs) * envPdf(env, L, v(0, 0, 1));
+                    le = add(envBg(env, L), envSun(env, p, L));
+                }
+                if (pS > 0) {
+                    double ph = hg(hz->g, dotp(dir, L));
+                    d3 c = scl(mul(beta, le), ph * misW(pS, ph) / pS);
+                    if (max3d(c) > 1e-7) {
+                        st->shadow++;
+                        if (occluded(S, p, L, tmax, &st->shadowSteps)) {
+                            c = scl(c, hazeTr(hz, p, L, tmax < 1e299 ? tmax : 1e30));
+                            #pragma omp atomic
+                            frame[4 * pix + 0] += c.x;
+                            #pragma omp atomic
+                            frame[4 * pix + 1] += c.y;
+                            #pragma omp atomic
+                            frame[4 * pix + 2] += c.z;
+                        }
+                    }
+                }
+            }
+            d3 nd = hgDir(hz->g, dir, u1[0], u1[1]);
+            double ph = hg(hz->g, dotp(dir, nd));
+            double pLout = (1 - fs) * (envLit(env) ? envPdf(env, nd, v(0, 0, 1)) : 0);
+            sunmis = (u->nee && pLout > 0) ? misW(ph, pLout) : 1;
+            prevPdf = u->nee ? ph : -1;
+            vtxLtot = ltot;
+            dir = nd;
+            if (bounces > u->maxbounce) break;
+            collide = ltot + hazeDraw(hz, p, dir, u1[2]) * n;
+            continue;
+        }
+        OrHit h = sceneAt(S, p);
+        d3 ns = h.n;
+        if (dotp(ns, ns) < 0.25) ns = normalAt(S, p, marchTol(S, p));
+        int inside = h.d < 0;
+        d3 nrm = inside ? neg(ns) : ns;
+        if (dotp(dir, nrm) > 0) { double off = offsetTol(S, p); p = add(p, scl(dir, off)); ltot += off * n; continue; }
+        bounces++;
+        const PtMaterial* m = &S->mats[h.m < S->nmats ? h.m : 0];
+        d3 Le = f3(m->emissive);
+        if (max3d(Le) > 0) {
+            double w = 1;
+            if (m->face >= 0) {
+                const PtLightFace* f = &S->faces[m->face];
+                double cosY = -dotp(f3(f->n), dir);
+                if (!(cosY > 0 || f->twoSided)) Le = v(0, 0, 0);
+                else if (prevPdf >= 0) {
+                    double dist = maxd(ltot - vtxLtot, 0) / maxd(n, 1e-4);
+                    w = misW(prevPdf, faceDensity(S, faceShare(S, bounces - 1), f, dist, cosY));
+                }
+            }
+            rad = add(rad, scl(mul(beta, Le), w));
+            direct = bounces == 1;
+            break;
+        }
+        double n0 = n, n1 = maxd(m->ior, 1);
+        if (inside) {
+            n1 = 1;
+            if (m->transmission > 0) {
+                OrHit hb = sceneAt(S, add(p, scl(dir, 2 * offsetTol(S, p))));
+                if (hb.insideW < 0 && hb.a < S->nmats) n1 = maxd(S->mats[hb.a].ior, 1);
+            }
+        }
+        double ud[4]; rng4(&rng, ud);
+        int draw = 0; double lambda0 = lambda; double sw[3] = {1, 1, 1};
+        if (m->abbe > 0 && m->transmission > 0 && !inside) {
+            if (lambda <= 0) { draw = 1; lambda = 0.38 + 0.40 * ud[3]; or_spectral_weight(lambda, sw); }
+            n1 = or_cauchy(n1, m->abbe, lambda);
+        }
+        OrLobes L = makeLobes(nrm, neg(dir), m, n0 / n1);
+        double fs = faceShare(S, bounces);
+        if (u->nee && !lobesNone(m)) {
+            double u0[4], u1[4]; rng4(&rng, u0); rng4(&rng, u1);
+            d3 Ld = v(0, 0, 0), le = v(0, 0, 0); double pS = 0, tmax = 1e300;
+            if (u0[0] < fs) {
+                const PtLightFace* f = &S->faces[facePick(S, u1[0] * S->faceTotal)];
+                d3 y = f3(f->centre);
+                if (f->shape == 1) { double rr = f->halfx * sqrt(u0[1]), ph = 2 * PI_D * u0[2]; y = add(y, scl(add(scl(f3(f->u), cos(ph)), scl(f3(f->v), sin(ph))), rr)); }
+                else y = add(y, add(scl(f3(f->u), (2 * u0[1] - 1) * f->halfx), scl(f3(f->v), (2 * u0[2] - 1) * f->halfy)));
+                d3 dy = sub(y, p); double dist = len(dy); Ld = scl(dy, 1 / maxd(dist, 1e-20));
+                double cosY = -dotp(f3(f->n), Ld);
+                if (dist > 0 && dotp(nrm, Ld) > 0 && fabs(cosY) > 1e-6 && (cosY > 0 || f->twoSided)) {
+                    pS = faceDensity(S, fs, f, dist, cosY);
+                    le = scl(f3(f->le), 1 / maxd(n0 * n0, 1));
+                    d3 o = add(p, scl(nrm, 2 * offsetTol(S, p)));
+                    tmax = len(sub(y, o)) - 2 * offsetTol(S, p);
+                }
+            } else if (envLit(env)) {
+                double pick = (u0[0] - fs) / maxd(1 - fs, 1e-6);
+                Ld = pick < sunPick(env) ? sunDir(env, u0[1], u0[2]) : cosDir(nrm, u0[1], u0[2]);
+                if (dotp(nrm, Ld) > 0) { pS = (1 - fs) * envPdf(env, Ld, nrm); le = add(envBg(env, Ld), envSun(env, p, Ld)); }
+            }
+            if (pS > 0 && max3d(le) > 0) {
+                d3 f = lobesEval(&L, Ld);
+                double pB = lobesPdf(&L, Ld), NoL = dotp(nrm, Ld);
+                d3 c = scl(mul(mul(beta, f), le), NoL * misW(pS, pB) / pS);
+                if (max3d(c) > 1e-7) {
+                    d3 o = add(p, scl(nrm, 2 * offsetTol(S, p)));
+                    if (tmax < 1e299) Ld = unit(sub(add(o, scl(Ld, tmax + 2 * offsetTol(S, p))), o));
+                    st->shadow++;
+                    if (occluded(S, o, Ld, tmax, &st->shadowSteps)) {
+                        if (hazeOn(S)) c = scl(c, hazeTr(&u->haze, o, Ld, tmax < 1e299 ? tmax : 1e30));
+                        #pragma omp atomic
+                        frame[4 * pix + 0] += c.x;
+                        #pragma omp atomic
+                        frame[4 * pix + 1] += c.y;
+                        #pragma omp atomic
+                        frame[4 * pix + 2] += c.z;
+                    }
+                }
+            }
+        }
+        double us[4]; rng4(&rng, us);
+        OrSample s = lobesSample(&L, us);
+        dir = s.dir;
+        beta = mul(beta, s.w);
+        if (s.transmitted) { n = n1; if (draw) beta = mul(beta, v(sw[0], sw[1], sw[2])); }
+        else if (draw) lambda = lambda0;
+        int terminate = !(max3d(s.w) > 0);
+        int misOut = u->nee && !s.delta;
+        double pOut = misOut ? lobesPdf(&L, dir) : 0;
+        int above = dotp(nrm, dir) > 0 && !s.delta;
+        if (misOut) {
+            double pLout = above && envLit(env) ? (1 - fs) * envPdf(env, dir, nrm) : 0;
+            sunmis = pLout > 0 ? misW(pOut, pLout) : 1;
+            prevPdf = above ? pOut : -1;
+        } else { sunmis = 1; prevPdf = -1; }
+        vtxLtot = ltot;
+        if (s.eval) steps += 0x10000u;
+        if (bounces > u->maxbounce) terminate = 1;
+        if (!terminate && u->diffuseCap && (steps >> 16) > u->diffuseCap) terminate = 1;
+        if (!terminate && u->rr && bounces >= u->rrStart) {
+            double pr = clampd(max3d(beta), 0.05, 0.95);
+            if (ud[0] > pr) terminate = 1; else beta = scl(beta, 1 / pr);
+        }
+        if (terminate) break;
+        double off = offsetTol(S, p);
+        p = add(p, scl(dir, off)); ltot += off * n;
+        collide = (hazeOn(S) && n <= 1) ? ltot + hazeDraw(&u->haze, p, dir, ud[1]) * n : 1e300;
+    }
+    (void)direct;
+    if (!(isfinite(rad.x) && isfinite(rad.y) && isfinite(rad.z)) || lumad(rad) < 0) { st->nonfinite++; rad = v(0, 0, 0); }
+    #pragma omp atomic
+    frame[4 * pix + 0] += rad.x;
+    #pragma omp atomic
+    frame[4 * pix + 1] += rad.y;
+    #pragma omp atomic
+    frame[4 * pix + 2] += rad.z;
+    #pragma omp atomic
+    frame[4 * pix + 3] += 1;
+}
+
+static void sceneInit(OrScene* S, const int* ops, int nops, const float* U, const PtMaterial* mats, int nmats,
+                      const PtLightFace* faces, int nfaces, const PtUniforms* u, OrOp* buf) {
+    for (int i = 0; i < nops; ++i) {
+        const int* o = ops + 8 * i;
+        buf[i].op = o[0]; buf[i].shape = o[1]; buf[i].xf = o[2]; buf[i].p = o[3]; buf[i].mat = o[4]; buf[i].prim = o[5];
+        buf[i].join = o[6]; buf[i].k = o[7];
+    }
+    S->ops = buf; S->nops = nops; S->U = U; S->mats = mats; S->nmats = nmats; S->faces = faces; S->nfaces = nfaces; S->u = u;
+    S->faceTotal = 0;
+    for (int k = 0; k < nfaces; ++k) if (faces[k].weight > 0) S->faceTotal += faces[k].weight;
+}
+
+/* Renders `samples` samples from sample index `first`, FP64 accumulation as the engine's resolve
+ * (core/wavefront.js:414-494, core/measure.js:36-57); accum [w*h*4] mean rgb + count, moments [w*h*2].
+ * ops: 8 ints per program entry (op, shape, xf, p, mat, prim, join, k). stats: 6 counters. */
+int or_render(const int* ops, int nops, const float* U, const PtMaterial* mats, int nmats, const PtLightFace* faces,
+              int nfaces, const PtUniforms* u, uint32_t first, uint32_t samples, double* accum, double* moments,
+              long long* stats, int threads) {
+    OrOp* buf = (OrOp*)malloc(sizeof(OrOp) * (nops > 0 ? nops : 1));
+    OrScene S; sceneInit(&S, ops, nops, U, mats, nmats, faces, nfaces, u, buf);
+    or_mf_tables(NULL);
+    uint32_t npx = u->width * u->height, N = npx;
+    OrTiles T = tilesOf(N, u->width, u->height);
+    double* frame = (double*)calloc((size_t)npx * 4, sizeof(double));
+    OrStats tot; memset(&tot, 0, sizeof tot);
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+    for (uint32_t s = 0; s < samples; ++s) {
+        uint32_t seed[4]; sampleSeed(u->seed, first + s, seed);
+        memset(frame, 0, sizeof(double) * npx * 4);
+        #pragma omp parallel
+        {
+            OrStats st; memset(&st, 0, sizeof st);
+            #pragma omp for schedule(dynamic, 64)
+            for (uint32_t i = 0; i < N; ++i) tracePath(&S, &T, seed, i, frame, &st);
+            #pragma omp critical
+            { tot.segments += st.segments; tot.shadow += st.shadow; tot.steps += st.steps; tot.shadowSteps += st.shadowSteps;
+              tot.maxiter += st.maxiter; tot.nonfinite += st.nonfinite; }
+        }
+        for (uint32_t i = 0; i < npx; ++i) {
+            double cnt = frame[4 * i + 3];
+            if (cnt == 0) continue;
+            double s0 = frame[4 * i] / cnt, s1 = frame[4 * i + 1] / cnt, s2 = frame[4 * i + 2] / cnt;
+            double* a = accum + 4 * (size_t)i;
+            double prevn = a[3];
+            if (u->maxaverages > 0) prevn = mind(prevn, u->maxaverages);
+            double c = prevn + 1, y = 0.2126 * s0 + 0.7152 * s1 + 0.0722 * s2, yPrev = 0.2126 * a[0] + 0.7152 * a[1] + 0.0722 * a[2];
+            a[0] = (a[0] * prevn + s0) / c; a[1] = (a[1] * prevn + s1) / c; a[2] = (a[2] * prevn + s2) / c; a[3] = c;
+            double w = 1 / c, dl = prevn > 0 ? y - yPrev : 0;
+            double* mo = moments + 2 * (size_t)i;
+            mo[0] = (1 - w) * (mo[0] + w * dl * dl);
+            mo[1] = (1 - w) * (1 - w) * mo[1] + w * w;
+        }
+    }
+    if (stats) { stats[0] = tot.segments; stats[1] = tot.shadow; stats[2] = tot.steps; stats[3] = tot.shadowSteps; stats[4] = tot.maxiter; stats[5] = tot.nonfinite; }
+    free(frame); free(buf);
+    return 0;
+}
+
+/* Each camera ray of sample `sample` to its first event (the engine's pt_pick_rays): pos4 (x, y, z, ltot),
+ * info (state | material << 8, primitive or 0xFFFFFFFF). Rays [first, first + count). */
+int or_pick(const int* ops, int nops, const float* U, const PtUniforms* u, uint32_t sample, uint32_t first, uint32_t count,
+            double* pos4, uint32_t* info) {
+    OrOp* buf = (OrOp*)malloc(sizeof(OrOp) * (nops > 0 ? nops : 1));
+    OrScene S; sceneInit(&S, ops, nops, U, NULL, 0, NULL, 0, u, buf);
+    uint32_t N = u->width * u->height;
+    OrTiles T = tilesOf(N, u->width, u->height);
+    uint32_t seed[4]; sampleSeed(u->seed, sample, seed);
+    #pragma omp parallel for schedule(dynamic, 64)
+    for (uint32_t k = 0; k < count; ++k) {
+        uint32_t i = first + k;
+        d3 p, d; uint32_t pix; double uh;
+        cameraRay(&S, &T, seed, i, &p, &d, &pix, &uh);
+        double ltot = 0; uint32_t steps = 0;
+        double collide = hazeOn(&S) ? hazeDraw(&u->haze, p, d, uh) : 1e300;
+        int st = march(&S, &p, d, 1, &ltot, &steps, collide);
+        uint32_t prim = 0xFFFFFFFFu, mat = 0xFFFFFFu;
+        if (st == ST_ACTIVE) { OrHit h = sceneAt(&S, p); prim = (uint32_t)h.prim; mat = (uint32_t)h.m; }
+        pos4[4 * k] = p.x; pos4[4 * k + 1] = p.y; pos4[4 * k + 2] = p.z; pos4[4 * k + 3] = ltot;
+        info[2 * k] = (uint32_t)st | (mat << 8); info[2 * k + 1] = prim;
+    }
+    free(buf);
+    return 0;
+}
+
+/* The scene's signed distance and closest primitive at points (for brute-force checks). */
+int or_scene_at(const int* ops, int nops, const float* U, const double* pts, int n, double* d, double* nrm, int* prim) {
+    OrOp* buf = (OrOp*)malloc(sizeof(OrOp) * (nops > 0 ? nops : 1));
+    PtUniforms u; memset(&u, 0, sizeof u);
+    OrScene S; sceneInit(&S, ops, nops, U, NULL, 0, NULL, 0, &u, buf);
+    for (int i = 0; i < n; ++i) {
+        OrHit h = sceneAt(&S, v(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]));
+        d[i] = h.d; if (nrm) { nrm[3 * i] = h.n.x; nrm[3 * i + 1] = h.n.y; nrm[3 * i + 2] = h.n.z; }
+        if (prim) prim[i] = h.prim;
+    }
+    free(buf);
+    return 0;
+}
+
+/* The BSDF for quadrature tests: eval (rgb), pdf, and a sample from u (dir, weight, flags). view/l/n world. */
+int or_bsdf(const PtMaterial* m, const double* n3, const double* view3, double eta, const double* l3, double* f, double* pdf,
+            const double* u4, double* sdir, double* sw, int* flags) {
+    or_mf_tables(NULL);
+    OrLobes L = makeLobes(v(n3[0], n3[1], n3[2]), v(view3[0], view3[1], view3[2]), m, eta);
+    if (l3 && f) { d3 e = lobesEval(&L, v(l3[0], l3[1], l3[2])); f[0] = e.x; f[1] = e.y; f[2] = e.z; }
+    if (l3 && pdf) *pdf = lobesPdf(&L, v(l3[0], l3[1], l3[2]));
+    if (u4 && sdir) {
+        OrSample s = lobesSample(&L, u4);
+        sdir[0] = s.dir.x; sdir[1] = s.dir.y; sdir[2] = s.dir.z; sw[0] = s.w.x; sw[1] = s.w.y; sw[2] = s.w.z;
+        *flags = s.delta | (s.transmitted << 1) | (s.eval << 2);
+    }
+    return 0;
+}
+
+/* Haze helpers for closed-form checks: transmittance along a ray, and the free-flight draw. */
+double or_haze_tr(const PtHaze* h, const double* o, const double* d, double lenMax) { return hazeTr(h, v(o[0], o[1], o[2]), v(d[0], d[1], d[2]), lenMax); }
+double or_haze_draw(const PtHaze* h, const double* o, const double* d, double u) { return hazeDraw(h, v(o[0], o[1], o[2]), v(d[0], d[1], d[2]), u); }
+double or_hg(double g, double c) { return hg(g, c); }
+void or_hg_sample(double g, const double* w, double u1, double u2, double* out) { d3 r = hgDir(g, v(w[0], w[1], w[2]), u1, u2); out[0] = r.x; out[1] = r.y; out[2] = r.z; }
+void or_pcg4d(const uint32_t* in, uint32_t* out) { OrRng s = {in[0], in[1], in[2], in[3]}; pcg4d(&s); out[0] = s.a; out[1] = s.b; out[2] = s.c; out[3] = s.d; }
+void or_tiles(uint32_t N, uint32_t w, uint32_t h, uint32_t* tx, uint32_t* ty) { OrTiles t = tilesOf(N, w, h); *tx = t.tilesX; *ty = t.tilesY; }

@@ -1,0 +1,148 @@
+"""ctypes binding of the CPU oracle (oracle.c) -- test infrastructure only: tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg use it as the checker. The product never imports it.
+
+The scene arrives as the emitter's output (shadytracer_amd/js/hipgen.js): the postfix program, the parameter
+block, materials and light faces; the uniforms as the engine's PtUniforms (shadytracer_amd/pt.py)."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, '_build', 'liboracle.so')
+
+SHAPES = ['plane', 'sphere', 'box', 'cylinder', 'capsule', 'torus', 'rectangle', 'disc']
+JOINS = ['union', 'subtract', 'intersect']
+OPS = {'prim': 0, 'join': 1, 'fold': 2}
+
+_lib = None
+
+
+def build():
+    subprocess.run(['make', '-s', '-C', HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        P = ctypes.POINTER
+        c_i, c_u, c_d, c_f, c_v = ctypes.c_int, ctypes.c_uint32, ctypes.c_double, ctypes.c_float, ctypes.c_void_p
+        L.or_render.argtypes = [P(c_i), c_i, P(c_f), c_v, c_i, c_v, c_i, c_v, c_u, c_u, P(c_d), P(c_d), P(ctypes.c_longlong), c_i]
+        L.or_pick.argtypes = [P(c_i), c_i, P(c_f), c_v, c_u, c_u, c_u, P(c_d), P(c_u)]
+        L.or_scene_at.argtypes = [P(c_i), c_i, P(c_f), P(c_d), c_i, P(c_d), P(c_d), P(c_i)]
+        L.or_bsdf.argtypes = [c_v, P(c_d), P(c_d), c_d, P(c_d), P(c_d), P(c_d), P(c_d), P(c_d), P(c_d), P(c_i)]
+        L.or_fresnel.restype = c_d
+        L.or_fresnel.argtypes = [c_d, c_d]
+        L.or_cauchy.restype = c_d
+        L.or_cauchy.argtypes = [c_d, c_d, c_d]
+        L.or_spectral_weight.argtypes = [c_d, P(c_d)]
+        L.or_mf_tables.argtypes = [P(c_d)]
+        L.or_haze_tr.restype = c_d
+        L.or_haze_tr.argtypes = [c_v, P(c_d), P(c_d), c_d]
+        L.or_haze_draw.restype = c_d
+        L.or_haze_draw.argtypes = [c_v, P(c_d), P(c_d), c_d]
+        L.or_hg.restype = c_d
+        L.or_hg.argtypes = [c_d, c_d]
+        L.or_hg_sample.argtypes = [c_d, P(c_d), c_d, c_d, P(c_d)]
+        L.or_pcg4d.argtypes = [P(c_u), P(c_u)]
+        L.or_tiles.argtypes = [c_u, c_u, c_u, P(c_u), P(c_u)]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def pack_program(program):
+    out = np.zeros((len(program), 8), np.int32)
+    for i, o in enumerate(program):
+        out[i, 0] = OPS[o['op']]
+        if o['op'] == 'prim':
+            out[i, 1] = SHAPES.index(o['shape']); out[i, 2] = o['xf']; out[i, 3] = o['p']; out[i, 4] = o['mat']; out[i, 5] = o['prim']
+        elif o['op'] == 'join':
+            out[i, 6] = JOINS.index(o['join']); out[i, 7] = o['k']
+    return np.ascontiguousarray(out)
+
+
+class Scene:
+    """A scene as the oracle reads it, with the ctypes buffers kept alive."""
+
+    def __init__(self, scene):
+        from shadytracer_amd import pt
+        self.ops = pack_program(scene['program'])
+        self.U = np.ascontiguousarray(np.asarray(scene['params'], np.float32))
+        mats = scene['materials']
+        self.mats = (pt.PtMaterial * len(mats))()
+        for i, m in enumerate(mats):
+            for k in ('base', 'emissive'):
+                for j in range(3):
+                    getattr(self.mats[i], k)[j] = m[k][j]
+            self.mats[i].metallic, self.mats[i].roughness, self.mats[i].transmission = m['metallic'], m['roughness'], m['transmission']
+            self.mats[i].ior, self.mats[i].specular, self.mats[i].abbe, self.mats[i].face = m['ior'], m['specular'], m['abbe'], m['face']
+        nf = len(scene['faces'])
+        self.faces = (pt.PtLightFace * max(nf, 1))()
+        for i, f in enumerate(scene['faces']):
+            for k in ('centre', 'u', 'v', 'n', 'le'):
+                for j in range(3):
+                    getattr(self.faces[i], k)[j] = f[k][j]
+            self.faces[i].halfx, self.faces[i].halfy, self.faces[i].shape = f['halfx'], f['halfy'], f['shape']
+            self.faces[i].twoSided, self.faces[i].weight = f['twoSided'], f['weight']
+        self.nmats, self.nfaces = len(mats), nf
+
+    def args(self):
+        return (self.ops.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), len(self.ops),
+                self.U.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+
+
+def render(scene, u, samples, first=0, threads=0):
+    """FP64 render: (accum [h, w, 4], moments [h, w, 2], stats dict)."""
+    S = Scene(scene)
+    n = u.width * u.height
+    acc = np.zeros((n, 4), np.float64)
+    mom = np.zeros((n, 2), np.float64)
+    st = (ctypes.c_longlong * 6)()
+    lib().or_render(*S.args(), ctypes.byref(S.mats), S.nmats, ctypes.byref(S.faces), S.nfaces, ctypes.byref(u),
+                    first, samples, _dp(acc), _dp(mom), st, threads)
+    names = ['segments', 'shadow_segments', 'steps', 'shadow_steps', 'maxiter', 'nonfinite']
+    return acc.reshape(u.height, u.width, 4), mom.reshape(u.height, u.width, 2), dict(zip(names, list(st)))
+
+
+def pick(scene, u, sample=0, first=0, count=None):
+    """Each camera ray's first event: (pos4 [n, 4], state [n], material [n], prim [n])."""
+    S = Scene(scene)
+    n = u.width * u.height if count is None else count
+    pos = np.zeros((n, 4), np.float64)
+    info = np.zeros((n, 2), np.uint32)
+    lib().or_pick(*S.args(), ctypes.byref(u), sample, first, n, _dp(pos), info.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+    return pos, info[:, 0] & 0xFF, info[:, 0] >> 8, info[:, 1]
+
+
+def scene_at(scene, pts):
+    S = Scene(scene)
+    pts = np.ascontiguousarray(np.asarray(pts, np.float64).reshape(-1, 3))
+    d = np.zeros(len(pts)); nrm = np.zeros((len(pts), 3)); prim = np.zeros(len(pts), np.int32)
+    lib().or_scene_at(*S.args(), _dp(pts), len(pts), _dp(d), _dp(nrm), prim.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return d, nrm, prim
+
+
+def bsdf(material, n, view, eta, l=None, u=None):
+    """eval/pdf at l and a sample from u, for one material (a PtMaterial)."""
+    n = np.asarray(n, np.float64); view = np.asarray(view, np.float64)
+    f = np.zeros(3); pdf = np.zeros(1); sd = np.zeros(3); sw = np.zeros(3); fl = ctypes.c_int(0)
+    la = np.asarray(l, np.float64) if l is not None else None
+    ua = np.asarray(u, np.float64) if u is not None else None
+    lib().or_bsdf(ctypes.byref(material), _dp(n), _dp(view), eta, _dp(la) if la is not None else None, _dp(f), _dp(pdf),
+                  _dp(ua) if ua is not None else None, _dp(sd), _dp(sw), ctypes.byref(fl))
+    return f, pdf[0], sd, sw, fl.value
+
+
+def mf_tables():
+    from shadytracer_amd.pt import load_library  # noqa: F401  (sizes come from pt_types.h)
+    out = np.zeros(32 * 32 * 16 + 16 * 32 + 32 * 32, np.float64)
+    lib().or_mf_tables(_dp(out))
+    return out

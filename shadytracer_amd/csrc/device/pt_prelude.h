@@ -268,12 +268,13 @@ struct SdfHit {
     unsigned int m;      // material of the closest surface
     unsigned int a;      // material the point is inside of
     float insideW;
+    unsigned int prim;   // the closest surface's primitive (its place in the emitter's program)
 };
 __device__ __forceinline__ SdfHit sdfEmpty() {
-    SdfHit h; h.n = v3(0.0f, 0.0f, 1.0f); h.d = 1.0e7f; h.m = 0u; h.a = 0u; h.insideW = -1.0e7f; return h;
+    SdfHit h; h.n = v3(0.0f, 0.0f, 1.0f); h.d = 1.0e7f; h.m = 0u; h.a = 0u; h.insideW = -1.0e7f; h.prim = 0xFFFFFFFFu; return h;
 }
-__device__ __forceinline__ void foldFull(SdfHit& h, V4 s, unsigned int mat) {
+__device__ __forceinline__ void foldFull(SdfHit& h, V4 s, unsigned int mat, unsigned int prim) {
     if (s.w < 0.0f && s.w > h.insideW) { h.insideW = s.w; h.a = mat; }
-    if (fabsf(s.w) < fabsf(h.d)) { h.n = xyz(s); h.d = s.w; h.m = mat; }
+    if (fabsf(s.w) < fabsf(h.d)) { h.n = xyz(s); h.d = s.w; h.m = mat; h.prim = prim; }
 }
 __device__ __forceinline__ void foldDist(float& best, float d) { if (fabsf(d) < fabsf(best)) best = d; }

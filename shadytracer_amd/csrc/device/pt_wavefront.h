@@ -571,3 +571,21 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_mf_lut_avg(float* T) {
     }
     T[PT_MF_ECAVG + t] = 2.0f * acc / PT_MF_NMU;
 }
+
+// ---------------------------------------------------------------------------------------------------
+// PICK (core/pick.js, R.pick): after a sample's first march, each camera ray's event, where it stopped, and the
+// primitive and material of the surface it hit -- the decisions the oracle checks ray by ray.
+extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_pick(const PtDev* __restrict__ D, float* __restrict__ pos4, unsigned int* __restrict__ info) {
+    unsigned int i = blockIdx.x * PT_BLOCK + threadIdx.x;
+    if (i >= D->nrays) return;
+    float4 P = ld4(D->pos, i);
+    unsigned int state = mzState(D->meta[4u * i + 2u]);
+    unsigned int prim = 0xFFFFFFFFu, mat = 0xFFFFFFu;
+    if (state == PT_ACTIVE) {
+        SdfHit h = sceneFull(D->params, v3(P.x, P.y, P.z));
+        prim = h.prim; mat = h.m;
+    }
+    st4(pos4, i, P.x, P.y, P.z, P.w);
+    info[2u * i] = state | (mat << 8);
+    info[2u * i + 1u] = prim;
+}

@@ -23,8 +23,8 @@ namespace {
 
 #include "pt_device_src.inc"   // kPtDevicePre, kPtDevicePost: the device sources as strings
 
-const char* kKernelNames[] = {"pt_genrays", "pt_march", "pt_shade", "pt_occlude", "pt_resolve", "pt_mf_lut", "pt_mf_lut_avg"};
-enum { K_GENRAYS, K_MARCH, K_SHADE, K_OCCLUDE, K_RESOLVE, K_MFLUT, K_MFAVG, K_COUNT };
+const char* kKernelNames[] = {"pt_genrays", "pt_march", "pt_shade", "pt_occlude", "pt_resolve", "pt_mf_lut", "pt_mf_lut_avg", "pt_pick"};
+enum { K_GENRAYS, K_MARCH, K_SHADE, K_OCCLUDE, K_RESOLVE, K_MFLUT, K_MFAVG, K_PICK, K_COUNT };
 
 thread_local std::string g_createError;
 
@@ -568,6 +568,27 @@ int pt_time_kernels(pt_ctx* c, uint32_t samples, double* ms, size_t n) {
         }
     }
     for (auto& e : ev) hipEventDestroy(e);
+    return PT_OK;
+}
+
+int pt_pick_rays(pt_ctx* c, float* pos4, uint32_t* info, size_t n) {
+    if (!c || !pos4 || !info) return fail(c, PT_ERR_ARG, "pos4 or info is NULL");
+    if (!c->module) return fail(c, PT_ERR_STATE, "no scene loaded");
+    if (n != c->host.nrays) return fail(c, PT_ERR_ARG, "n %zu, a sample has %u camera rays", n, c->host.nrays);
+    int rc;
+    if ((rc = syncDev(c))) return rc;
+    float* dpos = nullptr;
+    unsigned int* dinfo = nullptr;
+    if ((rc = allocBuf(c, dpos, n * 16)) || (rc = allocBuf(c, dinfo, n * 8))) return rc;
+    HIPCHK(c, launch(c, K_GENRAYS, gridFor(c, K_GENRAYS), 0));
+    HIPCHK(c, launch(c, K_MARCH, gridFor(c, K_MARCH), 0));
+    void* args[] = {&c->dev, &dpos, &dinfo};
+    HIPCHK(c, hipModuleLaunchKernel(c->fn[K_PICK], (unsigned int)((n + 255) / 256), 1, 1, 256, 1, 1, 0, c->stream, args, nullptr));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(pos4, dpos, n * 16, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(info, dinfo, n * 8, hipMemcpyDeviceToHost));
+    void* p1 = dpos; freeBuf(c, p1);
+    void* p2 = dinfo; freeBuf(c, p2);
     return PT_OK;
 }
 

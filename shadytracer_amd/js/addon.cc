@@ -290,18 +290,18 @@ napi_value ReadStats(napi_env env, napi_callback_info info) {
 
 napi_value Init(napi_env env, napi_value exports) {
     napi_property_descriptor props[] = {
-        {"create", nullptr, Create, nullptr, nullptr, nullptr, napi_default, nullptr},
-        {"destroy", nullptr, Destroy, nullptr, nullptr, nullptr, napi_default, nullptr},
-        {"abiVersion", nullptr, AbiVersion, nullptr, nullptr, nullptr, napi_default, nullptr},
-        {"structSize", nullptr, StructSize, nullptr, nullptr, nullptr, napi_default, nullptr},
-        {"compileScene", nullptr, CompileScene, nullptr, nullptr, nullptr, napi_default, nullptr},
-        {"setScene", nullptr, SetScene, nullptr, nullptr, nullptr, napi_default, nullptr},
-        {"setParams", nullptr, SetParams, nullptr, nullptr, nullptr, napi_default, nullptr},
-        {"setUniforms", nullptr, SetUniforms, nullptr, nullptr, nullptr, napi_default, nullptr},
-        {"clearAccum", nullptr, ClearAccum, nullptr, nullptr, nullptr, napi_default, nullptr},
-        {"trace", nullptr, Trace, nullptr, nullptr, nullptr, napi_default, nullptr},
-        {"readAccum", nullptr, ReadAccum, nullptr, nullptr, nullptr, napi_default, nullptr},
-        {"readStats", nullptr, ReadStats, nullptr, nullptr, nullptr, napi_default, nullptr},
+        {"create", nullptr, Create, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"destroy", nullptr, Destroy, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"abiVersion", nullptr, AbiVersion, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"structSize", nullptr, StructSize, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"compileScene", nullptr, CompileScene, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"setScene", nullptr, SetScene, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"setParams", nullptr, SetParams, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"setUniforms", nullptr, SetUniforms, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"clearAccum", nullptr, ClearAccum, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"trace", nullptr, Trace, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"readAccum", nullptr, ReadAccum, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"readStats", nullptr, ReadStats, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
     };
     napi_define_properties(env, exports, sizeof props / sizeof props[0], props);
     return exports;

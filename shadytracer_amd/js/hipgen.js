@@ -229,7 +229,7 @@ function generate(doc, opts) {
             if (!fn) throw new Error(`hipgen: node '${c.id}': unknown join '${join}'`)
             const wins = { union: `${v.f}.w < ${acc.f}.w`, subtract: `-${v.f}.w > ${acc.f}.w`, intersect: `${v.f}.w > ${acc.f}.w` }[join]
             dist.push(`    ${acc.d} = ${fn[0]}(${acc.d}, ${v.d}, U[${ko}]);`)
-            full.push(`    if (${wins}) ${acc.m} = ${v.m};`)
+            full.push(`    if (${wins}) { ${acc.m} = ${v.m}; ${acc.k} = ${v.k}; }`)
             full.push(`    ${acc.f} = ${fn[1]}(${acc.f}, ${v.f}, U[${ko}]);`)
             program.push({ op: 'join', join, k: ko })
             flops += JOIN_FLOPS
@@ -248,12 +248,13 @@ function generate(doc, opts) {
         const s = shape(n, X)
         const mat = matOf(n)
         const i = tmp++
-        const d = `d${i}`, f = `f${i}`, m = `m${i}`
+        const d = `d${i}`, f = `f${i}`, m = `m${i}`, k = `k${i}`
+        const prim = program.filter(o => o.op === 'prim').length
         dist.push(`    float ${d} = ${SHAPES[s.key].dist(s.po)} * U[${s.xo + 12}];`.replace(/\bq\b/, `xfIn(U + ${s.xo}, p)`))
         full.push(`    V4 ${f} = xfOut(U + ${s.xo}, ${SHAPES[s.key].full(s.po)});`.replace(/\bq\b/, `xfIn(U + ${s.xo}, p)`))
-        full.push(`    unsigned int ${m} = ${mat}u;`)
-        program.push({ op: 'prim', shape: s.key, xf: s.xo, p: s.po, mat })
-        return { d, f, m }
+        full.push(`    unsigned int ${m} = ${mat}u, ${k} = ${prim}u;`)
+        program.push({ op: 'prim', shape: s.key, xf: s.xo, p: s.po, mat, prim })
+        return { d, f, m, k }
     }
     // the closest-surface fold over a plain group's descendants
     function foldNode(n, parentX) {
@@ -267,7 +268,7 @@ function generate(doc, opts) {
         const v = valueOf(n, parentX)
         if (!v) return
         dist.push(`    foldDist(best, ${v.d});`)
-        full.push(`    foldFull(h, ${v.f}, ${v.m});`)
+        full.push(`    foldFull(h, ${v.f}, ${v.m}, ${v.k});`)
         program.push({ op: 'fold' })
         flops += FOLD_FLOPS
     }

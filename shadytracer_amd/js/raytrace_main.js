@@ -1,0 +1,192 @@
+'use strict'
+/*
+ * RAYTRACER -- the engine object of lib/pathtracer (core/raytrace_main.js:121), kept as the reference's callers use
+ * it, over the MI355X engine: each method calls libpt through the N-API addon (addon.cc -> include/pt.h).
+ *
+ *   const R = RAYTRACER(WG, _U, Q, scene)      WG: { device, blocksPerCu, cacheDir } (the reference's tinywebgpu
+ *                                              instance has no meaning here), _U: the uniform bag, Q: unused
+ *   await R.init()                             R.init (raytrace_main.js:1291-1313)
+ *   R.setScene(scene) -> ms                    R.setScene (:1145-1183); scene: hipgen output or a builder document.
+ *                                              A compile failure keeps the previous scene: e.restored = true
+ *   await R.trace(Q, uniforms, count)          R.trace (:1574-2342): count samples; resolves when the GPU is done
+ *   R.imageStats() -> { meanLuma, trimmedLuma, median, noise, relSE, ... }   (:876-956)
+ *   R.readPixel(x, y) / R.readRegion(x0, y0, x1, y1)   (:1417-1448; core/measure.js reading / regionReading)
+ *   R.clearAccums(), R.ensureBuffers(w, h), R.releaseBuffers(), R.wavefrontLog()
+ */
+const path = require('path')
+const { generate } = require('./hipgen')
+
+const LUMA = [0.2126, 0.7152, 0.0722]
+let addon = null
+function loadAddon() {
+    if (!addon) addon = require(path.join(__dirname, 'pt_addon.node'))
+    return addon
+}
+
+// ---- the C structures' layouts (include/pt_types.h), every field 4 bytes: [name, kind, count]
+const CAMERA = [['pos', 'f', 3], ['fx', 'f', 1], ['right', 'f', 3], ['fy', 'f', 1], ['up', 'f', 3], ['type', 'u', 1], ['fwd', 'f', 3], ['pad', 'f', 1]]
+const ENV = [['type', 'u', 1], ['L', 'f', 1], ['col', 'f', 3], ['sunDir', 'f', 3], ['sunCos', 'f', 1], ['sunE', 'f', 1], ['sunRadius', 'f', 1],
+    ['sunPoint', 'f', 3], ['pad', 'f', 2]]
+const HAZE = [['centre', 'f', 3], ['sigmaS', 'f', 1], ['half', 'f', 3], ['g', 'f', 1], ['albedo', 'f', 1], ['pad', 'f', 3]]
+const UNIFORMS = [['width', 'u', 1], ['height', 'u', 1], ['maxbounce', 'u', 1], ['maxiter', 'u', 1], ['stepscale', 'f', 1], ['eps', 'f', 1],
+    ['epsAuto', 'f', 1], ['envRadius', 'f', 1], ['nee', 'u', 1], ['rr', 'u', 1], ['rrStart', 'u', 1], ['diffuseCap', 'u', 1],
+    ['fireflyClamp', 'f', 1], ['fireflyWhite', 'f', 1], ['maxaverages', 'u', 1], ['seed', 'u', 1], ['cam', CAMERA, 1], ['env', ENV, 1],
+    ['haze', HAZE, 1], ['faceShare', 'f', 1], ['sppm', 'u', 1], ['sppmRadius', 'f', 1], ['sppmPhotons', 'u', 1], ['marchRefill', 'u', 1],
+    ['guide', 'u', 1], ['guideFrac', 'f', 1], ['pad0', 'u', 1]]
+const MATERIAL = [['base', 'f', 3], ['metallic', 'f', 1], ['roughness', 'f', 1], ['transmission', 'f', 1], ['ior', 'f', 1], ['specular', 'f', 1],
+    ['emissive', 'f', 3], ['abbe', 'f', 1], ['face', 'i', 1], ['specularTint', 'f', 1], ['pad', 'f', 2]]
+const FACE = [['centre', 'f', 3], ['halfx', 'f', 1], ['u', 'f', 3], ['halfy', 'f', 1], ['v', 'f', 3], ['shape', 'u', 1], ['n', 'f', 3],
+    ['twoSided', 'u', 1], ['le', 'f', 3], ['weight', 'f', 1]]
+
+function sizeOf(layout) {
+    return layout.reduce((s, [, k, c]) => s + (Array.isArray(k) ? sizeOf(k) : 4) * c, 0)
+}
+function pack(layout, obj, view, off) {
+    obj = obj || {}
+    for (const [name, kind, count] of layout) {
+        if (Array.isArray(kind)) { off = pack(kind, obj[name], view, off); continue }
+        const v = obj[name]
+        for (let i = 0; i < count; i++) {
+            const x = count > 1 ? (v ? v[i] : 0) : v
+            const val = x === undefined || x === null ? 0 : x
+            if (kind === 'f') view.setFloat32(off, val, true)
+            else if (kind === 'i') view.setInt32(off, val, true)
+            else view.setUint32(off, val >>> 0, true)
+            off += 4
+        }
+    }
+    return off
+}
+function packArray(layout, items) {
+    const n = sizeOf(layout)
+    const buf = new ArrayBuffer(Math.max(n * items.length, n))
+    const view = new DataView(buf)
+    items.forEach((it, i) => pack(layout, it, view, i * n))
+    return buf
+}
+
+// The engine's uniform defaults: the reference schema's (core/uniforms.schema.js:37-418) for the names it shares
+const DEFAULTS = {
+    width: 256, height: 256, maxbounce: 10, maxiter: 400, stepscale: 0.9, eps: 0, epsAuto: 1e-4, envRadius: 400, nee: 1, rr: 0,
+    rrStart: 3, diffuseCap: 0, fireflyClamp: 0, fireflyWhite: 1, maxaverages: 0, seed: 20261203, faceShare: 0.5, sppm: 0,
+    sppmRadius: 0.02, sppmPhotons: 0, marchRefill: 16, guide: 0, guideFrac: 0.5,
+}
+const KNOWN = new Set(UNIFORMS.map(f => f[0]).filter(k => k !== 'pad0'))
+
+function RAYTRACER(WG, _U, Q, scene) {
+    WG = WG || {}
+    const A = loadAddon()
+    let h = null
+    let current = null            // the loaded scene (hipgen output)
+    let building = false
+    const U = Object.assign({}, DEFAULTS, _U || {})
+    const R = {}
+
+    R.init = async () => {
+        if (sizeOf(UNIFORMS) !== A.structSize(0) || sizeOf(MATERIAL) !== A.structSize(1) || sizeOf(FACE) !== A.structSize(2)) {
+            throw new Error('libpt ABI mismatch: the shim packs structures of another size')
+        }
+        h = A.create(WG.device || 0, WG.blocksPerCu || 0, WG.cacheDir || '')
+        if (scene) R.setScene(scene)
+        return R
+    }
+
+    // A builder document is emitted first; a hipgen output is loaded as is.
+    R.setScene = (sc) => {
+        if (!h) throw new Error('RAYTRACER: init() first')
+        const t0 = Date.now()
+        const made = sc && sc.hip ? sc : generate(sc)
+        building = true
+        try {
+            A.setScene(h, made.hip, Float32Array.from(made.params), packArray(MATERIAL, made.materials), made.materials.length,
+                packArray(FACE, made.faces), made.faces.length)
+        } finally { building = false }
+        current = made
+        U.envRadius = made.envRadius
+        U.env = made.env
+        U.haze = made.haze || {}
+        return Date.now() - t0
+    }
+
+    R.uniforms = () => U
+
+    R.trace = async (_Q, uniforms, count = 1) => {
+        if (building) throw new Error('RAYTRACER: trace during a scene build')
+        if (!current) throw new Error('RAYTRACER: no scene')
+        if (uniforms) Object.assign(U, uniforms)
+        const buf = new ArrayBuffer(sizeOf(UNIFORMS))
+        pack(UNIFORMS, U, new DataView(buf), 0)
+        A.setUniforms(h, buf)
+        await A.trace(h, count)
+    }
+
+    R.clearAccums = () => A.clearAccum(h)
+    R.ensureBuffers = (w, hh) => { U.width = w; U.height = hh }
+    R.releaseBuffers = () => { if (h) { A.destroy(h); h = null } }
+
+    function readback() {
+        const n = U.width * U.height
+        const acc = new Float64Array(n * 4), mom = new Float64Array(n * 2)
+        A.readAccum(h, acc, mom)
+        return { acc, mom, n }
+    }
+    const stdErr = (V, S) => (S < 1 && S > 0 && V >= 0) ? Math.sqrt(S * V / (1 - S)) : null
+
+    // Luminance statistics of the accumulated picture (raytrace_main.js:876-956)
+    R.imageStats = () => {
+        const { acc, mom, n } = readback()
+        const L = []
+        let sum = 0, rel = 0, nrel = 0, samples = 0
+        for (let i = 0; i < n; i++) {
+            if (!acc[4 * i + 3]) continue
+            const y = LUMA[0] * acc[4 * i] + LUMA[1] * acc[4 * i + 1] + LUMA[2] * acc[4 * i + 2]
+            L.push(y); sum += y
+            samples = Math.max(samples, acc[4 * i + 3])
+            const se = stdErr(mom[2 * i], mom[2 * i + 1])
+            if (se !== null && y > 0) { rel += se / y; nrel++ }
+        }
+        if (!L.length) return { meanLuma: 0, trimmedLuma: 0, median: 0, relSE: null, samples: 0, pixels: n }
+        const sorted = Float64Array.from(L).sort()
+        const keep = sorted.subarray(0, Math.max(1, Math.floor(sorted.length * 0.999)))
+        const trimmed = keep.reduce((a, b) => a + b, 0) / keep.length
+        return { meanLuma: sum / L.length, trimmedLuma: trimmed, median: sorted[sorted.length >> 1], relSE: nrel ? rel / nrel : null,
+            samples, pixels: n, covered: L.length / n }
+    }
+    // One pixel's reading with its standard error (core/measure.js:59-67); y counts from the bottom row
+    R.readPixel = (x, y) => {
+        const { acc, mom } = readback()
+        const i = x + y * U.width
+        if (!acc[4 * i + 3]) return null
+        const rgb = [acc[4 * i], acc[4 * i + 1], acc[4 * i + 2]]
+        const L = LUMA[0] * rgb[0] + LUMA[1] * rgb[1] + LUMA[2] * rgb[2]
+        const se = stdErr(mom[2 * i], mom[2 * i + 1])
+        return { L, se, rel: se !== null && L > 0 ? se / L : null, nEff: mom[2 * i + 1] > 0 ? 1 / mom[2 * i + 1] : null, count: acc[4 * i + 3], rgb }
+    }
+    // A region's mean and its standard error (core/measure.js:91-134 regionReading, without the colour)
+    R.readRegion = (x0, y0, x1, y1) => {
+        const { acc, mom } = readback()
+        let n = 0, sum = 0, se2 = 0, missing = 0, min = Infinity, max = -Infinity, pixels = 0
+        for (let y = y0; y <= y1; y++) for (let x = x0; x <= x1; x++) {
+            const i = x + y * U.width
+            pixels++
+            if (!acc[4 * i + 3]) continue
+            const L = LUMA[0] * acc[4 * i] + LUMA[1] * acc[4 * i + 1] + LUMA[2] * acc[4 * i + 2]
+            const se = stdErr(mom[2 * i], mom[2 * i + 1])
+            n++; sum += L
+            if (se === null) missing++; else se2 += se * se
+            min = Math.min(min, L); max = Math.max(max, L)
+        }
+        if (!n) return null
+        const mean = sum / n
+        const se = missing ? null : Math.sqrt(se2) / n
+        return { mean, se, rel: se !== null && mean > 0 ? se / mean : null, min, max, uniformity: mean > 0 ? min / mean : null, pixels, sampled: n }
+    }
+    // The engine's counters (the reference's wavefront log and counter readback, raytrace_main.js:106-116)
+    R.wavefrontLog = () => {
+        const s = A.readStats(h)
+        return { segments: s[0], shadowSegments: s[1], steps: s[2], shadowSteps: s[3], nonfinite: s[4], maxiter: s[5], leaving: s[7] }
+    }
+    return R
+}
+
+module.exports = { RAYTRACER, UNIFORMS, MATERIAL, FACE, CAMERA, sizeOf, pack, packArray, DEFAULTS, KNOWN, loadAddon }

@@ -84,6 +84,7 @@ SIGNATURES = {
     'pt_read_stats': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.c_size_t]),
     'pt_reset_stats': (ctypes.c_int, [ctypes.c_void_p]),
     'pt_time_kernels': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double), ctypes.c_size_t]),
+    'pt_pick_rays': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_uint32), ctypes.c_size_t]),
 }
 
 STAT_NAMES = ['segments', 'shadow_segments', 'steps', 'shadow_steps', 'nonfinite', 'maxiter', 'paths', 'leaving',
@@ -273,6 +274,15 @@ class Engine:
         ms = (ctypes.c_double * 5)()
         self._chk(self.lib.pt_time_kernels(self.h, samples, ms, 5))
         return dict(zip(['genrays', 'march', 'shade', 'occlude', 'resolve'], list(ms)))
+
+    def pick_rays(self):
+        """Each camera ray of one sample to its first event: (pos4 [n, 4], state [n], material [n], prim [n])."""
+        n = self.width * self.height
+        pos = np.zeros((n, 4), np.float32)
+        info = np.zeros((n, 2), np.uint32)
+        self._chk(self.lib.pt_pick_rays(self.h, pos.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                        info.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), n))
+        return pos, info[:, 0] & 0xFF, info[:, 0] >> 8, info[:, 1]
 
     def image_stats(self):
         """Luminance statistics of the accumulated image (R.imageStats, raytrace_main.js:876-956): mean, and the

@@ -1,0 +1,144 @@
+"""The HIP engine on the GPU, through its C ABI, against the oracle and the closed forms. Every test names its case
+before it touches the GPU."""
+import json
+import math
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from shadytracer_amd import pt, scenes
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def load(engine, name, **over):
+    made = scenes.emit(name, **{k: v for k, v in over.items() if k in ('width', 'height', 'g', 'roughness', 'specular', 'metallic', 'albedo', 'mfp')})
+    engine.set_scene(made['scene'])
+    u = pt.uniforms_for(made, **{k: v for k, v in over.items() if k not in ('g', 'roughness', 'specular', 'metallic', 'albedo', 'mfp')})
+    engine.set_uniforms(u)
+    engine.clear_accums()
+    engine.reset_stats()
+    return made, u
+
+
+def test_c1_floor_reads_rho_times_L(engine):
+    say('case: plane-c1 128x128 32 spp, floor = 0.8 x 1000')
+    load(engine, 'plane-c1', width=128, height=128)
+    engine.trace(32)
+    acc, mom = engine.read_accum()
+    L = pt.luminance(acc)
+    assert abs(L[:32].mean() / 800 - 1) < 1e-3, L[:32].mean()
+    st = engine.stats()
+    assert st['nonfinite'] == 0 and st['segments'] > 128 * 128 * 32
+
+
+def test_primary_ray_decisions_match_the_oracle(engine):
+    say('case: room-c2 480x270, sample 0, pick vs oracle')
+    made, u = load(engine, 'room-c2', width=480, height=270)
+    gpos, gstate, gmat, gprim = engine.pick_rays()
+    opos, ostate, omat, oprim = oracle.pick(made['scene'], u)
+    same = (gstate == ostate) & ((gstate != 0) | (gprim == oprim))
+    say('agreement', same.mean(), 'mismatches', int((~same).sum()), 'of', len(same))
+    assert same.mean() >= 0.9999
+    hit = (gstate == 0) & same
+    d = np.linalg.norm(gpos[hit, :3] - opos[hit, :3], axis=1)
+    assert np.quantile(d, 0.999) < 5e-3
+
+
+def test_c2_per_pixel_means_agree_with_the_oracle(engine):
+    say('case: room-c2 48x27, GPU 256 spp vs oracle 64 spp')
+    made, u = load(engine, 'room-c2', width=48, height=27)
+    engine.trace(256)
+    ga, gm = engine.read_accum()
+    oa, om, _ = oracle.render(made['scene'], u, 64, first=100000)
+    gL, oL = pt.luminance(ga), pt.luminance(oa)
+    s = np.sqrt(pt.standard_error(gm) ** 2 + pt.standard_error(om) ** 2)
+    ok = s > 1e-9
+    z = (gL - oL)[ok] / s[ok]
+    say('z std', z.std(), 'frac |z|<3', np.mean(np.abs(z) < 3), 'means', gL.mean(), oL.mean())
+    assert 0.75 < z.std() < 1.3
+    assert np.mean(np.abs(z) < 3) > 0.97
+    se = math.sqrt(np.nansum(s ** 2)) / s.size
+    assert abs(gL.mean() - oL.mean()) < 4 * se + 1e-3 * oL.mean()
+
+
+@pytest.mark.parametrize('case,opts,tol', [('Lambertian', {'roughness': 1, 'specular': 0}, 2e-4),
+                                           ('plastic, rough 0.5', {'roughness': 0.5, 'specular': 0.5}, 3e-3),
+                                           ('metal, rough 0.5', {'roughness': 0.5, 'metallic': 1}, 5e-3)])
+def test_white_furnace(engine, case, opts, tol):
+    say('case: furnace plane', case)
+    load(engine, 'furnace-plane', width=64, height=64, **opts)
+    engine.trace(64)
+    acc, mom = engine.read_accum()
+    L = pt.luminance(acc)[:20]
+    assert abs(L.mean() / 1000 - 1) < tol, L.mean()
+
+
+@pytest.mark.parametrize('g', [0.6, 0.0, -0.5])
+def test_haze_box_conserves_energy(engine, g):
+    say('case: haze furnace g', g)
+    load(engine, 'furnace-haze', width=64, height=64, g=g, maxbounce=256)
+    engine.trace(256)
+    acc, mom = engine.read_accum()
+    L = pt.luminance(acc)
+    se = math.sqrt(np.nansum(pt.standard_error(mom) ** 2)) / L.size
+    say('haze mean', L.mean(), 'se', se)
+    assert abs(L.mean() / 1000 - 1) < max(2e-4, 4 * se / 1000), (L.mean(), se)
+    assert engine.stats()['nonfinite'] == 0
+
+
+def test_compile_failure_keeps_the_previous_scene(engine):
+    say('case: bad scene source after plane-c1')
+    made, u = load(engine, 'plane-c1', width=32, height=32)
+    bad = dict(made['scene'])
+    bad['hip'] = made['scene']['hip'].replace('return best;', 'return bestx;')
+    with pytest.raises(pt.PtError) as e:
+        engine.set_scene(bad)
+    assert e.value.restored and 'bestx' in str(e.value)
+    engine.trace(4)
+    acc, _ = engine.read_accum()
+    assert abs(pt.luminance(acc)[:8].mean() / 800 - 1) < 0.02
+
+
+def test_graph_replay_matches_direct_launches(engine):
+    say('case: room-c2 64x36, 3 samples through the graph and launch by launch')
+    made, u = load(engine, 'room-c2', width=64, height=36)
+    engine.trace(3)
+    a1, _ = engine.read_accum()
+    engine.clear_accums()
+    engine.time_kernels(3)
+    a2, _ = engine.read_accum()
+    L1, L2 = pt.luminance(a1), pt.luminance(a2)
+    assert np.allclose(L1, L2, rtol=1e-4, atol=1e-4 * L1.max())
+
+
+def test_params_move_geometry_without_a_rebuild(engine):
+    say('case: sphere moved by its parameter block')
+    made, u = load(engine, 'plane-c1', width=64, height=64)
+    p0, s0, _, _ = engine.pick_rays()
+    params = np.asarray(made['scene']['params'], np.float32).copy()
+    params[2] = 0.25                      # the plane's frame lifted 0.25 m
+    engine.set_params(params)
+    p1, s1, _, _ = engine.pick_rays()
+    hit = (s0 == 0) & (s1 == 0)
+    assert np.allclose(p1[hit, 2], 0.25, atol=2e-3) and np.allclose(p0[hit, 2], 0.0, atol=2e-3)
+
+
+def test_node_addon_drives_the_engine():
+    say('case: node tests/js/gpu_step.js (createPathTracer through the N-API addon)')
+    r = subprocess.run(['node', os.path.join(ROOT, 'tests', 'js', 'gpu_step.js')], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout)
+    assert abs(out['floor']['mean'] / 800 - 1) < 2e-3, out['floor']
+    assert out['stats']['samples'] == 16 and out['log']['segments'] > 0
