@@ -95,18 +95,22 @@ def main():
     ms = e.time_kernels(ksamples)
     kst = e.stats()
     rounds = u.maxbounce + 1
-    flops_per_step = made['scene']['flops']['dist'] + 12      # the scene's SDF plus the step's own arithmetic
     dom = max(('march', 'occlude'), key=lambda k: ms[k])
+    # FP32 operations the kernel's sphere tracing performed, counted by the kernel itself from the emitter's per-node
+    # counts (nodes evaluated after bounding-sphere culling, the bound tests, each step's own arithmetic)
+    flops_k = kst['flops'] if dom == 'march' else kst['shadow_flops']
     steps_k = kst['steps'] if dom == 'march' else kst['shadow_steps']
+    flops_per_step = flops_k / max(steps_k, 1)
     launches = ksamples * rounds
     avg_launch_ms = ms[dom] / launches
-    flops_per_launch = steps_k * flops_per_step / launches
+    flops_per_launch = flops_k / launches
     achieved = flops_per_launch / (avg_launch_ms * 1e-3) / 1e12
     traffic = traffic_from_profiles()
     roof = {'kernel': 'pt_' + dom, 'bound': 'valu', 'achieved': round(achieved, 3), 'peak': PEAK_FP32_TFLOPS, 'unit': 'TFLOP/s',
             'frac': round(achieved / PEAK_FP32_TFLOPS, 4), 'traffic': traffic,
             'flops_per_launch': flops_per_launch, 'avg_launch_ms': avg_launch_ms,
-            'flops_per_step': flops_per_step, 'kernel_ms_per_sample': {k: v / ksamples for k, v in ms.items()}}
+            'flops_per_step': flops_per_step, 'flops_per_step_unculled': made['scene']['flops']['dist'] + 30,
+            'kernel_ms_per_sample': {k: v / ksamples for k, v in ms.items()}}
     cpu = None if a.no_cpu else cpu_baseline(made)
     segs = st['segments']
     out = {

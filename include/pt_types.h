@@ -46,6 +46,8 @@
 #define PT_S_LEAVING 7       // hits from behind a surface, moved on without a bounce
 #define PT_S_GATHERS 8       // SPPM camera gathers
 #define PT_S_PHOTONS 9       // SPPM photons stored
+#define PT_S_FLOPS 10        // FP32 operations of the path rays' sphere tracing (scene SDF and step arithmetic)
+#define PT_S_SHADOWFLOPS 11  // the same for shadow rays
 #define PT_S_WORDS 16
 
 typedef struct PtCamera {

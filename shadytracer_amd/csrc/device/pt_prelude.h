@@ -248,11 +248,12 @@ __device__ __forceinline__ float dIntersect(float a, float b, float k) {
 __device__ __forceinline__ float dSubtract(float a, float b, float k) { return dIntersect(a, -b, k); }
 
 // ---------------------------------------------------------------------------------------------------
-// A node's frame: params t[0..2] position, t[3..11] rotation (columns: local axes in world), t[12] scale.
+// A node's frame: params t[0..2] position, t[3..11] rotation (columns: local axes in world), t[12] scale,
+// t[13] its inverse.
 // Local point q = R^T (p - pos) / s; a local (n, d) goes back as (R n, d s) (scene_api.js add_new).
 __device__ __forceinline__ V3 xfIn(const float* t, V3 p) {
     V3 d = p - v3(t[0], t[1], t[2]);
-    float is = 1.0f / t[12];
+    float is = t[13];
     return v3(dot(d, v3(t[3], t[4], t[5])) * is, dot(d, v3(t[6], t[7], t[8])) * is, dot(d, v3(t[9], t[10], t[11])) * is);
 }
 __device__ __forceinline__ V4 xfOut(const float* t, V4 s) {

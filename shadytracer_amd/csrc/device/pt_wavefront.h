@@ -52,6 +52,7 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_genrays(const PtDev* _
 // collision when the leg reaches its drawn free flight. Persistent: each wave refills its idle lanes from the
 // queue (one atomic per refill) and appends events to the shade queue (one atomic per wave and step batch).
 #define PT_MARCH_STEPS 4
+#define PT_STEP_FLOPS 30u   // a step's own FP32 work beside the SDF: tolerance, escape test, the move
 extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_march(const PtDev* __restrict__ D, unsigned int qsel) {
     unsigned int* C = D->counters;
     if (blockIdx.x == 0 && threadIdx.x == 0) { C[PT_C_SHADOW] = 0u; C[PT_C_SHADOW_HEAD] = 0u; }
@@ -69,7 +70,7 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_march(const PtDev* __r
     float ltot = 0.0f, n = 1.0f, collide = 3.0e38f;
     unsigned int steps = 0u, z = 0u;
     bool exhausted = false;
-    unsigned long long nsteps = 0ull, nmax = 0ull;
+    unsigned long long nsteps = 0ull, nmax = 0ull, nflops = 0ull;
     for (;;) {
         unsigned long long idle = __ballot(slot < 0);
         if (!exhausted && (unsigned int)__popcll(idle) >= refill) {
@@ -109,7 +110,9 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_march(const PtDev* __r
                     event = true;
                     break;
                 }
-                float d = sceneDist(U, p);
+                unsigned int w = PT_STEP_FLOPS;
+                float d = sceneDistW(U, p, w);
+                nflops += w;
                 float ea = fabsf(d);
                 float eps = marchTol(setting, p);
                 if (ltot > 1000.0f || (steps & 0xFFFFu) > maxiter) state = PT_MAXITER;
@@ -134,6 +137,7 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_march(const PtDev* __r
     }
     statAdd(D, PT_S_STEPS, nsteps);
     statAdd(D, PT_S_MAXITER, nmax);
+    statAdd(D, PT_S_FLOPS, nflops);
 }
 
 // ---------------------------------------------------------------------------------------------------
@@ -445,7 +449,7 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_occlude(const PtDev* _
     float tmax = 0.0f, trav = 0.0f;
     unsigned int steps = 0u;
     bool exhausted = false;
-    unsigned long long nsteps = 0ull, nsegs = 0ull;
+    unsigned long long nsteps = 0ull, nsegs = 0ull, nflops = 0ull;
     for (;;) {
         unsigned long long idle = __ballot(slot < 0);
         if (!exhausted && (unsigned int)__popcll(idle) >= refill) {
@@ -474,7 +478,9 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_occlude(const PtDev* _
         if (slot >= 0) {
             int result = 0;   // 1 reached, 2 blocked
             for (int k = 0; k < PT_MARCH_STEPS; ++k) {
-                float d = sceneDist(U, p);
+                unsigned int w = PT_STEP_FLOPS;
+                float d = sceneDistW(U, p, w);
+                nflops += w;
                 float ea = fabsf(d);
                 if (ea < marchTol(setting, p) || steps > maxiter) { result = 2; break; }
                 if (ea >= tmax - trav || (dot(p, dir) > 0.0f && dot(p, p) > R2)) { result = 1; break; }
@@ -495,6 +501,7 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_occlude(const PtDev* _
     }
     statAdd(D, PT_S_SHADOWSTEPS, nsteps);
     statAdd(D, PT_S_SHADOWSEGS, nsegs);
+    statAdd(D, PT_S_SHADOWFLOPS, nflops);
 }
 
 // ---------------------------------------------------------------------------------------------------

@@ -23,11 +23,13 @@
 const UNIT_METRES = { m: 1, cm: 0.01, mm: 0.001, um: 1e-6, in: 0.0254 }
 const LUMA = [0.2126, 0.7152, 0.0722]
 
-// FP32 operations per distance-only primitive evaluation in its frame (the transform in and out counted apart).
-const PRIM_FLOPS = { plane: 1, sphere: 7, box: 27, cylinder: 22, capsule: 14, torus: 14, rectangle: 14, disc: 13 }
-const XF_FLOPS = 20      // p - pos, three dots, 1/s, the distance times s
+// FP32 operations per distance-only primitive evaluation in its frame (an FMA counts 2; the transform in and out
+// counted apart), as the emitted code does them. Counted by the kernels per evaluated node: the march's roofline.
+const PRIM_FLOPS = { plane: 1, sphere: 7, box: 27, cylinder: 24, capsule: 14, torus: 14, rectangle: 14, disc: 13 }
+const XF_FLOPS = 22      // p - pos, three dot products, times 1/s; the distance times s
 const FOLD_FLOPS = 2
 const JOIN_FLOPS = 9
+const CULL_FLOPS = 9     // |p - c| - r against |best|
 
 // The builtin shapes: their parameters in U (after the 13 of the transform), and the HIP calls.
 const SHAPES = {
@@ -86,10 +88,34 @@ function compose(parent, t, unit) {
     const off = apply3(parent.R, local).map(v => v * parent.s)
     return { R: mul3(parent.R, R), t: [0, 1, 2].map(i => parent.t[i] + off[i]), s: parent.s * s }
 }
-// 13 floats: position, the rotation's three columns, scale
+// 14 floats: position, the rotation's three columns, scale, and its inverse (the march multiplies, never divides)
 function xfParams(X) {
     const R = X.R
-    return [X.t[0], X.t[1], X.t[2], R[0][0], R[1][0], R[2][0], R[0][1], R[1][1], R[2][1], R[0][2], R[1][2], R[2][2], X.s]
+    return [X.t[0], X.t[1], X.t[2], R[0][0], R[1][0], R[2][0], R[0][1], R[1][1], R[2][1], R[0][2], R[1][2], R[2][2], X.s, 1 / X.s]
+}
+// A bounding sphere of a shape in its own frame (a radius about its origin); Infinity for the unbounded plane.
+function localRadius(key, P) {
+    switch (key) {
+        case 'plane': return Infinity
+        case 'sphere': return P.radius
+        case 'box': return Math.hypot(P.size.x, P.size.y, P.size.z)
+        case 'cylinder': return Math.hypot(P.radius, 0.5 * P.height)
+        case 'capsule': return 0.5 * P.height + P.radius
+        case 'torus': return P.ring + P.tube
+        case 'rectangle': return 0.5 * Math.hypot(P.width, P.height)
+        case 'disc': return P.radius
+    }
+    return Infinity
+}
+// The smallest sphere found by enclosing two spheres.
+function encloseSpheres(a, b) {
+    if (!isFinite(a.r) || !isFinite(b.r)) return { c: a.c, r: Infinity }
+    const d = Math.hypot(b.c[0] - a.c[0], b.c[1] - a.c[1], b.c[2] - a.c[2])
+    if (d + b.r <= a.r) return a
+    if (d + a.r <= b.r) return b
+    const r = 0.5 * (d + a.r + b.r)
+    const t = d > 0 ? (r - a.r) / d : 0
+    return { c: [0, 1, 2].map(i => a.c[i] + (b.c[i] - a.c[i]) * t), r }
 }
 
 function fmt(x) { return Number.isFinite(x) ? x : 0 }
@@ -174,8 +200,7 @@ function generate(doc, opts) {
         const xo = push(xfParams(X))
         const po = push(SHAPES[key].params(P))
         layout[n.id] = { xf: xo, params: po, shape: key }
-        flops += PRIM_FLOPS[key] + XF_FLOPS
-        return { key, xo, po, P }
+        return { key, xo, po, P, bs: { c: X.t.slice(), r: localRadius(key, P) * X.s }, flops: PRIM_FLOPS[key] + XF_FLOPS }
     }
     function matOf(n) {
         if (n.type === 'light') return lightMaterial(n)
@@ -221,18 +246,21 @@ function generate(doc, opts) {
             const c = kids[i]
             const v = valueOf(c, X)
             if (!v) continue
-            if (!acc) { acc = v; continue }
+            if (!acc) { acc = Object.assign({}, v); continue }
             const join = i === 0 ? 'union' : (c.join || 'union')
             const k = c.joinBlend !== undefined && c.joinBlend !== null ? c.joinBlend * unit * X.s : blendG
+            // a union's value is at least the smaller of its sides less k/4; an intersection's or a cut's at least the
+            // first side's: the bound that keeps culling exact
+            if (join === 'union') { acc.bs = encloseSpheres(acc.bs, v.bs); acc.bs = { c: acc.bs.c, r: acc.bs.r + 0.25 * Math.max(k, 0) } }
+            acc.flops += v.flops + JOIN_FLOPS
             const ko = push([k])
             const fn = { union: ['dUnion', 'csgUnion'], subtract: ['dSubtract', 'csgSubtract'], intersect: ['dIntersect', 'csgIntersect'] }[join]
             if (!fn) throw new Error(`hipgen: node '${c.id}': unknown join '${join}'`)
             const wins = { union: `${v.f}.w < ${acc.f}.w`, subtract: `-${v.f}.w > ${acc.f}.w`, intersect: `${v.f}.w > ${acc.f}.w` }[join]
-            dist.push(`    ${acc.d} = ${fn[0]}(${acc.d}, ${v.d}, U[${ko}]);`)
-            full.push(`    if (${wins}) { ${acc.m} = ${v.m}; ${acc.k} = ${v.k}; }`)
-            full.push(`    ${acc.f} = ${fn[1]}(${acc.f}, ${v.f}, U[${ko}]);`)
+            dist.push(`        ${acc.d} = ${fn[0]}(${acc.d}, ${v.d}, U[${ko}]);`)
+            full.push(`        if (${wins}) { ${acc.m} = ${v.m}; ${acc.k} = ${v.k}; }`)
+            full.push(`        ${acc.f} = ${fn[1]}(${acc.f}, ${v.f}, U[${ko}]);`)
             program.push({ op: 'join', join, k: ko })
-            flops += JOIN_FLOPS
         }
         return acc
     }
@@ -250,11 +278,11 @@ function generate(doc, opts) {
         const i = tmp++
         const d = `d${i}`, f = `f${i}`, m = `m${i}`, k = `k${i}`
         const prim = program.filter(o => o.op === 'prim').length
-        dist.push(`    float ${d} = ${SHAPES[s.key].dist(s.po)} * U[${s.xo + 12}];`.replace(/\bq\b/, `xfIn(U + ${s.xo}, p)`))
-        full.push(`    V4 ${f} = xfOut(U + ${s.xo}, ${SHAPES[s.key].full(s.po)});`.replace(/\bq\b/, `xfIn(U + ${s.xo}, p)`))
-        full.push(`    unsigned int ${m} = ${mat}u, ${k} = ${prim}u;`)
+        dist.push(`        float ${d} = ${SHAPES[s.key].dist(s.po)} * U[${s.xo + 12}];`.replace(/\bq\b/, `xfIn(U + ${s.xo}, p)`))
+        full.push(`        V4 ${f} = xfOut(U + ${s.xo}, ${SHAPES[s.key].full(s.po)});`.replace(/\bq\b/, `xfIn(U + ${s.xo}, p)`))
+        full.push(`        unsigned int ${m} = ${mat}u, ${k} = ${prim}u;`)
         program.push({ op: 'prim', shape: s.key, xf: s.xo, p: s.po, mat, prim })
-        return { d, f, m, k }
+        return { d, f, m, k, bs: s.bs, flops: s.flops }
     }
     // the closest-surface fold over a plain group's descendants
     function foldNode(n, parentX) {
@@ -265,12 +293,26 @@ function generate(doc, opts) {
             for (const c of n.children || []) foldNode(c, X)
             return
         }
+        // the node's code, guarded by its bounding sphere: skipped where |p - c| - r cannot beat the closest |d| so
+        // far, which leaves the fold's answer exactly as it was (a skipped node is never the closest, nor negative)
+        const d0 = dist.length, f0 = full.length
         const v = valueOf(n, parentX)
         if (!v) return
-        dist.push(`    foldDist(best, ${v.d});`)
-        full.push(`    foldFull(h, ${v.f}, ${v.m}, ${v.k});`)
+        const body = dist.splice(d0), fbody = full.splice(f0)
+        const nodeFlops = v.flops + FOLD_FLOPS
+        if (isFinite(v.bs.r)) {
+            const co = push([v.bs.c[0], v.bs.c[1], v.bs.c[2], v.bs.r])
+            const test = `length(p - v3(U[${co}], U[${co + 1}], U[${co + 2}])) - U[${co + 3}]`
+            dist.push(`    if (${test} < fabsf(best)) {`, ...body, `        foldDist(best, ${v.d});`, `        w += ${nodeFlops}u;`, '    }')
+            dist.push(`    w += ${CULL_FLOPS}u;`)
+            full.push(`    if (${test} < fabsf(h.d)) {`, ...fbody, `        foldFull(h, ${v.f}, ${v.m}, ${v.k});`, '    }')
+            flops += nodeFlops + CULL_FLOPS
+        } else {
+            dist.push('    {', ...body, `        foldDist(best, ${v.d});`, `        w += ${nodeFlops}u;`, '    }')
+            full.push('    {', ...fbody, `        foldFull(h, ${v.f}, ${v.m}, ${v.k});`, '    }')
+            flops += nodeFlops
+        }
         program.push({ op: 'fold' })
-        flops += FOLD_FLOPS
     }
     function hazeOf(n, X) {
         if (haze) throw new Error('hipgen: one haze box a scene')
@@ -294,11 +336,13 @@ function generate(doc, opts) {
     const hip = [
         `// emitted by shadytracer_amd/js/hipgen.js from '${(doc.title || 'scene').replace(/[\n\r]/g, ' ')}'`,
         `#define PT_SCENE_MATERIALS ${materials.length}`,
-        '__device__ __forceinline__ float sceneDist(const float* __restrict__ U, V3 p) {',
+        '// the closest signed distance; w counts the FP32 operations spent (nodes evaluated, bounds tested)',
+        '__device__ __forceinline__ float sceneDistW(const float* __restrict__ U, V3 p, unsigned int& w) {',
         '    float best = 1.0e7f;',
         ...dist,
         '    return best;',
         '}',
+        '__device__ __forceinline__ float sceneDist(const float* __restrict__ U, V3 p) { unsigned int w = 0u; return sceneDistW(U, p, w); }',
         '__device__ __forceinline__ SdfHit sceneFull(const float* __restrict__ U, V3 p) {',
         '    SdfHit h = sdfEmpty();',
         ...full,
@@ -318,7 +362,7 @@ function generate(doc, opts) {
         envRadius,
         unit,
         program,
-        flops: { dist: flops },
+        flops: { dist: flops },   // every node evaluated: the bound a culled step stays under
     }
 }
 

@@ -88,7 +88,7 @@ SIGNATURES = {
 }
 
 STAT_NAMES = ['segments', 'shadow_segments', 'steps', 'shadow_steps', 'nonfinite', 'maxiter', 'paths', 'leaving',
-              'gathers', 'photons']
+              'gathers', 'photons', 'flops', 'shadow_flops']
 LUMA = (0.2126, 0.7152, 0.0722)
 
 _lib = None
