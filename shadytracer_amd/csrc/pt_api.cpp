@@ -481,7 +481,7 @@ int pt_set_uniforms(pt_ctx* c, const PtUniforms* u, size_t size) {
     if (!c || !u) return fail(c, PT_ERR_ARG, "uniforms is NULL");
     if (size != sizeof(PtUniforms)) return fail(c, PT_ERR_ARG, "uniforms: %zu bytes, expected %zu", size, sizeof(PtUniforms));
     if (u->width == 0 || u->height == 0 || u->width > 16384 || u->height > 16384) return fail(c, PT_ERR_ARG, "size %u x %u out of range", u->width, u->height);
-    if (u->maxbounce > 64) return fail(c, PT_ERR_ARG, "maxbounce %u above 64", u->maxbounce);
+    if (u->maxbounce > 250) return fail(c, PT_ERR_ARG, "maxbounce %u above 250 (a path counts its bounces in 8 bits)", u->maxbounce);
     if (u->marchRefill == 0 || u->marchRefill > 64) return fail(c, PT_ERR_ARG, "marchRefill %u not in 1..64", u->marchRefill);
     if (!(u->stepscale > 0.0f && u->stepscale <= 1.0f)) return fail(c, PT_ERR_ARG, "stepscale %g not in (0, 1]", u->stepscale);
     int rc;

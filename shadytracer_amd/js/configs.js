@@ -174,7 +174,7 @@ function furnaceHaze(opts = {}) {
     const mats = [material('none', [0, 0, 0], 1, 0, { specular: 0 })]
     const d = doc('furnace-haze', mats, [obj('speck', 'b-sphere', 'none', xf(0, 0, -50), { radius: 0.01 }), hazeBox], { sky: 'sphere', uniformSky: 1000 }, 20)
     const w = opts.width || 64, h = opts.height || 64
-    return { doc: d, camera: camera([0, -4, 0], [0, 0, 0], 30, w, h), settings: { width: w, height: h, spp: 64, maxbounce: 256, nee: 1 } }
+    return { doc: d, camera: camera([0, -4, 0], [0, 0, 0], 30, w, h), settings: { width: w, height: h, spp: 64, maxbounce: 250, nee: 1 } }
 }
 
 const CONFIGS = { 'plane-c1': planeC1, 'room-c2': roomC2, 'caustic-c3': causticC3, 'furnace-plane': furnacePlane, 'furnace-haze': furnaceHaze }
