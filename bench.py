@@ -55,8 +55,25 @@ def traffic_from_profiles():
         return None
 
 
+def calibration_copy(mib):
+    """A device-to-device copy of known size through the HIP runtime (its copy kernel): under rocprofv3 --pmc its
+    FETCH_SIZE / WRITE_SIZE give the counters' scale, bytes counted per byte moved."""
+    import ctypes
+    hip = ctypes.CDLL('libamdhip64.so')
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) or hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)):
+        raise RuntimeError('hipMalloc failed')
+    hip.hipMemset(a, 1, ctypes.c_size_t(n))
+    hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3)          # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument('--calibrate-mib', type=int, default=0, help='a device copy of this many MiB first (counter scale)')
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=32)
     ap.add_argument('--warmup', type=int, default=4)
@@ -70,6 +87,8 @@ def main():
         sys.exit(2)
 
     from shadytracer_amd import pt, scenes
+    if a.calibrate_mib:
+        calibration_copy(a.calibrate_mib)
     made = scenes.emit('room-c2', width=a.width, height=a.height)
     log('bench: room-c2 %dx%d, warmup %d, steps %d' % (a.width, a.height, a.warmup, a.steps))
     e = pt.Engine()

@@ -33,6 +33,8 @@
 #define PT_C_SHADOW_HEAD 5
 #define PT_C_SAMPLE 6      // samples resolved since the accumulators were cleared (drives the per-sample seed)
 #define PT_C_PHOTON 7      // photons stored in the SPPM map this pass
+#define PT_BINS 1024          // bins of the march queue's counting sort by node mask
+#define PT_BIN_BLOCKS 256     // blocks of that sort (each sorts one contiguous chunk of the queue)
 #define PT_C_WORDS 16
 
 // 64-bit statistics

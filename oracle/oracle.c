@@ -78,9 +78,11 @@ static void sampleSeed(uint32_t base, uint32_t sample, uint32_t* out) {
 }
 
 /* ---- the scene program (hipgen.js `program`) ---- */
-enum { OP_PRIM = 0, OP_JOIN = 1, OP_FOLD = 2 };
+enum { OP_PRIM = 0, OP_JOIN = 1, OP_FOLD = 2, OP_NODE = 3 };
 enum { SH_PLANE = 0, SH_SPHERE, SH_BOX, SH_CYLINDER, SH_CAPSULE, SH_TORUS, SH_RECT, SH_DISC };
 enum { J_UNION = 0, J_SUBTRACT, J_INTERSECT };
+/* prim: shape, xf, p, mat, prim; join: join, k; node (a top-level node's start): xf = its bounding sphere's offset in
+ * U (-1: unbounded), join = its mask bit (-1: none), k = the index of its fold */
 typedef struct { int op, shape, xf, p, mat, prim, join, k; } OrOp;
 
 typedef struct {
@@ -187,13 +189,22 @@ static d4 joinI(d4 a, d4 b, double k) {
 
 typedef struct { d3 n; double d; int m, a, prim; double insideW; } OrHit;
 
-/* The closest-surface fold over the program (scene_api.js:39-63 fold; codegen.js:1821-1828 joins). */
-static OrHit sceneAt(const OrScene* S, d3 p) {
+/* The closest-surface fold over the program (scene_api.js:39-63 fold; codegen.js:1821-1828 joins). A top-level node
+ * is skipped when the ray's mask leaves it out, or when its bounding sphere cannot beat the closest |d| so far (which
+ * changes nothing: such a node is neither the closest surface nor one p is inside). */
+static OrHit sceneAtM(const OrScene* S, d3 p, uint64_t mask) {
     d4 st[64]; int sm[64], sk[64]; int top = 0;
     OrHit h; h.n = v(0, 0, 1); h.d = 1e7; h.m = 0; h.a = 0; h.prim = -1; h.insideW = -1e7;
     for (int i = 0; i < S->nops; ++i) {
         const OrOp* o = &S->ops[i];
-        if (o->op == OP_PRIM) { st[top] = primWorld(S, o, p); sm[top] = o->mat; sk[top] = o->prim; ++top; }
+        if (o->op == OP_NODE) {
+            int skip = o->join >= 0 && !((mask >> o->join) & 1ull);
+            if (!skip && o->xf >= 0) {
+                const float* b = S->U + o->xf;
+                skip = len(sub(p, v(b[0], b[1], b[2]))) - b[3] >= fabs(h.d);
+            }
+            if (skip) i = o->k;
+        } else if (o->op == OP_PRIM) { st[top] = primWorld(S, o, p); sm[top] = o->mat; sk[top] = o->prim; ++top; }
         else if (o->op == OP_JOIN) {
             d4 b = st[--top], a = st[top - 1];
             double k = S->U[o->k];
@@ -210,7 +221,29 @@ static OrHit sceneAt(const OrScene* S, d3 p) {
     }
     return h;
 }
+static OrHit sceneAt(const OrScene* S, d3 p) { return sceneAtM(S, p, ~0ull); }
+/* The nodes a ray from o along d can reach within tmax: those whose bounding sphere, widened by margin, its segment
+ * meets. A node left out can neither be hit nor stop the ray (it never comes within a tolerance of it); leaving it
+ * out only lengthens the march's steps. */
+static uint64_t sceneMask(const OrScene* S, d3 o, d3 d, double tmax, double margin) {
+    uint64_t m = ~0ull;
+    for (int i = 0; i < S->nops; ++i) {
+        const OrOp* op = &S->ops[i];
+        if (op->op != OP_NODE || op->join < 0 || op->xf < 0) continue;
+        const float* b = S->U + op->xf;
+        d3 oc = sub(v(b[0], b[1], b[2]), o);
+        double t = clampd(dotp(oc, d), 0, tmax);
+        d3 q = sub(oc, scl(d, t));
+        double rr = b[3] + margin;
+        if (dotp(q, q) > rr * rr) m &= ~(1ull << op->join);
+    }
+    return m;
+}
+static double maskMargin(const OrScene* S) {
+    return maxd(S->u->eps > 0 ? S->u->eps : S->u->epsAuto, 1e-9) + 128.0 * pow(2.0, -23) * (S->u->envRadius + 1.0);
+}
 static double sceneDist(const OrScene* S, d3 p) { return sceneAt(S, p).d; }
+static double sceneDistM(const OrScene* S, d3 p, uint64_t mask) { return sceneAtM(S, p, mask).d; }
 
 /* THE HIT TOLERANCE (core/tolerances.js:10-34) */
 static double tolSetting(const OrScene* S) { return S->u->eps > 0 ? S->u->eps : S->u->epsAuto; }
@@ -523,9 +556,10 @@ enum { ST_ACTIVE = 0, ST_ESCAPED = 1, ST_MAXITER = 2, ST_SCATTER = 4 };
 static int march(const OrScene* S, d3* p, d3 dir, double n, double* ltot, uint32_t* steps, double collide) {
     const PtUniforms* u = S->u;
     double R2 = (double)u->envRadius * u->envRadius;
+    uint64_t mask = sceneMask(S, *p, dir, 1e300, maskMargin(S));
     for (;;) {
         if (*ltot >= collide) { *p = sub(*p, scl(dir, (*ltot - collide) / n)); *ltot = collide; return ST_SCATTER; }
-        double d = sceneDist(S, *p), ea = fabs(d), eps = marchTol(S, *p);
+        double d = sceneDistM(S, *p, mask), ea = fabs(d), eps = marchTol(S, *p);
         int st = ST_ACTIVE;
         if (*ltot > 1000 || (*steps & 0xFFFFu) > u->maxiter) st = ST_MAXITER;
         if (dotp(*p, dir) > 0 && dotp(*p, *p) > R2) st = ST_ESCAPED;
@@ -542,8 +576,9 @@ static int occluded(const OrScene* S, d3 o, d3 dir, double tmax, long long* step
     double R2 = (double)u->envRadius * u->envRadius, trav = 0;
     d3 p = o;
     uint32_t k = 0;
+    uint64_t mask = sceneMask(S, o, dir, tmax, maskMargin(S));
     for (;;) {
-        double d = sceneDist(S, p), ea = fabs(d);
+        double d = sceneDistM(S, p, mask), ea = fabs(d);
         if (ea < marchTol(S, p) || k > u->maxiter) return 0;
         if (ea >= tmax - trav || (dotp(p, dir) > 0 && dotp(p, p) > R2)) return 1;
         double t = ea * u->stepscale;

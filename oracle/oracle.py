@@ -14,7 +14,7 @@ LIB = os.path.join(HERE, '_build', 'liboracle.so')
 
 SHAPES = ['plane', 'sphere', 'box', 'cylinder', 'capsule', 'torus', 'rectangle', 'disc']
 JOINS = ['union', 'subtract', 'intersect']
-OPS = {'prim': 0, 'join': 1, 'fold': 2}
+OPS = {'prim': 0, 'join': 1, 'fold': 2, 'node': 3}
 
 _lib = None
 
@@ -66,6 +66,8 @@ def pack_program(program):
             out[i, 1] = SHAPES.index(o['shape']); out[i, 2] = o['xf']; out[i, 3] = o['p']; out[i, 4] = o['mat']; out[i, 5] = o['prim']
         elif o['op'] == 'join':
             out[i, 6] = JOINS.index(o['join']); out[i, 7] = o['k']
+        elif o['op'] == 'node':
+            out[i, 2] = o['bs']; out[i, 6] = o['bit']; out[i, 7] = o['end']
     return np.ascontiguousarray(out)
 
 

@@ -25,6 +25,10 @@ struct PtDev {
     unsigned int* qMarch1;
     unsigned int* qShade;
     unsigned int* qShadow;
+    unsigned int* qSorted;          // this round's march queue, binned by the rays' node masks
+    unsigned long long* maskBuf;    // per slot: the nodes its current segment can reach
+    unsigned int* binHist;          // PT_BINS counts, then (scan) the bins' write offsets
+    unsigned int* binOff;
     // shadow rays by slot
     float* shOrig;     // 4/slot: xyz, tmax
     float* shDir;      // 4/slot: xyz, n

@@ -55,6 +55,12 @@ __device__ __forceinline__ void sampleSeed(const PtDev* __restrict__ D, unsigned
 __device__ __forceinline__ float tolSetting(const PtDev* __restrict__ D) { return D->u.eps > 0.0f ? D->u.eps : D->u.epsAuto; }
 __device__ __forceinline__ float offsetTol(const PtDev* __restrict__ D, V3 p) { return 10.0f * marchTol(tolSetting(D), p); }
 
+// How far a node may be from a ray and still matter to its march: the largest hit tolerance anywhere the ray can be
+// (inside the sky sphere), so a node left out of the ray's mask can never come within a tolerance of it.
+__device__ __forceinline__ float maskMargin(const PtDev* __restrict__ D) {
+    return fmaxf(tolSetting(D), 1e-9f) + 128.0f * PT_F32_STEP * (D->u.envRadius + 1.0f);
+}
+
 // meta.z: bounces (8 bits) | state << 8 | flags << 16
 #define PT_F_DELTAONLY 0x1u     // every scatter so far was a delta event (SPPM gathers at the next diffuse hit)
 #define PT_F_NEEPREV 0x2u       // the previous vertex ran light sampling (prevPdf meaningful)

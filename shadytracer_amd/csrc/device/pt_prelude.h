@@ -3,6 +3,11 @@
 // kernels (pt_kernels.h). FP32 throughout, as the reference's WGSL.
 #pragma once
 
+// The scene's parameter block is read through the constant address space: wave-uniform indices then become scalar
+// loads (s_load) into SGPRs instead of per-lane vector loads.
+#define PT_CONST __attribute__((address_space(4)))
+typedef const PT_CONST float* __restrict__ ParamPtr;
+
 #define PT_PI 3.14159265358979f
 #define PT_INV_PI 0.318309886183791f
 #define PT_F32_STEP 1.1920928955078125e-7f
@@ -27,8 +32,13 @@ __device__ __forceinline__ V3& operator*=(V3& a, float s) { a = a * s; return a;
 __device__ __forceinline__ V4 operator-(V4 a) { V4 r; r.x = -a.x; r.y = -a.y; r.z = -a.z; r.w = -a.w; return r; }
 __device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ __forceinline__ V3 cross(V3 a, V3 b) { return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
-__device__ __forceinline__ float length(V3 a) { return sqrtf(dot(a, a)); }
-__device__ __forceinline__ V3 normalize(V3 a) { return a * (1.0f / sqrtf(dot(a, a))); }
+// The hardware's square root, reciprocal and reciprocal square root (v_sqrt/v_rcp/v_rsq_f32, 1 ulp): the IEEE-exact
+// library forms add a denormal rescale and a Newton step per call, which the distance fields do not need.
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float length(V3 a) { return fsqrt(dot(a, a)); }
+__device__ __forceinline__ V3 normalize(V3 a) { return a * frsq(dot(a, a)); }
 __device__ __forceinline__ V3 vabs(V3 a) { return v3(fabsf(a.x), fabsf(a.y), fabsf(a.z)); }
 __device__ __forceinline__ V3 vmax(V3 a, V3 b) { return v3(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)); }
 __device__ __forceinline__ V3 vmin(V3 a, V3 b) { return v3(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)); }
@@ -79,7 +89,7 @@ __device__ __forceinline__ float marchTol(float setting, V3 p) {
 // (core/sdf.js:352-628), restated.
 __device__ __forceinline__ V4 Nsphere(V3 p, float r) {
     float l = length(p);
-    V3 n = l > 1e-30f ? p * (1.0f / l) : v3(0.0f, 0.0f, 1.0f);
+    V3 n = l > 1e-30f ? p * frcp(l) : v3(0.0f, 0.0f, 1.0f);
     return v4(n, l - r);
 }
 __device__ __forceinline__ float Dsphere(V3 p, float r) { return length(p) - r; }
@@ -116,17 +126,17 @@ __device__ __forceinline__ float DboxRounded(V3 p, V3 b, float r) {
 
 // Cylinder along z: half height h, radius r (sdf.js Ncylinder).
 __device__ __forceinline__ V4 Ncylinder(V3 p, float h, float r) {
-    float qx = sqrtf(p.x * p.x + p.y * p.y), qy = fabsf(p.z);
+    float qx = fsqrt(p.x * p.x + p.y * p.y), qy = fabsf(p.z);
     float dx = qx - r, dy = qy - h;
     float ox = fmaxf(dx, 0.0f), oy = fmaxf(dy, 0.0f);
-    float ol = sqrtf(ox * ox + oy * oy);
+    float ol = fsqrt(ox * ox + oy * oy);
     float dist = fminf(fmaxf(dx, dy), 0.0f) + ol;
-    float lxy = fmaxf(qx, 1e-8f);
-    V3 nside = v3(p.x / lxy, p.y / lxy, 0.0f);
+    float ilxy = frcp(fmaxf(qx, 1e-8f));
+    V3 nside = v3(p.x * ilxy, p.y * ilxy, 0.0f);
     V3 ncap = v3(0.0f, 0.0f, signf(p.z));
     V3 n;
     if (dx > 0.0f && dy > 0.0f) {
-        float il = 1.0f / fmaxf(ol, 1e-8f);
+        float il = frcp(fmaxf(ol, 1e-8f));
         n = normalize(v3(nside.x * ox * il, nside.y * ox * il, 0.0f) + ncap * (oy * il));
     } else if (dx > dy) {
         n = nside;
@@ -136,10 +146,10 @@ __device__ __forceinline__ V4 Ncylinder(V3 p, float h, float r) {
     return v4(n, dist);
 }
 __device__ __forceinline__ float Dcylinder(V3 p, float h, float r) {
-    float qx = sqrtf(p.x * p.x + p.y * p.y), qy = fabsf(p.z);
+    float qx = fsqrt(p.x * p.x + p.y * p.y), qy = fabsf(p.z);
     float dx = qx - r, dy = qy - h;
     float ox = fmaxf(dx, 0.0f), oy = fmaxf(dy, 0.0f);
-    return fminf(fmaxf(dx, dy), 0.0f) + sqrtf(ox * ox + oy * oy);
+    return fminf(fmaxf(dx, dy), 0.0f) + fsqrt(ox * ox + oy * oy);
 }
 // A cylinder whose rim is rounded by rr (the rounded cylinder of the configs: the cylinder shrunk by rr, then
 // inflated by it).
@@ -156,23 +166,25 @@ __device__ __forceinline__ float DroundCylinder(V3 p, float h, float r, float rr
 
 // Torus about z: ring radius R, tube radius r (sdf.js Ntorus).
 __device__ __forceinline__ V4 Ntorus(V3 p, float R, float r) {
-    float lxy = sqrtf(p.x * p.x + p.y * p.y);
-    float rx = lxy > 1e-12f ? p.x / lxy : 1.0f, ry = lxy > 1e-12f ? p.y / lxy : 0.0f;
+    float lxy = fsqrt(p.x * p.x + p.y * p.y);
+    float il = frcp(lxy);
+    float rx = lxy > 1e-12f ? p.x * il : 1.0f, ry = lxy > 1e-12f ? p.y * il : 0.0f;
     float qx = lxy - R, qy = p.z;
-    float ql = sqrtf(qx * qx + qy * qy);
-    V3 n = ql > 1e-12f ? v3(rx * qx / ql, ry * qx / ql, qy / ql) : v3(0.0f, 0.0f, 1.0f);
+    float ql = fsqrt(qx * qx + qy * qy);
+    float iq = frcp(ql);
+    V3 n = ql > 1e-12f ? v3(rx * qx * iq, ry * qx * iq, qy * iq) : v3(0.0f, 0.0f, 1.0f);
     return v4(n, ql - r);
 }
 __device__ __forceinline__ float Dtorus(V3 p, float R, float r) {
-    float qx = sqrtf(p.x * p.x + p.y * p.y) - R;
-    return sqrtf(qx * qx + p.z * p.z) - r;
+    float qx = fsqrt(p.x * p.x + p.y * p.y) - R;
+    return fsqrt(qx * qx + p.z * p.z) - r;
 }
 
 // Capsule along z: half length h of the segment, radius r (sdf.js Ncapsule).
 __device__ __forceinline__ V4 Ncapsule(V3 p, float h, float r) {
     V3 v = p - v3(0.0f, 0.0f, fminf(fmaxf(p.z, -h), h));
     float l = length(v);
-    V3 n = l > 1e-12f ? v * (1.0f / l) : v3(1.0f, 0.0f, 0.0f);
+    V3 n = l > 1e-12f ? v * frcp(l) : v3(1.0f, 0.0f, 0.0f);
     return v4(n, l - r);
 }
 __device__ __forceinline__ float Dcapsule(V3 p, float h, float r) {
@@ -188,75 +200,74 @@ __device__ __forceinline__ float Dplane(V3 p) { return p.z; }
 __device__ __forceinline__ V4 Nrectangle(V3 p, float hx, float hy) {
     float ex = fmaxf(fabsf(p.x) - hx, 0.0f), ey = fmaxf(fabsf(p.y) - hy, 0.0f);
     float s = p.z < 0.0f ? -1.0f : 1.0f;
-    float d = sqrtf(ex * ex + ey * ey + p.z * p.z);
+    float d = fsqrt(ex * ex + ey * ey + p.z * p.z);
     if (fmaxf(ex, ey) <= 0.0f || d < 1e-12f) return v4(v3(0.0f, 0.0f, s), fabsf(p.z));
-    return v4(v3(ex / d * signf(p.x), ey / d * signf(p.y), p.z / d), d);
+    float id = frcp(d);
+    return v4(v3(ex * id * signf(p.x), ey * id * signf(p.y), p.z * id), d);
 }
 __device__ __forceinline__ float Drectangle(V3 p, float hx, float hy) {
     float ex = fmaxf(fabsf(p.x) - hx, 0.0f), ey = fmaxf(fabsf(p.y) - hy, 0.0f);
-    return sqrtf(ex * ex + ey * ey + p.z * p.z);
+    return fsqrt(ex * ex + ey * ey + p.z * p.z);
 }
 // A disc sheet of radius r in z = 0.
 __device__ __forceinline__ V4 Ndisc(V3 p, float r) {
-    float l = sqrtf(p.x * p.x + p.y * p.y);
+    float l = fsqrt(p.x * p.x + p.y * p.y);
     float e = fmaxf(l - r, 0.0f);
     float s = p.z < 0.0f ? -1.0f : 1.0f;
-    float d = sqrtf(e * e + p.z * p.z);
+    float d = fsqrt(e * e + p.z * p.z);
     if (e <= 0.0f || d < 1e-12f) return v4(v3(0.0f, 0.0f, s), fabsf(p.z));
-    float il = 1.0f / fmaxf(l, 1e-12f);
-    return v4(v3(e * p.x * il / d, e * p.y * il / d, p.z / d), d);
+    float il = frcp(fmaxf(l, 1e-12f)), id = frcp(d);
+    return v4(v3(e * p.x * il * id, e * p.y * il * id, p.z * id), d);
 }
 __device__ __forceinline__ float Ddisc(V3 p, float r) {
-    float e = fmaxf(sqrtf(p.x * p.x + p.y * p.y) - r, 0.0f);
-    return sqrtf(e * e + p.z * p.z);
+    float e = fmaxf(fsqrt(p.x * p.x + p.y * p.y) - r, 0.0f);
+    return fsqrt(e * e + p.z * p.z);
 }
 
 // ---------------------------------------------------------------------------------------------------
 // CSG joins (scene_api.js:352-373): a k > 0 blend is the polynomial smooth min/max, the normal the gradient of
-// the blend (mix of the two), none where either side has none.
+// the blend (mix of the two), none where either side has none. ik is 1/k (0 when k is 0), from the parameter block.
 __device__ __forceinline__ V3 csgNormal(V3 a, V3 b, float t) {
     if (dot(a, a) < 1e-12f || dot(b, b) < 1e-12f) return v3s(0.0f);
     V3 n = vmix(a, b, t);
     float l = length(n);
-    return l > 1e-12f ? n * (1.0f / l) : v3s(0.0f);
+    return l > 1e-12f ? n * frcp(l) : v3s(0.0f);
 }
-__device__ __forceinline__ V4 csgUnion(V4 a, V4 b, float k) {
+__device__ __forceinline__ V4 csgUnion(V4 a, V4 b, float k, float ik) {
     if (k <= 0.0f) return a.w < b.w ? a : b;
     float h = fmaxf(k - fabsf(a.w - b.w), 0.0f);
     float m = a.w < b.w ? 0.0f : 1.0f;
-    float t = m + (1.0f - 2.0f * m) * 0.5f * h / k;
-    return v4(csgNormal(xyz(a), xyz(b), t), fminf(a.w, b.w) - h * h * 0.25f / k);
+    float t = m + (1.0f - 2.0f * m) * 0.5f * h * ik;
+    return v4(csgNormal(xyz(a), xyz(b), t), fminf(a.w, b.w) - h * h * 0.25f * ik);
 }
-__device__ __forceinline__ V4 csgIntersect(V4 a, V4 b, float k) {
+__device__ __forceinline__ V4 csgIntersect(V4 a, V4 b, float k, float ik) {
     if (k <= 0.0f) return a.w > b.w ? a : b;
     float h = fmaxf(k - fabsf(a.w - b.w), 0.0f);
     float m = a.w > b.w ? 0.0f : 1.0f;
-    float t = m + (1.0f - 2.0f * m) * 0.5f * h / k;
-    return v4(csgNormal(xyz(a), xyz(b), t), fmaxf(a.w, b.w) + h * h * 0.25f / k);
+    float t = m + (1.0f - 2.0f * m) * 0.5f * h * ik;
+    return v4(csgNormal(xyz(a), xyz(b), t), fmaxf(a.w, b.w) + h * h * 0.25f * ik);
 }
-__device__ __forceinline__ V4 csgSubtract(V4 a, V4 b, float k) { return csgIntersect(a, -b, k); }
-__device__ __forceinline__ float dUnion(float a, float b, float k) {
-    if (k <= 0.0f) return fminf(a, b);
+__device__ __forceinline__ V4 csgSubtract(V4 a, V4 b, float k, float ik) { return csgIntersect(a, -b, k, ik); }
+__device__ __forceinline__ float dUnion(float a, float b, float k, float ik) {
     float h = fmaxf(k - fabsf(a - b), 0.0f);
-    return fminf(a, b) - h * h * 0.25f / k;
+    return fminf(a, b) - h * h * 0.25f * ik;
 }
-__device__ __forceinline__ float dIntersect(float a, float b, float k) {
-    if (k <= 0.0f) return fmaxf(a, b);
+__device__ __forceinline__ float dIntersect(float a, float b, float k, float ik) {
     float h = fmaxf(k - fabsf(a - b), 0.0f);
-    return fmaxf(a, b) + h * h * 0.25f / k;
+    return fmaxf(a, b) + h * h * 0.25f * ik;
 }
-__device__ __forceinline__ float dSubtract(float a, float b, float k) { return dIntersect(a, -b, k); }
+__device__ __forceinline__ float dSubtract(float a, float b, float k, float ik) { return dIntersect(a, -b, k, ik); }
 
 // ---------------------------------------------------------------------------------------------------
 // A node's frame: params t[0..2] position, t[3..11] rotation (columns: local axes in world), t[12] scale,
 // t[13] its inverse.
 // Local point q = R^T (p - pos) / s; a local (n, d) goes back as (R n, d s) (scene_api.js add_new).
-__device__ __forceinline__ V3 xfIn(const float* t, V3 p) {
+__device__ __forceinline__ V3 xfIn(const PT_CONST float* t, V3 p) {
     V3 d = p - v3(t[0], t[1], t[2]);
     float is = t[13];
     return v3(dot(d, v3(t[3], t[4], t[5])) * is, dot(d, v3(t[6], t[7], t[8])) * is, dot(d, v3(t[9], t[10], t[11])) * is);
 }
-__device__ __forceinline__ V4 xfOut(const float* t, V4 s) {
+__device__ __forceinline__ V4 xfOut(const PT_CONST float* t, V4 s) {
     V3 n = v3(t[3], t[4], t[5]) * s.x + v3(t[6], t[7], t[8]) * s.y + v3(t[9], t[10], t[11]) * s.z;
     return v4(n, s.w * t[12]);
 }

@@ -47,6 +47,79 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_genrays(const PtDev* _
 }
 
 // ---------------------------------------------------------------------------------------------------
+// BINNING: the round's march queue reordered so that rays needing the same scene nodes share waves. Each ray's node
+// mask (sceneMask: the nodes its segment can reach) is hashed to one of PT_BINS keys and the queue counting-sorted by
+// key: per-block histograms (LDS), one scan over (bin, block), a stable-per-block scatter. A wave of the march then
+// evaluates the union of few masks instead of sixty-four different ones. Random numbers are keyed by slot, so the
+// order changes no path's draws.
+__device__ __forceinline__ unsigned int maskKey(unsigned long long m) {
+    m ^= m >> 33; m *= 0xff51afd7ed558ccdull; m ^= m >> 33; m *= 0xc4ceb9fe1a85ec53ull; m ^= m >> 33;
+    return (unsigned int)(m >> 54);
+}
+__device__ __forceinline__ void binChunk(unsigned int count, unsigned int& lo, unsigned int& hi) {
+    unsigned int chunk = (count + PT_BIN_BLOCKS - 1) / PT_BIN_BLOCKS;
+    lo = min(blockIdx.x * chunk, count);
+    hi = min(lo + chunk, count);
+}
+extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_bin_count(const PtDev* __restrict__ D, unsigned int qsel) {
+    __shared__ unsigned int hist[PT_BINS];
+    for (unsigned int k = threadIdx.x; k < PT_BINS; k += PT_BLOCK) hist[k] = 0u;
+    __syncthreads();
+    const unsigned int count = D->counters[qsel ? PT_C_MARCH1 : PT_C_MARCH0];
+    const unsigned int* __restrict__ q = qsel ? D->qMarch1 : D->qMarch0;
+    ParamPtr U = (const PT_CONST float*)D->params;
+    const float margin = maskMargin(D);
+    unsigned int lo, hi;
+    binChunk(count, lo, hi);
+    for (unsigned int i = lo + threadIdx.x; i < hi; i += PT_BLOCK) {
+        unsigned int slot = q[i];
+        float4 P = ld4(D->pos, slot), Dd = ld4(D->dir, slot);
+        unsigned long long m = sceneMask(U, v3(P.x, P.y, P.z), v3(Dd.x, Dd.y, Dd.z), 3.0e38f, margin);
+        D->maskBuf[slot] = m;
+        atomicAdd(&hist[maskKey(m)], 1u);
+    }
+    __syncthreads();
+    for (unsigned int k = threadIdx.x; k < PT_BINS; k += PT_BLOCK) D->binHist[blockIdx.x * PT_BINS + k] = hist[k];
+}
+// one block of PT_BINS threads: bin k's start is the sum of all bins before it; block b's offset within bin k the sum
+// of the blocks before b
+extern "C" __global__ __launch_bounds__(PT_BINS) void pt_bin_scan(const PtDev* __restrict__ D) {
+    __shared__ unsigned int s[PT_BINS];
+    const unsigned int t = threadIdx.x;
+    unsigned int tot = 0u;
+    for (unsigned int b = 0; b < PT_BIN_BLOCKS; ++b) tot += D->binHist[b * PT_BINS + t];
+    s[t] = tot;
+    __syncthreads();
+    for (unsigned int off = 1; off < PT_BINS; off <<= 1) {
+        unsigned int x = t >= off ? s[t - off] : 0u;
+        __syncthreads();
+        s[t] += x;
+        __syncthreads();
+    }
+    unsigned int run = s[t] - tot;
+    for (unsigned int b = 0; b < PT_BIN_BLOCKS; ++b) {
+        unsigned int c = D->binHist[b * PT_BINS + t];
+        D->binOff[b * PT_BINS + t] = run;
+        run += c;
+    }
+}
+extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_bin_scatter(const PtDev* __restrict__ D, unsigned int qsel) {
+    __shared__ unsigned int rank[PT_BINS];
+    for (unsigned int k = threadIdx.x; k < PT_BINS; k += PT_BLOCK) rank[k] = 0u;
+    __syncthreads();
+    const unsigned int count = D->counters[qsel ? PT_C_MARCH1 : PT_C_MARCH0];
+    const unsigned int* __restrict__ q = qsel ? D->qMarch1 : D->qMarch0;
+    unsigned int lo, hi;
+    binChunk(count, lo, hi);
+    for (unsigned int i = lo + threadIdx.x; i < hi; i += PT_BLOCK) {
+        unsigned int slot = q[i];
+        unsigned int key = maskKey(D->maskBuf[slot]);
+        unsigned int r = atomicAdd(&rank[key], 1u);
+        D->qSorted[D->binOff[blockIdx.x * PT_BINS + key] + r] = slot;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------
 // MARCH (core/wavefront.js:834-1005): sphere tracing by |d| * stepscale; a hit where |d| < marchTol(p); escaped
 // when outside envRadius and heading out; out of steps past maxiter or an optical length of 1000; a haze
 // collision when the leg reaches its drawn free flight. Persistent: each wave refills its idle lanes from the
@@ -57,18 +130,19 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_march(const PtDev* __r
     unsigned int* C = D->counters;
     if (blockIdx.x == 0 && threadIdx.x == 0) { C[PT_C_SHADOW] = 0u; C[PT_C_SHADOW_HEAD] = 0u; }
     const unsigned int count = C[qsel ? PT_C_MARCH1 : PT_C_MARCH0];
-    const unsigned int* __restrict__ q = qsel ? D->qMarch1 : D->qMarch0;
+    const unsigned int* __restrict__ q = D->qSorted;
     const float setting = tolSetting(D);
     const float R2 = D->u.envRadius * D->u.envRadius;
     const float stepscale = D->u.stepscale;
     const unsigned int maxiter = D->u.maxiter;
     const unsigned int refill = D->u.marchRefill;
-    const float* __restrict__ U = D->params;
+    ParamPtr U = (const PT_CONST float*)D->params;
 
     int slot = -1;
     V3 p = v3s(0.0f), dir = v3s(0.0f);
     float ltot = 0.0f, n = 1.0f, collide = 3.0e38f;
     unsigned int steps = 0u, z = 0u;
+    unsigned long long mask = ~0ull;
     bool exhausted = false;
     unsigned long long nsteps = 0ull, nmax = 0ull, nflops = 0ull;
     for (;;) {
@@ -90,6 +164,7 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_march(const PtDev* __r
                     dir = v3(Dd.x, Dd.y, Dd.z); n = Dd.w;
                     steps = M.y; z = M.z;
                     collide = D->aux[4u * slot + 1u];
+                    mask = D->maskBuf[slot];
                 }
             }
         }
@@ -100,6 +175,7 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_march(const PtDev* __r
         unsigned int state = PT_ACTIVE;
         bool event = false;
         if (slot >= 0) {
+            #pragma unroll 1
             for (int k = 0; k < PT_MARCH_STEPS; ++k) {
                 if (ltot >= collide) {
                     // back to the collision point, to be shaded as a point in the haze
@@ -111,7 +187,7 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_march(const PtDev* __r
                     break;
                 }
                 unsigned int w = PT_STEP_FLOPS;
-                float d = sceneDistW(U, p, w);
+                float d = sceneDistM(U, p, mask, w);
                 nflops += w;
                 float ea = fabsf(d);
                 float eps = marchTol(setting, p);
@@ -145,7 +221,7 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_march(const PtDev* __r
 // weighted by the last vertex's MIS; the hit's normal, facing and material; an emitter's light weighted against
 // face sampling; light sampling (faces, or the sky and sun) with the power heuristic; the BSDF's next direction;
 // the bounce cap and Russian roulette; a point in the haze scatters by Henyey-Greenstein.
-__device__ __forceinline__ V3 normalAt(const float* __restrict__ U, V3 p, float h) {
+__device__ __forceinline__ V3 normalAt(ParamPtr U, V3 p, float h) {
     const float a = sceneDist(U, p + v3(h, -h, -h)), b = sceneDist(U, p + v3(-h, -h, h));
     const float c = sceneDist(U, p + v3(-h, h, -h)), d = sceneDist(U, p + v3(h, h, h));
     V3 g = v3(a - b - c + d, -a - b + c + d, -a + b - c + d);
@@ -235,7 +311,7 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
         }
     } else {
         // A SURFACE HIT
-        const float* __restrict__ U = D->params;
+        ParamPtr U = (const PT_CONST float*)D->params;
         SdfHit h = sceneFull(U, p);
         V3 ns = h.n;
         if (dot(ns, ns) < 0.25f) ns = normalAt(U, p, marchTol(tolSetting(D), p));
@@ -443,10 +519,12 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_occlude(const PtDev* _
     const float stepscale = D->u.stepscale;
     const unsigned int maxiter = D->u.maxiter;
     const unsigned int refill = D->u.marchRefill;
-    const float* __restrict__ U = D->params;
+    ParamPtr U = (const PT_CONST float*)D->params;
+    const float margin = maskMargin(D);
     int slot = -1;
     V3 p = v3s(0.0f), dir = v3s(0.0f), o = v3s(0.0f);
     float tmax = 0.0f, trav = 0.0f;
+    unsigned long long mask = ~0ull;
     unsigned int steps = 0u;
     bool exhausted = false;
     unsigned long long nsteps = 0ull, nsegs = 0ull, nflops = 0ull;
@@ -467,6 +545,7 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_occlude(const PtDev* _
                     o = v3(Oo.x, Oo.y, Oo.z); p = o; tmax = Oo.w;
                     dir = v3(Dd.x, Dd.y, Dd.z);
                     trav = 0.0f; steps = 0u;
+                    mask = sceneMask(U, o, dir, tmax, margin);
                     nsegs += 1ull;
                 }
             }
@@ -477,9 +556,10 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_occlude(const PtDev* _
         }
         if (slot >= 0) {
             int result = 0;   // 1 reached, 2 blocked
+            #pragma unroll 1
             for (int k = 0; k < PT_MARCH_STEPS; ++k) {
                 unsigned int w = PT_STEP_FLOPS;
-                float d = sceneDistW(U, p, w);
+                float d = sceneDistM(U, p, mask, w);
                 nflops += w;
                 float ea = fabsf(d);
                 if (ea < marchTol(setting, p) || steps > maxiter) { result = 2; break; }
@@ -589,7 +669,7 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_pick(const PtDev* __re
     unsigned int state = mzState(D->meta[4u * i + 2u]);
     unsigned int prim = 0xFFFFFFFFu, mat = 0xFFFFFFu;
     if (state == PT_ACTIVE) {
-        SdfHit h = sceneFull(D->params, v3(P.x, P.y, P.z));
+        SdfHit h = sceneFull((const PT_CONST float*)D->params, v3(P.x, P.y, P.z));
         prim = h.prim; mat = h.m;
     }
     st4(pos4, i, P.x, P.y, P.z, P.w);

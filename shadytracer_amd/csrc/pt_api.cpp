@@ -23,8 +23,9 @@ namespace {
 
 #include "pt_device_src.inc"   // kPtDevicePre, kPtDevicePost: the device sources as strings
 
-const char* kKernelNames[] = {"pt_genrays", "pt_march", "pt_shade", "pt_occlude", "pt_resolve", "pt_mf_lut", "pt_mf_lut_avg", "pt_pick"};
-enum { K_GENRAYS, K_MARCH, K_SHADE, K_OCCLUDE, K_RESOLVE, K_MFLUT, K_MFAVG, K_PICK, K_COUNT };
+const char* kKernelNames[] = {"pt_genrays", "pt_march", "pt_shade", "pt_occlude", "pt_resolve", "pt_mf_lut", "pt_mf_lut_avg", "pt_pick",
+                              "pt_bin_count", "pt_bin_scan", "pt_bin_scatter"};
+enum { K_GENRAYS, K_MARCH, K_SHADE, K_OCCLUDE, K_RESOLVE, K_MFLUT, K_MFAVG, K_PICK, K_BINCOUNT, K_BINSCAN, K_BINSCATTER, K_COUNT };
 
 thread_local std::string g_createError;
 
@@ -233,12 +234,14 @@ int ensureBuffers(pt_ctx* c, unsigned int w, unsigned int h) {
         destroyGraph(c);
         float** f4[] = {&c->host.pos, &c->host.dir, &c->host.beta, &c->host.rad, &c->host.aux, &c->host.shOrig, &c->host.shDir, &c->host.shRad};
         for (auto pp : f4) { void* p = *pp; freeBuf(c, p); *pp = nullptr; }
-        unsigned int** u1[] = {&c->host.qMarch0, &c->host.qMarch1, &c->host.qShade, &c->host.qShadow};
+        unsigned int** u1[] = {&c->host.qMarch0, &c->host.qMarch1, &c->host.qShade, &c->host.qShadow, &c->host.qSorted};
         for (auto pp : u1) { void* p = *pp; freeBuf(c, p); *pp = nullptr; }
         { void* p = c->host.meta; freeBuf(c, p); c->host.meta = nullptr; }
+        { void* p = c->host.maskBuf; freeBuf(c, p); c->host.maskBuf = nullptr; }
         int rc;
         for (auto pp : f4) if ((rc = allocBuf(c, *pp, size_t(px) * 16))) return rc;
         if ((rc = allocBuf(c, c->host.meta, size_t(px) * 16))) return rc;
+        if ((rc = allocBuf(c, c->host.maskBuf, size_t(px) * 8))) return rc;
         for (auto pp : u1) if ((rc = allocBuf(c, *pp, size_t(px) * 4))) return rc;
         c->capacity = px;
     }
@@ -260,14 +263,17 @@ unsigned int rounds(const pt_ctx* c) { return c->host.u.maxbounce + 1u; }
 hipError_t launch(pt_ctx* c, int k, unsigned int grid, unsigned int qsel) {
     void* args1[] = {&c->dev};
     void* args2[] = {&c->dev, &qsel};
-    bool two = (k == K_MARCH || k == K_SHADE);
-    return hipModuleLaunchKernel(c->fn[k], grid, 1, 1, 256, 1, 1, 0, c->stream, two ? args2 : args1, nullptr);
+    bool two = (k == K_MARCH || k == K_SHADE || k == K_BINCOUNT || k == K_BINSCATTER);
+    unsigned int block = k == K_BINSCAN ? PT_BINS : 256;
+    return hipModuleLaunchKernel(c->fn[k], grid, 1, 1, block, 1, 1, 0, c->stream, two ? args2 : args1, nullptr);
 }
 
 unsigned int gridFor(const pt_ctx* c, int k) {
     switch (k) {
         case K_GENRAYS: return (c->host.nrays + 255u) / 256u;
         case K_RESOLVE: return (c->host.u.width * c->host.u.height + 255u) / 256u;
+        case K_BINCOUNT: case K_BINSCATTER: return PT_BIN_BLOCKS;
+        case K_BINSCAN: return 1;
         default: return c->host.gridBlocks;
     }
 }
@@ -285,6 +291,7 @@ hipError_t enqueueSample(pt_ctx* c, std::vector<hipEvent_t>* events, std::vector
     unsigned int R = rounds(c);
     for (unsigned int it = 0; it < R; ++it) {
         unsigned int in = it & 1u;
+        if ((e = one(K_BINCOUNT, in)) || (e = one(K_BINSCAN, 0)) || (e = one(K_BINSCATTER, in))) return e;
         if ((e = one(K_MARCH, in))) return e;
         if ((e = one(K_SHADE, in ^ 1u))) return e;
         if ((e = one(K_OCCLUDE, 0))) return e;
@@ -383,7 +390,8 @@ int pt_create(pt_ctx** out, const pt_opts* opts) {
     if ((rc = allocBuf(c, c->dev, sizeof(PtDev))) || (rc = allocBuf(c, c->host.counters, PT_C_WORDS * 4)) ||
         (rc = allocBuf(c, c->host.stats, PT_S_WORDS * 8)) || (rc = allocBuf(c, c->host.mf, PT_MF_WORDS * 4)) ||
         (rc = allocBuf(c, c->host.mats, PT_MAX_MATERIALS * sizeof(PtMaterial))) ||
-        (rc = allocBuf(c, c->host.faces, PT_MAX_FACES * sizeof(PtLightFace)))) {
+        (rc = allocBuf(c, c->host.faces, PT_MAX_FACES * sizeof(PtLightFace))) ||
+        (rc = allocBuf(c, c->host.binHist, PT_BIN_BLOCKS * PT_BINS * 4)) || (rc = allocBuf(c, c->host.binOff, PT_BIN_BLOCKS * PT_BINS * 4))) {
         g_createError = c->err;
         pt_destroy(c);
         return rc;
@@ -552,7 +560,7 @@ int pt_time_kernels(pt_ctx* c, uint32_t samples, double* ms, size_t n) {
     int rc;
     if ((rc = syncDev(c))) return rc;
     for (size_t k = 0; k < n; ++k) ms[k] = 0.0;
-    size_t launches = 2 + 3 * size_t(rounds(c));
+    size_t launches = 2 + 6 * size_t(rounds(c));
     std::vector<hipEvent_t> ev(2 * launches);
     for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
     for (uint32_t s = 0; s < samples; ++s) {
@@ -563,7 +571,9 @@ int pt_time_kernels(pt_ctx* c, uint32_t samples, double* ms, size_t n) {
         for (size_t i = 0; i < kinds.size(); ++i) {
             float t = 0.0f;
             hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]);
-            int slot = kinds[i] == K_GENRAYS ? 0 : kinds[i] == K_MARCH ? 1 : kinds[i] == K_SHADE ? 2 : kinds[i] == K_OCCLUDE ? 3 : 4;
+            int k = kinds[i];
+            int slot = k == K_GENRAYS ? 0 : k == K_MARCH ? 1 : k == K_SHADE ? 2 : k == K_OCCLUDE ? 3 : k == K_RESOLVE ? 4 : 5;
+            if (slot >= (int)n) continue;
             ms[slot] += t;
         }
     }
@@ -581,6 +591,9 @@ int pt_pick_rays(pt_ctx* c, float* pos4, uint32_t* info, size_t n) {
     unsigned int* dinfo = nullptr;
     if ((rc = allocBuf(c, dpos, n * 16)) || (rc = allocBuf(c, dinfo, n * 8))) return rc;
     HIPCHK(c, launch(c, K_GENRAYS, gridFor(c, K_GENRAYS), 0));
+    HIPCHK(c, launch(c, K_BINCOUNT, gridFor(c, K_BINCOUNT), 0));
+    HIPCHK(c, launch(c, K_BINSCAN, gridFor(c, K_BINSCAN), 0));
+    HIPCHK(c, launch(c, K_BINSCATTER, gridFor(c, K_BINSCATTER), 0));
     HIPCHK(c, launch(c, K_MARCH, gridFor(c, K_MARCH), 0));
     void* args[] = {&c->dev, &dpos, &dinfo};
     HIPCHK(c, hipModuleLaunchKernel(c->fn[K_PICK], (unsigned int)((n + 255) / 256), 1, 1, 256, 1, 1, 0, c->stream, args, nullptr));

@@ -163,8 +163,8 @@ function generate(doc, opts) {
     const faces = []
     let haze = null
     let flops = 0
-    const dist = [], full = []
-    let tmp = 0
+    const dist = [], full = [], maskCode = []
+    let tmp = 0, maskBits = 0
 
     const push = vals => { const o = params.length; for (const v of vals) params.push(fmt(v)); return o }
 
@@ -253,13 +253,13 @@ function generate(doc, opts) {
             // first side's: the bound that keeps culling exact
             if (join === 'union') { acc.bs = encloseSpheres(acc.bs, v.bs); acc.bs = { c: acc.bs.c, r: acc.bs.r + 0.25 * Math.max(k, 0) } }
             acc.flops += v.flops + JOIN_FLOPS
-            const ko = push([k])
+            const ko = push([k, k > 0 ? 1 / k : 0])
             const fn = { union: ['dUnion', 'csgUnion'], subtract: ['dSubtract', 'csgSubtract'], intersect: ['dIntersect', 'csgIntersect'] }[join]
             if (!fn) throw new Error(`hipgen: node '${c.id}': unknown join '${join}'`)
             const wins = { union: `${v.f}.w < ${acc.f}.w`, subtract: `-${v.f}.w > ${acc.f}.w`, intersect: `${v.f}.w > ${acc.f}.w` }[join]
-            dist.push(`        ${acc.d} = ${fn[0]}(${acc.d}, ${v.d}, U[${ko}]);`)
+            dist.push(`        ${acc.d} = ${fn[0]}(${acc.d}, ${v.d}, U[${ko}], U[${ko + 1}]);`)
             full.push(`        if (${wins}) { ${acc.m} = ${v.m}; ${acc.k} = ${v.k}; }`)
-            full.push(`        ${acc.f} = ${fn[1]}(${acc.f}, ${v.f}, U[${ko}]);`)
+            full.push(`        ${acc.f} = ${fn[1]}(${acc.f}, ${v.f}, U[${ko}], U[${ko + 1}]);`)
             program.push({ op: 'join', join, k: ko })
         }
         return acc
@@ -294,25 +294,38 @@ function generate(doc, opts) {
             return
         }
         // the node's code, guarded by its bounding sphere: skipped where |p - c| - r cannot beat the closest |d| so
-        // far, which leaves the fold's answer exactly as it was (a skipped node is never the closest, nor negative)
+        // far, which leaves the fold's answer exactly as it was (a skipped node is never the closest, nor negative).
+        // And by the ray's node mask (sceneMask): a node whose sphere, widened by the hit tolerance, the ray's segment
+        // never meets can neither be hit nor stop the ray, so its march leaves it out.
+        const start = program.length
+        program.push({ op: 'node' })
         const d0 = dist.length, f0 = full.length
         const v = valueOf(n, parentX)
-        if (!v) return
+        if (!v) { program.splice(start); return }
         const body = dist.splice(d0), fbody = full.splice(f0)
         const nodeFlops = v.flops + FOLD_FLOPS
         if (isFinite(v.bs.r)) {
             const co = push([v.bs.c[0], v.bs.c[1], v.bs.c[2], v.bs.r])
             const test = `length(p - v3(U[${co}], U[${co + 1}], U[${co + 2}])) - U[${co + 3}]`
-            dist.push(`    if (${test} < fabsf(best)) {`, ...body, `        foldDist(best, ${v.d});`, `        w += ${nodeFlops}u;`, '    }')
+            const bit = maskBits < 64 ? maskBits++ : -1
+            const inMask = bit >= 0 ? `((mask >> ${bit}) & 1ull) && ` : ''
+            dist.push(`    if (${inMask}${test} < fabsf(best)) {`, ...body, `        foldDist(best, ${v.d});`, `        w += ${nodeFlops}u;`, '    }')
             dist.push(`    w += ${CULL_FLOPS}u;`)
             full.push(`    if (${test} < fabsf(h.d)) {`, ...fbody, `        foldFull(h, ${v.f}, ${v.m}, ${v.k});`, '    }')
+            if (bit >= 0) {
+                maskCode.push(`    { V3 oc = v3(U[${co}], U[${co + 1}], U[${co + 2}]) - o; float t = fminf(fmaxf(dot(oc, d), 0.0f), tmax);`,
+                    `      V3 q = oc - d * t; float rr = U[${co + 3}] + margin; if (dot(q, q) > rr * rr) m &= ~(1ull << ${bit}); }`)
+            }
+            program[start] = { op: 'node', bs: co, bit, end: -1 }
             flops += nodeFlops + CULL_FLOPS
         } else {
             dist.push('    {', ...body, `        foldDist(best, ${v.d});`, `        w += ${nodeFlops}u;`, '    }')
             full.push('    {', ...fbody, `        foldFull(h, ${v.f}, ${v.m}, ${v.k});`, '    }')
+            program[start] = { op: 'node', bs: -1, bit: -1, end: -1 }
             flops += nodeFlops
         }
         program.push({ op: 'fold' })
+        program[start].end = program.length - 1
     }
     function hazeOf(n, X) {
         if (haze) throw new Error('hipgen: one haze box a scene')
@@ -336,14 +349,22 @@ function generate(doc, opts) {
     const hip = [
         `// emitted by shadytracer_amd/js/hipgen.js from '${(doc.title || 'scene').replace(/[\n\r]/g, ' ')}'`,
         `#define PT_SCENE_MATERIALS ${materials.length}`,
-        '// the closest signed distance; w counts the FP32 operations spent (nodes evaluated, bounds tested)',
-        '__device__ __forceinline__ float sceneDistW(const float* __restrict__ U, V3 p, unsigned int& w) {',
+        `#define PT_SCENE_MASK_BITS ${maskBits}`,
+        '// the nodes a ray from o along d can reach within tmax: those whose bounding sphere, widened by margin, its',
+        '// segment meets (bit k of the mask: the k-th bounded top-level node); unbounded nodes are always marched',
+        '__device__ __forceinline__ unsigned long long sceneMask(ParamPtr U, V3 o, V3 d, float tmax, float margin) {',
+        '    unsigned long long m = ~0ull;',
+        ...maskCode,
+        '    return m;',
+        '}',
+        '// the closest signed distance over the nodes in mask; w counts the FP32 operations spent',
+        '__device__ __forceinline__ float sceneDistM(ParamPtr U, V3 p, unsigned long long mask, unsigned int& w) {',
         '    float best = 1.0e7f;',
         ...dist,
         '    return best;',
         '}',
-        '__device__ __forceinline__ float sceneDist(const float* __restrict__ U, V3 p) { unsigned int w = 0u; return sceneDistW(U, p, w); }',
-        '__device__ __forceinline__ SdfHit sceneFull(const float* __restrict__ U, V3 p) {',
+        '__device__ __forceinline__ float sceneDist(ParamPtr U, V3 p) { unsigned int w = 0u; return sceneDistM(U, p, ~0ull, w); }',
+        '__device__ __forceinline__ SdfHit sceneFull(ParamPtr U, V3 p) {',
         '    SdfHit h = sdfEmpty();',
         ...full,
         '    return h;',
@@ -362,7 +383,7 @@ function generate(doc, opts) {
         envRadius,
         unit,
         program,
-        flops: { dist: flops },   // every node evaluated: the bound a culled step stays under
+        flops: { dist: flops, maskNode: 14 },   // every node evaluated: the bound a culled step stays under
     }
 }
 

@@ -271,9 +271,9 @@ class Engine:
         self._chk(self.lib.pt_reset_stats(self.h))
 
     def time_kernels(self, samples=1):
-        ms = (ctypes.c_double * 5)()
-        self._chk(self.lib.pt_time_kernels(self.h, samples, ms, 5))
-        return dict(zip(['genrays', 'march', 'shade', 'occlude', 'resolve'], list(ms)))
+        ms = (ctypes.c_double * 6)()
+        self._chk(self.lib.pt_time_kernels(self.h, samples, ms, 6))
+        return dict(zip(['genrays', 'march', 'shade', 'occlude', 'resolve', 'bin'], list(ms)))
 
     def pick_rays(self):
         """Each camera ray of one sample to its first event: (pos4 [n, 4], state [n], material [n], prim [n])."""
