@@ -27,15 +27,15 @@
 // Device counters (u32 words)
 #define PT_C_MARCH0 0
 #define PT_C_MARCH1 1
-#define PT_C_MARCH_HEAD 2
-#define PT_C_SHADE 3
 #define PT_C_SHADOW 4
-#define PT_C_SHADOW_HEAD 5
 #define PT_C_SAMPLE 6      // samples resolved since the accumulators were cleared (drives the per-sample seed)
 #define PT_C_PHOTON 7      // photons stored in the SPPM map this pass
 #define PT_BINS 1024          // bins of the march queue's counting sort by node mask
 #define PT_BIN_BLOCKS 256     // blocks of that sort (each sorts one contiguous chunk of the queue)
-#define PT_C_WORDS 16
+#define PT_PARTS 64           // partitions of a queue that the persistent kernels' waves take batches from
+#define PT_C_HEADS_MARCH 64   // PT_PARTS fetch heads of the march queue
+#define PT_C_HEADS_SHADOW 128 // PT_PARTS fetch heads of the shadow queue
+#define PT_C_WORDS 256
 
 // 64-bit statistics
 #define PT_S_SEGMENTS 0      // path segments: march events handed to shade (hit, escape, cap, haze collision)

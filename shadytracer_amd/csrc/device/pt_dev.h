@@ -23,7 +23,6 @@ struct PtDev {
     // queues of slots
     unsigned int* qMarch0;
     unsigned int* qMarch1;
-    unsigned int* qShade;
     unsigned int* qShadow;
     unsigned int* qSorted;          // this round's march queue, binned by the rays' node masks
     unsigned long long* maskBuf;    // per slot: the nodes its current segment can reach

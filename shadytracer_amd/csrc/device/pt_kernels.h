@@ -45,6 +45,45 @@ __device__ __forceinline__ void st4(float* p, unsigned int i, float a, float b, 
 }
 __device__ __forceinline__ void st4u(unsigned int* p, unsigned int i, uint4 v) { reinterpret_cast<uint4*>(p)[i] = v; }
 
+// A wave's share of a queue, for the persistent kernels: batches of PT_GRAB consecutive entries taken by one atomic
+// from one of PT_PARTS partitions of the queue; a wave starts on its own partition and moves to the next when that
+// is empty, reading the head first so that an empty partition costs no atomic. Wave-uniform state.
+#define PT_GRAB 64u
+struct WaveFetch { unsigned int cur, end, part, left; bool done; };
+__device__ __forceinline__ WaveFetch fetchInit() {
+    WaveFetch f;
+    unsigned int wave = (blockIdx.x * PT_BLOCK + threadIdx.x) >> 6;
+    f.cur = 0u; f.end = 0u; f.part = wave % PT_PARTS; f.left = PT_PARTS; f.done = false;
+    return f;
+}
+// Every lane calls it; lanes with `want` may receive a queue index (else 0xFFFFFFFF).
+__device__ __forceinline__ unsigned int fetchNext(WaveFetch& f, unsigned int* heads, unsigned int count, bool want) {
+    unsigned int got = 0xFFFFFFFFu;
+    unsigned long long need = __ballot(want);
+    const unsigned int chunk = (count + PT_PARTS - 1u) / PT_PARTS;
+    while (need != 0ull && !f.done) {
+        if (f.cur >= f.end) {
+            unsigned int lo = min(f.part * chunk, count), hi = min(lo + chunk, count);
+            unsigned int base = __atomic_load_n(&heads[f.part], __ATOMIC_RELAXED);
+            if (lo + base < hi) {
+                if (laneId() == 0u) base = atomicAdd(&heads[f.part], PT_GRAB);
+                base = __shfl(base, 0);
+            }
+            if (lo + base < hi) { f.cur = lo + base; f.end = min(lo + base + PT_GRAB, hi); }
+            else { f.part = (f.part + 1u) % PT_PARTS; if (--f.left == 0u) f.done = true; }
+            continue;
+        }
+        unsigned int take = min((unsigned int)__popcll(need), f.end - f.cur);
+        bool mine = (need >> laneId()) & 1ull;
+        unsigned int r = rankIn(need);
+        bool served = mine && r < take;
+        if (served) got = f.cur + r;
+        need &= ~__ballot(served);
+        f.cur += take;
+    }
+    return got;
+}
+
 // This sample's seed: the base seed and the number of samples resolved so far, through PCG4D.
 __device__ __forceinline__ void sampleSeed(const PtDev* __restrict__ D, unsigned int* s) {
     Rng r; r.a = D->u.seed; r.b = D->counters[PT_C_SAMPLE]; r.c = 0x5EED5EEDu; r.d = 0x2026F00Du;

@@ -7,10 +7,8 @@
 extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_genrays(const PtDev* __restrict__ D) {
     unsigned int i = blockIdx.x * PT_BLOCK + threadIdx.x;
     unsigned int* C = D->counters;
-    if (i == 0) {
-        C[PT_C_MARCH0] = D->nrays; C[PT_C_MARCH1] = 0u; C[PT_C_MARCH_HEAD] = 0u;
-        C[PT_C_SHADE] = 0u; C[PT_C_SHADOW] = 0u; C[PT_C_SHADOW_HEAD] = 0u;
-    }
+    if (i == 0) { C[PT_C_MARCH0] = D->nrays; C[PT_C_MARCH1] = 0u; C[PT_C_SHADOW] = 0u; }
+    if (i < 2u * PT_PARTS) C[PT_C_HEADS_MARCH + i] = 0u;
     if (i >= D->nrays) return;
     unsigned int seed[4];
     sampleSeed(D, seed);
@@ -123,12 +121,17 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_bin_scatter(const PtDe
 // MARCH (core/wavefront.js:834-1005): sphere tracing by |d| * stepscale; a hit where |d| < marchTol(p); escaped
 // when outside envRadius and heading out; out of steps past maxiter or an optical length of 1000; a haze
 // collision when the leg reaches its drawn free flight. Persistent: each wave refills its idle lanes from the
-// queue (one atomic per refill) and appends events to the shade queue (one atomic per wave and step batch).
+// (binned) queue in batches (fetchNext). Every ray ends its march in exactly one event, so shade reads the same
+// queue: no event queue, no append.
 #define PT_MARCH_STEPS 4
 #define PT_STEP_FLOPS 30u   // a step's own FP32 work beside the SDF: tolerance, escape test, the move
 extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_march(const PtDev* __restrict__ D, unsigned int qsel) {
     unsigned int* C = D->counters;
-    if (blockIdx.x == 0 && threadIdx.x == 0) { C[PT_C_SHADOW] = 0u; C[PT_C_SHADOW_HEAD] = 0u; }
+    // the shadow queue this round's shade fills, and its fetch heads, start empty
+    if (blockIdx.x == 0) {
+        if (threadIdx.x == 0) C[PT_C_SHADOW] = 0u;
+        if (threadIdx.x < PT_PARTS) C[PT_C_HEADS_SHADOW + threadIdx.x] = 0u;
+    }
     const unsigned int count = C[qsel ? PT_C_MARCH1 : PT_C_MARCH0];
     const unsigned int* __restrict__ q = D->qSorted;
     const float setting = tolSetting(D);
@@ -143,20 +146,14 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_march(const PtDev* __r
     float ltot = 0.0f, n = 1.0f, collide = 3.0e38f;
     unsigned int steps = 0u, z = 0u;
     unsigned long long mask = ~0ull;
-    bool exhausted = false;
+    WaveFetch f = fetchInit();
     unsigned long long nsteps = 0ull, nmax = 0ull, nflops = 0ull;
     for (;;) {
         unsigned long long idle = __ballot(slot < 0);
-        if (!exhausted && (unsigned int)__popcll(idle) >= refill) {
-            unsigned int n_idle = (unsigned int)__popcll(idle);
-            int leader = __ffsll((long long)idle) - 1;
-            unsigned int base = 0u;
-            if ((int)laneId() == leader) base = atomicAdd(&C[PT_C_MARCH_HEAD], n_idle);
-            base = __shfl(base, leader);
-            if (base + n_idle >= count) exhausted = true;
+        if (!f.done && (unsigned int)__popcll(idle) >= refill) {
+            unsigned int k = fetchNext(f, C + PT_C_HEADS_MARCH, count, slot < 0);
             if (slot < 0) {
-                unsigned int k = base + rankIn(idle);
-                if (k < count) {
+                if (k != 0xFFFFFFFFu) {
                     slot = (int)q[k];
                     float4 P = ld4(D->pos, slot), Dd = ld4(D->dir, slot);
                     uint4 M = ld4u(D->meta, slot);
@@ -169,7 +166,7 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_march(const PtDev* __r
             }
         }
         if (__ballot(slot >= 0) == 0ull) {
-            if (exhausted) break;
+            if (f.done) break;
             continue;
         }
         unsigned int state = PT_ACTIVE;
@@ -201,10 +198,8 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_march(const PtDev* __r
                 ltot += t * n;
             }
         }
-        unsigned int qi = waveAppend(&C[PT_C_SHADE], event);
         if (event) {
             if (state == PT_MAXITER) nmax += 1ull;
-            D->qShade[qi] = (unsigned int)slot;
             st4(D->pos, slot, p.x, p.y, p.z, ltot);
             D->meta[4u * slot + 1u] = steps;
             D->meta[4u * slot + 2u] = mzPack(mzBounces(z), state, mzFlags(z));
@@ -481,9 +476,8 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_shade(const PtDev* __r
     }
     __syncthreads();
     unsigned int* C = D->counters;
-    // the queue march read this round, and its fetch head, start empty for the round after next
-    if (blockIdx.x == 0 && threadIdx.x == 0) { C[qsel ? PT_C_MARCH0 : PT_C_MARCH1] = 0u; C[PT_C_MARCH_HEAD] = 0u; }
-    const unsigned int count = C[PT_C_SHADE];
+    // this round's rays: the queue march read (binned), every one of them now at its event
+    const unsigned int count = C[qsel ? PT_C_MARCH0 : PT_C_MARCH1];
     unsigned int seed[4];
     sampleSeed(D, seed);
     unsigned int* qout = qsel ? D->qMarch1 : D->qMarch0;
@@ -494,7 +488,7 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_shade(const PtDev* __r
         ShadeOut o; o.requeue = false; o.shadow = false;
         unsigned int slot = 0u;
         if (i < count) {
-            slot = D->qShade[i];
+            slot = D->qSorted[i];
             o = shadeOne(D, smat, seed, slot);
             segs += 1ull;
         }
@@ -510,9 +504,14 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_shade(const PtDev* __r
 // OCCLUDE (core/wavefront.js:1481-1692): a shadow ray is blocked where |d| < marchTol, reaches its light when it
 // leaves the scene (sky, sun) or when the sphere of free space around it covers what is left to the face; a
 // reached ray deposits its contribution, dimmed by the haze it crossed. Persistent, like march.
-extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_occlude(const PtDev* __restrict__ D) {
+extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_occlude(const PtDev* __restrict__ D, unsigned int qsel) {
     unsigned int* C = D->counters;
-    if (blockIdx.x == 0 && threadIdx.x == 0) C[PT_C_SHADE] = 0u;
+    // the march queue this round read (shade has consumed it) and the march's fetch heads start empty for the round
+    // after next
+    if (blockIdx.x == 0) {
+        if (threadIdx.x == 0) C[qsel ? PT_C_MARCH1 : PT_C_MARCH0] = 0u;
+        if (threadIdx.x < PT_PARTS) C[PT_C_HEADS_MARCH + threadIdx.x] = 0u;
+    }
     const unsigned int count = C[PT_C_SHADOW];
     const float setting = tolSetting(D);
     const float R2 = D->u.envRadius * D->u.envRadius;
@@ -526,20 +525,14 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_occlude(const PtDev* _
     float tmax = 0.0f, trav = 0.0f;
     unsigned long long mask = ~0ull;
     unsigned int steps = 0u;
-    bool exhausted = false;
+    WaveFetch f = fetchInit();
     unsigned long long nsteps = 0ull, nsegs = 0ull, nflops = 0ull;
     for (;;) {
         unsigned long long idle = __ballot(slot < 0);
-        if (!exhausted && (unsigned int)__popcll(idle) >= refill) {
-            unsigned int n_idle = (unsigned int)__popcll(idle);
-            int leader = __ffsll((long long)idle) - 1;
-            unsigned int base = 0u;
-            if ((int)laneId() == leader) base = atomicAdd(&C[PT_C_SHADOW_HEAD], n_idle);
-            base = __shfl(base, leader);
-            if (base + n_idle >= count) exhausted = true;
+        if (!f.done && (unsigned int)__popcll(idle) >= refill) {
+            unsigned int k = fetchNext(f, C + PT_C_HEADS_SHADOW, count, slot < 0);
             if (slot < 0) {
-                unsigned int k = base + rankIn(idle);
-                if (k < count) {
+                if (k != 0xFFFFFFFFu) {
                     slot = (int)D->qShadow[k];
                     float4 Oo = ld4(D->shOrig, slot), Dd = ld4(D->shDir, slot);
                     o = v3(Oo.x, Oo.y, Oo.z); p = o; tmax = Oo.w;
@@ -551,7 +544,7 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_occlude(const PtDev* _
             }
         }
         if (__ballot(slot >= 0) == 0ull) {
-            if (exhausted) break;
+            if (f.done) break;
             continue;
         }
         if (slot >= 0) {

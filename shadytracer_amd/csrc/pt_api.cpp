@@ -234,7 +234,7 @@ int ensureBuffers(pt_ctx* c, unsigned int w, unsigned int h) {
         destroyGraph(c);
         float** f4[] = {&c->host.pos, &c->host.dir, &c->host.beta, &c->host.rad, &c->host.aux, &c->host.shOrig, &c->host.shDir, &c->host.shRad};
         for (auto pp : f4) { void* p = *pp; freeBuf(c, p); *pp = nullptr; }
-        unsigned int** u1[] = {&c->host.qMarch0, &c->host.qMarch1, &c->host.qShade, &c->host.qShadow, &c->host.qSorted};
+        unsigned int** u1[] = {&c->host.qMarch0, &c->host.qMarch1, &c->host.qShadow, &c->host.qSorted};
         for (auto pp : u1) { void* p = *pp; freeBuf(c, p); *pp = nullptr; }
         { void* p = c->host.meta; freeBuf(c, p); c->host.meta = nullptr; }
         { void* p = c->host.maskBuf; freeBuf(c, p); c->host.maskBuf = nullptr; }
@@ -263,7 +263,7 @@ unsigned int rounds(const pt_ctx* c) { return c->host.u.maxbounce + 1u; }
 hipError_t launch(pt_ctx* c, int k, unsigned int grid, unsigned int qsel) {
     void* args1[] = {&c->dev};
     void* args2[] = {&c->dev, &qsel};
-    bool two = (k == K_MARCH || k == K_SHADE || k == K_BINCOUNT || k == K_BINSCATTER);
+    bool two = (k == K_MARCH || k == K_SHADE || k == K_BINCOUNT || k == K_BINSCATTER || k == K_OCCLUDE);
     unsigned int block = k == K_BINSCAN ? PT_BINS : 256;
     return hipModuleLaunchKernel(c->fn[k], grid, 1, 1, block, 1, 1, 0, c->stream, two ? args2 : args1, nullptr);
 }
@@ -294,7 +294,7 @@ hipError_t enqueueSample(pt_ctx* c, std::vector<hipEvent_t>* events, std::vector
         if ((e = one(K_BINCOUNT, in)) || (e = one(K_BINSCAN, 0)) || (e = one(K_BINSCATTER, in))) return e;
         if ((e = one(K_MARCH, in))) return e;
         if ((e = one(K_SHADE, in ^ 1u))) return e;
-        if ((e = one(K_OCCLUDE, 0))) return e;
+        if ((e = one(K_OCCLUDE, in))) return e;
     }
     return one(K_RESOLVE, 0);
 }
