@@ -59,20 +59,25 @@ __device__ __forceinline__ void binChunk(unsigned int count, unsigned int& lo, u
     lo = min(blockIdx.x * chunk, count);
     hi = min(lo + chunk, count);
 }
+// qsel 0/1: the march queue of that parity; 2: the shadow queue (its segments end at tmax)
 extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_bin_count(const PtDev* __restrict__ D, unsigned int qsel) {
     __shared__ unsigned int hist[PT_BINS];
     for (unsigned int k = threadIdx.x; k < PT_BINS; k += PT_BLOCK) hist[k] = 0u;
     __syncthreads();
-    const unsigned int count = D->counters[qsel ? PT_C_MARCH1 : PT_C_MARCH0];
-    const unsigned int* __restrict__ q = qsel ? D->qMarch1 : D->qMarch0;
+    const bool shadow = qsel == 2u;
+    const unsigned int count = D->counters[shadow ? PT_C_SHADOW : (qsel ? PT_C_MARCH1 : PT_C_MARCH0)];
+    const unsigned int* __restrict__ q = shadow ? D->qShadow : (qsel ? D->qMarch1 : D->qMarch0);
+    const float* __restrict__ org = shadow ? D->shOrig : D->pos;
+    const float* __restrict__ dir = shadow ? D->shDir : D->dir;
     ParamPtr U = (const PT_CONST float*)D->params;
     const float margin = maskMargin(D);
     unsigned int lo, hi;
     binChunk(count, lo, hi);
     for (unsigned int i = lo + threadIdx.x; i < hi; i += PT_BLOCK) {
         unsigned int slot = q[i];
-        float4 P = ld4(D->pos, slot), Dd = ld4(D->dir, slot);
-        unsigned long long m = sceneMask(U, v3(P.x, P.y, P.z), v3(Dd.x, Dd.y, Dd.z), 3.0e38f, margin);
+        float4 P = ld4(org, slot), Dd = ld4(dir, slot);
+        float tmax = shadow ? P.w : 3.0e38f;
+        unsigned long long m = sceneMask(U, v3(P.x, P.y, P.z), v3(Dd.x, Dd.y, Dd.z), tmax, margin);
         D->maskBuf[slot] = m;
         atomicAdd(&hist[maskKey(m)], 1u);
     }
@@ -105,8 +110,9 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_bin_scatter(const PtDe
     __shared__ unsigned int rank[PT_BINS];
     for (unsigned int k = threadIdx.x; k < PT_BINS; k += PT_BLOCK) rank[k] = 0u;
     __syncthreads();
-    const unsigned int count = D->counters[qsel ? PT_C_MARCH1 : PT_C_MARCH0];
-    const unsigned int* __restrict__ q = qsel ? D->qMarch1 : D->qMarch0;
+    const bool shadow = qsel == 2u;
+    const unsigned int count = D->counters[shadow ? PT_C_SHADOW : (qsel ? PT_C_MARCH1 : PT_C_MARCH0)];
+    const unsigned int* __restrict__ q = shadow ? D->qShadow : (qsel ? D->qMarch1 : D->qMarch0);
     unsigned int lo, hi;
     binChunk(count, lo, hi);
     for (unsigned int i = lo + threadIdx.x; i < hi; i += PT_BLOCK) {
@@ -519,7 +525,6 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_occlude(const PtDev* _
     const unsigned int maxiter = D->u.maxiter;
     const unsigned int refill = D->u.marchRefill;
     ParamPtr U = (const PT_CONST float*)D->params;
-    const float margin = maskMargin(D);
     int slot = -1;
     V3 p = v3s(0.0f), dir = v3s(0.0f), o = v3s(0.0f);
     float tmax = 0.0f, trav = 0.0f;
@@ -533,12 +538,12 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_occlude(const PtDev* _
             unsigned int k = fetchNext(f, C + PT_C_HEADS_SHADOW, count, slot < 0);
             if (slot < 0) {
                 if (k != 0xFFFFFFFFu) {
-                    slot = (int)D->qShadow[k];
+                    slot = (int)D->qSorted[k];
                     float4 Oo = ld4(D->shOrig, slot), Dd = ld4(D->shDir, slot);
                     o = v3(Oo.x, Oo.y, Oo.z); p = o; tmax = Oo.w;
                     dir = v3(Dd.x, Dd.y, Dd.z);
                     trav = 0.0f; steps = 0u;
-                    mask = sceneMask(U, o, dir, tmax, margin);
+                    mask = D->maskBuf[slot];
                     nsegs += 1ull;
                 }
             }

@@ -294,6 +294,7 @@ hipError_t enqueueSample(pt_ctx* c, std::vector<hipEvent_t>* events, std::vector
         if ((e = one(K_BINCOUNT, in)) || (e = one(K_BINSCAN, 0)) || (e = one(K_BINSCATTER, in))) return e;
         if ((e = one(K_MARCH, in))) return e;
         if ((e = one(K_SHADE, in ^ 1u))) return e;
+        if ((e = one(K_BINCOUNT, 2)) || (e = one(K_BINSCAN, 0)) || (e = one(K_BINSCATTER, 2))) return e;
         if ((e = one(K_OCCLUDE, in))) return e;
     }
     return one(K_RESOLVE, 0);
@@ -560,7 +561,7 @@ int pt_time_kernels(pt_ctx* c, uint32_t samples, double* ms, size_t n) {
     int rc;
     if ((rc = syncDev(c))) return rc;
     for (size_t k = 0; k < n; ++k) ms[k] = 0.0;
-    size_t launches = 2 + 6 * size_t(rounds(c));
+    size_t launches = 2 + 9 * size_t(rounds(c));
     std::vector<hipEvent_t> ev(2 * launches);
     for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
     for (uint32_t s = 0; s < samples; ++s) {
