@@ -30,6 +30,8 @@
 #define PT_C_SHADOW 4
 #define PT_C_SAMPLE 6      // samples resolved since the accumulators were cleared (drives the per-sample seed)
 #define PT_C_PHOTON 7      // photons stored in the SPPM map this pass
+#define PT_C_RADIUS 8      // this sample's SPPM gather radius r_k (float bits), set by the photon pass
+#define PT_SPPM_HEADS (1u << 20)   // the photon map's hash-grid lists (the reference's SPPM_HEADS, sppm.js:79)
 #define PT_BINS 1024          // bins of the march queue's counting sort by node mask
 #define PT_BIN_BLOCKS 256     // blocks of that sort (each sorts one contiguous chunk of the queue)
 #define PT_PARTS 64           // partitions of a queue that the persistent kernels' waves take batches from
@@ -50,6 +52,10 @@
 #define PT_S_PHOTONS 9       // SPPM photons stored
 #define PT_S_FLOPS 10        // FP32 operations of the path rays' sphere tracing (scene SDF and step arithmetic)
 #define PT_S_SHADOWFLOPS 11  // the same for shadow rays
+#define PT_S_PHOTONSEGS 12   // photon segments traced (SPPM photon passes)
+#define PT_S_GATHERCUT 13    // SPPM gathers that reached the bound on records visited
+#define PT_S_RADIUS 14       // the last photon pass's gather radius r_k (float bits; set, not summed)
+#define PT_S_OVERFLOW 15     // SPPM stores dropped for want of room (the map holds 2 records a photon)
 #define PT_S_WORDS 16
 
 typedef struct PtCamera {
@@ -100,8 +106,10 @@ typedef struct PtEnv {
     float sunDir[3];      // towards the sun
     float sunCos;         // cos of the sun's angular radius
     float sunE;           // illuminance on a surface facing the sun, lux (0: no sun)
-    float sunRadius;      // aperture radius around sunAxisPoint (0: unlimited)
-    float sunPoint[3];    // a point on the aperture's axis
+    float sunRadius;      // the beam's radius (0: no aperture, the sun lights everything)
+    float sunPoint[3];    // the centre of the beam's disc: with an aperture, the sun is seen along the rays that
+                          // cross the disc of radius sunRadius about sunPoint, perpendicular to sunDir (and the
+                          // photon pass emits the beam from that disc)
     float pad[2];
 } PtEnv;
 
@@ -137,12 +145,12 @@ typedef struct PtUniforms {
     PtHaze haze;
     float faceShare;            // share of light sampling that goes to light faces (0.5)
     unsigned int sppm;          // 1: camera paths after delta-only chains gather from the photon map
-    float sppmRadius;           // current gather radius, metres (host schedule)
-    unsigned int sppmPhotons;   // photons emitted in the map's pass (normalises the gather)
+    float sppmRadius;           // r_1: the first sample's gather radius, metres
+    unsigned int sppmPhotons;   // photons emitted in each sample's photon pass
     unsigned int marchRefill;   // lanes a wave lets go idle before it fetches more rays
     unsigned int guide;         // 1: MLP path guiding mixed with BSDF sampling
     float guideFrac;            // share of guided directions
-    unsigned int pad0;
+    float sppmAlpha;            // the radius schedule's alpha: r_k^2 = r_1^2 prod_{i<k} (i + alpha) / (i + 1)
 } PtUniforms;
 
 

@@ -476,11 +476,15 @@ static d3 envBg(const PtEnv* e, d3 d) {
     if (e->type == 1 || (e->type == 2 && d.z > 0)) return scl(f3(e->col), e->L);
     return v(0, 0, 0);
 }
+/* the sun through its aperture: the lines crossing the disc of radius sunRadius about sunPoint, perpendicular to the
+ * sun (the disc the photon pass emits the beam from) */
 static d3 envSun(const PtEnv* e, d3 p, d3 d) {
     if (!inSun(e, d)) return v(0, 0, 0);
     if (e->sunRadius > 0) {
-        d3 a = f3(e->sunDir), q = sub(p, f3(e->sunPoint)), perp = sub(q, scl(a, dotp(q, a)));
-        if (dotp(perp, perp) > (double)e->sunRadius * e->sunRadius) return v(0, 0, 0);
+        d3 a = f3(e->sunDir), c = f3(e->sunPoint);
+        double t = dotp(sub(c, p), a) / dotp(d, a);
+        d3 q = sub(add(p, scl(d, t)), c);
+        if (!(dotp(q, q) <= (double)e->sunRadius * e->sunRadius)) return v(0, 0, 0);
     }
     double L = e->sunE / sunOmega(e);
     return v(L, L, L);
@@ -619,12 +623,156 @@ static void cameraRay(const OrScene* S, const OrTiles* T, const uint32_t* seed, 
     *uhaze = u4[2];
 }
 
-typedef struct { long long segments, shadow, steps, shadowSteps, maxiter, nonfinite; } OrStats;
+typedef struct { long long segments, shadow, steps, shadowSteps, maxiter, nonfinite, photonSegs, stored, gathers; } OrStats;
+
+/* ---- SPPM (core/sppm.js:79-192 map and schedule, :487-520 gather; core/wavefront.js:542-626 photon emission) ----
+ * The oracle's map is exact: records sorted by cell (side 2r), every record of the 8 cells about a gather point
+ * walked (the engine subsamples crowded cells by sub-lists, exact in expectation). */
+typedef struct { double p[3], pw[3], l[3], n[3]; long long key; } OrPhoton;
+typedef struct { OrPhoton* rec; size_t n, cap; double r; } OrMap;
+
+static long long cellKey(long long i, long long j, long long k) {
+    return ((i + (1LL << 20)) << 42) | ((j + (1LL << 20)) << 21) | (k + (1LL << 20));
+}
+static int photonCmp(const void* a, const void* b) {
+    long long x = ((const OrPhoton*)a)->key, y = ((const OrPhoton*)b)->key;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+static void mapPush(OrMap* m, d3 p, d3 pw, d3 l, d3 n) {
+    if (m->n == m->cap) { m->cap = m->cap ? 2 * m->cap : 1024; m->rec = (OrPhoton*)realloc(m->rec, m->cap * sizeof(OrPhoton)); }
+    OrPhoton* r = &m->rec[m->n++];
+    r->p[0] = p.x; r->p[1] = p.y; r->p[2] = p.z; r->pw[0] = pw.x; r->pw[1] = pw.y; r->pw[2] = pw.z;
+    r->l[0] = l.x; r->l[1] = l.y; r->l[2] = l.z; r->n[0] = n.x; r->n[1] = n.y; r->n[2] = n.z;
+    double inv = 1 / (2 * m->r);
+    r->key = cellKey((long long)floor(p.x * inv), (long long)floor(p.y * inv), (long long)floor(p.z * inv));
+}
+/* r_k = r_1 sqrt(prod_{i=1}^{k-1} (i + alpha) / (i + 1)) (sppm.js:177-192), by the product itself */
+double or_sppm_radius(double r1, double alpha, uint32_t k) {
+    double s = 1;
+    for (uint32_t i = 1; i < (k > 1 ? k : 1); ++i) s *= (i + alpha) / (i + 1.0);
+    return r1 * sqrt(s);
+}
+static double faceFlux(const PtLightFace* f) {
+    return f->weight > 0 ? lumad(f3(f->le)) * PI_D * faceArea(f) * (f->twoSided ? 2 : 1) : 0;
+}
+static double sunFlux(const PtEnv* e) { return (e->sunE > 0 && e->sunRadius > 0) ? e->sunE * PI_D * (double)e->sunRadius * e->sunRadius : 0; }
+static int evaluable(const PtMaterial* m) { return !lobesNone(m); }
+
+/* One photon: drawn from the lights by flux (the sun's beam through its aperture, the light faces), marched and
+ * scattered through delta events only, stored where it lands on an evaluable surface after one at least. */
+static void tracePhoton(const OrScene* S, const uint32_t* seed, uint32_t i, uint32_t N, OrMap* out, OrStats* st) {
+    const PtUniforms* u = S->u;
+    const PtEnv* env = &u->env;
+    double total = sunFlux(env), fsun = total;
+    for (int k = 0; k < S->nfaces; ++k) total += faceFlux(&S->faces[k]);
+    if (!(total > 0)) return;
+    OrRng r = rngAt(seed, i, 0, 0x94070000u);
+    double u0[4], u1[4]; rng4(&r, u0); rng4(&r, u1);
+    double pick = u0[0] * total, each = total / N;
+    d3 o, d, beta;
+    if (pick < fsun) {
+        d3 a = f3(env->sunDir), t, b; basis(a, &t, &b);
+        double rr = env->sunRadius * sqrt(u0[1]), ph = 2 * PI_D * u0[2];
+        o = add(f3(env->sunPoint), add(scl(t, rr * cos(ph)), scl(b, rr * sin(ph))));
+        d = neg(sunDir(env, u1[0], u1[1]));
+        double w = each * maxd(-dotp(d, a), 0);   /* L cos(theta) Omega A with L = E / Omega */
+        beta = v(w, w, w);
+    } else {
+        pick -= fsun;
+        int k = 0;
+        for (; k + 1 < S->nfaces; ++k) { double ff = faceFlux(&S->faces[k]); if (pick < ff) break; pick -= ff; }
+        const PtLightFace* f = &S->faces[k];
+        d3 y = f3(f->centre);
+        if (f->shape == 1) { double rr = f->halfx * sqrt(u0[1]), ph = 2 * PI_D * u0[2]; y = add(y, scl(add(scl(f3(f->u), cos(ph)), scl(f3(f->v), sin(ph))), rr)); }
+        else y = add(y, add(scl(f3(f->u), (2 * u0[1] - 1) * f->halfx), scl(f3(f->v), (2 * u0[2] - 1) * f->halfy)));
+        d3 side = (f->twoSided && u1[2] < 0.5) ? neg(f3(f->n)) : f3(f->n);
+        d = cosDir(side, u1[0], u1[1]);
+        o = add(y, scl(side, 2 * offsetTol(S, y)));
+        d3 le = f3(f->le);
+        beta = scl(le, each / maxd(lumad(le), 1e-30));
+    }
+    d3 p = o, dir = d;
+    double n = 1, ltot = 0, lambda = 0;
+    uint32_t steps = 0, bounces = 0;
+    int hadDelta = 0;
+    double collide = hazeOn(S) ? hazeDraw(&u->haze, p, dir, u1[3]) : 1e300;
+    for (;;) {
+        int state = march(S, &p, dir, n, &ltot, &steps, collide);
+        st->photonSegs++;
+        if (state != ST_ACTIVE) return;       /* escaped, out of steps, or collided in the haze */
+        OrRng rng = rngAt(seed, i, bounces, (0x5AADEu ^ ((steps & 0xFFFFu) << 12)) ^ 0x80000000u);
+        OrHit h = sceneAt(S, p);
+        d3 ns = h.n;
+        if (dotp(ns, ns) < 0.25) ns = normalAt(S, p, marchTol(S, p));
+        int inside = h.d < 0;
+        d3 nrm = inside ? neg(ns) : ns;
+        if (dotp(dir, nrm) > 0) { double off = offsetTol(S, p); p = add(p, scl(dir, off)); ltot += off * n; continue; }
+        bounces++;
+        const PtMaterial* m = &S->mats[h.m < S->nmats ? h.m : 0];
+        if (max3d(f3(m->emissive)) > 0) return;
+        double n0 = n, n1 = maxd(m->ior, 1);
+        if (inside) {
+            n1 = 1;
+            if (m->transmission > 0) {
+                OrHit hb = sceneAt(S, add(p, scl(dir, 2 * offsetTol(S, p))));
+                if (hb.insideW > -1e6 && hb.a < S->nmats) n1 = maxd(S->mats[hb.a].ior, 1);   /* inside something (-1e7: nothing, air) */
+            }
+        }
+        double ud[4]; rng4(&rng, ud);
+        int draw = 0; double lambda0 = lambda; double sw[3] = {1, 1, 1};
+        if (m->abbe > 0 && m->transmission > 0 && !inside) {
+            if (lambda <= 0) { draw = 1; lambda = 0.38 + 0.40 * ud[3]; or_spectral_weight(lambda, sw); }
+            n1 = or_cauchy(n1, m->abbe, lambda);
+        }
+        OrLobes L = makeLobes(nrm, neg(dir), m, n0 / n1);
+        if (evaluable(m) && hadDelta) { mapPush(out, p, beta, neg(dir), nrm); st->stored++; }
+        double us[4]; rng4(&rng, us);
+        OrSample s = lobesSample(&L, us);
+        if (!s.delta) return;                 /* the map holds L S+ D paths: a photon goes on through delta lobes only */
+        dir = s.dir;
+        beta = mul(beta, s.w);
+        if (s.transmitted) { n = n1; if (draw) beta = mul(beta, v(sw[0], sw[1], sw[2])); }
+        else if (draw) lambda = lambda0;
+        if (!(max3d(s.w) > 0)) return;
+        hadDelta = 1;
+        if (bounces > u->maxbounce) return;
+        if (u->rr && bounces >= u->rrStart) {
+            double pr = clampd(max3d(beta), 0.05, 0.95);
+            if (ud[0] > pr) return; else beta = scl(beta, 1 / pr);
+        }
+        double off = offsetTol(S, p);
+        p = add(p, scl(dir, off)); ltot += off * n;
+        collide = (hazeOn(S) && n <= 1) ? ltot + hazeDraw(&u->haze, p, dir, ud[1]) * n : 1e300;
+    }
+}
+
+/* sum f(l_p) P_p / (pi r^2) over the map's photons within r of p whose normal agrees (dot >= 0.9) */
+static d3 mapGather(const OrMap* m, const OrLobes* L, d3 p, d3 nrm) {
+    double r = m->r, inv = 1 / (2 * r);
+    long long i0 = (long long)floor(p.x * inv - 0.5), j0 = (long long)floor(p.y * inv - 0.5), k0 = (long long)floor(p.z * inv - 0.5);
+    d3 sum = v(0, 0, 0);
+    for (int c = 0; c < 8; ++c) {
+        long long key = cellKey(i0 + (c & 1), j0 + ((c >> 1) & 1), k0 + ((c >> 2) & 1));
+        size_t lo = 0, hi = m->n;
+        while (lo < hi) { size_t mid = (lo + hi) / 2; if (m->rec[mid].key < key) lo = mid + 1; else hi = mid; }
+        for (size_t q = lo; q < m->n && m->rec[q].key == key; ++q) {
+            const OrPhoton* ph = &m->rec[q];
+            d3 dd = sub(v(ph->p[0], ph->p[1], ph->p[2]), p);
+            if (dotp(dd, dd) >= r * r || dotp(v(ph->n[0], ph->n[1], ph->n[2]), nrm) < 0.9) continue;
+            sum = add(sum, mul(lobesEval(L, v(ph->l[0], ph->l[1], ph->l[2])), v(ph->pw[0], ph->pw[1], ph->pw[2])));
+        }
+    }
+    return scl(sum, 1 / (PI_D * r * r));
+}
 
 /* One camera path to its end (core/wavefront.js:1244-1442 shade; core/tracebasics.js:834-1108 path phases;
  * core/materials_pbr.js:1014-1070 emitterHit, :1110-1454 processMaterial_advanced). Adds to rgb. */
-static void tracePath(const OrScene* S, const OrTiles* T, const uint32_t* seed, uint32_t ray, double* frame, OrStats* st) {
+static void tracePath(const OrScene* S, const OrTiles* T, const uint32_t* seed, uint32_t ray, const OrMap* map,
+                      double* frame, OrStats* st) {
     const PtUniforms* u = S->u;
+    /* SPPM bookkeeping: deltaOnly (every event so far delta), caustic (a gather was left by a non-delta lobe),
+     * sdelta (every event since it delta): light met with caustic && sdelta is the map's */
+    int deltaOnly = 1, caustic = 0, sdelta = 0;
     const PtEnv* env = &u->env;
     d3 p, dir; uint32_t pix; double uh;
     cameraRay(S, T, seed, ray, &p, &dir, &pix, &uh);
@@ -639,7 +787,8 @@ static void tracePath(const OrScene* S, const OrTiles* T, const uint32_t* seed, 
         st->steps += steps - s0;
         st->segments++;
         if (state == ST_ESCAPED) {
-            rad = add(rad, scl(mul(beta, add(envBg(env, dir), envSun(env, p, dir))), sunmis));
+            int mapped = caustic && sdelta && sunFlux(env) > 0;
+            rad = add(rad, scl(mul(beta, add(envBg(env, dir), mapped ? v(0, 0, 0) : envSun(env, p, dir))), sunmis));
             direct = bounces == 0;
             break;
         }
@@ -691,6 +840,7 @@ static void tracePath(const OrScene* S, const OrTiles* T, const uint32_t* seed, 
             prevPdf = u->nee ? ph : -1;
             vtxLtot = ltot;
             dir = nd;
+            deltaOnly = caustic = sdelta = 0;
             if (bounces > u->maxbounce) break;
             collide = ltot + hazeDraw(hz, p, dir, u1[2]) * n;
             continue;
@@ -704,6 +854,7 @@ static void tracePath(const OrScene* S, const OrTiles* T, const uint32_t* seed, 
         bounces++;
         const PtMaterial* m = &S->mats[h.m < S->nmats ? h.m : 0];
         d3 Le = f3(m->emissive);
+        if (max3d(Le) > 0 && caustic && sdelta && m->face >= 0 && faceFlux(&S->faces[m->face]) > 0) break;
         if (max3d(Le) > 0) {
             double w = 1;
             if (m->face >= 0) {
@@ -724,7 +875,7 @@ static void tracePath(const OrScene* S, const OrTiles* T, const uint32_t* seed, 
             n1 = 1;
             if (m->transmission > 0) {
                 OrHit hb = sceneAt(S, add(p, scl(dir, 2 * offsetTol(S, p))));
-                if (hb.insideW < 0 && hb.a < S->nmats) n1 = maxd(S->mats[hb.a].ior, 1);
+                if (hb.insideW > -1e6 && hb.a < S->nmats) n1 = maxd(S->mats[hb.a].ior, 1);   /* inside something (-1e7: nothing, air) */
             }
         }
         double ud[4]; rng4(&rng, ud);
@@ -735,6 +886,12 @@ static void tracePath(const OrScene* S, const OrTiles* T, const uint32_t* seed, 
         }
         OrLobes L = makeLobes(nrm, neg(dir), m, n0 / n1);
         double fs = faceShare(S, bounces);
+        int gathered = 0;
+        if (map && deltaOnly && evaluable(m)) {
+            rad = add(rad, mul(beta, mapGather(map, &L, p, nrm)));
+            gathered = 1;
+            st->gathers++;
+        }
         if (u->nee && !lobesNone(m)) {
             double u0[4], u1[4]; rng4(&rng, u0); rng4(&rng, u1);
             d3 Ld = v(0, 0, 0), le = v(0, 0, 0); double pS = 0, tmax = 1e300;
@@ -793,6 +950,9 @@ static void tracePath(const OrScene* S, const OrTiles* T, const uint32_t* seed, 
         } else { sunmis = 1; prevPdf = -1; }
         vtxLtot = ltot;
         if (s.eval) steps += 0x10000u;
+        if (!s.delta) deltaOnly = 0;
+        if (gathered) { if (!s.delta) { caustic = 1; sdelta = 0; } }
+        else if (caustic) { if (s.delta) sdelta = 1; else caustic = sdelta = 0; }
         if (bounces > u->maxbounce) terminate = 1;
         if (!terminate && u->diffuseCap && (steps >> 16) > u->diffuseCap) terminate = 1;
         if (!terminate && u->rr && bounces >= u->rrStart) {
@@ -830,7 +990,8 @@ static void sceneInit(OrScene* S, const int* ops, int nops, const float* U, cons
 
 /* Renders `samples` samples from sample index `first`, FP64 accumulation as the engine's resolve
  * (core/wavefront.js:414-494, core/measure.js:36-57); accum [w*h*4] mean rgb + count, moments [w*h*2].
- * ops: 8 ints per program entry (op, shape, xf, p, mat, prim, join, k). stats: 6 counters. */
+ * With u->sppm each sample first runs its photon pass (u->sppmPhotons photons, radius r_k for sample k = first + s + 1).
+ * ops: 8 ints per program entry (op, shape, xf, p, mat, prim, join, k). stats: 9 counters. */
 int or_render(const int* ops, int nops, const float* U, const PtMaterial* mats, int nmats, const PtLightFace* faces,
               int nfaces, const PtUniforms* u, uint32_t first, uint32_t samples, double* accum, double* moments,
               long long* stats, int threads) {
@@ -846,17 +1007,41 @@ int or_render(const int* ops, int nops, const float* U, const PtMaterial* mats, 
 #else
     (void)threads;
 #endif
+    OrMap map; memset(&map, 0, sizeof map);
     for (uint32_t s = 0; s < samples; ++s) {
         uint32_t seed[4]; sampleSeed(u->seed, first + s, seed);
         memset(frame, 0, sizeof(double) * npx * 4);
+        if (u->sppm) {
+            /* the sample's photon pass: per-thread maps merged, then sorted by cell */
+            map.n = 0;
+            map.r = or_sppm_radius(u->sppmRadius, u->sppmAlpha, first + s + 1);
+            uint32_t NP = u->sppmPhotons;
+            #pragma omp parallel
+            {
+                OrStats st; memset(&st, 0, sizeof st);
+                OrMap mine; memset(&mine, 0, sizeof mine); mine.r = map.r;
+                #pragma omp for schedule(dynamic, 256)
+                for (uint32_t i = 0; i < NP; ++i) tracePhoton(&S, seed, i, NP, &mine, &st);
+                #pragma omp critical
+                {
+                    for (size_t q = 0; q < mine.n; ++q) {
+                        if (map.n == map.cap) { map.cap = map.cap ? 2 * map.cap : 4096; map.rec = (OrPhoton*)realloc(map.rec, map.cap * sizeof(OrPhoton)); }
+                        map.rec[map.n++] = mine.rec[q];
+                    }
+                    tot.photonSegs += st.photonSegs; tot.stored += st.stored;
+                }
+                free(mine.rec);
+            }
+            qsort(map.rec, map.n, sizeof(OrPhoton), photonCmp);
+        }
         #pragma omp parallel
         {
             OrStats st; memset(&st, 0, sizeof st);
             #pragma omp for schedule(dynamic, 64)
-            for (uint32_t i = 0; i < N; ++i) tracePath(&S, &T, seed, i, frame, &st);
+            for (uint32_t i = 0; i < N; ++i) tracePath(&S, &T, seed, i, u->sppm ? &map : NULL, frame, &st);
             #pragma omp critical
             { tot.segments += st.segments; tot.shadow += st.shadow; tot.steps += st.steps; tot.shadowSteps += st.shadowSteps;
-              tot.maxiter += st.maxiter; tot.nonfinite += st.nonfinite; }
+              tot.maxiter += st.maxiter; tot.nonfinite += st.nonfinite; tot.gathers += st.gathers; }
         }
         for (uint32_t i = 0; i < npx; ++i) {
             double cnt = frame[4 * i + 3];
@@ -873,8 +1058,9 @@ int or_render(const int* ops, int nops, const float* U, const PtMaterial* mats, 
             mo[1] = (1 - w) * (1 - w) * mo[1] + w * w;
         }
     }
-    if (stats) { stats[0] = tot.segments; stats[1] = tot.shadow; stats[2] = tot.steps; stats[3] = tot.shadowSteps; stats[4] = tot.maxiter; stats[5] = tot.nonfinite; }
-    free(frame); free(buf);
+    if (stats) { stats[0] = tot.segments; stats[1] = tot.shadow; stats[2] = tot.steps; stats[3] = tot.shadowSteps; stats[4] = tot.maxiter; stats[5] = tot.nonfinite;
+                 stats[6] = tot.photonSegs; stats[7] = tot.stored; stats[8] = tot.gathers; }
+    free(frame); free(buf); free(map.rec);
     return 0;
 }
 

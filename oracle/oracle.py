@@ -50,6 +50,8 @@ def lib():
         L.or_hg_sample.argtypes = [c_d, P(c_d), c_d, c_d, P(c_d)]
         L.or_pcg4d.argtypes = [P(c_u), P(c_u)]
         L.or_tiles.argtypes = [c_u, c_u, c_u, P(c_u), P(c_u)]
+        L.or_sppm_radius.restype = c_d
+        L.or_sppm_radius.argtypes = [c_d, c_d, c_u]
         _lib = L
     return _lib
 
@@ -107,11 +109,16 @@ def render(scene, u, samples, first=0, threads=0):
     n = u.width * u.height
     acc = np.zeros((n, 4), np.float64)
     mom = np.zeros((n, 2), np.float64)
-    st = (ctypes.c_longlong * 6)()
+    st = (ctypes.c_longlong * 9)()
     lib().or_render(*S.args(), ctypes.byref(S.mats), S.nmats, ctypes.byref(S.faces), S.nfaces, ctypes.byref(u),
                     first, samples, _dp(acc), _dp(mom), st, threads)
-    names = ['segments', 'shadow_segments', 'steps', 'shadow_steps', 'maxiter', 'nonfinite']
+    names = ['segments', 'shadow_segments', 'steps', 'shadow_steps', 'maxiter', 'nonfinite', 'photon_segments', 'photons', 'gathers']
     return acc.reshape(u.height, u.width, 4), mom.reshape(u.height, u.width, 2), dict(zip(names, list(st)))
+
+
+def sppm_radius(r1, alpha, k):
+    """r_k of the progressive schedule, by its product."""
+    return lib().or_sppm_radius(r1, alpha, k)
 
 
 def pick(scene, u, sample=0, first=0, count=None):

@@ -45,10 +45,13 @@ struct PtDev {
     const float* mf;   // GGX albedo tables
     unsigned int* seedCur;  // 4 words: this sample's seed
     // SPPM photon map
-    unsigned int* sppmHeads;
-    float* sppmRecs;
-    unsigned int sppmCap, sppmHashSize;
-    float* sppmAcc;    // per-pixel gathered flux (rgb) and count for the current pass
+    unsigned int* sppmHeads;        // 3 x PT_SPPM_HEADS: the cells' list heads (record index + 1; 0 empty), the cells'
+                                    // record counts, the sub-lists' heads
+    float* sppmRecs;                // 8 words a photon: position, power rgb, direction to the light (oct 2 x 16),
+                                    // normal (oct 2 x 13) << 5 | sub-list
+    unsigned int* sppmNext;         // 2 words a photon: its successor in its cell's list and in its sub-list
+    unsigned int sppmCap;           // records the map holds
+    unsigned int nphotons;          // photons emitted in a pass (slots [0, nphotons) of the path state)
     // guiding
     const void* guideNet;
     float* guideBuf;

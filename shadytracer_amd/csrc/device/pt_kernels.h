@@ -103,6 +103,11 @@ __device__ __forceinline__ float maskMargin(const PtDev* __restrict__ D) {
 // meta.z: bounces (8 bits) | state << 8 | flags << 16
 #define PT_F_DELTAONLY 0x1u     // every scatter so far was a delta event (SPPM gathers at the next diffuse hit)
 #define PT_F_NEEPREV 0x2u       // the previous vertex ran light sampling (prevPdf meaningful)
+#define PT_F_PHOTON 0x4u        // the slot carries a photon of an SPPM pass, not a camera path
+#define PT_F_HADDELTA 0x8u      // a photon that has passed at least one delta event (a caustic photon)
+#define PT_F_CAUSTIC 0x10u      // a camera path that gathered: light reaching its gather vertex through delta
+                                // events only is the photon map's
+#define PT_F_SDELTA 0x20u       // ... and it has scattered by a delta event since that vertex
 __device__ __forceinline__ unsigned int mzBounces(unsigned int z) { return z & 0xFFu; }
 __device__ __forceinline__ unsigned int mzState(unsigned int z) { return (z >> 8) & 0xFFu; }
 __device__ __forceinline__ unsigned int mzFlags(unsigned int z) { return z >> 16; }
@@ -116,14 +121,17 @@ __device__ __forceinline__ V3 envBackground(const PtEnv& e, V3 d) {
     if (e.type == 1u || (e.type == 2u && d.z > 0.0f)) return ld3(e.col) * e.L;
     return v3s(0.0f);
 }
-// The sun's radiance where it is seen through its aperture (a beam of radius sunRadius about sunPoint).
+// The sun's radiance along the ray from p towards d. With an aperture the sun is a beam: the lines that cross the disc
+// of radius sunRadius about sunPoint, perpendicular to the sun's direction -- the disc the photon pass emits the beam
+// from, so both see one light field, and any point of a ray (its vertex, or where it left the scene) gives the same
+// answer. The disc is placed above whatever the beam lights: nothing above it is lit by photons.
 __device__ __forceinline__ V3 envSun(const PtEnv& e, V3 p, V3 d) {
     if (!inSun(e, d)) return v3s(0.0f);
     if (e.sunRadius > 0.0f) {
-        V3 a = ld3(e.sunDir);
-        V3 q = p - ld3(e.sunPoint);
-        V3 perp = q - a * dot(q, a);
-        if (dot(perp, perp) > e.sunRadius * e.sunRadius) return v3s(0.0f);
+        V3 a = ld3(e.sunDir), c = ld3(e.sunPoint);
+        float t = dot(c - p, a) / dot(d, a);
+        V3 q = p + d * t - c;
+        if (!(dot(q, q) <= e.sunRadius * e.sunRadius)) return v3s(0.0f);
     }
     return v3s(e.sunE / sunSolidAngle(e));
 }
@@ -155,6 +163,14 @@ __device__ __forceinline__ float misWeight(float a, float b) { a *= a; b *= b; r
 // light faces (NEE towards emissive rectangles and discs): area sampling, density in solid angle
 __device__ __forceinline__ float faceArea(const PtLightFace& f) {
     return f.shape == 1u ? PT_PI * f.halfx * f.halfx : 4.0f * f.halfx * f.halfy;
+}
+// Emitted flux (luminance-weighted, lm) of a light face (Lambertian: pi L A a side) and of the sun's beam through
+// its aperture (E pi R^2; none without an aperture): the photon pass draws its lights in proportion to these.
+__device__ __forceinline__ float faceFlux(const PtLightFace& f) {
+    return f.weight > 0.0f ? luma(ld3(f.le)) * PT_PI * faceArea(f) * (f.twoSided ? 2.0f : 1.0f) : 0.0f;
+}
+__device__ __forceinline__ float sunFlux(const PtEnv& e) {
+    return (e.sunE > 0.0f && e.sunRadius > 0.0f) ? e.sunE * PT_PI * e.sunRadius * e.sunRadius : 0.0f;
 }
 __device__ __forceinline__ float faceShareAt(const PtDev* __restrict__ D, unsigned int bounces) {
     if (bounces == 0u || D->nfaces == 0u || D->faceTotal <= 0.0f) return 0.0f;
@@ -281,3 +297,47 @@ __device__ __forceinline__ V3 hgSample(float g, V3 w, float u1, float u2) {
     float sp, cp; __sincosf(2.0f * PT_PI * u2, &sp, &cp);
     return normalize(t * (sn * cp) + b * (sn * sp) + w * c);
 }
+
+// ---------------------------------------------------------------------------------------------------
+// SPPM (core/sppm.js:79-192, :350-520): photons stored at non-delta landings in a hash grid of PT_SPPM_HEADS
+// linked lists (cell 2r, Teschner et al. 2003's spatial hash), each record also on one of PT_SPPM_SUBS sub-lists of
+// its cell; camera paths after delta-only chains gather sum f(l_p) P_p / (pi r^2) over photons within r whose
+// surface normal agrees (dot >= 0.9).
+// Octahedral unit vectors (Meyer et al. 2010) in two signed fields of `bits` bits each.
+__device__ __forceinline__ unsigned int octEncode(V3 v, unsigned int bits = 16u) {
+    float s = fabsf(v.x) + fabsf(v.y) + fabsf(v.z);
+    float ex = v.x / s, ey = v.y / s;
+    if (v.z < 0.0f) {
+        float ox = (1.0f - fabsf(ey)) * (ex >= 0.0f ? 1.0f : -1.0f);
+        float oy = (1.0f - fabsf(ex)) * (ey >= 0.0f ? 1.0f : -1.0f);
+        ex = ox; ey = oy;
+    }
+    const float q = (float)((1u << (bits - 1u)) - 1u);
+    const unsigned int mask = (1u << bits) - 1u;
+    int ix = (int)rintf(fminf(fmaxf(ex, -1.0f), 1.0f) * q), iy = (int)rintf(fminf(fmaxf(ey, -1.0f), 1.0f) * q);
+    return ((unsigned int)ix & mask) | (((unsigned int)iy & mask) << bits);
+}
+__device__ __forceinline__ V3 octDecode(unsigned int w, unsigned int bits = 16u) {
+    const float q = (float)((1u << (bits - 1u)) - 1u);
+    const unsigned int mask = (1u << bits) - 1u, sh = 32u - bits;
+    int ix = (int)((w & mask) << sh) >> sh, iy = (int)(((w >> bits) & mask) << sh) >> sh;
+    float ex = (float)ix / q, ey = (float)iy / q;
+    V3 v = v3(ex, ey, 1.0f - fabsf(ex) - fabsf(ey));
+    if (v.z < 0.0f) {
+        v.x = (1.0f - fabsf(ey)) * (ex >= 0.0f ? 1.0f : -1.0f);
+        v.y = (1.0f - fabsf(ex)) * (ey >= 0.0f ? 1.0f : -1.0f);
+    }
+    return normalize(v);
+}
+__device__ __forceinline__ unsigned int sppmHashRaw(int i, int j, int k) {
+    return ((unsigned int)i * 73856093u) ^ ((unsigned int)j * 19349663u) ^ ((unsigned int)k * 83492791u);
+}
+__device__ __forceinline__ unsigned int sppmHash(int i, int j, int k) { return sppmHashRaw(i, j, k) & (PT_SPPM_HEADS - 1u); }
+// a cell's sub-list s: the cell's hash mixed with s (a golden-ratio multiple), in a table of its own
+__device__ __forceinline__ unsigned int sppmHashSub(int i, int j, int k, unsigned int s) {
+    return (sppmHashRaw(i, j, k) ^ ((s + 1u) * 0x9E3779B1u)) & (PT_SPPM_HEADS - 1u);
+}
+#define PT_SPPM_SUBS 32u       // sub-lists a cell's records are spread over
+#define PT_SPPM_BUDGET 512u    // records a gather walks in the ordinary case, shared by its eight cells
+#define PT_SPPM_HARD 8u        // ...and the bound on its records visited, in budgets
+#define PT_SPPM_NORMAL_DOT 0.9f

@@ -230,11 +230,64 @@ __device__ __forceinline__ V3 normalAt(ParamPtr U, V3 p, float h) {
     return l > 0.0f ? g * (1.0f / l) : v3(0.0f, 0.0f, 1.0f);
 }
 
-struct ShadeOut { bool requeue, shadow; };
+struct ShadeOut {
+    bool requeue, shadow, store, photon;
+    V3 sp, spow, sl, sn;     // a photon to store: position, power, direction to its light, surface normal
+    unsigned int ssub;       // ...and its sub-list
+};
+
+// The photon map's estimate at p for the lobes lb: sum over stored photons within r of f(l_p) P_p / (pi r^2) over the
+// 2 x 2 x 2 cells of side 2r around p (core/sppm.js:487-520 sppmGather). A record counts only when walked from its
+// own cell, so lists shared by hash collisions never count one twice. A cell holding more records than its share of
+// the gather's budget is subsampled, not cut: m of its PT_SPPM_SUBS sub-lists walked (m from the count alone), each
+// record found weighing PT_SPPM_SUBS / m -- every record is walked with probability m / PT_SPPM_SUBS, so the sum is
+// exact in expectation. A hard bound on the records visited ends a gather early; such a gather is counted.
+__device__ __forceinline__ V3 sppmGather(const PtDev* __restrict__ D, const Lobes& lb, V3 p, V3 nrm, bool& cut) {
+    const float r = __uint_as_float(D->counters[PT_C_RADIUS]), inv = 1.0f / (2.0f * r);
+    const float r2 = r * r;
+    const unsigned int* heads = D->sppmHeads;
+    const unsigned int* counts = heads + PT_SPPM_HEADS;
+    const unsigned int* subs = heads + 2u * PT_SPPM_HEADS;
+    const uint2* links = reinterpret_cast<const uint2*>(D->sppmNext);
+    const float4* recs = reinterpret_cast<const float4*>(D->sppmRecs);
+    int i0 = (int)floorf(p.x * inv - 0.5f), j0 = (int)floorf(p.y * inv - 0.5f), k0 = (int)floorf(p.z * inv - 0.5f);
+    const unsigned int share = PT_SPPM_BUDGET / 8u;
+    unsigned int visits = 0u;
+    V3 sum = v3s(0.0f);
+    for (int c = 0; c < 8 && !cut; ++c) {
+        int ci = i0 + (c & 1), cj = j0 + ((c >> 1) & 1), ck = k0 + ((c >> 2) & 1);
+        unsigned int h = sppmHash(ci, cj, ck);
+        unsigned int cnt = counts[h];
+        if (cnt == 0u) continue;
+        unsigned int m = cnt <= share ? PT_SPPM_SUBS : max(1u, (PT_SPPM_SUBS * share) / cnt);
+        float wgt = (float)PT_SPPM_SUBS / (float)m;
+        for (unsigned int sl = 0; sl < (m == PT_SPPM_SUBS ? 1u : m) && !cut; ++sl) {
+            bool whole = m == PT_SPPM_SUBS;
+            unsigned int idx = whole ? heads[h] : subs[sppmHashSub(ci, cj, ck, sl)];
+            while (idx != 0u) {
+                if (++visits > PT_SPPM_BUDGET * PT_SPPM_HARD) { cut = true; break; }
+                const float4 a = recs[2ull * (idx - 1u)], b = recs[2ull * (idx - 1u) + 1ull];
+                uint2 ln = links[idx - 1u];
+                unsigned int tag = __float_as_uint(b.w) & 31u;
+                if (whole || tag == sl) {
+                    V3 q = v3(a.x, a.y, a.z);
+                    V3 d = q - p;
+                    bool own = (int)floorf(q.x * inv) == ci && (int)floorf(q.y * inv) == cj && (int)floorf(q.z * inv) == ck;
+                    if (own && dot(d, d) < r2 && dot(octDecode(__float_as_uint(b.w) >> 5, 13u), nrm) >= PT_SPPM_NORMAL_DOT) {
+                        V3 f = lobesEval(lb, D->mf, octDecode(__float_as_uint(b.z)));
+                        sum += f * v3(a.w, b.x, b.y) * wgt;
+                    }
+                }
+                idx = whole ? ln.x : ln.y;
+            }
+        }
+    }
+    return sum * (1.0f / (PT_PI * r2));
+}
 
 __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const PtMaterial* __restrict__ smat,
                                              const unsigned int* seed, unsigned int slot) {
-    ShadeOut out; out.requeue = false; out.shadow = false;
+    ShadeOut out; out.requeue = false; out.shadow = false; out.store = false;
     float4 P = ld4(D->pos, slot), Dd = ld4(D->dir, slot), B = ld4(D->beta, slot), R = ld4(D->rad, slot), A = ld4(D->aux, slot);
     uint4 M = ld4u(D->meta, slot);
     V3 p = v3(P.x, P.y, P.z), dir = v3(Dd.x, Dd.y, Dd.z), beta = v3(B.x, B.y, B.z), rad = v3(R.x, R.y, R.z);
@@ -242,16 +295,25 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
     unsigned int pix = M.x, counts = M.y, z = M.z, tag = M.w;
     unsigned int bounces = mzBounces(z), state = mzState(z), flags = mzFlags(z);
     const PtEnv& env = D->u.env;
-    Rng rng = rngAt(seed, slot, bounces, 0x5AADEu ^ ((counts & 0xFFFFu) << 12));
+    const bool photon = (flags & PT_F_PHOTON) != 0u;
+    out.photon = photon;
+    // photons draw from their own stream, so the map and the camera paths of one sample are independent
+    Rng rng = rngAt(seed, slot, bounces, (0x5AADEu ^ ((counts & 0xFFFFu) << 12)) ^ (photon ? 0x80000000u : 0u));
     bool terminate = false;
     bool direct = false;
+    // a gathering path's light that reached its gather vertex through delta events only is the photon map's
+    const bool mapped = (flags & (PT_F_CAUSTIC | PT_F_SDELTA)) == (PT_F_CAUSTIC | PT_F_SDELTA);
 
     if (state == PT_ESCAPED) {
-        V3 Le = envBackground(env, dir) + envSun(env, p, dir);
-        rad += beta * Le * sunmis;
+        if (!photon) {
+            // the sun through its aperture emits photons, so a mapped path leaves its light to the map; the sky
+            // emits none and stays path traced
+            V3 Le = envBackground(env, dir) + ((mapped && sunFlux(env) > 0.0f) ? v3s(0.0f) : envSun(env, p, dir));
+            rad += beta * Le * sunmis;
+        }
         direct = bounces == 0u;
         terminate = true;
-    } else if (state == PT_MAXITER) {
+    } else if (state == PT_MAXITER || (state == PT_SCATTER && photon)) {
         terminate = true;
     } else if (state == PT_SCATTER) {
         // A POINT IN THE HAZE (core/haze.js:292-600 mediumVertex): albedo, light sampling with the phase function,
@@ -304,7 +366,7 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
         prevPdf = D->u.nee != 0u ? ph : -1.0f;
         vtxLtot = ltot;
         dir = nd;
-        flags &= ~PT_F_DELTAONLY;
+        flags &= ~(PT_F_DELTAONLY | PT_F_CAUSTIC | PT_F_SDELTA);
         if (bounces > D->u.maxbounce) terminate = true;
         else {
             collide = ltot + hazeDraw(hz, p, dir, u1.z) * n;
@@ -328,7 +390,9 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
             bounces += 1u;
             const PtMaterial& m = smat[h.m < D->nmats ? h.m : 0u];
             V3 Le = ld3(m.emissive);
-            if (max3(Le) > 0.0f) {
+            if (max3(Le) > 0.0f && (photon || (mapped && m.face >= 0 && faceFlux(D->faces[m.face]) > 0.0f))) {
+                terminate = true;      // a photon ends at a light; a mapped path's light is the map's (its photons')
+            } else if (max3(Le) > 0.0f) {
                 // AN EMITTER HIT: weighted against the light sampling the previous vertex ran
                 float w = 1.0f;
                 if (m.face >= 0) {
@@ -352,7 +416,7 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
                     n1 = 1.0f;
                     if (m.transmission > 0.0f) {
                         SdfHit hb = sceneFull(U, p + dir * (2.0f * offsetTol(D, p)));
-                        if (hb.insideW < 0.0f && hb.a < D->nmats) n1 = fmaxf(smat[hb.a].ior, 1.0f);
+                        if (hb.insideW > -1.0e6f && hb.a < D->nmats) n1 = fmaxf(smat[hb.a].ior, 1.0f);   // inside something (-1e7: nothing, air)
                     }
                 }
                 // DISPERSION (core/spectral.js:121-135): a wavelength drawn at the first refraction into a
@@ -371,7 +435,25 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
                 }
                 Lobes lb = makeLobes(nrm, -dir, m, n0 / n1, D->mf);
                 float fs = faceShareAt(D, bounces);
-                bool neeOn = D->u.nee != 0u;
+                bool neeOn = D->u.nee != 0u && !photon;
+                bool gathered = false;
+                // a surface with a lobe the map can be gathered through (any but a purely delta one)
+                const bool evaluable = !lobesNone(m);
+                if (photon && evaluable && (flags & PT_F_HADDELTA)) {
+                    // a caustic photon lands where the material has a non-delta lobe: stored (the store itself is
+                    // wave-aggregated by the caller), on one of the cell's sub-lists drawn uniformly; it goes on only
+                    // through a delta lobe (below)
+                    out.store = true;
+                    out.sp = p; out.spow = beta; out.sl = -dir; out.sn = nrm;
+                    out.ssub = min((unsigned int)(ud.z * (float)PT_SPPM_SUBS), PT_SPPM_SUBS - 1u);
+                } else if (!photon && D->u.sppm != 0u && (flags & PT_F_DELTAONLY) && evaluable) {
+                    // THE GATHER (core/sppm.js:487-494): the camera path's first non-delta vertex after delta events
+                    bool cut = false;
+                    rad += beta * sppmGather(D, lb, p, nrm, cut);
+                    gathered = true;
+                    atomicAdd(&D->stats[PT_S_GATHERS], 1ull);
+                    if (cut) atomicAdd(&D->stats[PT_S_GATHERCUT], 1ull);
+                }
                 if (neeOn && !lobesNone(m)) {
                     V4 u0 = rngNext(rng), u1 = rngNext(rng);
                     V3 L = v3s(0.0f), le = v3s(0.0f);
@@ -417,6 +499,7 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
                     }
                 }
                 LobeSample s = lobesSample(lb, D->mf, rngNext(rng));
+                if (photon && !s.isDelta) terminate = true;     // the map holds caustics: L S+ D only
                 dir = s.dir;
                 beta *= s.w;
                 if (s.transmitted) {
@@ -440,6 +523,13 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
                 vtxLtot = ltot;
                 if (s.ev == PT_EV_EVAL) counts += 0x10000u;
                 if (!s.isDelta) flags &= ~PT_F_DELTAONLY;
+                if (photon && s.isDelta) flags |= PT_F_HADDELTA;
+                // after a gather left by a non-delta lobe, light reaching the path through delta events only is
+                // the map's (it gathered it); left by a delta lobe, the path goes on as before the gather
+                if (gathered) { if (!s.isDelta) { flags |= PT_F_CAUSTIC; flags &= ~PT_F_SDELTA; } }
+                else if (flags & PT_F_CAUSTIC) {
+                    if (s.isDelta) flags |= PT_F_SDELTA; else flags &= ~(PT_F_CAUSTIC | PT_F_SDELTA);
+                }
                 if (bounces > D->u.maxbounce) terminate = true;
                 if (!terminate && D->u.diffuseCap != 0u && (counts >> 16) > D->u.diffuseCap) terminate = true;
                 if (!terminate && D->u.rr != 0u && bounces >= D->u.rrStart) {
@@ -458,9 +548,11 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
     }
     if (terminate) {
         out.requeue = false;
-        if (badDeposit(rad)) atomicAdd(&D->stats[PT_S_NONFINITE], 1ull);
-        else depositRGB(D, pix, rad, direct);
-        atomicAdd(reinterpret_cast<unsigned int*>(D->frame) + 4ull * pix + 3ull, 1u);
+        if (!photon) {
+            if (badDeposit(rad)) atomicAdd(&D->stats[PT_S_NONFINITE], 1ull);
+            else depositRGB(D, pix, rad, direct);
+            atomicAdd(reinterpret_cast<unsigned int*>(D->frame) + 4ull * pix + 3ull, 1u);
+        }
     } else if (out.requeue) {
         st4(D->pos, slot, p.x, p.y, p.z, ltot);
         st4(D->dir, slot, dir.x, dir.y, dir.z, n);
@@ -488,22 +580,46 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_shade(const PtDev* __r
     sampleSeed(D, seed);
     unsigned int* qout = qsel ? D->qMarch1 : D->qMarch0;
     unsigned int* cout = &C[qsel ? PT_C_MARCH1 : PT_C_MARCH0];
-    unsigned long long segs = 0ull;
+    unsigned long long segs = 0ull, psegs = 0ull, stored = 0ull, dropped = 0ull;
     for (unsigned int base = blockIdx.x * PT_BLOCK; base < count; base += gridDim.x * PT_BLOCK) {
         unsigned int i = base + threadIdx.x;
-        ShadeOut o; o.requeue = false; o.shadow = false;
+        ShadeOut o; o.requeue = false; o.shadow = false; o.store = false; o.photon = false;
         unsigned int slot = 0u;
         if (i < count) {
             slot = D->qSorted[i];
             o = shadeOne(D, smat, seed, slot);
-            segs += 1ull;
+            if (o.photon) psegs += 1ull; else segs += 1ull;
         }
         unsigned int qi = waveAppend(cout, o.requeue);
         if (o.requeue) qout[qi] = slot;
         unsigned int si = waveAppend(&C[PT_C_SHADOW], o.shadow);
         if (o.shadow) D->qShadow[si] = slot;
+        // a photon into the map: its record, then the heads of its cell's list and of its sub-list swapped for it
+        // and the cell's count raised
+        unsigned int ri = waveAppend(&C[PT_C_PHOTON], o.store);
+        if (o.store && ri < D->sppmCap) {
+            float4* rec = reinterpret_cast<float4*>(D->sppmRecs) + 2ull * ri;
+            rec[0] = make_float4(o.sp.x, o.sp.y, o.sp.z, o.spow.x);
+            rec[1] = make_float4(o.spow.y, o.spow.z, __uint_as_float(octEncode(o.sl)),
+                                 __uint_as_float((octEncode(o.sn, 13u) << 5) | o.ssub));
+            const float inv = 1.0f / (2.0f * __uint_as_float(C[PT_C_RADIUS]));
+            int ci = (int)floorf(o.sp.x * inv), cj = (int)floorf(o.sp.y * inv), ck = (int)floorf(o.sp.z * inv);
+            unsigned int h = sppmHash(ci, cj, ck);
+            unsigned int* heads = D->sppmHeads;
+            uint2 ln;
+            ln.x = atomicExch(&heads[h], ri + 1u);
+            ln.y = atomicExch(&heads[2u * PT_SPPM_HEADS + sppmHashSub(ci, cj, ck, o.ssub)], ri + 1u);
+            atomicAdd(&heads[PT_SPPM_HEADS + h], 1u);
+            reinterpret_cast<uint2*>(D->sppmNext)[ri] = ln;
+            stored += 1ull;
+        } else if (o.store) {
+            dropped += 1ull;
+        }
     }
     statAdd(D, PT_S_SEGMENTS, segs);
+    statAdd(D, PT_S_PHOTONSEGS, psegs);
+    statAdd(D, PT_S_PHOTONS, stored);
+    statAdd(D, PT_S_OVERFLOW, dropped);
 }
 
 // ---------------------------------------------------------------------------------------------------
@@ -673,4 +789,82 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_pick(const PtDev* __re
     st4(pos4, i, P.x, P.y, P.z, P.w);
     info[2u * i] = state | (mat << 8);
     info[2u * i + 1u] = prim;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// THE PHOTON PASS (core/wavefront.js:542-626 genphotons, core/photons.js:31-76; sppm.js plan): the map cleared, then
+// nphotons photons from the lights by their flux -- the sun's beam (flux E pi R^2, white; from the aperture's disc,
+// directions over the sun's cone) and each light face (Lambertian: pi L A a side) -- each carrying the total flux
+// over nphotons, then marched and shaded as any path (shade stores the caustic ones).
+// r_k for the k-th sample since the clear (k = samples resolved + 1), in closed form:
+// prod_{i=1}^{k-1} (i + a) / (i + 1) = Gamma(k + a) / (Gamma(1 + a) Gamma(k + 1)).
+__device__ __forceinline__ float sppmRadiusAt(float r1, float alpha, unsigned int k) {
+    double a = (double)alpha, kk = (double)k;
+    double lg = lgamma(kk + a) - lgamma(1.0 + a) - lgamma(kk + 1.0);
+    return (float)((double)r1 * sqrt(exp(lg)));
+}
+extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_sppm_clear(const PtDev* __restrict__ D) {
+    unsigned int i = blockIdx.x * PT_BLOCK + threadIdx.x;
+    if (i == 0) {
+        D->counters[PT_C_PHOTON] = 0u;
+        float r = sppmRadiusAt(D->u.sppmRadius, D->u.sppmAlpha, D->counters[PT_C_SAMPLE] + 1u);
+        D->counters[PT_C_RADIUS] = __float_as_uint(r);
+        D->stats[PT_S_RADIUS] = (unsigned long long)__float_as_uint(r);   // reported, not accumulated
+    }
+    if (i < 3u * PT_SPPM_HEADS) D->sppmHeads[i] = 0u;
+}
+extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_photon_gen(const PtDev* __restrict__ D) {
+    unsigned int i = blockIdx.x * PT_BLOCK + threadIdx.x;
+    unsigned int* C = D->counters;
+    const PtEnv& env = D->u.env;
+    float total = sunFlux(env);
+    for (unsigned int k = 0; k < D->nfaces; ++k) total += faceFlux(D->faces[k]);
+    // no light to emit photons from: an empty pass
+    if (i == 0) { C[PT_C_MARCH0] = total > 0.0f ? D->nphotons : 0u; C[PT_C_MARCH1] = 0u; C[PT_C_SHADOW] = 0u; }
+    if (i < 2u * PT_PARTS) C[PT_C_HEADS_MARCH + i] = 0u;
+    if (i >= D->nphotons || !(total > 0.0f)) return;
+    unsigned int seed[4];
+    sampleSeed(D, seed);
+    Rng r = rngAt(seed, i, 0u, 0x94070000u);
+    V4 u0 = rngNext(r), u1 = rngNext(r);
+    float pick = u0.x * total;
+    float each = total / (float)D->nphotons;
+    V3 o, d, power;
+    float fsun = sunFlux(env);
+    if (pick < fsun) {
+        V3 a = ld3(env.sunDir), t, b;
+        frameOf(a, t, b);
+        float rr = env.sunRadius * sqrtf(u0.y), sp, cp;
+        __sincosf(2.0f * PT_PI * u0.z, &sp, &cp);
+        o = ld3(env.sunPoint) + t * (rr * cp) + b * (rr * sp);
+        d = -sampleSunCone(env, u1.x, u1.y);
+        // directions uniform over the cone, positions over the disc: a photon's weight is L cos(theta) Omega A,
+        // L = E / Omega, so E A cos(theta) -- each carries the flux share times cos(theta)
+        power = v3s(each * fmaxf(-dot(d, a), 0.0f));
+    } else {
+        pick -= fsun;
+        unsigned int k = 0u;
+        for (; k + 1u < D->nfaces; ++k) { float ff = faceFlux(D->faces[k]); if (pick < ff) break; pick -= ff; }
+        const PtLightFace& f = D->faces[k];
+        V3 y = ld3(f.centre);
+        if (f.shape == 1u) {
+            float rr = f.halfx * sqrtf(u0.y), sp, cp; __sincosf(2.0f * PT_PI * u0.z, &sp, &cp);
+            y += (ld3(f.u) * cp + ld3(f.v) * sp) * rr;
+        } else {
+            y += ld3(f.u) * ((2.0f * u0.y - 1.0f) * f.halfx) + ld3(f.v) * ((2.0f * u0.z - 1.0f) * f.halfy);
+        }
+        V3 side = (f.twoSided && u1.z < 0.5f) ? -ld3(f.n) : ld3(f.n);
+        d = sampleCosine(side, u1.x, u1.y);
+        o = y + side * (2.0f * offsetTol(D, y));
+        V3 le = ld3(f.le);
+        power = le * (each / fmaxf(luma(le), 1e-30f));
+    }
+    st4(D->pos, i, o.x, o.y, o.z, 0.0f);
+    st4(D->dir, i, d.x, d.y, d.z, 1.0f);
+    st4(D->beta, i, power.x, power.y, power.z, -1.0f);
+    st4(D->rad, i, 0.0f, 0.0f, 0.0f, 0.0f);
+    st4u(D->meta, i, make_uint4(0u, 0u, mzPack(0u, PT_ACTIVE, PT_F_PHOTON), 0u));
+    float collide = hazeOn(D) ? hazeDraw(D->u.haze, o, d, u1.w) : 3.0e38f;
+    st4(D->aux, i, 1.0f, collide, 0.0f, 0.0f);
+    D->qMarch0[i] = i;
 }

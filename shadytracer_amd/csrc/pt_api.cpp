@@ -24,8 +24,9 @@ namespace {
 #include "pt_device_src.inc"   // kPtDevicePre, kPtDevicePost: the device sources as strings
 
 const char* kKernelNames[] = {"pt_genrays", "pt_march", "pt_shade", "pt_occlude", "pt_resolve", "pt_mf_lut", "pt_mf_lut_avg", "pt_pick",
-                              "pt_bin_count", "pt_bin_scan", "pt_bin_scatter"};
-enum { K_GENRAYS, K_MARCH, K_SHADE, K_OCCLUDE, K_RESOLVE, K_MFLUT, K_MFAVG, K_PICK, K_BINCOUNT, K_BINSCAN, K_BINSCATTER, K_COUNT };
+                              "pt_bin_count", "pt_bin_scan", "pt_bin_scatter", "pt_sppm_clear", "pt_photon_gen"};
+enum { K_GENRAYS, K_MARCH, K_SHADE, K_OCCLUDE, K_RESOLVE, K_MFLUT, K_MFAVG, K_PICK, K_BINCOUNT, K_BINSCAN, K_BINSCATTER,
+       K_SPPMCLEAR, K_PHOTONGEN, K_COUNT };
 
 thread_local std::string g_createError;
 
@@ -158,7 +159,9 @@ struct pt_ctx {
 
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
-    unsigned int graphRounds = 0;
+    uint64_t graphKey = 0;       // what the captured graph's launches were sized for (rounds, rays, photons)
+
+    unsigned int sppmCapacity = 0;   // photon records allocated
 
     std::vector<void*> owned;
 };
@@ -212,7 +215,10 @@ void raygenTiles(unsigned int N, unsigned int w, unsigned int h, unsigned int& t
     if (ty < 1) ty = 1;
 }
 
-int ensureBuffers(pt_ctx* c, unsigned int w, unsigned int h) {
+// Frame and accumulators for w x h pixels; path slots for the larger of the pixels and the photons a pass emits; the
+// photon map (three tables of 2^20 words; room for two records, 32 bytes and two links each, a photon) when photons
+// are asked for.
+int ensureBuffers(pt_ctx* c, unsigned int w, unsigned int h, unsigned int photons) {
     unsigned int px = w * h;
     if (px == 0) return fail(c, PT_ERR_ARG, "width and height must be positive (got %u x %u)", w, h);
     if (px > c->pixels) {
@@ -230,7 +236,23 @@ int ensureBuffers(pt_ctx* c, unsigned int w, unsigned int h) {
         HIPCHK(c, hipMemsetAsync(c->host.moments, 0, size_t(px) * 16, c->stream));
         c->pixels = px;
     }
-    if (px > c->capacity) {
+    unsigned int slots = photons > px ? photons : px;
+    // a photon stores once at each evaluable surface it meets along its delta chain: room for two a photon
+    unsigned int records = photons > (1u << 30) ? (1u << 31) : 2u * photons;
+    if (records > c->sppmCapacity) {
+        destroyGraph(c);
+        void* p;
+        p = c->host.sppmRecs; freeBuf(c, p); c->host.sppmRecs = nullptr;
+        p = c->host.sppmNext; freeBuf(c, p); c->host.sppmNext = nullptr;
+        int rc;
+        // heads: the cells' lists, the cells' counts, the sub-lists; a record's links: its list's next, its sub-list's
+        if (!c->host.sppmHeads && (rc = allocBuf(c, c->host.sppmHeads, size_t(3) * PT_SPPM_HEADS * 4))) return rc;
+        if ((rc = allocBuf(c, c->host.sppmRecs, size_t(records) * 32))) return rc;
+        if ((rc = allocBuf(c, c->host.sppmNext, size_t(records) * 8))) return rc;
+        c->sppmCapacity = records;
+    }
+    c->host.sppmCap = c->sppmCapacity;
+    if (slots > c->capacity) {
         destroyGraph(c);
         float** f4[] = {&c->host.pos, &c->host.dir, &c->host.beta, &c->host.rad, &c->host.aux, &c->host.shOrig, &c->host.shDir, &c->host.shRad};
         for (auto pp : f4) { void* p = *pp; freeBuf(c, p); *pp = nullptr; }
@@ -239,11 +261,11 @@ int ensureBuffers(pt_ctx* c, unsigned int w, unsigned int h) {
         { void* p = c->host.meta; freeBuf(c, p); c->host.meta = nullptr; }
         { void* p = c->host.maskBuf; freeBuf(c, p); c->host.maskBuf = nullptr; }
         int rc;
-        for (auto pp : f4) if ((rc = allocBuf(c, *pp, size_t(px) * 16))) return rc;
-        if ((rc = allocBuf(c, c->host.meta, size_t(px) * 16))) return rc;
-        if ((rc = allocBuf(c, c->host.maskBuf, size_t(px) * 8))) return rc;
-        for (auto pp : u1) if ((rc = allocBuf(c, *pp, size_t(px) * 4))) return rc;
-        c->capacity = px;
+        for (auto pp : f4) if ((rc = allocBuf(c, *pp, size_t(slots) * 16))) return rc;
+        if ((rc = allocBuf(c, c->host.meta, size_t(slots) * 16))) return rc;
+        if ((rc = allocBuf(c, c->host.maskBuf, size_t(slots) * 8))) return rc;
+        for (auto pp : u1) if ((rc = allocBuf(c, *pp, size_t(slots) * 4))) return rc;
+        c->capacity = slots;
     }
     c->host.capacity = c->capacity;
     c->devDirty = true;
@@ -274,6 +296,8 @@ unsigned int gridFor(const pt_ctx* c, int k) {
         case K_RESOLVE: return (c->host.u.width * c->host.u.height + 255u) / 256u;
         case K_BINCOUNT: case K_BINSCATTER: return PT_BIN_BLOCKS;
         case K_BINSCAN: return 1;
+        case K_SPPMCLEAR: return 3u * PT_SPPM_HEADS / 256u;
+        case K_PHOTONGEN: return (c->host.nphotons + 255u) / 256u;
         default: return c->host.gridBlocks;
     }
 }
@@ -287,8 +311,18 @@ hipError_t enqueueSample(pt_ctx* c, std::vector<hipEvent_t>* events, std::vector
         if (events) { hipEventRecord((*events)[kinds->size() * 2 + 1], c->stream); kinds->push_back(k); }
         return r;
     };
-    if ((e = one(K_GENRAYS, 0))) return e;
     unsigned int R = rounds(c);
+    if (c->host.nphotons) {
+        // THE PHOTON PASS: the map cleared and r_k set, photons emitted, marched and shaded round by round (no shadow
+        // rays; occlude runs on an empty queue for its resets of the march queue)
+        if ((e = one(K_SPPMCLEAR, 0)) || (e = one(K_PHOTONGEN, 0))) return e;
+        for (unsigned int it = 0; it < R; ++it) {
+            unsigned int in = it & 1u;
+            if ((e = one(K_BINCOUNT, in)) || (e = one(K_BINSCAN, 0)) || (e = one(K_BINSCATTER, in))) return e;
+            if ((e = one(K_MARCH, in)) || (e = one(K_SHADE, in ^ 1u)) || (e = one(K_OCCLUDE, in))) return e;
+        }
+    }
+    if ((e = one(K_GENRAYS, 0))) return e;
     for (unsigned int it = 0; it < R; ++it) {
         unsigned int in = it & 1u;
         if ((e = one(K_BINCOUNT, in)) || (e = one(K_BINSCAN, 0)) || (e = one(K_BINSCATTER, in))) return e;
@@ -300,8 +334,13 @@ hipError_t enqueueSample(pt_ctx* c, std::vector<hipEvent_t>* events, std::vector
     return one(K_RESOLVE, 0);
 }
 
+uint64_t graphKey(const pt_ctx* c) {
+    return (uint64_t(rounds(c)) << 56) ^ (uint64_t(c->host.nrays) << 28) ^ uint64_t(c->host.nphotons) ^
+           (uint64_t(c->host.gridBlocks) << 48);
+}
+
 int buildGraph(pt_ctx* c) {
-    if (c->graphExec && c->graphRounds == rounds(c)) return PT_OK;
+    if (c->graphExec && c->graphKey == graphKey(c)) return PT_OK;
     destroyGraph(c);
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = enqueueSample(c, nullptr, nullptr);
@@ -313,7 +352,7 @@ int buildGraph(pt_ctx* c) {
     }
     c->graph = g;
     HIPCHK(c, hipGraphInstantiate(&c->graphExec, g, nullptr, nullptr, 0));
-    c->graphRounds = rounds(c);
+    c->graphKey = graphKey(c);
     return PT_OK;
 }
 
@@ -330,6 +369,8 @@ void setDefaults(PtUniforms& u) {
     u.faceShare = 0.5f;
     u.marchRefill = 16;
     u.guideFrac = 0.5f;
+    u.sppmRadius = 0.02f;
+    u.sppmAlpha = 2.0f / 3.0f;
 }
 
 }  // namespace
@@ -399,7 +440,7 @@ int pt_create(pt_ctx** out, const pt_opts* opts) {
     }
     hipMemsetAsync(c->host.counters, 0, PT_C_WORDS * 4, c->stream);
     hipMemsetAsync(c->host.stats, 0, PT_S_WORDS * 8, c->stream);
-    if ((rc = ensureBuffers(c, c->host.u.width, c->host.u.height))) {
+    if ((rc = ensureBuffers(c, c->host.u.width, c->host.u.height, 0))) {
         g_createError = c->err;
         pt_destroy(c);
         return rc;
@@ -493,8 +534,13 @@ int pt_set_uniforms(pt_ctx* c, const PtUniforms* u, size_t size) {
     if (u->maxbounce > 250) return fail(c, PT_ERR_ARG, "maxbounce %u above 250 (a path counts its bounces in 8 bits)", u->maxbounce);
     if (u->marchRefill == 0 || u->marchRefill > 64) return fail(c, PT_ERR_ARG, "marchRefill %u not in 1..64", u->marchRefill);
     if (!(u->stepscale > 0.0f && u->stepscale <= 1.0f)) return fail(c, PT_ERR_ARG, "stepscale %g not in (0, 1]", u->stepscale);
+    if (u->sppm && (u->sppmPhotons == 0 || u->sppmPhotons > (1u << 26)))
+        return fail(c, PT_ERR_ARG, "sppmPhotons %u not in 1..2^26", u->sppmPhotons);
+    if (u->sppm && !(u->sppmRadius > 0.0f && u->sppmAlpha > 0.0f && u->sppmAlpha < 1.0f))
+        return fail(c, PT_ERR_ARG, "sppmRadius %g must be > 0 and sppmAlpha %g in (0, 1)", u->sppmRadius, u->sppmAlpha);
+    unsigned int photons = u->sppm ? u->sppmPhotons : 0u;
     int rc;
-    if ((rc = ensureBuffers(c, u->width, u->height))) return rc;
+    if ((rc = ensureBuffers(c, u->width, u->height, photons))) return rc;
     bool resized = u->width != c->host.u.width || u->height != c->host.u.height;
     c->host.u = *u;
     unsigned int N = u->width * u->height;
@@ -502,6 +548,7 @@ int pt_set_uniforms(pt_ctx* c, const PtUniforms* u, size_t size) {
     c->host.invTX = 1.0f / float(c->host.tilesX);
     c->host.invTY = 1.0f / float(c->host.tilesY);
     c->host.nrays = N;
+    c->host.nphotons = photons;
     c->devDirty = true;
     if (resized) return pt_clear_accum(c);
     return PT_OK;
@@ -561,7 +608,7 @@ int pt_time_kernels(pt_ctx* c, uint32_t samples, double* ms, size_t n) {
     int rc;
     if ((rc = syncDev(c))) return rc;
     for (size_t k = 0; k < n; ++k) ms[k] = 0.0;
-    size_t launches = 2 + 9 * size_t(rounds(c));
+    size_t launches = 4 + 15 * size_t(rounds(c));
     std::vector<hipEvent_t> ev(2 * launches);
     for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
     for (uint32_t s = 0; s < samples; ++s) {
@@ -573,7 +620,8 @@ int pt_time_kernels(pt_ctx* c, uint32_t samples, double* ms, size_t n) {
             float t = 0.0f;
             hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]);
             int k = kinds[i];
-            int slot = k == K_GENRAYS ? 0 : k == K_MARCH ? 1 : k == K_SHADE ? 2 : k == K_OCCLUDE ? 3 : k == K_RESOLVE ? 4 : 5;
+            int slot = k == K_GENRAYS ? 0 : k == K_MARCH ? 1 : k == K_SHADE ? 2 : k == K_OCCLUDE ? 3 : k == K_RESOLVE ? 4
+                     : (k == K_SPPMCLEAR || k == K_PHOTONGEN) ? 6 : 5;
             if (slot >= (int)n) continue;
             ms[slot] += t;
         }

@@ -15,6 +15,7 @@
  *   caustic-c3   the room with a 2 x 2 x 2 m haze box (sigma_s 0.05 /m, g 0.6, albedo 1), a dispersive glass sphere
  *                (r 0.4 m, n = 1.5 + 0.0042 / lambda^2 um) and a 100 klx sun through a 0.3 m aperture in the ceiling;
  *                1280 x 720.
+ *   sppm-slab    a collimated beam through a glass slab onto a floor: the photon map's closed form.
  *   furnace-*    closed-form checks: a white furnace (albedo-1 Lambertian plane under a uniform sky, and a haze box
  *                under a uniform sphere of sky).
  */
@@ -141,9 +142,15 @@ function causticC3(opts = {}) {
         // n = 1.5 + 0.0042 / lambda^2 (um) exactly: n_d = 1.5 + 0.0042 / 0.5876^2, V_d = (n_d - 1) / (n_F - n_C)
         material('glass', [1, 1, 1], 0, 0, { transmission: 1, ior: 1.5121643, abbe: 63.83197, specular: 0.5 })]
     const objs = roomObjects(r, mats)
-    // the aperture: a 0.3 m hole through the ceiling above the glass
-    const hole = obj('aperture', 'b-cylinder', 'walls', xf(0.6, 0.3, 1.6), { radius: 0.15, height: 0.6 }, { join: 'subtract' })
-    const glass = obj('glass-sphere', 'b-sphere', 'glass', xf(0.3, 0.15, 0.9), { radius: 0.4 })
+    // the sun (azimuth 26.57, elevation 70) through a 0.3 m beam aimed at the glass: the beam's axis meets the
+    // ceiling's mid-plane (z 3.1) where the line from the sphere's centre towards the sun does; the hole about it is
+    // 0.4 m across, so the slanted beam clears the 0.2 m slab
+    const az = 26.57 * Math.PI / 180, el = 70 * Math.PI / 180
+    const sd = [Math.cos(el) * Math.cos(az), Math.cos(el) * Math.sin(az), Math.sin(el)]
+    const sphere = [0.3, 0.15, 0.9], t = (3.1 - sphere[2]) / sd[2]
+    const hx = sphere[0] + sd[0] * t, hy = sphere[1] + sd[1] * t
+    const hole = obj('aperture', 'b-cylinder', 'walls', xf(hx, hy, 1.6), { radius: 0.2, height: 0.6 }, { join: 'subtract' })
+    const glass = obj('glass-sphere', 'b-sphere', 'glass', xf(sphere[0], sphere[1], sphere[2]), { radius: 0.4 })
     const hazeBox = { id: 'haze', kind: 'object', type: 'haze', label: 'haze', enabled: true, material: null, transform: xf(0, 0, 1.2),
         params: { size: { x: 2, y: 2, z: 2 } }, haze: { mfp: 20, g: 0.6, albedo: { r: 1, g: 1, b: 1 } } }
     // keep the glass and the beam clear of the random objects: drop those near them
@@ -151,11 +158,34 @@ function causticC3(opts = {}) {
         const c = o.kind === 'group' ? o.children[0].transform.pos : o.transform.pos
         return Math.hypot(c.x - 0.3, c.y - 0.15) > 0.9
     })
-    const env = { sky: 'none', sunIlluminance: 100000, uSunAzimuth: 26.57, uSunElevation: 70, sunDiameter: 0.53 }
+    const env = { sky: 'none', sunIlluminance: 100000, uSunAzimuth: 26.57, uSunElevation: 70, sunDiameter: 0.53,
+        sunAperture: { point: { x: hx, y: hy, z: 3.1 }, radius: 0.15 } }
     const d = doc('caustic-c3', mats, [roomChildren(hole), ...clear, glass, hazeBox, panel()], env, 10)
     const w = opts.width || 1280, h = opts.height || 720
     return { doc: d, camera: camera([0, -1.95, 1.6], [0.2, 0.5, 0.5], 70, w, h),
-        settings: { width: w, height: h, spp: 64, maxbounce: 8, nee: 1, sppmIterations: 10, sppmPhotons: 1 << 22, sppmRadius: 0.02, sppmAlpha: 0.7 } }
+        settings: { width: w, height: h, spp: 10, maxbounce: 8, nee: 1, sppm: 1, sppmIterations: 10, sppmPhotons: 1 << 22, sppmRadius: 0.02, sppmAlpha: 0.7 } }
+}
+
+// A collimated beam through a glass slab onto a floor (the closed form of core/test/sppm.test.mjs's collimated case):
+// the sun straight down (100 klx, 0.3 m radius beam about (0, 0, 2)), a 2 x 2 x 0.1 m slab of n = 1.5 at z = 1, a
+// Lambertian floor of albedo 0.5 at z = 0 seen from under the slab. At normal incidence each face reflects
+// R = ((n - 1) / (n + 1))^2 = 0.04, the slab passes (1 - R) / (1 + R) of the beam with every internal reflection
+// counted, and the floor in the beam reads 0.5 x 100000 x 0.96 / 1.04 / pi = 14691.3 cd/m^2 -- all of it caustic,
+// the photon map's alone (the shadow rays towards the sun meet the glass).
+// Options (for the map-against-path-tracing check): floorRoughness / floorSpecular (a glossy floor), sunDiameter
+// (degrees), beam (the beam's radius).
+function sppmSlab(opts = {}) {
+    const fr = opts.floorRoughness !== undefined ? opts.floorRoughness : 1
+    const fsp = opts.floorSpecular !== undefined ? opts.floorSpecular : 0
+    const mats = [material('floor', [0.5, 0.5, 0.5], fr, 0, { specular: fsp }),
+        material('glass', [1, 1, 1], 0, 0, { transmission: 1, ior: 1.5, specular: 0.5 })]
+    const env = { sky: 'none', sunIlluminance: 100000, uSunAzimuth: 0, uSunElevation: 90, sunDiameter: opts.sunDiameter || 0.53,
+        sunAperture: { point: { x: 0, y: 0, z: 2 }, radius: opts.beam || 0.3 } }
+    const d = doc('sppm-slab', mats, [obj('floor', 'b-plane', 'floor', xf(0, 0, 0)),
+        obj('slab', 'b-box', 'glass', xf(0, 0, 1), { size: { x: 1, y: 1, z: 0.05 }, rounding: 0 })], env, 20)
+    const w = opts.width || 64, h = opts.height || 48
+    return { doc: d, camera: camera([0, -0.8, 0.5], [0, 0, 0], 40, w, h),
+        settings: { width: w, height: h, spp: 16, maxbounce: 8, nee: 1, sppm: 1, sppmPhotons: 1 << 20, sppmRadius: 0.02, sppmAlpha: 0.7 } }
 }
 
 function furnacePlane(opts = {}) {
@@ -177,6 +207,7 @@ function furnaceHaze(opts = {}) {
     return { doc: d, camera: camera([0, -4, 0], [0, 0, 0], 30, w, h), settings: { width: w, height: h, spp: 64, maxbounce: 250, nee: 1 } }
 }
 
-const CONFIGS = { 'plane-c1': planeC1, 'room-c2': roomC2, 'caustic-c3': causticC3, 'furnace-plane': furnacePlane, 'furnace-haze': furnaceHaze }
+const CONFIGS = { 'plane-c1': planeC1, 'room-c2': roomC2, 'caustic-c3': causticC3, 'furnace-plane': furnacePlane, 'furnace-haze': furnaceHaze,
+    'sppm-slab': sppmSlab }
 
 module.exports = { CONFIGS, camera, rng, SEED }

@@ -50,7 +50,7 @@ class PtUniforms(ctypes.Structure):
                 ('fireflyClamp', ctypes.c_float), ('fireflyWhite', ctypes.c_float), ('maxaverages', ctypes.c_uint),
                 ('seed', ctypes.c_uint), ('cam', PtCamera), ('env', PtEnv), ('haze', PtHaze), ('faceShare', ctypes.c_float),
                 ('sppm', ctypes.c_uint), ('sppmRadius', ctypes.c_float), ('sppmPhotons', ctypes.c_uint),
-                ('marchRefill', ctypes.c_uint), ('guide', ctypes.c_uint), ('guideFrac', ctypes.c_float), ('pad0', ctypes.c_uint)]
+                ('marchRefill', ctypes.c_uint), ('guide', ctypes.c_uint), ('guideFrac', ctypes.c_float), ('sppmAlpha', ctypes.c_float)]
 
 
 class PtSceneDesc(ctypes.Structure):
@@ -88,7 +88,8 @@ SIGNATURES = {
 }
 
 STAT_NAMES = ['segments', 'shadow_segments', 'steps', 'shadow_steps', 'nonfinite', 'maxiter', 'paths', 'leaving',
-              'gathers', 'photons', 'flops', 'shadow_flops']
+              'gathers', 'photons', 'flops', 'shadow_flops', 'photon_segments', 'gather_cut', 'radius_bits',
+              'overflow']
 LUMA = (0.2126, 0.7152, 0.0722)
 
 _lib = None
@@ -155,6 +156,7 @@ def default_uniforms():
     u.nee, u.rr, u.rrStart, u.diffuseCap = 1, 0, 3, 0
     u.fireflyClamp, u.fireflyWhite, u.maxaverages, u.seed = 0.0, 1.0, 0, 20261203
     u.faceShare, u.marchRefill, u.guideFrac = 0.5, 16, 0.5
+    u.sppmRadius, u.sppmAlpha = 0.02, 2.0 / 3.0
     return u
 
 
@@ -172,10 +174,10 @@ def uniforms_for(made, **over):
     u.cam = make_camera(made['camera'])
     u.env = make_env(sc['env'])
     u.haze = make_haze(sc.get('haze'))
-    for k in ('maxiter', 'marchRefill', 'rr', 'rrStart', 'diffuseCap', 'maxaverages'):
+    for k in ('maxiter', 'marchRefill', 'rr', 'rrStart', 'diffuseCap', 'maxaverages', 'sppm', 'sppmPhotons'):
         if k in s:
             setattr(u, k, int(s[k]))
-    for k in ('stepscale', 'eps', 'epsAuto', 'faceShare', 'fireflyClamp'):
+    for k in ('stepscale', 'eps', 'epsAuto', 'faceShare', 'fireflyClamp', 'sppmRadius', 'sppmAlpha'):
         if k in s:
             setattr(u, k, float(s[k]))
     return u
@@ -265,15 +267,17 @@ class Engine:
     def stats(self):
         a = (ctypes.c_uint64 * 16)()
         self._chk(self.lib.pt_read_stats(self.h, a, 16))
-        return {k: int(a[i]) for i, k in enumerate(STAT_NAMES)}
+        out = {k: int(a[i]) for i, k in enumerate(STAT_NAMES)}
+        out['sppm_radius'] = float(np.array([out['radius_bits'] & 0xFFFFFFFF], np.uint32).view(np.float32)[0])
+        return out
 
     def reset_stats(self):
         self._chk(self.lib.pt_reset_stats(self.h))
 
     def time_kernels(self, samples=1):
-        ms = (ctypes.c_double * 6)()
-        self._chk(self.lib.pt_time_kernels(self.h, samples, ms, 6))
-        return dict(zip(['genrays', 'march', 'shade', 'occlude', 'resolve', 'bin'], list(ms)))
+        ms = (ctypes.c_double * 7)()
+        self._chk(self.lib.pt_time_kernels(self.h, samples, ms, 7))
+        return dict(zip(['genrays', 'march', 'shade', 'occlude', 'resolve', 'bin', 'photon'], list(ms)))
 
     def pick_rays(self):
         """Each camera ray of one sample to its first event: (pos4 [n, 4], state [n], material [n], prim [n])."""

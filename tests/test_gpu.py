@@ -21,10 +21,13 @@ def say(*a):
     sys.stdout.flush()
 
 
+SCENE_OPTS = ('g', 'roughness', 'specular', 'metallic', 'albedo', 'mfp', 'floorRoughness', 'floorSpecular', 'sunDiameter', 'beam')
+
+
 def load(engine, name, **over):
-    made = scenes.emit(name, **{k: v for k, v in over.items() if k in ('width', 'height', 'g', 'roughness', 'specular', 'metallic', 'albedo', 'mfp')})
+    made = scenes.emit(name, **{k: v for k, v in over.items() if k in SCENE_OPTS + ('width', 'height')})
     engine.set_scene(made['scene'])
-    u = pt.uniforms_for(made, **{k: v for k, v in over.items() if k not in ('g', 'roughness', 'specular', 'metallic', 'albedo', 'mfp')})
+    u = pt.uniforms_for(made, **{k: v for k, v in over.items() if k not in SCENE_OPTS})
     engine.set_uniforms(u)
     engine.clear_accums()
     engine.reset_stats()
@@ -144,3 +147,112 @@ def test_node_addon_drives_the_engine():
     out = json.loads(r.stdout)
     assert abs(out['floor']['mean'] / 800 - 1) < 2e-3, out['floor']
     assert out['stats']['samples'] == 16 and out['log']['segments'] > 0
+
+
+# ---------------------------------------------------------------------------------------------------- SPPM
+SLAB_FLOOR = 0.5 * 100000 * (1 - 0.04) / (1 + 0.04) / math.pi   # rho E T / pi, T = (1 - R) / (1 + R), R = 0.04
+
+
+def beam_pixels(made, radius):
+    u = pt.uniforms_for(made)
+    pos, state, mat, prim = oracle.pick(made['scene'], u)
+    return (np.hypot(pos[:, 0], pos[:, 1]) < radius).reshape(u.height, u.width)
+
+
+def gpu_samples(engine, samples):
+    """Each sample's own luminance image, from the running accumulator read after every sample."""
+    out, prev = [], None
+    for k in range(1, samples + 1):
+        engine.trace(1)
+        Y = pt.luminance(engine.read_accum()[0])
+        out.append(Y if prev is None else k * Y - (k - 1) * prev)
+        prev = Y
+    return out
+
+
+def oracle_samples(made, u, samples):
+    """The oracle's sample k alone (its photon pass at r_k), for k = 1..samples."""
+    return [pt.luminance(oracle.render(made['scene'], u, 1, first=k)[0]) for k in range(samples)]
+
+
+def agree(g, o, mask, rel):
+    """Per-sample means over mask: equal within 4 standard errors (sample to sample spread, map noise included)
+    and rel of the mean (the kernel's blur differs between the engine's subsampled map and the oracle's exact one
+    only in noise; rel covers the float rounding of long paths)."""
+    gm = np.array([x[mask].mean() for x in g])
+    om = np.array([x[mask].mean() for x in o])
+    se = math.sqrt(gm.var(ddof=1) / len(gm) + om.var(ddof=1) / len(om))
+    tol = 4 * se + rel * max(om.mean(), 1e-3)
+    return abs(gm.mean() - om.mean()) / tol, gm.mean(), om.mean(), se
+
+
+def test_sppm_slab_closed_form(engine):
+    say('case: sppm-slab 64x48, 2^20 photons x 16 samples, floor = rho E T / pi')
+    made, u = load(engine, 'sppm-slab', width=64, height=48)
+    inb = beam_pixels(made, 0.2)
+    engine.trace(16)
+    acc, mom = engine.read_accum()
+    st = engine.stats()
+    Y = pt.luminance(acc)[inb]
+    se = Y.std() / math.sqrt(inb.sum())
+    assert abs(Y.mean() - SLAB_FLOOR) < 4 * se + 0.01 * SLAB_FLOOR, (Y.mean(), se, SLAB_FLOOR)
+    frac = st['photons'] / (16 * (1 << 20))
+    assert abs(frac - 0.96 / 1.04) < 2e-3, frac
+    assert st['overflow'] == 0 and st['nonfinite'] == 0 and st['gathers'] >= 16 * 64 * 48 * 0.9
+
+
+def test_sppm_radius_follows_the_schedule(engine):
+    say('case: sppm-slab 32x24, r_k after k = 1..6 samples')
+    made, u = load(engine, 'sppm-slab', width=32, height=24, sppmPhotons=1 << 14, sppmRadius=0.05, sppmAlpha=0.7)
+    for k in range(1, 7):
+        engine.trace(1)
+        got = engine.stats()['sppm_radius']
+        assert got == pytest.approx(oracle.sppm_radius(0.05, 0.7, k), rel=2e-6), k
+
+
+def test_sppm_map_on_a_glossy_floor_matches_the_oracle(engine):
+    say('case: sppm-slab glossy floor, 20-degree sun, 0.6 m beam, 32x24, 2^18 photons')
+    opts = dict(width=32, height=24, floorRoughness=0.4, floorSpecular=0.5, sunDiameter=20, beam=0.6)
+    made, u = load(engine, 'sppm-slab', sppmPhotons=1 << 18, sppmRadius=0.03, **opts)
+    inb = beam_pixels(made, 0.3)
+    g = gpu_samples(engine, 16)
+    o = oracle_samples(made, pt.uniforms_for(made, sppmPhotons=1 << 18, sppmRadius=0.03), 16)
+    r = agree(g, o, inb, 0.005)
+    assert r[0] < 1, r
+
+
+def test_sppm_switches_off_and_on(engine):
+    say('case: sppm-slab 32x24 with the map, without, and with it again')
+    made, u = load(engine, 'sppm-slab', width=32, height=24, sppmPhotons=1 << 16)
+    inb = beam_pixels(made, 0.2)
+    engine.trace(2)
+    on = pt.luminance(engine.read_accum()[0])[inb].mean()
+    engine.set_uniforms(pt.uniforms_for(made, sppm=0))
+    engine.clear_accums()
+    engine.trace(2)
+    off = pt.luminance(engine.read_accum()[0])[inb].max()
+    engine.set_uniforms(pt.uniforms_for(made, sppmPhotons=1 << 16))
+    engine.clear_accums()
+    engine.trace(2)
+    again = pt.luminance(engine.read_accum()[0])[inb].mean()
+    assert off == 0
+    assert abs(on / SLAB_FLOOR - 1) < 0.05 and abs(again / SLAB_FLOOR - 1) < 0.05, (on, again)
+
+
+def test_c3_caustic_room_matches_the_oracle(engine):
+    say('case: caustic-c3 160x90, 2^18 photons, 6 samples, 16 regions vs oracle')
+    made, u = load(engine, 'caustic-c3', width=160, height=90, sppmPhotons=1 << 18)
+    regions = []
+    for by in range(4):
+        for bx in range(4):
+            m = np.zeros((90, 160), bool)
+            m[by * 90 // 4:(by + 1) * 90 // 4, bx * 40:(bx + 1) * 40] = True
+            regions.append(m)
+    g = gpu_samples(engine, 8)
+    st = engine.stats()
+    assert st['nonfinite'] == 0 and st['photons'] > 0 and st['gathers'] > 0 and st['overflow'] == 0
+    o = oracle_samples(made, pt.uniforms_for(made, sppmPhotons=1 << 18), 8)
+    results = [agree(g, o, m, 0.03) for m in regions + [np.ones((90, 160), bool)]]
+    say('worst region at %.2f of its tolerance' % max(r[0] for r in results))
+    for r in results:
+        assert r[0] < 1, r
