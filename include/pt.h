@@ -105,7 +105,8 @@ int pt_reset_stats(pt_ctx* ctx);
 
 /* Runs `samples` samples kernel by kernel with HIP events around each launch on the engine's stream and
  * returns the summed milliseconds per kernel: ms[0] genrays, [1] march, [2] shade, [3] occlude, [4] resolve,
- * [5] the queues' binning, [6] the photon pass's map clear and emission (n >= 5; the slots past n are dropped).
+ * [5] the queues' binning, [6] the photon pass's map clear and emission, [7] surface (n >= 5; the slots past n are
+ * dropped).
  * Replaces R.timeKernels (raytrace_main.js:549-557). */
 int pt_time_kernels(pt_ctx* ctx, uint32_t samples, double* ms, size_t n);
 

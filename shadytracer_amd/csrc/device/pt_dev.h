@@ -26,6 +26,7 @@ struct PtDev {
     unsigned int* qShadow;
     unsigned int* qSorted;          // this round's march queue, binned by the rays' node masks
     unsigned long long* maskBuf;    // per slot: the nodes its current segment can reach
+    float* hitRec;                  // per slot, 4 words: the surface kernel's normal and material word
     unsigned int* binHist;          // PT_BINS counts, then (scan) the bins' write offsets
     unsigned int* binOff;
     // shadow rays by slot

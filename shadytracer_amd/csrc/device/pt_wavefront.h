@@ -217,11 +217,6 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_march(const PtDev* __r
     statAdd(D, PT_S_FLOPS, nflops);
 }
 
-// ---------------------------------------------------------------------------------------------------
-// SHADE (core/wavefront.js:1244-1442 with the surface kernel :1037-1147 folded in): escape to the environment
-// weighted by the last vertex's MIS; the hit's normal, facing and material; an emitter's light weighted against
-// face sampling; light sampling (faces, or the sky and sun) with the power heuristic; the BSDF's next direction;
-// the bounce cap and Russian roulette; a point in the haze scatters by Henyey-Greenstein.
 __device__ __forceinline__ V3 normalAt(ParamPtr U, V3 p, float h) {
     const float a = sceneDist(U, p + v3(h, -h, -h)), b = sceneDist(U, p + v3(-h, -h, h));
     const float c = sceneDist(U, p + v3(-h, h, -h)), d = sceneDist(U, p + v3(h, h, h));
@@ -229,6 +224,43 @@ __device__ __forceinline__ V3 normalAt(ParamPtr U, V3 p, float h) {
     float l = length(g);
     return l > 0.0f ? g * (1.0f / l) : v3(0.0f, 0.0f, 1.0f);
 }
+
+// ---------------------------------------------------------------------------------------------------
+// SURFACE (core/wavefront.js:1037-1147 prog_surface): for each ray of the round that stopped at a surface, the one
+// full-SDF look at the hit -- its normal (the SDF's own, else a tetrahedral difference), whether the ray is inside,
+// its material -- and, for a transmissive material left from inside, the medium a hair beyond (the innermost
+// object there, or air). Written as a 16-byte hit record: the normal, then material | inside << 8 | beyond-known
+// << 9 | beyond material << 16. Shade reads it; the guide network reads it too.
+#define PT_HR_INSIDE 0x100u
+#define PT_HR_BEYOND 0x200u
+extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_surface(const PtDev* __restrict__ D, unsigned int qsel) {
+    const unsigned int count = D->counters[qsel ? PT_C_MARCH0 : PT_C_MARCH1];
+    ParamPtr U = (const PT_CONST float*)D->params;
+    for (unsigned int i = blockIdx.x * PT_BLOCK + threadIdx.x; i < count; i += gridDim.x * PT_BLOCK) {
+        const unsigned int slot = D->qSorted[i];
+        const uint4 M = ld4u(D->meta, slot);
+        if (mzState(M.z) != PT_ACTIVE) continue;
+        const float4 P = ld4(D->pos, slot), Dd = ld4(D->dir, slot);
+        const V3 p = v3(P.x, P.y, P.z), dir = v3(Dd.x, Dd.y, Dd.z);
+        SdfHit h = sceneFull(U, p);
+        V3 ns = h.n;
+        if (dot(ns, ns) < 0.25f) ns = normalAt(U, p, marchTol(tolSetting(D), p));
+        const bool inside = h.d < 0.0f;
+        const unsigned int mi = h.m < D->nmats ? h.m : 0u;
+        unsigned int w = mi | (inside ? PT_HR_INSIDE : 0u);
+        if (inside && D->mats[mi].transmission > 0.0f && dot(dir, -ns) <= 0.0f) {
+            SdfHit hb = sceneFull(U, p + dir * (2.0f * offsetTol(D, p)));
+            if (hb.insideW > -1.0e6f && hb.a < D->nmats) w |= PT_HR_BEYOND | (hb.a << 16);   // inside something (-1e7: nothing, air)
+        }
+        st4(D->hitRec, slot, ns.x, ns.y, ns.z, __uint_as_float(w));
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------
+// SHADE (core/wavefront.js:1244-1442): escape to the environment weighted by the last vertex's MIS; from the hit
+// record, the facing and material; an emitter's light weighted against face sampling; light sampling (faces, or the
+// sky and sun) with the power heuristic; the BSDF's next direction; the bounce cap and Russian roulette; a point in
+// the haze scatters by Henyey-Greenstein.
 
 struct ShadeOut {
     bool requeue, shadow, store, photon;
@@ -373,12 +405,11 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
             out.requeue = true;
         }
     } else {
-        // A SURFACE HIT
-        ParamPtr U = (const PT_CONST float*)D->params;
-        SdfHit h = sceneFull(U, p);
-        V3 ns = h.n;
-        if (dot(ns, ns) < 0.25f) ns = normalAt(U, p, marchTol(tolSetting(D), p));
-        bool inside = h.d < 0.0f;
+        // A SURFACE HIT: what the surface kernel found there
+        const float4 HR = ld4(D->hitRec, slot);
+        const unsigned int hw = __float_as_uint(HR.w);
+        V3 ns = v3(HR.x, HR.y, HR.z);
+        bool inside = (hw & PT_HR_INSIDE) != 0u;
         V3 nrm = inside ? -ns : ns;
         if (dot(dir, nrm) > 0.0f) {
             // hit from behind (core/scene_api.js:127-136 pathFacing): moved on, no bounce
@@ -388,7 +419,7 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
             atomicAdd(&D->stats[PT_S_LEAVING], 1ull);
         } else {
             bounces += 1u;
-            const PtMaterial& m = smat[h.m < D->nmats ? h.m : 0u];
+            const PtMaterial& m = smat[hw & 0xFFu];
             V3 Le = ld3(m.emissive);
             if (max3(Le) > 0.0f && (photon || (mapped && m.face >= 0 && faceFlux(D->faces[m.face]) > 0.0f))) {
                 terminate = true;      // a photon ends at a light; a mapped path's light is the map's (its photons')
@@ -414,10 +445,7 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
                 float n1 = fmaxf(m.ior, 1.0f);
                 if (inside) {
                     n1 = 1.0f;
-                    if (m.transmission > 0.0f) {
-                        SdfHit hb = sceneFull(U, p + dir * (2.0f * offsetTol(D, p)));
-                        if (hb.insideW > -1.0e6f && hb.a < D->nmats) n1 = fmaxf(smat[hb.a].ior, 1.0f);   // inside something (-1e7: nothing, air)
-                    }
+                    if (hw & PT_HR_BEYOND) n1 = fmaxf(smat[hw >> 16].ior, 1.0f);
                 }
                 // DISPERSION (core/spectral.js:121-135): a wavelength drawn at the first refraction into a
                 // dispersive medium; its index n(lambda) by Cauchy through n_d and V_d, the ray's own n after

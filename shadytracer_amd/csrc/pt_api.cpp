@@ -24,9 +24,9 @@ namespace {
 #include "pt_device_src.inc"   // kPtDevicePre, kPtDevicePost: the device sources as strings
 
 const char* kKernelNames[] = {"pt_genrays", "pt_march", "pt_shade", "pt_occlude", "pt_resolve", "pt_mf_lut", "pt_mf_lut_avg", "pt_pick",
-                              "pt_bin_count", "pt_bin_scan", "pt_bin_scatter", "pt_sppm_clear", "pt_photon_gen"};
+                              "pt_bin_count", "pt_bin_scan", "pt_bin_scatter", "pt_sppm_clear", "pt_photon_gen", "pt_surface"};
 enum { K_GENRAYS, K_MARCH, K_SHADE, K_OCCLUDE, K_RESOLVE, K_MFLUT, K_MFAVG, K_PICK, K_BINCOUNT, K_BINSCAN, K_BINSCATTER,
-       K_SPPMCLEAR, K_PHOTONGEN, K_COUNT };
+       K_SPPMCLEAR, K_PHOTONGEN, K_SURFACE, K_COUNT };
 
 thread_local std::string g_createError;
 
@@ -254,7 +254,8 @@ int ensureBuffers(pt_ctx* c, unsigned int w, unsigned int h, unsigned int photon
     c->host.sppmCap = c->sppmCapacity;
     if (slots > c->capacity) {
         destroyGraph(c);
-        float** f4[] = {&c->host.pos, &c->host.dir, &c->host.beta, &c->host.rad, &c->host.aux, &c->host.shOrig, &c->host.shDir, &c->host.shRad};
+        float** f4[] = {&c->host.pos, &c->host.dir, &c->host.beta, &c->host.rad, &c->host.aux, &c->host.shOrig, &c->host.shDir, &c->host.shRad,
+                        &c->host.hitRec};
         for (auto pp : f4) { void* p = *pp; freeBuf(c, p); *pp = nullptr; }
         unsigned int** u1[] = {&c->host.qMarch0, &c->host.qMarch1, &c->host.qShadow, &c->host.qSorted};
         for (auto pp : u1) { void* p = *pp; freeBuf(c, p); *pp = nullptr; }
@@ -285,7 +286,7 @@ unsigned int rounds(const pt_ctx* c) { return c->host.u.maxbounce + 1u; }
 hipError_t launch(pt_ctx* c, int k, unsigned int grid, unsigned int qsel) {
     void* args1[] = {&c->dev};
     void* args2[] = {&c->dev, &qsel};
-    bool two = (k == K_MARCH || k == K_SHADE || k == K_BINCOUNT || k == K_BINSCATTER || k == K_OCCLUDE);
+    bool two = (k == K_MARCH || k == K_SHADE || k == K_BINCOUNT || k == K_BINSCATTER || k == K_OCCLUDE || k == K_SURFACE);
     unsigned int block = k == K_BINSCAN ? PT_BINS : 256;
     return hipModuleLaunchKernel(c->fn[k], grid, 1, 1, block, 1, 1, 0, c->stream, two ? args2 : args1, nullptr);
 }
@@ -319,7 +320,7 @@ hipError_t enqueueSample(pt_ctx* c, std::vector<hipEvent_t>* events, std::vector
         for (unsigned int it = 0; it < R; ++it) {
             unsigned int in = it & 1u;
             if ((e = one(K_BINCOUNT, in)) || (e = one(K_BINSCAN, 0)) || (e = one(K_BINSCATTER, in))) return e;
-            if ((e = one(K_MARCH, in)) || (e = one(K_SHADE, in ^ 1u)) || (e = one(K_OCCLUDE, in))) return e;
+            if ((e = one(K_MARCH, in)) || (e = one(K_SURFACE, in ^ 1u)) || (e = one(K_SHADE, in ^ 1u)) || (e = one(K_OCCLUDE, in))) return e;
         }
     }
     if ((e = one(K_GENRAYS, 0))) return e;
@@ -327,7 +328,7 @@ hipError_t enqueueSample(pt_ctx* c, std::vector<hipEvent_t>* events, std::vector
         unsigned int in = it & 1u;
         if ((e = one(K_BINCOUNT, in)) || (e = one(K_BINSCAN, 0)) || (e = one(K_BINSCATTER, in))) return e;
         if ((e = one(K_MARCH, in))) return e;
-        if ((e = one(K_SHADE, in ^ 1u))) return e;
+        if ((e = one(K_SURFACE, in ^ 1u)) || (e = one(K_SHADE, in ^ 1u))) return e;
         if ((e = one(K_BINCOUNT, 2)) || (e = one(K_BINSCAN, 0)) || (e = one(K_BINSCATTER, 2))) return e;
         if ((e = one(K_OCCLUDE, in))) return e;
     }
@@ -608,7 +609,7 @@ int pt_time_kernels(pt_ctx* c, uint32_t samples, double* ms, size_t n) {
     int rc;
     if ((rc = syncDev(c))) return rc;
     for (size_t k = 0; k < n; ++k) ms[k] = 0.0;
-    size_t launches = 4 + 15 * size_t(rounds(c));
+    size_t launches = 4 + 17 * size_t(rounds(c));
     std::vector<hipEvent_t> ev(2 * launches);
     for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
     for (uint32_t s = 0; s < samples; ++s) {
@@ -621,7 +622,7 @@ int pt_time_kernels(pt_ctx* c, uint32_t samples, double* ms, size_t n) {
             hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]);
             int k = kinds[i];
             int slot = k == K_GENRAYS ? 0 : k == K_MARCH ? 1 : k == K_SHADE ? 2 : k == K_OCCLUDE ? 3 : k == K_RESOLVE ? 4
-                     : (k == K_SPPMCLEAR || k == K_PHOTONGEN) ? 6 : 5;
+                     : (k == K_SPPMCLEAR || k == K_PHOTONGEN) ? 6 : k == K_SURFACE ? 7 : 5;
             if (slot >= (int)n) continue;
             ms[slot] += t;
         }

@@ -275,9 +275,9 @@ class Engine:
         self._chk(self.lib.pt_reset_stats(self.h))
 
     def time_kernels(self, samples=1):
-        ms = (ctypes.c_double * 7)()
-        self._chk(self.lib.pt_time_kernels(self.h, samples, ms, 7))
-        return dict(zip(['genrays', 'march', 'shade', 'occlude', 'resolve', 'bin', 'photon'], list(ms)))
+        ms = (ctypes.c_double * 8)()
+        self._chk(self.lib.pt_time_kernels(self.h, samples, ms, 8))
+        return dict(zip(['genrays', 'march', 'shade', 'occlude', 'resolve', 'bin', 'photon', 'surface'], list(ms)))
 
     def pick_rays(self):
         """Each camera ray of one sample to its first event: (pos4 [n, 4], state [n], material [n], prim [n])."""
