@@ -51,7 +51,7 @@ typedef struct pt_scene_desc {
 int pt_abi_version(void);
 
 /* sizeof of the ABI's structures, for bindings to check their mirrors: 0 PtUniforms, 1 PtMaterial,
- * 2 PtLightFace, 3 PtCamera, 4 PtEnv, 5 PtHaze, 6 pt_scene_desc, 7 pt_opts; 0 for anything else.
+ * 2 PtLightFace, 3 PtCamera, 4 PtEnv, 5 PtHaze, 6 pt_scene_desc, 7 pt_opts, 8 PtGuideConfig; 0 for anything else.
  * No counterpart in the reference (its uniform packing lives in the absent tinywebgpu submodule). */
 size_t pt_struct_size(int which);
 
@@ -115,6 +115,25 @@ int pt_time_kernels(pt_ctx* ctx, uint32_t samples, double* ms, size_t n);
  * the camera rays of a sample (width*height). The sample counter is not advanced. Replaces R.pick
  * (lib/pathtracer/core/pick.js) for every pixel at once. */
 int pt_pick_rays(pt_ctx* ctx, float* pos4, uint32_t* info, size_t n);
+
+/* ---- path guiding (no counterpart in the reference: its guiding was removed, lib/pathtracer/README.md:853-861;
+ * the engine's uniforms switch it on: `guide`, and `guideFrac` the share of guided directions) ---- */
+
+/* Sets the guide network's training (PtGuideConfig; size = sizeof(PtGuideConfig)) and starts it afresh: weights
+ * drawn from cfg->seed (He-uniform, biases 0), Adam's moments and step count zeroed. batch: a multiple of 64 in
+ * [64, 2^20]; steps in [1, 64]; lr > 0; the box not empty. */
+int pt_guide_config(pt_ctx* ctx, const PtGuideConfig* cfg, size_t size);
+/* The network's FP32 master weights, 16104 floats in the order W1[80][64], b1, W2[64][64], b2, W3, b3, W4[64][40],
+ * b4: read (pt_guide_weights) or replaced, with Adam's state reset (pt_guide_load). */
+int pt_guide_weights(pt_ctx* ctx, float* theta, size_t n);
+int pt_guide_load(pt_ctx* ctx, const float* theta, size_t n);
+/* The network's forward pass on given vertices: raw 10 floats a row (position, view direction, normal,
+ * roughness) in, 40 head outputs a row out (mixture logits, 8 x 3 mean vectors, 8 log-concentrations). */
+int pt_guide_forward(pt_ctx* ctx, const float* raw, size_t n, float* out40);
+/* One Adam step on given training samples, 16 floats a row: position mapped into the box (3), roughness, view
+ * direction (3), weight F, normal (3), 0, sampled direction (3), 0. n in [1, batch]. grad (NULL or 16104 floats)
+ * receives the summed gradient the step applied; loss (NULL or 1 double) the batch's loss. */
+int pt_guide_train(pt_ctx* ctx, const float* samples, size_t n, float* grad, double* loss);
 
 #ifdef __cplusplus
 }

@@ -31,6 +31,8 @@
 #define PT_C_SAMPLE 6      // samples resolved since the accumulators were cleared (drives the per-sample seed)
 #define PT_C_PHOTON 7      // photons stored in the SPPM map this pass
 #define PT_C_RADIUS 8      // this sample's SPPM gather radius r_k (float bits), set by the photon pass
+#define PT_C_GUIDE_N 9     // guide training samples collected this sample
+#define PT_C_GUIDE_T 10    // Adam steps the guide network has taken
 #define PT_SPPM_HEADS (1u << 20)   // the photon map's hash-grid lists (the reference's SPPM_HEADS, sppm.js:79)
 #define PT_BINS 1024          // bins of the march queue's counting sort by node mask
 #define PT_BIN_BLOCKS 256     // blocks of that sort (each sorts one contiguous chunk of the queue)
@@ -56,7 +58,10 @@
 #define PT_S_GATHERCUT 13    // SPPM gathers that reached the bound on records visited
 #define PT_S_RADIUS 14       // the last photon pass's gather radius r_k (float bits; set, not summed)
 #define PT_S_OVERFLOW 15     // SPPM stores dropped for want of room (the map holds 2 records a photon)
-#define PT_S_WORDS 16
+#define PT_S_GSTEPS 16       // guide network: Adam steps taken
+#define PT_S_GUIDED 17       // path vertices sampled with guiding on
+#define PT_S_GRECORDS 18     // path vertices recorded for training
+#define PT_S_WORDS 24
 
 typedef struct PtCamera {
     float pos[3];
@@ -124,6 +129,20 @@ typedef struct PtHaze {
 } PtHaze;
 
 // The engine's uniforms: the subset of the reference's schema (core/uniforms.schema.js:37-418) this path reads.
+// The path-guiding network's training (set with pt_guide_config): the box vertex positions are encoded in, Adam's
+// learning rate, samples a step and steps a sample, which paths record their vertices, the warm-up before guided
+// sampling starts and the weights' initialisation seed.
+typedef struct PtGuideConfig {
+    float boxMin[3];
+    float lr;              // 1e-2
+    float boxMax[3];
+    unsigned int batch;    // training samples an Adam step (2^16)
+    unsigned int steps;    // Adam steps a sample (each on its own batch)
+    unsigned int stride;   // one camera path in `stride` records (0: chosen from batch x steps and the image)
+    unsigned int warmup;   // Adam steps before guided sampling starts
+    unsigned int seed;     // weights' initialisation
+} PtGuideConfig;
+
 typedef struct PtUniforms {
     unsigned int width, height;
     unsigned int maxbounce;     // 10

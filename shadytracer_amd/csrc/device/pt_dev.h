@@ -53,8 +53,21 @@ struct PtDev {
     unsigned int* sppmNext;         // 2 words a photon: its successor in its cell's list and in its sub-list
     unsigned int sppmCap;           // records the map holds
     unsigned int nphotons;          // photons emitted in a pass (slots [0, nphotons) of the path state)
-    // guiding
-    const void* guideNet;
-    float* guideBuf;
-    unsigned int guideCap, guidePad;
+    // path guiding (pt_guide.h)
+    PtGuideConfig g;
+    float* gMaster;        // the network's FP32 master weights (16104)
+    float* gM;             // Adam's moments
+    float* gV;
+    float* gPart;          // PT_G_PARTS rows of partial gradients
+    float* gGradOut;       // NULL, or 16104: the last step's summed gradient
+    unsigned short* gW;    // the BF16 copy, each layer transposed, for the MFMA
+    float* gBias;          // 256: b1, b2, b3, b4 (padded)
+    float* gMix;           // per slot, 10 float4: the vertex's mixture
+    float* gRec;           // recorded vertices: [path][vertex], 7 float4
+    float* gPath;          // per recorded path, 2 float4: its total (rgb, done), its light sampling's (rgb)
+    unsigned int* gVerts;  // per recorded path: its vertices recorded
+    float* gBatch;         // training samples, 4 float4 each
+    double* gSumF;         // per step: the sum of the training weights of its batch
+    double* gLoss;         // per step: the batch's loss
+    unsigned int gStride, gPaths, gMaxV, gBatchCap;
 };

@@ -33,6 +33,10 @@ __device__ __forceinline__ unsigned long long waveSum64(unsigned long long v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
     return v;
 }
+__device__ __forceinline__ float waveSum64f(float v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
 __device__ __forceinline__ void statAdd(const PtDev* __restrict__ D, int which, unsigned long long v) {
     v = waveSum64(v);
     if (laneId() == 0 && v) atomicAdd(&D->stats[which], v);
@@ -108,6 +112,9 @@ __device__ __forceinline__ float maskMargin(const PtDev* __restrict__ D) {
 #define PT_F_CAUSTIC 0x10u      // a camera path that gathered: light reaching its gather vertex through delta
                                 // events only is the photon map's
 #define PT_F_SDELTA 0x20u       // ... and it has scattered by a delta event since that vertex
+#define PT_F_GPEND 0x40u        // a recorded guide vertex waits for its S (the radiance so far, at the next event)
+#define PT_HR_INSIDE 0x100u     // hit record: the ray is inside the surface's object
+#define PT_HR_BEYOND 0x200u     // hit record: the medium beyond is the material in bits 16..
 __device__ __forceinline__ unsigned int mzBounces(unsigned int z) { return z & 0xFFu; }
 __device__ __forceinline__ unsigned int mzState(unsigned int z) { return (z >> 8) & 0xFFu; }
 __device__ __forceinline__ unsigned int mzFlags(unsigned int z) { return z >> 16; }

@@ -9,7 +9,18 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_genrays(const PtDev* _
     unsigned int* C = D->counters;
     if (i == 0) { C[PT_C_MARCH0] = D->nrays; C[PT_C_MARCH1] = 0u; C[PT_C_SHADOW] = 0u; }
     if (i < 2u * PT_PARTS) C[PT_C_HEADS_MARCH + i] = 0u;
+    if (D->u.guide != 0u) {
+        // the guide's training data starts empty each sample
+        if (i == 0) C[PT_C_GUIDE_N] = 0u;
+        if (i < D->g.steps) { D->gSumF[i] = 0.0; D->gLoss[i] = 0.0; }
+    }
     if (i >= D->nrays) return;
+    unsigned int gr;
+    if (guideRecorded(D, i, gr)) {
+        D->gVerts[gr] = 0u;
+        reinterpret_cast<float4*>(D->gPath)[2 * gr] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        reinterpret_cast<float4*>(D->gPath)[2 * gr + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
     unsigned int seed[4];
     sampleSeed(D, seed);
     Rng r = rngAt(seed, i, 0u, 0xCA3E8Au);
@@ -230,9 +241,7 @@ __device__ __forceinline__ V3 normalAt(ParamPtr U, V3 p, float h) {
 // full-SDF look at the hit -- its normal (the SDF's own, else a tetrahedral difference), whether the ray is inside,
 // its material -- and, for a transmissive material left from inside, the medium a hair beyond (the innermost
 // object there, or air). Written as a 16-byte hit record: the normal, then material | inside << 8 | beyond-known
-// << 9 | beyond material << 16. Shade reads it; the guide network reads it too.
-#define PT_HR_INSIDE 0x100u
-#define PT_HR_BEYOND 0x200u
+// << 9 | beyond material << 16 (PT_HR_*). Shade reads it; the guide network reads it too.
 extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_surface(const PtDev* __restrict__ D, unsigned int qsel) {
     const unsigned int count = D->counters[qsel ? PT_C_MARCH0 : PT_C_MARCH1];
     ParamPtr U = (const PT_CONST float*)D->params;
@@ -329,6 +338,17 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
     const PtEnv& env = D->u.env;
     const bool photon = (flags & PT_F_PHOTON) != 0u;
     out.photon = photon;
+    // guide training: a recorded path's previous vertex learns S, the radiance the path had when this event came
+    unsigned int gr = 0u;
+    const bool grec = !photon && guideRecorded(D, slot, gr);
+    if (grec && (flags & PT_F_GPEND)) {
+        const float4 nee = reinterpret_cast<const float4*>(D->gPath)[2 * gr + 1];
+        const unsigned int k = D->gVerts[gr] - 1u;
+        float4* rec = guideRec(D, gr, k);
+        rec[6] = make_float4(rad.x + nee.x, rad.y + nee.y, rad.z + nee.z, 0.0f);
+        rec[4].w = 1.0f;
+        flags &= ~PT_F_GPEND;
+    }
     // photons draw from their own stream, so the map and the camera paths of one sample are independent
     Rng rng = rngAt(seed, slot, bounces, (0x5AADEu ^ ((counts & 0xFFFFu) << 12)) ^ (photon ? 0x80000000u : 0u));
     bool terminate = false;
@@ -464,6 +484,13 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
                 Lobes lb = makeLobes(nrm, -dir, m, n0 / n1, D->mf);
                 float fs = faceShareAt(D, bounces);
                 bool neeOn = D->u.nee != 0u && !photon;
+                // PATH GUIDING: where it applies and the network is trained, directions come from its mixture with
+                // probability gc, from the BSDF otherwise; every density below is the two strategies' mixture
+                const bool gElig = !photon && guideMaterial(m);
+                const bool guided = gElig && guideActive(D);
+                const float gc = guided ? D->u.guideFrac : 0.0f;
+                GuideMix gmix;
+                if (guided) gmix = guideLoadMix(D, slot);
                 bool gathered = false;
                 // a surface with a lobe the map can be gathered through (any but a purely delta one)
                 const bool evaluable = !lobesNone(m);
@@ -514,6 +541,7 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
                     if (pS > 0.0f && max3(le) > 0.0f) {
                         V3 f = lobesEval(lb, D->mf, L);
                         float pB = lobesPdf(lb, L);
+                        if (guided) pB = gc * guidePdf(gmix, L) + (1.0f - gc) * pB;
                         float NoL = dot(nrm, L);
                         V3 contrib = beta * f * le * (NoL * misWeight(pS, pB) / pS);
                         if (max3(contrib) > 1e-7f) {
@@ -526,7 +554,24 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
                         }
                     }
                 }
+                const V3 wi = dir;
                 LobeSample s = lobesSample(lb, D->mf, rngNext(rng));
+                float pTot = -1.0f;    // the sampled direction's density (non-delta): the BSDF's, or the mixture's
+                if (guided) {
+                    const V4 ug = rngNext(rng);
+                    if (ug.w < gc) {
+                        s.dir = guideSample(gmix, ug);
+                        s.isDelta = false; s.transmitted = false; s.ev = PT_EV_EVAL;
+                    }
+                    if (s.isDelta) {
+                        s.w = s.w * (1.0f / (1.0f - gc));
+                    } else {
+                        pTot = gc * guidePdf(gmix, s.dir) + (1.0f - gc) * lobesPdf(lb, s.dir);
+                        const float cosO = dot(nrm, s.dir);
+                        s.w = (pTot > 0.0f && cosO > 0.0f) ? lobesEval(lb, D->mf, s.dir) * (cosO / pTot) : v3s(0.0f);
+                    }
+                    atomicAdd(&D->stats[PT_S_GUIDED], 1ull);
+                }
                 if (photon && !s.isDelta) terminate = true;     // the map holds caustics: L S+ D only
                 dir = s.dir;
                 beta *= s.w;
@@ -538,7 +583,7 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
                 }
                 if (!(max3(s.w) > 0.0f)) terminate = true;
                 bool misOut = neeOn && !s.isDelta;
-                float pOut = misOut ? lobesPdf(lb, dir) : 0.0f;
+                float pOut = misOut ? (pTot >= 0.0f ? pTot : lobesPdf(lb, dir)) : 0.0f;
                 bool above = dot(nrm, dir) > 0.0f && !s.isDelta;
                 if (misOut) {
                     float pLout = above && envLit(env) ? (1.0f - fs) * envPdf(env, dir, nrm) : 0.0f;
@@ -564,6 +609,23 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
                     float pr = fminf(fmaxf(max3(beta), 0.05f), 0.95f);
                     if (ud.x > pr) terminate = true; else beta *= 1.0f / pr;
                 }
+                if (!terminate && grec && gElig && !s.isDelta && s.ev == PT_EV_EVAL && D->gVerts[gr] < D->gMaxV) {
+                    // a training vertex: where, the view, the density it sampled w with, the normal, w, f cos at w,
+                    // the throughput after it; S comes at the next event
+                    const float pw = pTot >= 0.0f ? pTot : lobesPdf(lb, dir);
+                    const V3 fc = lobesEval(lb, D->mf, dir) * fmaxf(dot(nrm, dir), 0.0f);
+                    const unsigned int k = D->gVerts[gr];
+                    float4* rec = guideRec(D, gr, k);
+                    rec[0] = make_float4(p.x, p.y, p.z, m.roughness);
+                    rec[1] = make_float4(-wi.x, -wi.y, -wi.z, pw);
+                    rec[2] = make_float4(nrm.x, nrm.y, nrm.z, 0.0f);
+                    rec[3] = make_float4(dir.x, dir.y, dir.z, 0.0f);
+                    rec[4] = make_float4(fc.x, fc.y, fc.z, 0.0f);
+                    rec[5] = make_float4(beta.x, beta.y, beta.z, 0.0f);
+                    D->gVerts[gr] = k + 1u;
+                    flags |= PT_F_GPEND;
+                    atomicAdd(&D->stats[PT_S_GRECORDS], 1ull);
+                }
                 if (!terminate) {
                     float off = offsetTol(D, p);
                     p = p + dir * off; ltot += off * n;
@@ -576,6 +638,7 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
     }
     if (terminate) {
         out.requeue = false;
+        if (grec) reinterpret_cast<float4*>(D->gPath)[2 * gr] = make_float4(rad.x, rad.y, rad.z, 1.0f);
         if (!photon) {
             if (badDeposit(rad)) atomicAdd(&D->stats[PT_S_NONFINITE], 1ull);
             else depositRGB(D, pix, rad, direct);
@@ -716,6 +779,12 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_occlude(const PtDev* _
                     V3 c = v3(Rr.x, Rr.y, Rr.z);
                     if (hazeOn(D)) c *= hazeTr(D->u.haze, o, dir, tmax < 3.0e38f ? tmax : 1.0e30f);
                     depositRGB(D, __float_as_uint(Rr.w), c, false);
+                    unsigned int gr;
+                    if (guideRecorded(D, (unsigned int)slot, gr)) {
+                        // the path's light-sampling total, for its vertices' training radiance
+                        float4* nee = reinterpret_cast<float4*>(D->gPath) + 2 * gr + 1;
+                        *nee = make_float4(nee->x + c.x, nee->y + c.y, nee->z + c.z, 0.0f);
+                    }
                 }
                 slot = -1;
             }

@@ -8,6 +8,7 @@
 #include <sys/stat.h>
 #include <unistd.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -24,9 +25,13 @@ namespace {
 #include "pt_device_src.inc"   // kPtDevicePre, kPtDevicePost: the device sources as strings
 
 const char* kKernelNames[] = {"pt_genrays", "pt_march", "pt_shade", "pt_occlude", "pt_resolve", "pt_mf_lut", "pt_mf_lut_avg", "pt_pick",
-                              "pt_bin_count", "pt_bin_scan", "pt_bin_scatter", "pt_sppm_clear", "pt_photon_gen", "pt_surface"};
+                              "pt_bin_count", "pt_bin_scan", "pt_bin_scatter", "pt_sppm_clear", "pt_photon_gen", "pt_surface",
+                              "pt_guide_infer", "pt_guide_eval", "pt_guide_collect", "pt_guide_grad", "pt_guide_adam", "pt_guide_commit"};
 enum { K_GENRAYS, K_MARCH, K_SHADE, K_OCCLUDE, K_RESOLVE, K_MFLUT, K_MFAVG, K_PICK, K_BINCOUNT, K_BINSCAN, K_BINSCATTER,
-       K_SPPMCLEAR, K_PHOTONGEN, K_SURFACE, K_COUNT };
+       K_SPPMCLEAR, K_PHOTONGEN, K_SURFACE, K_GINFER, K_GEVAL, K_GCOLLECT, K_GGRAD, K_GADAM, K_GCOMMIT, K_COUNT };
+
+// the guide network's sizes (device/pt_guide.h)
+const unsigned int kGuideNP = 16104, kGuideWT = 17408, kGuideBias = 256, kGuideParts = 256, kGuideMaxSteps = 64;
 
 thread_local std::string g_createError;
 
@@ -163,6 +168,10 @@ struct pt_ctx {
 
     unsigned int sppmCapacity = 0;   // photon records allocated
 
+    bool guideReady = false;          // guide network allocated and initialised
+    unsigned int guideMixSlots = 0;   // slots gMix holds
+    unsigned int guideRecPaths = 0, guideRecVerts = 0, guideBatchCap = 0;
+
     std::vector<void*> owned;
 };
 
@@ -286,8 +295,9 @@ unsigned int rounds(const pt_ctx* c) { return c->host.u.maxbounce + 1u; }
 hipError_t launch(pt_ctx* c, int k, unsigned int grid, unsigned int qsel) {
     void* args1[] = {&c->dev};
     void* args2[] = {&c->dev, &qsel};
-    bool two = (k == K_MARCH || k == K_SHADE || k == K_BINCOUNT || k == K_BINSCATTER || k == K_OCCLUDE || k == K_SURFACE);
-    unsigned int block = k == K_BINSCAN ? PT_BINS : 256;
+    bool two = (k == K_MARCH || k == K_SHADE || k == K_BINCOUNT || k == K_BINSCATTER || k == K_OCCLUDE || k == K_SURFACE ||
+                k == K_GINFER || k == K_GGRAD || k == K_GADAM || k == K_GCOMMIT);
+    unsigned int block = k == K_BINSCAN ? PT_BINS : k == K_GINFER ? 512u : (k == K_GGRAD || k == K_GCOMMIT) ? 64u : 256u;
     return hipModuleLaunchKernel(c->fn[k], grid, 1, 1, block, 1, 1, 0, c->stream, two ? args2 : args1, nullptr);
 }
 
@@ -299,6 +309,11 @@ unsigned int gridFor(const pt_ctx* c, int k) {
         case K_BINSCAN: return 1;
         case K_SPPMCLEAR: return 3u * PT_SPPM_HEADS / 256u;
         case K_PHOTONGEN: return (c->host.nphotons + 255u) / 256u;
+        case K_GINFER: return c->cus;
+        case K_GCOLLECT: return (c->host.gPaths * c->host.gMaxV + 255u) / 256u;
+        case K_GGRAD: return kGuideParts;
+        case K_GADAM: return (kGuideNP + 255u) / 256u;
+        case K_GCOMMIT: return 1u;
         default: return c->host.gridBlocks;
     }
 }
@@ -313,6 +328,7 @@ hipError_t enqueueSample(pt_ctx* c, std::vector<hipEvent_t>* events, std::vector
         return r;
     };
     unsigned int R = rounds(c);
+    const bool guiding = c->host.u.guide != 0u && c->guideReady;
     if (c->host.nphotons) {
         // THE PHOTON PASS: the map cleared and r_k set, photons emitted, marched and shaded round by round (no shadow
         // rays; occlude runs on an empty queue for its resets of the march queue)
@@ -327,17 +343,27 @@ hipError_t enqueueSample(pt_ctx* c, std::vector<hipEvent_t>* events, std::vector
     for (unsigned int it = 0; it < R; ++it) {
         unsigned int in = it & 1u;
         if ((e = one(K_BINCOUNT, in)) || (e = one(K_BINSCAN, 0)) || (e = one(K_BINSCATTER, in))) return e;
-        if ((e = one(K_MARCH, in))) return e;
-        if ((e = one(K_SURFACE, in ^ 1u)) || (e = one(K_SHADE, in ^ 1u))) return e;
+        if ((e = one(K_MARCH, in)) || (e = one(K_SURFACE, in ^ 1u))) return e;
+        if (guiding && (e = one(K_GINFER, in ^ 1u))) return e;
+        if ((e = one(K_SHADE, in ^ 1u))) return e;
         if ((e = one(K_BINCOUNT, 2)) || (e = one(K_BINSCAN, 0)) || (e = one(K_BINSCATTER, 2))) return e;
         if ((e = one(K_OCCLUDE, in))) return e;
     }
-    return one(K_RESOLVE, 0);
+    if ((e = one(K_RESOLVE, 0))) return e;
+    if (guiding) {
+        // THE GUIDE'S TRAINING: the sample's recorded vertices into samples, then its Adam steps
+        if ((e = one(K_GCOLLECT, 0))) return e;
+        for (unsigned int j = 0; j < c->host.g.steps; ++j)
+            if ((e = one(K_GGRAD, j)) || (e = one(K_GADAM, j)) || (e = one(K_GCOMMIT, j))) return e;
+    }
+    return hipSuccess;
 }
 
 uint64_t graphKey(const pt_ctx* c) {
-    return (uint64_t(rounds(c)) << 56) ^ (uint64_t(c->host.nrays) << 28) ^ uint64_t(c->host.nphotons) ^
-           (uint64_t(c->host.gridBlocks) << 48);
+    // what the captured launches were sized for
+    const unsigned int k[] = {rounds(c), c->host.nrays, c->host.nphotons, c->host.gridBlocks,
+                              (c->host.u.guide && c->guideReady) ? 1u : 0u, c->host.g.steps, c->host.gPaths, c->host.gMaxV};
+    return fnv1a(k, sizeof k);
 }
 
 int buildGraph(pt_ctx* c) {
@@ -355,6 +381,126 @@ int buildGraph(pt_ctx* c) {
     HIPCHK(c, hipGraphInstantiate(&c->graphExec, g, nullptr, nullptr, 0));
     c->graphKey = graphKey(c);
     return PT_OK;
+}
+
+// ---- the guide network's host side
+uint16_t bf16Bits(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return 0x7FC0u;
+    u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+
+// He-uniform weights from a PCG32 stream of the seed (biases 0; the head's layer at a tenth, so the first mixture is
+// broad), the engine's parameter order.
+void guideInit(uint32_t seed, std::vector<float>& th) {
+    th.assign(kGuideNP, 0.0f);
+    uint64_t state = 0x853C49E6748FEA9Bull ^ (uint64_t(seed) << 1), inc = (uint64_t(seed) << 1) | 1u;
+    auto next = [&]() {
+        uint64_t old = state;
+        state = old * 6364136223846793005ull + inc;
+        uint32_t x = (uint32_t)(((old >> 18u) ^ old) >> 27u), r = (uint32_t)(old >> 59u);
+        return ((x >> r) | (x << ((32u - r) & 31u))) * (1.0 / 4294967296.0);
+    };
+    const unsigned int sizes[5] = {80, 64, 64, 64, 40};
+    size_t o = 0;
+    for (int L = 0; L < 4; ++L) {
+        const double lim = std::sqrt(6.0 / sizes[L]) * (L == 3 ? 0.1 : 1.0);
+        for (unsigned int k = 0; k < sizes[L] * sizes[L + 1]; ++k) th[o++] = (float)((2.0 * next() - 1.0) * lim);
+        o += sizes[L + 1];
+    }
+}
+
+// Master weights in: Adam's state cleared, the BF16 transposed copy and the biases written, the step count zeroed.
+int guideUpload(pt_ctx* c, const float* th) {
+    std::vector<uint16_t> w(kGuideWT, 0);
+    std::vector<float> b(kGuideBias, 0.0f);
+    const unsigned int sizes[5] = {80, 64, 64, 64, 40}, wt[4] = {0, 5120, 9216, 13312};
+    size_t o = 0;
+    for (int L = 0; L < 4; ++L) {
+        const unsigned int in = sizes[L], out = sizes[L + 1];
+        for (unsigned int i = 0; i < in; ++i)
+            for (unsigned int j = 0; j < out; ++j) w[wt[L] + j * in + i] = bf16Bits(th[o + i * out + j]);
+        o += in * out;
+        for (unsigned int j = 0; j < out; ++j) b[64 * L + j] = th[o + j];
+        o += out;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->host.gMaster, th, kGuideNP * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->host.gM, 0, kGuideNP * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->host.gV, 0, kGuideNP * 4, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->host.gW, w.data(), kGuideWT * 2, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->host.gBias, b.data(), kGuideBias * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->host.counters + PT_C_GUIDE_T, 0, 4, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PT_OK;
+}
+
+// The guide's buffers for the current image and settings: the network's (once), the mixtures by slot, the recorded
+// paths' vertices, the training samples.
+int ensureGuide(pt_ctx* c) {
+    int rc;
+    PtDev& h = c->host;
+    if (!h.gMaster) {
+        if ((rc = allocBuf(c, h.gMaster, kGuideNP * 4)) || (rc = allocBuf(c, h.gM, kGuideNP * 4)) ||
+            (rc = allocBuf(c, h.gV, kGuideNP * 4)) || (rc = allocBuf(c, h.gPart, size_t(kGuideParts) * kGuideNP * 4)) ||
+            (rc = allocBuf(c, h.gW, kGuideWT * 2)) || (rc = allocBuf(c, h.gBias, kGuideBias * 4)) ||
+            (rc = allocBuf(c, h.gSumF, kGuideMaxSteps * 8)) || (rc = allocBuf(c, h.gLoss, kGuideMaxSteps * 8)))
+            return rc;
+        HIPCHK(c, hipMemsetAsync(h.gSumF, 0, kGuideMaxSteps * 8, c->stream));
+        HIPCHK(c, hipMemsetAsync(h.gLoss, 0, kGuideMaxSteps * 8, c->stream));
+        std::vector<float> th;
+        guideInit(h.g.seed, th);
+        if ((rc = guideUpload(c, th.data()))) return rc;
+    }
+    if (c->capacity > c->guideMixSlots) {
+        destroyGraph(c);
+        void* p = h.gMix; freeBuf(c, p); h.gMix = nullptr;
+        if ((rc = allocBuf(c, h.gMix, size_t(c->capacity) * 160))) return rc;
+        c->guideMixSlots = c->capacity;
+    }
+    // which paths record: about 4 training samples a recorded path, batch x steps of them a sample
+    const unsigned long long want = (unsigned long long)h.g.batch * h.g.steps;
+    unsigned int stride = h.g.stride ? h.g.stride : (unsigned int)std::max<unsigned long long>(1ull, (4ull * h.nrays) / std::max(want, 1ull));
+    const unsigned int paths = (h.nrays + stride - 1u) / stride, verts = h.u.maxbounce + 1u;
+    if (paths > c->guideRecPaths || verts > c->guideRecVerts) {
+        destroyGraph(c);
+        void* p;
+        p = h.gRec; freeBuf(c, p); h.gRec = nullptr;
+        p = h.gPath; freeBuf(c, p); h.gPath = nullptr;
+        p = h.gVerts; freeBuf(c, p); h.gVerts = nullptr;
+        const unsigned int np = std::max(paths, c->guideRecPaths), nv = std::max(verts, c->guideRecVerts);
+        if ((rc = allocBuf(c, h.gRec, size_t(np) * nv * 7 * 16)) || (rc = allocBuf(c, h.gPath, size_t(np) * 32)) ||
+            (rc = allocBuf(c, h.gVerts, size_t(np) * 4)))
+            return rc;
+        HIPCHK(c, hipMemsetAsync(h.gVerts, 0, size_t(np) * 4, c->stream));
+        c->guideRecPaths = np;
+        c->guideRecVerts = nv;
+    }
+    h.gStride = stride;
+    h.gPaths = paths;
+    h.gMaxV = verts;
+    const unsigned int cap = (unsigned int)std::min<unsigned long long>(want, 1ull << 26);
+    if (cap > c->guideBatchCap) {
+        void* p = h.gBatch; freeBuf(c, p); h.gBatch = nullptr;
+        if ((rc = allocBuf(c, h.gBatch, size_t(cap) * 64))) return rc;
+        c->guideBatchCap = cap;
+    }
+    h.gBatchCap = cap;
+    c->guideReady = true;
+    c->devDirty = true;
+    return PT_OK;
+}
+
+void setGuideDefaults(PtGuideConfig& g) {
+    memset(&g, 0, sizeof g);
+    for (int a = 0; a < 3; ++a) { g.boxMin[a] = -10.0f; g.boxMax[a] = 10.0f; }
+    g.lr = 1e-2f;
+    g.batch = 1u << 16;
+    g.steps = 4;
+    g.stride = 0;
+    g.warmup = 16;
+    g.seed = 1;
 }
 
 void setDefaults(PtUniforms& u) {
@@ -390,6 +536,7 @@ size_t pt_struct_size(int which) {
         case 5: return sizeof(PtHaze);
         case 6: return sizeof(pt_scene_desc);
         case 7: return sizeof(pt_opts);
+        case 8: return sizeof(PtGuideConfig);
         default: return 0;
     }
 }
@@ -428,6 +575,7 @@ int pt_create(pt_ctx** out, const pt_opts* opts) {
     c->cus = cus > 0 ? (unsigned int)cus : 256u;
     c->blocksPerCu = (opts && opts->blocks_per_cu) ? opts->blocks_per_cu : 8u;
     setDefaults(c->host.u);
+    setGuideDefaults(c->host.g);
     c->host.gridBlocks = c->cus * c->blocksPerCu;
     int rc;
     if ((rc = allocBuf(c, c->dev, sizeof(PtDev))) || (rc = allocBuf(c, c->host.counters, PT_C_WORDS * 4)) ||
@@ -551,6 +699,7 @@ int pt_set_uniforms(pt_ctx* c, const PtUniforms* u, size_t size) {
     c->host.nrays = N;
     c->host.nphotons = photons;
     c->devDirty = true;
+    if (u->guide && (rc = ensureGuide(c))) return rc;
     if (resized) return pt_clear_accum(c);
     return PT_OK;
 }
@@ -560,7 +709,9 @@ int pt_clear_accum(pt_ctx* c) {
     HIPCHK(c, hipMemsetAsync(c->host.frame, 0, size_t(c->pixels) * 16, c->stream));
     HIPCHK(c, hipMemsetAsync(c->host.accum, 0, size_t(c->pixels) * 32, c->stream));
     HIPCHK(c, hipMemsetAsync(c->host.moments, 0, size_t(c->pixels) * 16, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->host.counters, 0, PT_C_WORDS * 4, c->stream));
+    // every counter but the guide network's step count (its training outlives the picture)
+    HIPCHK(c, hipMemsetAsync(c->host.counters, 0, PT_C_GUIDE_T * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->host.counters + PT_C_GUIDE_T + 1, 0, (PT_C_WORDS - PT_C_GUIDE_T - 1) * 4, c->stream));
     return PT_OK;
 }
 
@@ -609,7 +760,7 @@ int pt_time_kernels(pt_ctx* c, uint32_t samples, double* ms, size_t n) {
     int rc;
     if ((rc = syncDev(c))) return rc;
     for (size_t k = 0; k < n; ++k) ms[k] = 0.0;
-    size_t launches = 4 + 17 * size_t(rounds(c));
+    size_t launches = 6 + 18 * size_t(rounds(c)) + 3 * size_t(kGuideMaxSteps);
     std::vector<hipEvent_t> ev(2 * launches);
     for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
     for (uint32_t s = 0; s < samples; ++s) {
@@ -622,7 +773,8 @@ int pt_time_kernels(pt_ctx* c, uint32_t samples, double* ms, size_t n) {
             hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]);
             int k = kinds[i];
             int slot = k == K_GENRAYS ? 0 : k == K_MARCH ? 1 : k == K_SHADE ? 2 : k == K_OCCLUDE ? 3 : k == K_RESOLVE ? 4
-                     : (k == K_SPPMCLEAR || k == K_PHOTONGEN) ? 6 : k == K_SURFACE ? 7 : 5;
+                     : (k == K_SPPMCLEAR || k == K_PHOTONGEN) ? 6 : k == K_SURFACE ? 7
+                     : (k == K_GINFER || k == K_GCOLLECT || k == K_GGRAD || k == K_GADAM || k == K_GCOMMIT) ? 8 : 5;
             if (slot >= (int)n) continue;
             ms[slot] += t;
         }
@@ -652,6 +804,97 @@ int pt_pick_rays(pt_ctx* c, float* pos4, uint32_t* info, size_t n) {
     HIPCHK(c, hipMemcpy(info, dinfo, n * 8, hipMemcpyDeviceToHost));
     void* p1 = dpos; freeBuf(c, p1);
     void* p2 = dinfo; freeBuf(c, p2);
+    return PT_OK;
+}
+
+int pt_guide_config(pt_ctx* c, const PtGuideConfig* g, size_t size) {
+    if (!c || !g) return fail(c, PT_ERR_ARG, "config is NULL");
+    if (size != sizeof(PtGuideConfig)) return fail(c, PT_ERR_ARG, "config: %zu bytes, expected %zu", size, sizeof(PtGuideConfig));
+    if (g->batch < 64 || g->batch > (1u << 20) || g->batch % 64) return fail(c, PT_ERR_ARG, "batch %u: a multiple of 64 in [64, 2^20]", g->batch);
+    if (g->steps < 1 || g->steps > kGuideMaxSteps) return fail(c, PT_ERR_ARG, "steps %u not in 1..%u", g->steps, kGuideMaxSteps);
+    if (!(g->lr > 0.0f)) return fail(c, PT_ERR_ARG, "lr %g must be positive", g->lr);
+    for (int a = 0; a < 3; ++a)
+        if (!(g->boxMax[a] > g->boxMin[a])) return fail(c, PT_ERR_ARG, "the box is empty on axis %d", a);
+    hipStreamSynchronize(c->stream);
+    destroyGraph(c);
+    c->host.g = *g;
+    c->devDirty = true;
+    int rc;
+    if ((rc = ensureGuide(c))) return rc;
+    std::vector<float> th;
+    guideInit(g->seed, th);
+    return guideUpload(c, th.data());
+}
+
+int pt_guide_weights(pt_ctx* c, float* theta, size_t n) {
+    if (!c || !theta || n != kGuideNP) return fail(c, PT_ERR_ARG, "theta: NULL or n != %u", kGuideNP);
+    if (!c->guideReady) return fail(c, PT_ERR_STATE, "the guide network is not set up (pt_guide_config or uniforms.guide)");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(theta, c->host.gMaster, kGuideNP * 4, hipMemcpyDeviceToHost));
+    return PT_OK;
+}
+
+int pt_guide_load(pt_ctx* c, const float* theta, size_t n) {
+    if (!c || !theta || n != kGuideNP) return fail(c, PT_ERR_ARG, "theta: NULL or n != %u", kGuideNP);
+    int rc;
+    if (!c->guideReady && (rc = ensureGuide(c))) return rc;
+    return guideUpload(c, theta);
+}
+
+int pt_guide_forward(pt_ctx* c, const float* raw, size_t n, float* out40) {
+    if (!c || !raw || !out40 || n == 0 || n > (1u << 24)) return fail(c, PT_ERR_ARG, "raw/out40 NULL or n not in 1..2^24");
+    if (!c->module) return fail(c, PT_ERR_STATE, "no scene loaded");
+    int rc;
+    if (!c->guideReady && (rc = ensureGuide(c))) return rc;
+    if ((rc = syncDev(c))) return rc;
+    float *din = nullptr, *dout = nullptr;
+    if ((rc = allocBuf(c, din, n * 40)) || (rc = allocBuf(c, dout, n * 160))) return rc;
+    HIPCHK(c, hipMemcpy(din, raw, n * 40, hipMemcpyHostToDevice));
+    unsigned int nn = (unsigned int)n;
+    void* args[] = {&c->dev, &din, &nn, &dout};
+    unsigned int grid = (unsigned int)std::min<size_t>((n + 63) / 64, 4096);
+    HIPCHK(c, hipModuleLaunchKernel(c->fn[K_GEVAL], grid, 1, 1, 64, 1, 1, 0, c->stream, args, nullptr));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(out40, dout, n * 160, hipMemcpyDeviceToHost));
+    void* p1 = din; freeBuf(c, p1);
+    void* p2 = dout; freeBuf(c, p2);
+    return PT_OK;
+}
+
+int pt_guide_train(pt_ctx* c, const float* samples, size_t n, float* grad, double* loss) {
+    if (!c || !samples) return fail(c, PT_ERR_ARG, "samples is NULL");
+    if (!c->module) return fail(c, PT_ERR_STATE, "no scene loaded");
+    int rc;
+    if (!c->guideReady && (rc = ensureGuide(c))) return rc;
+    if (n == 0 || n > c->host.g.batch || n > c->host.gBatchCap) return fail(c, PT_ERR_ARG, "n %zu not in 1..%u", n, c->host.g.batch);
+    if ((rc = syncDev(c))) return rc;
+    double sumF = 0.0;
+    for (size_t i = 0; i < n; ++i) sumF += samples[16 * i + 7];
+    if (!(sumF > 0.0)) return fail(c, PT_ERR_ARG, "the samples' weights sum to %g", sumF);
+    const unsigned int cnt = (unsigned int)n;
+    const double zero = 0.0;
+    float* dgrad = nullptr;
+    if (grad && (rc = allocBuf(c, dgrad, kGuideNP * 4))) return rc;
+    HIPCHK(c, hipMemcpy(c->host.gBatch, samples, n * 64, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->host.counters + PT_C_GUIDE_N, &cnt, 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->host.gSumF, &sumF, 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->host.gLoss, &zero, 8, hipMemcpyHostToDevice));
+    // the gradient's copy goes through the device table for this one step
+    c->host.gGradOut = dgrad;
+    HIPCHK(c, hipMemcpy(c->dev, &c->host, sizeof(PtDev), hipMemcpyHostToDevice));
+    hipError_t e;
+    if ((e = launch(c, K_GGRAD, gridFor(c, K_GGRAD), 0)) || (e = launch(c, K_GADAM, gridFor(c, K_GADAM), 0)) ||
+        (e = launch(c, K_GCOMMIT, gridFor(c, K_GCOMMIT), 0))) {
+        c->host.gGradOut = nullptr;
+        c->devDirty = true;
+        return fail(c, PT_ERR_HIP, "guide step: %s", hipGetErrorString(e));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->host.gGradOut = nullptr;
+    HIPCHK(c, hipMemcpy(c->dev, &c->host, sizeof(PtDev), hipMemcpyHostToDevice));
+    if (grad) HIPCHK(c, hipMemcpy(grad, dgrad, kGuideNP * 4, hipMemcpyDeviceToHost));
+    if (loss) HIPCHK(c, hipMemcpy(loss, c->host.gLoss, 8, hipMemcpyDeviceToHost));
+    if (dgrad) { void* p = dgrad; freeBuf(c, p); }
     return PT_OK;
 }
 

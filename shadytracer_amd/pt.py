@@ -59,12 +59,18 @@ class PtSceneDesc(ctypes.Structure):
                 ('faces', ctypes.POINTER(PtLightFace)), ('nfaces', ctypes.c_uint32)]
 
 
+class PtGuideConfig(ctypes.Structure):
+    _fields_ = [('boxMin', ctypes.c_float * 3), ('lr', ctypes.c_float), ('boxMax', ctypes.c_float * 3),
+                ('batch', ctypes.c_uint), ('steps', ctypes.c_uint), ('stride', ctypes.c_uint), ('warmup', ctypes.c_uint),
+                ('seed', ctypes.c_uint)]
+
+
 class PtOpts(ctypes.Structure):
     _fields_ = [('device', ctypes.c_int), ('blocks_per_cu', ctypes.c_uint32), ('cache_dir', ctypes.c_char_p),
                 ('reserved', ctypes.c_uint32 * 4)]
 
 
-STRUCTS = [PtUniforms, PtMaterial, PtLightFace, PtCamera, PtEnv, PtHaze, PtSceneDesc, PtOpts]
+STRUCTS = [PtUniforms, PtMaterial, PtLightFace, PtCamera, PtEnv, PtHaze, PtSceneDesc, PtOpts, PtGuideConfig]
 
 # the C ABI, name -> (restype, argtypes)
 SIGNATURES = {
@@ -85,11 +91,18 @@ SIGNATURES = {
     'pt_reset_stats': (ctypes.c_int, [ctypes.c_void_p]),
     'pt_time_kernels': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double), ctypes.c_size_t]),
     'pt_pick_rays': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_uint32), ctypes.c_size_t]),
+    'pt_guide_config': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PtGuideConfig), ctypes.c_size_t]),
+    'pt_guide_weights': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_size_t]),
+    'pt_guide_load': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_size_t]),
+    'pt_guide_forward': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_size_t, ctypes.POINTER(ctypes.c_float)]),
+    'pt_guide_train': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_size_t, ctypes.POINTER(ctypes.c_float),
+                                      ctypes.POINTER(ctypes.c_double)]),
 }
+GUIDE_NP = 16104
 
 STAT_NAMES = ['segments', 'shadow_segments', 'steps', 'shadow_steps', 'nonfinite', 'maxiter', 'paths', 'leaving',
               'gathers', 'photons', 'flops', 'shadow_flops', 'photon_segments', 'gather_cut', 'radius_bits',
-              'overflow']
+              'overflow', 'guide_steps', 'guided', 'guide_records']
 LUMA = (0.2126, 0.7152, 0.0722)
 
 _lib = None
@@ -265,8 +278,8 @@ class Engine:
         return acc.reshape(self.height, self.width, 4), mom.reshape(self.height, self.width, 2)
 
     def stats(self):
-        a = (ctypes.c_uint64 * 16)()
-        self._chk(self.lib.pt_read_stats(self.h, a, 16))
+        a = (ctypes.c_uint64 * 24)()
+        self._chk(self.lib.pt_read_stats(self.h, a, 24))
         out = {k: int(a[i]) for i, k in enumerate(STAT_NAMES)}
         out['sppm_radius'] = float(np.array([out['radius_bits'] & 0xFFFFFFFF], np.uint32).view(np.float32)[0])
         return out
@@ -278,6 +291,39 @@ class Engine:
         ms = (ctypes.c_double * 8)()
         self._chk(self.lib.pt_time_kernels(self.h, samples, ms, 8))
         return dict(zip(['genrays', 'march', 'shade', 'occlude', 'resolve', 'bin', 'photon', 'surface'], list(ms)))
+
+    # ---- path guiding
+    def guide_config(self, box, lr=1e-2, batch=1 << 16, steps=4, stride=0, warmup=16, seed=1):
+        g = PtGuideConfig()
+        for a in range(3):
+            g.boxMin[a], g.boxMax[a] = float(box[0][a]), float(box[1][a])
+        g.lr, g.batch, g.steps, g.stride, g.warmup, g.seed = lr, batch, steps, stride, warmup, seed
+        self._chk(self.lib.pt_guide_config(self.h, ctypes.byref(g), ctypes.sizeof(g)))
+
+    def guide_weights(self):
+        th = np.zeros(GUIDE_NP, np.float32)
+        self._chk(self.lib.pt_guide_weights(self.h, th.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), GUIDE_NP))
+        return th
+
+    def guide_load(self, theta):
+        th = np.ascontiguousarray(np.asarray(theta, np.float32))
+        self._chk(self.lib.pt_guide_load(self.h, th.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), th.size))
+
+    def guide_forward(self, raw):
+        raw = np.ascontiguousarray(np.asarray(raw, np.float32).reshape(-1, 10))
+        out = np.zeros((len(raw), 40), np.float32)
+        self._chk(self.lib.pt_guide_forward(self.h, raw.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(raw),
+                                            out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def guide_train(self, samples):
+        """One Adam step on rows of 16 floats; returns (summed gradient, loss)."""
+        smp = np.ascontiguousarray(np.asarray(samples, np.float32).reshape(-1, 16))
+        grad = np.zeros(GUIDE_NP, np.float32)
+        loss = ctypes.c_double(0.0)
+        self._chk(self.lib.pt_guide_train(self.h, smp.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(smp),
+                                          grad.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.byref(loss)))
+        return grad, loss.value
 
     def pick_rays(self):
         """Each camera ray of one sample to its first event: (pos4 [n, 4], state [n], material [n], prim [n])."""
@@ -313,3 +359,22 @@ def image_stats(acc, mom):
     have = acc[..., 3] > 0
     return {'meanLuma': float(L[have].mean()) if have.any() else 0.0, 'samples': float(acc[..., 3].max()),
             'relSE': float(np.nanmean(se[have] / np.maximum(L[have], 1e-30))) if have.any() else math.nan}
+
+
+def scene_box(scene):
+    """The box guiding encodes positions in: the bounding spheres of the scene's bounded top-level nodes (the emitter's
+    program), a little enlarged; the environment sphere when nothing is bounded."""
+    U = scene['params']
+    lo, hi = [np.inf] * 3, [-np.inf] * 3
+    for o in scene['program']:
+        if o['op'] == 'node' and o['bs'] >= 0:
+            c, r = U[o['bs']:o['bs'] + 3], U[o['bs'] + 3]
+            for a in range(3):
+                lo[a] = min(lo[a], c[a] - r)
+                hi[a] = max(hi[a], c[a] + r)
+    if not np.isfinite(lo[0]):
+        R = scene['envRadius']
+        return [-R] * 3, [R] * 3
+    pad = [0.01 * (h - l) + 1e-3 for l, h in zip(lo, hi)]
+    return [l - p for l, p in zip(lo, pad)], [h + p for h, p in zip(hi, pad)]
+
