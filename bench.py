@@ -7,12 +7,18 @@ FP64 accumulators. `value` counts path segments (a camera ray or bounce marched 
 the timed steps; shadow rays are reported beside it.
 
     python bench.py [--gpus 1] [--steps 32] [--warmup 4] [--no-cpu]
+    python bench.py --guide [--scene caustic-c3] [--width 640 --height 360] [--train 300] [--steps 128]
 
-Prints one JSON line. The roofline is the dominant kernel's (sphere tracing: FP32 VALU), from HIP events on the
+Prints one JSON line. With --guide it measures the learned guiding against the plain tracer instead: each runs the
+same scene sample by sample; per-pixel variances come from the samples' own spread (both estimators are unbiased, the
+GPU tests show their means agree), and the seconds each needs to reach 1 % relative RMSE -- RMSE over the image over
+its mean, at the image's pixels and over 4 x 4 bins -- follow as (training time) + var / (0.01 mean)^2 x (seconds a
+sample). The roofline is the dominant kernel's (sphere tracing: FP32 VALU), from HIP events on the
 engine's stream; `cpu_baseline` times the FP64 C oracle on this host's cores on a bounded slice of the same workload.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -71,8 +77,80 @@ def calibration_copy(mib):
     hip.hipFree(b)
 
 
+def bench_guide(a):
+    """Guided against plain on one scene: variance, time a sample, seconds to 1 % relative RMSE."""
+    import numpy as np
+    from shadytracer_amd import pt, scenes
+    made = scenes.emit(a.scene, width=a.width, height=a.height)
+    extra = {}
+    if made['settings'].get('sppm'):
+        # the configuration's photons a pixel, at this image size
+        extra['sppmPhotons'] = max(1 << 14, int(made['settings']['sppmPhotons'] * a.width * a.height / (1280 * 720)))
+    e = pt.Engine()
+    e.set_scene(made['scene'])
+    e.set_uniforms(pt.uniforms_for(made, **extra))
+    e.guide_config(pt.scene_box(made['scene']), batch=1 << 16 if a.width * a.height >= 1 << 20 else 1 << 14, steps=4, warmup=32)
+
+    def run(guided):
+        u = pt.uniforms_for(made, **extra)
+        if guided:
+            u.guide, u.guideFrac = 1, 0.9
+        e.set_uniforms(u)
+        e.clear_accums()
+        t_train = 0.0
+        if guided:
+            t0 = time.perf_counter()
+            e.trace(a.train)
+            e.sync()
+            t_train = time.perf_counter() - t0
+            e.clear_accums()
+        n = a.steps
+        s1 = np.zeros((a.height, a.width)); s2 = np.zeros_like(s1)
+        b1 = np.zeros((a.height // 4, a.width // 4)); b2 = np.zeros_like(b1)
+        prev, t = None, 0.0
+        for k in range(1, n + 1):
+            t0 = time.perf_counter()
+            e.trace(1)
+            e.sync()
+            t += time.perf_counter() - t0
+            Y = pt.luminance(e.read_accum()[0])
+            x = Y if prev is None else k * Y - (k - 1) * prev
+            prev = Y
+            s1 += x; s2 += x * x
+            xb = x[:a.height // 4 * 4, :a.width // 4 * 4].reshape(a.height // 4, 4, a.width // 4, 4).mean(axis=(1, 3))
+            b1 += xb; b2 += xb * xb
+        mean = s1 / n
+        var = (s2 - n * mean * mean) / (n - 1)
+        bmean = b1 / n
+        bvar = (b2 - n * bmean * bmean) / (n - 1)
+        ts = t / n
+        mu = float(mean.mean())
+        to1 = lambda v: float(v) / (0.01 * mu) ** 2 * ts + t_train
+        log('bench: %s %.3f ms a sample, mean %.4g, mean var %.4g, 4x4-bin var %.4g' % (
+            'guided' if guided else 'plain', ts * 1e3, mu, var.mean(), bvar.mean()))
+        return {'ms_per_sample': ts * 1e3, 'train_s': t_train, 'mean': mu, 'var': float(var.mean()),
+                'var_4x4': float(bvar.mean()), 'seconds_to_1pct': to1(var.mean()), 'seconds_to_1pct_4x4': to1(bvar.mean()),
+                'stats': {k: v for k, v in e.stats().items() if k in ('segments', 'guided', 'guide_records', 'guide_steps')}}, var
+    plain, vp = run(False)
+    guided, vg = run(True)
+    z = abs(guided['mean'] - plain['mean']) / math.sqrt((plain['var'] + guided['var']) / a.steps / (a.width * a.height))
+    ratio = vp / np.maximum(vg, 1e-30)
+    out = {'metric': 'seconds to 1% rel. RMSE, guided vs plain', 'unit': 's', 'higher_is_better': False,
+           'value': guided['seconds_to_1pct_4x4'], 'plain': plain, 'guided': guided,
+           'speedup_1pct': plain['seconds_to_1pct'] / guided['seconds_to_1pct'],
+           'speedup_1pct_4x4': plain['seconds_to_1pct_4x4'] / guided['seconds_to_1pct_4x4'],
+           'variance_reduction': plain['var'] / guided['var'], 'median_pixel_variance_ratio': float(np.median(ratio)),
+           'mean_agreement_sigma': z, 'n_gpus': 1, 'dtype': 'f32 (network bf16, accumulation f64)',
+           'config': {'workload': '%s %dx%d' % (a.scene, a.width, a.height), 'train_samples': a.train, 'samples': a.steps,
+                      'photons': extra.get('sppmPhotons', 0)}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument('--guide', action='store_true', help='guided against plain: seconds to 1% relative RMSE')
+    ap.add_argument('--scene', default='caustic-c3')
+    ap.add_argument('--train', type=int, default=300)
     ap.add_argument('--calibrate-mib', type=int, default=0, help='a device copy of this many MiB first (counter scale)')
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=32)
@@ -86,6 +164,13 @@ def main():
               'all-reduce of the FP64 accumulators, not built here)' % a.gpus, file=sys.stderr)
         sys.exit(2)
 
+    if a.guide:
+        if a.width == 1920 and a.height == 1080 and a.scene == 'caustic-c3':
+            a.width, a.height = 640, 360
+        if a.steps == 32:
+            a.steps = 128
+        bench_guide(a)
+        return
     from shadytracer_amd import pt, scenes
     if a.calibrate_mib:
         calibration_copy(a.calibrate_mib)

@@ -105,8 +105,8 @@ int pt_reset_stats(pt_ctx* ctx);
 
 /* Runs `samples` samples kernel by kernel with HIP events around each launch on the engine's stream and
  * returns the summed milliseconds per kernel: ms[0] genrays, [1] march, [2] shade, [3] occlude, [4] resolve,
- * [5] the queues' binning, [6] the photon pass's map clear and emission, [7] surface (n >= 5; the slots past n are
- * dropped).
+ * [5] the queues' binning, [6] the photon pass's map clear and emission, [7] surface, [8] the guide network's
+ * inference and training (n >= 5; the slots past n are dropped).
  * Replaces R.timeKernels (raytrace_main.js:549-557). */
 int pt_time_kernels(pt_ctx* ctx, uint32_t samples, double* ms, size_t n);
 
@@ -123,17 +123,21 @@ int pt_pick_rays(pt_ctx* ctx, float* pos4, uint32_t* info, size_t n);
  * drawn from cfg->seed (He-uniform, biases 0), Adam's moments and step count zeroed. batch: a multiple of 64 in
  * [64, 2^20]; steps in [1, 64]; lr > 0; the box not empty. */
 int pt_guide_config(pt_ctx* ctx, const PtGuideConfig* cfg, size_t size);
-/* The network's FP32 master weights, 16104 floats in the order W1[80][64], b1, W2[64][64], b2, W3, b3, W4[64][40],
+/* The network's FP32 master weights, 16169 floats in the order W1[80][64], b1, W2[64][64], b2, W3, b3, W4[64][41],
  * b4: read (pt_guide_weights) or replaced, with Adam's state reset (pt_guide_load). */
 int pt_guide_weights(pt_ctx* ctx, float* theta, size_t n);
 int pt_guide_load(pt_ctx* ctx, const float* theta, size_t n);
 /* The network's forward pass on given vertices: raw 10 floats a row (position, view direction, normal,
- * roughness) in, 40 head outputs a row out (mixture logits, 8 x 3 mean vectors, 8 log-concentrations). */
-int pt_guide_forward(pt_ctx* ctx, const float* raw, size_t n, float* out40);
+ * roughness) in, 41 head outputs a row out (mixture logits, 8 x 3 mean vectors, 8 log-concentrations, the guided
+ * fraction's logit). */
+int pt_guide_forward(pt_ctx* ctx, const float* raw, size_t n, float* out41);
 /* One Adam step on given training samples, 16 floats a row: position mapped into the box (3), roughness, view
- * direction (3), weight F, normal (3), 0, sampled direction (3), 0. n in [1, batch]. grad (NULL or 16104 floats)
- * receives the summed gradient the step applied; loss (NULL or 1 double) the batch's loss. */
+ * direction (3), weight F, normal (3), 0, sampled direction (3), the BSDF's density there. n in [1, batch]. grad
+ * (NULL or 16169 floats) receives the summed gradient the step applied; loss (NULL or 1 double) the batch's loss. */
 int pt_guide_train(pt_ctx* ctx, const float* samples, size_t n, float* grad, double* loss);
+/* The training's state after the last sample: out[0] Adam steps taken, out[1] training samples the sample
+ * collected, out[2 + j] the loss of its step j (j < steps); n >= 2. */
+int pt_guide_status(pt_ctx* ctx, double* out, size_t n);
 
 #ifdef __cplusplus
 }

@@ -168,7 +168,7 @@ typedef struct PtUniforms {
     unsigned int sppmPhotons;   // photons emitted in each sample's photon pass
     unsigned int marchRefill;   // lanes a wave lets go idle before it fetches more rays
     unsigned int guide;         // 1: MLP path guiding mixed with BSDF sampling
-    float guideFrac;            // share of guided directions
+    float guideFrac;            // the largest share of guided directions (the network picks its own below it)
     float sppmAlpha;            // the radius schedule's alpha: r_k^2 = r_1^2 prod_{i<k} (i + alpha) / (i + 1)
 } PtUniforms;
 

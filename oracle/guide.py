@@ -7,12 +7,16 @@ the method the engine implements, written from the mathematics:
 
   * input: a vertex's position in the scene's box (x in [0, 1]^3) encoded at 12 octaves (sin, cos of 2^k pi x_i),
     its view direction, its normal, its roughness and a constant 1 -- 80 features;
-  * network: 80 -> 64 -> 64 -> 64 -> 40, ReLU between (weights W[in, out], biases b[out]);
+  * network: 80 -> 64 -> 64 -> 64 -> 41, ReLU between (weights W[in, out], biases b[out]);
   * head: an 8-lobe von Mises-Fisher mixture on the sphere -- weights softmax(o[0:8]), mean directions
     normalize(o[8 + 3j : 11 + 3j]), concentrations exp(clamp(o[32 + j], KAPPA_LO, KAPPA_HI));
-    vMF(w; mu, k) = k / (2 pi (1 - e^{-2k})) exp(k (mu . w - 1));
-  * loss over a batch of sampled directions w_i with weights F_i = luminance(f cos L_i) / p(w_i):
-    L = -sum_i F_i log q(w_i) / sum_i F_i (the KL divergence to the normalised product, up to a constant);
+    vMF(w; mu, k) = k / (2 pi (1 - e^{-2k})) exp(k (mu . w - 1)) -- and the guided fraction c = sigmoid(o[40]);
+  * the sampling density is the two strategies' mixture p(w) = c q(w) + (1 - c) b(w), b the BSDF's density;
+  * losses over a batch of sampled directions w_i with weights F_i = luminance(f cos L_i) / p_used(w_i): the mixture
+    learns the normalised product by its own KL divergence, L_q = -sum_i F_i log q(w_i) / sum_i F_i, and the fraction
+    learns how far to trust it by the combined density's, L_c = -sum_i F_i log(c q(w_i) + (1 - c) b(w_i)) / sum_i F_i
+    with q held fixed there (trained through L_c alone, a mixture the fraction has turned off would never learn
+    again); the loss reported is L_c;
   * Adam (beta1 0.9, beta2 0.99, eps 1e-8) on the FP32 master weights.
 
 Pinned by (tests/test_guide_oracle.py): central finite differences of the loss against `backward`; the mixture's
@@ -23,7 +27,7 @@ import math
 
 import numpy as np
 
-SIZES = [80, 64, 64, 64, 40]
+SIZES = [80, 64, 64, 64, 41]
 LOBES = 8
 OCTAVES = 12
 KAPPA_LO, KAPPA_HI = -3.0, 7.0
@@ -72,6 +76,11 @@ def forward(theta, X):
     return h, acts
 
 
+def frac(o):
+    """The guided fraction c = sigmoid(o[40])."""
+    return 1.0 / (1.0 + np.exp(-np.asarray(o, np.float64)[:, 40]))
+
+
 def mixture(o):
     """(pi [N, 8], mu [N, 8, 3], kappa [N, 8], |m| [N, 8], clamp mask [N, 8]) from head outputs."""
     o = np.asarray(o, np.float64)
@@ -103,20 +112,42 @@ def mixture_pdf(o, w):
     return (pi * vmf_pdf(mu, kappa, np.asarray(w, np.float64)[:, None, :])).sum(axis=1)
 
 
-def loss(o, w, F):
-    q = mixture_pdf(o, w)
+def combined_pdf(o, w, b):
+    c = frac(o)
+    return c * mixture_pdf(o, w) + (1 - c) * np.asarray(b, np.float64)
+
+
+def loss(o, w, F, b):
+    """L_c: the combined density's loss (the one reported)."""
+    p = combined_pdf(o, w, b)
     F = np.asarray(F, np.float64)
-    return -(F * np.log(q)).sum() / F.sum()
+    return -(F * np.log(p)).sum() / F.sum()
 
 
-def dloss_do(o, w, F):
-    """dL/do [N, 40] of the self-normalised batch loss."""
+def loss_q(o, w, F):
+    """L_q: the mixture's own loss."""
+    F = np.asarray(F, np.float64)
+    return -(F * np.log(mixture_pdf(o, w))).sum() / F.sum()
+
+
+def loss_surrogate(o, w, F, b, q0):
+    """L_q + L_c with q fixed at q0 inside L_c: what dloss_do differentiates (at q = q0)."""
+    c = frac(o)
+    F = np.asarray(F, np.float64)
+    return loss_q(o, w, F) - (F * np.log(c * q0 + (1 - c) * np.asarray(b, np.float64))).sum() / F.sum()
+
+
+def dloss_do(o, w, F, b):
+    """d/do [N, 41] of L_q + L_c, q held fixed in L_c."""
     pi, mu, kappa, ml, inside = mixture(o)
     w = np.asarray(w, np.float64)
     F = np.asarray(F, np.float64)
+    b = np.asarray(b, np.float64)
     comp = pi * vmf_pdf(mu, kappa, w[:, None, :])
     q = comp.sum(axis=1, keepdims=True)
     r = comp / q                                       # responsibilities
+    c = frac(o)[:, None]
+    p = c * q + (1 - c) * b[:, None]
     scale = (F / F.sum())[:, None]
     g = np.zeros_like(np.asarray(o, np.float64))
     g[:, :LOBES] = -(r - pi) * scale
@@ -126,14 +157,16 @@ def dloss_do(o, w, F):
     g[:, LOBES:4 * LOBES] = (gm * scale[:, :, None]).reshape(-1, 3 * LOBES)
     # d log C / dk = 1/k - 2 / (e^{2k} - 1) = 1/k + 2 e^{-2k} / expm1(-2k); dk/ds = k inside the clamp
     dlogc = 1.0 / kappa + 2.0 * np.exp(-2 * kappa) / np.expm1(-2 * kappa)
-    g[:, 4 * LOBES:] = -(r * (dlogc + (dot - 1.0)) * kappa * inside) * scale
+    g[:, 4 * LOBES:5 * LOBES] = -(r * (dlogc + (dot - 1.0)) * kappa * inside) * scale
+    # dc/ds = c (1 - c): -F (q - b) c (1 - c) / p
+    g[:, 40] = (-(F / F.sum())[:, None] * (q - b[:, None]) * c * (1 - c) / p)[:, 0]
     return g
 
 
-def backward(theta, X, w, F):
+def backward(theta, X, w, F, b):
     """dL/dtheta (flat, the engine's order)."""
     o, acts = forward(theta, X)
-    d = dloss_do(o, w, F)
+    d = dloss_do(o, w, F, b)
     layers = unpack(theta)
     grads = []
     for i in range(len(layers) - 1, -1, -1):

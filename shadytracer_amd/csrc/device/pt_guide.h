@@ -9,17 +9,22 @@
 //
 //   features (80): position in the scene box at 12 octaves -- sin, cos of 2^k pi x_i, k major, axis minor --, the view
 //                  direction, the normal, the roughness, 1
-//   network:       80 -> 64 -> 64 -> 64 -> 40, ReLU between; parameters flat W1[80][64], b1, W2[64][64], b2, W3, b3,
-//                  W4[64][40], b4 (W[in][out])
+//   network:       80 -> 64 -> 64 -> 64 -> 41, ReLU between; parameters flat W1[80][64], b1, W2[64][64], b2, W3, b3,
+//                  W4[64][41], b4 (W[in][out])
 //   head:          weights softmax(o[0:8]), means normalize(o[8 + 3j : 11 + 3j]), concentrations
-//                  exp(clamp(o[32 + j], -3, 7)); vMF(w) = k / (2 pi (1 - e^{-2k})) exp(k (mu . w - 1))
+//                  exp(clamp(o[32 + j], -3, 7)); vMF(w) = k / (2 pi (1 - e^{-2k})) exp(k (mu . w - 1)); the guided
+//                  fraction c = sigmoid(o[40]) (capped by the uniforms' guideFrac)
+//   losses:        the mixture learns the normalised product f cos L by its KL divergence -sum F log q(w) / sum F;
+//                  the fraction learns how far to trust it by the combined density's -sum F log(c q + (1 - c) b) /
+//                  sum F, b the BSDF's density at w, with q held fixed there (through that loss alone, a mixture the
+//                  fraction had turned off would never learn again); the second is the loss reported
 #pragma once
 
 #define PT_G_IN 80
 #define PT_G_H 64
-#define PT_G_OUT 40
+#define PT_G_OUT 41
 #define PT_G_LOBES 8
-#define PT_G_NP 16104
+#define PT_G_NP 16169
 #define PT_G_OFF_W1 0
 #define PT_G_OFF_B1 5120
 #define PT_G_OFF_W2 5184
@@ -27,18 +32,18 @@
 #define PT_G_OFF_W3 9344
 #define PT_G_OFF_B3 13440
 #define PT_G_OFF_W4 13504
-#define PT_G_OFF_B4 16064
+#define PT_G_OFF_B4 16128
 // the BF16 copy the MFMA reads, each layer's weights transposed (WT[out][in]): W1T 64 x 80, W2T, W3T 64 x 64, W4T
-// 64 x 64 with rows 40..63 zero
+// 64 x 64 with rows 41..63 zero
 #define PT_G_WT1 0
 #define PT_G_WT2 5120
 #define PT_G_WT3 9216
 #define PT_G_WT4 13312
 #define PT_G_WT_WORDS 17408
-#define PT_G_BIAS 256          // b1, b2, b3 at 0, 64, 128; b4 at 192 (64, the last 24 zero)
+#define PT_G_BIAS 256          // b1, b2, b3 at 0, 64, 128; b4 at 192 (64, the last 23 zero)
 #define PT_G_KLO -3.0f
 #define PT_G_KHI 7.0f
-#define PT_G_MIX_F4 10         // a slot's mixture: (mu, kappa) x 8, then the 8 weights
+#define PT_G_MIX_F4 11         // a slot's mixture: (mu, kappa) x 8, the 8 weights, the guided fraction
 #define PT_G_REC_F4 7          // a recorded vertex (below)
 #define PT_G_SAMPLE_F4 4       // a training sample (below)
 #define PT_G_TILE_LD 88        // the activation tile's row stride (bf16): 80 features and 8 of padding
@@ -178,6 +183,7 @@ __device__ __forceinline__ void guideForward(const GuideLds& s, __bf16* tile, fl
 struct GuideMix {
     float4 lobe[PT_G_LOBES];   // mu, kappa
     float w[PT_G_LOBES];
+    float c;                   // the guided fraction the network asks for
 };
 __device__ __forceinline__ float vmfLogNorm(float k) {
     // log(k / (2 pi (1 - e^{-2k})))
@@ -199,6 +205,7 @@ __device__ __forceinline__ GuideMix guideMixOf(const float* o) {
         float k = expf(fminf(fmaxf(o[32 + j], PT_G_KLO), PT_G_KHI));
         m.lobe[j] = make_float4(v.x / l, v.y / l, v.z / l, k);
     }
+    m.c = 1.0f / (1.0f + expf(-o[40]));
     return m;
 }
 __device__ __forceinline__ float guidePdf(const GuideMix& m, V3 d) {
@@ -232,6 +239,7 @@ __device__ __forceinline__ void guideStoreMix(const PtDev* __restrict__ D, unsig
     for (int j = 0; j < PT_G_LOBES; ++j) dst[j] = m.lobe[j];
     dst[8] = make_float4(m.w[0], m.w[1], m.w[2], m.w[3]);
     dst[9] = make_float4(m.w[4], m.w[5], m.w[6], m.w[7]);
+    dst[10] = make_float4(m.c, 0.0f, 0.0f, 0.0f);
 }
 __device__ __forceinline__ GuideMix guideLoadMix(const PtDev* __restrict__ D, unsigned int slot) {
     GuideMix m;
@@ -240,6 +248,7 @@ __device__ __forceinline__ GuideMix guideLoadMix(const PtDev* __restrict__ D, un
     for (int j = 0; j < PT_G_LOBES; ++j) m.lobe[j] = src[j];
     float4 a = src[8], b = src[9];
     m.w[0] = a.x; m.w[1] = a.y; m.w[2] = a.z; m.w[3] = a.w; m.w[4] = b.x; m.w[5] = b.y; m.w[6] = b.z; m.w[7] = b.w;
+    m.c = src[10].x;
     return m;
 }
 
@@ -302,9 +311,9 @@ extern "C" __global__ __launch_bounds__(64 * PT_G_WAVES) void pt_guide_infer(con
 }
 
 // The forward pass on given vertices (the C ABI's evaluation for tests and tools): raw rows of 10 floats (position,
-// view direction, normal, roughness) in, 40 head outputs a row out.
+// view direction, normal, roughness) in, 41 head outputs a row out.
 extern "C" __global__ __launch_bounds__(64) void pt_guide_eval(const PtDev* __restrict__ D, const float* __restrict__ raw,
-                                                             unsigned int n, float* __restrict__ o40) {
+                                                             unsigned int n, float* __restrict__ o41) {
     __shared__ GuideLds s;
     __shared__ __bf16 tile[64 * PT_G_TILE_LD];
     guideLoadWeights(D, s);
@@ -321,7 +330,7 @@ extern "C" __global__ __launch_bounds__(64) void pt_guide_eval(const PtDev* __re
         }
         guideForward(s, tile, out);
         if (i < n)
-            for (int j = 0; j < PT_G_OUT; ++j) o40[40ull * i + j] = out[lane * PT_G_OUT + j];
+            for (int j = 0; j < PT_G_OUT; ++j) o41[(unsigned long long)PT_G_OUT * i + j] = out[lane * PT_G_OUT + j];
     }
 }
 
@@ -336,7 +345,8 @@ __device__ __forceinline__ float4* guideRec(const PtDev* __restrict__ D, unsigne
 }
 
 // Collect: every recorded vertex with a known S becomes a training sample (position in the box, roughness | view
-// direction, F | normal | w), appended in order; each Adam step of the sample takes its own run of `batch` of them.
+// direction, F | normal | w, the BSDF's density at w), appended; the sum of F over all of them kept for the steps'
+// normalisation.
 extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_guide_collect(const PtDev* __restrict__ D) {
     const unsigned int idx = blockIdx.x * PT_BLOCK + threadIdx.x;
     const unsigned int r = idx / D->gMaxV, k = idx % D->gMaxV;
@@ -364,9 +374,19 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_guide_collect(const Pt
     if (ok && at < D->gBatchCap) {
         float4* dst = reinterpret_cast<float4*>(D->gBatch) + (size_t)PT_G_SAMPLE_F4 * at;
         dst[0] = a; dst[1] = b; dst[2] = c; dst[3] = d;
-        const unsigned int step = at / D->g.batch;
-        if (step < D->g.steps) atomicAdd(&D->gSumF[step], (double)F);
+        atomicAdd(&D->gSumF[0], (double)F);
     }
+}
+// Step `step`'s batch: all N samples in order when they fit one batch (the exact batch gradient, F / sum F), else
+// `batch` drawn uniformly with replacement from all N -- every image region in every step -- each weighing
+// F N / (batch sum F), an unbiased estimate of the whole collection's gradient.
+__device__ __forceinline__ unsigned int guidePick(const PtDev* __restrict__ D, unsigned int step, unsigned int i,
+                                                  unsigned int N) {
+    if (N <= D->g.batch) return i;
+    Rng r;
+    r.a = i; r.b = step * 0x9E3779B9u; r.c = D->counters[PT_C_GUIDE_T] ^ 0x6A09E667u; r.d = D->counters[PT_C_SAMPLE];
+    pcg4d(r);
+    return (unsigned int)(((unsigned long long)r.a * N) >> 32);
 }
 
 // ---------------------------------------------------------------------------------------------------
@@ -381,8 +401,10 @@ struct GuideTrainLds {
     float out[64 * PT_G_OUT];      // the head's outputs
     __bf16 dout[64 * 48];          // the head's gradient (K padded to 48)
 };
-__device__ __forceinline__ void guideHeadGrad(const float* o, V3 w, float scale, float* g) {
-    // responsibilities r_j = pi_j vMF_j(w) / q(w)
+// d/do at one sample (s = its share of the batch's weight) of -s log q(w) for the mixture's outputs and of
+// -s log(c q(w) + (1 - c) b) for the fraction's; returns the combined density
+__device__ __forceinline__ float guideHeadGrad(const float* o, V3 w, float bsdf, float s, float* g) {
+    // responsibilities r_j = pi_j vMF_j(w) / q(w); the guide's share of the combined density rho = c q / p
     GuideMix m = guideMixOf(o);
     float comp[PT_G_LOBES], q = 0.0f;
 #pragma unroll
@@ -392,6 +414,9 @@ __device__ __forceinline__ void guideHeadGrad(const float* o, V3 w, float scale,
         q += comp[j];
     }
     q = fmaxf(q, 1e-30f);
+    const float p = fmaxf(m.c * q + (1.0f - m.c) * bsdf, 1e-30f);
+    const float scale = s;
+    g[40] = -s * (q - bsdf) * m.c * (1.0f - m.c) / p;
 #pragma unroll
     for (int j = 0; j < PT_G_LOBES; ++j) {
         const float4 L = m.lobe[j];
@@ -410,6 +435,7 @@ __device__ __forceinline__ void guideHeadGrad(const float* o, V3 w, float scale,
         const float dlogc = 1.0f / k - 2.0f * e2 / fmaxf(1.0f - e2, 1e-30f);
         g[32 + j] = in ? -r * (dlogc + (dt - 1.0f)) * k * scale : 0.0f;
     }
+    return p;
 }
 
 extern "C" __global__ __launch_bounds__(64) void pt_guide_grad(const PtDev* __restrict__ D, unsigned int step) {
@@ -419,12 +445,12 @@ extern "C" __global__ __launch_bounds__(64) void pt_guide_grad(const PtDev* __re
     float* part = D->gPart + (size_t)blockIdx.x * PT_G_NP;
     for (unsigned int k = lane; k < PT_G_NP; k += 64u) part[k] = 0.0f;
     __syncthreads();
-    const unsigned int total = D->counters[PT_C_GUIDE_N] < D->gBatchCap ? D->counters[PT_C_GUIDE_N] : D->gBatchCap;
-    const unsigned int lo = step * D->g.batch;
-    const unsigned int n = total > lo ? min(total - lo, D->g.batch) : 0u;
-    const double sumF = D->gSumF[step];
+    const unsigned int N = D->counters[PT_C_GUIDE_N] < D->gBatchCap ? D->counters[PT_C_GUIDE_N] : D->gBatchCap;
+    const unsigned int n = min(N, D->g.batch);
+    const double sumF = D->gSumF[0];
     if (n == 0u || !(sumF > 0.0)) return;
-    const float inv = (float)(1.0 / sumF);
+    const float inv = (float)((double)N / ((double)n * sumF));
+    const float meanF = (float)(sumF / (double)N);
     const __bf16* w = S.s.w;
     const float* b = S.s.b;
     float lossPart = 0.0f;
@@ -432,10 +458,10 @@ extern "C" __global__ __launch_bounds__(64) void pt_guide_grad(const PtDev* __re
         const unsigned int i = t * 64u + lane;
         const bool on = i < n;
         V3 wdir = v3(0.0f, 0.0f, 1.0f);
-        float F = 0.0f;
+        float F = 0.0f, bsdf = 0.0f;
         if (on) {
             // the features, from the position already mapped into the box
-            const float4* smp = reinterpret_cast<const float4*>(D->gBatch) + (size_t)PT_G_SAMPLE_F4 * (lo + i);
+            const float4* smp = reinterpret_cast<const float4*>(D->gBatch) + (size_t)PT_G_SAMPLE_F4 * guidePick(D, step, i, N);
             const float4 a = smp[0], bb = smp[1], c = smp[2], d = smp[3];
             __bf16* r = S.x + lane * PT_G_TILE_LD;
             const float xs[3] = {a.x, a.y, a.z};
@@ -455,7 +481,8 @@ extern "C" __global__ __launch_bounds__(64) void pt_guide_grad(const PtDev* __re
             r[75] = toBf16(c.x); r[76] = toBf16(c.y); r[77] = toBf16(c.z);
             r[78] = toBf16(a.w); r[79] = toBf16(1.0f);
             wdir = v3(d.x, d.y, d.z);
-            F = bb.w;
+            bsdf = d.w;
+            F = fminf(bb.w, meanF * 16.0f);   // a firefly's weight capped: it would steer a whole batch
         } else {
             guideZeroRow(S.x, (int)lane);
         }
@@ -476,9 +503,8 @@ extern "C" __global__ __launch_bounds__(64) void pt_guide_grad(const PtDev* __re
         {
             float g[PT_G_OUT];
             if (on) {
-                guideHeadGrad(S.out + lane * PT_G_OUT, wdir, F * inv, g);
-                GuideMix m = guideMixOf(S.out + lane * PT_G_OUT);
-                lossPart += -F * inv * logf(fmaxf(guidePdf(m, wdir), 1e-30f));
+                const float p = guideHeadGrad(S.out + lane * PT_G_OUT, wdir, bsdf, F * inv, g);
+                lossPart += -F * inv * logf(p);
             } else {
 #pragma unroll
                 for (int j = 0; j < PT_G_OUT; ++j) g[j] = 0.0f;
@@ -547,7 +573,7 @@ extern "C" __global__ __launch_bounds__(64) void pt_guide_grad(const PtDev* __re
 // BF16 transposed copy and the biases refreshed. Step t = (steps so far) + 1; the commit kernel counts it afterwards.
 extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_guide_adam(const PtDev* __restrict__ D, unsigned int step) {
     const unsigned int p = blockIdx.x * PT_BLOCK + threadIdx.x;
-    if (p >= PT_G_NP || !(D->gSumF[step] > 0.0)) return;
+    if (p >= PT_G_NP || !(D->gSumF[0] > 0.0)) return;
     float g = 0.0f;
     for (unsigned int b = 0; b < PT_G_PARTS; ++b) g += D->gPart[(size_t)b * PT_G_NP + p];
     const float t = (float)(D->counters[PT_C_GUIDE_T] + 1u);
@@ -568,13 +594,13 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_guide_adam(const PtDev
     else if (p < PT_G_OFF_W3) { bias = true; idx = 64u + p - PT_G_OFF_B2; }
     else if (p < PT_G_OFF_B3) { unsigned int q = p - PT_G_OFF_W3, i = q / 64u, o = q % 64u; idx = PT_G_WT3 + o * 64u + i; }
     else if (p < PT_G_OFF_W4) { bias = true; idx = 128u + p - PT_G_OFF_B3; }
-    else if (p < PT_G_OFF_B4) { unsigned int q = p - PT_G_OFF_W4, i = q / 40u, o = q % 40u; idx = PT_G_WT4 + o * 64u + i; }
+    else if (p < PT_G_OFF_B4) { unsigned int q = p - PT_G_OFF_W4, i = q / PT_G_OUT, o = q % PT_G_OUT; idx = PT_G_WT4 + o * 64u + i; }
     else { bias = true; idx = 192u + p - PT_G_OFF_B4; }
     if (bias) D->gBias[idx] = th;
     else reinterpret_cast<__bf16*>(D->gW)[idx] = toBf16(th);
 }
 extern "C" __global__ void pt_guide_commit(const PtDev* __restrict__ D, unsigned int step) {
-    if (threadIdx.x == 0 && blockIdx.x == 0 && D->gSumF[step] > 0.0) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && D->gSumF[0] > 0.0) {
         D->counters[PT_C_GUIDE_T] += 1u;
         atomicAdd(&D->stats[PT_S_GSTEPS], 1ull);
     }

@@ -273,6 +273,7 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_surface(const PtDev* _
 
 struct ShadeOut {
     bool requeue, shadow, store, photon;
+    bool leaving, gathered, cut, guided, recorded;   // counted per wave by the caller
     V3 sp, spow, sl, sn;     // a photon to store: position, power, direction to its light, surface normal
     unsigned int ssub;       // ...and its sub-list
 };
@@ -329,6 +330,7 @@ __device__ __forceinline__ V3 sppmGather(const PtDev* __restrict__ D, const Lobe
 __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const PtMaterial* __restrict__ smat,
                                              const unsigned int* seed, unsigned int slot) {
     ShadeOut out; out.requeue = false; out.shadow = false; out.store = false;
+    out.leaving = out.gathered = out.cut = out.guided = out.recorded = false;
     float4 P = ld4(D->pos, slot), Dd = ld4(D->dir, slot), B = ld4(D->beta, slot), R = ld4(D->rad, slot), A = ld4(D->aux, slot);
     uint4 M = ld4u(D->meta, slot);
     V3 p = v3(P.x, P.y, P.z), dir = v3(Dd.x, Dd.y, Dd.z), beta = v3(B.x, B.y, B.z), rad = v3(R.x, R.y, R.z);
@@ -341,12 +343,13 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
     // guide training: a recorded path's previous vertex learns S, the radiance the path had when this event came
     unsigned int gr = 0u;
     const bool grec = !photon && guideRecorded(D, slot, gr);
-    if (grec && (flags & PT_F_GPEND)) {
+    const bool spend = grec && (flags & PT_F_GPEND);
+    V3 skStart = v3s(0.0f);
+    const V3 radStart = rad;
+    bool emitted = false;
+    if (spend) {
         const float4 nee = reinterpret_cast<const float4*>(D->gPath)[2 * gr + 1];
-        const unsigned int k = D->gVerts[gr] - 1u;
-        float4* rec = guideRec(D, gr, k);
-        rec[6] = make_float4(rad.x + nee.x, rad.y + nee.y, rad.z + nee.z, 0.0f);
-        rec[4].w = 1.0f;
+        skStart = rad + v3(nee.x, nee.y, nee.z);
         flags &= ~PT_F_GPEND;
     }
     // photons draw from their own stream, so the map and the camera paths of one sample are independent
@@ -436,13 +439,14 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
             float off = offsetTol(D, p);
             p = p + dir * off; ltot += off * n;
             out.requeue = true;
-            atomicAdd(&D->stats[PT_S_LEAVING], 1ull);
+            out.leaving = true;
         } else {
             bounces += 1u;
             const PtMaterial& m = smat[hw & 0xFFu];
             V3 Le = ld3(m.emissive);
             if (max3(Le) > 0.0f && (photon || (mapped && m.face >= 0 && faceFlux(D->faces[m.face]) > 0.0f))) {
                 terminate = true;      // a photon ends at a light; a mapped path's light is the map's (its photons')
+                emitted = true;
             } else if (max3(Le) > 0.0f) {
                 // AN EMITTER HIT: weighted against the light sampling the previous vertex ran
                 float w = 1.0f;
@@ -458,6 +462,7 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
                 rad += beta * Le * w;
                 direct = bounces == 1u;
                 terminate = true;
+                emitted = true;
             } else {
                 // the medium beyond the surface: the material's own index going in, air (or the enclosing
                 // material) going out
@@ -485,12 +490,13 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
                 float fs = faceShareAt(D, bounces);
                 bool neeOn = D->u.nee != 0u && !photon;
                 // PATH GUIDING: where it applies and the network is trained, directions come from its mixture with
-                // probability gc, from the BSDF otherwise; every density below is the two strategies' mixture
+                // probability gc (the network's own fraction, capped by guideFrac), from the BSDF otherwise; every
+                // density below is the two strategies' mixture
                 const bool gElig = !photon && guideMaterial(m);
                 const bool guided = gElig && guideActive(D);
-                const float gc = guided ? D->u.guideFrac : 0.0f;
                 GuideMix gmix;
-                if (guided) gmix = guideLoadMix(D, slot);
+                float gc = 0.0f;
+                if (guided) { gmix = guideLoadMix(D, slot); gc = fminf(gmix.c, D->u.guideFrac); }
                 bool gathered = false;
                 // a surface with a lobe the map can be gathered through (any but a purely delta one)
                 const bool evaluable = !lobesNone(m);
@@ -506,8 +512,8 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
                     bool cut = false;
                     rad += beta * sppmGather(D, lb, p, nrm, cut);
                     gathered = true;
-                    atomicAdd(&D->stats[PT_S_GATHERS], 1ull);
-                    if (cut) atomicAdd(&D->stats[PT_S_GATHERCUT], 1ull);
+                    out.gathered = true;
+                    out.cut = cut;
                 }
                 if (neeOn && !lobesNone(m)) {
                     V4 u0 = rngNext(rng), u1 = rngNext(rng);
@@ -559,18 +565,24 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
                 float pTot = -1.0f;    // the sampled direction's density (non-delta): the BSDF's, or the mixture's
                 if (guided) {
                     const V4 ug = rngNext(rng);
+                    // a BSDF sample that failed (a microfacet direction below the horizon) stays a failed sample: its
+                    // direction is a placeholder, not a draw of the BSDF's density
+                    bool valid = max3(s.w) > 0.0f;
                     if (ug.w < gc) {
                         s.dir = guideSample(gmix, ug);
                         s.isDelta = false; s.transmitted = false; s.ev = PT_EV_EVAL;
+                        valid = true;
                     }
-                    if (s.isDelta) {
+                    if (!valid) {
+                        s.w = v3s(0.0f);
+                    } else if (s.isDelta) {
                         s.w = s.w * (1.0f / (1.0f - gc));
                     } else {
                         pTot = gc * guidePdf(gmix, s.dir) + (1.0f - gc) * lobesPdf(lb, s.dir);
                         const float cosO = dot(nrm, s.dir);
                         s.w = (pTot > 0.0f && cosO > 0.0f) ? lobesEval(lb, D->mf, s.dir) * (cosO / pTot) : v3s(0.0f);
                     }
-                    atomicAdd(&D->stats[PT_S_GUIDED], 1ull);
+                    out.guided = true;
                 }
                 if (photon && !s.isDelta) terminate = true;     // the map holds caustics: L S+ D only
                 dir = s.dir;
@@ -619,12 +631,12 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
                     rec[0] = make_float4(p.x, p.y, p.z, m.roughness);
                     rec[1] = make_float4(-wi.x, -wi.y, -wi.z, pw);
                     rec[2] = make_float4(nrm.x, nrm.y, nrm.z, 0.0f);
-                    rec[3] = make_float4(dir.x, dir.y, dir.z, 0.0f);
+                    rec[3] = make_float4(dir.x, dir.y, dir.z, lobesPdf(lb, dir));
                     rec[4] = make_float4(fc.x, fc.y, fc.z, 0.0f);
                     rec[5] = make_float4(beta.x, beta.y, beta.z, 0.0f);
                     D->gVerts[gr] = k + 1u;
                     flags |= PT_F_GPEND;
-                    atomicAdd(&D->stats[PT_S_GRECORDS], 1ull);
+                    out.recorded = true;
                 }
                 if (!terminate) {
                     float off = offsetTol(D, p);
@@ -635,6 +647,16 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
                 }
             }
         }
+    }
+    if (spend) {
+        // the previous vertex's S: the path's radiance before this event -- and, with light sampling on, after the
+        // light this event met on leaving the scene or at an emitter, which light sampling also reaches: the network
+        // learns the light it must find itself, free of the MIS weights its own density sets
+        V3 sk = skStart;
+        if (D->u.nee != 0u && (state == PT_ESCAPED || emitted)) sk += rad - radStart;
+        float4* rec = guideRec(D, gr, D->gVerts[gr] - 1u);
+        rec[6] = make_float4(sk.x, sk.y, sk.z, 0.0f);
+        rec[4].w = 1.0f;
     }
     if (terminate) {
         out.requeue = false;
@@ -672,6 +694,7 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_shade(const PtDev* __r
     unsigned int* qout = qsel ? D->qMarch1 : D->qMarch0;
     unsigned int* cout = &C[qsel ? PT_C_MARCH1 : PT_C_MARCH0];
     unsigned long long segs = 0ull, psegs = 0ull, stored = 0ull, dropped = 0ull;
+    unsigned long long nLeave = 0ull, nGather = 0ull, nCut = 0ull, nGuided = 0ull, nRec = 0ull;
     for (unsigned int base = blockIdx.x * PT_BLOCK; base < count; base += gridDim.x * PT_BLOCK) {
         unsigned int i = base + threadIdx.x;
         ShadeOut o; o.requeue = false; o.shadow = false; o.store = false; o.photon = false;
@@ -680,6 +703,7 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_shade(const PtDev* __r
             slot = D->qSorted[i];
             o = shadeOne(D, smat, seed, slot);
             if (o.photon) psegs += 1ull; else segs += 1ull;
+            nLeave += o.leaving; nGather += o.gathered; nCut += o.cut; nGuided += o.guided; nRec += o.recorded;
         }
         unsigned int qi = waveAppend(cout, o.requeue);
         if (o.requeue) qout[qi] = slot;
@@ -711,6 +735,11 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_shade(const PtDev* __r
     statAdd(D, PT_S_PHOTONSEGS, psegs);
     statAdd(D, PT_S_PHOTONS, stored);
     statAdd(D, PT_S_OVERFLOW, dropped);
+    statAdd(D, PT_S_LEAVING, nLeave);
+    statAdd(D, PT_S_GATHERS, nGather);
+    statAdd(D, PT_S_GATHERCUT, nCut);
+    statAdd(D, PT_S_GUIDED, nGuided);
+    statAdd(D, PT_S_GRECORDS, nRec);
 }
 
 // ---------------------------------------------------------------------------------------------------

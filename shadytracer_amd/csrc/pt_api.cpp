@@ -31,7 +31,7 @@ enum { K_GENRAYS, K_MARCH, K_SHADE, K_OCCLUDE, K_RESOLVE, K_MFLUT, K_MFAVG, K_PI
        K_SPPMCLEAR, K_PHOTONGEN, K_SURFACE, K_GINFER, K_GEVAL, K_GCOLLECT, K_GGRAD, K_GADAM, K_GCOMMIT, K_COUNT };
 
 // the guide network's sizes (device/pt_guide.h)
-const unsigned int kGuideNP = 16104, kGuideWT = 17408, kGuideBias = 256, kGuideParts = 256, kGuideMaxSteps = 64;
+const unsigned int kGuideNP = 16169, kGuideWT = 17408, kGuideBias = 256, kGuideParts = 256, kGuideMaxSteps = 64;
 
 thread_local std::string g_createError;
 
@@ -403,7 +403,7 @@ void guideInit(uint32_t seed, std::vector<float>& th) {
         uint32_t x = (uint32_t)(((old >> 18u) ^ old) >> 27u), r = (uint32_t)(old >> 59u);
         return ((x >> r) | (x << ((32u - r) & 31u))) * (1.0 / 4294967296.0);
     };
-    const unsigned int sizes[5] = {80, 64, 64, 64, 40};
+    const unsigned int sizes[5] = {80, 64, 64, 64, 41};
     size_t o = 0;
     for (int L = 0; L < 4; ++L) {
         const double lim = std::sqrt(6.0 / sizes[L]) * (L == 3 ? 0.1 : 1.0);
@@ -416,7 +416,7 @@ void guideInit(uint32_t seed, std::vector<float>& th) {
 int guideUpload(pt_ctx* c, const float* th) {
     std::vector<uint16_t> w(kGuideWT, 0);
     std::vector<float> b(kGuideBias, 0.0f);
-    const unsigned int sizes[5] = {80, 64, 64, 64, 40}, wt[4] = {0, 5120, 9216, 13312};
+    const unsigned int sizes[5] = {80, 64, 64, 64, 41}, wt[4] = {0, 5120, 9216, 13312};
     size_t o = 0;
     for (int L = 0; L < 4; ++L) {
         const unsigned int in = sizes[L], out = sizes[L + 1];
@@ -456,10 +456,11 @@ int ensureGuide(pt_ctx* c) {
     if (c->capacity > c->guideMixSlots) {
         destroyGraph(c);
         void* p = h.gMix; freeBuf(c, p); h.gMix = nullptr;
-        if ((rc = allocBuf(c, h.gMix, size_t(c->capacity) * 160))) return rc;
+        if ((rc = allocBuf(c, h.gMix, size_t(c->capacity) * 176))) return rc;
         c->guideMixSlots = c->capacity;
     }
-    // which paths record: about 4 training samples a recorded path, batch x steps of them a sample
+    // which paths record: at about 4 training samples a recorded path, batch x steps of them a sample (each step draws
+    // its batch from all of them); room for four times that
     const unsigned long long want = (unsigned long long)h.g.batch * h.g.steps;
     unsigned int stride = h.g.stride ? h.g.stride : (unsigned int)std::max<unsigned long long>(1ull, (4ull * h.nrays) / std::max(want, 1ull));
     const unsigned int paths = (h.nrays + stride - 1u) / stride, verts = h.u.maxbounce + 1u;
@@ -480,7 +481,7 @@ int ensureGuide(pt_ctx* c) {
     h.gStride = stride;
     h.gPaths = paths;
     h.gMaxV = verts;
-    const unsigned int cap = (unsigned int)std::min<unsigned long long>(want, 1ull << 26);
+    const unsigned int cap = (unsigned int)std::min<unsigned long long>(4ull * want, 1ull << 26);
     if (cap > c->guideBatchCap) {
         void* p = h.gBatch; freeBuf(c, p); h.gBatch = nullptr;
         if ((rc = allocBuf(c, h.gBatch, size_t(cap) * 64))) return rc;
@@ -515,7 +516,7 @@ void setDefaults(PtUniforms& u) {
     u.cam.fx = 0.57735f; u.cam.fy = 0.57735f;
     u.faceShare = 0.5f;
     u.marchRefill = 16;
-    u.guideFrac = 0.5f;
+    u.guideFrac = 0.9f;
     u.sppmRadius = 0.02f;
     u.sppmAlpha = 2.0f / 3.0f;
 }
@@ -841,21 +842,21 @@ int pt_guide_load(pt_ctx* c, const float* theta, size_t n) {
     return guideUpload(c, theta);
 }
 
-int pt_guide_forward(pt_ctx* c, const float* raw, size_t n, float* out40) {
-    if (!c || !raw || !out40 || n == 0 || n > (1u << 24)) return fail(c, PT_ERR_ARG, "raw/out40 NULL or n not in 1..2^24");
+int pt_guide_forward(pt_ctx* c, const float* raw, size_t n, float* out41) {
+    if (!c || !raw || !out41 || n == 0 || n > (1u << 24)) return fail(c, PT_ERR_ARG, "raw/out41 NULL or n not in 1..2^24");
     if (!c->module) return fail(c, PT_ERR_STATE, "no scene loaded");
     int rc;
     if (!c->guideReady && (rc = ensureGuide(c))) return rc;
     if ((rc = syncDev(c))) return rc;
     float *din = nullptr, *dout = nullptr;
-    if ((rc = allocBuf(c, din, n * 40)) || (rc = allocBuf(c, dout, n * 160))) return rc;
+    if ((rc = allocBuf(c, din, n * 40)) || (rc = allocBuf(c, dout, n * 164))) return rc;
     HIPCHK(c, hipMemcpy(din, raw, n * 40, hipMemcpyHostToDevice));
     unsigned int nn = (unsigned int)n;
     void* args[] = {&c->dev, &din, &nn, &dout};
     unsigned int grid = (unsigned int)std::min<size_t>((n + 63) / 64, 4096);
     HIPCHK(c, hipModuleLaunchKernel(c->fn[K_GEVAL], grid, 1, 1, 64, 1, 1, 0, c->stream, args, nullptr));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(out40, dout, n * 160, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(out41, dout, n * 164, hipMemcpyDeviceToHost));
     void* p1 = din; freeBuf(c, p1);
     void* p2 = dout; freeBuf(c, p2);
     return PT_OK;
@@ -898,4 +899,18 @@ int pt_guide_train(pt_ctx* c, const float* samples, size_t n, float* grad, doubl
     return PT_OK;
 }
 
+int pt_guide_status(pt_ctx* c, double* out, size_t n) {
+    if (!c || !out || n < 2) return fail(c, PT_ERR_ARG, "out: NULL or n < 2");
+    if (!c->guideReady) return fail(c, PT_ERR_STATE, "the guide network is not set up");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    unsigned int w[2];
+    HIPCHK(c, hipMemcpy(w, c->host.counters + PT_C_GUIDE_N, 8, hipMemcpyDeviceToHost));
+    out[0] = w[1];
+    out[1] = w[0];
+    const size_t k = std::min<size_t>(n - 2, c->host.g.steps);
+    if (k) HIPCHK(c, hipMemcpy(out + 2, c->host.gLoss, k * 8, hipMemcpyDeviceToHost));
+    return PT_OK;
+}
+
 }  // extern "C"
+

@@ -69,7 +69,7 @@ function packArray(layout, items) {
 const DEFAULTS = {
     width: 256, height: 256, maxbounce: 10, maxiter: 400, stepscale: 0.9, eps: 0, epsAuto: 1e-4, envRadius: 400, nee: 1, rr: 0,
     rrStart: 3, diffuseCap: 0, fireflyClamp: 0, fireflyWhite: 1, maxaverages: 0, seed: 20261203, faceShare: 0.5, sppm: 0,
-    sppmRadius: 0.02, sppmPhotons: 0, marchRefill: 16, guide: 0, guideFrac: 0.5, sppmAlpha: 2 / 3,
+    sppmRadius: 0.02, sppmPhotons: 0, marchRefill: 16, guide: 0, guideFrac: 0.9, sppmAlpha: 2 / 3,
 }
 const KNOWN = new Set(UNIFORMS.map(f => f[0]))
 

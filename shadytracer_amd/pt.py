@@ -97,8 +97,9 @@ SIGNATURES = {
     'pt_guide_forward': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_size_t, ctypes.POINTER(ctypes.c_float)]),
     'pt_guide_train': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_size_t, ctypes.POINTER(ctypes.c_float),
                                       ctypes.POINTER(ctypes.c_double)]),
+    'pt_guide_status': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_size_t]),
 }
-GUIDE_NP = 16104
+GUIDE_NP = 16169
 
 STAT_NAMES = ['segments', 'shadow_segments', 'steps', 'shadow_steps', 'nonfinite', 'maxiter', 'paths', 'leaving',
               'gathers', 'photons', 'flops', 'shadow_flops', 'photon_segments', 'gather_cut', 'radius_bits',
@@ -168,7 +169,7 @@ def default_uniforms():
     u.stepscale, u.eps, u.epsAuto, u.envRadius = 0.9, 0.0, 1e-4, 400.0
     u.nee, u.rr, u.rrStart, u.diffuseCap = 1, 0, 3, 0
     u.fireflyClamp, u.fireflyWhite, u.maxaverages, u.seed = 0.0, 1.0, 0, 20261203
-    u.faceShare, u.marchRefill, u.guideFrac = 0.5, 16, 0.5
+    u.faceShare, u.marchRefill, u.guideFrac = 0.5, 16, 0.9
     u.sppmRadius, u.sppmAlpha = 0.02, 2.0 / 3.0
     return u
 
@@ -288,9 +289,9 @@ class Engine:
         self._chk(self.lib.pt_reset_stats(self.h))
 
     def time_kernels(self, samples=1):
-        ms = (ctypes.c_double * 8)()
-        self._chk(self.lib.pt_time_kernels(self.h, samples, ms, 8))
-        return dict(zip(['genrays', 'march', 'shade', 'occlude', 'resolve', 'bin', 'photon', 'surface'], list(ms)))
+        ms = (ctypes.c_double * 9)()
+        self._chk(self.lib.pt_time_kernels(self.h, samples, ms, 9))
+        return dict(zip(['genrays', 'march', 'shade', 'occlude', 'resolve', 'bin', 'photon', 'surface', 'guide'], list(ms)))
 
     # ---- path guiding
     def guide_config(self, box, lr=1e-2, batch=1 << 16, steps=4, stride=0, warmup=16, seed=1):
@@ -311,7 +312,7 @@ class Engine:
 
     def guide_forward(self, raw):
         raw = np.ascontiguousarray(np.asarray(raw, np.float32).reshape(-1, 10))
-        out = np.zeros((len(raw), 40), np.float32)
+        out = np.zeros((len(raw), 41), np.float32)
         self._chk(self.lib.pt_guide_forward(self.h, raw.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(raw),
                                             out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
         return out
@@ -324,6 +325,11 @@ class Engine:
         self._chk(self.lib.pt_guide_train(self.h, smp.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(smp),
                                           grad.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.byref(loss)))
         return grad, loss.value
+
+    def guide_status(self, steps=4):
+        out = (ctypes.c_double * (2 + steps))()
+        self._chk(self.lib.pt_guide_status(self.h, out, 2 + steps))
+        return {'steps': int(out[0]), 'collected': int(out[1]), 'losses': list(out)[2:]}
 
     def pick_rays(self):
         """Each camera ray of one sample to its first event: (pos4 [n, 4], state [n], material [n], prim [n])."""

@@ -74,12 +74,14 @@ def samples_rows(n, box, target=None):
     else:
         w = guide.sample_vmf(np.tile(target, (n, 1)), np.full(n, 20.0), RNG.random(n), RNG.random(n))
     F = RNG.random(n) + 0.1 if target is None else np.ones(n)
+    # the BSDF's density at w: Lambertian about the normal, or uniform over the sphere for the lobe target
+    b = np.maximum(np.einsum('ij,ij->i', w, raw[:, 6:9]), 0) / np.pi if target is None else np.full(n, 1 / (4 * np.pi))
     rows = np.zeros((n, 16))
     rows[:, 0:3], rows[:, 3] = pos01, raw[:, 9]
     rows[:, 4:7], rows[:, 7] = raw[:, 3:6], F
-    rows[:, 8:11], rows[:, 12:15] = raw[:, 6:9], w
+    rows[:, 8:11], rows[:, 12:15], rows[:, 15] = raw[:, 6:9], w, b
     X = guide.encode(pos01, raw[:, 3:6], raw[:, 6:9], raw[:, 9])
-    return rows, X, w, F
+    return rows, X, w, F, b
 
 
 def test_gradient_and_adam_match_the_oracle(engine):
@@ -87,14 +89,14 @@ def test_gradient_and_adam_match_the_oracle(engine):
     made, box = setup(engine, lr=1e-2)
     theta = RNG.normal(0, 0.2, guide.nparams()).astype(np.float32)
     engine.guide_load(theta)
-    rows, X, w, F = samples_rows(512, box)
+    rows, X, w, F, b = samples_rows(512, box)
     grad, loss = engine.guide_train(rows)
-    want = guide.backward(bf16(theta), X, w, F)
+    want = guide.backward(bf16(theta), X, w, F, b)
     cos = grad @ want / (np.linalg.norm(grad) * np.linalg.norm(want))
     rel = np.linalg.norm(grad - want) / np.linalg.norm(want)
     assert cos > 0.995 and rel < 0.08, (cos, rel)
     o, _ = guide.forward(bf16(theta), X)
-    assert abs(loss / guide.loss(o, w, F) - 1) < 2e-2, (loss, guide.loss(o, w, F))
+    assert abs(loss / guide.loss(o, w, F, b) - 1) < 2e-2, (loss, guide.loss(o, w, F, b))
     # Adam's first step: every weight moves by lr g / (|g| + eps), g the summed gradient the step applied
     th1 = engine.guide_weights()
     step = 1e-2 * grad.astype(np.float64) / (np.abs(grad) + 1e-8)
@@ -103,14 +105,15 @@ def test_gradient_and_adam_match_the_oracle(engine):
 
 
 def test_training_on_a_lobe_lowers_the_loss(engine):
-    say('case: 60 Adam steps on directions drawn from one vMF lobe (kappa 20)')
+    say('case: 60 Adam steps on directions drawn from one vMF lobe (kappa 20), the BSDF uniform')
     made, box = setup(engine, lr=1e-2)
     target = unit(np.array([0.6, 0.0, 0.8]))
     losses = []
     for k in range(60):
-        rows, X, w, F = samples_rows(1024, box, target)
+        rows, X, w, F, b = samples_rows(1024, box, target)
         _, loss = engine.guide_train(rows)
         losses.append(loss)
+    # the combined density's floor: the lobe's own entropy, reached as c -> 1
     floor = -(guide.log_c(np.array(20.0)) - 1.0)
     say('loss %.3f -> %.3f (floor %.3f)' % (losses[0], losses[-1], floor))
     assert losses[-1] < losses[0] - 1.0 and np.mean(losses[-5:]) < floor + 0.8, (losses[0], losses[-5:], floor)
@@ -162,3 +165,74 @@ def test_guided_room_agrees_with_the_plain_tracer(engine):
         assert z < 4.5, (gm.mean(), pm.mean(), se)
     say('guided vs plain: worst region at %.2f sigma; per-sample spread guided %.4g plain %.4g'
         % (worst, g.mean(axis=(1, 2)).std(), p.mean(axis=(1, 2)).std()))
+
+
+def furnace_samples(engine, u, n):
+    """Each sample's mean over the floor rows (the running FP64 mean read after every sample)."""
+    engine.set_uniforms(u)
+    engine.clear_accums()
+    out, prev = [], None
+    for k in range(1, n + 1):
+        engine.trace(1)
+        Y = pt.luminance(engine.read_accum()[0])[:20].mean()
+        out.append(Y if prev is None else k * Y - (k - 1) * prev)
+        prev = Y
+    return np.array(out)
+
+
+@pytest.mark.parametrize('case,opts', [('metal, rough 0.3', {'roughness': 0.3, 'metallic': 1, 'specular': 0.5}),
+                                       ('plastic, rough 0.2', {'roughness': 0.2, 'specular': 0.5})])
+def test_guided_furnace_keeps_its_energy(engine, case, opts):
+    say('case: guided white furnace, %s, 64x64: trained 32 samples, then 64 guided against 64 plain' % case)
+    made = scenes.emit('furnace-plane', width=64, height=64, **opts)
+    engine.set_scene(made['scene'])
+    engine.set_uniforms(pt.uniforms_for(made))
+    engine.guide_config(pt.scene_box(made['scene']), batch=2048, steps=2, warmup=4)
+    plain = furnace_samples(engine, pt.uniforms_for(made), 64)
+    ug = pt.uniforms_for(made)
+    ug.guide, ug.guideFrac = 1, 0.9
+    engine.set_uniforms(ug)
+    engine.clear_accums()
+    engine.trace(32)
+    engine.reset_stats()
+    guided = furnace_samples(engine, ug, 64)
+    st = engine.stats()
+    se = math.sqrt(plain.var(ddof=1) / len(plain) + guided.var(ddof=1) / len(guided))
+    say('plain %.3f guided %.3f, se %.3f (guided vertices %d)' % (plain.mean(), guided.mean(), se, st['guided']))
+    assert st['guided'] > 0 and st['nonfinite'] == 0
+    assert abs(guided.mean() - plain.mean()) < 4 * se + 1e-3 * plain.mean(), (plain.mean(), guided.mean(), se)
+
+
+def fixed_theta(lobes, c):
+    """All weights 0; the head's biases alone set the mixture at every vertex: lobes [(logit, mean, log kappa)], c."""
+    th = np.zeros(guide.nparams(), np.float32)
+    b4 = th[-41:]
+    for j in range(8):
+        lg, mu, lk = lobes[j % len(lobes)]
+        b4[j] = lg if j < len(lobes) else -30.0
+        b4[8 + 3 * j:11 + 3 * j] = mu
+        b4[32 + j] = lk
+    b4[40] = math.log(c / (1 - c))
+    return th
+
+
+@pytest.mark.parametrize('name,lobes,c', [('up, kappa 2', [(0.0, [0, 0, 1], math.log(2.0))], 0.5),
+                                          ('tilted, kappa 8', [(0.0, [0.7, 0, 0.7], math.log(8.0))], 0.5),
+                                          ('two lobes', [(0.0, [0, 0, 1], math.log(20.0)), (0.5, [-0.6, 0.3, 0.7], math.log(4.0))], 0.7)])
+def test_fixed_mixture_furnace(engine, name, lobes, c):
+    # a proposal that is known and does not learn (learning rate ~0, no warm-up): the estimator's unbiasedness alone
+    say('case: white furnace, metal rough 0.3, guided by a fixed mixture (%s, c %.1f)' % (name, c))
+    made = scenes.emit('furnace-plane', width=64, height=64, roughness=0.3, metallic=1, specular=0.5)
+    engine.set_scene(made['scene'])
+    engine.set_uniforms(pt.uniforms_for(made))
+    engine.guide_config(pt.scene_box(made['scene']), batch=2048, steps=1, warmup=0, lr=1e-30)
+    engine.guide_load(fixed_theta(lobes, c))
+    o = engine.guide_forward(np.array([[0, 0, 0, 0, 0, 1, 0, 0, 1, 0.3]])).astype(np.float64)
+    assert abs(guide.frac(o)[0] - c) < 1e-2
+    plain = furnace_samples(engine, pt.uniforms_for(made), 48)
+    ug = pt.uniforms_for(made)
+    ug.guide, ug.guideFrac = 1, 0.9
+    guided = furnace_samples(engine, ug, 48)
+    se = math.sqrt(plain.var(ddof=1) / len(plain) + guided.var(ddof=1) / len(guided))
+    say('plain %.3f guided %.3f, se %.3f' % (plain.mean(), guided.mean(), se))
+    assert abs(guided.mean() - plain.mean()) < 4 * se + 1e-3 * plain.mean(), (plain.mean(), guided.mean(), se)

@@ -20,12 +20,13 @@ def random_batch(n):
     rough = RNG.random(n)
     w = RNG.normal(size=(n, 3)); w /= np.linalg.norm(w, axis=1, keepdims=True)
     F = RNG.random(n) + 0.1
-    return guide.encode(pos, wo, nn, rough), w, F
+    b = np.maximum(np.einsum('ij,ij->i', w, nn), 0) / np.pi     # a Lambertian BSDF's density (0 below)
+    return guide.encode(pos, wo, nn, rough), w, F, b
 
 
 def test_parameter_count():
-    # 80*64+64 + 2*(64*64+64) + 64*40+40
-    assert guide.nparams() == 16104
+    # 80*64+64 + 2*(64*64+64) + 64*41+41
+    assert guide.nparams() == 16169
 
 
 def test_encoding_at_exact_points():
@@ -40,31 +41,36 @@ def test_encoding_at_exact_points():
 
 
 def test_backward_matches_finite_differences():
+    # the surrogate L_q + L_c with q frozen at the point of differentiation
     theta = random_theta()
-    X, w, F = random_batch(24)
-    g = guide.backward(theta, X, w, F)
-    idx = RNG.choice(len(theta), 300, replace=False)
+    X, w, F, b = random_batch(24)
+    g = guide.backward(theta, X, w, F, b)
+    q0 = guide.mixture_pdf(guide.forward(theta, X)[0], w)
+    idx = np.concatenate([RNG.choice(len(theta), 300, replace=False), np.arange(len(theta) - 41, len(theta))])
     h = 1e-6
+    L = lambda th: guide.loss_surrogate(guide.forward(th, X)[0], w, F, b, q0)
     for i in idx:
         tp, tm = theta.copy(), theta.copy()
         tp[i] += h; tm[i] -= h
-        fd = (guide.loss(guide.forward(tp, X)[0], w, F) - guide.loss(guide.forward(tm, X)[0], w, F)) / (2 * h)
+        fd = (L(tp) - L(tm)) / (2 * h)
         assert abs(fd - g[i]) < 1e-6 + 1e-5 * abs(fd), (i, fd, g[i])
 
 
 def test_head_gradient_matches_finite_differences_at_high_and_low_kappa():
     # the head alone, with concentrations near both ends of the clamp and lobes near and far from w
-    o = RNG.normal(0, 1, (6, 40))
+    o = RNG.normal(0, 1, (6, 41))
     o[:, 32:40] = np.linspace(-2.9, 6.9, 8)
     w = RNG.normal(size=(6, 3)); w /= np.linalg.norm(w, axis=1, keepdims=True)
     F = RNG.random(6) + 0.5
-    g = guide.dloss_do(o, w, F)
+    b = np.array([0.0, 0.05, 0.2, 0.0, 1.0, 0.3])        # including directions the BSDF cannot sample
+    g = guide.dloss_do(o, w, F, b)
+    q0 = guide.mixture_pdf(o, w)
     h = 1e-6
     for n in range(6):
-        for j in range(40):
+        for j in range(41):
             op, om = o.copy(), o.copy()
             op[n, j] += h; om[n, j] -= h
-            fd = (guide.loss(op, w, F) - guide.loss(om, w, F)) / (2 * h)
+            fd = (guide.loss_surrogate(op, w, F, b, q0) - guide.loss_surrogate(om, w, F, b, q0)) / (2 * h)
             assert abs(fd - g[n, j]) < 1e-6 + 1e-5 * abs(fd), (n, j, fd, g[n, j])
 
 
@@ -81,7 +87,7 @@ def _sphere_quadrature(n):
 def test_mixture_integrates_to_one():
     w, dA = _sphere_quadrature(600)
     for kap in (-2.5, 0.0, 3.0, 5.5):
-        o = RNG.normal(0, 1, (1, 40))
+        o = RNG.normal(0, 1, (1, 41))
         o[0, 32:40] = kap
         q = guide.mixture_pdf(np.repeat(o, len(w), axis=0), w)
         assert abs(q.sum() * dA - 1) < 2e-3, (kap, q.sum() * dA)
@@ -119,18 +125,22 @@ def test_adam_first_step_moves_each_weight_by_lr():
 
 
 def test_training_lowers_the_loss_on_a_fixed_target():
-    # the loss of a lobe-shaped target falls under plain Adam on the oracle's own gradients
+    # the loss of a lobe-shaped target falls under plain Adam on the oracle's own gradients, and the network learns
+    # to trust itself (c -> 1) where the BSDF (here uniform over the sphere) is the poorer density
     theta = random_theta(0.1)
     n = 256
     pos = RNG.random((n, 3)); X = guide.encode(pos, np.tile([0, 0, 1.0], (n, 1)), np.tile([0, 0, 1.0], (n, 1)), np.full(n, 0.5))
     target = np.array([0.6, 0.0, 0.8])
     w = guide.sample_vmf(np.tile(target, (n, 1)), np.full(n, 20.0), RNG.random(n), RNG.random(n))
     F = np.ones(n)
-    l0 = guide.loss(guide.forward(theta, X)[0], w, F)
+    b = np.full(n, 1 / (4 * np.pi))
+    l0 = guide.loss(guide.forward(theta, X)[0], w, F, b)
     m = np.zeros_like(theta); v = np.zeros_like(theta)
     for t in range(1, 201):
-        theta, m, v = guide.adam(theta, guide.backward(theta, X, w, F), m, v, t, 1e-2)
-    l1 = guide.loss(guide.forward(theta, X)[0], w, F)
+        theta, m, v = guide.adam(theta, guide.backward(theta, X, w, F, b), m, v, t, 1e-2)
+    o1 = guide.forward(theta, X)[0]
+    l1 = guide.loss(o1, w, F, b)
     # the target's own entropy bound: -E log vMF(20) = -(log C(20) + E[20 (c - 1)]) = -(log C(20) - 1)
     floor = -(guide.log_c(np.array(20.0)) - 1.0)
     assert l1 < l0 - 1.0 and l1 < floor + 0.5, (l0, l1, floor)
+    assert guide.frac(o1).mean() > 0.9
