@@ -51,7 +51,8 @@ typedef struct pt_scene_desc {
 int pt_abi_version(void);
 
 /* sizeof of the ABI's structures, for bindings to check their mirrors: 0 PtUniforms, 1 PtMaterial,
- * 2 PtLightFace, 3 PtCamera, 4 PtEnv, 5 PtHaze, 6 pt_scene_desc, 7 pt_opts, 8 PtGuideConfig; 0 for anything else.
+ * 2 PtLightFace, 3 PtCamera, 4 PtEnv, 5 PtHaze, 6 pt_scene_desc, 7 pt_opts, 8 PtGuideConfig, 9 PtOpticsScene,
+ * 10 PtOpticsResult; 0 for anything else.
  * No counterpart in the reference (its uniform packing lives in the absent tinywebgpu submodule). */
 size_t pt_struct_size(int which);
 
@@ -138,6 +139,20 @@ int pt_guide_train(pt_ctx* ctx, const float* samples, size_t n, float* grad, dou
 /* The training's state after the last sample: out[0] Adam steps taken, out[1] training samples the sample
  * collected, out[2 + j] the loss of its step j (j < steps); n >= 2. */
 int pt_guide_status(pt_ctx* ctx, double* out, size_t n);
+
+/* ---- the optics bench ---- */
+
+/* Traces n luminaire scenes (PtOpticsScene) in one persistent launch on `device` and returns each one's numbers
+ * (PtOpticsResult): the flux balance, the far field's maximum, beam and field angles in the C0 and C90 planes, the
+ * escaped light's chromaticity, CCT and Duv, the lux map's maximum and centre. far (NULL, or n x PT_OPT_NG x
+ * PT_OPT_NC doubles) receives each scene's far field, lm per (gamma, C) bin; lux (NULL, or n x PT_OPT_LUX^2) its lux
+ * map at z = 1 m, lx; ms (NULL or 1) the launch's time. Needs no engine. On failure pt_optics_error() says why.
+ * Replaces createOptics's run and results (lib/pathtracer/optics/index.js) over optics/kernels.js:43-87 (gen, shade,
+ * tallies), optics/farfield.js:33-168 (farBin, intensity, farStats), optics/detmap.js:104-213 (the map) and
+ * core/colorimetry.js:58-104 (xyToCctDuv). */
+int pt_optics_trace(int device, const PtOpticsScene* scenes, uint32_t n, PtOpticsResult* out, double* far, double* lux,
+                    double* ms);
+const char* pt_optics_error(void);
 
 #ifdef __cplusplus
 }

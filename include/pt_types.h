@@ -143,6 +143,43 @@ typedef struct PtGuideConfig {
     unsigned int seed;     // weights' initialisation
 } PtGuideConfig;
 
+// One luminaire of the optics bench (metres, lumens; the LED at the origin emitting along +z): a Lambertian square
+// LED, a lens of two conic faces (z = vertex + sag(r), sag = c r^2 / (1 + sqrt(1 - (1 + k) c^2 r^2)); c 0 is a
+// plane) with an absorbing rim, an optional paraboloid reflector (focus at the LED) and an absorbing board at z = 0.
+typedef struct PtOpticsScene {
+    float ledHalf;         // half the LED's side (0.5 mm)
+    float flux;            // lm
+    float xyz[3];          // the LED's colour as X, Y, Z of one lumen (Y = 1)
+    float lensZ;           // the front face's vertex
+    float lensT;           // centre thickness (front vertex to back vertex)
+    float lensD;           // diameter
+    float c1, k1;          // front face: curvature (1 / R), conic constant
+    float c2, k2;          // back face
+    float ior;
+    float reflF;           // paraboloid focal length (0: no reflector)
+    float reflR0;          // its base radius (the hole round the LED)
+    float reflZ1;          // its mouth's z
+    float rho[3];          // its reflectance per X, Y, Z channel
+    float boardR;          // the absorbing board's radius at z = 0
+    unsigned int photons;
+    unsigned int seed;
+} PtOpticsScene;
+
+// An optics bench result (FP64 on the GPU): flux balance, the far field's numbers (farfield.js farStats) and colour
+// (colorimetry.js xyToCctDuv), the lux map's.
+typedef struct PtOpticsResult {
+    double emitted, escaped, absorbed, efficiency;   // lm, lm, lm, escaped / emitted
+    double Imax, gammaMax, cMax;                     // cd at the brightest bin, and where (deg)
+    double beam[2], field[2];                        // full widths at 50 % and 10 % of Imax in the C0-C180 and
+                                                     // C90-C270 planes, deg (-1: the plane never falls that far)
+    double X, Y, Z, x, y, cct, duv;                  // the escaped light's tristimulus totals and chromaticity
+    double luxMax, luxCentre;                        // the map at 1 m: its largest cell, the mean of its 4 central
+    double photons;
+} PtOpticsResult;
+#define PT_OPT_NG 181        // far field gamma rows (0, 1, ..., 180 deg; the first and last half wide)
+#define PT_OPT_NC 360        // far field C columns (0, 1, ..., 359 deg)
+#define PT_OPT_LUX 256       // lux map cells a side over [-1, 1] m at z = 1 m
+
 typedef struct PtUniforms {
     unsigned int width, height;
     unsigned int maxbounce;     // 10

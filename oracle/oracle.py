@@ -52,6 +52,9 @@ def lib():
         L.or_tiles.argtypes = [c_u, c_u, c_u, P(c_u), P(c_u)]
         L.or_sppm_radius.restype = c_d
         L.or_sppm_radius.argtypes = [c_d, c_d, c_u]
+        L.or_cct_duv.argtypes = [c_d, c_d, P(c_d), P(c_d)]
+        L.or_fresnel_unpolarised.restype = c_d
+        L.or_fresnel_unpolarised.argtypes = [c_d, c_d, P(c_d)]
         _lib = L
     return _lib
 
@@ -155,3 +158,39 @@ def mf_tables():
     out = np.zeros(32 * 32 * 16 + 16 * 32 + 32 * 32, np.float64)
     lib().or_mf_tables(_dp(out))
     return out
+
+
+# ---------------------------------------------------------------------------------------------------- optics bench
+def optics_trace(scenes, threads=0, tallies=False):
+    """The bench in FP64 on the CPU: (results as dicts, far [n, 181, 360] lm, lux [n, 256, 256] lx) -- the tallies only
+    when asked."""
+    from shadytracer_amd import pt
+    n = len(scenes)
+    arr = (pt.PtOpticsScene * n)(*scenes)
+    res = (pt.PtOpticsResult * n)()
+    F = np.zeros((n, pt.OPT_NG, pt.OPT_NC)) if tallies else None
+    Lx = np.zeros((n, pt.OPT_LUX, pt.OPT_LUX)) if tallies else None
+    lib().or_optics_trace(arr, n, res, _dp(F) if tallies else None, _dp(Lx) if tallies else None, threads)
+    return [pt.result_dict(r) for r in res], F, Lx
+
+
+def optics_numbers(scene, far, lux, tot):
+    """The numbers of one scene from given tallies (far lm, lux lx, tot: X, Y, Z escaped, Y absorbed, photons)."""
+    from shadytracer_amd import pt
+    r = pt.PtOpticsResult()
+    far = np.ascontiguousarray(far, np.float64); lux = np.ascontiguousarray(lux, np.float64)
+    tot = np.ascontiguousarray(tot, np.float64)
+    lib().or_optics_numbers(ctypes.byref(scene), _dp(far), _dp(lux), _dp(tot), ctypes.byref(r))
+    return pt.result_dict(r)
+
+
+def cct_duv(x, y):
+    c, d = ctypes.c_double(), ctypes.c_double()
+    lib().or_cct_duv(x, y, ctypes.byref(c), ctypes.byref(d))
+    return c.value, d.value
+
+
+def fresnel_unpolarised(cos_i, eta):
+    t = ctypes.c_double()
+    return lib().or_fresnel_unpolarised(cos_i, eta, ctypes.byref(t))
+

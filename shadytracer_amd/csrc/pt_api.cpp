@@ -538,6 +538,8 @@ size_t pt_struct_size(int which) {
         case 6: return sizeof(pt_scene_desc);
         case 7: return sizeof(pt_opts);
         case 8: return sizeof(PtGuideConfig);
+        case 9: return sizeof(PtOpticsScene);
+        case 10: return sizeof(PtOpticsResult);
         default: return 0;
     }
 }

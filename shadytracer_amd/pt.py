@@ -65,12 +65,36 @@ class PtGuideConfig(ctypes.Structure):
                 ('seed', ctypes.c_uint)]
 
 
+class PtOpticsScene(ctypes.Structure):
+    _fields_ = [('ledHalf', ctypes.c_float), ('flux', ctypes.c_float), ('xyz', ctypes.c_float * 3), ('lensZ', ctypes.c_float),
+                ('lensT', ctypes.c_float), ('lensD', ctypes.c_float), ('c1', ctypes.c_float), ('k1', ctypes.c_float),
+                ('c2', ctypes.c_float), ('k2', ctypes.c_float), ('ior', ctypes.c_float), ('reflF', ctypes.c_float),
+                ('reflR0', ctypes.c_float), ('reflZ1', ctypes.c_float), ('rho', ctypes.c_float * 3), ('boardR', ctypes.c_float),
+                ('photons', ctypes.c_uint), ('seed', ctypes.c_uint)]
+
+
+class PtOpticsResult(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in ('emitted', 'escaped', 'absorbed', 'efficiency', 'Imax', 'gammaMax', 'cMax')] + \
+               [('beam', ctypes.c_double * 2), ('field', ctypes.c_double * 2)] + \
+               [(k, ctypes.c_double) for k in ('X', 'Y', 'Z', 'x', 'y', 'cct', 'duv', 'luxMax', 'luxCentre', 'photons')]
+
+
+OPT_NG, OPT_NC, OPT_LUX = 181, 360, 256
+
+
+def result_dict(r):
+    d = {k: getattr(r, k) for k, _ in PtOpticsResult._fields_ if k not in ('beam', 'field')}
+    d['beam'], d['field'] = list(r.beam), list(r.field)
+    return d
+
+
 class PtOpts(ctypes.Structure):
     _fields_ = [('device', ctypes.c_int), ('blocks_per_cu', ctypes.c_uint32), ('cache_dir', ctypes.c_char_p),
                 ('reserved', ctypes.c_uint32 * 4)]
 
 
-STRUCTS = [PtUniforms, PtMaterial, PtLightFace, PtCamera, PtEnv, PtHaze, PtSceneDesc, PtOpts, PtGuideConfig]
+STRUCTS = [PtUniforms, PtMaterial, PtLightFace, PtCamera, PtEnv, PtHaze, PtSceneDesc, PtOpts, PtGuideConfig, PtOpticsScene,
+           PtOpticsResult]
 
 # the C ABI, name -> (restype, argtypes)
 SIGNATURES = {
@@ -98,6 +122,9 @@ SIGNATURES = {
     'pt_guide_train': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_size_t, ctypes.POINTER(ctypes.c_float),
                                       ctypes.POINTER(ctypes.c_double)]),
     'pt_guide_status': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_size_t]),
+    'pt_optics_trace': (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(PtOpticsScene), ctypes.c_uint32, ctypes.POINTER(PtOpticsResult),
+                                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    'pt_optics_error': (ctypes.c_char_p, []),
 }
 GUIDE_NP = 16169
 
@@ -383,4 +410,20 @@ def scene_box(scene):
         return [-R] * 3, [R] * 3
     pad = [0.01 * (h - l) + 1e-3 for l, h in zip(lo, hi)]
     return [l - p for l, p in zip(lo, pad)], [h + p for h, p in zip(hi, pad)]
+
+
+def optics_trace(scenes, far=False, lux=False, device=0):
+    """The optics bench on the GPU: (results as dicts, far [n, 181, 360] lm or None, lux [n, 256, 256] lx or None, ms)."""
+    lib = load_library()
+    n = len(scenes)
+    arr = (PtOpticsScene * n)(*scenes)
+    res = (PtOpticsResult * n)()
+    F = np.zeros((n, OPT_NG, OPT_NC)) if far else None
+    L = np.zeros((n, OPT_LUX, OPT_LUX)) if lux else None
+    ms = ctypes.c_double(0.0)
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if a is not None else None
+    rc = lib.pt_optics_trace(device, arr, n, res, dp(F), dp(L), ctypes.byref(ms))
+    if rc:
+        raise PtError(rc, lib.pt_optics_error().decode())
+    return [result_dict(r) for r in res], F, L, ms.value
 
