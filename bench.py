@@ -8,6 +8,7 @@ the timed steps; shadow rays are reported beside it.
 
     python bench.py [--gpus 1] [--steps 32] [--warmup 4] [--no-cpu]
     python bench.py --guide [--scene caustic-c3] [--width 640 --height 360] [--train 300] [--steps 128]
+    python bench.py --optics [--steps 8] [--no-cpu]
 
 Prints one JSON line. With --guide it measures the learned guiding against the plain tracer instead: each runs the
 same scene sample by sample; per-pixel variances come from the samples' own spread (both estimators are unbiased, the
@@ -15,6 +16,8 @@ GPU tests show their means agree), and the seconds each needs to reach 1 % relat
 its mean, at the image's pixels and over 4 x 4 bins -- follow as (training time) + var / (0.01 mean)^2 x (seconds a
 sample). The roofline is the dominant kernel's (sphere tracing: FP32 VALU), from HIP events on the
 engine's stream; `cpu_baseline` times the FP64 C oracle on this host's cores on a bounded slice of the same workload.
+With --optics it measures the optics bench (photons a second over the walk and numbers kernels, every number's error
+against the FP64 oracle on the same photons).
 """
 import argparse
 import json
@@ -146,9 +149,81 @@ def bench_guide(a):
     print(json.dumps(out), flush=True)
 
 
+def optics_traffic_from_profiles():
+    """HBM bytes per walk launch, from the committed rocprofv3 --pmc summary of the optics bench, if there is one."""
+    p = os.path.join(ROOT, 'profiles', 'optics_walk_64scenes_traffic.json')
+    if not os.path.exists(p):
+        return None
+    try:
+        return json.load(open(p))['bytes_per_launch']
+    except Exception:
+        return None
+
+
+def bench_optics(a):
+    """The optics bench: 64 seeded luminaires (LED, plano-convex or aspheric lens, half of them in a paraboloid
+    reflector; 2^18..2^24 photons each) in one persistent launch, FP64 far fields and lux maps, numbers on the GPU.
+    A step is one such launch. The FP64 CPU oracle traces the same photons (same PCG4D draws) for the error of every
+    number and the CPU baseline."""
+    import numpy as np
+    from shadytracer_amd import optics, pt
+    sc = optics.bench_scenes(64)
+    total = sum(s.photons for s in sc)
+    log('bench: optics, 64 luminaires, %d photons, warmup %d, steps %d' % (total, a.warmup, a.steps))
+    for _ in range(max(1, a.warmup)):
+        pt.optics_trace(sc)
+    walk, numbers, wall = [], [], []
+    res = None
+    for _ in range(a.steps):
+        t0 = time.perf_counter()
+        res, _, _, (mw, mn) = pt.optics_trace(sc)
+        wall.append(time.perf_counter() - t0)
+        walk.append(mw)
+        numbers.append(mn)
+    kernel_ms = float(np.mean(walk)) + float(np.mean(numbers))
+    flops = sum(r['flops'] for r in res)
+    events = sum(r['events'] for r in res)
+    achieved = flops / (np.mean(walk) * 1e-3) / 1e12
+    roof = {'kernel': 'pt_optics_walk', 'bound': 'valu', 'achieved': round(achieved, 3), 'peak': PEAK_FP32_TFLOPS,
+            'unit': 'TFLOP/s', 'frac': round(achieved / PEAK_FP32_TFLOPS, 4), 'traffic': optics_traffic_from_profiles(),
+            'flops_per_launch': flops, 'avg_launch_ms': float(np.mean(walk)), 'flops_per_surface_test': flops / events,
+            'surface_tests_per_photon': events / total, 'numbers_kernel_ms': float(np.mean(numbers))}
+    out = {'metric': 'Mphotons/s (optics bench, 64 luminaires, one persistent launch)', 'value': total / kernel_ms / 1e3,
+           'unit': 'Mphotons/s', 'n_gpus': 1, 'steps': a.steps, 'warmup': a.warmup, 'ms_per_step': kernel_ms,
+           'ms_per_step_with_uploads': float(np.mean(wall)) * 1e3, 'higher_is_better': True, 'scaling': 'weak',
+           'vs_baseline': None, 'dtype': 'f32 walk, f64 tallies and numbers',
+           'data': 'synthetic: 64 seeded luminaires (shadytracer_amd/optics.py bench_scenes, seed 20261203)',
+           'config': {'workload': 'optics bench: 64 luminaires, Lambertian 1x1 mm LED 100 lm, plano-convex or aspheric '
+                                  'lens, optional paraboloid reflector, 181x360 candela bins + 256x256 lux at 1 m, '
+                                  '2^18..2^24 photons each', 'scenes': 64, 'photons': total},
+           'roofline': roof, 'cpu_baseline': None}
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+        import oracle
+        threads = int(os.environ.get('OMP_NUM_THREADS', '0') or 0) or os.cpu_count() or 1
+        t0 = time.perf_counter()
+        ores, _, _ = oracle.optics_trace(sc, threads=threads)
+        dt = time.perf_counter() - t0
+        out['cpu_baseline'] = {'value': total / dt / 1e6, 'unit': 'Mphotons/s', 'cores': threads, 'kind': 'port',
+                               'sample': 'the whole workload (the same 64 scenes and photons), FP64 C oracle with OpenMP; '
+                                         '%.1f s' % dt}
+        # every number against the oracle's on the same photons (the GPU walks in FP32: a few photons change bins)
+        err = {}
+        for key in ('beam', 'field'):
+            err[key + '_deg'] = max(abs(g[key][0] - o[key][0]) for g, o in zip(res, ores))
+        err['cct_K'] = max(abs(g['cct'] - o['cct']) for g, o in zip(res, ores))
+        err['duv'] = max(abs(g['duv'] - o['duv']) for g, o in zip(res, ores))
+        err['Imax_rel'] = max(abs(g['Imax'] / o['Imax'] - 1) for g, o in zip(res, ores))
+        err['efficiency'] = max(abs(g['efficiency'] - o['efficiency']) for g, o in zip(res, ores))
+        err['luxCentre_rel'] = max(abs(g['luxCentre'] / o['luxCentre'] - 1) for g, o in zip(res, ores))
+        out['max_error_vs_fp64_oracle'] = err
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--guide', action='store_true', help='guided against plain: seconds to 1% relative RMSE')
+    ap.add_argument('--optics', action='store_true', help='the optics bench: 64 luminaires in one persistent launch')
     ap.add_argument('--scene', default='caustic-c3')
     ap.add_argument('--train', type=int, default=300)
     ap.add_argument('--calibrate-mib', type=int, default=0, help='a device copy of this many MiB first (counter scale)')
@@ -171,9 +246,14 @@ def main():
             a.steps = 128
         bench_guide(a)
         return
-    from shadytracer_amd import pt, scenes
     if a.calibrate_mib:
         calibration_copy(a.calibrate_mib)
+    if a.optics:
+        if a.steps == 32:
+            a.steps = 8
+        bench_optics(a)
+        return
+    from shadytracer_amd import pt, scenes
     made = scenes.emit('room-c2', width=a.width, height=a.height)
     log('bench: room-c2 %dx%d, warmup %d, steps %d' % (a.width, a.height, a.warmup, a.steps))
     e = pt.Engine()

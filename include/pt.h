@@ -146,7 +146,8 @@ int pt_guide_status(pt_ctx* ctx, double* out, size_t n);
  * (PtOpticsResult): the flux balance, the far field's maximum, beam and field angles in the C0 and C90 planes, the
  * escaped light's chromaticity, CCT and Duv, the lux map's maximum and centre. far (NULL, or n x PT_OPT_NG x
  * PT_OPT_NC doubles) receives each scene's far field, lm per (gamma, C) bin; lux (NULL, or n x PT_OPT_LUX^2) its lux
- * map at z = 1 m, lx; ms (NULL or 1) the launch's time. Needs no engine. On failure pt_optics_error() says why.
+ * map at z = 1 m, lx; ms (NULL, or 2 doubles) the walk kernel's time and the numbers kernel's, in ms (HIP events on
+ * the launch's stream; uploads and allocation outside). Needs no engine. On failure pt_optics_error() says why.
  * Replaces createOptics's run and results (lib/pathtracer/optics/index.js) over optics/kernels.js:43-87 (gen, shade,
  * tallies), optics/farfield.js:33-168 (farBin, intensity, farStats), optics/detmap.js:104-213 (the map) and
  * core/colorimetry.js:58-104 (xyToCctDuv). */

@@ -175,6 +175,9 @@ typedef struct PtOpticsResult {
     double X, Y, Z, x, y, cct, duv;                  // the escaped light's tristimulus totals and chromaticity
     double luxMax, luxCentre;                        // the map at 1 m: its largest cell, the mean of its 4 central
     double photons;
+    double events;                                   // surface tests the walks made (one a bounce: every surface tried)
+    double flops;                                    // FP32 operations the walks performed, from per-step source counts
+                                                     // (0 from the CPU oracle)
 } PtOpticsResult;
 #define PT_OPT_NG 181        // far field gamma rows (0, 1, ..., 180 deg; the first and last half wide)
 #define PT_OPT_NC 360        // far field C columns (0, 1, ..., 359 deg)

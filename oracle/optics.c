@@ -108,7 +108,7 @@ double or_fresnel_unpolarised(double cosI, double eta, double* cosT) {
     return 0.5 * (rs * rs + rp * rp);
 }
 
-/* one photon; far (NG x NC) and lux (LUX x LUX) in lm, acc: X, Y, Z escaped, Y absorbed */
+/* one photon; far (NG x NC) and lux (LUX x LUX) in lm, acc: X, Y, Z escaped, Y absorbed, surface tests */
 static void tracePhoton(const PtOpticsScene* S, uint32_t i, double* far, double* lux, double* acc) {
     double u[4];
     ORng r = orngAt(S->seed, i, 0);
@@ -128,6 +128,7 @@ static void tracePhoton(const PtOpticsScene* S, uint32_t i, double* far, double*
     for (int b = 0; b < OPT_BOUNCES; ++b) {
         double tBest = 1e300, t;
         int what = -1;
+        acc[4] += 1;
         t = hitConic(o, d, zv1, S->c1, S->k1, rim, eps); if (t < tBest) { tBest = t; what = 0; }
         t = hitConic(o, d, zv2, S->c2, S->k2, rim, eps); if (t < tBest) { tBest = t; what = 1; }
         {
@@ -323,18 +324,19 @@ int or_optics_trace(const PtOpticsScene* scenes, int n, PtOpticsResult* out, dou
         const PtOpticsScene* S = &scenes[s];
         memset(far, 0, FW * sizeof(double));
         memset(lux, 0, LW * sizeof(double));
-        double tot[5] = {0, 0, 0, 0, (double)S->photons};
+        double tot[5] = {0, 0, 0, 0, (double)S->photons}, events = 0;
         #pragma omp parallel
         {
-            double acc[4] = {0, 0, 0, 0};
+            double acc[5] = {0, 0, 0, 0, 0};
             #pragma omp for schedule(dynamic, 1024)
             for (int64_t i = 0; i < (int64_t)S->photons; ++i) tracePhoton(S, (uint32_t)i, far, lux, acc);
             #pragma omp critical
-            { for (int k = 0; k < 4; ++k) tot[k] += acc[k]; }
+            { for (int k = 0; k < 4; ++k) tot[k] += acc[k]; events += acc[4]; }
         }
         double area = (2.0 / PT_OPT_LUX) * (2.0 / PT_OPT_LUX);
         for (size_t k = 0; k < LW; ++k) lux[k] /= area;
         or_optics_numbers(S, far, lux, tot, &out[s]);
+        out[s].events = events;
         if (farOut) memcpy(farOut + FW * s, far, FW * sizeof(double));
         if (luxOut) memcpy(luxOut + LW * s, lux, LW * sizeof(double));
     }

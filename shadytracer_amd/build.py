@@ -59,7 +59,8 @@ def build_lib():
     write_embedded()
     tmp = LIB + '.tmp'
     run([HIPCC, '--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared', '-Wall', '-I', INCLUDE,
-         '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops', '-o', tmp,
+         '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
+         '-mllvm', '-amdgpu-atomic-optimizer-strategy=DPP', '-o', tmp,
          os.path.join(CSRC, 'pt_api.cpp'), os.path.join(CSRC, 'pt_optics.hip'), '-lhiprtc'])
     os.replace(tmp, LIB)
     return LIB

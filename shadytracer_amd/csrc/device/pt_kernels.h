@@ -19,15 +19,23 @@ __device__ __forceinline__ unsigned int rankIn(unsigned long long m) {
     unsigned int l = laneId();
     return (unsigned int)__popcll(m & ((1ull << l) - 1ull));
 }
+// An atomic add on device memory through a global (address space 1) pointer: the compiler's atomic optimizer rewrites
+// these, not flat ones, into one atomic per wave (and broadcasts the result itself).
+typedef __attribute__((address_space(1))) unsigned int PtGlobalU32;
+__device__ __forceinline__ unsigned int gAtomicAdd(unsigned int* p, unsigned int v) {
+    return __hip_atomic_fetch_add((PtGlobalU32*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Reserve one slot per predicated lane at *counter; returns this lane's index (meaningless where !pred).
+// The atomic optimizer makes the predicated lanes' increments one atomic per wave, the slots in lane order; without it
+// each lane's own atomic still gives it a slot of its own. No branch on a lane id precedes a cross-lane read here:
+// with `if (lane == leader) base = atomicAdd(..); base = __shfl(base, leader)`, the compiler may thread the leader's
+// branch round an enclosing loop apart from the other lanes, whose broadcast then reads the leader while it is
+// masked off.
 __device__ __forceinline__ unsigned int waveAppend(unsigned int* counter, bool pred) {
-    unsigned long long m = __ballot(pred);
-    if (m == 0ull) return 0u;
-    int leader = __ffsll((long long)m) - 1;
-    unsigned int base = 0u;
-    if ((int)laneId() == leader) base = atomicAdd(counter, (unsigned int)__popcll(m));
-    base = __shfl(base, leader);
-    return base + rankIn(m);
+    unsigned int slot = 0u;
+    if (pred) slot = gAtomicAdd(counter, 1u);
+    return slot;
 }
 __device__ __forceinline__ unsigned long long waveSum64(unsigned long long v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -67,11 +75,14 @@ __device__ __forceinline__ unsigned int fetchNext(WaveFetch& f, unsigned int* he
     const unsigned int chunk = (count + PT_PARTS - 1u) / PT_PARTS;
     while (need != 0ull && !f.done) {
         if (f.cur >= f.end) {
-            unsigned int lo = min(f.part * chunk, count), hi = min(lo + chunk, count);
-            unsigned int base = __atomic_load_n(&heads[f.part], __ATOMIC_RELAXED);
+            const unsigned int part = __builtin_amdgcn_readfirstlane(f.part);   // uniform: a scalar address
+            unsigned int lo = min(part * chunk, count), hi = min(lo + chunk, count);
+            unsigned int base = __builtin_amdgcn_readfirstlane(__atomic_load_n(&heads[part], __ATOMIC_RELAXED));
             if (lo + base < hi) {
-                if (laneId() == 0u) base = atomicAdd(&heads[f.part], PT_GRAB);
-                base = __shfl(base, 0);
+                // every active lane executes the grab, the first adding PT_GRAB and the rest 0, and the wave takes
+                // the first's return: no branch on the lane before the broadcast (see waveAppend)
+                const unsigned int first = (unsigned int)__ffsll((long long)__ballot(1)) - 1u;
+                base = __builtin_amdgcn_readfirstlane(gAtomicAdd(&heads[part], laneId() == first ? PT_GRAB : 0u));
             }
             if (lo + base < hi) { f.cur = lo + base; f.end = min(lo + base + PT_GRAB, hi); }
             else { f.part = (f.part + 1u) % PT_PARTS; if (--f.left == 0u) f.done = true; }

@@ -89,7 +89,10 @@ bool writeFileAtomic(const std::string& path, const std::vector<char>& data) {
     return rename(tmp.c_str(), path.c_str()) == 0;
 }
 
-const char* kRtcOpts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+// The atomic optimizer's DPP strategy scans a wave's divergent addends in a few DPP adds (the default loops over the
+// active lanes), for the queue grabs whose first lane alone adds.
+const char* kRtcOpts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                          "-mllvm", "-amdgpu-atomic-optimizer-strategy=DPP"};
 const int kRtcNopts = sizeof(kRtcOpts) / sizeof(kRtcOpts[0]);
 
 std::string fullSource(const char* scene) {

@@ -25,7 +25,7 @@
 
 #define PT_OPT_CHUNK 4096u
 #define PT_OPT_BOUNCES 32
-#define PT_OPT_TOT 8          // per chunk: X, Y, Z escaped, Y absorbed, photons traced, spare
+#define PT_OPT_TOT 8          // per chunk: X, Y, Z escaped, Y absorbed, photons traced, surface tests, FP32 ops, spare
 
 namespace {
 
@@ -73,16 +73,40 @@ __device__ __forceinline__ float sag(float c, float k, float r2) {
     float q = 1.0f - (1.0f + k) * c * c * r2;
     return c * r2 / (1.0f + sqrtf(fmaxf(q, 0.0f)));
 }
+// WORK COUNT: each step below adds the FP32 operations its source performs to the lane's tally `fl` (+, -, *, /, sqrt
+// and each transcendental one apiece; comparisons, min/max, abs, negation and selects none). The bench's roofline
+// divides their sum by the walk's time.
+#define OPF_PLANE 12.0f       // hitConic, a flat face
+#define OPF_CONIC 38.0f       // hitConic, a curved face: the quadratic's set-up and roots
+#define OPF_ROOT 9.0f         //   each root tried: the point and its radius
+#define OPF_SAG 14.0f         //   each root within the rim: the sag test
+#define OPF_RIM 16.0f         // the rim cylinder's quadratic
+#define OPF_RIM_ROOT 5.0f     //   each of its roots tried (one sqrt before the first)
+#define OPF_REFL 24.0f        // the paraboloid's quadratic and roots
+#define OPF_REFL_ROOT 2.0f
+#define OPF_BOARD 11.0f
+#define OPF_MOVE 6.0f
+#define OPF_MIRROR 40.0f      // the paraboloid's normal, the reflection, the weights
+#define OPF_NORMAL 19.0f      // a curved face's normal
+#define OPF_FRESNEL 30.0f     // cos, eta, Fresnel, the uniform
+#define OPF_REFLECT 17.0f
+#define OPF_REFRACT 21.0f
+#define OPF_ESCAPE 8.0f       // gamma, C and their bins
+#define OPF_LUX 10.0f         // the crossing of z = 1 m and its cell
+#define OPF_EMIT 45.0f        // the LED's draw, the weights, the faces' rim heights
+
 // The nearest t > tmin where the ray meets the conic face (vertex zv, within radius rmax), on the branch through the
 // vertex; 1e30 for none. The face: c (x^2 + y^2) - 2 (z - zv) + c (1 + k) (z - zv)^2 = 0.
-__device__ __forceinline__ float hitConic(F3 o, F3 d, float zv, float c, float k, float rmax, float tmin) {
+__device__ __forceinline__ float hitConic(F3 o, F3 d, float zv, float c, float k, float rmax, float tmin, float& fl) {
     const float R2 = rmax * rmax;
     if (c == 0.0f) {
+        fl += OPF_PLANE;
         if (fabsf(d.z) < 1e-12f) return 1e30f;
         float t = (zv - o.z) / d.z;
         F3 p = o + d * t;
         return (t > tmin && p.x * p.x + p.y * p.y <= R2) ? t : 1e30f;
     }
+    fl += OPF_CONIC;
     const float Z = o.z - zv, ck = c * (1.0f + k);
     const float A = c * (d.x * d.x + d.y * d.y) + ck * d.z * d.z;
     const float B = 2.0f * (c * (o.x * d.x + o.y * d.y) - d.z + ck * Z * d.z);
@@ -102,9 +126,11 @@ __device__ __forceinline__ float hitConic(F3 o, F3 d, float zv, float c, float k
     for (int i = 0; i < 2; ++i) {
         const float t = i == 0 ? t0 : t1;
         if (!(t > tmin)) continue;
+        fl += OPF_ROOT;
         F3 p = o + d * t;
         const float r2 = p.x * p.x + p.y * p.y;
         if (r2 > R2) continue;
+        fl += OPF_SAG;
         // on the branch through the vertex
         if (fabsf((p.z - zv) - sag(c, k, r2)) > 1e-4f * (1.0f + fabsf(1.0f / c))) continue;
         return t;
@@ -135,8 +161,9 @@ __device__ __forceinline__ double waveSumD(double v) {
 }
 
 // One photon of scene `s`: emitted from the LED, walked through lens, rim, reflector and board, tallied where it
-// leaves. acc: X, Y, Z escaped, Y absorbed (this lane's running sums).
-__device__ void tracePhoton(const OptDev& D, unsigned int s, const PtOpticsScene& S, unsigned int i, double* acc) {
+// leaves. acc: X, Y, Z escaped, Y absorbed (this lane's running sums); ev, fl: its surface tests and FP32 operations.
+__device__ void tracePhoton(const OptDev& D, unsigned int s, const PtOpticsScene& S, unsigned int i, double* acc,
+                            float& ev, float& fl) {
     ORng r = orngAt(S.seed, i, 0u);
     const float4 u = onext(r);
     F3 o = f3(S.ledHalf * (2.0f * u.x - 1.0f), S.ledHalf * (2.0f * u.y - 1.0f), 0.0f);
@@ -152,22 +179,27 @@ __device__ void tracePhoton(const OptDev& D, unsigned int s, const PtOpticsScene
     const float f4 = 4.0f * S.reflF, zb = refl ? S.reflR0 * S.reflR0 / f4 - S.reflF : 0.0f;
     bool inLens = false;
     const float eps = 1e-6f;
+    fl += OPF_EMIT;
     for (int b = 0; b < PT_OPT_BOUNCES; ++b) {
         // the nearest surface
         float tBest = 1e30f;
         int what = -1;   // 0 front face, 1 back face, 2 rim, 3 reflector, 4 board
-        float t = hitConic(o, d, zv1, S.c1, S.k1, rim, eps);
+        ev += 1.0f;
+        float t = hitConic(o, d, zv1, S.c1, S.k1, rim, eps, fl);
         if (t < tBest) { tBest = t; what = 0; }
-        t = hitConic(o, d, zv2, S.c2, S.k2, rim, eps);
+        t = hitConic(o, d, zv2, S.c2, S.k2, rim, eps, fl);
         if (t < tBest) { tBest = t; what = 1; }
         {   // the rim: x^2 + y^2 = rim^2 between the faces' edges
             const float A = d.x * d.x + d.y * d.y, B = 2.0f * (o.x * d.x + o.y * d.y), C = o.x * o.x + o.y * o.y - rim * rim;
             const float disc = B * B - 4.0f * A * C;
+            fl += OPF_RIM;
             if (A > 1e-12f && disc >= 0.0f) {
                 const float sq = sqrtf(disc);
+                fl += 1.0f;
                 for (int k = 0; k < 2; ++k) {
                     const float tt = (-B + (k == 0 ? -sq : sq)) / (2.0f * A);
                     const float z = o.z + d.z * tt;
+                    fl += OPF_RIM_ROOT;
                     if (tt > eps && tt < tBest && z >= fminf(zr1, zr2) && z <= fmaxf(zr1, zr2)) { tBest = tt; what = 2; break; }
                 }
             }
@@ -177,6 +209,7 @@ __device__ void tracePhoton(const OptDev& D, unsigned int s, const PtOpticsScene
             const float B = 2.0f * (o.x * d.x + o.y * d.y) - f4 * d.z;
             const float C = o.x * o.x + o.y * o.y - f4 * (o.z + S.reflF);
             float t0 = 1e30f, t1 = 1e30f;
+            fl += OPF_REFL;
             if (fabsf(A) < 1e-12f) { if (fabsf(B) > 1e-20f) t0 = t1 = -C / B; }
             else {
                 const float disc = B * B - 4.0f * A * C;
@@ -189,12 +222,14 @@ __device__ void tracePhoton(const OptDev& D, unsigned int s, const PtOpticsScene
             for (int k = 0; k < 2; ++k) {
                 const float tt = k == 0 ? t0 : t1;
                 const float z = o.z + d.z * tt;
+                fl += OPF_REFL_ROOT;
                 if (tt > eps && tt < tBest && z >= zb && z <= S.reflZ1) { tBest = tt; what = 3; break; }
             }
         }
         if (d.z < 0.0f && !inLens) {   // the board at z = 0
             const float tt = -o.z / d.z;
             const F3 p = o + d * tt;
+            fl += OPF_BOARD;
             if (tt > eps && tt < tBest && p.x * p.x + p.y * p.y <= S.boardR * S.boardR) { tBest = tt; what = 4; }
         }
         if (what < 0) {
@@ -205,10 +240,12 @@ __device__ void tracePhoton(const OptDev& D, unsigned int s, const PtOpticsScene
             const int ig = min((int)floorf(g * (1.0f / 3.14159265f) * (PT_OPT_NG - 1) + 0.5f), PT_OPT_NG - 1);
             const int ic = (int)floorf(c * (1.0f / (2.0f * 3.14159265f)) * PT_OPT_NC + 0.5f) % PT_OPT_NC;
             atomicAdd(&D.far[(size_t)s * PT_OPT_NG * PT_OPT_NC + ig * PT_OPT_NC + ic], (double)wY);
+            fl += OPF_ESCAPE;
             // the lux map: where the line crosses z = 1 m
             if (d.z > 0.0f && o.z < 1.0f) {
                 const float tt = (1.0f - o.z) / d.z;
                 const float x = o.x + d.x * tt, y = o.y + d.y * tt;
+                fl += OPF_LUX;
                 if (fabsf(x) < 1.0f && fabsf(y) < 1.0f) {
                     const int cx = min((int)((x + 1.0f) * (0.5f * PT_OPT_LUX)), PT_OPT_LUX - 1);
                     const int cy = min((int)((y + 1.0f) * (0.5f * PT_OPT_LUX)), PT_OPT_LUX - 1);
@@ -219,6 +256,7 @@ __device__ void tracePhoton(const OptDev& D, unsigned int s, const PtOpticsScene
             return;
         }
         o = o + d * tBest;
+        fl += OPF_MOVE;
         if (what == 2 || what == 4) { acc[3] += wY; return; }   // rim and board absorb
         if (what == 3) {
             // specular reflection off the paraboloid, tinted
@@ -226,9 +264,11 @@ __device__ void tracePhoton(const OptDev& D, unsigned int s, const PtOpticsScene
             d = norm3(d - n * (2.0f * dot3(d, n)));
             acc[3] += wY * (1.0f - S.rho[1]);   // what the mirror keeps
             wX *= S.rho[0]; wY *= S.rho[1]; wZ *= S.rho[2];
+            fl += OPF_MIRROR;
             continue;
         }
         // a lens face: Fresnel's choice between reflection and refraction
+        const float cf = what == 0 ? S.c1 : S.c2;
         F3 n = what == 0 ? conicNormal(o, zv1, S.c1, S.k1) : conicNormal(o, zv2, S.c2, S.k2);
         // the outward normal: the front face's points away from the lens (-z), the back face's to +z
         if (what == 0) n = n * -1.0f;
@@ -240,11 +280,14 @@ __device__ void tracePhoton(const OptDev& D, unsigned int s, const PtOpticsScene
         const float F = fresnelU(cosI, eta, cosT);
         ORng rr = orngAt(S.seed, i, (unsigned int)b + 1u);
         const float4 uf = onext(rr);
+        fl += OPF_FRESNEL + (cf != 0.0f ? OPF_NORMAL : 0.0f);
         if (uf.x < F) {
             d = norm3(d + n * (2.0f * cosI));
+            fl += OPF_REFLECT;
         } else {
             d = norm3(d * eta + n * (eta * cosI - cosT));
             inLens = entering;
+            fl += OPF_REFRACT;
         }
     }
     acc[3] += wY;   // out of bounces: counted as absorbed
@@ -258,7 +301,7 @@ __device__ void tracePhoton(const OptDev& D, unsigned int s, const PtOpticsScene
 // `part` with plain stores; the numbers kernel adds a scene's rows in a fixed order.
 extern "C" __global__ __launch_bounds__(256) void pt_optics_walk(OptDev D) {
     const unsigned long long total = D.chunkStart[D.n];
-    double acc[4];
+    double acc[6];
     for (;;) {
         const unsigned int mine = atomicAdd(D.next, laneId() == 0u ? 1u : 0u);
         const unsigned int chunk = __builtin_amdgcn_readfirstlane(mine);
@@ -270,12 +313,19 @@ extern "C" __global__ __launch_bounds__(256) void pt_optics_walk(OptDev D) {
         const PtOpticsScene S = D.sc[s];
         const unsigned int first = (unsigned int)(chunk - D.chunkStart[s]) * PT_OPT_CHUNK;
         const unsigned int last = min(first + PT_OPT_CHUNK, S.photons);
-        acc[0] = acc[1] = acc[2] = acc[3] = 0.0;
-        for (unsigned int i = first + laneId(); i < last; i += 64u) tracePhoton(D, s, S, i, acc);
-        const double v0 = waveSumD(acc[0]), v1 = waveSumD(acc[1]), v2 = waveSumD(acc[2]), v3 = waveSumD(acc[3]);
+        acc[0] = acc[1] = acc[2] = acc[3] = acc[4] = acc[5] = 0.0;
+        for (unsigned int i = first + laneId(); i < last; i += 64u) {
+            float ev = 0.0f, fl = 0.0f;
+            tracePhoton(D, s, S, i, acc, ev, fl);
+            acc[4] += ev; acc[5] += fl;
+        }
+        double v[6];
+        for (int k = 0; k < 6; ++k) v[k] = waveSumD(acc[k]);
+        // lanes 0..6 store the chunk's row: X, Y, Z escaped, Y absorbed, photons, surface tests, FP32 operations
         const unsigned int l = laneId();
-        const double mineV = l == 0u ? v0 : l == 1u ? v1 : l == 2u ? v2 : l == 3u ? v3 : (double)(last - first);
-        if (l < 5u) D.part[(size_t)chunk * PT_OPT_TOT + l] = mineV;
+        const double mineV = l == 0u ? v[0] : l == 1u ? v[1] : l == 2u ? v[2] : l == 3u ? v[3]
+                           : l == 4u ? (double)(last - first) : l == 5u ? v[4] : v[5];
+        if (l < 7u) D.part[(size_t)chunk * PT_OPT_TOT + l] = mineV;
     }
 }
 
@@ -364,7 +414,7 @@ extern "C" __global__ __launch_bounds__(256) void pt_optics_numbers(OptDev D) {
     double* lux = D.lux + (size_t)s * PT_OPT_LUX * PT_OPT_LUX;
     __shared__ double rowI[PT_OPT_NG];
     __shared__ double bestL[256];
-    __shared__ double sums[5][256];
+    __shared__ double sums[7][256];
     // the lux map: lm per cell over its area, in m^2 (cells of 2 / LUX metres)
     const double area = (2.0 / PT_OPT_LUX) * (2.0 / PT_OPT_LUX);
     double lm = 0.0;
@@ -375,10 +425,10 @@ extern "C" __global__ __launch_bounds__(256) void pt_optics_numbers(OptDev D) {
     bestL[threadIdx.x] = lm;
     // the scene's chunk sums, each thread a strided share of its rows
     {
-        double t[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        double t[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         for (unsigned long long c = D.chunkStart[s] + threadIdx.x; c < D.chunkStart[s + 1]; c += blockDim.x)
-            for (int k = 0; k < 5; ++k) t[k] += D.part[c * PT_OPT_TOT + k];
-        for (int k = 0; k < 5; ++k) sums[k][threadIdx.x] = t[k];
+            for (int k = 0; k < 7; ++k) t[k] += D.part[c * PT_OPT_TOT + k];
+        for (int k = 0; k < 7; ++k) sums[k][threadIdx.x] = t[k];
     }
     // each gamma row's intensity, its C bins pooled
     for (int j = threadIdx.x; j < PT_OPT_NG; j += blockDim.x) {
@@ -390,7 +440,7 @@ extern "C" __global__ __launch_bounds__(256) void pt_optics_numbers(OptDev D) {
     for (int w = blockDim.x / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) {
             bestL[threadIdx.x] = fmax(bestL[threadIdx.x], bestL[threadIdx.x + w]);
-            for (int k = 0; k < 5; ++k) sums[k][threadIdx.x] += sums[k][threadIdx.x + w];
+            for (int k = 0; k < 7; ++k) sums[k][threadIdx.x] += sums[k][threadIdx.x + w];
         }
         __syncthreads();
     }
@@ -398,7 +448,7 @@ extern "C" __global__ __launch_bounds__(256) void pt_optics_numbers(OptDev D) {
     PtOpticsResult R;
     memset(&R, 0, sizeof R);
     const PtOpticsScene S = D.sc[s];
-    const double tot[5] = {sums[0][0], sums[1][0], sums[2][0], sums[3][0], sums[4][0]};
+    const double tot[7] = {sums[0][0], sums[1][0], sums[2][0], sums[3][0], sums[4][0], sums[5][0], sums[6][0]};
     R.emitted = S.flux;
     R.escaped = tot[1];
     R.absorbed = tot[3];
@@ -419,6 +469,8 @@ extern "C" __global__ __launch_bounds__(256) void pt_optics_numbers(OptDev D) {
     const int h = PT_OPT_LUX / 2;
     R.luxCentre = 0.25 * (lux[(h - 1) * PT_OPT_LUX + h - 1] + lux[(h - 1) * PT_OPT_LUX + h] + lux[h * PT_OPT_LUX + h - 1] + lux[h * PT_OPT_LUX + h]);
     R.photons = tot[4];
+    R.events = tot[5];
+    R.flops = tot[6];
     D.res[s] = R;
 }
 
@@ -456,7 +508,7 @@ int pt_optics_trace(int device, const PtOpticsScene* scenes, uint32_t n, PtOptic
     int rc = PT_ERR_HIP;
     OptDev D = {};
     void *dsc = nullptr, *dstart = nullptr, *dnext = nullptr, *dfar = nullptr, *dlux = nullptr, *dpart = nullptr, *dres = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
     const size_t farW = (size_t)n * PT_OPT_NG * PT_OPT_NC, luxW = (size_t)n * PT_OPT_LUX * PT_OPT_LUX;
     const size_t partW = (size_t)start[n] * PT_OPT_TOT;
     int cus = 256;
@@ -479,14 +531,20 @@ int pt_optics_trace(int device, const PtOpticsScene* scenes, uint32_t n, PtOptic
     D.far = (double*)dfar; D.lux = (double*)dlux; D.part = (double*)dpart; D.res = (PtOpticsResult*)dres;
     OCHK(hipEventCreate(&e0));
     OCHK(hipEventCreate(&e1));
+    OCHK(hipEventCreate(&e2));
     OCHK(hipEventRecord(e0, 0));
     hipLaunchKernelGGL(pt_optics_walk, dim3(cus * 8), dim3(256), 0, 0, D);
     OCHK(hipGetLastError());
+    OCHK(hipEventRecord(e1, 0));
     hipLaunchKernelGGL(pt_optics_numbers, dim3(n), dim3(256), 0, 0, D);
     OCHK(hipGetLastError());
-    OCHK(hipEventRecord(e1, 0));
-    OCHK(hipEventSynchronize(e1));
-    if (ms) { float t = 0.0f; hipEventElapsedTime(&t, e0, e1); *ms = t; }
+    OCHK(hipEventRecord(e2, 0));
+    OCHK(hipEventSynchronize(e2));
+    if (ms) {
+        float t = 0.0f;
+        hipEventElapsedTime(&t, e0, e1); ms[0] = t;
+        hipEventElapsedTime(&t, e1, e2); ms[1] = t;
+    }
     OCHK(hipMemcpy(out, dres, n * sizeof(PtOpticsResult), hipMemcpyDeviceToHost));
     if (far) OCHK(hipMemcpy(far, dfar, farW * 8, hipMemcpyDeviceToHost));
     if (lux) OCHK(hipMemcpy(lux, dlux, luxW * 8, hipMemcpyDeviceToHost));
@@ -494,6 +552,7 @@ int pt_optics_trace(int device, const PtOpticsScene* scenes, uint32_t n, PtOptic
 done:
     if (e0) hipEventDestroy(e0);
     if (e1) hipEventDestroy(e1);
+    if (e2) hipEventDestroy(e2);
     for (void* p : {dsc, dstart, dnext, dfar, dlux, dpart, dres}) if (p) hipFree(p);
     return rc;
 #undef OCHK

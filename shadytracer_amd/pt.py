@@ -76,7 +76,8 @@ class PtOpticsScene(ctypes.Structure):
 class PtOpticsResult(ctypes.Structure):
     _fields_ = [(k, ctypes.c_double) for k in ('emitted', 'escaped', 'absorbed', 'efficiency', 'Imax', 'gammaMax', 'cMax')] + \
                [('beam', ctypes.c_double * 2), ('field', ctypes.c_double * 2)] + \
-               [(k, ctypes.c_double) for k in ('X', 'Y', 'Z', 'x', 'y', 'cct', 'duv', 'luxMax', 'luxCentre', 'photons')]
+               [(k, ctypes.c_double) for k in ('X', 'Y', 'Z', 'x', 'y', 'cct', 'duv', 'luxMax', 'luxCentre', 'photons',
+                                               'events', 'flops')]
 
 
 OPT_NG, OPT_NC, OPT_LUX = 181, 360, 256
@@ -413,17 +414,18 @@ def scene_box(scene):
 
 
 def optics_trace(scenes, far=False, lux=False, device=0):
-    """The optics bench on the GPU: (results as dicts, far [n, 181, 360] lm or None, lux [n, 256, 256] lx or None, ms)."""
+    """The optics bench on the GPU: (results as dicts, far [n, 181, 360] lm or None, lux [n, 256, 256] lx or None,
+    (walk ms, numbers ms))."""
     lib = load_library()
     n = len(scenes)
     arr = (PtOpticsScene * n)(*scenes)
     res = (PtOpticsResult * n)()
     F = np.zeros((n, OPT_NG, OPT_NC)) if far else None
     L = np.zeros((n, OPT_LUX, OPT_LUX)) if lux else None
-    ms = ctypes.c_double(0.0)
+    ms = (ctypes.c_double * 2)()
     dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if a is not None else None
-    rc = lib.pt_optics_trace(device, arr, n, res, dp(F), dp(L), ctypes.byref(ms))
+    rc = lib.pt_optics_trace(device, arr, n, res, dp(F), dp(L), ms)
     if rc:
         raise PtError(rc, lib.pt_optics_error().decode())
-    return [result_dict(r) for r in res], F, L, ms.value
+    return [result_dict(r) for r in res], F, L, (ms[0], ms[1])
 
