@@ -110,7 +110,7 @@ def bench_guide(a):
         n = a.steps
         s1 = np.zeros((a.height, a.width)); s2 = np.zeros_like(s1)
         b1 = np.zeros((a.height // 4, a.width // 4)); b2 = np.zeros_like(b1)
-        prev, t = None, 0.0
+        prev, t, smeans = None, 0.0, []
         for k in range(1, n + 1):
             t0 = time.perf_counter()
             e.trace(1)
@@ -119,6 +119,7 @@ def bench_guide(a):
             Y = pt.luminance(e.read_accum()[0])
             x = Y if prev is None else k * Y - (k - 1) * prev
             prev = Y
+            smeans.append(float(x.mean()))
             s1 += x; s2 += x * x
             xb = x[:a.height // 4 * 4, :a.width // 4 * 4].reshape(a.height // 4, 4, a.width // 4, 4).mean(axis=(1, 3))
             b1 += xb; b2 += xb * xb
@@ -133,10 +134,13 @@ def bench_guide(a):
             'guided' if guided else 'plain', ts * 1e3, mu, var.mean(), bvar.mean()))
         return {'ms_per_sample': ts * 1e3, 'train_s': t_train, 'mean': mu, 'var': float(var.mean()),
                 'var_4x4': float(bvar.mean()), 'seconds_to_1pct': to1(var.mean()), 'seconds_to_1pct_4x4': to1(bvar.mean()),
+                'sample_mean_sd': float(np.std(smeans, ddof=1)),
                 'stats': {k: v for k, v in e.stats().items() if k in ('segments', 'guided', 'guide_records', 'guide_steps')}}, var
     plain, vp = run(False)
     guided, vg = run(True)
-    z = abs(guided['mean'] - plain['mean']) / math.sqrt((plain['var'] + guided['var']) / a.steps / (a.width * a.height))
+    # the means' agreement from the spread of each run's per-sample image means: pixels of one sample are not
+    # independent (they share the sample's seed-level draws), so per-pixel variances understate the mean's error
+    z = abs(guided['mean'] - plain['mean']) / math.sqrt((plain['sample_mean_sd'] ** 2 + guided['sample_mean_sd'] ** 2) / a.steps)
     ratio = vp / np.maximum(vg, 1e-30)
     out = {'metric': 'seconds to 1% rel. RMSE, guided vs plain', 'unit': 's', 'higher_is_better': False,
            'value': guided['seconds_to_1pct_4x4'], 'plain': plain, 'guided': guided,
@@ -233,6 +237,7 @@ def main():
     ap.add_argument('--width', type=int, default=1920)
     ap.add_argument('--height', type=int, default=1080)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--stragglers', type=int, default=None, help='rounds beyond maxbounce + 1 (default: the library\'s)')
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py: this build runs on one GPU; --gpus %d is not supported (samples would shard across GPUs with an '
@@ -256,7 +261,7 @@ def main():
     from shadytracer_amd import pt, scenes
     made = scenes.emit('room-c2', width=a.width, height=a.height)
     log('bench: room-c2 %dx%d, warmup %d, steps %d' % (a.width, a.height, a.warmup, a.steps))
-    e = pt.Engine()
+    e = pt.Engine(straggler_rounds=a.stragglers)
     e.set_scene(made['scene'])
     u = pt.uniforms_for(made)
     e.set_uniforms(u)
@@ -278,7 +283,7 @@ def main():
     ksamples = 4
     ms = e.time_kernels(ksamples)
     kst = e.stats()
-    rounds = u.maxbounce + 1
+    rounds = u.maxbounce + 1 + e.straggler_rounds
     dom = max(('march', 'occlude'), key=lambda k: ms[k])
     # FP32 operations the kernel's sphere tracing performed, counted by the kernel itself from the emitter's per-node
     # counts (nodes evaluated after bounding-sphere culling, the bound tests, each step's own arithmetic)

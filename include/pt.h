@@ -32,8 +32,11 @@ typedef struct pt_opts {
     int device;                 /* HIP device ordinal */
     uint32_t blocks_per_cu;     /* persistent kernels' blocks per compute unit (0: library default) */
     const char* cache_dir;      /* code-object cache (NULL: $PT_KCACHE, else kcache/ beside the library) */
-    uint32_t reserved[4];
+    uint32_t straggler_rounds;  /* rounds a sample runs beyond maxbounce + 1 for paths a hit from behind held up
+                                   (0: PT_STRAGGLER_ROUNDS; PT_STRAGGLER_NONE: none, the flush closes them) */
+    uint32_t reserved[3];
 } pt_opts;
+#define PT_STRAGGLER_NONE 0xFFFFFFFFu
 
 /* A scene as the HIP emitter (shadytracer_amd/js/hipgen.js) produces it: HIP C++ defining sceneDist and
  * sceneFull, the parameter block they read, the material table and the light faces. */

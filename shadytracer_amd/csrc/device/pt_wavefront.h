@@ -41,9 +41,17 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_genrays(const PtDev* _
         o = ld3(cam.pos) + right * (x * cam.fx) + up * (y * cam.fy);
         d = -fwd;
     }
+    // the pixel: the tile's own when the tile grid is the pixel grid (the jittered u * width can round up onto the
+    // next pixel's edge in FP32, which would give that pixel two rays of the sample and this one none); else where
+    // the jittered point falls
     float fw = (float)D->u.width, fh = (float)D->u.height;
-    unsigned int px = (unsigned int)fminf(fmaxf(u * fw, 0.0f), fw - 1.0f);
-    unsigned int py = (unsigned int)fminf(fmaxf(v * fh, 0.0f), fh - 1.0f);
+    unsigned int px, py;
+    if (D->tilesX == D->u.width && D->nrays == D->u.width * D->u.height) {
+        px = tx; py = ty;
+    } else {
+        px = (unsigned int)fminf(fmaxf(u * fw, 0.0f), fw - 1.0f);
+        py = (unsigned int)fminf(fmaxf(v * fh, 0.0f), fh - 1.0f);
+    }
     float collide = 3.0e38f;
     if (hazeOn(D)) collide = hazeDraw(D->u.haze, o, d, rnd.z);
     st4(D->pos, i, o.x, o.y, o.z, 0.0f);
@@ -855,6 +863,30 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_resolve(const PtDev* _
     double V = mo[0], S = mo[1];
     mo[0] = (1.0 - w) * (V + w * dlt * dlt);
     mo[1] = (1.0 - w) * (1.0 - w) * S + w * w;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// FLUSH (core/wavefront.js:948-975, the last pass's flush): the camera paths still going when the sample's rounds ran
+// out -- paths that met more hits from behind than the straggler rounds allow -- deposit the light they carry as they
+// stand, with their count, so that every pixel closes every sample; a recorded training path is closed as at its end.
+// qsel: the march queue the last round's shade filled.
+extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_flush(const PtDev* __restrict__ D, unsigned int qsel) {
+    const unsigned int count = D->counters[qsel ? PT_C_MARCH1 : PT_C_MARCH0];
+    const unsigned int* __restrict__ q = qsel ? D->qMarch1 : D->qMarch0;
+    unsigned long long n = 0ull;
+    for (unsigned int i = blockIdx.x * PT_BLOCK + threadIdx.x; i < count; i += gridDim.x * PT_BLOCK) {
+        const unsigned int slot = q[i];
+        const float4 R = ld4(D->rad, slot);
+        const unsigned int pix = ld4u(D->meta, slot).x;
+        const V3 rad = v3(R.x, R.y, R.z);
+        if (badDeposit(rad)) atomicAdd(&D->stats[PT_S_NONFINITE], 1ull);
+        else depositRGB(D, pix, rad, false);
+        atomicAdd(reinterpret_cast<unsigned int*>(D->frame) + 4ull * pix + 3ull, 1u);
+        unsigned int gr;
+        if (guideRecorded(D, slot, gr)) reinterpret_cast<float4*>(D->gPath)[2 * gr] = make_float4(rad.x, rad.y, rad.z, 1.0f);
+        n += 1ull;
+    }
+    statAdd(D, PT_S_FLUSHED, n);
 }
 
 // ---------------------------------------------------------------------------------------------------

@@ -26,9 +26,10 @@ namespace {
 
 const char* kKernelNames[] = {"pt_genrays", "pt_march", "pt_shade", "pt_occlude", "pt_resolve", "pt_mf_lut", "pt_mf_lut_avg", "pt_pick",
                               "pt_bin_count", "pt_bin_scan", "pt_bin_scatter", "pt_sppm_clear", "pt_photon_gen", "pt_surface",
-                              "pt_guide_infer", "pt_guide_eval", "pt_guide_collect", "pt_guide_grad", "pt_guide_adam", "pt_guide_commit"};
+                              "pt_guide_infer", "pt_guide_eval", "pt_guide_collect", "pt_guide_grad", "pt_guide_adam", "pt_guide_commit",
+                              "pt_flush"};
 enum { K_GENRAYS, K_MARCH, K_SHADE, K_OCCLUDE, K_RESOLVE, K_MFLUT, K_MFAVG, K_PICK, K_BINCOUNT, K_BINSCAN, K_BINSCATTER,
-       K_SPPMCLEAR, K_PHOTONGEN, K_SURFACE, K_GINFER, K_GEVAL, K_GCOLLECT, K_GGRAD, K_GADAM, K_GCOMMIT, K_COUNT };
+       K_SPPMCLEAR, K_PHOTONGEN, K_SURFACE, K_GINFER, K_GEVAL, K_GCOLLECT, K_GGRAD, K_GADAM, K_GCOMMIT, K_FLUSH, K_COUNT };
 
 // the guide network's sizes (device/pt_guide.h)
 const unsigned int kGuideNP = 16169, kGuideWT = 17408, kGuideBias = 256, kGuideParts = 256, kGuideMaxSteps = 64;
@@ -152,6 +153,7 @@ struct pt_ctx {
     std::string err;
     std::string cacheDir;
     unsigned int cus = 256, blocksPerCu = 8;
+    unsigned int stragglers = PT_STRAGGLER_ROUNDS;   // rounds beyond maxbounce + 1 (pt_opts.straggler_rounds)
 
     hipModule_t module = nullptr;
     hipFunction_t fn[K_COUNT] = {};
@@ -293,13 +295,15 @@ int syncDev(pt_ctx* c) {
     return PT_OK;
 }
 
-unsigned int rounds(const pt_ctx* c) { return c->host.u.maxbounce + 1u; }
+// maxbounce + 1 rounds end every path that bounces each round; the straggler rounds end those that met hits from
+// behind (moved on without a bounce), and the flush closes what is left
+unsigned int rounds(const pt_ctx* c) { return c->host.u.maxbounce + 1u + c->stragglers; }
 
 hipError_t launch(pt_ctx* c, int k, unsigned int grid, unsigned int qsel) {
     void* args1[] = {&c->dev};
     void* args2[] = {&c->dev, &qsel};
     bool two = (k == K_MARCH || k == K_SHADE || k == K_BINCOUNT || k == K_BINSCATTER || k == K_OCCLUDE || k == K_SURFACE ||
-                k == K_GINFER || k == K_GGRAD || k == K_GADAM || k == K_GCOMMIT);
+                k == K_GINFER || k == K_GGRAD || k == K_GADAM || k == K_GCOMMIT || k == K_FLUSH);
     unsigned int block = k == K_BINSCAN ? PT_BINS : k == K_GINFER ? 512u : (k == K_GGRAD || k == K_GCOMMIT) ? 64u : 256u;
     return hipModuleLaunchKernel(c->fn[k], grid, 1, 1, block, 1, 1, 0, c->stream, two ? args2 : args1, nullptr);
 }
@@ -352,7 +356,8 @@ hipError_t enqueueSample(pt_ctx* c, std::vector<hipEvent_t>* events, std::vector
         if ((e = one(K_BINCOUNT, 2)) || (e = one(K_BINSCAN, 0)) || (e = one(K_BINSCATTER, 2))) return e;
         if ((e = one(K_OCCLUDE, in))) return e;
     }
-    if ((e = one(K_RESOLVE, 0))) return e;
+    // the last round's shade filled march queue R & 1 with the paths still going
+    if ((e = one(K_FLUSH, R & 1u)) || (e = one(K_RESOLVE, 0))) return e;
     if (guiding) {
         // THE GUIDE'S TRAINING: the sample's recorded vertices into samples, then its Adam steps
         if ((e = one(K_GCOLLECT, 0))) return e;
@@ -580,6 +585,8 @@ int pt_create(pt_ctx** out, const pt_opts* opts) {
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
     c->cus = cus > 0 ? (unsigned int)cus : 256u;
     c->blocksPerCu = (opts && opts->blocks_per_cu) ? opts->blocks_per_cu : 8u;
+    c->stragglers = !(opts && opts->straggler_rounds) ? PT_STRAGGLER_ROUNDS
+                  : opts->straggler_rounds == PT_STRAGGLER_NONE ? 0u : std::min(opts->straggler_rounds, 16u);
     setDefaults(c->host.u);
     setGuideDefaults(c->host.g);
     c->host.gridBlocks = c->cus * c->blocksPerCu;
@@ -778,7 +785,7 @@ int pt_time_kernels(pt_ctx* c, uint32_t samples, double* ms, size_t n) {
             float t = 0.0f;
             hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]);
             int k = kinds[i];
-            int slot = k == K_GENRAYS ? 0 : k == K_MARCH ? 1 : k == K_SHADE ? 2 : k == K_OCCLUDE ? 3 : k == K_RESOLVE ? 4
+            int slot = k == K_GENRAYS ? 0 : k == K_MARCH ? 1 : k == K_SHADE ? 2 : k == K_OCCLUDE ? 3 : (k == K_RESOLVE || k == K_FLUSH) ? 4
                      : (k == K_SPPMCLEAR || k == K_PHOTONGEN) ? 6 : k == K_SURFACE ? 7
                      : (k == K_GINFER || k == K_GCOLLECT || k == K_GGRAD || k == K_GADAM || k == K_GCOMMIT) ? 8 : 5;
             if (slot >= (int)n) continue;

@@ -288,6 +288,46 @@ napi_value ReadStats(napi_env env, napi_callback_info info) {
     return arr;
 }
 
+// opticsTrace(device, scenes, n, results, far | null, lux | null) -> [walk ms, numbers ms]. scenes: n PtOpticsScene
+// packed; results: room for n PtOpticsResult; far / lux: Float64Arrays of n x 181 x 360 / n x 256 x 256. One launch of
+// ~30 ms for the bench's 64 scenes: run synchronously, as the optics page's run() awaits its whole batch.
+napi_value OpticsTrace(napi_env env, napi_callback_info info) {
+    napi_value a[6];
+    if (!args(env, info, 6, a)) return nullptr;
+    int32_t dev = 0; uint32_t n = 0;
+    napi_get_value_int32(env, a[0], &dev);
+    napi_get_value_uint32(env, a[2], &n);
+    void *sp = nullptr, *rp = nullptr, *fp = nullptr, *lp = nullptr;
+    size_t sl = 0, rl = 0, fl = 0, ll = 0;
+    if (!bytesOf(env, a[1], &sp, &sl) || !bytesOf(env, a[3], &rp, &rl)) return nullptr;
+    if (sl < (size_t)n * sizeof(PtOpticsScene) || rl < (size_t)n * sizeof(PtOpticsResult)) {
+        napi_throw_range_error(env, nullptr, "scenes or results smaller than n structures");
+        return nullptr;
+    }
+    napi_valuetype t;
+    napi_typeof(env, a[4], &t);
+    if (t != napi_null && !bytesOf(env, a[4], &fp, &fl)) return nullptr;
+    napi_typeof(env, a[5], &t);
+    if (t != napi_null && !bytesOf(env, a[5], &lp, &ll)) return nullptr;
+    if ((fp && fl < (size_t)n * PT_OPT_NG * PT_OPT_NC * 8) || (lp && ll < (size_t)n * PT_OPT_LUX * PT_OPT_LUX * 8)) {
+        napi_throw_range_error(env, nullptr, "far or lux smaller than n tallies");
+        return nullptr;
+    }
+    double ms[2] = {0.0, 0.0};
+    int rc = pt_optics_trace(dev, static_cast<const PtOpticsScene*>(sp), n, static_cast<PtOpticsResult*>(rp),
+                             static_cast<double*>(fp), static_cast<double*>(lp), ms);
+    if (rc) {
+        std::string text = std::string("libpt: ") + pt_optics_error();
+        napi_throw_error(env, nullptr, text.c_str());
+        return nullptr;
+    }
+    napi_value arr;
+    NAPI_OK(env, napi_create_array_with_length(env, 2, &arr));
+    napi_set_element(env, arr, 0, num(env, ms[0]));
+    napi_set_element(env, arr, 1, num(env, ms[1]));
+    return arr;
+}
+
 napi_value Init(napi_env env, napi_value exports) {
     napi_property_descriptor props[] = {
         {"create", nullptr, Create, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
@@ -302,6 +342,7 @@ napi_value Init(napi_env env, napi_value exports) {
         {"trace", nullptr, Trace, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"readAccum", nullptr, ReadAccum, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"readStats", nullptr, ReadStats, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"opticsTrace", nullptr, OpticsTrace, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
     };
     napi_define_properties(env, exports, sizeof props / sizeof props[0], props);
     return exports;

@@ -91,7 +91,10 @@ def result_dict(r):
 
 class PtOpts(ctypes.Structure):
     _fields_ = [('device', ctypes.c_int), ('blocks_per_cu', ctypes.c_uint32), ('cache_dir', ctypes.c_char_p),
-                ('reserved', ctypes.c_uint32 * 4)]
+                ('straggler_rounds', ctypes.c_uint32), ('reserved', ctypes.c_uint32 * 3)]
+
+
+STRAGGLER_NONE = 0xFFFFFFFF
 
 
 STRUCTS = [PtUniforms, PtMaterial, PtLightFace, PtCamera, PtEnv, PtHaze, PtSceneDesc, PtOpts, PtGuideConfig, PtOpticsScene,
@@ -131,7 +134,8 @@ GUIDE_NP = 16169
 
 STAT_NAMES = ['segments', 'shadow_segments', 'steps', 'shadow_steps', 'nonfinite', 'maxiter', 'paths', 'leaving',
               'gathers', 'photons', 'flops', 'shadow_flops', 'photon_segments', 'gather_cut', 'radius_bits',
-              'overflow', 'guide_steps', 'guided', 'guide_records']
+              'overflow', 'guide_steps', 'guided', 'guide_records', 'flushed']
+STRAGGLER_ROUNDS = 1    # PT_STRAGGLER_ROUNDS (include/pt_types.h): rounds a sample runs beyond maxbounce + 1
 LUMA = (0.2126, 0.7152, 0.0722)
 
 _lib = None
@@ -228,10 +232,14 @@ def uniforms_for(made, **over):
 class Engine:
     """One libpt context on a GPU."""
 
-    def __init__(self, device=0, blocks_per_cu=0, cache_dir=None):
+    def __init__(self, device=0, blocks_per_cu=0, cache_dir=None, straggler_rounds=None):
+        """straggler_rounds: rounds beyond maxbounce + 1 for paths held up by hits from behind (None: the library's
+        default, 0: none -- the flush closes them)."""
         self.lib = load_library()
         o = PtOpts()
         o.device, o.blocks_per_cu = device, blocks_per_cu
+        o.straggler_rounds = 0 if straggler_rounds is None else (STRAGGLER_NONE if straggler_rounds == 0 else straggler_rounds)
+        self.straggler_rounds = STRAGGLER_ROUNDS if straggler_rounds is None else min(straggler_rounds, 16)
         o.cache_dir = cache_dir.encode() if cache_dir else None
         h = ctypes.c_void_p()
         rc = self.lib.pt_create(ctypes.byref(h), ctypes.byref(o))

@@ -10,6 +10,9 @@ const { createPathTracer } = require(path.join(js, 'index'))
 
 const out = {}
 const a = m.loadAddon()
+const optics = require(path.join(js, 'optics'))
+optics.createOptics({ device: 0 })   // checks the optics layouts against the addon's structure sizes; no GPU call
+out.opticsSizes = [m.sizeOf(optics.SCENE), a.structSize(9), optics.RESULT.length * 8, a.structSize(10)]
 out.sizes = [m.sizeOf(m.UNIFORMS), a.structSize(0), m.sizeOf(m.MATERIAL), a.structSize(1), m.sizeOf(m.FACE), a.structSize(2)]
 out.addonKeys = Object.keys(a).sort()
 // packing: a known value lands at its C offset (width 0, height 4, seed 60, cam.pos 64, env.type 128)

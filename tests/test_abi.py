@@ -47,8 +47,10 @@ def test_addon_loads_and_packs_the_same_structures():
                                     stdout=subprocess.PIPE, text=True).stdout)
     s = out['sizes']
     assert s[0] == s[1] == ctypes.sizeof(pt.PtUniforms) and s[2] == s[3] and s[4] == s[5]
-    assert out['addonKeys'] == sorted(['abiVersion', 'clearAccum', 'compileScene', 'create', 'destroy', 'readAccum', 'readStats',
-                                       'setParams', 'setScene', 'setUniforms', 'structSize', 'trace'])
+    assert out['addonKeys'] == sorted(['abiVersion', 'clearAccum', 'compileScene', 'create', 'destroy', 'opticsTrace', 'readAccum',
+                                       'readStats', 'setParams', 'setScene', 'setUniforms', 'structSize', 'trace'])
+    o = out['opticsSizes']
+    assert o[0] == o[1] == ctypes.sizeof(pt.PtOpticsScene) and o[2] == o[3] == ctypes.sizeof(pt.PtOpticsResult)
     # fields land at their C offsets
     u = pt.PtUniforms
     assert [u.width.offset, u.height.offset, u.seed.offset, u.cam.offset, u.env.offset] == [0, 4, 60, 64, 128]
@@ -104,3 +106,13 @@ def test_emitter_output_is_consistent():
     mats = sc['materials']
     assert sum(m['metallic'] == 1 for m in mats[2:34]) in range(4, 20)
     assert all(0.05 <= m['roughness'] <= 0.9 for m in mats[2:34])
+
+
+def test_header_constants_match_the_binding():
+    import re
+    text = open(os.path.join(ROOT, 'include', 'pt_types.h')).read()
+    define = lambda name: int(re.search(r'#define %s (\d+)' % name, text).group(1))
+    assert define('PT_STRAGGLER_ROUNDS') == pt.STRAGGLER_ROUNDS
+    assert pt.STAT_NAMES.index('flushed') == define('PT_S_FLUSHED')
+    assert pt.STAT_NAMES.index('guide_records') == define('PT_S_GRECORDS')
+    assert len(pt.STAT_NAMES) <= define('PT_S_WORDS')

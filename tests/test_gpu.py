@@ -77,6 +77,19 @@ def test_c2_per_pixel_means_agree_with_the_oracle(engine):
 
 # the Lambertian's 1e-3: the march's 400-step cap (the reference's maxiter) drops a path that leaves the plane within
 # about 0.7 degrees of grazing before it reaches the sky sphere, ~0.03 % of the light here
+def test_every_pixel_closes_every_sample(engine):
+    # maxbounce + 1 rounds end the paths that bounce every round; a hit from behind moves a path on without a bounce,
+    # so the straggler rounds end those and the flush deposits the rare path still going: every pixel counts every sample
+    say('case: room-c2 320x180, 8 samples, every pixel counted 8 times')
+    made, u = load(engine, 'room-c2', width=320, height=180)
+    engine.trace(8)
+    acc, _ = engine.read_accum()
+    st = engine.stats()
+    assert (acc[..., 3] == 8).all(), (int((acc[..., 3] != 8).sum()), float(acc[..., 3].min()))
+    # one straggler round leaves ~0.2 % of this room's paths to the flush (those met two hits from behind)
+    assert st['leaving'] > 0 and st['flushed'] <= 5e-3 * 8 * 320 * 180, st
+
+
 @pytest.mark.parametrize('case,opts,tol', [('Lambertian', {'roughness': 1, 'specular': 0}, 1e-3),
                                            ('plastic, rough 0.5', {'roughness': 0.5, 'specular': 0.5}, 3e-3),
                                            ('metal, rough 0.5', {'roughness': 0.5, 'metallic': 1}, 5e-3)])

@@ -1,6 +1,9 @@
 """The optics bench on the GPU through the C ABI: the same photons as the FP64 oracle (PCG4D keyed alike), the numbers
 recomputed from the GPU's own tallies by the oracle, closed forms, and the 64-scene persistent launch."""
+import json
 import math
+import os
+import subprocess
 import sys
 
 import numpy as np
@@ -81,3 +84,26 @@ def test_sixty_four_scenes_in_one_launch():
         assert r['photons'] == s.photons
         assert abs(r['escaped'] + r['absorbed'] - r['emitted']) < 1e-5 * r['emitted']
         assert 0.5 < r['efficiency'] <= 1.0 and r['Imax'] > 0 and 0 < r['beam'][0] < 40 and 2500 < r['cct'] < 7000
+
+
+def test_node_addon_runs_the_optics_bench(tmp_path):
+    # the JavaScript path (shadytracer_amd/js/optics.js -> addon -> pt_optics_trace) gives the ctypes path's numbers
+    say('case: node tests/js/optics_run.js, 4 bench scenes at 2^18 photons, against the ctypes binding')
+    sc = optics.bench_scenes(4, log2_min=18, log2_max=18, seed=7)
+    doc = [{k: (list(getattr(s, k)) if k in ('xyz', 'rho') else getattr(s, k)) for k, _ in pt.PtOpticsScene._fields_} for s in sc]
+    path = tmp_path / 'scenes.json'
+    path.write_text(json.dumps(doc))
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run(['node', os.path.join(root, 'tests', 'js', 'optics_run.js'), str(path)], stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    js = json.loads(r.stdout)
+    py, F, _, _ = pt.optics_trace(sc, far=True)
+    assert js['farSum'] == pytest.approx(F.sum(), rel=1e-12)
+    for a, b in zip(js['results'], py):
+        # per-chunk sums are added in a fixed order: the totals repeat exactly; the far-field numbers to rounding
+        for k in ('photons', 'events', 'X', 'Y', 'Z', 'efficiency', 'cct', 'duv'):
+            assert a[k] == b[k], (k, a[k], b[k])
+        for k in ('Imax', 'luxMax', 'luxCentre'):
+            assert a[k] == pytest.approx(b[k], rel=1e-12), (k, a[k], b[k])
+        assert a['beam'] == pytest.approx(b['beam'], rel=1e-9) and a['field'] == pytest.approx(b['field'], rel=1e-9)
