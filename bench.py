@@ -96,6 +96,8 @@ def bench_guide(a):
 
     def run(guided):
         u = pt.uniforms_for(made, **extra)
+        if a.nee is not None:
+            u.nee = a.nee
         if guided:
             u.guide, u.guideFrac = 1, 0.9
         e.set_uniforms(u)
@@ -148,7 +150,8 @@ def bench_guide(a):
            'speedup_1pct_4x4': plain['seconds_to_1pct_4x4'] / guided['seconds_to_1pct_4x4'],
            'variance_reduction': plain['var'] / guided['var'], 'median_pixel_variance_ratio': float(np.median(ratio)),
            'mean_agreement_sigma': z, 'n_gpus': 1, 'dtype': 'f32 (network bf16, accumulation f64)',
-           'config': {'workload': '%s %dx%d' % (a.scene, a.width, a.height), 'train_samples': a.train, 'samples': a.steps,
+           'config': {'workload': '%s %dx%d%s' % (a.scene, a.width, a.height, '' if a.nee is None else ', nee %d' % a.nee),
+                      'train_samples': a.train, 'samples': a.steps,
                       'photons': extra.get('sppmPhotons', 0)}}
     print(json.dumps(out), flush=True)
 
@@ -230,6 +233,7 @@ def main():
     ap.add_argument('--optics', action='store_true', help='the optics bench: 64 luminaires in one persistent launch')
     ap.add_argument('--scene', default='caustic-c3')
     ap.add_argument('--train', type=int, default=300)
+    ap.add_argument('--nee', type=int, default=None, help='with --guide: light sampling on (1) or off (0)')
     ap.add_argument('--calibrate-mib', type=int, default=0, help='a device copy of this many MiB first (counter scale)')
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=32)

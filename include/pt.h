@@ -55,7 +55,7 @@ int pt_abi_version(void);
 
 /* sizeof of the ABI's structures, for bindings to check their mirrors: 0 PtUniforms, 1 PtMaterial,
  * 2 PtLightFace, 3 PtCamera, 4 PtEnv, 5 PtHaze, 6 pt_scene_desc, 7 pt_opts, 8 PtGuideConfig, 9 PtOpticsScene,
- * 10 PtOpticsResult; 0 for anything else.
+ * 10 PtOpticsResult, 11 PtTuning; 0 for anything else.
  * No counterpart in the reference (its uniform packing lives in the absent tinywebgpu submodule). */
 size_t pt_struct_size(int which);
 
@@ -119,6 +119,23 @@ int pt_time_kernels(pt_ctx* ctx, uint32_t samples, double* ms, size_t n);
  * the camera rays of a sample (width*height). The sample counter is not advanced. Replaces R.pick
  * (lib/pathtracer/core/pick.js) for every pixel at once. */
 int pt_pick_rays(pt_ctx* ctx, float* pos4, uint32_t* info, size_t n);
+
+/* ---- launch choices ----
+ * No reference counterpart: the reference's dispatch sizes are fixed in its pipelines (wavefront.js workgroup sizes,
+ * raytrace_main.js's 32-step passes). Here the persistent kernels' grid and their refill thresholds are chosen per
+ * scene and image by measurement. */
+
+/* The choices in force (PtTuning; size = sizeof(PtTuning)). */
+int pt_get_tuning(pt_ctx* ctx, PtTuning* out, size_t size);
+/* Sets them (blocksPerCu 1..32, refills 1..64); the next trace captures its graph anew. A uniforms' marchRefill of
+ * 0 takes the refills from here. */
+int pt_set_tuning(pt_ctx* ctx, const PtTuning* tuning, size_t size);
+/* The autotuner: for the scene and uniforms set, times `samples` (1..64) samples of each candidate setting -- blocks
+ * per CU in {2, 4, 6, 8, 12, 16}, then the march refill, then the occlusion refill, each in {1, 4, 8, 16, 24, 32, 48},
+ * coordinate by coordinate from the setting in force -- keeps the fastest, and clears the accumulators (the timed
+ * samples went into them; with guiding on, its training goes on through them). out (NULL or a PtTuning) receives the
+ * choice, the candidates timed and the times before and after. */
+int pt_autotune(pt_ctx* ctx, uint32_t samples, PtTuning* out, size_t size);
 
 /* ---- path guiding (no counterpart in the reference: its guiding was removed, lib/pathtracer/README.md:853-861;
  * the engine's uniforms switch it on: `guide`, and `guideFrac` the share of guided directions) ---- */

@@ -132,6 +132,17 @@ typedef struct PtHaze {
     float pad[3];
 } PtHaze;
 
+// The persistent kernels' launch choices: what pt_autotune picks for the scene and image in hand, or a host sets with
+// pt_set_tuning. A uniforms' marchRefill of 0 takes the refills from here; a nonzero one overrides both.
+typedef struct PtTuning {
+    unsigned int blocksPerCu;     // blocks of 256 threads a compute unit for march, surface, shade, occlude (1..32)
+    unsigned int refillMarch;     // a march wave takes new rays once this many of its 64 lanes are idle (1..64)
+    unsigned int refillOcclude;   // the same for the occlusion kernel
+    unsigned int candidates;      // pt_autotune: the settings it timed
+    double msPerSample;           // pt_autotune: the chosen setting's time a sample (ms)
+    double msBefore;              // pt_autotune: the setting in force before it, timed alike
+} PtTuning;
+
 // The engine's uniforms: the subset of the reference's schema (core/uniforms.schema.js:37-418) this path reads.
 // The path-guiding network's training (set with pt_guide_config): the box vertex positions are encoded in, Adam's
 // learning rate, samples a step and steps a sample, which paths record their vertices, the warm-up before guided

@@ -11,6 +11,8 @@ struct PtDev {
     unsigned int nmats, nfaces;
     float faceTotal;
     unsigned int gridBlocks;    // persistent kernels' grid
+    unsigned int refillMarch;   // march: refill a wave once this many lanes are idle
+    unsigned int refillOcclude; // occlude: the same
     unsigned int mfReady;
     unsigned int pad1;
     // path state, structure of arrays indexed by slot

@@ -163,7 +163,7 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_march(const PtDev* __r
     const float R2 = D->u.envRadius * D->u.envRadius;
     const float stepscale = D->u.stepscale;
     const unsigned int maxiter = D->u.maxiter;
-    const unsigned int refill = D->u.marchRefill;
+    const unsigned int refill = D->refillMarch;
     ParamPtr U = (const PT_CONST float*)D->params;
 
     int slot = -1;
@@ -767,7 +767,7 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_occlude(const PtDev* _
     const float R2 = D->u.envRadius * D->u.envRadius;
     const float stepscale = D->u.stepscale;
     const unsigned int maxiter = D->u.maxiter;
-    const unsigned int refill = D->u.marchRefill;
+    const unsigned int refill = D->refillOcclude;
     ParamPtr U = (const PT_CONST float*)D->params;
     int slot = -1;
     V3 p = v3s(0.0f), dir = v3s(0.0f), o = v3s(0.0f);

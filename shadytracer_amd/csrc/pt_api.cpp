@@ -154,6 +154,7 @@ struct pt_ctx {
     std::string cacheDir;
     unsigned int cus = 256, blocksPerCu = 8;
     unsigned int stragglers = PT_STRAGGLER_ROUNDS;   // rounds beyond maxbounce + 1 (pt_opts.straggler_rounds)
+    PtTuning tune{};          // the persistent kernels' launch choices (pt_set_tuning, pt_autotune)
 
     hipModule_t module = nullptr;
     hipFunction_t fn[K_COUNT] = {};
@@ -298,6 +299,14 @@ int syncDev(pt_ctx* c) {
 // maxbounce + 1 rounds end every path that bounces each round; the straggler rounds end those that met hits from
 // behind (moved on without a bounce), and the flush closes what is left
 unsigned int rounds(const pt_ctx* c) { return c->host.u.maxbounce + 1u + c->stragglers; }
+
+// the refill thresholds the kernels read: the uniforms' marchRefill where set, else the tuning's
+void applyRefills(pt_ctx* c) {
+    const unsigned int r = c->host.u.marchRefill;
+    c->host.refillMarch = r ? r : c->tune.refillMarch;
+    c->host.refillOcclude = r ? r : c->tune.refillOcclude;
+    c->devDirty = true;
+}
 
 hipError_t launch(pt_ctx* c, int k, unsigned int grid, unsigned int qsel) {
     void* args1[] = {&c->dev};
@@ -523,7 +532,7 @@ void setDefaults(PtUniforms& u) {
     u.cam.right[0] = 1.0f; u.cam.up[2] = 1.0f; u.cam.fwd[1] = -1.0f;
     u.cam.fx = 0.57735f; u.cam.fy = 0.57735f;
     u.faceShare = 0.5f;
-    u.marchRefill = 16;
+    u.marchRefill = 0;
     u.guideFrac = 0.9f;
     u.sppmRadius = 0.02f;
     u.sppmAlpha = 2.0f / 3.0f;
@@ -548,6 +557,7 @@ size_t pt_struct_size(int which) {
         case 8: return sizeof(PtGuideConfig);
         case 9: return sizeof(PtOpticsScene);
         case 10: return sizeof(PtOpticsResult);
+        case 11: return sizeof(PtTuning);
         default: return 0;
     }
 }
@@ -590,6 +600,9 @@ int pt_create(pt_ctx** out, const pt_opts* opts) {
     setDefaults(c->host.u);
     setGuideDefaults(c->host.g);
     c->host.gridBlocks = c->cus * c->blocksPerCu;
+    c->tune.blocksPerCu = c->blocksPerCu;
+    c->tune.refillMarch = c->tune.refillOcclude = 16u;
+    applyRefills(c);
     int rc;
     if ((rc = allocBuf(c, c->dev, sizeof(PtDev))) || (rc = allocBuf(c, c->host.counters, PT_C_WORDS * 4)) ||
         (rc = allocBuf(c, c->host.stats, PT_S_WORDS * 8)) || (rc = allocBuf(c, c->host.mf, PT_MF_WORDS * 4)) ||
@@ -694,7 +707,7 @@ int pt_set_uniforms(pt_ctx* c, const PtUniforms* u, size_t size) {
     if (size != sizeof(PtUniforms)) return fail(c, PT_ERR_ARG, "uniforms: %zu bytes, expected %zu", size, sizeof(PtUniforms));
     if (u->width == 0 || u->height == 0 || u->width > 16384 || u->height > 16384) return fail(c, PT_ERR_ARG, "size %u x %u out of range", u->width, u->height);
     if (u->maxbounce > 250) return fail(c, PT_ERR_ARG, "maxbounce %u above 250 (a path counts its bounces in 8 bits)", u->maxbounce);
-    if (u->marchRefill == 0 || u->marchRefill > 64) return fail(c, PT_ERR_ARG, "marchRefill %u not in 1..64", u->marchRefill);
+    if (u->marchRefill > 64) return fail(c, PT_ERR_ARG, "marchRefill %u not in 0..64 (0: the tuning's)", u->marchRefill);
     if (!(u->stepscale > 0.0f && u->stepscale <= 1.0f)) return fail(c, PT_ERR_ARG, "stepscale %g not in (0, 1]", u->stepscale);
     if (u->sppm && (u->sppmPhotons == 0 || u->sppmPhotons > (1u << 26)))
         return fail(c, PT_ERR_ARG, "sppmPhotons %u not in 1..2^26", u->sppmPhotons);
@@ -711,7 +724,7 @@ int pt_set_uniforms(pt_ctx* c, const PtUniforms* u, size_t size) {
     c->host.invTY = 1.0f / float(c->host.tilesY);
     c->host.nrays = N;
     c->host.nphotons = photons;
-    c->devDirty = true;
+    applyRefills(c);
     if (u->guide && (rc = ensureGuide(c))) return rc;
     if (resized) return pt_clear_accum(c);
     return PT_OK;
@@ -736,6 +749,90 @@ int pt_trace(pt_ctx* c, uint32_t samples) {
     if ((rc = syncDev(c)) || (rc = buildGraph(c))) return rc;
     for (uint32_t s = 0; s < samples; ++s) HIPCHK(c, hipGraphLaunch(c->graphExec, c->stream));
     return PT_OK;
+}
+
+int pt_get_tuning(pt_ctx* c, PtTuning* out, size_t size) {
+    if (!c || !out) return fail(c, PT_ERR_ARG, "tuning is NULL");
+    if (size != sizeof(PtTuning)) return fail(c, PT_ERR_ARG, "tuning: %zu bytes, expected %zu", size, sizeof(PtTuning));
+    *out = c->tune;
+    return PT_OK;
+}
+
+namespace {
+int setTuning(pt_ctx* c, unsigned int bpc, unsigned int rm, unsigned int ro) {
+    c->tune.blocksPerCu = bpc; c->tune.refillMarch = rm; c->tune.refillOcclude = ro;
+    c->blocksPerCu = bpc;
+    c->host.gridBlocks = c->cus * bpc;     // part of the graph's key: the next trace captures it anew
+    applyRefills(c);
+    return PT_OK;
+}
+// the time a sample of the current setting takes: one sample to build and warm, then `samples` timed on the stream
+int timeSetting(pt_ctx* c, uint32_t samples, double& ms) {
+    int rc;
+    if ((rc = syncDev(c)) || (rc = buildGraph(c))) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(c, hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) { hipEventDestroy(e0); return fail(c, PT_ERR_HIP, "event create"); }
+    hipError_t e = hipGraphLaunch(c->graphExec, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+    for (uint32_t k = 0; e == hipSuccess && k < samples; ++k) e = hipGraphLaunch(c->graphExec, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    hipEventDestroy(e0); hipEventDestroy(e1);
+    if (e != hipSuccess) return fail(c, PT_ERR_HIP, "autotune: %s", hipGetErrorString(e));
+    ms = double(t) / samples;
+    return PT_OK;
+}
+}  // namespace
+
+int pt_set_tuning(pt_ctx* c, const PtTuning* t, size_t size) {
+    if (!c || !t) return fail(c, PT_ERR_ARG, "tuning is NULL");
+    if (size != sizeof(PtTuning)) return fail(c, PT_ERR_ARG, "tuning: %zu bytes, expected %zu", size, sizeof(PtTuning));
+    if (t->blocksPerCu < 1 || t->blocksPerCu > 32 || t->refillMarch < 1 || t->refillMarch > 64 || t->refillOcclude < 1 || t->refillOcclude > 64)
+        return fail(c, PT_ERR_ARG, "tuning: blocksPerCu %u not in 1..32 or a refill (%u, %u) not in 1..64", t->blocksPerCu,
+                    t->refillMarch, t->refillOcclude);
+    return setTuning(c, t->blocksPerCu, t->refillMarch, t->refillOcclude);
+}
+
+int pt_autotune(pt_ctx* c, uint32_t samples, PtTuning* out, size_t size) {
+    if (!c) return fail(c, PT_ERR_ARG, "ctx is NULL");
+    if (out && size != sizeof(PtTuning)) return fail(c, PT_ERR_ARG, "tuning: %zu bytes, expected %zu", size, sizeof(PtTuning));
+    if (c->building) return fail(c, PT_ERR_STATE, "autotune during a scene build");
+    if (!c->module) return fail(c, PT_ERR_STATE, "no scene loaded");
+    if (samples < 1 || samples > 64) return fail(c, PT_ERR_ARG, "samples %u not in 1..64", samples);
+    // coordinate descent: the grid of the persistent kernels first (occupancy against the tail of each round), then
+    // each kernel's refill threshold (how many idle lanes a wave carries before it takes new rays)
+    static const unsigned int kBpc[] = {2, 4, 6, 8, 12, 16};
+    static const unsigned int kRefill[] = {1, 4, 8, 16, 24, 32, 48};
+    PtTuning best = c->tune;
+    unsigned int n = 0;
+    int rc;
+    double ms = 0.0;
+    if ((rc = timeSetting(c, samples, ms))) return rc;
+    const double before = ms;
+    double bestMs = ms;
+    ++n;
+    auto tryOne = [&](unsigned int bpc, unsigned int rm, unsigned int ro) -> int {
+        if (bpc == best.blocksPerCu && rm == best.refillMarch && ro == best.refillOcclude) return PT_OK;
+        int r;
+        double t = 0.0;
+        if ((r = setTuning(c, bpc, rm, ro)) || (r = timeSetting(c, samples, t))) return r;
+        ++n;
+        if (t < bestMs) { bestMs = t; best.blocksPerCu = bpc; best.refillMarch = rm; best.refillOcclude = ro; }
+        return PT_OK;
+    };
+    for (unsigned int b : kBpc) if ((rc = tryOne(b, best.refillMarch, best.refillOcclude))) return rc;
+    for (unsigned int r : kRefill) if ((rc = tryOne(best.blocksPerCu, r, best.refillOcclude))) return rc;
+    for (unsigned int r : kRefill) if ((rc = tryOne(best.blocksPerCu, best.refillMarch, r))) return rc;
+    if ((rc = setTuning(c, best.blocksPerCu, best.refillMarch, best.refillOcclude))) return rc;
+    c->tune.candidates = n;
+    c->tune.msPerSample = bestMs;
+    c->tune.msBefore = before;
+    if (out) *out = c->tune;
+    // the tuning's samples went into the accumulators: the picture starts again
+    return pt_clear_accum(c);
 }
 
 int pt_sync(pt_ctx* c) {

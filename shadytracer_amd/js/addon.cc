@@ -288,6 +288,44 @@ napi_value ReadStats(napi_env env, napi_callback_info info) {
     return arr;
 }
 
+// autotune(h, samples) -> { blocksPerCu, refillMarch, refillOcclude, candidates, msPerSample, msBefore }
+napi_value Autotune(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    if (!args(env, info, 2, a)) return nullptr;
+    pt_ctx* ctx = ctxOf(env, a[0]);
+    if (!ctx) return nullptr;
+    uint32_t samples = 4;
+    napi_get_value_uint32(env, a[1], &samples);
+    PtTuning t;
+    int rc = pt_autotune(ctx, samples, &t, sizeof t);
+    if (rc) return throwPt(env, ctx, rc);
+    napi_value o;
+    NAPI_OK(env, napi_create_object(env, &o));
+    napi_set_named_property(env, o, "blocksPerCu", num(env, t.blocksPerCu));
+    napi_set_named_property(env, o, "refillMarch", num(env, t.refillMarch));
+    napi_set_named_property(env, o, "refillOcclude", num(env, t.refillOcclude));
+    napi_set_named_property(env, o, "candidates", num(env, t.candidates));
+    napi_set_named_property(env, o, "msPerSample", num(env, t.msPerSample));
+    napi_set_named_property(env, o, "msBefore", num(env, t.msBefore));
+    return o;
+}
+
+// setTuning(h, blocksPerCu, refillMarch, refillOcclude)
+napi_value SetTuning(napi_env env, napi_callback_info info) {
+    napi_value a[4];
+    if (!args(env, info, 4, a)) return nullptr;
+    pt_ctx* ctx = ctxOf(env, a[0]);
+    if (!ctx) return nullptr;
+    PtTuning t;
+    memset(&t, 0, sizeof t);
+    napi_get_value_uint32(env, a[1], &t.blocksPerCu);
+    napi_get_value_uint32(env, a[2], &t.refillMarch);
+    napi_get_value_uint32(env, a[3], &t.refillOcclude);
+    int rc = pt_set_tuning(ctx, &t, sizeof t);
+    if (rc) return throwPt(env, ctx, rc);
+    return undef(env);
+}
+
 // opticsTrace(device, scenes, n, results, far | null, lux | null) -> [walk ms, numbers ms]. scenes: n PtOpticsScene
 // packed; results: room for n PtOpticsResult; far / lux: Float64Arrays of n x 181 x 360 / n x 256 x 256. One launch of
 // ~30 ms for the bench's 64 scenes: run synchronously, as the optics page's run() awaits its whole batch.
@@ -343,6 +381,8 @@ napi_value Init(napi_env env, napi_value exports) {
         {"readAccum", nullptr, ReadAccum, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"readStats", nullptr, ReadStats, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"opticsTrace", nullptr, OpticsTrace, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"autotune", nullptr, Autotune, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"setTuning", nullptr, SetTuning, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
     };
     napi_define_properties(env, exports, sizeof props / sizeof props[0], props);
     return exports;

@@ -69,7 +69,7 @@ function packArray(layout, items) {
 const DEFAULTS = {
     width: 256, height: 256, maxbounce: 10, maxiter: 400, stepscale: 0.9, eps: 0, epsAuto: 1e-4, envRadius: 400, nee: 1, rr: 0,
     rrStart: 3, diffuseCap: 0, fireflyClamp: 0, fireflyWhite: 1, maxaverages: 0, seed: 20261203, faceShare: 0.5, sppm: 0,
-    sppmRadius: 0.02, sppmPhotons: 0, marchRefill: 16, guide: 0, guideFrac: 0.9, sppmAlpha: 2 / 3,
+    sppmRadius: 0.02, sppmPhotons: 0, marchRefill: 0, guide: 0, guideFrac: 0.9, sppmAlpha: 2 / 3,
 }
 const KNOWN = new Set(UNIFORMS.map(f => f[0]))
 
@@ -114,13 +114,23 @@ function RAYTRACER(WG, _U, Q, scene) {
         if (building) throw new Error('RAYTRACER: trace during a scene build')
         if (!current) throw new Error('RAYTRACER: no scene')
         if (uniforms) Object.assign(U, uniforms)
+        A.setUniforms(h, packUniforms())
+        await A.trace(h, count)
+    }
+    function packUniforms() {
         const buf = new ArrayBuffer(sizeOf(UNIFORMS))
         pack(UNIFORMS, U, new DataView(buf), 0)
-        A.setUniforms(h, buf)
-        await A.trace(h, count)
+        return buf
     }
 
     R.clearAccums = () => A.clearAccum(h)
+    // the persistent kernels' grid and refill thresholds picked by timing candidates (pt_autotune); clears the picture
+    R.autotune = (samples = 4) => {
+        if (!current) throw new Error('RAYTRACER: no scene')
+        A.setUniforms(h, packUniforms())
+        return A.autotune(h, samples)
+    }
+    R.setTuning = (blocksPerCu, refillMarch, refillOcclude) => A.setTuning(h, blocksPerCu, refillMarch, refillOcclude)
     R.ensureBuffers = (w, hh) => { U.width = w; U.height = hh }
     R.releaseBuffers = () => { if (h) { A.destroy(h); h = null } }
 

@@ -5,6 +5,8 @@ set_uniforms, trace, image_stats, read_region, clear_accums. There is no CPU fal
 library or a GPU the calls raise.
 """
 import ctypes
+import hashlib
+import json
 import math
 import os
 
@@ -97,8 +99,13 @@ class PtOpts(ctypes.Structure):
 STRAGGLER_NONE = 0xFFFFFFFF
 
 
+class PtTuning(ctypes.Structure):
+    _fields_ = [('blocksPerCu', ctypes.c_uint), ('refillMarch', ctypes.c_uint), ('refillOcclude', ctypes.c_uint),
+                ('candidates', ctypes.c_uint), ('msPerSample', ctypes.c_double), ('msBefore', ctypes.c_double)]
+
+
 STRUCTS = [PtUniforms, PtMaterial, PtLightFace, PtCamera, PtEnv, PtHaze, PtSceneDesc, PtOpts, PtGuideConfig, PtOpticsScene,
-           PtOpticsResult]
+           PtOpticsResult, PtTuning]
 
 # the C ABI, name -> (restype, argtypes)
 SIGNATURES = {
@@ -129,6 +136,9 @@ SIGNATURES = {
     'pt_optics_trace': (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(PtOpticsScene), ctypes.c_uint32, ctypes.POINTER(PtOpticsResult),
                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     'pt_optics_error': (ctypes.c_char_p, []),
+    'pt_get_tuning': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PtTuning), ctypes.c_size_t]),
+    'pt_set_tuning': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PtTuning), ctypes.c_size_t]),
+    'pt_autotune': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(PtTuning), ctypes.c_size_t]),
 }
 GUIDE_NP = 16169
 
@@ -201,7 +211,7 @@ def default_uniforms():
     u.stepscale, u.eps, u.epsAuto, u.envRadius = 0.9, 0.0, 1e-4, 400.0
     u.nee, u.rr, u.rrStart, u.diffuseCap = 1, 0, 3, 0
     u.fireflyClamp, u.fireflyWhite, u.maxaverages, u.seed = 0.0, 1.0, 0, 20261203
-    u.faceShare, u.marchRefill, u.guideFrac = 0.5, 16, 0.9
+    u.faceShare, u.marchRefill, u.guideFrac = 0.5, 0, 0.9
     u.sppmRadius, u.sppmAlpha = 0.02, 2.0 / 3.0
     return u
 
@@ -287,6 +297,7 @@ class Engine:
         d.faces, d.nfaces = faces, nf
         self._keep = [params, mats, faces, d]
         self._chk(self.lib.pt_set_scene(self.h, ctypes.byref(d)))
+        self._scene_hash = hashlib.sha1(scene['hip'].encode()).hexdigest()[:16]
 
     def set_params(self, params):
         a = np.ascontiguousarray(np.asarray(params, dtype=np.float32))
@@ -295,6 +306,7 @@ class Engine:
     def set_uniforms(self, u):
         self._chk(self.lib.pt_set_uniforms(self.h, ctypes.byref(u), ctypes.sizeof(u)))
         self.width, self.height = u.width, u.height
+        self._u = u
 
     def clear_accums(self):
         self._chk(self.lib.pt_clear_accum(self.h))
@@ -328,6 +340,50 @@ class Engine:
         ms = (ctypes.c_double * 9)()
         self._chk(self.lib.pt_time_kernels(self.h, samples, ms, 9))
         return dict(zip(['genrays', 'march', 'shade', 'occlude', 'resolve', 'bin', 'photon', 'surface', 'guide'], list(ms)))
+
+    # ---- launch choices
+    def tuning(self):
+        t = PtTuning()
+        self._chk(self.lib.pt_get_tuning(self.h, ctypes.byref(t), ctypes.sizeof(t)))
+        return {k: getattr(t, k) for k, _ in PtTuning._fields_}
+
+    def set_tuning(self, blocksPerCu, refillMarch, refillOcclude):
+        t = PtTuning()
+        t.blocksPerCu, t.refillMarch, t.refillOcclude = blocksPerCu, refillMarch, refillOcclude
+        self._chk(self.lib.pt_set_tuning(self.h, ctypes.byref(t), ctypes.sizeof(t)))
+
+    def tuning_key(self):
+        """What a tuning is good for: the scene's source, the image, and the settings that shape a sample's work."""
+        u = self._u
+        return '%s/%dx%d/b%d/n%d/s%d:%d/g%d/x%d' % (self._scene_hash, u.width, u.height, u.maxbounce, u.nee, u.sppm,
+                                                   u.sppmPhotons if u.sppm else 0, u.guide, self.straggler_rounds)
+
+    def autotune(self, samples=4, cache=None):
+        """Picks the persistent kernels' grid and refill thresholds for the scene and uniforms set (pt_autotune) and
+        returns the choice. With `cache` (a JSON file), a choice stored for the same tuning_key() is applied instead of
+        timing again, and a new one is stored. Clears the accumulators either way."""
+        key = self.tuning_key()
+        table = {}
+        if cache and os.path.exists(cache):
+            try:
+                table = json.load(open(cache))
+            except (OSError, ValueError):
+                table = {}
+        if key in table:
+            t = table[key]
+            self.set_tuning(t['blocksPerCu'], t['refillMarch'], t['refillOcclude'])
+            self.clear_accums()
+            return dict(t, cached=True)
+        t = PtTuning()
+        self._chk(self.lib.pt_autotune(self.h, samples, ctypes.byref(t), ctypes.sizeof(t)))
+        out = {k: getattr(t, k) for k, _ in PtTuning._fields_}
+        if cache:
+            table[key] = out
+            tmp = cache + '.tmp%d' % os.getpid()
+            with open(tmp, 'w') as f:
+                json.dump(table, f, indent=1, sort_keys=True)
+            os.replace(tmp, cache)
+        return dict(out, cached=False)
 
     # ---- path guiding
     def guide_config(self, box, lr=1e-2, batch=1 << 16, steps=4, stride=0, warmup=16, seed=1):

@@ -47,8 +47,9 @@ def test_addon_loads_and_packs_the_same_structures():
                                     stdout=subprocess.PIPE, text=True).stdout)
     s = out['sizes']
     assert s[0] == s[1] == ctypes.sizeof(pt.PtUniforms) and s[2] == s[3] and s[4] == s[5]
-    assert out['addonKeys'] == sorted(['abiVersion', 'clearAccum', 'compileScene', 'create', 'destroy', 'opticsTrace', 'readAccum',
-                                       'readStats', 'setParams', 'setScene', 'setUniforms', 'structSize', 'trace'])
+    assert out['addonKeys'] == sorted(['abiVersion', 'autotune', 'clearAccum', 'compileScene', 'create', 'destroy', 'opticsTrace',
+                                       'readAccum', 'readStats', 'setParams', 'setScene', 'setTuning', 'setUniforms', 'structSize',
+                                       'trace'])
     o = out['opticsSizes']
     assert o[0] == o[1] == ctypes.sizeof(pt.PtOpticsScene) and o[2] == o[3] == ctypes.sizeof(pt.PtOpticsResult)
     # fields land at their C offsets

@@ -269,3 +269,19 @@ def test_c3_caustic_room_matches_the_oracle(engine):
     say('worst region at %.2f of its tolerance' % max(r[0] for r in results))
     for r in results:
         assert r[0] < 1, r
+
+
+def test_autotune_picks_a_setting_and_caches_it(engine, tmp_path):
+    say('case: room-c2 160x90, pt_autotune with 2 samples a candidate, then the cached choice')
+    made, u = load(engine, 'room-c2', width=160, height=90)
+    t = engine.autotune(samples=2, cache=str(tmp_path / 'tuning.json'))
+    say('tuning', t)
+    assert not t['cached'] and t['candidates'] >= 10
+    assert t['blocksPerCu'] in (2, 4, 6, 8, 12, 16) and t['refillMarch'] in (1, 4, 8, 16, 24, 32, 48)
+    assert 0 < t['msPerSample'] <= t['msBefore']
+    engine.trace(4)
+    acc, _ = engine.read_accum()
+    assert (acc[..., 3] == 4).all()               # the tuner's samples were cleared; the picture is the 4 traced
+    again = engine.autotune(samples=2, cache=str(tmp_path / 'tuning.json'))
+    assert again['cached'] and again['blocksPerCu'] == t['blocksPerCu'] and engine.tuning()['refillOcclude'] == t['refillOcclude']
+    engine.set_tuning(8, 16, 16)                  # back to the library's defaults for the tests after this one

@@ -167,6 +167,39 @@ def test_guided_room_agrees_with_the_plain_tracer(engine):
         % (worst, g.mean(axis=(1, 2)).std(), p.mean(axis=(1, 2)).std()))
 
 
+def test_trained_guiding_gains_over_bsdf_sampling(engine):
+    # What the trained network gains over the method without it: the room lit by its 0.5 m ceiling panel with light
+    # sampling off, so that the plain tracer finds the panel only by BSDF sampling. The mixture learns where the panel
+    # is; the guided image must have a fraction of the plain one's per-pixel variance, at the same mean.
+    say('case: room-c2 96x54, light sampling off: guided (after 48 training samples) vs plain, 48 samples each')
+    made = scenes.emit('room-c2', width=96, height=54)
+    engine.set_scene(made['scene'])
+    engine.guide_config(pt.scene_box(made['scene']), batch=4096, steps=4, warmup=16)
+    ug = pt.uniforms_for(made)
+    ug.nee, ug.guide, ug.guideFrac = 0, 1, 0.9
+    engine.set_uniforms(ug)
+    engine.clear_accums()
+    engine.trace(48)
+    engine.clear_accums()
+    engine.reset_stats()
+    g = per_sample(engine, 48)
+    assert engine.stats()['guided'] > 0
+    up = pt.uniforms_for(made)
+    up.nee = 0
+    engine.set_uniforms(up)
+    engine.clear_accums()
+    p = per_sample(engine, 48)
+    vg, vp = g.var(axis=0, ddof=1), p.var(axis=0, ddof=1)
+    gm, pm = g.mean(axis=(1, 2)), p.mean(axis=(1, 2))
+    z = abs(gm.mean() - pm.mean()) / math.sqrt(gm.var(ddof=1) / len(gm) + pm.var(ddof=1) / len(pm))
+    lit = vp > 0
+    ratio = vp.mean() / vg.mean()
+    median = float(np.median(vp[lit] / np.maximum(vg[lit], 1e-30)))
+    say('plain/guided variance: mean %.2f, median pixel %.2f; means %.4g vs %.4g (%.2f sigma)' % (ratio, median, gm.mean(), pm.mean(), z))
+    assert z < 4.5, (gm.mean(), pm.mean())
+    assert ratio > 2.0 and median > 1.5, (ratio, median)
+
+
 def furnace_samples(engine, u, n):
     """Each sample's mean over the floor rows (the running FP64 mean read after every sample)."""
     engine.set_uniforms(u)
