@@ -98,6 +98,13 @@ def bench_guide(a):
         u = pt.uniforms_for(made, **extra)
         if a.nee is not None:
             u.nee = a.nee
+        for item in a.set:
+            # name=value, a dotted name reaching into a structure (haze.sigmaS=0)
+            name, value = item.split('=', 1)
+            obj, parts = u, name.split('.')
+            for part in parts[:-1]:
+                obj = getattr(obj, part)
+            setattr(obj, parts[-1], type(getattr(obj, parts[-1]))(float(value)))
         if guided:
             u.guide, u.guideFrac = 1, 0.9
         e.set_uniforms(u)
@@ -113,6 +120,7 @@ def bench_guide(a):
         s1 = np.zeros((a.height, a.width)); s2 = np.zeros_like(s1)
         b1 = np.zeros((a.height // 4, a.width // 4)); b2 = np.zeros_like(b1)
         prev, t, smeans = None, 0.0, []
+        top = np.zeros(0)          # the largest squared deviations from the running mean, for the share of the ten largest
         for k in range(1, n + 1):
             t0 = time.perf_counter()
             e.trace(1)
@@ -123,10 +131,14 @@ def bench_guide(a):
             prev = Y
             smeans.append(float(x.mean()))
             s1 += x; s2 += x * x
+            top = np.sort(np.concatenate([top, np.partition(x.ravel(), -16)[-16:]]))[-16:]
             xb = x[:a.height // 4 * 4, :a.width // 4 * 4].reshape(a.height // 4, 4, a.width // 4, 4).mean(axis=(1, 3))
             b1 += xb; b2 += xb * xb
         mean = s1 / n
         var = (s2 - n * mean * mean) / (n - 1)
+        # how much of the squared deviation the ten largest pixel values carry (fireflies: a share near 1)
+        dev_top = (top[-10:] - float(mean.mean())) ** 2
+        top10 = float(dev_top.sum() / max(var.sum() * (n - 1), 1e-300))
         bmean = b1 / n
         bvar = (b2 - n * bmean * bmean) / (n - 1)
         ts = t / n
@@ -136,7 +148,7 @@ def bench_guide(a):
             'guided' if guided else 'plain', ts * 1e3, mu, var.mean(), bvar.mean()))
         return {'ms_per_sample': ts * 1e3, 'train_s': t_train, 'mean': mu, 'var': float(var.mean()),
                 'var_4x4': float(bvar.mean()), 'seconds_to_1pct': to1(var.mean()), 'seconds_to_1pct_4x4': to1(bvar.mean()),
-                'sample_mean_sd': float(np.std(smeans, ddof=1)),
+                'sample_mean_sd': float(np.std(smeans, ddof=1)), 'largest_value': float(top[-1]), 'top10_share': top10,
                 'stats': {k: v for k, v in e.stats().items() if k in ('segments', 'guided', 'guide_records', 'guide_steps')}}, var
     plain, vp = run(False)
     guided, vg = run(True)
@@ -144,7 +156,7 @@ def bench_guide(a):
     # independent (they share the sample's seed-level draws), so per-pixel variances understate the mean's error
     z = abs(guided['mean'] - plain['mean']) / math.sqrt((plain['sample_mean_sd'] ** 2 + guided['sample_mean_sd'] ** 2) / a.steps)
     ratio = vp / np.maximum(vg, 1e-30)
-    out = {'metric': 'seconds to 1% rel. RMSE, guided vs plain', 'unit': 's', 'higher_is_better': False,
+    out = {'metric': 'seconds to 1% rel. RMSE, guided vs plain', 'unit': 's', 'higher_is_better': False, 'uniform_overrides': a.set,
            'value': guided['seconds_to_1pct_4x4'], 'plain': plain, 'guided': guided,
            'speedup_1pct': plain['seconds_to_1pct'] / guided['seconds_to_1pct'],
            'speedup_1pct_4x4': plain['seconds_to_1pct_4x4'] / guided['seconds_to_1pct_4x4'],
@@ -234,6 +246,7 @@ def main():
     ap.add_argument('--scene', default='caustic-c3')
     ap.add_argument('--train', type=int, default=300)
     ap.add_argument('--nee', type=int, default=None, help='with --guide: light sampling on (1) or off (0)')
+    ap.add_argument('--set', action='append', default=[], help='with --guide: a uniform, name=value (haze.sigmaS=0)')
     ap.add_argument('--calibrate-mib', type=int, default=0, help='a device copy of this many MiB first (counter scale)')
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=32)
@@ -241,6 +254,8 @@ def main():
     ap.add_argument('--width', type=int, default=1920)
     ap.add_argument('--height', type=int, default=1080)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-autotune', action='store_true', help='keep the library\'s default launch choices')
+    ap.add_argument('--tuning', default=None, help='blocksPerCu,refillMarch,refillOcclude: set, not tuned')
     ap.add_argument('--stragglers', type=int, default=None, help='rounds beyond maxbounce + 1 (default: the library\'s)')
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
@@ -269,6 +284,15 @@ def main():
     e.set_scene(made['scene'])
     u = pt.uniforms_for(made)
     e.set_uniforms(u)
+    # the persistent kernels' launch choices for this scene and image, by measurement (before the warm-up; untimed)
+    if a.tuning:
+        b, rm, ro = (int(v) for v in a.tuning.split(','))
+        e.set_tuning(b, rm, ro)
+        tuning = dict(e.tuning(), msPerSample=None, msBefore=None, candidates=0)
+    else:
+        tuning = None if a.no_autotune else e.autotune(samples=2)
+    if tuning:
+        log('bench: autotune %s' % tuning)
     e.clear_accums()
     e.trace(a.warmup)
     e.sync()
@@ -312,7 +336,10 @@ def main():
         'dtype': 'f32', 'data': 'synthetic: seeded procedural room (room-c2), emitted by the project\'s JS emitter',
         'config': {'workload': 'room-c2: 5x4x3 m room, 32 SDF primitives, 8 smooth unions, GGX, 0.5 m panel 5000 cd/m^2, '
                                '%dx%d, depth 8, NEE on, 1 sample per step' % (a.width, a.height),
-                   'width': a.width, 'height': a.height, 'maxbounce': u.maxbounce, 'accumulation': 'f64'},
+                   'width': a.width, 'height': a.height, 'maxbounce': u.maxbounce, 'accumulation': 'f64',
+                   'straggler_rounds': e.straggler_rounds,
+                   'tuning': None if not tuning else {k: tuning[k] for k in ('blocksPerCu', 'refillMarch', 'refillOcclude',
+                                                                            'msPerSample', 'msBefore', 'candidates')}},
         'shadow_segments_per_s': st['shadow_segments'] / dt, 'march_steps_per_s': st['steps'] / dt,
         'mean_steps_per_segment': st['steps'] / max(segs, 1),
         'roofline': roof, 'cpu_baseline': cpu,
