@@ -64,7 +64,7 @@ struct PtDev {
     float* gGradOut;       // NULL, or 16104: the last step's summed gradient
     unsigned short* gW;    // the BF16 copy, each layer transposed, for the MFMA
     float* gBias;          // 256: b1, b2, b3, b4 (padded)
-    float* gMix;           // per slot, 10 float4: the vertex's mixture
+    float* gMix;           // per slot, 11 float4: the vertex's mixture (pt_guide.h guideStoreMix)
     float* gRec;           // recorded vertices: [path][vertex], 7 float4
     float* gPath;          // per recorded path, 2 float4: its total (rgb, done), its light sampling's (rgb)
     unsigned int* gVerts;  // per recorded path: its vertices recorded

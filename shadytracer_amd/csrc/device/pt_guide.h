@@ -43,7 +43,7 @@
 #define PT_G_BIAS 256          // b1, b2, b3 at 0, 64, 128; b4 at 192 (64, the last 23 zero)
 #define PT_G_KLO -3.0f
 #define PT_G_KHI 7.0f
-#define PT_G_MIX_F4 11         // a slot's mixture: (mu, kappa) x 8, the 8 weights, the guided fraction
+#define PT_G_MIX_F4 11         // a slot's mixture: (kappa mu, folded exponent) x 8, the 8 weights, the guided fraction
 #define PT_G_REC_F4 7          // a recorded vertex (below)
 #define PT_G_SAMPLE_F4 4       // a training sample (below)
 #define PT_G_TILE_LD 88        // the activation tile's row stride (bf16): 80 features and 8 of padding
@@ -208,22 +208,8 @@ __device__ __forceinline__ GuideMix guideMixOf(const float* o) {
     m.c = 1.0f / (1.0f + expf(-o[40]));
     return m;
 }
-__device__ __forceinline__ float guidePdf(const GuideMix& m, V3 d) {
-    float q = 0.0f;
-#pragma unroll
-    for (int j = 0; j < PT_G_LOBES; ++j) {
-        const float4 L = m.lobe[j];
-        q += m.w[j] * expf(vmfLogNorm(L.w) + L.w * (L.x * d.x + L.y * d.y + L.z * d.z - 1.0f));
-    }
-    return q;
-}
-// A direction from the mixture: a lobe by its weight (u.x), then Jakob 2012's inversion w = 1 + log(u1 + (1 - u1)
-// e^{-2k}) / k about its mean.
-__device__ __forceinline__ V3 guideSample(const GuideMix& m, V4 u) {
-    int j = 0;
-    float acc = m.w[0];
-    while (j < PT_G_LOBES - 1 && u.x >= acc) { ++j; acc += m.w[j]; }
-    const float4 L = m.lobe[j];
+// A direction from one lobe (mu, kappa): Jakob 2012's inversion w = 1 + log(u1 + (1 - u1) e^{-2k}) / k about its mean.
+__device__ __forceinline__ V3 guideLobeSample(float4 L, V4 u) {
     float k = L.w;
     float wz = 1.0f + logf(u.y + (1.0f - u.y) * expf(-2.0f * k)) / k;
     wz = fminf(fmaxf(wz, -1.0f), 1.0f);
@@ -233,23 +219,56 @@ __device__ __forceinline__ V3 guideSample(const GuideMix& m, V4 u) {
     frameOf(mu, t, b);
     return normalize(t * (sr * cp) + b * (sr * sp) + mu * wz);
 }
+// The lobe u falls in by the weights' running sum: compares unrolled, no index into a private array (which would put
+// the weights in scratch memory)
+__device__ __forceinline__ int guideLobePick(const float* w, float u) {
+    int j = PT_G_LOBES - 1;
+    float acc = 0.0f;
+#pragma unroll
+    for (int k = 0; k < PT_G_LOBES - 1; ++k) {
+        acc += w[k];
+        if (j == PT_G_LOBES - 1 && u < acc) j = k;
+    }
+    return j;
+}
+
+// A vertex's mixture as shade reads it, straight from the slot's record where it is used (shade holds no copy of it).
+// The record (PT_G_MIX_F4 float4s, written by the inference) folds each lobe's weight and normalisation into its
+// exponent: lobe j as (k mu, ln(w_j C(k_j)) - k_j), C(k) = k / (2 pi (1 - e^{-2k})), so that the mixture's density at d
+// is sum_j exp(v_j . d + b_j), one exponential a lobe; then the weights as they are (to pick a lobe); then the fraction.
 __device__ __forceinline__ void guideStoreMix(const PtDev* __restrict__ D, unsigned int slot, const GuideMix& m) {
     float4* dst = reinterpret_cast<float4*>(D->gMix) + (size_t)PT_G_MIX_F4 * slot;
 #pragma unroll
-    for (int j = 0; j < PT_G_LOBES; ++j) dst[j] = m.lobe[j];
+    for (int j = 0; j < PT_G_LOBES; ++j) {
+        const float4 L = m.lobe[j];
+        const float b = logf(fmaxf(m.w[j], 1e-38f)) + vmfLogNorm(L.w) - L.w;
+        dst[j] = make_float4(L.x * L.w, L.y * L.w, L.z * L.w, b);
+    }
     dst[8] = make_float4(m.w[0], m.w[1], m.w[2], m.w[3]);
     dst[9] = make_float4(m.w[4], m.w[5], m.w[6], m.w[7]);
     dst[10] = make_float4(m.c, 0.0f, 0.0f, 0.0f);
 }
-__device__ __forceinline__ GuideMix guideLoadMix(const PtDev* __restrict__ D, unsigned int slot) {
-    GuideMix m;
-    const float4* src = reinterpret_cast<const float4*>(D->gMix) + (size_t)PT_G_MIX_F4 * slot;
+__device__ __forceinline__ const float4* guideMixAt(const PtDev* __restrict__ D, unsigned int slot) {
+    return reinterpret_cast<const float4*>(D->gMix) + (size_t)PT_G_MIX_F4 * slot;
+}
+__device__ __forceinline__ float guideFracAt(const PtDev* __restrict__ D, unsigned int slot) { return guideMixAt(D, slot)[10].x; }
+__device__ __forceinline__ float guidePdfAt(const PtDev* __restrict__ D, unsigned int slot, V3 d) {
+    const float4* src = guideMixAt(D, slot);
+    float q = 0.0f;
 #pragma unroll
-    for (int j = 0; j < PT_G_LOBES; ++j) m.lobe[j] = src[j];
-    float4 a = src[8], b = src[9];
-    m.w[0] = a.x; m.w[1] = a.y; m.w[2] = a.z; m.w[3] = a.w; m.w[4] = b.x; m.w[5] = b.y; m.w[6] = b.z; m.w[7] = b.w;
-    m.c = src[10].x;
-    return m;
+    for (int j = 0; j < PT_G_LOBES; ++j) {
+        const float4 L = src[j];
+        q += __expf(L.x * d.x + L.y * d.y + L.z * d.z + L.w);
+    }
+    return q;
+}
+__device__ __forceinline__ V3 guideSampleAt(const PtDev* __restrict__ D, unsigned int slot, V4 u) {
+    const float4* src = guideMixAt(D, slot);
+    const float4 wa = src[8], wb = src[9];
+    const float w[PT_G_LOBES] = {wa.x, wa.y, wa.z, wa.w, wb.x, wb.y, wb.z, wb.w};
+    const float4 V = src[guideLobePick(w, u.x)];
+    const float k = sqrtf(V.x * V.x + V.y * V.y + V.z * V.z), inv = 1.0f / k;
+    return guideLobeSample(make_float4(V.x * inv, V.y * inv, V.z * inv, k), u);
 }
 
 // Where guiding applies: a camera path's surface vertex whose material has a lobe to evaluate, reflects only (a

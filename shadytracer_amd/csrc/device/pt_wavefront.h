@@ -502,9 +502,8 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
                 // density below is the two strategies' mixture
                 const bool gElig = !photon && guideMaterial(m);
                 const bool guided = gElig && guideActive(D);
-                GuideMix gmix;
-                float gc = 0.0f;
-                if (guided) { gmix = guideLoadMix(D, slot); gc = fminf(gmix.c, D->u.guideFrac); }
+                // the mixture is read from the slot's record where it is used, not held across the shading
+                const float gc = guided ? fminf(guideFracAt(D, slot), D->u.guideFrac) : 0.0f;
                 bool gathered = false;
                 // a surface with a lobe the map can be gathered through (any but a purely delta one)
                 const bool evaluable = !lobesNone(m);
@@ -555,7 +554,7 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
                     if (pS > 0.0f && max3(le) > 0.0f) {
                         V3 f = lobesEval(lb, D->mf, L);
                         float pB = lobesPdf(lb, L);
-                        if (guided) pB = gc * guidePdf(gmix, L) + (1.0f - gc) * pB;
+                        if (guided) pB = gc * guidePdfAt(D, slot, L) + (1.0f - gc) * pB;
                         float NoL = dot(nrm, L);
                         V3 contrib = beta * f * le * (NoL * misWeight(pS, pB) / pS);
                         if (max3(contrib) > 1e-7f) {
@@ -577,7 +576,7 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
                     // direction is a placeholder, not a draw of the BSDF's density
                     bool valid = max3(s.w) > 0.0f;
                     if (ug.w < gc) {
-                        s.dir = guideSample(gmix, ug);
+                        s.dir = guideSampleAt(D, slot, ug);
                         s.isDelta = false; s.transmitted = false; s.ev = PT_EV_EVAL;
                         valid = true;
                     }
@@ -586,7 +585,7 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
                     } else if (s.isDelta) {
                         s.w = s.w * (1.0f / (1.0f - gc));
                     } else {
-                        pTot = gc * guidePdf(gmix, s.dir) + (1.0f - gc) * lobesPdf(lb, s.dir);
+                        pTot = gc * guidePdfAt(D, slot, s.dir) + (1.0f - gc) * lobesPdf(lb, s.dir);
                         const float cosO = dot(nrm, s.dir);
                         s.w = (pTot > 0.0f && cosO > 0.0f) ? lobesEval(lb, D->mf, s.dir) * (cosO / pTot) : v3s(0.0f);
                     }

@@ -197,7 +197,8 @@ def test_trained_guiding_gains_over_bsdf_sampling(engine):
     median = float(np.median(vp[lit] / np.maximum(vg[lit], 1e-30)))
     say('plain/guided variance: mean %.2f, median pixel %.2f; means %.4g vs %.4g (%.2f sigma)' % (ratio, median, gm.mean(), pm.mean(), z))
     assert z < 4.5, (gm.mean(), pm.mean())
-    assert ratio > 2.0 and median > 1.5, (ratio, median)
+    # the median pixel's ratio is the steady figure (~25); the mean's rides on the rare BSDF hits on the panel (1.9-2.4)
+    assert median > 5.0 and ratio > 1.5, (ratio, median)
 
 
 def furnace_samples(engine, u, n):
