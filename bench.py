@@ -98,6 +98,8 @@ def bench_guide(a):
         u = pt.uniforms_for(made, **extra)
         if a.nee is not None:
             u.nee = a.nee
+        if guided:
+            u.guide, u.guideFrac = 1, 0.9
         for item in a.set:
             # name=value, a dotted name reaching into a structure (haze.sigmaS=0)
             name, value = item.split('=', 1)
@@ -105,8 +107,8 @@ def bench_guide(a):
             for part in parts[:-1]:
                 obj = getattr(obj, part)
             setattr(obj, parts[-1], type(getattr(obj, parts[-1]))(float(value)))
-        if guided:
-            u.guide, u.guideFrac = 1, 0.9
+        if not guided:
+            u.guide = 0
         e.set_uniforms(u)
         e.clear_accums()
         t_train = 0.0
@@ -115,7 +117,9 @@ def bench_guide(a):
             e.trace(a.train)
             e.sync()
             t_train = time.perf_counter() - t0
-            e.clear_accums()
+        # each estimator timed with its own launch choices (the tuner's samples train the guided one further; untimed)
+        tune = e.autotune(samples=2)
+        e.clear_accums()
         n = a.steps
         s1 = np.zeros((a.height, a.width)); s2 = np.zeros_like(s1)
         b1 = np.zeros((a.height // 4, a.width // 4)); b2 = np.zeros_like(b1)
@@ -147,6 +151,7 @@ def bench_guide(a):
         log('bench: %s %.3f ms a sample, mean %.4g, mean var %.4g, 4x4-bin var %.4g' % (
             'guided' if guided else 'plain', ts * 1e3, mu, var.mean(), bvar.mean()))
         return {'ms_per_sample': ts * 1e3, 'train_s': t_train, 'mean': mu, 'var': float(var.mean()),
+                'tuning': {k: tune[k] for k in ('blocksPerCu', 'refillMarch', 'refillOcclude')},
                 'var_4x4': float(bvar.mean()), 'seconds_to_1pct': to1(var.mean()), 'seconds_to_1pct_4x4': to1(bvar.mean()),
                 'sample_mean_sd': float(np.std(smeans, ddof=1)), 'largest_value': float(top[-1]), 'top10_share': top10,
                 'stats': {k: v for k, v in e.stats().items() if k in ('segments', 'guided', 'guide_records', 'guide_steps')}}, var

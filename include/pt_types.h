@@ -218,7 +218,8 @@ typedef struct PtUniforms {
     PtEnv env;
     PtHaze haze;
     float faceShare;            // share of light sampling that goes to light faces (0.5)
-    unsigned int sppm;          // 1: camera paths after delta-only chains gather from the photon map
+    unsigned int sppm;          // 1: camera paths after delta-only chains gather from the photon map (the reference's);
+                                // 2: every non-delta camera vertex gathers (caustics seen after a diffuse bounce too)
     float sppmRadius;           // r_1: the first sample's gather radius, metres
     unsigned int sppmPhotons;   // photons emitted in each sample's photon pass
     unsigned int marchRefill;   // lanes a wave lets go idle before it fetches more rays

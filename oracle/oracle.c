@@ -887,7 +887,7 @@ static void tracePath(const OrScene* S, const OrTiles* T, const uint32_t* seed, 
         OrLobes L = makeLobes(nrm, neg(dir), m, n0 / n1);
         double fs = faceShare(S, bounces);
         int gathered = 0;
-        if (map && deltaOnly && evaluable(m)) {
+        if (map && (u->sppm >= 2 || deltaOnly) && evaluable(m)) {   /* sppm 2: at every non-delta camera vertex */
             rad = add(rad, mul(beta, mapGather(map, &L, p, nrm)));
             gathered = 1;
             st->gathers++;

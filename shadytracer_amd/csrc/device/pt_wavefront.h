@@ -514,8 +514,10 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
                     out.store = true;
                     out.sp = p; out.spow = beta; out.sl = -dir; out.sn = nrm;
                     out.ssub = min((unsigned int)(ud.z * (float)PT_SPPM_SUBS), PT_SPPM_SUBS - 1u);
-                } else if (!photon && D->u.sppm != 0u && (flags & PT_F_DELTAONLY) && evaluable) {
+                } else if (!photon && D->u.sppm != 0u && (D->u.sppm >= 2u || (flags & PT_F_DELTAONLY)) && evaluable) {
                     // THE GATHER (core/sppm.js:487-494): the camera path's first non-delta vertex after delta events
+                    // (sppm 1, the reference's); with sppm 2 every non-delta camera vertex, so that caustic light a
+                    // path meets after a diffuse bounce is the map's too, not a rare BSDF hit on the sun through glass
                     bool cut = false;
                     rad += beta * sppmGather(D, lb, p, nrm, cut);
                     gathered = true;

@@ -163,7 +163,7 @@ function causticC3(opts = {}) {
     const d = doc('caustic-c3', mats, [roomChildren(hole), ...clear, glass, hazeBox, panel()], env, 10)
     const w = opts.width || 1280, h = opts.height || 720
     return { doc: d, camera: camera([0, -1.95, 1.6], [0.2, 0.5, 0.5], 70, w, h),
-        settings: { width: w, height: h, spp: 10, maxbounce: 8, nee: 1, sppm: 1, sppmIterations: 10, sppmPhotons: 1 << 22, sppmRadius: 0.02, sppmAlpha: 0.7 } }
+        settings: { width: w, height: h, spp: 10, maxbounce: 8, nee: 1, sppm: 2, sppmIterations: 10, sppmPhotons: 1 << 22, sppmRadius: 0.02, sppmAlpha: 0.7 } }
 }
 
 // A collimated beam through a glass slab onto a floor (the closed form of core/test/sppm.test.mjs's collimated case):

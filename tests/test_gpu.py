@@ -252,9 +252,11 @@ def test_sppm_switches_off_and_on(engine):
     assert abs(on / SLAB_FLOOR - 1) < 0.05 and abs(again / SLAB_FLOOR - 1) < 0.05, (on, again)
 
 
-def test_c3_caustic_room_matches_the_oracle(engine):
-    say('case: caustic-c3 160x90, 2^18 photons, 6 samples, 16 regions vs oracle')
-    made, u = load(engine, 'caustic-c3', width=160, height=90, sppmPhotons=1 << 18)
+@pytest.mark.parametrize('mode', [1, 2])
+def test_c3_caustic_room_matches_the_oracle(engine, mode):
+    # sppm 1 gathers at a camera path's first non-delta vertex (the reference's); 2 at every non-delta vertex
+    say('case: caustic-c3 160x90, 2^18 photons, 8 samples, sppm %d, 16 regions vs oracle' % mode)
+    made, u = load(engine, 'caustic-c3', width=160, height=90, sppmPhotons=1 << 18, sppm=mode)
     regions = []
     for by in range(4):
         for bx in range(4):
@@ -264,7 +266,7 @@ def test_c3_caustic_room_matches_the_oracle(engine):
     g = gpu_samples(engine, 8)
     st = engine.stats()
     assert st['nonfinite'] == 0 and st['photons'] > 0 and st['gathers'] > 0 and st['overflow'] == 0
-    o = oracle_samples(made, pt.uniforms_for(made, sppmPhotons=1 << 18), 8)
+    o = oracle_samples(made, pt.uniforms_for(made, sppmPhotons=1 << 18, sppm=mode), 8)
     results = [agree(g, o, m, 0.03) for m in regions + [np.ones((90, 160), bool)]]
     say('worst region at %.2f of its tolerance' % max(r[0] for r in results))
     for r in results:
