@@ -289,4 +289,13 @@ __device__ __forceinline__ void foldFull(SdfHit& h, V4 s, unsigned int mat, unsi
     if (s.w < 0.0f && s.w > h.insideW) { h.insideW = s.w; h.a = mat; }
     if (fabsf(s.w) < fabsf(h.d)) { h.n = xyz(s); h.d = s.w; h.m = mat; h.prim = prim; }
 }
+// A node's bounding sphere (centre c, radius r) can beat the closest |d| so far: |p - c| - r < |best|, compared
+// squared (|p - c|^2 < (r + |best|)^2; both sides non-negative) so that no square root is taken. The squared test is
+// a hair wide (2e-6 relative, above its own rounding), never narrower than the exact inequality: a node it skips is
+// one the exact test skips, which can neither be the closest surface nor hold the point.
+__device__ __forceinline__ bool boundNear(V3 p, float cx, float cy, float cz, float r, float best) {
+    const float dx = p.x - cx, dy = p.y - cy, dz = p.z - cz;
+    const float R = r + fabsf(best);
+    return dx * dx + dy * dy + dz * dz < R * R * (1.0f + 2e-6f);
+}
 __device__ __forceinline__ void foldDist(float& best, float d) { if (fabsf(d) < fabsf(best)) best = d; }

@@ -29,7 +29,7 @@ const PRIM_FLOPS = { plane: 1, sphere: 7, box: 27, cylinder: 24, capsule: 14, to
 const XF_FLOPS = 22      // p - pos, three dot products, times 1/s; the distance times s
 const FOLD_FLOPS = 2
 const JOIN_FLOPS = 9
-const CULL_FLOPS = 9     // |p - c| - r against |best|
+const CULL_FLOPS = 11    // |p - c|^2 against (r + |best|)^2, widened a hair (boundNear)
 
 // The builtin shapes: their parameters in U (after the 13 of the transform), and the HIP calls.
 const SHAPES = {
@@ -306,12 +306,12 @@ function generate(doc, opts) {
         const nodeFlops = v.flops + FOLD_FLOPS
         if (isFinite(v.bs.r)) {
             const co = push([v.bs.c[0], v.bs.c[1], v.bs.c[2], v.bs.r])
-            const test = `length(p - v3(U[${co}], U[${co + 1}], U[${co + 2}])) - U[${co + 3}]`
+            const near = best => `boundNear(p, U[${co}], U[${co + 1}], U[${co + 2}], U[${co + 3}], ${best})`
             const bit = maskBits < 64 ? maskBits++ : -1
             const inMask = bit >= 0 ? `((mask >> ${bit}) & 1ull) && ` : ''
-            dist.push(`    if (${inMask}${test} < fabsf(best)) {`, ...body, `        foldDist(best, ${v.d});`, `        w += ${nodeFlops}u;`, '    }')
+            dist.push(`    if (${inMask}${near('best')}) {`, ...body, `        foldDist(best, ${v.d});`, `        w += ${nodeFlops}u;`, '    }')
             dist.push(`    w += ${CULL_FLOPS}u;`)
-            full.push(`    if (${test} < fabsf(h.d)) {`, ...fbody, `        foldFull(h, ${v.f}, ${v.m}, ${v.k});`, '    }')
+            full.push(`    if (${near('h.d')}) {`, ...fbody, `        foldFull(h, ${v.f}, ${v.m}, ${v.k});`, '    }')
             if (bit >= 0) {
                 maskCode.push(`    { V3 oc = v3(U[${co}], U[${co + 1}], U[${co + 2}]) - o; float t = fminf(fmaxf(dot(oc, d), 0.0f), tmax);`,
                     `      V3 q = oc - d * t; float rr = U[${co + 3}] + margin; if (dot(q, q) > rr * rr) m &= ~(1ull << ${bit}); }`)
