@@ -107,7 +107,8 @@ typedef struct PtLightFace {
     float weight;
 } PtLightFace;
 
-// Environment: 0 black, 1 uniform sphere, 2 uniform upper hemisphere (z > 0), plus an optional sun.
+// Environment: 0 black, 1 uniform sphere, 2 uniform upper hemisphere (z > 0), 3 the studio -- L (1 + top d.z), never
+// below 0, over the whole sphere (core/environment.js studioLight:165-174) -- plus an optional sun.
 typedef struct PtEnv {
     unsigned int type;
     float L;              // luminance, cd/m^2
@@ -119,7 +120,8 @@ typedef struct PtEnv {
     float sunPoint[3];    // the centre of the beam's disc: with an aperture, the sun is seen along the rays that
                           // cross the disc of radius sunRadius about sunPoint, perpendicular to sunDir (and the
                           // photon pass emits the beam from that disc)
-    float pad[2];
+    float top;            // the studio's top light, 0..1 (type 3)
+    float pad;
 } PtEnv;
 
 // The homogeneous haze box (the reference's haze.js hazeUniforms): axis-aligned, centre and half size.

@@ -79,7 +79,7 @@ static void sampleSeed(uint32_t base, uint32_t sample, uint32_t* out) {
 
 /* ---- the scene program (hipgen.js `program`) ---- */
 enum { OP_PRIM = 0, OP_JOIN = 1, OP_FOLD = 2, OP_NODE = 3 };
-enum { SH_PLANE = 0, SH_SPHERE, SH_BOX, SH_CYLINDER, SH_CAPSULE, SH_TORUS, SH_RECT, SH_DISC };
+enum { SH_PLANE = 0, SH_SPHERE, SH_BOX, SH_CYLINDER, SH_CAPSULE, SH_TORUS, SH_RECT, SH_DISC, SH_LENS, SH_PRISM };
 enum { J_UNION = 0, J_SUBTRACT, J_INTERSECT };
 /* prim: shape, xf, p, mat, prim; join: join, k; node (a top-level node's start): xf = its bounding sphere's offset in
  * U (-1: unbounded), join = its mask bit (-1: none), k = the index of its fold */
@@ -97,7 +97,22 @@ typedef struct {
     const PtUniforms* u;
 } OrScene;
 
-/* Primitive (outward normal, distance) in its own frame: core/sdf.js:352-628 restated. */
+/* One face of a spherical lens (core/sdf.js lensFace:450): s the side (+1 the +z face), its vertex at s*vz, R its
+ * signed radius (0 flat, positive bulging out, negative hollowed in, closed by the plane through its rim). */
+static d4 lensFace(d3 p, double s, double vz, double R, double rim) {
+    if (R == 0) return d4of(v(0, 0, s), s * p.z - vz);
+    d3 q = sub(p, v(0, 0, s * (vz - R)));
+    double l = maxd(len(q), 1e-12), sg = R > 0 ? 1 : -1;
+    d4 f = d4of(scl(q, sg / l), sg * (l - fabs(R)));
+    if (R < 0) {
+        double sag = fabs(R) - sqrt(maxd(R * R - rim * rim, 0)), w = s * p.z - (vz + sag);
+        if (w > f.w) f = d4of(v(0, 0, s), w);
+    }
+    return f;
+}
+
+/* Primitive (outward normal, distance) in its own frame: core/sdf.js:352-628 restated; the lens core/sdf.js:463, the
+ * prism core/shapes.js:263-287 (a regular polygon extruded). */
 static d4 primLocal(int shape, d3 p, const float* P) {
     switch (shape) {
     case SH_PLANE: return d4of(v(0, 0, 1), p.z);
@@ -152,6 +167,31 @@ static d4 primLocal(int shape, d3 p, const float* P) {
         if (e <= 0 || d < 1e-12) return d4of(v(0, 0, s), fabs(p.z));
         double il = 1 / maxd(l, 1e-12);
         return d4of(v(e * p.x * il / d, e * p.y * il / d, p.z / d), d);
+    }
+    case SH_LENS: {     /* P: rim, vz, r1, r2 (the planes after them are the emitter's, recomputed here) */
+        double rim = P[0], vz = P[1], lxy = maxd(sqrt(p.x * p.x + p.y * p.y), 1e-12);
+        d4 n = d4of(v(p.x / lxy, p.y / lxy, 0), lxy - rim);
+        d4 a = lensFace(p, 1, vz, P[2], rim);
+        if (a.w > n.w) n = a;
+        d4 b = lensFace(p, -1, vz, P[3], rim);
+        if (b.w > n.w) n = b;
+        return n;
+    }
+    case SH_PRISM: {    /* P[4] the half height, P[5] the document's sides, P[6] its corner radius */
+        double sides = P[5], nn = clampd(nearbyint(sides), 3, 64), R = maxd(P[6], 0), h = P[4];
+        double an = PI_D / nn, a = R * cos(an), he = R * sin(an);
+        double k = nearbyint((atan2(p.y, p.x) + 0.5 * PI_D) / (2 * an)), phi = -0.5 * PI_D + 2 * an * k;
+        double nx = cos(phi), ny = sin(phi), u = p.x * nx + p.y * ny, w = -p.x * ny + p.y * nx;
+        double gx = nx, gy = ny, g = u - a;
+        if (g > 0) {
+            double ey = w - clampd(w, -he, he), d = sqrt(g * g + ey * ey);
+            if (d > 1e-12) { gx = (nx * g - ny * ey) / d; gy = (ny * g + nx * ey) / d; }
+            g = d;
+        }
+        double wz = fabs(p.z) - h, sz = p.z >= 0 ? 1 : -1;
+        if (g > 0 && wz > 0) { double l = sqrt(g * g + wz * wz); return d4of(v(gx * g / l, gy * g / l, sz * wz / l), l); }
+        if (g > wz) return d4of(v(gx, gy, 0), g);
+        return d4of(v(0, 0, sz), wz);
     }
     }
     return d4of(v(0, 0, 1), 1e30);
@@ -474,6 +514,7 @@ static double sunOmega(const PtEnv* e) { return 2 * PI_D * (1 - e->sunCos); }
 static int inSun(const PtEnv* e, d3 d) { return e->sunE > 0 && dotp(d, f3(e->sunDir)) >= e->sunCos; }
 static d3 envBg(const PtEnv* e, d3 d) {
     if (e->type == 1 || (e->type == 2 && d.z > 0)) return scl(f3(e->col), e->L);
+    if (e->type == 3) return scl(f3(e->col), e->L * maxd(1 + e->top * d.z, 0));   /* environment.js studioLight:168 */
     return v(0, 0, 0);
 }
 /* the sun through its aperture: the lines crossing the disc of radius sunRadius about sunPoint, perpendicular to the

@@ -137,6 +137,7 @@ __device__ __forceinline__ float sunSolidAngle(const PtEnv& e) { return 2.0f * P
 __device__ __forceinline__ bool inSun(const PtEnv& e, V3 d) { return e.sunE > 0.0f && dot(d, ld3(e.sunDir)) >= e.sunCos; }
 __device__ __forceinline__ V3 envBackground(const PtEnv& e, V3 d) {
     if (e.type == 1u || (e.type == 2u && d.z > 0.0f)) return ld3(e.col) * e.L;
+    if (e.type == 3u) return ld3(e.col) * (e.L * fmaxf(1.0f + e.top * d.z, 0.0f));
     return v3s(0.0f);
 }
 // The sun's radiance along the ray from p towards d. With an aperture the sun is a beam: the lines that cross the disc

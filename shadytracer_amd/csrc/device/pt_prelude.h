@@ -224,6 +224,90 @@ __device__ __forceinline__ float Ddisc(V3 p, float r) {
     return fsqrt(e * e + p.z * p.z);
 }
 
+// A spherical lens along z (sdf.js Nlens:463, lensFace:450): rim radius rim, the faces' vertices at z = +-vz, r1 the
+// +z face's signed radius and r2 the -z face's (positive bulges out, negative is hollowed in, 0 is flat). The solid is
+// the rim's cylinder cut by each face's solid -- a bulging face's ball; a hollow face's ball taken away, closed by the
+// plane through the face's rim at z = +-cap -- and its value the largest of the pieces' (exact off the edges, a bound
+// near them). s is the face's side (+1, -1); cap matters only for a hollow face.
+__device__ __forceinline__ V4 NlensFace(V3 p, float s, float vz, float R, float cap) {
+    if (R == 0.0f) return v4(v3(0.0f, 0.0f, s), s * p.z - vz);
+    V3 q = v3(p.x, p.y, p.z - s * (vz - R));        // from the face's centre of curvature
+    float l = fmaxf(length(q), 1e-12f);
+    float sg = R > 0.0f ? 1.0f : -1.0f;
+    V4 f = v4(q * (sg * frcp(l)), sg * (l - fabsf(R)));
+    float w = s * p.z - cap;
+    if (R < 0.0f && w > f.w) f = v4(v3(0.0f, 0.0f, s), w);
+    return f;
+}
+__device__ __forceinline__ V4 Nlens(V3 p, float rim, float vz, float r1, float r2, float cap1, float cap2) {
+    float lxy = fmaxf(fsqrt(p.x * p.x + p.y * p.y), 1e-12f);
+    float il = frcp(lxy);
+    V4 n = v4(v3(p.x * il, p.y * il, 0.0f), lxy - rim);
+    V4 a = NlensFace(p, 1.0f, vz, r1, cap1);
+    if (a.w > n.w) n = a;
+    V4 b = NlensFace(p, -1.0f, vz, r2, cap2);
+    if (b.w > n.w) n = b;
+    return n;
+}
+// one face's value, z already times the face's side, l2 = x^2 + y^2
+__device__ __forceinline__ float DlensFace(float l2, float z, float vz, float R, float cap) {
+    if (R == 0.0f) return z - vz;
+    float dz = z - (vz - R);
+    float f = fmaxf(fsqrt(l2 + dz * dz), 1e-12f) - fabsf(R);
+    return R > 0.0f ? f : fmaxf(-f, z - cap);
+}
+__device__ __forceinline__ float Dlens(V3 p, float rim, float vz, float r1, float r2, float cap1, float cap2) {
+    float l2 = p.x * p.x + p.y * p.y;
+    float d = fmaxf(fsqrt(l2), 1e-12f) - rim;
+    d = fmaxf(d, DlensFace(l2, p.z, vz, r1, cap1));
+    return fmaxf(d, DlensFace(l2, -p.z, vz, r2, cap2));
+}
+
+// A prism along z (shapes.js Nprism:285 over g2RegularPolygon and g2Extrude): a regular polygon of n sides, its
+// corners at radius R and one flat side facing -y, extruded to half height h. in = 1/n, a = R cos(pi/n) the apothem,
+// e = R sin(pi/n) half a side. The nearest side is the one whose outward normal is nearest the point's direction:
+// inside, that side has the largest projection; outside, a point in its wedge is nearest that side or one of its ends.
+// Side k's normal points at -pi/2 + 2 pi k / n; (u, v) is the point along that normal and along its tangent.
+__device__ __forceinline__ void prismSide(float x, float y, float n, float in, float& nx, float& ny, float& u, float& v) {
+    float k = rintf((atan2f(y, x) * (0.5f * PT_INV_PI) + 0.25f) * n);
+    float s, c;
+    sincospif(2.0f * k * in, &s, &c);
+    nx = s;
+    ny = -c;
+    u = x * nx + y * ny;
+    v = x * c + y * s;
+}
+__device__ __forceinline__ V4 Nprism(V3 p, float n, float in, float a, float e, float h) {
+    float nx, ny, u, v;
+    prismSide(p.x, p.y, n, in, nx, ny, u, v);
+    float gx = nx, gy = ny, g = u - a;
+    if (g > 0.0f) {
+        float ey = v - fminf(fmaxf(v, -e), e);
+        float d = fsqrt(g * g + ey * ey);
+        if (d > 1e-12f) {
+            float id = frcp(d);
+            gx = (nx * g - ny * ey) * id;
+            gy = (ny * g + nx * ey) * id;
+        }
+        g = d;
+    }
+    float wz = fabsf(p.z) - h, sz = p.z >= 0.0f ? 1.0f : -1.0f;
+    if (g > 0.0f && wz > 0.0f) {
+        float l = fsqrt(g * g + wz * wz), il = frcp(l);
+        return v4(v3(gx * g * il, gy * g * il, sz * wz * il), l);
+    }
+    if (g > wz) return v4(v3(gx, gy, 0.0f), g);
+    return v4(v3(0.0f, 0.0f, sz), wz);
+}
+__device__ __forceinline__ float Dprism(V3 p, float n, float in, float a, float e, float h) {
+    float nx, ny, u, v;
+    prismSide(p.x, p.y, n, in, nx, ny, u, v);
+    float g = u - a, ey = v - fminf(fmaxf(v, -e), e);
+    g = g > 0.0f ? fsqrt(g * g + ey * ey) : g;
+    float wz = fabsf(p.z) - h;
+    return g > 0.0f && wz > 0.0f ? fsqrt(g * g + wz * wz) : fmaxf(g, wz);
+}
+
 // ---------------------------------------------------------------------------------------------------
 // CSG joins (scene_api.js:352-373): a k > 0 blend is the polynomial smooth min/max, the normal the gradient of
 // the blend (mix of the two), none where either side has none. ik is 1/k (0 when k is 0), from the parameter block.

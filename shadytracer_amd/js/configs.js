@@ -18,6 +18,9 @@
  *   sppm-slab    a collimated beam through a glass slab onto a floor: the photon map's closed form.
  *   furnace-*    closed-form checks: a white furnace (albedo-1 Lambertian plane under a uniform sky, and a haze box
  *                under a uniform sphere of sky).
+ *   studio-floor, lens-prism
+ *                documents in the builder's own form (millimetres, the studio environment, lights by CCT and flux,
+ *                an orbit view), the second with the lens and prism builtins.
  */
 
 const SEED = 20261203
@@ -207,7 +210,46 @@ function furnaceHaze(opts = {}) {
     return { doc: d, camera: camera([0, -4, 0], [0, 0, 0], 30, w, h), settings: { width: w, height: h, spp: 64, maxbounce: 250, nee: 1 } }
 }
 
+// Documents as the builder writes them (millimetres, the studio environment -- envMode 1, no `sky` of ours -- lights by
+// CCT and flux, the camera as an orbit view), for the emitter's reading of the reference's own scene format.
+// studio-floor: a Lambertian floor of albedo 0.6 under a studio of 500 lx (top 0.5): the floor reads 0.6 x 500 / pi
+// cd/m^2 exactly, whatever the top light; the sky above the horizon L0 (1 + 0.5 d.z).
+function studioDoc(title, materials, children, blocks, view) {
+    return { format: 'raytracer-scene', version: 31, title, unit: 'mm', envRadius: 20000, materials, blocks,
+        root: { id: 'root', kind: 'group', label: title, enabled: true, blend: 0, children },
+        environment: { envMode: 1, studioIlluminance: 500, studioTop: 0.5, studioColor: { r: 1, g: 1, b: 1 } }, view: { camera: view } }
+}
+function studioFloor(opts = {}) {
+    const { cameraOf } = require('./hipgen')
+    const albedo = opts.albedo !== undefined ? opts.albedo : 0.6
+    const d = studioDoc('studio-floor', [material('floor', [albedo, albedo, albedo], 1, 0, { specular: 0 })],
+        [obj('floor', 'b-plane', 'floor', xf(0, 0, 0))], BLOCKS,
+        { mode: 'orbit', camtype: 0, fov: 50, target: [0, 0, 0], azimuth: -90, elevation: 70, distance: 2000, up: [0, 0, 1], aperture: 0 })
+    const w = opts.width || 64, h = opts.height || 48
+    return { doc: d, camera: cameraOf(d, w, h), settings: { width: w, height: h, spp: 64, maxbounce: 4, nee: 1 } }
+}
+// lens-prism: a plano-convex glass lens (60 mm across, 12 mm thick, r1 62 mm, flat back) standing with its axis along
+// -y in front of a glass prism (triangle of 30 mm corner radius, 80 mm long) lying on a grey floor; a 40 x 40 mm
+// panel of 3000 K, 200 lm behind and above them, and the studio of 500 lx. The lens and the prism are the builder's
+// builtins (lens@1, prism@1).
+function lensPrism(opts = {}) {
+    const { cameraOf } = require('./hipgen')
+    const blocks = BLOCKS.concat([
+        { id: 'b-lens', label: 'lens', kind: 'object', builtin: 'lens@1', params: { diameter: 1, thickness: 0.2, r1: 2, r2: 2 } },
+        { id: 'b-prism', label: 'prism', kind: 'object', builtin: 'prism@1', params: { sides: 3, radius: 0.5, height: 1 } }])
+    const glass = material('glass', [1, 1, 1], 0, 0, { transmission: 1, ior: 1.5168, abbe: 64.17, specular: 0.5 })
+    const d = studioDoc('lens-prism', [material('floor', [0.5, 0.5, 0.5], 0.8, 0, { specular: 0.5 }), glass], [
+        obj('floor', 'b-plane', 'floor', xf(0, 0, 0)),
+        obj('lens', 'b-lens', 'glass', xf(-40, -30, 45, 90, 0, 0), { diameter: 60, thickness: 12, r1: 62, r2: 0 }),
+        obj('prism', 'b-prism', 'glass', xf(45, 20, 0.0, 0, 90, 0), { sides: 3, radius: 30, height: 80 }),
+        { id: 'panel', kind: 'object', type: 'light', label: 'panel', enabled: true, material: null, transform: xf(0, 150, 160, 120, 0, 0),
+            params: { width: 40, height: 40 }, light: { shape: 'rectangle', colourBy: 'cct', cct: 3000, duv: 0, flux: 200 } },
+    ], blocks, { mode: 'orbit', camtype: 0, fov: 40, target: [0, 0, 30], azimuth: -100, elevation: 62, distance: 260, up: [0, 0, 1], aperture: 0 })
+    const w = opts.width || 96, h = opts.height || 64
+    return { doc: d, camera: cameraOf(d, w, h), settings: { width: w, height: h, spp: 64, maxbounce: 8, nee: 1 } }
+}
+
 const CONFIGS = { 'plane-c1': planeC1, 'room-c2': roomC2, 'caustic-c3': causticC3, 'furnace-plane': furnacePlane, 'furnace-haze': furnaceHaze,
-    'sppm-slab': sppmSlab }
+    'sppm-slab': sppmSlab, 'studio-floor': studioFloor, 'lens-prism': lensPrism }
 
 module.exports = { CONFIGS, camera, rng, SEED }

@@ -14,10 +14,10 @@
  *   program    the same tree as a postfix list, for the CPU oracle
  *   flops      FP32 operations of one sceneDist call (the march's roofline count)
  *
- * Scope: the builder's builtin shapes (plane, box, sphere, cylinder, capsule, torus, rectangle, disc), groups with
- * their joins (union, subtract, intersect; smooth blends), transforms (rotation Rz*Ry*Rx in degrees, uniform scale:
- * builder/transform.js:9-38), lights (rectangle and disc faces), one haze box and the studio environment. A code
- * block that is not a builtin -- user WGSL -- is refused with its node named.
+ * Scope: the builder's builtin shapes (plane, box, sphere, cylinder, capsule, torus, rectangle, disc, lens, prism),
+ * groups with their joins (union, subtract, intersect; smooth blends), transforms (rotation Rz*Ry*Rx in degrees,
+ * uniform scale: builder/transform.js:9-38), lights (rectangle and disc faces), one haze box and the studio
+ * environment. A code block that is not a builtin -- user WGSL -- is refused with its node named.
  */
 
 const UNIT_METRES = { m: 1, cm: 0.01, mm: 0.001, um: 1e-6, in: 0.0254 }
@@ -25,7 +25,8 @@ const LUMA = [0.2126, 0.7152, 0.0722]
 
 // FP32 operations per distance-only primitive evaluation in its frame (an FMA counts 2; the transform in and out
 // counted apart), as the emitted code does them. Counted by the kernels per evaluated node: the march's roofline.
-const PRIM_FLOPS = { plane: 1, sphere: 7, box: 27, cylinder: 24, capsule: 14, torus: 14, rectangle: 14, disc: 13 }
+// The prism's atan2f and sincospif are counted as 25 each (their polynomials and the division), an estimate.
+const PRIM_FLOPS = { plane: 1, sphere: 7, box: 27, cylinder: 24, capsule: 14, torus: 14, rectangle: 14, disc: 13, lens: 25, prism: 75 }
 const XF_FLOPS = 22      // p - pos, three dot products, times 1/s; the distance times s
 const FOLD_FLOPS = 2
 const JOIN_FLOPS = 9
@@ -53,10 +54,42 @@ const SHAPES = {
         dist: o => `Drectangle(q, U[${o}], U[${o + 1}])`, full: o => `Nrectangle(q, U[${o}], U[${o + 1}])`,
     },
     disc: { params: P => [P.radius], dist: o => `Ddisc(q, U[${o}])`, full: o => `Ndisc(q, U[${o}])` },
+    // rim radius, vertex half thickness, the faces' signed radii (0 flat), the planes closing hollow faces at the rim
+    lens: {
+        params: (P, unit) => lensParams(P, unit).U,
+        dist: o => `Dlens(q, ${[0, 1, 2, 3, 4, 5].map(i => `U[${o + i}]`).join(', ')})`,
+        full: o => `Nlens(q, ${[0, 1, 2, 3, 4, 5].map(i => `U[${o + i}]`).join(', ')})`,
+    },
+    // sides, 1 / sides, apothem, half a side, half height; the raw sides and corner radius after them (the oracle's)
+    prism: {
+        params: P => prismParams(P).U,
+        dist: o => `Dprism(q, ${[0, 1, 2, 3, 4].map(i => `U[${o + i}]`).join(', ')})`,
+        full: o => `Nprism(q, ${[0, 1, 2, 3, 4].map(i => `U[${o + i}]`).join(', ')})`,
+    },
+}
+// the builtins' own defaults for parameters a document leaves out (builder/primitives.js:172 prism, :237 lens)
+const DEFAULTS = { lens: { diameter: 1.0, thickness: 0.2, r1: 2.0, r2: 2.0 }, prism: { sides: 3, radius: 0.5, height: 1.0 } }
+
+// A lens (sdf.js Nlens:463): a radius under 1e-9 scene units is a flat face; a hollow face (negative radius) is closed
+// by the plane through its rim, its sag |R| - sqrt(R^2 - rim^2) beyond the vertex.
+function lensParams(P, unit) {
+    const rim = 0.5 * P.diameter, vz = 0.5 * P.thickness
+    const R = r => Math.abs(r) < 1e-9 * unit ? 0 : r
+    const r1 = R(P.r1), r2 = R(P.r2)
+    const sag = r => r < 0 ? Math.abs(r) - Math.sqrt(Math.max(r * r - rim * rim, 0)) : 0
+    return { U: [rim, vz, r1, r2, vz + sag(r1), vz + sag(r2)], rim, reach: vz + Math.max(sag(r1), sag(r2)) }
+}
+// A prism (shapes.js Nprism:285): sides rounded (half to even, as WGSL's round) and held to 3..64, the corner radius
+// at least 0.
+function roundHalfEven(x) { const f = Math.floor(x), r = x - f; return r > 0.5 || (r === 0.5 && f % 2 !== 0) ? f + 1 : f }
+function prismParams(P) {
+    const n = Math.min(Math.max(roundHalfEven(P.sides), 3), 64), R = Math.max(P.radius, 0)
+    return { U: [n, 1 / n, R * Math.cos(Math.PI / n), R * Math.sin(Math.PI / n), Math.max(0.5 * P.height, 0), P.sides, P.radius], R }
 }
 // the length-valued parameters of each shape (scaled by the unit); the rest are plain numbers
 const LENGTHS = { sphere: ['radius'], box: ['size', 'rounding'], cylinder: ['radius', 'height', 'rounding'], capsule: ['radius', 'height'],
-    torus: ['ring', 'tube'], rectangle: ['width', 'height'], disc: ['radius'], plane: [] }
+    torus: ['ring', 'tube'], rectangle: ['width', 'height'], disc: ['radius'], plane: [], lens: ['diameter', 'thickness', 'r1', 'r2'],
+    prism: ['radius', 'height'] }
 
 // ---- transforms: world = parent * local; R = Rz Ry Rx (columns: local axes in the parent)
 function rotationOf(t) {
@@ -94,7 +127,7 @@ function xfParams(X) {
     return [X.t[0], X.t[1], X.t[2], R[0][0], R[1][0], R[2][0], R[0][1], R[1][1], R[2][1], R[0][2], R[1][2], R[2][2], X.s, 1 / X.s]
 }
 // A bounding sphere of a shape in its own frame (a radius about its origin); Infinity for the unbounded plane.
-function localRadius(key, P) {
+function localRadius(key, P, unit) {
     switch (key) {
         case 'plane': return Infinity
         case 'sphere': return P.radius
@@ -104,6 +137,10 @@ function localRadius(key, P) {
         case 'torus': return P.ring + P.tube
         case 'rectangle': return 0.5 * Math.hypot(P.width, P.height)
         case 'disc': return P.radius
+        // the rim's cylinder from the farther face's reach to the other's: no face passes its vertex, or its rim's
+        // plane for a hollow one
+        case 'lens': { const L = lensParams(P, unit); return Math.hypot(L.rim, L.reach) }
+        case 'prism': return Math.hypot(prismParams(P).R, 0.5 * Math.max(P.height, 0))
     }
     return Infinity
 }
@@ -153,8 +190,11 @@ function generate(doc, opts) {
     const blocks = {}
     for (const b of doc.blocks || []) blocks[b.id] = b
     const materials = [materialOf(null)]           // 0: the default
-    const matIndex = {}
-    for (const m of doc.materials || []) { matIndex[m.id] = materials.length; materials.push(materialOf(m)) }
+    const matIndex = {}, docMats = {}, checked = {}
+    for (const m of doc.materials || []) { matIndex[m.id] = materials.length; docMats[m.id] = m; materials.push(materialOf(m)) }
+    // what the document asks for that the engine does not draw: refused, or with opts.approximate listed and left out
+    const approximations = []
+    const cannot = (where, what) => approximations.push(`${where}: ${what}`)
     if (materials.length > 64) throw new Error('hipgen: more than 64 materials')
 
     const params = []
@@ -178,7 +218,7 @@ function generate(doc, opts) {
     }
     function paramsOf(n, key) {
         const b = blocks[n.block] || {}
-        const P = Object.assign({}, b.params || {}, n.params || {})
+        const P = Object.assign({}, DEFAULTS[key] || {}, b.params || {}, n.params || {})
         const L = LENGTHS[key]
         const out = {}
         for (const k of Object.keys(P)) {
@@ -198,13 +238,16 @@ function generate(doc, opts) {
             P = n.light.shape === 'disc' ? { radius: (n.params.radius || 0.5) * unit } : { width: (n.params.width || 1) * unit, height: (n.params.height || 1) * unit }
         }
         const xo = push(xfParams(X))
-        const po = push(SHAPES[key].params(P))
+        const po = push(SHAPES[key].params(P, unit))
         layout[n.id] = { xf: xo, params: po, shape: key }
-        return { key, xo, po, P, bs: { c: X.t.slice(), r: localRadius(key, P) * X.s }, flops: PRIM_FLOPS[key] + XF_FLOPS }
+        return { key, xo, po, P, bs: { c: X.t.slice(), r: localRadius(key, P, unit) * X.s }, flops: PRIM_FLOPS[key] + XF_FLOPS }
     }
     function matOf(n) {
         if (n.type === 'light') return lightMaterial(n)
-        if (n.material && matIndex[n.material] !== undefined) return matIndex[n.material]
+        if (n.material && matIndex[n.material] !== undefined) {
+            if (!checked[n.material]) { checked[n.material] = true; for (const why of materialLimits(docMats[n.material])) cannot(`material '${n.material}'`, why) }
+            return matIndex[n.material]
+        }
         return 0
     }
     function lightMaterial(n) {
@@ -214,11 +257,12 @@ function generate(doc, opts) {
         const shape = L.shape === 'disc' ? 1 : 2
         const half = shape === 1 ? [(n.params.radius || 0.5) * unit * X.s, 0] : [0.5 * (n.params.width || 1) * unit * X.s, 0.5 * (n.params.height || 1) * unit * X.s]
         const area = shape === 1 ? Math.PI * half[0] * half[0] : 4 * half[0] * half[1]
-        const col = colourOf(L.color, [1, 1, 1])
-        const y = LUMA[0] * col[0] + LUMA[1] * col[1] + LUMA[2] * col[2]
-        // luminance given, or from the flux of a one-sided Lambertian face: L = flux / (pi A)
-        const lum = L.luminance !== undefined ? L.luminance : (L.flux || 0) / (Math.PI * area * (L.twoSided ? 2 : 1))
-        const le = col.map(c => c / (y || 1) * lum)
+        for (const why of lightLimits(L)) cannot(`light '${n.id}'`, why)
+        const col = lightRGB(L)
+        // a flux (lm) rules over the luminance: the flux of a Lambertian face is pi L A a side (lights.js fluxFactor:201)
+        const lum = L.flux !== undefined && L.flux !== null ? L.flux / (Math.PI * area * (L.twoSided ? 2 : 1))
+            : (L.luminance !== undefined ? L.luminance : 1000)
+        const le = col.map(c => c * lum)
         const R = X.R
         faces.push({
             centre: X.t.slice(), halfx: half[0], u: [R[0][0], R[1][0], R[2][0]], halfy: half[1], v: [R[0][1], R[1][1], R[2][1]], shape,
@@ -294,7 +338,9 @@ function generate(doc, opts) {
             return
         }
         // the node's code, guarded by its bounding sphere: skipped where |p - c| - r cannot beat the closest |d| so
-        // far, which leaves the fold's answer exactly as it was (a skipped node is never the closest, nor negative).
+        // far, which leaves the fold's answer exactly as it was (a skipped node is never the closest, nor negative)
+        // for an exact distance. For a bound (the lens's largest-of-pieces, a smooth join) the skipped value may be
+        // smaller, but the node's true distance is not, so the step stays safe and its surfaces are where they were.
         // And by the ray's node mask (sceneMask): a node whose sphere, widened by the hit tolerance, the ray's segment
         // never meets can neither be hit nor stop the ray, so its march leaves it out.
         const start = program.length
@@ -344,7 +390,10 @@ function generate(doc, opts) {
     for (const c of doc.root.children || []) foldNode(c, compose(IDENT, doc.root.transform, unit))
 
     const envRadius = (doc.envRadius !== undefined ? doc.envRadius * unit : 400)
-    const env = environmentOf(doc.environment || {}, doc)
+    const env = environmentOf(doc.environment || {}, doc, cannot)
+    if (approximations.length && !opts.approximate) {
+        throw new Error(`hipgen: the engine does not draw ${approximations.join('; ')} -- generate(doc, { approximate: true }) renders the scene without them`)
+    }
 
     const hip = [
         `// emitted by shadytracer_amd/js/hipgen.js from '${(doc.title || 'scene').replace(/[\n\r]/g, ' ')}'`,
@@ -384,17 +433,124 @@ function generate(doc, opts) {
         unit,
         program,
         flops: { dist: flops, maskNode: 14 },   // every node evaluated: the bound a culled step stays under
+        approximations,
     }
 }
 
-// The studio environment (the builder's envMode 0, studioIlluminance lux on a horizontal surface): here a uniform
-// sky over the upper hemisphere whose luminance gives that illuminance (E = pi L); `uniformSky` (cd/m^2) and
-// `sky: 'sphere' | 'hemisphere' | 'none'` name it directly. An optional sun: direction from azimuth/elevation
-// (sdf.js:72-76, Z-up), illuminance in lux, angular diameter in degrees, and an aperture (a beam radius about a point).
-function environmentOf(E, doc) {
-    const env = { type: 0, L: 0, col: [1, 1, 1], sunDir: [0, 0, 1], sunCos: 1, sunE: 0, sunRadius: 0, sunPoint: [0, 0, 0] }
-    const kind = E.sky || (E.envMode === 0 || E.envMode === undefined ? 'hemisphere' : 'none')
-    let L = E.uniformSky !== undefined ? E.uniformSky : (E.studioIlluminance !== undefined ? E.studioIlluminance / Math.PI : 0)
+// ---- what the engine's material table, light faces and environments cannot carry (each a reason, or none)
+// A code block with statements (not only comments) is user WGSL (builder/materials.js: the material's code runs per hit).
+const hasStatements = code => typeof code === 'string' && code.split('\n').some(l => l.trim() && !l.trim().startsWith('//'))
+function materialLimits(m) {
+    const out = [], P = (m && m.params) || {}
+    if (hasStatements(m.code)) out.push('material code (user WGSL)')
+    if (m.relief) out.push('a relief')
+    if (m.images && m.images.length) out.push('images')
+    if (m.type && m.type !== 'emitter') out.push(`type '${m.type}'`)
+    if (P.coating > 0) out.push('a thin-film coating')
+    if (P.anisotropy > 0) out.push('anisotropy')
+    if (P.attenuationDistance > 0) out.push('absorption inside (attenuationDistance)')
+    if (P.haze > 0 || P.scatterDistance > 0) out.push('scattering inside')
+    return out
+}
+// lights.js LIGHT_DEFAULTS:62-67: a face emitting cos^1 (Lambertian) over the hemisphere, into air, opaque
+function lightLimits(L) {
+    const out = []
+    if (L.shape && L.shape !== 'rectangle' && L.shape !== 'disc') out.push(`a ${L.shape} light`)
+    if ((L.distribution || 'cosine') !== 'cosine' || (L.n !== undefined && L.n !== 1)) out.push(`a ${L.distribution === 'cosine' ? 'cos^' + L.n : L.distribution} beam`)
+    if (L.halfAngle !== undefined && L.halfAngle < 90) out.push('a beam limited to a cone')
+    if (L.transparent) out.push('a transparent light')
+    if (L.emitN !== undefined && L.emitN !== 1) out.push('emission into a medium (emitN)')
+    if (['spectrum', 'file', 'wavelength'].includes(L.colourBy)) out.push(`colour by ${L.colourBy}`)
+    return out
+}
+
+// ---- a light's colour, linear Rec.709 of luminance 1 (lights.js lightRGB:85-100 over core/colorimetry.js): given as
+// RGB, as a chromaticity xy, or as a CCT and Duv -- the Planckian locus by Krystek's 1985 rational fit in CIE 1960
+// uv, Duv along the locus's unit normal (positive towards green), then xy -> XYZ at Y = 1 -> RGB by the inverse of
+// IEC 61966-2-1's four-decimal matrix (whose Y row is LUMA).
+const RGB709_TO_XYZ = [[0.4124, 0.3576, 0.1805], LUMA, [0.0193, 0.1192, 0.9505]]
+function inverse3(m) {
+    const [[a, b, c], [d, e, f], [g, h, i]] = m
+    const A = e * i - f * h, B = f * g - d * i, C = d * h - e * g, det = a * A + b * B + c * C
+    return [[A / det, (c * h - b * i) / det, (b * f - c * e) / det], [B / det, (a * i - c * g) / det, (c * d - a * f) / det],
+        [C / det, (b * g - a * h) / det, (a * e - b * d) / det]]
+}
+const XYZ_TO_RGB709 = inverse3(RGB709_TO_XYZ)
+function planckUv(T) {
+    const t = Math.min(15000, Math.max(1000, T))
+    return [(0.860117757 + 1.54118254e-4 * t + 1.28641212e-7 * t * t) / (1 + 8.42420234e-4 * t + 7.08145163e-7 * t * t),
+        (0.317398726 + 4.22806245e-5 * t + 4.20481691e-8 * t * t) / (1 - 2.89741816e-5 * t + 1.61456053e-7 * t * t)]
+}
+function cctDuvToXy(T, duv) {
+    const [u, v] = planckUv(T)
+    const h = Math.max(1e-3, T * 1e-5), a = planckUv(T - h), b = planckUv(T + h)
+    const du = b[0] - a[0], dv = b[1] - a[1], l = Math.hypot(du, dv) || 1
+    const uu = u + duv * dv / l, vv = v - duv * du / l
+    const d = 2 * uu - 8 * vv + 4
+    return [3 * uu / d, 2 * vv / d]
+}
+function rgbFromXy(x, y) {
+    const XYZ = y > 0 ? [x / y, 1, (1 - x - y) / y] : [0, 0, 0]
+    return XYZ_TO_RGB709.map(r => r[0] * XYZ[0] + r[1] * XYZ[1] + r[2] * XYZ[2])
+}
+function lightRGB(L) {
+    let rgb
+    if (L.colourBy === 'cct') rgb = rgbFromXy(...cctDuvToXy(L.cct !== undefined ? L.cct : 4000, L.duv || 0))
+    else if (L.colourBy === 'xy') { const c = L.xy || { x: 0.3127, y: 0.329 }; rgb = rgbFromXy(c.x, c.y) }
+    else rgb = colourOf(L.colour || L.color, [1, 1, 1])
+    const Y = LUMA[0] * rgb[0] + LUMA[1] * rgb[1] + LUMA[2] * rgb[2]
+    return Y > 0 ? rgb.map(v => v / Y) : [1, 1, 1]
+}
+
+// ---- the document's own camera (its view: lib/camera.js syncOrbitPose:137 and frame:106): an orbit about target,
+// azimuth about +z from +x, elevation the angle down from +z, the vertical field of view fov; metres
+function cameraOf(doc, width, height, cannot) {
+    const C = (doc.view && doc.view.camera) || {}
+    const unit = UNIT_METRES[doc.unit || 'm']
+    const r = Math.PI / 180
+    if (C.mode && C.mode !== 'orbit') cannot('camera', `a ${C.mode} camera`)
+    if (C.camtype) cannot('camera', 'an orthographic camera')
+    if (C.aperture > 0) cannot('camera', 'depth of field (aperture)')
+    if (C.roll) cannot('camera', 'roll')
+    const t = (C.target || [0, 0, 0]).map(v => v * unit)
+    const az = (C.azimuth !== undefined ? C.azimuth : 40) * r, el = (90 - (C.elevation !== undefined ? C.elevation : 70)) * r
+    const dist = (C.distance !== undefined ? C.distance : 4) * unit
+    const pos = [t[0] + dist * Math.cos(el) * Math.cos(az), t[1] + dist * Math.cos(el) * Math.sin(az), t[2] + dist * Math.sin(el)]
+    const norm = v => { const l = Math.hypot(v[0], v[1], v[2]) || 1; return v.map(x => x / l) }
+    const cross = (a, b) => [a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]]
+    const z = norm([pos[0] - t[0], pos[1] - t[1], pos[2] - t[2]])
+    let x = cross(C.up || [0, 0, 1], z)
+    if (Math.hypot(x[0], x[1], x[2]) < 1e-6) x = cross(Math.abs(z[2]) < 0.9 ? [0, 0, 1] : [1, 0, 0], z)
+    x = norm(x)
+    const fy = Math.tan(0.5 * (C.fov || 60) * r)
+    return { pos, right: x, up: cross(z, x), fwd: z, fx: fy * width / height, fy, type: 0 }
+}
+
+// The environment. A builder document's (uniforms.schema.js envMode:429): 1 the studio, L(d) = L0 (1 + top d.z) with
+// L0 = E / (pi (1 + 2 top / 3)) so that a level surface facing up receives studioIlluminance lux (photometry.js
+// studioLuminance:154, environment.js studioLight:165), its colour a tint of luminance 1, and no sun; 0 the sun & sky
+// and 2 the scene's own code, neither of which the engine draws. The configurations here name theirs directly:
+// `sky: 'sphere' | 'hemisphere' | 'none'` of `uniformSky` cd/m^2, and an optional sun -- direction from
+// azimuth/elevation (sdf.js:72-76, Z-up), illuminance in lux, angular diameter in degrees, and an aperture (a beam
+// radius about a point).
+function environmentOf(E, doc, cannot) {
+    cannot = cannot || (() => {})
+    const env = { type: 0, L: 0, col: [1, 1, 1], sunDir: [0, 0, 1], sunCos: 1, sunE: 0, sunRadius: 0, sunPoint: [0, 0, 0], top: 0 }
+    if (E.sky === undefined) {
+        if (hasStatements(doc.sky)) cannot('environment', 'the scene\'s own sky code')
+        const mode = E.envMode || 0
+        if (mode !== 1) { cannot('environment', mode === 2 ? 'the scene\'s own code (envMode 2)' : 'the sun & sky (envMode 0)'); return env }
+        const top = Math.min(1, Math.max(0, E.studioTop !== undefined ? +E.studioTop : 0.5))
+        const c = colourOf(E.studioColor, [1, 1, 1]).map(v => Math.max(v, 0))
+        const y = LUMA[0] * c[0] + LUMA[1] * c[1] + LUMA[2] * c[2]
+        env.L = Math.max(0, E.studioIlluminance !== undefined ? +E.studioIlluminance : 1000) / (Math.PI * (1 + 2 * top / 3))
+        env.type = env.L > 0 ? 3 : 0
+        env.col = c.map(v => v / Math.max(y, 1e-6))
+        env.top = top
+        return env
+    }
+    const kind = E.sky
+    let L = E.uniformSky !== undefined ? E.uniformSky : 0
     if (kind === 'none') L = 0
     env.type = L > 0 ? (kind === 'sphere' ? 1 : 2) : 0
     env.L = L
@@ -418,4 +574,4 @@ function environmentOf(E, doc) {
     return env
 }
 
-module.exports = { generate, environmentOf, rotationOf, UNIT_METRES, PRIM_FLOPS }
+module.exports = { generate, environmentOf, cameraOf, lightRGB, cctDuvToXy, rotationOf, UNIT_METRES, PRIM_FLOPS }

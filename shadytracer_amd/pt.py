@@ -37,7 +37,8 @@ class PtLightFace(ctypes.Structure):
 
 class PtEnv(ctypes.Structure):
     _fields_ = [('type', ctypes.c_uint), ('L', ctypes.c_float), ('col', F3), ('sunDir', F3), ('sunCos', ctypes.c_float),
-                ('sunE', ctypes.c_float), ('sunRadius', ctypes.c_float), ('sunPoint', F3), ('pad', F2)]
+                ('sunE', ctypes.c_float), ('sunRadius', ctypes.c_float), ('sunPoint', F3), ('top', ctypes.c_float),
+                ('pad', ctypes.c_float)]
 
 
 class PtHaze(ctypes.Structure):
@@ -194,6 +195,7 @@ def make_env(e):
     v.type, v.L = int(e['type']), float(e['L'])
     _set3(v.col, e['col']); _set3(v.sunDir, e['sunDir']); _set3(v.sunPoint, e['sunPoint'])
     v.sunCos, v.sunE, v.sunRadius = e['sunCos'], e['sunE'], e['sunRadius']
+    v.top = float(e.get('top', 0.0))
     return v
 
 

@@ -7,7 +7,7 @@ import subprocess
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 JS = os.path.join(PKG, 'js')
-STANDARD = ['plane-c1', 'room-c2', 'caustic-c3', 'furnace-plane', 'furnace-haze', 'sppm-slab']
+STANDARD = ['plane-c1', 'room-c2', 'caustic-c3', 'furnace-plane', 'furnace-haze', 'sppm-slab', 'studio-floor', 'lens-prism']
 NODE = os.environ.get('NODE', 'node')
 
 _memo = {}
@@ -25,8 +25,12 @@ def emit(name, **opts):
     return _memo[key]
 
 
-def emit_doc(path, camera=None):
-    args = [NODE, os.path.join(JS, 'emit.js'), '--doc', path] + ([json.dumps(camera)] if camera else [])
+def emit_doc(path, camera=None, **opts):
+    """A builder document file through the emitter. camera: the engine's pinhole (default: the document's own view);
+    opts: width, height, approximate (draw without what the engine does not, listed in scene['approximations'])."""
+    if camera:
+        opts['camera'] = camera
+    args = [NODE, os.path.join(JS, 'emit.js'), '--doc', path] + ([json.dumps(opts)] if opts else [])
     r = subprocess.run(args, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
     if r.returncode != 0:
         raise RuntimeError('emitter failed for %s: %s' % (path, r.stderr))

@@ -75,6 +75,51 @@ def test_c2_per_pixel_means_agree_with_the_oracle(engine):
     assert abs(gL.mean() - oL.mean()) < 4 * se + 1e-3 * oL.mean()
 
 
+def test_studio_floor_reads_rho_E_over_pi(engine):
+    # the builder's studio (envMode 1) puts its illuminance on a level surface whatever its top light: a Lambertian
+    # floor of albedo 0.6 under 500 lx reads 0.6 x 500 / pi; a document in millimetres, seen from its orbit view
+    say('case: studio-floor 64x48 256 spp, floor = 0.6 x 500 / pi')
+    made, u = load(engine, 'studio-floor')
+    _, state, _, _ = engine.pick_rays()
+    engine.trace(256)
+    acc, mom = engine.read_accum()
+    L = pt.luminance(acc).ravel()
+    floor = state == 0
+    assert floor.sum() > 1500
+    assert abs(L[floor].mean() / (0.6 * 500 / math.pi) - 1) < 2e-3, L[floor].mean()
+
+
+def test_lens_and_prism_primary_rays_match_the_oracle(engine):
+    say('case: lens-prism 96x64, sample 0, pick vs oracle')
+    made, u = load(engine, 'lens-prism')
+    gpos, gstate, gmat, gprim = engine.pick_rays()
+    opos, ostate, omat, oprim = oracle.pick(made['scene'], u)
+    same = (gstate == ostate) & ((gstate != 0) | (gprim == oprim))
+    say('agreement', same.mean(), 'lens', int((oprim[ostate == 0] == 1).sum()), 'prism', int((oprim[ostate == 0] == 2).sum()))
+    assert same.mean() >= 0.9995
+    assert ((gprim == 1) & (gstate == 0)).sum() > 300 and ((gprim == 2) & (gstate == 0)).sum() > 100
+    hit = (gstate == 0) & same
+    d = np.linalg.norm(gpos[hit, :3] - opos[hit, :3], axis=1)
+    assert np.quantile(d, 0.999) < 1e-4      # metres: the scene is 0.2 m across
+
+
+def test_lens_and_prism_per_pixel_means_agree_with_the_oracle(engine):
+    say('case: lens-prism 48x32, GPU 256 spp vs oracle 64 spp')
+    made, u = load(engine, 'lens-prism', width=48, height=32)
+    engine.trace(256)
+    ga, gm = engine.read_accum()
+    oa, om, _ = oracle.render(made['scene'], u, 64, first=100000)
+    gL, oL = pt.luminance(ga), pt.luminance(oa)
+    s = np.sqrt(pt.standard_error(gm) ** 2 + pt.standard_error(om) ** 2)
+    ok = s > 1e-9
+    z = (gL - oL)[ok] / s[ok]
+    say('z std', z.std(), 'frac |z|<3', np.mean(np.abs(z) < 3), 'means', gL.mean(), oL.mean())
+    assert 0.75 < z.std() < 1.3
+    assert np.mean(np.abs(z) < 3) > 0.97
+    se = math.sqrt(np.nansum(s ** 2)) / s.size
+    assert abs(gL.mean() - oL.mean()) < 4 * se + 1e-3 * oL.mean()
+
+
 # the Lambertian's 1e-3: the march's 400-step cap (the reference's maxiter) drops a path that leaves the plane within
 # about 0.7 degrees of grazing before it reaches the sky sphere, ~0.03 % of the light here
 def test_every_pixel_closes_every_sample(engine):

@@ -66,7 +66,7 @@ def test_raygen_tiles_one_per_pixel():
 # ---------------------------------------------------------------------------------------------------- geometry
 def _single(shape_doc_params, block, rot=(0, 0, 0), pos=(0, 0, 0), unit='m'):
     from shadytracer_amd import scenes as S  # noqa: F401
-    doc = {'format': 'raytracer-scene', 'version': 31, 'unit': unit, 'materials': [], 'blocks': [
+    doc = {'format': 'raytracer-scene', 'version': 31, 'unit': unit, 'materials': [], 'environment': {'sky': 'none'}, 'blocks': [
         {'id': 'b', 'builtin': block + '@1', 'params': {}}],
         'root': {'id': 'root', 'kind': 'group', 'children': [
             {'id': 'n', 'kind': 'object', 'type': 'code', 'block': 'b', 'params': shape_doc_params,
@@ -130,10 +130,199 @@ def test_transform_rotation_and_units(tmp_path):
     np.testing.assert_allclose(d, exact, atol=2e-6)
 
 
+def _node_sphere(sc):
+    """The emitted bounding sphere of a one-node scene: centre and radius."""
+    node = [o for o in sc['program'] if o['op'] == 'node'][0]
+    U = sc['params']
+    return np.array(U[node['bs']:node['bs'] + 3]), U[node['bs'] + 3]
+
+
+def _polygon_sdf(q, n, R):
+    """Brute force: the signed distance to a regular n-gon of corner radius R with a flat side facing -y -- outside the
+    nearest of its side segments, inside the largest of its side lines' projections less the apothem."""
+    ang = -np.pi / 2 + np.pi / n + 2 * np.pi * np.arange(n) / n     # the corners
+    V = R * np.stack([np.cos(ang), np.sin(ang)], 1)
+    W = np.roll(V, -1, axis=0)
+    best = np.full(len(q), np.inf)
+    for a, b in zip(V, W):
+        e = b - a
+        t = np.clip(((q - a) @ e) / (e @ e), 0, 1)
+        best = np.minimum(best, np.linalg.norm(q - a - t[:, None] * e, axis=1))
+    mid = -np.pi / 2 + 2 * np.pi * np.arange(n) / n                  # the sides' outward normals
+    proj = (q @ np.stack([np.cos(mid), np.sin(mid)], 0)).max(1) - R * np.cos(np.pi / n)
+    return np.where(proj > 0, best, proj)
+
+
+def _extrude(g, z, h):
+    w = np.stack([g, np.abs(z) - h], 1)
+    return np.linalg.norm(np.maximum(w, 0), axis=1) + np.minimum(w.max(1), 0)
+
+
+def _normals_are_gradients(sc, pts):
+    d, nrm, _ = oracle.scene_at(sc, pts)
+    h = 1e-6
+    grad = np.stack([(oracle.scene_at(sc, pts + h * e)[0] - oracle.scene_at(sc, pts - h * e)[0]) / (2 * h) for e in np.eye(3)], 1)
+    ok = np.linalg.norm(grad, axis=1) > 0.999
+    return np.mean((grad[ok] * nrm[ok]).sum(1) > 0.999)
+
+
+@pytest.mark.parametrize('sides,n', [(3, 3), (4, 4), (5, 5), (6.4, 6), (12, 12), (2, 3), (70, 64), (4.5, 4), (5.5, 6)])
+def test_prism_is_the_extruded_regular_polygon(sides, n, tmp_path):
+    # shapes.js Nprism: sides rounded half to even and held to 3..64, a flat side facing -y; against the brute-force
+    # polygon (nearest side segment outside, largest side projection inside) extruded exactly
+    R, H = 0.45, 0.7
+    sc = _emit_doc(_single({'sides': sides, 'radius': R, 'height': H}, 'prism'), tmp_path)
+    pts = RNG.uniform(-0.8, 0.8, (1500, 3))
+    d, nrm, _ = oracle.scene_at(sc, pts)
+    Rf, Hf = float(np.float32(R)), float(np.float32(H))
+    exact = _extrude(_polygon_sdf(pts[:, :2], n, Rf), pts[:, 2], 0.5 * Hf)
+    np.testing.assert_allclose(d, exact, atol=1e-9)
+    # the flat side down: the point under the centre a little below that side is that little away
+    a = Rf * math.cos(math.pi / n)
+    assert abs(oracle.scene_at(sc, [[0, -a - 0.1, 0]])[0][0] - 0.1) < 1e-9
+    assert _normals_are_gradients(sc, pts) > 0.97
+    c, r = _node_sphere(sc)
+    assert (np.linalg.norm(pts[d <= 0] - c, axis=1) <= r * (1 + 1e-6)).all()
+
+
+def test_prism_of_four_sides_is_a_box(tmp_path):
+    sc = _emit_doc(_single({'sides': 4, 'radius': 0.5, 'height': 0.8}, 'prism'), tmp_path)
+    pts = RNG.uniform(-1, 1, (800, 3))
+    d, _, _ = oracle.scene_at(sc, pts)
+    h = 0.5 / math.sqrt(2)
+    np.testing.assert_allclose(d, _box_exact(pts, np.array([h, h, float(np.float32(0.4))])), atol=1e-7)
+
+
+def _trace(sc, o, dr, inside=False, tmax=10.0):
+    """Sphere tracing on the oracle's distance: the first point where the surface is within 1e-12 along o + t dr."""
+    t = 0.0
+    for _ in range(2000):
+        p = o + t * dr
+        d = oracle.scene_at(sc, p[None])[0][0]
+        d = -d if inside else d
+        if d < 1e-12:
+            return p, oracle.scene_at(sc, p[None])[1][0]
+        t += max(d, 1e-13)
+        if t > tmax:
+            return None, None
+    raise AssertionError('no convergence')
+
+
+def _refract(dr, n, eta):
+    """Snell's law for unit dr against the normal n facing the incoming side; eta = n_in / n_out."""
+    c = -dr @ n
+    k = 1 - eta * eta * (1 - c * c)
+    assert k > 0
+    return eta * dr + (eta * c - math.sqrt(k)) * n
+
+
+@pytest.mark.parametrize('r1,r2', [(2.0, 2.0), (0.8, 0.0), (1.5, -3.0), (-2.0, -2.0), (0.0, 1.2)])
+def test_lens_focuses_where_the_thick_lens_formula_says(r1, r2, tmp_path):
+    # sdf.js Nlens: a paraxial ray down the axis, refracted at the oracle's surfaces with its normals (n = 1.5), crosses
+    # the axis at the thick lens's back focus: 1/f = (n-1) (1/r1 + 1/r2 - (n-1) t / (n r1 r2)), the focus f (1 - (n-1)
+    # t / (n r1)) beyond the -z vertex (radii signed as the builtin has them; a flat face's 1/r is 0)
+    D, T, nn = 1.0, 0.2, 1.5
+    sc = _emit_doc(_single({'diameter': D, 'thickness': T, 'r1': r1, 'r2': r2}, 'lens'), tmp_path)
+    f32 = lambda x: float(np.float32(x))   # noqa: E731  the parameters as U holds them
+    t, c1, c2 = 2 * f32(0.5 * T), (1 / f32(r1) if r1 else 0.0), (1 / f32(r2) if r2 else 0.0)
+    P = (nn - 1) * (c1 + c2 - (nn - 1) * t * c1 * c2 / nn)
+    f = 1 / P
+    bfd = f * (1 - (nn - 1) * t * c1 / nn)
+    y0 = 1e-3
+    o, dr = np.array([y0, 0, 2.0]), np.array([0, 0, -1.0])
+    p, n = _trace(sc, o, dr)
+    dr = _refract(dr, n, 1 / nn)
+    p, n = _trace(sc, p + 1e-9 * dr, dr, inside=True)
+    dr = _refract(dr, -n, nn)
+    assert p[2] < 0
+    s = -p[0] / dr[0]                 # where the ray (or its backward line) meets the axis
+    z = p[2] + s * dr[2]
+    vz = f32(0.5 * T)
+    assert abs((-vz - z) / bfd - 1) < 2e-4, (z, -vz - bfd)
+
+
+@pytest.mark.parametrize('r1,r2', [(2.0, 2.0), (1.5, -3.0), (-0.7, -0.9), (0.3, 0.0)])
+def test_lens_solid_and_surfaces(r1, r2, tmp_path):
+    D, T = 1.0, 0.3
+    sc = _emit_doc(_single({'diameter': D, 'thickness': T, 'r1': r1, 'r2': r2}, 'lens'), tmp_path)
+    f32 = lambda x: float(np.float32(x))   # noqa: E731
+    rim, vz = f32(0.5 * D), f32(0.5 * T)
+    pts = RNG.uniform(-0.8, 0.8, (3000, 3))
+    d, nrm, _ = oracle.scene_at(sc, pts)
+    rho = np.hypot(pts[:, 0], pts[:, 1])
+
+    def face_inside(s, R):
+        z = s * pts[:, 2]
+        if R == 0:
+            return z <= vz, np.abs(z - vz)
+        c = np.stack([0 * z, 0 * z, s * (vz - R) + 0 * z], 1)
+        l = np.linalg.norm(pts - c, axis=1)
+        if R > 0:
+            return l <= R, np.abs(l - R)
+        sag = abs(R) - math.sqrt(max(R * R - rim * rim, 0))
+        return (l >= -R) & (z <= vz + sag), np.minimum(np.abs(l + R), np.abs(z - vz - sag))
+    i1, m1 = face_inside(1, f32(r1))
+    i2, m2 = face_inside(-1, f32(r2))
+    inside = (rho <= rim) & i1 & i2
+    clear = np.minimum(np.minimum(m1, m2), np.abs(rho - rim)) > 1e-6
+    # the solid: inside exactly where the value is negative
+    assert ((d < 0) == inside)[clear].all()
+    # on the axis beyond the +z vertex the value is the distance to it, whatever the face's shape
+    for z in (vz + 0.01, vz + 0.3):
+        assert abs(oracle.scene_at(sc, [[0, 0, z]])[0][0] - (z - vz)) < 1e-12
+    # a point of the +z face's surface inside the rim is on the surface
+    for r in (0.0, 0.2, 0.4):
+        R = f32(r1)
+        if abs(r) > abs(R) > 0:
+            continue
+        zf = vz if R == 0 else vz - R + math.copysign(math.sqrt(R * R - r * r), R)
+        assert abs(oracle.scene_at(sc, [[r, 0, zf]])[0][0]) < 1e-12
+    # outside, the value never exceeds the distance to the surface: the surface points found by bisection along
+    # vertical lines through the solid (a subset of the surface, so their nearest is at least the true distance)
+    rr, ph = np.meshgrid(np.linspace(0, rim, 120), np.linspace(0, 2 * np.pi, 90, endpoint=False))
+    xy = np.stack([rr.ravel() * np.cos(ph.ravel()), rr.ravel() * np.sin(ph.ravel())], 1)
+    xy = xy[oracle.scene_at(sc, np.c_[xy, 0 * xy[:, 0]])[0] < 0]
+    surf = []
+    for sgn in (1, -1):
+        lo, hi = np.zeros(len(xy)), np.full(len(xy), sgn * 0.8)
+        for _ in range(45):
+            mid = 0.5 * (lo + hi)
+            neg = oracle.scene_at(sc, np.c_[xy, mid])[0] < 0
+            lo, hi = np.where(neg, mid, lo), np.where(neg, hi, mid)
+        surf.append(np.c_[xy, lo])
+    surf = np.concatenate(surf)
+    assert len(surf) > 1000
+    far = d > 0.02
+    near = np.array([np.linalg.norm(surf - q, axis=1).min() for q in pts[far]])
+    assert (d[far] <= near + 1e-9).all()
+    assert _normals_are_gradients(sc, pts) > 0.97
+    c, r = _node_sphere(sc)
+    assert (np.linalg.norm(pts[d <= 0] - c, axis=1) <= r * (1 + 1e-6)).all()
+
+
+def test_studio_floor_reads_rho_E_over_pi():
+    # environment.js studioLight: L0 (1 + top d.z) with L0 = E / (pi (1 + 2 top / 3)) lights a level surface with E
+    # whatever top is, so a Lambertian floor of albedo 0.6 under 500 lx reads 0.6 x 500 / pi; the rays that leave
+    # over the horizon read L0 (1 + top d.z), d.z within the top row's few degrees of 0
+    made = scenes.emit('studio-floor')
+    u = pt.uniforms_for(made)
+    acc, mom, _ = oracle.render(made['scene'], u, 64)
+    _, state, _, _ = oracle.pick(made['scene'], u)
+    L = pt.luminance(acc).ravel()
+    floor = state == 0
+    se = math.sqrt((pt.standard_error(mom).ravel()[floor] ** 2).sum()) / floor.sum()
+    assert abs(L[floor].mean() - 0.6 * 500 / math.pi) < 4 * se + 1e-9, (L[floor].mean(), se)
+    # the top three rows look 1 to 5 degrees over the horizon (the view is pitched 20 degrees down, 50 tall)
+    L0 = 500 / math.pi / (1 + 1 / 3)
+    sky = L.reshape(u.height, u.width)[-3:] / L0
+    assert (sky > 1 + 0.5 * math.sin(math.radians(0.9))).all() and (sky < 1 + 0.5 * math.sin(math.radians(5.1))).all()
+
+
 def test_smooth_union_bounds(tmp_path):
     # a smooth union lies between min(a, b) - k/4 and min(a, b), and equals the min where |a - b| >= k
     k = 0.2
-    doc = {'format': 'raytracer-scene', 'version': 31, 'unit': 'm', 'materials': [], 'blocks': [{'id': 'b', 'builtin': 'sphere@1'}],
+    doc = {'format': 'raytracer-scene', 'version': 31, 'unit': 'm', 'materials': [], 'environment': {'sky': 'none'},
+           'blocks': [{'id': 'b', 'builtin': 'sphere@1'}],
            'root': {'id': 'root', 'kind': 'group', 'children': [{'id': 'g', 'kind': 'group', 'blend': k, 'children': [
                {'id': 'a', 'kind': 'object', 'type': 'code', 'block': 'b', 'params': {'radius': 0.5}, 'transform': {'pos': {'x': -0.4, 'y': 0, 'z': 0}}},
                {'id': 'c', 'kind': 'object', 'type': 'code', 'block': 'b', 'params': {'radius': 0.5}, 'transform': {'pos': {'x': 0.4, 'y': 0, 'z': 0}}}]}]}}
