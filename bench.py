@@ -262,6 +262,7 @@ def main():
     ap.add_argument('--no-autotune', action='store_true', help='keep the library\'s default launch choices')
     ap.add_argument('--tuning', default=None, help='blocksPerCu,refillMarch,refillOcclude: set, not tuned')
     ap.add_argument('--stragglers', type=int, default=None, help='rounds beyond maxbounce + 1 (default: the library\'s)')
+    ap.add_argument('--relax', type=float, default=0.0, help='over-relaxed sphere tracing factor (0: the plain march)')
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py: this build runs on one GPU; --gpus %d is not supported (samples would shard across GPUs with an '
@@ -287,7 +288,7 @@ def main():
     log('bench: room-c2 %dx%d, warmup %d, steps %d' % (a.width, a.height, a.warmup, a.steps))
     e = pt.Engine(straggler_rounds=a.stragglers)
     e.set_scene(made['scene'])
-    u = pt.uniforms_for(made)
+    u = pt.uniforms_for(made, relax=a.relax)
     e.set_uniforms(u)
     # the persistent kernels' launch choices for this scene and image, by measurement (before the warm-up; untimed)
     if a.tuning:
@@ -342,7 +343,7 @@ def main():
         'config': {'workload': 'room-c2: 5x4x3 m room, 32 SDF primitives, 8 smooth unions, GGX, 0.5 m panel 5000 cd/m^2, '
                                '%dx%d, depth 8, NEE on, 1 sample per step' % (a.width, a.height),
                    'width': a.width, 'height': a.height, 'maxbounce': u.maxbounce, 'accumulation': 'f64',
-                   'straggler_rounds': e.straggler_rounds,
+                   'straggler_rounds': e.straggler_rounds, 'relax': u.relax,
                    'tuning': None if not tuning else {k: tuning[k] for k in ('blocksPerCu', 'refillMarch', 'refillOcclude',
                                                                             'msPerSample', 'msBefore', 'candidates')}},
         'shadow_segments_per_s': st['shadow_segments'] / dt, 'march_steps_per_s': st['steps'] / dt,

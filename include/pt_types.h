@@ -228,6 +228,8 @@ typedef struct PtUniforms {
     unsigned int guide;         // 1: MLP path guiding mixed with BSDF sampling
     float guideFrac;            // the largest share of guided directions (the network picks its own below it)
     float sppmAlpha;            // the radius schedule's alpha: r_k^2 = r_1^2 prod_{i<k} (i + alpha) / (i + 1)
+    float relax;                // over-relaxed sphere tracing: steps of relax * stepscale * |d| while the free balls
+                                // overlap (Keinert et al. 2014); 0 or 1 the plain march, at most 1.95
 } PtUniforms;
 
 

@@ -50,12 +50,22 @@ __device__ __forceinline__ void statAdd(const PtDev* __restrict__ D, int which, 
     if (laneId() == 0 && v) atomicAdd(&D->stats[which], v);
 }
 
-__device__ __forceinline__ float4 ld4(const float* p, unsigned int i) { return reinterpret_cast<const float4*>(p)[i]; }
-__device__ __forceinline__ uint4 ld4u(const unsigned int* p, unsigned int i) { return reinterpret_cast<const uint4*>(p)[i]; }
-__device__ __forceinline__ void st4(float* p, unsigned int i, float a, float b, float c, float d) {
-    reinterpret_cast<float4*>(p)[i] = make_float4(a, b, c, d);
+// The engine's buffers are device memory, read and written with global instructions, not flat ones: a flat access
+// counts against the same counter as the scalar loads of the scene's parameters, so one in flight would hold up
+// every wait of the distance function on those (a ray read ahead would stall the march it was read ahead for).
+#define PT_GLOBAL __attribute__((address_space(1)))
+__device__ __forceinline__ float4 ld4(const float* p, unsigned int i) { return ((const PT_GLOBAL float4*)p)[i]; }
+__device__ __forceinline__ uint4 ld4u(const unsigned int* p, unsigned int i) { return ((const PT_GLOBAL uint4*)p)[i]; }
+__device__ __forceinline__ float ldf(const float* p, unsigned int i) { return ((const PT_GLOBAL float*)p)[i]; }
+__device__ __forceinline__ unsigned int ldu(const unsigned int* p, unsigned int i) { return ((const PT_GLOBAL unsigned int*)p)[i]; }
+__device__ __forceinline__ unsigned long long ldu64(const unsigned long long* p, unsigned int i) {
+    return ((const PT_GLOBAL unsigned long long*)p)[i];
 }
-__device__ __forceinline__ void st4u(unsigned int* p, unsigned int i, uint4 v) { reinterpret_cast<uint4*>(p)[i] = v; }
+__device__ __forceinline__ void st4(float* p, unsigned int i, float a, float b, float c, float d) {
+    ((PT_GLOBAL float4*)p)[i] = make_float4(a, b, c, d);
+}
+__device__ __forceinline__ void st4u(unsigned int* p, unsigned int i, uint4 v) { ((PT_GLOBAL uint4*)p)[i] = v; }
+__device__ __forceinline__ void stu(unsigned int* p, unsigned int i, unsigned int v) { ((PT_GLOBAL unsigned int*)p)[i] = v; }
 
 // A wave's share of a queue, for the persistent kernels: batches of PT_GRAB consecutive entries taken by one atomic
 // from one of PT_PARTS partitions of the queue; a wave starts on its own partition and moves to the next when that

@@ -148,6 +148,13 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_bin_scatter(const PtDe
 // collision when the leg reaches its drawn free flight. Persistent: each wave refills its idle lanes from the
 // (binned) queue in batches (fetchNext). Every ray ends its march in exactly one event, so shade reads the same
 // queue: no event queue, no append.
+//
+// OVER-RELAXATION (Keinert, Schaefer, Korndoerfer, Ganse, Stamminger, "Enhanced Sphere Tracing", 2014), when the
+// uniforms' relax w > 1: a step of w r (r = stepscale |d|, the plain step) goes past the free ball it stands on. It
+// is kept if the free ball at the new point overlaps the old one (r + r_prev > the step): the two cover the segment,
+// so no surface lies on it. Else the march goes back to where the plain step would have put it (inside the old
+// ball) and marches plainly for the rest of the ray. A point reached by a gap is never tested for an event, so every
+// event is one the plain march could meet; a ray with a haze collision pending marches plainly.
 #define PT_MARCH_STEPS 4
 #define PT_STEP_FLOPS 30u   // a step's own FP32 work beside the SDF: tolerance, escape test, the move
 extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_march(const PtDev* __restrict__ D, unsigned int qsel) {
@@ -164,11 +171,13 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_march(const PtDev* __r
     const float stepscale = D->u.stepscale;
     const unsigned int maxiter = D->u.maxiter;
     const unsigned int refill = D->refillMarch;
+    const float relax = D->u.relax > 1.0f ? D->u.relax : 1.0f;
     ParamPtr U = (const PT_CONST float*)D->params;
 
     int slot = -1;
     V3 p = v3s(0.0f), dir = v3s(0.0f);
     float ltot = 0.0f, n = 1.0f, collide = 3.0e38f;
+    float om = 1.0f, rPrev = 0.0f, sPrev = 0.0f;    // the relaxation, the last plain step, the last step taken
     unsigned int steps = 0u, z = 0u;
     unsigned long long mask = ~0ull;
     WaveFetch f = fetchInit();
@@ -177,17 +186,17 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_march(const PtDev* __r
         unsigned long long idle = __ballot(slot < 0);
         if (!f.done && (unsigned int)__popcll(idle) >= refill) {
             unsigned int k = fetchNext(f, C + PT_C_HEADS_MARCH, count, slot < 0);
-            if (slot < 0) {
-                if (k != 0xFFFFFFFFu) {
-                    slot = (int)q[k];
-                    float4 P = ld4(D->pos, slot), Dd = ld4(D->dir, slot);
-                    uint4 M = ld4u(D->meta, slot);
-                    p = v3(P.x, P.y, P.z); ltot = P.w;
-                    dir = v3(Dd.x, Dd.y, Dd.z); n = Dd.w;
-                    steps = M.y; z = M.z;
-                    collide = D->aux[4u * slot + 1u];
-                    mask = D->maskBuf[slot];
-                }
+            if (slot < 0 && k != 0xFFFFFFFFu) {
+                slot = (int)ldu(q, k);
+                float4 P = ld4(D->pos, slot), Dd = ld4(D->dir, slot);
+                uint4 M = ld4u(D->meta, slot);
+                p = v3(P.x, P.y, P.z); ltot = P.w;
+                dir = v3(Dd.x, Dd.y, Dd.z); n = Dd.w;
+                steps = M.y; z = M.z;
+                collide = ldf(D->aux, 4u * slot + 1u);
+                mask = ldu64(D->maskBuf, slot);
+                om = collide >= 3.0e38f ? relax : 1.0f;
+                rPrev = sPrev = 0.0f;
             }
         }
         if (__ballot(slot >= 0) == 0ull) {
@@ -212,22 +221,35 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_march(const PtDev* __r
                 float d = sceneDistM(U, p, mask, w);
                 nflops += w;
                 float ea = fabsf(d);
+                float r = ea * stepscale;
+                if (om > 1.0f && r + rPrev <= sPrev) {
+                    // a gap between the free balls: back to the plain step's point, plainly from here
+                    float back = sPrev - rPrev;
+                    p = p - dir * back;
+                    ltot -= back * n;
+                    om = 1.0f;
+                    steps += 1u;
+                    nsteps += 1ull;
+                    continue;
+                }
                 float eps = marchTol(setting, p);
                 if (ltot > 1000.0f || (steps & 0xFFFFu) > maxiter) state = PT_MAXITER;
                 if (dot(p, dir) > 0.0f && dot(p, p) > R2) state = PT_ESCAPED;
                 if (ea < eps || state != PT_ACTIVE) { event = true; break; }
                 steps += 1u;
                 nsteps += 1ull;
-                float t = ea * stepscale;
+                float t = r * om;
                 p = p + dir * t;
                 ltot += t * n;
+                rPrev = r;
+                sPrev = t;
             }
         }
         if (event) {
             if (state == PT_MAXITER) nmax += 1ull;
             st4(D->pos, slot, p.x, p.y, p.z, ltot);
-            D->meta[4u * slot + 1u] = steps;
-            D->meta[4u * slot + 2u] = mzPack(mzBounces(z), state, mzFlags(z));
+            stu(D->meta, 4u * slot + 1u, steps);
+            stu(D->meta, 4u * slot + 2u, mzPack(mzBounces(z), state, mzFlags(z)));
             slot = -1;
         }
     }
@@ -769,10 +791,12 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_occlude(const PtDev* _
     const float stepscale = D->u.stepscale;
     const unsigned int maxiter = D->u.maxiter;
     const unsigned int refill = D->refillOcclude;
+    const float relax = D->u.relax > 1.0f ? D->u.relax : 1.0f;
     ParamPtr U = (const PT_CONST float*)D->params;
     int slot = -1;
     V3 p = v3s(0.0f), dir = v3s(0.0f), o = v3s(0.0f);
     float tmax = 0.0f, trav = 0.0f;
+    float om = 1.0f, rPrev = 0.0f, sPrev = 0.0f;    // over-relaxation, as the march's
     unsigned long long mask = ~0ull;
     unsigned int steps = 0u;
     WaveFetch f = fetchInit();
@@ -781,16 +805,16 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_occlude(const PtDev* _
         unsigned long long idle = __ballot(slot < 0);
         if (!f.done && (unsigned int)__popcll(idle) >= refill) {
             unsigned int k = fetchNext(f, C + PT_C_HEADS_SHADOW, count, slot < 0);
-            if (slot < 0) {
-                if (k != 0xFFFFFFFFu) {
-                    slot = (int)D->qSorted[k];
-                    float4 Oo = ld4(D->shOrig, slot), Dd = ld4(D->shDir, slot);
-                    o = v3(Oo.x, Oo.y, Oo.z); p = o; tmax = Oo.w;
-                    dir = v3(Dd.x, Dd.y, Dd.z);
-                    trav = 0.0f; steps = 0u;
-                    mask = D->maskBuf[slot];
-                    nsegs += 1ull;
-                }
+            if (slot < 0 && k != 0xFFFFFFFFu) {
+                slot = (int)ldu(D->qSorted, k);
+                float4 Oo = ld4(D->shOrig, slot), Dd = ld4(D->shDir, slot);
+                o = v3(Oo.x, Oo.y, Oo.z); p = o; tmax = Oo.w;
+                dir = v3(Dd.x, Dd.y, Dd.z);
+                trav = 0.0f; steps = 0u;
+                mask = ldu64(D->maskBuf, slot);
+                nsegs += 1ull;
+                om = relax;
+                rPrev = sPrev = 0.0f;
             }
         }
         if (__ballot(slot >= 0) == 0ull) {
@@ -805,10 +829,19 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_occlude(const PtDev* _
                 float d = sceneDistM(U, p, mask, w);
                 nflops += w;
                 float ea = fabsf(d);
+                float r = ea * stepscale;
+                if (om > 1.0f && r + rPrev <= sPrev) {
+                    float back = sPrev - rPrev;
+                    p = p - dir * back; trav -= back;
+                    om = 1.0f;
+                    steps += 1u; nsteps += 1ull;
+                    continue;
+                }
                 if (ea < marchTol(setting, p) || steps > maxiter) { result = 2; break; }
                 if (ea >= tmax - trav || (dot(p, dir) > 0.0f && dot(p, p) > R2)) { result = 1; break; }
-                float t = ea * stepscale;
+                float t = r * om;
                 p = p + dir * t; trav += t;
+                rPrev = r; sPrev = t;
                 steps += 1u; nsteps += 1ull;
             }
             if (result != 0) {

@@ -53,7 +53,8 @@ class PtUniforms(ctypes.Structure):
                 ('fireflyClamp', ctypes.c_float), ('fireflyWhite', ctypes.c_float), ('maxaverages', ctypes.c_uint),
                 ('seed', ctypes.c_uint), ('cam', PtCamera), ('env', PtEnv), ('haze', PtHaze), ('faceShare', ctypes.c_float),
                 ('sppm', ctypes.c_uint), ('sppmRadius', ctypes.c_float), ('sppmPhotons', ctypes.c_uint),
-                ('marchRefill', ctypes.c_uint), ('guide', ctypes.c_uint), ('guideFrac', ctypes.c_float), ('sppmAlpha', ctypes.c_float)]
+                ('marchRefill', ctypes.c_uint), ('guide', ctypes.c_uint), ('guideFrac', ctypes.c_float), ('sppmAlpha', ctypes.c_float),
+                ('relax', ctypes.c_float)]
 
 
 class PtSceneDesc(ctypes.Structure):
@@ -215,6 +216,7 @@ def default_uniforms():
     u.fireflyClamp, u.fireflyWhite, u.maxaverages, u.seed = 0.0, 1.0, 0, 20261203
     u.faceShare, u.marchRefill, u.guideFrac = 0.5, 0, 0.9
     u.sppmRadius, u.sppmAlpha = 0.02, 2.0 / 3.0
+    u.relax = 0.0
     return u
 
 
@@ -235,7 +237,7 @@ def uniforms_for(made, **over):
     for k in ('maxiter', 'marchRefill', 'rr', 'rrStart', 'diffuseCap', 'maxaverages', 'sppm', 'sppmPhotons'):
         if k in s:
             setattr(u, k, int(s[k]))
-    for k in ('stepscale', 'eps', 'epsAuto', 'faceShare', 'fireflyClamp', 'sppmRadius', 'sppmAlpha'):
+    for k in ('stepscale', 'eps', 'epsAuto', 'faceShare', 'fireflyClamp', 'sppmRadius', 'sppmAlpha', 'relax'):
         if k in s:
             setattr(u, k, float(s[k]))
     return u

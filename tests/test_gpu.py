@@ -58,6 +58,25 @@ def test_primary_ray_decisions_match_the_oracle(engine):
     assert np.quantile(d, 0.999) < 5e-3
 
 
+def test_relaxed_march_meets_the_same_surfaces(engine):
+    # over-relaxed sphere tracing (relax 1.6) keeps a step only where the free balls overlap, so each ray meets the
+    # surface the plain march meets: primary-ray decisions against the oracle's plain march, and the C1 floor
+    say('case: room-c2 480x270 relax 1.6, pick vs oracle; plane-c1 128x128 relax 1.6')
+    made, u = load(engine, 'room-c2', width=480, height=270, relax=1.6)
+    gpos, gstate, gmat, gprim = engine.pick_rays()
+    opos, ostate, omat, oprim = oracle.pick(made['scene'], pt.uniforms_for(made))
+    same = (gstate == ostate) & ((gstate != 0) | (gprim == oprim))
+    say('agreement', same.mean())
+    assert same.mean() >= 0.9999
+    hit = (gstate == 0) & same
+    assert np.quantile(np.linalg.norm(gpos[hit, :3] - opos[hit, :3], axis=1), 0.999) < 5e-3
+    load(engine, 'plane-c1', width=128, height=128, relax=1.6)
+    engine.trace(32)
+    acc, _ = engine.read_accum()
+    L = pt.luminance(acc)
+    assert abs(L[:32].mean() / 800 - 1) < 1e-3, L[:32].mean()
+
+
 def test_c2_per_pixel_means_agree_with_the_oracle(engine):
     say('case: room-c2 48x27, GPU 256 spp vs oracle 64 spp')
     made, u = load(engine, 'room-c2', width=48, height=27)
