@@ -263,6 +263,7 @@ def main():
     ap.add_argument('--tuning', default=None, help='blocksPerCu,refillMarch,refillOcclude: set, not tuned')
     ap.add_argument('--stragglers', type=int, default=None, help='rounds beyond maxbounce + 1 (default: the library\'s)')
     ap.add_argument('--relax', type=float, default=0.0, help='over-relaxed sphere tracing factor (0: the plain march)')
+    ap.add_argument('--no-fold', action='store_true', help='keep the kernels that read the parameter block (no pt_fold_params)')
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py: this build runs on one GPU; --gpus %d is not supported (samples would shard across GPUs with an '
@@ -290,6 +291,9 @@ def main():
     e.set_scene(made['scene'])
     u = pt.uniforms_for(made, relax=a.relax)
     e.set_uniforms(u)
+    # the scene does not move during the run: its kernels with the parameter block as constants (untimed compile)
+    if not a.no_fold:
+        e.fold_params()
     # the persistent kernels' launch choices for this scene and image, by measurement (before the warm-up; untimed)
     if a.tuning:
         b, rm, ro = (int(v) for v in a.tuning.split(','))
@@ -343,7 +347,7 @@ def main():
         'config': {'workload': 'room-c2: 5x4x3 m room, 32 SDF primitives, 8 smooth unions, GGX, 0.5 m panel 5000 cd/m^2, '
                                '%dx%d, depth 8, NEE on, 1 sample per step' % (a.width, a.height),
                    'width': a.width, 'height': a.height, 'maxbounce': u.maxbounce, 'accumulation': 'f64',
-                   'straggler_rounds': e.straggler_rounds, 'relax': u.relax,
+                   'straggler_rounds': e.straggler_rounds, 'relax': u.relax, 'params_folded': not a.no_fold,
                    'tuning': None if not tuning else {k: tuning[k] for k in ('blocksPerCu', 'refillMarch', 'refillOcclude',
                                                                             'msPerSample', 'msBefore', 'candidates')}},
         'shadow_segments_per_s': st['shadow_segments'] / dt, 'march_steps_per_s': st['steps'] / dt,

@@ -83,6 +83,13 @@ int pt_set_scene(pt_ctx* ctx, const pt_scene_desc* scene);
  * rebuild. Replaces the scene-uniform half of the uniform bag (raytrace_main.js:1997-2023). */
 int pt_set_params(pt_ctx* ctx, const float* params, uint32_t nparams);
 
+/* Compiles (or finds in the cache) the scene's kernels with its current parameter block folded in as constants and
+ * runs those until the next pt_set_params or pt_set_scene, which go back to the kernels that read the block. Same
+ * images (the same arithmetic on the same values); the distance function's parameter reads become immediates. No
+ * reference counterpart (the reference keeps its scene parameters live uniforms): for a host about to render a
+ * scene that will not move, after its edits. */
+int pt_fold_params(pt_ctx* ctx);
+
 /* Sets the engine uniforms (size must be sizeof(PtUniforms)); width/height resize the buffers.
  * Replaces the base half of the uniform bag and R.ensureBuffers (raytrace_main.js:234-302, :1997-2023). */
 int pt_set_uniforms(pt_ctx* ctx, const PtUniforms* u, size_t size);

@@ -7,6 +7,11 @@
 // loads (s_load) into SGPRs instead of per-lane vector loads.
 #define PT_CONST __attribute__((address_space(4)))
 typedef const PT_CONST float* __restrict__ ParamPtr;
+// The scene's functions read their parameters through U, the block pt_set_params updates. A compile with the
+// parameters folded in (pt_fold_params) redefines this to point U at a constant copy, so each read is an immediate.
+#ifndef PT_SCENE_PARAMS
+#define PT_SCENE_PARAMS(U) (void)0
+#endif
 
 #define PT_PI 3.14159265358979f
 #define PT_INV_PI 0.318309886183791f

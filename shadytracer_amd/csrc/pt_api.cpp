@@ -159,6 +159,10 @@ struct pt_ctx {
     hipModule_t module = nullptr;
     hipFunction_t fn[K_COUNT] = {};
     bool building = false;
+    std::string sceneSrc;            // the loaded scene's emitted source and parameter block, for pt_fold_params
+    std::vector<float> params;
+    hipModule_t foldedModule = nullptr;   // the scene compiled with those parameters as constants, while in use
+    hipFunction_t genericFn[K_COUNT] = {};
 
     PtDev host{};           // host mirror of the pointer table
     PtDev* dev = nullptr;   // device copy
@@ -629,9 +633,38 @@ void pt_destroy(pt_ctx* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     destroyGraph(c);
     for (void* p : c->owned) if (p) hipFree(p);
+    if (c->foldedModule) hipModuleUnload(c->foldedModule);
     if (c->module) hipModuleUnload(c->module);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
+}
+
+// Compiles (or finds) and loads a scene's code object and its kernels; on failure nothing is kept.
+static int loadSceneModule(pt_ctx* c, const char* src, hipModule_t& mod, hipFunction_t (&fns)[K_COUNT]) {
+    std::vector<char> code;
+    std::string path, err;
+    if (!compileScene(src, c->cacheDir, code, path, err)) return fail(c, PT_ERR_COMPILE, "%s", err.c_str());
+    mod = nullptr;
+    hipError_t e = hipModuleLoadData(&mod, code.data());
+    if (e != hipSuccess) return fail(c, PT_ERR_COMPILE, "hipModuleLoadData(%s): %s", path.c_str(), hipGetErrorString(e));
+    for (int k = 0; k < K_COUNT; ++k) {
+        if ((e = hipModuleGetFunction(&fns[k], mod, kKernelNames[k])) != hipSuccess) {
+            hipModuleUnload(mod);
+            mod = nullptr;
+            return fail(c, PT_ERR_COMPILE, "kernel %s missing: %s", kKernelNames[k], hipGetErrorString(e));
+        }
+    }
+    return PT_OK;
+}
+
+// Back to the kernels that read the parameter block (after the block changes, or a new scene).
+static void unfold(pt_ctx* c) {
+    if (!c->foldedModule) return;
+    hipStreamSynchronize(c->stream);
+    destroyGraph(c);
+    memcpy(c->fn, c->genericFn, sizeof c->fn);
+    hipModuleUnload(c->foldedModule);
+    c->foldedModule = nullptr;
 }
 
 int pt_set_scene(pt_ctx* c, const pt_scene_desc* s) {
@@ -642,29 +675,20 @@ int pt_set_scene(pt_ctx* c, const pt_scene_desc* s) {
     for (uint32_t i = 0; i < s->nmaterials; ++i)
         if (s->materials[i].face >= (int)s->nfaces) return fail(c, PT_ERR_ARG, "material %u names light face %d of %u", i, s->materials[i].face, s->nfaces);
     c->building = true;
-    std::vector<char> code;
-    std::string path, err;
-    if (!compileScene(s->hip_src, c->cacheDir, code, path, err)) {
-        c->building = false;
-        return fail(c, PT_ERR_COMPILE, "%s", err.c_str());
-    }
     hipModule_t mod = nullptr;
-    hipError_t e = hipModuleLoadData(&mod, code.data());
-    if (e != hipSuccess) { c->building = false; return fail(c, PT_ERR_COMPILE, "hipModuleLoadData(%s): %s", path.c_str(), hipGetErrorString(e)); }
     hipFunction_t fns[K_COUNT];
-    for (int k = 0; k < K_COUNT; ++k) {
-        if ((e = hipModuleGetFunction(&fns[k], mod, kKernelNames[k])) != hipSuccess) {
-            hipModuleUnload(mod);
-            c->building = false;
-            return fail(c, PT_ERR_COMPILE, "kernel %s missing: %s", kKernelNames[k], hipGetErrorString(e));
-        }
-    }
+    int lrc = loadSceneModule(c, s->hip_src, mod, fns);
+    c->building = false;
+    if (lrc) return lrc;
+    unfold(c);
     hipStreamSynchronize(c->stream);
     destroyGraph(c);
     if (c->module) hipModuleUnload(c->module);
     c->module = mod;
     memcpy(c->fn, fns, sizeof fns);
-    c->building = false;
+    memcpy(c->genericFn, fns, sizeof fns);
+    c->sceneSrc = s->hip_src;
+    c->params.assign(s->params, s->params + s->nparams);
     // tables
     if (s->nparams > c->nparams) {
         void* p = (void*)c->host.params; freeBuf(c, p); c->host.params = nullptr;
@@ -698,7 +722,38 @@ int pt_set_params(pt_ctx* c, const float* params, uint32_t n) {
     if (!c || (n && !params)) return fail(c, PT_ERR_ARG, "params is NULL");
     if (!c->module) return fail(c, PT_ERR_STATE, "no scene loaded");
     if (n != c->nparams) return fail(c, PT_ERR_ARG, "params: %u values, the scene has %u", n, c->nparams);
+    unfold(c);
     if (n) HIPCHK(c, hipMemcpyAsync((void*)c->host.params, params, size_t(n) * 4, hipMemcpyHostToDevice, c->stream));
+    c->params.assign(params, params + n);
+    return PT_OK;
+}
+
+int pt_fold_params(pt_ctx* c) {
+    if (!c) return fail(c, PT_ERR_ARG, "ctx is NULL");
+    if (!c->module) return fail(c, PT_ERR_STATE, "no scene loaded");
+    if (c->foldedModule) return PT_OK;
+    // each value as a hexadecimal float literal: exactly the float the block holds
+    std::string src = "// the parameter block as constants (pt_fold_params)\nstatic constexpr float PT_UC[" +
+                      std::to_string(c->params.size() ? c->params.size() : 1) + "] = {";
+    char buf[48];
+    for (size_t i = 0; i < c->params.size(); ++i) {
+        if (!std::isfinite(c->params[i])) return fail(c, PT_ERR_ARG, "parameter %zu is not finite", i);
+        snprintf(buf, sizeof buf, "%s%af", i ? ", " : "", (double)c->params[i]);
+        src += buf;
+    }
+    if (c->params.empty()) src += "0.0f";
+    src += "};\n#undef PT_SCENE_PARAMS\n#define PT_SCENE_PARAMS(U) ((U) = (ParamPtr)PT_UC)\n";
+    src += c->sceneSrc;
+    c->building = true;
+    hipModule_t mod = nullptr;
+    hipFunction_t fns[K_COUNT];
+    int rc = loadSceneModule(c, src.c_str(), mod, fns);
+    c->building = false;
+    if (rc) return rc;
+    hipStreamSynchronize(c->stream);
+    destroyGraph(c);
+    c->foldedModule = mod;
+    memcpy(c->fn, fns, sizeof fns);
     return PT_OK;
 }
 

@@ -182,6 +182,16 @@ napi_value SetParams(napi_env env, napi_callback_info info) {
     return undef(env);
 }
 
+napi_value FoldParams(napi_env env, napi_callback_info info) {
+    napi_value a[1];
+    if (!args(env, info, 1, a)) return nullptr;
+    pt_ctx* ctx = ctxOf(env, a[0]);
+    if (!ctx) return nullptr;
+    int rc = pt_fold_params(ctx);
+    if (rc) return throwPt(env, ctx, rc);
+    return undef(env);
+}
+
 napi_value SetUniforms(napi_env env, napi_callback_info info) {
     napi_value a[2];
     if (!args(env, info, 2, a)) return nullptr;
@@ -375,6 +385,7 @@ napi_value Init(napi_env env, napi_value exports) {
         {"compileScene", nullptr, CompileScene, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"setScene", nullptr, SetScene, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"setParams", nullptr, SetParams, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"foldParams", nullptr, FoldParams, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"setUniforms", nullptr, SetUniforms, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"clearAccum", nullptr, ClearAccum, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"trace", nullptr, Trace, nullptr, nullptr, nullptr, napi_enumerable, nullptr},

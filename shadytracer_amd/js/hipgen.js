@@ -402,18 +402,21 @@ function generate(doc, opts) {
         '// the nodes a ray from o along d can reach within tmax: those whose bounding sphere, widened by margin, its',
         '// segment meets (bit k of the mask: the k-th bounded top-level node); unbounded nodes are always marched',
         '__device__ __forceinline__ unsigned long long sceneMask(ParamPtr U, V3 o, V3 d, float tmax, float margin) {',
+        '    PT_SCENE_PARAMS(U);',
         '    unsigned long long m = ~0ull;',
         ...maskCode,
         '    return m;',
         '}',
         '// the closest signed distance over the nodes in mask; w counts the FP32 operations spent',
         '__device__ __forceinline__ float sceneDistM(ParamPtr U, V3 p, unsigned long long mask, unsigned int& w) {',
+        '    PT_SCENE_PARAMS(U);',
         '    float best = 1.0e7f;',
         ...dist,
         '    return best;',
         '}',
         '__device__ __forceinline__ float sceneDist(ParamPtr U, V3 p) { unsigned int w = 0u; return sceneDistM(U, p, ~0ull, w); }',
         '__device__ __forceinline__ SdfHit sceneFull(ParamPtr U, V3 p) {',
+        '    PT_SCENE_PARAMS(U);',
         '    SdfHit h = sdfEmpty();',
         ...full,
         '    return h;',

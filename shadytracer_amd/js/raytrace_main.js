@@ -131,6 +131,9 @@ function RAYTRACER(WG, _U, Q, scene) {
         return A.autotune(h, samples)
     }
     R.setTuning = (blocksPerCu, refillMarch, refillOcclude) => A.setTuning(h, blocksPerCu, refillMarch, refillOcclude)
+    // the scene's kernels with its parameters as constants (pt_fold_params), until its parameters or the scene change
+    R.setParams = (params) => { if (!current) throw new Error('RAYTRACER: no scene'); A.setParams(h, Float32Array.from(params)) }
+    R.foldParams = () => { if (!current) throw new Error('RAYTRACER: no scene'); A.foldParams(h) }
     R.ensureBuffers = (w, hh) => { U.width = w; U.height = hh }
     R.releaseBuffers = () => { if (h) { A.destroy(h); h = null } }
 

@@ -119,6 +119,7 @@ SIGNATURES = {
     'pt_compile_scene': (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]),
     'pt_set_scene': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PtSceneDesc)]),
     'pt_set_params': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_uint32]),
+    'pt_fold_params': (ctypes.c_int, [ctypes.c_void_p]),
     'pt_set_uniforms': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PtUniforms), ctypes.c_size_t]),
     'pt_clear_accum': (ctypes.c_int, [ctypes.c_void_p]),
     'pt_trace': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
@@ -306,6 +307,10 @@ class Engine:
     def set_params(self, params):
         a = np.ascontiguousarray(np.asarray(params, dtype=np.float32))
         self._chk(self.lib.pt_set_params(self.h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(a)))
+
+    def fold_params(self):
+        """The scene's kernels with its current parameter block as constants, until set_params or set_scene."""
+        self._chk(self.lib.pt_fold_params(self.h))
 
     def set_uniforms(self, u):
         self._chk(self.lib.pt_set_uniforms(self.h, ctypes.byref(u), ctypes.sizeof(u)))

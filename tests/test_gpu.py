@@ -216,6 +216,42 @@ def test_params_move_geometry_without_a_rebuild(engine):
     assert np.allclose(p1[hit, 2], 0.25, atol=2e-3) and np.allclose(p0[hit, 2], 0.0, atol=2e-3)
 
 
+def test_folded_params_render_the_same_scene(engine):
+    # pt_fold_params compiles the scene with its parameter block as constants: the same primary decisions as the
+    # oracle's, the same per-pixel means as the kernels that read the block, and a later set_params goes back to
+    # those (the moved geometry shows at once)
+    say('case: room-c2 480x270 folded, pick vs oracle; 48x27 folded vs unfolded means; plane-c1 fold then move')
+    made, u = load(engine, 'room-c2', width=480, height=270)
+    engine.fold_params()
+    gpos, gstate, _, gprim = engine.pick_rays()
+    opos, ostate, _, oprim = oracle.pick(made['scene'], u)
+    same = (gstate == ostate) & ((gstate != 0) | (gprim == oprim))
+    say('agreement', same.mean())
+    assert same.mean() >= 0.9999
+    made, u = load(engine, 'room-c2', width=48, height=27)
+    engine.trace(128)
+    ua, um = engine.read_accum()
+    engine.fold_params()
+    engine.clear_accums()
+    engine.trace(128)
+    fa, fm = engine.read_accum()
+    uL, fL = pt.luminance(ua), pt.luminance(fa)
+    sd = np.sqrt(pt.standard_error(um) ** 2 + pt.standard_error(fm) ** 2)
+    ok = sd > 1e-9
+    z = (uL - fL)[ok] / sd[ok]
+    say('folded vs unfolded: identical pixels %.4f, z std %.3f' % (np.mean(uL == fL), z.std()))
+    assert np.mean(np.abs(z) < 3) > 0.97 and z.std() < 1.3
+    made, u = load(engine, 'plane-c1', width=64, height=64)
+    engine.fold_params()
+    p0, s0, _, _ = engine.pick_rays()
+    params = np.asarray(made['scene']['params'], np.float32).copy()
+    params[2] = 0.25
+    engine.set_params(params)
+    p1, s1, _, _ = engine.pick_rays()
+    hit = (s0 == 0) & (s1 == 0)
+    assert np.allclose(p1[hit, 2], 0.25, atol=2e-3) and np.allclose(p0[hit, 2], 0.0, atol=2e-3)
+
+
 def test_node_addon_drives_the_engine():
     say('case: node tests/js/gpu_step.js (createPathTracer through the N-API addon)')
     r = subprocess.run(['node', os.path.join(ROOT, 'tests', 'js', 'gpu_step.js')], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
