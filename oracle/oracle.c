@@ -79,7 +79,7 @@ static void sampleSeed(uint32_t base, uint32_t sample, uint32_t* out) {
 
 /* ---- the scene program (hipgen.js `program`) ---- */
 enum { OP_PRIM = 0, OP_JOIN = 1, OP_FOLD = 2, OP_NODE = 3 };
-enum { SH_PLANE = 0, SH_SPHERE, SH_BOX, SH_CYLINDER, SH_CAPSULE, SH_TORUS, SH_RECT, SH_DISC, SH_LENS, SH_PRISM };
+enum { SH_PLANE = 0, SH_SPHERE, SH_BOX, SH_CYLINDER, SH_CAPSULE, SH_TORUS, SH_RECT, SH_DISC, SH_LENS, SH_PRISM, SH_REFLECTOR, SH_ASPHERE };
 enum { J_UNION = 0, J_SUBTRACT, J_INTERSECT };
 /* prim: shape, xf, p, mat, prim; join: join, k; node (a top-level node's start): xf = its bounding sphere's offset in
  * U (-1: unbounded), join = its mask bit (-1: none), k = the index of its fold */
@@ -111,8 +111,56 @@ static d4 lensFace(d3 p, double s, double vz, double R, double rim) {
     return f;
 }
 
+/* The nearest point of the parabola y = k x^2 (k > 0) to (x, y) in a meridian plane, by Newton's method on
+ * g(t) = 2k^2 t^3 + (1 - 2ky) t - x from t = x (g' > 0 wherever the minimum is unique, g is increasing past the
+ * largest root), polished in FP64 -- a different route to the root the GPU takes by Cardano (shapes.js g2Parabola:61).
+ * Returns the signed distance (positive below the curve) and the outward normal (gx, gy). */
+static double parabolaD(double x, double y, double k, double* gx, double* gy) {
+    double sx = x >= 0 ? 1 : -1, xa = fabs(x);
+    double t = maxd(xa, sqrt(maxd(y / k, 0)));   /* at or past the largest root: g(t) >= 0 there */
+    for (int i = 0; i < 200; ++i) {
+        double g = 2 * k * k * t * t * t + (1 - 2 * k * y) * t - xa, dg = 6 * k * k * t * t + (1 - 2 * k * y);
+        if (dg <= 0) { t *= 1.5; continue; }
+        double nt = t - g / dg;
+        if (fabs(nt - t) <= 1e-15 * maxd(1, fabs(t))) { t = nt; break; }
+        t = nt;
+    }
+    double ex = xa - t, ey = y - k * t * t, side = k * xa * xa - y >= 0 ? 1 : -1;
+    double in = 1 / sqrt(1 + 4 * k * k * t * t);
+    *gx = 2 * k * t * in * sx; *gy = -in;
+    return side * sqrt(ex * ex + ey * ey);
+}
+typedef struct { double x, y, d; } g2v;
+static g2v g2max(g2v a, g2v b) { return a.d >= b.d ? a : b; }
+static g2v parabolaOff(double x, double y, double k, double c) {
+    g2v a, b;
+    a.d = parabolaD(x - c, y, k, &a.x, &a.y);
+    if (c == 0) return a;
+    b.d = parabolaD(x + c, y, k, &b.x, &b.y);
+    if (c < 0) return a.d >= b.d ? a : b;
+    return a.d <= b.d ? a : b;
+}
+/* One face of an aspheric lens (core/shapes.js asphSag, asphSlopeMax, asphFace: 658-700): c the curvature (0 flat), k
+ * the conic constant, a[4] the even terms; the height above the surface over the steepest slope within the rim. */
+static double asphSagD(double r, double c, double k, const float* a, double* slope) {
+    double r2 = r * r, w = 1 - (1 + k) * c * c * r2, u = sqrt(maxd(w, 1e-4));
+    double poly = r2 * r2 * (a[0] + r2 * (a[1] + r2 * (a[2] + r2 * a[3])));
+    double dpoly = r * r2 * (4 * a[0] + r2 * (6 * a[1] + r2 * (8 * a[2] + r2 * 10 * a[3])));
+    *slope = (w >= 1e-4 ? c * r / u : 2 * c * r / (1 + u)) + dpoly;
+    return c * r2 / (1 + u) + poly;
+}
+static d4 asphFaceD(d3 p, double l, double rx, double ry, double rho, double ht, double R, double k, const float* a, double s) {
+    double c = R == 0 ? 0 : 1 / R, slope;
+    double sag = asphSagD(mind(l, rho), c, k, a, &slope);
+    double r2 = rho * rho, u = sqrt(maxd(1 - (1 + k) * c * c * r2, 1e-4));
+    double m = fabs(c) * rho / u + rho * r2 * (4 * fabs(a[0]) + r2 * (6 * fabs(a[1]) + r2 * (8 * fabs(a[2]) + r2 * 10 * fabs(a[3]))));
+    double g = l < rho ? slope : 0, in = 1 / sqrt(g * g + 1);
+    return d4of(v(rx * g * in, ry * g * in, s * in), (s * p.z - ht + sag) / sqrt(1 + m * m));
+}
+
 /* Primitive (outward normal, distance) in its own frame: core/sdf.js:352-628 restated; the lens core/sdf.js:463, the
- * prism core/shapes.js:263-287 (a regular polygon extruded). */
+ * prism core/shapes.js:263-287 (a regular polygon extruded), the reflector core/shapes.js:392-421, the aspheric lens
+ * core/shapes.js:658. */
 static d4 primLocal(int shape, d3 p, const float* P) {
     switch (shape) {
     case SH_PLANE: return d4of(v(0, 0, 1), p.z);
@@ -192,6 +240,30 @@ static d4 primLocal(int shape, d3 p, const float* P) {
         if (g > 0 && wz > 0) { double l = sqrt(g * g + wz * wz); return d4of(v(gx * g / l, gy * g / l, sz * wz / l), l); }
         if (g > wz) return d4of(v(gx, gy, 0), g);
         return d4of(v(0, 0, sz), wz);
+    }
+    case SH_REFLECTOR: {    /* P: focal, aperture, wall, solid, focus ring radius */
+        double fo = maxd(P[0], 1e-6), A = maxd(P[1], 0), c = mind(P[4], A);
+        double x = sqrt(p.x * p.x + p.y * p.y), y = p.z + fo, l = x > 1e-12 ? x : 1;
+        g2v s = parabolaOff(x, y, 0.25 / fo, c), r;
+        if (P[3] > 0.5) {
+            g2v cap = {0, 1, y - 0.25 * (A - c) * (A - c) / fo};
+            r = g2max(s, cap);
+        } else {
+            g2v ns = {-s.x, -s.y, -s.d}, off = {s.x, s.y, s.d - maxd(P[2], 1e-6)}, cyl = {1, 0, x - A};
+            r = g2max(g2max(ns, off), cyl);
+        }
+        double rx = x > 1e-12 ? p.x / l : 1, ry = x > 1e-12 ? p.y / l : 0;
+        return d4of(v(rx * r.x, ry * r.x, r.y), r.d);
+    }
+    case SH_ASPHERE: {      /* P: rho, ht, 7 per face (the emitter's), then the faces' radii R1, R2 */
+        double rho = P[0], ht = P[1], l = sqrt(p.x * p.x + p.y * p.y);
+        double rx = l > 1e-12 ? p.x / l : 1, ry = l > 1e-12 ? p.y / l : 0;
+        d4 n = d4of(v(rx, ry, 0), l - rho);
+        d4 a = asphFaceD(p, l, rx, ry, rho, ht, P[16], P[3], P + 4, 1);
+        if (a.w > n.w) n = a;
+        d4 b = asphFaceD(p, l, rx, ry, rho, ht, P[17], P[10], P + 11, -1);
+        if (b.w > n.w) n = b;
+        return n;
     }
     }
     return d4of(v(0, 0, 1), 1e30);

@@ -304,6 +304,115 @@ __device__ __forceinline__ V4 Nprism(V3 p, float n, float in, float a, float e, 
     if (g > wz) return v4(v3(gx, gy, 0.0f), g);
     return v4(v3(0.0f, 0.0f, sz), wz);
 }
+// A meridian-plane result (gx, gy: the outward normal in (r, z); d the signed distance), as the revolved shapes
+// build theirs; ties go to the first (shapes.js g2Max:102).
+struct G2 { float x, y, d; };
+__device__ __forceinline__ G2 g2(float x, float y, float d) { G2 r; r.x = x; r.y = y; r.d = d; return r; }
+__device__ __forceinline__ G2 g2max(G2 a, G2 b) { return a.d >= b.d ? a : b; }
+// Lifted about the z axis: the normal's radial part along p.xy's direction (+x on the axis) (shapes.js g2Revolve).
+__device__ __forceinline__ V4 g2revolve(V3 p, G2 g) {
+    float l = fsqrt(p.x * p.x + p.y * p.y);
+    float rx = 1.0f, ry = 0.0f;
+    if (l > 1e-12f) { float il = frcp(l); rx = p.x * il; ry = p.y * il; }
+    return v4(v3(rx * g.x, ry * g.x, g.y), g.d);
+}
+// The signed distance in a meridian plane from (x, y) to the parabola y = k x^2 (k > 0), positive below it, with its
+// outward normal (shapes.js g2Parabola:61). The nearest point's abscissa t solves the depressed cubic
+// 2k^2 t^3 + (1 - 2ky) t - x = 0: with a = (2ky - 1) / 6k^2 and b = x / 4k^2 its discriminant is b^2 - a^3; above
+// zero one real root (Cardano, as 2b / (u^2 - a + w^2) with u = cbrt(b + sqrt(h)), w = a / u, which keeps its
+// digits), else the largest of three, 2 sqrt(a) cos(atan2(sqrt(-h), b) / 3).
+__device__ __forceinline__ G2 g2parabola(float x, float y, float k) {
+    const float sx = x >= 0.0f ? 1.0f : -1.0f, xa = fabsf(x);
+    const float ik = 1.0f / k;
+    const float a = ik * (y - 0.5f * ik) * (1.0f / 3.0f);
+    const float b = 0.25f * ik * ik * xa;
+    const float h = b * b - a * a * a;
+    float t;
+    if (h > 0.0f) {
+        const float u = cbrtf(b + fsqrt(h));
+        const float w = a / u;
+        t = 2.0f * b / (u * u - a + w * w);
+    } else {
+        t = 2.0f * fsqrt(fmaxf(a, 0.0f)) * cosf(atan2f(fsqrt(-h), b) * (1.0f / 3.0f));
+    }
+    const float ex = xa - t, ey = y - k * t * t;
+    const float side = k * xa * xa - y >= 0.0f ? 1.0f : -1.0f;
+    const float in = frsq(1.0f + 4.0f * k * k * t * t);
+    return g2(2.0f * k * t * in * sx, -in, side * fsqrt(ex * ex + ey * ey));
+}
+// ...with its focus moved out to a ring of radius c in the plane of the foci (shapes.js g2ParabolaOffset:93): the two
+// half-planes' parabolas shifted by +-c, the larger of the two values for c < 0, the smaller for c > 0.
+__device__ __forceinline__ G2 g2parabolaOff(float x, float y, float k, float c) {
+    const G2 a = g2parabola(x - c, y, k);
+    if (c == 0.0f) return a;
+    const G2 b = g2parabola(x + c, y, k);
+    if (c < 0.0f) return a.d >= b.d ? a : b;
+    return a.d <= b.d ? a : b;
+}
+// A parabolic reflector (shapes.js NparabolicOffset:418 over NdishOffset:392 and NparaboloidOffset): its focus at the
+// origin, opening to +z, the mirror z = (r - c)^2 / 4f - f out to radius A. solid <= 0.5: the mirror with its wall w
+// thick behind it (the band 0 <= s <= w of the distance s to the parabola), cut by the cylinder r <= A; else the
+// paraboloid's inside, capped at the rim's height. U: f, A, w, solid, c (c held at most A by the emitter).
+__device__ __forceinline__ G2 parabolicG2(V3 p, float f, float A, float wall, float solid, float c) {
+    const float fo = fmaxf(f, 1e-6f), Aa = fmaxf(A, 0.0f), cc = fminf(c, Aa);
+    const float x = fsqrt(p.x * p.x + p.y * p.y), y = p.z + fo;
+    const G2 s = g2parabolaOff(x, y, 0.25f / fo, cc);
+    if (solid > 0.5f) return g2max(s, g2(0.0f, 1.0f, y - 0.25f * (Aa - cc) * (Aa - cc) / fo));
+    const G2 band = g2max(g2(-s.x, -s.y, -s.d), g2(s.x, s.y, s.d - fmaxf(wall, 1e-6f)));
+    return g2max(band, g2(1.0f, 0.0f, x - Aa));
+}
+__device__ __forceinline__ V4 Nparabolic(V3 p, float f, float A, float wall, float solid, float c) {
+    return g2revolve(p, parabolicG2(p, f, A, wall, solid, c));
+}
+__device__ __forceinline__ float Dparabolic(V3 p, float f, float A, float wall, float solid, float c) {
+    return parabolicG2(p, f, A, wall, solid, c).d;
+}
+
+// An aspheric lens along z (shapes.js Nasphere:658): a round rim of radius rho and two even-asphere faces, their
+// vertices at z = +-ht, each face's sag z(r) = c r^2 / (1 + sqrt(1 - (1 + k) c^2 r^2)) + a4 r^4 + a6 r^6 + a8 r^8 +
+// a10 r^10 measured into the lens (c = 1 / R; 0 flat). A face's value is its height above the surface,
+// s z - ht + sag(r) (r held at rho beyond the rim), over sqrt(1 + m^2), m the steepest the face can be within the
+// rim: 1-Lipschitz, so a bound both sides. Its normal is the surface's own. The root is held at 1e-4 where a conic
+// too small for the aperture has none. U per face: c, k, a4, a6, a8, a10, 1 / sqrt(1 + m^2).
+__device__ __forceinline__ void asphSag(float r, float c, float k, float a4, float a6, float a8, float a10, float& sag, float& slope) {
+    const float r2 = r * r;
+    const float w = 1.0f - (1.0f + k) * c * c * r2;
+    const float u = fsqrt(fmaxf(w, 1e-4f));
+    const float poly = r2 * r2 * (a4 + r2 * (a6 + r2 * (a8 + r2 * a10)));
+    const float dpoly = r * r2 * (4.0f * a4 + r2 * (6.0f * a6 + r2 * (8.0f * a8 + r2 * 10.0f * a10)));
+    sag = c * r2 / (1.0f + u) + poly;
+    slope = (w >= 1e-4f ? c * r / u : 2.0f * c * r / (1.0f + u)) + dpoly;
+}
+__device__ __forceinline__ V4 asphFace(V3 p, float l, float rx, float ry, float rho, float ht, const PT_CONST float* F, float s) {
+    float sag, slope;
+    asphSag(fminf(l, rho), F[0], F[1], F[2], F[3], F[4], F[5], sag, slope);
+    const float g = l < rho ? slope : 0.0f;
+    const float in = frsq(g * g + 1.0f);
+    return v4(v3(rx * g * in, ry * g * in, s * in), (s * p.z - ht + sag) * F[6]);
+}
+__device__ __forceinline__ V4 Nasphere(V3 p, const PT_CONST float* P) {
+    const float rho = P[0], ht = P[1];
+    const float l = fsqrt(p.x * p.x + p.y * p.y);
+    float rx = 1.0f, ry = 0.0f;
+    if (l > 1e-12f) { const float il = frcp(l); rx = p.x * il; ry = p.y * il; }
+    V4 n = v4(v3(rx, ry, 0.0f), l - rho);
+    const V4 top = asphFace(p, l, rx, ry, rho, ht, P + 2, 1.0f);
+    if (top.w > n.w) n = top;
+    const V4 bottom = asphFace(p, l, rx, ry, rho, ht, P + 9, -1.0f);
+    if (bottom.w > n.w) n = bottom;
+    return n;
+}
+__device__ __forceinline__ float Dasphere(V3 p, const PT_CONST float* P) {
+    const float rho = P[0], ht = P[1];
+    const float l = fsqrt(p.x * p.x + p.y * p.y);
+    float d = l - rho, sag, slope;
+    const float rr = fminf(l, rho);
+    asphSag(rr, P[2], P[3], P[4], P[5], P[6], P[7], sag, slope);
+    d = fmaxf(d, (p.z - ht + sag) * P[8]);
+    asphSag(rr, P[9], P[10], P[11], P[12], P[13], P[14], sag, slope);
+    return fmaxf(d, (-p.z - ht + sag) * P[15]);
+}
+
 __device__ __forceinline__ float Dprism(V3 p, float n, float in, float a, float e, float h) {
     float nx, ny, u, v;
     prismSide(p.x, p.y, n, in, nx, ny, u, v);

@@ -18,9 +18,9 @@
  *   sppm-slab    a collimated beam through a glass slab onto a floor: the photon map's closed form.
  *   furnace-*    closed-form checks: a white furnace (albedo-1 Lambertian plane under a uniform sky, and a haze box
  *                under a uniform sphere of sky).
- *   studio-floor, lens-prism
+ *   studio-floor, lens-prism, reflector-condenser
  *                documents in the builder's own form (millimetres, the studio environment, lights by CCT and flux,
- *                an orbit view), the second with the lens and prism builtins.
+ *                an orbit view), the others with the lens, prism, reflector and aspheric lens builtins.
  */
 
 const SEED = 20261203
@@ -249,7 +249,30 @@ function lensPrism(opts = {}) {
     return { doc: d, camera: cameraOf(d, w, h), settings: { width: w, height: h, spp: 64, maxbounce: 8, nee: 1 } }
 }
 
+// reflector-condenser: a mirror reflector (focal 8 mm, aperture 25 mm, wall 1 mm) on a post, opening up, with a 3 mm
+// disc of 200 lm at its focus facing into the bowl; and the aspheric condenser builtin at its defaults, 30 times (a
+// 30 mm condenser of k -2.25 over a flat back) standing on a grey floor; the studio of 500 lx.
+function reflectorCondenser(opts = {}) {
+    const { cameraOf } = require('./hipgen')
+    const blocks = BLOCKS.concat([
+        { id: 'b-refl', label: 'reflector', kind: 'object', builtin: 'reflector@1', params: { focal: 0.125, aperture: 0.5, wall: 0.01, solid: 0, focusOffset: 0 } },
+        { id: 'b-asph', label: 'aspheric lens', kind: 'object', builtin: 'aspheric lens@1', params: { diameter: 1, thickness: 0.3, r1: 0.5, k1: -2.25,
+            a4_1: 0, a6_1: 0, a8_1: 0, a10_1: 0, r2: 0, k2: 0, a4_2: 0, a6_2: 0, a8_2: 0, a10_2: 0 } }])
+    const mirror = material('mirror', [0.95, 0.95, 0.95], 0.02, 1)
+    const glass = material('glass', [1, 1, 1], 0, 0, { transmission: 1, ior: 1.5, specular: 0.5 })
+    const d = studioDoc('reflector-condenser', [material('floor', [0.5, 0.5, 0.5], 0.8, 0, { specular: 0.5 }), mirror, glass], [
+        obj('floor', 'b-plane', 'floor', xf(0, 0, 0)),
+        obj('post', 'b-cylinder', 'floor', xf(-30, 0, 14), { radius: 4, height: 28 }),
+        obj('reflector', 'b-refl', 'mirror', xf(-30, 0, 37), { focal: 8, aperture: 25, wall: 1 }),
+        { id: 'lamp', kind: 'object', type: 'light', label: 'lamp', enabled: true, material: null, transform: xf(-30, 0, 37, 180, 0, 0),
+            params: { radius: 1.5 }, light: { shape: 'disc', colourBy: 'cct', cct: 4000, flux: 200 } },
+        obj('condenser', 'b-asph', 'glass', xf(35, 0, 15, 90, 0, 0, 30), {}),
+    ], blocks, { mode: 'orbit', camtype: 0, fov: 40, target: [0, 0, 25], azimuth: -90, elevation: 62, distance: 150, up: [0, 0, 1], aperture: 0 })
+    const w = opts.width || 96, h = opts.height || 64
+    return { doc: d, camera: cameraOf(d, w, h), settings: { width: w, height: h, spp: 64, maxbounce: 8, nee: 1 } }
+}
+
 const CONFIGS = { 'plane-c1': planeC1, 'room-c2': roomC2, 'caustic-c3': causticC3, 'furnace-plane': furnacePlane, 'furnace-haze': furnaceHaze,
-    'sppm-slab': sppmSlab, 'studio-floor': studioFloor, 'lens-prism': lensPrism }
+    'sppm-slab': sppmSlab, 'studio-floor': studioFloor, 'lens-prism': lensPrism, 'reflector-condenser': reflectorCondenser }
 
 module.exports = { CONFIGS, camera, rng, SEED }

@@ -25,8 +25,10 @@ const LUMA = [0.2126, 0.7152, 0.0722]
 
 // FP32 operations per distance-only primitive evaluation in its frame (an FMA counts 2; the transform in and out
 // counted apart), as the emitted code does them. Counted by the kernels per evaluated node: the march's roofline.
-// The prism's atan2f and sincospif are counted as 25 each (their polynomials and the division), an estimate.
-const PRIM_FLOPS = { plane: 1, sphere: 7, box: 27, cylinder: 24, capsule: 14, torus: 14, rectangle: 14, disc: 13, lens: 25, prism: 75 }
+// The library calls (the prism's atan2f and sincospif, the reflector's cbrtf or cosf and atan2f) are counted as 25 each
+// (their polynomials and the division), an estimate.
+const PRIM_FLOPS = { plane: 1, sphere: 7, box: 27, cylinder: 24, capsule: 14, torus: 14, rectangle: 14, disc: 13, lens: 25, prism: 75,
+    reflector: 80, 'aspheric lens': 52 }
 const XF_FLOPS = 22      // p - pos, three dot products, times 1/s; the distance times s
 const FOLD_FLOPS = 2
 const JOIN_FLOPS = 9
@@ -66,9 +68,44 @@ const SHAPES = {
         dist: o => `Dprism(q, ${[0, 1, 2, 3, 4].map(i => `U[${o + i}]`).join(', ')})`,
         full: o => `Nprism(q, ${[0, 1, 2, 3, 4].map(i => `U[${o + i}]`).join(', ')})`,
     },
+    // focal length, aperture radius, wall, solid (> 0.5), the focus ring's radius held at most the aperture
+    reflector: {
+        params: P => [P.focal, P.aperture, P.wall, P.solid, Math.min(P.focusOffset, Math.max(P.aperture, 0))],
+        dist: o => `Dparabolic(q, ${[0, 1, 2, 3, 4].map(i => `U[${o + i}]`).join(', ')})`,
+        full: o => `Nparabolic(q, ${[0, 1, 2, 3, 4].map(i => `U[${o + i}]`).join(', ')})`,
+    },
+    // rim radius, vertex half thickness, then per face c, k, a4..a10 (metres), 1 / sqrt(1 + m^2); the radii after them
+    'aspheric lens': { params: (P, unit) => asphereParams(P, unit).U, dist: o => `Dasphere(q, U + ${o})`, full: o => `Nasphere(q, U + ${o})` },
 }
-// the builtins' own defaults for parameters a document leaves out (builder/primitives.js:172 prism, :237 lens)
-const DEFAULTS = { lens: { diameter: 1.0, thickness: 0.2, r1: 2.0, r2: 2.0 }, prism: { sides: 3, radius: 0.5, height: 1.0 } }
+// the builtins' own defaults for parameters a document leaves out (builder/primitives.js:172 prism, :237 lens, :255
+// aspheric lens, :294 reflector)
+const DEFAULTS = { lens: { diameter: 1.0, thickness: 0.2, r1: 2.0, r2: 2.0 }, prism: { sides: 3, radius: 0.5, height: 1.0 },
+    reflector: { focal: 0.125, aperture: 0.5, wall: 0.01, solid: 0, focusOffset: 0 },
+    'aspheric lens': { diameter: 1.0, thickness: 0.3, r1: 0.5, k1: -2.25, a4_1: 0, a6_1: 0, a8_1: 0, a10_1: 0, r2: 0, k2: 0, a4_2: 0, a6_2: 0,
+        a8_2: 0, a10_2: 0 } }
+
+// An aspheric lens (shapes.js Nasphere:658, asphSag, asphSlopeMax): a radius under 1e-9 scene units is a flat face;
+// the a's go from the scene's unit to metres as length^(1 - 2i) (a4 a unit^-3, ...); m, a face's steepest slope
+// within the rim, is the conic's at the rim plus every polynomial term's size there. reach: how far past its vertex
+// a face's surface goes (a hollow face), for the bounding sphere.
+function asphereParams(P, unit) {
+    const rho = Math.max(0.5 * P.diameter, 1e-6), ht = 0.5 * P.thickness
+    const face = i => {
+        const R = P['r' + i], c = Math.abs(R) < 1e-9 * unit ? 0 : 1 / R, k = P['k' + i]
+        const a = [4, 6, 8, 10].map(e => P[`a${e}_${i}`] * Math.pow(unit, 1 - e))
+        const sag = r => {
+            const r2 = r * r, u = Math.sqrt(Math.max(1 - (1 + k) * c * c * r2, 1e-4))
+            return c * r2 / (1 + u) + r2 * r2 * (a[0] + r2 * (a[1] + r2 * (a[2] + r2 * a[3])))
+        }
+        const r2 = rho * rho, u = Math.sqrt(Math.max(1 - (1 + k) * c * c * r2, 1e-4))
+        const m = Math.abs(c) * rho / u + rho * r2 * (4 * Math.abs(a[0]) + r2 * (6 * Math.abs(a[1]) + r2 * (8 * Math.abs(a[2]) + r2 * 10 * Math.abs(a[3]))))
+        let lowest = 0
+        for (let j = 0; j <= 1024; j++) lowest = Math.min(lowest, sag(rho * j / 1024))
+        return { U: [c, k, ...a, 1 / Math.sqrt(1 + m * m)], R: c === 0 ? 0 : R, reach: -lowest }
+    }
+    const f1 = face(1), f2 = face(2)
+    return { U: [rho, ht, ...f1.U, ...f2.U, f1.R, f2.R], rho, reach: ht + 1.01 * Math.max(f1.reach, f2.reach) }
+}
 
 // A lens (sdf.js Nlens:463): a radius under 1e-9 scene units is a flat face; a hollow face (negative radius) is closed
 // by the plane through its rim, its sag |R| - sqrt(R^2 - rim^2) beyond the vertex.
@@ -89,7 +126,7 @@ function prismParams(P) {
 // the length-valued parameters of each shape (scaled by the unit); the rest are plain numbers
 const LENGTHS = { sphere: ['radius'], box: ['size', 'rounding'], cylinder: ['radius', 'height', 'rounding'], capsule: ['radius', 'height'],
     torus: ['ring', 'tube'], rectangle: ['width', 'height'], disc: ['radius'], plane: [], lens: ['diameter', 'thickness', 'r1', 'r2'],
-    prism: ['radius', 'height'] }
+    prism: ['radius', 'height'], reflector: ['focal', 'aperture', 'wall', 'focusOffset'], 'aspheric lens': ['diameter', 'thickness', 'r1', 'r2'] }
 
 // ---- transforms: world = parent * local; R = Rz Ry Rx (columns: local axes in the parent)
 function rotationOf(t) {
@@ -141,6 +178,12 @@ function localRadius(key, P, unit) {
         // plane for a hollow one
         case 'lens': { const L = lensParams(P, unit); return Math.hypot(L.rim, L.reach) }
         case 'prism': return Math.hypot(prismParams(P).R, 0.5 * Math.max(P.height, 0))
+        // the rim's cylinder from the vertex less the wall to the rim's height (a focus ring widens the bowl)
+        case 'reflector': {
+            const f = Math.max(P.focal, 1e-6), A = Math.max(P.aperture, 0), w = Math.max(P.wall, 0), c = Math.abs(Math.min(P.focusOffset, A))
+            return 1.001 * Math.hypot(A + w, Math.max(f + w, Math.abs((A + c) * (A + c) / (4 * f) - f) + w))
+        }
+        case 'aspheric lens': { const L = asphereParams(P, unit); return 1.001 * Math.hypot(L.rho, L.reach) }
     }
     return Infinity
 }

@@ -300,6 +300,130 @@ def test_lens_solid_and_surfaces(r1, r2, tmp_path):
     assert (np.linalg.norm(pts[d <= 0] - c, axis=1) <= r * (1 + 1e-6)).all()
 
 
+def _parabola_brute(x, y, k):
+    """Distance from (x, y) to y = k t^2 by dense sampling of t and a local golden-section polish (no cubic)."""
+    T = max(abs(x), math.sqrt(max(y / k, 0)), 1.0) * 2
+    t = np.linspace(-T, T, 200001)
+    d2 = (t - x) ** 2 + (k * t * t - y) ** 2
+    i = int(np.argmin(d2))
+    lo, hi = t[max(i - 1, 0)], t[min(i + 1, len(t) - 1)]
+    g = (math.sqrt(5) - 1) / 2
+    f = lambda s: (s - x) ** 2 + (k * s * s - y) ** 2   # noqa: E731
+    for _ in range(100):
+        a, b = hi - g * (hi - lo), lo + g * (hi - lo)
+        if f(a) < f(b):
+            hi = b
+        else:
+            lo = a
+    return math.sqrt(f(0.5 * (lo + hi)))
+
+
+def test_reflector_mirror_is_the_paraboloid_and_focuses(tmp_path):
+    # shapes.js NdishOffset: the mirror z = r^2 / 4f - f (focus at the origin), its wall behind it, cut at the aperture
+    f, A, w = 0.125, 0.5, 0.01
+    sc = _emit_doc(_single({'focal': f, 'aperture': A, 'wall': w}, 'reflector'), tmp_path)
+    k = 0.25 / float(np.float32(f))
+    # inside the bowl and under the wall, away from the rim: the value is the distance to the parabola (in the bowl)
+    # or to its offset (under it), against a brute-force nearest point
+    pts = np.c_[RNG.uniform(-0.3, 0.3, (300, 2)), RNG.uniform(-0.2, 0.3, 300)]
+    d, nrm, _ = oracle.scene_at(sc, pts)
+    for q, dq in zip(pts, d):
+        r, y = math.hypot(q[0], q[1]), q[2] + float(np.float32(f))
+        s = _parabola_brute(r, y, k) * (1 if k * r * r >= y else -1)
+        want = max(max(-s, s - float(np.float32(w))), r - float(np.float32(A)))
+        if r < 0.4:
+            assert abs(dq - want) < 1e-9, (q, dq, want)
+    # on the axis below the vertex the wall is w thick: inside at half of it, w/2 + 0.01 outside below
+    assert oracle.scene_at(sc, [[0, 0, -f - 0.5 * w]])[0][0] < 0
+    assert abs(oracle.scene_at(sc, [[0, 0, -f - w - 0.01]])[0][0] - 0.01) < 1e-7
+    # rays from the focus meet the mirror and leave parallel to the axis (the oracle's normals)
+    for th in np.radians([100, 120, 140, 160, 175]):
+        dr = np.array([math.sin(th), 0.0, math.cos(th)])
+        p, n = _trace(sc, np.zeros(3), dr)
+        out = dr - 2 * (dr @ n) * n
+        assert np.allclose(out, [0, 0, 1], atol=1e-6), (th, out)
+    assert _normals_are_gradients(sc, pts) > 0.97
+    c, r = _node_sphere(sc)
+    assert (np.linalg.norm(pts[d <= 0] - c, axis=1) <= r).all()
+    # solid: the paraboloid's inside capped at the rim's height, 0.5^2 / 0.5 - 0.125 = 0.375
+    sc = _emit_doc(_single({'focal': f, 'aperture': A, 'wall': w, 'solid': 1}, 'reflector'), tmp_path)
+    assert oracle.scene_at(sc, [[0, 0, 0.2]])[0][0] < 0 and abs(oracle.scene_at(sc, [[0, 0, 0.5]])[0][0] - 0.125) < 1e-7
+
+
+@pytest.mark.parametrize('offset', [0.05, -0.05])
+def test_reflector_focus_ring(offset, tmp_path):
+    # shapes.js g2ParabolaOffset: each meridian half focused on the ring of radius |offset| in z = 0 (across the axis
+    # when negative): a ray from that half's focus leaves parallel to the axis
+    f = 0.125
+    sc = _emit_doc(_single({'focal': f, 'aperture': 0.5, 'wall': 0.01, 'focusOffset': offset}, 'reflector'), tmp_path)
+    for th in np.radians([110, 130, 150]):
+        dr = np.array([math.sin(th), 0.0, math.cos(th)])
+        o = np.array([offset, 0.0, 0.0])
+        p, n = _trace(sc, o, dr)
+        assert p[0] > 0
+        out = dr - 2 * (dr @ n) * n
+        assert np.allclose(out, [0, 0, 1], atol=1e-6), (th, out)
+
+
+def test_aspheric_condenser_makes_a_parallel_beam(tmp_path):
+    # shapes.js Nasphere at the builtin's defaults: a hyperboloid of k = -n^2, r = f (n - 1) over a flat face; in glass
+    # of n 1.5 a light on the axis 1 beyond the curved face's vertex leaves through the flat face as a parallel beam,
+    # at every height (the hyperboloid's exact focus, not a paraxial one)
+    sc = _emit_doc(_single({}, 'aspheric lens'), tmp_path)
+    src = np.array([0, 0, 0.15 + 1.0])
+    for th in np.radians([1, 5, 10, 13]):
+        dr = np.array([math.sin(th), 0.0, -math.cos(th)])
+        p, n = _trace(sc, src, dr)
+        inside = _refract(dr, n, 1 / 1.5)
+        assert np.allclose(inside, [0, 0, -1], atol=2e-6), (th, inside)
+
+
+@pytest.mark.parametrize('R,k,a4', [(2.0, 0.0, 0.0), (-1.5, 0.0, 0.0), (0.8, -1.0, 0.05), (1.2, 0.5, -0.02)])
+def test_aspheric_lens_surfaces_and_bound(R, k, a4, tmp_path):
+    # a k = 0, a = 0 face is the spherical lens's face: points of the sphere are on it; the polynomial moves the
+    # surface by a4 r^4; outside, the value never exceeds the distance to surface points found by bisection
+    D, T = 1.0, 0.3
+    sc = _emit_doc(_single({'diameter': D, 'thickness': T, 'r1': R, 'k1': k, 'a4_1': a4, 'r2': 0, 'k2': 0}, 'aspheric lens'), tmp_path)
+    f32 = lambda x: float(np.float32(x))   # noqa: E731
+    ht, Rf, kf, af = f32(0.5 * T), f32(R), f32(k), f32(a4)
+    for r in (0.0, 0.2, 0.45):
+        c = 1 / Rf
+        sag = c * r * r / (1 + math.sqrt(1 - (1 + kf) * c * c * r * r)) + af * r ** 4
+        assert abs(oracle.scene_at(sc, [[r, 0, ht - sag]])[0][0]) < 1e-12
+        assert abs(oracle.scene_at(sc, [[0, r, -ht]])[0][0]) < 1e-12      # the flat back
+    pts = RNG.uniform(-0.8, 0.8, (2000, 3))
+    d, _, _ = oracle.scene_at(sc, pts)
+    rr, ph = np.meshgrid(np.linspace(0, 0.5, 100), np.linspace(0, 2 * np.pi, 72, endpoint=False))
+    xy = np.stack([rr.ravel() * np.cos(ph.ravel()), rr.ravel() * np.sin(ph.ravel())], 1)
+    xy = xy[oracle.scene_at(sc, np.c_[xy, 0 * xy[:, 0]])[0] < 0]
+    surf = []
+    for sgn in (1, -1):
+        lo, hi = np.zeros(len(xy)), np.full(len(xy), sgn * 0.8)
+        for _ in range(45):
+            mid = 0.5 * (lo + hi)
+            neg = oracle.scene_at(sc, np.c_[xy, mid])[0] < 0
+            lo, hi = np.where(neg, mid, lo), np.where(neg, hi, mid)
+        surf.append(np.c_[xy, lo])
+    surf = np.concatenate(surf)
+    far = d > 0.02
+    near = np.array([np.linalg.norm(surf - q, axis=1).min() for q in pts[far]])
+    assert (d[far] <= near + 1e-9).all()
+    assert _normals_are_gradients(sc, pts[np.abs(d) < 0.3]) > 0.9
+    c, r = _node_sphere(sc)
+    assert (np.linalg.norm(pts[d <= 0] - c, axis=1) <= r).all()
+
+
+def test_aspheric_terms_follow_the_scene_unit(tmp_path):
+    # a4 is in the scene's unit to the -3 (a length to the 1 - 4): the same lens written in millimetres, its a4 a
+    # billionth of the metre one's, is the same solid
+    m = {'diameter': 1, 'thickness': 0.3, 'r1': 0.8, 'k1': -1, 'a4_1': 0.05, 'r2': -3, 'k2': 0.2, 'a4_2': 0.01}
+    mm = dict(m, diameter=1000, thickness=300, r1=800, r2=-3000, a4_1=0.05e-9, a4_2=0.01e-9)
+    a = _emit_doc(_single(m, 'aspheric lens'), tmp_path)
+    b = _emit_doc(_single(mm, 'aspheric lens', unit='mm'), tmp_path)
+    pts = RNG.uniform(-0.6, 0.6, (300, 3))
+    np.testing.assert_allclose(oracle.scene_at(b, pts)[0], oracle.scene_at(a, pts)[0], rtol=1e-5, atol=1e-7)
+
+
 def test_studio_floor_reads_rho_E_over_pi():
     # environment.js studioLight: L0 (1 + top d.z) with L0 = E / (pi (1 + 2 top / 3)) lights a level surface with E
     # whatever top is, so a Lambertian floor of albedo 0.6 under 500 lx reads 0.6 x 500 / pi; the rays that leave
