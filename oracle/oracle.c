@@ -79,7 +79,8 @@ static void sampleSeed(uint32_t base, uint32_t sample, uint32_t* out) {
 
 /* ---- the scene program (hipgen.js `program`) ---- */
 enum { OP_PRIM = 0, OP_JOIN = 1, OP_FOLD = 2, OP_NODE = 3 };
-enum { SH_PLANE = 0, SH_SPHERE, SH_BOX, SH_CYLINDER, SH_CAPSULE, SH_TORUS, SH_RECT, SH_DISC, SH_LENS, SH_PRISM, SH_REFLECTOR, SH_ASPHERE };
+enum { SH_PLANE = 0, SH_SPHERE, SH_BOX, SH_CYLINDER, SH_CAPSULE, SH_TORUS, SH_RECT, SH_DISC, SH_LENS, SH_PRISM, SH_REFLECTOR, SH_ASPHERE,
+       SH_REVOLUTION, SH_FRESNEL, SH_EXTRUSION };
 enum { J_UNION = 0, J_SUBTRACT, J_INTERSECT };
 /* prim: shape, xf, p, mat, prim; join: join, k; node (a top-level node's start): xf = its bounding sphere's offset in
  * U (-1: unbounded), join = its mask bit (-1: none), k = the index of its fold */
@@ -156,6 +157,51 @@ static d4 asphFaceD(d3 p, double l, double rx, double ry, double rho, double ht,
     double m = fabs(c) * rho / u + rho * r2 * (4 * fabs(a[0]) + r2 * (6 * fabs(a[1]) + r2 * (8 * fabs(a[2]) + r2 * 10 * fabs(a[3]))));
     double g = l < rho ? slope : 0, in = 1 / sqrt(g * g + 1);
     return d4of(v(rx * g * in, ry * g * in, s * in), (s * p.z - ht + sag) / sqrt(1 + m * m));
+}
+
+/* An outline walked piece by piece (core/shapes.js pfStart..pfEnd:684-715): the nearest point of its surface pieces
+ * and inside by the even-odd rule over the edges and arcs' chords. U: count, then 9 numbers a piece -- kind 0 a
+ * segment (a, b, surface) or 1 an arc (centre, radius, a, b, convex). Returns (normal, signed distance) in the plane. */
+static g2v dir2(double x, double y, double fx, double fy) { g2v r; double l = sqrt(x * x + y * y); if (l > 1e-12) { r.x = x / l; r.y = y / l; } else { r.x = fx; r.y = fy; } r.d = 0; return r; }
+static int cross2q(double qx, double qy, double ax, double ay, double bx, double by) {
+    return (ay > qy) != (by > qy) && qx < ax + (qy - ay) * (bx - ax) / (by - ay);
+}
+static g2v outlineAt(const float* P, double qx, double qy) {
+    int n = (int)P[0];
+    double bvx = 0, bvy = 0, bd2 = 3e38, nx = 1, ny = 0;
+    int inside = 0;
+    for (int i = 0; i < n; ++i) {
+        const float* k = P + 1 + 9 * i;
+        double px, py, nnx, nny;
+        if (k[0] == 0) {
+            double ax = k[1], ay = k[2], bx = k[3], by = k[4];
+            if (cross2q(qx, qy, ax, ay, bx, by)) inside = !inside;
+            if (k[5] == 0) continue;
+            double ex = bx - ax, ey = by - ay, t = clampd(((qx - ax) * ex + (qy - ay) * ey) / maxd(ex * ex + ey * ey, 1e-30), 0, 1);
+            px = ax + ex * t; py = ay + ey * t;
+            g2v nn = dir2(ey, -ex, 1, 0); nnx = nn.x; nny = nn.y;
+        } else {
+            double cx = k[1], cy = k[2], r = k[3], ax = k[4], ay = k[5], bx = k[6], by = k[7];
+            double wx = qx - cx, wy = qy - cy, mx = 0.5 * (ax + bx) - cx, my = 0.5 * (ay + by) - cy;
+            int segIn = wx * wx + wy * wy < r * r && wx * mx + wy * my > mx * mx + my * my;
+            if (cross2q(qx, qy, ax, ay, bx, by) != segIn) inside = !inside;
+            double eax = ax - cx, eay = ay - cy, ebx = bx - cx, eby = by - cy, s = eax * eby - eay * ebx >= 0 ? 1 : -1;
+            if (s * (eax * wy - eay * wx) >= 0 && s * (wx * eby - wy * ebx) >= 0) {
+                g2v md = dir2(mx, my, 1, 0), d = dir2(wx, wy, md.x, md.y);
+                px = cx + d.x * r; py = cy + d.y * r;
+            } else if ((qx - ax) * (qx - ax) + (qy - ay) * (qy - ay) <= (qx - bx) * (qx - bx) + (qy - by) * (qy - by)) { px = ax; py = ay; }
+            else { px = bx; py = by; }
+            g2v nn = dir2(px - cx, py - cy, 1, 0);
+            nnx = k[8] != 0 ? nn.x : -nn.x; nny = k[8] != 0 ? nn.y : -nn.y;
+        }
+        double vx = qx - px, vy = qy - py, d2 = vx * vx + vy * vy;
+        if (d2 < bd2) { bvx = vx; bvy = vy; bd2 = d2; nx = nnx; ny = nny; }
+    }
+    double d = sqrt(bd2), s = inside ? -1 : 1;
+    g2v out;
+    if (d > 1e-12) { out.x = bvx * s / d; out.y = bvy * s / d; } else { out.x = nx; out.y = ny; }
+    out.d = s * d;
+    return out;
 }
 
 /* Primitive (outward normal, distance) in its own frame: core/sdf.js:352-628 restated; the lens core/sdf.js:463, the
@@ -254,6 +300,18 @@ static d4 primLocal(int shape, d3 p, const float* P) {
         }
         double rx = x > 1e-12 ? p.x / l : 1, ry = x > 1e-12 ? p.y / l : 0;
         return d4of(v(rx * r.x, ry * r.x, r.y), r.d);
+    }
+    case SH_REVOLUTION: case SH_FRESNEL: case SH_EXTRUSION: {   /* P: mode (0 revolve, 1 extrude), half height, the outline */
+        if (P[0] == 0) {
+            double l = sqrt(p.x * p.x + p.y * p.y), rx = l > 1e-12 ? p.x / l : 1, ry = l > 1e-12 ? p.y / l : 0;
+            g2v g = outlineAt(P + 2, l, p.z);
+            return d4of(v(rx * g.x, ry * g.x, g.y), g.d);
+        }
+        g2v g = outlineAt(P + 2, p.x, p.y);
+        double h = P[1], wz = fabs(p.z) - h, sz = p.z >= 0 ? 1 : -1;
+        if (g.d > 0 && wz > 0) { double l = sqrt(g.d * g.d + wz * wz); return d4of(v(g.x * g.d / l, g.y * g.d / l, sz * wz / l), l); }
+        if (g.d > wz) return d4of(v(g.x, g.y, 0), g.d);
+        return d4of(v(0, 0, sz), wz);
     }
     case SH_ASPHERE: {      /* P: rho, ht, 7 per face (the emitter's), then the faces' radii R1, R2 */
         double rho = P[0], ht = P[1], l = sqrt(p.x * p.x + p.y * p.y);

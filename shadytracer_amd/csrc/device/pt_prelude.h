@@ -316,6 +316,70 @@ __device__ __forceinline__ V4 g2revolve(V3 p, G2 g) {
     if (l > 1e-12f) { float il = frcp(l); rx = p.x * il; ry = p.y * il; }
     return v4(v3(rx * g.x, ry * g.x, g.y), g.d);
 }
+// Extruded along z to half height h, exactly (shapes.js g2Extrude:717): past both the outline and the caps, the
+// corner's distance; else the larger of the two.
+__device__ __forceinline__ V4 g2extrude(V3 p, G2 g, float h) {
+    const float wz = fabsf(p.z) - h, sz = p.z >= 0.0f ? 1.0f : -1.0f;
+    if (g.d > 0.0f && wz > 0.0f) {
+        const float l = fsqrt(g.d * g.d + wz * wz), il = frcp(l);
+        return v4(v3(g.x * g.d * il, g.y * g.d * il, sz * wz * il), l);
+    }
+    if (g.d > wz) return v4(v3(g.x, g.y, 0.0f), g.d);
+    return v4(v3(0.0f, 0.0f, sz), wz);
+}
+// An outline of segments and fillet arcs (counter-clockwise), walked piece by piece (shapes.js pfStart..pfEnd:684-715):
+// the nearest point so far (v = q minus it, d2 its squared distance, n the piece's outward normal for q on it), and
+// inside by the even-odd rule over the edges and each arc's chord, flipped within the arc's circular segment. A
+// segment that is no surface (a revolved outline's axis edge) still counts for inside.
+struct Prof { float vx, vy, d2, nx, ny; bool inside; };
+__device__ __forceinline__ Prof pfStart() { Prof A; A.vx = A.vy = 0.0f; A.d2 = 3.0e38f; A.nx = 1.0f; A.ny = 0.0f; A.inside = false; return A; }
+__device__ __forceinline__ bool pfCross(float qx, float qy, float ax, float ay, float bx, float by) {
+    return (ay > qy) != (by > qy) && qx < ax + (qy - ay) * (bx - ax) / (by - ay);
+}
+__device__ __forceinline__ void pfDir(float x, float y, float fx, float fy, float& ox, float& oy) {
+    const float l = fsqrt(x * x + y * y);
+    if (l > 1e-12f) { const float il = frcp(l); ox = x * il; oy = y * il; } else { ox = fx; oy = fy; }
+}
+__device__ __forceinline__ void pfSegment(Prof& A, float qx, float qy, float ax, float ay, float bx, float by, bool surface) {
+    if (pfCross(qx, qy, ax, ay, bx, by)) A.inside = !A.inside;
+    if (!surface) return;
+    const float ex = bx - ax, ey = by - ay;
+    const float t = fminf(fmaxf(((qx - ax) * ex + (qy - ay) * ey) / fmaxf(ex * ex + ey * ey, 1e-30f), 0.0f), 1.0f);
+    const float vx = qx - (ax + ex * t), vy = qy - (ay + ey * t), d2 = vx * vx + vy * vy;
+    if (d2 < A.d2) { A.vx = vx; A.vy = vy; A.d2 = d2; pfDir(ey, -ex, 1.0f, 0.0f, A.nx, A.ny); }
+}
+// a fillet: the minor arc from a to b about c, radius r; convex: the shape lies on c's side of it
+__device__ __forceinline__ void pfArc(Prof& A, float qx, float qy, float cx, float cy, float r, float ax, float ay, float bx, float by, bool convex) {
+    const float wx = qx - cx, wy = qy - cy;
+    const float mx = 0.5f * (ax + bx) - cx, my = 0.5f * (ay + by) - cy;
+    const bool seg = wx * wx + wy * wy < r * r && wx * mx + wy * my > mx * mx + my * my;
+    if (pfCross(qx, qy, ax, ay, bx, by) != seg) A.inside = !A.inside;
+    const float eax = ax - cx, eay = ay - cy, ebx = bx - cx, eby = by - cy;
+    const float s = eax * eby - eay * ebx >= 0.0f ? 1.0f : -1.0f;
+    const bool wedge = s * (eax * wy - eay * wx) >= 0.0f && s * (wx * eby - wy * ebx) >= 0.0f;
+    float px, py;
+    if (wedge) {
+        float mdx, mdy, dx, dy;
+        pfDir(mx, my, 1.0f, 0.0f, mdx, mdy);
+        pfDir(wx, wy, mdx, mdy, dx, dy);
+        px = cx + dx * r; py = cy + dy * r;
+    } else {
+        const bool toA = (qx - ax) * (qx - ax) + (qy - ay) * (qy - ay) <= (qx - bx) * (qx - bx) + (qy - by) * (qy - by);
+        px = toA ? ax : bx; py = toA ? ay : by;
+    }
+    const float vx = qx - px, vy = qy - py, d2 = vx * vx + vy * vy;
+    if (d2 < A.d2) {
+        A.vx = vx; A.vy = vy; A.d2 = d2;
+        float nx, ny;
+        pfDir(px - cx, py - cy, 1.0f, 0.0f, nx, ny);
+        A.nx = convex ? nx : -nx; A.ny = convex ? ny : -ny;
+    }
+}
+__device__ __forceinline__ G2 pfEnd(const Prof& A) {
+    const float d = fsqrt(A.d2), s = A.inside ? -1.0f : 1.0f;
+    if (d > 1e-12f) { const float k = s / fmaxf(d, 1e-30f); return g2(A.vx * k, A.vy * k, s * d); }
+    return g2(A.nx, A.ny, s * d);
+}
 // The signed distance in a meridian plane from (x, y) to the parabola y = k x^2 (k > 0), positive below it, with its
 // outward normal (shapes.js g2Parabola:61). The nearest point's abscissa t solves the depressed cubic
 // 2k^2 t^3 + (1 - 2ky) t - x = 0: with a = (2ky - 1) / 6k^2 and b = x / 4k^2 its discriminant is b^2 - a^3; above

@@ -18,9 +18,9 @@
  *   sppm-slab    a collimated beam through a glass slab onto a floor: the photon map's closed form.
  *   furnace-*    closed-form checks: a white furnace (albedo-1 Lambertian plane under a uniform sky, and a haze box
  *                under a uniform sphere of sky).
- *   studio-floor, lens-prism, reflector-condenser
+ *   studio-floor, lens-prism, reflector-condenser, turned-parts
  *                documents in the builder's own form (millimetres, the studio environment, lights by CCT and flux,
- *                an orbit view), the others with the lens, prism, reflector and aspheric lens builtins.
+ *                an orbit view), the others with the lens, prism, reflector, aspheric lens and outline builtins.
  */
 
 const SEED = 20261203
@@ -272,7 +272,36 @@ function reflectorCondenser(opts = {}) {
     return { doc: d, camera: cameraOf(d, w, h), settings: { width: w, height: h, spp: 64, maxbounce: 8, nee: 1 } }
 }
 
+// turned-parts: outlines -- a Fresnel lens (six facet rings over a flat back, 40 mm across, glass) held up on a post, a
+// knob turned from a filleted outline and a notched plate extruded from one, on a grey floor under a 3000 K panel
+// and the studio.
+function turnedParts(opts = {}) {
+    const { cameraOf } = require('./hipgen')
+    const fresnel = [[0, 0], [0.5, 0], [0.5, 0.025], [0.4167, 0.0918], [0.4167, 0.025], [0.3333, 0.0746], [0.3333, 0.025], [0.25, 0.0611],
+        [0.25, 0.025], [0.1667, 0.0497], [0.1667, 0.025], [0.0833, 0.0394], [0.0833, 0.025], [0, 0.0297]]
+    const knob = [[0, -0.42], [0.5, -0.42, 0.08], [0.5, 0.08], [0.21, 0.25, 0.125], [0.33, 0.5, 0.08], [0, 0.5]]
+    const plate = [[-0.5, -0.35, 0.08], [0.5, -0.35, 0.08], [0.5, 0.35, 0.08], [0.125, 0.35, 0.035], [0, 0.125, 0.05], [-0.125, 0.35, 0.035], [-0.5, 0.35, 0.08]]
+    const blocks = BLOCKS.concat([
+        { id: 'b-fresnel', label: 'fresnel lens', kind: 'object', builtin: 'fresnel lens@1', params: { outline: fresnel } },
+        { id: 'b-turn', label: 'revolution', kind: 'object', builtin: 'revolution@1', params: { outline: knob } },
+        { id: 'b-extr', label: 'extrusion', kind: 'object', builtin: 'extrusion@1', params: { outline: plate, height: 0.2 } }])
+    const glass = material('glass', [1, 1, 1], 0, 0, { transmission: 1, ior: 1.49, specular: 0.5 })
+    const d = studioDoc('turned-parts', [material('floor', [0.5, 0.5, 0.5], 0.8, 0, { specular: 0.5 }), glass,
+        material('brass', [0.9, 0.7, 0.35], 0.3, 1), material('paint', [0.2, 0.35, 0.7], 0.5, 0)], [
+        obj('floor', 'b-plane', 'floor', xf(0, 0, 0)),
+        obj('post', 'b-cylinder', 'paint', xf(-35, 10, 15, 0, 0, 0), { radius: 3, height: 30 }),
+        obj('fresnel', 'b-fresnel', 'glass', xf(-35, 10, 30, 0, 0, 0, 40), {}),
+        obj('knob', 'b-turn', 'brass', xf(5, 15, 17, 0, 0, 0, 40), {}),
+        obj('plate', 'b-extr', 'paint', xf(40, -5, 4, 0, 0, 25, 40), { height: 0.2 }),
+        { id: 'panel', kind: 'object', type: 'light', label: 'panel', enabled: true, material: null, transform: xf(0, 60, 120, 150, 0, 0),
+            params: { width: 40, height: 40 }, light: { shape: 'rectangle', colourBy: 'cct', cct: 3000, flux: 300 } },
+    ], blocks, { mode: 'orbit', camtype: 0, fov: 40, target: [0, 5, 15], azimuth: -95, elevation: 62, distance: 170, up: [0, 0, 1], aperture: 0 })
+    const w = opts.width || 96, h = opts.height || 64
+    return { doc: d, camera: cameraOf(d, w, h), settings: { width: w, height: h, spp: 64, maxbounce: 8, nee: 1 } }
+}
+
 const CONFIGS = { 'plane-c1': planeC1, 'room-c2': roomC2, 'caustic-c3': causticC3, 'furnace-plane': furnacePlane, 'furnace-haze': furnaceHaze,
-    'sppm-slab': sppmSlab, 'studio-floor': studioFloor, 'lens-prism': lensPrism, 'reflector-condenser': reflectorCondenser }
+    'sppm-slab': sppmSlab, 'studio-floor': studioFloor, 'lens-prism': lensPrism, 'reflector-condenser': reflectorCondenser,
+    'turned-parts': turnedParts }
 
 module.exports = { CONFIGS, camera, rng, SEED }

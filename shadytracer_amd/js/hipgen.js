@@ -28,7 +28,7 @@ const LUMA = [0.2126, 0.7152, 0.0722]
 // The library calls (the prism's atan2f and sincospif, the reflector's cbrtf or cosf and atan2f) are counted as 25 each
 // (their polynomials and the division), an estimate.
 const PRIM_FLOPS = { plane: 1, sphere: 7, box: 27, cylinder: 24, capsule: 14, torus: 14, rectangle: 14, disc: 13, lens: 25, prism: 75,
-    reflector: 80, 'aspheric lens': 52 }
+    reflector: 80, 'aspheric lens': 52, revolution: 0, 'fresnel lens': 0, extrusion: 0 }
 const XF_FLOPS = 22      // p - pos, three dot products, times 1/s; the distance times s
 const FOLD_FLOPS = 2
 const JOIN_FLOPS = 9
@@ -76,6 +76,11 @@ const SHAPES = {
     },
     // rim radius, vertex half thickness, then per face c, k, a4..a10 (metres), 1 / sqrt(1 + m^2); the radii after them
     'aspheric lens': { params: (P, unit) => asphereParams(P, unit).U, dist: o => `Dasphere(q, U + ${o})`, full: o => `Nasphere(q, U + ${o})` },
+    // an outline turned about z, or extruded along z: U holds the mode, half the height, the count of pieces and 9
+    // numbers a piece (outlinePieces); the node's own function, written by the emitter, walks them (profileCode)
+    revolution: { profile: 'revolve', params: (P, unit) => profileParams(P, unit, 'revolve').U, dist: o => `Dprofile${o}(U, q)`, full: o => `Nprofile${o}(U, q)` },
+    'fresnel lens': { profile: 'revolve', params: (P, unit) => profileParams(P, unit, 'revolve').U, dist: o => `Dprofile${o}(U, q)`, full: o => `Nprofile${o}(U, q)` },
+    extrusion: { profile: 'extrude', params: (P, unit) => profileParams(P, unit, 'extrude').U, dist: o => `Dprofile${o}(U, q)`, full: o => `Nprofile${o}(U, q)` },
 }
 // the builtins' own defaults for parameters a document leaves out (builder/primitives.js:172 prism, :237 lens, :255
 // aspheric lens, :294 reflector)
@@ -83,6 +88,103 @@ const DEFAULTS = { lens: { diameter: 1.0, thickness: 0.2, r1: 2.0, r2: 2.0 }, pr
     reflector: { focal: 0.125, aperture: 0.5, wall: 0.01, solid: 0, focusOffset: 0 },
     'aspheric lens': { diameter: 1.0, thickness: 0.3, r1: 0.5, k1: -2.25, a4_1: 0, a6_1: 0, a8_1: 0, a10_1: 0, r2: 0, k2: 0, a4_2: 0, a6_2: 0,
         a8_2: 0, a10_2: 0 } }
+
+// ---- outlines (shapes.js profilePieces:1487): points (x, y, and a fillet radius if rounded), closed back to the first,
+// as the pieces a profile function walks -- segments, and arcs where a corner is rounded -- counter-clockwise, each
+// segment lying on x = 0 marked as the axis (a revolved outline's axis edge is no surface). An outline that is not
+// sound (fewer than three points, a repeated neighbour, no area, crossing edges, fillets that do not fit their edges)
+// is refused, naming the points as written.
+const sub2 = (a, b) => [a[0] - b[0], a[1] - b[1]]
+const cross2 = (a, b) => a[0] * b[1] - a[1] * b[0]
+const unit2 = v => { const l = Math.hypot(v[0], v[1]); return [v[0] / l, v[1] / l] }
+function segmentsMeet(a, b, c, d) {
+    const o = (p, q, r) => Math.sign(cross2(sub2(q, p), sub2(r, p)))
+    const within = (p, q, r) => Math.min(p[0], q[0]) <= r[0] && r[0] <= Math.max(p[0], q[0]) && Math.min(p[1], q[1]) <= r[1] && r[1] <= Math.max(p[1], q[1])
+    const o1 = o(a, b, c), o2 = o(a, b, d), o3 = o(c, d, a), o4 = o(c, d, b)
+    if (o1 !== o2 && o3 !== o4) return true
+    return (o1 === 0 && within(a, b, c)) || (o2 === 0 && within(a, b, d)) || (o3 === 0 && within(c, d, a)) || (o4 === 0 && within(c, d, b))
+}
+function outlinePieces(points, where) {
+    const bad = why => { throw new Error(`hipgen: ${where}: ${why}`) }
+    if (!Array.isArray(points) || points.length < 3) bad('an outline needs at least three points')
+    points.forEach((pt, i) => {
+        if (!Array.isArray(pt) || (pt.length !== 2 && pt.length !== 3) || !pt.every(Number.isFinite)) bad(`point ${i + 1}: x, y and a fillet radius if it has one`)
+        if (pt.length === 3 && pt[2] < 0) bad(`point ${i + 1}: a negative fillet radius`)
+    })
+    const n = points.length, at = i => ((i % n) + n) % n
+    for (let i = 0; i < n; i++) {
+        const a = points[i], b = points[at(i + 1)]
+        if (a[0] === b[0] && a[1] === b[1]) bad(`points ${i + 1} and ${at(i + 1) + 1} are the same point`)
+    }
+    let area = 0
+    for (let i = 0; i < n; i++) area += 0.5 * cross2(points[i], points[at(i + 1)])
+    const size = Math.max(...points.map(p => Math.max(Math.abs(p[0]), Math.abs(p[1]))), 1e-30)
+    if (!(Math.abs(area) > 1e-12 * size * size)) bad('the outline encloses no area')
+    for (let i = 0; i < n; i++) for (let j = i + 2; j < n; j++) {
+        if (i === 0 && j === n - 1) continue
+        if (segmentsMeet(points[i], points[at(i + 1)], points[j], points[at(j + 1)])) bad(`edges ${i + 1}-${at(i + 1) + 1} and ${j + 1}-${at(j + 1) + 1} cross`)
+    }
+    const order = [...Array(n).keys()]
+    if (area < 0) order.reverse()
+    const Pts = order.map(i => points[i])
+    const corner = Pts.map((p, i) => {
+        const r = p[2] || 0
+        if (!(r > 0)) return null
+        const prev = Pts[at(i - 1)], next = Pts[at(i + 1)]
+        const u = unit2(sub2(prev, p)), v = unit2(sub2(next, p))
+        const ang = Math.acos(Math.min(1, Math.max(-1, u[0] * v[0] + u[1] * v[1])))
+        if (ang > Math.PI - 1e-9) return null                      // a straight run: nothing to round
+        const t = r / Math.tan(0.5 * ang), bis = unit2([u[0] + v[0], u[1] + v[1]]), k = r / Math.sin(0.5 * ang)
+        return { t, r, A: [p[0] + u[0] * t, p[1] + u[1] * t], B: [p[0] + v[0] * t, p[1] + v[1] * t], c: [p[0] + bis[0] * k, p[1] + bis[1] * k],
+            convex: cross2(sub2(p, prev), sub2(next, p)) > 0 }
+    })
+    for (let i = 0; i < n; i++) {
+        const j = at(i + 1), need = (corner[i] ? corner[i].t : 0) + (corner[j] ? corner[j].t : 0)
+        if (need > Math.hypot(Pts[j][0] - Pts[i][0], Pts[j][1] - Pts[i][1]) * (1 + 1e-9)) bad(`the fillets at points ${order[i] + 1} and ${order[j] + 1} do not fit their edge`)
+    }
+    const pieces = []
+    for (let i = 0; i < n; i++) {
+        const k = corner[i], j = at(i + 1)
+        if (k) pieces.push({ kind: 'arc', c: k.c, r: k.r, a: k.A, b: k.B, convex: k.convex })
+        const a = k ? k.B : [Pts[i][0], Pts[i][1]], b = corner[j] ? corner[j].A : [Pts[j][0], Pts[j][1]]
+        if (Math.hypot(b[0] - a[0], b[1] - a[1]) > 1e-12 * size)
+            pieces.push({ kind: 'segment', a, b, axis: Math.abs(a[0]) <= 1e-12 * size && Math.abs(b[0]) <= 1e-12 * size })
+    }
+    return pieces
+}
+// U for an outline node: mode (0 revolve, 1 extrude), half the height, the count of pieces, then 9 numbers a piece:
+// kind (0 segment, 1 arc), a segment's a, b and whether it is a surface, or an arc's centre, radius, a, b and
+// whether it is convex. Lengths in metres. reach: the outline's farthest point from its origin (bounding sphere).
+function profileParams(P, unit, mode) {
+    const pts = (P.outline || []).map(p => p.map(v => v * unit))
+    const pieces = outlinePieces(pts, 'an outline')
+    const U = [mode === 'extrude' ? 1 : 0, mode === 'extrude' ? Math.max(0.5 * (P.height || 0), 0) : 0, pieces.length]
+    for (const k of pieces) {
+        if (k.kind === 'segment') U.push(0, k.a[0], k.a[1], k.b[0], k.b[1], k.axis && mode === 'revolve' ? 0 : 1, 0, 0, 0)
+        else U.push(1, k.c[0], k.c[1], k.r, k.a[0], k.a[1], k.b[0], k.b[1], k.convex ? 1 : 0)
+    }
+    const reach = Math.max(...pts.map(p => Math.hypot(p[0], p[1]))) * 1.001
+    return { U, pieces, reach, mode }
+}
+// The node's profile function (straight-line code, a call a piece: the kinds are the outline's shape, the numbers read
+// from U) and its solid: revolved (q = (|p.xy|, z), the axis edges no surface) or extruded.
+function profileCode(o, pr) {
+    const u = i => `U[${o + i}]`
+    const lines = [`__device__ __forceinline__ G2 profile${o}(ParamPtr U, float qx, float qy) {`, '    PT_SCENE_PARAMS(U);', '    Prof A = pfStart();']
+    pr.pieces.forEach((k, i) => {
+        const b = 3 + 9 * i
+        if (k.kind === 'segment') lines.push(`    pfSegment(A, qx, qy, ${u(b + 1)}, ${u(b + 2)}, ${u(b + 3)}, ${u(b + 4)}, ${k.axis && pr.mode === 'revolve' ? 'false' : 'true'});`)
+        else lines.push(`    pfArc(A, qx, qy, ${u(b + 1)}, ${u(b + 2)}, ${u(b + 3)}, ${u(b + 4)}, ${u(b + 5)}, ${u(b + 6)}, ${u(b + 7)}, ${k.convex ? 'true' : 'false'});`)
+    })
+    lines.push('    return pfEnd(A);', '}')
+    if (pr.mode === 'revolve') {
+        lines.push(`__device__ __forceinline__ V4 Nprofile${o}(ParamPtr U, V3 p) { return g2revolve(p, profile${o}(U, fsqrt(p.x * p.x + p.y * p.y), p.z)); }`)
+    } else {
+        lines.push(`__device__ __forceinline__ V4 Nprofile${o}(ParamPtr U, V3 p) { PT_SCENE_PARAMS(U); return g2extrude(p, profile${o}(U, p.x, p.y), ${u(1)}); }`)
+    }
+    lines.push(`__device__ __forceinline__ float Dprofile${o}(ParamPtr U, V3 p) { return Nprofile${o}(U, p).w; }`)
+    return lines.join('\n')
+}
 
 // An aspheric lens (shapes.js Nasphere:658, asphSag, asphSlopeMax): a radius under 1e-9 scene units is a flat face;
 // the a's go from the scene's unit to metres as length^(1 - 2i) (a4 a unit^-3, ...); m, a face's steepest slope
@@ -126,7 +228,8 @@ function prismParams(P) {
 // the length-valued parameters of each shape (scaled by the unit); the rest are plain numbers
 const LENGTHS = { sphere: ['radius'], box: ['size', 'rounding'], cylinder: ['radius', 'height', 'rounding'], capsule: ['radius', 'height'],
     torus: ['ring', 'tube'], rectangle: ['width', 'height'], disc: ['radius'], plane: [], lens: ['diameter', 'thickness', 'r1', 'r2'],
-    prism: ['radius', 'height'], reflector: ['focal', 'aperture', 'wall', 'focusOffset'], 'aspheric lens': ['diameter', 'thickness', 'r1', 'r2'] }
+    prism: ['radius', 'height'], reflector: ['focal', 'aperture', 'wall', 'focusOffset'], 'aspheric lens': ['diameter', 'thickness', 'r1', 'r2'],
+    revolution: [], 'fresnel lens': [], extrusion: ['height'] }
 
 // ---- transforms: world = parent * local; R = Rz Ry Rx (columns: local axes in the parent)
 function rotationOf(t) {
@@ -184,6 +287,9 @@ function localRadius(key, P, unit) {
             return 1.001 * Math.hypot(A + w, Math.max(f + w, Math.abs((A + c) * (A + c) / (4 * f) - f) + w))
         }
         case 'aspheric lens': { const L = asphereParams(P, unit); return 1.001 * Math.hypot(L.rho, L.reach) }
+        // an outline's farthest point from its origin; extruded, with half its height
+        case 'revolution': case 'fresnel lens': return profileParams(P, unit, 'revolve').reach
+        case 'extrusion': { const L = profileParams(P, unit, 'extrude'); return 1.001 * Math.hypot(L.reach, L.U[1]) }
     }
     return Infinity
 }
@@ -246,7 +352,7 @@ function generate(doc, opts) {
     const faces = []
     let haze = null
     let flops = 0
-    const dist = [], full = [], maskCode = []
+    const dist = [], full = [], maskCode = [], helpers = []
     let tmp = 0, maskBits = 0
 
     const push = vals => { const o = params.length; for (const v of vals) params.push(fmt(v)); return o }
@@ -267,7 +373,8 @@ function generate(doc, opts) {
         for (const k of Object.keys(P)) {
             const v = P[k]
             const isLen = L.indexOf(k) >= 0
-            if (v && typeof v === 'object') out[k] = { x: v.x * (isLen ? unit : 1), y: v.y * (isLen ? unit : 1), z: v.z * (isLen ? unit : 1) }
+            if (Array.isArray(v)) out[k] = v                      // an outline: its own function scales it
+            else if (v && typeof v === 'object') out[k] = { x: v.x * (isLen ? unit : 1), y: v.y * (isLen ? unit : 1), z: v.z * (isLen ? unit : 1) }
             else out[k] = isLen ? v * unit : v
         }
         return out
@@ -280,10 +387,20 @@ function generate(doc, opts) {
         if (n.type === 'light') {
             P = n.light.shape === 'disc' ? { radius: (n.params.radius || 0.5) * unit } : { width: (n.params.width || 1) * unit, height: (n.params.height || 1) * unit }
         }
+        let pr = null
+        if (SHAPES[key].profile) {
+            try { pr = profileParams(P, unit, SHAPES[key].profile) } catch (e) { throw new Error(`hipgen: node '${n.id}': ${e.message.replace(/^hipgen: /, '')}`) }
+        }
         const xo = push(xfParams(X))
-        const po = push(SHAPES[key].params(P, unit))
+        const po = push(pr ? pr.U : SHAPES[key].params(P, unit))
         layout[n.id] = { xf: xo, params: po, shape: key }
-        return { key, xo, po, P, bs: { c: X.t.slice(), r: localRadius(key, P, unit) * X.s }, flops: PRIM_FLOPS[key] + XF_FLOPS }
+        let primFlops = PRIM_FLOPS[key]
+        if (pr) {
+            helpers.push(profileCode(po, pr))
+            // a segment's crossing test, projection and compare; an arc's; the revolve or extrude and the end
+            primFlops = pr.pieces.reduce((s, k) => s + (k.kind === 'segment' ? 22 : 40), 0) + 12
+        }
+        return { key, xo, po, P, bs: { c: X.t.slice(), r: localRadius(key, P, unit) * X.s }, flops: primFlops + XF_FLOPS }
     }
     function matOf(n) {
         if (n.type === 'light') return lightMaterial(n)
@@ -442,6 +559,7 @@ function generate(doc, opts) {
         `// emitted by shadytracer_amd/js/hipgen.js from '${(doc.title || 'scene').replace(/[\n\r]/g, ' ')}'`,
         `#define PT_SCENE_MATERIALS ${materials.length}`,
         `#define PT_SCENE_MASK_BITS ${maskBits}`,
+        ...helpers,
         '// the nodes a ray from o along d can reach within tmax: those whose bounding sphere, widened by margin, its',
         '// segment meets (bit k of the mask: the k-th bounded top-level node); unbounded nodes are always marched',
         '__device__ __forceinline__ unsigned long long sceneMask(ParamPtr U, V3 o, V3 d, float tmax, float margin) {',

@@ -106,4 +106,4 @@ def test_reference_scenes_emit_or_name_what_stops_them():
             assert ('is user code, not a builtin shape' in msg) or ('the engine does not draw' in msg), (f, msg[-300:])
     # those with builtin shapes (lens, prism, reflector and aspheric lens among them), the studio and Lambertian lights
     assert {'led-lens-screen.json', 'parametric-part.json', 'rgbw-mixing.json', 'dusty-beam.json', 'parabolic-reflector.json',
-            'aspheric-condenser.json'} <= set(ok), ok
+            'aspheric-condenser.json', 'fresnel-plano-convex.json'} <= set(ok), ok

@@ -424,6 +424,109 @@ def test_aspheric_terms_follow_the_scene_unit(tmp_path):
     np.testing.assert_allclose(oracle.scene_at(b, pts)[0], oracle.scene_at(a, pts)[0], rtol=1e-5, atol=1e-7)
 
 
+def _rounded_rect(q, hx, hy, r):
+    d = np.abs(q) - np.array([hx - r, hy - r])
+    return np.linalg.norm(np.maximum(d, 0), axis=1) + np.minimum(d.max(1), 0) - r
+
+
+def _dense_outline(points, per=400):
+    """A filleted outline as a dense polyline, built here from the geometry of a fillet (tangent points t = r /
+    tan(angle / 2) along both edges, centre on the bisector at r / sin(angle / 2)), for a brute-force distance."""
+    P = [np.array(p[:2], float) for p in points]
+    n = len(P)
+    out = []
+    for i in range(n):
+        p, a, b = P[i], P[i - 1], P[(i + 1) % n]
+        r = points[i][2] if len(points[i]) == 3 else 0
+        if r > 0:
+            u, v = (a - p) / np.linalg.norm(a - p), (b - p) / np.linalg.norm(b - p)
+            ang = math.acos(np.clip(u @ v, -1, 1))
+            t = r / math.tan(ang / 2)
+            c = p + (u + v) / np.linalg.norm(u + v) * (r / math.sin(ang / 2))
+            A, B = p + u * t, p + v * t
+            a0, a1 = math.atan2(*(A - c)[::-1]), math.atan2(*(B - c)[::-1])
+            da = (a1 - a0 + math.pi) % (2 * math.pi) - math.pi      # the minor arc
+            out += [c + r * np.array([math.cos(a0 + da * s), math.sin(a0 + da * s)]) for s in np.linspace(0, 1, per)]
+        else:
+            out.append(p)
+    return np.array(out)
+
+
+def _poly_sdf(q, V, surface=None):
+    """Brute force: signed distance to a closed polyline (even-odd inside); surface[i] False leaves edge i out of the
+    distance (a revolved outline's axis edge)."""
+    n = len(V)
+    best = np.full(len(q), np.inf)
+    inside = np.zeros(len(q), bool)
+    for i in range(n):
+        a, b = V[i], V[(i + 1) % n]
+        cross = ((a[1] > q[:, 1]) != (b[1] > q[:, 1])) & (q[:, 0] < a[0] + (q[:, 1] - a[1]) * (b[0] - a[0]) / (b[1] - a[1] + 1e-300))
+        inside ^= cross
+        if surface is not None and not surface[i]:
+            continue
+        e = b - a
+        t = np.clip(((q - a) @ e) / max(e @ e, 1e-300), 0, 1)
+        best = np.minimum(best, np.linalg.norm(q - a - t[:, None] * e, axis=1))
+    return np.where(inside, -best, best)
+
+
+def test_revolved_rectangle_is_the_cylinder(tmp_path):
+    # shapes.js revolve over the profile pieces: (0, -h) (R, -h) (R, h) (0, h) turned about z, its axis edge no
+    # surface, is the cylinder exactly; with fillets at the rim corners the rounded cylinder
+    R, h, rr = 0.4, 0.3, 0.08
+    sc = _emit_doc(_single({'outline': [[0, -h], [R, -h], [R, h], [0, h]]}, 'revolution'), tmp_path)
+    pts = RNG.uniform(-0.8, 0.8, (800, 3))
+    q = np.stack([np.hypot(pts[:, 0], pts[:, 1]), pts[:, 2]], 1)
+    d, _, _ = oracle.scene_at(sc, pts)
+    np.testing.assert_allclose(d, _rounded_rect(q, R, h, 0), atol=2e-7)
+    sc = _emit_doc(_single({'outline': [[0, -h], [R, -h, rr], [R, h, rr], [0, h]]}, 'revolution'), tmp_path)
+    d, _, _ = oracle.scene_at(sc, pts)
+    np.testing.assert_allclose(d, _rounded_rect(q, R, h, rr), atol=2e-7)
+    assert _normals_are_gradients(sc, pts) > 0.97
+    c, r = _node_sphere(sc)
+    assert (np.linalg.norm(pts[d <= 0] - c, axis=1) <= r).all()
+
+
+def test_extruded_outline_against_brute_force(tmp_path):
+    # the extrusion builtin's own default outline: convex and concave fillets (the notch), extruded 0.2
+    outline = [[-0.5, -0.35, 0.08], [0.5, -0.35, 0.08], [0.5, 0.35, 0.08], [0.125, 0.35, 0.035], [0, 0.125, 0.05],
+               [-0.125, 0.35, 0.035], [-0.5, 0.35, 0.08]]
+    sc = _emit_doc(_single({'outline': outline, 'height': 0.2}, 'extrusion'), tmp_path)
+    pts = np.c_[RNG.uniform(-0.7, 0.7, (600, 2)), RNG.uniform(-0.3, 0.3, 600)]
+    d, _, _ = oracle.scene_at(sc, pts)
+    g = _poly_sdf(pts[:, :2], _dense_outline(outline, 2000))
+    exact = _extrude(g, pts[:, 2], float(np.float32(0.1)))
+    np.testing.assert_allclose(d, exact, atol=2e-6)          # the dense polyline's chord error, at most r (1 - cos(da/2))
+    # a filleted rectangle extruded is the box with rounded vertical edges
+    sc = _emit_doc(_single({'outline': [[-0.3, -0.2, 0.05], [0.3, -0.2, 0.05], [0.3, 0.2, 0.05], [-0.3, 0.2, 0.05]], 'height': 0.4},
+                           'extrusion'), tmp_path)
+    d, _, _ = oracle.scene_at(sc, pts)
+    np.testing.assert_allclose(d, _extrude(_rounded_rect(pts[:, :2], 0.3, 0.2, 0.05), pts[:, 2], 0.2), atol=2e-7)
+
+
+def test_fresnel_outline_against_brute_force(tmp_path):
+    # the reference's Fresnel lens (scenes/fresnel-plano-convex.json's outline: a flat back, six facet rings), revolved:
+    # a polygon's exact distance with the axis edge out of it, the even-odd rule for inside
+    outline = [[0, 0], [0.5, 0], [0.5, 0.025], [0.4167, 0.0918], [0.4167, 0.025], [0.3333, 0.0746], [0.3333, 0.025], [0.25, 0.0611],
+               [0.25, 0.025], [0.1667, 0.0497], [0.1667, 0.025], [0.0833, 0.0394], [0.0833, 0.025], [0, 0.0297]]
+    sc = _emit_doc(_single({'outline': outline}, 'fresnel lens'), tmp_path)
+    pts = np.c_[RNG.uniform(-0.6, 0.6, (1500, 2)), RNG.uniform(-0.1, 0.2, 1500)]
+    d, _, _ = oracle.scene_at(sc, pts)
+    q = np.stack([np.hypot(pts[:, 0], pts[:, 1]), pts[:, 2]], 1)
+    V = np.array([[float(np.float32(x)) for x in p] for p in outline])
+    surface = [not (V[i][0] == 0 and V[(i + 1) % len(V)][0] == 0) for i in range(len(V))]
+    np.testing.assert_allclose(d, _poly_sdf(q, V, surface), atol=1e-9)
+    assert (d[(q[:, 0] < 0.48) & (q[:, 1] > 0.001) & (q[:, 1] < 0.024)] < 0).all()     # the back plate is solid
+
+
+@pytest.mark.parametrize('outline,why', [([[0, 0], [1, 0]], 'at least three'), ([[0, 0], [1, 0], [1, 0], [0, 1]], 'same point'),
+                                         ([[0, 0], [2, 2], [2, 0], [0, 1]], 'cross'), ([[0, 0], [1, 0, 0.6], [1, 1, 0.6], [0, 1]], 'do not fit')])
+def test_unsound_outlines_are_refused(outline, why, tmp_path):
+    # shapes.js profilePieces' problems, refused with the node named
+    with pytest.raises(RuntimeError, match="node 'n': .*" + why):
+        _emit_doc(_single({'outline': outline}, 'revolution'), tmp_path)
+
+
 def test_studio_floor_reads_rho_E_over_pi():
     # environment.js studioLight: L0 (1 + top d.z) with L0 = E / (pi (1 + 2 top / 3)) lights a level surface with E
     # whatever top is, so a Lambertian floor of albedo 0.6 under 500 lx reads 0.6 x 500 / pi; the rays that leave
