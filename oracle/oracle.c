@@ -80,7 +80,7 @@ static void sampleSeed(uint32_t base, uint32_t sample, uint32_t* out) {
 /* ---- the scene program (hipgen.js `program`) ---- */
 enum { OP_PRIM = 0, OP_JOIN = 1, OP_FOLD = 2, OP_NODE = 3 };
 enum { SH_PLANE = 0, SH_SPHERE, SH_BOX, SH_CYLINDER, SH_CAPSULE, SH_TORUS, SH_RECT, SH_DISC, SH_LENS, SH_PRISM, SH_REFLECTOR, SH_ASPHERE,
-       SH_REVOLUTION, SH_FRESNEL, SH_EXTRUSION };
+       SH_REVOLUTION, SH_FRESNEL, SH_EXTRUSION, SH_CONE, SH_ROUNDCONE, SH_OCTA, SH_CTORUS, SH_LINK, SH_TUBE, SH_DOME };
 enum { J_UNION = 0, J_SUBTRACT, J_INTERSECT };
 /* prim: shape, xf, p, mat, prim; join: join, k; node (a top-level node's start): xf = its bounding sphere's offset in
  * U (-1: unbounded), join = its mask bit (-1: none), k = the index of its fold */
@@ -204,6 +204,26 @@ static g2v outlineAt(const float* P, double qx, double qy) {
     return out;
 }
 
+/* The builder's other primitives (core/shapes.js:119-373, core/sdf.js sdCappedCone:633), in a meridian plane where they
+ * are revolved: r = |p.xy|, z; the result lifted about the axis by revolveD. */
+static d4 revolveD(d3 p, g2v g) {
+    double l = sqrt(p.x * p.x + p.y * p.y), rx = l > 1e-12 ? p.x / l : 1, ry = l > 1e-12 ? p.y / l : 0;
+    return d4of(v(rx * g.x, ry * g.x, g.y), g.d);
+}
+static g2v circle2(double qx, double qy, double cx, double cy, double r) {
+    g2v g = dir2(qx - cx, qy - cy, 1, 0);
+    g.d = sqrt((qx - cx) * (qx - cx) + (qy - cy) * (qy - cy)) - r;
+    return g;
+}
+static g2v box2(double qx, double qy, double cx, double cy, double bx, double by) {
+    double sx = qx >= cx ? 1 : -1, sy = qy >= cy ? 1 : -1, ex = fabs(qx - cx) - bx, ey = fabs(qy - cy) - by;
+    g2v g;
+    if (ex > 0 || ey > 0) { double ox = maxd(ex, 0), oy = maxd(ey, 0), l = sqrt(ox * ox + oy * oy); g.x = ox / l * sx; g.y = oy / l * sy; g.d = l; return g; }
+    if (ex > ey) { g.x = sx; g.y = 0; g.d = ex; return g; }
+    g.x = 0; g.y = sy; g.d = ey; return g;
+}
+static d3 safeDirD(d3 a, d3 fb) { double l = len(a); return l > 1e-12 ? scl(a, 1 / l) : fb; }
+
 /* Primitive (outward normal, distance) in its own frame: core/sdf.js:352-628 restated; the lens core/sdf.js:463, the
  * prism core/shapes.js:263-287 (a regular polygon extruded), the reflector core/shapes.js:392-421, the aspheric lens
  * core/shapes.js:658. */
@@ -300,6 +320,59 @@ static d4 primLocal(int shape, d3 p, const float* P) {
         }
         double rx = x > 1e-12 ? p.x / l : 1, ry = x > 1e-12 ? p.y / l : 0;
         return d4of(v(rx * r.x, ry * r.x, r.y), r.d);
+    }
+    case SH_CONE: {         /* P: half height, base radius, top radius; distance only */
+        double h = P[0], r1 = P[1], r2 = P[2], qx = sqrt(p.x * p.x + p.y * p.y), qy = p.z;
+        double k2x = r2 - r1, k2y = 2 * h, cax = qx - mind(qx, qy < 0 ? r1 : r2), cay = fabs(qy) - h;
+        double t = clampd(((r2 - qx) * k2x + (h - qy) * k2y) / (k2x * k2x + k2y * k2y), 0, 1);
+        double cbx = qx - r2 + k2x * t, cby = qy - h + k2y * t, s = cbx < 0 && cay < 0 ? -1 : 1;
+        return d4of(v(0, 0, 0), s * sqrt(mind(cax * cax + cay * cay, cbx * cbx + cby * cby)));
+    }
+    case SH_ROUNDCONE: {    /* P: bottom radius, top radius, height */
+        double r1 = P[0], r2 = P[1], h = P[2], qx = sqrt(p.x * p.x + p.y * p.y), qy = p.z + 0.5 * h;
+        if (h <= fabs(r1 - r2)) return revolveD(p, r1 >= r2 ? circle2(qx, qy, 0, 0, r1) : circle2(qx, qy, 0, h, r2));
+        double b = (r1 - r2) / h, a = sqrt(1 - b * b), k = -b * qx + a * qy;
+        if (k < 0) return revolveD(p, circle2(qx, qy, 0, 0, r1));
+        if (k > a * h) return revolveD(p, circle2(qx, qy, 0, h, r2));
+        g2v g = {a, b, a * qx + b * qy - r1};
+        return revolveD(p, g);
+    }
+    case SH_OCTA: {         /* P: size */
+        double s = maxd(P[0], 0), c = 0.5773502691896258;
+        d3 sg = v(p.x >= 0 ? 1 : -1, p.y >= 0 ? 1 : -1, p.z >= 0 ? 1 : -1), a = v(fabs(p.x), fabs(p.y), fabs(p.z)), fb = v(c, c, c);
+        double m = a.x + a.y + a.z - s;
+        if (3 * a.x < m) { double k = clampd(0.5 * (a.z - a.y + s), 0, s); d3 w = v(a.x, a.y - s + k, a.z - k); return d4of(mul(sg, safeDirD(w, fb)), len(w)); }
+        if (3 * a.y < m) { double k = clampd(0.5 * (a.x - a.z + s), 0, s); d3 w = v(a.y, a.z - s + k, a.x - k); return d4of(mul(sg, safeDirD(v(w.z, w.x, w.y), fb)), len(w)); }
+        if (3 * a.z < m) { double k = clampd(0.5 * (a.y - a.x + s), 0, s); d3 w = v(a.z, a.x - s + k, a.y - k); return d4of(mul(sg, safeDirD(v(w.y, w.z, w.x), fb)), len(w)); }
+        return d4of(scl(sg, c), m * c);
+    }
+    case SH_CTORUS: {       /* P: ring, tube, sin and cos of half the angle */
+        double R = P[0], sn = P[2], cs = P[3], sx = p.x >= 0 ? 1 : -1, qx = fabs(p.x), cx, cy;
+        if (cs * qx > sn * p.y) { cx = R * sn; cy = R * cs; }
+        else { g2v d = dir2(qx, p.y, 0, 1); cx = R * d.x; cy = R * d.y; }
+        d3 w = v(qx - cx, p.y - cy, p.z), n = safeDirD(w, v(0, 0, 1));
+        return d4of(v(n.x * sx, n.y, n.z), len(w) - P[1]);
+    }
+    case SH_LINK: {         /* P: span, ring, tube: the torus with its y pulled apart */
+        double le = maxd(0.5 * P[0], 0), qy = (p.y >= 0 ? 1 : -1) * maxd(fabs(p.y) - le, 0);
+        double lxy = sqrt(p.x * p.x + qy * qy), rx = lxy > 1e-12 ? p.x / lxy : 1, ry = lxy > 1e-12 ? qy / lxy : 0;
+        double ax = lxy - P[1], ay = p.z, ql = sqrt(ax * ax + ay * ay);
+        d3 n = ql > 1e-12 ? v(rx * ax / ql, ry * ax / ql, ay / ql) : v(0, 0, 1);
+        return d4of(n, ql - P[2]);
+    }
+    case SH_TUBE: {         /* P: outer, inner, height */
+        double qx = sqrt(p.x * p.x + p.y * p.y), hh = maxd(0.5 * P[2], 0);
+        if (P[1] <= 0) return revolveD(p, box2(qx, p.z, 0, 0, maxd(P[0], 0), hh));
+        double ro = maxd(P[0], P[1] + 1e-6);
+        return revolveD(p, box2(qx, p.z, 0.5 * (P[1] + ro), 0, 0.5 * (ro - P[1]), hh));
+    }
+    case SH_DOME: {         /* P: radius, height */
+        double R = maxd(P[0], 1e-6), c = clampd(P[1] - R, -R, R * (1 - 1e-4)), hc = -c, w = sqrt(maxd(R * R - hc * hc, 0));
+        double qx = sqrt(p.x * p.x + p.y * p.y), qy = p.z - c;
+        double s = maxd((hc - R) * qx * qx + w * w * (hc + R - 2 * qy), hc * qx - w * qy);
+        if (s < 0) return revolveD(p, circle2(qx, qy, 0, 0, R));
+        if (qx < w) { g2v g = {0, -1, hc - qy}; return revolveD(p, g); }
+        return revolveD(p, circle2(qx, qy, w, hc, 0));
     }
     case SH_REVOLUTION: case SH_FRESNEL: case SH_EXTRUSION: {   /* P: mode (0 revolve, 1 extrude), half height, the outline */
         if (P[0] == 0) {

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, '_build', 'liboracle.so')
 
 SHAPES = ['plane', 'sphere', 'box', 'cylinder', 'capsule', 'torus', 'rectangle', 'disc', 'lens', 'prism', 'reflector', 'aspheric lens',
-          'revolution', 'fresnel lens', 'extrusion']
+          'revolution', 'fresnel lens', 'extrusion', 'cone', 'round cone', 'octahedron', 'capped torus', 'link', 'tube', 'dome']
 JOINS = ['union', 'subtract', 'intersect']
 OPS = {'prim': 0, 'join': 1, 'fold': 2, 'node': 3}
 

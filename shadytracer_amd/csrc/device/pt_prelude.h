@@ -316,6 +316,117 @@ __device__ __forceinline__ V4 g2revolve(V3 p, G2 g) {
     if (l > 1e-12f) { float il = frcp(l); rx = p.x * il; ry = p.y * il; }
     return v4(v3(rx * g.x, ry * g.x, g.y), g.d);
 }
+// ---- the builder's other primitives (builder/primitives.js:139-236; shapes.js, sdf.js sdCappedCone:633)
+__device__ __forceinline__ G2 g2circle(float qx, float qy, float cx, float cy, float r) {
+    const float ex = qx - cx, ey = qy - cy, l = fsqrt(ex * ex + ey * ey);
+    if (l > 1e-12f) { const float il = frcp(l); return g2(ex * il, ey * il, l - r); }
+    return g2(1.0f, 0.0f, l - r);
+}
+// the 2D box of half sizes (bx, by) about (cx, cy): exact both sides (shapes.js g2Box:46)
+__device__ __forceinline__ G2 g2box(float qx, float qy, float cx, float cy, float bx, float by) {
+    const float sx = qx >= cx ? 1.0f : -1.0f, sy = qy >= cy ? 1.0f : -1.0f;
+    const float ex = fabsf(qx - cx) - bx, ey = fabsf(qy - cy) - by;
+    if (ex > 0.0f || ey > 0.0f) {
+        const float ox = fmaxf(ex, 0.0f), oy = fmaxf(ey, 0.0f), l = fsqrt(ox * ox + oy * oy), il = frcp(l);
+        return g2(ox * il * sx, oy * il * sy, l);
+    }
+    if (ex > ey) return g2(sx, 0.0f, ex);
+    return g2(0.0f, sy, ey);
+}
+// A capped cone along z, base radius at z = -h, top radius at +h (Quilez's capped cone, sdf.js sdCappedCone:633):
+// distance only; the surface kernel takes its normal by differences.
+__device__ __forceinline__ float Dcone(V3 p, float h, float r1, float r2) {
+    const float qx = fsqrt(p.x * p.x + p.y * p.y), qy = p.z;
+    const float k2x = r2 - r1, k2y = 2.0f * h;
+    const float cax = qx - fminf(qx, qy < 0.0f ? r1 : r2), cay = fabsf(qy) - h;
+    const float t = fminf(fmaxf(((r2 - qx) * k2x + (h - qy) * k2y) / (k2x * k2x + k2y * k2y), 0.0f), 1.0f);
+    const float cbx = qx - r2 + k2x * t, cby = qy - h + k2y * t;
+    const float s = cbx < 0.0f && cay < 0.0f ? -1.0f : 1.0f;
+    return s * fsqrt(fminf(cax * cax + cay * cay, cbx * cbx + cby * cby));
+}
+__device__ __forceinline__ V4 Ncone(V3 p, float h, float r1, float r2) { return v4(v3s(0.0f), Dcone(p, h, r1, r2)); }
+// Two spheres along z, r1 at z = -h/2 and r2 at +h/2, and the cone touching both (shapes.js NroundCone:119); a height
+// under the radii's difference leaves the bigger sphere.
+__device__ __forceinline__ V4 NroundCone(V3 p, float r1, float r2, float h) {
+    const float qx = fsqrt(p.x * p.x + p.y * p.y), qy = p.z + 0.5f * h;
+    if (h <= fabsf(r1 - r2)) return g2revolve(p, r1 >= r2 ? g2circle(qx, qy, 0.0f, 0.0f, r1) : g2circle(qx, qy, 0.0f, h, r2));
+    const float b = (r1 - r2) / h, a = fsqrt(1.0f - b * b);
+    const float k = -b * qx + a * qy;
+    if (k < 0.0f) return g2revolve(p, g2circle(qx, qy, 0.0f, 0.0f, r1));
+    if (k > a * h) return g2revolve(p, g2circle(qx, qy, 0.0f, h, r2));
+    return g2revolve(p, g2(a, b, a * qx + b * qy - r1));
+}
+// A regular octahedron, size from its centre to each vertex (shapes.js Noctahedron:135): exact both sides -- inside
+// the nearest face's plane, outside the nearest point of the face, edge or vertex in the point's octant.
+__device__ __forceinline__ V3 safeDir3(V3 v, V3 fb) { const float l = length(v); return l > 1e-12f ? v * frcp(l) : fb; }
+__device__ __forceinline__ V4 Noctahedron(V3 p, float s0) {
+    const float s = fmaxf(s0, 0.0f);
+    const V3 sg = v3(p.x >= 0.0f ? 1.0f : -1.0f, p.y >= 0.0f ? 1.0f : -1.0f, p.z >= 0.0f ? 1.0f : -1.0f);
+    const V3 a = vabs(p);
+    const float m = a.x + a.y + a.z - s;
+    const float c = 0.5773502691896258f;
+    const V3 face = sg * c, fb = v3s(c);
+    if (3.0f * a.x < m) {
+        const float k = fminf(fmaxf(0.5f * (a.z - a.y + s), 0.0f), s);
+        const V3 v = v3(a.x, a.y - s + k, a.z - k);
+        return v4(sg * safeDir3(v, fb), length(v));
+    }
+    if (3.0f * a.y < m) {
+        const float k = fminf(fmaxf(0.5f * (a.x - a.z + s), 0.0f), s);
+        const V3 v = v3(a.y, a.z - s + k, a.x - k);
+        return v4(sg * safeDir3(v3(v.z, v.x, v.y), fb), length(v));
+    }
+    if (3.0f * a.z < m) {
+        const float k = fminf(fmaxf(0.5f * (a.y - a.x + s), 0.0f), s);
+        const V3 v = v3(a.z, a.x - s + k, a.y - k);
+        return v4(sg * safeDir3(v3(v.y, v.z, v.x), fb), length(v));
+    }
+    return v4(face, m * c);
+}
+// Part of a torus in the xy plane: an arc of radius R, angle degrees of it symmetric about +y, swept by a tube of
+// radius r (shapes.js NcappedTorus:335): the nearest point of the arc -- its end past the wedge -- then the tube.
+// sc = (sin, cos) of half the angle, from the emitter.
+__device__ __forceinline__ V4 NcappedTorus(V3 p, float R, float r, float sn, float cs) {
+    const float sx = p.x >= 0.0f ? 1.0f : -1.0f;
+    const V3 q = v3(fabsf(p.x), p.y, p.z);
+    float cx, cy;
+    if (cs * q.x > sn * q.y) { cx = R * sn; cy = R * cs; }
+    else {
+        const float l = fsqrt(q.x * q.x + q.y * q.y);
+        if (l > 1e-12f) { const float il = frcp(l); cx = R * q.x * il; cy = R * q.y * il; } else { cx = 0.0f; cy = R; }
+    }
+    const V3 v = v3(q.x - cx, q.y - cy, q.z);
+    const V3 n = safeDir3(v, v3(0.0f, 0.0f, 1.0f));
+    return v4(v3(n.x * sx, n.y, n.z), length(v) - r);
+}
+// A chain link in the xy plane: a ring of radius R pulled apart along y by span, swept by a tube of radius r (shapes.js
+// Nlink:349).
+__device__ __forceinline__ V4 Nlink(V3 p, float span, float R, float r) {
+    const float le = fmaxf(0.5f * span, 0.0f);
+    const float qy = (p.y >= 0.0f ? 1.0f : -1.0f) * fmaxf(fabsf(p.y) - le, 0.0f);
+    return Ntorus(v3(p.x, qy, p.z), R, r);
+}
+// A hollow cylinder along z, centred: the 2D box [inner, outer] x [-h/2, h/2] revolved (shapes.js Ntube:373); an inner
+// radius of 0 or less is the solid cylinder.
+__device__ __forceinline__ V4 Ntube(V3 p, float outer, float inner, float height) {
+    const float qx = fsqrt(p.x * p.x + p.y * p.y), hh = fmaxf(0.5f * height, 0.0f);
+    if (inner <= 0.0f) return g2revolve(p, g2box(qx, p.z, 0.0f, 0.0f, fmaxf(outer, 0.0f), hh));
+    const float ro = fmaxf(outer, inner + 1e-6f);
+    return g2revolve(p, g2box(qx, p.z, 0.5f * (inner + ro), 0.0f, 0.5f * (ro - inner), hh));
+}
+// The part of a sphere of radius R above z = 0, height tall (shapes.js Ndome:358, Quilez's cut sphere): its centre at
+// c = height - R (held below R), exact both sides -- the sphere, the flat base, or the rim's circle.
+__device__ __forceinline__ V4 Ndome(V3 p, float R0, float height) {
+    const float R = fmaxf(R0, 1e-6f);
+    const float c = fminf(fmaxf(height - R, -R), R * (1.0f - 1e-4f));
+    const float hc = -c, w = fsqrt(fmaxf(R * R - hc * hc, 0.0f));
+    const float qx = fsqrt(p.x * p.x + p.y * p.y), qy = p.z - c;
+    const float s = fmaxf((hc - R) * qx * qx + w * w * (hc + R - 2.0f * qy), hc * qx - w * qy);
+    if (s < 0.0f) return g2revolve(p, g2circle(qx, qy, 0.0f, 0.0f, R));
+    if (qx < w) return g2revolve(p, g2(0.0f, -1.0f, hc - qy));
+    return g2revolve(p, g2circle(qx, qy, w, hc, 0.0f));
+}
+
 // Extruded along z to half height h, exactly (shapes.js g2Extrude:717): past both the outline and the caps, the
 // corner's distance; else the larger of the two.
 __device__ __forceinline__ V4 g2extrude(V3 p, G2 g, float h) {

@@ -18,9 +18,10 @@
  *   sppm-slab    a collimated beam through a glass slab onto a floor: the photon map's closed form.
  *   furnace-*    closed-form checks: a white furnace (albedo-1 Lambertian plane under a uniform sky, and a haze box
  *                under a uniform sphere of sky).
- *   studio-floor, lens-prism, reflector-condenser, turned-parts
+ *   studio-floor, lens-prism, reflector-condenser, turned-parts, gallery
  *                documents in the builder's own form (millimetres, the studio environment, lights by CCT and flux,
- *                an orbit view), the others with the lens, prism, reflector, aspheric lens and outline builtins.
+ *                an orbit view), the others with the lens, prism, reflector, aspheric lens, outline and the
+ *                remaining solid builtins.
  */
 
 const SEED = 20261203
@@ -300,8 +301,29 @@ function turnedParts(opts = {}) {
     return { doc: d, camera: cameraOf(d, w, h), settings: { width: w, height: h, spp: 64, maxbounce: 8, nee: 1 } }
 }
 
+// gallery: the remaining solid builtins at their defaults, 40 times (millimetres) -- a cone, a round cone, an
+// octahedron, a capped torus (270 degrees), a chain link, a tube and a dome -- in a row on a grey floor, a 3000 K
+// panel and the studio.
+function gallery(opts = {}) {
+    const { cameraOf } = require('./hipgen')
+    const kinds = ['cone', 'round cone', 'octahedron', 'capped torus', 'link', 'tube', 'dome']
+    const blocks = BLOCKS.concat(kinds.map(k => ({ id: `b-${k}`, label: k, kind: 'object', builtin: `${k}@1`, params: {} })))
+    const mats = [material('floor', [0.5, 0.5, 0.5], 0.8, 0, { specular: 0.5 }), material('brass', [0.9, 0.7, 0.35], 0.3, 1),
+        material('paint', [0.2, 0.35, 0.7], 0.5, 0)]
+    // each sits on the floor: its local z extent at scale 40 (the octahedron on a corner, the torus and link on edge)
+    const place = { cone: [20, 0], 'round cone': [23, 0], octahedron: [20, 0], 'capped torus': [15.1, 90], link: [20, 90], tube: [20, 0], dome: [0, 0] }
+    const children = [obj('floor', 'b-plane', 'floor', xf(0, 0, 0))].concat(kinds.map((k, i) =>
+        obj(k.replace(' ', '-'), `b-${k}`, i % 2 ? 'brass' : 'paint', xf(-126 + 42 * i, 8 * (i % 2), place[k][0], place[k][1], 0, 0, 40), {})))
+    children.push({ id: 'panel', kind: 'object', type: 'light', label: 'panel', enabled: true, material: null, transform: xf(0, 80, 140, 150, 0, 0),
+        params: { width: 60, height: 40 }, light: { shape: 'rectangle', colourBy: 'cct', cct: 3000, flux: 400 } })
+    const d = studioDoc('gallery', mats, children, blocks,
+        { mode: 'orbit', camtype: 0, fov: 40, target: [0, 0, 15], azimuth: -90, elevation: 65, distance: 230, up: [0, 0, 1], aperture: 0 })
+    const w = opts.width || 160, h = opts.height || 64
+    return { doc: d, camera: cameraOf(d, w, h), settings: { width: w, height: h, spp: 64, maxbounce: 8, nee: 1 } }
+}
+
 const CONFIGS = { 'plane-c1': planeC1, 'room-c2': roomC2, 'caustic-c3': causticC3, 'furnace-plane': furnacePlane, 'furnace-haze': furnaceHaze,
     'sppm-slab': sppmSlab, 'studio-floor': studioFloor, 'lens-prism': lensPrism, 'reflector-condenser': reflectorCondenser,
-    'turned-parts': turnedParts }
+    'turned-parts': turnedParts, gallery }
 
 module.exports = { CONFIGS, camera, rng, SEED }

@@ -28,7 +28,8 @@ const LUMA = [0.2126, 0.7152, 0.0722]
 // The library calls (the prism's atan2f and sincospif, the reflector's cbrtf or cosf and atan2f) are counted as 25 each
 // (their polynomials and the division), an estimate.
 const PRIM_FLOPS = { plane: 1, sphere: 7, box: 27, cylinder: 24, capsule: 14, torus: 14, rectangle: 14, disc: 13, lens: 25, prism: 75,
-    reflector: 80, 'aspheric lens': 52, revolution: 0, 'fresnel lens': 0, extrusion: 0 }
+    reflector: 80, 'aspheric lens': 52, revolution: 0, 'fresnel lens': 0, extrusion: 0, cone: 30, 'round cone': 20, octahedron: 24,
+    'capped torus': 22, link: 18, tube: 16, dome: 24 }
 const XF_FLOPS = 22      // p - pos, three dot products, times 1/s; the distance times s
 const FOLD_FLOPS = 2
 const JOIN_FLOPS = 9
@@ -81,13 +82,29 @@ const SHAPES = {
     revolution: { profile: 'revolve', params: (P, unit) => profileParams(P, unit, 'revolve').U, dist: o => `Dprofile${o}(U, q)`, full: o => `Nprofile${o}(U, q)` },
     'fresnel lens': { profile: 'revolve', params: (P, unit) => profileParams(P, unit, 'revolve').U, dist: o => `Dprofile${o}(U, q)`, full: o => `Nprofile${o}(U, q)` },
     extrusion: { profile: 'extrude', params: (P, unit) => profileParams(P, unit, 'extrude').U, dist: o => `Dprofile${o}(U, q)`, full: o => `Nprofile${o}(U, q)` },
+    // the builder's other primitives (builder/primitives.js:139-236): their parameters as the shape functions take them
+    cone: { params: P => [0.5 * P.height, P.base, P.top], dist: o => `Dcone(q, ${us(o, 3)})`, full: o => `Ncone(q, ${us(o, 3)})` },
+    'round cone': { params: P => [P.bottom, P.top, P.height], dist: o => `NroundCone(q, ${us(o, 3)}).w`, full: o => `NroundCone(q, ${us(o, 3)})` },
+    octahedron: { params: P => [P.size], dist: o => `Noctahedron(q, U[${o}]).w`, full: o => `Noctahedron(q, U[${o}])` },
+    // ring, tube, and the sine and cosine of half the angle (the sine held above 1e-6, as shapes.js NcappedTorus does)
+    'capped torus': {
+        params: P => { const h = Math.min(Math.max(P.angle, 0), 360) * Math.PI / 360; return [P.ring, P.tube, Math.max(Math.sin(h), 1e-6), Math.cos(h)] },
+        dist: o => `NcappedTorus(q, ${us(o, 4)}).w`, full: o => `NcappedTorus(q, ${us(o, 4)})`,
+    },
+    link: { params: P => [P.span, P.ring, P.tube], dist: o => `Nlink(q, ${us(o, 3)}).w`, full: o => `Nlink(q, ${us(o, 3)})` },
+    tube: { params: P => [P.outer, P.inner, P.height], dist: o => `Ntube(q, ${us(o, 3)}).w`, full: o => `Ntube(q, ${us(o, 3)})` },
+    dome: { params: P => [P.radius, P.height], dist: o => `Ndome(q, ${us(o, 2)}).w`, full: o => `Ndome(q, ${us(o, 2)})` },
 }
+function us(o, n) { return [...Array(n).keys()].map(i => `U[${o + i}]`).join(', ') }
 // the builtins' own defaults for parameters a document leaves out (builder/primitives.js:172 prism, :237 lens, :255
 // aspheric lens, :294 reflector)
 const DEFAULTS = { lens: { diameter: 1.0, thickness: 0.2, r1: 2.0, r2: 2.0 }, prism: { sides: 3, radius: 0.5, height: 1.0 },
     reflector: { focal: 0.125, aperture: 0.5, wall: 0.01, solid: 0, focusOffset: 0 },
     'aspheric lens': { diameter: 1.0, thickness: 0.3, r1: 0.5, k1: -2.25, a4_1: 0, a6_1: 0, a8_1: 0, a10_1: 0, r2: 0, k2: 0, a4_2: 0, a6_2: 0,
-        a8_2: 0, a10_2: 0 } }
+        a8_2: 0, a10_2: 0 },
+    cone: { base: 0.5, top: 0.0, height: 1.0 }, 'round cone': { bottom: 0.3, top: 0.15, height: 0.55 }, octahedron: { size: 0.5 },
+    'capped torus': { ring: 0.42, tube: 0.08, angle: 270 }, link: { span: 0.36, ring: 0.25, tube: 0.07 }, tube: { outer: 0.5, inner: 0.4, height: 1.0 },
+    dome: { radius: 0.5, height: 0.3 } }
 
 // ---- outlines (shapes.js profilePieces:1487): points (x, y, and a fillet radius if rounded), closed back to the first,
 // as the pieces a profile function walks -- segments, and arcs where a corner is rounded -- counter-clockwise, each
@@ -229,7 +246,8 @@ function prismParams(P) {
 const LENGTHS = { sphere: ['radius'], box: ['size', 'rounding'], cylinder: ['radius', 'height', 'rounding'], capsule: ['radius', 'height'],
     torus: ['ring', 'tube'], rectangle: ['width', 'height'], disc: ['radius'], plane: [], lens: ['diameter', 'thickness', 'r1', 'r2'],
     prism: ['radius', 'height'], reflector: ['focal', 'aperture', 'wall', 'focusOffset'], 'aspheric lens': ['diameter', 'thickness', 'r1', 'r2'],
-    revolution: [], 'fresnel lens': [], extrusion: ['height'] }
+    revolution: [], 'fresnel lens': [], extrusion: ['height'], cone: ['base', 'top', 'height'], 'round cone': ['bottom', 'top', 'height'],
+    octahedron: ['size'], 'capped torus': ['ring', 'tube'], link: ['span', 'ring', 'tube'], tube: ['outer', 'inner', 'height'], dome: ['radius', 'height'] }
 
 // ---- transforms: world = parent * local; R = Rz Ry Rx (columns: local axes in the parent)
 function rotationOf(t) {
@@ -290,6 +308,14 @@ function localRadius(key, P, unit) {
         // an outline's farthest point from its origin; extruded, with half its height
         case 'revolution': case 'fresnel lens': return profileParams(P, unit, 'revolve').reach
         case 'extrusion': { const L = profileParams(P, unit, 'extrude'); return 1.001 * Math.hypot(L.reach, L.U[1]) }
+        case 'cone': return Math.hypot(Math.max(P.base, P.top, 0), 0.5 * Math.abs(P.height))
+        case 'round cone': return 0.5 * Math.abs(P.height) + Math.max(P.bottom, P.top, 0)
+        case 'octahedron': return Math.max(P.size, 0)
+        case 'capped torus': return P.ring + P.tube
+        case 'link': return 0.5 * Math.max(P.span, 0) + P.ring + P.tube
+        case 'tube': return Math.hypot(Math.max(P.outer, P.inner, 0), 0.5 * Math.abs(P.height))
+        // the sphere's centre at c = height - R (held to +-R): within |c| + R of the origin
+        case 'dome': { const R = Math.max(P.radius, 1e-6); return Math.abs(Math.min(Math.max(P.height - R, -R), R)) + R }
     }
     return Infinity
 }

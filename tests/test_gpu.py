@@ -196,6 +196,35 @@ def test_outlines_match_the_oracle(engine):
     assert abs(gL.mean() - oL.mean()) < 4 * se + 1e-3 * oL.mean()
 
 
+def test_solid_builtins_match_the_oracle(engine):
+    # cone, round cone, octahedron, capped torus, link, tube and dome: primary decisions ray by ray, then
+    # per-pixel means
+    say('case: gallery 160x64 pick vs oracle; 80x32 GPU 256 spp vs oracle 64 spp')
+    made, u = load(engine, 'gallery')
+    gpos, gstate, _, gprim = engine.pick_rays()
+    opos, ostate, _, oprim = oracle.pick(made['scene'], u)
+    same = (gstate == ostate) & ((gstate != 0) | (gprim == oprim))
+    say('agreement', same.mean(), 'per shape', [int(((oprim == k) & (ostate == 0)).sum()) for k in range(1, 8)])
+    assert same.mean() >= 0.9995
+    for k in range(1, 8):
+        assert ((gprim == k) & (gstate == 0)).sum() > 50, k
+    hit = (gstate == 0) & same
+    assert np.quantile(np.linalg.norm(gpos[hit, :3] - opos[hit, :3], axis=1), 0.999) < 1e-4
+    made, u = load(engine, 'gallery', width=80, height=32)
+    engine.trace(256)
+    ga, gm = engine.read_accum()
+    oa, om, _ = oracle.render(made['scene'], u, 64, first=100000)
+    gL, oL = pt.luminance(ga), pt.luminance(oa)
+    s = np.sqrt(pt.standard_error(gm) ** 2 + pt.standard_error(om) ** 2)
+    ok = s > 1e-9
+    z = (gL - oL)[ok] / s[ok]
+    say('z std', z.std(), 'frac |z|<3', np.mean(np.abs(z) < 3), 'means', gL.mean(), oL.mean())
+    assert 0.75 < z.std() < 1.3
+    assert np.mean(np.abs(z) < 3) > 0.97
+    se = math.sqrt(np.nansum(s ** 2)) / s.size
+    assert abs(gL.mean() - oL.mean()) < 4 * se + 1e-3 * oL.mean()
+
+
 # the Lambertian's 1e-3: the march's 400-step cap (the reference's maxiter) drops a path that leaves the plane within
 # about 0.7 degrees of grazing before it reaches the sky sphere, ~0.03 % of the light here
 def test_every_pixel_closes_every_sample(engine):

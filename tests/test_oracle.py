@@ -527,6 +527,121 @@ def test_unsound_outlines_are_refused(outline, why, tmp_path):
         _emit_doc(_single({'outline': outline}, 'revolution'), tmp_path)
 
 
+def _closest_on_triangle(p, a, b, c):
+    """The closest point of triangle abc to p (Ericson, Real-Time Collision Detection, 5.1.5): by Voronoi region."""
+    ab, ac, ap = b - a, c - a, p - a
+    d1, d2 = ab @ ap, ac @ ap
+    if d1 <= 0 and d2 <= 0:
+        return a
+    bp = p - b
+    d3, d4 = ab @ bp, ac @ bp
+    if d3 >= 0 and d4 <= d3:
+        return b
+    vc = d1 * d4 - d3 * d2
+    if vc <= 0 and d1 >= 0 and d3 <= 0:
+        return a + d1 / (d1 - d3) * ab
+    cp = p - c
+    d5, d6 = ab @ cp, ac @ cp
+    if d6 >= 0 and d5 <= d6:
+        return c
+    vb = d5 * d2 - d1 * d6
+    if vb <= 0 and d2 >= 0 and d6 <= 0:
+        return a + d2 / (d2 - d6) * ac
+    va = d3 * d6 - d5 * d4
+    if va <= 0 and (d4 - d3) >= 0 and (d5 - d6) >= 0:
+        return b + (d4 - d3) / ((d4 - d3) + (d5 - d6)) * (c - b)
+    den = 1 / (va + vb + vc)
+    return a + ab * (vb * den) + ac * (vc * den)
+
+
+def _curve_dist(q, curve):
+    """Distance from each query point to a dense point set (a sampled curve or surface)."""
+    return np.array([np.sqrt(((curve - x) ** 2).sum(1)).min() for x in q])
+
+
+def test_cone_and_tube_are_their_profiles_revolved(tmp_path):
+    # sdf.js sdCappedCone and shapes.js Ntube: exact both sides -- the trapezoid (0, -h) (base, -h) (top, h) (0, h)
+    # and the box [inner, outer] x [-h, h], turned about z (their axis edge no surface), by brute force
+    pts = RNG.uniform(-0.9, 0.9, (800, 3))
+    q = np.stack([np.hypot(pts[:, 0], pts[:, 1]), pts[:, 2]], 1)
+    f32 = lambda x: float(np.float32(x))   # noqa: E731
+    for base, top in ((0.5, 0.0), (0.5, 0.2), (0.1, 0.6), (0.4, 0.4)):
+        sc = _emit_doc(_single({'base': base, 'top': top, 'height': 1.0}, 'cone'), tmp_path)
+        V = np.array([[0, -0.5], [f32(base), -0.5], [f32(top), 0.5], [0, 0.5]])
+        np.testing.assert_allclose(oracle.scene_at(sc, pts)[0], _poly_sdf(q, V, [True, True, True, False]), atol=2e-7)
+    for inner in (0.0, 0.25):
+        sc = _emit_doc(_single({'outer': 0.5, 'inner': inner, 'height': 0.8}, 'tube'), tmp_path)
+        lo = max(inner, 0.0)
+        V = np.array([[f32(lo), -f32(0.4)], [0.5, -f32(0.4)], [0.5, f32(0.4)], [f32(lo), f32(0.4)]])
+        d, nrm, _ = oracle.scene_at(sc, pts)
+        np.testing.assert_allclose(d, _poly_sdf(q, V, [True, True, True, inner > 0]), atol=2e-7)
+        assert _normals_are_gradients(sc, pts) > 0.97
+
+
+def test_round_cone_octahedron_and_dome_against_brute_force(tmp_path):
+    pts = RNG.uniform(-0.9, 0.9, (400, 3))
+    # the round cone: equal radii make it the capsule; otherwise the hull of its two spheres, sampled densely
+    sc = _emit_doc(_single({'bottom': 0.2, 'top': 0.2, 'height': 0.6}, 'round cone'), tmp_path)
+    c = np.stack([0 * pts[:, 0], 0 * pts[:, 0], np.clip(pts[:, 2], -0.3, 0.3)], 1)
+    np.testing.assert_allclose(oracle.scene_at(sc, pts)[0], np.linalg.norm(pts - c, axis=1) - float(np.float32(0.2)), atol=2e-7)
+    r1, r2, h = 0.3, 0.15, 0.55
+    sc = _emit_doc(_single({'bottom': r1, 'top': r2, 'height': h}, 'round cone'), tmp_path)
+    t = np.linspace(0, 2 * np.pi, 40000, endpoint=False)
+    bnd = np.concatenate([np.c_[r1 * np.cos(t), r1 * np.sin(t) - h / 2], np.c_[r2 * np.cos(t), r2 * np.sin(t) + h / 2]])
+    b = (r1 - r2) / h
+    a = math.sqrt(1 - b * b)
+    s = np.linspace(0, 1, 40000)[:, None]
+    bnd = np.concatenate([bnd, (1 - s) * np.array([r1 * a, -h / 2 + r1 * b]) + s * np.array([r2 * a, h / 2 + r2 * b])])
+    d, _, _ = oracle.scene_at(sc, pts)
+    q = np.stack([np.hypot(pts[:, 0], pts[:, 1]), pts[:, 2]], 1)
+    out = d > 0.01
+    np.testing.assert_allclose(d[out], _curve_dist(q[out], bnd), atol=1e-4)
+    # the octahedron: inside its nearest face plane, outside the nearest point of its eight triangles (Ericson)
+    s8 = 0.5
+    sc = _emit_doc(_single({'size': s8}, 'octahedron'), tmp_path)
+    d, nrm, _ = oracle.scene_at(sc, pts)
+    corners = [np.array(v, float) * s8 for v in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1))]
+    tris = [(corners[i], corners[j], corners[k]) for i in (0, 1) for j in (2, 3) for k in (4, 5)]
+    for x, dx in zip(pts, d):
+        m = np.abs(x).sum() - s8
+        want = m / math.sqrt(3) if m <= 0 else min(np.linalg.norm(x - _closest_on_triangle(x, *t3)) for t3 in tris)
+        assert abs(dx - want) < 2e-7, (x, dx, want)
+    assert _normals_are_gradients(sc, pts) > 0.97
+    # the dome: the sphere's part above z = 0 -- its cap's arc and its base disc, sampled densely in the meridian plane
+    R, H = 0.5, 0.3
+    sc = _emit_doc(_single({'radius': R, 'height': H}, 'dome'), tmp_path)
+    cz = H - R
+    w = math.sqrt(R * R - cz * cz)
+    th = np.linspace(math.atan2(-cz, w), math.pi / 2, 60000)
+    bnd = np.concatenate([np.c_[R * np.cos(th), cz + R * np.sin(th)], np.c_[np.linspace(0, w, 30000), np.zeros(30000)]])
+    d, _, _ = oracle.scene_at(sc, pts)
+    q = np.stack([np.hypot(pts[:, 0], pts[:, 1]), pts[:, 2]], 1)
+    inside = (np.hypot(q[:, 0], q[:, 1] - cz) < R) & (q[:, 1] > 0)
+    np.testing.assert_allclose(np.abs(d), _curve_dist(q, bnd), atol=5e-5)
+    assert ((d < 0) == inside).all()
+
+
+def test_capped_torus_and_link_against_brute_force(tmp_path):
+    pts = RNG.uniform(-0.8, 0.8, (400, 3))
+    f32 = lambda x: float(np.float32(x))   # noqa: E731
+    # 360 degrees is the torus; 200 degrees the arc symmetric about +y, sampled densely, less the tube
+    sc = _emit_doc(_single({'ring': 0.42, 'tube': 0.08, 'angle': 360}, 'capped torus'), tmp_path)
+    np.testing.assert_allclose(oracle.scene_at(sc, pts)[0], np.hypot(np.hypot(pts[:, 0], pts[:, 1]) - f32(0.42), pts[:, 2]) - f32(0.08), atol=2e-6)
+    sc = _emit_doc(_single({'ring': 0.42, 'tube': 0.08, 'angle': 200}, 'capped torus'), tmp_path)
+    a = np.radians(np.linspace(90 - 100, 90 + 100, 200001))
+    arc = np.c_[0.42 * np.cos(a), 0.42 * np.sin(a), 0 * a]
+    np.testing.assert_allclose(oracle.scene_at(sc, pts)[0], _curve_dist(pts, arc) - 0.08, atol=1e-5)
+    # a link of span 0 is the torus; with span, the stadium curve less the tube
+    sc = _emit_doc(_single({'span': 0.0, 'ring': 0.25, 'tube': 0.07}, 'link'), tmp_path)
+    np.testing.assert_allclose(oracle.scene_at(sc, pts)[0], np.hypot(np.hypot(pts[:, 0], pts[:, 1]) - f32(0.25), pts[:, 2]) - f32(0.07), atol=2e-6)
+    sc = _emit_doc(_single({'span': 0.36, 'ring': 0.25, 'tube': 0.07}, 'link'), tmp_path)
+    t = np.linspace(0, np.pi, 100001)
+    st = np.concatenate([np.c_[0.25 * np.cos(t), 0.18 + 0.25 * np.sin(t)], np.c_[0.25 * np.cos(t), -0.18 - 0.25 * np.sin(t)],
+                         np.c_[np.full(20001, 0.25), np.linspace(-0.18, 0.18, 20001)], np.c_[np.full(20001, -0.25), np.linspace(-0.18, 0.18, 20001)]])
+    curve = np.c_[st, 0 * st[:, 0]]
+    np.testing.assert_allclose(oracle.scene_at(sc, pts)[0], _curve_dist(pts, curve) - 0.07, atol=1e-5)
+
+
 def test_studio_floor_reads_rho_E_over_pi():
     # environment.js studioLight: L0 (1 + top d.z) with L0 = E / (pi (1 + 2 top / 3)) lights a level surface with E
     # whatever top is, so a Lambertian floor of albedo 0.6 under 500 lx reads 0.6 x 500 / pi; the rays that leave
