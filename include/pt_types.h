@@ -108,7 +108,9 @@ typedef struct PtLightFace {
 } PtLightFace;
 
 // Environment: 0 black, 1 uniform sphere, 2 uniform upper hemisphere (z > 0), 3 the studio -- L (1 + top d.z), never
-// below 0, over the whole sphere (core/environment.js studioLight:165-174) -- plus an optional sun.
+// below 0, over the whole sphere (core/environment.js studioLight:165-174) -- 4 the sun & sky (environment.js
+// SUNSKY_FN:110: L is the sunlight above the air, lux; the sky scattered once in sixteen bands) -- plus an optional
+// sun.
 typedef struct PtEnv {
     unsigned int type;
     float L;              // luminance, cd/m^2
@@ -122,6 +124,12 @@ typedef struct PtEnv {
                           // photon pass emits the beam from that disc)
     float top;            // the studio's top light, 0..1 (type 3)
     float pad;
+    float sunL[3];        // type 4: the sun disc's radiance, RGB cd/m^2 (other types: sunE over the solid angle, white)
+    float sunSin2;        // sin^2 of the sun's angular radius: a direction d is in the disc when |d x sunDir|^2 <= this
+    float sunOmega;       // the disc's solid angle, 2 pi (1 - cos), worked out in double
+    float skyAerosol;     // type 4: the haze's optical depth at 550 nm
+    float skyMs;          // type 4: the air mass the sunlight crosses
+    float skyUp;          // type 4: 1 while the sun is up, fading to 0 over its last 3 degrees below the horizon
 } PtEnv;
 
 // The homogeneous haze box (the reference's haze.js hazeUniforms): axis-aligned, centre and half size.

@@ -80,7 +80,7 @@ static void sampleSeed(uint32_t base, uint32_t sample, uint32_t* out) {
 /* ---- the scene program (hipgen.js `program`) ---- */
 enum { OP_PRIM = 0, OP_JOIN = 1, OP_FOLD = 2, OP_NODE = 3 };
 enum { SH_PLANE = 0, SH_SPHERE, SH_BOX, SH_CYLINDER, SH_CAPSULE, SH_TORUS, SH_RECT, SH_DISC, SH_LENS, SH_PRISM, SH_REFLECTOR, SH_ASPHERE,
-       SH_REVOLUTION, SH_FRESNEL, SH_EXTRUSION, SH_CONE, SH_ROUNDCONE, SH_OCTA, SH_CTORUS, SH_LINK, SH_TUBE, SH_DOME };
+       SH_REVOLUTION, SH_FRESNEL, SH_EXTRUSION, SH_CONE, SH_ROUNDCONE, SH_OCTA, SH_CTORUS, SH_LINK, SH_TUBE, SH_DOME, SH_FRUSTUM };
 enum { J_UNION = 0, J_SUBTRACT, J_INTERSECT };
 /* prim: shape, xf, p, mat, prim; join: join, k; node (a top-level node's start): xf = its bounding sphere's offset in
  * U (-1: unbounded), join = its mask bit (-1: none), k = the index of its fold */
@@ -223,6 +223,22 @@ static g2v box2(double qx, double qy, double cx, double cy, double bx, double by
     g.x = 0; g.y = sy; g.d = ey; return g;
 }
 static d3 safeDirD(d3 a, d3 fb) { double l = len(a); return l > 1e-12 ? scl(a, 1 / l) : fb; }
+/* The nearest point of the planar convex polygon c[0..3] (either winding; corners may coincide) with unit normal n:
+   the projection when the turn about n has one sign along every edge, else the nearest point of the nearest edge. */
+static d3 polyNearD(d3 p, const d3* c, d3 n) {
+    d3 q = sub(p, scl(n, dotp(sub(p, c[0]), n)));
+    int pos = 0, neg = 0;
+    for (int i = 0; i < 4; ++i) { double t = dotp(crossp(sub(c[(i + 1) & 3], c[i]), sub(q, c[i])), n); pos += t > 0; neg += t < 0; }
+    if ((pos == 0) != (neg == 0)) return q;      /* one sign only: inside (none at all: no area) */
+    d3 best = c[0]; double bd = 1e300;
+    for (int i = 0; i < 4; ++i) {
+        d3 a = c[i], ab = sub(c[(i + 1) & 3], a);
+        double t = clampd(dotp(sub(p, a), ab) / maxd(dotp(ab, ab), 1e-300), 0, 1);
+        d3 e = add(a, scl(ab, t)); double de = dotp(sub(p, e), sub(p, e));
+        if (de < bd) { bd = de; best = e; }
+    }
+    return best;
+}
 
 /* Primitive (outward normal, distance) in its own frame: core/sdf.js:352-628 restated; the lens core/sdf.js:463, the
  * prism core/shapes.js:263-287 (a regular polygon extruded), the reflector core/shapes.js:392-421, the aspheric lens
@@ -365,6 +381,20 @@ static d4 primLocal(int shape, d3 p, const float* P) {
         if (P[1] <= 0) return revolveD(p, box2(qx, p.z, 0, 0, maxd(P[0], 0), hh));
         double ro = maxd(P[0], P[1] + 1e-6);
         return revolveD(p, box2(qx, p.z, 0.5 * (P[1] + ro), 0, 0.5 * (ro - P[1]), hh));
+    }
+    case SH_FRUSTUM: {      /* P: base x, y, top x, y, height -- all six faces, no folding */
+        double bx = 0.5 * fabs(P[0]), by = 0.5 * fabs(P[1]), tx = 0.5 * fabs(P[2]), ty = 0.5 * fabs(P[3]), hh = 0.5 * maxd(P[4], 1e-6);
+        d3 Bc[4] = {v(bx, -by, -hh), v(bx, by, -hh), v(-bx, by, -hh), v(-bx, -by, -hh)};
+        d3 Tc[4] = {v(tx, -ty, hh), v(tx, ty, hh), v(-tx, ty, hh), v(-tx, -ty, hh)};
+        d3 nrm[6] = {unit(v(2 * hh, 0, bx - tx)), unit(v(0, 2 * hh, by - ty)), unit(v(-2 * hh, 0, bx - tx)), unit(v(0, -2 * hh, by - ty)), v(0, 0, 1), v(0, 0, -1)};
+        d3 F[6][4];
+        for (int i = 0; i < 4; ++i) { F[i][0] = Bc[i]; F[i][1] = Bc[(i + 1) & 3]; F[i][2] = Tc[(i + 1) & 3]; F[i][3] = Tc[i]; F[4][i] = Tc[i]; F[5][i] = Bc[i]; }
+        double best = -1e300; int bi = 0, inside = 1;
+        for (int f = 0; f < 6; ++f) { double d = dotp(sub(p, F[f][0]), nrm[f]); if (d > 0) inside = 0; if (d > best) { best = d; bi = f; } }
+        if (inside) return d4of(nrm[bi], best);
+        double d2 = 1e300; d3 c = p;
+        for (int f = 0; f < 6; ++f) { d3 q = polyNearD(p, F[f], nrm[f]); double e = dotp(sub(p, q), sub(p, q)); if (e < d2) { d2 = e; c = q; } }
+        return d4of(safeDirD(sub(p, c), nrm[bi]), sqrt(d2));
     }
     case SH_DOME: {         /* P: radius, height */
         double R = maxd(P[0], 1e-6), c = clampd(P[1] - R, -R, R * (1 - 1e-4)), hc = -c, w = sqrt(maxd(R * R - hc * hc, 0));
@@ -713,11 +743,77 @@ void or_spectral_weight(double lamUm, double* rgb) {
 
 /* ---- environment, sun, light faces ---- */
 static int envLit(const PtEnv* e) { return (e->type != 0 && e->L > 0) || e->sunE > 0; }
-static double sunOmega(const PtEnv* e) { return 2 * PI_D * (1 - e->sunCos); }
-static int inSun(const PtEnv* e, d3 d) { return e->sunE > 0 && dotp(d, f3(e->sunDir)) >= e->sunCos; }
+static double sunOmega(const PtEnv* e) { return e->sunOmega; }
+/* in the disc: |d x s|^2 <= sin^2 on the sun's side (sdf.js inSunDisc:88) */
+static int inSun(const PtEnv* e, d3 d) { d3 s = f3(e->sunDir), c = crossp(d, s); return e->sunE > 0 && dotp(d, s) > 0 && dotp(c, c) <= e->sunSin2; }
+
+/* ---- the sun & sky (environment.js SKY_BAND_TABLE:66, SUNSKY_FN:110-170), in double: sixteen bands from 380 to 730
+ * nm, each with its Rayleigh depth (Hansen & Travis 1974 eq. 2.29), haze factor (Angstrom, 1.3), ozone depth (300 DU of
+ * the band's Chappuis cross-section) and its share of 5778 K sunlight in linear Rec.709 (Wyman-Sloan-Shirley's 2013
+ * fit to the CIE 1931 functions, summed at 1 nm, normalised to total luminance 1). */
+static double skyTauR[16], skyHaze[16], skyOzone[16], skyRgb[16][3];
+static int skyReady = 0;
+static double wssLobe(double x, double mu, double s1, double s2) { double t = (x - mu) / (x < mu ? s1 : s2); return exp(-0.5 * t * t); }
+static void skyTable(void) {
+    static const double sigma[16] = {6.2648e-24, 3.1169e-23, 1.0195e-22, 2.4886e-22, 5.4883e-22, 1.1517e-21, 2.0211e-21, 3.0497e-21,
+                                     4.2480e-21, 4.5252e-21, 4.7153e-21, 3.4726e-21, 2.3885e-21, 1.5936e-21, 9.8370e-22, 6.3061e-22};
+    /* linear Rec.709 from XYZ: the inverse of IEC 61966-2-1's four-decimal matrix */
+    double M[3][3] = {{0.4124, 0.3576, 0.1805}, {0.2126, 0.7152, 0.0722}, {0.0193, 0.1192, 0.9505}}, I[3][3];
+    double det = M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) - M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) + M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) {
+        int a = (j + 1) % 3, b = (j + 2) % 3, c = (i + 1) % 3, d = (i + 2) % 3;
+        I[i][j] = (M[a][c] * M[b][d] - M[a][d] * M[b][c]) / det;
+    }
+    double step = (730.0 - 380.0) / 16, tot[3] = {0, 0, 0};
+    for (int k = 0; k < 16; ++k) {
+        double X = 0, Y = 0, Z = 0, wl = 0, ws = 0;
+        for (double l = 380 + k * step; l < 380 + (k + 1) * step; l += 1) {
+            double n = floor(l + 0.5), m = l * 1e-9;
+            double x = 1.056 * wssLobe(n, 599.8, 37.9, 31.0) + 0.362 * wssLobe(n, 442.0, 16.0, 26.7) - 0.065 * wssLobe(n, 501.1, 20.4, 26.2);
+            double y = 0.821 * wssLobe(n, 568.8, 46.9, 40.5) + 0.286 * wssLobe(n, 530.9, 16.3, 31.1);
+            double z = 1.217 * wssLobe(n, 437.0, 11.8, 36.0) + 0.681 * wssLobe(n, 459.0, 26.0, 13.8);
+            double w = 1 / (pow(m, 5) * (exp(1.438776877e-2 / (m * 5778)) - 1));
+            X += w * x; Y += w * y; Z += w * z; wl += w * (x + y + z) * l; ws += w * (x + y + z);
+        }
+        double nm = wl / ws, um = nm / 1000;
+        skyTauR[k] = 0.008569 * pow(um, -4) * (1 + 0.0113 * pow(um, -2) + 0.00013 * pow(um, -4));
+        skyHaze[k] = pow(nm / 550, -1.3);
+        skyOzone[k] = sigma[k] * 300 * 2.6867e16;
+        for (int i = 0; i < 3; ++i) { skyRgb[k][i] = I[i][0] * X + I[i][1] * Y + I[i][2] * Z; tot[i] += skyRgb[k][i]; }
+    }
+    double Yt = 0.2126 * tot[0] + 0.7152 * tot[1] + 0.0722 * tot[2];
+    for (int k = 0; k < 16; ++k) for (int i = 0; i < 3; ++i) skyRgb[k][i] /= Yt;
+#pragma omp flush
+    skyReady = 1;
+}
+/* Kasten-Young relative air mass (sdf.js airMass:96) */
+static double airMassD(double cz) {
+    double c = clampd(cz, 0, 1), z = acos(c) * 180 / PI_D;
+    return 1 / (c + 0.15 * pow(maxd(93.885 - z, 0.1), -1.253));
+}
+static void skyInit(void) {
+    if (skyReady) return;
+#pragma omp critical(skytable)
+    { if (!skyReady) skyTable(); }
+}
+static d3 sunSkyD(const PtEnv* e, d3 d) {
+    skyInit();
+    d3 s = f3(e->sunDir);
+    double mv = airMassD(d.z), ms = airMassD(s.z), dm = mv - ms, c = dotp(d, s), g = 0.7;
+    double pR = 3 / (16 * PI_D) * (1 + c * c), pM = (1 - g * g) / (4 * PI_D * pow(1 + g * g - 2 * g * c, 1.5));
+    double up = clampd((s.z + 0.05) / 0.05, 0, 1), aer = e->skyAerosol, L[3] = {0, 0, 0};
+    for (int k = 0; k < 16; ++k) {
+        double ta = aer * skyHaze[k], tau = skyTauR[k] + ta + skyOzone[k];
+        double k1 = fabs(dm) < 1e-3 ? tau * mv * exp(-tau * mv) : mv * (exp(-tau * ms) - exp(-tau * mv)) / dm;
+        double w = (skyTauR[k] * pR + 0.9 * ta * pM) / tau * k1 * up;
+        for (int i = 0; i < 3; ++i) L[i] += skyRgb[k][i] * w;
+    }
+    return v(maxd(e->L * L[0], 0), maxd(e->L * L[1], 0), maxd(e->L * L[2], 0));
+}
 static d3 envBg(const PtEnv* e, d3 d) {
     if (e->type == 1 || (e->type == 2 && d.z > 0)) return scl(f3(e->col), e->L);
     if (e->type == 3) return scl(f3(e->col), e->L * maxd(1 + e->top * d.z, 0));   /* environment.js studioLight:168 */
+    if (e->type == 4) return inSun(e, d) ? v(0, 0, 0) : sunSkyD(e, d);
     return v(0, 0, 0);
 }
 /* the sun through its aperture: the lines crossing the disc of radius sunRadius about sunPoint, perpendicular to the
@@ -730,6 +826,7 @@ static d3 envSun(const PtEnv* e, d3 p, d3 d) {
         d3 q = sub(add(p, scl(d, t)), c);
         if (!(dotp(q, q) <= (double)e->sunRadius * e->sunRadius)) return v(0, 0, 0);
     }
+    if (e->type == 4) return f3(e->sunL);
     double L = e->sunE / sunOmega(e);
     return v(L, L, L);
 }
@@ -745,7 +842,7 @@ static d3 cosDir(d3 n, double u1, double u2) {
 }
 static d3 sunDir(const PtEnv* e, double u1, double u2) {
     d3 a = f3(e->sunDir), t, b; basis(a, &t, &b);
-    double c = 1 - u1 * (1 - e->sunCos), s = sqrt(maxd(0, 1 - c * c)), ph = 2 * PI_D * u2;
+    double omc = u1 * e->sunOmega / (2 * PI_D), c = 1 - omc, s = sqrt(maxd(0, omc * (2 - omc))), ph = 2 * PI_D * u2;
     return unit(add(add(scl(t, s * cos(ph)), scl(b, s * sin(ph))), scl(a, c)));
 }
 static double misW(double a, double b) { a *= a; b *= b; return a / maxd(a + b, 1e-20); }
@@ -1239,6 +1336,7 @@ static void sceneInit(OrScene* S, const int* ops, int nops, const float* U, cons
 int or_render(const int* ops, int nops, const float* U, const PtMaterial* mats, int nmats, const PtLightFace* faces,
               int nfaces, const PtUniforms* u, uint32_t first, uint32_t samples, double* accum, double* moments,
               long long* stats, int threads) {
+    skyInit();
     OrOp* buf = (OrOp*)malloc(sizeof(OrOp) * (nops > 0 ? nops : 1));
     OrScene S; sceneInit(&S, ops, nops, U, mats, nmats, faces, nfaces, u, buf);
     or_mf_tables(NULL);
@@ -1364,6 +1462,16 @@ int or_bsdf(const PtMaterial* m, const double* n3, const double* view3, double e
 }
 
 /* Haze helpers for closed-form checks: transmittance along a ray, and the free-flight draw. */
+/* The environment's radiance in each unit direction: the sky (envBg) and the sun's disc (envSun, from the origin). */
+int or_env(const PtUniforms* u, const double* dirs, int n, double* out) {
+    skyInit();
+    for (int i = 0; i < n; ++i) {
+        d3 d = unit(v(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+        d3 L = add(envBg(&u->env, d), envSun(&u->env, v(0, 0, 0), d));
+        out[3 * i] = L.x; out[3 * i + 1] = L.y; out[3 * i + 2] = L.z;
+    }
+    return 0;
+}
 double or_haze_tr(const PtHaze* h, const double* o, const double* d, double lenMax) { return hazeTr(h, v(o[0], o[1], o[2]), v(d[0], d[1], d[2]), lenMax); }
 double or_haze_draw(const PtHaze* h, const double* o, const double* d, double u) { return hazeDraw(h, v(o[0], o[1], o[2]), v(d[0], d[1], d[2]), u); }
 double or_hg(double g, double c) { return hg(g, c); }

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, '_build', 'liboracle.so')
 
 SHAPES = ['plane', 'sphere', 'box', 'cylinder', 'capsule', 'torus', 'rectangle', 'disc', 'lens', 'prism', 'reflector', 'aspheric lens',
-          'revolution', 'fresnel lens', 'extrusion', 'cone', 'round cone', 'octahedron', 'capped torus', 'link', 'tube', 'dome']
+          'revolution', 'fresnel lens', 'extrusion', 'cone', 'round cone', 'octahedron', 'capped torus', 'link', 'tube', 'dome', 'pyramid']
 JOINS = ['union', 'subtract', 'intersect']
 OPS = {'prim': 0, 'join': 1, 'fold': 2, 'node': 3}
 
@@ -133,6 +133,14 @@ def pick(scene, u, sample=0, first=0, count=None):
     info = np.zeros((n, 2), np.uint32)
     lib().or_pick(*S.args(), ctypes.byref(u), sample, first, n, _dp(pos), info.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
     return pos, info[:, 0] & 0xFF, info[:, 0] >> 8, info[:, 1]
+
+
+def env(u, dirs):
+    """The environment's radiance (sky and sun disc), RGB, in each direction."""
+    dirs = np.ascontiguousarray(np.asarray(dirs, np.float64).reshape(-1, 3))
+    out = np.zeros((len(dirs), 3))
+    lib().or_env(ctypes.byref(u), _dp(dirs), len(dirs), _dp(out))
+    return out
 
 
 def scene_at(scene, pts):

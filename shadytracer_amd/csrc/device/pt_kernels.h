@@ -143,11 +143,49 @@ __device__ __forceinline__ unsigned int mzPack(unsigned int b, unsigned int s, u
 
 // ---------------------------------------------------------------------------------------------------
 // environment (uniform sphere or hemisphere, and a sun disc), its light-sampling draw and density
-__device__ __forceinline__ float sunSolidAngle(const PtEnv& e) { return 2.0f * PT_PI * (1.0f - e.sunCos); }
-__device__ __forceinline__ bool inSun(const PtEnv& e, V3 d) { return e.sunE > 0.0f && dot(d, ld3(e.sunDir)) >= e.sunCos; }
+// The disc's solid angle comes from the host in double: 1 - cos of the real sun's radius is 1.07e-5, which f32 holds
+// only to 0.3 %. Membership is |d x s|^2 <= sin^2 on the sun's side (sdf.js inSunDisc:88), which keeps its relative
+// precision near the axis where a cosine test does not.
+__device__ __forceinline__ float sunSolidAngle(const PtEnv& e) { return e.sunOmega; }
+__device__ __forceinline__ bool inSun(const PtEnv& e, V3 d) {
+    const V3 s = ld3(e.sunDir), c = cross(d, s);
+    return e.sunE > 0.0f && dot(d, s) > 0.0f && dot(c, c) <= e.sunSin2;
+}
+// Kasten-Young relative air mass (sdf.js airMass:96): 1 straight up, about 38 at the horizon
+__device__ __forceinline__ float airMass(float cz) {
+    const float c = fminf(fmaxf(cz, 0.0f), 1.0f);
+    const float zdeg = acosf(c) * (180.0f / PT_PI);
+    return 1.0f / (c + 0.15f * __powf(fmaxf(93.885f - zdeg, 0.1f), -1.253f));
+}
+// The sun & sky's sky (environment.js SUNSKY_FN:130-168): the sunlight scattered once towards the eye by the air
+// (Rayleigh phase) and the haze (Henyey-Greenstein g 0.7, albedo 0.9) over the whole column, each band attenuated by
+// the extinction along the sun's path and the view's, the ozone absorbing only.
+__device__ __forceinline__ V3 sunSky(const PtEnv& e, V3 d) {
+    const V3 s = ld3(e.sunDir);
+    const float mv = airMass(d.z), ms = e.skyMs, dm = mv - ms;
+    const float c = dot(d, s), g = 0.7f;
+    const float pR = (3.0f / (16.0f * PT_PI)) * (1.0f + c * c);
+    const float x = 1.0f + g * g - 2.0f * g * c;
+    const float pM = (1.0f - g * g) / (4.0f * PT_PI * x * sqrtf(x));
+    const bool same = fabsf(dm) < 1e-3f;
+    const float idm = same ? 0.0f : 1.0f / dm;
+    float r = 0.0f, gr = 0.0f, b = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const float tr = PT_SKY_TAUR[k], ta = e.skyAerosol * PT_SKY_HAZE[k];
+        const float tau = tr + ta + PT_SKY_OZONE[k];
+        const float ev = __expf(-tau * mv);
+        const float k1 = same ? tau * mv * ev : mv * (__expf(-tau * ms) - ev) * idm;
+        const float v = (tr * pR + 0.9f * ta * pM) / tau * k1;
+        r += PT_SKY_RGB[3 * k] * v; gr += PT_SKY_RGB[3 * k + 1] * v; b += PT_SKY_RGB[3 * k + 2] * v;
+    }
+    const float f = e.L * e.skyUp;
+    return v3(fmaxf(f * r, 0.0f), fmaxf(f * gr, 0.0f), fmaxf(f * b, 0.0f));
+}
 __device__ __forceinline__ V3 envBackground(const PtEnv& e, V3 d) {
     if (e.type == 1u || (e.type == 2u && d.z > 0.0f)) return ld3(e.col) * e.L;
     if (e.type == 3u) return ld3(e.col) * (e.L * fmaxf(1.0f + e.top * d.z, 0.0f));
+    if (e.type == 4u) return inSun(e, d) ? v3s(0.0f) : sunSky(e, d);      // the disc is envSun's
     return v3s(0.0f);
 }
 // The sun's radiance along the ray from p towards d. With an aperture the sun is a beam: the lines that cross the disc
@@ -162,7 +200,7 @@ __device__ __forceinline__ V3 envSun(const PtEnv& e, V3 p, V3 d) {
         V3 q = p + d * t - c;
         if (!(dot(q, q) <= e.sunRadius * e.sunRadius)) return v3s(0.0f);
     }
-    return v3s(e.sunE / sunSolidAngle(e));
+    return e.type == 4u ? ld3(e.sunL) : v3s(e.sunE / sunSolidAngle(e));
 }
 __device__ __forceinline__ float sunPick(const PtEnv& e) { return e.sunE > 0.0f ? 0.5f : 0.0f; }
 __device__ __forceinline__ V3 sampleCosine(V3 n, float u1, float u2) {
@@ -174,8 +212,10 @@ __device__ __forceinline__ V3 sampleCosine(V3 n, float u1, float u2) {
 __device__ __forceinline__ V3 sampleSunCone(const PtEnv& e, float u1, float u2) {
     V3 a = ld3(e.sunDir);
     V3 t, b; frameOf(a, t, b);
-    float c = 1.0f - u1 * (1.0f - e.sunCos);
-    float s = sqrtf(fmaxf(0.0f, 1.0f - c * c));
+    // in 1 - cos, whose relative precision f32 keeps: sin = sqrt(omc (2 - omc)) (sdf.js sampleSunCone:132)
+    float omc = u1 * e.sunOmega * (0.5f * PT_INV_PI);
+    float c = 1.0f - omc;
+    float s = sqrtf(fmaxf(0.0f, omc * (2.0f - omc)));
     float phi = 2.0f * PT_PI * u2;
     float sp, cp; __sincosf(phi, &sp, &cp);
     return normalize(t * (s * cp) + b * (s * sp) + a * c);

@@ -383,6 +383,57 @@ __device__ __forceinline__ V4 Noctahedron(V3 p, float s0) {
     }
     return v4(face, m * c);
 }
+// The builder's pyramid, a frustum along z (shapes.js Nfrustum:204): a base.x by base.y rectangle at z = -h/2 and a
+// top.x by top.y one at +h/2 (0 by 0: an apex). Convex and mirror-symmetric in x and y, so after folding to the
+// positive quadrant the nearest surface point is on the +x side, the +y side, the top or the base. Inside, the
+// nearest of those planes; outside, the nearest of the four faces' nearest points. Exact both sides.
+__device__ __forceinline__ V3 segNear(V3 P, V3 a, V3 b) {
+    const V3 ab = b - a;
+    return a + ab * fminf(fmaxf(dot(P - a, ab) / fmaxf(dot(ab, ab), 1e-30f), 0.0f), 1.0f);
+}
+// the nearest point of the planar convex quad a b c d (counter-clockwise about its unit normal n; two corners may
+// coincide): the point's projection when it lies within all four edges, else the nearest point of the nearest edge
+__device__ __forceinline__ V3 quadNear(V3 P, V3 a, V3 b, V3 c, V3 d, V3 n) {
+    const V3 q = P - n * dot(P - a, n);
+    if (dot(cross(b - a, q - a), n) >= 0.0f && dot(cross(c - b, q - b), n) >= 0.0f &&
+        dot(cross(d - c, q - c), n) >= 0.0f && dot(cross(a - d, q - d), n) >= 0.0f) return q;
+    V3 m = segNear(P, a, b);
+    float dm = dot(P - m, P - m);
+    const V3 e[3] = { segNear(P, b, c), segNear(P, c, d), segNear(P, d, a) };
+#pragma unroll
+    for (int i = 0; i < 3; ++i) { const float de = dot(P - e[i], P - e[i]); if (de < dm) { dm = de; m = e[i]; } }
+    return m;
+}
+__device__ __forceinline__ V4 Nfrustum(V3 p, float bx0, float by0, float tx0, float ty0, float height) {
+    const float bx = 0.5f * fabsf(bx0), by = 0.5f * fabsf(by0), tx = 0.5f * fabsf(tx0), ty = 0.5f * fabsf(ty0);
+    const float h = fmaxf(height, 1e-6f);
+    const float sx = p.x >= 0.0f ? 1.0f : -1.0f, sy = p.y >= 0.0f ? 1.0f : -1.0f;
+    const V3 P = v3(fabsf(p.x), fabsf(p.y), p.z + 0.5f * h);
+    const V3 nx = safeDir3(v3(h, 0.0f, bx - tx), v3(1.0f, 0.0f, 0.0f));
+    const V3 ny = safeDir3(v3(0.0f, h, by - ty), v3(0.0f, 1.0f, 0.0f));
+    const float dx = dot(P - v3(bx, 0.0f, 0.0f), nx), dy = dot(P - v3(0.0f, by, 0.0f), ny);
+    V3 g;
+    float d;
+    if (dx <= 0.0f && dy <= 0.0f && P.z >= 0.0f && P.z <= h) {
+        d = dx; g = nx;
+        if (dy > d) { d = dy; g = ny; }
+        if (P.z - h > d) { d = P.z - h; g = v3(0.0f, 0.0f, 1.0f); }
+        if (-P.z > d) { d = -P.z; g = v3(0.0f, 0.0f, -1.0f); }
+    } else {
+        const V3 cx = quadNear(P, v3(bx, -by, 0.0f), v3(bx, by, 0.0f), v3(tx, ty, h), v3(tx, -ty, h), nx);
+        const V3 cy = quadNear(P, v3(bx, by, 0.0f), v3(-bx, by, 0.0f), v3(-tx, ty, h), v3(tx, ty, h), ny);
+        const V3 ct = v3(fminf(P.x, tx), fminf(P.y, ty), h), cb = v3(fminf(P.x, bx), fminf(P.y, by), 0.0f);
+        V3 c = cx, fb = nx;
+        float dc = dot(P - cx, P - cx);
+        const float d1 = dot(P - cy, P - cy), d2 = dot(P - ct, P - ct), d3 = dot(P - cb, P - cb);
+        if (d1 < dc) { dc = d1; c = cy; fb = ny; }
+        if (d2 < dc) { dc = d2; c = ct; fb = v3(0.0f, 0.0f, 1.0f); }
+        if (d3 < dc) { dc = d3; c = cb; fb = v3(0.0f, 0.0f, -1.0f); }
+        g = safeDir3(P - c, fb);
+        d = fsqrt(dc);
+    }
+    return v4(v3(g.x * sx, g.y * sy, g.z), d);
+}
 // Part of a torus in the xy plane: an arc of radius R, angle degrees of it symmetric about +y, swept by a tube of
 // radius r (shapes.js NcappedTorus:335): the nearest point of the arc -- its end past the wedge -- then the tube.
 // sc = (sin, cos) of half the angle, from the emitter.

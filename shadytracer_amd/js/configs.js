@@ -302,28 +302,45 @@ function turnedParts(opts = {}) {
 }
 
 // gallery: the remaining solid builtins at their defaults, 40 times (millimetres) -- a cone, a round cone, an
-// octahedron, a capped torus (270 degrees), a chain link, a tube and a dome -- in a row on a grey floor, a 3000 K
-// panel and the studio.
+// octahedron, a capped torus (270 degrees), a chain link, a tube, a dome and a pyramid -- in a row on a grey floor,
+// a 3000 K panel and the studio.
 function gallery(opts = {}) {
     const { cameraOf } = require('./hipgen')
-    const kinds = ['cone', 'round cone', 'octahedron', 'capped torus', 'link', 'tube', 'dome']
+    const kinds = ['cone', 'round cone', 'octahedron', 'capped torus', 'link', 'tube', 'dome', 'pyramid']
     const blocks = BLOCKS.concat(kinds.map(k => ({ id: `b-${k}`, label: k, kind: 'object', builtin: `${k}@1`, params: {} })))
     const mats = [material('floor', [0.5, 0.5, 0.5], 0.8, 0, { specular: 0.5 }), material('brass', [0.9, 0.7, 0.35], 0.3, 1),
         material('paint', [0.2, 0.35, 0.7], 0.5, 0)]
     // each sits on the floor: its local z extent at scale 40 (the octahedron on a corner, the torus and link on edge)
-    const place = { cone: [20, 0], 'round cone': [23, 0], octahedron: [20, 0], 'capped torus': [15.1, 90], link: [20, 90], tube: [20, 0], dome: [0, 0] }
+    const place = { cone: [20, 0], 'round cone': [23, 0], octahedron: [20, 0], 'capped torus': [15.1, 90], link: [20, 90], tube: [20, 0], dome: [0, 0], pyramid: [20, 0] }
     const children = [obj('floor', 'b-plane', 'floor', xf(0, 0, 0))].concat(kinds.map((k, i) =>
-        obj(k.replace(' ', '-'), `b-${k}`, i % 2 ? 'brass' : 'paint', xf(-126 + 42 * i, 8 * (i % 2), place[k][0], place[k][1], 0, 0, 40), {})))
+        obj(k.replace(' ', '-'), `b-${k}`, i % 2 ? 'brass' : 'paint', xf(-147 + 42 * i, 8 * (i % 2), place[k][0], place[k][1], 0, 0, 40), {})))
     children.push({ id: 'panel', kind: 'object', type: 'light', label: 'panel', enabled: true, material: null, transform: xf(0, 80, 140, 150, 0, 0),
         params: { width: 60, height: 40 }, light: { shape: 'rectangle', colourBy: 'cct', cct: 3000, flux: 400 } })
-    const d = studioDoc('gallery', mats, children, blocks,
+    const d = studioDoc(opts.sun ? 'sunlit' : 'gallery', mats, children, blocks,
         { mode: 'orbit', camtype: 0, fov: 40, target: [0, 0, 15], azimuth: -90, elevation: 65, distance: 230, up: [0, 0, 1], aperture: 0 })
+    if (opts.sun) d.environment = { envMode: 0, uSunAzimuth: -140, uSunElevation: 35, sunDiameter: 0.53, sunOutside: 128000, uSkyTurbidity: 0.7, sunHint: 1 }
     const w = opts.width || 160, h = opts.height || 64
     return { doc: d, camera: cameraOf(d, w, h), settings: { width: w, height: h, spp: 64, maxbounce: 8, nee: 1 } }
 }
 
+// sunlit: the gallery under the builder's default environment, the sun & sky (envMode 0): the sun 35 degrees up at
+// azimuth -140, 128 klx above the air, turbidity 0.7.
+const sunlit = (opts = {}) => gallery(Object.assign({}, opts, { sun: true }))
+// sunsky-floor: a Lambertian floor of albedo 0.6 under the sun & sky, the sun 40 degrees up: every pixel that sees the
+// floor reads 0.6 / pi (E_sun sin 40 + E_sky), the sun's light head-on times the sine of its elevation plus the sky's
+// on the level.
+function sunskyFloor(opts = {}) {
+    const { cameraOf } = require('./hipgen')
+    const d = studioDoc('sunsky-floor', [material('floor', [0.6, 0.6, 0.6], 1, 0, { specular: 0 })],
+        [obj('floor', 'b-plane', 'floor', xf(0, 0, 0))], BLOCKS,
+        { mode: 'orbit', camtype: 0, fov: 50, target: [0, 0, 0], azimuth: -90, elevation: 50, distance: 2000, up: [0, 0, 1], aperture: 0 })
+    d.environment = { envMode: 0, uSunAzimuth: 30, uSunElevation: 40, sunOutside: 128000, uSkyTurbidity: 0.7 }
+    const w = opts.width || 64, h = opts.height || 48
+    return { doc: d, camera: cameraOf(d, w, h), settings: { width: w, height: h, spp: 64, maxbounce: 4, nee: 1 } }
+}
+
 const CONFIGS = { 'plane-c1': planeC1, 'room-c2': roomC2, 'caustic-c3': causticC3, 'furnace-plane': furnacePlane, 'furnace-haze': furnaceHaze,
     'sppm-slab': sppmSlab, 'studio-floor': studioFloor, 'lens-prism': lensPrism, 'reflector-condenser': reflectorCondenser,
-    'turned-parts': turnedParts, gallery }
+    'turned-parts': turnedParts, gallery, sunlit, 'sunsky-floor': sunskyFloor }
 
 module.exports = { CONFIGS, camera, rng, SEED }

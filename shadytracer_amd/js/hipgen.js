@@ -20,6 +20,7 @@
  * environment. A code block that is not a builtin -- user WGSL -- is refused with its node named.
  */
 
+const sky = require('./sky')
 const UNIT_METRES = { m: 1, cm: 0.01, mm: 0.001, um: 1e-6, in: 0.0254 }
 const LUMA = [0.2126, 0.7152, 0.0722]
 
@@ -29,7 +30,7 @@ const LUMA = [0.2126, 0.7152, 0.0722]
 // (their polynomials and the division), an estimate.
 const PRIM_FLOPS = { plane: 1, sphere: 7, box: 27, cylinder: 24, capsule: 14, torus: 14, rectangle: 14, disc: 13, lens: 25, prism: 75,
     reflector: 80, 'aspheric lens': 52, revolution: 0, 'fresnel lens': 0, extrusion: 0, cone: 30, 'round cone': 20, octahedron: 24,
-    'capped torus': 22, link: 18, tube: 16, dome: 24 }
+    'capped torus': 22, link: 18, tube: 16, dome: 24, pyramid: 90 }
 const XF_FLOPS = 22      // p - pos, three dot products, times 1/s; the distance times s
 const FOLD_FLOPS = 2
 const JOIN_FLOPS = 9
@@ -94,6 +95,7 @@ const SHAPES = {
     link: { params: P => [P.span, P.ring, P.tube], dist: o => `Nlink(q, ${us(o, 3)}).w`, full: o => `Nlink(q, ${us(o, 3)})` },
     tube: { params: P => [P.outer, P.inner, P.height], dist: o => `Ntube(q, ${us(o, 3)}).w`, full: o => `Ntube(q, ${us(o, 3)})` },
     dome: { params: P => [P.radius, P.height], dist: o => `Ndome(q, ${us(o, 2)}).w`, full: o => `Ndome(q, ${us(o, 2)})` },
+    pyramid: { params: P => [P.base.x, P.base.y, P.top.x, P.top.y, P.height], dist: o => `Nfrustum(q, ${us(o, 5)}).w`, full: o => `Nfrustum(q, ${us(o, 5)})` },
 }
 function us(o, n) { return [...Array(n).keys()].map(i => `U[${o + i}]`).join(', ') }
 // the builtins' own defaults for parameters a document leaves out (builder/primitives.js:172 prism, :237 lens, :255
@@ -104,7 +106,7 @@ const DEFAULTS = { lens: { diameter: 1.0, thickness: 0.2, r1: 2.0, r2: 2.0 }, pr
         a8_2: 0, a10_2: 0 },
     cone: { base: 0.5, top: 0.0, height: 1.0 }, 'round cone': { bottom: 0.3, top: 0.15, height: 0.55 }, octahedron: { size: 0.5 },
     'capped torus': { ring: 0.42, tube: 0.08, angle: 270 }, link: { span: 0.36, ring: 0.25, tube: 0.07 }, tube: { outer: 0.5, inner: 0.4, height: 1.0 },
-    dome: { radius: 0.5, height: 0.3 } }
+    dome: { radius: 0.5, height: 0.3 }, pyramid: { base: { x: 1, y: 1 }, top: { x: 0, y: 0 }, height: 1 } }
 
 // ---- outlines (shapes.js profilePieces:1487): points (x, y, and a fillet radius if rounded), closed back to the first,
 // as the pieces a profile function walks -- segments, and arcs where a corner is rounded -- counter-clockwise, each
@@ -247,7 +249,8 @@ const LENGTHS = { sphere: ['radius'], box: ['size', 'rounding'], cylinder: ['rad
     torus: ['ring', 'tube'], rectangle: ['width', 'height'], disc: ['radius'], plane: [], lens: ['diameter', 'thickness', 'r1', 'r2'],
     prism: ['radius', 'height'], reflector: ['focal', 'aperture', 'wall', 'focusOffset'], 'aspheric lens': ['diameter', 'thickness', 'r1', 'r2'],
     revolution: [], 'fresnel lens': [], extrusion: ['height'], cone: ['base', 'top', 'height'], 'round cone': ['bottom', 'top', 'height'],
-    octahedron: ['size'], 'capped torus': ['ring', 'tube'], link: ['span', 'ring', 'tube'], tube: ['outer', 'inner', 'height'], dome: ['radius', 'height'] }
+    octahedron: ['size'], 'capped torus': ['ring', 'tube'], link: ['span', 'ring', 'tube'], tube: ['outer', 'inner', 'height'], dome: ['radius', 'height'],
+    pyramid: ['base', 'top', 'height'] }
 
 // ---- transforms: world = parent * local; R = Rz Ry Rx (columns: local axes in the parent)
 function rotationOf(t) {
@@ -316,6 +319,7 @@ function localRadius(key, P, unit) {
         case 'tube': return Math.hypot(Math.max(P.outer, P.inner, 0), 0.5 * Math.abs(P.height))
         // the sphere's centre at c = height - R (held to +-R): within |c| + R of the origin
         case 'dome': { const R = Math.max(P.radius, 1e-6); return Math.abs(Math.min(Math.max(P.height - R, -R), R)) + R }
+        case 'pyramid': { const h = 0.5 * Math.max(P.height, 1e-6); return Math.max(Math.hypot(0.5 * P.base.x, 0.5 * P.base.y, h), Math.hypot(0.5 * P.top.x, 0.5 * P.top.y, h)) }
     }
     return Infinity
 }
@@ -725,11 +729,33 @@ function cameraOf(doc, width, height, cannot) {
 // radius about a point).
 function environmentOf(E, doc, cannot) {
     cannot = cannot || (() => {})
-    const env = { type: 0, L: 0, col: [1, 1, 1], sunDir: [0, 0, 1], sunCos: 1, sunE: 0, sunRadius: 0, sunPoint: [0, 0, 0], top: 0 }
+    const env = { type: 0, L: 0, col: [1, 1, 1], sunDir: [0, 0, 1], sunCos: 1, sunE: 0, sunRadius: 0, sunPoint: [0, 0, 0], top: 0,
+        sunL: [0, 0, 0], sunSin2: 0, sunOmega: 0, skyAerosol: 0, skyMs: 1, skyUp: 0 }
+    const cone = d => { const c = sky.sunCone(d); env.sunCos = c.cos; env.sunSin2 = c.sin2; env.sunOmega = c.omega; return c }
     if (E.sky === undefined) {
         if (hasStatements(doc.sky)) cannot('environment', 'the scene\'s own sky code')
         const mode = E.envMode || 0
-        if (mode !== 1) { cannot('environment', mode === 2 ? 'the scene\'s own code (envMode 2)' : 'the sun & sky (envMode 0)'); return env }
+        if (mode === 2) { cannot('environment', 'the scene\'s own code (envMode 2)'); return env }
+        if (mode === 0) {
+            // the sun & sky (environment.js SUNSKY_FN; photometry.js derive:183 puts the sun's cone in use while the
+            // hint is on and its disc not wholly below the horizon)
+            const num = (v, d) => v !== undefined && v !== null && isFinite(+v) ? +v : d
+            const c = cone(num(E.sunDiameter, 0.53))
+            const el = num(E.uSunElevation, 45), turb = num(E.uSkyTurbidity, 0.7)
+            const s = sky.sunDirection(num(E.uSunAzimuth, 45), el)
+            env.type = 4
+            env.L = Math.max(num(E.sunOutside, 128000), 0)
+            env.sunDir = s
+            env.skyAerosol = sky.aerosolOf(turb)
+            env.skyMs = sky.airMass(Math.max(s[2], 0))
+            env.skyUp = Math.min(Math.max((s[2] + 0.05) / 0.05, 0), 1)
+            if (el * Math.PI / 180 + c.half > 0 && env.L > 0) {
+                env.sunL = sky.sunRadiance(s, turb, env.L, c.sin2)
+                env.sunE = (LUMA[0] * env.sunL[0] + LUMA[1] * env.sunL[1] + LUMA[2] * env.sunL[2]) * c.omega
+            }
+            if (env.L <= 0) env.type = 0
+            return env
+        }
         const top = Math.min(1, Math.max(0, E.studioTop !== undefined ? +E.studioTop : 0.5))
         const c = colourOf(E.studioColor, [1, 1, 1]).map(v => Math.max(v, 0))
         const y = LUMA[0] * c[0] + LUMA[1] * c[1] + LUMA[2] * c[2]
@@ -753,7 +779,7 @@ function environmentOf(E, doc, cannot) {
         const r = Math.PI / 180
         const az = (E.uSunAzimuth || 0) * r, el = (E.uSunElevation || 45) * r
         env.sunDir = [Math.cos(el) * Math.cos(az), Math.cos(el) * Math.sin(az), Math.sin(el)]
-        env.sunCos = Math.cos(0.5 * (E.sunDiameter || 0.53) * r)
+        cone(E.sunDiameter || 0.53)
         env.sunE = E.sunIlluminance
         const unit = UNIT_METRES[doc.unit || 'm']
         if (E.sunAperture) {

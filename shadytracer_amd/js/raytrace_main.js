@@ -26,7 +26,8 @@ function loadAddon() {
 // ---- the C structures' layouts (include/pt_types.h), every field 4 bytes: [name, kind, count]
 const CAMERA = [['pos', 'f', 3], ['fx', 'f', 1], ['right', 'f', 3], ['fy', 'f', 1], ['up', 'f', 3], ['type', 'u', 1], ['fwd', 'f', 3], ['pad', 'f', 1]]
 const ENV = [['type', 'u', 1], ['L', 'f', 1], ['col', 'f', 3], ['sunDir', 'f', 3], ['sunCos', 'f', 1], ['sunE', 'f', 1], ['sunRadius', 'f', 1],
-    ['sunPoint', 'f', 3], ['top', 'f', 1], ['pad', 'f', 1]]
+    ['sunPoint', 'f', 3], ['top', 'f', 1], ['pad', 'f', 1], ['sunL', 'f', 3], ['sunSin2', 'f', 1], ['sunOmega', 'f', 1], ['skyAerosol', 'f', 1],
+    ['skyMs', 'f', 1], ['skyUp', 'f', 1]]
 const HAZE = [['centre', 'f', 3], ['sigmaS', 'f', 1], ['half', 'f', 3], ['g', 'f', 1], ['albedo', 'f', 1], ['pad', 'f', 3]]
 const UNIFORMS = [['width', 'u', 1], ['height', 'u', 1], ['maxbounce', 'u', 1], ['maxiter', 'u', 1], ['stepscale', 'f', 1], ['eps', 'f', 1],
     ['epsAuto', 'f', 1], ['envRadius', 'f', 1], ['nee', 'u', 1], ['rr', 'u', 1], ['rrStart', 'u', 1], ['diffuseCap', 'u', 1],

@@ -38,7 +38,8 @@ class PtLightFace(ctypes.Structure):
 class PtEnv(ctypes.Structure):
     _fields_ = [('type', ctypes.c_uint), ('L', ctypes.c_float), ('col', F3), ('sunDir', F3), ('sunCos', ctypes.c_float),
                 ('sunE', ctypes.c_float), ('sunRadius', ctypes.c_float), ('sunPoint', F3), ('top', ctypes.c_float),
-                ('pad', ctypes.c_float)]
+                ('pad', ctypes.c_float), ('sunL', F3), ('sunSin2', ctypes.c_float), ('sunOmega', ctypes.c_float),
+                ('skyAerosol', ctypes.c_float), ('skyMs', ctypes.c_float), ('skyUp', ctypes.c_float)]
 
 
 class PtHaze(ctypes.Structure):
@@ -198,6 +199,12 @@ def make_env(e):
     _set3(v.col, e['col']); _set3(v.sunDir, e['sunDir']); _set3(v.sunPoint, e['sunPoint'])
     v.sunCos, v.sunE, v.sunRadius = e['sunCos'], e['sunE'], e['sunRadius']
     v.top = float(e.get('top', 0.0))
+    # the sun's cone in double: sin^2 of its radius and its solid angle, from the cosine when a caller gives only that
+    half = math.acos(min(max(float(e['sunCos']), -1.0), 1.0))
+    v.sunSin2 = float(e.get('sunSin2', math.sin(half) ** 2))
+    v.sunOmega = float(e.get('sunOmega', 4 * math.pi * math.sin(0.5 * half) ** 2))
+    _set3(v.sunL, e.get('sunL', [0.0, 0.0, 0.0]))
+    v.skyAerosol, v.skyMs, v.skyUp = float(e.get('skyAerosol', 0.0)), float(e.get('skyMs', 1.0)), float(e.get('skyUp', 0.0))
     return v
 
 

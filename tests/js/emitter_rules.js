@@ -52,5 +52,13 @@ out.approximations = approx.approximations
 // a light's flux: pi L A on its one face (1 m^2 here)
 const lit = H.generate(doc({ root: light({ shape: 'rectangle', colourBy: 'cct', cct: 3000, flux: 100 }) }))
 out.lightLe = lit.faces[0].le
+// the sun & sky at its defaults but for the sun's place, and at three elevations; the sky's band table and the
+// device's copy of it
+const S = require(path.join(js, 'sky'))
+out.sunsky = [45, 15, 5, -3.44].map(el => H.generate(doc({ environment: { envMode: 0, uSunAzimuth: 30, uSunElevation: el } })).env)
+out.skyBands = S.bandTable()
+out.skyHeader = S.headerText()
+out.skyJs = [[0, 0, 1], [0.6, 0, 0.8], [0, -0.96, 0.28], [0.3, 0.3, -0.905]].map(d => d.map(x => x / Math.hypot(...d)))
+    .map(d => S.skyRadiance(d, out.sunsky[0].sunDir, 0.7, 128000))
 out.studioEnv = H.generate(doc({ environment: { envMode: 1, studioIlluminance: 300, studioTop: 0.6, studioColor: { r: 1, g: 0.5, b: 0.25 } } })).env
 process.stdout.write(JSON.stringify(out))

@@ -197,16 +197,16 @@ def test_outlines_match_the_oracle(engine):
 
 
 def test_solid_builtins_match_the_oracle(engine):
-    # cone, round cone, octahedron, capped torus, link, tube and dome: primary decisions ray by ray, then
+    # cone, round cone, octahedron, capped torus, link, tube, dome and pyramid: primary decisions ray by ray, then
     # per-pixel means
     say('case: gallery 160x64 pick vs oracle; 80x32 GPU 256 spp vs oracle 64 spp')
     made, u = load(engine, 'gallery')
     gpos, gstate, _, gprim = engine.pick_rays()
     opos, ostate, _, oprim = oracle.pick(made['scene'], u)
     same = (gstate == ostate) & ((gstate != 0) | (gprim == oprim))
-    say('agreement', same.mean(), 'per shape', [int(((oprim == k) & (ostate == 0)).sum()) for k in range(1, 8)])
+    say('agreement', same.mean(), 'per shape', [int(((oprim == k) & (ostate == 0)).sum()) for k in range(1, 9)])
     assert same.mean() >= 0.9995
-    for k in range(1, 8):
+    for k in range(1, 9):
         assert ((gprim == k) & (gstate == 0)).sum() > 50, k
     hit = (gstate == 0) & same
     assert np.quantile(np.linalg.norm(gpos[hit, :3] - opos[hit, :3], axis=1), 0.999) < 1e-4
@@ -223,6 +223,49 @@ def test_solid_builtins_match_the_oracle(engine):
     assert np.mean(np.abs(z) < 3) > 0.97
     se = math.sqrt(np.nansum(s ** 2)) / s.size
     assert abs(gL.mean() - oL.mean()) < 4 * se + 1e-3 * oL.mean()
+
+
+def test_sunsky_floor_reads_rho_E_over_pi(engine):
+    # the builder's default environment, the sun & sky (envMode 0): a level Lambertian floor of albedo 0.6 reads
+    # 0.6 / pi (E_sun sin 40 + E_sky), the disc's light from the environment itself and the sky's by quadrature
+    from sky_quadrature import level_sky_lux, level_sun_lux
+    say('case: sunsky-floor 64x48 512 spp, floor = 0.6 (E_sun sin 40 + E_sky) / pi')
+    made, u = load(engine, 'sunsky-floor')
+    env = made['scene']['env']
+    want = 0.6 / math.pi * (level_sun_lux(env) + level_sky_lux(env))
+    engine.trace(512)
+    acc, mom = engine.read_accum()
+    L = pt.luminance(acc).ravel()
+    se = math.sqrt((pt.standard_error(mom).ravel() ** 2).sum()) / L.size
+    say('floor', L.mean(), 'want', want, 'se', se)
+    assert abs(L.mean() - want) < 4 * se + 1e-3 * want
+    assert engine.stats()['nonfinite'] == 0
+
+
+def test_sunlit_gallery_matches_the_oracle(engine):
+    # the gallery under the sun & sky: shadows from the disc, the sky's colour; primary decisions, then per-pixel means
+    say('case: sunlit 160x64 pick vs oracle; 80x32 GPU 256 spp vs oracle 64 spp')
+    made, u = load(engine, 'sunlit')
+    gpos, gstate, _, gprim = engine.pick_rays()
+    opos, ostate, _, oprim = oracle.pick(made['scene'], u)
+    same = (gstate == ostate) & ((gstate != 0) | (gprim == oprim))
+    assert same.mean() >= 0.9995
+    made, u = load(engine, 'sunlit', width=80, height=32)
+    engine.trace(256)
+    ga, gm = engine.read_accum()
+    oa, om, _ = oracle.render(made['scene'], u, 64, first=100000)
+    gL, oL = pt.luminance(ga), pt.luminance(oa)
+    s = np.sqrt(pt.standard_error(gm) ** 2 + pt.standard_error(om) ** 2)
+    ok = s > 1e-9
+    z = (gL - oL)[ok] / s[ok]
+    say('z std', z.std(), 'frac |z|<3', np.mean(np.abs(z) < 3), 'means', gL.mean(), oL.mean())
+    assert 0.75 < z.std() < 1.3
+    assert np.mean(np.abs(z) < 3) > 0.97
+    se = math.sqrt(np.nansum(s ** 2)) / s.size
+    assert abs(gL.mean() - oL.mean()) < 4 * se + 1e-3 * oL.mean()
+    # the colour: the sky's blue and the sun's warmth come through channel by channel
+    for c in range(3):
+        assert abs(ga[..., c].mean() / oa[..., c].mean() - 1) < 0.015, (c, ga[..., c].mean(), oa[..., c].mean())
 
 
 # the Lambertian's 1e-3: the march's 400-step cap (the reference's maxiter) drops a path that leaves the plane within

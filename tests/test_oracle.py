@@ -621,6 +621,33 @@ def test_round_cone_octahedron_and_dome_against_brute_force(tmp_path):
     assert ((d < 0) == inside).all()
 
 
+@pytest.mark.parametrize('base,top,height', [((1.0, 1.0), (0.0, 0.0), 1.0), ((1.0, 0.6), (0.4, 0.2), 0.8),
+                                             ((0.3, 0.5), (0.9, 0.7), 0.6), ((0.8, 0.8), (0.0, 0.5), 0.7)])
+def test_pyramid_is_the_convex_hull_of_its_corners(tmp_path, base, top, height):
+    # shapes.js Nfrustum: outside, the nearest of its twelve triangles' nearest points (Ericson); inside, the nearest
+    # face plane, each plane from a triangle's own corners (an apex or a ridge top leaves some triangles without area)
+    f = lambda x: float(np.float32(x))   # noqa: E731
+    bx, by, tx, ty, hh = f(base[0]) / 2, f(base[1]) / 2, f(top[0]) / 2, f(top[1]) / 2, f(height) / 2
+    B = [np.array([sx * bx, sy * by, -hh]) for sx, sy in ((1, -1), (1, 1), (-1, 1), (-1, -1))]
+    T = [np.array([sx * tx, sy * ty, hh]) for sx, sy in ((1, -1), (1, 1), (-1, 1), (-1, -1))]
+    quads = [(B[i], B[(i + 1) % 4], T[(i + 1) % 4], T[i]) for i in range(4)] + [tuple(T), tuple(B[::-1])]
+    tris = [t for a, b, c, d in quads for t in ((a, b, c), (a, c, d))]
+    planes = []
+    for a, b, c in tris:
+        n = np.cross(b - a, c - a)
+        if np.linalg.norm(n) > 1e-9:
+            n /= np.linalg.norm(n)
+            planes.append((n if n @ (a - [0, 0, 0]) >= -1e-12 else -n, a))
+    sc = _emit_doc(_single({'base': {'x': base[0], 'y': base[1]}, 'top': {'x': top[0], 'y': top[1]}, 'height': height}, 'pyramid'), tmp_path)
+    pts = RNG.uniform(-0.8, 0.8, (500, 3))
+    d, nrm, _ = oracle.scene_at(sc, pts)
+    for x, dx in zip(pts, d):
+        s = max(n @ (x - a) for n, a in planes)
+        want = s if s <= 0 else min(np.linalg.norm(x - _closest_on_triangle(x, *t3)) for t3 in tris)
+        assert abs(dx - want) < 2e-7, (x, dx, want)
+    assert _normals_are_gradients(sc, pts) > 0.97
+
+
 def test_capped_torus_and_link_against_brute_force(tmp_path):
     pts = RNG.uniform(-0.8, 0.8, (400, 3))
     f32 = lambda x: float(np.float32(x))   # noqa: E731
@@ -658,6 +685,22 @@ def test_studio_floor_reads_rho_E_over_pi():
     L0 = 500 / math.pi / (1 + 1 / 3)
     sky = L.reshape(u.height, u.width)[-3:] / L0
     assert (sky > 1 + 0.5 * math.sin(math.radians(0.9))).all() and (sky < 1 + 0.5 * math.sin(math.radians(5.1))).all()
+
+
+def test_sunsky_floor_reads_rho_E_over_pi():
+    # under the sun & sky a level Lambertian floor of albedo 0.6 receives the sun's light head-on times the sine of its
+    # elevation plus the sky's on the level (both from the environment itself: its disc and a quadrature of its sky),
+    # and reads 0.6 / pi of their sum -- light sampling's sun cone and cosine draws, MIS against the BSDF's
+    from sky_quadrature import level_sky_lux, level_sun_lux
+    made = scenes.emit('sunsky-floor', width=32, height=24)
+    u = pt.uniforms_for(made)
+    env = made['scene']['env']
+    want = 0.6 / math.pi * (level_sun_lux(env) + level_sky_lux(env))
+    acc, mom, _ = oracle.render(made['scene'], u, 128)
+    L = pt.luminance(acc).ravel()
+    se = math.sqrt((pt.standard_error(mom).ravel() ** 2).sum()) / L.size
+    assert abs(L.mean() - want) < 4 * se, (L.mean(), want, se)
+    assert se < 0.005 * want
 
 
 def test_smooth_union_bounds(tmp_path):
