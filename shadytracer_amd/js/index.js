@@ -57,6 +57,8 @@ function createPathTracer(opts = {}) {
     }
     RT.imageStats = () => R.imageStats()
     RT.readRegion = (...a) => R.readRegion(...a)
+    // the render's light as a file, 'pfm' | 'hdr' | 'exr' (darkroom/export/index.js: the light formats): a Buffer
+    RT.saveLight = (format, opts) => R.saveLight(format, opts)
     RT.readPixel = (...a) => R.readPixel(...a)
     RT.measure = (x0, y0, x1, y1) => R.readRegion(x0, y0, x1, y1)
     RT.wavefrontLog = () => R.wavefrontLog()

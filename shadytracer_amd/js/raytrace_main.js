@@ -147,6 +147,12 @@ function RAYTRACER(WG, _U, Q, scene) {
     const stdErr = (V, S) => (S < 1 && S > 0 && V >= 0) ? Math.sqrt(S * V / (1 - S)) : null
 
     // Luminance statistics of the accumulated picture (raytrace_main.js:876-956)
+    // The render as a light file (the darkroom's light formats, darkroom/export/index.js): 'pfm' | 'hdr' | 'exr',
+    // absolute cd/m^2 (HDR in Radiance's units); a Buffer
+    R.saveLight = (format, opts = {}) => {
+        const { acc } = readback()
+        return require('./export').writeLight(format, acc, U.width, U.height, opts)
+    }
     R.imageStats = () => {
         const { acc, mom, n } = readback()
         const L = []

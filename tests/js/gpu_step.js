@@ -16,6 +16,10 @@ async function main() {
     const st = RT.imageStats()
     const floor = RT.readRegion(0, 0, 63, 15)
     const log = RT.wavefrontLog()
+    if (process.argv[2]) {
+        require('fs').writeFileSync(path.join(process.argv[2], 'c1.exr'), RT.saveLight('exr'))
+        require('fs').writeFileSync(path.join(process.argv[2], 'c1.pfm'), RT.saveLight('pfm'))
+    }
     RT.dispose()
     process.stdout.write(JSON.stringify({ ms, stats: st, floor, log }))
 }

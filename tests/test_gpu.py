@@ -381,14 +381,22 @@ def test_folded_params_render_the_same_scene(engine):
     assert np.allclose(p1[hit, 2], 0.25, atol=2e-3) and np.allclose(p0[hit, 2], 0.0, atol=2e-3)
 
 
-def test_node_addon_drives_the_engine():
-    say('case: node tests/js/gpu_step.js (createPathTracer through the N-API addon)')
-    r = subprocess.run(['node', os.path.join(ROOT, 'tests', 'js', 'gpu_step.js')], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
-                       text=True, timeout=300)
+def test_node_addon_drives_the_engine(tmp_path):
+    say('case: node tests/js/gpu_step.js (createPathTracer through the N-API addon), then its light files')
+    r = subprocess.run(['node', os.path.join(ROOT, 'tests', 'js', 'gpu_step.js'), str(tmp_path)], stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     out = json.loads(r.stdout)
     assert abs(out['floor']['mean'] / 800 - 1) < 2e-3, out['floor']
     assert out['stats']['samples'] == 16 and out['log']['segments'] > 0
+    # the render saved as light: the EXR's float32 and the PFM's are the same numbers, the floor rows 800 cd/m^2
+    from test_export import _read_exr
+    exr = _read_exr((tmp_path / 'c1.exr').read_bytes())['img']
+    head = (tmp_path / 'c1.pfm').read_bytes().split(b'\n', 3)
+    pfm = np.frombuffer(head[3], '<f4').reshape(64, 64, 3)[::-1]
+    assert np.array_equal(exr.astype(np.float32), pfm)
+    Y = pfm.astype(np.float64) @ np.array([0.2126, 0.7152, 0.0722])
+    assert abs(Y[-16:].mean() / 800 - 1) < 2e-3
 
 
 # ---------------------------------------------------------------------------------------------------- SPPM
