@@ -75,7 +75,9 @@ typedef struct PtCamera {
     float up[3];
     unsigned int type;   // 0 perspective, 1 orthographic
     float fwd[3];        // from the target towards the camera; the view direction is -fwd
-    float pad;
+    float lensR;         // the thin lens's radius, m (0: a pinhole; perspective only)
+    float focus;         // the distance in focus along the view direction, m
+    float pad[3];
 } PtCamera;
 
 // A material as the shade kernel reads it (the reference's Material2 subset, core/materials.js:12-110).

@@ -956,7 +956,16 @@ static void cameraRay(const OrScene* S, const OrTiles* T, const uint32_t* seed, 
     double vv = clampd((ty + 0.5) * T->invTY + (u4[1] - 0.5) * T->invTY, 0, 1);
     double x = 2 * uu - 1, y = 2 * vv - 1;
     d3 right = f3(c->right), up = f3(c->up), fwd = f3(c->fwd);
-    if (c->type == 0) { *o = f3(c->pos); *d = unit(sub(add(scl(right, x * c->fx), scl(up, y * c->fy)), fwd)); }
+    if (c->type == 0) {
+        d3 dc = sub(add(scl(right, x * c->fx), scl(up, y * c->fy)), fwd);
+        *o = f3(c->pos); *d = unit(dc);
+        if (c->lensR > 0) {          /* the thin lens: a point of its disc (lens.js lensDisc:111), aimed at the focus plane */
+            double l2[4]; rng4(&r, l2);
+            double rr = c->lensR * sqrt(u4[3]), ph = 2 * PI_D * l2[0];
+            d3 off = add(scl(right, rr * cos(ph)), scl(up, rr * sin(ph)));
+            *o = add(*o, off); *d = unit(sub(scl(dc, c->focus), off));
+        }
+    }
     else { *o = add(f3(c->pos), add(scl(right, x * c->fx), scl(up, y * c->fy))); *d = neg(fwd); }
     double fw = S->u->width, fh = S->u->height;
     uint32_t px = (uint32_t)clampd(uu * fw, 0, fw - 1), py = (uint32_t)clampd(vv * fh, 0, fh - 1);

@@ -36,7 +36,18 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_genrays(const PtDev* _
     V3 o, d;
     if (cam.type == 0u) {
         o = ld3(cam.pos);
-        d = normalize(right * (x * cam.fx) + up * (y * cam.fy) - fwd);
+        const V3 dc = right * (x * cam.fx) + up * (y * cam.fy) - fwd;
+        d = normalize(dc);
+        if (cam.lensR > 0.0f) {
+            // through the thin lens (genrays.js:78-86, lens.js lensDisc:111): from a point of its disc, uniform in
+            // area, towards the point where the pinhole's ray meets the plane in focus
+            const V4 l2 = rngNext(r);
+            const float rr = cam.lensR * sqrtf(rnd.w), phi = 2.0f * PT_PI * l2.x;
+            float sp, cp; __sincosf(phi, &sp, &cp);
+            const V3 off = right * (rr * cp) + up * (rr * sp);
+            o = o + off;
+            d = normalize(dc * cam.focus - off);
+        }
     } else {
         o = ld3(cam.pos) + right * (x * cam.fx) + up * (y * cam.fy);
         d = -fwd;

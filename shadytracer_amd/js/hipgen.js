@@ -704,7 +704,6 @@ function cameraOf(doc, width, height, cannot) {
     const r = Math.PI / 180
     if (C.mode && C.mode !== 'orbit') cannot('camera', `a ${C.mode} camera`)
     if (C.camtype) cannot('camera', 'an orthographic camera')
-    if (C.aperture > 0) cannot('camera', 'depth of field (aperture)')
     if (C.roll) cannot('camera', 'roll')
     const t = (C.target || [0, 0, 0]).map(v => v * unit)
     const az = (C.azimuth !== undefined ? C.azimuth : 40) * r, el = (90 - (C.elevation !== undefined ? C.elevation : 70)) * r
@@ -717,13 +716,17 @@ function cameraOf(doc, width, height, cannot) {
     if (Math.hypot(x[0], x[1], x[2]) < 1e-6) x = cross(Math.abs(z[2]) < 0.9 ? [0, 0, 1] : [1, 0, 0], z)
     x = norm(x)
     const fy = Math.tan(0.5 * (C.fov || 60) * r)
-    return { pos, right: x, up: cross(z, x), fwd: z, fx: fy * width / height, fy, type: 0 }
+    // the thin lens (lib/camera.js:336-338): its radius, half the aperture, and the distance in focus, the target's
+    // for a focus of 0; both in the document's units
+    const lensR = Math.max(+C.aperture || 0, 0) / 2 * unit
+    const focus = (+C.focus > 0 ? +C.focus * unit : dist) || 1
+    return { pos, right: x, up: cross(z, x), fwd: z, fx: fy * width / height, fy, type: 0, lensR, focus }
 }
 
 // The environment. A builder document's (uniforms.schema.js envMode:429): 1 the studio, L(d) = L0 (1 + top d.z) with
 // L0 = E / (pi (1 + 2 top / 3)) so that a level surface facing up receives studioIlluminance lux (photometry.js
 // studioLuminance:154, environment.js studioLight:165), its colour a tint of luminance 1, and no sun; 0 the sun & sky
-// and 2 the scene's own code, neither of which the engine draws. The configurations here name theirs directly:
+// (js/sky.js); 2 the scene's own code, which the engine does not draw. The configurations here name theirs directly:
 // `sky: 'sphere' | 'hemisphere' | 'none'` of `uniformSky` cd/m^2, and an optional sun -- direction from
 // azimuth/elevation (sdf.js:72-76, Z-up), illuminance in lux, angular diameter in degrees, and an aperture (a beam
 // radius about a point).

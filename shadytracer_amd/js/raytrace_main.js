@@ -24,7 +24,8 @@ function loadAddon() {
 }
 
 // ---- the C structures' layouts (include/pt_types.h), every field 4 bytes: [name, kind, count]
-const CAMERA = [['pos', 'f', 3], ['fx', 'f', 1], ['right', 'f', 3], ['fy', 'f', 1], ['up', 'f', 3], ['type', 'u', 1], ['fwd', 'f', 3], ['pad', 'f', 1]]
+const CAMERA = [['pos', 'f', 3], ['fx', 'f', 1], ['right', 'f', 3], ['fy', 'f', 1], ['up', 'f', 3], ['type', 'u', 1], ['fwd', 'f', 3], ['lensR', 'f', 1],
+    ['focus', 'f', 1], ['pad', 'f', 3]]
 const ENV = [['type', 'u', 1], ['L', 'f', 1], ['col', 'f', 3], ['sunDir', 'f', 3], ['sunCos', 'f', 1], ['sunE', 'f', 1], ['sunRadius', 'f', 1],
     ['sunPoint', 'f', 3], ['top', 'f', 1], ['pad', 'f', 1], ['sunL', 'f', 3], ['sunSin2', 'f', 1], ['sunOmega', 'f', 1], ['skyAerosol', 'f', 1],
     ['skyMs', 'f', 1], ['skyUp', 'f', 1]]

@@ -21,7 +21,7 @@ F2 = ctypes.c_float * 2
 
 class PtCamera(ctypes.Structure):
     _fields_ = [('pos', F3), ('fx', ctypes.c_float), ('right', F3), ('fy', ctypes.c_float), ('up', F3),
-                ('type', ctypes.c_uint), ('fwd', F3), ('pad', ctypes.c_float)]
+                ('type', ctypes.c_uint), ('fwd', F3), ('lensR', ctypes.c_float), ('focus', ctypes.c_float), ('pad', F3)]
 
 
 class PtMaterial(ctypes.Structure):
@@ -190,6 +190,7 @@ def make_camera(cam):
     c = PtCamera()
     _set3(c.pos, cam['pos']); _set3(c.right, cam['right']); _set3(c.up, cam['up']); _set3(c.fwd, cam['fwd'])
     c.fx, c.fy, c.type = cam['fx'], cam['fy'], int(cam.get('type', 0))
+    c.lensR, c.focus = float(cam.get('lensR', 0.0)), float(cam.get('focus', 1.0))
     return c
 
 

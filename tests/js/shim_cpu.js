@@ -20,7 +20,7 @@ const buf = new ArrayBuffer(m.sizeOf(m.UNIFORMS))
 m.pack(m.UNIFORMS, { width: 640, height: 480, seed: 77, cam: { pos: [1, 2, 3] }, env: { type: 2, L: 1000 } }, new DataView(buf), 0)
 const dv = new DataView(buf)
 out.packed = [dv.getUint32(0, true), dv.getUint32(4, true), dv.getUint32(60, true), dv.getFloat32(64, true), dv.getFloat32(72, true),
-    dv.getUint32(128, true), dv.getFloat32(132, true)]
+    dv.getUint32(144, true), dv.getFloat32(148, true)]
 const RT = createPathTracer({ width: 8, height: 8 })
 out.unknown = RT.set({ maxbounce: 3, notAUniform: 1, stepscale: 0.8 })
 out.maxbounce = RT.get('maxbounce')

@@ -54,7 +54,8 @@ def test_addon_loads_and_packs_the_same_structures():
     assert o[0] == o[1] == ctypes.sizeof(pt.PtOpticsScene) and o[2] == o[3] == ctypes.sizeof(pt.PtOpticsResult)
     # fields land at their C offsets
     u = pt.PtUniforms
-    assert [u.width.offset, u.height.offset, u.seed.offset, u.cam.offset, u.env.offset] == [0, 4, 60, 64, 128]
+    assert [u.width.offset, u.height.offset, u.seed.offset, u.cam.offset, u.env.offset] == [0, 4, 60, 64, 144]
+    assert [pt.PtCamera.lensR.offset, pt.PtCamera.focus.offset, ctypes.sizeof(pt.PtCamera), ctypes.sizeof(pt.PtEnv)] == [60, 64, 80, 96]
     assert out['packed'] == [640, 480, 77, 1.0, 3.0, 2, 1000.0]
     # the facade's set returns unknown names (lib/pathtracer/index.js set)
     assert out['unknown'] == ['notAUniform'] and out['maxbounce'] == 3

@@ -225,6 +225,22 @@ def test_solid_builtins_match_the_oracle(engine):
     assert abs(gL.mean() - oL.mean()) < 4 * se + 1e-3 * oL.mean()
 
 
+def test_thin_lens_rays_match_the_oracle(engine):
+    # the thin lens: the same lens points and focus-plane aims as the oracle's, ray by ray, out of focus on the floor
+    from test_oracle import _looking_down
+    say('case: plane-c1 64x64 looking down, lens radius 0.05 m focused at 1 m, pick vs oracle')
+    made, _ = load(engine, 'plane-c1', width=64, height=64)
+    u = _looking_down(made, 0.05, 1.0)
+    engine.set_uniforms(u)
+    gpos, gstate, _, _ = engine.pick_rays()
+    opos, ostate, _, _ = oracle.pick(made['scene'], u)
+    assert (gstate == ostate).all() and (gstate == 0).all()
+    assert np.abs(gpos[:, :3] - opos[:, :3]).max() < 1e-4
+    u0 = _looking_down(made)
+    p0, _, _, _ = oracle.pick(made['scene'], u0)
+    assert np.linalg.norm(gpos[:, :3] - p0[:, :3], axis=1).max() > 0.04     # the lens moved them
+
+
 def test_sunsky_floor_reads_rho_E_over_pi(engine):
     # the builder's default environment, the sun & sky (envMode 0): a level Lambertian floor of albedo 0.6 reads
     # 0.6 / pi (E_sun sin 40 + E_sky), the disc's light from the environment itself and the sky's by quadrature

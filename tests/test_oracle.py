@@ -687,6 +687,31 @@ def test_studio_floor_reads_rho_E_over_pi():
     assert (sky > 1 + 0.5 * math.sin(math.radians(0.9))).all() and (sky < 1 + 0.5 * math.sin(math.radians(5.1))).all()
 
 
+def _looking_down(made, lensR=0.0, focus=1.0):
+    # a camera 2 m above the C1 floor looking straight down: the floor is the plane 2 m along the view axis
+    t = math.tan(math.radians(30))
+    cam = dict(pos=[0.0, 0.0, 2.0], right=[1.0, 0.0, 0.0], up=[0.0, 1.0, 0.0], fwd=[0.0, 0.0, 1.0], fx=t, fy=t, type=0,
+               lensR=lensR, focus=focus)
+    return pt.uniforms_for(dict(made, camera=cam))
+
+
+def test_thin_lens_focus_and_circle_of_confusion():
+    # genrays.js:78-86: a ray leaves a point of the lens disc (radius R, uniform in area) towards the point where the
+    # pinhole's ray meets the plane in focus. In focus at the floor every hit is the pinhole's; focused at half the
+    # distance, each hit is off the pinhole's by the lens point itself (|off| (1 - 2 / F) = -off): within R, on
+    # average 2R/3 (the mean radius of a uniform disc)
+    made = scenes.emit('plane-c1', width=64, height=64)
+    R = 0.05
+    p0, s0, _, _ = oracle.pick(made['scene'], _looking_down(made))
+    p1, s1, _, _ = oracle.pick(made['scene'], _looking_down(made, R, 2.0))
+    p2, s2, _, _ = oracle.pick(made['scene'], _looking_down(made, R, 1.0))
+    assert (s0 == 0).all() and (s1 == 0).all() and (s2 == 0).all()
+    # the march stops within its tolerance (here up to 4e-5 m) of the floor, along each ray's own direction
+    assert np.abs(p1[:, :3] - p0[:, :3]).max() < 2e-4
+    dev = np.linalg.norm(p2[:, :3] - p0[:, :3], axis=1)
+    assert dev.max() <= R + 2e-4 and abs(dev.mean() / (2 * R / 3) - 1) < 0.03
+
+
 def test_sunsky_floor_reads_rho_E_over_pi():
     # under the sun & sky a level Lambertian floor of albedo 0.6 receives the sun's light head-on times the sine of its
     # elevation plus the sky's on the level (both from the environment itself: its disc and a quadrature of its sky),
