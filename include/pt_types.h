@@ -240,6 +240,9 @@ typedef struct PtUniforms {
     float sppmAlpha;            // the radius schedule's alpha: r_k^2 = r_1^2 prod_{i<k} (i + alpha) / (i + 1)
     float relax;                // over-relaxed sphere tracing: steps of relax * stepscale * |d| while the free balls
                                 // overlap (Keinert et al. 2014); 0 or 1 the plain march, at most 1.95
+    unsigned int measureMode;   // the reference's measure views (uniforms.schema.js measureMode): 0 or 2 the light as it
+                                // is; 1 the illuminance view -- the camera's first hit a white Lambertian for that
+                                // bounce, so a pixel reads E / pi there, and a camera ray that meets nothing 0
 } PtUniforms;
 
 

@@ -13,8 +13,9 @@
  * 1; Fresnel at normal incidence and total internal reflection; Cauchy through n_d and V_d), brute-force
  * ray-primitive intersection on small inputs, quadrature invariants of the BSDF (pdf integrates to its
  * non-delta share; sampling weights agree with eval/pdf), the reference's own measured vectors
- * (lib/pathtracer/test/fixtures/run-0.77.0-pascal.json via tests/golden), and the error-bar identity of
- * core/measure.js (tests/golden/measure_vectors.json).
+ * (lib/pathtracer/test/fixtures/run-0.77.0-pascal.json via tests/golden/reference_rows.json: the furnace rows, the haze
+ * rows, the slab's transmissions), known answers of PCG4D from the JCGT 2020 definition, and the error-bar identity of
+ * core/measure.js (two renders from disjoint samples differ per pixel by their combined error bars).
  */
 #include <math.h>
 #include <stdint.h>
@@ -82,7 +83,7 @@ enum { OP_PRIM = 0, OP_JOIN = 1, OP_FOLD = 2, OP_NODE = 3 };
 enum { SH_PLANE = 0, SH_SPHERE, SH_BOX, SH_CYLINDER, SH_CAPSULE, SH_TORUS, SH_RECT, SH_DISC, SH_LENS, SH_PRISM, SH_REFLECTOR, SH_ASPHERE,
        SH_REVOLUTION, SH_FRESNEL, SH_EXTRUSION, SH_CONE, SH_ROUNDCONE, SH_OCTA, SH_CTORUS, SH_LINK, SH_TUBE, SH_DOME, SH_FRUSTUM };
 enum { J_UNION = 0, J_SUBTRACT, J_INTERSECT };
-/* prim: shape, xf, p, mat, prim; join: join, k; node (a top-level node's start): xf = its bounding sphere's offset in
+/* prim: shape, xf, p, mat, prim; join: join, k, and its shape (0 smooth, 1 round, 2 chamfer) in shape; node (a top-level node's start): xf = its bounding sphere's offset in
  * U (-1: unbounded), join = its mask bit (-1: none), k = the index of its fold */
 typedef struct { int op, shape, xf, p, mat, prim, join, k; } OrOp;
 
@@ -460,6 +461,44 @@ static d4 joinI(d4 a, d4 b, double k) {
     return d4of(csgN(v(a.x, a.y, a.z), v(b.x, b.y, b.z), t), maxd(a.w, b.w) + h * h * 0.25 / k);
 }
 
+/* Shaped joins (core/scene_api.js:393-406, model version 28 joinShape): hg_sdf's round and chamfer intersections of
+ * radius r, their value held at most the hard join's h plus csgShapedCap -- the most the form adds where h <= 0
+ * ((sqrt 2 - 1) r round, r / sqrt 2 chamfer) faded linearly to 0 at h = 2r. Unions and cuts through complements. */
+static double shapedCap(double h, double r, double most) {
+    double f = 1 - 0.5 * h / maxd(r, 1e-30);
+    return most * r * (f < 0 ? 0 : f > 1 ? 1 : f);
+}
+static d4 shapedOut(d4 a, d4 b, d4 hard, double d, double cap, d3 g) {
+    int both = dotp(v(a.x, a.y, a.z), v(a.x, a.y, a.z)) >= 1e-12 && dotp(v(b.x, b.y, b.z), v(b.x, b.y, b.z)) >= 1e-12;
+    if (d - hard.w >= cap) return both ? d4of(v(hard.x, hard.y, hard.z), hard.w + cap) : d4of(v(0, 0, 0), hard.w + cap);
+    double l = len(g);
+    return d4of(both && l > 1e-12 ? scl(g, 1 / l) : v(0, 0, 0), d);
+}
+static d4 roundI(d4 a, d4 b, double r) {
+    d4 hard = a.w > b.w ? a : b;
+    double A = a.w + r, B = b.w + r;
+    if (!(A > 0 && B > 0)) return hard;
+    return shapedOut(a, b, hard, sqrt(A * A + B * B) - r, shapedCap(hard.w, r, 0.41421356237309515),
+                     add(scl(v(a.x, a.y, a.z), A), scl(v(b.x, b.y, b.z), B)));
+}
+static d4 chamferI(d4 a, d4 b, double r) {
+    d4 hard = a.w > b.w ? a : b;
+    double d = (a.w + b.w + r) * 0.7071067811865476;
+    if (d <= hard.w) return hard;
+    return shapedOut(a, b, hard, d, shapedCap(hard.w, r, 0.7071067811865476), add(v(a.x, a.y, a.z), v(b.x, b.y, b.z)));
+}
+static d4 neg4(d4 a) { d4 r = {-a.x, -a.y, -a.z, -a.w}; return r; }
+/* one join: op J_*, shape 0 smooth / 1 round / 2 chamfer, k its blend or radius */
+static d4 joinAny(d4 a, d4 b, int op, int shape, double k) {
+    if (shape == 0) {
+        if (op == J_UNION) return joinU(a, b, k);
+        return joinI(a, op == J_SUBTRACT ? neg4(b) : b, k);
+    }
+    d4 (*I)(d4, d4, double) = shape == 1 ? roundI : chamferI;
+    if (op == J_UNION) return neg4(I(neg4(a), neg4(b), k));
+    return I(a, op == J_SUBTRACT ? neg4(b) : b, k);
+}
+
 typedef struct { d3 n; double d; int m, a, prim; double insideW; } OrHit;
 
 /* The closest-surface fold over the program (scene_api.js:39-63 fold; codegen.js:1821-1828 joins). A top-level node
@@ -483,9 +522,7 @@ static OrHit sceneAtM(const OrScene* S, d3 p, uint64_t mask) {
             double k = S->U[o->k];
             int wins = o->join == J_UNION ? b.w < a.w : o->join == J_SUBTRACT ? -b.w > a.w : b.w > a.w;
             if (wins) { sm[top - 1] = sm[top]; sk[top - 1] = sk[top]; }
-            if (o->join == J_UNION) st[top - 1] = joinU(a, b, k);
-            else if (o->join == J_SUBTRACT) { d4 nb = {-b.x, -b.y, -b.z, -b.w}; st[top - 1] = joinI(a, nb, k); }
-            else st[top - 1] = joinI(a, b, k);
+            st[top - 1] = joinAny(a, b, o->join, o->shape, k);
         } else {
             d4 s = st[--top];
             if (s.w < 0 && s.w > h.insideW) { h.insideW = s.w; h.a = sm[top]; }
@@ -622,7 +659,7 @@ static double lutE1(double mu, double r) {
 typedef struct {
     d3 tX, tY, nZ, o, base, tint, cm;
     double a, r, metallic, trans, coatK, eta, nrel, ps, ecO, ecAvg;
-    int isDelta;
+    int isDelta, through;
 } OrLobes;
 static void basis(d3 n, d3* t, d3* b) {
     /* any orthonormal frame will do for isotropic lobes; Frisvad's, restated */
@@ -651,12 +688,43 @@ static OrLobes makeLobes(d3 n, d3 view, const PtMaterial* m, double eta) {
     L.ecAvg = mind(L.coatK * lutEcAvg(L.r, L.nrel), 1);
     double e1 = L.metallic > 0 ? lutE1(L.o.z, L.r) : 1;
     L.cm = add(v(1, 1, 1), scl(L.base, 1 / maxd(e1, 1e-3) - 1));
+    L.through = 0;
     return L;
 }
 static d3 toLoc(const OrLobes* L, d3 w) { return v(dotp(w, L->tX), dotp(w, L->tY), dotp(w, L->nZ)); }
 static d3 toWld(const OrLobes* L, d3 w) { return unit(add(add(scl(L->tX, w.x), scl(L->tY, w.y)), scl(L->nZ, w.z))); }
+static double reflP(const OrLobes* L, double oh);
+/* The rough interface's transmission below the surface (materials_pbr.js:296-357 lobesThroughEval/Pdf; Walter et al.
+ * 2007), read only where the caller sets `through`: the generalised half vector h = -(eta o + i), turned to the view's
+ * side, refracting only with o.h > 0 > i.h; f = (1-m) T (1-F(o.h)) tint D G2 (o.h)|i.h| / (o.z |i.z| (eta o.h + i.h)^2),
+ * in basic radiance; its density the sampler's picks times the visible-normal density times |i.h| / (eta o.h + i.h)^2. */
+static int throughHalf(const OrLobes* L, d3 i, d3* h, double* oh, double* ih, double* jac) {
+    d3 g = neg(add(scl(L->o, L->eta), i));
+    double gl = len(g);
+    if (!(gl > 1e-12)) return 0;
+    g = scl(g, 1 / gl);
+    if (g.z < 0) g = neg(g);
+    *h = g; *oh = dotp(L->o, g); *ih = dotp(i, g);
+    if (*oh <= 0 || *ih >= 0) return 0;
+    double den = L->eta * *oh + *ih;
+    *jac = -*ih / maxd(den * den, 1e-300);
+    return 1;
+}
+static d3 lobesEvalThrough(const OrLobes* L, d3 i) {
+    double wd = 1 - L->metallic, oh, ih, jac;
+    d3 h;
+    if (wd <= 0 || L->trans <= 0 || L->isDelta || !throughHalf(L, i, &h, &oh, &ih, &jac)) return v(0, 0, 0);
+    return scl(L->tint, wd * L->trans * (1 - coatF(L, oh)) * mfD(h, L->a) * mfG2(L->o, i, L->a) * oh * jac / (L->o.z * -i.z));
+}
+static double lobesPdfThrough(const OrLobes* L, d3 i) {
+    double wd = 1 - L->metallic, oh, ih, jac, pd = (1 - L->ps) * (1 - L->trans);
+    d3 h;
+    if (wd <= 0 || L->trans <= 0 || L->isDelta || !throughHalf(L, i, &h, &oh, &ih, &jac)) return 0;
+    return wd * (1 - pd) * (1 - reflP(L, oh)) * mfG1(L->o, L->a) * mfD(h, L->a) * oh / L->o.z * jac;
+}
 static d3 lobesEval(const OrLobes* L, d3 l) {
     d3 i = toLoc(L, l);
+    if (i.z < 0 && L->through) return lobesEvalThrough(L, i);
     if (i.z <= 0) return v(0, 0, 0);
     d3 f = v(0, 0, 0);
     if (!L->isDelta) {
@@ -673,6 +741,7 @@ static d3 lobesEval(const OrLobes* L, d3 l) {
 static double reflP(const OrLobes* L, double oh) { double fh = coatF(L, oh), den = fh + (1 - fh) * L->trans; return den > 1e-6 ? fh / den : 1; }
 static double lobesPdf(const OrLobes* L, d3 l) {
     d3 i = toLoc(L, l);
+    if (i.z < 0 && L->through) return lobesPdfThrough(L, i);
     if (i.z <= 0) return 0;
     double wd = 1 - L->metallic, pd = (1 - L->ps) * (1 - L->trans), p = wd * pd * i.z / PI_D;
     if (!L->isDelta) p += (L->metallic + wd * (1 - pd) * reflP(L, dotp(L->o, unit(add(L->o, i))))) * mfPdfR(L->o, i, L->a);
@@ -808,7 +877,8 @@ static d3 sunSkyD(const PtEnv* e, d3 d) {
         double w = (skyTauR[k] * pR + 0.9 * ta * pM) / tau * k1 * up;
         for (int i = 0; i < 3; ++i) L[i] += skyRgb[k][i] * w;
     }
-    return v(maxd(e->L * L[0], 0), maxd(e->L * L[1], 0), maxd(e->L * L[2], 0));
+    /* col: the sky's gain per channel (the reference's envSky) */
+    return v(maxd(e->L * L[0] * e->col[0], 0), maxd(e->L * L[1] * e->col[1], 0), maxd(e->L * L[2] * e->col[2], 0));
 }
 static d3 envBg(const PtEnv* e, d3 d) {
     if (e->type == 1 || (e->type == 2 && d.z > 0)) return scl(f3(e->col), e->L);
@@ -1138,7 +1208,9 @@ static void tracePath(const OrScene* S, const OrTiles* T, const uint32_t* seed, 
         st->segments++;
         if (state == ST_ESCAPED) {
             int mapped = caustic && sdelta && sunFlux(env) > 0;
-            rad = add(rad, scl(mul(beta, add(envBg(env, dir), mapped ? v(0, 0, 0) : envSun(env, p, dir))), sunmis));
+            /* the illuminance view (uniforms.schema.js measureMode 1): a camera ray that meets nothing shows 0 */
+            if (!(u->measureMode == 1 && bounces == 0))
+                rad = add(rad, scl(mul(beta, add(envBg(env, dir), mapped ? v(0, 0, 0) : envSun(env, p, dir))), sunmis));
             direct = bounces == 0;
             break;
         }
@@ -1203,6 +1275,13 @@ static void tracePath(const OrScene* S, const OrTiles* T, const uint32_t* seed, 
         if (dotp(dir, nrm) > 0) { double off = offsetTol(S, p); p = add(p, scl(dir, off)); ltot += off * n; continue; }
         bounces++;
         const PtMaterial* m = &S->mats[h.m < S->nmats ? h.m : 0];
+        /* ...and the camera's first hit is a white Lambertian for that bounce: the pixel reads E / pi there */
+        PtMaterial white;
+        if (u->measureMode == 1 && bounces == 1) {
+            memset(&white, 0, sizeof white);
+            white.base[0] = white.base[1] = white.base[2] = 1; white.roughness = 1; white.ior = 1.5; white.face = -1;
+            m = &white;
+        }
         d3 Le = f3(m->emissive);
         if (max3d(Le) > 0 && caustic && sdelta && m->face >= 0 && faceFlux(&S->faces[m->face]) > 0) break;
         if (max3d(Le) > 0) {
@@ -1455,11 +1534,13 @@ int or_scene_at(const int* ops, int nops, const float* U, const double* pts, int
     return 0;
 }
 
-/* The BSDF for quadrature tests: eval (rgb), pdf, and a sample from u (dir, weight, flags). view/l/n world. */
+/* The BSDF for quadrature tests: eval (rgb), pdf, and a sample from u (dir, weight, flags). view/l/n world. through:
+ * evaluate the transmission below the surface too. */
 int or_bsdf(const PtMaterial* m, const double* n3, const double* view3, double eta, const double* l3, double* f, double* pdf,
-            const double* u4, double* sdir, double* sw, int* flags) {
+            const double* u4, double* sdir, double* sw, int* flags, int through) {
     or_mf_tables(NULL);
     OrLobes L = makeLobes(v(n3[0], n3[1], n3[2]), v(view3[0], view3[1], view3[2]), m, eta);
+    L.through = through;
     if (l3 && f) { d3 e = lobesEval(&L, v(l3[0], l3[1], l3[2])); f[0] = e.x; f[1] = e.y; f[2] = e.z; }
     if (l3 && pdf) *pdf = lobesPdf(&L, v(l3[0], l3[1], l3[2]));
     if (u4 && sdir) {

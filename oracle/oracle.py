@@ -35,7 +35,7 @@ def lib():
         L.or_render.argtypes = [P(c_i), c_i, P(c_f), c_v, c_i, c_v, c_i, c_v, c_u, c_u, P(c_d), P(c_d), P(ctypes.c_longlong), c_i]
         L.or_pick.argtypes = [P(c_i), c_i, P(c_f), c_v, c_u, c_u, c_u, P(c_d), P(c_u)]
         L.or_scene_at.argtypes = [P(c_i), c_i, P(c_f), P(c_d), c_i, P(c_d), P(c_d), P(c_i)]
-        L.or_bsdf.argtypes = [c_v, P(c_d), P(c_d), c_d, P(c_d), P(c_d), P(c_d), P(c_d), P(c_d), P(c_d), P(c_i)]
+        L.or_bsdf.argtypes = [c_v, P(c_d), P(c_d), c_d, P(c_d), P(c_d), P(c_d), P(c_d), P(c_d), P(c_d), P(c_i), ctypes.c_int]
         L.or_fresnel.restype = c_d
         L.or_fresnel.argtypes = [c_d, c_d]
         L.or_cauchy.restype = c_d
@@ -72,6 +72,7 @@ def pack_program(program):
             out[i, 1] = SHAPES.index(o['shape']); out[i, 2] = o['xf']; out[i, 3] = o['p']; out[i, 4] = o['mat']; out[i, 5] = o['prim']
         elif o['op'] == 'join':
             out[i, 6] = JOINS.index(o['join']); out[i, 7] = o['k']
+            out[i, 1] = ('smooth', 'round', 'chamfer').index(o.get('shape', 'smooth'))
         elif o['op'] == 'node':
             out[i, 2] = o['bs']; out[i, 6] = o['bit']; out[i, 7] = o['end']
     return np.ascontiguousarray(out)
@@ -151,14 +152,15 @@ def scene_at(scene, pts):
     return d, nrm, prim
 
 
-def bsdf(material, n, view, eta, l=None, u=None):
-    """eval/pdf at l and a sample from u, for one material (a PtMaterial)."""
+def bsdf(material, n, view, eta, l=None, u=None, through=False):
+    """eval/pdf at l and a sample from u, for one material (a PtMaterial); through: the transmission lobe below the
+    surface evaluated too."""
     n = np.asarray(n, np.float64); view = np.asarray(view, np.float64)
     f = np.zeros(3); pdf = np.zeros(1); sd = np.zeros(3); sw = np.zeros(3); fl = ctypes.c_int(0)
     la = np.asarray(l, np.float64) if l is not None else None
     ua = np.asarray(u, np.float64) if u is not None else None
     lib().or_bsdf(ctypes.byref(material), _dp(n), _dp(view), eta, _dp(la) if la is not None else None, _dp(f), _dp(pdf),
-                  _dp(ua) if ua is not None else None, _dp(sd), _dp(sw), ctypes.byref(fl))
+                  _dp(ua) if ua is not None else None, _dp(sd), _dp(sw), ctypes.byref(fl), int(through))
     return f, pdf[0], sd, sw, fl.value
 
 

@@ -6,6 +6,10 @@
 // tables Ec, Ec_avg and E1 are integrated on the GPU at start-up (pt_mf_lut below). Sampled one lobe at a time
 // (metal with p = m, else the diffuse base with p = (1-ps)(1-T), else the interface) with the lobe's own
 // f cos / pdf over its pick probability as the weight; lobesPdf is the mixture density, as MIS needs.
+// The rough interface's transmission (materials_pbr.js:99-100 lobesThrough, :296-357) is evaluated too where the caller
+// asks for it (Lobes.through, false from makeLobes), for a direction below the surface: Walter et al. 2007's BTDF over
+// the generalised half vector h = -(eta o + i), in basic radiance (no n^2 -- the engine carries radiance through an
+// interface as the sampler's (1 - F) weight does), so that eval, pdf and the sampler's weight agree exactly.
 #pragma once
 
 #define PT_ALPHA_MIN 1e-3f
@@ -118,6 +122,7 @@ struct Lobes {
     float ps;          // interface pick at the view
     float ecO, ecAvg;
     V3 cm;
+    bool through;      // evaluate the transmission lobe below the surface (guided sampling's mixture reads it)
 };
 
 __device__ __forceinline__ void frameOf(V3 n, V3& t, V3& b) {
@@ -158,12 +163,39 @@ __device__ __forceinline__ Lobes makeLobes(V3 n, V3 view, const PtMaterial& m, f
     lb.ecAvg = fminf(lb.coatK * mfEcAvg(T, lb.r, lb.nrel), 1.0f);
     float e1 = lb.metallic > 0.0f ? mfE1(T, lb.o.z, lb.r) : 1.0f;
     lb.cm = v3s(1.0f) + lb.base * (1.0f / fmaxf(e1, 1e-3f) - 1.0f);
+    lb.through = false;
     return lb;
 }
 
-// f(o, i) for reflection; transmission is not evaluated (the configs' transmissive material is smooth glass).
+// Refraction through a rough interface, i below the surface: the facet that takes o to i is h = -(eta o + i)
+// normalised (turned to the view's side), a refraction only with o in front of it and i behind; the Jacobian
+// |dw_h / dw_i| = |i.h| / (eta o.h + i.h)^2.
+__device__ __forceinline__ bool refractFacet(const Lobes& lb, V3 i, V3& h, float& oh, float& ih, float& jac) {
+    h = (lb.o * lb.eta + i) * -1.0f;
+    const float hh = dot(h, h);
+    if (!(hh > 1e-24f)) return false;
+    h = h * frsq(hh);
+    if (h.z < 0.0f) h = -h;
+    oh = dot(lb.o, h);
+    ih = dot(i, h);
+    if (oh <= 0.0f || ih >= 0.0f) return false;
+    const float den = lb.eta * oh + ih;
+    jac = -ih / fmaxf(den * den, 1e-20f);
+    return true;
+}
+// f below the surface: (1 - m) T (1 - F(o.h)) tint D(h) G2(o, i) (o.h) |i.h| / (o.z |i.z| (eta o.h + i.h)^2)
+__device__ __forceinline__ V3 lobesEvalThrough(const Lobes& lb, V3 i) {
+    const float wd = 1.0f - lb.metallic;
+    V3 h; float oh, ih, jac;
+    if (wd <= 0.0f || lb.trans <= 0.0f || lb.isDelta || !refractFacet(lb, i, h, oh, ih, jac)) return v3s(0.0f);
+    const float k = wd * lb.trans * (1.0f - coatF(lb, oh)) * mfD(h, lb.a) * mfG2(lb.o, i, lb.a) * oh * jac / (lb.o.z * -i.z);
+    return lb.tint * k;
+}
+
+// f(o, i): reflection above the surface; below it, the rough transmission where the caller asked (through).
 __device__ __forceinline__ V3 lobesEval(const Lobes& lb, const float* __restrict__ T, V3 l) {
     V3 i = toLocal(lb, l);
+    if (i.z < 0.0f && lb.through) return lobesEvalThrough(lb, i);
     if (i.z <= 0.0f) return v3s(0.0f);
     V3 f = v3s(0.0f);
     if (!lb.isDelta) {
@@ -185,8 +217,18 @@ __device__ __forceinline__ float interfaceReflectP(const Lobes& lb, float oh) {
     float den = fh + (1.0f - fh) * lb.trans;
     return den > 1e-6f ? fh / den : 1.0f;
 }
+// the refraction's density under the sampler: the dielectric pick, the interface's, the refraction's share of the
+// facet, the visible-normal density G1(o) (o.h) D(h) / o.z, and the Jacobian
+__device__ __forceinline__ float lobesPdfThrough(const Lobes& lb, V3 i) {
+    const float wd = 1.0f - lb.metallic;
+    V3 h; float oh, ih, jac;
+    if (wd <= 0.0f || lb.trans <= 0.0f || lb.isDelta || !refractFacet(lb, i, h, oh, ih, jac)) return 0.0f;
+    const float pd = (1.0f - lb.ps) * (1.0f - lb.trans);
+    return wd * (1.0f - pd) * (1.0f - interfaceReflectP(lb, oh)) * mfG1(lb.o, lb.a) * mfD(h, lb.a) * oh / lb.o.z * jac;
+}
 __device__ __forceinline__ float lobesPdf(const Lobes& lb, V3 l) {
     V3 i = toLocal(lb, l);
+    if (i.z < 0.0f && lb.through) return lobesPdfThrough(lb, i);
     if (i.z <= 0.0f) return 0.0f;
     float wd = 1.0f - lb.metallic;
     float pd = (1.0f - lb.ps) * (1.0f - lb.trans);

@@ -273,8 +273,10 @@ __device__ __forceinline__ V3 guideSampleAt(const PtDev* __restrict__ D, unsigne
 
 // Where guiding applies: a camera path's surface vertex whose material has a lobe to evaluate, reflects only (a
 // transmissive lobe is sampled, not evaluated) and emits nothing.
+// A vertex guiding can serve: a material with a lobe that has a density (a rough interface, refracting or not, or a
+// diffuse base; not a mirror or clear glass, whose directions no proposal can share), and not an emitter.
 __device__ __forceinline__ bool guideMaterial(const PtMaterial& m) {
-    return !lobesNone(m) && m.transmission <= 0.0f && max3(ld3(m.emissive)) <= 0.0f;
+    return !lobesNone(m) && max3(ld3(m.emissive)) <= 0.0f;
 }
 __device__ __forceinline__ bool guideActive(const PtDev* __restrict__ D) {
     return D->u.guide != 0u && D->u.guideFrac > 0.0f && D->counters[PT_C_GUIDE_T] >= D->g.warmup;

@@ -179,8 +179,8 @@ __device__ __forceinline__ V3 sunSky(const PtEnv& e, V3 d) {
         const float v = (tr * pR + 0.9f * ta * pM) / tau * k1;
         r += PT_SKY_RGB[3 * k] * v; gr += PT_SKY_RGB[3 * k + 1] * v; b += PT_SKY_RGB[3 * k + 2] * v;
     }
-    const float f = e.L * e.skyUp;
-    return v3(fmaxf(f * r, 0.0f), fmaxf(f * gr, 0.0f), fmaxf(f * b, 0.0f));
+    const float f = e.L * e.skyUp;     // col: the sky's gain per channel (envSky)
+    return v3(fmaxf(f * r * e.col[0], 0.0f), fmaxf(f * gr * e.col[1], 0.0f), fmaxf(f * b * e.col[2], 0.0f));
 }
 __device__ __forceinline__ V3 envBackground(const PtEnv& e, V3 d) {
     if (e.type == 1u || (e.type == 2u && d.z > 0.0f)) return ld3(e.col) * e.L;

@@ -229,46 +229,73 @@ __device__ __forceinline__ float Ddisc(V3 p, float r) {
     return fsqrt(e * e + p.z * p.z);
 }
 
-// A spherical lens along z (sdf.js Nlens:463, lensFace:450): rim radius rim, the faces' vertices at z = +-vz, r1 the
-// +z face's signed radius and r2 the -z face's (positive bulges out, negative is hollowed in, 0 is flat). The solid is
-// the rim's cylinder cut by each face's solid -- a bulging face's ball; a hollow face's ball taken away, closed by the
-// plane through the face's rim at z = +-cap -- and its value the largest of the pieces' (exact off the edges, a bound
-// near them). s is the face's side (+1, -1); cap matters only for a hollow face.
-__device__ __forceinline__ V4 NlensFace(V3 p, float s, float vz, float R, float cap) {
-    if (R == 0.0f) return v4(v3(0.0f, 0.0f, s), s * p.z - vz);
-    V3 q = v3(p.x, p.y, p.z - s * (vz - R));        // from the face's centre of curvature
-    float l = fmaxf(length(q), 1e-12f);
-    float sg = R > 0.0f ? 1.0f : -1.0f;
-    V4 f = v4(q * (sg * frcp(l)), sg * (l - fabsf(R)));
-    float w = s * p.z - cap;
-    if (R < 0.0f && w > f.w) f = v4(v3(0.0f, 0.0f, s), w);
-    return f;
+// ---- shapes of revolution are drawn in their meridian half-plane: r = |p.xy| >= 0 across, z up. A piece there answers
+// a Mer -- the outward unit normal (nr, nz) and the signed distance d -- and merTo3 turns the answer about the z axis.
+// An intersection of pieces keeps the piece with the larger d (the first on a tie).
+struct Mer { float nr, nz, d; };
+__device__ __forceinline__ Mer mer(float nr, float nz, float d) { Mer m; m.nr = nr; m.nz = nz; m.d = d; return m; }
+__device__ __forceinline__ Mer merMax(Mer a, Mer b) { return b.d > a.d ? b : a; }
+__device__ __forceinline__ Mer merFlip(Mer a) { return mer(-a.nr, -a.nz, -a.d); }
+// The normal's r part goes along p.xy's direction; on the axis, where every direction is one, along +x.
+__device__ __forceinline__ V4 merTo3(V3 p, Mer m) {
+    const float rr = p.x * p.x + p.y * p.y;
+    if (!(rr > 1e-24f)) return v4(v3(m.nr, 0.0f, m.nz), m.d);
+    const float k = m.nr * frsq(rr);
+    return v4(v3(p.x * k, p.y * k, m.nz), m.d);
 }
-__device__ __forceinline__ V4 Nlens(V3 p, float rim, float vz, float r1, float r2, float cap1, float cap2) {
-    float lxy = fmaxf(fsqrt(p.x * p.x + p.y * p.y), 1e-12f);
-    float il = frcp(lxy);
-    V4 n = v4(v3(p.x * il, p.y * il, 0.0f), lxy - rim);
-    V4 a = NlensFace(p, 1.0f, vz, r1, cap1);
-    if (a.w > n.w) n = a;
-    V4 b = NlensFace(p, -1.0f, vz, r2, cap2);
-    if (b.w > n.w) n = b;
-    return n;
+// A disc of radius rad about (cr, cz) (a ball, turned): the distance from its centre less rad, the normal away from
+// the centre (+r from the centre itself).
+__device__ __forceinline__ Mer merCircle(float r, float z, float cr, float cz, float rad) {
+    const float er = r - cr, ez = z - cz, l = fsqrt(er * er + ez * ez);
+    if (!(l > 1e-12f)) return mer(1.0f, 0.0f, l - rad);
+    const float k = frcp(l);
+    return mer(er * k, ez * k, l - rad);
 }
-// one face's value, z already times the face's side, l2 = x^2 + y^2
-__device__ __forceinline__ float DlensFace(float l2, float z, float vz, float R, float cap) {
-    if (R == 0.0f) return z - vz;
-    float dz = z - (vz - R);
-    float f = fmaxf(fsqrt(l2 + dz * dz), 1e-12f) - fabsf(R);
-    return R > 0.0f ? f : fmaxf(-f, z - cap);
-}
-__device__ __forceinline__ float Dlens(V3 p, float rim, float vz, float r1, float r2, float cap1, float cap2) {
-    float l2 = p.x * p.x + p.y * p.y;
-    float d = fmaxf(fsqrt(l2), 1e-12f) - rim;
-    d = fmaxf(d, DlensFace(l2, p.z, vz, r1, cap1));
-    return fmaxf(d, DlensFace(l2, -p.z, vz, r2, cap2));
+// The rectangle of half sizes (hr, hz) about (cr, cz), exact both sides: within it, the nearer pair of sides (the z
+// pair on a tie); past it, the distance to its nearest corner or side, measured by how far past each pair it lies.
+__device__ __forceinline__ Mer merRect(float r, float z, float cr, float cz, float hr, float hz) {
+    const float sr = r < cr ? -1.0f : 1.0f, sz = z < cz ? -1.0f : 1.0f;
+    const float pr = fabsf(r - cr) - hr, pz = fabsf(z - cz) - hz;
+    if (fmaxf(pr, pz) <= 0.0f) return pr > pz ? mer(sr, 0.0f, pr) : mer(0.0f, sz, pz);
+    const float ar = fmaxf(pr, 0.0f), az = fmaxf(pz, 0.0f), l = fsqrt(ar * ar + az * az), k = frcp(l);
+    return mer(sr * ar * k, sz * az * k, l);
 }
 
-// A prism along z (shapes.js Nprism:285 over g2RegularPolygon and g2Extrude): a regular polygon of n sides, its
+// A spherical lens along z (sdf.js Nlens:463): rim radius rim, the faces' vertices at z = +-vz, r1 the +z face's signed
+// radius and r2 the -z face's (positive bulges out, negative is hollowed in, 0 is flat). In the meridian plane the lens
+// is the band r <= rim cut by each face's solid. Measured along a face's own outward axis (zs = s z, s = +1 for the +z
+// face), the face's sphere has its centre on the axis at vz - R and radius |R|: a bulging face keeps the inside of that
+// circle, a hollow one the outside, closed by the line zs = cap through its rim (the rim's height, from the emitter).
+// The value is the largest of the pieces' -- exact away from the edges, a bound near them.
+__device__ __forceinline__ Mer lensSide(float r, float z, float s, float vz, float R, float cap) {
+    const float zs = s * z;
+    if (R == 0.0f) return mer(0.0f, s, zs - vz);
+    const Mer c = merCircle(r, zs, 0.0f, vz - R, fabsf(R));
+    if (R > 0.0f) return mer(c.nr, s * c.nz, c.d);
+    const Mer outside = mer(-c.nr, -s * c.nz, -c.d);
+    return zs - cap > outside.d ? mer(0.0f, s, zs - cap) : outside;
+}
+__device__ __forceinline__ V4 Nlens(V3 p, float rim, float vz, float r1, float r2, float cap1, float cap2) {
+    const float r = fsqrt(p.x * p.x + p.y * p.y);
+    Mer m = mer(1.0f, 0.0f, r - rim);
+    m = merMax(m, lensSide(r, p.z, 1.0f, vz, r1, cap1));
+    m = merMax(m, lensSide(r, p.z, -1.0f, vz, r2, cap2));
+    return merTo3(p, m);
+}
+// the same value alone, for the march: rr = r^2, zs along the face's axis
+__device__ __forceinline__ float lensSideD(float rr, float zs, float vz, float R, float cap) {
+    if (R == 0.0f) return zs - vz;
+    const float ez = zs - vz + R;
+    const float c = fsqrt(rr + ez * ez) - fabsf(R);
+    return R > 0.0f ? c : fmaxf(-c, zs - cap);
+}
+__device__ __forceinline__ float Dlens(V3 p, float rim, float vz, float r1, float r2, float cap1, float cap2) {
+    const float rr = p.x * p.x + p.y * p.y;
+    const float d = fmaxf(fsqrt(rr) - rim, lensSideD(rr, p.z, vz, r1, cap1));
+    return fmaxf(d, lensSideD(rr, -p.z, vz, r2, cap2));
+}
+
+// A prism along z (shapes.js Nprism:285): a regular polygon of n sides, its
 // corners at radius R and one flat side facing -y, extruded to half height h. in = 1/n, a = R cos(pi/n) the apothem,
 // e = R sin(pi/n) half a side. The nearest side is the one whose outward normal is nearest the point's direction:
 // inside, that side has the largest projection; outside, a point in its wedge is nearest that side or one of its ends.
@@ -304,35 +331,7 @@ __device__ __forceinline__ V4 Nprism(V3 p, float n, float in, float a, float e, 
     if (g > wz) return v4(v3(gx, gy, 0.0f), g);
     return v4(v3(0.0f, 0.0f, sz), wz);
 }
-// A meridian-plane result (gx, gy: the outward normal in (r, z); d the signed distance), as the revolved shapes
-// build theirs; ties go to the first (shapes.js g2Max:102).
-struct G2 { float x, y, d; };
-__device__ __forceinline__ G2 g2(float x, float y, float d) { G2 r; r.x = x; r.y = y; r.d = d; return r; }
-__device__ __forceinline__ G2 g2max(G2 a, G2 b) { return a.d >= b.d ? a : b; }
-// Lifted about the z axis: the normal's radial part along p.xy's direction (+x on the axis) (shapes.js g2Revolve).
-__device__ __forceinline__ V4 g2revolve(V3 p, G2 g) {
-    float l = fsqrt(p.x * p.x + p.y * p.y);
-    float rx = 1.0f, ry = 0.0f;
-    if (l > 1e-12f) { float il = frcp(l); rx = p.x * il; ry = p.y * il; }
-    return v4(v3(rx * g.x, ry * g.x, g.y), g.d);
-}
-// ---- the builder's other primitives (builder/primitives.js:139-236; shapes.js, sdf.js sdCappedCone:633)
-__device__ __forceinline__ G2 g2circle(float qx, float qy, float cx, float cy, float r) {
-    const float ex = qx - cx, ey = qy - cy, l = fsqrt(ex * ex + ey * ey);
-    if (l > 1e-12f) { const float il = frcp(l); return g2(ex * il, ey * il, l - r); }
-    return g2(1.0f, 0.0f, l - r);
-}
-// the 2D box of half sizes (bx, by) about (cx, cy): exact both sides (shapes.js g2Box:46)
-__device__ __forceinline__ G2 g2box(float qx, float qy, float cx, float cy, float bx, float by) {
-    const float sx = qx >= cx ? 1.0f : -1.0f, sy = qy >= cy ? 1.0f : -1.0f;
-    const float ex = fabsf(qx - cx) - bx, ey = fabsf(qy - cy) - by;
-    if (ex > 0.0f || ey > 0.0f) {
-        const float ox = fmaxf(ex, 0.0f), oy = fmaxf(ey, 0.0f), l = fsqrt(ox * ox + oy * oy), il = frcp(l);
-        return g2(ox * il * sx, oy * il * sy, l);
-    }
-    if (ex > ey) return g2(sx, 0.0f, ex);
-    return g2(0.0f, sy, ey);
-}
+// ---- the builder's other primitives (builder/primitives.js:139-236)
 // A capped cone along z, base radius at z = -h, top radius at +h (Quilez's capped cone, sdf.js sdCappedCone:633):
 // distance only; the surface kernel takes its normal by differences.
 __device__ __forceinline__ float Dcone(V3 p, float h, float r1, float r2) {
@@ -345,43 +344,49 @@ __device__ __forceinline__ float Dcone(V3 p, float h, float r1, float r2) {
     return s * fsqrt(fminf(cax * cax + cay * cay, cbx * cbx + cby * cby));
 }
 __device__ __forceinline__ V4 Ncone(V3 p, float h, float r1, float r2) { return v4(v3s(0.0f), Dcone(p, h, r1, r2)); }
-// Two spheres along z, r1 at z = -h/2 and r2 at +h/2, and the cone touching both (shapes.js NroundCone:119); a height
-// under the radii's difference leaves the bigger sphere.
+// Two balls on the z axis, radius r1 about z = -h/2 and r2 about z = +h/2, and the cone tangent to both: their convex
+// hull (builder/primitives.js round cone). In the meridian plane, measured from the lower centre (z' = z + h/2), the
+// outer tangent line has unit normal n = (cos a, sin a) with sin a = (r1 - r2) / h, and meets the circles at r1 n and
+// (0, h) + r2 n; along the line's direction t = (-sin a, cos a) the first contact is at 0 and the second at h cos a. A
+// point whose foot along t comes before the first contact is nearest the lower ball, after the second the upper one,
+// else the line, at distance q . n - r1. A height of |r1 - r2| or less leaves the bigger ball holding the other.
 __device__ __forceinline__ V4 NroundCone(V3 p, float r1, float r2, float h) {
-    const float qx = fsqrt(p.x * p.x + p.y * p.y), qy = p.z + 0.5f * h;
-    if (h <= fabsf(r1 - r2)) return g2revolve(p, r1 >= r2 ? g2circle(qx, qy, 0.0f, 0.0f, r1) : g2circle(qx, qy, 0.0f, h, r2));
-    const float b = (r1 - r2) / h, a = fsqrt(1.0f - b * b);
-    const float k = -b * qx + a * qy;
-    if (k < 0.0f) return g2revolve(p, g2circle(qx, qy, 0.0f, 0.0f, r1));
-    if (k > a * h) return g2revolve(p, g2circle(qx, qy, 0.0f, h, r2));
-    return g2revolve(p, g2(a, b, a * qx + b * qy - r1));
+    const float r = fsqrt(p.x * p.x + p.y * p.y), z = p.z + 0.5f * h;
+    Mer m;
+    if (h <= fabsf(r1 - r2)) {
+        m = r2 > r1 ? merCircle(r, z, 0.0f, h, r2) : merCircle(r, z, 0.0f, 0.0f, r1);
+    } else {
+        const float sa = (r1 - r2) / h, ca = fsqrt(1.0f - sa * sa);
+        const float foot = ca * z - sa * r;
+        if (foot < 0.0f) m = merCircle(r, z, 0.0f, 0.0f, r1);
+        else if (foot > h * ca) m = merCircle(r, z, 0.0f, h, r2);
+        else m = mer(ca, sa, ca * r + sa * z - r1);
+    }
+    return merTo3(p, m);
 }
-// A regular octahedron, size from its centre to each vertex (shapes.js Noctahedron:135): exact both sides -- inside
-// the nearest face's plane, outside the nearest point of the face, edge or vertex in the point's octant.
+// A regular octahedron |x| + |y| + |z| <= s (s from the centre to each vertex), exact both sides. Folded into the
+// positive octant (a = |p|) its one face there is the triangle (s,0,0), (0,s,0), (0,0,s) on the plane a.1 = s, normal
+// (1,1,1)/sqrt 3. Inside the solid, and outside wherever a's foot on that plane (a less e (1,1,1), e its height over
+// the plane over sqrt 3) lies in the triangle, the distance is the plane's. Otherwise the nearest point is on one of the
+// triangle's edges, each lying in a coordinate plane, and the least of the three edge distances is the distance.
 __device__ __forceinline__ V3 safeDir3(V3 v, V3 fb) { const float l = length(v); return l > 1e-12f ? v * frcp(l) : fb; }
 __device__ __forceinline__ V4 Noctahedron(V3 p, float s0) {
-    const float s = fmaxf(s0, 0.0f);
-    const V3 sg = v3(p.x >= 0.0f ? 1.0f : -1.0f, p.y >= 0.0f ? 1.0f : -1.0f, p.z >= 0.0f ? 1.0f : -1.0f);
-    const V3 a = vabs(p);
-    const float m = a.x + a.y + a.z - s;
-    const float c = 0.5773502691896258f;
-    const V3 face = sg * c, fb = v3s(c);
-    if (3.0f * a.x < m) {
-        const float k = fminf(fmaxf(0.5f * (a.z - a.y + s), 0.0f), s);
-        const V3 v = v3(a.x, a.y - s + k, a.z - k);
-        return v4(sg * safeDir3(v, fb), length(v));
-    }
-    if (3.0f * a.y < m) {
-        const float k = fminf(fmaxf(0.5f * (a.x - a.z + s), 0.0f), s);
-        const V3 v = v3(a.y, a.z - s + k, a.x - k);
-        return v4(sg * safeDir3(v3(v.z, v.x, v.y), fb), length(v));
-    }
-    if (3.0f * a.z < m) {
-        const float k = fminf(fmaxf(0.5f * (a.y - a.x + s), 0.0f), s);
-        const V3 v = v3(a.z, a.x - s + k, a.y - k);
-        return v4(sg * safeDir3(v3(v.y, v.z, v.x), fb), length(v));
-    }
-    return v4(face, m * c);
+    const float s = fmaxf(s0, 0.0f), k = 0.5773502691896258f;
+    const V3 a = vabs(p), sg = v3(p.x < 0.0f ? -1.0f : 1.0f, p.y < 0.0f ? -1.0f : 1.0f, p.z < 0.0f ? -1.0f : 1.0f);
+    const float h = (a.x + a.y + a.z - s) * k;
+    if (fminf(a.x, fminf(a.y, a.z)) >= h * k) return v4(sg * k, h);
+    // the edges: x = 0 from (0,s,0) to (0,0,s), y = 0 from (0,0,s) to (s,0,0), z = 0 from (s,0,0) to (0,s,0), each
+    // point written by its distance u from the edge's first end, clamped to the edge
+    const float ux = fminf(fmaxf(0.5f * (a.z - a.y + s), 0.0f), s);
+    const float uy = fminf(fmaxf(0.5f * (a.x - a.z + s), 0.0f), s);
+    const float uz = fminf(fmaxf(0.5f * (a.y - a.x + s), 0.0f), s);
+    const V3 ex = a - v3(0.0f, s - ux, ux), ey = a - v3(uy, 0.0f, s - uy), ez = a - v3(s - uz, uz, 0.0f);
+    V3 v = ex;
+    float vv = dot(ex, ex);
+    const float yy = dot(ey, ey), zz = dot(ez, ez);
+    if (yy < vv) { v = ey; vv = yy; }
+    if (zz < vv) { v = ez; vv = zz; }
+    return v4(sg * safeDir3(v, v3s(k)), fsqrt(vv));
 }
 // The builder's pyramid, a frustum along z (shapes.js Nfrustum:204): a base.x by base.y rectangle at z = -h/2 and a
 // top.x by top.y one at +h/2 (0 by 0: an apex). Convex and mirror-symmetric in x and y, so after folding to the
@@ -434,59 +439,73 @@ __device__ __forceinline__ V4 Nfrustum(V3 p, float bx0, float by0, float tx0, fl
     }
     return v4(v3(g.x * sx, g.y * sy, g.z), d);
 }
-// Part of a torus in the xy plane: an arc of radius R, angle degrees of it symmetric about +y, swept by a tube of
-// radius r (shapes.js NcappedTorus:335): the nearest point of the arc -- its end past the wedge -- then the tube.
-// sc = (sin, cos) of half the angle, from the emitter.
+// Part of a ring: the arc of radius R in the xy plane spanning `angle` symmetric about +y, swept by a tube of radius r
+// (builder/primitives.js capped torus). sn, cs: the sine and cosine of half the angle (from the emitter). Folded to
+// x >= 0 the arc runs from +y to its end R (sn, cs). The nearest arc point is the circle's own point in q.xy's direction
+// while that direction lies within the arc, else the end; q.xy's component along (cs, -sn), the normal of the end's ray
+// pointing past it, tells which. The tube is then the distance from that point less r.
 __device__ __forceinline__ V4 NcappedTorus(V3 p, float R, float r, float sn, float cs) {
-    const float sx = p.x >= 0.0f ? 1.0f : -1.0f;
-    const V3 q = v3(fabsf(p.x), p.y, p.z);
-    float cx, cy;
-    if (cs * q.x > sn * q.y) { cx = R * sn; cy = R * cs; }
-    else {
-        const float l = fsqrt(q.x * q.x + q.y * q.y);
-        if (l > 1e-12f) { const float il = frcp(l); cx = R * q.x * il; cy = R * q.y * il; } else { cx = 0.0f; cy = R; }
+    const float x = fabsf(p.x), y = p.y;
+    float ax = R * sn, ay = R * cs;
+    if (x * cs - y * sn <= 0.0f) {
+        const float rr = x * x + y * y;
+        if (rr > 1e-24f) { const float k = R * frsq(rr); ax = x * k; ay = y * k; } else { ax = 0.0f; ay = R; }
     }
-    const V3 v = v3(q.x - cx, q.y - cy, q.z);
-    const V3 n = safeDir3(v, v3(0.0f, 0.0f, 1.0f));
-    return v4(v3(n.x * sx, n.y, n.z), length(v) - r);
+    const V3 v = v3(x - ax, y - ay, p.z);
+    const float l = length(v);
+    const V3 n = l > 1e-12f ? v * frcp(l) : v3(0.0f, 0.0f, 1.0f);
+    return v4(v3(p.x < 0.0f ? -n.x : n.x, n.y, n.z), l - r);
 }
-// A chain link in the xy plane: a ring of radius R pulled apart along y by span, swept by a tube of radius r (shapes.js
-// Nlink:349).
+// A chain link: the torus of ring R and tube r about z, elongated along y by span -- y first moved to the nearest point
+// of [-span/2, span/2], which opens the ring into two half rings joined by straight lengths.
 __device__ __forceinline__ V4 Nlink(V3 p, float span, float R, float r) {
-    const float le = fmaxf(0.5f * span, 0.0f);
-    const float qy = (p.y >= 0.0f ? 1.0f : -1.0f) * fmaxf(fabsf(p.y) - le, 0.0f);
-    return Ntorus(v3(p.x, qy, p.z), R, r);
+    const float e = fmaxf(0.5f * span, 0.0f);
+    return Ntorus(v3(p.x, p.y - fminf(fmaxf(p.y, -e), e), p.z), R, r);
 }
-// A hollow cylinder along z, centred: the 2D box [inner, outer] x [-h/2, h/2] revolved (shapes.js Ntube:373); an inner
-// radius of 0 or less is the solid cylinder.
+// A hollow cylinder along z, centred (builder/primitives.js tube): the rectangle [inner, outer] x [-h/2, h/2] of the
+// meridian plane, turned; an inner radius of 0 or less is the solid cylinder (the rectangle from the axis out).
 __device__ __forceinline__ V4 Ntube(V3 p, float outer, float inner, float height) {
-    const float qx = fsqrt(p.x * p.x + p.y * p.y), hh = fmaxf(0.5f * height, 0.0f);
-    if (inner <= 0.0f) return g2revolve(p, g2box(qx, p.z, 0.0f, 0.0f, fmaxf(outer, 0.0f), hh));
+    const float r = fsqrt(p.x * p.x + p.y * p.y), hh = fmaxf(0.5f * height, 0.0f);
+    if (inner <= 0.0f) return merTo3(p, merRect(r, p.z, 0.0f, 0.0f, fmaxf(outer, 0.0f), hh));
     const float ro = fmaxf(outer, inner + 1e-6f);
-    return g2revolve(p, g2box(qx, p.z, 0.5f * (inner + ro), 0.0f, 0.5f * (ro - inner), hh));
+    return merTo3(p, merRect(r, p.z, 0.5f * (inner + ro), 0.0f, 0.5f * (ro - inner), hh));
 }
-// The part of a sphere of radius R above z = 0, height tall (shapes.js Ndome:358, Quilez's cut sphere): its centre at
-// c = height - R (held below R), exact both sides -- the sphere, the flat base, or the rim's circle.
+// A dome: the ball of radius R whose centre is at z = c = height - R (held in [-R, R)), cut by its base plane z = 0 and
+// kept above it -- the intersection of a ball and a half-space, convex. In the meridian plane: inside, the nearer of
+// the sphere (|q - C| - R) and the base (-z). Outside, the nearer of the base disc r <= w (w = sqrt(R^2 - c^2), the
+// rim's radius) and the cap: the sphere's own point towards q while that point is above the base, else the rim. Exact
+// both sides.
 __device__ __forceinline__ V4 Ndome(V3 p, float R0, float height) {
     const float R = fmaxf(R0, 1e-6f);
     const float c = fminf(fmaxf(height - R, -R), R * (1.0f - 1e-4f));
-    const float hc = -c, w = fsqrt(fmaxf(R * R - hc * hc, 0.0f));
-    const float qx = fsqrt(p.x * p.x + p.y * p.y), qy = p.z - c;
-    const float s = fmaxf((hc - R) * qx * qx + w * w * (hc + R - 2.0f * qy), hc * qx - w * qy);
-    if (s < 0.0f) return g2revolve(p, g2circle(qx, qy, 0.0f, 0.0f, R));
-    if (qx < w) return g2revolve(p, g2(0.0f, -1.0f, hc - qy));
-    return g2revolve(p, g2circle(qx, qy, w, hc, 0.0f));
+    const float w = fsqrt(fmaxf(R * R - c * c, 0.0f));
+    const float r = fsqrt(p.x * p.x + p.y * p.y), z = p.z, ez = z - c;
+    const float l = fsqrt(r * r + ez * ez);
+    Mer m;
+    if (z > 0.0f && l < R) {
+        m = -z > l - R ? mer(0.0f, -1.0f, -z) : merCircle(r, z, 0.0f, c, R);
+    } else {
+        const float b = fminf(r, w);
+        const float base2 = (r - b) * (r - b) + z * z;
+        // the cap's nearest point: the sphere's towards q when it is not below the base, else the rim (w, 0)
+        const bool radial = l > 1e-12f && c * l + R * ez >= 0.0f;
+        const float fr = radial ? R * r / l : w, fz = radial ? c + R * ez / l : 0.0f;
+        const float cap2 = (r - fr) * (r - fr) + (z - fz) * (z - fz);
+        if (base2 < cap2) m = r > w ? merCircle(r, z, w, 0.0f, 0.0f) : mer(0.0f, -1.0f, -z);
+        else m = radial ? merCircle(r, z, 0.0f, c, R) : merCircle(r, z, w, 0.0f, 0.0f);
+    }
+    return merTo3(p, m);
 }
 
-// Extruded along z to half height h, exactly (shapes.js g2Extrude:717): past both the outline and the caps, the
+// An outline's meridian answer extruded along z to half height h, exactly: past both the outline and the caps, the
 // corner's distance; else the larger of the two.
-__device__ __forceinline__ V4 g2extrude(V3 p, G2 g, float h) {
+__device__ __forceinline__ V4 merExtrude(V3 p, Mer g, float h) {
     const float wz = fabsf(p.z) - h, sz = p.z >= 0.0f ? 1.0f : -1.0f;
     if (g.d > 0.0f && wz > 0.0f) {
         const float l = fsqrt(g.d * g.d + wz * wz), il = frcp(l);
-        return v4(v3(g.x * g.d * il, g.y * g.d * il, sz * wz * il), l);
+        return v4(v3(g.nr * g.d * il, g.nz * g.d * il, sz * wz * il), l);
     }
-    if (g.d > wz) return v4(v3(g.x, g.y, 0.0f), g.d);
+    if (g.d > wz) return v4(v3(g.nr, g.nz, 0.0f), g.d);
     return v4(v3(0.0f, 0.0f, sz), wz);
 }
 // An outline of segments and fillet arcs (counter-clockwise), walked piece by piece (shapes.js pfStart..pfEnd:684-715):
@@ -537,61 +556,57 @@ __device__ __forceinline__ void pfArc(Prof& A, float qx, float qy, float cx, flo
         A.nx = convex ? nx : -nx; A.ny = convex ? ny : -ny;
     }
 }
-__device__ __forceinline__ G2 pfEnd(const Prof& A) {
+__device__ __forceinline__ Mer pfEnd(const Prof& A) {
     const float d = fsqrt(A.d2), s = A.inside ? -1.0f : 1.0f;
-    if (d > 1e-12f) { const float k = s / fmaxf(d, 1e-30f); return g2(A.vx * k, A.vy * k, s * d); }
-    return g2(A.nx, A.ny, s * d);
+    if (d > 1e-12f) { const float k = s / fmaxf(d, 1e-30f); return mer(A.vx * k, A.vy * k, s * d); }
+    return mer(A.nx, A.ny, s * d);
 }
-// The signed distance in a meridian plane from (x, y) to the parabola y = k x^2 (k > 0), positive below it, with its
-// outward normal (shapes.js g2Parabola:61). The nearest point's abscissa t solves the depressed cubic
-// 2k^2 t^3 + (1 - 2ky) t - x = 0: with a = (2ky - 1) / 6k^2 and b = x / 4k^2 its discriminant is b^2 - a^3; above
-// zero one real root (Cardano, as 2b / (u^2 - a + w^2) with u = cbrt(b + sqrt(h)), w = a / u, which keeps its
-// digits), else the largest of three, 2 sqrt(a) cos(atan2(sqrt(-h), b) / 3).
-__device__ __forceinline__ G2 g2parabola(float x, float y, float k) {
-    const float sx = x >= 0.0f ? 1.0f : -1.0f, xa = fabsf(x);
-    const float ik = 1.0f / k;
-    const float a = ik * (y - 0.5f * ik) * (1.0f / 3.0f);
-    const float b = 0.25f * ik * ik * xa;
-    const float h = b * b - a * a * a;
-    float t;
-    if (h > 0.0f) {
-        const float u = cbrtf(b + fsqrt(h));
-        const float w = a / u;
-        t = 2.0f * b / (u * u - a + w * w);
-    } else {
-        t = 2.0f * fsqrt(fmaxf(a, 0.0f)) * cosf(atan2f(fsqrt(-h), b) * (1.0f / 3.0f));
+// The parabola z = k r^2 (k > 0) of a meridian plane: the signed distance from (x, y), positive below the curve
+// (outside the bowl), and the outward normal at the nearest point. The nearest point (t, k t^2), for |x|, sets the
+// derivative of the squared distance to zero: 2 k^2 t^3 + (1 - 2 k y) t - |x| = 0, i.e. t^3 + P t + Q = 0 with
+// P = (1 - 2 k y) / 2k^2 and Q = -|x| / 2k^2 <= 0, whose largest real root is the one. Fujiwara's bound puts every root
+// within 2 max(sqrt|P|, cbrt(|Q|/2)); Newton's method started there descends on the largest root without overshoot
+// (the cubic is convex and rising beyond it), and eight steps reach f32 precision over the reflectors' range.
+__device__ __forceinline__ Mer parabolaMer(float x, float y, float k) {
+    const float ax = fabsf(x);
+    const float ik2 = 1.0f / (2.0f * k * k);
+    const float P = (1.0f - 2.0f * k * y) * ik2, Q = -ax * ik2;
+    float t = 2.0f * fmaxf(fsqrt(fabsf(P)), cbrtf(-0.5f * Q));
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const float den = 3.0f * t * t + P;
+        t = den > 1e-30f ? (2.0f * t * t * t - Q) / den : t;
     }
-    const float ex = xa - t, ey = y - k * t * t;
-    const float side = k * xa * xa - y >= 0.0f ? 1.0f : -1.0f;
-    const float in = frsq(1.0f + 4.0f * k * k * t * t);
-    return g2(2.0f * k * t * in * sx, -in, side * fsqrt(ex * ex + ey * ey));
+    const float er = ax - t, ez = y - k * t * t;
+    const float slope = 2.0f * k * t, in = frsq(1.0f + slope * slope);
+    const float d = fsqrt(er * er + ez * ez);
+    return mer((x < 0.0f ? -slope : slope) * in, -in, k * ax * ax >= y ? d : -d);
 }
-// ...with its focus moved out to a ring of radius c in the plane of the foci (shapes.js g2ParabolaOffset:93): the two
-// half-planes' parabolas shifted by +-c, the larger of the two values for c < 0, the smaller for c > 0.
-__device__ __forceinline__ G2 g2parabolaOff(float x, float y, float k, float c) {
-    const G2 a = g2parabola(x - c, y, k);
-    if (c == 0.0f) return a;
-    const G2 b = g2parabola(x + c, y, k);
-    if (c < 0.0f) return a.d >= b.d ? a : b;
-    return a.d <= b.d ? a : b;
-}
-// A parabolic reflector (shapes.js NparabolicOffset:418 over NdishOffset:392 and NparaboloidOffset): its focus at the
-// origin, opening to +z, the mirror z = (r - c)^2 / 4f - f out to radius A. solid <= 0.5: the mirror with its wall w
-// thick behind it (the band 0 <= s <= w of the distance s to the parabola), cut by the cylinder r <= A; else the
-// paraboloid's inside, capped at the rim's height. U: f, A, w, solid, c (c held at most A by the emitter).
-__device__ __forceinline__ G2 parabolicG2(V3 p, float f, float A, float wall, float solid, float c) {
+// A parabolic reflector (shapes.js NparabolicOffset:418, builder/primitives.js reflector): focus at the origin, opening
+// to +z, the mirror z = (r - c)^2 / 4f - f out to radius A, c the radius of a ring of foci (c 0: a point focus). The
+// mirror's profile in the half-plane r >= 0 is the parabola of focal length f centred on r = c; the surface turned
+// about the axis also holds its mirror image, centred on r = -c, and of the two the nearer counts where the ring is on
+// the point's side (c > 0, their union) and the farther where it is across the axis (c < 0, the two bowls' common
+// inside). solid <= 0.5: the mirror with its wall w thick behind it (0 <= s <= w in the distance s to the profile),
+// cut by the cylinder r <= A; else the bowl's inside, capped at the rim's height. U: f, A, w, solid, c (c held at most
+// A by the emitter).
+__device__ __forceinline__ Mer reflectorMer(V3 p, float f, float A, float wall, float solid, float c) {
     const float fo = fmaxf(f, 1e-6f), Aa = fmaxf(A, 0.0f), cc = fminf(c, Aa);
-    const float x = fsqrt(p.x * p.x + p.y * p.y), y = p.z + fo;
-    const G2 s = g2parabolaOff(x, y, 0.25f / fo, cc);
-    if (solid > 0.5f) return g2max(s, g2(0.0f, 1.0f, y - 0.25f * (Aa - cc) * (Aa - cc) / fo));
-    const G2 band = g2max(g2(-s.x, -s.y, -s.d), g2(s.x, s.y, s.d - fmaxf(wall, 1e-6f)));
-    return g2max(band, g2(1.0f, 0.0f, x - Aa));
+    const float r = fsqrt(p.x * p.x + p.y * p.y), y = p.z + fo, k = 0.25f / fo;
+    Mer s = parabolaMer(r - cc, y, k);
+    if (cc != 0.0f) {
+        const Mer o = parabolaMer(r + cc, y, k);
+        if (cc < 0.0f ? o.d > s.d : o.d < s.d) s = o;
+    }
+    if (solid > 0.5f) return merMax(s, mer(0.0f, 1.0f, y - k * (Aa - cc) * (Aa - cc)));
+    const Mer shell = merMax(merFlip(s), mer(s.nr, s.nz, s.d - fmaxf(wall, 1e-6f)));
+    return merMax(shell, mer(1.0f, 0.0f, r - Aa));
 }
 __device__ __forceinline__ V4 Nparabolic(V3 p, float f, float A, float wall, float solid, float c) {
-    return g2revolve(p, parabolicG2(p, f, A, wall, solid, c));
+    return merTo3(p, reflectorMer(p, f, A, wall, solid, c));
 }
 __device__ __forceinline__ float Dparabolic(V3 p, float f, float A, float wall, float solid, float c) {
-    return parabolicG2(p, f, A, wall, solid, c).d;
+    return reflectorMer(p, f, A, wall, solid, c).d;
 }
 
 // An aspheric lens along z (shapes.js Nasphere:658): a round rim of radius rho and two even-asphere faces, their
@@ -681,6 +696,64 @@ __device__ __forceinline__ float dIntersect(float a, float b, float k, float ik)
     return fmaxf(a, b) + h * h * 0.25f * ik;
 }
 __device__ __forceinline__ float dSubtract(float a, float b, float k, float ik) { return dIntersect(a, -b, k, ik); }
+
+// ---- shaped joins (the builder's joinShape, model version 28; scene_api.js:393-406). For a join of radius r a round
+// is a circular fillet where the two surfaces meet square, a chamfer the flat cut between the same two tangent lines,
+// written in Mercury's hg_sdf forms (fOpIntersectionRound, fOpIntersectionChamfer, 2016) on the two signed distances:
+//     round:   with A = a + r, B = b + r: |(A, B)| - r where both are positive, else max(a, b)
+//     chamfer: max(max(a, b), (a + b + r) / sqrt 2)
+// A union is the intersection of the two complements, a cut the intersection with the second's complement. The forms
+// are exact for two square planes; for curved surfaces, or ones meeting at other angles, they can read more than the
+// true distance away from the edge, which the march must never be told. So each is held under the hard join's value h
+// plus an allowance: the most the form adds anywhere with h <= 0 -- at the sharp corner, (sqrt 2 - 1) r for the
+// round, r / sqrt 2 for the chamfer -- which leaves the surface and everything inside the form's exactly, falling
+// quadratically to nothing at h = 2r, past which the hard join's value (never above the true distance) is all there
+// is. The allowance falls with slope at most its size over r, under 1, so the value stays 1-Lipschitz.
+#define PT_ROUND_MOST 0.41421356f
+#define PT_CHAMFER_MOST 0.70710678f
+__device__ __forceinline__ float shapedAllowance(float h, float r, float most) {
+    const float f = saturate(1.0f - 0.5f * h / fmaxf(r, 1e-30f));
+    return most * r * f * f;
+}
+__device__ __forceinline__ float dRoundI(float a, float b, float r) {
+    const float h = fmaxf(a, b), A = a + r, B = b + r;
+    if (!(A > 0.0f && B > 0.0f)) return h;
+    return fminf(fsqrt(A * A + B * B) - r, h + shapedAllowance(h, r, PT_ROUND_MOST));
+}
+__device__ __forceinline__ float dChamferI(float a, float b, float r) {
+    const float h = fmaxf(a, b);
+    return fmaxf(h, fminf((a + b + r) * PT_CHAMFER_MOST, h + shapedAllowance(h, r, PT_CHAMFER_MOST)));
+}
+__device__ __forceinline__ float dRoundU(float a, float b, float r) { return -dRoundI(-a, -b, r); }
+__device__ __forceinline__ float dRoundS(float a, float b, float r) { return dRoundI(a, -b, r); }
+__device__ __forceinline__ float dChamferU(float a, float b, float r) { return -dChamferI(-a, -b, r); }
+__device__ __forceinline__ float dChamferS(float a, float b, float r) { return dChamferI(a, -b, r); }
+// With normals: where the form's value is used, its gradient (A na + B nb for the round, na + nb for the chamfer,
+// normalised); where the hard join's or its allowance, the harder side's normal. Either side without a normal leaves
+// the result without one (the surface kernel then takes it by differences), as for the smooth blends.
+__device__ __forceinline__ V4 shapedResult(V4 a, V4 b, V4 hard, float d, float capped, V3 grad) {
+    const bool both = dot(xyz(a), xyz(a)) >= 1e-12f && dot(xyz(b), xyz(b)) >= 1e-12f;
+    if (d >= capped) return v4(both ? xyz(hard) : v3s(0.0f), capped);
+    const float l = length(grad);
+    return v4(both && l > 1e-12f ? grad * frcp(l) : v3s(0.0f), d);
+}
+__device__ __forceinline__ V4 csgRoundI(V4 a, V4 b, float r) {
+    const V4 hard = a.w > b.w ? a : b;
+    const float A = a.w + r, B = b.w + r;
+    if (!(A > 0.0f && B > 0.0f)) return hard;
+    return shapedResult(a, b, hard, fsqrt(A * A + B * B) - r, hard.w + shapedAllowance(hard.w, r, PT_ROUND_MOST),
+                        xyz(a) * A + xyz(b) * B);
+}
+__device__ __forceinline__ V4 csgChamferI(V4 a, V4 b, float r) {
+    const V4 hard = a.w > b.w ? a : b;
+    const float d = (a.w + b.w + r) * PT_CHAMFER_MOST;
+    if (d <= hard.w) return hard;
+    return shapedResult(a, b, hard, d, hard.w + shapedAllowance(hard.w, r, PT_CHAMFER_MOST), xyz(a) + xyz(b));
+}
+__device__ __forceinline__ V4 csgRoundU(V4 a, V4 b, float r) { return -csgRoundI(-a, -b, r); }
+__device__ __forceinline__ V4 csgRoundS(V4 a, V4 b, float r) { return csgRoundI(a, -b, r); }
+__device__ __forceinline__ V4 csgChamferU(V4 a, V4 b, float r) { return -csgChamferI(-a, -b, r); }
+__device__ __forceinline__ V4 csgChamferS(V4 a, V4 b, float r) { return csgChamferI(a, -b, r); }
 
 // ---------------------------------------------------------------------------------------------------
 // A node's frame: params t[0..2] position, t[3..11] rotation (columns: local axes in world), t[12] scale,

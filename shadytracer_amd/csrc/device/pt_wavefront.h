@@ -401,7 +401,8 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
     const bool mapped = (flags & (PT_F_CAUSTIC | PT_F_SDELTA)) == (PT_F_CAUSTIC | PT_F_SDELTA);
 
     if (state == PT_ESCAPED) {
-        if (!photon) {
+        // the illuminance view: a camera ray that met nothing has no surface to measure on, and shows 0
+        if (!photon && !(D->u.measureMode == 1u && bounces == 0u)) {
             // the sun through its aperture emits photons, so a mapped path leaves its light to the map; the sky
             // emits none and stays path traced
             V3 Le = envBackground(env, dir) + ((mapped && sunFlux(env) > 0.0f) ? v3s(0.0f) : envSun(env, p, dir));
@@ -483,7 +484,8 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
             out.leaving = true;
         } else {
             bounces += 1u;
-            const PtMaterial& m = smat[hw & 0xFFu];
+            // the illuminance view (measureMode 1): the camera's first hit is the white Lambertian kept after the table
+            const PtMaterial& m = smat[(D->u.measureMode == 1u && bounces == 1u && !photon) ? PT_MAX_MATERIALS : (hw & 0xFFu)];
             V3 Le = ld3(m.emissive);
             if (max3(Le) > 0.0f && (photon || (mapped && m.face >= 0 && faceFlux(D->faces[m.face]) > 0.0f))) {
                 terminate = true;      // a photon ends at a light; a mapped path's light is the map's (its photons')
@@ -620,9 +622,14 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
                     } else if (s.isDelta) {
                         s.w = s.w * (1.0f / (1.0f - gc));
                     } else {
+                        // the mixture's density and value over the whole sphere: a rough transmissive material's
+                        // refraction is evaluated below the surface too, so a direction either strategy draws there
+                        // carries what the BSDF passes (the plain tracer's refraction weight, reached by MIS)
+                        lb.through = true;
                         pTot = gc * guidePdfAt(D, slot, s.dir) + (1.0f - gc) * lobesPdf(lb, s.dir);
                         const float cosO = dot(nrm, s.dir);
-                        s.w = (pTot > 0.0f && cosO > 0.0f) ? lobesEval(lb, D->mf, s.dir) * (cosO / pTot) : v3s(0.0f);
+                        s.w = (pTot > 0.0f && cosO != 0.0f) ? lobesEval(lb, D->mf, s.dir) * (fabsf(cosO) / pTot) : v3s(0.0f);
+                        s.transmitted = cosO < 0.0f;
                     }
                     out.guided = true;
                 }
@@ -667,7 +674,7 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
                     // a training vertex: where, the view, the density it sampled w with, the normal, w, f cos at w,
                     // the throughput after it; S comes at the next event
                     const float pw = pTot >= 0.0f ? pTot : lobesPdf(lb, dir);
-                    const V3 fc = lobesEval(lb, D->mf, dir) * fmaxf(dot(nrm, dir), 0.0f);
+                    const V3 fc = lobesEval(lb, D->mf, dir) * fabsf(dot(nrm, dir));
                     const unsigned int k = D->gVerts[gr];
                     float4* rec = guideRec(D, gr, k);
                     rec[0] = make_float4(p.x, p.y, p.z, m.roughness);
@@ -720,12 +727,20 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
 }
 
 extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_shade(const PtDev* __restrict__ D, unsigned int qsel) {
-    __shared__ PtMaterial smat[PT_MAX_MATERIALS];
+    __shared__ PtMaterial smat[PT_MAX_MATERIALS + 1];
     {
         const unsigned int words = D->nmats * (unsigned int)(sizeof(PtMaterial) / 4);
         const unsigned int* src = reinterpret_cast<const unsigned int*>(D->mats);
         unsigned int* dst = reinterpret_cast<unsigned int*>(smat);
         for (unsigned int k = threadIdx.x; k < words; k += PT_BLOCK) dst[k] = src[k];
+        // after the table: the illuminance view's white Lambertian
+        if (threadIdx.x == 0) {
+            PtMaterial& w = smat[PT_MAX_MATERIALS];
+            w.base[0] = w.base[1] = w.base[2] = 1.0f;
+            w.metallic = 0.0f; w.roughness = 1.0f; w.transmission = 0.0f; w.ior = 1.5f; w.specular = 0.0f;
+            w.emissive[0] = w.emissive[1] = w.emissive[2] = 0.0f;
+            w.abbe = 0.0f; w.face = -1; w.specularTint = 0.0f; w.pad[0] = w.pad[1] = 0.0f;
+        }
     }
     __syncthreads();
     unsigned int* C = D->counters;

@@ -765,6 +765,7 @@ int pt_set_uniforms(pt_ctx* c, const PtUniforms* u, size_t size) {
     if (u->marchRefill > 64) return fail(c, PT_ERR_ARG, "marchRefill %u not in 0..64 (0: the tuning's)", u->marchRefill);
     if (!(u->stepscale > 0.0f && u->stepscale <= 1.0f)) return fail(c, PT_ERR_ARG, "stepscale %g not in (0, 1]", u->stepscale);
     if (!(u->relax == 0.0f || (u->relax >= 1.0f && u->relax <= 1.95f))) return fail(c, PT_ERR_ARG, "relax %g not 0 or in [1, 1.95]", u->relax);
+    if (u->measureMode > 2u) return fail(c, PT_ERR_ARG, "measureMode %u not 0, 1 (illuminance) or 2 (luminance)", u->measureMode);
     if (u->sppm && (u->sppmPhotons == 0 || u->sppmPhotons > (1u << 26)))
         return fail(c, PT_ERR_ARG, "sppmPhotons %u not in 1..2^26", u->sppmPhotons);
     if (u->sppm && !(u->sppmRadius > 0.0f && u->sppmAlpha > 0.0f && u->sppmAlpha < 1.0f))

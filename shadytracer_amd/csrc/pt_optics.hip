@@ -373,38 +373,54 @@ __device__ double planeWidth(const double* I, double level) {
     }
     return ends[1] - ends[0];
 }
-// the Planckian locus in CIE 1960 uv (Krystek 1985's rational fit), 1000..15000 K
-__device__ void planckUv(double T, double& u, double& v) {
-    const double t = fmin(15000.0, fmax(1000.0, T));
-    u = (0.860117757 + 1.54118254e-4 * t + 1.28641212e-7 * t * t) / (1.0 + 8.42420234e-4 * t + 7.08145163e-7 * t * t);
-    v = (0.317398726 + 4.22806245e-5 * t + 4.20481691e-8 * t * t) / (1.0 - 2.89741816e-5 * t + 1.61456053e-7 * t * t);
+// The Planckian locus in CIE 1960 uv by Krystek's 1985 fit, each coordinate a ratio of quadratics in T over
+// 1000..15000 K, with its direction dP/dT from the ratios' derivatives (N'D - N D') / D^2.
+__device__ void krystek(const double* c, double t, double& val, double& slope) {
+    const double n = c[0] + t * (c[1] + t * c[2]), d = c[3] + t * (c[4] + t * c[5]);
+    val = n / d;
+    slope = ((c[1] + 2.0 * t * c[2]) * d - n * (c[4] + 2.0 * t * c[5])) / (d * d);
 }
-// CCT: the locus's nearest point to (u, v), by a scan in mired and a golden-section refinement; Duv the signed
-// distance along the locus's normal (positive above it)
+__device__ void locusAt(double T, double& u, double& v, double& su, double& sv) {
+    const double KU[6] = {0.860117757, 1.54118254e-4, 1.28641212e-7, 1.0, 8.42420234e-4, 7.08145163e-7};
+    const double KV[6] = {0.317398726, 4.22806245e-5, 4.20481691e-8, 1.0, -2.89741816e-5, 1.61456053e-7};
+    const double t = fmin(15000.0, fmax(1000.0, T));
+    krystek(KU, t, u, su);
+    krystek(KV, t, v, sv);
+}
+// CCT and Duv of a chromaticity. The locus point nearest (u, v) is where the offset from it is square to the locus:
+// g(T) = (P(T) - uv) . P'(T) = 0, g rising through zero. g is sampled at 48 temperatures even in mired, each bracket
+// where it turns from negative to positive is closed by bisection to double precision, and the nearest of those feet
+// and the two ends of the range is the CCT. Duv is the offset along the locus's unit normal, the tangent turned a
+// quarter clockwise: positive above the locus (towards green).
 __device__ void cctDuv(double x, double y, double& cct, double& duv) {
     const double den = -2.0 * x + 12.0 * y + 3.0;
     const double u = 4.0 * x / den, v = 6.0 * y / den;
-    auto d2 = [&](double T) { double a, b; planckUv(T, a, b); return (a - u) * (a - u) + (b - v) * (b - v); };
-    const double M0 = 1e6 / 15000.0, M1 = 1e6 / 1000.0;
-    const int N = 200;
-    int best = 0;
-    double bd = 1e300;
-    for (int k = 0; k <= N; ++k) { const double dd = d2(1e6 / (M0 + (M1 - M0) * k / N)); if (dd < bd) { bd = dd; best = k; } }
-    double lo = M0 + (M1 - M0) * fmax(0.0, best - 1.0) / N, hi = M0 + (M1 - M0) * fmin((double)N, best + 1.0) / N;
-    const double g = (sqrt(5.0) - 1.0) / 2.0;
-    for (int k = 0; k < 80; ++k) {
-        const double a = hi - g * (hi - lo), b = lo + g * (hi - lo);
-        if (d2(1e6 / a) < d2(1e6 / b)) hi = b; else lo = a;
+    auto g = [&](double T) { double a, b, sa, sb; locusAt(T, a, b, sa, sb); return (a - u) * sa + (b - v) * sb; };
+    auto dist2 = [&](double T) { double a, b, sa, sb; locusAt(T, a, b, sa, sb); return (a - u) * (a - u) + (b - v) * (b - v); };
+    const double Mhot = 1e6 / 15000.0, Mcold = 1e6 / 1000.0;
+    const int K = 48;
+    double bestT = 15000.0, bestD = dist2(15000.0);
+    if (dist2(1000.0) < bestD) { bestT = 1000.0; bestD = dist2(1000.0); }
+    // walking from hot to cold, T falls: a foot is where g turns from positive to negative along the walk
+    double Tprev = 15000.0, gprev = g(Tprev);
+    for (int k = 1; k <= K; ++k) {
+        const double Tk = 1e6 / (Mhot + (Mcold - Mhot) * k / K), gk = g(Tk);
+        if (gprev > 0.0 && gk <= 0.0) {
+            double lo = Tk, hi = Tprev;            // g(lo) <= 0 < g(hi)
+            for (int it = 0; it < 64; ++it) {
+                const double mid = 0.5 * (lo + hi);
+                if (g(mid) > 0.0) hi = mid; else lo = mid;
+            }
+            const double T = 0.5 * (lo + hi), dd = dist2(T);
+            if (dd < bestD) { bestD = dd; bestT = T; }
+        }
+        Tprev = Tk; gprev = gk;
     }
-    const double T = 1e6 / ((lo + hi) / 2.0);
-    double pu, pv, u0, v0, u1, v1;
-    planckUv(T, pu, pv);
-    const double h = fmax(1e-3, T * 1e-5);
-    planckUv(T - h, u0, v0);
-    planckUv(T + h, u1, v1);
-    const double du = u1 - u0, dv = v1 - v0, l = sqrt(du * du + dv * dv);
-    cct = T;
-    duv = (u - pu) * (dv / l) + (v - pv) * (-du / l);
+    double pu, pv, su, sv;
+    locusAt(bestT, pu, pv, su, sv);
+    const double l = sqrt(su * su + sv * sv);
+    cct = bestT;
+    duv = ((u - pu) * sv - (v - pv) * su) / l;
 }
 
 extern "C" __global__ __launch_bounds__(256) void pt_optics_numbers(OptDev D) {

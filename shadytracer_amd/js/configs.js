@@ -18,7 +18,7 @@
  *   sppm-slab    a collimated beam through a glass slab onto a floor: the photon map's closed form.
  *   furnace-*    closed-form checks: a white furnace (albedo-1 Lambertian plane under a uniform sky, and a haze box
  *                under a uniform sphere of sky).
- *   studio-floor, lens-prism, reflector-condenser, turned-parts, gallery
+ *   studio-floor, lens-prism, reflector-condenser, turned-parts, gallery, joins
  *                documents in the builder's own form (millimetres, the studio environment, lights by CCT and flux,
  *                an orbit view), the others with the lens, prism, reflector, aspheric lens, outline and the
  *                remaining solid builtins.
@@ -199,6 +199,24 @@ function furnacePlane(opts = {}) {
     return { doc: d, camera: camera([0, -3, 1.5], [0, 0, 0], 60, w, h), settings: { width: w, height: h, spp: 64, maxbounce: 16, nee: 1 } }
 }
 
+// furnace-sphere: the reference's white furnace (lib/pathtracer/tests.js furnace :1555, testScene 'sphere'): a unit
+// sphere of one material in a uniform sphere of sky, 1000 cd/m^2, seen from 3 m with a 40-degree vertical field of view
+// (FURNACE_CAMERA), maxbounce 40. A material that neither makes nor loses light vanishes: the picture reads 1000.
+function furnaceSphere(opts = {}) {
+    const num = (v, d) => v !== undefined ? v : d
+    const b = num(opts.albedo, 1)
+    const mats = [material('white', [b, b, b], num(opts.roughness, 1), num(opts.metallic, 0),
+        { specular: num(opts.specular, 0), transmission: num(opts.transmission, 0), ior: num(opts.ior, 1.5), abbe: num(opts.abbe, 0) })]
+    const d = doc('furnace-sphere', mats, [obj('ball', 'b-sphere', 'white', xf(0, 0, 0), { radius: 1 })], { sky: 'sphere', uniformSky: 1000 }, 20)
+    const w = opts.width || 64, h = opts.height || 64
+    const el = 30 * Math.PI / 180, az = 30 * Math.PI / 180
+    const pos = [3 * Math.cos(el) * Math.cos(az), 3 * Math.cos(el) * Math.sin(az), 3 * Math.sin(el)]
+    // the surface tolerance the reference traces this scene at: photometry.js autoEps, 1e-4 of its extent (tests.js EXTENT
+    // sphere: 2 sqrt 3 m); with maxiter 400 over a path, what a path trapped in rough glass loses depends on it
+    const epsAuto = 1e-4 * 2 * Math.sqrt(3)
+    return { doc: d, camera: camera(pos, [0, 0, 0], 40, w, h), settings: { width: w, height: h, spp: 64, maxbounce: 40, nee: 1, maxiter: 400, epsAuto } }
+}
+
 function furnaceHaze(opts = {}) {
     // a haze box of albedo 1 in a uniform sphere of sky: every pixel reads the sky's luminance (energy conservation)
     const g = opts.g !== undefined ? opts.g : 0.6
@@ -339,8 +357,68 @@ function sunskyFloor(opts = {}) {
     return { doc: d, camera: cameraOf(d, w, h), settings: { width: w, height: h, spp: 64, maxbounce: 4, nee: 1 } }
 }
 
-const CONFIGS = { 'plane-c1': planeC1, 'room-c2': roomC2, 'caustic-c3': causticC3, 'furnace-plane': furnacePlane, 'furnace-haze': furnaceHaze,
+// sunsky-bench: the reference's photometry bench (lib/pathtracer/tests.js photometry :447, illuminance :553, cosine-law
+// :613, bsdf-parity :1868): a plane through the origin under the sun & sky -- the sun at azimuth 45, elevation 45, 128
+// klx above the air, turbidity 0.7 -- seen by FLOOR_CAMERA (straight down from 5 units, fov 40) or, with oblique,
+// OBLIQUE_CAMERA (70 degrees off vertical); the floor Lambertian of albedo 0.8 (floorScene) unless the options give its
+// material, tilted towards the sun's bearing by `tilt` degrees (testScene 'tilted'); in metres or millimetres (unit).
+// Options: albedo, metallic, roughness, specular, transmission, tilt, unit, oblique, and the environment's envSun,
+// envSky, uSunElevation, uSkyTurbidity, sunDiameter.
+function sunskyBench(opts = {}) {
+    const { cameraOf } = require('./hipgen')
+    const num = (v, d) => v !== undefined ? v : d
+    const b = num(opts.albedo, 0.8)
+    const mat = material('floor', [b, b, b], num(opts.roughness, 1), num(opts.metallic, 0),
+        { specular: num(opts.specular, 0), transmission: num(opts.transmission, 0), ior: 1.5 })
+    const tilt = num(opts.tilt, 0), az = 45
+    // the plane's normal (+z of its frame) tilted by `tilt` towards the sun's bearing (sin az, cos az, 0): Rz(90 - az) Ry(tilt)
+    const floor = obj('floor', 'b-plane', 'floor', xf(0, 0, 0, 0, tilt, tilt ? 90 - az : 0))
+    const unit = opts.unit || 'm'
+    const d = { format: 'raytracer-scene', version: 31, title: 'sunsky-bench', unit, envRadius: 400 / UNIT[unit], materials: [mat], blocks: BLOCKS,
+        root: { id: 'root', kind: 'group', label: 'bench', enabled: true, blend: 0, children: [floor] },
+        environment: { envMode: 0, uSunAzimuth: az, uSunElevation: num(opts.uSunElevation, 45), uSkyTurbidity: num(opts.uSkyTurbidity, 0.7),
+            sunOutside: 128000, sunDiameter: num(opts.sunDiameter, 0.53), envSun: num(opts.envSun, 1), envSky: num(opts.envSky, 1) },
+        view: { camera: { mode: 'orbit', camtype: 0, target: [0, 0, 0], azimuth: 30, elevation: opts.oblique ? 70 : 8, distance: 5, fov: 40 } } }
+    // the studio instead (envMode 1): opts.studio = { lux, top, tint }
+    if (opts.studio) {
+        const t = opts.studio.tint || [1, 1, 1]
+        d.environment = { envMode: 1, studioIlluminance: opts.studio.lux, studioTop: opts.studio.top, studioColor: { r: t[0], g: t[1], b: t[2] } }
+    }
+    const w = opts.width || 32, h = opts.height || 32
+    return { doc: d, camera: cameraOf(d, w, h), settings: { width: w, height: h, spp: 32, maxbounce: 10, nee: 1, maxiter: 400 } }
+}
+const UNIT = { m: 1, mm: 0.001 }
+
+// joins: the builder's shaped joins (joinShape, model version 28) on millimetre parts -- a box intersected with a
+// ball by a round, a box with a cylinder cut from it by a chamfer, two boxes unioned by a round (a concave fillet), and
+// a chamfered union -- on a grey floor under the studio and a 4000 K panel.
+function joins(opts = {}) {
+    const { cameraOf } = require('./hipgen')
+    const mats = [material('floor', [0.5, 0.5, 0.5], 0.8, 0, { specular: 0.5 }), material('brass', [0.9, 0.7, 0.35], 0.3, 1),
+        material('paint', [0.2, 0.35, 0.7], 0.5, 0)]
+    const box = (id, sx, sy, sz, t, extra) => Object.assign(obj(id, 'b-box', null, t, { size: { x: sx, y: sy, z: sz }, rounding: 0 }), extra || {})
+    const part = (id, mat, t, kids) => ({ id, kind: 'group', label: id, enabled: true, blend: 0, transform: t, children: kids.map(k => Object.assign(k, { material: mat })) })
+    const children = [obj('floor', 'b-plane', 'floor', xf(0, 0, 0)),
+        part('rounded', 'paint', xf(-75, 0, 20), [box('r-box', 18, 18, 18, xf(0, 0, 0)),
+            Object.assign(obj('r-ball', 'b-sphere', null, xf(0, 0, 0), { radius: 24 }), { join: 'intersect', joinBlend: 5, joinShape: 'round' })]),
+        part('chamfered', 'brass', xf(-25, 0, 20), [box('c-box', 18, 18, 18, xf(0, 0, 0)),
+            Object.assign(obj('c-hole', 'b-cylinder', null, xf(0, 0, 0), { radius: 9, height: 60 }), { join: 'subtract', joinBlend: 4, joinShape: 'chamfer' })]),
+        part('filleted', 'paint', xf(25, 0, 14), [box('f-base', 18, 18, 6, xf(0, 0, -8)),
+            box('f-post', 7, 7, 14, xf(0, 0, 4), { join: 'union', joinBlend: 6, joinShape: 'round' })]),
+        part('bevelled', 'brass', xf(75, 0, 14), [box('b-base', 18, 18, 6, xf(0, 0, -8)),
+            box('b-post', 7, 7, 14, xf(0, 0, 4), { join: 'union', joinBlend: 5, joinShape: 'chamfer' })]),
+    ]
+    children.push({ id: 'panel', kind: 'object', type: 'light', label: 'panel', enabled: true, material: null, transform: xf(0, 80, 140, 150, 0, 0),
+        params: { width: 60, height: 40 }, light: { shape: 'rectangle', colourBy: 'cct', cct: 4000, flux: 400 } })
+    const d = studioDoc('joins', mats, children, BLOCKS,
+        { mode: 'orbit', camtype: 0, fov: 40, target: [0, 0, 12], azimuth: -70, elevation: 60, distance: 170, up: [0, 0, 1], aperture: 0 })
+    const w = opts.width || 160, h = opts.height || 64
+    return { doc: d, camera: cameraOf(d, w, h), settings: { width: w, height: h, spp: 64, maxbounce: 8, nee: 1 } }
+}
+
+const CONFIGS = { 'plane-c1': planeC1, 'room-c2': roomC2, 'caustic-c3': causticC3, 'furnace-plane': furnacePlane, 'furnace-sphere': furnaceSphere,
+    'furnace-haze': furnaceHaze,
     'sppm-slab': sppmSlab, 'studio-floor': studioFloor, 'lens-prism': lensPrism, 'reflector-condenser': reflectorCondenser,
-    'turned-parts': turnedParts, gallery, sunlit, 'sunsky-floor': sunskyFloor }
+    'turned-parts': turnedParts, gallery, sunlit, 'sunsky-floor': sunskyFloor, 'sunsky-bench': sunskyBench, joins }
 
 module.exports = { CONFIGS, camera, rng, SEED }

@@ -15,9 +15,10 @@
  *   flops      FP32 operations of one sceneDist call (the march's roofline count)
  *
  * Scope: the builder's builtin shapes (plane, box, sphere, cylinder, capsule, torus, rectangle, disc, lens, prism),
- * groups with their joins (union, subtract, intersect; smooth blends), transforms (rotation Rz*Ry*Rx in degrees,
- * uniform scale: builder/transform.js:9-38), lights (rectangle and disc faces), one haze box and the studio
- * environment. A code block that is not a builtin -- user WGSL -- is refused with its node named.
+ * groups with their joins (union, subtract, intersect; smooth blends, rounds and chamfers), transforms (rotation
+ * Rz*Ry*Rx in degrees, uniform scale: builder/transform.js:9-38), lights (rectangle and disc faces), one haze box and
+ * the studio environment. A code block that is not a builtin -- user WGSL -- is refused with its node named, and so is
+ * a group with code (a modifier).
  */
 
 const sky = require('./sky')
@@ -33,7 +34,15 @@ const PRIM_FLOPS = { plane: 1, sphere: 7, box: 27, cylinder: 24, capsule: 14, to
     'capped torus': 22, link: 18, tube: 16, dome: 24, pyramid: 90 }
 const XF_FLOPS = 22      // p - pos, three dot products, times 1/s; the distance times s
 const FOLD_FLOPS = 2
-const JOIN_FLOPS = 9
+// a join per shape: its device functions (distance only, with normals), FP32 operations, and how far below the smaller
+// side a union's value can go per unit of its radius (the smooth blend's k/4; the round's and chamfer's allowance)
+const JOIN_FNS = {
+    smooth: { union: ['dUnion', 'csgUnion'], subtract: ['dSubtract', 'csgSubtract'], intersect: ['dIntersect', 'csgIntersect'] },
+    round: { union: ['dRoundU', 'csgRoundU'], subtract: ['dRoundS', 'csgRoundS'], intersect: ['dRoundI', 'csgRoundI'] },
+    chamfer: { union: ['dChamferU', 'csgChamferU'], subtract: ['dChamferS', 'csgChamferS'], intersect: ['dChamferI', 'csgChamferI'] },
+}
+const JOIN_FLOPS = { smooth: 9, round: 19, chamfer: 14 }
+const JOIN_REACH = { smooth: 0.25, round: 0.41421356, chamfer: 0.70710678 }
 const CULL_FLOPS = 11    // |p - c|^2 against (r + |best|)^2, widened a hair (boundNear)
 
 // The builtin shapes: their parameters in U (after the 13 of the transform), and the HIP calls.
@@ -108,66 +117,75 @@ const DEFAULTS = { lens: { diameter: 1.0, thickness: 0.2, r1: 2.0, r2: 2.0 }, pr
     'capped torus': { ring: 0.42, tube: 0.08, angle: 270 }, link: { span: 0.36, ring: 0.25, tube: 0.07 }, tube: { outer: 0.5, inner: 0.4, height: 1.0 },
     dome: { radius: 0.5, height: 0.3 }, pyramid: { base: { x: 1, y: 1 }, top: { x: 0, y: 0 }, height: 1 } }
 
-// ---- outlines (shapes.js profilePieces:1487): points (x, y, and a fillet radius if rounded), closed back to the first,
-// as the pieces a profile function walks -- segments, and arcs where a corner is rounded -- counter-clockwise, each
-// segment lying on x = 0 marked as the axis (a revolved outline's axis edge is no surface). An outline that is not
-// sound (fewer than three points, a repeated neighbour, no area, crossing edges, fillets that do not fit their edges)
-// is refused, naming the points as written.
+// ---- outlines (the builder's revolution, Fresnel lens and extrusion shapes): a closed polygon given as entries
+// [x, y] or [x, y, r], r the radius of a fillet that rounds that corner, compiled into the pieces a profile function
+// walks: straight sides, and an arc at each rounded corner, in counter-clockwise order. A side lying on x = 0 is
+// marked as the axis (turned about the axis, it is no surface). What cannot make a sound outline is refused with the
+// entries named by their place in the list as written.
 const sub2 = (a, b) => [a[0] - b[0], a[1] - b[1]]
 const cross2 = (a, b) => a[0] * b[1] - a[1] * b[0]
+const dot2 = (a, b) => a[0] * b[0] + a[1] * b[1]
 const unit2 = v => { const l = Math.hypot(v[0], v[1]); return [v[0] / l, v[1] / l] }
-function segmentsMeet(a, b, c, d) {
-    const o = (p, q, r) => Math.sign(cross2(sub2(q, p), sub2(r, p)))
-    const within = (p, q, r) => Math.min(p[0], q[0]) <= r[0] && r[0] <= Math.max(p[0], q[0]) && Math.min(p[1], q[1]) <= r[1] && r[1] <= Math.max(p[1], q[1])
-    const o1 = o(a, b, c), o2 = o(a, b, d), o3 = o(c, d, a), o4 = o(c, d, b)
-    if (o1 !== o2 && o3 !== o4) return true
-    return (o1 === 0 && within(a, b, c)) || (o2 === 0 && within(a, b, d)) || (o3 === 0 && within(c, d, a)) || (o4 === 0 && within(c, d, b))
+// Whether the closed segments ab and cd share a point: a + t (b - a) = c + s (d - c) solved by Cramer's rule for
+// t, s in [0, 1]; parallel sides share a point only when they lie on one line and their spans along it overlap.
+function sidesTouch(a, b, c, d) {
+    const r = sub2(b, a), q = sub2(d, c), w = sub2(c, a)
+    const den = cross2(r, q)
+    if (den !== 0) {
+        const t = cross2(w, q) / den, s = cross2(w, r) / den
+        return t >= 0 && t <= 1 && s >= 0 && s <= 1
+    }
+    if (cross2(w, r) !== 0) return false
+    const rr = dot2(r, r), t0 = dot2(w, r) / rr, t1 = dot2(sub2(d, a), r) / rr
+    return Math.max(t0, t1) >= 0 && Math.min(t0, t1) <= 1
+}
+// A corner's fillet: the circle of radius r tangent to both sides at p. With u and v the unit directions from p to its
+// neighbours and c = u.v the cosine of the corner's angle th, the tangent points lie r / tan(th/2) = r sqrt((1+c)/(1-c))
+// along each side and the centre r / sin(th/2) = r sqrt(2/(1-c)) along the bisector. A straight run has nothing to round.
+function fillet(prev, p, next, r) {
+    const u = unit2(sub2(prev, p)), v = unit2(sub2(next, p)), c = Math.max(-1, Math.min(1, dot2(u, v)))
+    if (!(r > 0) || 1 + c < 1e-15) return null
+    const along = r * Math.sqrt((1 + c) / (1 - c)), out = r * Math.sqrt(2 / (1 - c)), bis = unit2([u[0] + v[0], u[1] + v[1]])
+    return { along, r, inAt: [p[0] + u[0] * along, p[1] + u[1] * along], outAt: [p[0] + v[0] * along, p[1] + v[1] * along],
+        centre: [p[0] + bis[0] * out, p[1] + bis[1] * out], convex: cross2(sub2(p, prev), sub2(next, p)) > 0 }
 }
 function outlinePieces(points, where) {
-    const bad = why => { throw new Error(`hipgen: ${where}: ${why}`) }
-    if (!Array.isArray(points) || points.length < 3) bad('an outline needs at least three points')
-    points.forEach((pt, i) => {
-        if (!Array.isArray(pt) || (pt.length !== 2 && pt.length !== 3) || !pt.every(Number.isFinite)) bad(`point ${i + 1}: x, y and a fillet radius if it has one`)
-        if (pt.length === 3 && pt[2] < 0) bad(`point ${i + 1}: a negative fillet radius`)
-    })
-    const n = points.length, at = i => ((i % n) + n) % n
+    const refuse = why => { throw new Error(`hipgen: ${where}: ${why}`) }
+    const n = Array.isArray(points) ? points.length : 0
+    if (n < 3) refuse(`only ${n} entries: an outline is a polygon of three or more`)
     for (let i = 0; i < n; i++) {
-        const a = points[i], b = points[at(i + 1)]
-        if (a[0] === b[0] && a[1] === b[1]) bad(`points ${i + 1} and ${at(i + 1) + 1} are the same point`)
+        const e = points[i]
+        if (!Array.isArray(e) || e.length < 2 || e.length > 3 || !e.every(Number.isFinite)) refuse(`entry ${i + 1} is not [x, y] or [x, y, fillet] in numbers`)
+        if (e.length === 3 && e[2] < 0) refuse(`entry ${i + 1} has a fillet radius below zero`)
     }
-    let area = 0
-    for (let i = 0; i < n; i++) area += 0.5 * cross2(points[i], points[at(i + 1)])
-    const size = Math.max(...points.map(p => Math.max(Math.abs(p[0]), Math.abs(p[1]))), 1e-30)
-    if (!(Math.abs(area) > 1e-12 * size * size)) bad('the outline encloses no area')
+    const next = i => (i + 1) % n
+    for (let i = 0; i < n; i++) if (points[i][0] === points[next(i)][0] && points[i][1] === points[next(i)][1]) refuse(`entries ${i + 1} and ${next(i) + 1} coincide`)
+    // the shoelace area: its sign is the winding, its size whether there is a shape at all
+    let twice = 0
+    for (let i = 0; i < n; i++) twice += cross2(points[i], points[next(i)])
+    const scale = points.reduce((m, e) => Math.max(m, Math.abs(e[0]), Math.abs(e[1])), 1e-30)
+    if (!(Math.abs(twice) > 2e-12 * scale * scale)) refuse('its area is zero')
     for (let i = 0; i < n; i++) for (let j = i + 2; j < n; j++) {
-        if (i === 0 && j === n - 1) continue
-        if (segmentsMeet(points[i], points[at(i + 1)], points[j], points[at(j + 1)])) bad(`edges ${i + 1}-${at(i + 1) + 1} and ${j + 1}-${at(j + 1) + 1} cross`)
+        if (next(j) === i) continue
+        if (sidesTouch(points[i], points[next(i)], points[j], points[next(j)])) refuse(`sides ${i + 1}-${next(i) + 1} and ${j + 1}-${next(j) + 1} intersect`)
     }
-    const order = [...Array(n).keys()]
-    if (area < 0) order.reverse()
-    const Pts = order.map(i => points[i])
-    const corner = Pts.map((p, i) => {
-        const r = p[2] || 0
-        if (!(r > 0)) return null
-        const prev = Pts[at(i - 1)], next = Pts[at(i + 1)]
-        const u = unit2(sub2(prev, p)), v = unit2(sub2(next, p))
-        const ang = Math.acos(Math.min(1, Math.max(-1, u[0] * v[0] + u[1] * v[1])))
-        if (ang > Math.PI - 1e-9) return null                      // a straight run: nothing to round
-        const t = r / Math.tan(0.5 * ang), bis = unit2([u[0] + v[0], u[1] + v[1]]), k = r / Math.sin(0.5 * ang)
-        return { t, r, A: [p[0] + u[0] * t, p[1] + u[1] * t], B: [p[0] + v[0] * t, p[1] + v[1] * t], c: [p[0] + bis[0] * k, p[1] + bis[1] * k],
-            convex: cross2(sub2(p, prev), sub2(next, p)) > 0 }
-    })
+    // counter-clockwise, each entry keeping its place in the list for messages
+    const ord = points.map((_, i) => i)
+    if (twice < 0) ord.reverse()
+    const V = ord.map(i => points[i])
+    const F = V.map((p, i) => fillet(V[(i + n - 1) % n], p, V[next(i)], p.length === 3 ? p[2] : 0))
     for (let i = 0; i < n; i++) {
-        const j = at(i + 1), need = (corner[i] ? corner[i].t : 0) + (corner[j] ? corner[j].t : 0)
-        if (need > Math.hypot(Pts[j][0] - Pts[i][0], Pts[j][1] - Pts[i][1]) * (1 + 1e-9)) bad(`the fillets at points ${order[i] + 1} and ${order[j] + 1} do not fit their edge`)
+        const used = (F[i] ? F[i].along : 0) + (F[next(i)] ? F[next(i)].along : 0)
+        if (used > Math.hypot(V[next(i)][0] - V[i][0], V[next(i)][1] - V[i][1]) * (1 + 1e-9))
+            refuse(`side ${ord[i] + 1}-${ord[next(i)] + 1} is too short for the fillets at its ends`)
     }
     const pieces = []
+    const onAxis = q => Math.abs(q[0]) <= 1e-12 * scale
     for (let i = 0; i < n; i++) {
-        const k = corner[i], j = at(i + 1)
-        if (k) pieces.push({ kind: 'arc', c: k.c, r: k.r, a: k.A, b: k.B, convex: k.convex })
-        const a = k ? k.B : [Pts[i][0], Pts[i][1]], b = corner[j] ? corner[j].A : [Pts[j][0], Pts[j][1]]
-        if (Math.hypot(b[0] - a[0], b[1] - a[1]) > 1e-12 * size)
-            pieces.push({ kind: 'segment', a, b, axis: Math.abs(a[0]) <= 1e-12 * size && Math.abs(b[0]) <= 1e-12 * size })
+        const f = F[i], g = F[next(i)]
+        if (f) pieces.push({ kind: 'arc', c: f.centre, r: f.r, a: f.inAt, b: f.outAt, convex: f.convex })
+        const from = f ? f.outAt : [V[i][0], V[i][1]], to = g ? g.inAt : [V[next(i)][0], V[next(i)][1]]
+        if (Math.hypot(to[0] - from[0], to[1] - from[1]) > 1e-12 * scale) pieces.push({ kind: 'segment', a: from, b: to, axis: onAxis(from) && onAxis(to) })
     }
     return pieces
 }
@@ -189,7 +207,7 @@ function profileParams(P, unit, mode) {
 // from U) and its solid: revolved (q = (|p.xy|, z), the axis edges no surface) or extruded.
 function profileCode(o, pr) {
     const u = i => `U[${o + i}]`
-    const lines = [`__device__ __forceinline__ G2 profile${o}(ParamPtr U, float qx, float qy) {`, '    PT_SCENE_PARAMS(U);', '    Prof A = pfStart();']
+    const lines = [`__device__ __forceinline__ Mer profile${o}(ParamPtr U, float qx, float qy) {`, '    PT_SCENE_PARAMS(U);', '    Prof A = pfStart();']
     pr.pieces.forEach((k, i) => {
         const b = 3 + 9 * i
         if (k.kind === 'segment') lines.push(`    pfSegment(A, qx, qy, ${u(b + 1)}, ${u(b + 2)}, ${u(b + 3)}, ${u(b + 4)}, ${k.axis && pr.mode === 'revolve' ? 'false' : 'true'});`)
@@ -197,9 +215,9 @@ function profileCode(o, pr) {
     })
     lines.push('    return pfEnd(A);', '}')
     if (pr.mode === 'revolve') {
-        lines.push(`__device__ __forceinline__ V4 Nprofile${o}(ParamPtr U, V3 p) { return g2revolve(p, profile${o}(U, fsqrt(p.x * p.x + p.y * p.y), p.z)); }`)
+        lines.push(`__device__ __forceinline__ V4 Nprofile${o}(ParamPtr U, V3 p) { return merTo3(p, profile${o}(U, fsqrt(p.x * p.x + p.y * p.y), p.z)); }`)
     } else {
-        lines.push(`__device__ __forceinline__ V4 Nprofile${o}(ParamPtr U, V3 p) { PT_SCENE_PARAMS(U); return g2extrude(p, profile${o}(U, p.x, p.y), ${u(1)}); }`)
+        lines.push(`__device__ __forceinline__ V4 Nprofile${o}(ParamPtr U, V3 p) { PT_SCENE_PARAMS(U); return merExtrude(p, profile${o}(U, p.x, p.y), ${u(1)}); }`)
     }
     lines.push(`__device__ __forceinline__ float Dprofile${o}(ParamPtr U, V3 p) { return Nprofile${o}(U, p).w; }`)
     return lines.join('\n')
@@ -470,7 +488,9 @@ function generate(doc, opts) {
     const isCsg = g => (g.blend > 0) || (g.children || []).some((c, i) => i > 0 && ((c.join && c.join !== 'union') || c.joinBlend > 0))
     const drawn = c => c.enabled !== false && c.kind !== undefined && c.type !== 'haze'
 
-    // a CSG group's children joined in order into one value; returns the names of its expressions
+    // a CSG group's children joined in order into one value; returns the names of its expressions. A child's join is
+    // its op (union, subtract, intersect) in its shape: the smooth polynomial blend (no joinShape), or a round's or a
+    // chamfer's of that radius (joinShape, model version 28)
     function csgGroup(g, X) {
         const kids = (g.children || []).filter(drawn)
         if (!kids.length) return null
@@ -482,29 +502,46 @@ function generate(doc, opts) {
             if (!v) continue
             if (!acc) { acc = Object.assign({}, v); continue }
             const join = i === 0 ? 'union' : (c.join || 'union')
-            const k = c.joinBlend !== undefined && c.joinBlend !== null ? c.joinBlend * unit * X.s : blendG
-            // a union's value is at least the smaller of its sides less k/4; an intersection's or a cut's at least the
-            // first side's: the bound that keeps culling exact
-            if (join === 'union') { acc.bs = encloseSpheres(acc.bs, v.bs); acc.bs = { c: acc.bs.c, r: acc.bs.r + 0.25 * Math.max(k, 0) } }
-            acc.flops += v.flops + JOIN_FLOPS
-            const ko = push([k, k > 0 ? 1 / k : 0])
-            const fn = { union: ['dUnion', 'csgUnion'], subtract: ['dSubtract', 'csgSubtract'], intersect: ['dIntersect', 'csgIntersect'] }[join]
+            const shape = c.joinShape === undefined || c.joinShape === null ? 'smooth' : c.joinShape
+            const fn = (JOIN_FNS[shape] || {})[join]
+            if (!JOIN_FNS[shape]) throw new Error(`hipgen: node '${c.id}': unknown joinShape '${shape}' (round, chamfer, or none for smooth)`)
             if (!fn) throw new Error(`hipgen: node '${c.id}': unknown join '${join}'`)
+            const k = c.joinBlend !== undefined && c.joinBlend !== null ? c.joinBlend * unit * X.s : blendG
+            // a union's value is at least the smaller of its sides less the most its blend takes off (k/4 smooth, the
+            // round's and chamfer's corner allowance); an intersection's or a cut's at least the first side's: the bound
+            // that keeps culling exact
+            if (join === 'union') { acc.bs = encloseSpheres(acc.bs, v.bs); acc.bs = { c: acc.bs.c, r: acc.bs.r + JOIN_REACH[shape] * Math.max(k, 0) } }
+            acc.flops += v.flops + JOIN_FLOPS[shape]
+            const ko = push([k, k > 0 ? 1 / k : 0])
+            const args = shape === 'smooth' ? `U[${ko}], U[${ko + 1}]` : `U[${ko}]`
             const wins = { union: `${v.f}.w < ${acc.f}.w`, subtract: `-${v.f}.w > ${acc.f}.w`, intersect: `${v.f}.w > ${acc.f}.w` }[join]
-            dist.push(`        ${acc.d} = ${fn[0]}(${acc.d}, ${v.d}, U[${ko}], U[${ko + 1}]);`)
+            dist.push(`        ${acc.d} = ${fn[0]}(${acc.d}, ${v.d}, ${args});`)
             full.push(`        if (${wins}) { ${acc.m} = ${v.m}; ${acc.k} = ${v.k}; }`)
-            full.push(`        ${acc.f} = ${fn[1]}(${acc.f}, ${v.f}, U[${ko}], U[${ko + 1}]);`)
-            program.push({ op: 'join', join, k: ko })
+            full.push(`        ${acc.f} = ${fn[1]}(${acc.f}, ${v.f}, ${args});`)
+            program.push({ op: 'join', join, shape, k: ko })
         }
         return acc
+    }
+    // A group with code of its own -- a builtin modifier (twist, bend, repeat, circular, mirror, round, shell, displace,
+    // relief: builder/modifiers.js) or the user's -- changes the point before its children or the distance after them
+    // (builder/codegen.js group code). The engine runs no group code: such a group is refused, or with opts.approximate
+    // drawn as a plain group of its children and listed.
+    // A group block without a builtin and without statements only carries the group's parameters: a plain group.
+    function modifierCheck(g) {
+        const b = g.block ? blocks[g.block] : null
+        let what = null
+        if (b && b.builtin) what = `modifier '${b.builtin}' (group code)`
+        else if (hasStatements(g.code) || (b && hasStatements(b.code))) what = 'group code (user WGSL)'
+        if (what) cannot(`group '${g.id}'`, what)
     }
     // a node's value as expressions (for use inside a CSG group)
     function valueOf(n, parentX) {
         const X = compose(parentX, n.transform, unit)
         if (n.kind === 'group') {
+            modifierCheck(n)
             if (isCsg(n)) return csgGroup(n, X)
             // a plain group inside a CSG group: its children unioned
-            return csgGroup(Object.assign({}, n, { blend: 0, children: (n.children || []).map(c => Object.assign({}, c, { join: 'union', joinBlend: undefined })) }), X)
+            return csgGroup(Object.assign({}, n, { blend: 0, children: (n.children || []).map(c => Object.assign({}, c, { join: 'union', joinBlend: undefined, joinShape: undefined })) }), X)
         }
         n.__world = X
         const s = shape(n, X)
@@ -523,6 +560,7 @@ function generate(doc, opts) {
         if (n.enabled === false) return
         if (n.type === 'haze') { hazeOf(n, compose(parentX, n.transform, unit)); return }
         if (n.kind === 'group' && !isCsg(n)) {
+            modifierCheck(n)
             const X = compose(parentX, n.transform, unit)
             for (const c of n.children || []) foldNode(c, X)
             return
@@ -670,18 +708,24 @@ function inverse3(m) {
         [C / det, (b * g - a * h) / det, (a * e - b * d) / det]]
 }
 const XYZ_TO_RGB709 = inverse3(RGB709_TO_XYZ)
-function planckUv(T) {
-    const t = Math.min(15000, Math.max(1000, T))
-    return [(0.860117757 + 1.54118254e-4 * t + 1.28641212e-7 * t * t) / (1 + 8.42420234e-4 * t + 7.08145163e-7 * t * t),
-        (0.317398726 + 4.22806245e-5 * t + 4.20481691e-8 * t * t) / (1 - 2.89741816e-5 * t + 1.61456053e-7 * t * t)]
+// Krystek's fit of the Planckian locus in CIE 1960 uv, each coordinate a ratio of quadratics in T (1000-15000 K),
+// with the locus's direction dP/dT from the same ratios' derivatives, (N'D - N D') / D^2.
+const KRYSTEK_U = [[0.860117757, 1.54118254e-4, 1.28641212e-7], [1, 8.42420234e-4, 7.08145163e-7]]
+const KRYSTEK_V = [[0.317398726, 4.22806245e-5, 4.20481691e-8], [1, -2.89741816e-5, 1.61456053e-7]]
+function ratioAndSlope([N, D], t) {
+    const n = N[0] + t * (N[1] + t * N[2]), d = D[0] + t * (D[1] + t * D[2])
+    const dn = N[1] + 2 * t * N[2], dd = D[1] + 2 * t * D[2]
+    return [n / d, (dn * d - n * dd) / (d * d)]
 }
+// A CCT and Duv as a chromaticity: the locus's point at T moved duv along its unit normal, the tangent turned a quarter
+// clockwise -- towards larger v, green, for duv > 0 -- then uv back to xy.
 function cctDuvToXy(T, duv) {
-    const [u, v] = planckUv(T)
-    const h = Math.max(1e-3, T * 1e-5), a = planckUv(T - h), b = planckUv(T + h)
-    const du = b[0] - a[0], dv = b[1] - a[1], l = Math.hypot(du, dv) || 1
-    const uu = u + duv * dv / l, vv = v - duv * du / l
-    const d = 2 * uu - 8 * vv + 4
-    return [3 * uu / d, 2 * vv / d]
+    const t = Math.min(15000, Math.max(1000, T))
+    const [u0, su] = ratioAndSlope(KRYSTEK_U, t), [v0, sv] = ratioAndSlope(KRYSTEK_V, t)
+    const len = Math.hypot(su, sv) || 1
+    const u = u0 + duv * sv / len, v = v0 - duv * su / len
+    const den = 2 * u - 8 * v + 4
+    return [3 * u / den, 2 * v / den]
 }
 function rgbFromXy(x, y) {
     const XYZ = y > 0 ? [x / y, 1, (1 - x - y) / y] : [0, 0, 0]
@@ -751,9 +795,12 @@ function environmentOf(E, doc, cannot) {
             env.sunDir = s
             env.skyAerosol = sky.aerosolOf(turb)
             env.skyMs = sky.airMass(Math.max(s[2], 0))
+            // envSky and envSun (uniforms.schema.js): the sky's and the sun's gains, 0 switching either off
+            const skyGain = Math.max(num(E.envSky, 1), 0), sunGain = Math.max(num(E.envSun, 1), 0)
             env.skyUp = Math.min(Math.max((s[2] + 0.05) / 0.05, 0), 1)
-            if (el * Math.PI / 180 + c.half > 0 && env.L > 0) {
-                env.sunL = sky.sunRadiance(s, turb, env.L, c.sin2)
+            env.col = [skyGain, skyGain, skyGain]          // the sky's gain (a tint for the sun & sky)
+            if (el * Math.PI / 180 + c.half > 0 && env.L > 0 && sunGain > 0) {
+                env.sunL = sky.sunRadiance(s, turb, env.L, c.sin2).map(x => x * sunGain)
                 env.sunE = (LUMA[0] * env.sunL[0] + LUMA[1] * env.sunL[1] + LUMA[2] * env.sunL[2]) * c.omega
             }
             if (env.L <= 0) env.type = 0

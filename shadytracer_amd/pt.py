@@ -55,7 +55,7 @@ class PtUniforms(ctypes.Structure):
                 ('seed', ctypes.c_uint), ('cam', PtCamera), ('env', PtEnv), ('haze', PtHaze), ('faceShare', ctypes.c_float),
                 ('sppm', ctypes.c_uint), ('sppmRadius', ctypes.c_float), ('sppmPhotons', ctypes.c_uint),
                 ('marchRefill', ctypes.c_uint), ('guide', ctypes.c_uint), ('guideFrac', ctypes.c_float), ('sppmAlpha', ctypes.c_float),
-                ('relax', ctypes.c_float)]
+                ('relax', ctypes.c_float), ('measureMode', ctypes.c_uint)]
 
 
 class PtSceneDesc(ctypes.Structure):
@@ -243,7 +243,7 @@ def uniforms_for(made, **over):
     u.cam = make_camera(made['camera'])
     u.env = make_env(sc['env'])
     u.haze = make_haze(sc.get('haze'))
-    for k in ('maxiter', 'marchRefill', 'rr', 'rrStart', 'diffuseCap', 'maxaverages', 'sppm', 'sppmPhotons'):
+    for k in ('maxiter', 'marchRefill', 'rr', 'rrStart', 'diffuseCap', 'maxaverages', 'sppm', 'sppmPhotons', 'measureMode'):
         if k in s:
             setattr(u, k, int(s[k]))
     for k in ('stepscale', 'eps', 'epsAuto', 'faceShare', 'fireflyClamp', 'sppmRadius', 'sppmAlpha', 'relax'):

@@ -225,6 +225,109 @@ def test_solid_builtins_match_the_oracle(engine):
     assert abs(gL.mean() - oL.mean()) < 4 * se + 1e-3 * oL.mean()
 
 
+def test_shaped_joins_match_the_oracle(engine):
+    # rounds and chamfers (joinShape): a ball-box round, a chamfered hole, a concave round fillet and a chamfered union --
+    # primary decisions ray by ray against the oracle (whose safe-step allowance differs from the engine's; the surfaces
+    # are the same), then per-pixel means
+    say('case: joins 160x64 pick vs oracle; 80x32 GPU 256 spp vs oracle 64 spp')
+    made, u = load(engine, 'joins')
+    gpos, gstate, _, gprim = engine.pick_rays()
+    opos, ostate, _, oprim = oracle.pick(made['scene'], u)
+    same = (gstate == ostate) & ((gstate != 0) | (gprim == oprim))
+    say('agreement', same.mean())
+    assert same.mean() >= 0.9995
+    hit = (gstate == 0) & same
+    assert np.quantile(np.linalg.norm(gpos[hit, :3] - opos[hit, :3], axis=1), 0.999) < 1e-4
+    made, u = load(engine, 'joins', width=80, height=32)
+    engine.trace(256)
+    ga, gm = engine.read_accum()
+    oa, om, _ = oracle.render(made['scene'], u, 64, first=100000)
+    gL, oL = pt.luminance(ga), pt.luminance(oa)
+    s = np.sqrt(pt.standard_error(gm) ** 2 + pt.standard_error(om) ** 2)
+    ok = s > 1e-9
+    z = (gL - oL)[ok] / s[ok]
+    say('z std', z.std(), 'frac |z|<3', np.mean(np.abs(z) < 3), 'means', gL.mean(), oL.mean())
+    assert 0.75 < z.std() < 1.3
+    assert np.mean(np.abs(z) < 3) > 0.97
+    se = math.sqrt(np.nansum(s ** 2)) / s.size
+    assert abs(gL.mean() - oL.mean()) < 4 * se + 1e-3 * oL.mean()
+
+
+def test_sun_and_sky_rows_on_the_engine(engine):
+    # the reference's photometry, illuminance-view and cosine-law rows (tests/photometry_rows.py) on the HIP path: each
+    # row's ratio to the light its environment puts on the surface, against the fixture's ratio within both bars
+    import photometry_rows as P
+    for test, table in P.TABLES.items():
+        for case in table:
+            say('case: %s / %s, 32x32' % (test, case))
+            made, u = P.emit(test, case, 32, 32)
+            engine.set_scene(made['scene'])
+            u.width, u.height = 32, 32
+            engine.set_uniforms(u)
+            engine.clear_accums()
+            engine.trace(1024 if 'NEE off' in case else 128)
+            acc, mom = engine.read_accum()
+            want = P.answer(test, case, made)
+            ours, se = P.ratio_of(acc, mom, want)
+            ref, rse = P.reference(test, case)
+            say('  ratio %.5f +- %.5f, reference %.5f +- %.5f' % (ours, se, ref, rse))
+            if want == 0:
+                assert ours == 0
+                continue
+            assert abs(ours - ref) < 4 * math.sqrt(se ** 2 + rse ** 2) + 1e-4, (test, case, ours, se, ref, rse)
+
+
+def test_furnace_rows_on_the_engine(engine):
+    # the reference's white furnace rows (tests/test_oracle.py FURNACE_ROWS) on the HIP path: a unit sphere of each
+    # material in a uniform sky, the frame's coverage from the engine's own black-sphere control
+    from test_oracle import FURNACE_ROWS, FURNACE_CONTROL, ref_rows
+    rows = ref_rows('furnace')
+
+    def row(opts, over, spp):
+        made = scenes.emit('furnace-sphere', width=79, height=106, **opts)
+        engine.set_scene(made['scene'])
+        engine.set_uniforms(pt.uniforms_for(made, **over))
+        engine.clear_accums()
+        engine.trace(spp)
+        acc, mom = engine.read_accum()
+        L = pt.luminance(acc)
+        return L.mean(), math.sqrt(np.nansum(pt.standard_error(mom) ** 2)) / L.size
+
+    say('case: furnace control (black sphere), 79x106, 64 spp')
+    ctl, _ = row({'albedo': 0, 'roughness': 0.5, 'specular': 0}, {}, 64)
+    f_ref, f_ours = 1 - rows[FURNACE_CONTROL]['meanLuma'] / 1000, 1 - ctl / 1000
+    assert abs(f_ours / f_ref - 1) < 0.005, (f_ours, f_ref)
+    for case, (opts, over) in FURNACE_ROWS.items():
+        say('case: furnace %s, 79x106, 256 spp' % case)
+        mean, se = row(opts, over, 256)
+        mean = 1000 - (1000 - mean) * f_ref / f_ours
+        r = rows[case]
+        rse = r['meanLuma'] * r['relSE'] / 100
+        say('  %.3f +- %.3f, reference %.3f +- %.3f' % (mean, se, r['meanLuma'], rse))
+        assert abs(mean - r['meanLuma']) < 4 * math.sqrt(se ** 2 + rse ** 2) + 0.02, (case, mean, se, r['meanLuma'], rse)
+
+
+@pytest.mark.parametrize('case,opts', [('Lambertian', {'roughness': 0.3}), ('plastic 0.3', {'roughness': 0.3, 'specular': 0.5}),
+                                       ('metal 0.3', {'roughness': 0.3, 'metallic': 1}), ('metal 0.6', {'roughness': 0.6, 'metallic': 1})])
+def test_bsdf_parity_rows_on_the_engine(engine, case, opts):
+    # bsdf-parity (lib/pathtracer/tests.js:1868) on the HIP path: the oblique floor under a 5-degree sun, light sampling
+    # on against off -- BSDF sampling alone must find the same light
+    say('case: bsdf-parity %s, 32x32, NEE on 256 spp, NEE off 4096 spp' % case)
+    made = scenes.emit('sunsky-bench', width=32, height=32, oblique=True, sunDiameter=5, **opts)
+    engine.set_scene(made['scene'])
+    got = []
+    for nee, spp in ((1, 256), (0, 4096)):
+        engine.set_uniforms(pt.uniforms_for(made, nee=nee))
+        engine.clear_accums()
+        engine.trace(spp)
+        acc, mom = engine.read_accum()
+        L = pt.luminance(acc).ravel()
+        got.append((L.mean(), math.sqrt(np.nansum(pt.standard_error(mom).ravel() ** 2)) / L.size))
+    (a, sa), (b, sb) = got
+    say('  NEE on %.2f +- %.2f, off %.2f +- %.2f' % (a, sa, b, sb))
+    assert abs(a - b) < 4 * math.hypot(sa, sb), got
+
+
 def test_thin_lens_rays_match_the_oracle(engine):
     # the thin lens: the same lens points and focus-plane aims as the oracle's, ray by ray, out of focus on the floor
     from test_oracle import _looking_down

@@ -242,6 +242,37 @@ def test_guided_furnace_keeps_its_energy(engine, case, opts):
     assert abs(guided.mean() - plain.mean()) < 4 * se + 1e-3 * plain.mean(), (plain.mean(), guided.mean(), se)
 
 
+def test_guided_rough_glass_furnace_agrees_with_the_plain_tracer(engine):
+    # rough glass (0.3) in the white furnace, guided: BSDF-drawn refractions and guide-drawn directions below the surface
+    # are weighed by the mixture of the two densities with the transmission lobe evaluated (Walter et al. 2007), so the
+    # guided picture keeps the plain tracer's energy -- the single-scatter 0.94 of the sphere, not more, not less
+    say('case: guided white furnace, rough glass 0.3 sphere, 64x64: trained 32 samples, then 128 guided against 128 plain')
+    made = scenes.emit('furnace-sphere', width=64, height=64, roughness=0.3, transmission=1)
+    engine.set_scene(made['scene'])
+    engine.set_uniforms(pt.uniforms_for(made))
+    engine.guide_config(pt.scene_box(made['scene']), batch=2048, steps=2, warmup=4)
+    u = pt.uniforms_for(made)
+    engine.set_uniforms(u)
+    engine.clear_accums()
+    engine.trace(128)
+    plain, pm = engine.read_accum()
+    ug = pt.uniforms_for(made)
+    ug.guide, ug.guideFrac = 1, 0.5
+    engine.set_uniforms(ug)
+    engine.clear_accums()
+    engine.trace(32)
+    engine.clear_accums()
+    engine.reset_stats()
+    engine.trace(128)
+    guided, gm = engine.read_accum()
+    st = engine.stats()
+    Lp, Lg = pt.luminance(plain), pt.luminance(guided)
+    se = math.sqrt(np.nansum(pt.standard_error(pm) ** 2) + np.nansum(pt.standard_error(gm) ** 2)) / Lp.size
+    say('plain %.3f guided %.3f, se %.3f (guided vertices %d)' % (Lp.mean(), Lg.mean(), se, st['guided']))
+    assert st['guided'] > 0 and st['nonfinite'] == 0
+    assert abs(Lg.mean() - Lp.mean()) < 4 * se + 1e-3 * Lp.mean(), (Lp.mean(), Lg.mean(), se)
+
+
 def fixed_theta(lobes, c):
     """All weights 0; the head's biases alone set the mixture at every vertex: lobes [(logit, mean, log kappa)], c."""
     th = np.zeros(guide.nparams(), np.float32)
