@@ -24,7 +24,7 @@ extern "C" {
 #define PT_ERR_COMPILE 3    /* the scene did not compile; the previous scene is still loaded */
 #define PT_ERR_STATE 4      /* the call is not valid now: trace with no scene, or during a scene build */
 
-#define PT_ABI_VERSION 1
+#define PT_ABI_VERSION 2
 
 typedef struct pt_ctx pt_ctx;
 
@@ -48,7 +48,11 @@ typedef struct pt_scene_desc {
     uint32_t nmaterials;
     const PtLightFace* faces;
     uint32_t nfaces;
+    uint32_t flags;             /* PT_SCENE_* */
 } pt_scene_desc;
+/* The new scene draws the same picture as the one loaded (the reference's setScene({keepPicture}): a scene baked with
+ * its values as constants): the accumulation is kept, not cleared. */
+#define PT_SCENE_KEEP_PICTURE 1u
 
 /* ABI version of this library (PT_ABI_VERSION). */
 int pt_abi_version(void);
@@ -75,8 +79,9 @@ const char* pt_last_error(const pt_ctx* ctx);
 int pt_compile_scene(const char* hip_src, const char* cache_dir, char* path_out, size_t path_cap);
 
 /* Loads a scene: compiles (or finds in the cache) and loads its kernels, uploads its tables and clears the
- * accumulation. On a compile failure the previous scene stays loaded and PT_ERR_COMPILE is returned (the
- * reference's e.restored). Replaces R.setScene (raytrace_main.js:1145-1183). */
+ * accumulation (kept with PT_SCENE_KEEP_PICTURE). On a compile failure the previous scene stays loaded and
+ * PT_ERR_COMPILE is returned (the reference's e.restored). Safe to call from a worker thread while the caller makes no
+ * other call on the context. Replaces R.setScene(s, {keepPicture}) (raytrace_main.js:1145-1183). */
 int pt_set_scene(pt_ctx* ctx, const pt_scene_desc* scene);
 
 /* Updates the scene's parameter block in place (same length): the builder's per-node uniforms, without a

@@ -62,6 +62,7 @@
 #define PT_S_GUIDED 17       // path vertices sampled with guiding on
 #define PT_S_GRECORDS 18     // path vertices recorded for training
 #define PT_S_FLUSHED 19      // camera paths still going when a sample's rounds ran out, deposited as they stood
+#define PT_S_SHADOWCUT 20    // shadow rays cut short by maxiter, counted as reaching their light (wavefront.js:1640-1645)
 #define PT_S_WORDS 24
 // Rounds of march -> shade a sample runs beyond maxbounce + 1: a hit from behind moves a path on without a bounce,
 // so a path that met some still finishes; one still going after them is flushed (PT_S_FLUSHED)

@@ -595,6 +595,15 @@ static d3 reflL(d3 o, d3 h) { return sub(scl(h, 2 * dotp(o, h)), o); }
 static double g_mf[PT_MF_WORDS];
 static int g_mfReady = 0;
 /* The tables by stratified visible-normal quadrature (64 x 64 per entry), FP64. */
+static double lutAt(d3 o, double a, int e1, double nrel, double u1, double u2, int* above) {
+    d3 h = mfVNDF(o, a, u1, u2), i = reflL(o, h);
+    *above = i.z > 0;
+    if (!*above) return 0;
+    return (e1 ? 1 : or_fresnel(dotp(o, h), 1 / nrel)) * mfG2(o, i, a) / mfG1(o, a);
+}
+/* the tables as computed before (a cache the Python side keeps on disk: they take seconds) */
+void or_mf_load(const double* in) { memcpy(g_mf, in, sizeof g_mf); g_mfReady = 1; }
+int or_mf_words(void) { return (int)(sizeof g_mf / sizeof g_mf[0]); }
 void or_mf_tables(double* out) {
     if (!g_mfReady) {
         #pragma omp parallel for schedule(dynamic)
@@ -605,14 +614,21 @@ void or_mf_tables(double* out) {
             double mu = (im + 0.5) / PT_MF_NMU, r = (double)ir / (PT_MF_NR - 1), nrel = 1 + 2.0 * ii / (PT_MF_NIOR - 1);
             double a = clampd(r * r, ALPHA_MIN, 1);
             d3 o = v(sqrt(maxd(0, 1 - mu * mu)), 0, mu);
+            /* the mean over the visible-normal square of F G2/G1 above the horizon: midpoints on 128 x 128, a cell
+             * the horizon's cut crosses (a corner disagreeing with the centre) taken again as 8 x 8 midpoints */
+            const int N = 128, S = 8;
             double acc = 0;
-            for (int sa = 0; sa < 64; ++sa) for (int sb = 0; sb < 64; ++sb) {
-                d3 h = mfVNDF(o, a, (sa + 0.5) / 64, (sb + 0.5) / 64);
-                d3 i = reflL(o, h);
-                if (i.z <= 0) continue;
-                acc += (e1 ? 1 : or_fresnel(dotp(o, h), 1 / nrel)) * mfG2(o, i, a) / mfG1(o, a);
+            for (int ca = 0; ca < N; ++ca) for (int cb = 0; cb < N; ++cb) {
+                int above, c, cut = 0;
+                double mid = lutAt(o, a, e1, nrel, (ca + 0.5) / N, (cb + 0.5) / N, &above);
+                for (int k = 0; k < 4; ++k) { lutAt(o, a, e1, nrel, (double)(ca + (k & 1)) / N, (double)(cb + (k >> 1)) / N, &c); cut |= c != above; }
+                if (!cut) { acc += mid; continue; }
+                double sub = 0;
+                for (int sa = 0; sa < S; ++sa) for (int sb = 0; sb < S; ++sb)
+                    sub += lutAt(o, a, e1, nrel, (ca + (sa + 0.5) / S) / N, (cb + (sb + 0.5) / S) / N, &c);
+                acc += sub / (S * S);
             }
-            acc /= 4096.0;
+            acc /= (double)N * N;
             if (e1) g_mf[PT_MF_E1 + ir * PT_MF_NMU + im] = acc;
             else g_mf[PT_MF_EC + (ii * PT_MF_NR + ir) * PT_MF_NMU + im] = acc;
         }
@@ -994,8 +1010,10 @@ static int occluded(const OrScene* S, d3 o, d3 dir, double tmax, long long* step
     uint64_t mask = sceneMask(S, o, dir, tmax, maskMargin(S));
     for (;;) {
         double d = sceneDistM(S, p, mask), ea = fabs(d);
-        if (ea < marchTol(S, p) || k > u->maxiter) return 0;
+        if (ea < marchTol(S, p)) return 0;
         if (ea >= tmax - trav || (dotp(p, dir) > 0 && dotp(p, p) > R2)) return 1;
+        /* cut short by maxiter: unoccluded, as wavefront.js:1640-1645 counts it (occluded would bias dark) */
+        if (k > u->maxiter) return 1;
         double t = ea * u->stepscale;
         p = add(p, scl(dir, t)); trav += t; ++k; ++*steps;
     }

@@ -535,6 +535,7 @@ void setDefaults(PtUniforms& u) {
     u.cam.pos[1] = -3.0f; u.cam.pos[2] = 1.5f;
     u.cam.right[0] = 1.0f; u.cam.up[2] = 1.0f; u.cam.fwd[1] = -1.0f;
     u.cam.fx = 0.57735f; u.cam.fy = 0.57735f;
+    u.cam.focus = 1.0f;
     u.faceShare = 0.5f;
     u.marchRefill = 0;
     u.guideFrac = 0.9f;
@@ -715,6 +716,7 @@ int pt_set_scene(pt_ctx* c, const pt_scene_desc* s) {
         HIPCHK(c, hipModuleLaunchKernel(c->fn[K_MFAVG], (PT_MF_NIOR * PT_MF_NR + 255) / 256, 1, 1, 256, 1, 1, 0, c->stream, args, nullptr));
         c->host.mfReady = 1;
     }
+    if (s->flags & PT_SCENE_KEEP_PICTURE) return PT_OK;
     return pt_clear_accum(c);
 }
 
@@ -766,6 +768,10 @@ int pt_set_uniforms(pt_ctx* c, const PtUniforms* u, size_t size) {
     if (!(u->stepscale > 0.0f && u->stepscale <= 1.0f)) return fail(c, PT_ERR_ARG, "stepscale %g not in (0, 1]", u->stepscale);
     if (!(u->relax == 0.0f || (u->relax >= 1.0f && u->relax <= 1.95f))) return fail(c, PT_ERR_ARG, "relax %g not 0 or in [1, 1.95]", u->relax);
     if (u->measureMode > 2u) return fail(c, PT_ERR_ARG, "measureMode %u not 0, 1 (illuminance) or 2 (luminance)", u->measureMode);
+    // the thin lens: a radius of 0 is the pinhole; a lens needs a finite radius and a distance in focus in front of it
+    if (!(std::isfinite(u->cam.lensR) && u->cam.lensR >= 0.0f)) return fail(c, PT_ERR_ARG, "camera lensR %g must be finite and >= 0", u->cam.lensR);
+    if (u->cam.lensR > 0.0f && !(std::isfinite(u->cam.focus) && u->cam.focus > 0.0f))
+        return fail(c, PT_ERR_ARG, "camera focus %g must be > 0 with a lens (lensR %g)", u->cam.focus, u->cam.lensR);
     if (u->sppm && (u->sppmPhotons == 0 || u->sppmPhotons > (1u << 26)))
         return fail(c, PT_ERR_ARG, "sppmPhotons %u not in 1..2^26", u->sppmPhotons);
     if (u->sppm && !(u->sppmRadius > 0.0f && u->sppmAlpha > 0.0f && u->sppmAlpha < 1.0f))

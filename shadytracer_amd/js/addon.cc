@@ -1,12 +1,15 @@
 // The Node N-API addon: libpt's C ABI (include/pt.h) as JavaScript functions, one to one. Structures cross as
-// ArrayBuffers that the JS shim packs (raytrace_main.js); trace returns a Promise resolved on a worker thread
-// once the GPU has finished, so a trace never blocks the event loop (the reference's step() resolves on
-// queue.onSubmittedWorkDone, lib/pathtracer/index.js:1005-1007).
+// ArrayBuffers that the JS shim packs (raytrace_main.js); trace, setScene and foldParams return Promises settled on a
+// worker thread once the GPU (or the compiler) has finished, so neither a trace nor a compile blocks the event loop
+// (the reference's step() resolves on queue.onSubmittedWorkDone, lib/pathtracer/index.js:1005-1007; its setScene is
+// async, raytrace_main.js:1145).
 #include <node_api.h>
 
+#include <chrono>
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "pt.h"
 
@@ -144,13 +147,78 @@ napi_value CompileScene(napi_env env, napi_callback_info info) {
     return r;
 }
 
-// setScene(h, src, paramsF32, materialsBuf, nmat, facesBuf, nfaces)
+// A scene build on a worker thread (the reference's setScene and its pipelines are asynchronous: hipRTC's compile,
+// seconds for a large scene, never holds the event loop). Everything it reads is copied in first: the JS side may
+// collect its buffers before the worker runs. The shim makes no other call on the context until the Promise settles.
+struct BuildWork {
+    napi_async_work work;
+    napi_deferred deferred;
+    pt_ctx* ctx;
+    bool fold;                          // pt_fold_params instead of pt_set_scene
+    std::string src;
+    std::vector<float> params;
+    std::vector<PtMaterial> mats;
+    std::vector<PtLightFace> faces;
+    uint32_t flags;
+    int rc;
+    std::string err;
+    double ms;
+};
+
+void buildExecute(napi_env, void* data) {
+    BuildWork* w = static_cast<BuildWork*>(data);
+    const auto t0 = std::chrono::steady_clock::now();
+    if (w->fold) {
+        w->rc = pt_fold_params(w->ctx);
+    } else {
+        pt_scene_desc d;
+        d.hip_src = w->src.c_str();
+        d.params = w->params.data(); d.nparams = (uint32_t)w->params.size();
+        d.materials = w->mats.data(); d.nmaterials = (uint32_t)w->mats.size();
+        d.faces = w->faces.empty() ? nullptr : w->faces.data(); d.nfaces = (uint32_t)w->faces.size();
+        d.flags = w->flags;
+        w->rc = pt_set_scene(w->ctx, &d);
+    }
+    if (!w->rc) w->rc = pt_sync(w->ctx);
+    if (w->rc) w->err = pt_last_error(w->ctx);
+    w->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void buildComplete(napi_env env, napi_status, void* data) {
+    BuildWork* w = static_cast<BuildWork*>(data);
+    if (w->rc) {
+        // the Error a failed build rejects with: code, and restored when the scene before is still the one loaded
+        napi_value msg, err, codeV, restored;
+        std::string text = "libpt: " + w->err;
+        napi_create_string_utf8(env, text.c_str(), text.size(), &msg);
+        napi_create_error(env, nullptr, msg, &err);
+        napi_create_int32(env, w->rc, &codeV);
+        napi_set_named_property(env, err, "code", codeV);
+        napi_get_boolean(env, w->rc == PT_ERR_COMPILE, &restored);
+        napi_set_named_property(env, err, "restored", restored);
+        napi_reject_deferred(env, w->deferred, err);
+    } else {
+        napi_resolve_deferred(env, w->deferred, num(env, w->ms));
+    }
+    napi_delete_async_work(env, w->work);
+    delete w;
+}
+
+napi_value queueBuild(napi_env env, BuildWork* w) {
+    napi_value promise, name;
+    if (napi_create_promise(env, &w->deferred, &promise) != napi_ok) { delete w; napi_throw_error(env, nullptr, "no promise"); return nullptr; }
+    napi_create_string_utf8(env, w->fold ? "pt_fold_params" : "pt_set_scene", NAPI_AUTO_LENGTH, &name);
+    NAPI_OK(env, napi_create_async_work(env, nullptr, name, buildExecute, buildComplete, w, &w->work));
+    NAPI_OK(env, napi_queue_async_work(env, w->work));
+    return promise;
+}
+
+// setScene(h, src, paramsF32, materialsBuf, nmat, facesBuf, nfaces, keepPicture) -> Promise of the build's ms
 napi_value SetScene(napi_env env, napi_callback_info info) {
-    napi_value a[7];
+    napi_value a[8];
     if (!args(env, info, 7, a)) return nullptr;
     pt_ctx* ctx = ctxOf(env, a[0]);
     if (!ctx) return nullptr;
-    std::string src = str(env, a[1]);
     void *pp, *mp, *fp; size_t pl, ml, fl;
     if (!bytesOf(env, a[2], &pp, &pl) || !bytesOf(env, a[3], &mp, &ml) || !bytesOf(env, a[5], &fp, &fl)) return nullptr;
     uint32_t nm = 0, nf = 0;
@@ -160,14 +228,20 @@ napi_value SetScene(napi_env env, napi_callback_info info) {
         napi_throw_range_error(env, nullptr, "material or face buffer shorter than its count");
         return nullptr;
     }
-    pt_scene_desc d;
-    d.hip_src = src.c_str();
-    d.params = static_cast<const float*>(pp); d.nparams = (uint32_t)(pl / 4);
-    d.materials = static_cast<const PtMaterial*>(mp); d.nmaterials = nm;
-    d.faces = static_cast<const PtLightFace*>(fp); d.nfaces = nf;
-    int rc = pt_set_scene(ctx, &d);
-    if (rc) return throwPt(env, ctx, rc);
-    return undef(env);
+    bool keep = false;
+    size_t argc = 8;
+    napi_value all[8];
+    napi_get_cb_info(env, info, &argc, all, nullptr, nullptr);
+    if (argc >= 8) napi_get_value_bool(env, all[7], &keep);
+    BuildWork* w = new BuildWork();
+    w->ctx = ctx;
+    w->fold = false;
+    w->src = str(env, a[1]);
+    w->params.assign(static_cast<const float*>(pp), static_cast<const float*>(pp) + pl / 4);
+    w->mats.assign(static_cast<const PtMaterial*>(mp), static_cast<const PtMaterial*>(mp) + nm);
+    w->faces.assign(static_cast<const PtLightFace*>(fp), static_cast<const PtLightFace*>(fp) + nf);
+    w->flags = keep ? PT_SCENE_KEEP_PICTURE : 0u;
+    return queueBuild(env, w);
 }
 
 napi_value SetParams(napi_env env, napi_callback_info info) {
@@ -182,14 +256,17 @@ napi_value SetParams(napi_env env, napi_callback_info info) {
     return undef(env);
 }
 
+// foldParams(h) -> Promise of the build's ms (a compile too)
 napi_value FoldParams(napi_env env, napi_callback_info info) {
     napi_value a[1];
     if (!args(env, info, 1, a)) return nullptr;
     pt_ctx* ctx = ctxOf(env, a[0]);
     if (!ctx) return nullptr;
-    int rc = pt_fold_params(ctx);
-    if (rc) return throwPt(env, ctx, rc);
-    return undef(env);
+    BuildWork* w = new BuildWork();
+    w->ctx = ctx;
+    w->fold = true;
+    w->flags = 0u;
+    return queueBuild(env, w);
 }
 
 napi_value SetUniforms(napi_env env, napi_callback_info info) {

@@ -385,7 +385,9 @@ function sunskyBench(opts = {}) {
         d.environment = { envMode: 1, studioIlluminance: opts.studio.lux, studioTop: opts.studio.top, studioColor: { r: t[0], g: t[1], b: t[2] } }
     }
     const w = opts.width || 32, h = opts.height || 32
-    return { doc: d, camera: cameraOf(d, w, h), settings: { width: w, height: h, spp: 32, maxbounce: 10, nee: 1, maxiter: 400 } }
+    // the reference's bench plane says it is 10 units across (tests.js EXTENT plane): its tolerance is autoEps of that
+    const epsAuto = Math.min(Math.max(1e-4 * 10 * UNIT[unit], 1e-8), 1e-3)
+    return { doc: d, camera: cameraOf(d, w, h), settings: { width: w, height: h, spp: 32, maxbounce: 10, nee: 1, maxiter: 400, epsAuto } }
 }
 const UNIT = { m: 1, mm: 0.001 }
 

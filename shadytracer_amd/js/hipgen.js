@@ -341,6 +341,28 @@ function localRadius(key, P, unit) {
     }
     return Infinity
 }
+// A shape's half extents in its own frame, for the scene's bounds (the builder's, model.js bounds): the common solids
+// exactly, the rest as the cube round their bounding sphere; null for the unbounded plane.
+function localHalf(key, P, unit) {
+    switch (key) {
+        case 'plane': return null
+        case 'sphere': return [P.radius, P.radius, P.radius]
+        case 'box': return [P.size.x, P.size.y, P.size.z]
+        case 'cylinder': return [P.radius, P.radius, 0.5 * P.height]
+        case 'capsule': return [P.radius, P.radius, 0.5 * P.height + P.radius]
+        case 'torus': return [P.ring + P.tube, P.ring + P.tube, P.tube]
+        case 'rectangle': return [0.5 * P.width, 0.5 * P.height, 0]
+        case 'disc': return [P.radius, P.radius, 0]
+    }
+    const r = localRadius(key, P, unit)
+    return isFinite(r) ? [r, r, r] : null
+}
+// THE SCENE'S TOLERANCE (photometry.js autoEps:117): 1e-4 of its size -- the diagonal of its bounds -- never under
+// 1e-8 m nor over 1 mm; a scene with no bounded part keeps the 1 mm.
+function autoEps(extentMetres) {
+    return extentMetres > 0 && isFinite(extentMetres) ? Math.min(Math.max(1e-4 * extentMetres, 1e-8), 1e-3) : 1e-3
+}
+
 // The smallest sphere found by enclosing two spheres.
 function encloseSpheres(a, b) {
     if (!isFinite(a.r) || !isFinite(b.r)) return { c: a.c, r: Infinity }
@@ -396,6 +418,7 @@ function generate(doc, opts) {
 
     const params = []
     const layout = {}
+    const bounds = [[Infinity, Infinity, Infinity], [-Infinity, -Infinity, -Infinity]]
     const program = []
     const faces = []
     let haze = null
@@ -442,6 +465,15 @@ function generate(doc, opts) {
         const xo = push(xfParams(X))
         const po = push(pr ? pr.U : SHAPES[key].params(P, unit))
         layout[n.id] = { xf: xo, params: po, shape: key }
+        // the scene's bounds: the local box's corners in the world
+        const half = localHalf(key, P, unit)
+        if (half) {
+            for (let c = 0; c < 8; c++) {
+                const l = [0, 1, 2].map(i => ((c >> i) & 1 ? 1 : -1) * half[i] * X.s)
+                const w = [0, 1, 2].map(i => X.t[i] + X.R[i][0] * l[0] + X.R[i][1] * l[1] + X.R[i][2] * l[2])
+                for (let i = 0; i < 3; i++) { bounds[0][i] = Math.min(bounds[0][i], w[i]); bounds[1][i] = Math.max(bounds[1][i], w[i]) }
+            }
+        }
         let primFlops = PRIM_FLOPS[key]
         if (pr) {
             helpers.push(profileCode(po, pr))
@@ -661,8 +693,15 @@ function generate(doc, opts) {
         faces,
         haze,
         env,
+        // what the environment was made from: the document's settings and the little of the document it reads, so a
+        // host's lighting settings (createPathTracer's set) can be laid over them later without the document
+        envSettings: doc.environment || {},
+        envDoc: { unit: doc.unit || 'm', sky: doc.sky },
         envRadius,
         unit,
+        // the diagonal of the scene's bounds, metres (null when nothing in it is bounded), and the tolerance it implies
+        extent: isFinite(bounds[0][0]) ? Math.hypot(...[0, 1, 2].map(i => bounds[1][i] - bounds[0][i])) : null,
+        epsAuto: autoEps(isFinite(bounds[0][0]) ? Math.hypot(...[0, 1, 2].map(i => bounds[1][i] - bounds[0][i])) : null),
         program,
         flops: { dist: flops, maskNode: 14 },   // every node evaluated: the bound a culled step stays under
         approximations,
@@ -840,4 +879,4 @@ function environmentOf(E, doc, cannot) {
     return env
 }
 
-module.exports = { generate, environmentOf, cameraOf, lightRGB, cctDuvToXy, rotationOf, UNIT_METRES, PRIM_FLOPS }
+module.exports = { generate, environmentOf, cameraOf, lightRGB, cctDuvToXy, rotationOf, autoEps, UNIT_METRES, PRIM_FLOPS }

@@ -93,22 +93,29 @@ function RAYTRACER(WG, _U, Q, scene) {
         return R
     }
 
-    // A builder document is emitted first; a hipgen output is loaded as is.
-    R.setScene = (sc) => {
+    // A builder document is emitted first; a hipgen output is loaded as is. Resolves with the milliseconds it took, as
+    // the reference's does (raytrace_main.js:1145-1183): the compile runs on a worker thread, and nothing else may be
+    // asked of the engine meanwhile (a trace then throws). A build that fails rejects with e.restored: the scene before
+    // is still loaded. keepPicture: the new scene draws the same picture (a baked scene), so the samples so far stay.
+    R.setScene = async (sc, { keepPicture = false } = {}) => {
         if (!h) throw new Error('RAYTRACER: init() first')
+        if (building) throw new Error('RAYTRACER: a scene build is already running')
         const t0 = Date.now()
         const made = sc && sc.hip ? sc : generate(sc)
         building = true
         try {
-            A.setScene(h, made.hip, Float32Array.from(made.params), packArray(MATERIAL, made.materials), made.materials.length,
-                packArray(FACE, made.faces), made.faces.length)
+            await A.setScene(h, made.hip, Float32Array.from(made.params), packArray(MATERIAL, made.materials), made.materials.length,
+                packArray(FACE, made.faces), made.faces.length, !!keepPicture)
         } finally { building = false }
         current = made
         U.envRadius = made.envRadius
+        if (made.epsAuto) U.epsAuto = made.epsAuto
         U.env = made.env
         U.haze = made.haze || {}
         return Date.now() - t0
     }
+    R.scene = () => current
+    R.building = () => building
 
     R.uniforms = () => U
 
@@ -125,7 +132,7 @@ function RAYTRACER(WG, _U, Q, scene) {
         return buf
     }
 
-    R.clearAccums = () => A.clearAccum(h)
+    R.clearAccums = () => { if (building) throw new Error('RAYTRACER: clear during a scene build'); A.clearAccum(h) }
     // the persistent kernels' grid and refill thresholds picked by timing candidates (pt_autotune); clears the picture
     R.autotune = (samples = 4) => {
         if (!current) throw new Error('RAYTRACER: no scene')
@@ -134,8 +141,17 @@ function RAYTRACER(WG, _U, Q, scene) {
     }
     R.setTuning = (blocksPerCu, refillMarch, refillOcclude) => A.setTuning(h, blocksPerCu, refillMarch, refillOcclude)
     // the scene's kernels with its parameters as constants (pt_fold_params), until its parameters or the scene change
-    R.setParams = (params) => { if (!current) throw new Error('RAYTRACER: no scene'); A.setParams(h, Float32Array.from(params)) }
-    R.foldParams = () => { if (!current) throw new Error('RAYTRACER: no scene'); A.foldParams(h) }
+    R.setParams = (params) => {
+        if (!current) throw new Error('RAYTRACER: no scene')
+        if (building) throw new Error('RAYTRACER: parameters during a scene build')
+        A.setParams(h, Float32Array.from(params))
+    }
+    R.foldParams = async () => {
+        if (!current) throw new Error('RAYTRACER: no scene')
+        if (building) throw new Error('RAYTRACER: a scene build is already running')
+        building = true
+        try { return await A.foldParams(h) } finally { building = false }
+    }
     R.ensureBuffers = (w, hh) => { U.width = w; U.height = hh }
     R.releaseBuffers = () => { if (h) { A.destroy(h); h = null } }
 

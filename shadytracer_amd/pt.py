@@ -61,7 +61,7 @@ class PtUniforms(ctypes.Structure):
 class PtSceneDesc(ctypes.Structure):
     _fields_ = [('hip_src', ctypes.c_char_p), ('params', ctypes.POINTER(ctypes.c_float)), ('nparams', ctypes.c_uint32),
                 ('materials', ctypes.POINTER(PtMaterial)), ('nmaterials', ctypes.c_uint32),
-                ('faces', ctypes.POINTER(PtLightFace)), ('nfaces', ctypes.c_uint32)]
+                ('faces', ctypes.POINTER(PtLightFace)), ('nfaces', ctypes.c_uint32), ('flags', ctypes.c_uint32)]
 
 
 class PtGuideConfig(ctypes.Structure):
@@ -148,7 +148,8 @@ GUIDE_NP = 16169
 
 STAT_NAMES = ['segments', 'shadow_segments', 'steps', 'shadow_steps', 'nonfinite', 'maxiter', 'paths', 'leaving',
               'gathers', 'photons', 'flops', 'shadow_flops', 'photon_segments', 'gather_cut', 'radius_bits',
-              'overflow', 'guide_steps', 'guided', 'guide_records', 'flushed']
+              'overflow', 'guide_steps', 'guided', 'guide_records', 'flushed', 'shadowcut']
+PT_SCENE_KEEP_PICTURE = 1   # pt_scene_desc.flags (include/pt.h)
 STRAGGLER_ROUNDS = 1    # PT_STRAGGLER_ROUNDS (include/pt_types.h): rounds a sample runs beyond maxbounce + 1
 LUMA = (0.2126, 0.7152, 0.0722)
 
@@ -240,6 +241,9 @@ def uniforms_for(made, **over):
     u.nee = int(s.get('nee', 1))
     u.seed = int(s.get('seed', 20261203))
     u.envRadius = float(sc['envRadius'])
+    # the scene's own tolerance (photometry.js autoEps: 1e-4 of its bounds' diagonal), unless the settings give one
+    if sc.get('epsAuto'):
+        u.epsAuto = float(sc['epsAuto'])
     u.cam = make_camera(made['camera'])
     u.env = make_env(sc['env'])
     u.haze = make_haze(sc.get('haze'))
@@ -287,7 +291,8 @@ class Engine:
         except Exception:
             pass
 
-    def set_scene(self, scene):
+    def set_scene(self, scene, keep_picture=False):
+        """Loads an emitted scene; keep_picture keeps the accumulation (a scene that draws the same picture)."""
         mats = (PtMaterial * len(scene['materials']))()
         for i, m in enumerate(scene['materials']):
             _set3(mats[i].base, m['base']); _set3(mats[i].emissive, m['emissive'])
@@ -308,6 +313,7 @@ class Engine:
         d.nparams = len(params)
         d.materials, d.nmaterials = mats, len(scene['materials'])
         d.faces, d.nfaces = faces, nf
+        d.flags = PT_SCENE_KEEP_PICTURE if keep_picture else 0
         self._keep = [params, mats, faces, d]
         self._chk(self.lib.pt_set_scene(self.h, ctypes.byref(d)))
         self._scene_hash = hashlib.sha1(scene['hip'].encode()).hexdigest()[:16]

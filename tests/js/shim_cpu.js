@@ -24,6 +24,9 @@ out.packed = [dv.getUint32(0, true), dv.getUint32(4, true), dv.getUint32(60, tru
 const RT = createPathTracer({ width: 8, height: 8 })
 out.unknown = RT.set({ maxbounce: 3, notAUniform: 1, stepscale: 0.8 })
 out.maxbounce = RT.get('maxbounce')
+// the reference's lighting settings (uniforms.schema.js Lighting) are known names, kept for the scene's environment
+out.lighting = RT.set({ uSunElevation: 20, envSky: 0, sunOutside: 128000, envMode: 0 })
+out.elevation = RT.get('uSunElevation')
 try {
     generate({ root: { children: [{ id: 'u1', kind: 'object', type: 'code', block: 'b' }] }, blocks: [{ id: 'b', code: 'C = add_new(C, myShape(p), MATERIAL);' }] })
     out.refused = null

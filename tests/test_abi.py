@@ -39,7 +39,7 @@ def test_struct_sizes_match_the_bindings():
     for i, s in enumerate(pt.STRUCTS):
         assert lib.pt_struct_size(i) == ctypes.sizeof(s), s.__name__
     assert lib.pt_struct_size(99) == 0
-    assert lib.pt_abi_version() == 1
+    assert lib.pt_abi_version() == 2
 
 
 def test_addon_loads_and_packs_the_same_structures():
@@ -50,6 +50,8 @@ def test_addon_loads_and_packs_the_same_structures():
     assert out['addonKeys'] == sorted(['abiVersion', 'autotune', 'clearAccum', 'compileScene', 'create', 'destroy', 'foldParams', 'opticsTrace',
                                        'readAccum', 'readStats', 'setParams', 'setScene', 'setTuning', 'setUniforms', 'structSize',
                                        'trace'])
+    # the reference's lighting settings are set() names (uniforms.schema.js Lighting), none returned as unknown
+    assert out['lighting'] == [] and out['elevation'] == 20
     o = out['opticsSizes']
     assert o[0] == o[1] == ctypes.sizeof(pt.PtOpticsScene) and o[2] == o[3] == ctypes.sizeof(pt.PtOpticsResult)
     # fields land at their C offsets

@@ -103,7 +103,10 @@ def test_studio_floor_reads_rho_E_over_pi(engine):
     engine.trace(256)
     acc, mom = engine.read_accum()
     L = pt.luminance(acc).ravel()
-    floor = state == 0
+    # the floor's pixels, less the row beside its far edge (the 20 m disc), whose samples also meet the sky
+    hit = (state == 0).reshape(u.height, u.width)
+    hit[:-1] &= hit[1:]
+    floor = hit.ravel()
     assert floor.sum() > 1500
     assert abs(L[floor].mean() / (0.6 * 500 / math.pi) - 1) < 2e-3, L[floor].mean()
 
@@ -234,8 +237,15 @@ def test_shaped_joins_match_the_oracle(engine):
     gpos, gstate, _, gprim = engine.pick_rays()
     opos, ostate, _, oprim = oracle.pick(made['scene'], u)
     same = (gstate == ostate) & ((gstate != 0) | (gprim == oprim))
-    say('agreement', same.mean())
-    assert same.mean() >= 0.9995
+    # a hit labelled differently stopped a little elsewhere on the same surface (within eps of it, further along a
+    # grazing ray) where a round or chamfer passes from one part to the other: the oracle's own label at the engine's
+    # hit point must then be the engine's
+    moved = (gstate == ostate) & (gstate == 0) & (gprim != oprim)
+    d, _, lab = oracle.scene_at(made['scene'], gpos[moved, :3])
+    say('agreement', same.mean(), 'relabelled hits', moved.sum(), 'labels at the engine hit', np.mean(lab == gprim[moved]))
+    assert np.all(lab == gprim[moved]) and np.all(np.abs(d) < 4 * u.epsAuto)
+    assert moved.mean() < 0.005
+    assert (same | moved).mean() >= 0.9995
     hit = (gstate == 0) & same
     assert np.quantile(np.linalg.norm(gpos[hit, :3] - opos[hit, :3], axis=1), 0.999) < 1e-4
     made, u = load(engine, 'joins', width=80, height=32)
@@ -516,6 +526,20 @@ def test_node_addon_drives_the_engine(tmp_path):
     assert np.array_equal(exr.astype(np.float32), pfm)
     Y = pfm.astype(np.float64) @ np.array([0.2126, 0.7152, 0.0722])
     assert abs(Y[-16:].mean() / 800 - 1) < 2e-3
+    # the compile ran on a worker thread: the event loop's 10 ms timer kept ticking through it; keepPicture kept the 16
+    # samples; a step during a build was refused
+    say('compile %.0f ms, timer ticks %d, samples kept %d, refused: %s' % (out['compileMs'], out['ticks'], out['kept'], out['refused']))
+    assert out['compileMs'] > 50 and out['ticks'] >= 0.5 * out['compileMs'] / 10, out
+    assert out['kept'] == 16
+    assert out['refused'] and 'during a scene build' in out['refused']
+    # set() took the reference's lighting settings and returned only the unknown name; the sun alone at 20 degrees on
+    # the bench's floor reads rho E_sun sin 20 / pi, as the fixture's row does (tests.js photometry)
+    import photometry_rows as P
+    assert out['unknown'] == ['noSuchSetting']
+    ref, rse = P.reference('photometry', 'sun only, elevation 20')
+    ratio = out['sunOnly']['mean'] / out['sunOnly']['want']
+    say('sun only at 20 degrees: ratio %.5f, reference %.5f +- %.5f' % (ratio, ref, rse))
+    assert abs(ratio - ref) < 4 * rse + 0.01
 
 
 # ---------------------------------------------------------------------------------------------------- SPPM
