@@ -187,6 +187,19 @@ int pt_optics_trace(int device, const PtOpticsScene* scenes, uint32_t n, PtOptic
                     double* ms);
 const char* pt_optics_error(void);
 
+/* The same as pt_optics_trace -- the same kernels, the same numbers -- with no allocation of any kind: every device
+ * buffer lives in one array of PT_OPT_STATIC_BYTES that the library holds on each GPU it runs on until the process
+ * ends (the last call's contents stay in it; what a call sums into it clears first). No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Limits: at most PT_OPT_STATIC_SCENES scenes a call, and their photons rounded up to chunks of 4096
+ * at most PT_OPT_STATIC_CHUNKS chunks in all (2^27 photons); beyond them PT_ERR_ARG with a message, as for any bad
+ * argument. No timings (ms): events would be an allocation. Calls on one GPU share its array: this library serialises
+ * its own callers in one process; overlapping calls through any other route race. */
+#define PT_OPT_STATIC_SCENES 8u
+#define PT_OPT_STATIC_CHUNKS 32768u
+#define PT_OPT_STATIC_BYTES (11u << 20)
+int pt_optics_trace_static(int device, const PtOpticsScene* scenes, uint32_t n, PtOpticsResult* out, double* far, double* lux);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,6 +18,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -502,11 +503,89 @@ int ofail(const char* fmt, ...) {
     return PT_ERR_ARG;
 }
 
+// ---- the optics bench without allocation (pt_optics_trace_static): every device buffer pt_optics_trace allocates,
+// laid out in one array of device memory the library holds for the process (each GPU its own copy), for callers that
+// cannot allocate. Offsets 256-byte aligned; sizes for PT_OPT_STATIC_SCENES scenes and PT_OPT_STATIC_CHUNKS chunks.
+constexpr size_t alignUp(size_t x) { return (x + 255) & ~size_t(255); }
+constexpr size_t kStSc = 0;
+constexpr size_t kStStart = alignUp(kStSc + PT_OPT_STATIC_SCENES * sizeof(PtOpticsScene));
+constexpr size_t kStNext = alignUp(kStStart + (PT_OPT_STATIC_SCENES + 1) * 8);
+constexpr size_t kStFar = alignUp(kStNext + 4);
+constexpr size_t kStLux = alignUp(kStFar + size_t(PT_OPT_STATIC_SCENES) * PT_OPT_NG * PT_OPT_NC * 8);
+constexpr size_t kStPart = alignUp(kStLux + size_t(PT_OPT_STATIC_SCENES) * PT_OPT_LUX * PT_OPT_LUX * 8);
+constexpr size_t kStRes = alignUp(kStPart + size_t(PT_OPT_STATIC_CHUNKS) * PT_OPT_TOT * 8);
+constexpr size_t kStBytes = alignUp(kStRes + PT_OPT_STATIC_SCENES * sizeof(PtOpticsResult));
+static_assert(kStBytes <= PT_OPT_STATIC_BYTES, "the static optics buffer outgrew its stated size");
+
+}  // namespace
+
+// the array itself (host code sees only a stand-in of its name; its device address comes from hipGetSymbolAddress)
+static __device__ __align__(256) unsigned char scratch[PT_OPT_STATIC_BYTES];
+
+namespace {
+// its address on each device, taken once with that device current; calls on one device take the lock, so that two
+// callers in this process never share it at once
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchLock[kMaxDevices];
+unsigned char* g_scratchAt[kMaxDevices] = {};
 }  // namespace
 
 extern "C" {
 
 const char* pt_optics_error(void) { return g_optErr.c_str(); }
+
+int pt_optics_trace_static(int device, const PtOpticsScene* scenes, uint32_t n, PtOpticsResult* out, double* far, double* lux) {
+    if (device < 0 || device >= kMaxDevices) return ofail("device %d not in 0..%d", device, kMaxDevices - 1);
+    if (!scenes || !out || n == 0 || n > PT_OPT_STATIC_SCENES) return ofail("scenes/out NULL or n %u not in 1..%u", n, PT_OPT_STATIC_SCENES);
+    unsigned long long start[PT_OPT_STATIC_SCENES + 1];
+    start[0] = 0;
+    for (uint32_t s = 0; s < n; ++s) {
+        const PtOpticsScene& S = scenes[s];
+        if (S.photons == 0 || S.photons > (1u << 30)) return ofail("scene %u: photons %u not in 1..2^30", s, S.photons);
+        if (!(S.flux > 0.0f) || !(S.ledHalf > 0.0f) || !(S.ior >= 1.0f) || !(S.lensD > 0.0f) || !(S.lensT > 0.0f))
+            return ofail("scene %u: flux, LED size, lens diameter and thickness must be positive, ior >= 1", s);
+        start[s + 1] = start[s] + (S.photons + PT_OPT_CHUNK - 1) / PT_OPT_CHUNK;
+    }
+    if (start[n] > PT_OPT_STATIC_CHUNKS)
+        return ofail("%llu chunks of %u photons over the scenes, more than the %u the static buffer holds", start[n], PT_OPT_CHUNK,
+                     PT_OPT_STATIC_CHUNKS);
+    std::lock_guard<std::mutex> hold(g_scratchLock[device]);
+    hipError_t e;
+#define OCHK(x) do { if ((e = (x)) != hipSuccess) { g_optErr = std::string(#x) + ": " + hipGetErrorString(e); return PT_ERR_HIP; } } while (0)
+    OCHK(hipSetDevice(device));
+    if (!g_scratchAt[device]) {
+        void* p = nullptr;
+        OCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratchAt[device] = static_cast<unsigned char*>(p);
+    }
+    unsigned char* base = g_scratchAt[device];
+    int cus = 256;
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    const size_t farW = (size_t)n * PT_OPT_NG * PT_OPT_NC, luxW = (size_t)n * PT_OPT_LUX * PT_OPT_LUX;
+    const size_t partW = (size_t)start[n] * PT_OPT_TOT;
+    OptDev D = {};
+    D.sc = reinterpret_cast<const PtOpticsScene*>(base + kStSc); D.n = n;
+    D.chunkStart = reinterpret_cast<const unsigned long long*>(base + kStStart);
+    D.next = reinterpret_cast<unsigned int*>(base + kStNext);
+    D.far = reinterpret_cast<double*>(base + kStFar); D.lux = reinterpret_cast<double*>(base + kStLux);
+    D.part = reinterpret_cast<double*>(base + kStPart); D.res = reinterpret_cast<PtOpticsResult*>(base + kStRes);
+    // the last call's tallies are still there: what is summed into starts at zero
+    OCHK(hipMemcpy(base + kStSc, scenes, n * sizeof(PtOpticsScene), hipMemcpyHostToDevice));
+    OCHK(hipMemcpy(base + kStStart, start, (n + 1) * 8, hipMemcpyHostToDevice));
+    OCHK(hipMemsetAsync(base + kStNext, 0, 4, 0));
+    OCHK(hipMemsetAsync(base + kStFar, 0, farW * 8, 0));
+    OCHK(hipMemsetAsync(base + kStLux, 0, luxW * 8, 0));
+    OCHK(hipMemsetAsync(base + kStPart, 0, partW * 8, 0));
+    hipLaunchKernelGGL(pt_optics_walk, dim3(cus * 8), dim3(256), 0, 0, D);
+    OCHK(hipGetLastError());
+    hipLaunchKernelGGL(pt_optics_numbers, dim3(n), dim3(256), 0, 0, D);
+    OCHK(hipGetLastError());
+    OCHK(hipMemcpy(out, base + kStRes, n * sizeof(PtOpticsResult), hipMemcpyDeviceToHost));
+    if (far) OCHK(hipMemcpy(far, base + kStFar, farW * 8, hipMemcpyDeviceToHost));
+    if (lux) OCHK(hipMemcpy(lux, base + kStLux, luxW * 8, hipMemcpyDeviceToHost));
+    return PT_OK;
+#undef OCHK
+}
 
 int pt_optics_trace(int device, const PtOpticsScene* scenes, uint32_t n, PtOpticsResult* out, double* far, double* lux,
                     double* ms) {

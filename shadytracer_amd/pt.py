@@ -140,6 +140,8 @@ SIGNATURES = {
     'pt_optics_trace': (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(PtOpticsScene), ctypes.c_uint32, ctypes.POINTER(PtOpticsResult),
                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     'pt_optics_error': (ctypes.c_char_p, []),
+    'pt_optics_trace_static': (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(PtOpticsScene), ctypes.c_uint32, ctypes.POINTER(PtOpticsResult),
+                                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     'pt_get_tuning': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PtTuning), ctypes.c_size_t]),
     'pt_set_tuning': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PtTuning), ctypes.c_size_t]),
     'pt_autotune': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(PtTuning), ctypes.c_size_t]),
