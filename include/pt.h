@@ -24,7 +24,7 @@ extern "C" {
 #define PT_ERR_COMPILE 3    /* the scene did not compile; the previous scene is still loaded */
 #define PT_ERR_STATE 4      /* the call is not valid now: trace with no scene, or during a scene build */
 
-#define PT_ABI_VERSION 2
+#define PT_ABI_VERSION 3
 
 typedef struct pt_ctx pt_ctx;
 

@@ -63,6 +63,9 @@
 #define PT_S_GRECORDS 18     // path vertices recorded for training
 #define PT_S_FLUSHED 19      // camera paths still going when a sample's rounds ran out, deposited as they stood
 #define PT_S_SHADOWCUT 20    // shadow rays cut short by maxiter, counted as reaching their light (wavefront.js:1640-1645)
+#define PT_S_GINFER 21       // guide network: path vertices run through the network for their mixture (inference)
+#define PT_S_GTILES 22       // guide network: tiles of 64 rows the inference ran (the rows the matrix cores computed)
+#define PT_S_GTRAIN 23       // guide network: training rows through the forward and backward pass (all Adam steps)
 #define PT_S_WORDS 24
 // Rounds of march -> shade a sample runs beyond maxbounce + 1: a hit from behind moves a path on without a bounce,
 // so a path that met some still finishes; one still going after them is flushed (PT_S_FLUSHED)
@@ -244,6 +247,9 @@ typedef struct PtUniforms {
     unsigned int measureMode;   // the reference's measure views (uniforms.schema.js measureMode): 0 or 2 the light as it
                                 // is; 1 the illuminance view -- the camera's first hit a white Lambertian for that
                                 // bounce, so a pixel reads E / pi there, and a camera ray that meets nothing 0
+    unsigned int lt;            // 1: light tracing (uniforms.schema.js lt): sun photons that passed a delta event connect to
+                                // the camera where they first meet a surface or the haze; camera paths leave that light
+                                // (first vertex, then delta events to the sun) to them. Pinhole and orthographic cameras
 } PtUniforms;
 
 

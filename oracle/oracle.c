@@ -1096,9 +1096,66 @@ static double faceFlux(const PtLightFace* f) {
 static double sunFlux(const PtEnv* e) { return (e->sunE > 0 && e->sunRadius > 0) ? e->sunE * PI_D * (double)e->sunRadius * e->sunRadius : 0; }
 static int evaluable(const PtMaterial* m) { return !lobesNone(m); }
 
+/* ---- LIGHT TRACING (core/materials_pbr.js:554-700 ltConnect, camFootprint; core/haze.js:404-466 ltConnectMedium) ----
+ * A sun photon that came through a delta event connects to the camera where it first meets a surface (by the non-delta
+ * lobes, seen from the camera) or the haze (the phase function, the albedo); a camera path whose first vertex left it
+ * by a non-delta lobe leaves the sun it meets through delta events only to these. Pinhole and orthographic cameras,
+ * one ray a pixel, not the illuminance view. */
+static int ltOn(const OrScene* S) {
+    const PtUniforms* u = S->u;
+    OrTiles T = tilesOf(u->width * u->height, u->width, u->height);
+    return u->lt && u->measureMode != 1 && !(u->cam.type == 0 && u->cam.lensR > 0) && T.tilesX == u->width;
+}
+/* The camera's view of P: its pixel, the unit way to the camera and its length, and G over a pixel's area on the image
+ * plane at distance 1 (4 fx fy / (w h)): G = |n.v| dist / z^3 for a pinhole, |n.v| orthographic; in a volume the same
+ * without |n.v|. 0 where the camera does not see P (behind it, off the picture). */
+static int camSees(const OrScene* S, d3 P, d3 nrm, int surface, uint32_t* pix, d3* vv, double* dist, double* g) {
+    const PtUniforms* u = S->u;
+    const PtCamera* c = &u->cam;
+    d3 right = f3(c->right), up = f3(c->up), fwd = f3(c->fwd), rel = sub(P, f3(c->pos));
+    double z = -dotp(rel, fwd), x, y, G;
+    if (!(z > 1e-4)) return 0;
+    if (c->type == 0) {
+        *dist = len(rel);
+        *vv = scl(rel, -1 / *dist);
+        x = dotp(rel, right) / (z * c->fx);
+        y = dotp(rel, up) / (z * c->fy);
+        G = (surface ? fabs(dotp(nrm, *vv)) : 1) * *dist / (z * z * z);
+    } else {
+        *dist = z;
+        *vv = fwd;
+        x = dotp(rel, right) / c->fx;
+        y = dotp(rel, up) / c->fy;
+        G = surface ? fabs(dotp(nrm, fwd)) : 1;
+    }
+    if (!(fabs(x) < 1 && fabs(y) < 1)) return 0;
+    double fw = u->width, fh = u->height;
+    uint32_t px = (uint32_t)clampd((x + 1) * 0.5 * fw, 0, fw - 1), py = (uint32_t)clampd((y + 1) * 0.5 * fh, 0, fh - 1);
+    *pix = px + py * u->width;
+    *g = G * fw * fh / (4.0 * c->fx * c->fy);
+    return 1;
+}
+/* A connection from o towards the camera, stopping two offsets short: its light into its pixel if nothing is in the way,
+ * dimmed by the haze it crosses (the engine's occlusion pass). */
+static void ltDeposit(const OrScene* S, d3 o, d3 w, double l, d3 c, uint32_t pix, double* frame, OrStats* st) {
+    double tmax = l - 2 * offsetTol(S, o);
+    st->shadow++;
+    if (occluded(S, o, w, tmax, &st->shadowSteps)) {
+        if (hazeOn(S)) c = scl(c, hazeTr(&S->u->haze, o, w, tmax));
+        #pragma omp atomic
+        frame[4 * pix + 0] += c.x;
+        #pragma omp atomic
+        frame[4 * pix + 1] += c.y;
+        #pragma omp atomic
+        frame[4 * pix + 2] += c.z;
+    }
+}
+
 /* One photon: drawn from the lights by flux (the sun's beam through its aperture, the light faces), marched and
- * scattered through delta events only, stored where it lands on an evaluable surface after one at least. */
-static void tracePhoton(const OrScene* S, const uint32_t* seed, uint32_t i, uint32_t N, OrMap* out, OrStats* st) {
+ * scattered through delta events only, stored where it lands on an evaluable surface after one at least (out: the
+ * map, NULL without one); with light tracing a sun photon's light there, or where it scatters in the haze after a
+ * delta event, goes to the camera (frame). */
+static void tracePhoton(const OrScene* S, const uint32_t* seed, uint32_t i, uint32_t N, OrMap* out, OrStats* st, double* frame) {
     const PtUniforms* u = S->u;
     const PtEnv* env = &u->env;
     double total = sunFlux(env), fsun = total;
@@ -1108,7 +1165,8 @@ static void tracePhoton(const OrScene* S, const uint32_t* seed, uint32_t i, uint
     double u0[4], u1[4]; rng4(&r, u0); rng4(&r, u1);
     double pick = u0[0] * total, each = total / N;
     d3 o, d, beta;
-    if (pick < fsun) {
+    const int sunPh = pick < fsun, lt = ltOn(S);
+    if (sunPh) {
         d3 a = f3(env->sunDir), t, b; basis(a, &t, &b);
         double rr = env->sunRadius * sqrt(u0[1]), ph = 2 * PI_D * u0[2];
         o = add(f3(env->sunPoint), add(scl(t, rr * cos(ph)), scl(b, rr * sin(ph))));
@@ -1137,7 +1195,16 @@ static void tracePhoton(const OrScene* S, const uint32_t* seed, uint32_t i, uint
     for (;;) {
         int state = march(S, &p, dir, n, &ltot, &steps, collide);
         st->photonSegs++;
-        if (state != ST_ACTIVE) return;       /* escaped, out of steps, or collided in the haze */
+        if (state != ST_ACTIVE) {             /* escaped, out of steps, or collided in the haze */
+            if (state == ST_SCATTER && lt && hadDelta && sunPh) {
+                uint32_t cp; d3 vv; double dist, g;
+                if (camSees(S, p, v(0, 0, 0), 0, &cp, &vv, &dist, &g)) {
+                    d3 c = scl(beta, u->haze.albedo * hg(u->haze.g, dotp(dir, vv)) * g);
+                    if (max3d(c) > 0) ltDeposit(S, p, vv, dist, c, cp, frame, st);
+                }
+            }
+            return;
+        }
         OrRng rng = rngAt(seed, i, bounces, (0x5AADEu ^ ((steps & 0xFFFFu) << 12)) ^ 0x80000000u);
         OrHit h = sceneAt(S, p);
         d3 ns = h.n;
@@ -1163,7 +1230,22 @@ static void tracePhoton(const OrScene* S, const uint32_t* seed, uint32_t i, uint
             n1 = or_cauchy(n1, m->abbe, lambda);
         }
         OrLobes L = makeLobes(nrm, neg(dir), m, n0 / n1);
-        if (evaluable(m) && hadDelta) { mapPush(out, p, beta, neg(dir), nrm); st->stored++; }
+        if (evaluable(m) && hadDelta) {
+            if (out) { mapPush(out, p, beta, neg(dir), nrm); st->stored++; }
+            if (lt && sunPh && fabs(n0 - 1) < 1e-3) {
+                uint32_t cp; d3 vv; double dist, g;
+                if (camSees(S, p, nrm, 1, &cp, &vv, &dist, &g) && dotp(nrm, vv) > 0) {
+                    OrLobes Lc = makeLobes(nrm, vv, m, n0 / n1);
+                    d3 c = scl(mul(beta, lobesEval(&Lc, neg(dir))), g);
+                    if (max3d(c) > 0) {
+                        d3 o2 = add(p, scl(nrm, 2 * offsetTol(S, p)));
+                        d3 toCam = u->cam.type == 0 ? sub(f3(u->cam.pos), o2) : scl(vv, dist);
+                        double l = len(toCam);
+                        ltDeposit(S, o2, scl(toCam, 1 / l), l, c, cp, frame, st);
+                    }
+                }
+            }
+        }
         double us[4]; rng4(&rng, us);
         OrSample s = lobesSample(&L, us);
         if (!s.delta) return;                 /* the map holds L S+ D paths: a photon goes on through delta lobes only */
@@ -1209,8 +1291,10 @@ static void tracePath(const OrScene* S, const OrTiles* T, const uint32_t* seed, 
                       double* frame, OrStats* st) {
     const PtUniforms* u = S->u;
     /* SPPM bookkeeping: deltaOnly (every event so far delta), caustic (a gather was left by a non-delta lobe),
-     * sdelta (every event since it delta): light met with caustic && sdelta is the map's */
-    int deltaOnly = 1, caustic = 0, sdelta = 0;
+     * sdelta (every event since it delta): light met with caustic && sdelta is the map's. With light tracing, ltcls
+     * (the first vertex was left by a non-delta lobe): the sun met with ltcls && sdelta is the photons' connections' */
+    int deltaOnly = 1, caustic = 0, sdelta = 0, ltcls = 0;
+    const int lt = ltOn(S);
     const PtEnv* env = &u->env;
     d3 p, dir; uint32_t pix; double uh;
     cameraRay(S, T, seed, ray, &p, &dir, &pix, &uh);
@@ -1225,7 +1309,7 @@ static void tracePath(const OrScene* S, const OrTiles* T, const uint32_t* seed, 
         st->steps += steps - s0;
         st->segments++;
         if (state == ST_ESCAPED) {
-            int mapped = caustic && sdelta && sunFlux(env) > 0;
+            int mapped = (caustic || ltcls) && sdelta && sunFlux(env) > 0;
             /* the illuminance view (uniforms.schema.js measureMode 1): a camera ray that meets nothing shows 0 */
             if (!(u->measureMode == 1 && bounces == 0))
                 rad = add(rad, scl(mul(beta, add(envBg(env, dir), mapped ? v(0, 0, 0) : envSun(env, p, dir))), sunmis));
@@ -1280,7 +1364,8 @@ static void tracePath(const OrScene* S, const OrTiles* T, const uint32_t* seed, 
             prevPdf = u->nee ? ph : -1;
             vtxLtot = ltot;
             dir = nd;
-            deltaOnly = caustic = sdelta = 0;
+            deltaOnly = caustic = sdelta = ltcls = 0;
+            if (lt && bounces == 1) ltcls = 1;   /* the first vertex in the haze */
             if (bounces > u->maxbounce) break;
             collide = ltot + hazeDraw(hz, p, dir, u1[2]) * n;
             continue;
@@ -1334,7 +1419,8 @@ static void tracePath(const OrScene* S, const OrTiles* T, const uint32_t* seed, 
         OrLobes L = makeLobes(nrm, neg(dir), m, n0 / n1);
         double fs = faceShare(S, bounces);
         int gathered = 0;
-        if (map && (u->sppm >= 2 || deltaOnly) && evaluable(m)) {   /* sppm 2: at every non-delta camera vertex */
+        /* sppm 2: at every non-delta camera vertex; with light tracing not at the first (the connections') */
+        if (map && (u->sppm >= 2 || deltaOnly) && evaluable(m) && !(lt && bounces == 1)) {
             rad = add(rad, mul(beta, mapGather(map, &L, p, nrm)));
             gathered = 1;
             st->gathers++;
@@ -1398,8 +1484,14 @@ static void tracePath(const OrScene* S, const OrTiles* T, const uint32_t* seed, 
         vtxLtot = ltot;
         if (s.eval) steps += 0x10000u;
         if (!s.delta) deltaOnly = 0;
-        if (gathered) { if (!s.delta) { caustic = 1; sdelta = 0; } }
-        else if (caustic) { if (s.delta) sdelta = 1; else caustic = sdelta = 0; }
+        /* the classes kept through delta events, closed at any other; opened where a non-delta lobe leaves the vertex on
+         * its own side: after a gather (the map's), at the first vertex with light tracing (the connections') */
+        if (s.delta) { if (caustic || ltcls) sdelta = 1; }
+        else caustic = ltcls = sdelta = 0;
+        if (!s.delta && !s.transmitted) {
+            if (gathered) caustic = 1;
+            else if (lt && bounces == 1 && evaluable(m)) ltcls = 1;
+        }
         if (bounces > u->maxbounce) terminate = 1;
         if (!terminate && u->diffuseCap && (steps >> 16) > u->diffuseCap) terminate = 1;
         if (!terminate && u->rr && bounces >= u->rrStart) {
@@ -1459,8 +1551,9 @@ int or_render(const int* ops, int nops, const float* U, const PtMaterial* mats, 
     for (uint32_t s = 0; s < samples; ++s) {
         uint32_t seed[4]; sampleSeed(u->seed, first + s, seed);
         memset(frame, 0, sizeof(double) * npx * 4);
-        if (u->sppm) {
-            /* the sample's photon pass: per-thread maps merged, then sorted by cell */
+        if (u->sppm || u->lt) {
+            /* the sample's photon pass: per-thread maps merged, then sorted by cell; light tracing's connections into
+             * the frame */
             map.n = 0;
             map.r = or_sppm_radius(u->sppmRadius, u->sppmAlpha, first + s + 1);
             uint32_t NP = u->sppmPhotons;
@@ -1469,14 +1562,14 @@ int or_render(const int* ops, int nops, const float* U, const PtMaterial* mats, 
                 OrStats st; memset(&st, 0, sizeof st);
                 OrMap mine; memset(&mine, 0, sizeof mine); mine.r = map.r;
                 #pragma omp for schedule(dynamic, 256)
-                for (uint32_t i = 0; i < NP; ++i) tracePhoton(&S, seed, i, NP, &mine, &st);
+                for (uint32_t i = 0; i < NP; ++i) tracePhoton(&S, seed, i, NP, u->sppm ? &mine : NULL, &st, frame);
                 #pragma omp critical
                 {
                     for (size_t q = 0; q < mine.n; ++q) {
                         if (map.n == map.cap) { map.cap = map.cap ? 2 * map.cap : 4096; map.rec = (OrPhoton*)realloc(map.rec, map.cap * sizeof(OrPhoton)); }
                         map.rec[map.n++] = mine.rec[q];
                     }
-                    tot.photonSegs += st.photonSegs; tot.stored += st.stored;
+                    tot.photonSegs += st.photonSegs; tot.stored += st.stored; tot.shadow += st.shadow; tot.shadowSteps += st.shadowSteps;
                 }
                 free(mine.rec);
             }

@@ -306,6 +306,7 @@ extern "C" __global__ __launch_bounds__(64 * PT_G_WAVES) void pt_guide_infer(con
     __bf16* tile = tiles[wave];
     float* out = reinterpret_cast<float*>(tile);
     const unsigned int ntiles = (count + 63u) / 64u;
+    unsigned long long nOn = 0ull, nTiles = 0ull;
     for (unsigned int t = blockIdx.x * PT_G_WAVES + wave; t < ntiles; t += gridDim.x * PT_G_WAVES) {
         const unsigned int i = t * 64u + lane;
         bool on = false;
@@ -328,7 +329,12 @@ extern "C" __global__ __launch_bounds__(64 * PT_G_WAVES) void pt_guide_infer(con
         if (!on) guideZeroRow(tile, (int)lane);
         guideForward(s, tile, out);
         if (on) guideStoreMix(D, slot, guideMixOf(out + lane * PT_G_OUT));
+        nOn += on ? 1ull : 0ull;
+        nTiles += lane == 0u ? 1ull : 0ull;
     }
+    // the loop's bound is the wave's own, so every lane of the wave arrives here together
+    statAdd(D, PT_S_GINFER, nOn);
+    statAdd(D, PT_S_GTILES, nTiles);
 }
 
 // The forward pass on given vertices (the C ABI's evaluation for tests and tools): raw rows of 10 floats (position,
@@ -588,6 +594,10 @@ extern "C" __global__ __launch_bounds__(64) void pt_guide_grad(const PtDev* __re
     }
     lossPart = waveSum64f(lossPart);
     if (lane == 0 && lossPart != 0.0f) atomicAdd(&D->gLoss[step], (double)lossPart);
+    // the rows this block ran: its tiles' in-range rows
+    unsigned long long rows = 0ull;
+    for (unsigned int t = blockIdx.x; t * 64u < n; t += gridDim.x) rows += (t * 64u + lane < n) ? 1ull : 0ull;
+    statAdd(D, PT_S_GTRAIN, rows);
 }
 
 // ADAM (beta1 0.9, beta2 0.99, eps 1e-8) on the FP32 masters from the summed partial rows, the gradient cleared, the

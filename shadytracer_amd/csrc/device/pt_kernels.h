@@ -133,6 +133,9 @@ __device__ __forceinline__ float maskMargin(const PtDev* __restrict__ D) {
 #define PT_F_CAUSTIC 0x10u      // a camera path that gathered: light reaching its gather vertex through delta
                                 // events only is the photon map's
 #define PT_F_SDELTA 0x20u       // ... and it has scattered by a delta event since that vertex
+#define PT_F_SUNPH 0x80u        // a photon from the sun (light tracing connects these)
+#define PT_F_LTCLS 0x100u       // a camera path whose first vertex left it by a non-delta lobe: with SDELTA, the light it
+                                // meets through delta events only is the light tracer's
 #define PT_F_GPEND 0x40u        // a recorded guide vertex waits for its S (the radiance so far, at the next event)
 #define PT_HR_INSIDE 0x100u     // hit record: the ray is inside the surface's object
 #define PT_HR_BEYOND 0x200u     // hit record: the medium beyond is the material in bits 16..

@@ -368,6 +368,60 @@ __device__ __forceinline__ V3 sppmGather(const PtDev* __restrict__ D, const Lobe
     return sum * (1.0f / (PT_PI * r2));
 }
 
+// LIGHT TRACING (core/materials_pbr.js:554-700 ltConnect and camFootprint; core/haze.js:404-466 ltConnectMedium): a
+// sun photon that came through a delta event (glass, a mirror) asks, where it first meets a surface or the haze, whether
+// the camera sees that point, and adds its light to the pixel that sees it. The camera's paths leave exactly that light
+// to the photons: a path whose first vertex left it by a non-delta lobe and then met the sun through delta events only
+// does not count the sun. Each path is counted by one side. Off for the thin lens and the illuminance view, and where
+// the rays are not one a pixel.
+__device__ __forceinline__ bool ltOn(const PtDev* __restrict__ D) {
+    return D->u.lt != 0u && D->u.measureMode != 1u && !(D->u.cam.type == 0u && D->u.cam.lensR > 0.0f) &&
+           D->tilesX == D->u.width && D->nrays == D->u.width * D->u.height;
+}
+// The camera's view of a point P: false where the camera cannot see it (behind it, off the picture); else the pixel,
+// the way to the camera v and its length, and what one photon's light there is multiplied by -- the density, per unit
+// area at P (surface: |n.v|) or per unit volume (in the haze: no normal), of the pixel's camera rays landing there,
+// over the pixel's area on the image plane at distance 1. A pinhole: G = |n.v| dist / z^3 (a plane's area at distance
+// 1 to solid angle, cos^3, to area at P, dist^2 / |n.v|); orthographic: G = |n.v|. A pixel is 4 fx fy / (w h) of the
+// plane. The pixel is the one genrays' mapping sends a ray through P from.
+__device__ __forceinline__ bool camSees(const PtDev* __restrict__ D, V3 P, V3 nrm, bool surface, unsigned int& pix, V3& v,
+                                        float& dist, float& g) {
+    const PtCamera& cam = D->u.cam;
+    const V3 right = ld3(cam.right), up = ld3(cam.up), fwd = ld3(cam.fwd);
+    const V3 rel = P - ld3(cam.pos);
+    const float z = -dot(rel, fwd);                       // depth in front of the camera
+    if (!(z > 1e-4f)) return false;
+    float x, y, G;
+    if (cam.type == 0u) {
+        dist = length(rel);
+        v = rel * (-1.0f / dist);
+        x = dot(rel, right) / (z * cam.fx);
+        y = dot(rel, up) / (z * cam.fy);
+        G = (surface ? fabsf(dot(nrm, v)) : 1.0f) * dist / (z * z * z);
+    } else {
+        dist = z;
+        v = fwd;
+        x = dot(rel, right) / cam.fx;
+        y = dot(rel, up) / cam.fy;
+        G = surface ? fabsf(dot(nrm, fwd)) : 1.0f;
+    }
+    if (!(fabsf(x) < 1.0f && fabsf(y) < 1.0f)) return false;
+    const float fw = (float)D->u.width, fh = (float)D->u.height;
+    const unsigned int px = (unsigned int)fminf(fmaxf((x + 1.0f) * 0.5f * fw, 0.0f), fw - 1.0f);
+    const unsigned int py = (unsigned int)fminf(fmaxf((y + 1.0f) * 0.5f * fh, 0.0f), fh - 1.0f);
+    pix = px + py * D->u.width;
+    g = G * fw * fh / (4.0f * cam.fx * cam.fy);
+    return true;
+}
+// The connection, queued as the slot's shadow ray: from o towards the camera, stopping two offsets short of it; what it
+// carries goes to its pixel if nothing is in the way (pt_occlude). shDir.w < 0 marks it a connection: no path's.
+__device__ __forceinline__ void ltQueue(const PtDev* __restrict__ D, unsigned int slot, V3 o, V3 v, float len, V3 c,
+                                        unsigned int pix) {
+    st4(D->shOrig, slot, o.x, o.y, o.z, len - 2.0f * offsetTol(D, o));
+    st4(D->shDir, slot, v.x, v.y, v.z, -1.0f);
+    st4(D->shRad, slot, c.x, c.y, c.z, __uint_as_float(pix));
+}
+
 __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const PtMaterial* __restrict__ smat,
                                              const unsigned int* seed, unsigned int slot) {
     ShadeOut out; out.requeue = false; out.shadow = false; out.store = false;
@@ -397,20 +451,34 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
     Rng rng = rngAt(seed, slot, bounces, (0x5AADEu ^ ((counts & 0xFFFFu) << 12)) ^ (photon ? 0x80000000u : 0u));
     bool terminate = false;
     bool direct = false;
-    // a gathering path's light that reached its gather vertex through delta events only is the photon map's
+    // a gathering path's light that reached its gather vertex through delta events only is the photon map's; the sun's
+    // that reached the camera's first vertex so is the light tracer's
     const bool mapped = (flags & (PT_F_CAUSTIC | PT_F_SDELTA)) == (PT_F_CAUSTIC | PT_F_SDELTA);
+    const bool traced = (flags & (PT_F_LTCLS | PT_F_SDELTA)) == (PT_F_LTCLS | PT_F_SDELTA);
+    const bool lt = ltOn(D);
 
     if (state == PT_ESCAPED) {
         // the illuminance view: a camera ray that met nothing has no surface to measure on, and shows 0
         if (!photon && !(D->u.measureMode == 1u && bounces == 0u)) {
             // the sun through its aperture emits photons, so a mapped path leaves its light to the map; the sky
             // emits none and stays path traced
-            V3 Le = envBackground(env, dir) + ((mapped && sunFlux(env) > 0.0f) ? v3s(0.0f) : envSun(env, p, dir));
+            V3 Le = envBackground(env, dir) + (((mapped || traced) && sunFlux(env) > 0.0f) ? v3s(0.0f) : envSun(env, p, dir));
             rad += beta * Le * sunmis;
         }
         direct = bounces == 0u;
         terminate = true;
-    } else if (state == PT_MAXITER || (state == PT_SCATTER && photon)) {
+    } else if (state == PT_MAXITER) {
+        terminate = true;
+    } else if (state == PT_SCATTER && photon) {
+        // a photon ends where it scatters in the haze; a sun photon that came through a delta event sends its light
+        // there to the camera through the phase function, the albedo its collision's weight (haze.js ltConnectMedium)
+        if (lt && (flags & (PT_F_HADDELTA | PT_F_SUNPH)) == (PT_F_HADDELTA | PT_F_SUNPH)) {
+            unsigned int cp; V3 v; float dist, g;
+            if (camSees(D, p, v3s(0.0f), false, cp, v, dist, g)) {
+                const V3 c = beta * (D->u.haze.albedo * hgEval(D->u.haze.g, dot(dir, v)) * g);
+                if (max3(c) > 0.0f) { ltQueue(D, slot, p, v, dist, c, cp); out.shadow = true; }
+            }
+        }
         terminate = true;
     } else if (state == PT_SCATTER) {
         // A POINT IN THE HAZE (core/haze.js:292-600 mediumVertex): albedo, light sampling with the phase function,
@@ -463,7 +531,9 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
         prevPdf = D->u.nee != 0u ? ph : -1.0f;
         vtxLtot = ltot;
         dir = nd;
-        flags &= ~(PT_F_DELTAONLY | PT_F_CAUSTIC | PT_F_SDELTA);
+        flags &= ~(PT_F_DELTAONLY | PT_F_CAUSTIC | PT_F_SDELTA | PT_F_LTCLS);
+        // the camera's first vertex in the haze: the sun it meets past here through delta events is the light tracer's
+        if (lt && bounces == 1u) flags |= PT_F_LTCLS;
         if (bounces > D->u.maxbounce) terminate = true;
         else {
             collide = ltot + hazeDraw(hz, p, dir, u1.z) * n;
@@ -546,13 +616,33 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
                     // a caustic photon lands where the material has a non-delta lobe: stored (the store itself is
                     // wave-aggregated by the caller), on one of the cell's sub-lists drawn uniformly; it goes on only
                     // through a delta lobe (below)
-                    out.store = true;
-                    out.sp = p; out.spow = beta; out.sl = -dir; out.sn = nrm;
-                    out.ssub = min((unsigned int)(ud.z * (float)PT_SPPM_SUBS), PT_SPPM_SUBS - 1u);
-                } else if (!photon && D->u.sppm != 0u && (D->u.sppm >= 2u || (flags & PT_F_DELTAONLY)) && evaluable) {
+                    if (D->u.sppm != 0u) {
+                        out.store = true;
+                        out.sp = p; out.spow = beta; out.sl = -dir; out.sn = nrm;
+                        out.ssub = min((unsigned int)(ud.z * (float)PT_SPPM_SUBS), PT_SPPM_SUBS - 1u);
+                    }
+                    // ...and a sun photon's light from here to the camera, by the non-delta lobes the camera's side
+                    // leaves to it, seen from where the camera is (ltConnect); from air only
+                    if (lt && (flags & PT_F_SUNPH) && fabsf(n0 - 1.0f) < 1e-3f) {
+                        unsigned int cp; V3 v; float dist, g;
+                        if (camSees(D, p, nrm, true, cp, v, dist, g) && dot(nrm, v) > 0.0f) {
+                            const Lobes lc = makeLobes(nrm, v, m, n0 / n1, D->mf);
+                            const V3 c = beta * lobesEval(lc, D->mf, -dir) * g;
+                            if (max3(c) > 0.0f) {
+                                const V3 o = p + nrm * (2.0f * offsetTol(D, p));
+                                const V3 toCam = D->u.cam.type == 0u ? ld3(D->u.cam.pos) - o : v * dist;
+                                const float len = length(toCam);
+                                ltQueue(D, slot, o, toCam * (1.0f / len), len, c, cp);
+                                out.shadow = true;
+                            }
+                        }
+                    }
+                } else if (!photon && D->u.sppm != 0u && (D->u.sppm >= 2u || (flags & PT_F_DELTAONLY)) && evaluable &&
+                           !(lt && bounces == 1u)) {
                     // THE GATHER (core/sppm.js:487-494): the camera path's first non-delta vertex after delta events
                     // (sppm 1, the reference's); with sppm 2 every non-delta camera vertex, so that caustic light a
-                    // path meets after a diffuse bounce is the map's too, not a rare BSDF hit on the sun through glass
+                    // path meets after a diffuse bounce is the map's too, not a rare BSDF hit on the sun through glass.
+                    // With light tracing the first vertex is the photons' connections' instead
                     bool cut = false;
                     rad += beta * sppmGather(D, lb, p, nrm, cut);
                     gathered = true;
@@ -658,11 +748,17 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
                 if (s.ev == PT_EV_EVAL) counts += 0x10000u;
                 if (!s.isDelta) flags &= ~PT_F_DELTAONLY;
                 if (photon && s.isDelta) flags |= PT_F_HADDELTA;
-                // after a gather left by a non-delta lobe, light reaching the path through delta events only is
-                // the map's (it gathered it); left by a delta lobe, the path goes on as before the gather
-                if (gathered) { if (!s.isDelta) { flags |= PT_F_CAUSTIC; flags &= ~PT_F_SDELTA; } }
-                else if (flags & PT_F_CAUSTIC) {
-                    if (s.isDelta) flags |= PT_F_SDELTA; else flags &= ~(PT_F_CAUSTIC | PT_F_SDELTA);
+                // THE CLASSES ANOTHER ESTIMATOR COUNTS: kept through delta events (SDELTA), closed at any other; opened
+                // where a non-delta lobe leaves a vertex on its own side -- after a gather, the light met through
+                // delta events only is the map's (it gathered it); at the camera's first vertex with light tracing,
+                // the sun's so met is the photons'. Light that came through a rough transmission is neither's (the
+                // map and the connections read the side the vertex is seen from), and a delta lobe leaves the path
+                // as it was.
+                if (s.isDelta) { if (flags & (PT_F_CAUSTIC | PT_F_LTCLS)) flags |= PT_F_SDELTA; }
+                else flags &= ~(PT_F_CAUSTIC | PT_F_LTCLS | PT_F_SDELTA);
+                if (!photon && !s.isDelta && !s.transmitted) {
+                    if (gathered) flags |= PT_F_CAUSTIC;
+                    else if (lt && bounces == 1u && evaluable) flags |= PT_F_LTCLS;
                 }
                 if (bounces > D->u.maxbounce) terminate = true;
                 if (!terminate && D->u.diffuseCap != 0u && (counts >> 16) > D->u.diffuseCap) terminate = true;
@@ -821,7 +917,7 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_occlude(const PtDev* _
     ParamPtr U = (const PT_CONST float*)D->params;
     int slot = -1;
     V3 p = v3s(0.0f), dir = v3s(0.0f), o = v3s(0.0f);
-    float tmax = 0.0f, trav = 0.0f;
+    float tmax = 0.0f, trav = 0.0f, dir4w = 0.0f;   // dir4w < 0: a light tracer's connection
     float om = 1.0f, rPrev = 0.0f, sPrev = 0.0f;    // over-relaxation, as the march's
     unsigned long long mask = ~0ull;
     unsigned int steps = 0u;
@@ -836,6 +932,7 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_occlude(const PtDev* _
                 float4 Oo = ld4(D->shOrig, slot), Dd = ld4(D->shDir, slot);
                 o = v3(Oo.x, Oo.y, Oo.z); p = o; tmax = Oo.w;
                 dir = v3(Dd.x, Dd.y, Dd.z);
+                dir4w = Dd.w;
                 trav = 0.0f; steps = 0u;
                 mask = ldu64(D->maskBuf, slot);
                 nsegs += 1ull;
@@ -880,7 +977,7 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_occlude(const PtDev* _
                     if (hazeOn(D)) c *= hazeTr(D->u.haze, o, dir, tmax < 3.0e38f ? tmax : 1.0e30f);
                     depositRGB(D, __float_as_uint(Rr.w), c, false);
                     unsigned int gr;
-                    if (guideRecorded(D, (unsigned int)slot, gr)) {
+                    if (!(dir4w < 0.0f) && guideRecorded(D, (unsigned int)slot, gr)) {
                         // the path's light-sampling total, for its vertices' training radiance
                         float4* nee = reinterpret_cast<float4*>(D->gPath) + 2 * gr + 1;
                         *nee = make_float4(nee->x + c.x, nee->y + c.y, nee->z + c.z, 0.0f);
@@ -1080,7 +1177,8 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_photon_gen(const PtDev
     float each = total / (float)D->nphotons;
     V3 o, d, power;
     float fsun = sunFlux(env);
-    if (pick < fsun) {
+    const bool sunPh = pick < fsun;
+    if (sunPh) {
         V3 a = ld3(env.sunDir), t, b;
         frameOf(a, t, b);
         float rr = env.sunRadius * sqrtf(u0.y), sp, cp;
@@ -1112,7 +1210,7 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_photon_gen(const PtDev
     st4(D->dir, i, d.x, d.y, d.z, 1.0f);
     st4(D->beta, i, power.x, power.y, power.z, -1.0f);
     st4(D->rad, i, 0.0f, 0.0f, 0.0f, 0.0f);
-    st4u(D->meta, i, make_uint4(0u, 0u, mzPack(0u, PT_ACTIVE, PT_F_PHOTON), 0u));
+    st4u(D->meta, i, make_uint4(0u, 0u, mzPack(0u, PT_ACTIVE, PT_F_PHOTON | (sunPh ? PT_F_SUNPH : 0u)), 0u));
     float collide = hazeOn(D) ? hazeDraw(D->u.haze, o, d, u1.w) : 3.0e38f;
     st4(D->aux, i, 1.0f, collide, 0.0f, 0.0f);
     D->qMarch0[i] = i;

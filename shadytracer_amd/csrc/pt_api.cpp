@@ -342,6 +342,7 @@ unsigned int gridFor(const pt_ctx* c, int k) {
 hipError_t enqueueSample(pt_ctx* c, std::vector<hipEvent_t>* events, std::vector<int>* kinds) {
     hipError_t e;
     auto one = [&](int k, unsigned int qsel) -> hipError_t {
+        if (events && kinds->size() * 2 + 1 >= events->size()) return hipErrorInvalidValue;   // more launches than timed
         if (events) { hipEventRecord((*events)[kinds->size() * 2], c->stream); }
         hipError_t r = launch(c, k, gridFor(c, k), qsel);
         if (events) { hipEventRecord((*events)[kinds->size() * 2 + 1], c->stream); kinds->push_back(k); }
@@ -350,13 +351,17 @@ hipError_t enqueueSample(pt_ctx* c, std::vector<hipEvent_t>* events, std::vector
     unsigned int R = rounds(c);
     const bool guiding = c->host.u.guide != 0u && c->guideReady;
     if (c->host.nphotons) {
-        // THE PHOTON PASS: the map cleared and r_k set, photons emitted, marched and shaded round by round (no shadow
-        // rays; occlude runs on an empty queue for its resets of the march queue)
+        // THE PHOTON PASS: the map cleared and r_k set, photons emitted, marched and shaded round by round. Without
+        // light tracing there are no shadow rays, and occlude runs on an empty queue for its resets of the march queue;
+        // with it the photons' connections to the camera are the shadow queue, binned as the camera's are.
+        const bool lt = c->host.u.lt != 0u;
         if ((e = one(K_SPPMCLEAR, 0)) || (e = one(K_PHOTONGEN, 0))) return e;
         for (unsigned int it = 0; it < R; ++it) {
             unsigned int in = it & 1u;
             if ((e = one(K_BINCOUNT, in)) || (e = one(K_BINSCAN, 0)) || (e = one(K_BINSCATTER, in))) return e;
-            if ((e = one(K_MARCH, in)) || (e = one(K_SURFACE, in ^ 1u)) || (e = one(K_SHADE, in ^ 1u)) || (e = one(K_OCCLUDE, in))) return e;
+            if ((e = one(K_MARCH, in)) || (e = one(K_SURFACE, in ^ 1u)) || (e = one(K_SHADE, in ^ 1u))) return e;
+            if (lt && ((e = one(K_BINCOUNT, 2)) || (e = one(K_BINSCAN, 0)) || (e = one(K_BINSCATTER, 2)))) return e;
+            if ((e = one(K_OCCLUDE, in))) return e;
         }
     }
     if ((e = one(K_GENRAYS, 0))) return e;
@@ -383,7 +388,8 @@ hipError_t enqueueSample(pt_ctx* c, std::vector<hipEvent_t>* events, std::vector
 uint64_t graphKey(const pt_ctx* c) {
     // what the captured launches were sized for
     const unsigned int k[] = {rounds(c), c->host.nrays, c->host.nphotons, c->host.gridBlocks,
-                              (c->host.u.guide && c->guideReady) ? 1u : 0u, c->host.g.steps, c->host.gPaths, c->host.gMaxV};
+                              (c->host.u.guide && c->guideReady) ? 1u : 0u, c->host.g.steps, c->host.gPaths, c->host.gMaxV,
+                              c->host.u.lt ? 1u : 0u};
     return fnv1a(k, sizeof k);
 }
 
@@ -772,11 +778,12 @@ int pt_set_uniforms(pt_ctx* c, const PtUniforms* u, size_t size) {
     if (!(std::isfinite(u->cam.lensR) && u->cam.lensR >= 0.0f)) return fail(c, PT_ERR_ARG, "camera lensR %g must be finite and >= 0", u->cam.lensR);
     if (u->cam.lensR > 0.0f && !(std::isfinite(u->cam.focus) && u->cam.focus > 0.0f))
         return fail(c, PT_ERR_ARG, "camera focus %g must be > 0 with a lens (lensR %g)", u->cam.focus, u->cam.lensR);
-    if (u->sppm && (u->sppmPhotons == 0 || u->sppmPhotons > (1u << 26)))
-        return fail(c, PT_ERR_ARG, "sppmPhotons %u not in 1..2^26", u->sppmPhotons);
+    if (u->lt > 1u) return fail(c, PT_ERR_ARG, "lt %u not 0 or 1", u->lt);
+    if ((u->sppm || u->lt) && (u->sppmPhotons == 0 || u->sppmPhotons > (1u << 26)))
+        return fail(c, PT_ERR_ARG, "sppmPhotons %u not in 1..2^26 (the photons a sample of the map and of light tracing)", u->sppmPhotons);
     if (u->sppm && !(u->sppmRadius > 0.0f && u->sppmAlpha > 0.0f && u->sppmAlpha < 1.0f))
         return fail(c, PT_ERR_ARG, "sppmRadius %g must be > 0 and sppmAlpha %g in (0, 1)", u->sppmRadius, u->sppmAlpha);
-    unsigned int photons = u->sppm ? u->sppmPhotons : 0u;
+    unsigned int photons = (u->sppm || u->lt) ? u->sppmPhotons : 0u;
     int rc;
     if ((rc = ensureBuffers(c, u->width, u->height, photons))) return rc;
     bool resized = u->width != c->host.u.width || u->height != c->host.u.height;
@@ -933,7 +940,9 @@ int pt_time_kernels(pt_ctx* c, uint32_t samples, double* ms, size_t n) {
     int rc;
     if ((rc = syncDev(c))) return rc;
     for (size_t k = 0; k < n; ++k) ms[k] = 0.0;
-    size_t launches = 6 + 18 * size_t(rounds(c)) + 3 * size_t(kGuideMaxSteps);
+    // a sample's launches at most: the photon pass 2 + 10 a round (with light tracing's binning of its connections),
+    // the camera's 1 + 11 a round (guide inference included), flush and resolve, collect and 3 a training step
+    size_t launches = 6 + 21 * size_t(rounds(c)) + 3 * size_t(kGuideMaxSteps);
     std::vector<hipEvent_t> ev(2 * launches);
     for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
     for (uint32_t s = 0; s < samples; ++s) {

@@ -14,7 +14,7 @@
  *                ceiling panel of 5000 cd/m^2; 1920 x 1080, 256 spp, depth 8, NEE on.
  *   caustic-c3   the room with a 2 x 2 x 2 m haze box (sigma_s 0.05 /m, g 0.6, albedo 1), a dispersive glass sphere
  *                (r 0.4 m, n = 1.5 + 0.0042 / lambda^2 um) and a 100 klx sun through a 0.3 m aperture in the ceiling;
- *                1280 x 720.
+ *                1280 x 720; the photon map at every camera vertex past the first, light tracing at the first.
  *   sppm-slab    a collimated beam through a glass slab onto a floor: the photon map's closed form.
  *   furnace-*    closed-form checks: a white furnace (albedo-1 Lambertian plane under a uniform sky, and a haze box
  *                under a uniform sphere of sky).
@@ -167,7 +167,8 @@ function causticC3(opts = {}) {
     const d = doc('caustic-c3', mats, [roomChildren(hole), ...clear, glass, hazeBox, panel()], env, 10)
     const w = opts.width || 1280, h = opts.height || 720
     return { doc: d, camera: camera([0, -1.95, 1.6], [0.2, 0.5, 0.5], 70, w, h),
-        settings: { width: w, height: h, spp: 10, maxbounce: 8, nee: 1, sppm: 2, sppmIterations: 10, sppmPhotons: 1 << 22, sppmRadius: 0.02, sppmAlpha: 0.7 } }
+        settings: { width: w, height: h, spp: 10, maxbounce: 8, nee: 1, sppm: 2, sppmIterations: 10, sppmPhotons: 1 << 22, sppmRadius: 0.02, sppmAlpha: 0.7,
+            lt: 1 } }
 }
 
 // A collimated beam through a glass slab onto a floor (the closed form of core/test/sppm.test.mjs's collimated case):
@@ -181,15 +182,21 @@ function causticC3(opts = {}) {
 function sppmSlab(opts = {}) {
     const fr = opts.floorRoughness !== undefined ? opts.floorRoughness : 1
     const fsp = opts.floorSpecular !== undefined ? opts.floorSpecular : 0
-    const mats = [material('floor', [0.5, 0.5, 0.5], fr, 0, { specular: fsp }),
+    const fa = opts.floorAlbedo !== undefined ? opts.floorAlbedo : 0.5
+    const mats = [material('floor', [fa, fa, fa], fr, 0, { specular: fsp }),
         material('glass', [1, 1, 1], 0, 0, { transmission: 1, ior: 1.5, specular: 0.5 })]
     const env = { sky: 'none', sunIlluminance: 100000, uSunAzimuth: 0, uSunElevation: 90, sunDiameter: opts.sunDiameter || 0.53,
         sunAperture: { point: { x: 0, y: 0, z: 2 }, radius: opts.beam || 0.3 } }
-    const d = doc('sppm-slab', mats, [obj('floor', 'b-plane', 'floor', xf(0, 0, 0)),
-        obj('slab', 'b-box', 'glass', xf(0, 0, 1), { size: { x: 1, y: 1, z: 0.05 }, rounding: 0 })], env, 20)
+    const children = [obj('floor', 'b-plane', 'floor', xf(0, 0, 0)),
+        obj('slab', 'b-box', 'glass', xf(0, 0, 1), { size: { x: 1, y: 1, z: 0.05 }, rounding: 0 })]
+    // with opts.mfp: a haze box under the slab, the beam's track through it (light tracing's connections from the dust)
+    if (opts.mfp) children.push({ id: 'haze', kind: 'object', type: 'haze', label: 'haze', enabled: true, material: null,
+        transform: xf(0, 0, 0.5), params: { size: { x: 1.2, y: 1.2, z: 0.8 } }, haze: { mfp: opts.mfp, g: opts.g || 0, albedo: { r: 1, g: 1, b: 1 } } })
+    const d = doc('sppm-slab', mats, children, env, 20)
     const w = opts.width || 64, h = opts.height || 48
     return { doc: d, camera: camera([0, -0.8, 0.5], [0, 0, 0], 40, w, h),
-        settings: { width: w, height: h, spp: 16, maxbounce: 8, nee: 1, sppm: 1, sppmPhotons: 1 << 20, sppmRadius: 0.02, sppmAlpha: 0.7 } }
+        settings: { width: w, height: h, spp: 16, maxbounce: 8, nee: 1, sppm: 1, sppmPhotons: 1 << 20, sppmRadius: 0.02, sppmAlpha: 0.7,
+            lt: opts.lt || 0 } }
 }
 
 function furnacePlane(opts = {}) {

@@ -34,7 +34,7 @@ const UNIFORMS = [['width', 'u', 1], ['height', 'u', 1], ['maxbounce', 'u', 1], 
     ['epsAuto', 'f', 1], ['envRadius', 'f', 1], ['nee', 'u', 1], ['rr', 'u', 1], ['rrStart', 'u', 1], ['diffuseCap', 'u', 1],
     ['fireflyClamp', 'f', 1], ['fireflyWhite', 'f', 1], ['maxaverages', 'u', 1], ['seed', 'u', 1], ['cam', CAMERA, 1], ['env', ENV, 1],
     ['haze', HAZE, 1], ['faceShare', 'f', 1], ['sppm', 'u', 1], ['sppmRadius', 'f', 1], ['sppmPhotons', 'u', 1], ['marchRefill', 'u', 1],
-    ['guide', 'u', 1], ['guideFrac', 'f', 1], ['sppmAlpha', 'f', 1], ['relax', 'f', 1], ['measureMode', 'u', 1]]
+    ['guide', 'u', 1], ['guideFrac', 'f', 1], ['sppmAlpha', 'f', 1], ['relax', 'f', 1], ['measureMode', 'u', 1], ['lt', 'u', 1]]
 const MATERIAL = [['base', 'f', 3], ['metallic', 'f', 1], ['roughness', 'f', 1], ['transmission', 'f', 1], ['ior', 'f', 1], ['specular', 'f', 1],
     ['emissive', 'f', 3], ['abbe', 'f', 1], ['face', 'i', 1], ['specularTint', 'f', 1], ['pad', 'f', 2]]
 const FACE = [['centre', 'f', 3], ['halfx', 'f', 1], ['u', 'f', 3], ['halfy', 'f', 1], ['v', 'f', 3], ['shape', 'u', 1], ['n', 'f', 3],
@@ -71,7 +71,7 @@ function packArray(layout, items) {
 const DEFAULTS = {
     width: 256, height: 256, maxbounce: 10, maxiter: 400, stepscale: 0.9, eps: 0, epsAuto: 1e-4, envRadius: 400, nee: 1, rr: 0,
     rrStart: 3, diffuseCap: 0, fireflyClamp: 0, fireflyWhite: 1, maxaverages: 0, seed: 20261203, faceShare: 0.5, sppm: 0,
-    sppmRadius: 0.02, sppmPhotons: 0, marchRefill: 0, guide: 0, guideFrac: 0.9, sppmAlpha: 2 / 3, relax: 0, measureMode: 0,
+    sppmRadius: 0.02, sppmPhotons: 0, marchRefill: 0, guide: 0, guideFrac: 0.9, sppmAlpha: 2 / 3, relax: 0, measureMode: 0, lt: 0,
 }
 const KNOWN = new Set(UNIFORMS.map(f => f[0]))
 

@@ -55,7 +55,7 @@ class PtUniforms(ctypes.Structure):
                 ('seed', ctypes.c_uint), ('cam', PtCamera), ('env', PtEnv), ('haze', PtHaze), ('faceShare', ctypes.c_float),
                 ('sppm', ctypes.c_uint), ('sppmRadius', ctypes.c_float), ('sppmPhotons', ctypes.c_uint),
                 ('marchRefill', ctypes.c_uint), ('guide', ctypes.c_uint), ('guideFrac', ctypes.c_float), ('sppmAlpha', ctypes.c_float),
-                ('relax', ctypes.c_float), ('measureMode', ctypes.c_uint)]
+                ('relax', ctypes.c_float), ('measureMode', ctypes.c_uint), ('lt', ctypes.c_uint)]
 
 
 class PtSceneDesc(ctypes.Structure):
@@ -150,7 +150,8 @@ GUIDE_NP = 16169
 
 STAT_NAMES = ['segments', 'shadow_segments', 'steps', 'shadow_steps', 'nonfinite', 'maxiter', 'paths', 'leaving',
               'gathers', 'photons', 'flops', 'shadow_flops', 'photon_segments', 'gather_cut', 'radius_bits',
-              'overflow', 'guide_steps', 'guided', 'guide_records', 'flushed', 'shadowcut']
+              'overflow', 'guide_steps', 'guided', 'guide_records', 'flushed', 'shadowcut', 'guide_infer',
+              'guide_tiles', 'guide_train']
 PT_SCENE_KEEP_PICTURE = 1   # pt_scene_desc.flags (include/pt.h)
 STRAGGLER_ROUNDS = 1    # PT_STRAGGLER_ROUNDS (include/pt_types.h): rounds a sample runs beyond maxbounce + 1
 LUMA = (0.2126, 0.7152, 0.0722)
@@ -249,7 +250,7 @@ def uniforms_for(made, **over):
     u.cam = make_camera(made['camera'])
     u.env = make_env(sc['env'])
     u.haze = make_haze(sc.get('haze'))
-    for k in ('maxiter', 'marchRefill', 'rr', 'rrStart', 'diffuseCap', 'maxaverages', 'sppm', 'sppmPhotons', 'measureMode'):
+    for k in ('maxiter', 'marchRefill', 'rr', 'rrStart', 'diffuseCap', 'maxaverages', 'sppm', 'sppmPhotons', 'measureMode', 'lt'):
         if k in s:
             setattr(u, k, int(s[k]))
     for k in ('stepscale', 'eps', 'epsAuto', 'faceShare', 'fireflyClamp', 'sppmRadius', 'sppmAlpha', 'relax'):
@@ -380,8 +381,9 @@ class Engine:
     def tuning_key(self):
         """What a tuning is good for: the scene's source, the image, and the settings that shape a sample's work."""
         u = self._u
-        return '%s/%dx%d/b%d/n%d/s%d:%d/g%d/x%d' % (self._scene_hash, u.width, u.height, u.maxbounce, u.nee, u.sppm,
-                                                   u.sppmPhotons if u.sppm else 0, u.guide, self.straggler_rounds)
+        return '%s/%dx%d/b%d/n%d/s%d:%d/l%d/g%d/x%d' % (self._scene_hash, u.width, u.height, u.maxbounce, u.nee, u.sppm,
+                                                       u.sppmPhotons if (u.sppm or u.lt) else 0, u.lt, u.guide,
+                                                       self.straggler_rounds)
 
     def autotune(self, samples=4, cache=None):
         """Picks the persistent kernels' grid and refill thresholds for the scene and uniforms set (pt_autotune) and

@@ -39,7 +39,7 @@ def test_struct_sizes_match_the_bindings():
     for i, s in enumerate(pt.STRUCTS):
         assert lib.pt_struct_size(i) == ctypes.sizeof(s), s.__name__
     assert lib.pt_struct_size(99) == 0
-    assert lib.pt_abi_version() == 2
+    assert lib.pt_abi_version() == 3
 
 
 def test_addon_loads_and_packs_the_same_structures():
@@ -119,4 +119,6 @@ def test_header_constants_match_the_binding():
     assert define('PT_STRAGGLER_ROUNDS') == pt.STRAGGLER_ROUNDS
     assert pt.STAT_NAMES.index('flushed') == define('PT_S_FLUSHED')
     assert pt.STAT_NAMES.index('guide_records') == define('PT_S_GRECORDS')
+    assert pt.STAT_NAMES.index('shadowcut') == define('PT_S_SHADOWCUT')
+    assert pt.STAT_NAMES.index('guide_train') == define('PT_S_GTRAIN')
     assert len(pt.STAT_NAMES) <= define('PT_S_WORDS')

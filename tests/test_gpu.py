@@ -632,10 +632,47 @@ def test_sppm_switches_off_and_on(engine):
     assert abs(on / SLAB_FLOOR - 1) < 0.05 and abs(again / SLAB_FLOOR - 1) < 0.05, (on, again)
 
 
+def test_light_tracing_slab_closed_form(engine):
+    # light tracing alone, and with the map on: the floor under the slab reads rho E T / pi once (the connections'),
+    # and its connections are traced as the photon pass's shadow rays
+    for over in (dict(lt=1, sppm=0), dict(lt=1, sppm=1)):
+        say('case: sppm-slab 64x48, light tracing %s, 2^18 photons x 16 samples' % over)
+        made, u = load(engine, 'sppm-slab', width=64, height=48, sppmPhotons=1 << 18, **over)
+        inb = beam_pixels(made, 0.2)
+        engine.trace(16)
+        acc, _ = engine.read_accum()
+        st = engine.stats()
+        Y = pt.luminance(acc)[inb]
+        se = Y.std() / math.sqrt(inb.sum())
+        say('  %.1f +- %.1f against %.1f; connections %d' % (Y.mean(), se, SLAB_FLOOR, st['shadow_segments']))
+        assert abs(Y.mean() - SLAB_FLOOR) < 4 * se + 0.01 * SLAB_FLOOR, (over, Y.mean(), se, SLAB_FLOOR)
+        assert st['shadow_segments'] > 16 * (1 << 17) and st['nonfinite'] == 0
+        assert st['gathers'] == 0      # the camera sees the floor directly: its first vertex is the connections'
+        assert pt.luminance(acc)[~beam_pixels(made, 0.45)].max() == 0
+
+
+def test_light_tracing_in_the_haze_matches_the_oracle(engine):
+    # the beam's track through a haze box under the slab and its spot on the floor, light traced with the map on: the
+    # connections from the dust and from the floor, and the camera's paths that leave them their light, sample by
+    # sample against the oracle's (the same seeds), over the picture and over the beam's spot
+    opts = dict(width=48, height=36, mfp=4.0, g=0.3, sunDiameter=20, beam=0.3)
+    say('case: sppm-slab with haze 48x36, lt 1, sppm 1, 2^16 photons, 48 samples vs oracle')
+    made, u = load(engine, 'sppm-slab', sppmPhotons=1 << 16, lt=1, **opts)
+    g = gpu_samples(engine, 48)
+    st = engine.stats()
+    assert st['nonfinite'] == 0 and st['shadow_segments'] > 0
+    o = oracle_samples(made, pt.uniforms_for(made, sppmPhotons=1 << 16, lt=1), 48)
+    for name, m in (('picture', np.ones((36, 48), bool)), ('spot', beam_pixels(made, 0.25))):
+        r = agree(g, o, m, 0.01)
+        say('  %s: engine %.2f, oracle %.2f, se %.2f (%.2f of the tolerance)' % (name, r[1], r[2], r[3], r[0]))
+        assert r[0] < 1, (name, r)
+
+
 @pytest.mark.parametrize('mode', [1, 2])
 def test_c3_caustic_room_matches_the_oracle(engine, mode):
-    # sppm 1 gathers at a camera path's first non-delta vertex (the reference's); 2 at every non-delta vertex
-    say('case: caustic-c3 160x90, 2^18 photons, 8 samples, sppm %d, 16 regions vs oracle' % mode)
+    # sppm 1 gathers at a camera path's first non-delta vertex after a delta event (the reference's); 2 at every
+    # non-delta vertex past the first; the first is light tracing's (the room's setting, as the reference's default)
+    say('case: caustic-c3 160x90, 2^18 photons, 8 samples, sppm %d, light tracing, 16 regions vs oracle' % mode)
     made, u = load(engine, 'caustic-c3', width=160, height=90, sppmPhotons=1 << 18, sppm=mode)
     regions = []
     for by in range(4):

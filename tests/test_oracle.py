@@ -1341,3 +1341,57 @@ def test_sppm_map_matches_path_tracing_on_a_glossy_floor():
             for k in range(6)]
     traced, se = np.mean(runs), np.std(runs, ddof=1) / math.sqrt(len(runs))
     assert abs(mapped - traced) < 4 * se + 0.01 * traced, (mapped, traced, se)
+
+
+def test_light_tracing_slab_closed_form():
+    # light tracing (lt 1): the sun's photons through the slab connect to the camera from the floor, and the camera's
+    # paths leave that light to them. Alone, and with the map on (sppm 1, which then gathers only past a delta event),
+    # the floor under the slab reads rho E T / pi once: not twice (both counting it), not zero (neither)
+    made = scenes.emit('sppm-slab', width=32, height=24)
+    inb = _beam_pixels(made, 0.2)
+    for over in (dict(lt=1, sppm=0), dict(lt=1, sppm=1)):
+        u = pt.uniforms_for(made, sppmPhotons=1 << 16, **over)
+        acc, _, st = oracle.render(made['scene'], u, 4)
+        Y = pt.luminance(acc)[inb]
+        se = Y.std() / math.sqrt(inb.sum())
+        assert abs(Y.mean() - SLAB_FLOOR) < 4 * se + 3e-3 * SLAB_FLOOR, (over, Y.mean(), se, SLAB_FLOOR)
+        assert st['gathers'] == 0 or over['sppm'] == 1
+    # the connections stop where the floor is not seen: a pixel outside the beam's spot reads 0
+    out = ~_beam_pixels(made, 0.45)
+    assert pt.luminance(acc)[out].max() == 0
+
+
+def _runs(made, region, runs, samples, seed, **over):
+    """A region's mean over independent runs, and its standard error from their spread."""
+    vals = [pt.luminance(oracle.render(made['scene'], pt.uniforms_for(made, seed=seed + k, **over), samples)[0])[region].mean()
+            for k in range(runs)]
+    return np.mean(vals), np.std(vals, ddof=1) / math.sqrt(runs)
+
+
+def _lt_against_tracing(made, region, photons, samples, traced_samples, runs=4):
+    """The light tracer's image over a region against plain path tracing's (lt 0, no map), which converges here (a
+    wide sun); each side's error bar from its own independent runs (the camera's paths the light tracer leaves alone
+    are noisy on both)."""
+    lt, se1 = _runs(made, region, runs, samples, 500, lt=1, sppm=0, sppmPhotons=photons)
+    traced, se2 = _runs(made, region, runs, traced_samples, 700, lt=0, sppm=0)
+    return lt, traced, math.hypot(se1, se2)
+
+
+def test_light_tracing_matches_path_tracing_on_a_glossy_floor():
+    # a glossy plastic floor under a 20-degree sun through the slab: the connection's f (the floor's BSDF from the
+    # camera's side at the photon's direction) and its footprint G / A_px against the camera's own paths
+    made = scenes.emit('sppm-slab', width=32, height=24, floorRoughness=0.4, floorSpecular=0.5, sunDiameter=20, beam=0.6)
+    inb = _beam_pixels(made, 0.3)
+    lt, traced, se = _lt_against_tracing(made, inb, 1 << 14, 64, 1024)
+    assert abs(lt - traced) < 4 * se + 0.01 * traced, (lt, traced, se)
+
+
+def test_light_tracing_matches_path_tracing_in_the_haze():
+    # the beam's track in a haze box under the slab (mean free path 4 m, g 0.3) over a black floor: a photon that
+    # scatters in the dust connects through the phase function (the albedo its weight, per unit volume dist / z^3);
+    # the camera's paths whose first vertex is in the dust leave the sun through the slab to it. The whole picture
+    # (both sides carry the camera's own rare multiple-scattering hits of the sun, hence the runs)
+    made = scenes.emit('sppm-slab', width=24, height=18, mfp=4.0, g=0.3, sunDiameter=20, beam=0.3, floorAlbedo=0)
+    allpx = np.ones((18, 24), bool)
+    lt, traced, se = _lt_against_tracing(made, allpx, 1 << 14, 192, 3072, runs=6)
+    assert abs(lt - traced) < 4 * se + 0.01 * traced, (lt, traced, se)
