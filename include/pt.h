@@ -34,9 +34,13 @@ typedef struct pt_opts {
     const char* cache_dir;      /* code-object cache (NULL: $PT_KCACHE, else kcache/ beside the library) */
     uint32_t straggler_rounds;  /* rounds a sample runs beyond maxbounce + 1 for paths a hit from behind held up
                                    (0: PT_STRAGGLER_ROUNDS; PT_STRAGGLER_NONE: none, the flush closes them) */
-    uint32_t reserved[3];
+    uint32_t flags;             /* PT_OPT_* */
+    uint32_t reserved[2];
 } pt_opts;
 #define PT_STRAGGLER_NONE 0xFFFFFFFFu
+/* Launch a sample's kernels one by one on the stream instead of replaying the captured hipGraph (for tools that
+ * follow kernel launches but not graph launches). Same kernels, same order, same results. */
+#define PT_OPT_DIRECT_LAUNCH 1u
 
 /* A scene as the HIP emitter (shadytracer_amd/js/hipgen.js) produces it: HIP C++ defining sceneDist and
  * sceneFull, the parameter block they read, the material table and the light faces. */
@@ -122,7 +126,8 @@ int pt_reset_stats(pt_ctx* ctx);
 /* Runs `samples` samples kernel by kernel with HIP events around each launch on the engine's stream and
  * returns the summed milliseconds per kernel: ms[0] genrays, [1] march, [2] shade, [3] occlude, [4] resolve,
  * [5] the queues' binning, [6] the photon pass's map clear and emission, [7] surface, [8] the guide network's
- * inference and training (n >= 5; the slots past n are dropped).
+ * inference and training, and of those [9] inference, [10] the training's gradient pass (forward and backward),
+ * [11] the rest of training: collection, Adam, commit (n >= 5; the slots past n are dropped).
  * Replaces R.timeKernels (raytrace_main.js:549-557). */
 int pt_time_kernels(pt_ctx* ctx, uint32_t samples, double* ms, size_t n);
 

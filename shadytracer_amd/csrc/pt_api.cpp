@@ -154,6 +154,7 @@ struct pt_ctx {
     std::string cacheDir;
     unsigned int cus = 256, blocksPerCu = 8;
     unsigned int stragglers = PT_STRAGGLER_ROUNDS;   // rounds beyond maxbounce + 1 (pt_opts.straggler_rounds)
+    bool direct = false;      // PT_OPT_DIRECT_LAUNCH: samples launched kernel by kernel, no graph replay
     PtTuning tune{};          // the persistent kernels' launch choices (pt_set_tuning, pt_autotune)
 
     hipModule_t module = nullptr;
@@ -394,6 +395,7 @@ uint64_t graphKey(const pt_ctx* c) {
 }
 
 int buildGraph(pt_ctx* c) {
+    if (c->direct) return PT_OK;                 // launched kernel by kernel (launchSample)
     if (c->graphExec && c->graphKey == graphKey(c)) return PT_OK;
     destroyGraph(c);
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -408,6 +410,11 @@ int buildGraph(pt_ctx* c) {
     HIPCHK(c, hipGraphInstantiate(&c->graphExec, g, nullptr, nullptr, 0));
     c->graphKey = graphKey(c);
     return PT_OK;
+}
+
+// One sample on the stream: the captured graph replayed, or with PT_OPT_DIRECT_LAUNCH its launches one by one
+hipError_t launchSample(pt_ctx* c) {
+    return c->direct ? enqueueSample(c, nullptr, nullptr) : hipGraphLaunch(c->graphExec, c->stream);
 }
 
 // ---- the guide network's host side
@@ -608,6 +615,7 @@ int pt_create(pt_ctx** out, const pt_opts* opts) {
     c->blocksPerCu = (opts && opts->blocks_per_cu) ? opts->blocks_per_cu : 8u;
     c->stragglers = !(opts && opts->straggler_rounds) ? PT_STRAGGLER_ROUNDS
                   : opts->straggler_rounds == PT_STRAGGLER_NONE ? 0u : std::min(opts->straggler_rounds, 16u);
+    c->direct = opts && (opts->flags & PT_OPT_DIRECT_LAUNCH);
     setDefaults(c->host.u);
     setGuideDefaults(c->host.g);
     c->host.gridBlocks = c->cus * c->blocksPerCu;
@@ -817,7 +825,7 @@ int pt_trace(pt_ctx* c, uint32_t samples) {
     if (!c->module) return fail(c, PT_ERR_STATE, "no scene loaded");
     int rc;
     if ((rc = syncDev(c)) || (rc = buildGraph(c))) return rc;
-    for (uint32_t s = 0; s < samples; ++s) HIPCHK(c, hipGraphLaunch(c->graphExec, c->stream));
+    for (uint32_t s = 0; s < samples; ++s) HIPCHK(c, launchSample(c));
     return PT_OK;
 }
 
@@ -843,9 +851,9 @@ int timeSetting(pt_ctx* c, uint32_t samples, double& ms) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIPCHK(c, hipEventCreate(&e0));
     if (hipEventCreate(&e1) != hipSuccess) { hipEventDestroy(e0); return fail(c, PT_ERR_HIP, "event create"); }
-    hipError_t e = hipGraphLaunch(c->graphExec, c->stream);
+    hipError_t e = launchSample(c);
     if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
-    for (uint32_t k = 0; e == hipSuccess && k < samples; ++k) e = hipGraphLaunch(c->graphExec, c->stream);
+    for (uint32_t k = 0; e == hipSuccess && k < samples; ++k) e = launchSample(c);
     if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
     if (e == hipSuccess) e = hipEventSynchronize(e1);
     float t = 0.0f;
@@ -957,8 +965,10 @@ int pt_time_kernels(pt_ctx* c, uint32_t samples, double* ms, size_t n) {
             int slot = k == K_GENRAYS ? 0 : k == K_MARCH ? 1 : k == K_SHADE ? 2 : k == K_OCCLUDE ? 3 : (k == K_RESOLVE || k == K_FLUSH) ? 4
                      : (k == K_SPPMCLEAR || k == K_PHOTONGEN) ? 6 : k == K_SURFACE ? 7
                      : (k == K_GINFER || k == K_GCOLLECT || k == K_GGRAD || k == K_GADAM || k == K_GCOMMIT) ? 8 : 5;
-            if (slot >= (int)n) continue;
-            ms[slot] += t;
+            if (slot < (int)n) ms[slot] += t;
+            // the guide's kernels one by one: inference, the gradient pass (forward and backward), the rest of training
+            int g = k == K_GINFER ? 9 : k == K_GGRAD ? 10 : (k == K_GCOLLECT || k == K_GADAM || k == K_GCOMMIT) ? 11 : -1;
+            if (g >= 0 && g < (int)n) ms[g] += t;
         }
     }
     for (auto& e : ev) hipEventDestroy(e);

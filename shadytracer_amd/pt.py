@@ -96,10 +96,11 @@ def result_dict(r):
 
 class PtOpts(ctypes.Structure):
     _fields_ = [('device', ctypes.c_int), ('blocks_per_cu', ctypes.c_uint32), ('cache_dir', ctypes.c_char_p),
-                ('straggler_rounds', ctypes.c_uint32), ('reserved', ctypes.c_uint32 * 3)]
+                ('straggler_rounds', ctypes.c_uint32), ('flags', ctypes.c_uint32), ('reserved', ctypes.c_uint32 * 2)]
 
 
 STRAGGLER_NONE = 0xFFFFFFFF
+OPT_DIRECT_LAUNCH = 1        # a sample's kernels launched one by one, not replayed from the captured graph
 
 
 class PtTuning(ctypes.Structure):
@@ -262,12 +263,14 @@ def uniforms_for(made, **over):
 class Engine:
     """One libpt context on a GPU."""
 
-    def __init__(self, device=0, blocks_per_cu=0, cache_dir=None, straggler_rounds=None):
+    def __init__(self, device=0, blocks_per_cu=0, cache_dir=None, straggler_rounds=None, direct=False):
         """straggler_rounds: rounds beyond maxbounce + 1 for paths held up by hits from behind (None: the library's
-        default, 0: none -- the flush closes them)."""
+        default, 0: none -- the flush closes them). direct: launch each sample's kernels one by one instead of
+        replaying the captured graph (PT_OPT_DIRECT_LAUNCH)."""
         self.lib = load_library()
         o = PtOpts()
         o.device, o.blocks_per_cu = device, blocks_per_cu
+        o.flags = OPT_DIRECT_LAUNCH if direct else 0
         o.straggler_rounds = 0 if straggler_rounds is None else (STRAGGLER_NONE if straggler_rounds == 0 else straggler_rounds)
         self.straggler_rounds = STRAGGLER_ROUNDS if straggler_rounds is None else min(straggler_rounds, 16)
         o.cache_dir = cache_dir.encode() if cache_dir else None
@@ -363,9 +366,10 @@ class Engine:
         self._chk(self.lib.pt_reset_stats(self.h))
 
     def time_kernels(self, samples=1):
-        ms = (ctypes.c_double * 9)()
-        self._chk(self.lib.pt_time_kernels(self.h, samples, ms, 9))
-        return dict(zip(['genrays', 'march', 'shade', 'occlude', 'resolve', 'bin', 'photon', 'surface', 'guide'], list(ms)))
+        ms = (ctypes.c_double * 12)()
+        self._chk(self.lib.pt_time_kernels(self.h, samples, ms, 12))
+        return dict(zip(['genrays', 'march', 'shade', 'occlude', 'resolve', 'bin', 'photon', 'surface', 'guide',
+                         'guide_infer', 'guide_grad', 'guide_adam'], list(ms)))
 
     # ---- launch choices
     def tuning(self):
