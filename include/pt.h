@@ -63,7 +63,7 @@ int pt_abi_version(void);
 
 /* sizeof of the ABI's structures, for bindings to check their mirrors: 0 PtUniforms, 1 PtMaterial,
  * 2 PtLightFace, 3 PtCamera, 4 PtEnv, 5 PtHaze, 6 pt_scene_desc, 7 pt_opts, 8 PtGuideConfig, 9 PtOpticsScene,
- * 10 PtOpticsResult, 11 PtTuning; 0 for anything else.
+ * 10 PtOpticsResult, 11 PtTuning, 12 PtOpticsSource, 13 PtOpticsTrace, 14 PtOpticsTally; 0 for anything else.
  * No counterpart in the reference (its uniform packing lives in the absent tinywebgpu submodule). */
 size_t pt_struct_size(int which);
 
@@ -176,6 +176,22 @@ int pt_guide_train(pt_ctx* ctx, const float* samples, size_t n, float* grad, dou
 /* The training's state after the last sample: out[0] Adam steps taken, out[1] training samples the sample
  * collected, out[2 + j] the loss of its step j (j < steps); n >= 2. */
 int pt_guide_status(pt_ctx* ctx, double* out, size_t n);
+
+/* ---- the optics trace over the loaded scene ---- */
+
+/* Traces t->rays rays from the sources through the loaded scene -- the engine's own binning, march, surface kernels and
+ * materials, with its uniforms in force (tolerance, maxiter, haze) -- and returns where their flux went: escaped, lost
+ * (past the bounce cap, maxDist or maxiter), detected, absorbed in the haze, and per material the flux arriving and
+ * leaving and the interactions (PtOpticsTally, lm). far (NULL when t->farMode is 0, else nfar = nG x nC doubles,
+ * gamma-major) receives the far field, lm per (gamma, C) bin of the type C grid; hits (NULL, or hitCap x
+ * PT_OPT_HIT_WORDS floats) the first min(t->detCap, hitCap) detector hits. The same seed traces the same rays. Sources
+ * with no flux are left out; a trace needs one with flux.
+ * Replaces createOptics(RT, WG) with setSources, run, results and readHits (lib/pathtracer/optics/index.js:92-825,
+ * :434-477 run, :613-631 readHits, :687-697 setSources) over optics/kernels.js prog_opt_gen :504-531 (sources:
+ * kernels.js SOURCE_RAY :270-420, optics/sources.js :1-60) and prog_opt_shade :533-723 (tallies, far field, detector
+ * hits), and the far-field binning of optics/farfield.js:44-56. */
+int pt_optics_run(pt_ctx* ctx, const PtOpticsSource* src, uint32_t nsrc, const PtOpticsTrace* t, size_t tsize,
+                  PtOpticsTally* out, size_t osize, double* far, size_t nfar, float* hits, uint32_t hitCap);
 
 /* ---- the optics bench ---- */
 

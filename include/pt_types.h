@@ -214,6 +214,77 @@ typedef struct PtOpticsResult {
 #define PT_OPT_NC 360        // far field C columns (0, 1, ..., 359 deg)
 #define PT_OPT_LUX 256       // lux map cells a side over [-1, 1] m at z = 1 m
 
+// ---- The optics trace over the scene (optics/index.js createOptics; optics/kernels.js prog_opt_gen/prog_opt_shade):
+// rays from sources walk the loaded scene through the engine's own march, surface and materials; where their flux goes
+// is tallied per material, into the far field and at detectors.
+// A source (optics/sources.js defaultSource / sourceRecords): a shape at a pose, an angular distribution about its axis,
+// a flux. Every ray carries the same share of the total flux, so the distribution is in where rays go.
+typedef struct PtOpticsSource {
+    float pos[3];
+    unsigned int shape;          // 0 point, 1 disc (size[0] its radius), 2 rectangle (size: half sizes along u, axis x u)
+    float axis[3];               // the emission axis (normalised by the library)
+    unsigned int distribution;   // 0 collimated (along the axis), 1 uniform in solid angle within the cone,
+                                 // 2 density ~ cos^n within the cone (n 1 over 90 degrees: Lambertian)
+    float u[3];                  // a tangent (any vector not along the axis)
+    float halfAngle;             // the cone's half angle, degrees, 0..90
+    float size[2];
+    float n;                     // the cosine power (distribution 2)
+    float flux;                  // lm (a source of no flux is left out)
+    float colour[3];             // the light's hue, linear Rec.709 (scaled to luminance 1 by the library)
+    float emitN;                 // the index of the medium it emits into (>= 1)
+    unsigned int twoSided;       // both faces emit, half the rays each
+    unsigned int onSurface;      // the source is a sheet of the scene: rays start two march tolerances off it
+    unsigned int pad[2];
+} PtOpticsSource;
+// What a trace is asked (optics/index.js O.settings).
+typedef struct PtOpticsTrace {
+    unsigned int rays;           // rays traced in all (the library runs them in batches)
+    unsigned int bounces;        // interactions a ray is followed through (64); one past them it is LOST
+    unsigned int seed;
+    unsigned int detCap;         // detector hits recorded at most (more are counted)
+    float maxDist;               // the longest optical path, metres; a hit beyond it is LOST (0: no limit)
+    unsigned int farMode;        // far field: 0 off, 1 every escaping ray, 2 every ray arriving at detector material
+                                 // farMaterial, by the direction it arrives along
+    unsigned int farMaterial;
+    unsigned int nC, nG;         // C bins about the axis (centred on 0, 360/nC, ...); gamma bins centred on
+                                 // 0, 180/(nG-1), ..., 180 degrees, the two end ones half as wide
+    float farAxis[3];            // gamma = 0 (normalised by the library)
+    float farC0[3];              // C = 0 (made square to the axis by the library)
+    unsigned int detectors[2];   // one bit per material id: a ray arriving at such a material from outside stops
+                                 // there, all of its flux DETECTED and the hit recorded
+    unsigned int marchSteps;     // sphere-tracing steps a ray takes over its whole path before it is LOST (the engine's
+                                 // maxiter for the trace's rays; 0: 2000, optics/index.js settings.marchSteps)
+    unsigned int pad;
+} PtOpticsTrace;
+// Where the flux went (optics/index.js readTallies / scaled), in lm: emitted = escaped + lost + volume + the sum over
+// materials of (incident - leaving); absorbed = that sum less what the detectors caught, plus volume.
+typedef struct PtOpticsTally {
+    double emitted, escaped, lost, detected, volume, absorbed;
+    unsigned long long rays;
+    unsigned long long escapedRays, lostRays, interactions, absorbedRays, detectedRays;
+    unsigned long long detectorHits;       // hits at detectors, recorded or not (recorded: min(this, detCap))
+    unsigned long long pad;
+    double incident[PT_MAX_MATERIALS];     // per material id: the flux arriving, and leaving after the interaction
+    double leaving[PT_MAX_MATERIALS];
+    unsigned long long hits[PT_MAX_MATERIALS];
+} PtOpticsTally;
+#define PT_OPT_HIT_WORDS 8   // a recorded detector hit: position xyz (m), flux (lm), the arriving direction xyz, material
+// The device tallies: fixed point, PT_OPT_FX per unit of ray weight in 64-bit integer atomics (exact, and the same sum
+// in any order). Word indices:
+#define PT_OPT_FX 16777216.0
+#define PT_OT_ESCAPED 0
+#define PT_OT_LOST 1
+#define PT_OT_DETECTED 2
+#define PT_OT_VOLUME 3
+#define PT_OT_CESC 4
+#define PT_OT_CLOST 5
+#define PT_OT_CINTER 6
+#define PT_OT_CABS 7
+#define PT_OT_CDET 8
+#define PT_OT_HITS 9
+#define PT_OT_MAT 16           // then per material: incident, leaving, hits
+#define PT_OT_WORDS (PT_OT_MAT + 3 * PT_MAX_MATERIALS)
+
 typedef struct PtUniforms {
     unsigned int width, height;
     unsigned int maxbounce;     // 10

@@ -73,3 +73,19 @@ struct PtDev {
     double* gLoss;         // per step: the batch's loss
     unsigned int gStride, gPaths, gMaxV, gBatchCap;
 };
+
+// The optics trace's own table (pt_optrace.h): the sources, the trace's settings and its tallies, for one batch.
+struct PtOptDev {
+    const PtOpticsSource* src;      // sources with flux, each's cumulative share of the total in cdf
+    const float* cdf;
+    unsigned int nsrc;
+    unsigned int rays;              // this batch's rays: slots [0, rays)
+    unsigned int firstId;           // the batch's first ray id (a ray's draws are keyed by its id)
+    unsigned int bounces;
+    unsigned int seed[4];
+    PtOpticsTrace t;                // the settings, axis and C0 normalised
+    unsigned long long* tally;      // PT_OT_WORDS
+    unsigned long long* far;        // nG x nC, gamma-major
+    float* hits;                    // detCap x PT_OPT_HIT_WORDS (flux in units of ray weight)
+};
+

@@ -27,9 +27,10 @@ namespace {
 const char* kKernelNames[] = {"pt_genrays", "pt_march", "pt_shade", "pt_occlude", "pt_resolve", "pt_mf_lut", "pt_mf_lut_avg", "pt_pick",
                               "pt_bin_count", "pt_bin_scan", "pt_bin_scatter", "pt_sppm_clear", "pt_photon_gen", "pt_surface",
                               "pt_guide_infer", "pt_guide_eval", "pt_guide_collect", "pt_guide_grad", "pt_guide_adam", "pt_guide_commit",
-                              "pt_flush"};
+                              "pt_flush", "pt_opt_gen", "pt_opt_shade", "pt_opt_flush"};
 enum { K_GENRAYS, K_MARCH, K_SHADE, K_OCCLUDE, K_RESOLVE, K_MFLUT, K_MFAVG, K_PICK, K_BINCOUNT, K_BINSCAN, K_BINSCATTER,
-       K_SPPMCLEAR, K_PHOTONGEN, K_SURFACE, K_GINFER, K_GEVAL, K_GCOLLECT, K_GGRAD, K_GADAM, K_GCOMMIT, K_FLUSH, K_COUNT };
+       K_SPPMCLEAR, K_PHOTONGEN, K_SURFACE, K_GINFER, K_GEVAL, K_GCOLLECT, K_GGRAD, K_GADAM, K_GCOMMIT, K_FLUSH, K_OPTGEN, K_OPTSHADE,
+       K_OPTFLUSH, K_COUNT };
 
 // the guide network's sizes (device/pt_guide.h)
 const unsigned int kGuideNP = 16169, kGuideWT = 17408, kGuideBias = 256, kGuideParts = 256, kGuideMaxSteps = 64;
@@ -179,6 +180,11 @@ struct pt_ctx {
 
     unsigned int sppmCapacity = 0;   // photon records allocated
 
+    // the optics trace (pt_optics_run): its table on the device and the buffers it points at
+    PtOptDev opt{};
+    PtOptDev* optDev = nullptr;
+    unsigned int optSrcCap = 0, optFarCap = 0, optHitCap = 0;
+
     bool guideReady = false;          // guide network allocated and initialised
     unsigned int guideMixSlots = 0;   // slots gMix holds
     unsigned int guideRecPaths = 0, guideRecVerts = 0, guideBatchCap = 0;
@@ -235,6 +241,8 @@ void raygenTiles(unsigned int N, unsigned int w, unsigned int h, unsigned int& t
     if (ty < 1) ty = 1;
 }
 
+int ensureSlots(pt_ctx* c, unsigned int slots);
+
 // Frame and accumulators for w x h pixels; path slots for the larger of the pixels and the photons a pass emits; the
 // photon map (three tables of 2^20 words; room for two records, 32 bytes and two links each, a photon) when photons
 // are asked for.
@@ -272,6 +280,11 @@ int ensureBuffers(pt_ctx* c, unsigned int w, unsigned int h, unsigned int photon
         c->sppmCapacity = records;
     }
     c->host.sppmCap = c->sppmCapacity;
+    return ensureSlots(c, slots);
+}
+
+// Path slots (state, queues, masks, hit records, shadow rays) for `slots` rays at once; grows, never shrinks.
+int ensureSlots(pt_ctx* c, unsigned int slots) {
     if (slots > c->capacity) {
         destroyGraph(c);
         float** f4[] = {&c->host.pos, &c->host.dir, &c->host.beta, &c->host.rad, &c->host.aux, &c->host.shOrig, &c->host.shDir, &c->host.shRad,
@@ -576,6 +589,9 @@ size_t pt_struct_size(int which) {
         case 9: return sizeof(PtOpticsScene);
         case 10: return sizeof(PtOpticsResult);
         case 11: return sizeof(PtTuning);
+        case 12: return sizeof(PtOpticsSource);
+        case 13: return sizeof(PtOpticsTrace);
+        case 14: return sizeof(PtOpticsTally);
         default: return 0;
     }
 }
@@ -972,6 +988,194 @@ int pt_time_kernels(pt_ctx* c, uint32_t samples, double* ms, size_t n) {
         }
     }
     for (auto& e : ev) hipEventDestroy(e);
+    return PT_OK;
+}
+
+// ---- the optics trace over the loaded scene (optics/index.js createOptics run / readHits / results)
+namespace {
+bool finite3(const float* v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
+double len3(const float* v) { return std::sqrt(double(v[0]) * v[0] + double(v[1]) * v[1] + double(v[2]) * v[2]); }
+// a unit vector square to `a` (unit), from `u` where it is not along `a`
+void squareTo(const float* a, const float* u, float* out) {
+    double v[3] = {u[0], u[1], u[2]};
+    double d = v[0] * a[0] + v[1] * a[1] + v[2] * a[2];
+    for (int k = 0; k < 3; ++k) v[k] -= d * a[k];
+    double l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (!(l > 1e-3)) {
+        // any tangent: the axis crossed with the world axis it is least along
+        double w[3] = {0, 0, 0};
+        w[std::fabs(a[0]) < 0.9 ? 0 : 1] = 1.0;
+        v[0] = a[1] * w[2] - a[2] * w[1]; v[1] = a[2] * w[0] - a[0] * w[2]; v[2] = a[0] * w[1] - a[1] * w[0];
+        l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    }
+    for (int k = 0; k < 3; ++k) out[k] = float(v[k] / l);
+}
+hipError_t launchOpt(pt_ctx* c, int k, unsigned int grid, bool withQ, unsigned int qsel) {
+    void* args2[] = {&c->dev, &c->optDev};
+    void* args3[] = {&c->dev, &c->optDev, &qsel};
+    return hipModuleLaunchKernel(c->fn[k], grid, 1, 1, 256, 1, 1, 0, c->stream, withQ ? args3 : args2, nullptr);
+}
+}  // namespace
+
+int pt_optics_run(pt_ctx* c, const PtOpticsSource* src, uint32_t nsrc, const PtOpticsTrace* t, size_t tsize,
+                  PtOpticsTally* out, size_t osize, double* far, size_t nfar, float* hits, uint32_t hitCap) {
+    if (!c) return fail(c, PT_ERR_ARG, "ctx is NULL");
+    if (!t || !out || (nsrc && !src)) return fail(c, PT_ERR_ARG, "sources, trace or tally is NULL");
+    if (tsize != sizeof(PtOpticsTrace)) return fail(c, PT_ERR_ARG, "trace: %zu bytes, expected %zu", tsize, sizeof(PtOpticsTrace));
+    if (osize != sizeof(PtOpticsTally)) return fail(c, PT_ERR_ARG, "tally: %zu bytes, expected %zu", osize, sizeof(PtOpticsTally));
+    if (c->building) return fail(c, PT_ERR_STATE, "optics trace during a scene build");
+    if (!c->module) return fail(c, PT_ERR_STATE, "no scene loaded");
+    if (t->rays == 0) return fail(c, PT_ERR_ARG, "rays: 0");
+    if (t->bounces > 250) return fail(c, PT_ERR_ARG, "bounces %u above 250 (a ray counts its bounces in 8 bits)", t->bounces);
+    if (!(std::isfinite(t->maxDist) && t->maxDist >= 0.0f)) return fail(c, PT_ERR_ARG, "maxDist %g must be finite and >= 0", t->maxDist);
+    if (t->farMode > 2u) return fail(c, PT_ERR_ARG, "farMode %u not 0 (off), 1 (escaping) or 2 (a detector)", t->farMode);
+    if (t->marchSteps > 60000u) return fail(c, PT_ERR_ARG, "marchSteps %u above 60000 (a ray counts its steps in 16 bits)", t->marchSteps);
+    const size_t bins = t->farMode ? size_t(t->nC) * t->nG : 0;
+    if (t->farMode) {
+        if (t->nC < 1 || t->nC > 3600 || t->nG < 2 || t->nG > 1801) return fail(c, PT_ERR_ARG, "far field %u x %u: nC 1..3600, nG 2..1801", t->nC, t->nG);
+        if (!far || nfar != bins) return fail(c, PT_ERR_ARG, "far: %zu bins given, the grid has %zu", far ? nfar : size_t(0), bins);
+        if (t->farMode == 2u && t->farMaterial >= PT_MAX_MATERIALS) return fail(c, PT_ERR_ARG, "farMaterial %u not below %d", t->farMaterial, PT_MAX_MATERIALS);
+        if (!finite3(t->farAxis) || !(len3(t->farAxis) > 0.0) || !finite3(t->farC0)) return fail(c, PT_ERR_ARG, "far axis or C0 not a direction");
+    }
+    const unsigned int detCap = hits ? std::min(t->detCap, hitCap) : 0u;
+    // the sources with flux, checked; their hues at luminance 1; their cumulative shares
+    std::vector<PtOpticsSource> live;
+    double total = 0.0;
+    for (uint32_t i = 0; i < nsrc; ++i) {
+        PtOpticsSource q = src[i];
+        if (!(std::isfinite(q.flux) && q.flux > 0.0f)) continue;
+        if (q.shape > 2u || q.distribution > 2u) return fail(c, PT_ERR_ARG, "source %u: shape %u (0..2) or distribution %u (0..2)", i, q.shape, q.distribution);
+        if (!finite3(q.pos) || !finite3(q.axis) || !(len3(q.axis) > 0.0) || !finite3(q.u))
+            return fail(c, PT_ERR_ARG, "source %u: position, axis or tangent not finite (or no axis)", i);
+        if (!(std::isfinite(q.halfAngle) && std::isfinite(q.n) && q.n >= 0.0f && std::isfinite(q.size[0]) && q.size[0] >= 0.0f &&
+              std::isfinite(q.size[1]) && q.size[1] >= 0.0f))
+            return fail(c, PT_ERR_ARG, "source %u: half angle, n or size not finite (or negative)", i);
+        const double al = len3(q.axis);
+        for (int k = 0; k < 3; ++k) q.axis[k] = float(q.axis[k] / al);
+        squareTo(q.axis, q.u, q.u);
+        double Y = 0.2126 * q.colour[0] + 0.7152 * q.colour[1] + 0.0722 * q.colour[2];
+        if (!finite3(q.colour) || !(Y > 0.0)) { q.colour[0] = q.colour[1] = q.colour[2] = 1.0f; Y = 1.0; }
+        for (int k = 0; k < 3; ++k) q.colour[k] = float(q.colour[k] / Y);
+        if (!(q.emitN >= 1.0f) || !std::isfinite(q.emitN)) q.emitN = 1.0f;
+        live.push_back(q);
+        total += q.flux;
+    }
+    if (live.empty()) return fail(c, PT_ERR_ARG, "no sources: none with flux");
+    std::vector<float> cdf(live.size());
+    double cum = 0.0;
+    for (size_t i = 0; i < live.size(); ++i) { cum += live[i].flux; cdf[i] = i + 1 == live.size() ? 1.0f : float(cum / total); }
+    // the device table and its buffers
+    int rc;
+    if (!c->optDev && (rc = allocBuf(c, c->optDev, sizeof(PtOptDev)))) return rc;
+    if (!c->opt.tally && (rc = allocBuf(c, c->opt.tally, PT_OT_WORDS * 8))) return rc;
+    if (live.size() > c->optSrcCap) {
+        void* p = (void*)c->opt.src; freeBuf(c, p);
+        p = (void*)c->opt.cdf; freeBuf(c, p);
+        PtOpticsSource* ps = nullptr; float* pc = nullptr;
+        if ((rc = allocBuf(c, ps, live.size() * sizeof(PtOpticsSource))) || (rc = allocBuf(c, pc, live.size() * 4))) return rc;
+        c->opt.src = ps; c->opt.cdf = pc;
+        c->optSrcCap = (unsigned int)live.size();
+    }
+    if (bins > c->optFarCap) {
+        void* p = c->opt.far; freeBuf(c, p); c->opt.far = nullptr;
+        if ((rc = allocBuf(c, c->opt.far, bins * 8))) return rc;
+        c->optFarCap = (unsigned int)bins;
+    }
+    if (detCap > c->optHitCap) {
+        void* p = c->opt.hits; freeBuf(c, p); c->opt.hits = nullptr;
+        if ((rc = allocBuf(c, c->opt.hits, size_t(detCap) * PT_OPT_HIT_WORDS * 4))) return rc;
+        c->optHitCap = detCap;
+    }
+    // a batch as many rays as there are slots (at least 2^20); the draws are keyed by the ray's id, so batching does
+    // not change which rays are traced
+    const unsigned int batch = std::min(t->rays, std::max(c->capacity, 1u << 20));
+    // the trace's own step budget, through the engine's maxiter while it runs (restored after, whatever happens)
+    struct KeepMaxiter {
+        pt_ctx* c; unsigned int v;
+        ~KeepMaxiter() { if (c->host.u.maxiter != v) { c->host.u.maxiter = v; c->devDirty = true; } }
+    } keep{c, c->host.u.maxiter};
+    const unsigned int steps = t->marchSteps ? t->marchSteps : 2000u;
+    if (c->host.u.maxiter != steps) { c->host.u.maxiter = steps; c->devDirty = true; }
+    if ((rc = ensureSlots(c, batch)) || (rc = syncDev(c))) return rc;
+    PtOptDev& O = c->opt;
+    O.nsrc = (unsigned int)live.size();
+    O.bounces = t->bounces;
+    O.t = *t;
+    O.t.detCap = detCap;
+    if (t->farMode) {
+        const double al = len3(t->farAxis);
+        for (int k = 0; k < 3; ++k) O.t.farAxis[k] = float(t->farAxis[k] / al);
+        squareTo(O.t.farAxis, t->farC0, O.t.farC0);
+    }
+    // the trace's seed through a 64-bit mix (splitmix64), four words
+    uint64_t z = 0x9E3779B97F4A7C15ull * (uint64_t(t->seed) + 1u);
+    for (int k = 0; k < 4; ++k) {
+        z += 0x9E3779B97F4A7C15ull;
+        uint64_t x = z;
+        x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+        x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+        O.seed[k] = uint32_t(x ^ (x >> 31));
+    }
+    HIPCHK(c, hipMemcpyAsync((void*)O.src, live.data(), live.size() * sizeof(PtOpticsSource), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync((void*)O.cdf, cdf.data(), cdf.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(O.tally, 0, PT_OT_WORDS * 8, c->stream));
+    if (bins) HIPCHK(c, hipMemsetAsync(O.far, 0, bins * 8, c->stream));
+    const unsigned int R = t->bounces + 2u;     // the first segment, one per interaction, one for hits from behind
+    for (unsigned int first = 0; first < t->rays; first += batch) {
+        O.rays = std::min(batch, t->rays - first);
+        O.firstId = first;
+        HIPCHK(c, hipMemcpyAsync(c->optDev, &O, sizeof(PtOptDev), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, launchOpt(c, K_OPTGEN, (O.rays + 255u) / 256u, false, 0));
+        for (unsigned int it = 0; it < R; ++it) {
+            const unsigned int in = it & 1u;
+            HIPCHK(c, launch(c, K_BINCOUNT, gridFor(c, K_BINCOUNT), in));
+            HIPCHK(c, launch(c, K_BINSCAN, gridFor(c, K_BINSCAN), 0));
+            HIPCHK(c, launch(c, K_BINSCATTER, gridFor(c, K_BINSCATTER), in));
+            HIPCHK(c, launch(c, K_MARCH, gridFor(c, K_MARCH), in));
+            HIPCHK(c, launch(c, K_SURFACE, gridFor(c, K_SURFACE), in ^ 1u));
+            HIPCHK(c, launchOpt(c, K_OPTSHADE, c->host.gridBlocks, true, in ^ 1u));
+            HIPCHK(c, launch(c, K_OCCLUDE, gridFor(c, K_OCCLUDE), in));
+        }
+        HIPCHK(c, launchOpt(c, K_OPTFLUSH, c->host.gridBlocks, true, R & 1u));
+        // the table is rewritten for the next batch: this one's kernels finish first
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    std::vector<unsigned long long> T(PT_OT_WORDS);
+    HIPCHK(c, hipMemcpy(T.data(), O.tally, PT_OT_WORDS * 8, hipMemcpyDeviceToHost));
+    // weights to lumens: every ray carries the same share of the total flux
+    const double k = total / double(t->rays) / PT_OPT_FX;
+    memset(out, 0, sizeof *out);
+    out->emitted = total;
+    out->escaped = T[PT_OT_ESCAPED] * k;
+    out->lost = T[PT_OT_LOST] * k;
+    out->detected = T[PT_OT_DETECTED] * k;
+    out->volume = T[PT_OT_VOLUME] * k;
+    out->rays = t->rays;
+    out->escapedRays = T[PT_OT_CESC];
+    out->lostRays = T[PT_OT_CLOST];
+    out->interactions = T[PT_OT_CINTER];
+    out->absorbedRays = T[PT_OT_CABS];
+    out->detectedRays = T[PT_OT_CDET];
+    out->detectorHits = T[PT_OT_HITS];
+    double taken = 0.0;
+    for (int m = 0; m < PT_MAX_MATERIALS; ++m) {
+        out->incident[m] = T[PT_OT_MAT + 3 * m] * k;
+        out->leaving[m] = T[PT_OT_MAT + 3 * m + 1] * k;
+        out->hits[m] = T[PT_OT_MAT + 3 * m + 2];
+        taken += out->incident[m] - out->leaving[m];
+    }
+    out->absorbed = taken - out->detected + out->volume;
+    if (bins) {
+        std::vector<unsigned long long> F(bins);
+        HIPCHK(c, hipMemcpy(F.data(), O.far, bins * 8, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < bins; ++i) far[i] = F[i] * k;
+    }
+    const unsigned int nh = (unsigned int)std::min<unsigned long long>(T[PT_OT_HITS], detCap);
+    if (nh) {
+        HIPCHK(c, hipMemcpy(hits, O.hits, size_t(nh) * PT_OPT_HIT_WORDS * 4, hipMemcpyDeviceToHost));
+        const float kf = float(total / double(t->rays));
+        for (unsigned int i = 0; i < nh; ++i) hits[i * PT_OPT_HIT_WORDS + 3] *= kf;
+    }
     return PT_OK;
 }
 

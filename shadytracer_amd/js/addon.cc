@@ -5,6 +5,7 @@
 // async, raytrace_main.js:1145).
 #include <node_api.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdint>
 #include <cstring>
@@ -453,6 +454,97 @@ napi_value OpticsTrace(napi_env env, napi_callback_info info) {
     return arr;
 }
 
+// opticsRun(h, sources, nsrc, trace, nfar, hitCap) -> Promise of { tally: ArrayBuffer (PtOpticsTally), far: Float64Array
+// | null, hits: Float32Array }: the optics trace over the loaded scene on a worker thread (pt_optics_run). The inputs are
+// copied in and the outputs made when it is done, so nothing of the caller's is held meanwhile.
+struct OpticsRunWork {
+    napi_async_work work;
+    napi_deferred deferred;
+    pt_ctx* ctx;
+    std::vector<PtOpticsSource> src;
+    PtOpticsTrace t;
+    PtOpticsTally out;
+    std::vector<double> far;
+    std::vector<float> hits;
+    uint32_t hitCap;
+    int rc;
+    std::string err;
+};
+
+void opticsRunExecute(napi_env, void* data) {
+    OpticsRunWork* w = static_cast<OpticsRunWork*>(data);
+    w->rc = pt_optics_run(w->ctx, w->src.data(), (uint32_t)w->src.size(), &w->t, sizeof w->t, &w->out, sizeof w->out,
+                          w->far.empty() ? nullptr : w->far.data(), w->far.size(), w->hitCap ? w->hits.data() : nullptr, w->hitCap);
+    if (w->rc) w->err = pt_last_error(w->ctx);
+}
+
+void opticsRunComplete(napi_env env, napi_status, void* data) {
+    OpticsRunWork* w = static_cast<OpticsRunWork*>(data);
+    if (w->rc) {
+        napi_value msg, err;
+        std::string text = "libpt: " + w->err;
+        napi_create_string_utf8(env, text.c_str(), text.size(), &msg);
+        napi_create_error(env, nullptr, msg, &err);
+        napi_reject_deferred(env, w->deferred, err);
+    } else {
+        napi_value res, tally, far, hits, ab;
+        void* d;
+        napi_create_object(env, &res);
+        napi_create_arraybuffer(env, sizeof w->out, &d, &tally);
+        memcpy(d, &w->out, sizeof w->out);
+        napi_set_named_property(env, res, "tally", tally);
+        if (w->far.empty()) {
+            napi_get_null(env, &far);
+        } else {
+            napi_create_arraybuffer(env, w->far.size() * 8, &d, &ab);
+            memcpy(d, w->far.data(), w->far.size() * 8);
+            napi_create_typedarray(env, napi_float64_array, w->far.size(), ab, 0, &far);
+        }
+        napi_set_named_property(env, res, "far", far);
+        const size_t nh = (size_t)std::min<unsigned long long>(w->out.detectorHits, w->hitCap) * PT_OPT_HIT_WORDS;
+        napi_create_arraybuffer(env, nh * 4 + 4, &d, &ab);
+        if (nh) memcpy(d, w->hits.data(), nh * 4);
+        napi_create_typedarray(env, napi_float32_array, nh, ab, 0, &hits);
+        napi_set_named_property(env, res, "hits", hits);
+        napi_resolve_deferred(env, w->deferred, res);
+    }
+    napi_delete_async_work(env, w->work);
+    delete w;
+}
+
+napi_value OpticsRun(napi_env env, napi_callback_info info) {
+    napi_value a[6];
+    if (!args(env, info, 6, a)) return nullptr;
+    pt_ctx* ctx = ctxOf(env, a[0]);
+    if (!ctx) return nullptr;
+    uint32_t nsrc = 0, nfar = 0, hitCap = 0;
+    napi_get_value_uint32(env, a[2], &nsrc);
+    napi_get_value_uint32(env, a[4], &nfar);
+    napi_get_value_uint32(env, a[5], &hitCap);
+    void *sp = nullptr, *tp = nullptr;
+    size_t sl = 0, tl = 0;
+    if (!bytesOf(env, a[1], &sp, &sl) || !bytesOf(env, a[3], &tp, &tl)) return nullptr;
+    if (sl < (size_t)nsrc * sizeof(PtOpticsSource) || tl != sizeof(PtOpticsTrace)) {
+        napi_throw_range_error(env, nullptr, "sources smaller than nsrc structures, or the trace not a PtOpticsTrace");
+        return nullptr;
+    }
+    if (hitCap > (1u << 24)) { napi_throw_range_error(env, nullptr, "hitCap above 2^24"); return nullptr; }
+    OpticsRunWork* w = new OpticsRunWork();
+    w->ctx = ctx;
+    w->src.assign(static_cast<const PtOpticsSource*>(sp), static_cast<const PtOpticsSource*>(sp) + nsrc);
+    memcpy(&w->t, tp, sizeof w->t);
+    w->far.assign(nfar, 0.0);
+    w->hitCap = hitCap;
+    w->hits.assign(size_t(hitCap) * PT_OPT_HIT_WORDS, 0.0f);
+    w->rc = 0;
+    napi_value promise, name;
+    NAPI_OK(env, napi_create_promise(env, &w->deferred, &promise));
+    napi_create_string_utf8(env, "pt_optics_run", NAPI_AUTO_LENGTH, &name);
+    NAPI_OK(env, napi_create_async_work(env, nullptr, name, opticsRunExecute, opticsRunComplete, w, &w->work));
+    NAPI_OK(env, napi_queue_async_work(env, w->work));
+    return promise;
+}
+
 napi_value Init(napi_env env, napi_value exports) {
     napi_property_descriptor props[] = {
         {"create", nullptr, Create, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
@@ -466,6 +558,7 @@ napi_value Init(napi_env env, napi_value exports) {
         {"setUniforms", nullptr, SetUniforms, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"clearAccum", nullptr, ClearAccum, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"trace", nullptr, Trace, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"opticsRun", nullptr, OpticsRun, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"readAccum", nullptr, ReadAccum, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"readStats", nullptr, ReadStats, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"opticsTrace", nullptr, OpticsTrace, nullptr, nullptr, nullptr, napi_enumerable, nullptr},

@@ -425,7 +425,43 @@ function joins(opts = {}) {
     return { doc: d, camera: cameraOf(d, w, h), settings: { width: w, height: h, spp: 64, maxbounce: 8, nee: 1 } }
 }
 
-const CONFIGS = { 'plane-c1': planeC1, 'room-c2': roomC2, 'caustic-c3': causticC3, 'furnace-plane': furnacePlane, 'furnace-sphere': furnaceSphere,
+// THE OPTICS TRACE'S TEST SCENES (lib/pathtracer/tests.js optics-balance :1060-1100, optics-farfield :1101-1137,
+// optics-refraction :1138-1200): a floor, a closed shell round a point source, a glass slab over a floor, a glass ball.
+// Material 0 is the one the rows name (the floor, the wall, the glass); the slab scene's floor is material 0 (black, a
+// detector in its rows) and its glass material 1. No environment: what leaves the scene has escaped.
+function opticsFloor(opts = {}) {
+    const num = (v, d) => v !== undefined ? v : d
+    const b = num(opts.albedo, 0.5)
+    const mats = [material('floor', [b, b, b], num(opts.roughness, 1), num(opts.metallic, 0), { specular: num(opts.specular, 0) })]
+    const d = doc('optics-floor', mats, [obj('floor', 'b-plane', 'floor', xf(0, 0, 0))], { sky: 'none' }, 20)
+    return { doc: d, camera: camera([0, -3, 1.5], [0, 0, 0], 60, 32, 32), settings: { width: 32, height: 32, spp: 1, maxbounce: 8, nee: 1, eps: 1e-5 } }
+}
+function opticsShell(opts = {}) {
+    const b = opts.albedo !== undefined ? opts.albedo : 0.8
+    const mats = [material('wall', [b, b, b], 1, 0, { specular: 0 })]
+    const shell = { id: 'shell', kind: 'group', label: 'shell', enabled: true, blend: 0, transform: xf(0, 0, 0), children: [
+        obj('outer', 'b-sphere', 'wall', xf(0, 0, 0), { radius: 1.05 }),
+        obj('inner', 'b-sphere', 'wall', xf(0, 0, 0), { radius: 1.0 }, { join: 'subtract' }),
+    ] }
+    const d = doc('optics-shell', mats, [shell], { sky: 'none' }, 20)
+    return { doc: d, camera: camera([0, -0.5, 0.2], [0, 0, 0], 60, 32, 32), settings: { width: 32, height: 32, spp: 1, maxbounce: 8, nee: 1, eps: 1e-5 } }
+}
+function opticsSlab(opts = {}) {
+    const t = opts.thickness !== undefined ? opts.thickness : 0.1
+    const mats = [material('floor', [0, 0, 0], 1, 0, { specular: 0 }),
+        material('glass', [1, 1, 1], 0, 0, { transmission: 1, ior: opts.ior || 1.5, specular: 0.5 })]
+    const d = doc('optics-slab', mats, [obj('floor', 'b-plane', 'floor', xf(0, 0, 0)),
+        obj('slab', 'b-box', 'glass', xf(0, 0, opts.z !== undefined ? opts.z : 0.5), { size: { x: 4, y: 4, z: t / 2 }, rounding: 0 })], { sky: 'none' }, 20)
+    return { doc: d, camera: camera([0, -3, 1.5], [0, 0, 0], 60, 32, 32), settings: { width: 32, height: 32, spp: 1, maxbounce: 8, nee: 1, eps: 1e-5 } }
+}
+function opticsBall(opts = {}) {
+    const mats = [material('glass', [1, 1, 1], 0, 0, { transmission: 1, ior: opts.ior || 1.5, specular: 0.5 })]
+    const d = doc('optics-ball', mats, [obj('ball', 'b-sphere', 'glass', xf(0, 0, 0), { radius: opts.radius || 0.5 })], { sky: 'none' }, 20)
+    return { doc: d, camera: camera([0, -3, 1.5], [0, 0, 0], 60, 32, 32), settings: { width: 32, height: 32, spp: 1, maxbounce: 8, nee: 1, eps: 1e-5 } }
+}
+
+const CONFIGS = { 'plane-c1': planeC1, 'optics-floor': opticsFloor, 'optics-shell': opticsShell, 'optics-slab': opticsSlab,
+    'optics-ball': opticsBall, 'room-c2': roomC2, 'caustic-c3': causticC3, 'furnace-plane': furnacePlane, 'furnace-sphere': furnaceSphere,
     'furnace-haze': furnaceHaze,
     'sppm-slab': sppmSlab, 'studio-floor': studioFloor, 'lens-prism': lensPrism, 'reflector-condenser': reflectorCondenser,
     'turned-parts': turnedParts, gallery, sunlit, 'sunsky-floor': sunskyFloor, 'sunsky-bench': sunskyBench, joins }

@@ -109,6 +109,9 @@ function createPathTracer(opts = {}) {
     RT.readPixel = (...a) => R.readPixel(...a)
     RT.measure = (x0, y0, x1, y1) => R.readRegion(x0, y0, x1, y1)
     RT.wavefrontLog = () => R.wavefrontLog()
+    // for createOptics(RT): the trace over this scene, and the scene it runs on
+    RT.opticsRun = (...a) => R.opticsRun(...a)
+    RT.scene = () => R.scene()
     RT.dispose = () => R.releaseBuffers()
     return RT
 }

@@ -132,6 +132,14 @@ function RAYTRACER(WG, _U, Q, scene) {
         return buf
     }
 
+    // The optics trace over the loaded scene (pt_optics_run, on a worker thread): the packed sources and trace, the
+    // far field's bin count and the detector hits kept; resolves with { tally, far, hits } (optics.js createOptics)
+    R.opticsRun = async (sources, nsrc, trace, nfar, hitCap) => {
+        if (building) throw new Error('RAYTRACER: optics trace during a scene build')
+        if (!current) throw new Error('RAYTRACER: no scene')
+        A.setUniforms(h, packUniforms())
+        return await A.opticsRun(h, sources, nsrc, trace, nfar, hitCap)
+    }
     R.clearAccums = () => { if (building) throw new Error('RAYTRACER: clear during a scene build'); A.clearAccum(h) }
     // the persistent kernels' grid and refill thresholds picked by timing candidates (pt_autotune); clears the picture
     R.autotune = (samples = 4) => {

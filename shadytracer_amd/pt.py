@@ -108,8 +108,34 @@ class PtTuning(ctypes.Structure):
                 ('candidates', ctypes.c_uint), ('msPerSample', ctypes.c_double), ('msBefore', ctypes.c_double)]
 
 
+class PtOpticsSource(ctypes.Structure):
+    _fields_ = [('pos', ctypes.c_float * 3), ('shape', ctypes.c_uint), ('axis', ctypes.c_float * 3), ('distribution', ctypes.c_uint),
+                ('u', ctypes.c_float * 3), ('halfAngle', ctypes.c_float), ('size', ctypes.c_float * 2), ('n', ctypes.c_float),
+                ('flux', ctypes.c_float), ('colour', ctypes.c_float * 3), ('emitN', ctypes.c_float), ('twoSided', ctypes.c_uint),
+                ('onSurface', ctypes.c_uint), ('pad', ctypes.c_uint * 2)]
+
+
+class PtOpticsTrace(ctypes.Structure):
+    _fields_ = [('rays', ctypes.c_uint), ('bounces', ctypes.c_uint), ('seed', ctypes.c_uint), ('detCap', ctypes.c_uint),
+                ('maxDist', ctypes.c_float), ('farMode', ctypes.c_uint), ('farMaterial', ctypes.c_uint), ('nC', ctypes.c_uint),
+                ('nG', ctypes.c_uint), ('farAxis', ctypes.c_float * 3), ('farC0', ctypes.c_float * 3),
+                ('detectors', ctypes.c_uint * 2), ('marchSteps', ctypes.c_uint), ('pad', ctypes.c_uint)]
+
+
+MAX_MATERIALS = 64
+OPT_HIT_WORDS = 8
+
+
+class PtOpticsTally(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in ('emitted', 'escaped', 'lost', 'detected', 'volume', 'absorbed')] + \
+               [(k, ctypes.c_uint64) for k in ('rays', 'escapedRays', 'lostRays', 'interactions', 'absorbedRays', 'detectedRays',
+                                               'detectorHits', 'pad')] + \
+               [('incident', ctypes.c_double * MAX_MATERIALS), ('leaving', ctypes.c_double * MAX_MATERIALS),
+                ('hits', ctypes.c_uint64 * MAX_MATERIALS)]
+
+
 STRUCTS = [PtUniforms, PtMaterial, PtLightFace, PtCamera, PtEnv, PtHaze, PtSceneDesc, PtOpts, PtGuideConfig, PtOpticsScene,
-           PtOpticsResult, PtTuning]
+           PtOpticsResult, PtTuning, PtOpticsSource, PtOpticsTrace, PtOpticsTally]
 
 # the C ABI, name -> (restype, argtypes)
 SIGNATURES = {
@@ -141,6 +167,9 @@ SIGNATURES = {
     'pt_optics_trace': (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(PtOpticsScene), ctypes.c_uint32, ctypes.POINTER(PtOpticsResult),
                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     'pt_optics_error': (ctypes.c_char_p, []),
+    'pt_optics_run': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PtOpticsSource), ctypes.c_uint32, ctypes.POINTER(PtOpticsTrace),
+                                     ctypes.c_size_t, ctypes.POINTER(PtOpticsTally), ctypes.c_size_t, ctypes.POINTER(ctypes.c_double),
+                                     ctypes.c_size_t, ctypes.POINTER(ctypes.c_float), ctypes.c_uint32]),
     'pt_optics_trace_static': (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(PtOpticsScene), ctypes.c_uint32, ctypes.POINTER(PtOpticsResult),
                                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     'pt_get_tuning': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PtTuning), ctypes.c_size_t]),
@@ -365,6 +394,58 @@ class Engine:
     def reset_stats(self):
         self._chk(self.lib.pt_reset_stats(self.h))
 
+    # ---- the optics trace over the loaded scene (createOptics: optics/index.js)
+    def optics_run(self, sources, rays=100000, bounces=64, seed=1, far=None, detectors=(), det_cap=0, max_dist=0.0,
+                   march_steps=0):
+        """Traces `rays` rays from `sources` (dicts as optics/sources.js describes them: type 'point' | 'disc' |
+        'rectangle', pos, axis, u, size, distribution 'collimated' | 'uniform' | 'cosine', halfAngle (deg), n, flux (lm),
+        colour, emitN, twoSided, onSurface; missing keys from default_source()) through the loaded scene, in metres.
+        far: None, or {'mode': 'escaped' | 'detector', 'material', 'nC', 'nG', 'axis', 'c0'}; detectors: material ids
+        whose surfaces stop and record rays; det_cap: hits kept. Returns the tallies in lm, per material, the far field
+        ({'flux': [nG, nC] lm, ...}) and the hits ([k, 8]: position, lm, direction, material). march_steps: the
+        sphere-tracing steps a ray may take over its whole path (0: 2000)."""
+        srcs = [dict(default_source(), **sd) for sd in sources]
+        arr = (PtOpticsSource * max(1, len(srcs)))()
+        for a, sd in zip(arr, srcs):
+            a.pos[:] = [float(x) for x in sd['pos']]
+            a.shape = SOURCE_SHAPES[sd['type']]
+            a.axis[:] = [float(x) for x in sd['axis']]
+            a.distribution = DISTRIBUTIONS[sd['distribution']]
+            a.u[:] = [float(x) for x in sd['u']]
+            a.halfAngle = float(sd['halfAngle'])
+            size = list(sd['size']) if hasattr(sd['size'], '__len__') else [sd['size'], sd['size']]
+            a.size[:] = [float(size[0]), float(size[1] if len(size) > 1 else size[0])]
+            a.n, a.flux, a.emitN = float(sd['n']), float(sd['flux']), float(sd['emitN'])
+            a.colour[:] = [float(x) for x in sd.get('colour', (1.0, 1.0, 1.0))]
+            a.twoSided, a.onSurface = int(bool(sd['twoSided'])), int(bool(sd.get('onSurface', False)))
+        t = PtOpticsTrace()
+        t.rays, t.bounces, t.seed, t.detCap, t.maxDist = rays, bounces, seed, det_cap, max_dist
+        t.marchSteps = march_steps
+        for m in detectors:
+            t.detectors[m >> 5] |= 1 << (m & 31)
+        F = None
+        if far:
+            f = dict(FAR_DEFAULT, **far)
+            t.farMode = {'off': 0, 'escaped': 1, 'detector': 2}[f['mode']]
+            t.farMaterial, t.nC, t.nG = f['material'], f['nC'], f['nG']
+            t.farAxis[:] = [float(x) for x in f['axis']]
+            t.farC0[:] = [float(x) for x in f['c0']]
+            if t.farMode:
+                F = np.zeros((f['nG'], f['nC']))
+        H = np.zeros((max(det_cap, 1), OPT_HIT_WORDS), np.float32)
+        out = PtOpticsTally()
+        self._chk(self.lib.pt_optics_run(self.h, arr, len(srcs), ctypes.byref(t), ctypes.sizeof(t), ctypes.byref(out),
+                                         ctypes.sizeof(out),
+                                         F.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if F is not None else None,
+                                         F.size if F is not None else 0,
+                                         H.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), det_cap))
+        res = {k: getattr(out, k) for k, _ in PtOpticsTally._fields_ if k not in ('incident', 'leaving', 'hits', 'pad')}
+        res['materials'] = [{'incident': out.incident[m], 'leaving': out.leaving[m], 'hits': int(out.hits[m]),
+                             'absorbed': out.incident[m] - out.leaving[m]} for m in range(MAX_MATERIALS)]
+        res['far'] = None if F is None else dict(dict(FAR_DEFAULT, **far), flux=F, emitted=out.emitted)
+        res['hits'] = H[:min(det_cap, out.detectorHits)].copy()
+        return res
+
     def time_kernels(self, samples=1):
         ms = (ctypes.c_double * 12)()
         self._chk(self.lib.pt_time_kernels(self.h, samples, ms, 12))
@@ -506,6 +587,18 @@ def scene_box(scene):
         return [-R] * 3, [R] * 3
     pad = [0.01 * (h - l) + 1e-3 for l, h in zip(lo, hi)]
     return [l - p for l, p in zip(lo, pad)], [h + p for h, p in zip(hi, pad)]
+
+
+SOURCE_SHAPES = {'point': 0, 'disc': 1, 'rectangle': 2}
+DISTRIBUTIONS = {'collimated': 0, 'uniform': 1, 'cosine': 2}
+FAR_DEFAULT = {'mode': 'off', 'material': 0, 'nC': 72, 'nG': 181, 'axis': (0.0, 0.0, 1.0), 'c0': (1.0, 0.0, 0.0)}
+
+
+def default_source():
+    """optics/sources.js defaultSource: a 1 x 1 Lambertian square 1 up, facing down, 1000 lm, into air, one face."""
+    return {'type': 'rectangle', 'pos': (0.0, 0.0, 1.0), 'axis': (0.0, 0.0, -1.0), 'u': (1.0, 0.0, 0.0), 'size': (0.5, 0.5),
+            'distribution': 'cosine', 'halfAngle': 90.0, 'n': 1.0, 'flux': 1000.0, 'emitN': 1.0, 'twoSided': False,
+            'onSurface': False, 'colour': (1.0, 1.0, 1.0)}
 
 
 def optics_trace(scenes, far=False, lux=False, device=0):

@@ -7,7 +7,8 @@ import os
 import sys
 
 SRC = '/root/reference/lib/pathtracer/test/fixtures/run-0.77.0-pascal.json'
-KEEP = ['furnace', 'haze', 'photometry', 'illuminance', 'cosine-law', 'bsdf-parity', 'optics-refraction', 'optics-farfield']
+KEEP = ['furnace', 'haze', 'photometry', 'illuminance', 'cosine-law', 'bsdf-parity', 'optics-refraction', 'optics-farfield',
+        'optics-balance']
 
 
 def main():
@@ -15,7 +16,8 @@ def main():
     out = {'source': 'lib/pathtracer/test/fixtures/run-0.77.0-pascal.json', 'tests': {}}
     for k in KEEP:
         t = d['tests'][k]
-        rows = [{f: r.get(f) for f in ('case', 'meanLuma', 'relSE', 'expected', 'value') if f in r} for r in t.get('rows', [])]
+        rows = [{f: r.get(f) for f in ('case', 'meanLuma', 'relSE', 'expected', 'value', 'measured', 'rays') if f in r}
+                for r in t.get('rows', [])]
         out['tests'][k] = {'pixels': t.get('pixels'), 'rows': rows}
     here = os.path.dirname(os.path.abspath(__file__))
     json.dump(out, open(os.path.join(here, 'reference_rows.json'), 'w'), indent=1)

@@ -3,10 +3,10 @@
 // tests/test_optics_gpu.py.
 const path = require('path')
 const fs = require('fs')
-const { createOptics } = require(path.join(__dirname, '..', '..', 'shadytracer_amd', 'js', 'optics'))
+const { createOpticsBench } = require(path.join(__dirname, '..', '..', 'shadytracer_amd', 'js', 'optics'))
 
 const scenes = JSON.parse(fs.readFileSync(process.argv[2], 'utf8'))
-const O = createOptics({ device: 0 })
+const O = createOpticsBench({ device: 0 })
 const { results, far, walkMs } = O.run(scenes, { far: true })
 let farSum = 0
 for (let i = 0; i < far.length; i++) farSum += far[i]

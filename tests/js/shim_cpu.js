@@ -11,7 +11,8 @@ const { createPathTracer } = require(path.join(js, 'index'))
 const out = {}
 const a = m.loadAddon()
 const optics = require(path.join(js, 'optics'))
-optics.createOptics({ device: 0 })   // checks the optics layouts against the addon's structure sizes; no GPU call
+optics.createOpticsBench({ device: 0 })   // checks the bench's layouts against the addon's structure sizes; no GPU call
+out.traceSizes = [m.sizeOf(optics.SOURCE), a.structSize(12), m.sizeOf(optics.TRACE), a.structSize(13), optics.TALLY_BYTES, a.structSize(14)]
 out.opticsSizes = [m.sizeOf(optics.SCENE), a.structSize(9), optics.RESULT.length * 8, a.structSize(10)]
 out.sizes = [m.sizeOf(m.UNIFORMS), a.structSize(0), m.sizeOf(m.MATERIAL), a.structSize(1), m.sizeOf(m.FACE), a.structSize(2)]
 out.addonKeys = Object.keys(a).sort()
