@@ -63,7 +63,8 @@ int pt_abi_version(void);
 
 /* sizeof of the ABI's structures, for bindings to check their mirrors: 0 PtUniforms, 1 PtMaterial,
  * 2 PtLightFace, 3 PtCamera, 4 PtEnv, 5 PtHaze, 6 pt_scene_desc, 7 pt_opts, 8 PtGuideConfig, 9 PtOpticsScene,
- * 10 PtOpticsResult, 11 PtTuning, 12 PtOpticsSource, 13 PtOpticsTrace, 14 PtOpticsTally; 0 for anything else.
+ * 10 PtOpticsResult, 11 PtTuning, 12 PtOpticsSource, 13 PtOpticsTrace, 14 PtOpticsTally, 15 PtDevelop,
+ * 16 PtDevelopInfo; 0 for anything else.
  * No counterpart in the reference (its uniform packing lives in the absent tinywebgpu submodule). */
 size_t pt_struct_size(int which);
 
@@ -176,6 +177,17 @@ int pt_guide_train(pt_ctx* ctx, const float* samples, size_t n, float* grad, dou
 /* The training's state after the last sample: out[0] Adam steps taken, out[1] training samples the sample
  * collected, out[2 + j] the loss of its step j (j < steps); n >= 2. */
 int pt_guide_status(pt_ctx* ctx, double* out, size_t n);
+
+/* ---- the developed picture ---- */
+
+/* The accumulated picture exposed, tone mapped and encoded as 8-bit sRGB RGBA (npx = width x height pixels, rows top
+ * first with d->flipY): with d->autoExposure the exposure is metered from the picture (a log2-luminance histogram's
+ * trimmed average, highlights protected, eased from the last call; the first call after pt_clear_accum sets it), else
+ * it is d->exposureEV. info (NULL allowed) says what was used and metered.
+ * Replaces the display pass and the auto-exposure kernels (lib/pathtracer/core/rendertoscreen.js displayPixel
+ * :209-260 with its tone curves :35-150; core/autoexposure.js prog_ae_histogram / prog_ae_update :54-153) for a
+ * picture that is saved rather than shown (darkroom/export/png.js takes its result). */
+int pt_develop(pt_ctx* ctx, const PtDevelop* d, size_t dsize, uint8_t* rgba, size_t npx, PtDevelopInfo* info);
 
 /* ---- the optics trace over the loaded scene ---- */
 

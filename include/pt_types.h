@@ -285,6 +285,23 @@ typedef struct PtOpticsTally {
 #define PT_OT_MAT 16           // then per material: incident, leaving, hits
 #define PT_OT_WORDS (PT_OT_MAT + 3 * PT_MAX_MATERIALS)
 
+// ---- The developed picture (core/rendertoscreen.js displayPixel, core/autoexposure.js): exposure, tone curve, 8-bit sRGB.
+typedef struct PtDevelop {
+    unsigned int toneMap;        // 0 linear (clipped at white), 1 PBR Neutral, 2 AgX, 3 AgX Punchy, 4 ACES (fitted),
+                                 // 5 Reinhard (luminance), 6 Hable (the reference's toneMap numbering)
+    unsigned int autoExposure;   // 1: metered from the picture (the exposure eases by aeSpeed a call); 0: exposureEV
+    float exposureEV;            // EV100 when not metered: white is 1.2 x 2^EV cd/m^2
+    float aeSpeed;               // share of the gap to the metered EV closed a call (1: jump; the first call jumps)
+    float aeHeadroom;            // stops the 95 % quantile may sit above white before the meter darkens the picture
+    unsigned int dither;         // 1: +-0.5 of an 8-bit step of per-pixel noise before rounding
+    unsigned int flipY;          // 1: rows top first (images, PNG); 0: as accumulated, bottom first
+    unsigned int pad;
+} PtDevelop;
+// What developing measured: the exposure used, the meter's trimmed log2 average and protected quantile, pixels metered.
+typedef struct PtDevelopInfo {
+    float ev, log2Average, log2High, metered;
+} PtDevelopInfo;
+
 typedef struct PtUniforms {
     unsigned int width, height;
     unsigned int maxbounce;     // 10

@@ -27,10 +27,11 @@ namespace {
 const char* kKernelNames[] = {"pt_genrays", "pt_march", "pt_shade", "pt_occlude", "pt_resolve", "pt_mf_lut", "pt_mf_lut_avg", "pt_pick",
                               "pt_bin_count", "pt_bin_scan", "pt_bin_scatter", "pt_sppm_clear", "pt_photon_gen", "pt_surface",
                               "pt_guide_infer", "pt_guide_eval", "pt_guide_collect", "pt_guide_grad", "pt_guide_adam", "pt_guide_commit",
-                              "pt_flush", "pt_opt_gen", "pt_opt_shade", "pt_opt_flush"};
+                              "pt_flush", "pt_opt_gen", "pt_opt_shade", "pt_opt_flush", "pt_ae_hist", "pt_ae_update",
+                              "pt_develop"};
 enum { K_GENRAYS, K_MARCH, K_SHADE, K_OCCLUDE, K_RESOLVE, K_MFLUT, K_MFAVG, K_PICK, K_BINCOUNT, K_BINSCAN, K_BINSCATTER,
        K_SPPMCLEAR, K_PHOTONGEN, K_SURFACE, K_GINFER, K_GEVAL, K_GCOLLECT, K_GGRAD, K_GADAM, K_GCOMMIT, K_FLUSH, K_OPTGEN, K_OPTSHADE,
-       K_OPTFLUSH, K_COUNT };
+       K_OPTFLUSH, K_AEHIST, K_AEUPDATE, K_DEVELOP, K_COUNT };
 
 // the guide network's sizes (device/pt_guide.h)
 const unsigned int kGuideNP = 16169, kGuideWT = 17408, kGuideBias = 256, kGuideParts = 256, kGuideMaxSteps = 64;
@@ -179,6 +180,13 @@ struct pt_ctx {
     uint64_t graphKey = 0;       // what the captured graph's launches were sized for (rounds, rays, photons)
 
     unsigned int sppmCapacity = 0;   // photon records allocated
+
+    // the developed picture (pt_develop): the meter's histogram and state, the settings and the 8-bit picture on the device
+    unsigned int* aeHist = nullptr;
+    float* aeState = nullptr;
+    PtDevelop* devSet = nullptr;
+    unsigned char* devRgba = nullptr;
+    size_t devRgbaPx = 0;
 
     // the optics trace (pt_optics_run): its table on the device and the buffers it points at
     PtOptDev opt{};
@@ -592,6 +600,8 @@ size_t pt_struct_size(int which) {
         case 12: return sizeof(PtOpticsSource);
         case 13: return sizeof(PtOpticsTrace);
         case 14: return sizeof(PtOpticsTally);
+        case 15: return sizeof(PtDevelop);
+        case 16: return sizeof(PtDevelopInfo);
         default: return 0;
     }
 }
@@ -832,6 +842,8 @@ int pt_clear_accum(pt_ctx* c) {
     // every counter but the guide network's step count (its training outlives the picture)
     HIPCHK(c, hipMemsetAsync(c->host.counters, 0, PT_C_GUIDE_T * 4, c->stream));
     HIPCHK(c, hipMemsetAsync(c->host.counters + PT_C_GUIDE_T + 1, 0, (PT_C_WORDS - PT_C_GUIDE_T - 1) * 4, c->stream));
+    // a new picture: the meter sets its exposure afresh at the next development
+    if (c->aeState) HIPCHK(c, hipMemsetAsync(c->aeState, 0, 8 * 4, c->stream));
     return PT_OK;
 }
 
@@ -1175,6 +1187,52 @@ int pt_optics_run(pt_ctx* c, const PtOpticsSource* src, uint32_t nsrc, const PtO
         HIPCHK(c, hipMemcpy(hits, O.hits, size_t(nh) * PT_OPT_HIT_WORDS * 4, hipMemcpyDeviceToHost));
         const float kf = float(total / double(t->rays));
         for (unsigned int i = 0; i < nh; ++i) hits[i * PT_OPT_HIT_WORDS + 3] *= kf;
+    }
+    return PT_OK;
+}
+
+int pt_develop(pt_ctx* c, const PtDevelop* d, size_t dsize, uint8_t* rgba, size_t npx, PtDevelopInfo* info) {
+    if (!c) return fail(c, PT_ERR_ARG, "ctx is NULL");
+    if (!d || !rgba) return fail(c, PT_ERR_ARG, "settings or rgba is NULL");
+    if (dsize != sizeof(PtDevelop)) return fail(c, PT_ERR_ARG, "develop: %zu bytes, expected %zu", dsize, sizeof(PtDevelop));
+    if (c->building) return fail(c, PT_ERR_STATE, "develop during a scene build");
+    if (!c->module) return fail(c, PT_ERR_STATE, "no scene loaded");
+    const size_t px = size_t(c->host.u.width) * c->host.u.height;
+    if (npx != px) return fail(c, PT_ERR_ARG, "npx %zu, the picture has %zu pixels", npx, px);
+    if (d->toneMap > 6u) return fail(c, PT_ERR_ARG, "toneMap %u not in 0..6", d->toneMap);
+    if (!(std::isfinite(d->exposureEV) && std::isfinite(d->aeSpeed) && std::isfinite(d->aeHeadroom)))
+        return fail(c, PT_ERR_ARG, "exposureEV, aeSpeed and aeHeadroom must be finite");
+    int rc;
+    if (!c->aeHist) {
+        if ((rc = allocBuf(c, c->aeHist, 128 * 4)) || (rc = allocBuf(c, c->aeState, 8 * 4)) || (rc = allocBuf(c, c->devSet, sizeof(PtDevelop))))
+            return rc;
+        HIPCHK(c, hipMemsetAsync(c->aeHist, 0, 128 * 4, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->aeState, 0, 8 * 4, c->stream));
+    }
+    if (px > c->devRgbaPx) {
+        void* p = c->devRgba; freeBuf(c, p); c->devRgba = nullptr;
+        if ((rc = allocBuf(c, c->devRgba, px * 4))) return rc;
+        c->devRgbaPx = px;
+    }
+    if ((rc = syncDev(c))) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->devSet, d, sizeof(PtDevelop), hipMemcpyHostToDevice, c->stream));
+    if (d->autoExposure) {
+        void* ah[] = {&c->dev, &c->aeHist};
+        HIPCHK(c, hipModuleLaunchKernel(c->fn[K_AEHIST], c->cus * 4u, 1, 1, 256, 1, 1, 0, c->stream, ah, nullptr));
+        void* au[] = {&c->aeHist, &c->aeState, &c->devSet};
+        HIPCHK(c, hipModuleLaunchKernel(c->fn[K_AEUPDATE], 1, 1, 1, 64, 1, 1, 0, c->stream, au, nullptr));
+    }
+    void* dv[] = {&c->dev, &c->devRgba, &c->aeState, &c->devSet};
+    HIPCHK(c, hipModuleLaunchKernel(c->fn[K_DEVELOP], (unsigned int)((px + 255) / 256), 1, 1, 256, 1, 1, 0, c->stream, dv, nullptr));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(rgba, c->devRgba, px * 4, hipMemcpyDeviceToHost));
+    if (info) {
+        float st[8];
+        HIPCHK(c, hipMemcpy(st, c->aeState, sizeof st, hipMemcpyDeviceToHost));
+        info->ev = d->autoExposure ? st[0] : d->exposureEV;
+        info->log2Average = st[2];
+        info->log2High = st[3];
+        info->metered = d->autoExposure ? st[4] : 0.0f;
     }
     return PT_OK;
 }

@@ -79,3 +79,26 @@ def i0_fit(I, max_gamma=20.0):
     Ir = I[rows].mean(axis=1)
     c = np.cos(g[rows])
     return float((Ir * c).sum() / (c * c).sum())
+
+
+def intensity_csv(I, unit='cd', se=None):
+    """The grid as CSV text (farfield.js intensityCSV :170): a comment line, a header of gamma and one column per C
+    (each followed by its standard error when `se` is given), then one row per gamma."""
+    nG, nC = I.shape
+    dC, dG = 360.0 / nC, 180.0 / (nG - 1)
+    fmt = lambda x: ('%.4f' % x).rstrip('0').rstrip('.')
+    head = ['gamma']
+    for i in range(nC):
+        head.append('C' + fmt(i * dC))
+        if se is not None:
+            head.append('se_C' + fmt(i * dC))
+    rows = ['# type C grid, %s: a row per gamma (degrees from the photometric axis), a column per C (degrees about it from '
+            'C0)' % unit, ','.join(head)]
+    for j in range(nG):
+        r = [fmt(j * dG)]
+        for i in range(nC):
+            r.append(repr(float(I[j, i])))
+            if se is not None:
+                r.append(repr(float(se[j, i])))
+        rows.append(','.join(r))
+    return '\n'.join(rows)

@@ -454,6 +454,28 @@ napi_value OpticsTrace(napi_env env, napi_callback_info info) {
     return arr;
 }
 
+// develop(h, settings (PtDevelop bytes), rgba (Uint8Array of w x h x 4)) -> [ev, log2Average, log2High, metered]
+napi_value Develop(napi_env env, napi_callback_info info) {
+    napi_value a[3];
+    if (!args(env, info, 3, a)) return nullptr;
+    pt_ctx* ctx = ctxOf(env, a[0]);
+    if (!ctx) return nullptr;
+    void *sp = nullptr, *op = nullptr;
+    size_t sl = 0, ol = 0;
+    if (!bytesOf(env, a[1], &sp, &sl) || !bytesOf(env, a[2], &op, &ol)) return nullptr;
+    if (sl != sizeof(PtDevelop) || ol % 4) { napi_throw_range_error(env, nullptr, "settings not a PtDevelop, or rgba not 4 bytes a pixel"); return nullptr; }
+    PtDevelopInfo di;
+    int rc = pt_develop(ctx, static_cast<const PtDevelop*>(sp), sl, static_cast<uint8_t*>(op), ol / 4, &di);
+    if (rc) return throwPt(env, ctx, rc);
+    napi_value arr;
+    NAPI_OK(env, napi_create_array_with_length(env, 4, &arr));
+    napi_set_element(env, arr, 0, num(env, di.ev));
+    napi_set_element(env, arr, 1, num(env, di.log2Average));
+    napi_set_element(env, arr, 2, num(env, di.log2High));
+    napi_set_element(env, arr, 3, num(env, di.metered));
+    return arr;
+}
+
 // opticsRun(h, sources, nsrc, trace, nfar, hitCap) -> Promise of { tally: ArrayBuffer (PtOpticsTally), far: Float64Array
 // | null, hits: Float32Array }: the optics trace over the loaded scene on a worker thread (pt_optics_run). The inputs are
 // copied in and the outputs made when it is done, so nothing of the caller's is held meanwhile.
@@ -559,6 +581,7 @@ napi_value Init(napi_env env, napi_value exports) {
         {"clearAccum", nullptr, ClearAccum, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"trace", nullptr, Trace, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"opticsRun", nullptr, OpticsRun, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"develop", nullptr, Develop, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"readAccum", nullptr, ReadAccum, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"readStats", nullptr, ReadStats, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"opticsTrace", nullptr, OpticsTrace, nullptr, nullptr, nullptr, napi_enumerable, nullptr},

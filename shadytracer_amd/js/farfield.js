@@ -61,4 +61,20 @@ function width(R, C = 0, level = 0.5) {
     return lo === null || hi === null ? null : hi - lo
 }
 
-module.exports = { intensity, plane, width, omegaOf }
+/** The grid as CSV text: a comment line, a header of gamma and one column per C (each followed by its standard error
+ *  when R.se is given), then one row per gamma (optics/farfield.js intensityCSV :170). */
+function intensityCSV(R, unit = 'cd') {
+    const fmt = x => String(+x.toFixed(4))
+    const head = ['gamma']
+    for (let i = 0; i < R.nC; i++) { head.push('C' + fmt(i * R.dC)); if (R.se) head.push('se_C' + fmt(i * R.dC)) }
+    const rows = [`# type C grid, ${unit}: a row per gamma (degrees from the photometric axis), a column per C (degrees about it from C0)`,
+        head.join(',')]
+    for (let j = 0; j < R.nG; j++) {
+        const r = [fmt(j * R.dG)]
+        for (let i = 0; i < R.nC; i++) { r.push(R.I[j * R.nC + i]); if (R.se) r.push(R.se[j * R.nC + i]) }
+        rows.push(r.join(','))
+    }
+    return rows.join('\n')
+}
+
+module.exports = { intensity, plane, width, omegaOf, intensityCSV }

@@ -106,6 +106,10 @@ function createPathTracer(opts = {}) {
     RT.readRegion = (...a) => R.readRegion(...a)
     // the render's light as a file, 'pfm' | 'hdr' | 'exr' (darkroom/export/index.js: the light formats): a Buffer
     RT.saveLight = (format, opts) => R.saveLight(format, opts)
+    // the developed picture (exposure, tone curve, sRGB) and as a PNG file
+    RT.develop = opts => R.develop(opts)
+    RT.readAccum = () => R.readAccum()
+    RT.savePicture = (format, opts) => R.savePicture(format, opts)
     RT.readPixel = (...a) => R.readPixel(...a)
     RT.measure = (x0, y0, x1, y1) => R.readRegion(x0, y0, x1, y1)
     RT.wavefrontLog = () => R.wavefrontLog()

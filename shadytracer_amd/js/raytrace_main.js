@@ -74,6 +74,10 @@ const DEFAULTS = {
     sppmRadius: 0.02, sppmPhotons: 0, marchRefill: 0, guide: 0, guideFrac: 0.9, sppmAlpha: 2 / 3, relax: 0, measureMode: 0, lt: 0,
 }
 const KNOWN = new Set(UNIFORMS.map(f => f[0]))
+// PtDevelop (include/pt_types.h) and the reference's toneMap numbering (core/rendertoscreen.js)
+const DEVELOP = [['toneMap', 'u', 1], ['autoExposure', 'u', 1], ['exposureEV', 'f', 1], ['aeSpeed', 'f', 1], ['aeHeadroom', 'f', 1],
+    ['dither', 'u', 1], ['flipY', 'u', 1], ['pad', 'u', 1]]
+const TONE_MAPS = { linear: 0, neutral: 1, agx: 2, 'agx-punchy': 3, aces: 4, reinhard: 5, hable: 6 }
 
 function RAYTRACER(WG, _U, Q, scene) {
     WG = WG || {}
@@ -178,6 +182,29 @@ function RAYTRACER(WG, _U, Q, scene) {
         const { acc } = readback()
         return require('./export').writeLight(format, acc, U.width, U.height, opts)
     }
+    // the accumulation itself: { acc: FP64 mean rgb + count per pixel, mom: FP64 (V, S) per pixel }, row 0 at the bottom
+    R.readAccum = () => { const { acc, mom } = readback(); return { acc, mom } }
+    // THE DEVELOPED PICTURE (core/rendertoscreen.js's display pass with core/autoexposure.js's meter, for a picture that is
+    // saved): { rgba (Uint8Array, rows top first), width, height, ev, log2Average, log2High, metered }.
+    // toneMap by the reference's numbering or name; autoExposure metered from the picture (eased by aeSpeed a call,
+    // set afresh after clearAccums), else exposureEV (EV100: white at 1.2 x 2^EV cd/m^2).
+    R.develop = ({ toneMap = 'neutral', autoExposure = true, exposureEV = 15, aeSpeed = 1, aeHeadroom = 2, dither = false } = {}) => {
+        if (!current) throw new Error('RAYTRACER: no scene')
+        const tone = typeof toneMap === 'number' ? toneMap : TONE_MAPS[toneMap]
+        if (tone === undefined) throw new Error(`RAYTRACER: no tone curve '${toneMap}' (${Object.keys(TONE_MAPS).join(', ')})`)
+        const buf = new ArrayBuffer(sizeOf(DEVELOP))
+        pack(DEVELOP, { toneMap: tone, autoExposure: autoExposure ? 1 : 0, exposureEV, aeSpeed, aeHeadroom, dither: dither ? 1 : 0, flipY: 1 },
+            new DataView(buf), 0)
+        const rgba = new Uint8Array(U.width * U.height * 4)
+        const [ev, log2Average, log2High, metered] = A.develop(h, buf, rgba)
+        return { rgba, width: U.width, height: U.height, ev, log2Average, log2High, metered }
+    }
+    // the developed picture as a PNG file (darkroom/export/png.js's role): a Buffer
+    R.savePicture = (format = 'png', opts = {}) => {
+        if (format !== 'png') throw new Error(`RAYTRACER: picture format '${format}' (png)`)
+        const p = R.develop(opts)
+        return require('./png').encodePNG(p.rgba, p.width, p.height)
+    }
     R.imageStats = () => {
         const { acc, mom, n } = readback()
         const L = []
@@ -234,4 +261,4 @@ function RAYTRACER(WG, _U, Q, scene) {
     return R
 }
 
-module.exports = { RAYTRACER, UNIFORMS, MATERIAL, FACE, CAMERA, sizeOf, pack, packArray, DEFAULTS, KNOWN, loadAddon }
+module.exports = { RAYTRACER, UNIFORMS, MATERIAL, FACE, CAMERA, DEVELOP, TONE_MAPS, sizeOf, pack, packArray, DEFAULTS, KNOWN, loadAddon }

@@ -134,8 +134,20 @@ class PtOpticsTally(ctypes.Structure):
                 ('hits', ctypes.c_uint64 * MAX_MATERIALS)]
 
 
+class PtDevelop(ctypes.Structure):
+    _fields_ = [('toneMap', ctypes.c_uint), ('autoExposure', ctypes.c_uint), ('exposureEV', ctypes.c_float), ('aeSpeed', ctypes.c_float),
+                ('aeHeadroom', ctypes.c_float), ('dither', ctypes.c_uint), ('flipY', ctypes.c_uint), ('pad', ctypes.c_uint)]
+
+
+class PtDevelopInfo(ctypes.Structure):
+    _fields_ = [('ev', ctypes.c_float), ('log2Average', ctypes.c_float), ('log2High', ctypes.c_float), ('metered', ctypes.c_float)]
+
+
+# the reference's toneMap numbering (core/rendertoscreen.js)
+TONE_MAPS = {'linear': 0, 'neutral': 1, 'agx': 2, 'agx-punchy': 3, 'aces': 4, 'reinhard': 5, 'hable': 6}
+
 STRUCTS = [PtUniforms, PtMaterial, PtLightFace, PtCamera, PtEnv, PtHaze, PtSceneDesc, PtOpts, PtGuideConfig, PtOpticsScene,
-           PtOpticsResult, PtTuning, PtOpticsSource, PtOpticsTrace, PtOpticsTally]
+           PtOpticsResult, PtTuning, PtOpticsSource, PtOpticsTrace, PtOpticsTally, PtDevelop, PtDevelopInfo]
 
 # the C ABI, name -> (restype, argtypes)
 SIGNATURES = {
@@ -167,6 +179,8 @@ SIGNATURES = {
     'pt_optics_trace': (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(PtOpticsScene), ctypes.c_uint32, ctypes.POINTER(PtOpticsResult),
                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     'pt_optics_error': (ctypes.c_char_p, []),
+    'pt_develop': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PtDevelop), ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint8),
+                                  ctypes.c_size_t, ctypes.POINTER(PtDevelopInfo)]),
     'pt_optics_run': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PtOpticsSource), ctypes.c_uint32, ctypes.POINTER(PtOpticsTrace),
                                      ctypes.c_size_t, ctypes.POINTER(PtOpticsTally), ctypes.c_size_t, ctypes.POINTER(ctypes.c_double),
                                      ctypes.c_size_t, ctypes.POINTER(ctypes.c_float), ctypes.c_uint32]),
@@ -393,6 +407,20 @@ class Engine:
 
     def reset_stats(self):
         self._chk(self.lib.pt_reset_stats(self.h))
+
+    # ---- the developed picture (core/rendertoscreen.js, core/autoexposure.js)
+    def develop(self, tone='neutral', auto=True, ev=15.0, speed=1.0, headroom=2.0, dither=False, flip=True):
+        """The accumulation exposed, tone mapped and sRGB-encoded (pt_develop): (uint8 [h, w, 4] RGBA, rows top first
+        with flip, and {'ev', 'log2Average', 'log2High', 'metered'}). auto: metered from the picture, else EV100 `ev`."""
+        d = PtDevelop()
+        d.toneMap = TONE_MAPS[tone] if isinstance(tone, str) else int(tone)
+        d.autoExposure, d.exposureEV, d.aeSpeed, d.aeHeadroom = int(bool(auto)), ev, speed, headroom
+        d.dither, d.flipY = int(bool(dither)), int(bool(flip))
+        out = np.zeros((self.height, self.width, 4), np.uint8)
+        info = PtDevelopInfo()
+        self._chk(self.lib.pt_develop(self.h, ctypes.byref(d), ctypes.sizeof(d), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                      self.width * self.height, ctypes.byref(info)))
+        return out, {k: getattr(info, k) for k, _ in PtDevelopInfo._fields_}
 
     # ---- the optics trace over the loaded scene (createOptics: optics/index.js)
     def optics_run(self, sources, rays=100000, bounces=64, seed=1, far=None, detectors=(), det_cap=0, max_dist=0.0,

@@ -14,6 +14,7 @@ const optics = require(path.join(js, 'optics'))
 optics.createOpticsBench({ device: 0 })   // checks the bench's layouts against the addon's structure sizes; no GPU call
 out.traceSizes = [m.sizeOf(optics.SOURCE), a.structSize(12), m.sizeOf(optics.TRACE), a.structSize(13), optics.TALLY_BYTES, a.structSize(14)]
 out.opticsSizes = [m.sizeOf(optics.SCENE), a.structSize(9), optics.RESULT.length * 8, a.structSize(10)]
+out.developSize = [m.sizeOf(m.DEVELOP), a.structSize(15)]
 out.sizes = [m.sizeOf(m.UNIFORMS), a.structSize(0), m.sizeOf(m.MATERIAL), a.structSize(1), m.sizeOf(m.FACE), a.structSize(2)]
 out.addonKeys = Object.keys(a).sort()
 // packing: a known value lands at its C offset (width 0, height 4, seed 60, cam.pos 64, env.type 128)

@@ -47,12 +47,13 @@ def test_addon_loads_and_packs_the_same_structures():
                                     stdout=subprocess.PIPE, text=True).stdout)
     s = out['sizes']
     assert s[0] == s[1] == ctypes.sizeof(pt.PtUniforms) and s[2] == s[3] and s[4] == s[5]
-    assert out['addonKeys'] == sorted(['abiVersion', 'autotune', 'clearAccum', 'compileScene', 'create', 'destroy', 'foldParams', 'opticsRun',
+    assert out['addonKeys'] == sorted(['abiVersion', 'autotune', 'clearAccum', 'compileScene', 'create', 'destroy', 'develop', 'foldParams', 'opticsRun',
                                        'opticsTrace',
                                        'readAccum', 'readStats', 'setParams', 'setScene', 'setTuning', 'setUniforms', 'structSize',
                                        'trace'])
     # the reference's lighting settings are set() names (uniforms.schema.js Lighting), none returned as unknown
     assert out['lighting'] == [] and out['elevation'] == 20
+    assert out['developSize'] == [ctypes.sizeof(pt.PtDevelop)] * 2
     t = out['traceSizes']
     assert t[0] == t[1] == ctypes.sizeof(pt.PtOpticsSource) and t[2] == t[3] == ctypes.sizeof(pt.PtOpticsTrace)
     assert t[4] == t[5] == ctypes.sizeof(pt.PtOpticsTally)

@@ -284,3 +284,28 @@ def test_node_create_optics_runs_the_trace():
     # the run gives its seed back: the same run again traces the same rays
     assert js['seed'] == 1
 
+
+
+def test_detector_map_of_a_lambertian_source(engine):
+    # a small Lambertian disc 1 m over a detector floor, facing it: E(r) = flux h^2 / (pi (h^2 + r^2)^2), 318.3 lx under
+    # it; the hits mapped (shadytracer_amd/detmap.py) in 5 cm cells over 2 x 2 m, ring by ring against the closed form
+    from shadytracer_amd import detmap
+    say('case: optics detector map under a Lambertian disc')
+    made = load(engine, 'optics-floor', albedo=0.5)
+    floor = mat(made, 'floor')
+    src = {'type': 'disc', 'pos': (0, 0, 1), 'axis': (0, 0, -1), 'size': (0.01, 0.01), 'distribution': 'cosine', 'flux': 1000}
+    n = 1 << 22
+    res = engine.optics_run([src], rays=n, seed=23, detectors=[floor], det_cap=n)
+    assert res['detectorHits'] > 0.99 * n and len(res['hits']) == res['detectorHits']
+    m = detmap.detector_map(res['hits'], cells=40, rng={'a': (-1.0, 1.0), 'b': (-1.0, 1.0)})
+    assert (m['nx'], m['ny']) == (40, 40) and abs(m['normal'][2]) == 1.0
+    ax = m['range']['a'][0] + (np.arange(40) + 0.5) * m['cell'] + m['centre'][0]
+    ay = m['range']['b'][0] + (np.arange(40) + 0.5) * m['cell'] + m['centre'][1]
+    X, Y = np.meshgrid(ax, ay)
+    r2 = X * X + Y * Y
+    closed = 1000.0 / (math.pi * (1.0 + r2) ** 2)
+    for lo, hi in ((0.0, 0.15), (0.15, 0.3), (0.3, 0.5), (0.5, 0.7), (0.7, 0.9)):
+        ring = (r2 >= lo * lo) & (r2 < hi * hi)
+        ratio = m['E'][ring].sum() / closed[ring].sum()
+        say('  r %.2f-%.2f m: E / closed form %.4f over %d cells' % (lo, hi, ratio, ring.sum()))
+        assert ratio == pytest.approx(1.0, abs=0.02)
