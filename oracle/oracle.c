@@ -79,12 +79,15 @@ static void sampleSeed(uint32_t base, uint32_t sample, uint32_t* out) {
 }
 
 /* ---- the scene program (hipgen.js `program`) ---- */
-enum { OP_PRIM = 0, OP_JOIN = 1, OP_FOLD = 2, OP_NODE = 3 };
+enum { OP_PRIM = 0, OP_JOIN = 1, OP_FOLD = 2, OP_NODE = 3, OP_MOD = 4, OP_MODEND = 5 };
+/* the builtin modifier groups (builder/modifiers.js; hipgen.js MODIFIERS), in oracle.py's MODS order */
+enum { MOD_TWIST = 0, MOD_BEND, MOD_REPEAT, MOD_MIRROR, MOD_CIRCULAR, MOD_ROUND, MOD_SHELL };
 enum { SH_PLANE = 0, SH_SPHERE, SH_BOX, SH_CYLINDER, SH_CAPSULE, SH_TORUS, SH_RECT, SH_DISC, SH_LENS, SH_PRISM, SH_REFLECTOR, SH_ASPHERE,
        SH_REVOLUTION, SH_FRESNEL, SH_EXTRUSION, SH_CONE, SH_ROUNDCONE, SH_OCTA, SH_CTORUS, SH_LINK, SH_TUBE, SH_DOME, SH_FRUSTUM };
 enum { J_UNION = 0, J_SUBTRACT, J_INTERSECT };
 /* prim: shape, xf, p, mat, prim; join: join, k, and its shape (0 smooth, 1 round, 2 chamfer) in shape; node (a top-level node's start): xf = its bounding sphere's offset in
- * U (-1: unbounded), join = its mask bit (-1: none), k = the index of its fold */
+ * U (-1: unbounded), join = its mask bit (-1: none), k = the index of its fold; mod (a modifier group's start): shape =
+ * its kind, xf = the group's frame, p = its parameters, k = the index of its modend */
 typedef struct { int op, shape, xf, p, mat, prim, join, k; } OrOp;
 
 typedef struct {
@@ -499,6 +502,89 @@ static d4 joinAny(d4 a, d4 b, int op, int shape, double k) {
     return I(a, op == J_SUBTRACT ? neg4(b) : b, k);
 }
 
+/* THE MODIFIER GROUPS (builder/modifiers.js twist, bend, repeat, mirror, circular, round, shell): the group's
+ * children, ops [i + 1, end), evaluated at the group-frame point the modifier makes, joined as they say, then the
+ * modifier's change to the value and the group's frame back to its parent's. A twist's or bend's value is the children's
+ * times its Lipschitz bound 1 / sqrt(1 + (rate extent)^2) (U[p + 1]), with no normal (differences); a mirror's likewise
+ * without a factor; the circular array is the least of its copies, each's normal turned with it. */
+static d4 evalRange(const OrScene* S, int lo, int hi, d3 p, int* mat, int* prim);
+static double repeat1(double x, double s, double n) {
+    if (s == 0) return x;
+    double k = nearbyint(x / s);      /* round half to even, as the device's rintf */
+    return x - s * (k < -n ? -n : k > n ? n : k);
+}
+static d4 modEval(const OrScene* S, int i, d3 pw, int* mat, int* prim) {
+    const OrOp* o = &S->ops[i];
+    const float* t = S->U + o->xf;
+    const float* P = S->U + o->p;
+    d3 c0 = v(t[3], t[4], t[5]), c1 = v(t[6], t[7], t[8]), c2 = v(t[9], t[10], t[11]);
+    d3 d = sub(pw, v(t[0], t[1], t[2]));
+    double sc = t[12];
+    d3 p = v(dotp(d, c0) / sc, dotp(d, c1) / sc, dotp(d, c2) / sc);
+    d4 r;
+    switch (o->shape) {
+    case MOD_TWIST: case MOD_BEND: {
+        double a = P[0] * (o->shape == MOD_TWIST ? p.z : p.x), ca = cos(a), sa = sin(a);
+        d3 q = o->shape == MOD_TWIST ? v(ca * p.x - sa * p.y, sa * p.x + ca * p.y, p.z) : v(ca * p.x - sa * p.z, p.y, sa * p.x + ca * p.z);
+        d4 u = evalRange(S, i + 1, o->k, q, mat, prim);
+        r = d4of(v(0, 0, 0), u.w * P[1]);
+        break;
+    }
+    case MOD_REPEAT: {
+        d3 q = v(repeat1(p.x, P[0], P[3]), repeat1(p.y, P[1], P[4]), repeat1(p.z, P[2], P[5]));
+        r = evalRange(S, i + 1, o->k, q, mat, prim);
+        break;
+    }
+    case MOD_MIRROR: {
+        d4 u = evalRange(S, i + 1, o->k, v(fabs(p.x), p.y, p.z), mat, prim);
+        r = d4of(v(0, 0, 0), u.w);
+        break;
+    }
+    case MOD_CIRCULAR: {
+        int n = (int)P[0];
+        r = d4of(v(0, 0, 0), 1e300);
+        for (int c = 0; c < n; ++c) {
+            double a = c * (double)P[1], ca = cos(a), sa = sin(a);
+            int m, k;
+            d4 u = evalRange(S, i + 1, o->k, v(ca * p.x + sa * p.y, ca * p.y - sa * p.x, p.z), &m, &k);
+            if (u.w < r.w) { r = d4of(v(ca * u.x - sa * u.y, sa * u.x + ca * u.y, u.z), u.w); *mat = m; *prim = k; }
+        }
+        break;
+    }
+    case MOD_ROUND: {
+        d4 u = evalRange(S, i + 1, o->k, p, mat, prim);
+        r = d4of(v(u.x, u.y, u.z), u.w - P[0]);
+        break;
+    }
+    default: {
+        d4 u = evalRange(S, i + 1, o->k, p, mat, prim);
+        double sg = u.w < 0 ? -1 : 1;
+        r = d4of(v(u.x * sg, u.y * sg, u.z * sg), fabs(u.w) - P[0]);
+        break;
+    }
+    }
+    d3 n = add(add(scl(c0, r.x), scl(c1, r.y)), scl(c2, r.z));
+    return d4of(n, r.w * sc);
+}
+/* a node's value ops in [lo, hi) (prims, joins, modifier groups) on a stack; the value, its material and primitive */
+static d4 evalRange(const OrScene* S, int lo, int hi, d3 p, int* mat, int* prim) {
+    d4 st[64]; int sm[64], sk[64]; int top = 0;
+    for (int i = lo; i < hi; ++i) {
+        const OrOp* o = &S->ops[i];
+        if (o->op == OP_PRIM) { st[top] = primWorld(S, o, p); sm[top] = o->mat; sk[top] = o->prim; ++top; }
+        else if (o->op == OP_MOD) { st[top] = modEval(S, i, p, &sm[top], &sk[top]); ++top; i = o->k; }
+        else if (o->op == OP_JOIN) {
+            d4 b = st[--top], a = st[top - 1];
+            double k = S->U[o->k];
+            int wins = o->join == J_UNION ? b.w < a.w : o->join == J_SUBTRACT ? -b.w > a.w : b.w > a.w;
+            if (wins) { sm[top - 1] = sm[top]; sk[top - 1] = sk[top]; }
+            st[top - 1] = joinAny(a, b, o->join, o->shape, k);
+        }
+    }
+    *mat = top ? sm[0] : 0; *prim = top ? sk[0] : -1;
+    return top ? st[0] : d4of(v(0, 0, 1), 1e30);
+}
+
 typedef struct { d3 n; double d; int m, a, prim; double insideW; } OrHit;
 
 /* The closest-surface fold over the program (scene_api.js:39-63 fold; codegen.js:1821-1828 joins). A top-level node
@@ -517,6 +603,8 @@ static OrHit sceneAtM(const OrScene* S, d3 p, uint64_t mask) {
             }
             if (skip) i = o->k;
         } else if (o->op == OP_PRIM) { st[top] = primWorld(S, o, p); sm[top] = o->mat; sk[top] = o->prim; ++top; }
+        else if (o->op == OP_MOD) { st[top] = modEval(S, i, p, &sm[top], &sk[top]); ++top; i = o->k; }
+        else if (o->op == OP_MODEND) { }
         else if (o->op == OP_JOIN) {
             d4 b = st[--top], a = st[top - 1];
             double k = S->U[o->k];

@@ -15,7 +15,8 @@ LIB = os.path.join(HERE, '_build', 'liboracle.so')
 SHAPES = ['plane', 'sphere', 'box', 'cylinder', 'capsule', 'torus', 'rectangle', 'disc', 'lens', 'prism', 'reflector', 'aspheric lens',
           'revolution', 'fresnel lens', 'extrusion', 'cone', 'round cone', 'octahedron', 'capped torus', 'link', 'tube', 'dome', 'pyramid']
 JOINS = ['union', 'subtract', 'intersect']
-OPS = {'prim': 0, 'join': 1, 'fold': 2, 'node': 3}
+OPS = {'prim': 0, 'join': 1, 'fold': 2, 'node': 3, 'mod': 4, 'modend': 5}
+MODS = ['twist', 'bend', 'repeat', 'mirror', 'circular', 'round', 'shell']
 
 _lib = None
 
@@ -108,6 +109,8 @@ def pack_program(program):
             out[i, 1] = ('smooth', 'round', 'chamfer').index(o.get('shape', 'smooth'))
         elif o['op'] == 'node':
             out[i, 2] = o['bs']; out[i, 6] = o['bit']; out[i, 7] = o['end']
+        elif o['op'] == 'mod':
+            out[i, 1] = MODS.index(o['kind']); out[i, 2] = o['xf']; out[i, 3] = o['p']; out[i, 7] = o['end']
     return np.ascontiguousarray(out)
 
 

@@ -769,6 +769,33 @@ __device__ __forceinline__ V4 xfOut(const PT_CONST float* t, V4 s) {
     return v4(n, s.w * t[12]);
 }
 
+// THE BUILTIN MODIFIER GROUPS (builder/modifiers.js): the point a group's children are evaluated at, in the group's
+// frame, and what happens to their value. A twist turns the point about z by rate z; a bend turns it in the xz plane by
+// amount x (both scale the distance by their Lipschitz bound, applied by the emitter); a repeat folds it into the cell
+// of copies `spacing` apart, at most `count` each way along each axis (0 spacing: none along it); a mirror reflects it
+// across x = 0. A round grows the children's surface by r; a shell makes it a skin of thickness t (the normal turned
+// outward on both sides).
+__device__ __forceinline__ V3 mTwist(V3 p, float rate) {
+    float s, c;
+    __sincosf(rate * p.z, &s, &c);
+    return v3(c * p.x - s * p.y, s * p.x + c * p.y, p.z);
+}
+__device__ __forceinline__ V3 mBend(V3 p, float amount) {
+    float s, c;
+    __sincosf(amount * p.x, &s, &c);
+    return v3(c * p.x - s * p.z, p.y, s * p.x + c * p.z);
+}
+__device__ __forceinline__ float mRepeat1(float x, float s, float n) {
+    return s != 0.0f ? x - s * fminf(fmaxf(rintf(x / s), -n), n) : x;
+}
+__device__ __forceinline__ V3 mRepeat(V3 p, V3 s, V3 n) { return v3(mRepeat1(p.x, s.x, n.x), mRepeat1(p.y, s.y, n.y), mRepeat1(p.z, s.z, n.z)); }
+__device__ __forceinline__ V3 mMirror(V3 p) { return v3(fabsf(p.x), p.y, p.z); }
+__device__ __forceinline__ V4 mGrow(V4 f, float r) { return v4(v3(f.x, f.y, f.z), f.w - r); }
+__device__ __forceinline__ V4 mShell(V4 f, float t) {
+    const float s = f.w < 0.0f ? -1.0f : 1.0f;
+    return v4(v3(f.x, f.y, f.z) * s, fabsf(f.w) - t);
+}
+
 // The closest-surface fold (scene_api.js:39-63): the closest |d| gives the surface and its material, the
 // deepest negative d the material the point is inside of.
 struct SdfHit {

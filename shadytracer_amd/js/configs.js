@@ -348,6 +348,49 @@ function gallery(opts = {}) {
     return { doc: d, camera: cameraOf(d, w, h), settings: { width: w, height: h, spp: 64, maxbounce: 8, nee: 1 } }
 }
 
+// modifiers: the builtin modifier groups (builder/modifiers.js) in millimetres, laid out as the reference's
+// scenes/modifiers.json lays its own: a twisted bar, a bent plank, a repeated ball, a mirrored torus, a rounded cube,
+// a shelled ball with a box cut out of it, and two circular arrays (balls, and boxes overlapping into one ring); the
+// studio's light and a CCT panel. Options: `only` a modifier's name (that group alone over the floor), and `rate` to
+// set the twist's (0: the untwisted bar).
+function modifiers(opts = {}) {
+    const { cameraOf } = require('./hipgen')
+    const mods = ['twist', 'bend', 'repeat', 'mirror', 'round', 'shell', 'circular']
+    const blocks = BLOCKS.concat(mods.map(k => ({ id: `b-${k}`, label: k, kind: 'group', builtin: `${k}@1`, params: {} })))
+    const mats = [material('floor', [0.5, 0.5, 0.5], 0.8, 0, { specular: 0.5 }), material('brass', [0.9, 0.7, 0.35], 0.3, 1),
+        material('paint', [0.2, 0.35, 0.7], 0.5, 0), material('clay', [0.75, 0.4, 0.3], 0.9, 0, { specular: 0.2 })]
+    const group = (id, kind, at, params, children) => ({ id, kind: 'group', label: kind, enabled: true, blend: 0, block: `b-${kind}`,
+        transform: at, params, children })
+    const rate = opts.rate !== undefined ? opts.rate : 0.039269908
+    const G = {
+        twist: group('twisted', 'twist', xf(-135, 50, 0), { rate, extent: 22 },
+            [obj('bar', 'b-box', 'paint', xf(0, 0, 40), { size: { x: 14, y: 14, z: 40 }, rounding: 1 })]),
+        bend: group('bent', 'bend', xf(-45, 50, 25), { amount: 0.01, extent: 40 },
+            [obj('plank', 'b-box', 'brass', xf(0, 0, 0), { size: { x: 40, y: 8, z: 4 }, rounding: 1 })]),
+        repeat: group('repeated', 'repeat', xf(45, 50, 7), { spacing: { x: 20, y: 20, z: 20 }, count: { x: 1, y: 1, z: 0 } },
+            [obj('ball', 'b-sphere', 'clay', xf(0, 0, 0), { radius: 7 })]),
+        mirror: group('mirrored', 'mirror', xf(135, 50, 0), {},
+            [obj('ring', 'b-torus', 'brass', xf(16, 0, 25, 90, 0, 30), { ring: 14, tube: 5 })]),
+        round: group('rounded', 'round', xf(-135, -50, 0), { radius: 5 },
+            [obj('cube', 'b-box', 'brass', xf(0, 0, 22), { size: { x: 17, y: 17, z: 17 }, rounding: 0 })]),
+        shell: { id: 'cut', kind: 'group', label: 'cut shell', enabled: true, blend: 0, transform: xf(-45, -50, 0), children: [
+            group('shelled', 'shell', xf(0, 0, 0), { thickness: 2 }, [obj('globe', 'b-sphere', 'clay', xf(0, 0, 25), { radius: 25 })]),
+            obj('window', 'b-box', 'clay', xf(-18, -18, 45), { size: { x: 20, y: 20, z: 20 }, rounding: 0 }, { join: 'subtract' })] },
+        circular: group('bolts', 'circular', xf(45, -50, 0), { count: 8, reach: 32 },
+            [obj('bolt', 'b-sphere', 'paint', xf(24, 0, 7), { radius: 7 })]),
+    }
+    const ring = group('ring', 'circular', xf(135, -50, 0), { count: 12, reach: 32 },
+        [obj('tooth', 'b-box', 'brass', xf(6, 0, 8), { size: { x: 18, y: 6, z: 8 }, rounding: 1 })])
+    const chosen = opts.only ? [G[opts.only]] : mods.map(k => G[k]).concat([ring])
+    const children = [obj('floor', 'b-plane', 'floor', xf(0, 0, 0))].concat(chosen)
+    children.push({ id: 'panel', kind: 'object', type: 'light', label: 'panel', enabled: true, material: null, transform: xf(0, 100, 160, 150, 0, 0),
+        params: { width: 80, height: 50 }, light: { shape: 'rectangle', colourBy: 'cct', cct: 3500, flux: 600 } })
+    const d = studioDoc('modifiers', mats, children, blocks,
+        { mode: 'orbit', camtype: 0, fov: 45, target: [0, 0, 20], azimuth: -90, elevation: 55, distance: 420, up: [0, 0, 1], aperture: 0 })
+    const w = opts.width || 160, h = opts.height || 96
+    return { doc: d, camera: cameraOf(d, w, h), settings: { width: w, height: h, spp: 64, maxbounce: 8, nee: 1 } }
+}
+
 // sunlit: the gallery under the builder's default environment, the sun & sky (envMode 0): the sun 35 degrees up at
 // azimuth -140, 128 klx above the air, turbidity 0.7.
 const sunlit = (opts = {}) => gallery(Object.assign({}, opts, { sun: true }))
@@ -464,6 +507,6 @@ const CONFIGS = { 'plane-c1': planeC1, 'optics-floor': opticsFloor, 'optics-shel
     'optics-ball': opticsBall, 'room-c2': roomC2, 'caustic-c3': causticC3, 'furnace-plane': furnacePlane, 'furnace-sphere': furnaceSphere,
     'furnace-haze': furnaceHaze,
     'sppm-slab': sppmSlab, 'studio-floor': studioFloor, 'lens-prism': lensPrism, 'reflector-condenser': reflectorCondenser,
-    'turned-parts': turnedParts, gallery, sunlit, 'sunsky-floor': sunskyFloor, 'sunsky-bench': sunskyBench, joins }
+    'turned-parts': turnedParts, gallery, sunlit, modifiers, 'sunsky-floor': sunskyFloor, 'sunsky-bench': sunskyBench, joins }
 
 module.exports = { CONFIGS, camera, rng, SEED }

@@ -16,9 +16,10 @@
  *
  * Scope: the builder's builtin shapes (plane, box, sphere, cylinder, capsule, torus, rectangle, disc, lens, prism),
  * groups with their joins (union, subtract, intersect; smooth blends, rounds and chamfers), transforms (rotation
- * Rz*Ry*Rx in degrees, uniform scale: builder/transform.js:9-38), lights (rectangle and disc faces), one haze box and
- * the studio environment. A code block that is not a builtin -- user WGSL -- is refused with its node named, and so is
- * a group with code (a modifier).
+ * Rz*Ry*Rx in degrees, uniform scale: builder/transform.js:9-38), the builtin modifier groups (twist, bend, repeat,
+ * mirror, circular array, round, shell: builder/modifiers.js), lights (rectangle and disc faces), one haze box and the
+ * studio environment. A code block that is not a builtin -- user WGSL -- is refused with its node named, and so is a
+ * group with code that is not one of those modifiers (the noise displacements, the user's own).
  */
 
 const sky = require('./sky')
@@ -44,6 +45,62 @@ const JOIN_FNS = {
 const JOIN_FLOPS = { smooth: 9, round: 19, chamfer: 14 }
 const JOIN_REACH = { smooth: 0.25, round: 0.41421356, chamfer: 0.70710678 }
 const CULL_FLOPS = 11    // |p - c|^2 against (r + |best|)^2, widened a hair (boundNear)
+
+// THE BUILTIN MODIFIER GROUPS (builder/modifiers.js MODIFIERS): a group whose block is one of these changes the point
+// its children are evaluated at (space: twist, bend, repeat, mirror, the circular array) or their distance (shape:
+// round, shell). Each is drawn by a device function of its own (pt_prelude.h m*), its parameters in U after the
+// group's transform: lengths in metres, per-length rates per metre. A warp that stretches space scales the children's
+// distances by its Lipschitz bound k (k = 1 / sqrt(1 + (rate extent)^2)) and leaves the normal to differences (a zero
+// normal: pt_surface takes it from the distance). `bound` is the modifier's bounding sphere in the group's frame from
+// its children's (c, r) and its parameters. Displacement and noise reliefs (noise functions) are not drawn.
+const MODIFIERS = {
+    twist: {
+        lengths: ['extent'], perLength: ['rate'], warp: true,
+        params: P => [P.rate, 1 / Math.sqrt(1 + (P.rate * P.extent) ** 2)],
+        point: (o, q) => `mTwist(${q}, U[${o}])`, flops: 30,
+        // a turn about z keeps |p|: the children's sphere's points are within |c| + r of the axis' origin
+        bound: (c, r) => ({ c: [0, 0, 0], r: Math.hypot(...c) + r }),
+    },
+    bend: {
+        lengths: ['extent'], perLength: ['amount'], warp: true,
+        params: P => [P.amount, 1 / Math.sqrt(1 + (P.amount * P.extent) ** 2)],
+        point: (o, q) => `mBend(${q}, U[${o}])`, flops: 30,
+        bound: (c, r) => ({ c: [0, 0, 0], r: Math.hypot(...c) + r }),
+    },
+    repeat: {
+        lengths: ['spacing'], perLength: [],
+        params: P => [P.spacing.x, P.spacing.y, P.spacing.z, P.count.x, P.count.y, P.count.z],
+        point: (o, q) => `mRepeat(${q}, v3(${us(o, 3)}), v3(${us(o + 3, 3)}))`, flops: 24,
+        bound: (c, r, P) => ({ c, r: r + Math.hypot(P.spacing.x * P.count.x, P.spacing.y * P.count.y, P.spacing.z * P.count.z) }),
+    },
+    mirror: {
+        lengths: [], perLength: [], warp: true, unscaled: true,
+        params: () => [1],
+        point: (o, q) => `mMirror(${q})`, flops: 1,
+        bound: (c, r) => ({ c: [0, c[1], c[2]], r: Math.abs(c[0]) + r }),
+    },
+    circular: {
+        lengths: [], perLength: [], copies: true,
+        // copies 1..64 (rounded) and the angle between them
+        params: P => { const n = Math.min(Math.max(Math.round(P.count), 1), 64); return [n, 2 * Math.PI / n] },
+        flops: 12,
+        bound: (c, r) => ({ c: [0, 0, c[2]], r: Math.hypot(c[0], c[1]) + r }),
+    },
+    round: {
+        lengths: ['radius'], perLength: [],
+        params: P => [P.radius], after: (o, d) => `(${d} - U[${o}])`, afterFull: (o, f) => `mGrow(${f}, U[${o}])`, flops: 1,
+        bound: (c, r, P) => ({ c, r: r + Math.max(P.radius, 0) }),
+    },
+    shell: {
+        lengths: ['thickness'], perLength: [],
+        params: P => [P.thickness], after: (o, d) => `(fabsf(${d}) - U[${o}])`, afterFull: (o, f) => `mShell(${f}, U[${o}])`, flops: 2,
+        bound: (c, r, P) => ({ c, r: r + Math.max(P.thickness, 0) }),
+    },
+}
+const MODIFIER_DEFAULTS = {
+    twist: { rate: 1, extent: 1 }, bend: { amount: 0.3, extent: 2 }, repeat: { spacing: { x: 2, y: 2, z: 2 }, count: { x: 2, y: 2, z: 0 } },
+    mirror: {}, circular: { count: 6, reach: 32 }, round: { radius: 0.1 }, shell: { thickness: 0.05 },
+}
 
 // The builtin shapes: their parameters in U (after the 13 of the transform), and the HIP calls.
 const SHAPES = {
@@ -424,7 +481,7 @@ function generate(doc, opts) {
     let haze = null
     let flops = 0
     const dist = [], full = [], maskCode = [], helpers = []
-    let tmp = 0, maskBits = 0
+    let tmp = 0, maskBits = 0, modDepth = 0
 
     const push = vals => { const o = params.length; for (const v of vals) params.push(fmt(v)); return o }
 
@@ -467,7 +524,8 @@ function generate(doc, opts) {
         layout[n.id] = { xf: xo, params: po, shape: key }
         // the scene's bounds: the local box's corners in the world
         const half = localHalf(key, P, unit)
-        if (half) {
+        // (inside a modifier group the frame is the group's, not the world's: the modifier adds its own bound)
+        if (half && !modDepth) {
             for (let c = 0; c < 8; c++) {
                 const l = [0, 1, 2].map(i => ((c >> i) & 1 ? 1 : -1) * half[i] * X.s)
                 const w = [0, 1, 2].map(i => X.t[i] + X.R[i][0] * l[0] + X.R[i][1] * l[1] + X.R[i][2] * l[2])
@@ -523,14 +581,14 @@ function generate(doc, opts) {
     // a CSG group's children joined in order into one value; returns the names of its expressions. A child's join is
     // its op (union, subtract, intersect) in its shape: the smooth polynomial blend (no joinShape), or a round's or a
     // chamfer's of that radius (joinShape, model version 28)
-    function csgGroup(g, X) {
+    function csgGroup(g, X, pt = 'p') {
         const kids = (g.children || []).filter(drawn)
         if (!kids.length) return null
         const blendG = (g.blend || 0) * unit * X.s
         let acc = null
         for (let i = 0; i < kids.length; i++) {
             const c = kids[i]
-            const v = valueOf(c, X)
+            const v = valueOf(c, X, pt)
             if (!v) continue
             if (!acc) { acc = Object.assign({}, v); continue }
             const join = i === 0 ? 'union' : (c.join || 'union')
@@ -559,21 +617,97 @@ function generate(doc, opts) {
     // (builder/codegen.js group code). The engine runs no group code: such a group is refused, or with opts.approximate
     // drawn as a plain group of its children and listed.
     // A group block without a builtin and without statements only carries the group's parameters: a plain group.
+    function modifierOf(g) {
+        const b = g.block ? blocks[g.block] : null
+        const key = b && b.builtin ? String(b.builtin).split('@')[0] : null
+        return key && MODIFIERS[key] ? key : null
+    }
     function modifierCheck(g) {
+        if (modifierOf(g)) return
         const b = g.block ? blocks[g.block] : null
         let what = null
         if (b && b.builtin) what = `modifier '${b.builtin}' (group code)`
         else if (hasStatements(g.code) || (b && hasStatements(b.code))) what = 'group code (user WGSL)'
         if (what) cannot(`group '${g.id}'`, what)
     }
-    // a node's value as expressions (for use inside a CSG group)
-    function valueOf(n, parentX) {
+    const unionOf = n => Object.assign({}, n, { blend: 0, children: (n.children || []).map(c => Object.assign({}, c, { join: 'union', joinBlend: undefined, joinShape: undefined })) })
+    // A modifier group's value: the group's frame point changed by the modifier (or the copies' points), its children
+    // evaluated there in the group's frame (their transforms relative to it) and joined as the group says, their distance
+    // changed by the modifier, then scaled to the world by the group's scale; the normal turned to the world, or left to
+    // differences after a warp.
+    function modifierValue(g, X, pt) {
+        const key = modifierOf(g)
+        const M = MODIFIERS[key]
+        const b = blocks[g.block] || {}
+        const raw = Object.assign({}, MODIFIER_DEFAULTS[key], b.params || {}, g.params || {})
+        const P = {}
+        for (const k of Object.keys(raw)) {
+            const v = raw[k], f = M.lengths.indexOf(k) >= 0 ? unit : M.perLength.indexOf(k) >= 0 ? 1 / unit : 1
+            P[k] = v && typeof v === 'object' ? { x: v.x * f, y: v.y * f, z: v.z * f } : v * f
+        }
+        for (const c of g.children || []) {
+            if (c.type === 'light' || c.type === 'haze') throw new Error(`hipgen: node '${c.id}': a ${c.type} inside modifier group '${g.id}' is not drawn`)
+        }
+        const xo = push(xfParams(X))
+        const po = push(M.params(P))
+        const i = tmp++
+        const q = `q${i}`, d = `d${i}`, f = `f${i}`, m = `m${i}`, k = `k${i}`
+        const start = program.length
+        program.push({ op: 'mod', kind: key, xf: xo, p: po, end: -1 })
+        const d0 = dist.length, f0 = full.length
+        modDepth++
+        const inner = isCsg(g) ? csgGroup(g, IDENT, q) : csgGroup(unionOf(g), IDENT, q)
+        modDepth--
+        if (!inner) { program.splice(start); dist.splice(d0); full.splice(f0); return null }
+        const body = dist.splice(d0), fbody = full.splice(f0)
+        program.push({ op: 'modend' })
+        program[start].end = program.length - 1
+        const local = `xfIn(U + ${xo}, ${pt})`
+        const scale = `U[${xo + 12}]`
+        if (M.copies) {
+            // the circular array: every copy, the point turned back by its angle, the least value; its normal turned
+            const g0 = `g${i}`
+            dist.push(`        const V3 ${g0} = ${local};`, `        float ${d} = 3.0e38f;`,
+                `        for (int c${i} = 0; c${i} < (int)U[${po}]; ++c${i}) {`,
+                `            float sn${i}, cs${i}; __sincosf((float)c${i} * U[${po + 1}], &sn${i}, &cs${i});`,
+                `            const V3 ${q} = v3(cs${i} * ${g0}.x + sn${i} * ${g0}.y, cs${i} * ${g0}.y - sn${i} * ${g0}.x, ${g0}.z);`,
+                ...body.map(l => '    ' + l), `            ${d} = fminf(${d}, ${inner.d});`, '        }', `        ${d} *= ${scale};`)
+            full.push(`        const V3 ${g0} = ${local};`, `        V4 ${f} = v4(v3s(0.0f), 3.0e38f);`, `        unsigned int ${m} = 0u, ${k} = 0u;`,
+                `        for (int c${i} = 0; c${i} < (int)U[${po}]; ++c${i}) {`,
+                `            float sn${i}, cs${i}; __sincosf((float)c${i} * U[${po + 1}], &sn${i}, &cs${i});`,
+                `            const V3 ${q} = v3(cs${i} * ${g0}.x + sn${i} * ${g0}.y, cs${i} * ${g0}.y - sn${i} * ${g0}.x, ${g0}.z);`,
+                ...fbody.map(l => '    ' + l),
+                `            if (${inner.f}.w < ${f}.w) { ${f} = v4(v3(cs${i} * ${inner.f}.x - sn${i} * ${inner.f}.y, sn${i} * ${inner.f}.x + cs${i} * ${inner.f}.y, ${inner.f}.z), ${inner.f}.w); ${m} = ${inner.m}; ${k} = ${inner.k}; }`,
+                '        }', `        ${f} = xfOut(U + ${xo}, ${f});`)
+        } else {
+            const qv = M.point ? M.point(po, local) : local
+            const after = v => M.after ? M.after(po, v) : v
+            const kf = M.warp && !M.unscaled ? ` * U[${po + 1}]` : ''
+            dist.push(`        const V3 ${q} = ${qv};`, ...body, `        float ${d} = ${after(inner.d)}${kf} * ${scale};`)
+            const fl = M.warp ? `v4(v3s(0.0f), ${inner.f}.w${kf})` : M.afterFull ? M.afterFull(po, inner.f) : inner.f
+            full.push(`        const V3 ${q} = ${qv};`, ...fbody, `        V4 ${f} = xfOut(U + ${xo}, ${fl});`,
+                `        unsigned int ${m} = ${inner.m}, ${k} = ${inner.k};`)
+        }
+        // the modifier's bound in the group's frame, then in the world
+        const lb = M.bound(inner.bs.c, inner.bs.r, P)
+        const R = X.R
+        const wc = [0, 1, 2].map(a => X.t[a] + X.s * (R[a][0] * lb.c[0] + R[a][1] * lb.c[1] + R[a][2] * lb.c[2]))
+        const wr = lb.r * X.s
+        if (isFinite(wr)) {
+            for (let a = 0; a < 3; a++) { bounds[0][a] = Math.min(bounds[0][a], wc[a] - wr); bounds[1][a] = Math.max(bounds[1][a], wc[a] + wr) }
+        }
+        const copies = M.copies ? Math.min(Math.max(Math.round(P.count), 1), 64) : 1
+        return { d, f, m, k, bs: { c: wc, r: wr }, flops: copies * (inner.flops + M.flops) + XF_FLOPS }
+    }
+    // a node's value as expressions (for use inside a CSG group); pt: the point it is evaluated at
+    function valueOf(n, parentX, pt = 'p') {
         const X = compose(parentX, n.transform, unit)
         if (n.kind === 'group') {
             modifierCheck(n)
-            if (isCsg(n)) return csgGroup(n, X)
+            if (modifierOf(n)) return modifierValue(n, X, pt)
+            if (isCsg(n)) return csgGroup(n, X, pt)
             // a plain group inside a CSG group: its children unioned
-            return csgGroup(Object.assign({}, n, { blend: 0, children: (n.children || []).map(c => Object.assign({}, c, { join: 'union', joinBlend: undefined, joinShape: undefined })) }), X)
+            return csgGroup(unionOf(n), X, pt)
         }
         n.__world = X
         const s = shape(n, X)
@@ -581,8 +715,8 @@ function generate(doc, opts) {
         const i = tmp++
         const d = `d${i}`, f = `f${i}`, m = `m${i}`, k = `k${i}`
         const prim = program.filter(o => o.op === 'prim').length
-        dist.push(`        float ${d} = ${SHAPES[s.key].dist(s.po)} * U[${s.xo + 12}];`.replace(/\bq\b/, `xfIn(U + ${s.xo}, p)`))
-        full.push(`        V4 ${f} = xfOut(U + ${s.xo}, ${SHAPES[s.key].full(s.po)});`.replace(/\bq\b/, `xfIn(U + ${s.xo}, p)`))
+        dist.push(`        float ${d} = ${SHAPES[s.key].dist(s.po)} * U[${s.xo + 12}];`.replace(/\bq\b/, `xfIn(U + ${s.xo}, ${pt})`))
+        full.push(`        V4 ${f} = xfOut(U + ${s.xo}, ${SHAPES[s.key].full(s.po)});`.replace(/\bq\b/, `xfIn(U + ${s.xo}, ${pt})`))
         full.push(`        unsigned int ${m} = ${mat}u, ${k} = ${prim}u;`)
         program.push({ op: 'prim', shape: s.key, xf: s.xo, p: s.po, mat, prim })
         return { d, f, m, k, bs: s.bs, flops: s.flops }
@@ -591,7 +725,7 @@ function generate(doc, opts) {
     function foldNode(n, parentX) {
         if (n.enabled === false) return
         if (n.type === 'haze') { hazeOf(n, compose(parentX, n.transform, unit)); return }
-        if (n.kind === 'group' && !isCsg(n)) {
+        if (n.kind === 'group' && !isCsg(n) && !modifierOf(n)) {
             modifierCheck(n)
             const X = compose(parentX, n.transform, unit)
             for (const c of n.children || []) foldNode(c, X)

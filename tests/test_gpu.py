@@ -228,6 +228,35 @@ def test_solid_builtins_match_the_oracle(engine):
     assert abs(gL.mean() - oL.mean()) < 4 * se + 1e-3 * oL.mean()
 
 
+def test_modifier_groups_match_the_oracle(engine):
+    # the builtin modifier groups (twist, bend, repeat, mirror, round, shell, two circular arrays): primary decisions ray
+    # by ray against the oracle's evaluation of the same program, then per-pixel means
+    say('case: modifiers 160x96 pick vs oracle; 80x48 GPU 256 spp vs oracle 64 spp')
+    made, u = load(engine, 'modifiers')
+    gpos, gstate, _, gprim = engine.pick_rays()
+    opos, ostate, _, oprim = oracle.pick(made['scene'], u)
+    same = (gstate == ostate) & ((gstate != 0) | (gprim == oprim))
+    say('agreement', same.mean(), 'per primitive', [int(((oprim == k) & (ostate == 0)).sum()) for k in range(1, 10)])
+    assert same.mean() >= 0.9995
+    for k in range(1, 10):
+        assert ((gprim == k) & (gstate == 0)).sum() > 20, k
+    hit = (gstate == 0) & same
+    assert np.quantile(np.linalg.norm(gpos[hit, :3] - opos[hit, :3], axis=1), 0.999) < 1e-4
+    made, u = load(engine, 'modifiers', width=80, height=48)
+    engine.trace(256)
+    ga, gm = engine.read_accum()
+    oa, om, _ = oracle.render(made['scene'], u, 64, first=100000)
+    gL, oL = pt.luminance(ga), pt.luminance(oa)
+    s = np.sqrt(pt.standard_error(gm) ** 2 + pt.standard_error(om) ** 2)
+    ok = s > 1e-9
+    z = (gL - oL)[ok] / s[ok]
+    say('z std', z.std(), 'frac |z|<3', np.mean(np.abs(z) < 3), 'means', gL.mean(), oL.mean())
+    assert 0.75 < z.std() < 1.3
+    assert np.mean(np.abs(z) < 3) > 0.97
+    se = math.sqrt(np.nansum(s ** 2)) / s.size
+    assert abs(gL.mean() - oL.mean()) < 4 * se + 1e-3 * oL.mean()
+
+
 def test_shaped_joins_match_the_oracle(engine):
     # rounds and chamfers (joinShape): a ball-box round, a chamfered hole, a concave round fillet and a chamfered union --
     # primary decisions ray by ray against the oracle (whose safe-step allowance differs from the engine's; the surfaces

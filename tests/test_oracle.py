@@ -926,18 +926,23 @@ def test_shaped_join_on_curved_surfaces_stays_a_lower_bound(tmp_path):
 
 
 def test_modifier_groups_are_refused_by_name(tmp_path):
-    # a group with code -- a builtin modifier such as twist@1 (builder/modifiers.js:80) -- changes its children's
-    # space; the engine runs no group code, so the document is refused naming the group, or drawn without the modifier
-    # and listed when approximations are allowed
+    # a group with code the engine does not draw -- a noise displacement (displace@1, builder/modifiers.js), the user's
+    # own -- is refused naming the group, or drawn without it and listed when approximations are allowed; the builtin
+    # space and shape modifiers (twist, bend, repeat, mirror, circular, round, shell) are drawn (tests/test_modifiers.py)
     doc = _two_box_doc('intersect', 'chamfer', 0.1)
     doc['blocks'].append({'id': 'tw', 'builtin': 'twist@1', 'params': {'rate': 1.0, 'extent': 1.0}})
+    doc['blocks'].append({'id': 'dp', 'builtin': 'displace@1', 'params': {'amount': 0.05, 'frequency': 2}})
     doc['root']['children'][0]['block'] = 'tw'
     p = tmp_path / 'doc.json'
     p.write_text(json.dumps(doc))
-    with pytest.raises(RuntimeError, match="group 'g': modifier 'twist@1'"):
+    made = scenes.emit_doc(str(p))
+    assert made['scene']['approximations'] == [] and [o['kind'] for o in made['scene']['program'] if o['op'] == 'mod'] == ['twist']
+    doc['root']['children'][0]['block'] = 'dp'
+    p.write_text(json.dumps(doc))
+    with pytest.raises(RuntimeError, match="group 'g': modifier 'displace@1'"):
         scenes.emit_doc(str(p))
     made = scenes.emit_doc(str(p), approximate=True)
-    assert any("group 'g': modifier 'twist@1'" in a for a in made['scene']['approximations'])
+    assert any("group 'g': modifier 'displace@1'" in a for a in made['scene']['approximations'])
     doc['root']['children'][0].pop('block')
     doc['root']['children'][0]['code'] = 'p = p * 2.0;\nC = children(C, p);'
     p.write_text(json.dumps(doc))
