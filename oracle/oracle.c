@@ -79,7 +79,8 @@ static void sampleSeed(uint32_t base, uint32_t sample, uint32_t* out) {
 }
 
 /* ---- the scene program (hipgen.js `program`) ---- */
-enum { OP_PRIM = 0, OP_JOIN = 1, OP_FOLD = 2, OP_NODE = 3, OP_MOD = 4, OP_MODEND = 5 };
+enum { OP_PRIM = 0, OP_JOIN = 1, OP_FOLD = 2, OP_NODE = 3, OP_MOD = 4, OP_MODEND = 5, OP_USER = 6, OP_UGROUP = 7, OP_UFOLD = 8,
+       OP_UGROUPEND = 9 };
 /* the builtin modifier groups (builder/modifiers.js; hipgen.js MODIFIERS), in oracle.py's MODS order */
 enum { MOD_TWIST = 0, MOD_BEND, MOD_REPEAT, MOD_MIRROR, MOD_CIRCULAR, MOD_ROUND, MOD_SHELL };
 enum { SH_PLANE = 0, SH_SPHERE, SH_BOX, SH_CYLINDER, SH_CAPSULE, SH_TORUS, SH_RECT, SH_DISC, SH_LENS, SH_PRISM, SH_REFLECTOR, SH_ASPHERE,
@@ -87,7 +88,9 @@ enum { SH_PLANE = 0, SH_SPHERE, SH_BOX, SH_CYLINDER, SH_CAPSULE, SH_TORUS, SH_RE
 enum { J_UNION = 0, J_SUBTRACT, J_INTERSECT };
 /* prim: shape, xf, p, mat, prim; join: join, k, and its shape (0 smooth, 1 round, 2 chamfer) in shape; node (a top-level node's start): xf = its bounding sphere's offset in
  * U (-1: unbounded), join = its mask bit (-1: none), k = the index of its fold; mod (a modifier group's start): shape =
- * its kind, xf = the group's frame, p = its parameters, k = the index of its modend */
+ * its kind, xf = the group's frame, p = its parameters, k = the index of its modend; user (an object's own code):
+ * shape = the index of its compiled function, xf its frame, mat, prim; ugroup (a group's own code): shape = its
+ * function, xf its frame, k = the index of its ugroupend, each child's value ops followed by a ufold */
 typedef struct { int op, shape, xf, p, mat, prim, join, k; } OrOp;
 
 typedef struct {
@@ -508,6 +511,71 @@ static d4 joinAny(d4 a, d4 b, int op, int shape, double k) {
  * times its Lipschitz bound 1 / sqrt(1 + (rate extent)^2) (U[p + 1]), with no normal (differences); a mirror's likewise
  * without a factor; the circular array is the least of its copies, each's normal turned with it. */
 static d4 evalRange(const OrScene* S, int lo, int hi, d3 p, int* mat, int* prim);
+
+/* THE SCENE'S OWN CODE (an object's or a group's WGSL), compiled in FP64 by the tests from the emitter's translation
+ * against oracle/wgsl_host.h: by index, an object's function (local point, metres -> normal, distance, material |
+ * primitive << 8) or a group's (which asks the oracle for its children's values at a point through the callback). */
+typedef void (*OrUserObj)(const float* U, const double* q, double* out, unsigned int* m);
+typedef void (*OrUserGroup)(const float* U, const double* q, void* self, int (*kids)(void*, const double*, double*, unsigned int*),
+                            double* out, unsigned int* m);
+static void* g_user[64];
+static int g_nuser = 0;
+void or_set_user(void* const* fns, int n) {
+    g_nuser = n < 64 ? n : 64;
+    for (int i = 0; i < g_nuser; ++i) g_user[i] = fns[i];
+}
+/* the materials' own code: (U, material, hit point, outward normal, the hit's primitive, the material to change) */
+typedef void (*OrUserMat)(const float* U, unsigned int mat, const double* p, const double* n, int prim, PtMaterial* m);
+static OrUserMat g_umat = 0;
+void or_set_material(void* fn) { g_umat = (OrUserMat)fn; }
+/* the oracle's primitives for the scene code's shape functions: shape index, local point, parameters, (normal, d) */
+void or_prim_local(int shape, const double* p, const float* P, double* out) {
+    d4 r = primLocal(shape, v(p[0], p[1], p[2]), P);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = r.w;
+}
+static d3 frameIn(const float* t, d3 pw) {
+    d3 c0 = v(t[3], t[4], t[5]), c1 = v(t[6], t[7], t[8]), c2 = v(t[9], t[10], t[11]);
+    d3 d = sub(pw, v(t[0], t[1], t[2]));
+    return v(dotp(d, c0) / t[12], dotp(d, c1) / t[12], dotp(d, c2) / t[12]);
+}
+static d4 frameOut(const float* t, const double* l) {
+    d3 c0 = v(t[3], t[4], t[5]), c1 = v(t[6], t[7], t[8]), c2 = v(t[9], t[10], t[11]);
+    return d4of(add(add(scl(c0, l[0]), scl(c1, l[1])), scl(c2, l[2])), l[3] * t[12]);
+}
+static int evalOp(const OrScene* S, int i, d3 p, d4* st, int* sm, int* sk, int* top);
+typedef struct { const OrScene* S; int lo, hi; } OrKids;
+/* a coded group's children at q (the group's frame, metres): each child's value, in order */
+static int orKids(void* self, const double* q, double* vals, unsigned int* ms) {
+    const OrKids* K = (const OrKids*)self;
+    d3 p = v(q[0], q[1], q[2]);
+    d4 st[64]; int sm[64], sk[64]; int top = 0, n = 0;
+    for (int i = K->lo; i < K->hi; ++i) {
+        if (K->S->ops[i].op == OP_UFOLD) {
+            d4 r = st[--top];
+            vals[4 * n] = r.x; vals[4 * n + 1] = r.y; vals[4 * n + 2] = r.z; vals[4 * n + 3] = r.w;
+            ms[n] = (unsigned int)sm[top] | ((unsigned int)sk[top] << 8);
+            if (++n == 64) break;
+        } else {
+            i = evalOp(K->S, i, p, st, sm, sk, &top);
+        }
+    }
+    return n;
+}
+static d4 userEval(const OrScene* S, int i, d3 pw, int* mat, int* prim) {
+    const OrOp* o = &S->ops[i];
+    const float* t = S->U + o->xf;
+    d3 p = frameIn(t, pw);
+    double q[3] = {p.x, p.y, p.z}, out[4];
+    unsigned int m = 0;
+    if (o->shape >= g_nuser || !g_user[o->shape]) { out[0] = 0; out[1] = 0; out[2] = 1; out[3] = NAN; }
+    else if (o->op == OP_USER) ((OrUserObj)g_user[o->shape])(S->U, q, out, &m);
+    else {
+        OrKids K = {S, i + 1, o->k};
+        ((OrUserGroup)g_user[o->shape])(S->U, q, &K, orKids, out, &m);
+    }
+    *mat = (int)(m & 0xFFu); *prim = (int)(m >> 8);
+    return frameOut(t, out);
+}
 static double repeat1(double x, double s, double n) {
     if (s == 0) return x;
     double k = nearbyint(x / s);      /* round half to even, as the device's rintf */
@@ -566,26 +634,42 @@ static d4 modEval(const OrScene* S, int i, d3 pw, int* mat, int* prim) {
     d3 n = add(add(scl(c0, r.x), scl(c1, r.y)), scl(c2, r.z));
     return d4of(n, r.w * sc);
 }
-/* a node's value ops in [lo, hi) (prims, joins, modifier groups) on a stack; the value, its material and primitive */
+/* one value op at i (a prim, a modifier or coded group, a join) on the stack; returns the index it ended at */
+static int evalOp(const OrScene* S, int i, d3 p, d4* st, int* sm, int* sk, int* top) {
+    const OrOp* o = &S->ops[i];
+    if (o->op == OP_PRIM) { st[*top] = primWorld(S, o, p); sm[*top] = o->mat; sk[*top] = o->prim; ++*top; }
+    else if (o->op == OP_MOD) { st[*top] = modEval(S, i, p, &sm[*top], &sk[*top]); ++*top; i = o->k; }
+    else if (o->op == OP_USER) { st[*top] = userEval(S, i, p, &sm[*top], &sk[*top]); ++*top; }
+    else if (o->op == OP_UGROUP) { st[*top] = userEval(S, i, p, &sm[*top], &sk[*top]); ++*top; i = o->k; }
+    else if (o->op == OP_JOIN) {
+        int t = --*top;
+        d4 b = st[t], a = st[t - 1];
+        double k = S->U[o->k];
+        int wins = o->join == J_UNION ? b.w < a.w : o->join == J_SUBTRACT ? -b.w > a.w : b.w > a.w;
+        if (wins) { sm[t - 1] = sm[t]; sk[t - 1] = sk[t]; }
+        st[t - 1] = joinAny(a, b, o->join, o->shape, k);
+    }
+    return i;
+}
+/* a node's value ops in [lo, hi) (prims, joins, modifier and coded groups) on a stack; the value, its material and
+ * primitive */
 static d4 evalRange(const OrScene* S, int lo, int hi, d3 p, int* mat, int* prim) {
     d4 st[64]; int sm[64], sk[64]; int top = 0;
-    for (int i = lo; i < hi; ++i) {
-        const OrOp* o = &S->ops[i];
-        if (o->op == OP_PRIM) { st[top] = primWorld(S, o, p); sm[top] = o->mat; sk[top] = o->prim; ++top; }
-        else if (o->op == OP_MOD) { st[top] = modEval(S, i, p, &sm[top], &sk[top]); ++top; i = o->k; }
-        else if (o->op == OP_JOIN) {
-            d4 b = st[--top], a = st[top - 1];
-            double k = S->U[o->k];
-            int wins = o->join == J_UNION ? b.w < a.w : o->join == J_SUBTRACT ? -b.w > a.w : b.w > a.w;
-            if (wins) { sm[top - 1] = sm[top]; sk[top - 1] = sk[top]; }
-            st[top - 1] = joinAny(a, b, o->join, o->shape, k);
-        }
-    }
+    for (int i = lo; i < hi; ++i) i = evalOp(S, i, p, st, sm, sk, &top);
     *mat = top ? sm[0] : 0; *prim = top ? sk[0] : -1;
     return top ? st[0] : d4of(v(0, 0, 1), 1e30);
 }
 
 typedef struct { d3 n; double d; int m, a, prim; double insideW; } OrHit;
+/* a hit's material, its code run on it when the scene has some */
+static const PtMaterial* hitMaterial(const OrScene* S, const OrHit* h, d3 p, d3 ns, PtMaterial* own) {
+    const int k = h->m < S->nmats ? h->m : 0;
+    if (!g_umat) return &S->mats[k];
+    *own = S->mats[k];
+    const double pw[3] = {p.x, p.y, p.z}, nw[3] = {ns.x, ns.y, ns.z};
+    g_umat(S->U, (unsigned int)k, pw, nw, h->prim, own);
+    return own;
+}
 
 /* The closest-surface fold over the program (scene_api.js:39-63 fold; codegen.js:1821-1828 joins). A top-level node
  * is skipped when the ray's mask leaves it out, or when its bounding sphere cannot beat the closest |d| so far (which
@@ -602,15 +686,9 @@ static OrHit sceneAtM(const OrScene* S, d3 p, uint64_t mask) {
                 skip = len(sub(p, v(b[0], b[1], b[2]))) - b[3] >= fabs(h.d);
             }
             if (skip) i = o->k;
-        } else if (o->op == OP_PRIM) { st[top] = primWorld(S, o, p); sm[top] = o->mat; sk[top] = o->prim; ++top; }
-        else if (o->op == OP_MOD) { st[top] = modEval(S, i, p, &sm[top], &sk[top]); ++top; i = o->k; }
-        else if (o->op == OP_MODEND) { }
-        else if (o->op == OP_JOIN) {
-            d4 b = st[--top], a = st[top - 1];
-            double k = S->U[o->k];
-            int wins = o->join == J_UNION ? b.w < a.w : o->join == J_SUBTRACT ? -b.w > a.w : b.w > a.w;
-            if (wins) { sm[top - 1] = sm[top]; sk[top - 1] = sk[top]; }
-            st[top - 1] = joinAny(a, b, o->join, o->shape, k);
+        } else if (o->op == OP_PRIM || o->op == OP_MOD || o->op == OP_USER || o->op == OP_UGROUP || o->op == OP_JOIN) {
+            i = evalOp(S, i, p, st, sm, sk, &top);
+        } else if (o->op == OP_MODEND) {
         } else {
             d4 s = st[--top];
             if (s.w < 0 && s.w > h.insideW) { h.insideW = s.w; h.a = sm[top]; }
@@ -1301,7 +1379,8 @@ static void tracePhoton(const OrScene* S, const uint32_t* seed, uint32_t i, uint
         d3 nrm = inside ? neg(ns) : ns;
         if (dotp(dir, nrm) > 0) { double off = offsetTol(S, p); p = add(p, scl(dir, off)); ltot += off * n; continue; }
         bounces++;
-        const PtMaterial* m = &S->mats[h.m < S->nmats ? h.m : 0];
+        PtMaterial mown;
+        const PtMaterial* m = hitMaterial(S, &h, p, ns, &mown);
         if (max3d(f3(m->emissive)) > 0) return;
         double n0 = n, n1 = maxd(m->ior, 1);
         if (inside) {
@@ -1465,7 +1544,8 @@ static void tracePath(const OrScene* S, const OrTiles* T, const uint32_t* seed, 
         d3 nrm = inside ? neg(ns) : ns;
         if (dotp(dir, nrm) > 0) { double off = offsetTol(S, p); p = add(p, scl(dir, off)); ltot += off * n; continue; }
         bounces++;
-        const PtMaterial* m = &S->mats[h.m < S->nmats ? h.m : 0];
+        PtMaterial mown;
+        const PtMaterial* m = hitMaterial(S, &h, p, ns, &mown);
         /* ...and the camera's first hit is a white Lambertian for that bounce: the pixel reads E / pi there */
         PtMaterial white;
         if (u->measureMode == 1 && bounces == 1) {

@@ -15,7 +15,7 @@ LIB = os.path.join(HERE, '_build', 'liboracle.so')
 SHAPES = ['plane', 'sphere', 'box', 'cylinder', 'capsule', 'torus', 'rectangle', 'disc', 'lens', 'prism', 'reflector', 'aspheric lens',
           'revolution', 'fresnel lens', 'extrusion', 'cone', 'round cone', 'octahedron', 'capped torus', 'link', 'tube', 'dome', 'pyramid']
 JOINS = ['union', 'subtract', 'intersect']
-OPS = {'prim': 0, 'join': 1, 'fold': 2, 'node': 3, 'mod': 4, 'modend': 5}
+OPS = {'prim': 0, 'join': 1, 'fold': 2, 'node': 3, 'mod': 4, 'modend': 5, 'user': 6, 'ugroup': 7, 'ufold': 8, 'ugroupend': 9}
 MODS = ['twist', 'bend', 'repeat', 'mirror', 'circular', 'round', 'shell']
 
 _lib = None
@@ -58,6 +58,8 @@ def lib():
         L.or_fresnel_unpolarised.restype = c_d
         L.or_fresnel_unpolarised.argtypes = [c_d, c_d, P(c_d)]
         L.or_mf_load.argtypes = [P(c_d)]
+        L.or_set_user.argtypes = [P(c_v), c_i]
+        L.or_set_material.argtypes = [c_v]
         L.or_mf_words.restype = c_i
         _lib = L
         _load_tables(L)
@@ -111,7 +113,48 @@ def pack_program(program):
             out[i, 2] = o['bs']; out[i, 6] = o['bit']; out[i, 7] = o['end']
         elif o['op'] == 'mod':
             out[i, 1] = MODS.index(o['kind']); out[i, 2] = o['xf']; out[i, 3] = o['p']; out[i, 7] = o['end']
+        elif o['op'] == 'user':
+            out[i, 1] = o['index']; out[i, 2] = o['xf']; out[i, 4] = o['mat']; out[i, 5] = o['prim']
+        elif o['op'] == 'ugroup':
+            out[i, 1] = o['index']; out[i, 2] = o['xf']; out[i, 7] = o['end']
     return np.ascontiguousarray(out)
+
+
+# A scene's own code (objects' and groups' WGSL, as the emitter translated it) compiled in FP64 against wgsl_host.h,
+# one module per distinct source, kept under _build/user/
+USER_DIR = os.path.join(HERE, '_build', 'user')
+DEVICE_DIR = os.path.join(os.path.dirname(HERE), 'shadytracer_amd', 'csrc', 'device')
+_user_mods = {}
+
+
+def user_module(user, material=None):
+    """The compiled module of a scene's `user` list and material hook: (module, its function pointers in index order,
+    their count, the material hook or None)."""
+    import hashlib
+    src = '#include "wgsl_host.h"\n' + '\n\n'.join(u['cpp'] for u in user) + '\n' + (material or '') + '\n'
+    key = hashlib.sha1(src.encode()).hexdigest()[:16]
+    if key in _user_mods:
+        return _user_mods[key]
+    os.makedirs(USER_DIR, exist_ok=True)
+    so = os.path.join(USER_DIR, 'u%s.so' % key)
+    if not os.path.exists(so):
+        cpp = os.path.join(USER_DIR, 'u%s.cpp' % key)
+        with open(cpp, 'w') as f:
+            f.write(src)
+        r = subprocess.run(['g++', '-O2', '-std=c++14', '-shared', '-fPIC', '-I', HERE, '-I', DEVICE_DIR, '-o', so + '.tmp', cpp],
+                           stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        if r.returncode != 0:
+            raise RuntimeError('the scene code did not compile for the oracle:\n' + r.stdout[-4000:])
+        os.replace(so + '.tmp', so)
+    M = ctypes.CDLL(so)
+    M.wg_set_prim.argtypes = [ctypes.c_void_p]
+    M.wg_set_prim(ctypes.cast(lib().or_prim_local, ctypes.c_void_p))
+    fns = (ctypes.c_void_p * max(len(user), 1))()
+    for i, u in enumerate(user):
+        fns[i] = ctypes.cast(getattr(M, ('ucode%d' if u['kind'] == 'object' else 'ugroup%d') % i), ctypes.c_void_p)
+    hook = ctypes.cast(M.umaterial, ctypes.c_void_p) if material else None
+    _user_mods[key] = (M, fns, len(user), hook)
+    return _user_mods[key]
 
 
 class Scene:
@@ -120,6 +163,7 @@ class Scene:
     def __init__(self, scene):
         from shadytracer_amd import pt
         self.ops = pack_program(scene['program'])
+        self.user = user_module(scene.get('user') or [], scene.get('materialCode')) if (scene.get('user') or scene.get('materialCode')) else None
         self.U = np.ascontiguousarray(np.asarray(scene['params'], np.float32))
         mats = scene['materials']
         self.mats = (pt.PtMaterial * len(mats))()
@@ -140,6 +184,13 @@ class Scene:
         self.nmats, self.nfaces = len(mats), nf
 
     def args(self):
+        # the scene's own code, for the call these arguments are for
+        if self.user is not None:
+            lib().or_set_user(self.user[1], self.user[2])
+            lib().or_set_material(self.user[3])
+        else:
+            lib().or_set_user(None, 0)
+            lib().or_set_material(None)
         return (self.ops.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), len(self.ops),
                 self.U.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
 

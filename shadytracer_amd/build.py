@@ -16,7 +16,8 @@ LIB = os.path.join(PKG, 'libpt.so')
 ADDON = os.path.join(PKG, 'js', 'pt_addon.node')
 
 # device sources in translation-unit order; the emitted scene goes between PRE and POST
-DEVICE_PRE = [os.path.join(INCLUDE, 'pt_types.h'), os.path.join(DEV, 'pt_dev.h'), os.path.join(DEV, 'pt_prelude.h')]
+DEVICE_PRE = [os.path.join(INCLUDE, 'pt_types.h'), os.path.join(DEV, 'pt_dev.h'), os.path.join(DEV, 'pt_prelude.h'),
+              os.path.join(DEV, 'pt_wgsl_rt.h'), os.path.join(DEV, 'pt_wgsl_lib.h')]
 DEVICE_POST = [os.path.join(DEV, f) for f in ('pt_bsdf.h', 'pt_sky.h', 'pt_kernels.h', 'pt_guide.h', 'pt_wavefront.h', 'pt_optrace.h',
                                             'pt_develop.h')]
 

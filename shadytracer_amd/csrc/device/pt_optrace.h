@@ -147,7 +147,13 @@ __device__ __forceinline__ bool opticsOne(const PtDev* __restrict__ D, const PtO
             return false;
         }
         const unsigned int mat = hw & 0xFFu;
+#if PT_SCENE_MATCODE
+        // the material's own code, at this hit
+        PtMaterial m = D->mats[mat];
+        sceneMaterial((const PT_CONST float*)D->params, mat, p, ns, m);
+#else
         const PtMaterial& m = D->mats[mat];
+#endif
         const unsigned long long fw = optFix(w);
         atomicAdd(&sT[PT_OT_MAT + 3u * mat], fw);
         atomicAdd(&sT[PT_OT_MAT + 3u * mat + 2u], 1ull);

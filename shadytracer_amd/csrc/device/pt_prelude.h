@@ -823,3 +823,8 @@ __device__ __forceinline__ bool boundNear(V3 p, float cx, float cy, float cz, fl
     return dx * dx + dy * dy + dz * dz < R * R * (1.0f + 2e-6f);
 }
 __device__ __forceinline__ void foldDist(float& best, float d) { if (fabsf(d) < fabsf(best)) best = d; }
+
+// The scene code's runtime (pt_wgsl_rt.h, pt_wgsl_lib.h) on the device: FP32, inlined device functions.
+#define WG_FN __device__ __forceinline__
+typedef float WF;
+#define WL(x) x##f

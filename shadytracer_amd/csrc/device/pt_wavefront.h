@@ -555,7 +555,14 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
         } else {
             bounces += 1u;
             // the illuminance view (measureMode 1): the camera's first hit is the white Lambertian kept after the table
+#if PT_SCENE_MATCODE
+            // the material's own code runs on the hit's material (not on the illuminance view's white)
+            const bool whiteView = D->u.measureMode == 1u && bounces == 1u && !photon;
+            PtMaterial m = smat[whiteView ? PT_MAX_MATERIALS : (hw & 0xFFu)];
+            if (!whiteView) sceneMaterial((const PT_CONST float*)D->params, hw & 0xFFu, p, ns, m);
+#else
             const PtMaterial& m = smat[(D->u.measureMode == 1u && bounces == 1u && !photon) ? PT_MAX_MATERIALS : (hw & 0xFFu)];
+#endif
             V3 Le = ld3(m.emissive);
             if (max3(Le) > 0.0f && (photon || (mapped && m.face >= 0 && faceFlux(D->faces[m.face]) > 0.0f))) {
                 terminate = true;      // a photon ends at a light; a mapped path's light is the map's (its photons')

@@ -391,6 +391,104 @@ function modifiers(opts = {}) {
     return { doc: d, camera: cameraOf(d, w, h), settings: { width: w, height: h, spp: 64, maxbounce: 8, nee: 1 } }
 }
 
+// usercode: the modifiers scene with every modifier written as the document's own group code (WGSL the emitter
+// translates, not the native modifiers), and two objects of their own code: a cone from its closed form and a Menger
+// sponge of three levels. With { native: true } the same groups use the builtin modifiers, so the two documents
+// describe one scene: what the translation draws can be held to what the native code draws.
+const USER_CODE = {
+    twist: `// turned about z by rate radians per unit up
+let a = rate * p.z;
+let c = cos(a);
+let s = sin(a);
+p = vec3<f32>(c * p.x - s * p.y, s * p.x + c * p.y, p.z);
+warp(1.0 / sqrt(1.0 + rate * rate * extent * extent));
+C = children(C, p);`,
+    bend: `let a = amount * p.x;
+p = VEC3(cos(a) * p.x - sin(a) * p.z, p.y, sin(a) * p.x + cos(a) * p.z);
+warp(1.0 / sqrt(1.0 + amount * amount * extent * extent));
+C = children(C, p);`,
+    repeat: `p = p - spacing * clamp(round(p / spacing), -count, count);
+C = children(C, p);`,
+    mirror: `p.x = abs(p.x);
+warp(1.0);
+C = children(C, p);`,
+    round: `var H = children(emptyHit(r), p);
+H.s.w -= radius * XF_S;
+C = fold(C, H.s, H.m);`,
+    shell: `var H = children(emptyHit(r), p);
+H.s = VEC4(H.s.xyz * sign(H.s.w), abs(H.s.w) - thickness * XF_S);
+C = fold(C, H.s, H.m);`,
+    circular: `/* every copy, turned by i / n of a turn */
+let n = u32(clamp(round(count), 1.0, 64.0));
+let slice = 6.2831853 / f32(n);
+let xfm = XF_M;
+for (var i = 0u; i < n; i++) {
+    let t = f32(i) * slice;
+    let c = cos(t); let s = sin(t);
+    XF_M = xfm * mat3x3<f32>(c, s, 0.0, -s, c, 0.0, 0.0, 0.0, 1.0);
+    C = children(C, VEC3(c * p.x + s * p.y, c * p.y - s * p.x, p.z));
+}
+XF_M = xfm;`,
+    cone: `// a cone's side between z = -height/2 and height/2, radius base below and top above, and its caps
+let k = (top - base) / height;
+let c = inverseSqrt(1.0 + k * k);
+let rho = length(p.xy);
+let side = (rho - (0.5 * (base + top) + k * p.z)) * c;
+let caps = abs(p.z) - 0.5 * height;
+let radial = select(vec2f(1.0, 0.0), p.xy / rho, rho > 1e-12);
+var f = VEC4(VEC3(radial * c, -k * c), side);
+if (caps > side) { f = VEC4(0.0, 0.0, sign(p.z), caps); }
+C = add_new(C, f, MATERIAL);`,
+    menger: `let s = size * 0.5;
+C = add_new(C, sdMenger(p / s, iterations) * s, MATERIAL);`,
+}
+// the materials' own code: a checker of the floor in the world, grain in the paint and stones in the clay in their
+// objects' frames (value and gradient noise, fBm, crossed Worley cells)
+const MATERIAL_CODE = {
+    floor: `let q = p / mm(1.0);
+let w = cos(PI * q / square);
+m.baseColor = mix(m.baseColor, second, step(0.0, w.x * w.y));`,
+    paint: `let q = p_local * S_local / mm(1.0);
+let bend = gnoise3(q / vec3f(6.0, 6.0, 24.0), 7u).w;
+let ring = fract(length(q.xy) / 2.5 + 0.6 * bend);
+let late = smoothstep(0.7, 1.0, ring);
+let fib = 0.5 + 0.5 * vnoise3(q / vec3f(0.4, 0.4, 8.0), 8u).w;
+m.baseColor = mix(m.baseColor, latewood, late) * (0.8 + 0.4 * fib);
+m.roughness = min(1.0, m.roughness + 0.2 * late);`,
+    clay: `let q = p_local * S_local / mm(1.0);
+let w = worley3x(q / 4.0, 12u);
+let f = 0.5 + 0.5 * fbm3(q / 10.0, 3u, 4u, 2.0, 0.5).w;
+m.baseColor = m.baseColor * (0.6 + 0.4 * f) * (1.0 - 0.5 * (1.0 - smoothstep(0.0, 0.3, w.y - w.x)));`,
+}
+function usercode(opts = {}) {
+    const made = modifiers(Object.assign({}, opts, { width: opts.width || 160, height: opts.height || 96 }))
+    const d = made.doc
+    if (!opts.noMaterialCode) {
+        const extra = { floor: { square: 20, second: { r: 0.15, g: 0.15, b: 0.15 } }, paint: { latewood: { r: 0.3, g: 0.15, b: 0.05 } }, clay: {} }
+        for (const m of d.materials) {
+            if (!MATERIAL_CODE[m.id]) continue
+            m.code = MATERIAL_CODE[m.id]
+            Object.assign(m.params, extra[m.id])
+        }
+    }
+    if (!opts.native) {
+        // each modifier group's block: its code in place of the builtin, the same parameters
+        d.blocks = d.blocks.map(b => {
+            const k = b.builtin && b.builtin.split('@')[0]
+            return USER_CODE[k] && b.kind === 'group' ? { id: b.id, label: b.label, kind: 'group', code: USER_CODE[k], params: b.params } : b
+        })
+    }
+    if (!opts.only) {
+        d.blocks = d.blocks.concat([
+            { id: 'u-cone', label: 'cone', kind: 'object', code: USER_CODE.cone, params: { base: 10, top: 4, height: 30 } },
+            { id: 'u-menger', label: 'Menger sponge', kind: 'object', code: USER_CODE.menger, params: { size: 30, iterations: 3 },
+                constants: { iterations: 'u32' } },
+        ])
+        d.root.children.splice(1, 0, obj('cone', 'u-cone', 'paint', xf(0, 0, 15), {}), obj('sponge', 'u-menger', 'clay', xf(0, 120, 15, 0, 0, 20), {}))
+    }
+    return made
+}
+
 // sunlit: the gallery under the builder's default environment, the sun & sky (envMode 0): the sun 35 degrees up at
 // azimuth -140, 128 klx above the air, turbidity 0.7.
 const sunlit = (opts = {}) => gallery(Object.assign({}, opts, { sun: true }))
@@ -507,6 +605,6 @@ const CONFIGS = { 'plane-c1': planeC1, 'optics-floor': opticsFloor, 'optics-shel
     'optics-ball': opticsBall, 'room-c2': roomC2, 'caustic-c3': causticC3, 'furnace-plane': furnacePlane, 'furnace-sphere': furnaceSphere,
     'furnace-haze': furnaceHaze,
     'sppm-slab': sppmSlab, 'studio-floor': studioFloor, 'lens-prism': lensPrism, 'reflector-condenser': reflectorCondenser,
-    'turned-parts': turnedParts, gallery, sunlit, modifiers, 'sunsky-floor': sunskyFloor, 'sunsky-bench': sunskyBench, joins }
+    'turned-parts': turnedParts, gallery, sunlit, modifiers, usercode, 'sunsky-floor': sunskyFloor, 'sunsky-bench': sunskyBench, joins }
 
 module.exports = { CONFIGS, camera, rng, SEED }

@@ -18,11 +18,13 @@
  * groups with their joins (union, subtract, intersect; smooth blends, rounds and chamfers), transforms (rotation
  * Rz*Ry*Rx in degrees, uniform scale: builder/transform.js:9-38), the builtin modifier groups (twist, bend, repeat,
  * mirror, circular array, round, shell: builder/modifiers.js), lights (rectangle and disc faces), one haze box and the
- * studio environment. A code block that is not a builtin -- user WGSL -- is refused with its node named, and so is a
- * group with code that is not one of those modifiers (the noise displacements, the user's own).
+ * studio environment. An object or a group whose code is not one of those builtins -- the author's own WGSL, or a
+ * builtin this file does not draw natively -- is translated (usercode.js, wgsl.js) and runs as written, in the scene's
+ * unit; code that uses what the translation does not know is refused with its node, line and column named.
  */
 
 const sky = require('./sky')
+const usercode = require('./usercode')
 const UNIT_METRES = { m: 1, cm: 0.01, mm: 0.001, um: 1e-6, in: 0.0254 }
 const LUMA = [0.2126, 0.7152, 0.0722]
 
@@ -34,6 +36,8 @@ const PRIM_FLOPS = { plane: 1, sphere: 7, box: 27, cylinder: 24, capsule: 14, to
     reflector: 80, 'aspheric lens': 52, revolution: 0, 'fresnel lens': 0, extrusion: 0, cone: 30, 'round cone': 20, octahedron: 24,
     'capped torus': 22, link: 18, tube: 16, dome: 24, pyramid: 90 }
 const XF_FLOPS = 22      // p - pos, three dot products, times 1/s; the distance times s
+// an object's or a group's own code: not counted operation by operation (an estimate of a small shape's cost)
+const USER_FLOPS = 40
 const FOLD_FLOPS = 2
 // a join per shape: its device functions (distance only, with normals), FP32 operations, and how far below the smaller
 // side a union's value can go per unit of its radius (the smooth blend's k/4; the round's and chamfer's allowance)
@@ -482,6 +486,10 @@ function generate(doc, opts) {
     let flops = 0
     const dist = [], full = [], maskCode = [], helpers = []
     let tmp = 0, maskBits = 0, modDepth = 0
+    // each primitive's object frame (its U offset): its node's own, or inside a modifier or coded group the outermost
+    // such group's (the frame a material's code reads p_local in)
+    const primFrames = []
+    let outerFrame = null
 
     const push = vals => { const o = params.length; for (const v of vals) params.push(fmt(v)); return o }
 
@@ -490,8 +498,9 @@ function generate(doc, opts) {
         const b = blocks[n.block]
         if (!b) throw new Error(`hipgen: node '${n.id}' has no block '${n.block}'`)
         const key = b.builtin ? String(b.builtin).split('@')[0] : null
-        if (!key || !SHAPES[key]) throw new Error(`hipgen: node '${n.id}' (${n.label || ''}) is user code, not a builtin shape: WGSL translation is not supported`)
-        return key
+        if (key && SHAPES[key]) return key
+        if (usercode.isCode(n.code) || usercode.isCode(b.code)) return 'code'
+        throw new Error(`hipgen: node '${n.id}' (${n.label || ''}) has no code and is not a builtin shape`)
     }
     function paramsOf(n, key) {
         const b = blocks[n.block] || {}
@@ -623,12 +632,104 @@ function generate(doc, opts) {
         return key && MODIFIERS[key] ? key : null
     }
     function modifierCheck(g) {
-        if (modifierOf(g)) return
+        if (modifierOf(g) || groupCodeOf(g)) return
         const b = g.block ? blocks[g.block] : null
-        let what = null
-        if (b && b.builtin) what = `modifier '${b.builtin}' (group code)`
-        else if (hasStatements(g.code) || (b && hasStatements(b.code))) what = 'group code (user WGSL)'
-        if (what) cannot(`group '${g.id}'`, what)
+        if (b && b.builtin) cannot(`group '${g.id}'`, `modifier '${b.builtin}' (group code)`)
+    }
+    // A group's own code (the group's, else its block's), when it has statements and is not a native modifier
+    function groupCodeOf(g) {
+        if (g.kind !== 'group' || modifierOf(g)) return null
+        const b = g.block ? blocks[g.block] : null
+        if (usercode.isCode(g.code)) return g.code
+        if (b && usercode.isCode(b.code)) return b.code
+        return null
+    }
+    const codeParams = (n, b) => Object.assign({}, (b && b.params) || {}, n.params || {})
+    const codeConstants = b => (b && b.constants) || {}
+    const userFns = []
+    // An object's code: a function of the node's local point (the emitter's frame, metres) giving what it folded
+    function userObjectValue(n, X, pt) {
+        const b = blocks[n.block] || {}
+        const code = usercode.isCode(n.code) ? n.code : b.code
+        const mat = matOf(n)
+        const prim = program.filter(o => o.op === 'prim' || o.op === 'user').length
+        const idx = userFns.length
+        let fn
+        try {
+            fn = usercode.objectFunction(idx, code, codeParams(n, b), codeConstants(b), mat | (prim << 8), unit, push)
+        } catch (e) {
+            // code the translation does not take: refused, or with opts.approximate left out and listed
+            cannot(`node '${n.id}' (${n.label || ''})`, `its code: ${e.message}`)
+            return null
+        }
+        const xo = push(xfParams(X))
+        primFrames[prim] = outerFrame !== null ? outerFrame : xo
+        userFns.push({ kind: 'object', node: n.id, host: fn.host })
+        helpers.push(fn.device)
+        layout[n.id] = { xf: xo, shape: 'code' }
+        const i = tmp++
+        const d = `d${i}`, f = `f${i}`, m = `m${i}`, k = `k${i}`
+        dist.push(`        unsigned int um${i}; float ${d} = ucode${idx}(U, xfIn(U + ${xo}, ${pt}), &um${i}).w * U[${xo + 12}];`)
+        full.push(`        unsigned int um${i}; V4 ${f} = xfOut(U + ${xo}, ucode${idx}(U, xfIn(U + ${xo}, ${pt}), &um${i}));`)
+        full.push(`        unsigned int ${m} = um${i} & 0xFFu, ${k} = um${i} >> 8;`)
+        program.push({ op: 'user', index: idx, xf: xo, mat, prim })
+        // the code's cost is not counted (an estimate stands for it); no bound is known, so the node is never culled
+        return { d, f, m, k, bs: { c: X.t.slice(), r: Infinity }, flops: USER_FLOPS + XF_FLOPS }
+    }
+    // A group's code around its children: children(C, q) evaluates them at q (the group's frame, scene units) and folds
+    // each into C; the code's Hit, taken back to metres, is the group's value
+    function userGroupValue(g, X, pt) {
+        const b = g.block ? blocks[g.block] : null
+        const code = groupCodeOf(g)
+        for (const c of g.children || []) {
+            if (c.type === 'light' || c.type === 'haze') throw new Error(`hipgen: node '${c.id}': a ${c.type} inside group '${g.id}' with code is not drawn`)
+        }
+        const idx = userFns.length
+        let gc
+        try {
+            gc = usercode.groupCode(code, codeParams(g, b), codeConstants(b), unit, push)
+        } catch (e) {
+            // refused, or with opts.approximate drawn as a plain group of its children and listed
+            cannot(`group '${g.id}' (${g.label || ''})`, `its code: ${e.message}`)
+            return isCsg(g) ? csgGroup(g, X, pt) : csgGroup(unionOf(g), X, pt)
+        }
+        const xo = push(xfParams(X))
+        const i = tmp++
+        const q = `q${i}`
+        const start = program.length
+        program.push({ op: 'ugroup', index: idx, xf: xo, end: -1 })
+        const d0 = dist.length, f0 = full.length
+        const kids = []
+        const frameWas = outerFrame
+        if (outerFrame === null) outerFrame = xo
+        modDepth++
+        for (const c of (g.children || []).filter(drawn)) {
+            const v = valueOf(c, IDENT, q)
+            if (!v) continue
+            program.push({ op: 'ufold' })
+            kids.push(v)
+        }
+        modDepth--
+        outerFrame = frameWas
+        const body = dist.splice(d0), fbody = full.splice(f0)
+        program.push({ op: 'ugroupend' })
+        program[start].end = program.length - 1
+        userFns.push({ kind: 'group', node: g.id, host: usercode.hostGroup(idx, gc) })
+        const d = `d${i}`, f = `f${i}`, m = `m${i}`, k = `k${i}`
+        const ind = l => '            ' + l
+        const head = [`        WHit uh${i};`, `        const WF wunit_g${i} = WL(${unit.toExponential()});`, '        {', '            WXf wx;', ...gc.decls.map(ind)]
+        const kidsD = kids.map(v => `                wCin = w_fold(wCin, wvec4(0.0f, 0.0f, 0.0f, ${v.d} / wunit * wx.S), 0u);`)
+        const kidsF = kids.map(v => `                wCin = w_fold(wCin, mk_wvec4(wx.M * wvec3(${v.f}.x, ${v.f}.y, ${v.f}.z), ${v.f}.w / wunit * wx.S), ${v.m} | (${v.k} << 8));`)
+        const tail = ['            wvec3 wp = ofV3(' + `xfIn(U + ${xo}, ${pt})` + ') / wunit;', '            WHit wC = w_emptyHit(0);',
+            ...gc.lines.map(ind), `            uh${i} = wC;`, '        }']
+        const lam = (lines, folds) => ['            auto wchildren = [&](WHit wCin, wvec3 wq) -> WHit {', `                const V3 ${q} = toV3(wq * wunit);`,
+            ...lines.map(l => '        ' + l), ...folds, '                return wCin;', '            };']
+        dist.push(...head, ...lam(body, kidsD), ...tail, `        float ${d} = uh${i}.s.w * wunit_g${i} * U[${xo + 12}];`)
+        full.push(...head, ...lam(fbody, kidsF), ...tail,
+            `        V4 ${f} = xfOut(U + ${xo}, v4(v3(uh${i}.s.x, uh${i}.s.y, uh${i}.s.z), uh${i}.s.w * wunit_g${i}));`,
+            `        unsigned int ${m} = uh${i}.m & 0xFFu, ${k} = uh${i}.m >> 8;`)
+        const flops = kids.reduce((a, v) => a + v.flops, 0) + USER_FLOPS + XF_FLOPS
+        return { d, f, m, k, bs: { c: X.t.slice(), r: Infinity }, flops }
     }
     const unionOf = n => Object.assign({}, n, { blend: 0, children: (n.children || []).map(c => Object.assign({}, c, { join: 'union', joinBlend: undefined, joinShape: undefined })) })
     // A modifier group's value: the group's frame point changed by the modifier (or the copies' points), its children
@@ -655,9 +756,12 @@ function generate(doc, opts) {
         const start = program.length
         program.push({ op: 'mod', kind: key, xf: xo, p: po, end: -1 })
         const d0 = dist.length, f0 = full.length
+        const frameWas = outerFrame
+        if (outerFrame === null) outerFrame = xo
         modDepth++
         const inner = isCsg(g) ? csgGroup(g, IDENT, q) : csgGroup(unionOf(g), IDENT, q)
         modDepth--
+        outerFrame = frameWas
         if (!inner) { program.splice(start); dist.splice(d0); full.splice(f0); return null }
         const body = dist.splice(d0), fbody = full.splice(f0)
         program.push({ op: 'modend' })
@@ -703,6 +807,7 @@ function generate(doc, opts) {
     function valueOf(n, parentX, pt = 'p') {
         const X = compose(parentX, n.transform, unit)
         if (n.kind === 'group') {
+            if (groupCodeOf(n)) return userGroupValue(n, X, pt)
             modifierCheck(n)
             if (modifierOf(n)) return modifierValue(n, X, pt)
             if (isCsg(n)) return csgGroup(n, X, pt)
@@ -710,11 +815,15 @@ function generate(doc, opts) {
             return csgGroup(unionOf(n), X, pt)
         }
         n.__world = X
+        // a ray warp (its block the region where rays bend; builder/warp.js) is not a surface: refused, or left out
+        if (n.warp) { cannot(`node '${n.id}' (${n.label || ''})`, 'a ray warp'); return null }
+        if (n.type !== 'light' && shapeOf(n) === 'code') return userObjectValue(n, X, pt)
         const s = shape(n, X)
         const mat = matOf(n)
         const i = tmp++
         const d = `d${i}`, f = `f${i}`, m = `m${i}`, k = `k${i}`
-        const prim = program.filter(o => o.op === 'prim').length
+        const prim = program.filter(o => o.op === 'prim' || o.op === 'user').length
+        primFrames[prim] = outerFrame !== null ? outerFrame : s.xo
         dist.push(`        float ${d} = ${SHAPES[s.key].dist(s.po)} * U[${s.xo + 12}];`.replace(/\bq\b/, `xfIn(U + ${s.xo}, ${pt})`))
         full.push(`        V4 ${f} = xfOut(U + ${s.xo}, ${SHAPES[s.key].full(s.po)});`.replace(/\bq\b/, `xfIn(U + ${s.xo}, ${pt})`))
         full.push(`        unsigned int ${m} = ${mat}u, ${k} = ${prim}u;`)
@@ -725,7 +834,7 @@ function generate(doc, opts) {
     function foldNode(n, parentX) {
         if (n.enabled === false) return
         if (n.type === 'haze') { hazeOf(n, compose(parentX, n.transform, unit)); return }
-        if (n.kind === 'group' && !isCsg(n) && !modifierOf(n)) {
+        if (n.kind === 'group' && !isCsg(n) && !modifierOf(n) && !groupCodeOf(n)) {
             modifierCheck(n)
             const X = compose(parentX, n.transform, unit)
             for (const c of n.children || []) foldNode(c, X)
@@ -783,6 +892,19 @@ function generate(doc, opts) {
 
     for (const c of doc.root.children || []) foldNode(c, compose(IDENT, doc.root.transform, unit))
 
+    // the materials' own code (those the drawn nodes use): one case each of the scene's material hook
+    const matCases = []
+    for (const id of Object.keys(checked)) {
+        const m = docMats[id]
+        if (!m || !usercode.isCode(m.code)) continue
+        try {
+            matCases.push(Object.assign({ mat: matIndex[id] }, usercode.materialCode(m.code, m.params || {}, unit, push)))
+        } catch (e) {
+            cannot(`material '${id}'`, `its code: ${e.message}`)
+        }
+    }
+    const hook = matCases.length ? usercode.materialHook(matCases, primFrames.map(f => (f === undefined ? -1 : f)), unit) : null
+
     const envRadius = (doc.envRadius !== undefined ? doc.envRadius * unit : 400)
     const env = environmentOf(doc.environment || {}, doc, cannot)
     if (approximations.length && !opts.approximate) {
@@ -793,6 +915,7 @@ function generate(doc, opts) {
         `// emitted by shadytracer_amd/js/hipgen.js from '${(doc.title || 'scene').replace(/[\n\r]/g, ' ')}'`,
         `#define PT_SCENE_MATERIALS ${materials.length}`,
         `#define PT_SCENE_MASK_BITS ${maskBits}`,
+        `#define PT_SCENE_MATCODE ${hook ? 1 : 0}`,
         ...helpers,
         '// the nodes a ray from o along d can reach within tmax: those whose bounding sphere, widened by margin, its',
         '// segment meets (bit k of the mask: the k-th bounded top-level node); unbounded nodes are always marched',
@@ -816,6 +939,7 @@ function generate(doc, opts) {
         ...full,
         '    return h;',
         '}',
+        ...(hook ? ['// the materials\' own code, run on a hit\'s material before it is used', hook.device] : []),
         '',
     ].join('\n')
     return {
@@ -837,6 +961,9 @@ function generate(doc, opts) {
         extent: isFinite(bounds[0][0]) ? Math.hypot(...[0, 1, 2].map(i => bounds[1][i] - bounds[0][i])) : null,
         epsAuto: autoEps(isFinite(bounds[0][0]) ? Math.hypot(...[0, 1, 2].map(i => bounds[1][i] - bounds[0][i])) : null),
         program,
+        // the objects' and groups' own code as the oracle compiles it (FP64 C++ over oracle/wgsl_host.h), by index
+        user: userFns.map(u => ({ kind: u.kind, node: u.node, cpp: u.host })),
+        materialCode: hook ? hook.host : null,
         flops: { dist: flops, maskNode: 14 },   // every node evaluated: the bound a culled step stays under
         approximations,
     }
@@ -847,7 +974,6 @@ function generate(doc, opts) {
 const hasStatements = code => typeof code === 'string' && code.split('\n').some(l => l.trim() && !l.trim().startsWith('//'))
 function materialLimits(m) {
     const out = [], P = (m && m.params) || {}
-    if (hasStatements(m.code)) out.push('material code (user WGSL)')
     if (m.relief) out.push('a relief')
     if (m.images && m.images.length) out.push('images')
     if (m.type && m.type !== 'emitter') out.push(`type '${m.type}'`)

@@ -38,7 +38,8 @@ out.refusals = {
     nosetting: refusal(doc({ environment: undefined })),
     envcode: refusal(doc({ environment: { envMode: 2 } })),
     comments: refusal(doc({ materials: mat({ code: '// only a comment\n   // and another\n' }) })),
-    code: refusal(doc({ materials: mat({ code: '// a checker\nm.baseColor = VEC3(1.0);\n' }) })),
+    code: refusal(doc({ materials: mat({ code: '// a texture\nm.baseColor = myTexture(p);\n' }) })),
+    translated: refusal(doc({ materials: mat({ code: '// a checker\nm.baseColor = VEC3(step(0.0, sin(p.x)));\n' }) })),
     coating: refusal(doc({ materials: mat({ params: { coating: 1 } }) })),
     relief: refusal(doc({ materials: mat({ relief: { code: '// bumps' } }) })),
     collimated: refusal(doc({ root: light({ shape: 'rectangle', distribution: 'collimated', flux: 10 }) })),

@@ -66,7 +66,7 @@ def test_addon_loads_and_packs_the_same_structures():
     assert out['packed'] == [640, 480, 77, 1.0, 3.0, 2, 1000.0]
     # the facade's set returns unknown names (lib/pathtracer/index.js set)
     assert out['unknown'] == ['notAUniform'] and out['maxbounce'] == 3
-    assert 'user code' in out['refused']
+    assert "its code: unknown function 'myShape'" in out['refused']
     assert out['c2'] == {'prims': 35, 'materials': 35, 'faces': 1, 'flops': out['c2']['flops']}
 
 

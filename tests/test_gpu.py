@@ -21,7 +21,7 @@ def say(*a):
     sys.stdout.flush()
 
 
-SCENE_OPTS = ('g', 'roughness', 'specular', 'metallic', 'albedo', 'mfp', 'floorRoughness', 'floorSpecular', 'sunDiameter', 'beam')
+SCENE_OPTS = ('g', 'roughness', 'specular', 'metallic', 'albedo', 'mfp', 'floorRoughness', 'floorSpecular', 'sunDiameter', 'beam', 'native')
 
 
 def load(engine, name, **over):
@@ -255,6 +255,56 @@ def test_modifier_groups_match_the_oracle(engine):
     assert np.mean(np.abs(z) < 3) > 0.97
     se = math.sqrt(np.nansum(s ** 2)) / s.size
     assert abs(gL.mean() - oL.mean()) < 4 * se + 1e-3 * oL.mean()
+
+
+def test_scene_code_matches_the_oracle(engine):
+    # the document's own WGSL, translated: the modifiers as group code, a cone and a Menger sponge of object code,
+    # materials whose code reads the world point and the object's frame through the noise library -- primary decisions
+    # ray by ray against the oracle's FP64 compile of the same translation, then per-pixel means
+    say('case: usercode 160x96 pick vs oracle; 80x48 GPU 256 spp vs oracle 64 spp')
+    made, u = load(engine, 'usercode')
+    assert len(made['scene']['user']) == 10 and made['scene']['materialCode']
+    gpos, gstate, _, gprim = engine.pick_rays()
+    opos, ostate, _, oprim = oracle.pick(made['scene'], u)
+    same = (gstate == ostate) & ((gstate != 0) | (gprim == oprim))
+    say('agreement', same.mean())
+    assert same.mean() >= 0.999
+    for k in range(0, 12):
+        assert ((gprim == k) & (gstate == 0)).sum() > 20, k
+    hit = (gstate == 0) & same
+    assert np.quantile(np.linalg.norm(gpos[hit, :3] - opos[hit, :3], axis=1), 0.999) < 1e-4
+    made, u = load(engine, 'usercode', width=80, height=48)
+    engine.trace(256)
+    ga, gm = engine.read_accum()
+    oa, om, _ = oracle.render(made['scene'], u, 64, first=100000)
+    gL, oL = pt.luminance(ga), pt.luminance(oa)
+    s = np.sqrt(pt.standard_error(gm) ** 2 + pt.standard_error(om) ** 2)
+    ok = s > 1e-9
+    z = (gL - oL)[ok] / s[ok]
+    say('z std', z.std(), 'frac |z|<3', np.mean(np.abs(z) < 3), 'means', gL.mean(), oL.mean())
+    assert 0.75 < z.std() < 1.3
+    assert np.mean(np.abs(z) < 3) > 0.97
+    se = math.sqrt(np.nansum(s ** 2)) / s.size
+    assert abs(gL.mean() - oL.mean()) < 4 * se + 1e-3 * oL.mean()
+
+
+def test_scene_code_draws_as_the_native_modifiers(engine):
+    # the same scene with its modifiers as group code and as the native modifier groups: one picture within the spread
+    # of the two renders (the gear's overlapping teeth fold copy by copy in code, so the two differ inside them only)
+    say('case: usercode vs native modifiers 80x48, 256 spp each')
+    out = []
+    for native in (False, True):
+        made, u = load(engine, 'usercode', width=80, height=48, native=native)
+        engine.trace(256)
+        out.append(engine.read_accum())
+    (ua, um), (na, nm) = out
+    uL, nL = pt.luminance(ua), pt.luminance(na)
+    s = np.sqrt(pt.standard_error(um) ** 2 + pt.standard_error(nm) ** 2)
+    ok = s > 1e-9
+    z = (uL - nL)[ok] / s[ok]
+    say('z std', z.std(), 'means', uL.mean(), nL.mean())
+    assert 0.75 < z.std() < 1.3
+    assert np.mean(np.abs(z) < 3) > 0.97
 
 
 def test_shaped_joins_match_the_oracle(engine):

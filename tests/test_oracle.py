@@ -943,10 +943,16 @@ def test_modifier_groups_are_refused_by_name(tmp_path):
         scenes.emit_doc(str(p))
     made = scenes.emit_doc(str(p), approximate=True)
     assert any("group 'g': modifier 'displace@1'" in a for a in made['scene']['approximations'])
+    # the group's own code is translated and drawn (tests/test_usercode.py); code calling what the translation does
+    # not know is refused with its line and column
     doc['root']['children'][0].pop('block')
     doc['root']['children'][0]['code'] = 'p = p * 2.0;\nC = children(C, p);'
     p.write_text(json.dumps(doc))
-    with pytest.raises(RuntimeError, match="group 'g': group code"):
+    made = scenes.emit_doc(str(p))
+    assert [o['op'] for o in made['scene']['program']].count('ugroup') == 1
+    doc['root']['children'][0]['code'] = 'p = p * 2.0;\nC = children(C, wobble(p));'
+    p.write_text(json.dumps(doc))
+    with pytest.raises(RuntimeError, match=r"group 'g' \(.*\): its code: unknown function 'wobble' \(line 2, column 17\)"):
         scenes.emit_doc(str(p))
     doc['root']['children'][0].pop('code')
     doc['root']['children'][0]['children'][1]['joinShape'] = 'bevel'
