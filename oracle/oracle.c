@@ -80,7 +80,7 @@ static void sampleSeed(uint32_t base, uint32_t sample, uint32_t* out) {
 
 /* ---- the scene program (hipgen.js `program`) ---- */
 enum { OP_PRIM = 0, OP_JOIN = 1, OP_FOLD = 2, OP_NODE = 3, OP_MOD = 4, OP_MODEND = 5, OP_USER = 6, OP_UGROUP = 7, OP_UFOLD = 8,
-       OP_UGROUPEND = 9 };
+       OP_UGROUPEND = 9, OP_RELIEF = 10 };
 /* the builtin modifier groups (builder/modifiers.js; hipgen.js MODIFIERS), in oracle.py's MODS order */
 enum { MOD_TWIST = 0, MOD_BEND, MOD_REPEAT, MOD_MIRROR, MOD_CIRCULAR, MOD_ROUND, MOD_SHELL };
 enum { SH_PLANE = 0, SH_SPHERE, SH_BOX, SH_CYLINDER, SH_CAPSULE, SH_TORUS, SH_RECT, SH_DISC, SH_LENS, SH_PRISM, SH_REFLECTOR, SH_ASPHERE,
@@ -526,6 +526,16 @@ void or_set_user(void* const* fns, int n) {
 }
 /* the materials' own code: (U, material, hit point, outward normal, the hit's primitive, the material to change) */
 typedef void (*OrUserMat)(const float* U, unsigned int mat, const double* p, const double* n, int prim, PtMaterial* m);
+/* a material's relief: (U, the object's point in real millimetres, the scene's point in millimetres) -> height, slope */
+typedef void (*OrRelief)(const float* U, const double* q, const double* world, double* out);
+static void* g_relief[64];
+static int g_nrelief = 0;
+void or_set_reliefs(void* const* fns, int n) {
+    g_nrelief = n < 64 ? n : 64;
+    for (int i = 0; i < g_nrelief; ++i) g_relief[i] = fns[i];
+}
+/* the scene's point the evaluation started from (a relief reads it), per thread */
+static _Thread_local d3 g_world;
 static OrUserMat g_umat = 0;
 void or_set_material(void* fn) { g_umat = (OrUserMat)fn; }
 /* the oracle's primitives for the scene code's shape functions: shape index, local point, parameters, (normal, d) */
@@ -641,6 +651,22 @@ static int evalOp(const OrScene* S, int i, d3 p, d4* st, int* sm, int* sk, int* 
     else if (o->op == OP_MOD) { st[*top] = modEval(S, i, p, &sm[*top], &sk[*top]); ++*top; i = o->k; }
     else if (o->op == OP_USER) { st[*top] = userEval(S, i, p, &sm[*top], &sk[*top]); ++*top; }
     else if (o->op == OP_UGROUP) { st[*top] = userEval(S, i, p, &sm[*top], &sk[*top]); ++*top; i = o->k; }
+    else if (o->op == OP_RELIEF) {
+        /* the relief of the object just evaluated (builder/codegen.js:1735-1760): beyond two amplitudes its distance moved
+         * towards zero by one, within them less the height at its point and divided by one plus the slope bound; no normal */
+        d4* v = &st[*top - 1];
+        const double a = S->U[o->p];
+        if (fabs(v->w) > 2 * a) v->w -= v->w < 0 ? -a : a;
+        else {
+            const float* t = S->U + o->xf;
+            d3 q = scl(frameIn(t, p), t[12] * 1000.0);
+            double qa[3] = {q.x, q.y, q.z}, wa[3] = {g_world.x * 1000.0, g_world.y * 1000.0, g_world.z * 1000.0}, hs[2] = {0, 0};
+            if (o->shape < g_nrelief && g_relief[o->shape]) ((OrRelief)g_relief[o->shape])(S->U, qa, wa, hs);
+            else hs[0] = NAN;
+            v->w = (v->w - 0.001 * hs[0]) / (1 + hs[1]);
+        }
+        v->x = v->y = v->z = 0;
+    }
     else if (o->op == OP_JOIN) {
         int t = --*top;
         d4 b = st[t], a = st[t - 1];
@@ -676,6 +702,7 @@ static const PtMaterial* hitMaterial(const OrScene* S, const OrHit* h, d3 p, d3 
  * changes nothing: such a node is neither the closest surface nor one p is inside). */
 static OrHit sceneAtM(const OrScene* S, d3 p, uint64_t mask) {
     d4 st[64]; int sm[64], sk[64]; int top = 0;
+    g_world = p;
     OrHit h; h.n = v(0, 0, 1); h.d = 1e7; h.m = 0; h.a = 0; h.prim = -1; h.insideW = -1e7;
     for (int i = 0; i < S->nops; ++i) {
         const OrOp* o = &S->ops[i];
@@ -686,7 +713,7 @@ static OrHit sceneAtM(const OrScene* S, d3 p, uint64_t mask) {
                 skip = len(sub(p, v(b[0], b[1], b[2]))) - b[3] >= fabs(h.d);
             }
             if (skip) i = o->k;
-        } else if (o->op == OP_PRIM || o->op == OP_MOD || o->op == OP_USER || o->op == OP_UGROUP || o->op == OP_JOIN) {
+        } else if (o->op == OP_PRIM || o->op == OP_MOD || o->op == OP_USER || o->op == OP_UGROUP || o->op == OP_JOIN || o->op == OP_RELIEF) {
             i = evalOp(S, i, p, st, sm, sk, &top);
         } else if (o->op == OP_MODEND) {
         } else {

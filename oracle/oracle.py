@@ -15,7 +15,7 @@ LIB = os.path.join(HERE, '_build', 'liboracle.so')
 SHAPES = ['plane', 'sphere', 'box', 'cylinder', 'capsule', 'torus', 'rectangle', 'disc', 'lens', 'prism', 'reflector', 'aspheric lens',
           'revolution', 'fresnel lens', 'extrusion', 'cone', 'round cone', 'octahedron', 'capped torus', 'link', 'tube', 'dome', 'pyramid']
 JOINS = ['union', 'subtract', 'intersect']
-OPS = {'prim': 0, 'join': 1, 'fold': 2, 'node': 3, 'mod': 4, 'modend': 5, 'user': 6, 'ugroup': 7, 'ufold': 8, 'ugroupend': 9}
+OPS = {'prim': 0, 'join': 1, 'fold': 2, 'node': 3, 'mod': 4, 'modend': 5, 'user': 6, 'ugroup': 7, 'ufold': 8, 'ugroupend': 9, 'relief': 10}
 MODS = ['twist', 'bend', 'repeat', 'mirror', 'circular', 'round', 'shell']
 
 _lib = None
@@ -60,6 +60,7 @@ def lib():
         L.or_mf_load.argtypes = [P(c_d)]
         L.or_set_user.argtypes = [P(c_v), c_i]
         L.or_set_material.argtypes = [c_v]
+        L.or_set_reliefs.argtypes = [P(c_v), c_i]
         L.or_mf_words.restype = c_i
         _lib = L
         _load_tables(L)
@@ -117,6 +118,8 @@ def pack_program(program):
             out[i, 1] = o['index']; out[i, 2] = o['xf']; out[i, 4] = o['mat']; out[i, 5] = o['prim']
         elif o['op'] == 'ugroup':
             out[i, 1] = o['index']; out[i, 2] = o['xf']; out[i, 7] = o['end']
+        elif o['op'] == 'relief':
+            out[i, 1] = o['index']; out[i, 2] = o['xf']; out[i, 3] = o['amp']
     return np.ascontiguousarray(out)
 
 
@@ -127,11 +130,12 @@ DEVICE_DIR = os.path.join(os.path.dirname(HERE), 'shadytracer_amd', 'csrc', 'dev
 _user_mods = {}
 
 
-def user_module(user, material=None):
-    """The compiled module of a scene's `user` list and material hook: (module, its function pointers in index order,
-    their count, the material hook or None)."""
+def user_module(user, material=None, reliefs=()):
+    """The compiled module of a scene's `user` list, material hook and reliefs: (module, its function pointers in index
+    order, their count, the material hook or None, the reliefs' pointers, their count)."""
     import hashlib
-    src = '#include "wgsl_host.h"\n' + '\n\n'.join(u['cpp'] for u in user) + '\n' + (material or '') + '\n'
+    src = ('#include "wgsl_host.h"\n' + '\n\n'.join(u['cpp'] for u in user) + '\n' + (material or '') + '\n' +
+           '\n\n'.join(r['cpp'] for r in reliefs) + '\n')
     key = hashlib.sha1(src.encode()).hexdigest()[:16]
     if key in _user_mods:
         return _user_mods[key]
@@ -153,7 +157,10 @@ def user_module(user, material=None):
     for i, u in enumerate(user):
         fns[i] = ctypes.cast(getattr(M, ('ucode%d' if u['kind'] == 'object' else 'ugroup%d') % i), ctypes.c_void_p)
     hook = ctypes.cast(M.umaterial, ctypes.c_void_p) if material else None
-    _user_mods[key] = (M, fns, len(user), hook)
+    rel = (ctypes.c_void_p * max(len(reliefs), 1))()
+    for i in range(len(reliefs)):
+        rel[i] = ctypes.cast(getattr(M, 'urelief%d' % i), ctypes.c_void_p)
+    _user_mods[key] = (M, fns, len(user), hook, rel, len(reliefs))
     return _user_mods[key]
 
 
@@ -163,7 +170,8 @@ class Scene:
     def __init__(self, scene):
         from shadytracer_amd import pt
         self.ops = pack_program(scene['program'])
-        self.user = user_module(scene.get('user') or [], scene.get('materialCode')) if (scene.get('user') or scene.get('materialCode')) else None
+        coded = scene.get('user') or scene.get('materialCode') or scene.get('reliefs')
+        self.user = user_module(scene.get('user') or [], scene.get('materialCode'), scene.get('reliefs') or []) if coded else None
         self.U = np.ascontiguousarray(np.asarray(scene['params'], np.float32))
         mats = scene['materials']
         self.mats = (pt.PtMaterial * len(mats))()
@@ -188,9 +196,11 @@ class Scene:
         if self.user is not None:
             lib().or_set_user(self.user[1], self.user[2])
             lib().or_set_material(self.user[3])
+            lib().or_set_reliefs(self.user[4], self.user[5])
         else:
             lib().or_set_user(None, 0)
             lib().or_set_material(None)
+            lib().or_set_reliefs(None, 0)
         return (self.ops.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), len(self.ops),
                 self.U.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
 

@@ -479,6 +479,16 @@ function usercode(opts = {}) {
         })
     }
     if (!opts.only) {
+        // two tiles wearing reliefs: ripples of a closed form, and fBm plaster
+        d.materials.push(Object.assign(material('tile', [0.6, 0.55, 0.5], 0.4, 0), { relief: { code: `let k = 6.2831853 / size;
+h = amplitude * sin(k * q.x) * cos(k * q.y);
+slope = amplitude * k;`, params: { amplitude: 0.5, size: 8 } } }),
+        Object.assign(material('plaster', [0.8, 0.78, 0.74], 0.9, 0), { relief: { code: `let peak = nzFbmLip(4u, 1.0, 0.5, 1.0);
+let n = fbm3(q / size, 1u, 4u, 2.0, 0.5);
+h = amplitude * n.w / peak;
+slope = amplitude / size * nzFbmLip(4u, 2.0, 0.5, 4.3) / peak;`, params: { amplitude: 1.0, size: 6 } } }))
+        d.root.children.push(obj('tile', 'b-box', 'tile', xf(-60, -125, 3), { size: { x: 25, y: 25, z: 3 }, rounding: 0 }),
+            obj('plaster', 'b-box', 'plaster', xf(60, -125, 3), { size: { x: 25, y: 25, z: 3 }, rounding: 0 }))
         d.blocks = d.blocks.concat([
             { id: 'u-cone', label: 'cone', kind: 'object', code: USER_CODE.cone, params: { base: 10, top: 4, height: 30 } },
             { id: 'u-menger', label: 'Menger sponge', kind: 'object', code: USER_CODE.menger, params: { size: 30, iterations: 3 },

@@ -644,6 +644,41 @@ function generate(doc, opts) {
         if (b && usercode.isCode(b.code)) return b.code
         return null
     }
+    // A material's relief (builder/codegen.js:1735-1760) on the object wearing it: the object's distance d (metres,
+    // its parent's frame), then beyond two amplitudes d moved towards zero by one, within them d less the height at the
+    // object's point (real millimetres in its own frame) and divided by one plus the height's slope bound; the normal by
+    // differences. Returns the lines that change d and f, or null for a material without one.
+    const reliefFns = {}
+    function reliefOn(n, xo, pt, d, f) {
+        const m = n.material && docMats[n.material]
+        if (!m || !m.relief || !usercode.isCode(m.relief.code)) return null
+        if (reliefFns[m.id] === undefined) {
+            const R = m.relief, P = R.params || {}
+            let fn
+            try {
+                fn = usercode.reliefFunction(reliefList.length, R.code, P, R.constants || {}, push)
+            } catch (e) {
+                cannot(`material '${m.id}'`, `its relief's code: ${e.message}`)
+                reliefFns[m.id] = null
+                return null
+            }
+            reliefFns[m.id] = { idx: reliefList.length, amp: push([0.001 * (+P.amplitude || 0)]) }
+            reliefList.push({ material: m.id, host: fn.host })
+            helpers.push(fn.device)
+        }
+        const R = reliefFns[m.id]
+        if (!R) return null
+        const i = tmp++
+        const band = (v) => [
+            `        { const float ra${i} = U[${R.amp}];`,
+            `          if (fabsf(${v}) > 2.0f * ra${i}) ${v} -= copysignf(ra${i}, ${v});`,
+            `          else { const V3 ql${i} = xfIn(U + ${xo}, ${pt}) * (U[${xo + 12}] * 1000.0f);`,
+            `                 const wvec2 hs${i} = urelief${R.idx}(U, ofV3(ql${i}), ofV3(p * 1000.0f));`,
+            `                 ${v} = (${v} - 0.001f * hs${i}.x) / (1.0f + hs${i}.y); } }`]
+        program.push({ op: 'relief', index: R.idx, xf: xo, amp: R.amp })
+        return { dist: band(d), full: [...band(`${f}.w`), `        ${f} = v4(v3s(0.0f), ${f}.w);`] }
+    }
+    const reliefList = []
     const codeParams = (n, b) => Object.assign({}, (b && b.params) || {}, n.params || {})
     const codeConstants = b => (b && b.constants) || {}
     const userFns = []
@@ -673,6 +708,8 @@ function generate(doc, opts) {
         full.push(`        unsigned int um${i}; V4 ${f} = xfOut(U + ${xo}, ucode${idx}(U, xfIn(U + ${xo}, ${pt}), &um${i}));`)
         full.push(`        unsigned int ${m} = um${i} & 0xFFu, ${k} = um${i} >> 8;`)
         program.push({ op: 'user', index: idx, xf: xo, mat, prim })
+        const rel = reliefOn(n, xo, pt, d, f)
+        if (rel) { dist.push(...rel.dist); full.push(...rel.full) }
         // the code's cost is not counted (an estimate stands for it); no bound is known, so the node is never culled
         return { d, f, m, k, bs: { c: X.t.slice(), r: Infinity }, flops: USER_FLOPS + XF_FLOPS }
     }
@@ -828,6 +865,12 @@ function generate(doc, opts) {
         full.push(`        V4 ${f} = xfOut(U + ${s.xo}, ${SHAPES[s.key].full(s.po)});`.replace(/\bq\b/, `xfIn(U + ${s.xo}, ${pt})`))
         full.push(`        unsigned int ${m} = ${mat}u, ${k} = ${prim}u;`)
         program.push({ op: 'prim', shape: s.key, xf: s.xo, p: s.po, mat, prim })
+        const rel = n.type !== 'light' ? reliefOn(n, s.xo, pt, d, f) : null
+        if (rel) {
+            dist.push(...rel.dist); full.push(...rel.full)
+            // the relief may lift the surface by its amplitude: the bound grows by it
+            s.bs = { c: s.bs.c, r: s.bs.r + 2 * (docMats[n.material].relief.params.amplitude || 0) * 0.001 }
+        }
         return { d, f, m, k, bs: s.bs, flops: s.flops }
     }
     // the closest-surface fold over a plain group's descendants
@@ -963,6 +1006,7 @@ function generate(doc, opts) {
         program,
         // the objects' and groups' own code as the oracle compiles it (FP64 C++ over oracle/wgsl_host.h), by index
         user: userFns.map(u => ({ kind: u.kind, node: u.node, cpp: u.host })),
+        reliefs: reliefList.map(r => ({ material: r.material, cpp: r.host })),
         materialCode: hook ? hook.host : null,
         flops: { dist: flops, maskNode: 14 },   // every node evaluated: the bound a culled step stays under
         approximations,
@@ -974,7 +1018,7 @@ function generate(doc, opts) {
 const hasStatements = code => typeof code === 'string' && code.split('\n').some(l => l.trim() && !l.trim().startsWith('//'))
 function materialLimits(m) {
     const out = [], P = (m && m.params) || {}
-    if (m.relief) out.push('a relief')
+    if (m.relief && !usercode.isCode(m.relief.code)) out.push('a relief without code')
     if (m.images && m.images.length) out.push('images')
     if (m.type && m.type !== 'emitter') out.push(`type '${m.type}'`)
     if (P.coating > 0) out.push('a thin-film coating')

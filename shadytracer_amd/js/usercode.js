@@ -175,6 +175,29 @@ function materialHook(cases, frames, unit) {
     return { device, host }
 }
 
+/**
+ * A material's relief (builder/codegen.js:1380-1406, :1735-1760): its code sets h, the height in millimetres, and slope,
+ * a bound on the height's slope, at q (the point in the frame of the object wearing it, real millimetres) and world
+ * (the scene's point, millimetres); its parameters by name. Device `urelief<i>(U, q, world)` -> (h, slope); host the
+ * same through pointers.
+ */
+function reliefFunction(i, code, params, constants, push) {
+    const P = paramDecls(params, constants, push)
+    const names = Object.assign(libraryNames(), P.names)
+    for (const k of ['add_new', 'fold', 'emptyHit', 'warp', 'r', 'XF_S', 'XF_M', 'XF_K', 'XF_SUB', 'mm', 'metres']) delete names[k]
+    const t = wgsl.translate(code, {
+        names, hitFns: new Set(), indent: '        ',
+        vars: { q: { cpp: 'wq' }, world: { cpp: 'wworld' }, h: { cpp: 'wh' }, slope: { cpp: 'wslope' } },
+    })
+    const body = ['    WF wh = 0, wslope = 0;', '    {', ...P.decls.map(l => '        ' + l), ...t.lines, '    }']
+    const device = [`__device__ __forceinline__ wvec2 urelief${i}(ParamPtr U, wvec3 wq, wvec3 wworld) {`, '    (void)wq; (void)wworld;',
+        ...body, '    return wvec2(wh, wslope);', '}'].join('\n')
+    const host = [`extern "C" void urelief${i}(const float* U, const double* q, const double* w, double* out) {`,
+        '    const wvec3 wq(q[0], q[1], q[2]), wworld(w[0], w[1], w[2]);', '    (void)wq; (void)wworld;', ...body,
+        '    out[0] = wh; out[1] = wslope;', '}'].join('\n')
+    return { device, host }
+}
+
 // The oracle's form of a group's code: its children come from the oracle's own evaluation of them, through `kids`
 // (each child's normal and distance in the group's frame, metres, and its material | primitive << 8)
 function hostGroup(i, gc) {
@@ -199,4 +222,4 @@ function hostGroup(i, gc) {
 // Is this text code (statements, not only comments)?
 const isCode = src => typeof src === 'string' && wgsl.hasStatements(src)
 
-module.exports = { objectFunction, groupCode, hostGroup, materialCode, materialHook, isCode, libraryNames }
+module.exports = { objectFunction, groupCode, hostGroup, materialCode, materialHook, reliefFunction, isCode, libraryNames }

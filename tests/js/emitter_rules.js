@@ -41,7 +41,7 @@ out.refusals = {
     code: refusal(doc({ materials: mat({ code: '// a texture\nm.baseColor = myTexture(p);\n' }) })),
     translated: refusal(doc({ materials: mat({ code: '// a checker\nm.baseColor = VEC3(step(0.0, sin(p.x)));\n' }) })),
     coating: refusal(doc({ materials: mat({ params: { coating: 1 } }) })),
-    relief: refusal(doc({ materials: mat({ relief: { code: '// bumps' } }) })),
+    relief: refusal(doc({ materials: mat({ relief: { code: 'h = heightAt(0u, q.xy).x;', params: { amplitude: 1 } } }) })),
     collimated: refusal(doc({ root: light({ shape: 'rectangle', distribution: 'collimated', flux: 10 }) })),
     cone: refusal(doc({ root: light({ shape: 'rectangle', halfAngle: 30 }) })),
     lambertian: refusal(doc({ root: light({ shape: 'rectangle', colourBy: 'cct', cct: 3000, flux: 100 }) })),

@@ -84,7 +84,7 @@ def test_what_the_engine_does_not_draw_is_refused_by_name(rules):
     assert R['studio'] is None and R['comments'] is None and R['lambertian'] is None and R['unusedcoating'] is None
     assert R['sunsky'] is None and R['nosetting'] is None and "scene's own code" in R['envcode']
     assert "material 'm': its code: unknown function 'myTexture' (line 2" in R['code'] and R['translated'] is None
-    assert 'coating' in R['coating'] and 'relief' in R['relief']
+    assert 'coating' in R['coating'] and "its relief's code: unknown function 'heightAt'" in R['relief']
     assert 'collimated' in R['collimated'] and 'cone' in R['cone']
     assert rules['approximations'] == ["material 'm': anisotropy"]
 
@@ -180,7 +180,7 @@ def test_reference_scenes_emit_or_name_what_stops_them():
             msg = str(e)
             assert 'the engine does not draw' in msg, (f, msg[-300:])
     # those with builtin shapes (lens, prism, reflector, aspheric lens, outlines and the solids among them), the studio
-    # or the sun & sky, and Lambertian lights; depth-of-field and first-steps through their materials' own code
+    # or the sun & sky, and Lambertian lights; five more through their materials' own code and reliefs
     assert {'led-lens-screen.json', 'parametric-part.json', 'rgbw-mixing.json', 'dusty-beam.json', 'parabolic-reflector.json',
             'aspheric-condenser.json', 'fresnel-plano-convex.json', 'combining-shapes.json', 'shape-gallery.json',
-            'depth-of-field.json', 'first-steps.json'} <= set(ok), ok
+            'depth-of-field.json', 'first-steps.json', 'lighting-room.json', 'office-lighting.json', 'relief-texture.json'} <= set(ok), ok

@@ -303,7 +303,7 @@ def test_material_code_runs_on_the_hit(drawn):
     from shadytracer_amd import pt
     user, _ = drawn
     sc = user['scene']
-    M, _, _, hook = oracle.user_module(sc['user'], sc['materialCode'])
+    hook = oracle.user_module(sc["user"], sc["materialCode"], sc["reliefs"])[3]
     assert hook is not None
     fn = ctypes.CFUNCTYPE(None, ctypes.POINTER(ctypes.c_float), ctypes.c_uint, ctypes.POINTER(ctypes.c_double),
                           ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.POINTER(pt.PtMaterial))(hook.value)
@@ -334,3 +334,29 @@ def test_material_code_runs_on_the_hit(drawn):
     c1, r1 = run(paint, [0.003, 0.002, 0.010], prim)
     c2, r2 = run(paint, [0.003, 0.002, 0.010], prim)
     assert np.array_equal(c1, c2) and r1 >= sc['materials'][paint]['roughness']
+
+
+def test_a_relief_moves_the_surface_by_its_height(drawn):
+    # the ripple tile: a box whose top face is 6 mm up, under h = A sin(k qx) cos(k qy) (A 0.5 mm, k = 2 pi / 8 mm,
+    # q the point in the tile's frame in mm) with slope bound A k. Within two amplitudes of the face the distance is
+    # (d - h) / (1 + A k), beyond them d less one amplitude towards zero (builder/codegen.js:1735-1760)
+    user, _ = drawn
+    sc = user['scene']
+    rng = np.random.default_rng(9)
+    n = 4000
+    x = rng.uniform(-0.075, -0.045, n)
+    y = rng.uniform(-0.140, -0.110, n)
+    z = rng.uniform(0.0045, 0.0075, n)
+    d, nrm, _ = oracle.scene_at(sc, np.c_[x, y, z])
+    A, k = 0.5, 2 * np.pi / 8
+    qx, qy = (x + 0.060) * 1000, (y + 0.125) * 1000
+    h = A * np.sin(k * qx) * np.cos(k * qy)
+    d0 = z - 0.006
+    band = np.abs(d0) <= 2 * A * 1e-3
+    want = np.where(band, (d0 - 1e-3 * h) / (1 + A * k), d0 - np.sign(d0) * A * 1e-3)
+    assert band.sum() > 1000 and (~band).sum() > 500
+    assert np.abs(d - want).max() < 1e-9
+    # under the surface where the ripple is above the point, and so on: the sign is the relief's
+    assert np.array_equal(d[band] < 0, (d0 - 1e-3 * h)[band] < 0)
+    # no analytic normal: the renderer takes it by differences
+    assert np.abs(nrm).max() == 0
