@@ -96,7 +96,8 @@ typedef struct PtMaterial {
     float abbe;           // > 0: dispersive (Cauchy through n_d and V_d)
     int face;             // light face this emissive material belongs to, -1 none
     float specularTint;
-    float pad[2];
+    float attenuation;    // > 0: absorbs inside, the depth (metres) at which it has filtered light to base (Beer-Lambert)
+    float pad[1];
 } PtMaterial;
 
 // A light's face for next-event estimation (the reference's LightFace, core/tracebasics.js:266-277).

@@ -524,6 +524,19 @@ void or_set_user(void* const* fns, int n) {
     g_nuser = n < 64 ? n : 64;
     for (int i = 0; i < g_nuser; ++i) g_user[i] = fns[i];
 }
+/* Beer-Lambert through distance d inside an absorbing material (materials.js mediumMu): exp(-mu d) per channel,
+ * mu = -ln(base held to [1e-6, 1]) / attenuation */
+static d3 mediumTrD(const PtMaterial* m, double d) {
+    if (!(m->attenuation > 0)) return v(1, 1, 1);
+    double t[3];
+    for (int k = 0; k < 3; ++k) t[k] = exp(log(clampd(m->base[k], 1e-6, 1)) * d / m->attenuation);
+    return v(t[0], t[1], t[2]);
+}
+/* the medium after a crossing: into the surface's material, or out to the one beyond (absorbing ones only), + 1 */
+static int mediumAfter(const OrScene* S, int inside, int hitMat, int beyond) {
+    const int into = inside ? beyond : hitMat;
+    return (into >= 0 && into < S->nmats && S->mats[into].attenuation > 0) ? into + 1 : 0;
+}
 /* the materials' own code: (U, material, hit point, outward normal, the hit's primitive, the material to change) */
 typedef void (*OrUserMat)(const float* U, unsigned int mat, const double* p, const double* n, int prim, PtMaterial* m);
 /* a material's relief: (U, the object's point in real millimetres, the scene's point in millimetres) -> height, slope */
@@ -888,7 +901,7 @@ static OrLobes makeLobes(d3 n, d3 view, const PtMaterial* m, double eta) {
     L.a = clampd(L.r * L.r, ALPHA_MIN, 1);
     L.metallic = clampd(m->metallic, 0, 1);
     L.trans = clampd(m->transmission, 0, 1);
-    L.base = f3(m->base); L.tint = L.base;
+    L.base = f3(m->base); L.tint = m->attenuation > 0 ? v(1, 1, 1) : L.base;   /* an absorbing body tints along the way, not at its faces */
     L.coatK = 2 * m->specular + (1 - 2 * m->specular) * L.trans;
     L.eta = eta; L.nrel = clampd(1 / eta, 1, 3);
     double fo = coatF(&L, L.o.z);
@@ -1381,7 +1394,8 @@ static void tracePhoton(const OrScene* S, const uint32_t* seed, uint32_t i, uint
         beta = scl(le, each / maxd(lumad(le), 1e-30));
     }
     d3 p = o, dir = d;
-    double n = 1, ltot = 0, lambda = 0;
+    double n = 1, ltot = 0, lambda = 0, vtxL = 0;
+    int med = 0;   /* the absorbing medium the photon travels in: material + 1, 0 none */
     uint32_t steps = 0, bounces = 0;
     int hadDelta = 0;
     double collide = hazeOn(S) ? hazeDraw(&u->haze, p, dir, u1[3]) : 1e300;
@@ -1406,15 +1420,17 @@ static void tracePhoton(const OrScene* S, const uint32_t* seed, uint32_t i, uint
         d3 nrm = inside ? neg(ns) : ns;
         if (dotp(dir, nrm) > 0) { double off = offsetTol(S, p); p = add(p, scl(dir, off)); ltot += off * n; continue; }
         bounces++;
+        if (med) beta = mul(beta, mediumTrD(&S->mats[med - 1], maxd(ltot - vtxL, 0) / maxd(n, 1e-4)));
         PtMaterial mown;
         const PtMaterial* m = hitMaterial(S, &h, p, ns, &mown);
         if (max3d(f3(m->emissive)) > 0) return;
         double n0 = n, n1 = maxd(m->ior, 1);
+        int beyond = -1;   /* the material beyond the surface when leaving one, if any */
         if (inside) {
             n1 = 1;
             if (m->transmission > 0) {
                 OrHit hb = sceneAt(S, add(p, scl(dir, 2 * offsetTol(S, p))));
-                if (hb.insideW > -1e6 && hb.a < S->nmats) n1 = maxd(S->mats[hb.a].ior, 1);   /* inside something (-1e7: nothing, air) */
+                if (hb.insideW > -1e6 && hb.a < S->nmats) { n1 = maxd(S->mats[hb.a].ior, 1); beyond = hb.a; }   /* inside something (-1e7: nothing, air) */
             }
         }
         double ud[4]; rng4(&rng, ud);
@@ -1445,8 +1461,9 @@ static void tracePhoton(const OrScene* S, const uint32_t* seed, uint32_t i, uint
         if (!s.delta) return;                 /* the map holds L S+ D paths: a photon goes on through delta lobes only */
         dir = s.dir;
         beta = mul(beta, s.w);
-        if (s.transmitted) { n = n1; if (draw) beta = mul(beta, v(sw[0], sw[1], sw[2])); }
+        if (s.transmitted) { n = n1; if (draw) beta = mul(beta, v(sw[0], sw[1], sw[2])); med = mediumAfter(S, inside, h.m < S->nmats ? h.m : 0, beyond); }
         else if (draw) lambda = lambda0;
+        vtxL = ltot;
         if (!(max3d(s.w) > 0)) return;
         hadDelta = 1;
         if (bounces > u->maxbounce) return;
@@ -1494,6 +1511,7 @@ static void tracePath(const OrScene* S, const OrTiles* T, const uint32_t* seed, 
     cameraRay(S, T, seed, ray, &p, &dir, &pix, &uh);
     d3 beta = v(1, 1, 1), rad = v(0, 0, 0);
     double n = 1, ltot = 0, prevPdf = -1, vtxLtot = 0, sunmis = 1, lambda = 0;
+    int med = 0;   /* the absorbing medium the path travels in: material + 1, 0 none */
     uint32_t steps = 0, bounces = 0;
     double collide = hazeOn(S) ? hazeDraw(&u->haze, p, dir, uh) : 1e300;
     int direct = 0;
@@ -1571,6 +1589,7 @@ static void tracePath(const OrScene* S, const OrTiles* T, const uint32_t* seed, 
         d3 nrm = inside ? neg(ns) : ns;
         if (dotp(dir, nrm) > 0) { double off = offsetTol(S, p); p = add(p, scl(dir, off)); ltot += off * n; continue; }
         bounces++;
+        if (med) beta = mul(beta, mediumTrD(&S->mats[med - 1], maxd(ltot - vtxLtot, 0) / maxd(n, 1e-4)));
         PtMaterial mown;
         const PtMaterial* m = hitMaterial(S, &h, p, ns, &mown);
         /* ...and the camera's first hit is a white Lambertian for that bounce: the pixel reads E / pi there */
@@ -1598,11 +1617,12 @@ static void tracePath(const OrScene* S, const OrTiles* T, const uint32_t* seed, 
             break;
         }
         double n0 = n, n1 = maxd(m->ior, 1);
+        int beyond = -1;   /* the material beyond the surface when leaving one, if any */
         if (inside) {
             n1 = 1;
             if (m->transmission > 0) {
                 OrHit hb = sceneAt(S, add(p, scl(dir, 2 * offsetTol(S, p))));
-                if (hb.insideW > -1e6 && hb.a < S->nmats) n1 = maxd(S->mats[hb.a].ior, 1);   /* inside something (-1e7: nothing, air) */
+                if (hb.insideW > -1e6 && hb.a < S->nmats) { n1 = maxd(S->mats[hb.a].ior, 1); beyond = hb.a; }   /* inside something (-1e7: nothing, air) */
             }
         }
         double ud[4]; rng4(&rng, ud);
@@ -1665,7 +1685,7 @@ static void tracePath(const OrScene* S, const OrTiles* T, const uint32_t* seed, 
         OrSample s = lobesSample(&L, us);
         dir = s.dir;
         beta = mul(beta, s.w);
-        if (s.transmitted) { n = n1; if (draw) beta = mul(beta, v(sw[0], sw[1], sw[2])); }
+        if (s.transmitted) { n = n1; if (draw) beta = mul(beta, v(sw[0], sw[1], sw[2])); med = mediumAfter(S, inside, h.m < S->nmats ? h.m : 0, beyond); }
         else if (draw) lambda = lambda0;
         int terminate = !(max3d(s.w) > 0);
         int misOut = u->nee && !s.delta;

@@ -181,6 +181,7 @@ class Scene:
                     getattr(self.mats[i], k)[j] = m[k][j]
             self.mats[i].metallic, self.mats[i].roughness, self.mats[i].transmission = m['metallic'], m['roughness'], m['transmission']
             self.mats[i].ior, self.mats[i].specular, self.mats[i].abbe, self.mats[i].face = m['ior'], m['specular'], m['abbe'], m['face']
+            self.mats[i].specularTint, self.mats[i].attenuation = m.get('specularTint', 0.0), m.get('attenuation', 0.0)
         nf = len(scene['faces'])
         self.faces = (pt.PtLightFace * max(nf, 1))()
         for i, f in enumerate(scene['faces']):

@@ -141,6 +141,15 @@ __device__ __forceinline__ float coatAlb(const Lobes& lb, const float* __restric
     return fminf(lb.coatK * mfEc(T, mu, lb.r, lb.nrel), 1.0f);
 }
 
+// Beer-Lambert through distance d of an absorbing material: base^(d / attenuation) per channel, the colour held to
+// [1e-6, 1] (materials.js mediumMu: exp(-mu d), mu = -ln(base) / attenuationDistance)
+__device__ __forceinline__ V3 mediumTr(const PtMaterial& m, float d) {
+    if (!(m.attenuation > 0.0f)) return v3s(1.0f);
+    const float k = d / m.attenuation;
+    return v3(exp2f(k * __log2f(fminf(fmaxf(m.base[0], 1e-6f), 1.0f))), exp2f(k * __log2f(fminf(fmaxf(m.base[1], 1e-6f), 1.0f))),
+              exp2f(k * __log2f(fminf(fmaxf(m.base[2], 1e-6f), 1.0f))));
+}
+
 __device__ __forceinline__ Lobes makeLobes(V3 n, V3 view, const PtMaterial& m, float eta, const float* __restrict__ T) {
     Lobes lb;
     frameOf(n, lb.tX, lb.tY);
@@ -153,7 +162,9 @@ __device__ __forceinline__ Lobes makeLobes(V3 n, V3 view, const PtMaterial& m, f
     lb.metallic = saturate(m.metallic);
     lb.trans = saturate(m.transmission);
     lb.base = ld3(m.base);
-    lb.tint = lb.base;
+    // a material that absorbs inside takes its colour off along the way through it, so its interface passes every
+    // channel alike (materials_pbr.js:206-210 transTint)
+    lb.tint = m.attenuation > 0.0f ? v3s(1.0f) : lb.base;
     lb.coatK = mixf(2.0f * m.specular, 1.0f, lb.trans);
     lb.eta = eta;
     lb.nrel = fminf(fmaxf(1.0f / eta, 1.0f), 3.0f);

@@ -430,6 +430,9 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
     uint4 M = ld4u(D->meta, slot);
     V3 p = v3(P.x, P.y, P.z), dir = v3(Dd.x, Dd.y, Dd.z), beta = v3(B.x, B.y, B.z), rad = v3(R.x, R.y, R.z);
     float ltot = P.w, n = Dd.w, prevPdf = B.w, vtxLtot = R.w, sunmis = A.x, collide = A.y, lambda = A.z;
+    // the absorbing medium the path travels in (materials.js attenuationDistance): its material + 1, 0 none
+    const unsigned int medIn = __float_as_uint(A.w);
+    unsigned int medOut = medIn;
     unsigned int pix = M.x, counts = M.y, z = M.z, tag = M.w;
     unsigned int bounces = mzBounces(z), state = mzState(z), flags = mzFlags(z);
     const PtEnv& env = D->u.env;
@@ -554,6 +557,9 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
             out.leaving = true;
         } else {
             bounces += 1u;
+            // ABSORPTION ALONG THE WAY (tracebasics.js absorbAlong): inside an absorbing medium the path loses
+            // base^(d / attenuation) per channel over the distance d since its last vertex
+            if (medIn != 0u) beta *= mediumTr(smat[medIn - 1u], fmaxf(ltot - vtxLtot, 0.0f) / fmaxf(n, 1e-4f));
             // the illuminance view (measureMode 1): the camera's first hit is the white Lambertian kept after the table
 #if PT_SCENE_MATCODE
             // the material's own code runs on the hit's material (not on the illuminance view's white)
@@ -736,6 +742,9 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
                 if (s.transmitted) {
                     n = n1;
                     if (draw) beta *= specW;
+                    // through the surface: into its material, or out to the one beyond (absorbing ones only)
+                    const unsigned int into = inside ? ((hw & PT_HR_BEYOND) ? (hw >> 16) : 0xFFFFFFFFu) : (hw & 0xFFu);
+                    medOut = (into != 0xFFFFFFFFu && smat[into].attenuation > 0.0f) ? into + 1u : 0u;
                 } else if (draw) {
                     lambda = lambda0;   // reflected: the wavelength is not the path's
                 }
@@ -824,7 +833,7 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
         st4(D->beta, slot, beta.x, beta.y, beta.z, prevPdf);
         st4(D->rad, slot, rad.x, rad.y, rad.z, vtxLtot);
         st4u(D->meta, slot, make_uint4(pix, counts, mzPack(bounces, PT_ACTIVE, flags), tag));
-        st4(D->aux, slot, sunmis, collide, lambda, 0.0f);
+        st4(D->aux, slot, sunmis, collide, lambda, __uint_as_float(medOut));
     }
     return out;
 }

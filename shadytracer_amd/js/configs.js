@@ -224,6 +224,22 @@ function furnaceSphere(opts = {}) {
     return { doc: d, camera: camera(pos, [0, 0, 0], 40, w, h), settings: { width: w, height: h, spp: 64, maxbounce: 40, nee: 1, maxiter: 400, epsAuto } }
 }
 
+// absorb-slab: a smooth glass slab (n 1.5) 20 mm thick in a uniform sphere of sky, its glass filtering light to a
+// colour of (0.5, 0.7, 0.9) at 10 mm (materials.js attenuationDistance), seen head-on: the centre reads
+// L (R + (1 - R)^2 (R tau^2 + tau) / (1 - R^2 tau^2)) per channel, R = 0.04 the face's Fresnel reflectance head-on and
+// tau = colour^(20 / 10) the inside's transmittance (the series of passes back and forth)
+function absorbSlab(opts = {}) {
+    const att = opts.attenuation !== undefined ? opts.attenuation : 10
+    const mats = [material('glass', [0.5, 0.7, 0.9], 0, 0, { specular: 0.5, transmission: 1, ior: 1.5, attenuationDistance: att })]
+    const d = doc('absorb-slab', mats, [obj('slab', 'b-box', 'glass', xf(0, 0, 0), { size: { x: 200, y: 10, z: 200 }, rounding: 0 })],
+        { sky: 'sphere', uniformSky: 1000 }, 20000)
+    d.unit = 'mm'
+    const w = opts.width || 32, h = opts.height || 32
+    // a fine tolerance: the march stops within it of a face, and the path counted inside differs from the slab's by
+    // up to it at each face (2 um against 20 mm here)
+    return { doc: d, camera: camera([0, -0.5, 0], [0, 0, 0], 4, w, h), settings: { width: w, height: h, spp: 64, maxbounce: 40, nee: 1, eps: 2e-6 } }
+}
+
 function furnaceHaze(opts = {}) {
     // a haze box of albedo 1 in a uniform sphere of sky: every pixel reads the sky's luminance (energy conservation)
     const g = opts.g !== undefined ? opts.g : 0.6
@@ -613,7 +629,7 @@ function opticsBall(opts = {}) {
 
 const CONFIGS = { 'plane-c1': planeC1, 'optics-floor': opticsFloor, 'optics-shell': opticsShell, 'optics-slab': opticsSlab,
     'optics-ball': opticsBall, 'room-c2': roomC2, 'caustic-c3': causticC3, 'furnace-plane': furnacePlane, 'furnace-sphere': furnaceSphere,
-    'furnace-haze': furnaceHaze,
+    'furnace-haze': furnaceHaze, 'absorb-slab': absorbSlab,
     'sppm-slab': sppmSlab, 'studio-floor': studioFloor, 'lens-prism': lensPrism, 'reflector-condenser': reflectorCondenser,
     'turned-parts': turnedParts, gallery, sunlit, modifiers, usercode, 'sunsky-floor': sunskyFloor, 'sunsky-bench': sunskyBench, joins }
 

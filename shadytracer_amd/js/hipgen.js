@@ -471,7 +471,14 @@ function generate(doc, opts) {
     for (const b of doc.blocks || []) blocks[b.id] = b
     const materials = [materialOf(null)]           // 0: the default
     const matIndex = {}, docMats = {}, checked = {}
-    for (const m of doc.materials || []) { matIndex[m.id] = materials.length; docMats[m.id] = m; materials.push(materialOf(m)) }
+    for (const m of doc.materials || []) {
+        matIndex[m.id] = materials.length; docMats[m.id] = m
+        const made = materialOf(m)
+        // absorption inside (materials.js attenuationDistance): its depth in metres
+        const ad = ((m.params || {}).attenuationDistance || 0)
+        made.attenuation = ad > 0 ? ad * unit : 0
+        materials.push(made)
+    }
     // what the document asks for that the engine does not draw: refused, or with opts.approximate listed and left out
     const approximations = []
     const cannot = (where, what) => approximations.push(`${where}: ${what}`)
@@ -1023,7 +1030,6 @@ function materialLimits(m) {
     if (m.type && m.type !== 'emitter') out.push(`type '${m.type}'`)
     if (P.coating > 0) out.push('a thin-film coating')
     if (P.anisotropy > 0) out.push('anisotropy')
-    if (P.attenuationDistance > 0) out.push('absorption inside (attenuationDistance)')
     if (P.haze > 0 || P.scatterDistance > 0) out.push('scattering inside')
     return out
 }

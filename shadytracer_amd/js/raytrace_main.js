@@ -36,7 +36,7 @@ const UNIFORMS = [['width', 'u', 1], ['height', 'u', 1], ['maxbounce', 'u', 1], 
     ['haze', HAZE, 1], ['faceShare', 'f', 1], ['sppm', 'u', 1], ['sppmRadius', 'f', 1], ['sppmPhotons', 'u', 1], ['marchRefill', 'u', 1],
     ['guide', 'u', 1], ['guideFrac', 'f', 1], ['sppmAlpha', 'f', 1], ['relax', 'f', 1], ['measureMode', 'u', 1], ['lt', 'u', 1]]
 const MATERIAL = [['base', 'f', 3], ['metallic', 'f', 1], ['roughness', 'f', 1], ['transmission', 'f', 1], ['ior', 'f', 1], ['specular', 'f', 1],
-    ['emissive', 'f', 3], ['abbe', 'f', 1], ['face', 'i', 1], ['specularTint', 'f', 1], ['pad', 'f', 2]]
+    ['emissive', 'f', 3], ['abbe', 'f', 1], ['face', 'i', 1], ['specularTint', 'f', 1], ['attenuation', 'f', 1], ['pad', 'f', 1]]
 const FACE = [['centre', 'f', 3], ['halfx', 'f', 1], ['u', 'f', 3], ['halfy', 'f', 1], ['v', 'f', 3], ['shape', 'u', 1], ['n', 'f', 3],
     ['twoSided', 'u', 1], ['le', 'f', 3], ['weight', 'f', 1]]
 

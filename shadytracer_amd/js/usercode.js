@@ -125,8 +125,10 @@ function groupCode(code, params, constants, unit, push) {
  */
 function materialCode(code, params, unit, push) {
     const P = paramDecls(params, {}, push)
-    const names = Object.assign(libraryNames(), P.names, { p_local: 'wpl', R_local: 'wRl', S_local: 'wSl' })
-    delete names.add_new; delete names.fold; delete names.emptyHit; delete names.warp; delete names.r
+    // the Hit's words are not a material's; its parameters may take their names
+    const names = libraryNames()
+    for (const k of ['add_new', 'fold', 'emptyHit', 'warp', 'r', 'XF_S', 'XF_M', 'XF_K', 'XF_SUB']) delete names[k]
+    Object.assign(names, P.names, { p_local: 'wpl', R_local: 'wRl', S_local: 'wSl' })
     const t = wgsl.translate(code, {
         names, hitFns: new Set(), indent: '',
         vars: { m: { cpp: 'wm', hit: true }, p: { cpp: 'wp' }, n: { cpp: 'wn' } },
@@ -183,8 +185,9 @@ function materialHook(cases, frames, unit) {
  */
 function reliefFunction(i, code, params, constants, push) {
     const P = paramDecls(params, constants, push)
-    const names = Object.assign(libraryNames(), P.names)
+    const names = libraryNames()
     for (const k of ['add_new', 'fold', 'emptyHit', 'warp', 'r', 'XF_S', 'XF_M', 'XF_K', 'XF_SUB', 'mm', 'metres']) delete names[k]
+    Object.assign(names, P.names)
     const t = wgsl.translate(code, {
         names, hitFns: new Set(), indent: '        ',
         vars: { q: { cpp: 'wq' }, world: { cpp: 'wworld' }, h: { cpp: 'wh' }, slope: { cpp: 'wslope' } },

@@ -27,7 +27,7 @@ class PtCamera(ctypes.Structure):
 class PtMaterial(ctypes.Structure):
     _fields_ = [('base', F3), ('metallic', ctypes.c_float), ('roughness', ctypes.c_float), ('transmission', ctypes.c_float),
                 ('ior', ctypes.c_float), ('specular', ctypes.c_float), ('emissive', F3), ('abbe', ctypes.c_float),
-                ('face', ctypes.c_int), ('specularTint', ctypes.c_float), ('pad', F2)]
+                ('face', ctypes.c_int), ('specularTint', ctypes.c_float), ('attenuation', ctypes.c_float), ('pad', ctypes.c_float * 1)]
 
 
 class PtLightFace(ctypes.Structure):
@@ -348,6 +348,7 @@ class Engine:
             mats[i].metallic, mats[i].roughness, mats[i].transmission = m['metallic'], m['roughness'], m['transmission']
             mats[i].ior, mats[i].specular, mats[i].abbe = m['ior'], m['specular'], m['abbe']
             mats[i].face, mats[i].specularTint = int(m['face']), m.get('specularTint', 0.0)
+            mats[i].attenuation = m.get('attenuation', 0.0)
         nf = len(scene['faces'])
         faces = (PtLightFace * max(nf, 1))()
         for i, f in enumerate(scene['faces']):

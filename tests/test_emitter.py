@@ -180,7 +180,8 @@ def test_reference_scenes_emit_or_name_what_stops_them():
             msg = str(e)
             assert 'the engine does not draw' in msg, (f, msg[-300:])
     # those with builtin shapes (lens, prism, reflector, aspheric lens, outlines and the solids among them), the studio
-    # or the sun & sky, and Lambertian lights; five more through their materials' own code and reliefs
+    # or the sun & sky, and Lambertian lights; eight more through their materials' own code, reliefs and absorption
     assert {'led-lens-screen.json', 'parametric-part.json', 'rgbw-mixing.json', 'dusty-beam.json', 'parabolic-reflector.json',
             'aspheric-condenser.json', 'fresnel-plano-convex.json', 'combining-shapes.json', 'shape-gallery.json',
-            'depth-of-field.json', 'first-steps.json', 'lighting-room.json', 'office-lighting.json', 'relief-texture.json'} <= set(ok), ok
+            'depth-of-field.json', 'first-steps.json', 'lighting-room.json', 'office-lighting.json', 'relief-texture.json',
+            'glass-liquid.json', 'product-shot.json', 'infinite-detail.json'} <= set(ok), ok

@@ -21,7 +21,7 @@ def say(*a):
     sys.stdout.flush()
 
 
-SCENE_OPTS = ('g', 'roughness', 'specular', 'metallic', 'albedo', 'mfp', 'floorRoughness', 'floorSpecular', 'sunDiameter', 'beam', 'native')
+SCENE_OPTS = ('g', 'roughness', 'specular', 'metallic', 'albedo', 'mfp', 'floorRoughness', 'floorSpecular', 'sunDiameter', 'beam', 'native', 'attenuation')
 
 
 def load(engine, name, **over):
@@ -783,3 +783,21 @@ def test_autotune_picks_a_setting_and_caches_it(engine, tmp_path):
     again = engine.autotune(samples=2, cache=str(tmp_path / 'tuning.json'))
     assert again['cached'] and again['blocksPerCu'] == t['blocksPerCu'] and engine.tuning()['refillOcclude'] == t['refillOcclude']
     engine.set_tuning(8, 16, 16)                  # back to the library's defaults for the tests after this one
+
+
+def test_an_absorbing_slab_transmits_its_closed_form(engine):
+    # absorption inside (attenuationDistance): a smooth slab 20 mm thick seen head-on in a uniform sky reads
+    # L (R + (1 - R)^2 (R tau^2 + tau) / (1 - R^2 tau^2)) per channel, tau = base^(20 / distance)
+    from test_absorption import slab_closed_form, centre
+    for att in (10.0, 40.0):
+        say('case: absorb-slab 32x32 4096 spp, attenuation', att)
+        made, u = load(engine, 'absorb-slab', attenuation=att)
+        engine.trace(4096)
+        acc, mom = engine.read_accum()
+        c, se = centre(acc, mom)
+        got = c.mean(0) / 1000
+        want = slab_closed_form(att)
+        rel = np.sqrt((se ** 2).sum()) / len(se) / (c @ [0.2126, 0.7152, 0.0722]).mean()
+        say('got', got, 'want', want, 'rel se', rel)
+        assert engine.stats()['nonfinite'] == 0
+        assert np.all(np.abs(got / want - 1) < 4 * rel + 2e-4), (att, got, want, rel)
