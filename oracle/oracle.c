@@ -1102,7 +1102,16 @@ static d3 sunSkyD(const PtEnv* e, d3 d) {
     /* col: the sky's gain per channel (the reference's envSky) */
     return v(maxd(e->L * L[0] * e->col[0], 0), maxd(e->L * L[1] * e->col[1], 0), maxd(e->L * L[2] * e->col[2], 0));
 }
+/* the scene's own sky code (its background(dir)), from the tests' compile of it */
+typedef void (*OrSky)(const double* d, double* out);
+static OrSky g_sky = 0;
+void or_set_sky(void* fn) { g_sky = (OrSky)fn; }
 static d3 envBg(const PtEnv* e, d3 d) {
+    if (e->type == 5) {
+        double in[3] = {d.x, d.y, d.z}, out[3] = {0, 0, 0};
+        if (g_sky) g_sky(in, out);
+        return v(out[0], out[1], out[2]);
+    }
     if (e->type == 1 || (e->type == 2 && d.z > 0)) return scl(f3(e->col), e->L);
     if (e->type == 3) return scl(f3(e->col), e->L * maxd(1 + e->top * d.z, 0));   /* environment.js studioLight:168 */
     if (e->type == 4) return inSun(e, d) ? v(0, 0, 0) : sunSkyD(e, d);

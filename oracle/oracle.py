@@ -61,6 +61,7 @@ def lib():
         L.or_set_user.argtypes = [P(c_v), c_i]
         L.or_set_material.argtypes = [c_v]
         L.or_set_reliefs.argtypes = [P(c_v), c_i]
+        L.or_set_sky.argtypes = [c_v]
         L.or_mf_words.restype = c_i
         _lib = L
         _load_tables(L)
@@ -130,12 +131,12 @@ DEVICE_DIR = os.path.join(os.path.dirname(HERE), 'shadytracer_amd', 'csrc', 'dev
 _user_mods = {}
 
 
-def user_module(user, material=None, reliefs=()):
+def user_module(user, material=None, reliefs=(), functions=None, sky=None):
     """The compiled module of a scene's `user` list, material hook and reliefs: (module, its function pointers in index
     order, their count, the material hook or None, the reliefs' pointers, their count)."""
     import hashlib
-    src = ('#include "wgsl_host.h"\n' + '\n\n'.join(u['cpp'] for u in user) + '\n' + (material or '') + '\n' +
-           '\n\n'.join(r['cpp'] for r in reliefs) + '\n')
+    src = ('#include "wgsl_host.h"\n' + (functions or '') + '\n' + '\n\n'.join(u['cpp'] for u in user) + '\n' + (material or '') +
+           '\n' + '\n\n'.join(r['cpp'] for r in reliefs) + '\n' + (sky or '') + '\n')
     key = hashlib.sha1(src.encode()).hexdigest()[:16]
     if key in _user_mods:
         return _user_mods[key]
@@ -160,7 +161,8 @@ def user_module(user, material=None, reliefs=()):
     rel = (ctypes.c_void_p * max(len(reliefs), 1))()
     for i in range(len(reliefs)):
         rel[i] = ctypes.cast(getattr(M, 'urelief%d' % i), ctypes.c_void_p)
-    _user_mods[key] = (M, fns, len(user), hook, rel, len(reliefs))
+    skyfn = ctypes.cast(M.usky, ctypes.c_void_p) if sky else None
+    _user_mods[key] = (M, fns, len(user), hook, rel, len(reliefs), skyfn)
     return _user_mods[key]
 
 
@@ -170,8 +172,9 @@ class Scene:
     def __init__(self, scene):
         from shadytracer_amd import pt
         self.ops = pack_program(scene['program'])
-        coded = scene.get('user') or scene.get('materialCode') or scene.get('reliefs')
-        self.user = user_module(scene.get('user') or [], scene.get('materialCode'), scene.get('reliefs') or []) if coded else None
+        coded = scene.get('user') or scene.get('materialCode') or scene.get('reliefs') or scene.get('skyCode')
+        self.user = (user_module(scene.get('user') or [], scene.get('materialCode'), scene.get('reliefs') or [], scene.get('functionsCode'),
+                                 scene.get('skyCode')) if coded else None)
         self.U = np.ascontiguousarray(np.asarray(scene['params'], np.float32))
         mats = scene['materials']
         self.mats = (pt.PtMaterial * len(mats))()
@@ -198,10 +201,12 @@ class Scene:
             lib().or_set_user(self.user[1], self.user[2])
             lib().or_set_material(self.user[3])
             lib().or_set_reliefs(self.user[4], self.user[5])
+            lib().or_set_sky(self.user[6])
         else:
             lib().or_set_user(None, 0)
             lib().or_set_material(None)
             lib().or_set_reliefs(None, 0)
+            lib().or_set_sky(None)
         return (self.ops.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), len(self.ops),
                 self.U.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
 
@@ -234,8 +239,10 @@ def pick(scene, u, sample=0, first=0, count=None):
     return pos, info[:, 0] & 0xFF, info[:, 0] >> 8, info[:, 1]
 
 
-def env(u, dirs):
-    """The environment's radiance (sky and sun disc), RGB, in each direction."""
+def env(u, dirs, scene=None):
+    """The environment's radiance (sky and sun disc), RGB, in each direction; `scene` for a scene's own sky code."""
+    if scene is not None:
+        Scene(scene).args()
     dirs = np.ascontiguousarray(np.asarray(dirs, np.float64).reshape(-1, 3))
     out = np.zeros((len(dirs), 3))
     lib().or_env(ctypes.byref(u), _dp(dirs), len(dirs), _dp(out))

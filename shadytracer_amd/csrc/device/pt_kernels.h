@@ -189,6 +189,9 @@ __device__ __forceinline__ V3 envBackground(const PtEnv& e, V3 d) {
     if (e.type == 1u || (e.type == 2u && d.z > 0.0f)) return ld3(e.col) * e.L;
     if (e.type == 3u) return ld3(e.col) * (e.L * fmaxf(1.0f + e.top * d.z, 0.0f));
     if (e.type == 4u) return inSun(e, d) ? v3s(0.0f) : sunSky(e, d);      // the disc is envSun's
+#if PT_SCENE_SKY
+    if (e.type == 5u) return sceneSky(d);                                   // the scene's own sky code
+#endif
     return v3s(0.0f);
 }
 // The sun's radiance along the ray from p towards d. With an aperture the sun is a beam: the lines that cross the disc

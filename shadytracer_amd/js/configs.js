@@ -224,6 +224,38 @@ function furnaceSphere(opts = {}) {
     return { doc: d, camera: camera(pos, [0, 0, 0], 40, w, h), settings: { width: w, height: h, spp: 64, maxbounce: 40, nee: 1, maxiter: 400, epsAuto } }
 }
 
+// sky-code: a floor and a sphere under the studio written as the scene's own sky code (builder/sky.js: background(dir)
+// with its parameters), the sphere's code calling a function of the document's functions block. With { native: true }
+// the same scene under the builtin studio (envMode 1, 1000 lx, top 0.5), so the two can be held to one another.
+function skyCode(opts = {}) {
+    const { cameraOf } = require('./hipgen')
+    const top = 0.5, E = 1000
+    const L0 = E / (Math.PI * (1 + 2 * top / 3))
+    const mats = [material('floor', [0.7, 0.7, 0.7], 1, 0, { specular: 0 }), material('ball', [0.8, 0.3, 0.2], 0.4, 0)]
+    const blocks = BLOCKS.concat([{ id: 'u-ball', label: 'ball', kind: 'object', code: 'C = add_new(C, ball(p, radius), MATERIAL);', params: { radius: 40 } }])
+    const d = studioDoc('sky-code', mats, [obj('floor', 'b-plane', 'floor', xf(0, 0, 0)), obj('ball', 'u-ball', 'ball', xf(0, 0, 40), {})], blocks,
+        { mode: 'orbit', camtype: 0, fov: 40, target: [0, 0, 30], azimuth: -90, elevation: 70, distance: 400, up: [0, 0, 1], aperture: 0 })
+    d.environment = { envMode: 1, studioIlluminance: E, studioTop: top, studioColor: { r: 1, g: 1, b: 1 } }
+    if (!opts.native) {
+        d.functions = `// a ball of radius r: its outward normal and distance
+fn ball(q: vec3f, r: f32) -> vec4f {
+    let l = length(q);
+    return vec4f(select(vec3f(0.0, 0.0, 1.0), q / l, l > 0.0), l - r);
+}`
+        d.sky = `// the studio: brighter above
+const BASE: f32 = 1.0;
+fn background(dir: VEC3) -> VEC3 {
+    return VEC3(L0 * max(BASE + top * dir.z, 0.0));
+}`
+        d.skyParams = [{ name: 'L0', value: L0 }, { name: 'top', value: top }]
+    } else {
+        d.blocks = BLOCKS.concat([{ id: 'u-ball', label: 'ball', kind: 'object', builtin: 'sphere@1', params: { radius: 40 } }])
+        d.root.children[1].params = { radius: 40 }
+    }
+    const w = opts.width || 64, h = opts.height || 48
+    return { doc: d, camera: cameraOf(d, w, h), settings: { width: w, height: h, spp: 64, maxbounce: 6, nee: 1 } }
+}
+
 // absorb-slab: a smooth glass slab (n 1.5) 20 mm thick in a uniform sphere of sky, its glass filtering light to a
 // colour of (0.5, 0.7, 0.9) at 10 mm (materials.js attenuationDistance), seen head-on: the centre reads
 // L (R + (1 - R)^2 (R tau^2 + tau) / (1 - R^2 tau^2)) per channel, R = 0.04 the face's Fresnel reflectance head-on and
@@ -629,7 +661,7 @@ function opticsBall(opts = {}) {
 
 const CONFIGS = { 'plane-c1': planeC1, 'optics-floor': opticsFloor, 'optics-shell': opticsShell, 'optics-slab': opticsSlab,
     'optics-ball': opticsBall, 'room-c2': roomC2, 'caustic-c3': causticC3, 'furnace-plane': furnacePlane, 'furnace-sphere': furnaceSphere,
-    'furnace-haze': furnaceHaze, 'absorb-slab': absorbSlab,
+    'furnace-haze': furnaceHaze, 'absorb-slab': absorbSlab, 'sky-code': skyCode,
     'sppm-slab': sppmSlab, 'studio-floor': studioFloor, 'lens-prism': lensPrism, 'reflector-condenser': reflectorCondenser,
     'turned-parts': turnedParts, gallery, sunlit, modifiers, usercode, 'sunsky-floor': sunskyFloor, 'sunsky-bench': sunskyBench, joins }
 

@@ -482,6 +482,19 @@ function generate(doc, opts) {
     // what the document asks for that the engine does not draw: refused, or with opts.approximate listed and left out
     const approximations = []
     const cannot = (where, what) => approximations.push(`${where}: ${what}`)
+    // the document's functions block (builder/codegen.js:1366-1370): C++ ahead of every node's code, its names in scope
+    // for every code of the document
+    usercode.setShared({})
+    let functionsCpp = ''
+    if (usercode.isCode(doc.functions)) {
+        try {
+            const mod = usercode.moduleCode(doc.functions)
+            functionsCpp = mod.text
+            usercode.setShared(mod.own)
+        } catch (e) {
+            cannot('the functions block', e.message)
+        }
+    }
     if (materials.length > 64) throw new Error('hipgen: more than 64 materials')
 
     const params = []
@@ -492,6 +505,7 @@ function generate(doc, opts) {
     let haze = null
     let flops = 0
     const dist = [], full = [], maskCode = [], helpers = []
+    if (functionsCpp) helpers.push('// the document\'s functions block', functionsCpp)
     let tmp = 0, maskBits = 0, modDepth = 0
     // each primitive's object frame (its U offset): its node's own, or inside a modifier or coded group the outermost
     // such group's (the frame a material's code reads p_local in)
@@ -707,6 +721,17 @@ function generate(doc, opts) {
         const xo = push(xfParams(X))
         primFrames[prim] = outerFrame !== null ? outerFrame : xo
         userFns.push({ kind: 'object', node: n.id, host: fn.host })
+        // no bound is known for code, but the scene's extent (its tolerance) counts the node as a ball about its place of
+        // its largest parameter (an estimate of its size, in the document's unit), never for culling
+        if (!modDepth) {
+            const sizes = []
+            for (const v of Object.values(codeParams(n, b))) {
+                if (typeof v === 'number') sizes.push(Math.abs(v))
+                else if (v && typeof v === 'object' && !Array.isArray(v)) for (const c of ['x', 'y', 'z']) if (typeof v[c] === 'number') sizes.push(Math.abs(v[c]))
+            }
+            const r = (sizes.length ? Math.max(...sizes) : 0) * unit * X.s
+            if (r > 0 && isFinite(r)) for (let a = 0; a < 3; a++) { bounds[0][a] = Math.min(bounds[0][a], X.t[a] - r); bounds[1][a] = Math.max(bounds[1][a], X.t[a] + r) }
+        }
         helpers.push(fn.device)
         layout[n.id] = { xf: xo, shape: 'code' }
         const i = tmp++
@@ -956,7 +981,21 @@ function generate(doc, opts) {
     const hook = matCases.length ? usercode.materialHook(matCases, primFrames.map(f => (f === undefined ? -1 : f)), unit) : null
 
     const envRadius = (doc.envRadius !== undefined ? doc.envRadius * unit : 400)
-    const env = environmentOf(doc.environment || {}, doc, cannot)
+    let env = environmentOf(doc.environment || {}, doc, cannot)
+    // the scene's own sky (builder/sky.js): its code lights the scene, whatever the environment's mode
+    let skyMod = null
+    if (usercode.isCode(doc.sky)) {
+        try {
+            skyMod = usercode.moduleCode(doc.sky, usercode.skyParamNames(doc.skyParams))
+            if (!skyMod.own.background) throw new Error('it defines no background(dir)')
+            env = Object.assign({}, env, { type: 5, L: 1, sunE: 0, sunL: [0, 0, 0] })
+        } catch (e) {
+            cannot('environment', `the sky's code: ${e.message}`)
+            skyMod = null
+        }
+    } else if (((doc.environment || {}).envMode || 0) === 2) {
+        cannot('environment', 'the scene\'s own code (envMode 2) with no sky code')
+    }
     if (approximations.length && !opts.approximate) {
         throw new Error(`hipgen: the engine does not draw ${approximations.join('; ')} -- generate(doc, { approximate: true }) renders the scene without them`)
     }
@@ -966,6 +1005,9 @@ function generate(doc, opts) {
         `#define PT_SCENE_MATERIALS ${materials.length}`,
         `#define PT_SCENE_MASK_BITS ${maskBits}`,
         `#define PT_SCENE_MATCODE ${hook ? 1 : 0}`,
+        `#define PT_SCENE_SKY ${skyMod ? 1 : 0}`,
+        ...(skyMod ? ['// the scene\'s own sky', skyMod.text,
+            '__device__ __forceinline__ V3 sceneSky(V3 d) { const wvec3 r = u_background(wvec3(d.x, d.y, d.z)); return v3(r.x, r.y, r.z); }'] : []),
         ...helpers,
         '// the nodes a ray from o along d can reach within tmax: those whose bounding sphere, widened by margin, its',
         '// segment meets (bit k of the mask: the k-th bounded top-level node); unbounded nodes are always marched',
@@ -1015,6 +1057,9 @@ function generate(doc, opts) {
         user: userFns.map(u => ({ kind: u.kind, node: u.node, cpp: u.host })),
         reliefs: reliefList.map(r => ({ material: r.material, cpp: r.host })),
         materialCode: hook ? hook.host : null,
+        functionsCode: functionsCpp || null,
+        skyCode: skyMod ? [skyMod.text, 'extern "C" void usky(const double* d, double* out) {',
+            '    const wvec3 r = u_background(wvec3(d[0], d[1], d[2]));', '    out[0] = r.x; out[1] = r.y; out[2] = r.z;', '}'].join('\n') : null,
         flops: { dist: flops, maskNode: 14 },   // every node evaluated: the bound a culled step stays under
         approximations,
     }
@@ -1129,9 +1174,8 @@ function environmentOf(E, doc, cannot) {
         sunL: [0, 0, 0], sunSin2: 0, sunOmega: 0, skyAerosol: 0, skyMs: 1, skyUp: 0 }
     const cone = d => { const c = sky.sunCone(d); env.sunCos = c.cos; env.sunSin2 = c.sin2; env.sunOmega = c.omega; return c }
     if (E.sky === undefined) {
-        if (hasStatements(doc.sky)) cannot('environment', 'the scene\'s own sky code')
         const mode = E.envMode || 0
-        if (mode === 2) { cannot('environment', 'the scene\'s own code (envMode 2)'); return env }
+        if (mode === 2) return env
         if (mode === 0) {
             // the sun & sky (environment.js SUNSKY_FN; photometry.js derive:183 puts the sun's cone in use while the
             // hint is on and its disc not wholly below the horizon)

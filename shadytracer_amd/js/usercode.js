@@ -20,8 +20,11 @@ const SHAPE_FNS = ['sd', 'sdSphere', 'sdBox', 'sdRoundedBox', 'sdCylinder', 'sdC
     'Noctahedron', 'Nlink', 'Ntube', 'Ndome', 'safeDir', 'g2Extrude', 'g2Revolve']
 const NOISE_FNS = ['vnoise3', 'gnoise3', 'fbm3', 'nzFbmLip', 'worley3', 'worley3x']
 
+// the document's own functions (its `functions` block), callable from every code of the document: set per document
+let shared = {}
+function setShared(own) { shared = Object.assign({}, own || {}) }
 function libraryNames() {
-    const names = {}
+    const names = Object.assign({}, shared)
     for (const b of BUILTINS.concat(SHAPE_FNS, NOISE_FNS)) names[b] = `w_${b}`
     Object.assign(names, {
         NZ_GRAD_LIP_MEASURED: 'WL(4.3)', NZ_GRAD_VALUE_BOUND: 'WL(1.3856406460551018)', NZ_GRAD_SCALE: 'WL(1.6)',
@@ -34,6 +37,29 @@ function libraryNames() {
     return names
 }
 const HIT_FNS = new Set(['add_new', 'fold', 'emptyHit', 'children'])
+
+/**
+ * A module of the document's WGSL -- its `functions` block (builder/codegen.js:1366-1370, callable from every node's,
+ * group's and material's code) or its sky (builder/sky.js, core/environment.js ENV_FN: `background(dir)` gives the
+ * environment's radiance) -- as C++ functions and constants. `extra`: names in scope besides the library (the sky's
+ * parameters). Returns { text (the C++), own (the module's names as C++) }.
+ */
+function moduleCode(src, extra) {
+    const names = Object.assign(libraryNames(), extra || {})
+    for (const k of ['add_new', 'fold', 'emptyHit', 'warp', 'r', 'XF_S', 'XF_M', 'XF_K', 'XF_SUB', 'children', 'mm', 'metres']) delete names[k]
+    const t = wgsl.translate(src, { names, hitFns: new Set(), module: true })
+    return { text: t.fns.join('\n'), own: t.own }
+}
+// The sky's parameters (doc.skyParams: [{name, value}], a number or three) as literals of the sky's code
+function skyParamNames(params) {
+    const out = {}
+    for (const p of params || []) {
+        if (!/^[A-Za-z_][A-Za-z0-9_]*$/.test(p.name || '')) throw new Error(`sky parameter name '${p.name}'`)
+        const v = p.value
+        out[p.name] = Array.isArray(v) ? `wvec3(WL(${(+v[0]).toExponential()}), WL(${(+v[1]).toExponential()}), WL(${(+v[2]).toExponential()}))` : `WL(${(+v).toExponential()})`
+    }
+    return out
+}
 
 // A parameter's C++ declaration: a number through U (live), a compile-time constant as a literal, a vector through U
 function paramDecls(params, constants, push) {
@@ -225,4 +251,4 @@ function hostGroup(i, gc) {
 // Is this text code (statements, not only comments)?
 const isCode = src => typeof src === 'string' && wgsl.hasStatements(src)
 
-module.exports = { objectFunction, groupCode, hostGroup, materialCode, materialHook, reliefFunction, isCode, libraryNames }
+module.exports = { objectFunction, groupCode, hostGroup, materialCode, materialHook, reliefFunction, moduleCode, skyParamNames, setShared, isCode, libraryNames }

@@ -65,7 +65,7 @@ function tokenize(src) {
 
 // ----------------------------------------------------------------------------------------------------------------
 // The parser: statements and expressions into plain objects
-function parse(src) {
+function parse(src, module = false) {
     const T = tokenize(src)
     let i = 0
     const peek = (k = 0) => T[i + k]
@@ -212,8 +212,26 @@ function parse(src) {
         }
     }
 
+    // a module's function: fn name(a: T, ...) -> R { ... }
+    function fnDecl() {
+        const at = next()
+        const name = ident()
+        want('(')
+        const params = []
+        while (!is(')')) {
+            const pn = ident(); want(':'); params.push({ name: pn.v, ty: type(), tok: pn })
+            if (!eat(',')) break
+        }
+        want(')')
+        const ret = eat('->') ? type() : null
+        return { k: 'fn', name: name.v, params, ret, body: block(), tok: at }
+    }
     const body = []
-    while (peek().t !== 'eof') body.push(statement())
+    while (peek().t !== 'eof') {
+        if (module && is('fn')) body.push(fnDecl())
+        else if (module && is('@')) throw new WgslError('attributes are not supported in scene code', peek())
+        else body.push(statement())
+    }
     return body
 }
 
@@ -266,7 +284,12 @@ function literal(v, tok) {
  *          hitFns: Set of functions returning a Hit, indent: '    ' }
  */
 function translate(src, scope) {
-    const body = parse(src)
+    const all = parse(src, !!scope.module)
+    // a module's functions and constants, in order, ahead of its statements; each callable from the code after it
+    const fnDecls = all.filter(s => s.k === 'fn')
+    const body = scope.module ? all.filter(s => s.k !== 'fn' && s.k !== 'const') : all
+    const constDecls = scope.module ? all.filter(s => s.k === 'const') : []
+    if (scope.module && body.length) throw new WgslError('a module holds functions and constants only', body[0].tok)
     const names = scope.names || {}
     const hitFns = scope.hitFns || new Set()
     const frames = [new Map()]
@@ -419,8 +442,44 @@ function translate(src, scope) {
         }
         return [`${ind}${simple(s)};`]
     }
+    // the module's own names: a function u_<name>, a constant a function of none u_<name>_()
+    const fns = []
+    const own = {}
+    for (const c of constDecls) {
+        own[c.name] = `u_${c.name}_()`
+        names[c.name] = own[c.name]
+        const tn = c.ty ? typeName(c.ty) : null
+        fns.push(`WG_FN ${tn || 'auto'} u_${c.name}_() { return ${tn ? `wconv<${tn}>(${ex(c.init)})` : ex(c.init)}; }`)
+    }
+    // every function declared ahead of the definitions, so each may call any other (WGSL's order is free)
+    for (const f of fnDecls) {
+        if (Object.prototype.hasOwnProperty.call(names, f.name) && !own[f.name]) throw new WgslError(`function '${f.name}' takes a name the library has`, f.tok)
+        own[f.name] = `u_${f.name}`
+        names[f.name] = own[f.name]
+    }
+    const protos = []
+    for (const f of fnDecls) {
+        const ret = f.ret ? typeName(f.ret) : 'void'
+        if (!ret) throw new WgslError(`unknown type '${f.ret.name}'`, f.ret.tok)
+        frames.push(new Map())
+        const ps = f.params.map(pp => {
+            const tn = typeName(pp.ty)
+            if (!tn) throw new WgslError(`unknown type '${pp.ty.name}'`, pp.ty.tok)
+            const d = declare(pp.name, false, pp.tok)
+            d.konst = true
+            return `${tn} ${d.cpp}`
+        })
+        const saved = scope.returns
+        scope.returns = (e) => (e === null ? 'return' : `return ${ret === 'void' ? e : `wconv<${ret}>(${e})`}`)
+        const inner = stmts(f.body, '    ')
+        scope.returns = saved
+        frames.pop()
+        protos.push(`WG_FN ${ret} u_${f.name}(${ps.join(', ')});`)
+        fns.push([`WG_FN ${ret} u_${f.name}(${ps.join(', ')}) {`, ...inner, '}'].join('\n'))
+    }
+    fns.splice(constDecls.length, 0, ...protos)
     const lines = stmts(body, scope.indent || '    ')
-    return { lines, calls }
+    return { lines, calls, fns, own }
 }
 
 // Does the text hold statements (not only comments and space)?

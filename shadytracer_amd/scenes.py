@@ -9,7 +9,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 JS = os.path.join(PKG, 'js')
 STANDARD = ['plane-c1', 'room-c2', 'caustic-c3', 'furnace-plane', 'furnace-sphere', 'furnace-haze', 'sppm-slab', 'studio-floor', 'lens-prism',
             'reflector-condenser', 'turned-parts', 'gallery', 'sunlit', 'sunsky-floor', 'joins', 'optics-floor', 'optics-shell',
-            'optics-slab', 'optics-ball', 'modifiers', 'usercode', 'absorb-slab']
+            'optics-slab', 'optics-ball', 'modifiers', 'usercode', 'absorb-slab', 'sky-code']
 NODE = os.environ.get('NODE', 'node')
 
 _memo = {}

@@ -801,3 +801,24 @@ def test_an_absorbing_slab_transmits_its_closed_form(engine):
         say('got', got, 'want', want, 'rel se', rel)
         assert engine.stats()['nonfinite'] == 0
         assert np.all(np.abs(got / want - 1) < 4 * rel + 2e-4), (att, got, want, rel)
+
+
+def test_the_scenes_own_sky_code_lights_as_the_studio(engine):
+    # the sky written as the scene's code (background(dir) with its parameters) and a ball of the document's functions
+    # block: the picture of the builtin studio and sphere within the spread, and the oracle's FP64 compile of it
+    say('case: sky-code 64x48, code vs builtin studio 256 spp each, vs oracle 64 spp')
+    out = []
+    for native in (False, True):
+        made, u = load(engine, 'sky-code', native=native, seed=11 + int(native))
+        engine.trace(256)
+        out.append(engine.read_accum())
+        if not native:
+            oa, om, _ = oracle.render(made['scene'], u, 64, first=100000)
+    (ca, cm), (na, nm) = out
+    for (a, m), (b, n) in (((ca, cm), (na, nm)), ((ca, cm), (oa, om))):
+        La, Lb = pt.luminance(a), pt.luminance(b)
+        s = np.sqrt(pt.standard_error(m) ** 2 + pt.standard_error(n) ** 2)
+        ok = s > 1e-9
+        z = (La - Lb)[ok] / s[ok]
+        say('z std', z.std(), 'means', La.mean(), Lb.mean())
+        assert 0.75 < z.std() < 1.3 and np.mean(np.abs(z) < 3) > 0.97

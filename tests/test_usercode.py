@@ -360,3 +360,50 @@ def test_a_relief_moves_the_surface_by_its_height(drawn):
     assert np.array_equal(d[band] < 0, (d0 - 1e-3 * h)[band] < 0)
     # no analytic normal: the renderer takes it by differences
     assert np.abs(nrm).max() == 0
+
+
+# ---- the document's functions block and its own sky (builder/sky.js: background(dir)) ------------------------------
+def test_module_functions_and_constants_translate_in_any_order(tmp_path):
+    import subprocess as sp
+    code = ('const TWO: f32 = 2.0;\nfn twice(x: f32) -> f32 { return half(x) * TWO * TWO; }\n'
+            'fn half(x: f32) -> f32 { return x / TWO; }\nfn v3of(x: f32) -> vec3f { return vec3f(x, twice(x), 0.0); }')
+    js = ("const u = require(%r); const m = u.moduleCode(%s); process.stdout.write(JSON.stringify(m))"
+          % (os.path.join(ROOT, 'shadytracer_amd', 'js', 'usercode.js'), json.dumps(code)))
+    m = json.loads(sp.run(['node', '-e', js], stdout=sp.PIPE, check=True, text=True).stdout)
+    assert m['own'] == {'TWO': 'u_TWO_()', 'twice': 'u_twice', 'half': 'u_half', 'v3of': 'u_v3of'}
+    src = tmp_path / 'm.cpp'
+    src.write_text('#include "wgsl_host.h"\n' + m['text'] + '\nextern "C" double go() { return u_v3of(3.0).y + u_TWO_(); }\n')
+    r = sp.run(['g++', '-O1', '-std=c++14', '-shared', '-fPIC', '-I', os.path.join(ROOT, 'oracle'), '-I',
+                os.path.join(ROOT, 'shadytracer_amd', 'csrc', 'device'), '-o', str(tmp_path / 'm.so'), str(src)],
+               stdout=sp.PIPE, stderr=sp.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout[-2000:]
+    M = ctypes.CDLL(str(tmp_path / 'm.so'))
+    M.go.restype = ctypes.c_double
+    assert M.go() == 3.0 * 2.0 + 2.0
+    # a module holds functions and constants only; a function cannot take a library function's name
+    for bad, why in (('let x = 1.0;', 'functions and constants only'), ('fn length(x: f32) -> f32 { return x; }', 'takes a name the library has')):
+        js = "const u = require(%r); try { u.moduleCode(%s) } catch (e) { process.stdout.write(e.message) }" % (
+            os.path.join(ROOT, 'shadytracer_amd', 'js', 'usercode.js'), json.dumps(bad))
+        assert why in sp.run(['node', '-e', js], stdout=sp.PIPE, check=True, text=True).stdout
+
+
+def test_the_scenes_own_sky_is_its_code():
+    from shadytracer_amd import pt
+    coded, native = scenes.emit('sky-code'), scenes.emit('sky-code', native=True)
+    assert coded['scene']['env']['type'] == 5 and native['scene']['env']['type'] == 3
+    # the code's background(dir) is the studio's L0 (1 + top d.z) in every direction (the oracle's FP64 compile)
+    rng = np.random.default_rng(4)
+    d = rng.normal(size=(500, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    uc, un = pt.uniforms_for(coded), pt.uniforms_for(native)
+    Lc = oracle.env(uc, d, scene=coded['scene'])
+    Ln = oracle.env(un, d)
+    assert np.abs(Lc - Ln).max() <= 1e-4 * Ln.max()
+    # ...and the pictures agree: the ball of the functions block's code and the floor under the two skies
+    ac, mc, _ = oracle.render(coded['scene'], uc, 48)
+    an, mn, _ = oracle.render(native['scene'], un, 48, first=5000)
+    Lc, Ln = pt.luminance(ac), pt.luminance(an)
+    s = np.sqrt(pt.standard_error(mc) ** 2 + pt.standard_error(mn) ** 2)
+    ok = s > 1e-9
+    z = (Lc - Ln)[ok] / s[ok]
+    assert 0.75 < z.std() < 1.3 and np.mean(np.abs(z) < 3) > 0.97, z.std()
