@@ -1412,7 +1412,7 @@ static void tracePhoton(const OrScene* S, const uint32_t* seed, uint32_t i, uint
         int state = march(S, &p, dir, n, &ltot, &steps, collide);
         st->photonSegs++;
         if (state != ST_ACTIVE) {             /* escaped, out of steps, or collided in the haze */
-            if (state == ST_SCATTER && lt && hadDelta && sunPh) {
+            if (state == ST_SCATTER && lt && hadDelta) {
                 uint32_t cp; d3 vv; double dist, g;
                 if (camSees(S, p, v(0, 0, 0), 0, &cp, &vv, &dist, &g)) {
                     d3 c = scl(beta, u->haze.albedo * hg(u->haze.g, dotp(dir, vv)) * g);
@@ -1451,7 +1451,7 @@ static void tracePhoton(const OrScene* S, const uint32_t* seed, uint32_t i, uint
         OrLobes L = makeLobes(nrm, neg(dir), m, n0 / n1);
         if (evaluable(m) && hadDelta) {
             if (out) { mapPush(out, p, beta, neg(dir), nrm); st->stored++; }
-            if (lt && sunPh && fabs(n0 - 1) < 1e-3) {
+            if (lt && fabs(n0 - 1) < 1e-3) {
                 uint32_t cp; d3 vv; double dist, g;
                 if (camSees(S, p, nrm, 1, &cp, &vv, &dist, &g) && dotp(nrm, vv) > 0) {
                     OrLobes Lc = makeLobes(nrm, vv, m, n0 / n1);
@@ -1609,7 +1609,7 @@ static void tracePath(const OrScene* S, const OrTiles* T, const uint32_t* seed, 
             m = &white;
         }
         d3 Le = f3(m->emissive);
-        if (max3d(Le) > 0 && caustic && sdelta && m->face >= 0 && faceFlux(&S->faces[m->face]) > 0) break;
+        if (max3d(Le) > 0 && (caustic || ltcls) && sdelta && m->face >= 0 && faceFlux(&S->faces[m->face]) > 0) break;
         if (max3d(Le) > 0) {
             double w = 1;
             if (m->face >= 0) {

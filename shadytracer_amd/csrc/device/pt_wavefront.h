@@ -369,10 +369,10 @@ __device__ __forceinline__ V3 sppmGather(const PtDev* __restrict__ D, const Lobe
 }
 
 // LIGHT TRACING (core/materials_pbr.js:554-700 ltConnect and camFootprint; core/haze.js:404-466 ltConnectMedium): a
-// sun photon that came through a delta event (glass, a mirror) asks, where it first meets a surface or the haze, whether
-// the camera sees that point, and adds its light to the pixel that sees it. The camera's paths leave exactly that light
-// to the photons: a path whose first vertex left it by a non-delta lobe and then met the sun through delta events only
-// does not count the sun. Each path is counted by one side. Off for the thin lens and the illuminance view, and where
+// photon -- from the sun or a light face -- that came through a delta event (glass, a mirror) asks, where it first meets
+// a surface or the haze, whether the camera sees that point, and adds its light to the pixel that sees it. The camera's
+// paths leave exactly that light to the photons: a path whose first vertex left it by a non-delta lobe and then met the
+// sun or a face through delta events only does not count it. Each path is counted by one side. Off for the thin lens and the illuminance view, and where
 // the rays are not one a pixel.
 __device__ __forceinline__ bool ltOn(const PtDev* __restrict__ D) {
     return D->u.lt != 0u && D->u.measureMode != 1u && !(D->u.cam.type == 0u && D->u.cam.lensR > 0.0f) &&
@@ -475,7 +475,7 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
     } else if (state == PT_SCATTER && photon) {
         // a photon ends where it scatters in the haze; a sun photon that came through a delta event sends its light
         // there to the camera through the phase function, the albedo its collision's weight (haze.js ltConnectMedium)
-        if (lt && (flags & (PT_F_HADDELTA | PT_F_SUNPH)) == (PT_F_HADDELTA | PT_F_SUNPH)) {
+        if (lt && (flags & PT_F_HADDELTA)) {
             unsigned int cp; V3 v; float dist, g;
             if (camSees(D, p, v3s(0.0f), false, cp, v, dist, g)) {
                 const V3 c = beta * (D->u.haze.albedo * hgEval(D->u.haze.g, dot(dir, v)) * g);
@@ -570,8 +570,10 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
             const PtMaterial& m = smat[(D->u.measureMode == 1u && bounces == 1u && !photon) ? PT_MAX_MATERIALS : (hw & 0xFFu)];
 #endif
             V3 Le = ld3(m.emissive);
-            if (max3(Le) > 0.0f && (photon || (mapped && m.face >= 0 && faceFlux(D->faces[m.face]) > 0.0f))) {
-                terminate = true;      // a photon ends at a light; a mapped path's light is the map's (its photons')
+            if (max3(Le) > 0.0f && (photon || ((mapped || traced) && m.face >= 0 && faceFlux(D->faces[m.face]) > 0.0f))) {
+                // a photon ends at a light; a mapped path's light is the map's, a traced one's the connections' (the
+                // face's photons')
+                terminate = true;
                 emitted = true;
             } else if (max3(Le) > 0.0f) {
                 // AN EMITTER HIT: weighted against the light sampling the previous vertex ran
@@ -636,7 +638,7 @@ __device__ __forceinline__ ShadeOut shadeOne(const PtDev* __restrict__ D, const 
                     }
                     // ...and a sun photon's light from here to the camera, by the non-delta lobes the camera's side
                     // leaves to it, seen from where the camera is (ltConnect); from air only
-                    if (lt && (flags & PT_F_SUNPH) && fabsf(n0 - 1.0f) < 1e-3f) {
+                    if (lt && fabsf(n0 - 1.0f) < 1e-3f) {
                         unsigned int cp; V3 v; float dist, g;
                         if (camSees(D, p, nrm, true, cp, v, dist, g) && dot(nrm, v) > 0.0f) {
                             const Lobes lc = makeLobes(nrm, v, m, n0 / n1, D->mf);

@@ -747,6 +747,20 @@ def test_light_tracing_in_the_haze_matches_the_oracle(engine):
         assert r[0] < 1, (name, r)
 
 
+def test_light_tracing_from_a_light_face_matches_the_oracle(engine):
+    # the lens and the prism under their panel, light traced: the panel's photons through the glass connect to the
+    # camera, and the camera's paths leave that light to them -- sample by sample against the oracle's (the same seeds)
+    say('case: lens-prism 48x36, lt 1, 2^16 photons, 48 samples vs oracle')
+    made, u = load(engine, 'lens-prism', width=48, height=36, sppmPhotons=1 << 16, lt=1, sppm=0)
+    g = gpu_samples(engine, 48)
+    st = engine.stats()
+    assert st['nonfinite'] == 0 and st['shadow_segments'] > 0
+    o = oracle_samples(made, pt.uniforms_for(made, sppmPhotons=1 << 16, lt=1, sppm=0), 48)
+    r = agree(g, o, np.ones((36, 48), bool), 0.01)
+    say('  picture: engine %.2f, oracle %.2f, se %.2f (%.2f of the tolerance)' % (r[1], r[2], r[3], r[0]))
+    assert r[0] < 1, r
+
+
 @pytest.mark.parametrize('mode', [1, 2])
 def test_c3_caustic_room_matches_the_oracle(engine, mode):
     # sppm 1 gathers at a camera path's first non-delta vertex after a delta event (the reference's); 2 at every

@@ -1406,3 +1406,13 @@ def test_light_tracing_matches_path_tracing_in_the_haze():
     allpx = np.ones((18, 24), bool)
     lt, traced, se = _lt_against_tracing(made, allpx, 1 << 14, 192, 3072, runs=6)
     assert abs(lt - traced) < 4 * se + 0.01 * traced, (lt, traced, se)
+
+
+def test_light_tracing_from_a_light_face_matches_path_tracing():
+    # a light face's photons through glass (the lens and the prism under their panel) connect to the camera too, and
+    # the camera's paths leave that light to them: the picture against plain path tracing's (lt 0, no map), which
+    # converges here (a panel of some size); the whole picture
+    made = scenes.emit('lens-prism', width=24, height=18)
+    allpx = np.ones((18, 24), bool)
+    lt, traced, se = _lt_against_tracing(made, allpx, 1 << 14, 96, 1536, runs=5)
+    assert abs(lt - traced) < 4 * se + 0.01 * traced, (lt, traced, se)
