@@ -97,7 +97,11 @@ __device__ __forceinline__ bool opticsOne(const PtDev* __restrict__ D, const PtO
     const unsigned int id = M.x, counts = M.y;
     unsigned int bounces = mzBounces(M.z), flags = mzFlags(M.z);
     const unsigned int state = mzState(M.z);
-    const float w = luma(beta);
+    float w = luma(beta);
+    // the absorbing medium the ray travels in (material + 1, 0 none), and the optical length at its last vertex
+    const unsigned int medIn = __float_as_uint(A.w);
+    unsigned int medOut = medIn;
+    float vtx = B.w;
     Rng rng = rngAt(O->seed, id, bounces, 0x0971C5ADu ^ ((counts & 0xFFFFu) << 12));
     if (state == PT_ESCAPED) {
         atomicAdd(&sT[PT_OT_ESCAPED], optFix(w));
@@ -139,6 +143,14 @@ __device__ __forceinline__ bool opticsOne(const PtDev* __restrict__ D, const PtO
             st4(D->pos, slot, p.x, p.y, p.z, ltot);
             stu(D->meta, 4u * slot + 2u, mzPack(bounces, PT_ACTIVE, flags));
             return true;
+        }
+        if (medIn != 0u) {
+            // ABSORPTION ALONG THE WAY (tracebasics.js absorbAlong), since the last vertex: what the medium took is its
+            // material's (counted in as incident, never as leaving)
+            beta *= mediumTr(D->mats[medIn - 1u], fmaxf(ltot - vtx, 0.0f) / fmaxf(n, 1e-4f));
+            const float wa = luma(beta);
+            atomicAdd(&sT[PT_OT_MAT + 3u * (medIn - 1u)], optFix(w) - optFix(wa));
+            w = wa;
         }
         if (O->t.maxDist > 0.0f && ltot > O->t.maxDist) {
             // a hit past the longest path traced: stopped at the limit
@@ -202,6 +214,9 @@ __device__ __forceinline__ bool opticsOne(const PtDev* __restrict__ D, const PtO
         if (s.transmitted) {
             n = n1;
             if (draw) beta *= specW;
+            // through the surface: into its material, or out to the one beyond (absorbing ones only)
+            const unsigned int into = inside ? ((hw & PT_HR_BEYOND) ? (hw >> 16) : 0xFFFFFFFFu) : mat;
+            medOut = (into != 0xFFFFFFFFu && D->mats[into].attenuation > 0.0f) ? into + 1u : 0u;
         } else if (draw) {
             lambda = lambda0;
         }
@@ -218,15 +233,16 @@ __device__ __forceinline__ bool opticsOne(const PtDev* __restrict__ D, const PtO
             return false;
         }
         dir = s.dir;
+        vtx = ltot;
         const float off = offsetTol(D, p);
         p = p + dir * off; ltot += off * n;
         collide = (hazeOn(D) && n <= 1.0f) ? ltot + hazeDraw(D->u.haze, p, dir, u.y) * n : 3.0e38f;
     }
     st4(D->pos, slot, p.x, p.y, p.z, ltot);
     st4(D->dir, slot, dir.x, dir.y, dir.z, n);
-    st4(D->beta, slot, beta.x, beta.y, beta.z, -1.0f);
+    st4(D->beta, slot, beta.x, beta.y, beta.z, vtx);
     st4u(D->meta, slot, make_uint4(id, counts, mzPack(bounces, PT_ACTIVE, flags), 0u));
-    st4(D->aux, slot, 1.0f, collide, lambda, 0.0f);
+    st4(D->aux, slot, 1.0f, collide, lambda, __uint_as_float(medOut));
     return true;
 }
 

@@ -647,8 +647,11 @@ function opticsShell(opts = {}) {
 }
 function opticsSlab(opts = {}) {
     const t = opts.thickness !== undefined ? opts.thickness : 0.1
+    // opts.attenuation: the glass absorbs inside, to `colour` (default white) at that depth (metres)
+    const att = opts.attenuation || 0
     const mats = [material('floor', [0, 0, 0], 1, 0, { specular: 0 }),
-        material('glass', [1, 1, 1], 0, 0, { transmission: 1, ior: opts.ior || 1.5, specular: 0.5 })]
+        material('glass', att > 0 ? (opts.colour || [0.5, 0.5, 0.5]) : [1, 1, 1], 0, 0,
+            { transmission: 1, ior: opts.ior || 1.5, specular: 0.5, attenuationDistance: att })]
     const d = doc('optics-slab', mats, [obj('floor', 'b-plane', 'floor', xf(0, 0, 0)),
         obj('slab', 'b-box', 'glass', xf(0, 0, opts.z !== undefined ? opts.z : 0.5), { size: { x: 4, y: 4, z: t / 2 }, rounding: 0 })], { sky: 'none' }, 20)
     return { doc: d, camera: camera([0, -3, 1.5], [0, 0, 0], 60, 32, 32), settings: { width: 32, height: 32, spp: 1, maxbounce: 8, nee: 1, eps: 1e-5 } }

@@ -224,6 +224,26 @@ def test_slab_transmission(engine, deg):
     assert T == pytest.approx(expect, abs=3e-3)
 
 
+def test_absorbing_slab_takes_its_share(engine):
+    # the same slab head-on, its glass absorbing to 0.5 at 50 mm (thickness 100 mm: tau = 0.25 inside): the detector
+    # takes (1 - R)^2 tau / (1 - R^2 tau^2), and the glass what its inside took, the balance whole
+    say('case: optics absorbing slab, 0 deg')
+    made = load(engine, 'optics-slab', attenuation=0.05)
+    src = {'type': 'disc', 'pos': (0, 0, 1), 'axis': (0, 0, -1), 'u': (0, 1, 0), 'size': (0.3, 0.3),
+           'distribution': 'collimated', 'flux': 1000}
+    res = engine.optics_run([src], rays=1 << 20, bounces=64, seed=23, detectors=[mat(made, 'floor')])
+    R, tau = 0.04, 0.25
+    T = res['detected'] / res['emitted']
+    want = (1 - R) ** 2 * tau / (1 - R * R * tau * tau)
+    back = R + (1 - R) ** 2 * R * tau * tau / (1 - R * R * tau * tau)
+    say('  T %.5f (closed form %.5f), escaped %.5f (closed form %.5f)' % (T, want, res['escaped'] / res['emitted'], back))
+    assert T == pytest.approx(want, abs=2e-3)
+    assert res['escaped'] / res['emitted'] == pytest.approx(back, abs=2e-3)
+    took = res['materials'][mat(made, 'glass')]['absorbed']
+    assert took / res['emitted'] == pytest.approx(1 - want - back, abs=3e-3)
+    assert res['emitted'] == pytest.approx(res['escaped'] + res['detected'] + res['absorbed'] + res['lost'], rel=1e-6)
+
+
 def test_glass_ball_interactions(engine):
     # a point source at the centre of a glass ball (emitting into n 1.5): every ray meets the surface head-on, 4 % comes
     # back through the centre to try again -- 1 / (1 - R) interactions a ray, and every ray escapes
