@@ -24,7 +24,7 @@ extern "C" {
 #define PT_ERR_COMPILE 3    /* the scene did not compile; the previous scene is still loaded */
 #define PT_ERR_STATE 4      /* the call is not valid now: trace with no scene, or during a scene build */
 
-#define PT_ABI_VERSION 3
+#define PT_ABI_VERSION 4
 
 typedef struct pt_ctx pt_ctx;
 
@@ -180,14 +180,28 @@ int pt_guide_status(pt_ctx* ctx, double* out, size_t n);
 
 /* ---- the developed picture ---- */
 
-/* The accumulated picture exposed, tone mapped and encoded as 8-bit sRGB RGBA (npx = width x height pixels, rows top
- * first with d->flipY): with d->autoExposure the exposure is metered from the picture (a log2-luminance histogram's
- * trimmed average, highlights protected, eased from the last call; the first call after pt_clear_accum sets it), else
- * it is d->exposureEV. info (NULL allowed) says what was used and metered.
- * Replaces the display pass and the auto-exposure kernels (lib/pathtracer/core/rendertoscreen.js displayPixel
- * :209-260 with its tone curves :35-150; core/autoexposure.js prog_ae_histogram / prog_ae_update :54-153) for a
- * picture that is saved rather than shown (darkroom/export/png.js takes its result). */
+/* The accumulated picture developed to 8-bit RGBA (npx = width x height pixels, rows top first with d->flipY): with
+ * d->autoExposure the exposure is metered from the picture (a log2-luminance histogram's trimmed average, highlights
+ * protected, eased from the last call; the first call after pt_clear_accum sets it), else it is d->exposureEV. Then the
+ * darkroom's stages that d->lightOn names (white balance and gains, the tone equaliser, Oklab saturation, vibrance and
+ * hue, the vignette), the gamut clip, the tone curve, the output transfer, Levels, the curves (d->curveLut), the colour
+ * wheels as an ASC CDL, the display's contrast, saturation and brightness, grain and dither; or, with d->measureMode,
+ * the false-colour view of the illuminance or luminance. lightOn 0 is the display's own picture. info (NULL allowed)
+ * says what was used and metered.
+ * Replaces the develop pass and the auto-exposure kernels (lib/pathtracer/core/rendertoscreen.js displayPixel
+ * :209-270, developPixel :423-455 with its stages :286-421 and tone curves :35-150; core/autoexposure.js
+ * prog_ae_histogram / prog_ae_update :54-153; the settings from darkroom/grade.js lightUniforms :231-271 and
+ * darkroom/curves.js bakeLut :75-87) for a picture that is saved rather than shown (darkroom/export/png.js takes its
+ * result). */
 int pt_develop(pt_ctx* ctx, const PtDevelop* d, size_t dsize, uint8_t* rgba, size_t npx, PtDevelopInfo* info);
+
+/* What pt_develop would write, counted: every `step`-th pixel each way developed as pt_develop does it (metering
+ * included), hist[0..255] red, [256..511] green, [512..767] blue, [768..1023] the Rec.709 luma of the display values,
+ * bin round(v 255); nbins = 4 x PT_HIST_BINS. A false-colour view (d->measureMode) is refused.
+ * Replaces the darkroom's histogram pass (lib/pathtracer/core/rendertoscreen.js develop_histogram_prog :597-643;
+ * develop.js histogramStep :97). */
+int pt_develop_histogram(pt_ctx* ctx, const PtDevelop* d, size_t dsize, uint32_t step, uint32_t* hist, size_t nbins,
+                         PtDevelopInfo* info);
 
 /* ---- the optics trace over the loaded scene ---- */
 

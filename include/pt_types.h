@@ -286,17 +286,49 @@ typedef struct PtOpticsTally {
 #define PT_OT_MAT 16           // then per material: incident, leaving, hits
 #define PT_OT_WORDS (PT_OT_MAT + 3 * PT_MAX_MATERIALS)
 
-// ---- The developed picture (core/rendertoscreen.js displayPixel, core/autoexposure.js): exposure, tone curve, 8-bit sRGB.
+// ---- The developed picture (core/rendertoscreen.js displayPixel and developPixel, core/autoexposure.js,
+// darkroom/grade.js lightUniforms, darkroom/curves.js bakeLut): exposure, the darkroom's stages, tone curve, 8-bit.
+// lightOn's bits (grade.js LIGHT_*): which darkroom stage runs; a stage off is never computed, so with lightOn 0 the
+// picture is the display's own.
+#define PT_LIGHT_ANY 1u
+#define PT_LIGHT_WB 2u
+#define PT_LIGHT_EQ 4u
+#define PT_LIGHT_LAB 8u
+#define PT_LIGHT_LEVELS 16u
+#define PT_LIGHT_CURVES 32u
+#define PT_LIGHT_CDL 64u
+#define PT_LIGHT_VIGNETTE 128u
+#define PT_LIGHT_GRAIN 256u
+#define PT_LIGHT_DITHER 512u
+#define PT_LIGHT_CURVE_LUMA 1024u
+#define PT_LIGHT_ALL 2047u
+#define PT_CURVE_LUT 1024        // entries per curve of curveLut: master, red, green, blue
+#define PT_HIST_BINS 256         // a developed picture's histogram: R, G, B and luma, this many bins each
 typedef struct PtDevelop {
     unsigned int toneMap;        // 0 linear (clipped at white), 1 PBR Neutral, 2 AgX, 3 AgX Punchy, 4 ACES (fitted),
-                                 // 5 Reinhard (luminance), 6 Hable (the reference's toneMap numbering)
+                                 // 5 Reinhard (luminance), 6 Hable, 7 Hable with tone7's white, toe and shoulder
+                                 // (the reference's toneMap numbering)
     unsigned int autoExposure;   // 1: metered from the picture (the exposure eases by aeSpeed a call); 0: exposureEV
     float exposureEV;            // EV100 when not metered: white is 1.2 x 2^EV cd/m^2
     float aeSpeed;               // share of the gap to the metered EV closed a call (1: jump; the first call jumps)
     float aeHeadroom;            // stops the 95 % quantile may sit above white before the meter darkens the picture
-    unsigned int dither;         // 1: +-0.5 of an 8-bit step of per-pixel noise before rounding
+    unsigned int lightOn;        // PT_LIGHT_* bits; 0: the display's picture, bit for bit
     unsigned int flipY;          // 1: rows top first (images, PNG); 0: as accumulated, bottom first
-    unsigned int pad;
+    unsigned int transfer;       // the output transfer: 0 sRGB, 1 gamma 2.2, 2 gamma 2.4, 3 linear
+    float contrast, saturation, brightness;  // the display's grading on the encoded picture (0 each: none)
+    unsigned int measureMode;    // 0 the picture; false colour of 1 the illuminance (lux), 2 the luminance (cd/m^2)
+    float measureMin, measureMax;            // the false colour's log scale, its ends
+    unsigned int measureBands;   // 0 continuous, else that many flat bands
+    float wb[9];                 // white balance and gains: rows of the 3 x 3 applied to the exposed light
+    float eq[8];                 // the tone equaliser's gains in stops at -7 .. 0 EV below white
+    float lab[3];                // Oklab: saturation, vibrance, hue (radians)
+    float tone7[3];              // curve 7: its white (linear), toe, shoulder
+    float levels[5];             // Levels: input black, input white, 1 / gamma, output black, output white
+    float cdlSlope[3], cdlOffset[3], cdlPower[3];  // the colour wheels as an ASC CDL
+    float vignette[3];           // stops at the corners, where it starts (0..1 of the way out), round (0..1)
+    float grain[2];              // amount, cell size in picture pixels (>= 1)
+    float pad[7];
+    float curveLut[4 * PT_CURVE_LUT];        // the curves baked: master, red, green, blue on the encoded value
 } PtDevelop;
 // What developing measured: the exposure used, the meter's trimmed log2 average and protected quantile, pixels metered.
 typedef struct PtDevelopInfo {

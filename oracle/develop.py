@@ -7,6 +7,14 @@ exposure: white at 1.2 2^EV cd/m^2, ISO 12232; the meter log2(L 100 / 12.5), ISO
 (the tone curves :35-150 and the sRGB encoding of displayPixel :209-260), from the published definitions each curve
 names: Khronos PBR Neutral, AgX in Wrensch's polynomial approximation (with Blender's Punchy look), Hill's fitted ACES,
 Reinhard on luminance, Hable's Uncharted 2 curve, IEC 61966-2-1.
+
+The darkroom (core/rendertoscreen.js developPixel :423-455 and its stages :286-421; darkroom/grade.js lightStages
+:279-290, finishEarly :296-310, finishLate :311-320, vignetteStops :160-168, grainNoise :218-226, ditherAt :230-233;
+core/colorimetry.js gamutClip :110-117; the false-colour views of displayPixel :224-241) restated from the
+definitions they name: Oklab (Ottosson 2020), Hable's curve with its toe and shoulder given, Levels, the ASC CDL,
+PCG's output permutation (Jarzynski and Olano 2020), Google's Turbo in its published polynomial fit. `grade` is the
+dict of PtDevelop's darkroom words the host makes (lightOn, wb rows, eq, lab, tone7, transfer, levels, cdl*,
+vignette, grain, curveLut, contrast, saturation, brightness, measure*).
 """
 import numpy as np
 
@@ -117,12 +125,222 @@ def srgb(v):
     return np.where(v <= 0.0031308, 12.92 * v, 1.055 * v ** (1 / 2.4) - 0.055)
 
 
-def develop(acc, tone=1, ev=None, headroom=2.0):
-    """acc [h, w, 4] FP64 accumulation (row 0 at the bottom): (8-bit RGB [h, w, 3] top row first, before rounding as
-    floats too, the EV used). ev None: metered from the picture."""
+LIGHT = dict(ANY=1, WB=2, EQ=4, LAB=8, LEVELS=16, CURVES=32, CDL=64, VIGNETTE=128, GRAIN=256, DITHER=512, CURVE_LUMA=1024)
+GRADE_DEFAULTS = dict(lightOn=0, wb=[1, 0, 0, 0, 1, 0, 0, 0, 1], eq=[0] * 8, lab=[0, 0, 0], tone7=[5.6, 0.2, 0.15], transfer=0,
+                      levels=[0, 1, 1, 0, 1], cdlSlope=[1, 1, 1], cdlOffset=[0, 0, 0], cdlPower=[1, 1, 1], vignette=[0, 0.5, 0],
+                      grain=[0, 1.5], curveLut=None, contrast=0.0, saturation=0.0, brightness=0.0, measureMode=0,
+                      measureMin=1.0, measureMax=100000.0, measureBands=0)
+
+
+def gamut_clip(c):
+    """A negative channel desaturated to 0 towards the grey of the colour's own luminance; no luminance: black."""
+    c = np.array(c, float)
+    lo = c.min(axis=-1, keepdims=True)
+    Y = (c @ LUMA)[..., None]
+    t = Y / np.where(Y - lo > 0, Y - lo, 1.0)
+    clipped = np.where(Y > 0, np.maximum(Y + (c - Y) * t, 0.0), 0.0)
+    return np.where(lo >= 0, c, clipped)
+
+
+_OK_M1 = np.array([[0.4122214708, 0.5363325363, 0.0514459929], [0.2119034982, 0.6806995451, 0.1073969566],
+                   [0.0883024619, 0.2817188376, 0.6299787005]])
+_OK_M2 = np.array([[0.2104542553, 0.7936177850, -0.0040720468], [1.9779984951, -2.4285922050, 0.4505937099],
+                   [0.0259040371, 0.7827717662, -0.8086757660]])
+
+
+def oklab(c):
+    """Ottosson's Oklab of linear sRGB, the cube root keeping its sign."""
+    return np.cbrt(np.array(c, float) @ _OK_M1.T) @ _OK_M2.T
+
+
+def from_oklab(q):
+    """Oklab's inverse, from the inverses of its two matrices (the published 10-digit inverses agree to 1e-9)."""
+    return (np.array(q, float) @ np.linalg.inv(_OK_M2).T) ** 3 @ np.linalg.inv(_OK_M1).T
+
+
+def eq_gain(bands, x):
+    """The equaliser's gain in stops at x stops below white: the 8 bands at -7..0, interpolated, held at the ends."""
+    t = np.clip(np.asarray(x, float) + 7.0, 0.0, 7.0)
+    k = np.minimum(np.floor(t), 6).astype(int)
+    b = np.asarray(bands, float)
+    return b[k] + (b[k + 1] - b[k]) * (t - k)
+
+
+def lab_adjust(c, sat, vib, hue):
+    q = oklab(c)
+    rel = np.hypot(q[..., 1], q[..., 2]) / np.maximum(np.abs(q[..., 0]), 1e-6)
+    k = (1 + sat) * (1 + vib * (1 - np.minimum(rel / 0.25, 1.0)))
+    cs, sn = np.cos(hue), np.sin(hue)
+    a = (q[..., 1] * cs - q[..., 2] * sn) * k
+    b = (q[..., 1] * sn + q[..., 2] * cs) * k
+    return from_oklab(np.stack([q[..., 0], a, b], axis=-1))
+
+
+def hable_p(x, sh, to):
+    """Hable's rational curve with shoulder strength sh and toe strength to (linear 0.50, angle 0.10, toe 0.02/0.30)."""
+    return (x * (sh * x + 0.10 * 0.50) + to * 0.02) / (x * (sh * x + 0.50) + to * 0.30) - 0.02 / 0.30
+
+
+def tone7(c, white, toe, shoulder):
+    return hable_p(2.0 * np.asarray(c, float), shoulder, toe) / hable_p(2.0 * white, shoulder, toe)
+
+
+def vignette_stops(px, py, w, h, amount, mid, rnd):
+    qx, qy = ((px + 0.5) / w - 0.5) * 2, ((py + 0.5) / h - 0.5) * 2
+    a = w / h
+    r0 = np.hypot(qx, qy) / np.sqrt(2.0)
+    r1 = np.hypot(qx * a, qy) / np.hypot(a, 1.0)
+    r = r0 + (r1 - r0) * rnd
+    t = np.clip((r - mid) / max(1e-6, 1 - mid), 0.0, 1.0)
+    return amount * t * t * (3 - 2 * t)
+
+
+def encode_out(x, transfer):
+    c = np.clip(x, 0.0, 1.0)
+    if transfer == 1:
+        return c ** (1 / 2.2)
+    if transfer == 2:
+        return c ** (1 / 2.4)
+    if transfer == 3:
+        return c
+    return srgb(c)
+
+
+def levels(x, in_black, in_white, inv_gamma, out_black, out_white):
+    t = np.clip((x - in_black) / max(1e-6, in_white - in_black), 0.0, 1.0)
+    return out_black + (out_white - out_black) * t ** inv_gamma
+
+
+def lut_at(lut, ch, x):
+    n = len(lut) // 4
+    t = np.clip(x, 0.0, 1.0) * (n - 1)
+    k = np.minimum(np.floor(t), n - 2).astype(int)
+    a, b = lut[ch * n + k], lut[ch * n + k + 1]
+    return a + (b - a) * (t - k)
+
+
+def cdl(x, slope, offset, power):
+    return np.maximum(x * slope + offset, 0.0) ** power
+
+
+def pcg_hash(n):
+    """PCG's output permutation on 32 bits (uint64 arrays keep the products exact before the mask)."""
+    M = np.uint64(0xFFFFFFFF)
+    st = (np.asarray(n, np.uint64) * np.uint64(747796405) + np.uint64(2891336453)) & M
+    w = (((st >> ((st >> np.uint64(28)) + np.uint64(4))) ^ st) * np.uint64(277803737)) & M
+    return (w >> np.uint64(22)) ^ w
+
+
+def unit24(h):
+    return (np.asarray(h, np.uint64) >> np.uint64(8)).astype(float) / 16777216.0
+
+
+def grain_noise(px, py, size):
+    gx, gy = (px + 0.5) / size, (py + 0.5) / size
+    ix, iy = np.floor(gx), np.floor(gy)
+    fx, fy = gx - ix, gy - iy
+    fx, fy = fx * fx * (3 - 2 * fx), fy * fy * (3 - 2 * fy)
+
+    def cell(i, j):
+        key = (np.asarray(i + j * 65536, np.int64) & 0xFFFFFFFF).astype(np.uint64) ^ np.uint64(0x632BE5AB)
+        return unit24(pcg_hash(key)) * 2 - 1
+    a, b, c, d = cell(ix, iy), cell(ix + 1, iy), cell(ix, iy + 1), cell(ix + 1, iy + 1)
+    top = a + (b - a) * fx
+    return top + ((c + (d - c) * fx) - top) * fy
+
+
+def dither_at(px, py):
+    key = (np.asarray(px + py * 65536, np.int64) & 0xFFFFFFFF).astype(np.uint64) ^ np.uint64(0x1B873593)
+    h1 = pcg_hash(key)
+    h2 = pcg_hash(h1)
+    return (unit24(h1) + unit24(h2) - 1) / 255
+
+
+_TURBO = np.array([[0.13572138, 4.61539260, -42.66032258, 132.13108234, -152.94239396, 59.28637943],
+                   [0.09140261, 2.19418839, 4.84296658, -14.18503333, 4.27729857, 2.82956604],
+                   [0.10667330, 12.64194608, -60.58204836, 110.36276771, -89.90310912, 27.34824973]])
+
+
+def turbo(t):
+    x = np.clip(np.asarray(t, float), 0.0, 1.0)
+    return np.stack([x ** k for k in range(6)], axis=-1) @ _TURBO.T
+
+
+def false_colour(light, u, v, g):
+    Y = np.maximum(light @ LUMA, 0.0)
+    val = np.pi * Y if g['measureMode'] == 1 else Y
+    lo = np.log2(max(g['measureMin'], 1e-6))
+    hi = max(np.log2(max(g['measureMax'], 1e-6)), lo + 1e-3)
+    strip = v < 0.025
+    t = np.where(strip, u, (np.log2(np.maximum(val, 1e-30)) - lo) / (hi - lo))
+    if g['measureBands'] > 0:
+        t = (np.floor(np.clip(t, 0.0, 0.99999) * g['measureBands']) + 0.5) / g['measureBands']
+    shown = np.where(((val <= 0) & ~strip)[..., None], 0.0, turbo(t))
+    return np.clip(shown, 0.0, 1.0)
+
+
+def develop_pixels(light, scale, px, py, W, H, tone, grade=None):
+    """The reference's developPixel on light [..., 3] (cd/m^2) at picture pixels (px, py), row 0 at the bottom: display
+    values in [0, 1]."""
+    g = dict(GRADE_DEFAULTS, **(grade or {}))
+    if g['measureMode']:
+        return false_colour(np.asarray(light, float), (px + 0.5) / W, (py + 0.5) / H, g)
+    on = int(g['lightOn'])
+    c = np.asarray(light, float) * scale
+    if on & LIGHT['WB']:
+        c = c @ np.asarray(g['wb'], float).reshape(3, 3).T
+    if on & LIGHT['EQ']:
+        c = c * 2.0 ** eq_gain(g['eq'], np.log2(np.maximum(c @ LUMA, 1e-30)))[..., None]
+    if on & LIGHT['LAB']:
+        c = lab_adjust(c, *g['lab'])
+    if on & LIGHT['VIGNETTE']:
+        c = c * 2.0 ** vignette_stops(px, py, W, H, *g['vignette'])[..., None]
+    c = gamut_clip(c)
+    c = tone7(c, *g['tone7']) if tone == 7 else TONES[tone](c)
+    c = encode_out(np.clip(c, 0.0, 1.0), g['transfer'])
+    if on & LIGHT['LEVELS']:
+        c = levels(c, *g['levels'])
+    if on & LIGHT['CURVES']:
+        lut = np.asarray(g['curveLut'], float)
+        if on & LIGHT['CURVE_LUMA']:
+            y = c @ LUMA
+            y2 = lut_at(lut, 0, y)
+            c = np.where((y > 1e-6)[..., None], c * (y2 / np.where(y > 1e-6, y, 1.0))[..., None], y2[..., None])
+        else:
+            c = lut_at(lut, 0, c)
+        c = np.stack([lut_at(lut, k + 1, c[..., k]) for k in range(3)], axis=-1)
+    if on & LIGHT['CDL']:
+        c = cdl(c, np.asarray(g['cdlSlope']), np.asarray(g['cdlOffset']), np.asarray(g['cdlPower']))
+    c = 0.5 + (c - 0.5) * (g['contrast'] + 1)
+    y = (c @ LUMA)[..., None]
+    c = y + (c - y) * (g['saturation'] + 1)
+    c = c + g['brightness']
+    if on & LIGHT['GRAIN']:
+        y = np.clip(c @ LUMA, 0.0, 1.0)
+        c = c + (grain_noise(px, py, g['grain'][1]) * g['grain'][0] * 0.08 * (0.3 + 2.8 * y * (1 - y)))[..., None]
+    if on & LIGHT['DITHER']:
+        c = c + dither_at(px, py)[..., None]
+    return np.clip(c, 0.0, 1.0)
+
+
+def develop(acc, tone=1, ev=None, headroom=2.0, grade=None):
+    """acc [h, w, 4] FP64 accumulation (row 0 at the bottom): (8-bit RGB [h, w, 3] top row first, the display values
+    times 255 before rounding, top row first, the EV used). ev None: metered from the picture."""
     if ev is None:
         ev = ae_update(ae_histogram(acc), headroom)[0]
-    c = np.maximum(acc[..., :3], 0.0) / (EV_CLIP * 2.0 ** ev)
-    v = srgb(TONES[tone](c)) * 255.0
-    v = np.where((acc[..., 3] > 0)[..., None], v, 0.0)
+    H, W = acc.shape[:2]
+    py, px = np.mgrid[0:H, 0:W].astype(float)
+    v = develop_pixels(acc[..., :3], 1.0 / (EV_CLIP * 2.0 ** ev), px, py, W, H, tone, grade) * 255.0
     return np.floor(np.minimum(v + 0.5, 255.0))[::-1].astype(np.uint8), v[::-1], ev
+
+
+def histogram(acc, tone=1, ev=None, headroom=2.0, grade=None, step=1):
+    """What develop() writes, counted over every `step`-th pixel each way: [4, 256] (R, G, B, luma), bin round(v 255)."""
+    if ev is None:
+        ev = ae_update(ae_histogram(acc), headroom)[0]
+    sub = acc[::step, ::step]
+    H, W = acc.shape[:2]
+    py, px = np.mgrid[0:H:step, 0:W:step].astype(float)
+    c = develop_pixels(sub[..., :3], 1.0 / (EV_CLIP * 2.0 ** ev), px, py, W, H, tone, grade)
+    chans = [c[..., 0], c[..., 1], c[..., 2], np.clip(c @ LUMA, 0.0, 1.0)]
+    return np.stack([np.bincount((np.clip(x, 0, 1) * 255 + 0.5).astype(int).ravel(), minlength=256) for x in chans])

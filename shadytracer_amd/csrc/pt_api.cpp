@@ -28,10 +28,10 @@ const char* kKernelNames[] = {"pt_genrays", "pt_march", "pt_shade", "pt_occlude"
                               "pt_bin_count", "pt_bin_scan", "pt_bin_scatter", "pt_sppm_clear", "pt_photon_gen", "pt_surface",
                               "pt_guide_infer", "pt_guide_eval", "pt_guide_collect", "pt_guide_grad", "pt_guide_adam", "pt_guide_commit",
                               "pt_flush", "pt_opt_gen", "pt_opt_shade", "pt_opt_flush", "pt_ae_hist", "pt_ae_update",
-                              "pt_develop"};
+                              "pt_develop", "pt_develop_hist"};
 enum { K_GENRAYS, K_MARCH, K_SHADE, K_OCCLUDE, K_RESOLVE, K_MFLUT, K_MFAVG, K_PICK, K_BINCOUNT, K_BINSCAN, K_BINSCATTER,
        K_SPPMCLEAR, K_PHOTONGEN, K_SURFACE, K_GINFER, K_GEVAL, K_GCOLLECT, K_GGRAD, K_GADAM, K_GCOMMIT, K_FLUSH, K_OPTGEN, K_OPTSHADE,
-       K_OPTFLUSH, K_AEHIST, K_AEUPDATE, K_DEVELOP, K_COUNT };
+       K_OPTFLUSH, K_AEHIST, K_AEUPDATE, K_DEVELOP, K_DEVHIST, K_COUNT };
 
 // the guide network's sizes (device/pt_guide.h)
 const unsigned int kGuideNP = 16169, kGuideWT = 17408, kGuideBias = 256, kGuideParts = 256, kGuideMaxSteps = 64;
@@ -1191,28 +1191,37 @@ int pt_optics_run(pt_ctx* c, const PtOpticsSource* src, uint32_t nsrc, const PtO
     return PT_OK;
 }
 
-int pt_develop(pt_ctx* c, const PtDevelop* d, size_t dsize, uint8_t* rgba, size_t npx, PtDevelopInfo* info) {
-    if (!c) return fail(c, PT_ERR_ARG, "ctx is NULL");
-    if (!d || !rgba) return fail(c, PT_ERR_ARG, "settings or rgba is NULL");
+// The settings checked, the buffers made, the settings on the device and, with autoExposure, the picture metered.
+static int developBegin(pt_ctx* c, const PtDevelop* d, size_t dsize) {
+    if (!d) return fail(c, PT_ERR_ARG, "settings is NULL");
     if (dsize != sizeof(PtDevelop)) return fail(c, PT_ERR_ARG, "develop: %zu bytes, expected %zu", dsize, sizeof(PtDevelop));
     if (c->building) return fail(c, PT_ERR_STATE, "develop during a scene build");
     if (!c->module) return fail(c, PT_ERR_STATE, "no scene loaded");
-    const size_t px = size_t(c->host.u.width) * c->host.u.height;
-    if (npx != px) return fail(c, PT_ERR_ARG, "npx %zu, the picture has %zu pixels", npx, px);
-    if (d->toneMap > 6u) return fail(c, PT_ERR_ARG, "toneMap %u not in 0..6", d->toneMap);
-    if (!(std::isfinite(d->exposureEV) && std::isfinite(d->aeSpeed) && std::isfinite(d->aeHeadroom)))
-        return fail(c, PT_ERR_ARG, "exposureEV, aeSpeed and aeHeadroom must be finite");
+    if (d->toneMap > 7u) return fail(c, PT_ERR_ARG, "toneMap %u not in 0..7", d->toneMap);
+    if (d->transfer > 3u) return fail(c, PT_ERR_ARG, "transfer %u not in 0..3", d->transfer);
+    if (d->measureMode > 2u) return fail(c, PT_ERR_ARG, "measureMode %u not in 0..2", d->measureMode);
+    if (d->lightOn & ~PT_LIGHT_ALL) return fail(c, PT_ERR_ARG, "lightOn 0x%x has bits beyond 0x%x", d->lightOn, PT_LIGHT_ALL);
+    const float scalars[] = {d->exposureEV, d->aeSpeed, d->aeHeadroom, d->contrast, d->saturation, d->brightness, d->measureMin,
+                             d->measureMax};
+    for (float v : scalars)
+        if (!std::isfinite(v)) return fail(c, PT_ERR_ARG, "exposureEV, aeSpeed, aeHeadroom, the grading and the measure scale must be finite");
+    for (const float* v = d->wb; v < d->pad; ++v)
+        if (!std::isfinite(*v)) return fail(c, PT_ERR_ARG, "a darkroom setting (word %d) is not finite", int(v - d->wb));
+    if (d->lightOn & PT_LIGHT_CURVES)
+        for (int k = 0; k < 4 * PT_CURVE_LUT; ++k)
+            if (!std::isfinite(d->curveLut[k])) return fail(c, PT_ERR_ARG, "curveLut[%d] is not finite", k);
+    if (d->measureMode && !(d->measureMin > 0.0f && d->measureMax > 0.0f))
+        return fail(c, PT_ERR_ARG, "the false colour's scale needs measureMin, measureMax > 0");
+    if ((d->lightOn & PT_LIGHT_GRAIN) && !(d->grain[1] >= 1.0f)) return fail(c, PT_ERR_ARG, "grain size %g below 1 pixel", d->grain[1]);
+    if ((d->lightOn & PT_LIGHT_LEVELS) && !(d->levels[2] > 0.0f)) return fail(c, PT_ERR_ARG, "levels 1/gamma %g not > 0", d->levels[2]);
+    if (d->toneMap == 7u && !(d->tone7[0] > 0.0f && d->tone7[1] >= 0.0f && d->tone7[2] >= 0.0f))
+        return fail(c, PT_ERR_ARG, "curve 7 needs white > 0, toe and shoulder >= 0");
     int rc;
     if (!c->aeHist) {
         if ((rc = allocBuf(c, c->aeHist, 128 * 4)) || (rc = allocBuf(c, c->aeState, 8 * 4)) || (rc = allocBuf(c, c->devSet, sizeof(PtDevelop))))
             return rc;
         HIPCHK(c, hipMemsetAsync(c->aeHist, 0, 128 * 4, c->stream));
         HIPCHK(c, hipMemsetAsync(c->aeState, 0, 8 * 4, c->stream));
-    }
-    if (px > c->devRgbaPx) {
-        void* p = c->devRgba; freeBuf(c, p); c->devRgba = nullptr;
-        if ((rc = allocBuf(c, c->devRgba, px * 4))) return rc;
-        c->devRgbaPx = px;
     }
     if ((rc = syncDev(c))) return rc;
     HIPCHK(c, hipMemcpyAsync(c->devSet, d, sizeof(PtDevelop), hipMemcpyHostToDevice, c->stream));
@@ -1222,19 +1231,60 @@ int pt_develop(pt_ctx* c, const PtDevelop* d, size_t dsize, uint8_t* rgba, size_
         void* au[] = {&c->aeHist, &c->aeState, &c->devSet};
         HIPCHK(c, hipModuleLaunchKernel(c->fn[K_AEUPDATE], 1, 1, 1, 64, 1, 1, 0, c->stream, au, nullptr));
     }
+    return PT_OK;
+}
+
+static int developInfo(pt_ctx* c, const PtDevelop* d, PtDevelopInfo* info) {
+    if (!info) return PT_OK;
+    float st[8];
+    HIPCHK(c, hipMemcpy(st, c->aeState, sizeof st, hipMemcpyDeviceToHost));
+    info->ev = d->autoExposure ? st[0] : d->exposureEV;
+    info->log2Average = st[2];
+    info->log2High = st[3];
+    info->metered = d->autoExposure ? st[4] : 0.0f;
+    return PT_OK;
+}
+
+int pt_develop(pt_ctx* c, const PtDevelop* d, size_t dsize, uint8_t* rgba, size_t npx, PtDevelopInfo* info) {
+    if (!c) return fail(c, PT_ERR_ARG, "ctx is NULL");
+    if (!rgba) return fail(c, PT_ERR_ARG, "rgba is NULL");
+    const size_t px = size_t(c->host.u.width) * c->host.u.height;
+    if (c->module && npx != px) return fail(c, PT_ERR_ARG, "npx %zu, the picture has %zu pixels", npx, px);
+    int rc;
+    if ((rc = developBegin(c, d, dsize))) return rc;
+    if (px > c->devRgbaPx) {
+        void* p = c->devRgba; freeBuf(c, p); c->devRgba = nullptr;
+        if ((rc = allocBuf(c, c->devRgba, px * 4))) return rc;
+        c->devRgbaPx = px;
+    }
     void* dv[] = {&c->dev, &c->devRgba, &c->aeState, &c->devSet};
     HIPCHK(c, hipModuleLaunchKernel(c->fn[K_DEVELOP], (unsigned int)((px + 255) / 256), 1, 1, 256, 1, 1, 0, c->stream, dv, nullptr));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(rgba, c->devRgba, px * 4, hipMemcpyDeviceToHost));
-    if (info) {
-        float st[8];
-        HIPCHK(c, hipMemcpy(st, c->aeState, sizeof st, hipMemcpyDeviceToHost));
-        info->ev = d->autoExposure ? st[0] : d->exposureEV;
-        info->log2Average = st[2];
-        info->log2High = st[3];
-        info->metered = d->autoExposure ? st[4] : 0.0f;
-    }
-    return PT_OK;
+    return developInfo(c, d, info);
+}
+
+int pt_develop_histogram(pt_ctx* c, const PtDevelop* d, size_t dsize, uint32_t step, uint32_t* hist, size_t nbins, PtDevelopInfo* info) {
+    if (!c) return fail(c, PT_ERR_ARG, "ctx is NULL");
+    if (!hist) return fail(c, PT_ERR_ARG, "hist is NULL");
+    if (nbins != 4 * PT_HIST_BINS) return fail(c, PT_ERR_ARG, "nbins %zu, the histogram has %d", nbins, 4 * PT_HIST_BINS);
+    if (step == 0) return fail(c, PT_ERR_ARG, "step 0");
+    if (d && d->measureMode) return fail(c, PT_ERR_ARG, "a false-colour view is a measurement: no histogram of it");
+    int rc;
+    if ((rc = developBegin(c, d, dsize))) return rc;
+    uint32_t* dh = nullptr;
+    if ((rc = allocBuf(c, dh, 4 * PT_HIST_BINS * 4))) return rc;
+    HIPCHK(c, hipMemsetAsync(dh, 0, 4 * PT_HIST_BINS * 4, c->stream));
+    const size_t W = c->host.u.width, H = c->host.u.height;
+    const size_t n = ((W + step - 1) / step) * ((H + step - 1) / step);
+    const unsigned int blocks = (unsigned int)std::min<size_t>((n + 255) / 256, size_t(c->cus) * 4u);
+    void* hv[] = {&c->dev, &dh, &c->aeState, &c->devSet, &step};
+    HIPCHK(c, hipModuleLaunchKernel(c->fn[K_DEVHIST], std::max(blocks, 1u), 1, 1, 256, 1, 1, 0, c->stream, hv, nullptr));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(hist, dh, 4 * PT_HIST_BINS * 4, hipMemcpyDeviceToHost));
+    void* p = dh;
+    freeBuf(c, p);
+    return developInfo(c, d, info);
 }
 
 int pt_pick_rays(pt_ctx* c, float* pos4, uint32_t* info, size_t n) {

@@ -454,6 +454,16 @@ napi_value OpticsTrace(napi_env env, napi_callback_info info) {
     return arr;
 }
 
+napi_value developInfo(napi_env env, const PtDevelopInfo& di) {
+    napi_value arr;
+    NAPI_OK(env, napi_create_array_with_length(env, 4, &arr));
+    napi_set_element(env, arr, 0, num(env, di.ev));
+    napi_set_element(env, arr, 1, num(env, di.log2Average));
+    napi_set_element(env, arr, 2, num(env, di.log2High));
+    napi_set_element(env, arr, 3, num(env, di.metered));
+    return arr;
+}
+
 // develop(h, settings (PtDevelop bytes), rgba (Uint8Array of w x h x 4)) -> [ev, log2Average, log2High, metered]
 napi_value Develop(napi_env env, napi_callback_info info) {
     napi_value a[3];
@@ -467,13 +477,28 @@ napi_value Develop(napi_env env, napi_callback_info info) {
     PtDevelopInfo di;
     int rc = pt_develop(ctx, static_cast<const PtDevelop*>(sp), sl, static_cast<uint8_t*>(op), ol / 4, &di);
     if (rc) return throwPt(env, ctx, rc);
-    napi_value arr;
-    NAPI_OK(env, napi_create_array_with_length(env, 4, &arr));
-    napi_set_element(env, arr, 0, num(env, di.ev));
-    napi_set_element(env, arr, 1, num(env, di.log2Average));
-    napi_set_element(env, arr, 2, num(env, di.log2High));
-    napi_set_element(env, arr, 3, num(env, di.metered));
-    return arr;
+    return developInfo(env, di);
+}
+
+// developHistogram(h, settings (PtDevelop bytes), step, hist (Uint32Array of 4 x 256)) -> [ev, log2Average, log2High, metered]
+napi_value DevelopHistogram(napi_env env, napi_callback_info info) {
+    napi_value a[4];
+    if (!args(env, info, 4, a)) return nullptr;
+    pt_ctx* ctx = ctxOf(env, a[0]);
+    if (!ctx) return nullptr;
+    void *sp = nullptr, *hp = nullptr;
+    size_t sl = 0, hl = 0;
+    uint32_t step = 1;
+    if (!bytesOf(env, a[1], &sp, &sl) || !bytesOf(env, a[3], &hp, &hl)) return nullptr;
+    if (napi_get_value_uint32(env, a[2], &step) != napi_ok) { napi_throw_type_error(env, nullptr, "step: a number"); return nullptr; }
+    if (sl != sizeof(PtDevelop) || hl != 4 * PT_HIST_BINS * 4) {
+        napi_throw_range_error(env, nullptr, "settings not a PtDevelop, or hist not 1024 32-bit bins");
+        return nullptr;
+    }
+    PtDevelopInfo di;
+    int rc = pt_develop_histogram(ctx, static_cast<const PtDevelop*>(sp), sl, step, static_cast<uint32_t*>(hp), hl / 4, &di);
+    if (rc) return throwPt(env, ctx, rc);
+    return developInfo(env, di);
 }
 
 // opticsRun(h, sources, nsrc, trace, nfar, hitCap) -> Promise of { tally: ArrayBuffer (PtOpticsTally), far: Float64Array
@@ -582,6 +607,7 @@ napi_value Init(napi_env env, napi_value exports) {
         {"trace", nullptr, Trace, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"opticsRun", nullptr, OpticsRun, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"develop", nullptr, Develop, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"developHistogram", nullptr, DevelopHistogram, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"readAccum", nullptr, ReadAccum, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"readStats", nullptr, ReadStats, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"opticsTrace", nullptr, OpticsTrace, nullptr, nullptr, nullptr, napi_enumerable, nullptr},

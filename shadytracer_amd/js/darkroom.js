@@ -1,0 +1,239 @@
+'use strict'
+// THE DARKROOM'S SETTINGS AS THE DEVELOP KERNEL READS THEM: a develop settings object (the reference's
+// darkroom/grade.js LIGHT_DEFAULTS keys, curves as darkroom/curves.js holds them) to PtDevelop's darkroom words
+// (include/pt_types.h): which stages are on, the white-balance matrix, the equaliser's bands, Oklab's, curve 7's, the
+// finish's Levels, curve table and CDL, the vignette and the grain.
+//
+// Follows lib/pathtracer/darkroom/grade.js (lightUniforms :231-271, whiteBalanceMatrix :79-87, whiteFromRGB :90-100,
+// cdlFromWheels :193-201), darkroom/curves.js (normalizeCurve :34-43, curveFunction :55-81, bakeLut :84-96) and
+// core/colorimetry.js (the Rec.709 matrix, Krystek's locus, CCT and Duv) -- each from the published method it names:
+// IEC 61966-2-1's matrix, Krystek (1985), Bradford's cone response, Fritsch and Carlson (1980), the ASC CDL.
+
+const LUMA = [0.2126, 0.7152, 0.0722]
+// linear Rec.709 (D65) to CIE XYZ: IEC 61966-2-1's four-decimal matrix, whose Y row is LUMA
+const RGB_TO_XYZ = [[0.4124, 0.3576, 0.1805], LUMA.slice(), [0.0193, 0.1192, 0.9505]]
+
+function inverse3(m) {
+    const [[a, b, c], [d, e, f], [g, h, k]] = m
+    const c00 = e * k - f * h, c01 = c * h - b * k, c02 = b * f - c * e
+    const c10 = f * g - d * k, c11 = a * k - c * g, c12 = c * d - a * f
+    const c20 = d * h - e * g, c21 = b * g - a * h, c22 = a * e - b * d
+    const det = a * c00 + b * c10 + c * c20
+    return [[c00 / det, c01 / det, c02 / det], [c10 / det, c11 / det, c12 / det], [c20 / det, c21 / det, c22 / det]]
+}
+const XYZ_TO_RGB = inverse3(RGB_TO_XYZ)
+const times = (m, v) => m.map(r => r[0] * v[0] + r[1] * v[1] + r[2] * v[2])
+const product = (a, b) => a.map(r => [0, 1, 2].map(j => r[0] * b[0][j] + r[1] * b[1][j] + r[2] * b[2][j]))
+
+const rgbToXYZ = rgb => times(RGB_TO_XYZ, rgb)
+const xyzToRgb = xyz => times(XYZ_TO_RGB, xyz)
+const xyToXYZ = (x, y, Y = 1) => (y > 0 ? [x / y * Y, Y, (1 - x - y) / y * Y] : [0, 0, 0])
+const chromaticity = ([X, Y, Z]) => { const s = X + Y + Z; return s > 0 ? [X / s, Y / s] : null }
+// CIE 1960 uv, where CCT and Duv are defined
+const xyToUv = (x, y) => { const d = 12 * y - 2 * x + 3; return [4 * x / d, 6 * y / d] }
+const uvToXy = (u, v) => { const d = 2 * u - 8 * v + 4; return [3 * u / d, 2 * v / d] }
+
+const CCT_MIN = 1000, CCT_MAX = 15000
+// the Planckian locus in CIE 1960 uv: Krystek's rational approximation (1985), 1000-15000 K
+function planckUv(T) {
+    const t = Math.min(CCT_MAX, Math.max(CCT_MIN, T))
+    return [(0.860117757 + 1.54118254e-4 * t + 1.28641212e-7 * t * t) / (1 + 8.42420234e-4 * t + 7.08145163e-7 * t * t),
+            (0.317398726 + 4.22806245e-5 * t + 4.20481691e-8 * t * t) / (1 - 2.89741816e-5 * t + 1.61456053e-7 * t * t)]
+}
+// the unit normal of the locus at T, on its green side (positive Duv, ANSI C78.377): the tangent by a central
+// difference, turned a quarter
+function locusNormal(T) {
+    const h = Math.max(1e-3, T * 1e-5)
+    const a = planckUv(T - h), b = planckUv(T + h)
+    const du = b[0] - a[0], dv = b[1] - a[1], l = Math.hypot(du, dv) || 1
+    return [dv / l, -du / l]
+}
+function cctDuvToXy(T, duv = 0) {
+    const [u, v] = planckUv(T), [nu, nv] = locusNormal(T)
+    return uvToXy(u + duv * nu, v + duv * nv)
+}
+// xy to { cct, duv, inRange }: the nearest point of the locus in uv, bracketed by a scan evenly spaced in mired and
+// narrowed by golden sections; inRange false when that point is an end of the locus
+function xyToCctDuv(x, y) {
+    const [u, v] = xyToUv(x, y)
+    const dist = mired => { const p = planckUv(1e6 / mired); return (p[0] - u) ** 2 + (p[1] - v) ** 2 }
+    const lo0 = 1e6 / CCT_MAX, hi0 = 1e6 / CCT_MIN, N = 400
+    let k = 0, best = Infinity
+    for (let i = 0; i <= N; i++) { const d = dist(lo0 + (hi0 - lo0) * i / N); if (d < best) { best = d; k = i } }
+    let a = lo0 + (hi0 - lo0) * Math.max(0, k - 1) / N, b = lo0 + (hi0 - lo0) * Math.min(N, k + 1) / N
+    const g = (Math.sqrt(5) - 1) / 2
+    let c = b - g * (b - a), d = a + g * (b - a), fc = dist(c), fd = dist(d)
+    for (let it = 0; it < 100; it++) {
+        if (fc < fd) { b = d; d = c; fd = fc; c = b - g * (b - a); fc = dist(c) } else { a = c; c = d; fc = fd; d = a + g * (b - a); fd = dist(d) }
+    }
+    const T = 1e6 / ((a + b) / 2)
+    const p = planckUv(T), n = locusNormal(T)
+    return { cct: T, duv: (u - p[0]) * n[0] + (v - p[1]) * n[1], inRange: T > CCT_MIN * 1.001 && T < CCT_MAX * 0.999 }
+}
+
+// D65 as the RGB matrix has it: the chromaticity of Rec.709's (1, 1, 1), and where it sits on the locus
+const D65 = chromaticity(rgbToXYZ([1, 1, 1]))
+const D65_CCT = xyToCctDuv(D65[0], D65[1])
+// the white balance's defaults: D65's own temperature (rounded) and its Duv, the tint's zero
+const WB_DEFAULT = { temp: Math.round(D65_CCT.cct), duv: D65_CCT.duv }
+
+// Bradford's cone response (Lam 1985)
+const BRADFORD = [[0.8951, 0.2664, -0.1614], [-0.7502, 1.7135, 0.0367], [0.0389, -0.0685, 1.0296]]
+const BRADFORD_INV = inverse3(BRADFORD)
+
+// the white a temperature and tint (Duv x 1000 from D65's) name, as xy
+const whiteXy = (temp, tint) => cctDuvToXy(temp, WB_DEFAULT.duv + tint / 1000)
+
+// rows of the 3 x 3 taking linear Rec.709 light lit by the white (temp, tint) to the same scene lit by D65 (a von
+// Kries scaling in Bradford's cone space), each row then times its channel's gain
+function whiteBalanceMatrix(temp, tint = 0, gains = [1, 1, 1]) {
+    const w = whiteXy(temp, tint)
+    const src = times(BRADFORD, xyToXYZ(w[0], w[1], 1)), dst = times(BRADFORD, xyToXYZ(D65[0], D65[1], 1))
+    const scale = [[dst[0] / src[0], 0, 0], [0, dst[1] / src[1], 0], [0, 0, dst[2] / src[2]]]
+    const m = product(XYZ_TO_RGB, product(product(BRADFORD_INV, product(scale, BRADFORD)), RGB_TO_XYZ))
+    return m.map((r, i) => r.map(x => x * gains[i]))
+}
+
+// the eyedropper: the temperature and tint that make a patch of linear Rec.709 light neutral, or { error }
+function whiteFromRGB(rgb) {
+    if (!rgb.every(Number.isFinite) || Math.min(...rgb) < 0) return { error: 'a colour outside the gamut cannot be a white' }
+    const XYZ = rgbToXYZ(rgb)
+    if (!(XYZ[1] > 0)) return { error: 'black: no white to take' }
+    const [x, y] = chromaticity(XYZ)
+    const { cct, duv, inRange } = xyToCctDuv(x, y)
+    if (!inRange) return { error: `not a white: outside ${CCT_MIN}-${CCT_MAX} K` }
+    if (Math.abs(duv) > 0.05) return { error: `not a white: Duv ${duv.toFixed(3)}` }
+    return { temp: Math.round(cct), tint: Math.round((duv - WB_DEFAULT.duv) * 10000) / 10 }
+}
+
+// every tool at its identity
+const LIGHT_DEFAULTS = {
+    wbTemp: WB_DEFAULT.temp, wbTint: 0, gainR: 1, gainG: 1, gainB: 1,
+    eq0: 0, eq1: 0, eq2: 0, eq3: 0, eq4: 0, eq5: 0, eq6: 0, eq7: 0,
+    labSat: 0, labVib: 0, labHue: 0,
+    tcWhite: Math.log2(5.6), tcToe: 0.20, tcShoulder: 0.15,
+    vigAmount: 0, vigMid: 0.5, vigRound: 0,
+    transfer: 0, lvInBlack: 0, lvInWhite: 1, lvGamma: 1, lvOutBlack: 0, lvOutWhite: 1,
+    liftX: 0, liftY: 0, liftM: 0, gammaX: 0, gammaY: 0, gammaM: 0, gainX: 0, gainY: 0, gainM: 0,
+    grainAmount: 0, grainSize: 1.5, dither: false, curveLuma: false,
+}
+const EQ_KEYS = ['eq0', 'eq1', 'eq2', 'eq3', 'eq4', 'eq5', 'eq6', 'eq7']
+const LIGHT = { ANY: 1, WB: 2, EQ: 4, LAB: 8, LEVELS: 16, CURVES: 32, CDL: 64, VIGNETTE: 128, GRAIN: 256, DITHER: 512, CURVE_LUMA: 1024 }
+const TRANSFERS = { 'sRGB': 0, 'gamma 2.2': 1, 'gamma 2.4 (BT.1886)': 2, 'linear': 3 }
+
+// THE WHEELS: a disc point (x, y) spread on red, green and blue along axes 120 degrees apart (the three sum to 0: a
+// tint, not a brightening), the wheel's master moving all three; lift is the CDL's offset, gamma its power, gain its slope
+const wheelTint = (x, y) => [x, -0.5 * x + Math.sqrt(3) / 2 * y, -0.5 * x - Math.sqrt(3) / 2 * y]
+function cdlFromWheels(s) {
+    const lift = wheelTint(s.liftX, s.liftY), gam = wheelTint(s.gammaX, s.gammaY), gain = wheelTint(s.gainX, s.gainY)
+    return {
+        offset: lift.map(t => 0.1 * (s.liftM + t)),
+        power: gam.map(t => 2 ** (-0.5 * (s.gammaM + t))),
+        slope: gain.map(t => 2 ** (s.gainM + 0.5 * t)),
+    }
+}
+
+// THE CURVES: points [x, y] in 0..1, sorted, at least 1/256 apart along x, at most 16, at least the two ends
+const LUT_SIZE = 1024, MAX_POINTS = 16, MIN_GAP = 1 / 256
+const CHANNELS = ['m', 'r', 'g', 'b']
+const straight = () => [[0, 0], [1, 1]]
+function normalizeCurve(points) {
+    const pts = (Array.isArray(points) ? points : [])
+        .filter(p => Array.isArray(p) && Number.isFinite(+p[0]) && Number.isFinite(+p[1]))
+        .map(p => [Math.min(1, Math.max(0, +p[0])), Math.min(1, Math.max(0, +p[1]))])
+        .sort((a, b) => a[0] - b[0])
+    const kept = []
+    for (const p of pts) if (!kept.length || p[0] - kept[kept.length - 1][0] >= MIN_GAP) kept.push(p)
+    if (kept.length < 2) return straight()
+    return kept.length > MAX_POINTS ? [kept[0], ...kept.slice(1, MAX_POINTS - 1), kept[kept.length - 1]] : kept
+}
+const isStraight = p => p.length === 2 && p[0][0] === 0 && p[0][1] === 0 && p[1][0] === 1 && p[1][1] === 1
+const curvesStraight = curves => CHANNELS.every(k => isStraight(normalizeCurve(curves && curves[k] !== undefined ? curves[k] : straight())))
+
+// the monotone cubic through the points (Fritsch and Carlson 1980): Hermite pieces whose end slopes start as the mean
+// of the secants (0 at a turn), 0 on a flat secant, and are scaled back where they would overshoot; held beyond the ends
+function curveFunction(points) {
+    const p = normalizeCurve(points), n = p.length
+    const X = p.map(q => q[0]), Y = p.map(q => q[1])
+    const sec = []
+    for (let k = 0; k + 1 < n; k++) sec.push((Y[k + 1] - Y[k]) / (X[k + 1] - X[k]))
+    const m = new Array(n)
+    m[0] = sec[0]
+    m[n - 1] = sec[n - 2]
+    for (let k = 1; k < n - 1; k++) m[k] = sec[k - 1] * sec[k] <= 0 ? 0 : (sec[k - 1] + sec[k]) / 2
+    for (let k = 0; k + 1 < n; k++) {
+        if (sec[k] === 0) { m[k] = 0; m[k + 1] = 0; continue }
+        const al = m[k] / sec[k], be = m[k + 1] / sec[k], r = al * al + be * be
+        if (r > 9) { const t = 3 / Math.sqrt(r); m[k] = t * al * sec[k]; m[k + 1] = t * be * sec[k] }
+    }
+    return x => {
+        if (x <= X[0]) return Y[0]
+        if (x >= X[n - 1]) return Y[n - 1]
+        let k = 0
+        while (k < n - 2 && x > X[k + 1]) k++
+        const h = X[k + 1] - X[k], t = (x - X[k]) / h, t2 = t * t, t3 = t2 * t
+        const y = (2 * t3 - 3 * t2 + 1) * Y[k] + (t3 - 2 * t2 + t) * h * m[k] + (3 * t2 - 2 * t3) * Y[k + 1] + (t3 - t2) * h * m[k + 1]
+        return Math.min(1, Math.max(0, y))
+    }
+}
+// the four curves baked: master, red, green, blue, LUT_SIZE values each; a straight curve is i / 1023 exactly
+function bakeLut(curves) {
+    const lut = new Float32Array(4 * LUT_SIZE)
+    CHANNELS.forEach((key, ch) => {
+        const pts = normalizeCurve(curves && curves[key] !== undefined ? curves[key] : straight())
+        const f = isStraight(pts) ? null : curveFunction(pts)
+        for (let i = 0; i < LUT_SIZE; i++) lut[ch * LUT_SIZE + i] = f ? f(i / (LUT_SIZE - 1)) : i / (LUT_SIZE - 1)
+    })
+    return lut
+}
+
+// PtDevelop's darkroom words for settings `s` (missing keys at their defaults). A stage at its identity is off, so it
+// is never computed and cannot change a bit; curve 7 (s.toneMap 7) turns `ANY` on with no stage.
+function lightUniforms(s = {}) {
+    const v = k => (Number.isFinite(+s[k]) && s[k] !== null && s[k] !== '' ? +s[k] : LIGHT_DEFAULTS[k])
+    const gains = [v('gainR'), v('gainG'), v('gainB')]
+    const wbOn = v('wbTemp') !== WB_DEFAULT.temp || v('wbTint') !== 0 || gains.some(g => g !== 1)
+    const bands = EQ_KEYS.map(v)
+    const eqOn = bands.some(b => b !== 0)
+    const lab = [v('labSat'), v('labVib'), v('labHue') * Math.PI / 180]
+    const labOn = lab.some(x => x !== 0)
+    const curve7 = (s.toneMap | 0) === 7
+    const transfer = Math.max(0, Math.min(3, v('transfer') | 0))
+    const lv = ['lvInBlack', 'lvInWhite', 'lvGamma', 'lvOutBlack', 'lvOutWhite'].map(v)
+    const levelsOn = lv.some((x, i) => x !== [0, 1, 1, 0, 1][i])
+    const curvesOn = !curvesStraight(s.curves)
+    const wheels = {}
+    for (const k of ['liftX', 'liftY', 'liftM', 'gammaX', 'gammaY', 'gammaM', 'gainX', 'gainY', 'gainM']) wheels[k] = v(k)
+    const cdlOn = Object.values(wheels).some(x => x !== 0)
+    const cdl = cdlFromWheels(wheels)
+    const vignette = [v('vigAmount'), v('vigMid'), v('vigRound')]
+    const grain = [v('grainAmount'), Math.max(1, v('grainSize'))]
+    const on = [wbOn && LIGHT.WB, eqOn && LIGHT.EQ, labOn && LIGHT.LAB, levelsOn && LIGHT.LEVELS, curvesOn && LIGHT.CURVES,
+        cdlOn && LIGHT.CDL, vignette[0] !== 0 && LIGHT.VIGNETTE, grain[0] !== 0 && LIGHT.GRAIN, !!s.dither && LIGHT.DITHER,
+        curvesOn && !!s.curveLuma && LIGHT.CURVE_LUMA].reduce((a, b) => a | (b || 0), 0)
+    const any = on !== 0 || curve7 || transfer !== 0
+    const wb = wbOn ? whiteBalanceMatrix(v('wbTemp'), v('wbTint'), gains) : [[1, 0, 0], [0, 1, 0], [0, 0, 1]]
+    return {
+        lightOn: any ? on | LIGHT.ANY : 0,
+        wb: [].concat(...wb), eq: bands, lab, tone7: [2 ** v('tcWhite'), v('tcToe'), v('tcShoulder')],
+        transfer, levels: [lv[0], lv[1], 1 / Math.max(0.01, lv[2]), lv[3], lv[4]],
+        cdlSlope: cdl.slope, cdlOffset: cdl.offset, cdlPower: cdl.power, vignette, grain,
+        curveLut: curvesOn ? bakeLut(s.curves) : null,
+    }
+}
+const lightIdentity = s => lightUniforms(s).lightOn === 0
+
+// every `step`-th pixel each way, so a histogram visits about `cap` pixels whatever the picture's size
+const histogramStep = (w, h, cap = 1 << 20) => Math.max(1, Math.ceil(Math.sqrt(w * h / cap)))
+// the share of pixels (luma) at white and at black in a histogram of 4 x 256 bins
+function clipShares(bins) {
+    let n = 0
+    for (let i = 768; i < 1024; i++) n += bins[i]
+    return n ? { white: bins[1023] / n, black: bins[768] / n, pixels: n } : { white: 0, black: 0, pixels: 0 }
+}
+
+module.exports = {
+    LUMA, RGB_TO_XYZ, XYZ_TO_RGB, rgbToXYZ, xyzToRgb, xyToXYZ, chromaticity, planckUv, cctDuvToXy, xyToCctDuv, D65, WB_DEFAULT,
+    whiteXy, whiteBalanceMatrix, whiteFromRGB, LIGHT_DEFAULTS, EQ_KEYS, LIGHT, TRANSFERS, wheelTint, cdlFromWheels, LUT_SIZE,
+    normalizeCurve, curveFunction, bakeLut, lightUniforms, lightIdentity, histogramStep, clipShares,
+}

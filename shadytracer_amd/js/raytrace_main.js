@@ -76,8 +76,12 @@ const DEFAULTS = {
 const KNOWN = new Set(UNIFORMS.map(f => f[0]))
 // PtDevelop (include/pt_types.h) and the reference's toneMap numbering (core/rendertoscreen.js)
 const DEVELOP = [['toneMap', 'u', 1], ['autoExposure', 'u', 1], ['exposureEV', 'f', 1], ['aeSpeed', 'f', 1], ['aeHeadroom', 'f', 1],
-    ['dither', 'u', 1], ['flipY', 'u', 1], ['pad', 'u', 1]]
-const TONE_MAPS = { linear: 0, neutral: 1, agx: 2, 'agx-punchy': 3, aces: 4, reinhard: 5, hable: 6 }
+    ['lightOn', 'u', 1], ['flipY', 'u', 1], ['transfer', 'u', 1], ['contrast', 'f', 1], ['saturation', 'f', 1], ['brightness', 'f', 1],
+    ['measureMode', 'u', 1], ['measureMin', 'f', 1], ['measureMax', 'f', 1], ['measureBands', 'u', 1], ['wb', 'f', 9], ['eq', 'f', 8],
+    ['lab', 'f', 3], ['tone7', 'f', 3], ['levels', 'f', 5], ['cdlSlope', 'f', 3], ['cdlOffset', 'f', 3], ['cdlPower', 'f', 3],
+    ['vignette', 'f', 3], ['grain', 'f', 2], ['pad', 'f', 7], ['curveLut', 'f', 4096]]
+const TONE_MAPS = { linear: 0, neutral: 1, agx: 2, 'agx-punchy': 3, aces: 4, reinhard: 5, hable: 6, 'hable-adjustable': 7 }
+const DARKROOM = require('./darkroom')
 
 function RAYTRACER(WG, _U, Q, scene) {
     WG = WG || {}
@@ -184,20 +188,45 @@ function RAYTRACER(WG, _U, Q, scene) {
     }
     // the accumulation itself: { acc: FP64 mean rgb + count per pixel, mom: FP64 (V, S) per pixel }, row 0 at the bottom
     R.readAccum = () => { const { acc, mom } = readback(); return { acc, mom } }
-    // THE DEVELOPED PICTURE (core/rendertoscreen.js's display pass with core/autoexposure.js's meter, for a picture that is
-    // saved): { rgba (Uint8Array, rows top first), width, height, ev, log2Average, log2High, metered }.
+    // THE DEVELOPED PICTURE (core/rendertoscreen.js's develop pass with core/autoexposure.js's meter, for a picture that
+    // is saved): { rgba (Uint8Array, rows top first), width, height, ev, log2Average, log2High, metered }.
     // toneMap by the reference's numbering or name; autoExposure metered from the picture (eased by aeSpeed a call,
-    // set afresh after clearAccums), else exposureEV (EV100: white at 1.2 x 2^EV cd/m^2).
-    R.develop = ({ toneMap = 'neutral', autoExposure = true, exposureEV = 15, aeSpeed = 1, aeHeadroom = 2, dither = false } = {}) => {
-        if (!current) throw new Error('RAYTRACER: no scene')
+    // set afresh after clearAccums), else exposureEV (or `ev`; EV100: white at 1.2 x 2^EV cd/m^2). The rest are the
+    // reference's develop settings (develop.js viewSettings): contrast, saturation, brightness; show (or measureMode)
+    // 1 or 2 with measureMin, measureMax, measureBands for the false-colour views; and the darkroom's tools by
+    // darkroom/grade.js's names (wbTemp, wbTint, gainR.., eq0..eq7, labSat, labVib, labHue, tcWhite, tcToe,
+    // tcShoulder, vigAmount, vigMid, vigRound, transfer, lvInBlack.., liftX.., grainAmount, grainSize, dither,
+    // curveLuma) with `curves` as darkroom/curves.js holds them ({ m, r, g, b }: lists of [x, y]).
+    const developBytes = (opts = {}) => {
+        const { toneMap = 'neutral', autoExposure = true, aeSpeed = 1, aeHeadroom = 2 } = opts
         const tone = typeof toneMap === 'number' ? toneMap : TONE_MAPS[toneMap]
         if (tone === undefined) throw new Error(`RAYTRACER: no tone curve '${toneMap}' (${Object.keys(TONE_MAPS).join(', ')})`)
+        const ev = opts.exposureEV !== undefined ? opts.exposureEV : opts.ev !== undefined ? opts.ev : 15
+        const light = DARKROOM.lightUniforms({ ...opts, toneMap: tone })
         const buf = new ArrayBuffer(sizeOf(DEVELOP))
-        pack(DEVELOP, { toneMap: tone, autoExposure: autoExposure ? 1 : 0, exposureEV, aeSpeed, aeHeadroom, dither: dither ? 1 : 0, flipY: 1 },
-            new DataView(buf), 0)
+        pack(DEVELOP, {
+            toneMap: tone, autoExposure: autoExposure ? 1 : 0, exposureEV: ev, aeSpeed, aeHeadroom, flipY: 1,
+            contrast: +opts.contrast || 0, saturation: +opts.saturation || 0, brightness: +opts.brightness || 0,
+            measureMode: (opts.show !== undefined ? opts.show : opts.measureMode) | 0, measureMin: +opts.measureMin || 1,
+            measureMax: +opts.measureMax || 100000, measureBands: opts.measureBands | 0, ...light,
+        }, new DataView(buf), 0)
+        return buf
+    }
+    R.develop = (opts = {}) => {
+        if (!current) throw new Error('RAYTRACER: no scene')
+        const buf = developBytes(opts)
         const rgba = new Uint8Array(U.width * U.height * 4)
         const [ev, log2Average, log2High, metered] = A.develop(h, buf, rgba)
         return { rgba, width: U.width, height: U.height, ev, log2Average, log2High, metered }
+    }
+    // THE DEVELOPED PICTURE'S HISTOGRAM (develop.js's darkroom histogram): what R.develop(opts) would write, counted
+    // over every `step`-th pixel each way: { bins (Uint32Array: 256 each of R, G, B, luma), step, white, black, ev }
+    R.developHistogram = (opts = {}, step = DARKROOM.histogramStep(U.width, U.height)) => {
+        if (!current) throw new Error('RAYTRACER: no scene')
+        const bins = new Uint32Array(1024)
+        const [ev] = A.developHistogram(h, developBytes(opts), step, bins)
+        const { white, black } = DARKROOM.clipShares(bins)
+        return { bins, step, white, black, ev }
     }
     // the developed picture as a PNG file (darkroom/export/png.js's role): a Buffer
     R.savePicture = (format = 'png', opts = {}) => {

@@ -136,7 +136,13 @@ class PtOpticsTally(ctypes.Structure):
 
 class PtDevelop(ctypes.Structure):
     _fields_ = [('toneMap', ctypes.c_uint), ('autoExposure', ctypes.c_uint), ('exposureEV', ctypes.c_float), ('aeSpeed', ctypes.c_float),
-                ('aeHeadroom', ctypes.c_float), ('dither', ctypes.c_uint), ('flipY', ctypes.c_uint), ('pad', ctypes.c_uint)]
+                ('aeHeadroom', ctypes.c_float), ('lightOn', ctypes.c_uint), ('flipY', ctypes.c_uint), ('transfer', ctypes.c_uint),
+                ('contrast', ctypes.c_float), ('saturation', ctypes.c_float), ('brightness', ctypes.c_float),
+                ('measureMode', ctypes.c_uint), ('measureMin', ctypes.c_float), ('measureMax', ctypes.c_float),
+                ('measureBands', ctypes.c_uint), ('wb', ctypes.c_float * 9), ('eq', ctypes.c_float * 8), ('lab', ctypes.c_float * 3),
+                ('tone7', ctypes.c_float * 3), ('levels', ctypes.c_float * 5), ('cdlSlope', ctypes.c_float * 3),
+                ('cdlOffset', ctypes.c_float * 3), ('cdlPower', ctypes.c_float * 3), ('vignette', ctypes.c_float * 3),
+                ('grain', ctypes.c_float * 2), ('pad', ctypes.c_float * 7), ('curveLut', ctypes.c_float * 4096)]
 
 
 class PtDevelopInfo(ctypes.Structure):
@@ -144,7 +150,8 @@ class PtDevelopInfo(ctypes.Structure):
 
 
 # the reference's toneMap numbering (core/rendertoscreen.js)
-TONE_MAPS = {'linear': 0, 'neutral': 1, 'agx': 2, 'agx-punchy': 3, 'aces': 4, 'reinhard': 5, 'hable': 6}
+TONE_MAPS = {'linear': 0, 'neutral': 1, 'agx': 2, 'agx-punchy': 3, 'aces': 4, 'reinhard': 5, 'hable': 6, 'hable-adjustable': 7}
+LIGHT_ANY, LIGHT_DITHER = 1, 512
 
 STRUCTS = [PtUniforms, PtMaterial, PtLightFace, PtCamera, PtEnv, PtHaze, PtSceneDesc, PtOpts, PtGuideConfig, PtOpticsScene,
            PtOpticsResult, PtTuning, PtOpticsSource, PtOpticsTrace, PtOpticsTally, PtDevelop, PtDevelopInfo]
@@ -179,6 +186,8 @@ SIGNATURES = {
     'pt_optics_trace': (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(PtOpticsScene), ctypes.c_uint32, ctypes.POINTER(PtOpticsResult),
                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     'pt_optics_error': (ctypes.c_char_p, []),
+    'pt_develop_histogram': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PtDevelop), ctypes.c_size_t, ctypes.c_uint32,
+                             ctypes.POINTER(ctypes.c_uint32), ctypes.c_size_t, ctypes.POINTER(PtDevelopInfo)]),
     'pt_develop': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PtDevelop), ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint8),
                                   ctypes.c_size_t, ctypes.POINTER(PtDevelopInfo)]),
     'pt_optics_run': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PtOpticsSource), ctypes.c_uint32, ctypes.POINTER(PtOpticsTrace),
@@ -410,18 +419,47 @@ class Engine:
         self._chk(self.lib.pt_reset_stats(self.h))
 
     # ---- the developed picture (core/rendertoscreen.js, core/autoexposure.js)
-    def develop(self, tone='neutral', auto=True, ev=15.0, speed=1.0, headroom=2.0, dither=False, flip=True):
-        """The accumulation exposed, tone mapped and sRGB-encoded (pt_develop): (uint8 [h, w, 4] RGBA, rows top first
-        with flip, and {'ev', 'log2Average', 'log2High', 'metered'}). auto: metered from the picture, else EV100 `ev`."""
+    def _develop_settings(self, tone, auto, ev, speed, headroom, dither, flip, grade):
         d = PtDevelop()
         d.toneMap = TONE_MAPS[tone] if isinstance(tone, str) else int(tone)
         d.autoExposure, d.exposureEV, d.aeSpeed, d.aeHeadroom = int(bool(auto)), ev, speed, headroom
-        d.dither, d.flipY = int(bool(dither)), int(bool(flip))
+        d.flipY = int(bool(flip))
+        d.wb[:] = [1, 0, 0, 0, 1, 0, 0, 0, 1]
+        d.measureMin, d.measureMax = 1.0, 100000.0
+        for k, v in (grade or {}).items():
+            if v is None:
+                continue
+            f = getattr(d, k)
+            if isinstance(f, ctypes.Array):
+                f[:] = [float(x) for x in v]
+            else:
+                setattr(d, k, type(f)(v))
+        if dither:
+            d.lightOn |= LIGHT_ANY | LIGHT_DITHER
+        return d
+
+    def develop(self, tone='neutral', auto=True, ev=15.0, speed=1.0, headroom=2.0, dither=False, flip=True, grade=None):
+        """The accumulation developed (pt_develop): (uint8 [h, w, 4] RGBA, rows top first with flip, and {'ev',
+        'log2Average', 'log2High', 'metered'}). auto: metered from the picture, else EV100 `ev`. dither: triangular,
+        one 8-bit step either way. grade: PtDevelop's darkroom words by name (lightOn, wb (rows), eq, lab, tone7,
+        transfer, levels, cdlSlope, cdlOffset, cdlPower, vignette, grain, curveLut, contrast, saturation, brightness,
+        measureMode, measureMin, measureMax, measureBands), as js/darkroom.js lightUniforms makes them."""
+        d = self._develop_settings(tone, auto, ev, speed, headroom, dither, flip, grade)
         out = np.zeros((self.height, self.width, 4), np.uint8)
         info = PtDevelopInfo()
         self._chk(self.lib.pt_develop(self.h, ctypes.byref(d), ctypes.sizeof(d), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
                                       self.width * self.height, ctypes.byref(info)))
         return out, {k: getattr(info, k) for k, _ in PtDevelopInfo._fields_}
+
+    def develop_histogram(self, tone='neutral', auto=True, ev=15.0, speed=1.0, headroom=2.0, dither=False, grade=None, step=1):
+        """What develop() would write, counted over every `step`-th pixel each way (pt_develop_histogram): (uint32
+        [4, 256]: R, G, B, luma, and the info develop() gives)."""
+        d = self._develop_settings(tone, auto, ev, speed, headroom, dither, True, grade)
+        hist = np.zeros((4, 256), np.uint32)
+        info = PtDevelopInfo()
+        self._chk(self.lib.pt_develop_histogram(self.h, ctypes.byref(d), ctypes.sizeof(d), int(step),
+                                                hist.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), hist.size, ctypes.byref(info)))
+        return hist, {k: getattr(info, k) for k, _ in PtDevelopInfo._fields_}
 
     # ---- the optics trace over the loaded scene (createOptics: optics/index.js)
     def optics_run(self, sources, rays=100000, bounces=64, seed=1, far=None, detectors=(), det_cap=0, max_dist=0.0,

@@ -39,7 +39,7 @@ def test_struct_sizes_match_the_bindings():
     for i, s in enumerate(pt.STRUCTS):
         assert lib.pt_struct_size(i) == ctypes.sizeof(s), s.__name__
     assert lib.pt_struct_size(99) == 0
-    assert lib.pt_abi_version() == 3
+    assert lib.pt_abi_version() == 4
 
 
 def test_addon_loads_and_packs_the_same_structures():
@@ -47,7 +47,7 @@ def test_addon_loads_and_packs_the_same_structures():
                                     stdout=subprocess.PIPE, text=True).stdout)
     s = out['sizes']
     assert s[0] == s[1] == ctypes.sizeof(pt.PtUniforms) and s[2] == s[3] and s[4] == s[5]
-    assert out['addonKeys'] == sorted(['abiVersion', 'autotune', 'clearAccum', 'compileScene', 'create', 'destroy', 'develop', 'foldParams', 'opticsRun',
+    assert out['addonKeys'] == sorted(['abiVersion', 'autotune', 'clearAccum', 'compileScene', 'create', 'destroy', 'develop', 'developHistogram', 'foldParams', 'opticsRun',
                                        'opticsTrace',
                                        'readAccum', 'readStats', 'setParams', 'setScene', 'setTuning', 'setUniforms', 'structSize',
                                        'trace'])

@@ -87,3 +87,70 @@ def test_node_saves_a_png(tmp_path):
     want, _, ev = D.develop(acc, tone=1)
     assert js['ev'] == pytest.approx(ev, abs=2e-3)
     assert np.abs(img[..., :3].astype(int) - want.astype(int)).max() <= 1
+
+
+# ---- the darkroom
+
+def darkroom_words(settings):
+    r = subprocess.run(['node', os.path.join(ROOT, 'tests', 'js', 'darkroom_uniforms.js'), json.dumps(settings)], capture_output=True,
+                       text=True, timeout=60, check=True)
+    return json.loads(r.stdout)
+
+
+BEND = {'m': [[0, 0], [0.3, 0.36], [1, 1]], 'r': [[0, 0.02], [0.5, 0.47], [1, 1]]}
+GRADES = {
+    'light': dict(wbTemp=3400, wbTint=3, gainB=1.05, eq1=0.5, eq5=-0.4, labSat=0.3, labVib=0.4, labHue=12),
+    'finish': dict(transfer=1, lvInBlack=0.02, lvInWhite=0.95, lvGamma=1.2, lvOutBlack=0.01, lvOutWhite=0.98, curves=BEND,
+                   liftX=0.1, liftM=0.05, gammaY=-0.2, gainM=0.1),
+    'luma-curve': dict(curves=BEND, curveLuma=True, vigAmount=-0.8, vigMid=0.2, vigRound=0.5),
+    'curve7-graded': dict(toneMap=7, tcWhite=2.0, tcToe=0.35, tcShoulder=0.25, grainAmount=0.5, grainSize=2.0, dither=True),
+}
+
+
+@pytest.mark.parametrize('name', list(GRADES))
+def test_darkroom_matches_the_checker(engine, name):
+    settings = GRADES[name]
+    say('case: develop room-c2 96x54 (16 samples), darkroom', name)
+    acc = render(engine, 'room-c2', 16, width=96, height=54)
+    grade = darkroom_words(settings)
+    if name == 'finish':
+        grade.update(contrast=0.15, saturation=-0.1, brightness=0.02)
+    tone = settings.get('toneMap', 1)
+    rgba, info = engine.develop(tone=tone, auto=False, ev=9.0, grade=grade)
+    want, v, _ = D.develop(acc, tone=tone, ev=9.0, grade=grade)
+    diff = np.abs(rgba[..., :3].astype(int) - want.astype(int))
+    # FP32 against FP64: a channel lands one 8-bit step off only where the exact value sits at a rounding edge
+    say('  max step', diff.max(), 'channels off', int((diff > 0).sum()), 'of', diff.size)
+    assert diff.max() <= 1 and (diff > 0).mean() < 0.01
+    # the stages did something: the graded picture is not the plain one
+    plain, _, _ = D.develop(acc, tone=tone if tone != 7 else 6, ev=9.0)
+    assert np.abs(want.astype(int) - plain.astype(int)).mean() > 1.0
+
+
+def test_false_colour_views_match_the_checker(engine):
+    say('case: develop room-c2 96x54 (16 samples), false colour of luminance, continuous and in 6 bands')
+    acc = render(engine, 'room-c2', 16, width=96, height=54)
+    for g in (dict(measureMode=2, measureMin=0.5, measureMax=5000.0), dict(measureMode=2, measureMin=0.5, measureMax=5000.0, measureBands=6)):
+        rgba, _ = engine.develop(tone=1, auto=False, ev=9.0, grade=g)
+        want, _, _ = D.develop(acc, tone=1, ev=9.0, grade=g)
+        diff = np.abs(rgba[..., :3].astype(int) - want.astype(int))
+        assert diff.max() <= 1 and (diff > 0).mean() < 0.01
+        # the legend strip along the bottom: the scale itself, left to right
+        assert np.array_equal(rgba[-1, :, :3], want[-1])
+
+
+def test_histogram_counts_the_developed_picture(engine):
+    say('case: develop_histogram room-c2 96x54 (16 samples), step 1 and 3, graded')
+    acc = render(engine, 'room-c2', 16, width=96, height=54)
+    grade = darkroom_words(GRADES['light'])
+    rgba, _ = engine.develop(tone=1, auto=False, ev=9.0, grade=grade, flip=False)
+    for step in (1, 3):
+        hist, _ = engine.develop_histogram(tone=1, auto=False, ev=9.0, grade=grade, step=step)
+        sub = rgba[::step, ::step, :3]
+        for ch in range(3):
+            assert np.array_equal(hist[ch], np.bincount(sub[..., ch].ravel(), minlength=256)), (step, ch)
+        assert int(hist[3].sum()) == sub.shape[0] * sub.shape[1]
+        want = D.histogram(acc, tone=1, ev=9.0, grade=grade, step=step)
+        assert np.abs(hist[3].astype(int) - want[3]).sum() <= 0.02 * sub.shape[0] * sub.shape[1]
+    with pytest.raises(pt.PtError, match='false-colour'):
+        engine.develop_histogram(tone=1, grade=dict(measureMode=1))
