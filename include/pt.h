@@ -195,6 +195,12 @@ int pt_guide_status(pt_ctx* ctx, double* out, size_t n);
  * result). */
 int pt_develop(pt_ctx* ctx, const PtDevelop* d, size_t dsize, uint8_t* rgba, size_t npx, PtDevelopInfo* info);
 
+/* pt_develop to 16 bits a channel: rgba16 holds npx RGBA pixels of uint16 (alpha 65535), round(v 65535) of what the
+ * stages computed, with d's dither off (16 bits have no 8-bit steps to break).
+ * Replaces the develop pass drawn into a float target for a 16-bit PNG (lib/pathtracer/develop.js developedRows and
+ * developUniforms' `float` :73-77; darkroom/export/run.js :30-41 rounding to 16 bits). */
+int pt_develop16(pt_ctx* ctx, const PtDevelop* d, size_t dsize, uint16_t* rgba16, size_t npx, PtDevelopInfo* info);
+
 /* What pt_develop would write, counted: every `step`-th pixel each way developed as pt_develop does it (metering
  * included), hist[0..255] red, [256..511] green, [512..767] blue, [768..1023] the Rec.709 luma of the display values,
  * bin round(v 255); nbins = 4 x PT_HIST_BINS. A false-colour view (d->measureMode) is refused.

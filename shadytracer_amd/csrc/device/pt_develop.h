@@ -389,6 +389,21 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_develop(const PtDev* _
                          (unsigned char)fminf(c.z * 255.0f + 0.5f, 255.0f), 255);
 }
 
+// The same development to 16 bits a channel (RGBA, alpha 65535): round(v 65535), as a 16-bit PNG keeps it.
+extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_develop16(const PtDev* __restrict__ D, ushort4* __restrict__ out,
+                                                                   const float* __restrict__ state, const PtDevelop* __restrict__ s) {
+    const unsigned int W = D->u.width, H = D->u.height;
+    const unsigned int i = blockIdx.x * PT_BLOCK + threadIdx.x;
+    if (i >= W * H) return;
+    bool sampled;
+    const V3 light = accumMean(D, i, sampled);
+    const unsigned int x = i % W, y = i / W;
+    const V3 c = developPixel(light, developScale(state, s), x, y, W, H, s);
+    const unsigned int o = (s->flipY ? (H - 1u - y) : y) * W + x;
+    out[o] = make_ushort4((unsigned short)fminf(c.x * 65535.0f + 0.5f, 65535.0f), (unsigned short)fminf(c.y * 65535.0f + 0.5f, 65535.0f),
+                          (unsigned short)fminf(c.z * 65535.0f + 0.5f, 65535.0f), 65535);
+}
+
 // THE DEVELOPED PICTURE'S HISTOGRAM (rendertoscreen.js develop_histogram_prog): every `step`-th pixel each way developed
 // as pt_develop does it and counted, 256 bins each of R, G, B and the Rec.709 luma of the display values (bin
 // round(v 255)). A block's 1024 bins in LDS, one atomic a non-empty bin to `hist`; every thread reaches both barriers.

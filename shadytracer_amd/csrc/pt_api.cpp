@@ -28,10 +28,10 @@ const char* kKernelNames[] = {"pt_genrays", "pt_march", "pt_shade", "pt_occlude"
                               "pt_bin_count", "pt_bin_scan", "pt_bin_scatter", "pt_sppm_clear", "pt_photon_gen", "pt_surface",
                               "pt_guide_infer", "pt_guide_eval", "pt_guide_collect", "pt_guide_grad", "pt_guide_adam", "pt_guide_commit",
                               "pt_flush", "pt_opt_gen", "pt_opt_shade", "pt_opt_flush", "pt_ae_hist", "pt_ae_update",
-                              "pt_develop", "pt_develop_hist"};
+                              "pt_develop", "pt_develop_hist", "pt_develop16"};
 enum { K_GENRAYS, K_MARCH, K_SHADE, K_OCCLUDE, K_RESOLVE, K_MFLUT, K_MFAVG, K_PICK, K_BINCOUNT, K_BINSCAN, K_BINSCATTER,
        K_SPPMCLEAR, K_PHOTONGEN, K_SURFACE, K_GINFER, K_GEVAL, K_GCOLLECT, K_GGRAD, K_GADAM, K_GCOMMIT, K_FLUSH, K_OPTGEN, K_OPTSHADE,
-       K_OPTFLUSH, K_AEHIST, K_AEUPDATE, K_DEVELOP, K_DEVHIST, K_COUNT };
+       K_OPTFLUSH, K_AEHIST, K_AEUPDATE, K_DEVELOP, K_DEVHIST, K_DEVELOP16, K_COUNT };
 
 // the guide network's sizes (device/pt_guide.h)
 const unsigned int kGuideNP = 16169, kGuideWT = 17408, kGuideBias = 256, kGuideParts = 256, kGuideMaxSteps = 64;
@@ -1264,6 +1264,29 @@ int pt_develop(pt_ctx* c, const PtDevelop* d, size_t dsize, uint8_t* rgba, size_
     return developInfo(c, d, info);
 }
 
+int pt_develop16(pt_ctx* c, const PtDevelop* d, size_t dsize, uint16_t* rgba16, size_t npx, PtDevelopInfo* info) {
+    if (!c) return fail(c, PT_ERR_ARG, "ctx is NULL");
+    if (!rgba16) return fail(c, PT_ERR_ARG, "rgba16 is NULL");
+    if (!d || dsize != sizeof(PtDevelop)) return developBegin(c, d, dsize);
+    const size_t px = size_t(c->host.u.width) * c->host.u.height;
+    if (c->module && npx != px) return fail(c, PT_ERR_ARG, "npx %zu, the picture has %zu pixels", npx, px);
+    // 16 bits keep what the stages computed: no dither (it breaks 8-bit steps, and there are none); the rest as asked
+    PtDevelop s = *d;
+    s.lightOn &= ~PT_LIGHT_DITHER;
+    int rc;
+    if ((rc = developBegin(c, &s, dsize))) return rc;
+    uint16_t* dout = nullptr;
+    if ((rc = allocBuf(c, dout, px * 8))) return rc;
+    void* dv[] = {&c->dev, &dout, &c->aeState, &c->devSet};
+    hipError_t e = hipModuleLaunchKernel(c->fn[K_DEVELOP16], (unsigned int)((px + 255) / 256), 1, 1, 256, 1, 1, 0, c->stream, dv, nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(rgba16, dout, px * 8, hipMemcpyDeviceToHost);
+    void* p = dout;
+    freeBuf(c, p);
+    HIPCHK(c, e);
+    return developInfo(c, &s, info);
+}
+
 int pt_develop_histogram(pt_ctx* c, const PtDevelop* d, size_t dsize, uint32_t step, uint32_t* hist, size_t nbins, PtDevelopInfo* info) {
     if (!c) return fail(c, PT_ERR_ARG, "ctx is NULL");
     if (!hist) return fail(c, PT_ERR_ARG, "hist is NULL");
@@ -1274,16 +1297,18 @@ int pt_develop_histogram(pt_ctx* c, const PtDevelop* d, size_t dsize, uint32_t s
     if ((rc = developBegin(c, d, dsize))) return rc;
     uint32_t* dh = nullptr;
     if ((rc = allocBuf(c, dh, 4 * PT_HIST_BINS * 4))) return rc;
-    HIPCHK(c, hipMemsetAsync(dh, 0, 4 * PT_HIST_BINS * 4, c->stream));
     const size_t W = c->host.u.width, H = c->host.u.height;
     const size_t n = ((W + step - 1) / step) * ((H + step - 1) / step);
     const unsigned int blocks = (unsigned int)std::min<size_t>((n + 255) / 256, size_t(c->cus) * 4u);
     void* hv[] = {&c->dev, &dh, &c->aeState, &c->devSet, &step};
-    HIPCHK(c, hipModuleLaunchKernel(c->fn[K_DEVHIST], std::max(blocks, 1u), 1, 1, 256, 1, 1, 0, c->stream, hv, nullptr));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(hist, dh, 4 * PT_HIST_BINS * 4, hipMemcpyDeviceToHost));
+    hipError_t e = hipMemsetAsync(dh, 0, 4 * PT_HIST_BINS * 4, c->stream);
+    if (e == hipSuccess)
+        e = hipModuleLaunchKernel(c->fn[K_DEVHIST], std::max(blocks, 1u), 1, 1, 256, 1, 1, 0, c->stream, hv, nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(hist, dh, 4 * PT_HIST_BINS * 4, hipMemcpyDeviceToHost);
     void* p = dh;
     freeBuf(c, p);
+    HIPCHK(c, e);
     return developInfo(c, d, info);
 }
 

@@ -480,6 +480,22 @@ napi_value Develop(napi_env env, napi_callback_info info) {
     return developInfo(env, di);
 }
 
+// develop16(h, settings (PtDevelop bytes), rgba16 (Uint16Array of w x h x 4)) -> [ev, log2Average, log2High, metered]
+napi_value Develop16(napi_env env, napi_callback_info info) {
+    napi_value a[3];
+    if (!args(env, info, 3, a)) return nullptr;
+    pt_ctx* ctx = ctxOf(env, a[0]);
+    if (!ctx) return nullptr;
+    void *sp = nullptr, *op = nullptr;
+    size_t sl = 0, ol = 0;
+    if (!bytesOf(env, a[1], &sp, &sl) || !bytesOf(env, a[2], &op, &ol)) return nullptr;
+    if (sl != sizeof(PtDevelop) || ol % 8) { napi_throw_range_error(env, nullptr, "settings not a PtDevelop, or rgba16 not 8 bytes a pixel"); return nullptr; }
+    PtDevelopInfo di;
+    int rc = pt_develop16(ctx, static_cast<const PtDevelop*>(sp), sl, static_cast<uint16_t*>(op), ol / 8, &di);
+    if (rc) return throwPt(env, ctx, rc);
+    return developInfo(env, di);
+}
+
 // developHistogram(h, settings (PtDevelop bytes), step, hist (Uint32Array of 4 x 256)) -> [ev, log2Average, log2High, metered]
 napi_value DevelopHistogram(napi_env env, napi_callback_info info) {
     napi_value a[4];
@@ -608,6 +624,7 @@ napi_value Init(napi_env env, napi_value exports) {
         {"opticsRun", nullptr, OpticsRun, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"develop", nullptr, Develop, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"developHistogram", nullptr, DevelopHistogram, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"develop16", nullptr, Develop16, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"readAccum", nullptr, ReadAccum, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"readStats", nullptr, ReadStats, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"opticsTrace", nullptr, OpticsTrace, nullptr, nullptr, nullptr, napi_enumerable, nullptr},

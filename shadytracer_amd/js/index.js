@@ -108,6 +108,8 @@ function createPathTracer(opts = {}) {
     RT.saveLight = (format, opts) => R.saveLight(format, opts)
     // the developed picture (exposure, tone curve, sRGB) and as a PNG file
     RT.develop = opts => R.develop(opts)
+    RT.develop16 = opts => R.develop16(opts)
+    RT.developHistogram = (opts, step) => R.developHistogram(opts, step)
     RT.readAccum = () => R.readAccum()
     RT.savePicture = (format, opts) => R.savePicture(format, opts)
     RT.readPixel = (...a) => R.readPixel(...a)

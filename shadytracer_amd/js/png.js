@@ -6,6 +6,11 @@
  * annex, polynomial 0xEDB88320). Each row takes the filter whose output has the least sum of absolute (signed) values.
  *
  *   const buf = encodePNG(rgba, width, height)      rgba: Uint8Array of width x height x 4, row 0 at the top
+ *   const b16 = encodePNG16(rgb, width, height)     rgb: Uint16Array of width x height x 3 (16 bits a channel, RGB)
+ *   insertText(buf, [[keyword, value], ...])         the same PNG with iTXt chunks (UTF-8 text) right after IHDR
+ *
+ * Both writers take { text } too: iTXt chunks (keyword, uncompressed, no language, UTF-8) after IHDR, which is how
+ * the reference's saves say how they were made (darkroom/export/png.js itxt, insertText).
  */
 const zlib = require('zlib')
 
@@ -44,17 +49,28 @@ function filterRow(row, prev, bpp, kind, out) {
     return out
 }
 
-function encodePNG(rgba, width, height, { level = 6 } = {}) {
-    if (rgba.length !== width * height * 4) throw new Error(`png: ${rgba.length} bytes for a ${width} x ${height} RGBA picture`)
-    const stride = width * 4
+// an iTXt chunk: keyword (1-79 Latin-1 bytes) and a NUL, compression flag 0, method 0, empty language tag and
+// translated keyword (each NUL-ended), then the text in UTF-8
+function itxt(keyword, text) {
+    const k = Buffer.from(String(keyword).slice(0, 79), 'latin1')
+    return chunk('iTXt', Buffer.concat([k, Buffer.alloc(5), Buffer.from(String(text), 'utf8')]))
+}
+function insertText(png, text = []) {
+    const at = 8 + 25                            // the signature and IHDR, always first
+    return Buffer.concat([Buffer.from(png.subarray(0, at)), ...text.map(([k, v]) => itxt(k, v)), Buffer.from(png.subarray(at))])
+}
+
+// rows of `bytes` (height rows of `stride` bytes, top first) filtered and deflated into a PNG of the given IHDR depth
+// and colour type; bpp: bytes a pixel, what the filters look back by
+function encodeRows(bytes, width, height, stride, bpp, depth, colourType, { level = 6, text = [] } = {}) {
     const raw = Buffer.alloc(height * (stride + 1))
     let prev = new Uint8Array(stride)
     const cand = new Uint8Array(stride), best = new Uint8Array(stride)
     for (let y = 0; y < height; y++) {
-        const row = rgba.subarray(y * stride, (y + 1) * stride)
+        const row = bytes.subarray(y * stride, (y + 1) * stride)
         let bestKind = 0, bestCost = Infinity
         for (let k = 0; k < 5; k++) {
-            filterRow(row, prev, 4, k, cand)
+            filterRow(row, prev, bpp, k, cand)
             let cost = 0
             for (let i = 0; i < stride; i++) cost += cand[i] < 128 ? cand[i] : 256 - cand[i]
             if (cost < bestCost) { bestCost = cost; bestKind = k; best.set(cand) }
@@ -66,9 +82,23 @@ function encodePNG(rgba, width, height, { level = 6 } = {}) {
     const ihdr = Buffer.alloc(13)
     ihdr.writeUInt32BE(width, 0)
     ihdr.writeUInt32BE(height, 4)
-    ihdr[8] = 8; ihdr[9] = 6; ihdr[10] = 0; ihdr[11] = 0; ihdr[12] = 0
+    ihdr[8] = depth; ihdr[9] = colourType; ihdr[10] = 0; ihdr[11] = 0; ihdr[12] = 0
     return Buffer.concat([Buffer.from([0x89, 0x50, 0x4E, 0x47, 0x0D, 0x0A, 0x1A, 0x0A]), chunk('IHDR', ihdr),
-        chunk('sRGB', Buffer.from([0])), chunk('IDAT', zlib.deflateSync(raw, { level })), chunk('IEND', Buffer.alloc(0))])
+        ...text.map(([k, v]) => itxt(k, v)), chunk('sRGB', Buffer.from([0])), chunk('IDAT', zlib.deflateSync(raw, { level })),
+        chunk('IEND', Buffer.alloc(0))])
 }
 
-module.exports = { encodePNG, crc32 }
+function encodePNG(rgba, width, height, opts = {}) {
+    if (rgba.length !== width * height * 4) throw new Error(`png: ${rgba.length} bytes for a ${width} x ${height} RGBA picture`)
+    return encodeRows(rgba, width, height, width * 4, 4, 8, 6, opts)
+}
+
+// 16 bits a channel, RGB (colour type 2): each sample big-endian, as PNG stores it
+function encodePNG16(rgb, width, height, opts = {}) {
+    if (rgb.length !== width * height * 3) throw new Error(`png16: ${rgb.length} samples for a ${width} x ${height} RGB picture`)
+    const bytes = new Uint8Array(rgb.length * 2)
+    for (let i = 0; i < rgb.length; i++) { bytes[2 * i] = rgb[i] >>> 8; bytes[2 * i + 1] = rgb[i] & 0xFF }
+    return encodeRows(bytes, width, height, width * 6, 6, 16, 2, opts)
+}
+
+module.exports = { encodePNG, encodePNG16, insertText, itxt, crc32 }

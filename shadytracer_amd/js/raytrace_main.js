@@ -228,11 +228,35 @@ function RAYTRACER(WG, _U, Q, scene) {
         const { white, black } = DARKROOM.clipShares(bins)
         return { bins, step, white, black, ev }
     }
-    // the developed picture as a PNG file (darkroom/export/png.js's role): a Buffer
+    // the development to 16 bits a channel: { rgba16 (Uint16Array, rows top first, alpha 65535), width, height, ev, ... };
+    // the dither is off (16 bits have no 8-bit steps to break)
+    R.develop16 = (opts = {}) => {
+        if (!current) throw new Error('RAYTRACER: no scene')
+        const rgba16 = new Uint16Array(U.width * U.height * 4)
+        const [ev, log2Average, log2High, metered] = A.develop16(h, developBytes(opts), rgba16)
+        return { rgba16, width: U.width, height: U.height, ev, log2Average, log2High, metered }
+    }
+    // THE DEVELOPED PICTURE AS A FILE (darkroom/export/run.js exportPicture's developed formats): a Buffer. 'png' 8 bits
+    // and 'png16' 16 bits a channel, each saying how it was made in iTXt chunks (Software; Comment: the settings and
+    // the exposure as JSON); 'jpeg' at opts.quality (0..1, 0.92 unless given), no text, as a browser's canvas writes it
+    const describe = (format, opts, ev) => [['Software', 'shadytracer_amd (MI355X path tracer)'],
+        ['Comment', JSON.stringify({ format, ev, settings: { ...opts, curves: opts.curves || undefined } })]]
     R.savePicture = (format = 'png', opts = {}) => {
-        if (format !== 'png') throw new Error(`RAYTRACER: picture format '${format}' (png)`)
-        const p = R.develop(opts)
-        return require('./png').encodePNG(p.rgba, p.width, p.height)
+        if (format === 'png') {
+            const p = R.develop(opts)
+            return require('./png').encodePNG(p.rgba, p.width, p.height, { text: describe(format, opts, p.ev) })
+        }
+        if (format === 'png16') {
+            const p = R.develop16(opts)
+            const rgb = new Uint16Array(p.width * p.height * 3)
+            for (let i = 0, j = 0; i < rgb.length; i += 3, j += 4) { rgb[i] = p.rgba16[j]; rgb[i + 1] = p.rgba16[j + 1]; rgb[i + 2] = p.rgba16[j + 2] }
+            return require('./png').encodePNG16(rgb, p.width, p.height, { text: describe(format, opts, p.ev) })
+        }
+        if (format === 'jpeg') {
+            const p = R.develop(opts)
+            return require('./jpeg').encodeJPEG(p.rgba, p.width, p.height, opts.quality !== undefined ? opts.quality : 0.92)
+        }
+        throw new Error(`RAYTRACER: picture format '${format}' (png, png16, jpeg; WebP is not written here)`)
     }
     R.imageStats = () => {
         const { acc, mom, n } = readback()

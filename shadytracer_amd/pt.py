@@ -186,6 +186,8 @@ SIGNATURES = {
     'pt_optics_trace': (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(PtOpticsScene), ctypes.c_uint32, ctypes.POINTER(PtOpticsResult),
                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     'pt_optics_error': (ctypes.c_char_p, []),
+    'pt_develop16': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PtDevelop), ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint16),
+                     ctypes.c_size_t, ctypes.POINTER(PtDevelopInfo)]),
     'pt_develop_histogram': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PtDevelop), ctypes.c_size_t, ctypes.c_uint32,
                              ctypes.POINTER(ctypes.c_uint32), ctypes.c_size_t, ctypes.POINTER(PtDevelopInfo)]),
     'pt_develop': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PtDevelop), ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint8),
@@ -449,6 +451,15 @@ class Engine:
         info = PtDevelopInfo()
         self._chk(self.lib.pt_develop(self.h, ctypes.byref(d), ctypes.sizeof(d), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
                                       self.width * self.height, ctypes.byref(info)))
+        return out, {k: getattr(info, k) for k, _ in PtDevelopInfo._fields_}
+
+    def develop16(self, tone='neutral', auto=True, ev=15.0, speed=1.0, headroom=2.0, flip=True, grade=None):
+        """develop() to 16 bits a channel (pt_develop16): (uint16 [h, w, 4] RGBA, alpha 65535, and the info); no dither."""
+        d = self._develop_settings(tone, auto, ev, speed, headroom, False, flip, grade)
+        out = np.zeros((self.height, self.width, 4), np.uint16)
+        info = PtDevelopInfo()
+        self._chk(self.lib.pt_develop16(self.h, ctypes.byref(d), ctypes.sizeof(d), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)),
+                                        self.width * self.height, ctypes.byref(info)))
         return out, {k: getattr(info, k) for k, _ in PtDevelopInfo._fields_}
 
     def develop_histogram(self, tone='neutral', auto=True, ev=15.0, speed=1.0, headroom=2.0, dither=False, grade=None, step=1):

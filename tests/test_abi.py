@@ -17,7 +17,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared():
     text = open(os.path.join(ROOT, 'include', 'pt.h')).read()
     text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
-    return sorted(set(re.findall(r'\b(pt_[a-z_]+)\s*\(', text)))
+    return sorted(set(re.findall(r'\b(pt_[a-z0-9_]+)\s*\(', text)))
 
 
 def have_gpu():
@@ -47,7 +47,7 @@ def test_addon_loads_and_packs_the_same_structures():
                                     stdout=subprocess.PIPE, text=True).stdout)
     s = out['sizes']
     assert s[0] == s[1] == ctypes.sizeof(pt.PtUniforms) and s[2] == s[3] and s[4] == s[5]
-    assert out['addonKeys'] == sorted(['abiVersion', 'autotune', 'clearAccum', 'compileScene', 'create', 'destroy', 'develop', 'developHistogram', 'foldParams', 'opticsRun',
+    assert out['addonKeys'] == sorted(['abiVersion', 'autotune', 'clearAccum', 'compileScene', 'create', 'destroy', 'develop', 'develop16', 'developHistogram', 'foldParams', 'opticsRun',
                                        'opticsTrace',
                                        'readAccum', 'readStats', 'setParams', 'setScene', 'setTuning', 'setUniforms', 'structSize',
                                        'trace'])

@@ -110,8 +110,9 @@ def decode_png(data):
             idat += body
         pos += 12 + n
     w, h, depth, ctype = ihdr[:4]
-    assert depth == 8 and ctype in (2, 6) and kinds[-1] == b'IEND'
-    ch = 4 if ctype == 6 else 3
+    assert depth in (8, 16) and ctype in (2, 6) and kinds[-1] == b'IEND'
+    # filters work on bytes, looking back a whole pixel: bpp bytes
+    ch = (4 if ctype == 6 else 3) * depth // 8
     raw = zlib.decompress(idat)
     out = np.zeros((h, w * ch), np.int32)
     prev = np.zeros(w * ch, np.int32)
@@ -138,7 +139,27 @@ def decode_png(data):
             cur[i] = (row[i] + p) & 0xFF
         out[y] = cur
         prev = cur
+    if depth == 16:                                      # samples big-endian
+        b = out.reshape(h, w * ch // 2, 2)
+        return (b[..., 0] * 256 + b[..., 1]).reshape(h, w, ch // 2).astype(np.uint16), kinds
     return out.reshape(h, w, ch).astype(np.uint8), kinds
+
+
+def png_text(data):
+    """The iTXt chunks of a PNG: {keyword: text}, each read as the specification lays it out (keyword NUL, compression
+    flag, method, language tag NUL, translated keyword NUL, UTF-8 text)."""
+    pos, out = 8, {}
+    while pos < len(data):
+        n, = struct.unpack('>I', data[pos:pos + 4])
+        kind, body = data[pos + 4:pos + 8], data[pos + 8:pos + 8 + n]
+        if kind == b'iTXt':
+            key, rest = body.split(b'\0', 1)
+            assert rest[0] == 0 and rest[1] == 0                 # uncompressed
+            lang, rest = rest[2:].split(b'\0', 1)
+            _, text = rest.split(b'\0', 1)
+            out[key.decode('latin1')] = text.decode('utf8')
+        pos += 12 + n
+    return out
 
 
 @pytest.mark.parametrize('filters', ['adaptive', 0, 1, 2, 3, 4])

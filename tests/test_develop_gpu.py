@@ -76,7 +76,9 @@ def test_node_saves_a_png(tmp_path):
     say('case: node tests/js/develop_png.js, plane-c1 through createPathTracer, savePicture')
     out = tmp_path / 'c1.png'
     accf = tmp_path / 'acc.f64'
-    r = subprocess.run(['node', os.path.join(ROOT, 'tests', 'js', 'develop_png.js'), str(out), str(accf)], stdout=subprocess.PIPE,
+    out16, jpg = tmp_path / 'c1-16.png', tmp_path / 'c1.jpg'
+    r = subprocess.run(['node', os.path.join(ROOT, 'tests', 'js', 'develop_png.js'), str(out), str(accf), str(out16), str(jpg)],
+                       stdout=subprocess.PIPE,
                        stderr=subprocess.PIPE, text=True, timeout=180)
     assert r.returncode == 0, r.stderr
     js = json.loads(r.stdout)
@@ -84,9 +86,23 @@ def test_node_saves_a_png(tmp_path):
     img, kinds = T.decode_png(out.read_bytes())
     assert img.shape == (js['height'], js['width'], 4) and b'sRGB' in kinds
     acc = np.fromfile(str(accf), np.float64).reshape(js['height'], js['width'], 4)
-    want, _, ev = D.develop(acc, tone=1)
+    want, v, ev = D.develop(acc, tone=1)
     assert js['ev'] == pytest.approx(ev, abs=2e-3)
     assert np.abs(img[..., :3].astype(int) - want.astype(int)).max() <= 1
+    # the 16-bit file: the same picture to 1/65535 (FP32 stages against FP64), saying how it was made
+    data16 = out16.read_bytes()
+    img16, _ = T.decode_png(data16)
+    assert img16.dtype == np.uint16 and img16.shape == (js['height'], js['width'], 3)
+    assert np.abs(img16.astype(float) - v * 257.0).max() < 0.002 * 65535
+    assert json.loads(T.png_text(data16)['Comment'])['format'] == 'png16'
+    # the JPEG: a JFIF file that decodes close to the picture
+    assert jpg.read_bytes()[:4] == b'\xff\xd8\xff\xe0'
+    try:
+        from PIL import Image
+    except ImportError:
+        return
+    got = np.asarray(Image.open(str(jpg)).convert('RGB')).astype(float)
+    assert np.abs(got - want).mean() < 3.0
 
 
 # ---- the darkroom
@@ -125,6 +141,21 @@ def test_darkroom_matches_the_checker(engine, name):
     # the stages did something: the graded picture is not the plain one
     plain, _, _ = D.develop(acc, tone=tone if tone != 7 else 6, ev=9.0)
     assert np.abs(want.astype(int) - plain.astype(int)).mean() > 1.0
+
+
+def test_develop16_matches_the_checker(engine):
+    say('case: develop16 room-c2 96x54 (16 samples), finish grade, dither asked and left off')
+    acc = render(engine, 'room-c2', 16, width=96, height=54)
+    grade = darkroom_words(dict(GRADES['finish'], dither=True))
+    rgba16, _ = engine.develop16(tone=1, auto=False, ev=9.0, grade=grade)
+    rgba8, _ = engine.develop(tone=1, auto=False, ev=9.0, grade=dict(grade, lightOn=grade['lightOn'] & ~512))
+    _, v, _ = D.develop(acc, tone=1, ev=9.0, grade=dict(grade, lightOn=grade['lightOn'] & ~512))
+    assert rgba16.dtype == np.uint16 and np.all(rgba16[..., 3] == 65535)
+    err = np.abs(rgba16[..., :3].astype(float) - v * 257.0)
+    say('  16-bit error max', err.max(), 'mean', err.mean())
+    assert err.max() < 0.002 * 65535 and err.mean() < 8
+    # the 8-bit picture is the 16-bit one rounded, but at rounding edges
+    assert np.abs(np.round(rgba16[..., :3] / 257.0) - rgba8[..., :3]).max() <= 1
 
 
 def test_false_colour_views_match_the_checker(engine):
