@@ -6,7 +6,8 @@
 //
 // Follows lib/pathtracer/darkroom/grade.js (lightUniforms :231-271, whiteBalanceMatrix :79-87, whiteFromRGB :90-100,
 // cdlFromWheels :193-201), darkroom/curves.js (normalizeCurve :34-43, curveFunction :55-81, bakeLut :84-96) and
-// core/colorimetry.js (the Rec.709 matrix, Krystek's locus, CCT and Duv) -- each from the published method it names:
+// core/colorimetry.js (the Rec.709 matrix, Krystek's locus, CCT and Duv) -- each from the published method it names
+// (the CCT search is this package's own, as in the optics kernels' colour summary):
 // IEC 61966-2-1's matrix, Krystek (1985), Bradford's cone response, Fritsch and Carlson (1980), the ASC CDL.
 
 const LUMA = [0.2126, 0.7152, 0.0722]
@@ -34,41 +35,50 @@ const xyToUv = (x, y) => { const d = 12 * y - 2 * x + 3; return [4 * x / d, 6 * 
 const uvToXy = (u, v) => { const d = 2 * u - 8 * v + 4; return [3 * u / d, 2 * v / d] }
 
 const CCT_MIN = 1000, CCT_MAX = 15000
-// the Planckian locus in CIE 1960 uv: Krystek's rational approximation (1985), 1000-15000 K
-function planckUv(T) {
+// the Planckian locus in CIE 1960 uv by Krystek's rational fit (1985), 1000-15000 K, each coordinate a ratio of
+// quadratics in T, with its direction dP/dT from the ratios' derivatives (N'D - N D') / D^2
+const KRYSTEK_U = [[0.860117757, 1.54118254e-4, 1.28641212e-7], [1, 8.42420234e-4, 7.08145163e-7]]
+const KRYSTEK_V = [[0.317398726, 4.22806245e-5, 4.20481691e-8], [1, -2.89741816e-5, 1.61456053e-7]]
+function ratio([N, D], t) {
+    const n = N[0] + t * (N[1] + t * N[2]), d = D[0] + t * (D[1] + t * D[2])
+    return [n / d, ((N[1] + 2 * t * N[2]) * d - n * (D[1] + 2 * t * D[2])) / (d * d)]
+}
+// { u, v, su, sv }: the locus point at T (held to the fit's range) and its tangent
+function locus(T) {
     const t = Math.min(CCT_MAX, Math.max(CCT_MIN, T))
-    return [(0.860117757 + 1.54118254e-4 * t + 1.28641212e-7 * t * t) / (1 + 8.42420234e-4 * t + 7.08145163e-7 * t * t),
-            (0.317398726 + 4.22806245e-5 * t + 4.20481691e-8 * t * t) / (1 - 2.89741816e-5 * t + 1.61456053e-7 * t * t)]
+    const [u, su] = ratio(KRYSTEK_U, t), [v, sv] = ratio(KRYSTEK_V, t)
+    return { u, v, su, sv }
 }
-// the unit normal of the locus at T, on its green side (positive Duv, ANSI C78.377): the tangent by a central
-// difference, turned a quarter
-function locusNormal(T) {
-    const h = Math.max(1e-3, T * 1e-5)
-    const a = planckUv(T - h), b = planckUv(T + h)
-    const du = b[0] - a[0], dv = b[1] - a[1], l = Math.hypot(du, dv) || 1
-    return [dv / l, -du / l]
-}
+const planckUv = T => { const p = locus(T); return [p.u, p.v] }
+// a CCT and Duv as xy: the locus point moved duv along the unit normal, the tangent turned a quarter clockwise --
+// towards larger v, the green side, for duv > 0 (ANSI C78.377's sign)
 function cctDuvToXy(T, duv = 0) {
-    const [u, v] = planckUv(T), [nu, nv] = locusNormal(T)
-    return uvToXy(u + duv * nu, v + duv * nv)
+    const p = locus(T), l = Math.hypot(p.su, p.sv) || 1
+    return uvToXy(p.u + duv * p.sv / l, p.v - duv * p.su / l)
 }
-// xy to { cct, duv, inRange }: the nearest point of the locus in uv, bracketed by a scan evenly spaced in mired and
-// narrowed by golden sections; inRange false when that point is an end of the locus
+// xy to { cct, duv, inRange }: the foot of the perpendicular from (u, v) to the locus, where g(T) = (P(T) - uv) . P'(T)
+// changes sign; g is sampled at 96 temperatures even in mired, every sign change is closed by bisection, and the
+// nearest of those feet and the range's two ends is the CCT. inRange is false when that is an end.
 function xyToCctDuv(x, y) {
     const [u, v] = xyToUv(x, y)
-    const dist = mired => { const p = planckUv(1e6 / mired); return (p[0] - u) ** 2 + (p[1] - v) ** 2 }
-    const lo0 = 1e6 / CCT_MAX, hi0 = 1e6 / CCT_MIN, N = 400
-    let k = 0, best = Infinity
-    for (let i = 0; i <= N; i++) { const d = dist(lo0 + (hi0 - lo0) * i / N); if (d < best) { best = d; k = i } }
-    let a = lo0 + (hi0 - lo0) * Math.max(0, k - 1) / N, b = lo0 + (hi0 - lo0) * Math.min(N, k + 1) / N
-    const g = (Math.sqrt(5) - 1) / 2
-    let c = b - g * (b - a), d = a + g * (b - a), fc = dist(c), fd = dist(d)
-    for (let it = 0; it < 100; it++) {
-        if (fc < fd) { b = d; d = c; fd = fc; c = b - g * (b - a); fc = dist(c) } else { a = c; c = d; fc = fd; d = a + g * (b - a); fd = dist(d) }
+    const g = T => { const p = locus(T); return (p.u - u) * p.su + (p.v - v) * p.sv }
+    const d2 = T => { const p = locus(T); return (p.u - u) ** 2 + (p.v - v) ** 2 }
+    let bestT = CCT_MAX, bestD = d2(CCT_MAX)
+    if (d2(CCT_MIN) < bestD) { bestT = CCT_MIN; bestD = d2(CCT_MIN) }
+    const hot = 1e6 / CCT_MAX, cold = 1e6 / CCT_MIN, K = 96
+    let Tp = CCT_MAX, gp = g(Tp)
+    for (let k = 1; k <= K; k++) {
+        const Tk = 1e6 / (hot + (cold - hot) * k / K), gk = g(Tk)
+        if (gp > 0 && gk <= 0) {                 // walking towards cold, T falls: the foot lies between
+            let lo = Tk, hi = Tp
+            for (let it = 0; it < 64; it++) { const mid = 0.5 * (lo + hi); if (g(mid) > 0) hi = mid; else lo = mid }
+            const T = 0.5 * (lo + hi), dd = d2(T)
+            if (dd < bestD) { bestD = dd; bestT = T }
+        }
+        Tp = Tk; gp = gk
     }
-    const T = 1e6 / ((a + b) / 2)
-    const p = planckUv(T), n = locusNormal(T)
-    return { cct: T, duv: (u - p[0]) * n[0] + (v - p[1]) * n[1], inRange: T > CCT_MIN * 1.001 && T < CCT_MAX * 0.999 }
+    const p = locus(bestT), l = Math.hypot(p.su, p.sv)
+    return { cct: bestT, duv: ((u - p.u) * p.sv - (v - p.v) * p.su) / l, inRange: bestT > CCT_MIN * 1.001 && bestT < CCT_MAX * 0.999 }
 }
 
 // D65 as the RGB matrix has it: the chromaticity of Rec.709's (1, 1, 1), and where it sits on the locus
