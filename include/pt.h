@@ -201,6 +201,16 @@ int pt_develop(pt_ctx* ctx, const PtDevelop* d, size_t dsize, uint8_t* rgba, siz
  * developUniforms' `float` :73-77; darkroom/export/run.js :30-41 rounding to 16 bits). */
 int pt_develop16(pt_ctx* ctx, const PtDevelop* d, size_t dsize, uint16_t* rgba16, size_t npx, PtDevelopInfo* info);
 
+/* A view of the developed picture, outW x outH 8-bit RGBA (npx = outW x outH, rows top first with d->flipY):
+ * view = {cx, cy, sx, sy}, the picture point (in picture pixels, y up) at the output's centre and the picture pixels
+ * one output pixel spans. Each output pixel averages ceil(max(sx, sy)) squared taps (1..8 a side) over its
+ * footprint, the backdrop grey 0.09 outside the picture; clipWarn stripes what clips (red at white, blue at black).
+ * {w/2, h/2, w/outW, h/outH} is the whole picture, a thumbnail at a smaller output.
+ * Replaces the develop pass's view (lib/pathtracer/core/rendertoscreen.js develop_prog :555-593; develop.js boxForSpan
+ * :61 and viewFill :67; darkroom/view.js toUniform :73). */
+int pt_develop_view(pt_ctx* ctx, const PtDevelop* d, size_t dsize, const float* view, uint32_t outW, uint32_t outH, uint32_t clipWarn,
+                    uint8_t* rgba, size_t npx, PtDevelopInfo* info);
+
 /* What pt_develop would write, counted: every `step`-th pixel each way developed as pt_develop does it (metering
  * included), hist[0..255] red, [256..511] green, [512..767] blue, [768..1023] the Rec.709 luma of the display values,
  * bin round(v 255); nbins = 4 x PT_HIST_BINS. A false-colour view (d->measureMode) is refused.

@@ -344,3 +344,37 @@ def histogram(acc, tone=1, ev=None, headroom=2.0, grade=None, step=1):
     c = develop_pixels(sub[..., :3], 1.0 / (EV_CLIP * 2.0 ** ev), px, py, W, H, tone, grade)
     chans = [c[..., 0], c[..., 1], c[..., 2], np.clip(c @ LUMA, 0.0, 1.0)]
     return np.stack([np.bincount((np.clip(x, 0, 1) * 255 + 0.5).astype(int).ravel(), minlength=256) for x in chans])
+
+
+def develop_view(acc, view, out_w, out_h, tone=1, ev=None, headroom=2.0, grade=None, clip_warn=False):
+    """rendertoscreen.js develop_prog's frag as pt_develop_view runs it: an out_h x out_w view (8-bit RGB, the top row
+    first, and the values x 255 before rounding) of the picture through view = (cx, cy, sx, sy), picture pixels, y up;
+    each output pixel the mean of box x box taps over its footprint (box = ceil(max(sx, sy)), 1..8), 0.09 outside the
+    picture, clip stripes (red at white, blue at black) on the diagonal bands when clip_warn."""
+    if ev is None:
+        ev = ae_update(ae_histogram(acc), headroom)[0]
+    g = dict(GRADE_DEFAULTS, **(grade or {}))
+    H, W = acc.shape[:2]
+    py, px = np.mgrid[0:H, 0:W].astype(float)
+    dev = develop_pixels(acc[..., :3], 1.0 / (EV_CLIP * 2.0 ** ev), px, py, W, H, tone, g)
+    cx0, cy0, sx, sy = [float(v) for v in view]
+    box = int(min(8, max(1, np.ceil(max(sx, sy) - 1e-9))))
+    oy, ox = np.mgrid[0:out_h, 0:out_w].astype(float)
+    fx, fy = ox + 0.5, oy + 0.5
+    cx, cy = cx0 + (fx - 0.5 * out_w) * sx, cy0 + (fy - 0.5 * out_h) * sy
+    inside = (cx >= 0) & (cy >= 0) & (cx < W) & (cy < H)
+    stripe = np.mod((fx + fy) / 8.0 + 0.0625, 1.0) < 0.5
+    total = np.zeros((out_h, out_w, 3))
+    for b in range(box * box):
+        qx = cx - 0.5 * sx + ((b % box) + 0.5) / box * sx
+        qy = cy - 0.5 * sy + ((b // box) + 0.5) / box * sy
+        ix = np.clip(qx, 0, W - 1).astype(int)
+        iy = np.clip(qy, 0, H - 1).astype(int)
+        c = dev[iy, ix]
+        if clip_warn and not g['measureMode']:
+            m = c.max(axis=-1)
+            c = np.where((stripe & (m >= 254.5 / 255))[..., None], [1.0, 0.0, 0.0], c)
+            c = np.where((stripe & (m < 254.5 / 255) & (m <= 0.5 / 255))[..., None], [0.0, 0.25, 1.0], c)
+        total += c
+    v = np.where(inside[..., None], total / (box * box), 0.09) * 255.0
+    return np.floor(np.minimum(v + 0.5, 255.0))[::-1].astype(np.uint8), v[::-1]

@@ -404,6 +404,51 @@ extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_develop16(const PtDev*
                           (unsigned short)fminf(c.z * 65535.0f + 0.5f, 65535.0f), 65535);
 }
 
+// A VIEW OF THE DEVELOPED PICTURE (rendertoscreen.js develop_prog's frag): an output of outW x outH pixels looking at
+// the picture through view = (cx, cy, sx, sy) -- the picture point at the output's centre and the picture pixels one
+// output pixel spans, both in picture pixels with y up. Each output pixel averages box x box taps spread over its
+// footprint (a preview smaller than the picture does not stand one pixel for its neighbourhood; at a span of 1 or
+// less, one tap, the nearest pixel); outside the picture it is the backdrop grey 0.09. clipWarn stripes diagonally
+// (8 output pixels a period) a tap at white on any channel red and one black on all blue, in the picture view only.
+// One output pixel per thread, rows top first when s->flipY.
+extern "C" __global__ __launch_bounds__(PT_BLOCK) void pt_develop_view(const PtDev* __restrict__ D, uchar4* __restrict__ out,
+                                                                      const float* __restrict__ state, const PtDevelop* __restrict__ s,
+                                                                      float4 view, unsigned int outW, unsigned int outH,
+                                                                      unsigned int box, unsigned int clipWarn) {
+    const unsigned int j = blockIdx.x * PT_BLOCK + threadIdx.x;
+    if (j >= outW * outH) return;
+    const unsigned int W = D->u.width, H = D->u.height;
+    const unsigned int ox = j % outW, oy = j / outW;                     // oy counted from the bottom, as the picture
+    const float fx = (float)ox + 0.5f, fy = (float)oy + 0.5f;
+    const float cx = view.x + (fx - 0.5f * (float)outW) * view.z, cy = view.y + (fy - 0.5f * (float)outH) * view.w;
+    V3 sum = v3s(0.0f);
+    if (cx < 0.0f || cy < 0.0f || cx >= (float)W || cy >= (float)H) {
+        sum = v3s(0.09f);
+    } else {
+        const float scale = developScale(state, s);
+        const bool stripe = (fx + fy) / 8.0f + 0.0625f - floorf((fx + fy) / 8.0f + 0.0625f) < 0.5f;
+        const float n = (float)box;
+        for (unsigned int b = 0; b < box * box; ++b) {
+            const float qx = cx - 0.5f * view.z + ((float)(b % box) + 0.5f) / n * view.z;
+            const float qy = cy - 0.5f * view.w + ((float)(b / box) + 0.5f) / n * view.w;
+            const unsigned int px = (unsigned int)fminf(fmaxf(qx, 0.0f), (float)W - 1.0f);
+            const unsigned int py = (unsigned int)fminf(fmaxf(qy, 0.0f), (float)H - 1.0f);
+            bool sampled;
+            V3 c = developPixel(accumMean(D, py * W + px, sampled), scale, px, py, W, H, s);
+            if (clipWarn && s->measureMode == 0u && stripe) {
+                const float m = fmaxf(c.x, fmaxf(c.y, c.z));
+                if (m >= 254.5f / 255.0f) c = v3(1.0f, 0.0f, 0.0f);
+                else if (m <= 0.5f / 255.0f) c = v3(0.0f, 0.25f, 1.0f);
+            }
+            sum = sum + c;
+        }
+        sum = sum * (1.0f / (n * n));
+    }
+    const unsigned int o = (s->flipY ? (outH - 1u - oy) : oy) * outW + ox;
+    out[o] = make_uchar4((unsigned char)fminf(sum.x * 255.0f + 0.5f, 255.0f), (unsigned char)fminf(sum.y * 255.0f + 0.5f, 255.0f),
+                         (unsigned char)fminf(sum.z * 255.0f + 0.5f, 255.0f), 255);
+}
+
 // THE DEVELOPED PICTURE'S HISTOGRAM (rendertoscreen.js develop_histogram_prog): every `step`-th pixel each way developed
 // as pt_develop does it and counted, 256 bins each of R, G, B and the Rec.709 luma of the display values (bin
 // round(v 255)). A block's 1024 bins in LDS, one atomic a non-empty bin to `hist`; every thread reaches both barriers.

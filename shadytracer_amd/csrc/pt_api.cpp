@@ -28,10 +28,11 @@ const char* kKernelNames[] = {"pt_genrays", "pt_march", "pt_shade", "pt_occlude"
                               "pt_bin_count", "pt_bin_scan", "pt_bin_scatter", "pt_sppm_clear", "pt_photon_gen", "pt_surface",
                               "pt_guide_infer", "pt_guide_eval", "pt_guide_collect", "pt_guide_grad", "pt_guide_adam", "pt_guide_commit",
                               "pt_flush", "pt_opt_gen", "pt_opt_shade", "pt_opt_flush", "pt_ae_hist", "pt_ae_update",
-                              "pt_develop", "pt_develop_hist", "pt_develop16"};
+                              "pt_develop", "pt_develop_hist", "pt_develop16",
+                              "pt_develop_view"};
 enum { K_GENRAYS, K_MARCH, K_SHADE, K_OCCLUDE, K_RESOLVE, K_MFLUT, K_MFAVG, K_PICK, K_BINCOUNT, K_BINSCAN, K_BINSCATTER,
        K_SPPMCLEAR, K_PHOTONGEN, K_SURFACE, K_GINFER, K_GEVAL, K_GCOLLECT, K_GGRAD, K_GADAM, K_GCOMMIT, K_FLUSH, K_OPTGEN, K_OPTSHADE,
-       K_OPTFLUSH, K_AEHIST, K_AEUPDATE, K_DEVELOP, K_DEVHIST, K_DEVELOP16, K_COUNT };
+       K_OPTFLUSH, K_AEHIST, K_AEUPDATE, K_DEVELOP, K_DEVHIST, K_DEVELOP16, K_DEVVIEW, K_COUNT };
 
 // the guide network's sizes (device/pt_guide.h)
 const unsigned int kGuideNP = 16169, kGuideWT = 17408, kGuideBias = 256, kGuideParts = 256, kGuideMaxSteps = 64;
@@ -1285,6 +1286,33 @@ int pt_develop16(pt_ctx* c, const PtDevelop* d, size_t dsize, uint16_t* rgba16, 
     freeBuf(c, p);
     HIPCHK(c, e);
     return developInfo(c, &s, info);
+}
+
+int pt_develop_view(pt_ctx* c, const PtDevelop* d, size_t dsize, const float* view, uint32_t outW, uint32_t outH, uint32_t clipWarn,
+                    uint8_t* rgba, size_t npx, PtDevelopInfo* info) {
+    if (!c) return fail(c, PT_ERR_ARG, "ctx is NULL");
+    if (!rgba || !view) return fail(c, PT_ERR_ARG, "view or rgba is NULL");
+    if (outW == 0 || outH == 0 || outW > 16384 || outH > 16384) return fail(c, PT_ERR_ARG, "output %u x %u not in 1..16384", outW, outH);
+    if (npx != size_t(outW) * outH) return fail(c, PT_ERR_ARG, "npx %zu, the output has %zu pixels", npx, size_t(outW) * outH);
+    for (int k = 0; k < 4; ++k)
+        if (!std::isfinite(view[k])) return fail(c, PT_ERR_ARG, "view[%d] is not finite", k);
+    if (!(view[2] > 0.0f && view[3] > 0.0f)) return fail(c, PT_ERR_ARG, "the view's span (%g, %g) must be > 0", view[2], view[3]);
+    int rc;
+    if ((rc = developBegin(c, d, dsize))) return rc;
+    // taps a side: the larger span rounded up, 1 at full size or closer, at most 8
+    const uint32_t box = (uint32_t)std::min(8.0, std::max(1.0, std::ceil(std::max(double(view[2]), double(view[3])) - 1e-9)));
+    const size_t px = size_t(outW) * outH;
+    uint8_t* dout = nullptr;
+    if ((rc = allocBuf(c, dout, px * 4))) return rc;
+    float4 v = make_float4(view[0], view[1], view[2], view[3]);
+    void* vv[] = {&c->dev, &dout, &c->aeState, &c->devSet, &v, &outW, &outH, (void*)&box, &clipWarn};
+    hipError_t e = hipModuleLaunchKernel(c->fn[K_DEVVIEW], (unsigned int)((px + 255) / 256), 1, 1, 256, 1, 1, 0, c->stream, vv, nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(rgba, dout, px * 4, hipMemcpyDeviceToHost);
+    void* p = dout;
+    freeBuf(c, p);
+    HIPCHK(c, e);
+    return developInfo(c, d, info);
 }
 
 int pt_develop_histogram(pt_ctx* c, const PtDevelop* d, size_t dsize, uint32_t step, uint32_t* hist, size_t nbins, PtDevelopInfo* info) {

@@ -496,6 +496,33 @@ napi_value Develop16(napi_env env, napi_callback_info info) {
     return developInfo(env, di);
 }
 
+// developView(h, settings (PtDevelop bytes), view (Float32Array of 4), outW, outH, clipWarn, rgba (Uint8Array of
+// outW x outH x 4)) -> [ev, log2Average, log2High, metered]
+napi_value DevelopView(napi_env env, napi_callback_info info) {
+    napi_value a[7];
+    if (!args(env, info, 7, a)) return nullptr;
+    pt_ctx* ctx = ctxOf(env, a[0]);
+    if (!ctx) return nullptr;
+    void *sp = nullptr, *vp = nullptr, *op = nullptr;
+    size_t sl = 0, vl = 0, ol = 0;
+    uint32_t w = 0, hh = 0, clip = 0;
+    if (!bytesOf(env, a[1], &sp, &sl) || !bytesOf(env, a[2], &vp, &vl) || !bytesOf(env, a[6], &op, &ol)) return nullptr;
+    if (napi_get_value_uint32(env, a[3], &w) != napi_ok || napi_get_value_uint32(env, a[4], &hh) != napi_ok ||
+        napi_get_value_uint32(env, a[5], &clip) != napi_ok) {
+        napi_throw_type_error(env, nullptr, "outW, outH, clipWarn: numbers");
+        return nullptr;
+    }
+    if (sl != sizeof(PtDevelop) || vl != 16 || ol % 4) {
+        napi_throw_range_error(env, nullptr, "settings not a PtDevelop, view not 4 floats, or rgba not 4 bytes a pixel");
+        return nullptr;
+    }
+    PtDevelopInfo di;
+    int rc = pt_develop_view(ctx, static_cast<const PtDevelop*>(sp), sl, static_cast<const float*>(vp), w, hh, clip,
+                             static_cast<uint8_t*>(op), ol / 4, &di);
+    if (rc) return throwPt(env, ctx, rc);
+    return developInfo(env, di);
+}
+
 // developHistogram(h, settings (PtDevelop bytes), step, hist (Uint32Array of 4 x 256)) -> [ev, log2Average, log2High, metered]
 napi_value DevelopHistogram(napi_env env, napi_callback_info info) {
     napi_value a[4];
@@ -625,6 +652,7 @@ napi_value Init(napi_env env, napi_value exports) {
         {"develop", nullptr, Develop, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"developHistogram", nullptr, DevelopHistogram, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"develop16", nullptr, Develop16, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"developView", nullptr, DevelopView, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"readAccum", nullptr, ReadAccum, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"readStats", nullptr, ReadStats, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"opticsTrace", nullptr, OpticsTrace, nullptr, nullptr, nullptr, napi_enumerable, nullptr},

@@ -110,6 +110,7 @@ function createPathTracer(opts = {}) {
     RT.develop = opts => R.develop(opts)
     RT.develop16 = opts => R.develop16(opts)
     RT.developHistogram = (opts, step) => R.developHistogram(opts, step)
+    RT.developView = (opts, out) => R.developView(opts, out)
     RT.readAccum = () => R.readAccum()
     RT.savePicture = (format, opts) => R.savePicture(format, opts)
     RT.readPixel = (...a) => R.readPixel(...a)

@@ -219,6 +219,18 @@ function RAYTRACER(WG, _U, Q, scene) {
         const [ev, log2Average, log2High, metered] = A.develop(h, buf, rgba)
         return { rgba, width: U.width, height: U.height, ev, log2Average, log2High, metered }
     }
+    // A VIEW OF THE DEVELOPED PICTURE (the develop pass's `view`: the darkroom viewer's pan and zoom, a thumbnail):
+    // { rgba (outW x outH, rows top first), width, height, ev }. view = [cx, cy, sx, sy] in picture pixels, y up (the
+    // whole picture by default, develop.js viewFill); each output pixel averages the taps of its footprint; clipWarn
+    // stripes what clips
+    R.developView = (opts = {}, { width, height, view, clipWarn = false } = {}) => {
+        if (!current) throw new Error('RAYTRACER: no scene')
+        const w = width || U.width, hh = height || U.height
+        const v = new Float32Array(view || [U.width / 2, U.height / 2, U.width / w, U.height / hh])
+        const rgba = new Uint8Array(w * hh * 4)
+        const [ev] = A.developView(h, developBytes(opts), v, w, hh, clipWarn ? 1 : 0, rgba)
+        return { rgba, width: w, height: hh, ev }
+    }
     // THE DEVELOPED PICTURE'S HISTOGRAM (develop.js's darkroom histogram): what R.develop(opts) would write, counted
     // over every `step`-th pixel each way: { bins (Uint32Array: 256 each of R, G, B, luma), step, white, black, ev }
     R.developHistogram = (opts = {}, step = DARKROOM.histogramStep(U.width, U.height)) => {

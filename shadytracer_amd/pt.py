@@ -186,6 +186,9 @@ SIGNATURES = {
     'pt_optics_trace': (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(PtOpticsScene), ctypes.c_uint32, ctypes.POINTER(PtOpticsResult),
                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     'pt_optics_error': (ctypes.c_char_p, []),
+    'pt_develop_view': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PtDevelop), ctypes.c_size_t, ctypes.POINTER(ctypes.c_float),
+                        ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint8), ctypes.c_size_t,
+                        ctypes.POINTER(PtDevelopInfo)]),
     'pt_develop16': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PtDevelop), ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint16),
                      ctypes.c_size_t, ctypes.POINTER(PtDevelopInfo)]),
     'pt_develop_histogram': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PtDevelop), ctypes.c_size_t, ctypes.c_uint32,
@@ -460,6 +463,18 @@ class Engine:
         info = PtDevelopInfo()
         self._chk(self.lib.pt_develop16(self.h, ctypes.byref(d), ctypes.sizeof(d), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)),
                                         self.width * self.height, ctypes.byref(info)))
+        return out, {k: getattr(info, k) for k, _ in PtDevelopInfo._fields_}
+
+    def develop_view(self, view, out_w, out_h, clip_warn=False, tone='neutral', auto=True, ev=15.0, speed=1.0, headroom=2.0,
+                     dither=False, flip=True, grade=None):
+        """A view of the developed picture (pt_develop_view): view = (cx, cy, sx, sy) in picture pixels, y up (the
+        whole picture at the output: (w/2, h/2, w/out_w, h/out_h)); (uint8 [out_h, out_w, 4], info)."""
+        d = self._develop_settings(tone, auto, ev, speed, headroom, dither, flip, grade)
+        v = (ctypes.c_float * 4)(*[float(x) for x in view])
+        out = np.zeros((out_h, out_w, 4), np.uint8)
+        info = PtDevelopInfo()
+        self._chk(self.lib.pt_develop_view(self.h, ctypes.byref(d), ctypes.sizeof(d), v, out_w, out_h, int(bool(clip_warn)),
+                                           out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), out.shape[0] * out.shape[1], ctypes.byref(info)))
         return out, {k: getattr(info, k) for k, _ in PtDevelopInfo._fields_}
 
     def develop_histogram(self, tone='neutral', auto=True, ev=15.0, speed=1.0, headroom=2.0, dither=False, grade=None, step=1):

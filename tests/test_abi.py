@@ -47,7 +47,7 @@ def test_addon_loads_and_packs_the_same_structures():
                                     stdout=subprocess.PIPE, text=True).stdout)
     s = out['sizes']
     assert s[0] == s[1] == ctypes.sizeof(pt.PtUniforms) and s[2] == s[3] and s[4] == s[5]
-    assert out['addonKeys'] == sorted(['abiVersion', 'autotune', 'clearAccum', 'compileScene', 'create', 'destroy', 'develop', 'develop16', 'developHistogram', 'foldParams', 'opticsRun',
+    assert out['addonKeys'] == sorted(['abiVersion', 'autotune', 'clearAccum', 'compileScene', 'create', 'destroy', 'develop', 'develop16', 'developHistogram', 'developView', 'foldParams', 'opticsRun',
                                        'opticsTrace',
                                        'readAccum', 'readStats', 'setParams', 'setScene', 'setTuning', 'setUniforms', 'structSize',
                                        'trace'])

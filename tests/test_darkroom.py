@@ -234,3 +234,19 @@ def test_develop_refuses_what_it_cannot_develop():
     assert lib.pt_develop(None, ctypes.byref(d), ctypes.sizeof(d), None, 0, None) != 0
     assert lib.pt_develop_histogram(None, ctypes.byref(d), ctypes.sizeof(d), 1, hist, 1024, None) != 0
     assert ctypes.sizeof(pt.PtDevelop) == lib.pt_struct_size(15) == 64 * 4 + 4 * 1024 * 4
+
+
+def test_views_of_the_checker():
+    rng = np.random.default_rng(5)
+    acc = np.concatenate([rng.lognormal(0, 1.5, (20, 32, 3)), np.ones((20, 32, 1))], axis=-1)
+    rgb, v, ev = D.develop(acc, tone=1)
+    same, sv = D.develop_view(acc, (16.0, 10.0, 1.0, 1.0), 32, 20, tone=1, ev=ev)
+    assert np.array_equal(same, rgb) and np.allclose(sv, v)
+    # a half-size thumbnail is the mean of each 2 x 2 block of the picture's values
+    _, tv = D.develop_view(acc, (16.0, 10.0, 2.0, 2.0), 16, 10, tone=1, ev=ev)
+    assert np.allclose(tv, v.reshape(10, 2, 16, 2, 3).mean(axis=(1, 3)))
+    # zoomed in 4 x: each picture pixel a 4 x 4 square; beyond the picture the backdrop
+    _, zv = D.develop_view(acc, (30.0, 18.0, 0.25, 0.25), 24, 24, tone=1, ev=ev)
+    assert np.allclose(zv[:, -4:], 0.09 * 255) and np.allclose(zv[:4, :], 0.09 * 255)     # x >= 32 or y >= 20 is outside
+    assert np.allclose(zv[-1, 0], v[20 - 1 - 15, 27])     # the bottom-left output pixel looks at picture pixel (27, 15)
+    assert np.allclose(zv[-4:, :4], v[20 - 1 - 15, 27])   # ... and so do the other 15 of its 4 x 4 square

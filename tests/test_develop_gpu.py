@@ -158,6 +158,45 @@ def test_develop16_matches_the_checker(engine):
     assert np.abs(np.round(rgba16[..., :3] / 257.0) - rgba8[..., :3]).max() <= 1
 
 
+def ambiguous_taps(view, w, h):
+    """[h, w] (top row first): output pixels with a tap within 1e-3 of a picture pixel's edge, where an FP32 tap and
+    the checker's FP64 one may read neighbouring pixels."""
+    cx0, cy0, sx, sy = view
+    box = int(min(8, max(1, np.ceil(max(sx, sy) - 1e-9))))
+    oy, ox = np.mgrid[0:h, 0:w].astype(float)
+    bad = np.zeros((h, w), bool)
+    for b in range(box * box):
+        qx = cx0 + (ox + 0.5 - 0.5 * w) * sx - 0.5 * sx + ((b % box) + 0.5) / box * sx
+        qy = cy0 + (oy + 0.5 - 0.5 * h) * sy - 0.5 * sy + ((b // box) + 0.5) / box * sy
+        bad |= (np.abs(qx - np.round(qx)) < 1e-3) | (np.abs(qy - np.round(qy)) < 1e-3)
+    return bad[::-1]
+
+
+def test_views_of_the_developed_picture(engine):
+    say('case: develop_view room-c2 96x54 (16 samples): full size, thumbnail, zoomed past the edge, clip stripes')
+    acc = render(engine, 'room-c2', 16, width=96, height=54)
+    grade = darkroom_words(GRADES['light'])
+    full, _ = engine.develop(tone=1, auto=False, ev=9.0, grade=grade)
+    same, _ = engine.develop_view((48.0, 27.0, 1.0, 1.0), 96, 54, tone=1, auto=False, ev=9.0, grade=grade)
+    assert np.array_equal(same, full)                  # the whole picture at its own size is the development itself
+    for view, w, h, clip, ev in (((48.0, 27.0, 2.0, 2.0), 48, 27, False, 9.0),          # a thumbnail: 2 x 2 taps
+                                 ((48.3, 27.1, 3.1337, 3.1337), 30, 17, False, 9.0),    # an odd one: 4 x 4 taps
+                                 ((94.0, 2.0, 0.25, 0.25), 64, 48, False, 9.0),         # zoomed in across the corner
+                                 ((48.0, 27.0, 1.0, 1.0), 96, 54, True, 6.0)):          # over-exposed, stripes
+        got, _ = engine.develop_view(view, w, h, clip_warn=clip, tone=1, auto=False, ev=ev, grade=grade)
+        want, _ = D.develop_view(acc, view, w, h, tone=1, ev=ev, grade=grade, clip_warn=clip)
+        sure = ~ambiguous_taps(view, w, h)
+        assert sure.mean() > 0.8, view
+        diff = np.abs(got[..., :3].astype(int) - want.astype(int))[sure]
+        say('  view', view, w, h, 'max step', diff.max(), 'off', int((diff > 0).sum()), 'compared', int(sure.sum()))
+        assert diff.max() <= 1 and (diff > 0).mean() < 0.01, view
+    # the backdrop beyond the picture's corner, and stripes where an over-exposed picture clips
+    corner, _ = engine.develop_view((94.0, 2.0, 0.25, 0.25), 64, 48, tone=1, auto=False, ev=9.0, grade=grade)
+    assert np.all(corner[:, -1, :3] == 23) and np.all(corner[-1, :, :3] == 23)
+    striped, _ = engine.develop_view((48.0, 27.0, 1.0, 1.0), 96, 54, clip_warn=True, tone=1, auto=False, ev=6.0, grade=grade)
+    assert (np.all(striped[..., :3] == [255, 0, 0], axis=-1)).sum() > 0
+
+
 def test_false_colour_views_match_the_checker(engine):
     say('case: develop room-c2 96x54 (16 samples), false colour of luminance, continuous and in 6 bands')
     acc = render(engine, 'room-c2', 16, width=96, height=54)
