@@ -242,8 +242,73 @@ function clipShares(bins) {
     return n ? { white: bins[1023] / n, black: bins[768] / n, pixels: n } : { white: 0, black: 0, pixels: 0 }
 }
 
+// ---- RECIPES (darkroom/recipe.js): the develop settings as a thing to keep and read back -- JSON text, or the Comment
+// text chunk of a PNG this package saved. What comes in is checked key by key against the settings a develop has: a
+// key it does not know is listed as ignored, a value of the wrong kind is ignored and the default stands, the curves
+// are normalised.
+const RECIPE_KIND = 'darkroom-recipe', RECIPE_VERSION = 1
+const DEVELOP_DEFAULTS = () => ({
+    ev: 15, toneMap: 1, autoExposure: false, contrast: 0, saturation: 0, brightness: 0, show: 0, measureMin: 1, measureMax: 100000,
+    measureBands: 0, ...LIGHT_DEFAULTS, curves: { m: straight(), r: straight(), g: straight(), b: straight() },
+})
+const normalizeCurves = c => {
+    const o = {}
+    for (const k of CHANNELS) o[k] = normalizeCurve(c && typeof c === 'object' && c[k] !== undefined ? c[k] : straight())
+    return o
+}
+function toRecipe(settings, defaults = DEVELOP_DEFAULTS()) {
+    const out = {}
+    for (const k of Object.keys(defaults)) if (k in settings) out[k] = k === 'curves' ? normalizeCurves(settings.curves) : settings[k]
+    return { app: 'shadytracer_amd', kind: RECIPE_KIND, version: RECIPE_VERSION, settings: out }
+}
+function fromRecipe(obj, defaults = DEVELOP_DEFAULTS()) {
+    if (!obj || typeof obj !== 'object') return { error: 'not a recipe' }
+    if ('kind' in obj && obj.kind !== RECIPE_KIND) return { error: `a ${obj.kind}, not a recipe` }
+    const src = obj.settings && typeof obj.settings === 'object' ? obj.settings : obj
+    const settings = {}, ignored = []
+    for (const [k, v] of Object.entries(src)) {
+        if (['app', 'kind', 'version'].includes(k)) continue
+        if (!(k in defaults)) { ignored.push(k); continue }
+        const d = defaults[k]
+        if (k === 'curves') settings.curves = normalizeCurves(v)
+        else if (typeof d === 'number' && typeof v !== 'boolean' && v !== null && v !== '' && Number.isFinite(+v)) settings[k] = +v
+        else if (typeof d === 'boolean' && typeof v === 'boolean') settings[k] = v
+        else ignored.push(k)
+    }
+    return { settings, ignored }
+}
+function recipeFromText(text, defaults) {
+    let obj
+    try { obj = JSON.parse(text) } catch (e) { return { error: 'not JSON' } }
+    return fromRecipe(obj, defaults)
+}
+// the recipe a PNG saved by savePicture carries in its Comment iTXt chunk ({ format, ev, settings })
+function recipeFromPicture(png, defaults) {
+    const b = Buffer.from(png)
+    if (b.length < 8 || b.readUInt32BE(0) !== 0x89504E47) return { error: 'not a PNG' }
+    for (let pos = 8; pos + 8 <= b.length;) {
+        const n = b.readUInt32BE(pos), type = b.toString('latin1', pos + 4, pos + 8), body = b.subarray(pos + 8, pos + 8 + n)
+        if (type === 'iTXt') {
+            const k = body.indexOf(0)
+            if (body.toString('latin1', 0, k) === 'Comment' && body[k + 1] === 0) {
+                let o = k + 3
+                o = body.indexOf(0, o) + 1                                     // the language tag
+                o = body.indexOf(0, o) + 1                                     // the translated keyword
+                const r = recipeFromText(body.toString('utf8', o), defaults)
+                if (!r.error && r.settings.ev === undefined) {
+                    try { const ev = JSON.parse(body.toString('utf8', o)).ev; if (Number.isFinite(ev)) r.settings.ev = ev } catch (e) { }
+                }
+                return r
+            }
+        }
+        pos += 12 + n
+    }
+    return { error: 'this picture says nothing of how it was developed' }
+}
+
 module.exports = {
     LUMA, RGB_TO_XYZ, XYZ_TO_RGB, rgbToXYZ, xyzToRgb, xyToXYZ, chromaticity, planckUv, cctDuvToXy, xyToCctDuv, D65, WB_DEFAULT,
     whiteXy, whiteBalanceMatrix, whiteFromRGB, LIGHT_DEFAULTS, EQ_KEYS, LIGHT, TRANSFERS, wheelTint, cdlFromWheels, LUT_SIZE,
-    normalizeCurve, curveFunction, bakeLut, lightUniforms, lightIdentity, histogramStep, clipShares,
+    normalizeCurve, curveFunction, bakeLut, lightUniforms, lightIdentity, histogramStep, clipShares, RECIPE_KIND, DEVELOP_DEFAULTS,
+    toRecipe, fromRecipe, recipeFromText, recipeFromPicture,
 }

@@ -45,4 +45,14 @@ const rest = D.cdlFromWheels({ liftX: 0, liftY: 0, liftM: 0, gammaX: 0, gammaY: 
 const lift = D.cdlFromWheels({ liftX: 0, liftY: 0, liftM: 0.5, gammaX: 0, gammaY: 0, gammaM: 0, gainX: 0, gainY: 0, gainM: 1 })
 out.cdl = { rest, lift, tint: D.wheelTint(0.6, -0.3), red: D.wheelTint(1, 0) }
 out.step = [D.histogramStep(1920, 1080), D.histogramStep(7680, 4320), D.histogramStep(100, 100)]
+// recipes
+const look = { ...D.DEVELOP_DEFAULTS(), wbTemp: 3400, labVib: 0.3, dither: true, curves: { m: [[0.5, 0.6], [0, 0], [1, 1]] } }
+const text = JSON.stringify(D.toRecipe(look))
+const back = D.recipeFromText(text)
+const odd = D.recipeFromText(JSON.stringify({ settings: { wbTemp: '3200', labVib: true, dither: 1, mystery: 4, curves: 'x' } }))
+const { encodePNG } = require('../../shadytracer_amd/js/png.js')
+const png = encodePNG(new Uint8Array(4 * 4 * 4), 4, 4, { text: [['Software', 's'], ['Comment', JSON.stringify({ format: 'png', ev: 9.25, settings: look })]] })
+const out2 = { recipe: JSON.parse(text), back, odd, fromPicture: D.recipeFromPicture(png), notRecipe: D.recipeFromText('{"kind":"scene"}'),
+    notJson: D.recipeFromText('{'), bare: D.recipeFromPicture(encodePNG(new Uint8Array(16), 2, 2)) }
+out.recipes = out2
 console.log(JSON.stringify(out))

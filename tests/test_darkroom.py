@@ -250,3 +250,16 @@ def test_views_of_the_checker():
     assert np.allclose(zv[:, -4:], 0.09 * 255) and np.allclose(zv[:4, :], 0.09 * 255)     # x >= 32 or y >= 20 is outside
     assert np.allclose(zv[-1, 0], v[20 - 1 - 15, 27])     # the bottom-left output pixel looks at picture pixel (27, 15)
     assert np.allclose(zv[-4:, :4], v[20 - 1 - 15, 27])   # ... and so do the other 15 of its 4 x 4 square
+
+
+def test_recipes_travel_and_are_read_with_care(js):
+    r = js['recipes']
+    assert r['recipe']['kind'] == 'darkroom-recipe' and r['recipe']['settings']['curves']['m'] == [[0, 0], [0.5, 0.6], [1, 1]]
+    assert r['back']['settings'] == r['recipe']['settings'] and r['back']['ignored'] == []
+    # a number as text is read as the number; a boolean where a number belongs, a number where a boolean belongs and an
+    # unknown key are left out; a curve that is not one is straight
+    assert r['odd']['settings']['wbTemp'] == 3200 and 'labVib' not in r['odd']['settings'] and 'dither' not in r['odd']['settings']
+    assert sorted(r['odd']['ignored']) == ['dither', 'labVib', 'mystery']
+    assert r['odd']['settings']['curves']['m'] == [[0, 0], [1, 1]]
+    assert r['fromPicture']['settings']['wbTemp'] == 3400 and r['fromPicture']['settings']['ev'] == 15
+    assert 'error' in r['notRecipe'] and 'error' in r['notJson'] and 'error' in r['bare']
