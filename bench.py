@@ -458,6 +458,17 @@ def main():
             'flops_per_launch': flops_per_launch, 'avg_launch_ms': avg_launch_ms,
             'flops_per_step': flops_per_step, 'flops_per_step_unculled': made['scene']['flops']['dist'] + 30,
             'kernel_ms_per_sample': {k: v / ksamples for k, v in ms.items()}}
+    # the developed picture of the same accumulation: pt_develop's wall clock a call (the meter's two kernels, the
+    # develop kernel, the sync and the 8-bit picture copied to the host), against the bytes its kernel must move
+    # (32 B of FP64 accumulation read and 4 B written a pixel)
+    e.develop(tone='agx', auto=True)
+    t0 = time.perf_counter()
+    for _ in range(10):
+        e.develop(tone='agx', auto=True, speed=0.5)
+    develop = {'workload': '%dx%d AgX, metered each call' % (a.width, a.height),
+               'ms_per_call': (time.perf_counter() - t0) * 1e3 / 10, 'kernel_bytes': 36 * a.width * a.height,
+               'includes': 'meter, develop kernel, sync, %d-byte copy to the host' % (4 * a.width * a.height)}
+    log('bench: develop %.3f ms a call at %dx%d' % (develop['ms_per_call'], a.width, a.height))
     cpu = None if a.no_cpu else cpu_baseline(made)
     segs = st['segments']
     out = {
@@ -472,7 +483,7 @@ def main():
                                                                             'msPerSample', 'msBefore', 'candidates')}},
         'shadow_segments_per_s': st['shadow_segments'] / dt, 'march_steps_per_s': st['steps'] / dt,
         'mean_steps_per_segment': st['steps'] / max(segs, 1),
-        'roofline': roof, 'cpu_baseline': cpu,
+        'roofline': roof, 'cpu_baseline': cpu, 'develop': develop,
     }
     e.close()
     if not a.no_guide:
