@@ -64,3 +64,17 @@ def test_png_text_and_16_bits(files):
     im = Image.open(io.BytesIO(data))
     im.load()
     assert im.size == (src.shape[1], src.shape[0]) and im.info.get('Software') == 'shadytracer_amd'
+
+
+@pytest.mark.parametrize('channels', [3, 4])
+def test_python_png_16_bits_with_text(channels):
+    from shadytracer_amd import png
+    rng = np.random.default_rng(11)
+    img = rng.integers(0, 65536, (13, 19, channels)).astype(np.uint16)
+    img[..., 0] = np.arange(19)[None, :] * 3000                  # a gradient, so the filters have work
+    data = png.encode(img, text=[('Software', 'shadytracer_amd'), ('Comment', 'ev 9.5, cd/m²')])
+    got, kinds = T.decode_png(data)
+    assert got.dtype == np.uint16 and np.array_equal(got, img) and kinds[1] == b'iTXt'
+    assert T.png_text(data) == {'Software': 'shadytracer_amd', 'Comment': 'ev 9.5, cd/m²'}
+    for k in range(5):                                           # every filter on 6- and 8-byte pixels
+        assert np.array_equal(T.decode_png(png.encode(img, filters=k))[0], img)
